@@ -1,0 +1,167 @@
+"""ctypes binding of libcrashcat_amd.so (the HIP path).  There is no CPU fallback:
+if the library or a GPU is missing, every entry point raises."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcrashcat_amd.so")
+_LIB = None
+
+SYMBOLS = [
+    "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
+    "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
+    "cc_get_phase_times",
+]
+
+
+class CrashcatError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise CrashcatError(f"{LIB_PATH} is not built; run crashcat_amd/build.py (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    ip = C.POINTER(C.c_int32)
+    vp = C.c_void_p
+    L.cc_create.argtypes = [C.POINTER(vp), C.POINTER(_abi.Config)]
+    L.cc_destroy.argtypes = [vp]
+    L.cc_last_error.argtypes = [vp]
+    L.cc_last_error.restype = C.c_char_p
+    L.cc_upload_scene.argtypes = [vp, C.POINTER(_abi.Settings), C.c_int32, ip, C.POINTER(_abi.BodyDesc), C.c_int32,
+                                  C.POINTER(_abi.ShapeDesc), C.c_int32, C.POINTER(_abi.HullData),
+                                  C.POINTER(_abi.PointConstraintDesc), C.c_int32]
+    L.cc_step.argtypes = [vp, C.c_double, C.c_int32]
+    L.cc_sync.argtypes = [vp]
+    L.cc_download_bodies.argtypes = [vp, C.POINTER(_abi.BodyState), C.c_int32]
+    L.cc_set_bodies.argtypes = [vp, C.POINTER(_abi.BodyState), C.c_int32]
+    L.cc_download_pairs.argtypes = [vp, ip, C.c_int32, ip]
+    L.cc_download_contacts.argtypes = [vp, C.POINTER(_abi.ContactOut), C.c_int32, ip]
+    L.cc_get_counters.argtypes = [vp, C.POINTER(_abi.StepCounters)]
+    fp = C.POINTER(C.c_float)
+    L.cc_broadphase_aabbs.argtypes = [vp, fp, fp, ip, ip, ip, C.c_int32, C.c_double, ip, C.c_int32, ip]
+    L.cc_get_phase_times.argtypes = [vp, C.POINTER(C.c_double), C.c_int32, ip]
+    for s in SYMBOLS:
+        getattr(L, s).restype = getattr(L, s).restype if s == "cc_last_error" else C.c_int
+    _LIB = L
+    return L
+
+
+BODY_STATE_DTYPE = np.dtype(_abi.BodyState)
+CONTACT_DTYPE = np.dtype(_abi.ContactOut)
+PHASES = ["broadphase", "narrowphase", "epa", "islands", "solver", "cache"]
+
+
+class Context:
+    """One device context (a batch of worlds) behind the C ABI."""
+
+    def __init__(self, precision="fp32", device=0, max_pairs_per_body=16, block_solver=0):
+        L = lib()
+        cfg = _abi.Config()
+        cfg.precision = _abi.CC_PRECISION_FP64 if precision == "fp64" else _abi.CC_PRECISION_FP32
+        cfg.device = device
+        cfg.max_pairs_per_body = max_pairs_per_body
+        cfg.max_contacts_per_body = 8
+        cfg.block_solver = block_solver
+        h = C.c_void_p()
+        rc = L.cc_create(C.byref(h), C.byref(cfg))
+        if rc != 0:
+            raise CrashcatError(f"cc_create failed ({rc}): no usable HIP device")
+        self.h = h
+        self.n = 0
+        self.precision = precision
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CrashcatError(lib().cc_last_error(self.h).decode() + f" (code {rc})")
+
+    def close(self):
+        if self.h:
+            lib().cc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene):
+        L = lib()
+        bodies = scene.bodies()
+        shapes, hulls, keep = scene.c_shapes()
+        pcs, npc = scene.c_point_constraints()
+        offs = scene.c_offsets()
+        self._keep = (bodies, keep, pcs, offs)
+        self.n = len(bodies)
+        self.num_worlds = len(offs) - 1
+        self._check(L.cc_upload_scene(self.h, C.byref(scene.settings), len(offs) - 1,
+                                      offs.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      bodies.ctypes.data_as(C.POINTER(_abi.BodyDesc)), self.n, shapes,
+                                      len(scene.shapes), C.byref(hulls), pcs, npc))
+
+    def step(self, dt=1.0 / 60.0, n=1):
+        self._check(lib().cc_step(self.h, dt, n))
+
+    def sync(self):
+        self._check(lib().cc_sync(self.h))
+
+    def bodies(self):
+        out = np.zeros(self.n, dtype=BODY_STATE_DTYPE)
+        self._check(lib().cc_download_bodies(self.h, out.ctypes.data_as(C.POINTER(_abi.BodyState)), self.n))
+        return out
+
+    def set_bodies(self, states):
+        st = np.ascontiguousarray(states, dtype=BODY_STATE_DTYPE)
+        self._check(lib().cc_set_bodies(self.h, st.ctypes.data_as(C.POINTER(_abi.BodyState)), self.n))
+
+    def pairs(self):
+        cnt = C.c_int32()
+        self._check(lib().cc_download_pairs(self.h, None, 0, C.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 2), dtype=np.int32)
+        self._check(lib().cc_download_pairs(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), cnt.value, C.byref(cnt)))
+        return out[:cnt.value]
+
+    def contacts(self):
+        cnt = C.c_int32()
+        self._check(lib().cc_download_contacts(self.h, None, 0, C.byref(cnt)))
+        out = np.zeros(max(cnt.value, 1), dtype=CONTACT_DTYPE)
+        self._check(lib().cc_download_contacts(self.h, out.ctypes.data_as(C.POINTER(_abi.ContactOut)), cnt.value,
+                                               C.byref(cnt)))
+        return out[:cnt.value]
+
+    def counters(self):
+        c = _abi.StepCounters()
+        self._check(lib().cc_get_counters(self.h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in _abi.StepCounters._fields_}
+
+    def phase_times(self):
+        buf = (C.c_double * 16)()
+        cnt = C.c_int32()
+        self._check(lib().cc_get_phase_times(self.h, buf, 16, C.byref(cnt)))
+        return {PHASES[i] if i < len(PHASES) else str(i): buf[i] for i in range(cnt.value)}
+
+    def broadphase_aabbs(self, mn, mx, motion, sleeping, layer, margin):
+        mn = np.ascontiguousarray(mn, dtype=np.float32)
+        mx = np.ascontiguousarray(mx, dtype=np.float32)
+        mo = np.ascontiguousarray(motion, dtype=np.int32)
+        sl = np.ascontiguousarray(sleeping, dtype=np.int32)
+        la = np.ascontiguousarray(layer, dtype=np.int32)
+        n = len(mo)
+        ip = C.POINTER(C.c_int32)
+        fp = C.POINTER(C.c_float)
+        cnt = C.c_int32()
+        cap = max(64 * n, 1)
+        out = np.zeros((cap, 2), dtype=np.int32)
+        self._check(lib().cc_broadphase_aabbs(self.h, mn.ctypes.data_as(fp), mx.ctypes.data_as(fp), mo.ctypes.data_as(ip),
+                                              sl.ctypes.data_as(ip), la.ctypes.data_as(ip), n, margin,
+                                              out.ctypes.data_as(ip), cap, C.byref(cnt)))
+        self.n = n
+        return out[:min(cnt.value, cap)]

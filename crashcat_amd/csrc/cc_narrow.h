@@ -1,0 +1,1045 @@
+// Convex narrowphase device code: support functions, GJK closest points, EPA
+// (triangle normal / distance / barycentrics in FP64), supporting faces,
+// Sutherland-Hodgman clipping, manifold generation and 4-point pruning.
+// One pair per lane.  Citations are to the reference (crashcat) sources.
+#pragma once
+#include "cc_math.h"
+
+namespace cc {
+
+enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3 };
+
+template <typename T>
+struct ShapeD {
+    int type, hfp, hnp, hff, hnf, pad;
+    T radius, hh, cr;
+    T he[3], com[3], amin[3], amax[3];
+};
+
+template <typename T>
+struct ShapeTable {
+    const ShapeD<T>* shapes;
+    const T* pts;     // hull points (COM relative), xyz
+    const T* shrunk;  // hull points with the convex radius removed
+    const int* ffv;   // face first vertex
+    const int* fnv;   // face vertex count
+    const int* fvi;   // face vertex indices (relative to the hull's first point)
+    const T* pn;      // face plane normals
+};
+
+// ---------------------------------------------------------------- supports
+enum { SK_ZERO, SK_SPHERE_R, SK_BOX, SK_CAP_NC, SK_CAP_R, SK_HULL };
+
+template <typename T>
+struct Sup {
+    int kind;
+    int np;
+    T r, hh, cr;
+    V3<T> he;
+    const T* pts;
+    bool transformed, addRadius;
+    T addR;
+    M3<T> R;
+    V3<T> t;
+};
+
+// getShapeSupportFunction for unit scale (sphere.ts:246-256, support.ts:336-364,
+// capsule.ts:392-404, convex-hull.ts:979-996)
+template <typename T>
+__device__ inline void sup_for_shape(Sup<T>& s, const ShapeTable<T>& st, int si, bool include) {
+    const ShapeD<T>& d = st.shapes[si];
+    s.transformed = false;
+    s.addRadius = false;
+    s.np = 0;
+    s.pts = nullptr;
+    s.r = s.hh = s.cr = T(0);
+    s.he = mk3<T>(0, 0, 0);
+    switch (d.type) {
+        case SHAPE_SPHERE:
+            if (include) { s.kind = SK_SPHERE_R; s.r = d.radius; }
+            else { s.kind = SK_ZERO; s.cr = d.radius; }
+            break;
+        case SHAPE_BOX:
+            s.kind = SK_BOX;
+            if (!include) {
+                T cr = tmin<T>(d.cr, T(0.05));
+                s.he = mk3<T>(tmax<T>(T(0), d.he[0] - cr), tmax<T>(T(0), d.he[1] - cr), tmax<T>(T(0), d.he[2] - cr));
+                s.cr = cr;
+            } else {
+                s.he = mk3<T>(d.he[0], d.he[1], d.he[2]);
+            }
+            break;
+        case SHAPE_CAPSULE:
+            s.hh = d.hh;
+            if (include) { s.kind = SK_CAP_R; s.r = d.radius; }
+            else { s.kind = SK_CAP_NC; s.cr = d.radius; }
+            break;
+        default:
+            s.kind = SK_HULL;
+            s.np = d.hnp;
+            if (include || d.cr == T(0)) { s.pts = st.pts + 3 * d.hfp; }
+            else { s.pts = st.shrunk + 3 * d.hfp; s.cr = d.cr; }
+            break;
+    }
+}
+
+template <typename T>
+__device__ inline V3<T> sup_base(const Sup<T>& s, V3<T> d) {
+    switch (s.kind) {
+        case SK_ZERO: return mk3<T>(0, 0, 0);
+        case SK_SPHERE_R: {
+            T l2 = len2(d);
+            if (l2 > T(0)) { T sc = s.r / sqrt(l2); return d * sc; }
+            return mk3<T>(0, 0, 0);
+        }
+        case SK_BOX:
+            return mk3<T>(d.x >= T(0) ? s.he.x : -s.he.x, d.y >= T(0) ? s.he.y : -s.he.y, d.z >= T(0) ? s.he.z : -s.he.z);
+        case SK_CAP_NC: return mk3<T>(0, d.y > T(0) ? s.hh : -s.hh, 0);
+        case SK_CAP_R: {
+            T l2 = len2(d);
+            if (l2 > T(0)) {
+                T sc = s.r / sqrt(l2);
+                return mk3<T>(d.x * sc, d.y * sc + (d.y > T(0) ? s.hh : -s.hh), d.z * sc);
+            }
+            return mk3<T>(0, s.hh, 0);
+        }
+        default: {
+            T best = -INFINITY;
+            V3<T> out = mk3<T>(0, 0, 0);
+            for (int i = 0; i < s.np; i++) {
+                V3<T> p = mk3<T>(s.pts[3 * i], s.pts[3 * i + 1], s.pts[3 * i + 2]);
+                T dt = dot(p, d);
+                if (dt > best) { best = dt; out = p; }
+            }
+            return out;
+        }
+    }
+}
+
+// TransformedSupport (support.ts:236-261) + AddConvexRadiusSupport (support.ts:288-303)
+template <typename T>
+__device__ inline V3<T> sup_get(const Sup<T>& s, V3<T> d) {
+    V3<T> out;
+    if (s.transformed) {
+        V3<T> p = sup_base(s, mul_t(s.R, d));
+        out = mul(s.R, p) + s.t;
+    } else {
+        out = sup_base(s, d);
+    }
+    if (s.addRadius) {
+        T l2 = len2(d);
+        if (l2 > T(0)) { T sc = s.addR / sqrt(l2); out = out + d * sc; }
+    }
+    return out;
+}
+
+// ------------------------------------------------------- barycentrics
+template <typename T> struct Bary { T u, v, w; };
+
+template <typename T>
+__device__ inline Bary<T> bary2(V3<T> a, V3<T> b, T tol2) {  // closest-points.ts:19-53
+    Bary<T> r = {0, 0, 0};
+    V3<T> ab = b - a;
+    T den = len2(ab);
+    if (den < tol2) {
+        if (len2(a) < len2(b)) { r.u = 1; r.v = 0; }
+        else { r.u = 0; r.v = 1; }
+        return r;
+    }
+    r.v = -dot(a, ab) / den;
+    r.u = T(1) - r.v;
+    return r;
+}
+
+template <typename T>
+__device__ inline Bary<T> bary3(V3<T> a, V3<T> b, V3<T> c, T tol2) {  // closest-points.ts:57-140
+    Bary<T> r = {0, 0, 0};
+    V3<T> ab = b - a, ac = c - a, bc = c - b;
+    T d00 = len2(ab), d11 = len2(ac), d22 = len2(bc);
+    if (d00 <= d22) {
+        T d01 = dot(ab, ac);
+        T den = d00 * d11 - d01 * d01;
+        if (fabs(den) < T(1.0e-12)) {
+            if (d00 > d11) { Bary<T> o = bary2(a, b, tol2); r.u = o.u; r.v = o.v; r.w = 0; }
+            else { Bary<T> o = bary2(a, c, tol2); r.u = o.u; r.w = o.v; r.v = 0; }
+            return r;
+        }
+        T a0 = dot(a, ab), a1 = dot(a, ac);
+        r.v = (d01 * a1 - d11 * a0) / den;
+        r.w = (d01 * a0 - d00 * a1) / den;
+        r.u = T(1) - r.v - r.w;
+    } else {
+        T d12 = dot(ac, bc);
+        T den = d11 * d22 - d12 * d12;
+        if (fabs(den) < T(1.0e-12)) {
+            if (d11 > d22) { Bary<T> o = bary2(a, c, tol2); r.u = o.u; r.w = o.v; r.v = 0; }
+            else { Bary<T> o = bary2(b, c, tol2); r.v = o.u; r.w = o.v; r.u = 0; }
+            return r;
+        }
+        T c1 = dot(c, ac), c2 = dot(c, bc);
+        r.u = (d22 * c1 - d12 * c2) / den;
+        r.v = (d11 * c2 - d12 * c1) / den;
+        r.w = T(1) - r.u - r.v;
+    }
+    return r;
+}
+
+// ------------------------------------------------------------------- GJK
+template <typename T> struct SPt { V3<T> y, p, q; };
+template <typename T> struct Simplex { SPt<T> pt[4]; int size; };
+template <typename T> struct CPRes { V3<T> point; int set; };
+
+template <typename T> CC_HD T clamp01(T x) { return x < T(0) ? T(0) : (x > T(1) ? T(1) : x); }
+
+template <typename T>
+__device__ inline CPRes<T> cp_line(V3<T> a, V3<T> b, T tol2) {  // gjk.ts:33-55
+    Bary<T> bc = bary2(a, b, tol2);
+    CPRes<T> r;
+    if (bc.v <= T(0)) { r.point = a; r.set = 1; }
+    else if (bc.u <= T(0)) { r.point = b; r.set = 2; }
+    else { r.point = a * bc.u + b * bc.v; r.set = 3; }
+    return r;
+}
+
+template <typename T>
+__device__ inline CPRes<T> cp_triangle(V3<T> inA, V3<T> inB, V3<T> inC, bool mustIncludeC, T tol2) {  // gjk.ts:57-309
+    CPRes<T> r;
+    bool swapAC = len2(inC - inB) < len2(inC - inA);
+    V3<T> a = swapAC ? inC : inA;
+    V3<T> c = swapAC ? inA : inC;
+    V3<T> ab = inB - a, ac = c - a;
+    V3<T> n = cross(ab, ac);
+    T n2 = len2(n);
+    if (n2 < T(1.0e-10)) {
+        int set = 4;
+        V3<T> best = inC;
+        T bestD = len2(inC);
+        if (!mustIncludeC) {
+            T a2 = len2(inA);
+            if (a2 < bestD) { set = 1; best = inA; bestD = a2; }
+            T b2 = len2(inB);
+            if (b2 < bestD) { set = 2; best = inB; bestD = b2; }
+        }
+        V3<T> ac2 = c - a;
+        T acl = len2(ac2);
+        if (acl > tol2) {
+            T v = clamp01(-dot(a, ac2) / acl);
+            V3<T> q = a + ac2 * v;
+            T d2 = len2(q);
+            if (d2 < bestD) { set = 5; best = q; bestD = d2; }
+        }
+        V3<T> bc2 = inC - inB;
+        T bcl = len2(bc2);
+        if (bcl > tol2) {
+            T v = clamp01(-dot(inB, bc2) / bcl);
+            V3<T> q = inB + bc2 * v;
+            T d2 = len2(q);
+            if (d2 < bestD) { set = 6; best = q; bestD = d2; }
+        }
+        if (!mustIncludeC) {
+            V3<T> ab2 = inB - inA;
+            T abl = len2(ab2);
+            if (abl > tol2) {
+                T v = clamp01(-dot(inA, ab2) / abl);
+                V3<T> q = inA + ab2 * v;
+                T d2 = len2(q);
+                if (d2 < bestD) { set = 3; best = q; }
+            }
+        }
+        r.set = set;
+        r.point = best;
+        return r;
+    }
+    V3<T> ap = -a;
+    T d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= T(0) && d2 <= T(0)) { r.set = swapAC ? 4 : 1; r.point = a; return r; }
+    V3<T> bp = -inB;
+    T d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= T(0) && d4 <= d3) { r.set = 2; r.point = inB; return r; }
+    if (d1 * d4 <= d3 * d2 && d1 >= T(0) && d3 <= T(0)) {
+        T v = d1 / (d1 - d3);
+        r.set = swapAC ? 6 : 3;
+        r.point = a + ab * v;
+        return r;
+    }
+    V3<T> cp = -c;
+    T d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= T(0) && d5 <= d6) { r.set = swapAC ? 1 : 4; r.point = c; return r; }
+    if (d5 * d2 <= d1 * d6 && d2 >= T(0) && d6 <= T(0)) {
+        T w = d2 / (d2 - d6);
+        r.set = 5;
+        r.point = a + ac * w;
+        return r;
+    }
+    T d43 = d4 - d3, d56 = d5 - d6;
+    if (d3 * d6 <= d5 * d4 && d43 >= T(0) && d56 >= T(0)) {
+        T w = d43 / (d43 + d56);
+        r.set = swapAC ? 3 : 6;
+        r.point = inB + (c - inB) * w;
+        return r;
+    }
+    r.set = 7;
+    V3<T> sum = a + inB + c;
+    T sc = dot(sum, n) / (T(3) * n2);
+    r.point = n * sc;
+    return r;
+}
+
+template <typename T>
+__device__ inline void origin_outside(bool out[4], V3<T> a, V3<T> b, V3<T> c, V3<T> d, T tol) {  // gjk.ts:314-394
+    V3<T> ab = b - a, ac = c - a, ad = d - a, bd = d - b, bc = c - b;
+    V3<T> abac = cross(ab, ac), acad = cross(ac, ad), adab = cross(ad, ab), bdbc = cross(bd, bc);
+    T sp[4] = {dot(a, abac), dot(a, acad), dot(a, adab), dot(b, bdbc)};
+    T sd[4] = {dot(ad, abac), dot(ab, acad), dot(ac, adab), -dot(ab, bdbc)};
+    if (sd[0] > T(0) && sd[1] > T(0) && sd[2] > T(0) && sd[3] > T(0)) {
+        for (int i = 0; i < 4; i++) out[i] = sp[i] >= -tol;
+        return;
+    }
+    if (sd[0] < T(0) && sd[1] < T(0) && sd[2] < T(0) && sd[3] < T(0)) {
+        for (int i = 0; i < 4; i++) out[i] = sp[i] <= tol;
+        return;
+    }
+    out[0] = out[1] = out[2] = out[3] = true;
+}
+
+template <typename T>
+__device__ inline CPRes<T> cp_tetra(V3<T> a, V3<T> b, V3<T> c, V3<T> d, bool mustIncludeD, T tol) {  // gjk.ts:399-487
+    T tol2 = tol * tol;
+    CPRes<T> r;
+    r.set = 15;
+    r.point = mk3<T>(0, 0, 0);
+    T best = INFINITY;
+    bool o[4];
+    origin_outside(o, a, b, c, d, tol);
+    if (o[0]) {
+        if (mustIncludeD) { r.set = 1; r.point = a; }
+        else r = cp_triangle(a, b, c, false, tol2);
+        best = len2(r.point);
+    }
+    if (o[1]) {
+        CPRes<T> t = cp_triangle(a, c, d, mustIncludeD, tol2);
+        T d2 = len2(t.point);
+        if (d2 < best) { best = d2; r.point = t.point; r.set = (t.set & 1) + ((t.set & 6) << 1); }
+    }
+    if (o[2]) {
+        CPRes<T> t = cp_triangle(a, b, d, mustIncludeD, tol2);
+        T d2 = len2(t.point);
+        if (d2 < best) { best = d2; r.point = t.point; r.set = (t.set & 3) + ((t.set & 4) << 1); }
+    }
+    if (o[3]) {
+        CPRes<T> t = cp_triangle(b, c, d, mustIncludeD, tol2);
+        T d2 = len2(t.point);
+        if (d2 < best) { r.point = t.point; r.set = t.set << 1; }
+    }
+    return r;
+}
+
+template <typename T> struct GjkOut { V3<T> pointA, pointB, axis; T dist2; Simplex<T> simplex; };
+
+// gjkClosestPoints gjk.ts:1294-1546
+template <typename T>
+__device__ inline void gjk_closest_points(GjkOut<T>& out, const Sup<T>& A, const Sup<T>& B, T tolerance, V3<T> direction,
+                                          T maxDist2) {
+    const T GJK_TOL = T(1e-5);
+    T tol2 = tolerance * tolerance;
+    Simplex<T>& s = out.simplex;
+    s.size = 0;
+    V3<T> cpt = direction;
+    T cd2 = len2(direction);
+    int cset = 0;
+    T prevDist2 = sizeof(T) == 8 ? T(1.7976931348623157e308) : T(3.402823466e38);
+    int it = 0;
+    while (it++ < 100) {
+        V3<T> p = sup_get(A, cpt);
+        V3<T> q = sup_get(B, -cpt);
+        V3<T> w = p - q;
+        T dt = dot(cpt, w);
+        if (dt < T(0) && dt * dt > cd2 * maxDist2) {
+            out.dist2 = INFINITY;
+            out.axis = cpt;
+            s.size = 0;
+            out.pointA = out.pointB = mk3<T>(0, 0, 0);
+            return;
+        }
+        s.pt[s.size].y = w;
+        s.pt[s.size].p = p;
+        s.pt[s.size].q = q;
+        s.size++;
+        CPRes<T> cp;
+        switch (s.size) {
+            case 1: cp.set = 1; cp.point = s.pt[0].y; break;
+            case 2: cp = cp_line(s.pt[0].y, s.pt[1].y, T(1e-10)); break;
+            case 3: cp = cp_triangle(s.pt[0].y, s.pt[1].y, s.pt[2].y, true, T(1e-10)); break;
+            default: cp = cp_tetra(s.pt[0].y, s.pt[1].y, s.pt[2].y, s.pt[3].y, true, T(1e-5)); break;
+        }
+        T d2 = len2(cp.point);
+        if (!(d2 < prevDist2)) { s.size--; break; }
+        cpt = cp.point;
+        cd2 = d2;
+        cset = cp.set;
+        if (cset == 0xf) { cpt = mk3<T>(0, 0, 0); cd2 = T(0); break; }
+        int ns = 0;
+        for (int i = 0; i < s.size; i++) {
+            if (cset & (1 << i)) {
+                if (ns != i) s.pt[ns] = s.pt[i];
+                ns++;
+            }
+        }
+        s.size = ns;
+        if (cd2 <= tol2) { cpt = mk3<T>(0, 0, 0); cd2 = T(0); break; }
+        T ymax = T(0);
+        for (int i = 0; i < s.size; i++) ymax = tmax<T>(ymax, len2(s.pt[i].y));
+        if (cd2 <= GJK_TOL * ymax) { cpt = mk3<T>(0, 0, 0); cd2 = T(0); break; }
+        cpt = -cpt;
+        if (prevDist2 - cd2 <= GJK_TOL * prevDist2) break;
+        prevDist2 = cd2;
+    }
+    out.pointA = out.pointB = mk3<T>(0, 0, 0);
+    if (s.size == 0) { out.dist2 = INFINITY; out.axis = mk3<T>(0, 0, 0); return; }
+    if (s.size == 1) { out.pointA = s.pt[0].p; out.pointB = s.pt[0].q; }
+    else if (s.size == 2) {
+        Bary<T> b = bary2(s.pt[0].y, s.pt[1].y, T(1e-10));
+        out.pointA = s.pt[0].p * b.u + s.pt[1].p * b.v;
+        out.pointB = s.pt[0].q * b.u + s.pt[1].q * b.v;
+    } else if (s.size == 3) {
+        Bary<T> b = bary3(s.pt[0].y, s.pt[1].y, s.pt[2].y, T(1e-10));
+        out.pointA = s.pt[0].p * b.u + s.pt[1].p * b.v + s.pt[2].p * b.w;
+        out.pointB = s.pt[0].q * b.u + s.pt[1].q * b.v + s.pt[2].q * b.w;
+    }
+    out.axis = cpt;
+    out.dist2 = cd2;
+}
+
+enum { PD_NOT_COLLIDING = 0, PD_COLLIDING = 1, PD_INDETERMINATE = 2 };
+template <typename T> struct PenDepth { int status; V3<T> axis, pointA, pointB; };
+
+// penetrationDepthStepGJK penetration.ts:35-110
+template <typename T>
+__device__ inline void pd_step_gjk(PenDepth<T>& out, Simplex<T>& outSimplex, const Sup<T>& A, const Sup<T>& B, T crA, T crB,
+                                   V3<T> dir, T tol) {
+    T cr = crA + crB, cr2 = cr * cr;
+    GjkOut<T> g;
+    gjk_closest_points(g, A, B, tol, dir, cr2);
+    if (g.dist2 > cr2) {
+        out.pointA = out.pointB = out.axis = mk3<T>(0, 0, 0);
+        out.status = PD_NOT_COLLIDING;
+        return;
+    }
+    out.pointA = g.pointA;
+    out.pointB = g.pointB;
+    out.axis = g.axis;
+    outSimplex = g.simplex;
+    if (g.dist2 > T(0)) {
+        T vl = sqrt(g.dist2);
+        out.pointA = out.pointA + out.axis * (crA / vl);
+        out.pointB = out.pointB + out.axis * (-(crB / vl));
+        out.status = PD_COLLIDING;
+        return;
+    }
+    out.status = PD_INDETERMINATE;
+}
+
+// ------------------------------------------------------------------- EPA
+// epa-convex-hull-builder.ts: triangle pool (256), min-heap on closestLengthSq,
+// horizon search with an explicit stack; normal, centroid, distance and the
+// barycentric lambdas of each face are kept in FP64.
+#define EPA_MAX_TRIANGLES 256
+#define EPA_MAX_EDGE_LENGTH 128
+#define EPA_POINT_CAP 160
+#define EPA_MAX_POINTS_TO_INCLUDE_ORIGIN_IN_HULL 32
+#define EPA_MAX_POINTS 128
+
+struct EEdge { short nbT; unsigned char nbE; unsigned char start; };
+struct ETri {
+    double nx, ny, nz, cx, cy, cz;
+    double cls;
+    double l0, l1;
+    EEdge e[3];
+    unsigned char rel0, interior, removed, inQueue;
+};
+struct EStack { short tri; short edge; short iter; short pad; };
+
+template <typename T>
+struct EpaState {
+    V3<T> y[EPA_POINT_CAP], p[EPA_POINT_CAP], q[EPA_POINT_CAP];
+    ETri tris[EPA_MAX_TRIANGLES];
+    short freeList[EPA_MAX_TRIANGLES * 2];
+    short queue[EPA_MAX_TRIANGLES * 2];
+    EStack stack[EPA_MAX_EDGE_LENGTH];
+    EEdge edges[EPA_MAX_EDGE_LENGTH];
+    short newT[EPA_MAX_EDGE_LENGTH];
+    int n, nfree, nq, nedges, nnew;
+};
+
+__device__ inline bool tri_facing(const ETri& t, double px, double py, double pz) {
+    return t.nx * (px - t.cx) + t.ny * (py - t.cy) + t.nz * (pz - t.cz) > 0.0;
+}
+
+template <typename T>
+__device__ inline void heap_push(EpaState<T>& h, int ti) {
+    if (h.nq >= EPA_MAX_TRIANGLES * 2) return;
+    h.queue[h.nq++] = (short)ti;
+    h.tris[ti].inQueue = 1;
+    int cur = h.nq - 1;
+    while (cur > 0) {
+        int par = (cur - 1) >> 1;
+        if (h.tris[h.queue[par]].cls > h.tris[h.queue[cur]].cls) {
+            short t = h.queue[par]; h.queue[par] = h.queue[cur]; h.queue[cur] = t;
+            cur = par;
+        } else break;
+    }
+}
+
+template <typename T>
+__device__ inline int heap_pop(EpaState<T>& h) {
+    if (h.nq == 0) return -1;
+    short* q = h.queue;
+    short tmp = q[h.nq - 1]; q[h.nq - 1] = q[0]; q[0] = tmp;
+    int count = h.nq - 1;
+    int largest = 0;
+    for (;;) {
+        int child = (largest << 1) + 1;
+        if (child >= count) break;
+        int prev = largest;
+        if (h.tris[q[largest]].cls > h.tris[q[child]].cls) largest = child;
+        ++child;
+        if (child < count && h.tris[q[largest]].cls > h.tris[q[child]].cls) largest = child;
+        if (prev == largest) break;
+        short t = q[prev]; q[prev] = q[largest]; q[largest] = t;
+    }
+    return q[--h.nq];
+}
+
+template <typename T>
+__device__ inline void free_tri(EpaState<T>& h, int ti) {
+    if (h.nfree < EPA_MAX_TRIANGLES * 2) h.freeList[h.nfree++] = (short)ti;
+}
+
+template <typename T>
+__device__ inline int create_tri(EpaState<T>& h, int i1, int i2, int i3) {  // createTriangle :179-345
+    if (h.nfree == 0) return -1;
+    int ti = h.freeList[--h.nfree];
+    ETri& t = h.tris[ti];
+    t.cls = INFINITY;
+    t.l0 = t.l1 = 0.0;
+    t.rel0 = 0; t.interior = 0; t.removed = 0; t.inQueue = 0;
+    t.e[0].start = (unsigned char)i1; t.e[1].start = (unsigned char)i2; t.e[2].start = (unsigned char)i3;
+    t.e[0].nbT = t.e[1].nbT = t.e[2].nbT = -1;
+    V3<double> y0 = cvt<double>(h.y[i1]), y1 = cvt<double>(h.y[i2]), y2 = cvt<double>(h.y[i3]);
+    t.cx = (y0.x + y1.x + y2.x) / 3.0; t.cy = (y0.y + y1.y + y2.y) / 3.0; t.cz = (y0.z + y1.z + y2.z) / 3.0;
+    V3<double> cen = mk3<double>(t.cx, t.cy, t.cz);
+    V3<double> y10 = y1 - y0, y20 = y2 - y0, y21 = y2 - y1;
+    double y20d = len2(y20), y21d = len2(y21);
+    if (y20d < y21d) {
+        V3<double> n = cross(y10, y20);
+        t.nx = n.x; t.ny = n.y; t.nz = n.z;
+        double nl = len2(n);
+        if (nl > 1e-10) {
+            double cdn = dot(cen, n);
+            t.cls = fabs(cdn) * cdn / nl;
+            double y10d = len2(y10), y10y20 = dot(y10, y20);
+            double det = y10d * y20d - y10y20 * y10y20;
+            if (det > 0.0) {
+                double y0y10 = dot(y0, y10), y0y20 = dot(y0, y20);
+                double l0 = (y10y20 * y0y20 - y20d * y0y10) / det;
+                double l1 = (y10y20 * y0y10 - y10d * y0y20) / det;
+                t.l0 = l0; t.l1 = l1; t.rel0 = 1;
+                if (l0 > -1e-3 && l1 > -1e-3 && l0 + l1 < 1.0 + 1e-3) t.interior = 1;
+            }
+        }
+    } else {
+        V3<double> n = cross(y10, y21);
+        t.nx = n.x; t.ny = n.y; t.nz = n.z;
+        double nl = len2(n);
+        if (nl > 1e-10) {
+            double cdn = dot(cen, n);
+            t.cls = fabs(cdn) * cdn / nl;
+            double y10d = len2(y10), y10y21 = dot(y10, y21);
+            double det = y10d * y21d - y10y21 * y10y21;
+            if (det > 0.0) {
+                double y1y10 = dot(y1, y10), y1y21 = dot(y1, y21);
+                double l0 = (y21d * y1y10 - y10y21 * y1y21) / det;
+                double l1 = (y10y21 * y1y10 - y10d * y1y21) / det;
+                t.l0 = l0; t.l1 = l1; t.rel0 = 0;
+                if (l0 > -1e-3 && l1 > -1e-3 && l0 + l1 < 1.0 + 1e-3) t.interior = 1;
+            }
+        }
+    }
+    return ti;
+}
+
+template <typename T>
+__device__ inline void link_tri(EpaState<T>& h, int t1, int e1, int t2, int e2) {
+    h.tris[t1].e[e1].nbT = (short)t2; h.tris[t1].e[e1].nbE = (unsigned char)e2;
+    h.tris[t2].e[e2].nbT = (short)t1; h.tris[t2].e[e2].nbE = (unsigned char)e1;
+}
+
+template <typename T>
+__device__ inline void unlink_tri(EpaState<T>& h, int ti) {  // unlinkTriangle :451-466
+    ETri& t = h.tris[ti];
+    for (int i = 0; i < 3; i++) {
+        if (t.e[i].nbT >= 0) {
+            h.tris[t.e[i].nbT].e[t.e[i].nbE].nbT = -1;
+            t.e[i].nbT = -1;
+        }
+    }
+    if (!t.inQueue) free_tri(h, ti);
+}
+
+template <typename T>
+__device__ inline bool find_edge(EpaState<T>& h, int facing, double px, double py, double pz) {  // findEdge :468-539
+    h.nedges = 0;
+    h.tris[facing].removed = 1;
+    int sp = 0;
+    h.stack[0].tri = (short)facing; h.stack[0].edge = 0; h.stack[0].iter = -1;
+    int nextExpected = -1;
+    for (;;) {
+        EStack& cur = h.stack[sp];
+        if (++cur.iter >= 3) {
+            unlink_tri(h, cur.tri);
+            if (--sp < 0) break;
+        } else {
+            EEdge e = h.tris[cur.tri].e[(cur.edge + cur.iter) % 3];
+            int n = e.nbT;
+            if (n >= 0 && !h.tris[n].removed) {
+                if (tri_facing(h.tris[n], px, py, pz)) {
+                    h.tris[n].removed = 1;
+                    if (sp + 1 >= EPA_MAX_EDGE_LENGTH) return false;
+                    sp++;
+                    h.stack[sp].tri = (short)n; h.stack[sp].edge = e.nbE; h.stack[sp].iter = 0;
+                } else {
+                    if (e.start != nextExpected && nextExpected != -1) return false;
+                    nextExpected = h.tris[n].e[e.nbE].start;
+                    if (h.nedges >= EPA_MAX_EDGE_LENGTH) return false;
+                    h.edges[h.nedges++] = e;
+                }
+            }
+        }
+    }
+    if (h.nedges > 0 && h.edges[0].start != nextExpected) return false;
+    return h.nedges >= 3;
+}
+
+template <typename T>
+__device__ inline bool add_point(EpaState<T>& h, int facing, int idx, double closestDist2) {  // addPoint :541-593
+    V3<double> p = cvt<double>(h.y[idx]);
+    h.nnew = 0;
+    if (!find_edge(h, facing, p.x, p.y, p.z)) return false;
+    int ne = h.nedges;
+    for (int i = 0; i < ne; i++) {
+        int in = (i + 1) % ne;
+        int nt = create_tri(h, h.edges[i].start, h.edges[in].start, idx);
+        if (nt < 0) return false;
+        h.newT[h.nnew++] = (short)nt;
+        ETri& t = h.tris[nt];
+        if ((t.interior && t.cls < closestDist2) || t.cls < 0.0) heap_push(h, nt);
+    }
+    for (int i = 0; i < ne; i++) {
+        int in = (i + 1) % ne;
+        link_tri(h, h.newT[i], 0, h.edges[i].nbT, h.edges[i].nbE);
+        link_tri(h, h.newT[i], 1, h.newT[in], 2);
+    }
+    return true;
+}
+
+template <typename T>
+__device__ inline int epa_add(EpaState<T>& P, const Sup<T>& A, const Sup<T>& B, V3<T> d) {
+    V3<T> p = sup_get(A, d);
+    V3<T> q = sup_get(B, -d);
+    int i = P.n++;
+    P.y[i] = p - q;
+    P.p[i] = p;
+    P.q[i] = q;
+    return i;
+}
+
+// penetrationDepthStepEPA penetration.ts:214-614; the state lives in device memory.
+template <typename T>
+__device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const Sup<T>& B, T tol, const Simplex<T>& s,
+                                   EpaState<T>& H) {
+    H.n = s.size;
+    for (int i = 0; i < s.size; i++) { H.y[i] = s.pt[i].y; H.p[i] = s.pt[i].p; H.q[i] = s.pt[i].q; }
+    if (H.n == 1) {
+        H.n--;
+        epa_add(H, A, B, mk3<T>(0, 1, 0));
+        epa_add(H, A, B, mk3<T>(-1, -1, -1));
+        epa_add(H, A, B, mk3<T>(1, -1, -1));
+        epa_add(H, A, B, mk3<T>(0, -1, 1));
+    } else if (H.n == 2) {
+        V3<T> axis = H.y[1] - H.y[0];
+        T al = vlen(axis);
+        V3<T> an = mk3<T>(axis.x / al, axis.y / al, axis.z / al);
+        V3<T> d1;
+        if (fabs(an.x) > fabs(an.y)) { T pl = sqrt(an.x * an.x + an.z * an.z); d1 = mk3<T>(an.z / pl, 0, -an.x / pl); }
+        else { T pl = sqrt(an.y * an.y + an.z * an.z); d1 = mk3<T>(0, an.z / pl, -an.y / pl); }
+        const T c = T(-0.4999999999999998), sn = T(0.8660254037844387), omc = T(1) - c;  // cos/sin of 120 degrees
+        T r00 = c + an.x * an.x * omc, r01 = an.x * an.y * omc - an.z * sn, r02 = an.x * an.z * omc + an.y * sn;
+        T r10 = an.y * an.x * omc + an.z * sn, r11 = c + an.y * an.y * omc, r12 = an.y * an.z * omc - an.x * sn;
+        T r20 = an.z * an.x * omc - an.y * sn, r21 = an.z * an.y * omc + an.x * sn, r22 = c + an.z * an.z * omc;
+        V3<T> d2 = mk3<T>(r00 * d1.x + r01 * d1.y + r02 * d1.z, r10 * d1.x + r11 * d1.y + r12 * d1.z, r20 * d1.x + r21 * d1.y + r22 * d1.z);
+        V3<T> d3 = mk3<T>(r00 * d2.x + r01 * d2.y + r02 * d2.z, r10 * d2.x + r11 * d2.y + r12 * d2.z, r20 * d2.x + r21 * d2.y + r22 * d2.z);
+        epa_add(H, A, B, d1);
+        epa_add(H, A, B, d2);
+        epa_add(H, A, B, d3);
+    }
+    // initialize :347-371
+    H.nfree = EPA_MAX_TRIANGLES;
+    for (int i = 0; i < EPA_MAX_TRIANGLES; i++) H.freeList[i] = (short)(EPA_MAX_TRIANGLES - 1 - i);
+    H.nq = 0;
+    {
+        int t1 = create_tri(H, 0, 1, 2);
+        int t2 = create_tri(H, 0, 2, 1);
+        link_tri(H, t1, 0, t2, 2);
+        link_tri(H, t1, 1, t2, 1);
+        link_tri(H, t1, 2, t2, 0);
+        heap_push(H, t1);
+        heap_push(H, t2);
+    }
+    for (int i = 3; i < H.n; i++) {
+        // findFacingTriangle :420-445
+        int best = -1;
+        double bestD = 0.0;
+        V3<double> p = cvt<double>(H.y[i]);
+        for (int k = 0; k < H.nq; k++) {
+            const ETri& t = H.tris[H.queue[k]];
+            if (t.removed) continue;
+            double dt = t.nx * (p.x - t.cx) + t.ny * (p.y - t.cy) + t.nz * (p.z - t.cz);
+            if (dt > 0.0) {
+                double d2 = dt * dt / (t.nx * t.nx + t.ny * t.ny + t.nz * t.nz);
+                if (d2 > bestD) { best = H.queue[k]; bestD = d2; }
+            }
+        }
+        if (best >= 0) {
+            if (!add_point(H, best, i, 1.7976931348623157e308)) { out.status = PD_NOT_COLLIDING; return false; }
+        }
+    }
+    for (;;) {
+        if (H.nq == 0) { out.status = PD_NOT_COLLIDING; return false; }
+        int t = H.queue[0];
+        if (H.tris[t].removed) {
+            heap_pop(H);
+            if (H.nq == 0) { out.status = PD_NOT_COLLIDING; return false; }
+            free_tri(H, t);
+            continue;
+        }
+        if (H.tris[t].cls >= 0.0) break;
+        heap_pop(H);
+        V3<T> nrm = mk3<T>(T(H.tris[t].nx), T(H.tris[t].ny), T(H.tris[t].nz));
+        if (H.n >= EPA_POINT_CAP) { out.status = PD_NOT_COLLIDING; return false; }
+        int ni = epa_add(H, A, B, nrm);
+        V3<double> w = cvt<double>(H.y[ni]);
+        if (!tri_facing(H.tris[t], w.x, w.y, w.z) || !add_point(H, t, ni, 1.7976931348623157e308)) {
+            out.status = PD_NOT_COLLIDING;
+            return false;
+        }
+        free_tri(H, t);
+        if (H.nq == 0 || H.n >= EPA_MAX_POINTS_TO_INCLUDE_ORIGIN_IN_HULL) { out.status = PD_NOT_COLLIDING; return false; }
+    }
+    double closestDist2 = 1.7976931348623157e308;
+    int last = -1;
+    bool flip = false;
+    do {
+        int t = heap_pop(H);
+        if (t < 0) { out.status = PD_NOT_COLLIDING; return false; }
+        ETri& TT = H.tris[t];
+        if (TT.removed) { free_tri(H, t); continue; }
+        if (TT.cls >= closestDist2) break;
+        if (last >= 0) free_tri(H, last);
+        last = t;
+        V3<T> nrm = mk3<T>(T(TT.nx), T(TT.ny), T(TT.nz));
+        if (H.n >= EPA_POINT_CAP) break;
+        int ni = epa_add(H, A, B, nrm);
+        V3<double> w = cvt<double>(H.y[ni]);
+        double dt = TT.nx * w.x + TT.ny * w.y + TT.nz * w.z;
+        if (dt < 0.0) { out.status = PD_NOT_COLLIDING; return false; }
+        double d2 = dt * dt / (TT.nx * TT.nx + TT.ny * TT.ny + TT.nz * TT.nz);
+        if (d2 - TT.cls < TT.cls * double(tol)) break;
+        closestDist2 = d2 < closestDist2 ? d2 : closestDist2;
+        if (!tri_facing(TT, w.x, w.y, w.z)) break;
+        if (!add_point(H, t, ni, closestDist2)) break;
+        bool defect = false;
+        for (int i = 0; i < H.nnew; i++) {
+            const ETri& nt = H.tris[H.newT[i]];
+            if (nt.nx * nt.cx + nt.ny * nt.cy + nt.nz * nt.cz < 0.0) { defect = true; break; }
+        }
+        if (defect) {
+            V3<T> p2 = sup_get(A, -nrm);
+            V3<T> q2 = sup_get(B, nrm);
+            V3<double> w2 = cvt<double>(p2 - q2);
+            double dot2 = -(TT.nx * w2.x + TT.ny * w2.y + TT.nz * w2.z);
+            if (dot2 < dt) flip = true;
+            break;
+        }
+    } while (H.nq > 0 && H.n < EPA_MAX_POINTS);
+    if (last < 0) { out.status = PD_NOT_COLLIDING; return false; }
+    const ETri& L = H.tris[last];
+    double nl2 = L.nx * L.nx + L.ny * L.ny + L.nz * L.nz;
+    double cdn = L.cx * L.nx + L.cy * L.ny + L.cz * L.nz;
+    double sc = cdn / nl2;
+    V3<double> pn = mk3<double>(L.nx * sc, L.ny * sc, L.nz * sc);
+    if (len2(pn) < 1e-10) { out.status = PD_NOT_COLLIDING; return false; }
+    if (flip) pn = -pn;
+    int i0 = L.e[0].start, i1 = L.e[1].start, i2 = L.e[2].start;
+    T l0 = T(L.l0), l1 = T(L.l1);
+    if (L.rel0) {
+        out.pointA = H.p[i0] + (H.p[i1] - H.p[i0]) * l0 + (H.p[i2] - H.p[i0]) * l1;
+        out.pointB = H.q[i0] + (H.q[i1] - H.q[i0]) * l0 + (H.q[i2] - H.q[i0]) * l1;
+    } else {
+        out.pointA = H.p[i1] + (H.p[i0] - H.p[i1]) * l0 + (H.p[i2] - H.p[i1]) * l1;
+        out.pointB = H.q[i1] + (H.q[i0] - H.q[i1]) * l0 + (H.q[i2] - H.q[i1]) * l1;
+    }
+    out.axis = cvt<T>(pn);
+    out.status = PD_COLLIDING;
+    return true;
+}
+
+// ------------------------------------------------------------------ faces
+#define CC_MAX_FACE 32
+template <typename T> struct Face { int n; V3<T> v[CC_MAX_FACE]; };
+
+template <typename T>
+__device__ inline void face_transform(Face<T>& f, V3<T> pos, Q4<T> q) {  // transformFace face.ts:30-84
+    M3<T> R = m3_from_quat(q);
+    for (int i = 0; i < f.n; i++) f.v[i] = mul(R, f.v[i]) + pos;
+}
+
+template <typename T>
+__device__ inline void supporting_face(Face<T>& f, const ShapeTable<T>& st, int si, V3<T> d, V3<T> pos, Q4<T> q) {
+    const ShapeD<T>& s = st.shapes[si];
+    if (s.type == SHAPE_SPHERE) { f.n = 0; return; }  // sphere.ts:143-146
+    if (s.type == SHAPE_BOX) {  // box.ts:129-332
+        T hx = s.he[0], hy = s.he[1], hz = s.he[2];
+        T ax = fabs(d.x), ay = fabs(d.y), az = fabs(d.z);
+        f.n = 4;
+        if (ax >= ay && ax >= az) {
+            if (d.x < T(0)) { f.v[0] = mk3<T>(hx, -hy, -hz); f.v[1] = mk3<T>(hx, hy, -hz); f.v[2] = mk3<T>(hx, hy, hz); f.v[3] = mk3<T>(hx, -hy, hz); }
+            else { f.v[0] = mk3<T>(-hx, -hy, -hz); f.v[1] = mk3<T>(-hx, -hy, hz); f.v[2] = mk3<T>(-hx, hy, hz); f.v[3] = mk3<T>(-hx, hy, -hz); }
+        } else if (ay >= ax && ay >= az) {
+            if (d.y < T(0)) { f.v[0] = mk3<T>(-hx, hy, -hz); f.v[1] = mk3<T>(-hx, hy, hz); f.v[2] = mk3<T>(hx, hy, hz); f.v[3] = mk3<T>(hx, hy, -hz); }
+            else { f.v[0] = mk3<T>(-hx, -hy, -hz); f.v[1] = mk3<T>(hx, -hy, -hz); f.v[2] = mk3<T>(hx, -hy, hz); f.v[3] = mk3<T>(-hx, -hy, hz); }
+        } else {
+            if (d.z < T(0)) { f.v[0] = mk3<T>(-hx, -hy, hz); f.v[1] = mk3<T>(hx, -hy, hz); f.v[2] = mk3<T>(hx, hy, hz); f.v[3] = mk3<T>(-hx, hy, hz); }
+            else { f.v[0] = mk3<T>(-hx, -hy, -hz); f.v[1] = mk3<T>(-hx, hy, -hz); f.v[2] = mk3<T>(hx, hy, -hz); f.v[3] = mk3<T>(hx, -hy, -hz); }
+        }
+    } else if (s.type == SHAPE_CAPSULE) {  // capsule.ts:193-253
+        T len = sqrt(d.x * d.x + d.z * d.z);
+        if (len == T(0)) { f.n = 0; return; }
+        T sx = (s.radius / len) * d.x, sz = (s.radius / len) * d.z;
+        V3<T> top = mk3<T>(-sx, s.hh, -sz), bot = mk3<T>(-sx, -s.hh, -sz);
+        T pt = dot(top, d), pb = dot(bot, d);
+        if (fabs(pt - pb) < T(0.02) * vlen(d)) { f.n = 2; f.v[0] = top; f.v[1] = bot; }
+        else { f.n = 0; return; }
+    } else {  // convex-hull.ts:479-548
+        int f0 = s.hff;
+        V3<T> n0 = mk3<T>(st.pn[3 * f0], st.pn[3 * f0 + 1], st.pn[3 * f0 + 2]);
+        T bestDot = dot(n0, d) / vlen(n0);
+        int best = 0;
+        for (int i = 1; i < s.hnf; i++) {
+            V3<T> n = mk3<T>(st.pn[3 * (f0 + i)], st.pn[3 * (f0 + i) + 1], st.pn[3 * (f0 + i) + 2]);
+            T dt = dot(n, d) / vlen(n);
+            if (dt < bestDot) { bestDot = dt; best = i; }
+        }
+        int first = st.ffv[f0 + best], nv = st.fnv[f0 + best];
+        int delta = (nv + 16 - 1) / 16;
+        f.n = 0;
+        for (int i = 0; i < nv && f.n < CC_MAX_FACE; i += delta) {
+            int vi = st.fvi[first + i];
+            const T* pp = st.pts + 3 * (s.hfp + vi);
+            f.v[f.n++] = mk3<T>(pp[0], pp[1], pp[2]);
+        }
+    }
+    face_transform(f, pos, q);
+}
+
+// ---------------------------------------------------------------- clipping (clip.ts)
+template <typename T>
+__device__ inline void clip_poly_vs_plane(Face<T>& out, const Face<T>& poly, V3<T> origin, V3<T> normal) {  // :13-92
+    out.n = 0;
+    if (poly.n < 2) return;
+    V3<T> e1 = poly.v[poly.n - 1];
+    T prevNum = dot(origin - e1, normal);
+    bool prevIn = prevNum < T(0);
+    for (int j = 0; j < poly.n; j++) {
+        V3<T> e2 = poly.v[j];
+        T num = dot(origin - e2, normal);
+        bool curIn = num < T(0);
+        if (curIn != prevIn) {
+            V3<T> e12 = e2 - e1;
+            T den = dot(e12, normal);
+            if (den != T(0)) {
+                T t = prevNum / den;
+                if (out.n < CC_MAX_FACE) out.v[out.n++] = e1 + e12 * t;
+            } else curIn = prevIn;
+        }
+        if (curIn && out.n < CC_MAX_FACE) out.v[out.n++] = e2;
+        prevNum = num;
+        prevIn = curIn;
+        e1 = e2;
+    }
+}
+
+template <typename T>
+__device__ inline void clip_poly_vs_poly(Face<T>& out, const Face<T>& toClip, const Face<T>& clip, V3<T> normal,
+                                         Face<T>* tmp) {  // :108-175
+    out.n = 0;
+    if (toClip.n < 2 || clip.n < 3) return;
+    int tmpIdx = 0;
+    for (int i = 0; i < clip.n; i++) {
+        V3<T> c1 = clip.v[i], c2 = clip.v[(i + 1) % clip.n];
+        V3<T> cn = cross(normal, c2 - c1);
+        const Face<T>& src = (i == 0) ? toClip : tmp[tmpIdx];
+        tmpIdx ^= 1;
+        Face<T>& dst = (i == clip.n - 1) ? out : tmp[tmpIdx];
+        clip_poly_vs_plane(dst, src, c1, cn);
+        if (dst.n < 3) { out.n = 0; break; }
+    }
+}
+
+template <typename T>
+__device__ inline void clip_poly_vs_edge(Face<T>& out, const Face<T>& poly, V3<T> ev1, V3<T> ev2, V3<T> clipNormal) {  // :190-363
+    out.n = 0;
+    if (poly.n < 3) return;
+    V3<T> en = cross(clipNormal, ev2 - ev1);
+    V3<T> p0 = poly.v[0], p1 = poly.v[1], p2 = poly.v[2];
+    V3<T> pn = cross(p2 - p0, p1 - p0);
+    T pl2 = len2(pn);
+    V3<T> v1 = ev1 + pn * (dot(p0 - ev1, pn) / pl2);
+    V3<T> v2 = ev2 + pn * (dot(p0 - ev2, pn) / pl2);
+    V3<T> v12 = v2 - v1;
+    T v12l = len2(v12);
+    V3<T> e1 = poly.v[poly.n - 1];
+    T prevNum = dot(ev1 - e1, en);
+    bool prevIn = prevNum < T(0);
+    for (int j = 0; j < poly.n; j++) {
+        V3<T> e2 = poly.v[j];
+        T num = dot(ev1 - e2, en);
+        bool curIn = num < T(0);
+        if (curIn != prevIn) {
+            V3<T> e12 = e2 - e1;
+            T den = dot(e12, en);
+            V3<T> cl = (den != T(0)) ? e1 + e12 * (prevNum / den) : e1;
+            T pr = dot(cl - v1, v12);
+            V3<T> o = pr < T(0) ? v1 : (pr > v12l ? v2 : cl);
+            if (out.n < CC_MAX_FACE) out.v[out.n++] = o;
+        }
+        prevNum = num;
+        prevIn = curIn;
+        e1 = e2;
+    }
+}
+
+// ------------------------------------------------------------- manifold
+#define CC_MANIFOLD_CAP 32
+template <typename T>
+struct Manifold {
+    V3<T> base, normal;
+    int n;
+    V3<T> relA[CC_MANIFOLD_CAP], relB[CC_MANIFOLD_CAP];
+};
+
+// manifoldBetweenTwoFaces manifold.ts:345-446
+template <typename T>
+__device__ inline void manifold_between_two_faces(Manifold<T>& out, V3<T> p1, V3<T> p2, V3<T> axis, T maxDist,
+                                                  const Face<T>& f1, const Face<T>& f2, Face<T>* scratch3) {
+    out.n = 0;
+    if (f1.n < 2 || f2.n < 3) {
+        out.relA[0] = p1 - out.base; out.relB[0] = p2 - out.base; out.n = 1;
+        return;
+    }
+    Face<T>& clipped = scratch3[2];
+    clipped.n = 0;
+    if (f1.n >= 3) clip_poly_vs_poly(clipped, f2, f1, axis, scratch3);
+    else if (f1.n == 2) clip_poly_vs_edge(clipped, f2, f1.v[0], f1.v[1], axis);
+    V3<T> origin = f1.v[0];
+    V3<T> first = f1.v[1] - origin;
+    V3<T> pnrm = (f1.n >= 3) ? cross(first, f1.v[2] - origin) : cross(cross(first, axis), first);
+    T apn = dot(axis, pnrm);
+    if (apn != T(0)) {
+        T al = vlen(axis);
+        for (int i = 0; i < clipped.n; i++) {
+            V3<T> q2 = clipped.v[i];
+            T dist = dot(q2 - origin, pnrm) / apn;
+            if (dist * al < maxDist) {
+                V3<T> q1 = q2 + axis * (-dist);
+                out.relA[out.n] = q1 - out.base;
+                out.relB[out.n] = q2 - out.base;
+                out.n++;
+                if (out.n >= CC_MANIFOLD_CAP) break;
+            }
+        }
+    }
+    if (out.n == 0) { out.relA[0] = p1 - out.base; out.relB[0] = p2 - out.base; out.n = 1; }
+}
+
+// pruneContactPoints manifold.ts:192-313
+template <typename T>
+__device__ inline void prune(Manifold<T>& m, V3<T> axis) {
+    if (m.n <= 4) return;
+    const T MIN_D2 = T(1.0e-6);
+    T pd2[CC_MANIFOLD_CAP];
+    V3<T> proj[CC_MANIFOLD_CAP];
+    for (int i = 0; i < m.n; i++) {
+        V3<T> v1 = m.relA[i], v2 = m.relB[i];
+        T dp = dot(v1, axis);
+        proj[i] = v1 + axis * (-dp);
+        pd2[i] = tmax<T>(MIN_D2, len2(v2 - v1));
+    }
+    int p1 = 0;
+    T best = tmax<T>(MIN_D2, len2(proj[0])) * pd2[0];
+    for (int i = 1; i < m.n; i++) {
+        T me = tmax<T>(MIN_D2, len2(proj[i])) * pd2[i];
+        if (me > best) { best = me; p1 = i; }
+    }
+    int p2 = -1;
+    best = -INFINITY;
+    for (int i = 0; i < m.n; i++) {
+        if (i == p1) continue;
+        T me = tmax<T>(MIN_D2, len2(proj[i] - proj[p1])) * pd2[i];
+        if (me > best) { best = me; p2 = i; }
+    }
+    int p3 = -1, p4 = -1;
+    T minD = T(0), maxD = T(0);
+    V3<T> perp = cross(proj[p2] - proj[p1], axis);
+    for (int i = 0; i < m.n; i++) {
+        if (i == p1 || i == p2) continue;
+        T d = dot(proj[i] - proj[p1], perp);
+        if (d < minD) { minD = d; p3 = i; }
+        else if (d > maxD) { maxD = d; p4 = i; }
+    }
+    int order[4] = {p1, p3, p2, p4};
+    V3<T> ra[4], rb[4];
+    int n = 0;
+    for (int k = 0; k < 4; k++)
+        if (order[k] != -1) { ra[n] = m.relA[order[k]]; rb[n] = m.relB[order[k]]; n++; }
+    for (int k = 0; k < n; k++) { m.relA[k] = ra[k]; m.relB[k] = rb[k]; }
+    m.n = n;
+}
+
+// -------------------------------------------------------------- hits
+template <typename T>
+struct Hit {
+    V3<T> pointA, pointB, axis;
+    T penetration;
+};
+
+// The GJK half of collideConvexVsConvexLocal (convex.ts:359-437); returns the status
+// and fills the local-space result (pd) or the simplex needed by EPA.
+template <typename T>
+struct ConvexCtx {
+    Q4<T> rel;
+    V3<T> relPos;
+    T maxSepUse;
+};
+
+template <typename T>
+__device__ inline void convex_setup(ConvexCtx<T>& c, V3<T> pa, Q4<T> qa, V3<T> pb, Q4<T> qb, T maxSep) {
+    Q4<T> ia = qconj(qa);
+    c.rel.x = ia.w * qb.x + ia.x * qb.w + ia.y * qb.z - ia.z * qb.y;
+    c.rel.y = ia.w * qb.y - ia.x * qb.z + ia.y * qb.w + ia.z * qb.x;
+    c.rel.z = ia.w * qb.z + ia.x * qb.y - ia.y * qb.x + ia.z * qb.w;
+    c.rel.w = ia.w * qb.w - ia.x * qb.x - ia.y * qb.y - ia.z * qb.z;
+    c.relPos = qrot2(ia, pb - pa);
+    c.maxSepUse = maxSep;
+}
+
+}  // namespace cc

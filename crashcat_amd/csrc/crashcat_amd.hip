@@ -1,0 +1,815 @@
+// crashcat_amd — host side of the C ABI (include/crashcat_amd.h) and the kernel
+// instantiations for FP32 (batched path) and FP64 (plain path).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/crashcat_amd.h"
+#include "cc_solver.h"
+
+using namespace cc;
+
+// ------------------------------------------------------------------ scans
+__global__ void __launch_bounds__(1024) cc::k_scan_blocks(const int* in, int* out, int* blockSums, int n) {
+    __shared__ int sh[1024 / 64 + 1];
+    int base = blockIdx.x * 4096 + threadIdx.x * 4;
+    int v[4];
+    int s = 0;
+    for (int k = 0; k < 4; k++) {
+        int i = base + k;
+        v[k] = i < n ? in[i] : 0;
+        s += v[k];
+    }
+    int tot;
+    int ex = block_excl_scan<1024>(s, sh, tot);
+    for (int k = 0; k < 4; k++) {
+        int i = base + k;
+        if (i < n) out[i] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) cc::k_scan_sums(int* blockSums, int nb, int* total) {
+    __shared__ int sh[1024 / 64 + 1];
+    int base = threadIdx.x * 4;
+    int v[4];
+    int s = 0;
+    for (int k = 0; k < 4; k++) {
+        int i = base + k;
+        v[k] = i < nb ? blockSums[i] : 0;
+        s += v[k];
+    }
+    int tot;
+    int ex = block_excl_scan<1024>(s, sh, tot);
+    for (int k = 0; k < 4; k++) {
+        int i = base + k;
+        if (i < nb) blockSums[i] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0) *total = tot;
+}
+
+__global__ void cc::k_scan_add(int* out, const int* blockSums, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += blockSums[i / 4096];
+}
+
+__global__ void cc::k_hash_clear(unsigned long long* keys, int cap) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) keys[i] = CC_EMPTY_KEY;
+}
+
+__global__ void k_fill_i32(int* p, int v, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <typename T>
+__global__ void k_hash_clear_cur(Dev<T> d) {
+    unsigned long long* k = d.hkeys[*d.parity];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.hashCap; i += gridDim.x * blockDim.x) k[i] = CC_EMPTY_KEY;
+}
+
+template <typename T>
+__global__ void k_set_pair_count(Dev<T> d) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int n = d.bpOff[d.N];
+        d.counters[CNT_PAIRS] = n;
+        d.counters[CNT_EPA] = 0;
+        if (n > d.pairCap) atomicOr(&d.counters[CNT_OVERFLOW], 1);
+    }
+}
+
+template <typename T>
+__global__ void k_count_epa(Dev<T> d) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) d.counters[CNT_EPA_TOTAL] += d.counters[CNT_EPA];
+}
+
+// body initialisation at upload: COM, AABB, sleep test spheres (rigid-body.ts:390-434)
+template <typename T>
+__global__ void k_init_bodies(Dev<T> d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N) return;
+    set_transform(d.B, d.st, i);
+    reset_sleep_timer(d.B, d.st, i);
+    d.B.wake[i] = 0;
+    d.B.label[i] = -1;
+}
+
+template <typename T>
+__global__ void k_refresh_bodies(Dev<T> d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N) return;
+    set_transform(d.B, d.st, i);
+}
+
+// ---------------------------------------------------------------- context
+struct Alloc {
+    std::vector<void*> ptrs;
+    hipError_t err = hipSuccess;
+    template <typename X>
+    X* get(size_t n) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&p, n * sizeof(X));
+        if (e != hipSuccess) { err = e; return nullptr; }
+        ptrs.push_back(p);
+        return (X*)p;
+    }
+    void release() {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+struct Impl {
+    virtual ~Impl() {}
+};
+
+template <typename T>
+struct CtxT : Impl {
+    Dev<T> d;
+    Alloc mem;
+    int* scanSums = nullptr;
+    int N = 0, W = 0, NPC = 0, numShapes = 0;
+    std::vector<int> hostWStart;
+    hipEvent_t ev[12];
+    bool events = false;
+    int solverBS = 256;
+    ~CtxT() {
+        mem.release();
+        if (events)
+            for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+struct cc_ctx {
+    cc_config cfg;
+    hipStream_t stream = nullptr;
+    std::string err;
+    Impl* impl = nullptr;
+    int steps = 0;
+    double phaseMs[12] = {0};
+    int nPhase = 0;
+};
+
+static int fail(cc_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+#define HIPCHK(ctx, x)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(ctx, CC_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" int cc_create(cc_ctx** out, const cc_config* cfg) {
+    if (!out) return CC_E_ARG;
+    cc_ctx* c = new cc_ctx();
+    cc_config def = {CC_PRECISION_FP32, 0, 16, 8, 0, 0};
+    c->cfg = cfg ? *cfg : def;
+    if (c->cfg.max_pairs_per_body <= 0) c->cfg.max_pairs_per_body = 16;
+    if (c->cfg.max_contacts_per_body <= 0) c->cfg.max_contacts_per_body = 8;
+    if (c->cfg.precision != CC_PRECISION_FP32 && c->cfg.precision != CC_PRECISION_FP64) {
+        delete c;
+        return CC_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        delete c;
+        return CC_E_HIP;
+    }
+    if (hipSetDevice(c->cfg.device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return CC_E_HIP;
+    }
+    *out = c;
+    return CC_OK;
+}
+
+extern "C" int cc_destroy(cc_ctx* c) {
+    if (!c) return CC_OK;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c->impl;
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CC_OK;
+}
+
+extern "C" const char* cc_last_error(const cc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+static int next_pow2(long long x) {
+    long long p = 1;
+    while (p < x) p <<= 1;
+    return (int)p;
+}
+
+template <typename T>
+static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* woff, const cc_body_desc* bodies, int32_t nb,
+                  const cc_shape_desc* shapes, int32_t ns, const cc_hull_data* hulls, const cc_point_constraint_desc* pcs,
+                  int32_t npc) {
+    auto* X = new CtxT<T>();
+    delete c->impl;
+    c->impl = X;
+    Dev<T>& d = X->d;
+    memset(&d, 0, sizeof(d));
+    Alloc& A = X->mem;
+    X->N = nb;
+    X->W = nw;
+    X->NPC = npc;
+    // settings
+    for (int k = 0; k < 3; k++) d.s.gravity[k] = T(s->gravity[k]);
+    d.s.spec = T(s->speculative_contact_distance);
+    d.s.manifoldTol = T(s->manifold_tolerance);
+    d.s.colTol = T(s->collision_tolerance);
+    d.s.penTol = T(s->penetration_tolerance);
+    d.s.preserveDist2 = T(s->contact_point_preserve_lambda_max_dist_sq);
+    d.s.minVelRest = T(s->min_velocity_for_restitution);
+    d.s.slop = T(s->penetration_slop);
+    d.s.beta = T(s->baumgarte_factor);
+    d.s.maxPen = T(s->max_penetration_distance);
+    d.s.tSleep = T(s->time_before_sleep);
+    d.s.sleepThresh = T(s->point_velocity_sleep_threshold);
+    d.s.velIters = s->velocity_iterations;
+    d.s.posIters = s->position_iterations;
+    d.s.gravityEnabled = s->gravity_enabled;
+    d.s.warmStarting = s->warm_starting;
+    d.s.allowSleeping = s->allow_sleeping;
+    memcpy(d.s.layerPairs, s->object_layer_pairs, 64);
+    // shapes
+    std::vector<ShapeD<T>> sh(std::max(ns, 1));
+    for (int i = 0; i < ns; i++) {
+        const cc_shape_desc& q = shapes[i];
+        if (q.type < 0 || q.type > 3) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
+        ShapeD<T>& o = sh[i];
+        o.type = q.type; o.hfp = q.hull_first_point; o.hnp = q.hull_num_points; o.hff = q.hull_first_face;
+        o.hnf = q.hull_num_faces; o.pad = 0;
+        o.radius = T(q.radius); o.hh = T(q.half_height); o.cr = T(q.convex_radius);
+        for (int k = 0; k < 3; k++) {
+            o.he[k] = T(q.half_extents[k]); o.com[k] = T(q.center_of_mass[k]);
+            o.amin[k] = T(q.aabb_min[k]); o.amax[k] = T(q.aabb_max[k]);
+        }
+    }
+    int P = hulls ? hulls->total_points : 0, F = hulls ? hulls->total_faces : 0, FV = hulls ? hulls->total_face_vertices : 0;
+    for (int i = 0; i < ns; i++)
+        if (shapes[i].type == CC_SHAPE_CONVEX_HULL &&
+            (shapes[i].hull_first_point + shapes[i].hull_num_points > P || shapes[i].hull_first_face + shapes[i].hull_num_faces > F ||
+             shapes[i].hull_num_points > 255))
+            return fail(c, CC_E_ARG, "hull data out of range");
+    std::vector<T> hp(3 * std::max(P, 1)), hs(3 * std::max(P, 1)), hn(3 * std::max(F, 1));
+    for (int i = 0; i < 3 * P; i++) { hp[i] = T(hulls->points[i]); hs[i] = T(hulls->shrunk_points[i]); }
+    for (int i = 0; i < 3 * F; i++) hn[i] = T(hulls->plane_normals[i]);
+    ShapeD<T>* dShapes = A.get<ShapeD<T>>(sh.size());
+    T* dPts = A.get<T>(hp.size());
+    T* dShr = A.get<T>(hs.size());
+    T* dPn = A.get<T>(hn.size());
+    int* dFfv = A.get<int>(std::max(F, 1));
+    int* dFnv = A.get<int>(std::max(F, 1));
+    int* dFvi = A.get<int>(std::max(FV, 1));
+    if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (shapes)");
+    HIPCHK(c, hipMemcpy(dShapes, sh.data(), sh.size() * sizeof(ShapeD<T>), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dPts, hp.data(), hp.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dShr, hs.data(), hs.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dPn, hn.data(), hn.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (F > 0) {
+        HIPCHK(c, hipMemcpy(dFfv, hulls->face_first_vertex, F * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dFnv, hulls->face_num_vertices, F * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (FV > 0) HIPCHK(c, hipMemcpy(dFvi, hulls->face_vertex_indices, FV * sizeof(int), hipMemcpyHostToDevice));
+    d.st = ShapeTable<T>{dShapes, dPts, dShr, dFfv, dFnv, dFvi, dPn};
+    // bodies
+    X->hostWStart.assign(woff, woff + nw + 1);
+    std::vector<int> worldOf(std::max(nb, 1));
+    for (int w = 0; w < nw; w++)
+        for (int i = woff[w]; i < woff[w + 1]; i++) worldOf[i] = w;
+    std::vector<V3<T>> pos(nb), com(nb), v(nb), wv(nb), invI(nb);
+    std::vector<Q4<T>> q(nb), ir(nb);
+    std::vector<T> im(nb), fr(nb), re(nb), gf(nb), ld(nb), ad(nb), ml(nb), ma(nb);
+    std::vector<int> shp(nb), flags(nb), slp(nb);
+    std::vector<unsigned> grp(nb), msk(nb);
+    for (int i = 0; i < nb; i++) {
+        const cc_body_desc& b = bodies[i];
+        if (b.shape < 0 || b.shape >= ns) return fail(c, CC_E_ARG, "body shape index out of range");
+        if (b.motion_type < 0 || b.motion_type > 2) return fail(c, CC_E_ARG, "invalid motion type");
+        if (b.object_layer < 0 || b.object_layer >= CC_MAX_OBJECT_LAYERS) return fail(c, CC_E_ARG, "invalid object layer");
+        pos[i] = mk3<T>(T(b.position[0]), T(b.position[1]), T(b.position[2]));
+        q[i].x = T(b.quaternion[0]); q[i].y = T(b.quaternion[1]); q[i].z = T(b.quaternion[2]); q[i].w = T(b.quaternion[3]);
+        v[i] = mk3<T>(T(b.linear_velocity[0]), T(b.linear_velocity[1]), T(b.linear_velocity[2]));
+        wv[i] = mk3<T>(T(b.angular_velocity[0]), T(b.angular_velocity[1]), T(b.angular_velocity[2]));
+        invI[i] = mk3<T>(T(b.inv_inertia_diagonal[0]), T(b.inv_inertia_diagonal[1]), T(b.inv_inertia_diagonal[2]));
+        ir[i].x = T(b.inertia_rotation[0]); ir[i].y = T(b.inertia_rotation[1]); ir[i].z = T(b.inertia_rotation[2]);
+        ir[i].w = T(b.inertia_rotation[3]);
+        im[i] = T(b.inv_mass); fr[i] = T(b.friction); re[i] = T(b.restitution); gf[i] = T(b.gravity_factor);
+        ld[i] = T(b.linear_damping); ad[i] = T(b.angular_damping); ml[i] = T(b.max_linear_velocity);
+        ma[i] = T(b.max_angular_velocity);
+        shp[i] = b.shape;
+        flags[i] = (b.motion_type & 3) | ((b.allowed_dofs & 63) << 2) | ((b.allow_sleeping ? 1 : 0) << 8) |
+                   ((b.friction_combine & 7) << 9) | ((b.restitution_combine & 7) << 12) | ((b.object_layer & 255) << 16);
+        slp[i] = b.motion_type == CC_MOTION_STATIC ? 0 : (b.sleeping ? 1 : 0);
+        grp[i] = b.collision_groups;
+        msk[i] = b.collision_mask;
+        com[i] = pos[i];
+    }
+    Bodies<T>& B = d.B;
+#define UP(field, vec, type)                                                                    \
+    B.field = A.get<type>(std::max<size_t>(vec.size(), 1));                                     \
+    if (!B.field) return fail(c, CC_E_HIP, "hipMalloc failed (bodies)");                        \
+    if (!vec.empty()) HIPCHK(c, hipMemcpy(B.field, vec.data(), vec.size() * sizeof(type), hipMemcpyHostToDevice));
+    UP(pos, pos, V3<T>) UP(q, q, Q4<T>) UP(com, com, V3<T>) UP(v, v, V3<T>) UP(w, wv, V3<T>) UP(invMass, im, T)
+    UP(invI, invI, V3<T>) UP(iRot, ir, Q4<T>) UP(friction, fr, T) UP(restitution, re, T) UP(gravityFactor, gf, T)
+    UP(linDamp, ld, T) UP(angDamp, ad, T) UP(maxLinV, ml, T) UP(maxAngV, ma, T) UP(shape, shp, int) UP(flags, flags, int)
+    UP(groups, grp, unsigned) UP(mask, msk, unsigned) UP(world, worldOf, int) UP(sleeping, slp, int)
+#undef UP
+    B.amin = A.get<V3<T>>(nb);
+    B.amax = A.get<V3<T>>(nb);
+    B.sc = A.get<V3<T>>(3 * (size_t)nb);
+    B.sr = A.get<T>(3 * (size_t)nb);
+    B.stimer = A.get<T>(nb);
+    B.label = A.get<int>(nb);
+    B.wake = A.get<int>(nb);
+    d.N = nb;
+    d.W = nw;
+    int* dW = A.get<int>(nw + 1);
+    if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed");
+    HIPCHK(c, hipMemcpy(dW, woff, (nw + 1) * sizeof(int), hipMemcpyHostToDevice));
+    d.wStart = dW;
+    // point constraints, grouped by world in index order
+    std::vector<std::pair<int, int>> pcw;
+    for (int i = 0; i < npc; i++) {
+        int a = pcs[i].body_a, b = pcs[i].body_b;
+        if (a < 0 || a >= nb || b < 0 || b >= nb || worldOf[a] != worldOf[b] || a == b)
+            return fail(c, CC_E_ARG, "point constraint bodies invalid or in different worlds");
+        pcw.push_back({worldOf[a], i});
+    }
+    std::stable_sort(pcw.begin(), pcw.end(), [](auto& x, auto& y) { return x.first < y.first; });
+    std::vector<PointCon<T>> hpc(std::max(npc, 1));
+    std::vector<int> pcStart(nw + 1, 0);
+    for (int k = 0; k < npc; k++) {
+        const cc_point_constraint_desc& p = pcs[pcw[k].second];
+        PointCon<T>& o = hpc[k];
+        memset(&o, 0, sizeof(o));
+        o.a = p.body_a; o.b = p.body_b; o.enabled = p.enabled;
+        o.localA = mk3<T>(T(p.local_point_a[0]), T(p.local_point_a[1]), T(p.local_point_a[2]));
+        o.localB = mk3<T>(T(p.local_point_b[0]), T(p.local_point_b[1]), T(p.local_point_b[2]));
+        pcStart[pcw[k].first + 1]++;
+    }
+    for (int w = 0; w < nw; w++) pcStart[w + 1] += pcStart[w];
+    d.pc = A.get<PointCon<T>>(hpc.size());
+    int* dPcs = A.get<int>(nw + 1);
+    if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (constraints)");
+    HIPCHK(c, hipMemcpy(d.pc, hpc.data(), hpc.size() * sizeof(PointCon<T>), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dPcs, pcStart.data(), (nw + 1) * sizeof(int), hipMemcpyHostToDevice));
+    d.pcStart = dPcs;
+    d.NPC = npc;
+    // capacities
+    long long pairCap = (long long)c->cfg.max_pairs_per_body * std::max(nb, 1);
+    if (pairCap > (1ll << 24) - 16) pairCap = (1ll << 24) - 16;  // two-level scan limit (4096 * 4096)
+    d.pairCap = (int)pairCap;
+    d.conCap = d.pairCap;
+    d.epaCap = std::min(d.pairCap, std::max(4096, c->cfg.max_contacts_per_body * 512));
+    d.hashCap = next_pow2(2ll * d.pairCap);
+    d.bpCount = A.get<int>(nb + 1);
+    d.bpOff = A.get<int>(nb + 1);
+    d.pairA = A.get<int>(d.pairCap);
+    d.pairB = A.get<int>(d.pairCap);
+    d.counters = A.get<int>(CNT_N);
+    d.con = A.get<Con<T>>(d.pairCap);
+    d.tap = A.get<ConTap<T>>(d.pairCap);
+    d.conValid = A.get<int>(d.pairCap + 1);
+    d.conScan = A.get<int>(d.pairCap + 1);
+    d.conIdx = A.get<int>(d.conCap);
+    for (int k = 0; k < 2; k++) {
+        d.rec[k] = A.get<CacheRec<T>>(d.pairCap);
+        d.hkeys[k] = A.get<unsigned long long>(d.hashCap);
+        d.hvals[k] = A.get<int>(d.hashCap);
+    }
+    d.epaPair = A.get<int>(d.epaCap);
+    d.epaSimplex = A.get<Simplex<T>>(d.epaCap);
+    d.epaWork = A.get<EpaState<T>>(d.epaCap);
+    size_t L = (size_t)d.conCap + npc;
+    d.skeys = A.get<unsigned long long>(2 * (size_t)d.conCap);
+    d.seq = A.get<int>(L);
+    d.seqA = A.get<int>(L);
+    d.seqB = A.get<int>(L);
+    d.prevA = A.get<int>(L);
+    d.prevB = A.get<int>(L);
+    d.level = A.get<int>(L);
+    d.sched = A.get<int>(L);
+    d.lvlStart = A.get<int>(2 * (L + nw + 1));
+    d.blist = A.get<int>(2 * L);
+    d.bcnt = A.get<int>(nb);
+    d.boff = A.get<int>(nb);
+    d.isl0 = A.get<int>(nb);
+    d.isl1 = A.get<int>(nb);
+    d.stats = A.get<int>(2 * (size_t)nw);
+    d.prevDt = A.get<T>(1);
+    d.parity = A.get<int>(1);
+    X->scanSums = A.get<int>(4096);
+    if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (workspace)");
+    HIPCHK(c, hipMemset(d.counters, 0, CNT_N * sizeof(int)));
+    HIPCHK(c, hipMemset(d.prevDt, 0, sizeof(T)));
+    HIPCHK(c, hipMemset(d.parity, 0, sizeof(int)));
+    for (int k = 0; k < 2; k++) HIPCHK(c, hipMemset(d.hkeys[k], 0xff, d.hashCap * sizeof(unsigned long long)));
+    HIPCHK(c, hipMemset(d.conValid, 0, (d.pairCap + 1) * sizeof(int)));
+    HIPCHK(c, hipMemset(d.stats, 0, 2 * nw * sizeof(int)));
+    int blocks = (nb + 255) / 256;
+    if (nb > 0) hipLaunchKernelGGL(k_init_bodies<T>, dim3(blocks), dim3(256), 0, c->stream, d);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!X->events) {
+        for (auto& e : X->ev) HIPCHK(c, hipEventCreate(&e));
+        X->events = true;
+    }
+    int bs = c->cfg.block_solver;
+    X->solverBS = (bs == 64 || bs == 128 || bs == 256 || bs == 512 || bs == 1024) ? bs : 256;
+    return CC_OK;
+}
+
+extern "C" int cc_upload_scene(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* woff, const cc_body_desc* bodies,
+                               int32_t nb, const cc_shape_desc* shapes, int32_t ns, const cc_hull_data* hulls,
+                               const cc_point_constraint_desc* pcs, int32_t npc) {
+    if (!c || !s || nw <= 0 || !woff || nb < 0 || (nb > 0 && !bodies) || ns < 0 || (ns > 0 && !shapes))
+        return fail(c, CC_E_ARG, "invalid arguments to cc_upload_scene");
+    if (woff[0] != 0 || woff[nw] != nb) return fail(c, CC_E_ARG, "world offsets must start at 0 and end at num_bodies");
+    for (int w = 0; w < nw; w++)
+        if (woff[w + 1] < woff[w]) return fail(c, CC_E_ARG, "world offsets must be non-decreasing");
+    if (npc > 0 && !pcs) return fail(c, CC_E_ARG, "point constraints pointer is null");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->steps = 0;
+    if (c->cfg.precision == CC_PRECISION_FP64) return upload<double>(c, s, nw, woff, bodies, nb, shapes, ns, hulls, pcs, npc);
+    return upload<float>(c, s, nw, woff, bodies, nb, shapes, ns, hulls, pcs, npc);
+}
+
+static int grid_for(long long n, int bs, int maxBlocks) {
+    long long g = (n + bs - 1) / bs;
+    if (g < 1) g = 1;
+    if (g > maxBlocks) g = maxBlocks;
+    return (int)g;
+}
+
+static void scan_launch(hipStream_t st, const int* in, int* out, int n, int* sums) {
+    int nb = (n + 4095) / 4096;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, in, out, sums, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, nb, out + n);
+    hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1), dim3(256), 0, st, out, sums, n);
+}
+
+template <typename T>
+static void launch_world(CtxT<T>* X, hipStream_t st, T dt) {
+    Dev<T>& d = X->d;
+    switch (X->solverBS) {
+        case 64: hipLaunchKernelGGL((k_world<T, 64>), dim3(d.W), dim3(64), 0, st, d, dt); break;
+        case 128: hipLaunchKernelGGL((k_world<T, 128>), dim3(d.W), dim3(128), 0, st, d, dt); break;
+        case 512: hipLaunchKernelGGL((k_world<T, 512>), dim3(d.W), dim3(512), 0, st, d, dt); break;
+        case 1024: hipLaunchKernelGGL((k_world<T, 1024>), dim3(d.W), dim3(1024), 0, st, d, dt); break;
+        default: hipLaunchKernelGGL((k_world<T, 256>), dim3(d.W), dim3(256), 0, st, d, dt); break;
+    }
+}
+
+template <typename T>
+static int step_impl(cc_ctx* c, double dtd, int nsteps, bool timed) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    hipStream_t st = c->stream;
+    T dt = T(dtd);
+    int N = d.N;
+    int nbB = (N + 255) / 256;
+    const double margin = (double)d.s.spec;
+    for (int s = 0; s < nsteps; s++) {
+        int e = 0;
+        bool rec = timed && s == nsteps - 1;
+        auto mark = [&]() { if (rec) (void)hipEventRecord(X->ev[e++], st); };
+        mark();
+        hipLaunchKernelGGL(k_accel<T>, dim3(nbB), dim3(256), 0, st, d, dt);
+        hipLaunchKernelGGL((k_bp<T, false>), dim3(nbB), dim3(256), 0, st, d, margin);
+        scan_launch(st, d.bpCount, d.bpOff, N, X->scanSums);
+        hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
+        hipLaunchKernelGGL((k_bp<T, true>), dim3(nbB), dim3(256), 0, st, d, margin);
+        mark();  // broadphase
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(d.pairCap + 1, 256, 2048)), dim3(256), 0, st, d.conValid, 0, d.pairCap + 1);
+        hipLaunchKernelGGL(k_narrowphase<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d, dt);
+        mark();  // narrowphase (GJK + manifold + constraint setup)
+        hipLaunchKernelGGL(k_epa<T>, dim3(grid_for(d.epaCap, 64, 4096)), dim3(64), 0, st, d, dt);
+        hipLaunchKernelGGL(k_count_epa<T>, dim3(1), dim3(64), 0, st, d);
+        mark();  // EPA
+        hipLaunchKernelGGL(k_wake<T>, dim3(nbB), dim3(256), 0, st, d);
+        scan_launch(st, d.conValid, d.conScan, d.pairCap, X->scanSums);
+        hipLaunchKernelGGL(k_compact<T>, dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_uf_init<T>, dim3(nbB), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_uf_link<T>, dim3(grid_for(d.conCap + d.NPC, 256, 2048)), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_uf_compress<T>, dim3(nbB), dim3(256), 0, st, d);
+        mark();  // islands
+        launch_world<T>(X, st, dt);
+        mark();  // solver + integration + sleep
+        hipLaunchKernelGGL(k_hash_clear_cur<T>, dim3(grid_for(d.hashCap, 256, 2048)), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_hash_insert<T>, dim3(grid_for(d.conCap, 256, 2048)), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_end_step<T>, dim3(1), dim3(64), 0, st, d, dt);
+        mark();  // cache
+        if (rec) c->nPhase = e;
+    }
+    HIPCHK(c, hipGetLastError());
+    c->steps += nsteps;
+    return CC_OK;
+}
+
+extern "C" int cc_step(cc_ctx* c, double dt, int32_t nsteps) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (nsteps < 0 || !(dt >= 0)) return fail(c, CC_E_ARG, "invalid dt or nsteps");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return step_impl<double>(c, dt, nsteps, true);
+    return step_impl<float>(c, dt, nsteps, true);
+}
+
+template <typename T>
+static int counters_impl(cc_ctx* c, cc_step_counters* o);
+
+extern "C" int cc_sync(cc_ctx* c) {
+    if (!c) return CC_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CC_OK;
+}
+
+template <typename T>
+static int download_impl(cc_ctx* c, cc_body_state* out, int n) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    if (n != d.N) return fail(c, CC_E_ARG, "num_bodies mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<V3<T>> pos(n), com(n), v(n), w(n), mn(n), mx(n);
+    std::vector<Q4<T>> q(n);
+    std::vector<int> sl(n), lab(n);
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(pos.data(), d.B.pos, n * sizeof(V3<T>), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(com.data(), d.B.com, n * sizeof(V3<T>), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(v.data(), d.B.v, n * sizeof(V3<T>), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(w.data(), d.B.w, n * sizeof(V3<T>), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(mn.data(), d.B.amin, n * sizeof(V3<T>), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(mx.data(), d.B.amax, n * sizeof(V3<T>), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(q.data(), d.B.q, n * sizeof(Q4<T>), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(sl.data(), d.B.sleeping, n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(lab.data(), d.B.label, n * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    for (int i = 0; i < n; i++) {
+        cc_body_state& o = out[i];
+        o.position[0] = pos[i].x; o.position[1] = pos[i].y; o.position[2] = pos[i].z;
+        o.quaternion[0] = q[i].x; o.quaternion[1] = q[i].y; o.quaternion[2] = q[i].z; o.quaternion[3] = q[i].w;
+        o.center_of_mass_position[0] = com[i].x; o.center_of_mass_position[1] = com[i].y; o.center_of_mass_position[2] = com[i].z;
+        o.linear_velocity[0] = v[i].x; o.linear_velocity[1] = v[i].y; o.linear_velocity[2] = v[i].z;
+        o.angular_velocity[0] = w[i].x; o.angular_velocity[1] = w[i].y; o.angular_velocity[2] = w[i].z;
+        o.aabb_min[0] = mn[i].x; o.aabb_min[1] = mn[i].y; o.aabb_min[2] = mn[i].z;
+        o.aabb_max[0] = mx[i].x; o.aabb_max[1] = mx[i].y; o.aabb_max[2] = mx[i].z;
+        o.sleeping = sl[i];
+        o.island = c->steps > 0 ? lab[i] : -1;
+    }
+    return CC_OK;
+}
+
+extern "C" int cc_download_bodies(cc_ctx* c, cc_body_state* out, int32_t n) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (n > 0 && !out) return fail(c, CC_E_ARG, "null output");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return download_impl<double>(c, out, n);
+    return download_impl<float>(c, out, n);
+}
+
+template <typename T>
+static int set_bodies_impl(cc_ctx* c, const cc_body_state* in, int n) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    if (n != d.N) return fail(c, CC_E_ARG, "num_bodies mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<V3<T>> pos(n), v(n), w(n);
+    std::vector<Q4<T>> q(n);
+    for (int i = 0; i < n; i++) {
+        pos[i] = mk3<T>(T(in[i].position[0]), T(in[i].position[1]), T(in[i].position[2]));
+        v[i] = mk3<T>(T(in[i].linear_velocity[0]), T(in[i].linear_velocity[1]), T(in[i].linear_velocity[2]));
+        w[i] = mk3<T>(T(in[i].angular_velocity[0]), T(in[i].angular_velocity[1]), T(in[i].angular_velocity[2]));
+        q[i].x = T(in[i].quaternion[0]); q[i].y = T(in[i].quaternion[1]); q[i].z = T(in[i].quaternion[2]);
+        q[i].w = T(in[i].quaternion[3]);
+    }
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(d.B.pos, pos.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.B.v, v.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.B.w, w.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.B.q, q.data(), n * sizeof(Q4<T>), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_refresh_bodies<T>, dim3((n + 255) / 256), dim3(256), 0, c->stream, d);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return CC_OK;
+}
+
+extern "C" int cc_set_bodies(cc_ctx* c, const cc_body_state* in, int32_t n) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (n > 0 && !in) return fail(c, CC_E_ARG, "null input");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return set_bodies_impl<double>(c, in, n);
+    return set_bodies_impl<float>(c, in, n);
+}
+
+template <typename T>
+static int pairs_impl(cc_ctx* c, int32_t* out, int cap, int* count) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int cnt[CNT_N];
+    HIPCHK(c, hipMemcpy(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    int n = std::min(cnt[CNT_PAIRS], d.pairCap);
+    *count = n;
+    if (!out || cap <= 0) return CC_OK;
+    std::vector<int> a(n), b(n);
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(a.data(), d.pairA, n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(b.data(), d.pairB, n * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    std::vector<std::pair<int, int>> p(n);
+    for (int i = 0; i < n; i++) p[i] = {std::min(a[i], b[i]), std::max(a[i], b[i])};
+    std::sort(p.begin(), p.end());
+    for (int i = 0; i < n && i < cap; i++) { out[2 * i] = p[i].first; out[2 * i + 1] = p[i].second; }
+    return CC_OK;
+}
+
+extern "C" int cc_download_pairs(cc_ctx* c, int32_t* out, int32_t cap, int32_t* count) {
+    if (!c || !c->impl || !count) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return pairs_impl<double>(c, out, cap, count);
+    return pairs_impl<float>(c, out, cap, count);
+}
+
+template <typename T>
+static int contacts_impl(cc_ctx* c, cc_contact_out* out, int cap, int* count) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int cnt[CNT_N];
+    HIPCHK(c, hipMemcpy(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    int n = std::min(cnt[CNT_CONS], d.conCap);
+    *count = n;
+    if (!out || cap <= 0) return CC_OK;
+    std::vector<int> idx(n);
+    if (n > 0) HIPCHK(c, hipMemcpy(idx.data(), d.conIdx, n * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<cc_contact_out> res;
+    res.reserve(n);
+    for (int k = 0; k < n; k++) {
+        Con<T> h;
+        ConTap<T> t;
+        HIPCHK(c, hipMemcpy(&h, d.con + idx[k], sizeof(h), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(&t, d.tap + idx[k], sizeof(t), hipMemcpyDeviceToHost));
+        cc_contact_out o;
+        memset(&o, 0, sizeof(o));
+        o.body_a = h.a; o.body_b = h.b; o.num_points = h.np;
+        o.normal[0] = h.normal.x; o.normal[1] = h.normal.y; o.normal[2] = h.normal.z;
+        for (int j = 0; j < h.np && j < 4; j++) {
+            o.position_a[j][0] = t.pa[j].x; o.position_a[j][1] = t.pa[j].y; o.position_a[j][2] = t.pa[j].z;
+            o.position_b[j][0] = t.pb[j].x; o.position_b[j][1] = t.pb[j].y; o.position_b[j][2] = t.pb[j].z;
+            o.lambda_n[j] = h.p[j].n.lambda; o.lambda_t1[j] = h.p[j].t1.lambda; o.lambda_t2[j] = h.p[j].t2.lambda;
+        }
+        res.push_back(o);
+    }
+    std::sort(res.begin(), res.end(), [](const cc_contact_out& x, const cc_contact_out& y) {
+        return x.body_a != y.body_a ? x.body_a < y.body_a : x.body_b < y.body_b;
+    });
+    for (int k = 0; k < n && k < cap; k++) out[k] = res[k];
+    return CC_OK;
+}
+
+extern "C" int cc_download_contacts(cc_ctx* c, cc_contact_out* out, int32_t cap, int32_t* count) {
+    if (!c || !c->impl || !count) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return contacts_impl<double>(c, out, cap, count);
+    return contacts_impl<float>(c, out, cap, count);
+}
+
+template <typename T>
+static int counters_impl(cc_ctx* c, cc_step_counters* o) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int cnt[CNT_N];
+    HIPCHK(c, hipMemcpy(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    memset(o, 0, sizeof(*o));
+    o->num_pairs = std::min(cnt[CNT_PAIRS], d.pairCap);
+    o->num_contacts = std::min(cnt[CNT_CONS], d.conCap);
+    o->num_epa = cnt[CNT_EPA];
+    o->overflow = cnt[CNT_OVERFLOW];
+    int n = d.N;
+    std::vector<int> fl(n), sl(n), lab(n);
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(fl.data(), d.B.flags, n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(sl.data(), d.B.sleeping, n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(lab.data(), d.B.label, n * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    for (int i = 0; i < n; i++) {
+        if (fl_motion(fl[i]) == MOTION_DYNAMIC) {
+            o->num_bodies++;
+            if (!sl[i]) o->num_awake++;
+        }
+        if (lab[i] == i) o->num_islands++;
+    }
+    std::vector<int> st(2 * d.W);
+    HIPCHK(c, hipMemcpy(st.data(), d.stats, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int w = 0; w < d.W; w++) {
+        o->max_levels = std::max<int64_t>(o->max_levels, st[2 * w]);
+        o->position_sweeps = std::max<int64_t>(o->position_sweeps, st[2 * w + 1]);
+    }
+    int nc = (int)o->num_contacts;
+    if (nc > 0) {
+        std::vector<int> idx(nc);
+        HIPCHK(c, hipMemcpy(idx.data(), d.conIdx, nc * sizeof(int), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nc; k++) {
+            int np;
+            HIPCHK(c, hipMemcpy(&np, &d.con[idx[k]].np, sizeof(int), hipMemcpyDeviceToHost));
+            o->num_points += np;
+        }
+    }
+    return CC_OK;
+}
+
+extern "C" int cc_get_counters(cc_ctx* c, cc_step_counters* o) {
+    if (!c || !c->impl || !o) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return counters_impl<double>(c, o);
+    return counters_impl<float>(c, o);
+}
+
+extern "C" int cc_get_phase_times(cc_ctx* c, double* out, int32_t cap, int32_t* count) {
+    if (!c || !c->impl || !count) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Impl* im = c->impl;
+    hipEvent_t* ev;
+    if (c->cfg.precision == CC_PRECISION_FP64) ev = static_cast<CtxT<double>*>(im)->ev;
+    else ev = static_cast<CtxT<float>*>(im)->ev;
+    int n = c->nPhase > 1 ? c->nPhase - 1 : 0;
+    *count = n;
+    for (int i = 0; i < n && i < cap; i++) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        out[i] = ms;
+    }
+    return CC_OK;
+}
+
+// Broadphase-only tap on caller-supplied AABBs (one world).
+template <typename T>
+static int bp_impl(cc_ctx* c, const float* mn, const float* mx, const int32_t* mo, const int32_t* sl, const int32_t* la,
+                   int n, double margin, int32_t* out, int cap, int32_t* count) {
+    // a one-world scene of unit spheres whose bounds are overwritten with the given AABBs
+    cc_settings s;
+    memset(&s, 0, sizeof(s));
+    for (int a = 0; a < CC_MAX_OBJECT_LAYERS; a++)
+        for (int b = 0; b < CC_MAX_OBJECT_LAYERS; b++) s.object_layer_pairs[a * CC_MAX_OBJECT_LAYERS + b] = 1;
+    s.object_layer_pairs[1 * CC_MAX_OBJECT_LAYERS + 1] = 0;
+    s.speculative_contact_distance = margin;
+    cc_shape_desc sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.type = CC_SHAPE_SPHERE;
+    sh.radius = 1;
+    std::vector<cc_body_desc> bd(std::max(n, 1));
+    for (int i = 0; i < n; i++) {
+        memset(&bd[i], 0, sizeof(cc_body_desc));
+        bd[i].motion_type = mo[i];
+        bd[i].sleeping = sl[i];
+        bd[i].object_layer = la[i];
+        bd[i].collision_groups = bd[i].collision_mask = 0xffffffffu;
+        bd[i].quaternion[3] = 1;
+        bd[i].allowed_dofs = 63;
+    }
+    int woff[2] = {0, n};
+    int r = upload<T>(c, &s, 1, woff, bd.data(), n, &sh, 1, nullptr, nullptr, 0);
+    if (r) return r;
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    std::vector<V3<T>> a(n), b(n);
+    for (int i = 0; i < n; i++) {
+        a[i] = mk3<T>(T(mn[3 * i]), T(mn[3 * i + 1]), T(mn[3 * i + 2]));
+        b[i] = mk3<T>(T(mx[3 * i]), T(mx[3 * i + 1]), T(mx[3 * i + 2]));
+    }
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(d.B.amin, a.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d.B.amax, b.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
+    }
+    hipStream_t st = c->stream;
+    int nbB = (n + 255) / 256;
+    if (n > 0) {
+        hipLaunchKernelGGL((k_bp<T, false>), dim3(nbB), dim3(256), 0, st, d, margin);
+        scan_launch(st, d.bpCount, d.bpOff, n, X->scanSums);
+        hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
+        hipLaunchKernelGGL((k_bp<T, true>), dim3(nbB), dim3(256), 0, st, d, margin);
+    }
+    HIPCHK(c, hipGetLastError());
+    return pairs_impl<T>(c, out, cap, count);
+}
+
+extern "C" int cc_broadphase_aabbs(cc_ctx* c, const float* mn, const float* mx, const int32_t* mo, const int32_t* sl,
+                                   const int32_t* la, int32_t n, double margin, int32_t* out, int32_t cap, int32_t* count) {
+    if (!c || !count || n < 0 || (n > 0 && (!mn || !mx || !mo || !sl || !la))) return fail(c, CC_E_ARG, "invalid arguments");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return bp_impl<double>(c, mn, mx, mo, sl, la, n, margin, out, cap, count);
+    return bp_impl<float>(c, mn, mx, mo, sl, la, n, margin, out, cap, count);
+}

@@ -1,0 +1,404 @@
+"""Scene description shared by the device path and its checkers, plus the seeded
+scene generators of the benchmark configurations.
+
+A Scene is the plain-data form of one or more crashcat worlds (bodies, shapes,
+hull data, point constraints, settings) laid out exactly as the C ABI expects
+(include/crashcat_amd.h).  Generators use splitmix64-seeded xoshiro256** so the
+same seed gives the same initial state on every machine.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi
+from . import shapes as S
+
+
+# ---------------------------------------------------------------- settings / layers
+def create_world_settings():
+    """createWorldSettings — src/world-settings.ts:224-264 (defaults)."""
+    s = _abi.Settings()
+    s.gravity[:] = (0.0, -9.81, 0.0)
+    s.speculative_contact_distance = 0.02
+    s.manifold_tolerance = 1e-3
+    s.collision_tolerance = 1e-4
+    s.penetration_tolerance = 1e-4
+    s.normal_cos_max_delta_rotation = math.cos(5 * math.pi / 180)
+    s.contact_point_preserve_lambda_max_dist_sq = 0.01 * 0.01
+    s.min_velocity_for_restitution = 1.0
+    s.penetration_slop = 0.02
+    s.baumgarte_factor = 0.2
+    s.max_penetration_distance = 0.2
+    s.time_before_sleep = 0.5
+    s.point_velocity_sleep_threshold = 0.03
+    s.velocity_iterations = 10
+    s.position_iterations = 2
+    s.gravity_enabled = 1
+    s.warm_starting = 1
+    s.allow_sleeping = 1
+    s.num_object_layers = 0
+    return s
+
+
+def add_object_layer(settings):
+    """addObjectLayer — src/layers.ts (broadphase layers follow the object-layer pair table)."""
+    i = settings.num_object_layers
+    if i >= _abi.CC_MAX_OBJECT_LAYERS:
+        raise ValueError("too many object layers")
+    settings.num_object_layers = i + 1
+    return i
+
+
+def enable_collision(settings, a, b):
+    """enableCollision — src/layers.ts:88-112."""
+    settings.object_layer_pairs[a * _abi.CC_MAX_OBJECT_LAYERS + b] = 1
+    settings.object_layer_pairs[b * _abi.CC_MAX_OBJECT_LAYERS + a] = 1
+
+
+def test_world_settings():
+    """The layer setup of the reference's test helper (tst/helpers.ts:3-26):
+    MOVING collides with MOVING and NOT_MOVING."""
+    s = create_world_settings()
+    moving = add_object_layer(s)
+    not_moving = add_object_layer(s)
+    enable_collision(s, moving, moving)
+    enable_collision(s, moving, not_moving)
+    return s, moving, not_moving
+
+
+BODY_DTYPE = np.dtype(_abi.BodyDesc)
+
+
+def _quat_yaw(theta):
+    return (0.0, math.sin(theta / 2), 0.0, math.cos(theta / 2))
+
+
+class Scene:
+    """Bodies of one or more independent worlds sharing a shape table and settings."""
+
+    def __init__(self, settings=None):
+        if settings is None:
+            settings, self.layer_moving, self.layer_static = test_world_settings()
+        else:
+            self.layer_moving, self.layer_static = 0, 1
+        self.settings = settings
+        self.shapes = []
+        self._shape_index = {}
+        self.rows = []  # list of dicts, converted lazily
+        self.world_offsets = [0]
+        self.point_constraints = []
+        self._bodies = None
+
+    # -------------------------------------------------------------- building
+    def add_shape(self, shape):
+        k = shape.key()
+        if k in self._shape_index:
+            return self._shape_index[k]
+        self._shape_index[k] = len(self.shapes)
+        self.shapes.append(shape)
+        return len(self.shapes) - 1
+
+    def add_body(self, shape, position, quaternion=(0.0, 0.0, 0.0, 1.0), motion_type=_abi.CC_MOTION_DYNAMIC,
+                 linear_velocity=(0.0, 0.0, 0.0), angular_velocity=(0.0, 0.0, 0.0), object_layer=None,
+                 friction=0.2, restitution=0.0, gravity_factor=1.0, linear_damping=0.05, angular_damping=0.05,
+                 max_linear_velocity=500.0, max_angular_velocity=0.25 * math.pi * 60.0, allow_sleeping=True,
+                 allowed_dofs=0b111111, friction_combine=4, restitution_combine=3, collision_groups=0xFFFFFFFF,
+                 collision_mask=0xFFFFFFFF):
+        """rigidBody.create — src/body/rigid-body.ts:390-434, defaults :120-139."""
+        si = self.add_shape(shape)
+        if object_layer is None:
+            object_layer = self.layer_static if motion_type == _abi.CC_MOTION_STATIC else self.layer_moving
+        if motion_type == _abi.CC_MOTION_STATIC:
+            inv_mass, inv_i, irot = 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)
+        else:
+            inv_mass, inv_i, irot = S.motion_mass(shape.mass, shape.inertia, allowed_dofs)
+        self.rows.append((si, motion_type, object_layer, collision_groups, collision_mask, int(allow_sleeping),
+                          allowed_dofs, friction_combine, restitution_combine, 0, tuple(position), tuple(quaternion),
+                          tuple(linear_velocity), tuple(angular_velocity), inv_mass, inv_i, irot, friction,
+                          restitution, gravity_factor, linear_damping, angular_damping, max_linear_velocity,
+                          max_angular_velocity))
+        self._bodies = None
+        return len(self.rows) - 1
+
+    def end_world(self):
+        """Close the current world; following bodies belong to a new world."""
+        if self.world_offsets[-1] != len(self.rows):
+            self.world_offsets.append(len(self.rows))
+
+    def add_point_constraint(self, a, b, local_a, local_b):
+        self.point_constraints.append((a, b, tuple(local_a), tuple(local_b)))
+
+    # -------------------------------------------------------------- C views
+    @property
+    def num_bodies(self):
+        return len(self.rows)
+
+    @property
+    def num_worlds(self):
+        offs = self.offsets()
+        return len(offs) - 1
+
+    def offsets(self):
+        offs = list(self.world_offsets)
+        if offs[-1] != len(self.rows):
+            offs.append(len(self.rows))
+        return offs
+
+    def bodies(self):
+        """numpy structured array with the layout of cc_body_desc."""
+        if self._bodies is None:
+            arr = np.zeros(len(self.rows), dtype=BODY_DTYPE)
+            if self.rows:
+                cols = list(zip(*self.rows))
+                names = ["shape", "motion_type", "object_layer", "collision_groups", "collision_mask",
+                         "allow_sleeping", "allowed_dofs", "friction_combine", "restitution_combine", "sleeping",
+                         "position", "quaternion", "linear_velocity", "angular_velocity", "inv_mass",
+                         "inv_inertia_diagonal", "inertia_rotation", "friction", "restitution", "gravity_factor",
+                         "linear_damping", "angular_damping", "max_linear_velocity", "max_angular_velocity"]
+                for n, c in zip(names, cols):
+                    arr[n] = np.asarray(c)
+            self._bodies = arr
+        return self._bodies
+
+    def c_bodies(self):
+        b = self.bodies()
+        return b.ctypes.data_as(C.POINTER(_abi.BodyDesc)), b
+
+    def c_shapes(self):
+        """(shape array, HullData, keepalive) in the C layout."""
+        n = len(self.shapes)
+        arr = (_abi.ShapeDesc * max(n, 1))()
+        pts, shr, ffv, fnv, fvi, pn, pc = [], [], [], [], [], [], []
+        np_, nf, nfv = 0, 0, 0
+        for i, s in enumerate(self.shapes):
+            d = arr[i]
+            d.type = s.type
+            d.radius = s.radius
+            d.half_height = s.half_height
+            d.half_extents[:] = s.half_extents
+            d.convex_radius = s.convex_radius
+            d.center_of_mass[:] = s.center_of_mass
+            d.aabb_min[:] = s.aabb_min
+            d.aabb_max[:] = s.aabb_max
+            if s.type == S.CONVEX_HULL:
+                d.hull_first_point = np_
+                d.hull_num_points = len(s.points)
+                d.hull_first_face = nf
+                d.hull_num_faces = len(s.faces)
+                pts.append(np.asarray(s.points, dtype=np.float64))
+                shr.append(np.asarray(s.shrunk_points, dtype=np.float64))
+                for f, nrm, c in zip(s.faces, s.plane_normals, s.plane_constants):
+                    ffv.append(nfv)
+                    fnv.append(len(f))
+                    fvi.extend(f)
+                    nfv += len(f)
+                    pn.append(nrm)
+                    pc.append(c)
+                np_ += len(s.points)
+                nf += len(s.faces)
+        keep = {}
+        keep["pts"] = np.ascontiguousarray(np.concatenate(pts) if pts else np.zeros((1, 3))).astype(np.float64)
+        keep["shr"] = np.ascontiguousarray(np.concatenate(shr) if shr else np.zeros((1, 3))).astype(np.float64)
+        keep["ffv"] = np.asarray(ffv if ffv else [0], dtype=np.int32)
+        keep["fnv"] = np.asarray(fnv if fnv else [0], dtype=np.int32)
+        keep["fvi"] = np.asarray(fvi if fvi else [0], dtype=np.int32)
+        keep["pn"] = np.ascontiguousarray(np.asarray(pn if pn else np.zeros((1, 3)), dtype=np.float64))
+        keep["pc"] = np.asarray(pc if pc else [0.0], dtype=np.float64)
+        h = _abi.HullData()
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int32)
+        h.points = keep["pts"].ctypes.data_as(dp)
+        h.shrunk_points = keep["shr"].ctypes.data_as(dp)
+        h.face_first_vertex = keep["ffv"].ctypes.data_as(ip)
+        h.face_num_vertices = keep["fnv"].ctypes.data_as(ip)
+        h.face_vertex_indices = keep["fvi"].ctypes.data_as(ip)
+        h.plane_normals = keep["pn"].ctypes.data_as(dp)
+        h.plane_constants = keep["pc"].ctypes.data_as(dp)
+        h.total_points = np_
+        h.total_faces = nf
+        h.total_face_vertices = nfv
+        keep["arr"] = arr
+        return arr, h, keep
+
+    def c_point_constraints(self):
+        n = len(self.point_constraints)
+        arr = (_abi.PointConstraintDesc * max(n, 1))()
+        for i, (a, b, la, lb) in enumerate(self.point_constraints):
+            arr[i].body_a = a
+            arr[i].body_b = b
+            arr[i].enabled = 1
+            arr[i].local_point_a[:] = la
+            arr[i].local_point_b[:] = lb
+        return arr, n
+
+    def c_offsets(self):
+        offs = np.asarray(self.offsets(), dtype=np.int32)
+        return offs
+
+
+# ------------------------------------------------------------------------ PRNG
+_M64 = (1 << 64) - 1
+
+
+def _splitmix64(x):
+    x = (x + 0x9E3779B97F4A7C15) & _M64
+    z = x
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return x, z ^ (z >> 31)
+
+
+class Rng:
+    """xoshiro256** seeded by splitmix64."""
+
+    def __init__(self, seed):
+        x = seed & _M64
+        self.s = []
+        for _ in range(4):
+            x, z = _splitmix64(x)
+            self.s.append(z)
+
+    def next_u64(self):
+        s = self.s
+        r = (((s[1] * 5) & _M64) << 7 | ((s[1] * 5) & _M64) >> 57) & _M64
+        r = (r * 9) & _M64
+        t = (s[1] << 17) & _M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = ((s[3] << 45) | (s[3] >> 19)) & _M64
+        return r
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+    def normal(self, mean=0.0, std=1.0):
+        u1 = self.uniform()
+        u2 = self.uniform()
+        if u1 < 1e-300:
+            u1 = 1e-300
+        return mean + std * math.sqrt(-2.0 * math.log(u1)) * math.cos(2 * math.pi * u2)
+
+    def quaternion(self):
+        """Uniform random rotation (Shoemake)."""
+        u1, u2, u3 = self.uniform(), self.uniform(), self.uniform()
+        a = math.sqrt(1 - u1)
+        b = math.sqrt(u1)
+        return (a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2), b * math.sin(2 * math.pi * u3),
+                b * math.cos(2 * math.pi * u3))
+
+
+# ------------------------------------------------------------------ generators
+def _floor(scene, half=(50.0, 0.5, 50.0), top=0.0, friction=0.2):
+    return scene.add_body(S.box(half), (0.0, top - half[1], 0.0), motion_type=_abi.CC_MOTION_STATIC,
+                          friction=friction)
+
+
+def cube_heap(seed=1001, nx=10, ny=10, nz=10, scene=None, friction=0.5):
+    """Config 1: 1,000 boxes (he 0.5, density 1000) on a 10x10x10 grid at 1.1 m pitch, the
+    lowest layer 1 m above a static 50x0.5x50 floor, +-0.05 m jitter, random yaw, friction 0.5."""
+    sc = scene if scene is not None else Scene()
+    rng = Rng(seed)
+    _floor(sc, friction=friction)
+    shape = S.box((0.5, 0.5, 0.5))
+    pitch = 1.1
+    for j in range(ny):
+        for i in range(nx):
+            for k in range(nz):
+                x = (i - (nx - 1) / 2) * pitch + rng.uniform(-0.05, 0.05)
+                y = 1.0 + 0.5 + j * pitch + rng.uniform(-0.05, 0.05)
+                z = (k - (nz - 1) / 2) * pitch + rng.uniform(-0.05, 0.05)
+                q = _quat_yaw(rng.uniform(0, 2 * math.pi))
+                sc.add_body(shape, (x, y, z), q, friction=friction)
+    sc.end_world()
+    return sc
+
+
+def _unit_sphere_points(rng, n):
+    pts = []
+    for _ in range(n):
+        z = rng.uniform(-1.0, 1.0)
+        phi = rng.uniform(0, 2 * math.pi)
+        r = math.sqrt(max(0.0, 1 - z * z))
+        pts.append((r * math.cos(phi), r * math.sin(phi), z))
+    return np.array(pts)
+
+
+def mixed_pile(seed=1002, n=20000, extent=(60.0, 40.0, 60.0), scene=None):
+    """Config 2: n bodies drawn uniformly from sphere (r U[0.3,0.8]), box (he U[0.2,0.6]^3),
+    capsule (r U[0.2,0.4], hh U[0.2,0.6]) and convex hull of 16 points on the unit sphere
+    scaled U[0.4,1.0]; positions uniform in a 60x40x60 m box above a static floor;
+    v ~ N(0,1) per axis, w ~ N(0,0.5) per axis."""
+    sc = scene if scene is not None else Scene()
+    rng = Rng(seed)
+    half = (max(50.0, extent[0] / 2 + 5), 0.5, max(50.0, extent[2] / 2 + 5))
+    _floor(sc, half=half)
+    for _ in range(n):
+        kind = min(3, int(rng.uniform() * 4))
+        if kind == 0:
+            shape = S.sphere(rng.uniform(0.3, 0.8))
+        elif kind == 1:
+            shape = S.box((rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6)))
+        elif kind == 2:
+            shape = S.capsule(rng.uniform(0.2, 0.4), rng.uniform(0.2, 0.6))
+        else:
+            pts = _unit_sphere_points(rng, 16) * rng.uniform(0.4, 1.0)
+            shape = S.convex_hull(pts)
+        x = rng.uniform(-extent[0] / 2, extent[0] / 2)
+        y = rng.uniform(1.0, 1.0 + extent[1])
+        z = rng.uniform(-extent[2] / 2, extent[2] / 2)
+        q = rng.quaternion()
+        v = (rng.normal(0, 1), rng.normal(0, 1), rng.normal(0, 1))
+        w = (rng.normal(0, 0.5), rng.normal(0, 0.5), rng.normal(0, 0.5))
+        sc.add_body(shape, (x, y, z), q, linear_velocity=v, angular_velocity=w)
+    sc.end_world()
+    return sc
+
+
+def batched_worlds(seed=1003, num_worlds=4096, min_bodies=8, max_bodies=512, chains=2, chain_len=8, scene=None):
+    """Config 3: independent worlds, body count per world log-uniform in [8, 512]; each world
+    is a pile of boxes and spheres (he/r 0.5) on a 1.1 m grid above a floor plus `chains`
+    chains of `chain_len` capsules (r 0.1, hh 0.3) joined end to end by point constraints."""
+    sc = scene if scene is not None else Scene()
+    box = S.box((0.5, 0.5, 0.5))
+    ball = S.sphere(0.5)
+    cap = S.capsule(0.1, 0.3)
+    for w in range(num_worlds):
+        rng = Rng((seed << 32) + w)
+        n = int(math.floor(math.exp(rng.uniform(math.log(min_bodies), math.log(max_bodies)))))
+        _floor(sc, friction=0.5)
+        side = max(1, int(math.ceil(n ** (1 / 3))))
+        for b in range(n):
+            i, k, j = b % side, (b // side) % side, b // (side * side)
+            x = (i - (side - 1) / 2) * 1.1 + rng.uniform(-0.05, 0.05)
+            y = 1.5 + j * 1.1 + rng.uniform(-0.05, 0.05)
+            z = (k - (side - 1) / 2) * 1.1 + rng.uniform(-0.05, 0.05)
+            shape = box if rng.uniform() < 0.5 else ball
+            sc.add_body(shape, (x, y, z), _quat_yaw(rng.uniform(0, 2 * math.pi)), friction=0.5)
+        top = 1.5 + (n // (side * side) + 1) * 1.1 + 1.0
+        for c in range(chains):
+            z0 = (c - (chains - 1) / 2) * 2.0 + side * 1.1
+            prev = None
+            for l in range(chain_len):
+                x = (l - (chain_len - 1) / 2) * 0.8
+                # capsule axis along x: rotate 90 degrees about z
+                q = (0.0, 0.0, math.sin(math.pi / 4), math.cos(math.pi / 4))
+                idx = sc.add_body(cap, (x, top, z0), q, friction=0.5)
+                if prev is not None:
+                    # shared end point: +x end of the previous capsule (local -y after the 90 degree
+                    # turn about z) and the -x end of this one (local +y)
+                    sc.add_point_constraint(prev, idx, (0.0, -0.4, 0.0), (0.0, 0.4, 0.0))
+                prev = idx
+        sc.end_world()
+    return sc
+
+
+def replicate(builder, copies, seed0, **kw):
+    """A batch of `copies` independent worlds made by `builder` with seeds seed0, seed0+1, ..."""
+    sc = Scene()
+    for c in range(copies):
+        builder(seed=seed0 + c, scene=sc, **kw)
+    return sc

@@ -1,0 +1,261 @@
+/*
+ * crashcat_amd — C ABI of the MI355X (gfx950) step hot path.
+ *
+ * The library replaces the body of crashcat's
+ *     updateWorld(world, listener, timeStep)          src/update.ts:45
+ * for worlds whose bodies use the convex shapes {sphere, box, capsule,
+ * convex hull} and whose listener has no callbacks.  Everything that
+ * crosses this boundary is plain data: the host (JavaScript over N-API,
+ * Python over ctypes, or C/C++) owns every buffer it passes in, calls copy
+ * synchronously and keep no alias after they return; device memory is owned
+ * by the context.  A context is used from one host thread at a time.
+ *
+ * All functions return 0 on success and a negative CC_E_* code on failure;
+ * cc_last_error() then describes the failure (the host maps this to a
+ * thrown Error, as crashcat's assert() does, src/utils/assert.ts:1-4).
+ */
+#ifndef CRASHCAT_AMD_H
+#define CRASHCAT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------- */
+#define CC_OK 0
+#define CC_E_ARG (-1)       /* invalid argument / descriptor                   */
+#define CC_E_HIP (-2)       /* HIP runtime failure                             */
+#define CC_E_CAPACITY (-3)  /* a per-step capacity (pairs, contacts, EPA) ran out */
+#define CC_E_STATE (-4)     /* call not valid in the current context state     */
+#define CC_E_UNSUPPORTED (-5) /* shape/constraint type not handled on the GPU path */
+
+/* ---- shape types: the dispatch matrix of src/shapes/shapes.ts:37-51 ----- */
+#define CC_SHAPE_SPHERE 0
+#define CC_SHAPE_BOX 1
+#define CC_SHAPE_CAPSULE 2
+#define CC_SHAPE_CONVEX_HULL 3
+
+/* ---- motion types: src/body/motion-type.ts ------------------------------ */
+#define CC_MOTION_STATIC 0
+#define CC_MOTION_KINEMATIC 1
+#define CC_MOTION_DYNAMIC 2
+
+/* ---- material combine modes: src/constraints/combine-material.ts:1-8 ---- */
+#define CC_COMBINE_AVERAGE 0
+#define CC_COMBINE_MULTIPLY 1
+#define CC_COMBINE_MIN 2
+#define CC_COMBINE_MAX 3
+#define CC_COMBINE_GEOMETRIC_MEAN 4
+
+/* ---- precision of a context ------------------------------------------- */
+#define CC_PRECISION_FP32 0 /* batched path: bulk FP32, FP64 in EPA faces and inertia */
+#define CC_PRECISION_FP64 1 /* plain path: FP64 throughout                        */
+
+#define CC_MAX_OBJECT_LAYERS 8
+
+/* World settings; mirrors createWorldSettings() src/world-settings.ts:224-264. */
+typedef struct cc_settings {
+    double gravity[3];
+    double speculative_contact_distance;    /* narrowphase.speculativeContactDistance (0.02) */
+    double manifold_tolerance;              /* narrowphase.manifoldTolerance (1e-3)          */
+    double collision_tolerance;             /* narrowphase.collisionTolerance (1e-4)         */
+    double penetration_tolerance;           /* narrowphase.penetrationTolerance (1e-4)       */
+    double normal_cos_max_delta_rotation;   /* cos(5 deg)                                     */
+    double contact_point_preserve_lambda_max_dist_sq; /* contacts.* (1e-4)                     */
+    double min_velocity_for_restitution;    /* solver.minVelocityForRestitution (1)           */
+    double penetration_slop;                /* solver.penetrationSlop (0.02)                  */
+    double baumgarte_factor;                /* solver.baumgarteFactor (0.2)                   */
+    double max_penetration_distance;        /* solver.maxPenetrationDistance (0.2)            */
+    double time_before_sleep;               /* sleeping.timeBeforeSleep (0.5)                 */
+    double point_velocity_sleep_threshold;  /* sleeping.pointVelocitySleepThreshold (0.03)    */
+    int32_t velocity_iterations;            /* solver.velocityIterations (10)                 */
+    int32_t position_iterations;            /* solver.positionIterations (2)                  */
+    int32_t gravity_enabled;
+    int32_t warm_starting;
+    int32_t allow_sleeping;
+    int32_t num_object_layers;
+    /* object_layer_pairs[a * CC_MAX_OBJECT_LAYERS + b] != 0 when object layers a and b
+     * collide (src/layers.ts:88-112, enableCollision). */
+    uint8_t object_layer_pairs[CC_MAX_OBJECT_LAYERS * CC_MAX_OBJECT_LAYERS];
+} cc_settings;
+
+/* One shape; mirrors the fields of the Sphere/Box/Capsule/ConvexHull shape records
+ * (src/shapes/sphere.ts:40-56, box.ts:42-52, capsule.ts:30-48, convex-hull.ts:37-62). */
+typedef struct cc_shape_desc {
+    int32_t type;               /* CC_SHAPE_* */
+    int32_t hull_first_point;   /* hulls: offset into cc_hull_data.points / shrunk_points */
+    int32_t hull_num_points;
+    int32_t hull_first_face;    /* hulls: offset into face_first_vertex / face_num_vertices / planes */
+    int32_t hull_num_faces;
+    int32_t pad_;
+    double radius;              /* sphere radius, capsule radius                    */
+    double half_height;         /* capsule half height of the cylinder              */
+    double half_extents[3];     /* box                                              */
+    double convex_radius;       /* box, hull                                        */
+    double center_of_mass[3];   /* shape.centerOfMass (hull: the hull's COM)        */
+    double aabb_min[3];         /* shape.aabb (local bounds)                        */
+    double aabb_max[3];
+} cc_shape_desc;
+
+/* Flattened convex-hull data shared by all hull shapes.  Points are relative to the
+ * hull's centre of mass (src/shapes/convex-hull.ts:228); shrunk_points are the
+ * convex-radius-excluded support points (src/shapes/convex-hull.ts:732-786). */
+typedef struct cc_hull_data {
+    const double* points;              /* [3 * total_points]            */
+    const double* shrunk_points;       /* [3 * total_points]            */
+    const int32_t* face_first_vertex;  /* [total_faces], into face_vertex_indices (absolute) */
+    const int32_t* face_num_vertices;  /* [total_faces]                 */
+    const int32_t* face_vertex_indices;/* [total_face_vertices], indices relative to the hull's first point */
+    const double* plane_normals;       /* [3 * total_faces]             */
+    const double* plane_constants;     /* [total_faces]                 */
+    int32_t total_points;
+    int32_t total_faces;
+    int32_t total_face_vertices;
+    int32_t pad_;
+} cc_hull_data;
+
+/* One body at upload; mirrors RigidBody + MotionProperties (src/body/rigid-body.ts:142-252,
+ * src/body/motion-properties.ts:25-69).  Mass properties are already decomposed into
+ * inv_mass / inv_inertia_diagonal / inertia_rotation (setMassProperties, :419-481). */
+typedef struct cc_body_desc {
+    int32_t shape;              /* index into the shape table                       */
+    int32_t motion_type;        /* CC_MOTION_*                                      */
+    int32_t object_layer;
+    uint32_t collision_groups;
+    uint32_t collision_mask;
+    int32_t allow_sleeping;
+    int32_t allowed_dofs;       /* 6-bit mask, bits 0-2 translation, 3-5 rotation   */
+    int32_t friction_combine;   /* CC_COMBINE_*                                     */
+    int32_t restitution_combine;
+    int32_t sleeping;           /* initial sleep state                              */
+    double position[3];
+    double quaternion[4];       /* x, y, z, w                                       */
+    double linear_velocity[3];
+    double angular_velocity[3];
+    double inv_mass;
+    double inv_inertia_diagonal[3];
+    double inertia_rotation[4];
+    double friction;
+    double restitution;
+    double gravity_factor;
+    double linear_damping;
+    double angular_damping;
+    double max_linear_velocity;
+    double max_angular_velocity;
+} cc_body_desc;
+
+/* Body state read back after stepping. */
+typedef struct cc_body_state {
+    double position[3];
+    double quaternion[4];
+    double center_of_mass_position[3];
+    double linear_velocity[3];
+    double angular_velocity[3];
+    double aabb_min[3];
+    double aabb_max[3];
+    int32_t sleeping;
+    int32_t island;             /* island label (lowest body index of the island), -1 if none */
+} cc_body_state;
+
+/* Ball joint between two bodies (src/constraints/point-constraint.ts:40-80);
+ * points are in each body's centre-of-mass space. */
+typedef struct cc_point_constraint_desc {
+    int32_t body_a;
+    int32_t body_b;
+    int32_t enabled;
+    int32_t pad_;
+    double local_point_a[3];
+    double local_point_b[3];
+} cc_point_constraint_desc;
+
+/* One contact constraint of the last step, for parity taps
+ * (ContactConstraint, src/constraints/contact-constraints.ts:31-95). */
+typedef struct cc_contact_out {
+    int32_t body_a;             /* lower id of the pair (src/constraints/contact-constraints.ts:448-453) */
+    int32_t body_b;
+    int32_t num_points;
+    int32_t pad_;
+    double normal[3];           /* world space, A -> B                              */
+    double position_a[4][3];    /* world-space contact points on A and B at creation */
+    double position_b[4][3];
+    double lambda_n[4];         /* solved impulses (after the velocity iterations)  */
+    double lambda_t1[4];
+    double lambda_t2[4];
+} cc_contact_out;
+
+/* Per-step counters used by the byte model. */
+typedef struct cc_step_counters {
+    int64_t num_bodies;         /* dynamic bodies                                   */
+    int64_t num_awake;
+    int64_t num_pairs;
+    int64_t num_contacts;       /* contact constraints (manifolds)                  */
+    int64_t num_points;         /* contact points                                   */
+    int64_t num_epa;            /* EPA invocations                                  */
+    int64_t num_islands;
+    int64_t max_levels;         /* deepest dependency-level schedule of any world   */
+    int64_t position_sweeps;    /* position sweeps actually executed (max over worlds) */
+    int64_t overflow;           /* non-zero if a capacity was exceeded              */
+} cc_step_counters;
+
+typedef struct cc_config {
+    int32_t precision;          /* CC_PRECISION_*                                   */
+    int32_t device;             /* HIP device ordinal                               */
+    int32_t max_pairs_per_body; /* capacity factor for broadphase pairs (default 24) */
+    int32_t max_contacts_per_body; /* capacity factor for contact constraints (default 12) */
+    int32_t block_solver;       /* threads per solver workgroup (0 = autotuned)     */
+    int32_t block_bodies;       /* threads per body-parallel workgroup (0 = autotuned) */
+} cc_config;
+
+typedef struct cc_ctx cc_ctx;
+
+/* Lifecycle ---------------------------------------------------------------- */
+/* replaces createWorld (src/world.ts:46-58) + device allocation */
+int cc_create(cc_ctx** out, const cc_config* cfg);
+int cc_destroy(cc_ctx* ctx);
+const char* cc_last_error(const cc_ctx* ctx);
+
+/* Upload a batch of independent worlds.  world_body_offsets has num_worlds+1
+ * entries; bodies of world w are [offsets[w], offsets[w+1]) and keep creation order
+ * (src/body/rigid-body.ts:398-407).  Point constraints refer to global body indices
+ * and must join bodies of the same world.  Replaces rigidBody.create (src/body/rigid-body.ts:390-434)
+ * + broadphase.addBody for all bodies at once. */
+int cc_upload_scene(cc_ctx* ctx, const cc_settings* settings,
+                    int32_t num_worlds, const int32_t* world_body_offsets,
+                    const cc_body_desc* bodies, int32_t num_bodies,
+                    const cc_shape_desc* shapes, int32_t num_shapes,
+                    const cc_hull_data* hulls,
+                    const cc_point_constraint_desc* point_constraints, int32_t num_point_constraints);
+
+/* Advance every world by nsteps steps of dt; replaces updateWorld (src/update.ts:45-239).
+ * Work is enqueued on the context's stream; call cc_sync to wait. */
+int cc_step(cc_ctx* ctx, double dt, int32_t nsteps);
+int cc_sync(cc_ctx* ctx);
+
+/* Read back body state (all worlds, global body order). */
+int cc_download_bodies(cc_ctx* ctx, cc_body_state* out, int32_t num_bodies);
+/* Overwrite body poses/velocities (all worlds), e.g. to restart from an oracle state. */
+int cc_set_bodies(cc_ctx* ctx, const cc_body_state* in, int32_t num_bodies);
+
+/* Parity taps of the last step. */
+/* Broadphase pairs as (a, b) global body indices, a < b, sorted ascending;
+ * replaces broadphase.findCollidingPairs (src/broadphase/broadphase.ts:217-277). */
+int cc_download_pairs(cc_ctx* ctx, int32_t* out_pairs, int32_t capacity, int32_t* out_count);
+/* Contact constraints sorted by (body_a, body_b). */
+int cc_download_contacts(cc_ctx* ctx, cc_contact_out* out, int32_t capacity, int32_t* out_count);
+int cc_get_counters(cc_ctx* ctx, cc_step_counters* out);
+
+/* Broadphase pair set on caller-supplied AABBs (one world), without stepping; the
+ * AABBs are used as given (float32 values widened to double on both sides). */
+int cc_broadphase_aabbs(cc_ctx* ctx, const float* aabb_min, const float* aabb_max,
+                        const int32_t* motion_type, const int32_t* sleeping, const int32_t* object_layer,
+                        int32_t n, double margin, int32_t* out_pairs, int32_t capacity, int32_t* out_count);
+
+/* Per-phase timing of the last cc_step (milliseconds, HIP events on the context stream). */
+int cc_get_phase_times(cc_ctx* ctx, double* out_ms, int32_t capacity, int32_t* out_count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRASHCAT_AMD_H */

@@ -1,0 +1,87 @@
+/*
+ * refcpu — FP64 CPU restatement of crashcat's per-step collision-and-contact
+ * hot path.  TEST INFRASTRUCTURE (the oracle): only tests/, the smoke check and
+ * bench.py's CPU-baseline leg load it, and only as the checker or the timed CPU
+ * baseline; the product library never links or calls it.
+ *
+ * It follows the reference files op for op (citations beside each function in
+ * refcpu_collide.c / refcpu_step.c) in double precision, on one thread, with
+ * crashcat's own orderings: active list in wake/creation order, contact
+ * constraints in pair order, warm start in creation order, islands numbered by
+ * lowest active index and sorted by size, contacts key-sorted per island.
+ * Deliberate differences, none of which changes a pair set or an island
+ * partition: broadphase pairs are enumerated brute force (the pair set of the
+ * dynamic AABB tree is tree-independent, src/broadphase/dbvt.ts:620-681), in
+ * (querying body, other body) index order instead of tree order; body ids equal
+ * body indices (no removals, so id order == index order).
+ */
+#ifndef REFCPU_H
+#define REFCPU_H
+#include <stdint.h>
+#include "../include/crashcat_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oc_world oc_world;
+
+/* world / step */
+oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb,
+                    const cc_shape_desc* shapes, int32_t ns, const cc_hull_data* hulls,
+                    const cc_point_constraint_desc* pcs, int32_t npc);
+void oc_destroy(oc_world* w);
+void oc_step(oc_world* w, double dt);
+int32_t oc_num_bodies(const oc_world* w);
+void oc_get_bodies(const oc_world* w, cc_body_state* out);
+void oc_set_bodies(oc_world* w, const cc_body_state* in);
+/* pairs of the last step as (min,max) index pairs, sorted */
+int32_t oc_get_pairs(const oc_world* w, int32_t* out, int32_t cap);
+/* contact constraints of the last step, sorted by (a,b) */
+int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap);
+void oc_get_counters(const oc_world* w, cc_step_counters* out);
+/* broadphase only, on given AABBs (widened from float) */
+int32_t oc_broadphase_aabbs(const float* mn, const float* mx, const int32_t* motion, const int32_t* sleeping,
+                            const int32_t* layer, int32_t n, double margin, const cc_settings* s,
+                            int32_t* out, int32_t cap);
+
+/* unit entry points used by the known-answer tests */
+typedef struct oc_hit {
+    int32_t hit;
+    int32_t status;          /* 0 GJK-colliding, 1 EPA, -1 none */
+    double point_a[3], point_b[3], axis[3], penetration;
+    int32_t face_a_n, face_b_n;
+    double face_a[32 * 3], face_b[32 * 3];
+} oc_hit;
+/* collide two shapes at world poses (collideShapeVsShape, src/collision/narrowphase.ts:172) */
+void oc_collide(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t sa, const double* pa, const double* qa,
+                int32_t sb, const double* pb, const double* qb, double max_separation, const cc_settings* s,
+                oc_hit* out);
+/* manifold of one hit, pruned to <= 4 points (update.ts:385-563 for a single hit);
+ * out_a/out_b are world points; returns the point count */
+int32_t oc_manifold_from_hit(const oc_hit* hit, double max_contact_distance, double* normal_out,
+                             double* out_a, double* out_b);
+/* clipping primitives (src/manifold/clip.ts) */
+int32_t oc_clip_poly_vs_poly(const double* poly, int32_t n, const double* clip, int32_t m, const double* normal,
+                             double* out);
+int32_t oc_clip_poly_vs_edge(const double* poly, int32_t n, const double* e1, const double* e2, const double* normal,
+                             double* out);
+int32_t oc_manifold_between_two_faces(const double* p1, const double* p2, const double* axis, double max_dist,
+                                      const double* face1, int32_t n1, const double* face2, int32_t n2,
+                                      double* out_a, double* out_b);
+/* prune >4 points to <=4 (src/manifold/manifold.ts:192-313); points relative to base offset */
+int32_t oc_prune(double* rel_a, double* rel_b, int32_t n, const double* axis);
+/* barycentric coordinates (src/collision/closest-points.ts) */
+void oc_barycentric2(const double* a, const double* b, double tol, double* uvw_valid);
+void oc_barycentric3(const double* a, const double* b, const double* c, double tol, double* uvw_valid);
+/* world inverse inertia (src/body/motion-properties.ts:494-621), column-major 3x3 */
+void oc_inverse_inertia(const double* inv_diag, const double* inertia_rot, const double* body_quat, int32_t dofs,
+                        double* out9);
+/* supporting face of a shape in local direction, transformed to world (shapes.ts:489) */
+int32_t oc_supporting_face(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t s, const double* dir,
+                           const double* pos, const double* quat, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1277 @@
+/*
+ * refcpu_collide.c — FP64 restatement of crashcat's convex narrowphase:
+ * support functions, GJK closest points, EPA, supporting faces, polygon
+ * clipping, manifold generation and contact pruning.  Oracle (test
+ * infrastructure); citations are to the reference's files.
+ */
+#include <float.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include "refcpu_internal.h"
+
+int64_t ocx_epa_calls = 0;
+
+/* ======================================================================== */
+/* support functions (src/collision/support.ts and the shape files)       */
+/* ======================================================================== */
+enum { SK_ZERO, SK_SPHERE_R, SK_BOX, SK_CAP_NC, SK_CAP_R, SK_HULL };
+typedef struct {
+    int kind;
+    double r, hh;
+    V3 he;
+    const double* pts;
+    int np;
+    double convexRadius;
+    int transformed;
+    M3 R;
+    V3 t;
+    int addRadius;
+    double addR;
+} OSup;
+
+#define MODE_INCLUDE 0
+#define MODE_EXCLUDE 1
+
+/* getShapeSupportFunction for unit scale: sphere.ts:246-256, support.ts:336-364 (box),
+ * capsule.ts:392-404, convex-hull.ts:979-996 */
+static void sup_for_shape(OSup* s, const OShapes* sh, int si, int mode) {
+    const cc_shape_desc* d = &sh->shapes[si];
+    memset(s, 0, sizeof(*s));
+    switch (d->type) {
+    case CC_SHAPE_SPHERE:
+        if (mode == MODE_INCLUDE) { s->kind = SK_SPHERE_R; s->r = d->radius; s->convexRadius = 0; }
+        else { s->kind = SK_ZERO; s->convexRadius = d->radius; }
+        break;
+    case CC_SHAPE_BOX:
+        s->kind = SK_BOX;
+        if (mode == MODE_EXCLUDE) {
+            double cr = dmin(d->convex_radius * 1.0, 0.05); /* min(r * minScale, DEFAULT_CONVEX_RADIUS) */
+            s->he = v3(dmax(0, d->half_extents[0] - cr), dmax(0, d->half_extents[1] - cr), dmax(0, d->half_extents[2] - cr));
+            s->convexRadius = cr;
+        } else {
+            s->he = vload(d->half_extents);
+            s->convexRadius = 0;
+        }
+        break;
+    case CC_SHAPE_CAPSULE:
+        s->hh = d->half_height;
+        if (mode == MODE_INCLUDE) { s->kind = SK_CAP_R; s->r = d->radius; s->convexRadius = 0; }
+        else { s->kind = SK_CAP_NC; s->convexRadius = d->radius; }
+        break;
+    case CC_SHAPE_CONVEX_HULL:
+        s->kind = SK_HULL;
+        s->np = d->hull_num_points;
+        if (mode == MODE_INCLUDE || d->convex_radius == 0) {
+            s->pts = sh->hulls->points + 3 * d->hull_first_point;
+            s->convexRadius = 0;
+        } else {
+            s->pts = sh->hulls->shrunk_points + 3 * d->hull_first_point;
+            s->convexRadius = d->convex_radius;
+        }
+        break;
+    default:
+        fprintf(stderr, "refcpu: unsupported shape type %d\n", d->type);
+        abort();
+    }
+}
+
+static V3 sup_base(const OSup* s, V3 d) {
+    switch (s->kind) {
+    case SK_ZERO:
+        return v3(0, 0, 0);
+    case SK_SPHERE_R: { /* sphere.ts:214-231 */
+        double l2 = vlen2(d);
+        if (l2 > 0) { double sc = s->r / sqrt(l2); return vscale(d, sc); }
+        return v3(0, 0, 0);
+    }
+    case SK_BOX: /* support.ts:319-323 */
+        return v3(d.x >= 0 ? s->he.x : -s->he.x, d.y >= 0 ? s->he.y : -s->he.y, d.z >= 0 ? s->he.z : -s->he.z);
+    case SK_CAP_NC: /* capsule.ts:291-297 */
+        return v3(0, d.y > 0 ? s->hh : -s->hh, 0);
+    case SK_CAP_R: { /* capsule.ts:330-345 */
+        double l2 = vlen2(d);
+        if (l2 > 0) {
+            double sc = s->r / sqrt(l2);
+            return v3(d.x * sc, d.y * sc + (d.y > 0 ? s->hh : -s->hh), d.z * sc);
+        }
+        return v3(0, s->hh, 0);
+    }
+    case SK_HULL: { /* convex-hull.ts:588-608, 698-716 */
+        double best = -INFINITY;
+        V3 out = v3(0, 0, 0);
+        for (int i = 0; i < s->np; i++) {
+            V3 p = vload(s->pts + 3 * i);
+            double dt = vdot(p, d);
+            if (dt > best) { best = dt; out = p; }
+        }
+        return out;
+    }
+    }
+    return v3(0, 0, 0);
+}
+
+/* TransformedSupport (support.ts:236-261) and AddConvexRadiusSupport (support.ts:288-303) */
+static V3 sup_get(const OSup* s, V3 d) {
+    V3 out;
+    if (s->transformed) {
+        V3 ld = m3_mul_t(&s->R, d);
+        V3 p = sup_base(s, ld);
+        out = vadd(m3_mul(&s->R, p), s->t);
+    } else {
+        out = sup_base(s, d);
+    }
+    if (s->addRadius) {
+        double l2 = vlen2(d);
+        if (l2 > 0) {
+            double sc = s->addR / sqrt(l2);
+            out = vadd(out, vscale(d, sc));
+        }
+    }
+    return out;
+}
+
+/* ======================================================================== */
+/* barycentric coordinates (src/collision/closest-points.ts:19-140)          */
+/* ======================================================================== */
+typedef struct { double u, v, w; int valid; } Bary;
+
+static Bary bary2(V3 a, V3 b, double tol2) {
+    Bary r = {0, 0, 0, 0};
+    V3 ab = vsub(b, a);
+    double den = vlen2(ab);
+    if (den < tol2) {
+        if (vlen2(a) < vlen2(b)) { r.u = 1; r.v = 0; }
+        else { r.u = 0; r.v = 1; }
+        r.valid = 0;
+        return r;
+    }
+    r.v = -vdot(a, ab) / den;
+    r.u = 1.0 - r.v;
+    r.valid = 1;
+    return r;
+}
+
+static Bary bary3(V3 a, V3 b, V3 c, double tol2) {
+    Bary r = {0, 0, 0, 0};
+    V3 ab = vsub(b, a), ac = vsub(c, a), bc = vsub(c, b);
+    double d00 = vlen2(ab), d11 = vlen2(ac), d22 = vlen2(bc);
+    if (d00 <= d22) {
+        double d01 = vdot(ab, ac);
+        double den = d00 * d11 - d01 * d01;
+        if (fabs(den) < 1.0e-12) {
+            if (d00 > d11) { Bary o = bary2(a, b, tol2); r.u = o.u; r.v = o.v; r.w = 0; }
+            else { Bary o = bary2(a, c, tol2); r.u = o.u; r.w = o.v; r.v = 0; }
+            r.valid = 0;
+            return r;
+        }
+        double a0 = vdot(a, ab), a1 = vdot(a, ac);
+        r.v = (d01 * a1 - d11 * a0) / den;
+        r.w = (d01 * a0 - d00 * a1) / den;
+        r.u = 1.0 - r.v - r.w;
+    } else {
+        double d12 = vdot(ac, bc);
+        double den = d11 * d22 - d12 * d12;
+        if (fabs(den) < 1.0e-12) {
+            if (d11 > d22) { Bary o = bary2(a, c, tol2); r.u = o.u; r.w = o.v; r.v = 0; }
+            else { Bary o = bary2(b, c, tol2); r.v = o.u; r.w = o.v; r.u = 0; }
+            r.valid = 0;
+            return r;
+        }
+        double c1 = vdot(c, ac), c2 = vdot(c, bc);
+        r.u = (d22 * c1 - d12 * c2) / den;
+        r.v = (d11 * c2 - d12 * c1) / den;
+        r.w = 1.0 - r.u - r.v;
+    }
+    r.valid = 1;
+    return r;
+}
+
+/* ======================================================================== */
+/* GJK (src/collision/gjk.ts)                                               */
+/* ======================================================================== */
+typedef struct { V3 y, p, q; } SPt;
+typedef struct { SPt pt[4]; int size; } Simplex;
+typedef struct { V3 point; int set; } CPRes;
+
+static double clamp01(double x) { return x < 0 ? 0 : (x > 1 ? 1 : x); }
+
+/* computeClosestPointOnLine gjk.ts:33-55 */
+static CPRes cp_line(V3 a, V3 b, double tol2) {
+    Bary bc = bary2(a, b, tol2);
+    CPRes r;
+    if (bc.v <= 0.0) { r.point = a; r.set = 1; }
+    else if (bc.u <= 0.0) { r.point = b; r.set = 2; }
+    else { r.point = vadd(vscale(a, bc.u), vscale(b, bc.v)); r.set = 3; }
+    return r;
+}
+
+/* computeClosestPointOnTriangle gjk.ts:57-309 */
+static CPRes cp_triangle(V3 inA, V3 inB, V3 inC, int mustIncludeC, double tol2) {
+    CPRes r;
+    V3 ac0 = vsub(inC, inA), bc0 = vsub(inC, inB);
+    int swapAC = vlen2(bc0) < vlen2(ac0);
+    V3 a = swapAC ? inC : inA;
+    V3 c = swapAC ? inA : inC;
+    V3 ab = vsub(inB, a), ac = vsub(c, a);
+    V3 n = vcross(ab, ac);
+    double n2 = vlen2(n);
+    if (n2 < 1.0e-10) {
+        int set = 4;
+        V3 best = inC;
+        double bestD = vlen2(inC);
+        if (!mustIncludeC) {
+            double a2 = vlen2(inA);
+            if (a2 < bestD) { set = 1; best = inA; bestD = a2; }
+            double b2 = vlen2(inB);
+            if (b2 < bestD) { set = 2; best = inB; bestD = b2; }
+        }
+        V3 ac2 = vsub(c, a);
+        double acl = vlen2(ac2);
+        if (acl > tol2) {
+            double v = clamp01(-vdot(a, ac2) / acl);
+            V3 q = vadd(a, vscale(ac2, v));
+            double d2 = vlen2(q);
+            if (d2 < bestD) { set = 5; best = q; bestD = d2; }
+        }
+        V3 bc2 = vsub(inC, inB);
+        double bcl = vlen2(bc2);
+        if (bcl > tol2) {
+            double v = clamp01(-vdot(inB, bc2) / bcl);
+            V3 q = vadd(inB, vscale(bc2, v));
+            double d2 = vlen2(q);
+            if (d2 < bestD) { set = 6; best = q; bestD = d2; }
+        }
+        if (!mustIncludeC) {
+            V3 ab2 = vsub(inB, inA);
+            double abl = vlen2(ab2);
+            if (abl > tol2) {
+                double v = clamp01(-vdot(inA, ab2) / abl);
+                V3 q = vadd(inA, vscale(ab2, v));
+                double d2 = vlen2(q);
+                if (d2 < bestD) { set = 3; best = q; }
+            }
+        }
+        r.set = set;
+        r.point = best;
+        return r;
+    }
+    V3 ap = vneg(a);
+    double d1 = vdot(ab, ap), d2 = vdot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) { r.set = swapAC ? 4 : 1; r.point = a; return r; }
+    V3 bp = vneg(inB);
+    double d3 = vdot(ab, bp), d4 = vdot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) { r.set = 2; r.point = inB; return r; }
+    if (d1 * d4 <= d3 * d2 && d1 >= 0.0 && d3 <= 0.0) {
+        double v = d1 / (d1 - d3);
+        r.set = swapAC ? 6 : 3;
+        r.point = vadd(a, vscale(ab, v));
+        return r;
+    }
+    V3 cp = vneg(c);
+    double d5 = vdot(ab, cp), d6 = vdot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) { r.set = swapAC ? 1 : 4; r.point = c; return r; }
+    if (d5 * d2 <= d1 * d6 && d2 >= 0.0 && d6 <= 0.0) {
+        double w = d2 / (d2 - d6);
+        r.set = 5;
+        r.point = vadd(a, vscale(ac, w));
+        return r;
+    }
+    double d43 = d4 - d3, d56 = d5 - d6;
+    if (d3 * d6 <= d5 * d4 && d43 >= 0.0 && d56 >= 0.0) {
+        double w = d43 / (d43 + d56);
+        r.set = swapAC ? 3 : 6;
+        r.point = vadd(inB, vscale(vsub(c, inB), w));
+        return r;
+    }
+    r.set = 7;
+    V3 sum = vadd(vadd(a, inB), c);
+    double sc = vdot(sum, n) / (3 * n2);
+    r.point = vscale(n, sc);
+    return r;
+}
+
+/* isOriginOutsideOfTrianglePlanes gjk.ts:314-394 */
+static void origin_outside(int out[4], V3 a, V3 b, V3 c, V3 d, double tol) {
+    V3 ab = vsub(b, a), ac = vsub(c, a), ad = vsub(d, a), bd = vsub(d, b), bc = vsub(c, b);
+    V3 abac = vcross(ab, ac), acad = vcross(ac, ad), adab = vcross(ad, ab), bdbc = vcross(bd, bc);
+    double sp[4] = {vdot(a, abac), vdot(a, acad), vdot(a, adab), vdot(b, bdbc)};
+    double sd[4] = {vdot(ad, abac), vdot(ab, acad), vdot(ac, adab), -vdot(ab, bdbc)};
+    if (sd[0] > 0 && sd[1] > 0 && sd[2] > 0 && sd[3] > 0) {
+        for (int i = 0; i < 4; i++) out[i] = sp[i] >= -tol;
+        return;
+    }
+    if (sd[0] < 0 && sd[1] < 0 && sd[2] < 0 && sd[3] < 0) {
+        for (int i = 0; i < 4; i++) out[i] = sp[i] <= tol;
+        return;
+    }
+    out[0] = out[1] = out[2] = out[3] = 1;
+}
+
+/* computeClosestPointOnTetrahedron gjk.ts:399-487 */
+static CPRes cp_tetra(V3 a, V3 b, V3 c, V3 d, int mustIncludeD, double tol) {
+    double tol2 = tol * tol;
+    CPRes r;
+    r.set = 15;
+    r.point = v3(0, 0, 0);
+    double best = INFINITY;
+    int o[4];
+    origin_outside(o, a, b, c, d, tol);
+    if (o[0]) {
+        if (mustIncludeD) { r.set = 1; r.point = a; }
+        else r = cp_triangle(a, b, c, 0, tol2);
+        best = vlen2(r.point);
+    }
+    if (o[1]) {
+        CPRes t = cp_triangle(a, c, d, mustIncludeD, tol2);
+        double d2 = vlen2(t.point);
+        if (d2 < best) { best = d2; r.point = t.point; r.set = (t.set & 1) + ((t.set & 6) << 1); }
+    }
+    if (o[2]) {
+        CPRes t = cp_triangle(a, b, d, mustIncludeD, tol2);
+        double d2 = vlen2(t.point);
+        if (d2 < best) { best = d2; r.point = t.point; r.set = (t.set & 3) + ((t.set & 4) << 1); }
+    }
+    if (o[3]) {
+        CPRes t = cp_triangle(b, c, d, mustIncludeD, tol2);
+        double d2 = vlen2(t.point);
+        if (d2 < best) { r.point = t.point; r.set = t.set << 1; }
+    }
+    return r;
+}
+
+typedef struct { V3 point; double dist2; int set; int found; } CPSRes;
+
+/* computeClosestPointToSimplex gjk.ts:512-596 */
+static void cp_simplex(CPSRes* res, double prevDist2, int lastPointPartOfClosest, const Simplex* s) {
+    CPRes cp;
+    switch (s->size) {
+    case 1: cp.set = 1; cp.point = s->pt[0].y; break;
+    case 2: cp = cp_line(s->pt[0].y, s->pt[1].y, 1e-10); break;
+    case 3: cp = cp_triangle(s->pt[0].y, s->pt[1].y, s->pt[2].y, lastPointPartOfClosest, 1e-10); break;
+    default: cp = cp_tetra(s->pt[0].y, s->pt[1].y, s->pt[2].y, s->pt[3].y, lastPointPartOfClosest, 1e-5); break;
+    }
+    double d2 = vlen2(cp.point);
+    if (d2 < prevDist2) {
+        res->point = cp.point;
+        res->dist2 = d2;
+        res->set = cp.set;
+        res->found = 1;
+        return;
+    }
+    res->found = 0;
+}
+
+#define GJK_TOLERANCE 1e-5
+#define GJK_MAX_ITERATIONS 100
+
+typedef struct { V3 pointA, pointB, axis; double dist2; Simplex simplex; } GjkOut;
+
+/* gjkClosestPoints gjk.ts:1294-1546 */
+static void gjk_closest_points(GjkOut* out, const OSup* A, const OSup* B, double tolerance, V3 direction,
+                               double maxDist2) {
+    double tol2 = tolerance * tolerance;
+    Simplex s;
+    s.size = 0;
+    CPSRes c;
+    c.point = direction;
+    c.dist2 = vlen2(direction);
+    c.set = 0;
+    c.found = 1;
+    double prevDist2 = DBL_MAX;
+    int it = 0;
+    while (it++ < GJK_MAX_ITERATIONS) {
+        V3 p = sup_get(A, c.point);
+        V3 q = sup_get(B, vneg(c.point));
+        V3 w = vsub(p, q);
+        double dot = vdot(c.point, w);
+        if (dot < 0.0 && dot * dot > c.dist2 * maxDist2) {
+            out->dist2 = DBL_MAX;
+            out->axis = c.point;
+            out->simplex.size = 0;
+            out->pointA = out->pointB = v3(0, 0, 0);
+            return;
+        }
+        s.pt[s.size].y = w;
+        s.pt[s.size].p = p;
+        s.pt[s.size].q = q;
+        s.size++;
+        cp_simplex(&c, prevDist2, 1, &s);
+        if (!c.found) { s.size--; break; }
+        if (c.set == 0xf) { c.point = v3(0, 0, 0); c.dist2 = 0.0; break; }
+        int ns = 0;
+        for (int i = 0; i < s.size; i++) {
+            if (c.set & (1 << i)) {
+                if (ns != i) s.pt[ns] = s.pt[i];
+                ns++;
+            }
+        }
+        s.size = ns;
+        if (c.dist2 <= tol2) { c.point = v3(0, 0, 0); c.dist2 = 0.0; break; }
+        double ymax = 0;
+        for (int i = 0; i < s.size; i++) ymax = dmax(ymax, vlen2(s.pt[i].y));
+        if (c.dist2 <= GJK_TOLERANCE * ymax) { c.point = v3(0, 0, 0); c.dist2 = 0.0; break; }
+        c.point = vneg(c.point);
+        if (prevDist2 - c.dist2 <= GJK_TOLERANCE * prevDist2) break;
+        prevDist2 = c.dist2;
+    }
+    out->pointA = out->pointB = v3(0, 0, 0);
+    out->simplex = s;
+    if (s.size == 0) {
+        out->dist2 = DBL_MAX;
+        out->axis = v3(0, 0, 0);
+        return;
+    }
+    if (s.size == 1) {
+        out->pointA = s.pt[0].p;
+        out->pointB = s.pt[0].q;
+    } else if (s.size == 2) {
+        Bary b = bary2(s.pt[0].y, s.pt[1].y, 1e-10);
+        out->pointA = vadd(vscale(s.pt[0].p, b.u), vscale(s.pt[1].p, b.v));
+        out->pointB = vadd(vscale(s.pt[0].q, b.u), vscale(s.pt[1].q, b.v));
+    } else if (s.size == 3) {
+        Bary b = bary3(s.pt[0].y, s.pt[1].y, s.pt[2].y, 1e-10);
+        out->pointA = vadd(vadd(vscale(s.pt[0].p, b.u), vscale(s.pt[1].p, b.v)), vscale(s.pt[2].p, b.w));
+        out->pointB = vadd(vadd(vscale(s.pt[0].q, b.u), vscale(s.pt[1].q, b.v)), vscale(s.pt[2].q, b.w));
+    }
+    out->axis = c.point;
+    out->dist2 = c.dist2;
+}
+
+/* ======================================================================== */
+/* penetration depth (src/collision/penetration.ts)                         */
+/* ======================================================================== */
+enum { PD_NOT_COLLIDING = 0, PD_COLLIDING = 1, PD_INDETERMINATE = 2 };
+typedef struct { int status; V3 axis, pointA, pointB; } PenDepth;
+
+/* penetrationDepthStepGJK penetration.ts:35-110 */
+static void pd_step_gjk(PenDepth* out, Simplex* outSimplex, const OSup* A, const OSup* B, double crA, double crB,
+                        V3 dir, double tol) {
+    double cr = crA + crB, cr2 = cr * cr;
+    GjkOut g;
+    gjk_closest_points(&g, A, B, tol, dir, cr2);
+    if (g.dist2 > cr2) {
+        out->pointA = out->pointB = out->axis = v3(0, 0, 0);
+        out->status = PD_NOT_COLLIDING;
+        return;
+    }
+    out->pointA = g.pointA;
+    out->pointB = g.pointB;
+    out->axis = g.axis;
+    *outSimplex = g.simplex;
+    if (g.dist2 > 0.0) {
+        double vl = sqrt(g.dist2);
+        out->pointA = vadd(out->pointA, vscale(out->axis, crA / vl));
+        out->pointB = vadd(out->pointB, vscale(out->axis, -(crB / vl)));
+        out->status = PD_COLLIDING;
+        return;
+    }
+    out->status = PD_INDETERMINATE;
+}
+
+/* ---- EPA convex hull builder (src/collision/epa-convex-hull-builder.ts) --- */
+#define EPA_MAX_TRIANGLES 256
+#define EPA_MAX_EDGE_LENGTH 128
+#define EPA_MIN_TRIANGLE_AREA 1e-10
+#define EPA_BARYCENTRIC_EPSILON 1e-3
+#define EPA_MAX_POINTS_TO_INCLUDE_ORIGIN_IN_HULL 32
+#define EPA_MAX_POINTS 128
+#define EPA_POINT_CAP 256
+
+typedef struct { int nbT, nbE, start; } EEdge;
+typedef struct {
+    EEdge e[3];
+    V3 normal, centroid;
+    double closestLengthSq;
+    double lambda[2];
+    int lambdaRel0, interior, removed, inQueue;
+} ETri;
+typedef struct { int tri, edge, iter; } EStack;
+
+typedef struct {
+    ETri tris[EPA_MAX_TRIANGLES];
+    int freeList[EPA_MAX_TRIANGLES * 4];
+    int nfree;
+    int queue[EPA_MAX_TRIANGLES * 4];
+    int nq;
+    const V3* pos;
+    EStack stack[EPA_MAX_EDGE_LENGTH];
+    EEdge edges[EPA_MAX_EDGE_LENGTH];
+    int nedges;
+} EHull;
+
+static int tri_facing(const ETri* t, V3 p) { return vdot(t->normal, vsub(p, t->centroid)) > 0.0; }
+
+static void heap_push(EHull* h, int ti) { /* pushTriangleToQueue :95-116 */
+    h->queue[h->nq++] = ti;
+    h->tris[ti].inQueue = 1;
+    int cur = h->nq - 1;
+    while (cur > 0) {
+        int par = (cur - 1) >> 1;
+        if (h->tris[h->queue[par]].closestLengthSq > h->tris[h->queue[cur]].closestLengthSq) {
+            int t = h->queue[par]; h->queue[par] = h->queue[cur]; h->queue[cur] = t;
+            cur = par;
+        } else break;
+    }
+}
+
+static int heap_pop(EHull* h) { /* popClosestTriangleFromQueue :381-418 */
+    if (h->nq == 0) return -1;
+    int* q = h->queue;
+    int tmp = q[h->nq - 1]; q[h->nq - 1] = q[0]; q[0] = tmp;
+    int count = h->nq - 1;
+    int largest = 0;
+    for (;;) {
+        int child = (largest << 1) + 1;
+        if (child >= count) break;
+        int prev = largest;
+        if (h->tris[q[largest]].closestLengthSq > h->tris[q[child]].closestLengthSq) largest = child;
+        ++child;
+        if (child < count && h->tris[q[largest]].closestLengthSq > h->tris[q[child]].closestLengthSq) largest = child;
+        if (prev == largest) break;
+        int t = q[prev]; q[prev] = q[largest]; q[largest] = t;
+    }
+    return q[--h->nq];
+}
+
+static void free_tri(EHull* h, int ti) { h->freeList[h->nfree++] = ti; }
+
+/* createTriangle :179-345 */
+static int create_tri(EHull* h, int i1, int i2, int i3) {
+    if (h->nfree == 0) return -1;
+    int ti = h->freeList[--h->nfree];
+    ETri* t = &h->tris[ti];
+    t->closestLengthSq = INFINITY;
+    t->lambda[0] = t->lambda[1] = 0;
+    t->lambdaRel0 = 0; t->interior = 0; t->removed = 0; t->inQueue = 0;
+    t->e[0].start = i1; t->e[1].start = i2; t->e[2].start = i3;
+    t->e[0].nbT = t->e[1].nbT = t->e[2].nbT = -1;
+    V3 y0 = h->pos[i1], y1 = h->pos[i2], y2 = h->pos[i3];
+    t->centroid = v3((y0.x + y1.x + y2.x) / 3.0, (y0.y + y1.y + y2.y) / 3.0, (y0.z + y1.z + y2.z) / 3.0);
+    V3 y10 = vsub(y1, y0), y20 = vsub(y2, y0), y21 = vsub(y2, y1);
+    double y20d = vlen2(y20), y21d = vlen2(y21);
+    if (y20d < y21d) {
+        V3 n = vcross(y10, y20);
+        t->normal = n;
+        double nl = vlen2(n);
+        if (nl > EPA_MIN_TRIANGLE_AREA) {
+            double cdn = vdot(t->centroid, n);
+            t->closestLengthSq = fabs(cdn) * cdn / nl;
+            double y10d = vlen2(y10), y10y20 = vdot(y10, y20);
+            double det = y10d * y20d - y10y20 * y10y20;
+            if (det > 0.0) {
+                double y0y10 = vdot(y0, y10), y0y20 = vdot(y0, y20);
+                double l0 = (y10y20 * y0y20 - y20d * y0y10) / det;
+                double l1 = (y10y20 * y0y10 - y10d * y0y20) / det;
+                t->lambda[0] = l0; t->lambda[1] = l1; t->lambdaRel0 = 1;
+                if (l0 > -EPA_BARYCENTRIC_EPSILON && l1 > -EPA_BARYCENTRIC_EPSILON && l0 + l1 < 1.0 + EPA_BARYCENTRIC_EPSILON)
+                    t->interior = 1;
+            }
+        }
+    } else {
+        V3 n = vcross(y10, y21);
+        t->normal = n;
+        double nl = vlen2(n);
+        if (nl > EPA_MIN_TRIANGLE_AREA) {
+            double cdn = vdot(t->centroid, n);
+            t->closestLengthSq = fabs(cdn) * cdn / nl;
+            double y10d = vlen2(y10), y10y21 = vdot(y10, y21);
+            double det = y10d * y21d - y10y21 * y10y21;
+            if (det > 0.0) {
+                double y1y10 = vdot(y1, y10), y1y21 = vdot(y1, y21);
+                double l0 = (y21d * y1y10 - y10y21 * y1y21) / det;
+                double l1 = (y10y21 * y1y10 - y10d * y1y21) / det;
+                t->lambda[0] = l0; t->lambda[1] = l1; t->lambdaRel0 = 0;
+                if (l0 > -EPA_BARYCENTRIC_EPSILON && l1 > -EPA_BARYCENTRIC_EPSILON && l0 + l1 < 1.0 + EPA_BARYCENTRIC_EPSILON)
+                    t->interior = 1;
+            }
+        }
+    }
+    return ti;
+}
+
+static void link_tri(EHull* h, int t1, int e1, int t2, int e2) {
+    h->tris[t1].e[e1].nbT = t2; h->tris[t1].e[e1].nbE = e2;
+    h->tris[t2].e[e2].nbT = t1; h->tris[t2].e[e2].nbE = e1;
+}
+
+static void hull_initialize(EHull* h, int i1, int i2, int i3) { /* initialize :347-371 */
+    h->nfree = EPA_MAX_TRIANGLES;
+    for (int i = 0; i < EPA_MAX_TRIANGLES; i++) h->freeList[i] = EPA_MAX_TRIANGLES - 1 - i;
+    h->nq = 0;
+    int t1 = create_tri(h, i1, i2, i3);
+    int t2 = create_tri(h, i1, i3, i2);
+    link_tri(h, t1, 0, t2, 2);
+    link_tri(h, t1, 1, t2, 1);
+    link_tri(h, t1, 2, t2, 0);
+    heap_push(h, t1);
+    heap_push(h, t2);
+}
+
+static int find_facing(EHull* h, V3 p) { /* findFacingTriangle :420-445 */
+    int best = -1;
+    double bestD = 0.0;
+    for (int i = 0; i < h->nq; i++) {
+        ETri* t = &h->tris[h->queue[i]];
+        if (t->removed) continue;
+        double dot = vdot(t->normal, vsub(p, t->centroid));
+        if (dot > 0.0) {
+            double d2 = dot * dot / vlen2(t->normal);
+            if (d2 > bestD) { best = h->queue[i]; bestD = d2; }
+        }
+    }
+    return best;
+}
+
+static void unlink_tri(EHull* h, int ti) { /* unlinkTriangle :451-466 */
+    ETri* t = &h->tris[ti];
+    for (int i = 0; i < 3; i++) {
+        if (t->e[i].nbT >= 0) {
+            h->tris[t->e[i].nbT].e[t->e[i].nbE].nbT = -1;
+            t->e[i].nbT = -1;
+        }
+    }
+    if (!t->inQueue) free_tri(h, ti);
+}
+
+static int find_edge(EHull* h, int facing, V3 vtx) { /* findEdge :468-539 */
+    h->nedges = 0;
+    h->tris[facing].removed = 1;
+    int sp = 0;
+    h->stack[0].tri = facing; h->stack[0].edge = 0; h->stack[0].iter = -1;
+    int nextExpected = -1;
+    for (;;) {
+        EStack* cur = &h->stack[sp];
+        if (++cur->iter >= 3) {
+            unlink_tri(h, cur->tri);
+            if (--sp < 0) break;
+        } else {
+            EEdge e = h->tris[cur->tri].e[(cur->edge + cur->iter) % 3];
+            int n = e.nbT;
+            if (n >= 0 && !h->tris[n].removed) {
+                if (tri_facing(&h->tris[n], vtx)) {
+                    h->tris[n].removed = 1;
+                    if (sp + 1 >= EPA_MAX_EDGE_LENGTH) return 0; /* the reference's fixed stack has 128 entries */
+                    sp++;
+                    h->stack[sp].tri = n; h->stack[sp].edge = e.nbE; h->stack[sp].iter = 0;
+                } else {
+                    if (e.start != nextExpected && nextExpected != -1) return 0;
+                    nextExpected = h->tris[n].e[e.nbE].start;
+                    if (h->nedges >= EPA_MAX_EDGE_LENGTH) return 0;
+                    h->edges[h->nedges++] = e;
+                }
+            }
+        }
+    }
+    if (h->nedges > 0 && h->edges[0].start != nextExpected) return 0;
+    return h->nedges >= 3;
+}
+
+/* addPoint :541-593; new triangles returned in out[] */
+static int add_point(EHull* h, int facing, int idx, double closestDist2, int* out, int* nout) {
+    V3 p = h->pos[idx];
+    *nout = 0;
+    if (!find_edge(h, facing, p)) return 0;
+    int ne = h->nedges;
+    for (int i = 0; i < ne; i++) {
+        int in = (i + 1) % ne;
+        int nt = create_tri(h, h->edges[i].start, h->edges[in].start, idx);
+        if (nt < 0) return 0;
+        out[(*nout)++] = nt;
+        ETri* t = &h->tris[nt];
+        if ((t->interior && t->closestLengthSq < closestDist2) || t->closestLengthSq < 0.0) heap_push(h, nt);
+    }
+    for (int i = 0; i < ne; i++) {
+        int in = (i + 1) % ne;
+        link_tri(h, out[i], 0, h->edges[i].nbT, h->edges[i].nbE);
+        link_tri(h, out[i], 1, out[in], 2);
+    }
+    return 1;
+}
+
+typedef struct { V3 y[EPA_POINT_CAP], p[EPA_POINT_CAP], q[EPA_POINT_CAP]; int n; } EPts;
+
+static int epa_add(EPts* P, const OSup* A, const OSup* B, V3 d) { /* addEpaSupportPoint penetration.ts:155-184 */
+    V3 p = sup_get(A, d);
+    V3 q = sup_get(B, vneg(d));
+    int i = P->n++;
+    P->y[i] = vsub(p, q);
+    P->p[i] = p;
+    P->q[i] = q;
+    return i;
+}
+
+/* penetrationDepthStepEPA penetration.ts:214-614 */
+static int pd_step_epa(PenDepth* out, const OSup* A, const OSup* B, double tol, const Simplex* s) {
+    static EHull H; /* single-threaded oracle */
+    static EPts P;
+    ocx_epa_calls++;
+    P.n = 0;
+    for (int i = 0; i < s->size; i++) { P.y[i] = s->pt[i].y; P.p[i] = s->pt[i].p; P.q[i] = s->pt[i].q; }
+    P.n = s->size;
+    if (P.n == 1) {
+        P.n--;
+        epa_add(&P, A, B, v3(0, 1, 0));
+        epa_add(&P, A, B, v3(-1, -1, -1));
+        epa_add(&P, A, B, v3(1, -1, -1));
+        epa_add(&P, A, B, v3(0, -1, 1));
+    } else if (P.n == 2) {
+        V3 axis = vsub(P.y[1], P.y[0]);
+        double al = vlen(axis);
+        V3 an = v3(axis.x / al, axis.y / al, axis.z / al);
+        V3 d1;
+        if (fabs(an.x) > fabs(an.y)) {
+            double pl = sqrt(an.x * an.x + an.z * an.z);
+            d1 = v3(an.z / pl, 0.0, -an.x / pl);
+        } else {
+            double pl = sqrt(an.y * an.y + an.z * an.z);
+            d1 = v3(0.0, an.z / pl, -an.y / pl);
+        }
+        const double c = cos(120.0 * M_PI / 180.0), sn = sin(120.0 * M_PI / 180.0), omc = 1 - c;
+        double r00 = c + an.x * an.x * omc, r01 = an.x * an.y * omc - an.z * sn, r02 = an.x * an.z * omc + an.y * sn;
+        double r10 = an.y * an.x * omc + an.z * sn, r11 = c + an.y * an.y * omc, r12 = an.y * an.z * omc - an.x * sn;
+        double r20 = an.z * an.x * omc - an.y * sn, r21 = an.z * an.y * omc + an.x * sn, r22 = c + an.z * an.z * omc;
+        V3 d2 = v3(r00 * d1.x + r01 * d1.y + r02 * d1.z, r10 * d1.x + r11 * d1.y + r12 * d1.z, r20 * d1.x + r21 * d1.y + r22 * d1.z);
+        V3 d3 = v3(r00 * d2.x + r01 * d2.y + r02 * d2.z, r10 * d2.x + r11 * d2.y + r12 * d2.z, r20 * d2.x + r21 * d2.y + r22 * d2.z);
+        epa_add(&P, A, B, d1);
+        epa_add(&P, A, B, d2);
+        epa_add(&P, A, B, d3);
+    }
+    H.pos = P.y;
+    hull_initialize(&H, 0, 1, 2);
+    int newT[EPA_MAX_EDGE_LENGTH];
+    int nnew;
+    for (int i = 3; i < P.n; i++) {
+        int t = find_facing(&H, P.y[i]);
+        if (t >= 0) {
+            if (!add_point(&H, t, i, DBL_MAX, newT, &nnew)) { out->status = PD_NOT_COLLIDING; return 0; }
+        }
+    }
+    for (;;) {
+        if (H.nq == 0) { out->status = PD_NOT_COLLIDING; return 0; }
+        int t = H.queue[0];
+        if (H.tris[t].removed) {
+            heap_pop(&H);
+            if (H.nq == 0) { out->status = PD_NOT_COLLIDING; return 0; }
+            free_tri(&H, t);
+            continue;
+        }
+        if (H.tris[t].closestLengthSq >= 0.0) break;
+        heap_pop(&H);
+        int ni = epa_add(&P, A, B, H.tris[t].normal);
+        V3 w = P.y[ni];
+        if (!tri_facing(&H.tris[t], w) || !add_point(&H, t, ni, DBL_MAX, newT, &nnew)) {
+            out->status = PD_NOT_COLLIDING;
+            return 0;
+        }
+        free_tri(&H, t);
+        if (H.nq == 0 || P.n >= EPA_MAX_POINTS_TO_INCLUDE_ORIGIN_IN_HULL) { out->status = PD_NOT_COLLIDING; return 0; }
+    }
+    double closestDist2 = DBL_MAX;
+    int last = -1;
+    int flip = 0;
+    do {
+        int t = heap_pop(&H);
+        if (t < 0) { out->status = PD_NOT_COLLIDING; return 0; }
+        ETri* T = &H.tris[t];
+        if (T->removed) { free_tri(&H, t); continue; }
+        if (T->closestLengthSq >= closestDist2) break;
+        if (last >= 0) free_tri(&H, last);
+        last = t;
+        int ni = epa_add(&P, A, B, T->normal);
+        V3 w = P.y[ni];
+        double dot = vdot(T->normal, w);
+        if (dot < 0.0) { out->status = PD_NOT_COLLIDING; return 0; }
+        double d2 = dot * dot / vlen2(T->normal);
+        if (d2 - T->closestLengthSq < T->closestLengthSq * tol) break;
+        closestDist2 = dmin(closestDist2, d2);
+        if (!tri_facing(T, w)) break;
+        if (!add_point(&H, t, ni, closestDist2, newT, &nnew)) break;
+        int defect = 0;
+        for (int i = 0; i < nnew; i++) {
+            ETri* nt = &H.tris[newT[i]];
+            if (vdot(nt->normal, nt->centroid) < 0.0) { defect = 1; break; }
+        }
+        if (defect) {
+            V3 nn = vneg(T->normal);
+            V3 p2 = sup_get(A, nn);
+            V3 q2 = sup_get(B, T->normal);
+            double dot2 = vdot(nn, vsub(p2, q2));
+            if (dot2 < dot) flip = 1;
+            break;
+        }
+    } while (H.nq > 0 && P.n < EPA_MAX_POINTS);
+    if (last < 0) { out->status = PD_NOT_COLLIDING; return 0; }
+    ETri* L = &H.tris[last];
+    double nl2 = vlen2(L->normal);
+    double cdn = vdot(L->centroid, L->normal);
+    V3 pn = vscale(L->normal, cdn / nl2);
+    if (vlen2(pn) < 1e-10) { out->status = PD_NOT_COLLIDING; return 0; }
+    if (flip) pn = vneg(pn);
+    int i0 = L->e[0].start, i1 = L->e[1].start, i2 = L->e[2].start;
+    V3 ca, cb;
+    double l0 = L->lambda[0], l1 = L->lambda[1];
+    if (L->lambdaRel0) {
+        ca = vadd(vadd(P.p[i0], vscale(vsub(P.p[i1], P.p[i0]), l0)), vscale(vsub(P.p[i2], P.p[i0]), l1));
+        cb = vadd(vadd(P.q[i0], vscale(vsub(P.q[i1], P.q[i0]), l0)), vscale(vsub(P.q[i2], P.q[i0]), l1));
+    } else {
+        ca = vadd(vadd(P.p[i1], vscale(vsub(P.p[i0], P.p[i1]), l0)), vscale(vsub(P.p[i2], P.p[i1]), l1));
+        cb = vadd(vadd(P.q[i1], vscale(vsub(P.q[i0], P.q[i1]), l0)), vscale(vsub(P.q[i2], P.q[i1]), l1));
+    }
+    out->axis = pn;
+    out->pointA = ca;
+    out->pointB = cb;
+    out->status = PD_COLLIDING;
+    return 1;
+}
+
+/* ======================================================================== */
+/* supporting faces                                                          */
+/* ======================================================================== */
+/* transformFace (src/utils/face.ts:30-84) for unit scale */
+static void face_transform(OFace* f, V3 pos, Q4 q) {
+    M3 R = m3_from_quat(q);
+    for (int i = 0; i < f->n; i++) f->v[i] = vadd(m3_mul(&R, f->v[i]), pos);
+}
+
+static void face_box(OFace* f, V3 d, V3 he) { /* box.ts:129-332, insideOut = false */
+    double hx = he.x, hy = he.y, hz = he.z;
+    double ax = fabs(d.x), ay = fabs(d.y), az = fabs(d.z);
+    f->n = 4;
+    if (ax >= ay && ax >= az) {
+        if (d.x < 0) { f->v[0] = v3(hx, -hy, -hz); f->v[1] = v3(hx, hy, -hz); f->v[2] = v3(hx, hy, hz); f->v[3] = v3(hx, -hy, hz); }
+        else { f->v[0] = v3(-hx, -hy, -hz); f->v[1] = v3(-hx, -hy, hz); f->v[2] = v3(-hx, hy, hz); f->v[3] = v3(-hx, hy, -hz); }
+    } else if (ay >= ax && ay >= az) {
+        if (d.y < 0) { f->v[0] = v3(-hx, hy, -hz); f->v[1] = v3(-hx, hy, hz); f->v[2] = v3(hx, hy, hz); f->v[3] = v3(hx, hy, -hz); }
+        else { f->v[0] = v3(-hx, -hy, -hz); f->v[1] = v3(hx, -hy, -hz); f->v[2] = v3(hx, -hy, hz); f->v[3] = v3(-hx, -hy, hz); }
+    } else {
+        if (d.z < 0) { f->v[0] = v3(-hx, -hy, hz); f->v[1] = v3(hx, -hy, hz); f->v[2] = v3(hx, hy, hz); f->v[3] = v3(-hx, hy, hz); }
+        else { f->v[0] = v3(-hx, -hy, -hz); f->v[1] = v3(-hx, hy, -hz); f->v[2] = v3(hx, hy, -hz); f->v[3] = v3(hx, -hy, -hz); }
+    }
+}
+
+static void face_capsule(OFace* f, V3 d, double hh, double r) { /* capsule.ts:193-253 */
+    double len = sqrt(d.x * d.x + d.z * d.z);
+    if (len == 0.0) { f->n = 0; return; }
+    double sx = (r / len) * d.x, sz = (r / len) * d.z;
+    V3 top = v3(-sx, hh, -sz), bot = v3(-sx, -hh, -sz);
+    double pt = vdot(top, d), pb = vdot(bot, d);
+    double dl = vlen(d);
+    if (fabs(pt - pb) < 0.02 * dl) { f->n = 2; f->v[0] = top; f->v[1] = bot; }
+    else f->n = 0;
+}
+
+static void face_hull(OFace* f, const OShapes* sh, const cc_shape_desc* s, V3 d) { /* convex-hull.ts:479-548 */
+    const cc_hull_data* H = sh->hulls;
+    int f0 = s->hull_first_face;
+    V3 n0 = vload(H->plane_normals + 3 * f0);
+    double bestDot = vdot(n0, d) / vlen(n0);
+    int best = 0;
+    for (int i = 1; i < s->hull_num_faces; i++) {
+        V3 n = vload(H->plane_normals + 3 * (f0 + i));
+        double dt = vdot(n, d) / vlen(n);
+        if (dt < bestDot) { bestDot = dt; best = i; }
+    }
+    int first = H->face_first_vertex[f0 + best], nv = H->face_num_vertices[f0 + best];
+    int maxv = 32 / 2;
+    int delta = (nv + maxv - 1) / maxv;
+    f->n = 0;
+    for (int i = 0; i < nv; i += delta) {
+        int vi = H->face_vertex_indices[first + i];
+        f->v[f->n++] = vload(H->points + 3 * (s->hull_first_point + vi));
+    }
+}
+
+/* getShapeSupportingFace (src/shapes/shapes.ts:489-509) */
+static void supporting_face(OFace* f, const OShapes* sh, int si, V3 dir, V3 pos, Q4 q) {
+    const cc_shape_desc* s = &sh->shapes[si];
+    switch (s->type) {
+    case CC_SHAPE_SPHERE: f->n = 0; return;
+    case CC_SHAPE_BOX: face_box(f, dir, vload(s->half_extents)); break;
+    case CC_SHAPE_CAPSULE: face_capsule(f, dir, s->half_height, s->radius); if (f->n == 0) return; break;
+    case CC_SHAPE_CONVEX_HULL: face_hull(f, sh, s, dir); break;
+    default: f->n = 0; return;
+    }
+    face_transform(f, pos, q);
+}
+
+/* ======================================================================== */
+/* collide                                                                   */
+/* ======================================================================== */
+/* v' = v + 2 * cross(q.xyz, cross(q.xyz, v) + q.w * v)   (convex.ts inline form) */
+static V3 qrot2(Q4 q, V3 v) {
+    double uvx = q.y * v.z - q.z * v.y + q.w * v.x;
+    double uvy = q.z * v.x - q.x * v.z + q.w * v.y;
+    double uvz = q.x * v.y - q.y * v.x + q.w * v.z;
+    double uuvx = q.y * uvz - q.z * uvy, uuvy = q.z * uvx - q.x * uvz, uuvz = q.x * uvy - q.y * uvx;
+    return v3(v.x + 2 * uuvx, v.y + 2 * uuvy, v.z + 2 * uuvz);
+}
+
+/* collideSphereVsSphere src/shapes/sphere.ts:319-410 */
+static void collide_sphere_sphere(const OShapes* sh, int sa, V3 pa, int sb, V3 pb, OHit* out) {
+    double ra = sh->shapes[sa].radius, rb = sh->shapes[sb].radius;
+    V3 d = vsub(pb, pa);
+    double d2 = vlen2(d), rs = ra + rb;
+    out->hit = 0;
+    if (d2 >= rs * rs) return;
+    double dist = sqrt(d2);
+    double pen = rs - dist;
+    V3 n;
+    if (dist > 0) { double inv = 1 / dist; n = vscale(d, inv); }
+    else n = v3(0, 1, 0);
+    out->hit = 1;
+    out->status = 0;
+    out->pointA = vadd(pa, vscale(n, ra));
+    out->pointB = vsub(pb, vscale(n, rb));
+    out->axis = n;
+    out->penetration = pen;
+    out->faceA.n = 0;
+    out->faceB.n = 0;
+}
+
+/* collideConvexVsConvex + collideConvexVsConvexLocal src/shapes/convex.ts:257-641 */
+static void collide_convex_convex(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, double maxSep,
+                                  double colTol, double penTol, OHit* out) {
+    out->hit = 0;
+    Q4 ia = {-qa.x, -qa.y, -qa.z, qa.w};
+    Q4 rel;
+    rel.x = ia.w * qb.x + ia.x * qb.w + ia.y * qb.z - ia.z * qb.y;
+    rel.y = ia.w * qb.y - ia.x * qb.z + ia.y * qb.w + ia.z * qb.x;
+    rel.z = ia.w * qb.z + ia.x * qb.y - ia.y * qb.x + ia.z * qb.w;
+    rel.w = ia.w * qb.w - ia.x * qb.x - ia.y * qb.y - ia.z * qb.z;
+    V3 relPos = qrot2(ia, vsub(pb, pa));
+
+    OSup A, B;
+    sup_for_shape(&A, sh, sa, MODE_EXCLUDE);
+    sup_for_shape(&B, sh, sb, MODE_EXCLUDE);
+    B.transformed = 1;
+    B.R = m3_from_quat(rel);
+    B.t = relPos;
+    V3 axis0 = relPos;
+    if (vlen2(axis0) <= 1e-12) axis0 = v3(1, 0, 0);
+    double maxSepUse = maxSep;
+    PenDepth pd;
+    Simplex simplex;
+    simplex.size = 0;
+    pd_step_gjk(&pd, &simplex, &A, &B, A.convexRadius + maxSepUse, B.convexRadius, axis0, colTol);
+    int status = 0;
+    if (pd.status == PD_NOT_COLLIDING) return;
+    if (pd.status == PD_INDETERMINATE) {
+        maxSepUse = dmin(maxSepUse, 1.0);
+        OSup A2, B2;
+        sup_for_shape(&A2, sh, sa, MODE_INCLUDE);
+        sup_for_shape(&B2, sh, sb, MODE_INCLUDE);
+        A2.addRadius = 1;
+        A2.addR = maxSepUse;
+        B2.transformed = 1;
+        B2.R = B.R;
+        B2.t = relPos;
+        if (!pd_step_epa(&pd, &A2, &B2, penTol, &simplex)) return;
+        status = 1;
+    }
+    V3 dd = vsub(pd.pointA, pd.pointB);
+    double pen = sqrt(vlen2(dd)) - maxSepUse;
+    double al = sqrt(vlen2(pd.axis));
+    if (al > 0.0) {
+        double corr = maxSepUse / al;
+        pd.pointA = vadd(pd.pointA, vscale(pd.axis, -corr));
+    }
+    out->hit = 1;
+    out->status = status;
+    out->pointA = vadd(qrot2(qa, pd.pointA), pa);
+    out->pointB = vadd(qrot2(qa, pd.pointB), pa);
+    out->axis = qrot2(qa, pd.axis);
+    out->penetration = pen;
+    supporting_face(&out->faceA, sh, sa, vneg(pd.axis), pa, qa);
+    Q4 crel = {-rel.x, -rel.y, -rel.z, rel.w};
+    V3 dirB = qrot2(crel, pd.axis);
+    V3 posB = vadd(qrot2(qa, relPos), pa);
+    Q4 qB;
+    qB.x = qa.w * rel.x + qa.x * rel.w + qa.y * rel.z - qa.z * rel.y;
+    qB.y = qa.w * rel.y - qa.x * rel.z + qa.y * rel.w + qa.z * rel.x;
+    qB.z = qa.w * rel.z + qa.x * rel.y - qa.y * rel.x + qa.z * rel.w;
+    qB.w = qa.w * rel.w - qa.x * rel.x - qa.y * rel.y - qa.z * rel.z;
+    supporting_face(&out->faceB, sh, sb, dirB, posB, qB);
+}
+
+/* collideShapeVsShape dispatch (src/collision/narrowphase.ts:172-238, src/shapes/sphere.ts:154-167) */
+void ocx_collide(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, double maxSep, double colTol,
+                 double penTol, OHit* out) {
+    out->faceA.n = out->faceB.n = 0;
+    if (sh->shapes[sa].type == CC_SHAPE_SPHERE && sh->shapes[sb].type == CC_SHAPE_SPHERE)
+        collide_sphere_sphere(sh, sa, pa, sb, pb, out);
+    else
+        collide_convex_convex(sh, sa, pa, qa, sb, pb, qb, maxSep, colTol, penTol, out);
+}
+
+/* ======================================================================== */
+/* clipping (src/manifold/clip.ts)                                           */
+/* ======================================================================== */
+static void clip_poly_vs_plane(OFace* out, const OFace* poly, V3 origin, V3 normal) { /* :13-92 */
+    out->n = 0;
+    if (poly->n < 2) return;
+    V3 e1 = poly->v[poly->n - 1];
+    double prevNum = vdot(vsub(origin, e1), normal);
+    int prevIn = prevNum < 0.0;
+    for (int j = 0; j < poly->n; j++) {
+        V3 e2 = poly->v[j];
+        double num = vdot(vsub(origin, e2), normal);
+        int curIn = num < 0.0;
+        if (curIn != prevIn) {
+            V3 e12 = vsub(e2, e1);
+            double den = vdot(e12, normal);
+            if (den != 0.0) {
+                double t = prevNum / den;
+                if (out->n < OC_MAX_FACE) out->v[out->n++] = vadd(e1, vscale(e12, t));
+            } else curIn = prevIn;
+        }
+        if (curIn && out->n < OC_MAX_FACE) out->v[out->n++] = e2;
+        prevNum = num;
+        prevIn = curIn;
+        e1 = e2;
+    }
+}
+
+static void clip_poly_vs_poly(OFace* out, const OFace* toClip, const OFace* clip, V3 normal) { /* :108-175 */
+    static OFace tmp[2];
+    out->n = 0;
+    if (toClip->n < 2 || clip->n < 3) return;
+    int tmpIdx = 0;
+    const OFace* src;
+    OFace* dst;
+    for (int i = 0; i < clip->n; i++) {
+        V3 c1 = clip->v[i], c2 = clip->v[(i + 1) % clip->n];
+        V3 cn = vcross(normal, vsub(c2, c1));
+        src = (i == 0) ? toClip : &tmp[tmpIdx];
+        tmpIdx ^= 1;
+        dst = (i == clip->n - 1) ? out : &tmp[tmpIdx];
+        clip_poly_vs_plane(dst, src, c1, cn);
+        if (dst->n < 3) { out->n = 0; break; }
+    }
+}
+
+static void clip_poly_vs_edge(OFace* out, const OFace* poly, V3 ev1, V3 ev2, V3 clipNormal) { /* :190-363 */
+    out->n = 0;
+    if (poly->n < 3) return;
+    V3 edge = vsub(ev2, ev1);
+    V3 en = vcross(clipNormal, edge);
+    V3 p0 = poly->v[0], p1 = poly->v[1], p2 = poly->v[2];
+    V3 pn = vcross(vsub(p2, p0), vsub(p1, p0));
+    double pl2 = vlen2(pn);
+    V3 v1 = vadd(ev1, vscale(pn, vdot(vsub(p0, ev1), pn) / pl2));
+    V3 v2 = vadd(ev2, vscale(pn, vdot(vsub(p0, ev2), pn) / pl2));
+    V3 v12 = vsub(v2, v1);
+    double v12l = vlen2(v12);
+    V3 e1 = poly->v[poly->n - 1];
+    double prevNum = vdot(vsub(ev1, e1), en);
+    int prevIn = prevNum < 0.0;
+    for (int j = 0; j < poly->n; j++) {
+        V3 e2 = poly->v[j];
+        double num = vdot(vsub(ev1, e2), en);
+        int curIn = num < 0.0;
+        if (curIn != prevIn) {
+            V3 e12 = vsub(e2, e1);
+            double den = vdot(e12, en);
+            V3 cl = (den != 0.0) ? vadd(e1, vscale(e12, prevNum / den)) : e1;
+            double pr = vdot(vsub(cl, v1), v12);
+            V3 o = pr < 0.0 ? v1 : (pr > v12l ? v2 : cl);
+            if (out->n < OC_MAX_FACE) out->v[out->n++] = o;
+        }
+        prevNum = num;
+        prevIn = curIn;
+        e1 = e2;
+    }
+}
+
+/* ======================================================================== */
+/* manifold (src/manifold/manifold.ts)                                       */
+/* ======================================================================== */
+static void set_point(OManifold* m, int i, V3 w1, V3 w2) { /* setContactPoint :152-163 */
+    m->relA[i] = vsub(w1, m->base);
+    m->relB[i] = vsub(w2, m->base);
+}
+
+/* manifoldBetweenTwoFaces :345-446 */
+void ocx_manifold_between_two_faces(OManifold* out, V3 p1, V3 p2, V3 axis, double maxDist, const OFace* f1,
+                                    const OFace* f2) {
+    out->n = 0;
+    if (f1->n < 2 || f2->n < 3) {
+        set_point(out, 0, p1, p2);
+        out->n = 1;
+        return;
+    }
+    OFace clipped;
+    clipped.n = 0;
+    if (f1->n >= 3) clip_poly_vs_poly(&clipped, f2, f1, axis);
+    else if (f1->n == 2) clip_poly_vs_edge(&clipped, f2, f1->v[0], f1->v[1], axis);
+    V3 origin = f1->v[0];
+    V3 first = vsub(f1->v[1], origin);
+    V3 pnrm;
+    if (f1->n >= 3) pnrm = vcross(first, vsub(f1->v[2], origin));
+    else pnrm = vcross(vcross(first, axis), first);
+    double apn = vdot(axis, pnrm);
+    if (apn != 0.0) {
+        double al = vlen(axis);
+        for (int i = 0; i < clipped.n; i++) {
+            V3 q2 = clipped.v[i];
+            double dist = vdot(vsub(q2, origin), pnrm) / apn;
+            if (dist * al < maxDist) {
+                V3 q1 = vadd(q2, vscale(axis, -dist));
+                set_point(out, out->n, q1, q2);
+                out->n++;
+                if (out->n >= OC_MAX_MANIFOLD_POINTS) break;
+            }
+        }
+    }
+    if (out->n == 0) {
+        set_point(out, 0, p1, p2);
+        out->n = 1;
+    }
+}
+
+/* pruneContactPoints :192-313 */
+void ocx_prune(OManifold* m, V3 axis) {
+    if (m->n <= 4) return;
+    const double MIN_D2 = 1.0e-6;
+    V3 proj[OC_MAX_MANIFOLD_POINTS];
+    double pd2[OC_MAX_MANIFOLD_POINTS];
+    for (int i = 0; i < m->n; i++) {
+        V3 v1 = m->relA[i], v2 = m->relB[i];
+        double dp = vdot(v1, axis);
+        proj[i] = vadd(v1, vscale(axis, -dp));
+        pd2[i] = dmax(MIN_D2, vlen2(vsub(v2, v1)));
+    }
+    int p1 = 0;
+    double best = dmax(MIN_D2, vlen2(proj[0])) * pd2[0];
+    for (int i = 1; i < m->n; i++) {
+        double me = dmax(MIN_D2, vlen2(proj[i])) * pd2[i];
+        if (me > best) { best = me; p1 = i; }
+    }
+    int p2 = -1;
+    best = -INFINITY;
+    for (int i = 0; i < m->n; i++) {
+        if (i == p1) continue;
+        double me = dmax(MIN_D2, vlen2(vsub(proj[i], proj[p1]))) * pd2[i];
+        if (me > best) { best = me; p2 = i; }
+    }
+    int p3 = -1, p4 = -1;
+    double minD = 0.0, maxD = 0.0;
+    V3 perp = vcross(vsub(proj[p2], proj[p1]), axis);
+    for (int i = 0; i < m->n; i++) {
+        if (i == p1 || i == p2) continue;
+        double d = vdot(vsub(proj[i], proj[p1]), perp);
+        if (d < minD) { minD = d; p3 = i; }
+        else if (d > maxD) { maxD = d; p4 = i; }
+    }
+    int order[4] = {p1, p3, p2, p4};
+    V3 ra[4], rb[4];
+    int n = 0;
+    for (int k = 0; k < 4; k++) {
+        if (order[k] != -1) { ra[n] = m->relA[order[k]]; rb[n] = m->relB[order[k]]; n++; }
+    }
+    for (int k = 0; k < n; k++) { m->relA[k] = ra[k]; m->relB[k] = rb[k]; }
+    m->n = n;
+}
+
+/* ======================================================================== */
+/* unit entry points                                                         */
+/* ======================================================================== */
+static OFace face_from(const double* v, int n) {
+    OFace f;
+    f.n = n;
+    for (int i = 0; i < n; i++) f.v[i] = vload(v + 3 * i);
+    return f;
+}
+
+void oc_collide(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t sa, const double* pa, const double* qa,
+                int32_t sb, const double* pb, const double* qb, double maxSep, const cc_settings* s, oc_hit* out) {
+    OShapes sh = {shapes, hulls};
+    OHit h;
+    Q4 QA = {qa[0], qa[1], qa[2], qa[3]}, QB = {qb[0], qb[1], qb[2], qb[3]};
+    ocx_collide(&sh, sa, vload(pa), QA, sb, vload(pb), QB, maxSep, s->collision_tolerance, s->penetration_tolerance, &h);
+    memset(out, 0, sizeof(*out));
+    out->hit = h.hit;
+    out->status = h.hit ? h.status : -1;
+    if (!h.hit) return;
+    vstore(out->point_a, h.pointA);
+    vstore(out->point_b, h.pointB);
+    vstore(out->axis, h.axis);
+    out->penetration = h.penetration;
+    out->face_a_n = h.faceA.n;
+    out->face_b_n = h.faceB.n;
+    for (int i = 0; i < h.faceA.n; i++) vstore(out->face_a + 3 * i, h.faceA.v[i]);
+    for (int i = 0; i < h.faceB.n; i++) vstore(out->face_b + 3 * i, h.faceB.v[i]);
+}
+
+int32_t oc_manifold_from_hit(const oc_hit* hit, double maxDist, double* normal_out, double* out_a, double* out_b) {
+    OManifold m;
+    m.base = v3(0, 0, 0);
+    m.n = 0;
+    V3 axis = vload(hit->axis);
+    V3 n = vnormalize(axis);
+    OFace fa = face_from(hit->face_a, hit->face_a_n), fb = face_from(hit->face_b, hit->face_b_n);
+    if (fa.n >= 2 && fb.n >= 3)
+        ocx_manifold_between_two_faces(&m, vload(hit->point_a), vload(hit->point_b), axis, maxDist, &fa, &fb);
+    else { set_point(&m, 0, vload(hit->point_a), vload(hit->point_b)); m.n = 1; }
+    n = vnormalize(n);
+    if (m.n > 4) ocx_prune(&m, n);
+    vstore(normal_out, n);
+    for (int i = 0; i < m.n; i++) { vstore(out_a + 3 * i, m.relA[i]); vstore(out_b + 3 * i, m.relB[i]); }
+    return m.n;
+}
+
+int32_t oc_clip_poly_vs_poly(const double* poly, int32_t n, const double* clip, int32_t m, const double* normal,
+                             double* out) {
+    OFace a = face_from(poly, n), b = face_from(clip, m), o;
+    clip_poly_vs_poly(&o, &a, &b, vload(normal));
+    for (int i = 0; i < o.n; i++) vstore(out + 3 * i, o.v[i]);
+    return o.n;
+}
+
+int32_t oc_clip_poly_vs_edge(const double* poly, int32_t n, const double* e1, const double* e2, const double* normal,
+                             double* out) {
+    OFace a = face_from(poly, n), o;
+    clip_poly_vs_edge(&o, &a, vload(e1), vload(e2), vload(normal));
+    for (int i = 0; i < o.n; i++) vstore(out + 3 * i, o.v[i]);
+    return o.n;
+}
+
+int32_t oc_manifold_between_two_faces(const double* p1, const double* p2, const double* axis, double maxDist,
+                                      const double* f1, int32_t n1, const double* f2, int32_t n2, double* out_a,
+                                      double* out_b) {
+    OManifold m;
+    m.base = v3(0, 0, 0);
+    OFace a = face_from(f1, n1), b = face_from(f2, n2);
+    ocx_manifold_between_two_faces(&m, vload(p1), vload(p2), vload(axis), maxDist, &a, &b);
+    for (int i = 0; i < m.n; i++) { vstore(out_a + 3 * i, m.relA[i]); vstore(out_b + 3 * i, m.relB[i]); }
+    return m.n;
+}
+
+int32_t oc_prune(double* rel_a, double* rel_b, int32_t n, const double* axis) {
+    OManifold m;
+    m.base = v3(0, 0, 0);
+    m.n = n;
+    for (int i = 0; i < n; i++) { m.relA[i] = vload(rel_a + 3 * i); m.relB[i] = vload(rel_b + 3 * i); }
+    ocx_prune(&m, vload(axis));
+    for (int i = 0; i < m.n; i++) { vstore(rel_a + 3 * i, m.relA[i]); vstore(rel_b + 3 * i, m.relB[i]); }
+    return m.n;
+}
+
+void oc_barycentric2(const double* a, const double* b, double tol, double* r) {
+    Bary o = bary2(vload(a), vload(b), tol);
+    r[0] = o.u; r[1] = o.v; r[2] = o.w; r[3] = o.valid;
+}
+
+void oc_barycentric3(const double* a, const double* b, const double* c, double tol, double* r) {
+    Bary o = bary3(vload(a), vload(b), vload(c), tol);
+    r[0] = o.u; r[1] = o.v; r[2] = o.w; r[3] = o.valid;
+}
+
+int32_t oc_supporting_face(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t s, const double* dir,
+                           const double* pos, const double* quat, double* out) {
+    OShapes sh = {shapes, hulls};
+    OFace f;
+    Q4 q = {quat[0], quat[1], quat[2], quat[3]};
+    supporting_face(&f, &sh, s, vload(dir), vload(pos), q);
+    for (int i = 0; i < f.n; i++) vstore(out + 3 * i, f.v[i]);
+    return f.n;
+}

@@ -1,0 +1,1192 @@
+/*
+ * refcpu_step.c — FP64 restatement of crashcat's updateWorld (src/update.ts:45-239)
+ * around the narrowphase in refcpu_collide.c: acceleration integration,
+ * broadphase pair predicate, manifold reduction collector, persistent contact
+ * cache, contact-constraint setup, islands, warm start, sequential-impulse
+ * velocity and position solves, integration and sleeping.  Oracle (test
+ * infrastructure); citations are to the reference's files.
+ */
+#include <float.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include "refcpu_internal.h"
+
+#define INACTIVE INT32_MAX /* INACTIVE_BODY_INDEX (src/body/sleep.ts:8) */
+#define EMPTY_SUB 0xffffffffu
+
+typedef struct {
+    int shape, motion, layer;
+    uint32_t groups, mask;
+    int allowSleep, dof, fmode, rmode;
+    V3 pos, com, v, w, force, torque;
+    Q4 q;
+    double invMass;
+    V3 invI;
+    Q4 inertiaRot;
+    double friction, restitution, gravityFactor, linDamp, angDamp, maxLinV, maxAngV;
+    int sleeping;
+    int activeIndex;
+    int islandIndex;
+    V3 aabbMin, aabbMax;
+    V3 sleepCenter[3];
+    double sleepRadius[3];
+    double sleepTimer;
+} OBody;
+
+typedef struct { V3 p1, p2; double ln, lt1, lt2; } OCached;
+typedef struct {
+    int a, b;
+    int numPoints;
+    V3 normalB;
+    OCached pts[4];
+    int processed;
+    int persisted;
+} OContact;
+
+typedef struct { V3 r1x, r2x, iI1r1x, iI2r2x; double effMass, lambda, bias; } OAxis;
+typedef struct { V3 posA, posB, localA, localB; OAxis n, t1, t2; } OCPoint;
+typedef struct {
+    int a, b, contact;
+    V3 normal, t1, t2;
+    double friction, restitution, invMassA, invMassB;
+    M3 invIA, invIB;
+    int np;
+    OCPoint pts[4];
+    uint32_t sortKey;
+} OCC;
+
+typedef struct {
+    int a, b, enabled, sleeping;
+    V3 localA, localB;
+    V3 r1, r2;
+    M3 iI1r1X, iI2r2X;
+    double eff[16]; /* mat4 (3x3 part used, [15] marks active) */
+    V3 lambda;
+} OPoint;
+
+typedef struct {
+    int* bodies; int nb, cb;      /* active indices */
+    int* contacts; int nc, cc;    /* contact constraint indices */
+    int* cons; int nu, cu;        /* user constraint indices */
+    int velSteps, posSteps;
+} OIsland;
+
+struct oc_world {
+    cc_settings s;
+    OShapes sh;
+    cc_shape_desc* shapes;
+    int ns;
+    cc_hull_data hulls;
+    double* hullBuf;
+    int32_t* hullIdx;
+    OBody* b;
+    int nb;
+    int* active;
+    int nactive;
+    double prevDt;
+    /* pairs */
+    int* pairs; int npairs, cpairs;
+    /* contacts cache */
+    OContact* ct; int nct, cct;
+    int* htab; int hcap;
+    /* constraints */
+    OCC* cc; int ncc, ccc;
+    /* user constraints */
+    OPoint* pc; int npc;
+    /* islands */
+    int* bodyLinks; int* bodyIslands; int* contactLinks; int* consLinks; int* consIds; int nconsIds;
+    OIsland* isl; int nisl, cisl;
+    /* counters */
+    cc_step_counters ctr;
+};
+
+static void* xrealloc(void* p, size_t n) {
+    void* r = realloc(p, n ? n : 1);
+    if (!r) { fprintf(stderr, "refcpu: out of memory\n"); abort(); }
+    return r;
+}
+
+/* ---- body helpers (src/body/rigid-body.ts, motion-properties.ts, sleep.ts) ---- */
+static const cc_shape_desc* body_shape(const oc_world* w, const OBody* b) { return &w->shapes[b->shape]; }
+
+static void update_com(oc_world* w, OBody* b) { /* updateCenterOfMassPosition :527-536 */
+    b->com = vadd(qrotate(b->q, vload(body_shape(w, b)->center_of_mass)), b->pos);
+}
+static void pos_from_com(oc_world* w, OBody* b) { /* updatePositionFromCenterOfMass :545-553 */
+    b->pos = vsub(b->com, qrotate(b->q, vload(body_shape(w, b)->center_of_mass)));
+}
+static void update_aabb(oc_world* w, OBody* b) { /* updateAABB :561-674 */
+    const cc_shape_desc* s = body_shape(w, b);
+    M3 R = m3_from_quat(b->q);
+    double mn[3] = {s->aabb_min[0], s->aabb_min[1], s->aabb_min[2]};
+    double mx[3] = {s->aabb_max[0], s->aabb_max[1], s->aabb_max[2]};
+    V3 lo = v3(0, 0, 0), hi = v3(0, 0, 0);
+    for (int c = 0; c < 8; c++) {
+        V3 p = v3((c & 1) ? mx[0] : mn[0], (c & 2) ? mx[1] : mn[1], (c & 4) ? mx[2] : mn[2]);
+        V3 wp = vadd(m3_mul(&R, p), b->pos);
+        if (c == 0) { lo = wp; hi = wp; }
+        else {
+            lo = v3(dmin(lo.x, wp.x), dmin(lo.y, wp.y), dmin(lo.z, wp.z));
+            hi = v3(dmax(hi.x, wp.x), dmax(hi.y, wp.y), dmax(hi.z, wp.z));
+        }
+    }
+    b->aabbMin = lo;
+    b->aabbMax = hi;
+}
+static void set_transform(oc_world* w, OBody* b) { /* setTransform :758-774 (broadphase update is implicit) */
+    update_com(w, b);
+    update_aabb(w, b);
+}
+
+/* getInverseInertiaForRotation (src/body/motion-properties.ts:494-621) */
+static M3 inv_inertia(V3 invDiag, Q4 iq, const M3* bodyR, int dof) {
+    M3 I = m3_from_quat(iq);
+    const double* b = bodyR->m;
+    const double* i = I.m;
+    double r[9];
+    for (int c = 0; c < 3; c++)
+        for (int row = 0; row < 3; row++)
+            r[c * 3 + row] = b[row] * i[c * 3 + 0] + b[3 + row] * i[c * 3 + 1] + b[6 + row] * i[c * 3 + 2];
+    double d[3] = {invDiag.x, invDiag.y, invDiag.z};
+    double s[9];
+    for (int c = 0; c < 3; c++)
+        for (int row = 0; row < 3; row++) s[c * 3 + row] = d[c] * r[c * 3 + row];
+    M3 o;
+    /* out(col j, row k) = sum_c s[c][j] * r[c][k] */
+    for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) o.m[j * 3 + k] = s[0 * 3 + j] * r[0 * 3 + k] + s[1 * 3 + j] * r[1 * 3 + k] + s[2 * 3 + j] * r[2 * 3 + k];
+    int rot = (dof >> 3) & 7;
+    if (rot != 7) {
+        double m[3] = {(rot & 1) ? 1.0 : 0.0, (rot & 2) ? 1.0 : 0.0, (rot & 4) ? 1.0 : 0.0};
+        for (int c = 0; c < 3; c++)
+            for (int row = 0; row < 3; row++) o.m[c * 3 + row] *= m[row] * m[c];
+    }
+    return o;
+}
+
+static void sleep_points(oc_world* w, OBody* b, V3 out[3]) { /* getSleepTestPoints src/body/sleep.ts:22-76 */
+    const cc_shape_desc* s = body_shape(w, b);
+    out[0] = b->com;
+    double x = (s->aabb_max[0] - s->aabb_min[0]) * 0.5, y = (s->aabb_max[1] - s->aabb_min[1]) * 0.5,
+           z = (s->aabb_max[2] - s->aabb_min[2]) * 0.5;
+    int low = (x <= y && x <= z) ? 0 : (y <= z ? 1 : 2);
+    V3 ax = v3(1, 0, 0), ay = v3(0, 1, 0), az = v3(0, 0, 1);
+    if (low == 0) { out[1] = vadd(b->com, vscale(qrotate(b->q, ay), y)); out[2] = vadd(b->com, vscale(qrotate(b->q, az), z)); }
+    else if (low == 1) { out[1] = vadd(b->com, vscale(qrotate(b->q, ax), x)); out[2] = vadd(b->com, vscale(qrotate(b->q, az), z)); }
+    else { out[1] = vadd(b->com, vscale(qrotate(b->q, ax), x)); out[2] = vadd(b->com, vscale(qrotate(b->q, ay), y)); }
+}
+static void reset_sleep_spheres(OBody* b, V3 p[3]) {
+    for (int i = 0; i < 3; i++) { b->sleepCenter[i] = p[i]; b->sleepRadius[i] = 0; }
+    b->sleepTimer = 0;
+}
+static void reset_sleep_timer(oc_world* w, OBody* b) { /* resetSleepTimer :120-127 */
+    if (b->motion != CC_MOTION_DYNAMIC) return;
+    V3 p[3];
+    sleep_points(w, b, p);
+    reset_sleep_spheres(b, p);
+}
+static int update_sleep_state(oc_world* w, OBody* b, double dt, double maxMove, double tBefore) { /* :88-117 */
+    if (!b->allowSleep) return 0;
+    V3 p[3];
+    sleep_points(w, b, p);
+    for (int i = 0; i < 3; i++) {
+        double d = vlen(vsub(b->sleepCenter[i], p[i]));
+        b->sleepRadius[i] = dmax(b->sleepRadius[i], d);
+        if (b->sleepRadius[i] > maxMove) { reset_sleep_spheres(b, p); return 0; }
+    }
+    b->sleepTimer += dt;
+    return b->sleepTimer >= tBefore;
+}
+static void add_active(oc_world* w, OBody* b, int idx) { /* addBodyToActiveBodies :132-142 */
+    if (b->activeIndex != INACTIVE) return;
+    b->activeIndex = w->nactive;
+    w->active[w->nactive++] = idx;
+}
+static void remove_active(oc_world* w, OBody* b) { /* removeBodyFromActiveBodies :145-166 */
+    if (b->activeIndex == INACTIVE) return;
+    int last = w->nactive - 1;
+    if (b->activeIndex != last) {
+        int lastBody = w->active[last];
+        w->active[b->activeIndex] = lastBody;
+        w->b[lastBody].activeIndex = b->activeIndex;
+    }
+    b->activeIndex = INACTIVE;
+    w->nactive--;
+}
+static void body_sleep(oc_world* w, OBody* b) { /* sleep :169-185 */
+    if (b->motion == CC_MOTION_STATIC || b->sleeping) return;
+    remove_active(w, b);
+    b->sleeping = 1;
+    b->v = v3(0, 0, 0);
+    b->w = v3(0, 0, 0);
+}
+static void body_wake(oc_world* w, OBody* b, int idx) { /* wake :188-203 */
+    if (b->motion == CC_MOTION_STATIC) return;
+    reset_sleep_timer(w, b);
+    if (!b->sleeping) return;
+    b->sleeping = 0;
+    add_active(w, b, idx);
+}
+static V3 dof_lin(V3 v, int dof) {
+    if (!(dof & 1)) v.x = 0;
+    if (!(dof & 2)) v.y = 0;
+    if (!(dof & 4)) v.z = 0;
+    return v;
+}
+static void clamp_lin(OBody* b) { /* clampLinearVelocity :624-630 */
+    if (vlen2(b->v) > b->maxLinV * b->maxLinV) b->v = vscale(vnormalize(b->v), b->maxLinV);
+}
+static void clamp_ang(OBody* b) {
+    if (vlen2(b->w) > b->maxAngV * b->maxAngV) b->w = vscale(vnormalize(b->w), b->maxAngV);
+}
+static void add_rot_step(OBody* b, V3 s, double sign) { /* add/subRotationStep src/body/rigid-body-step.ts:55-97 */
+    double len = vlen(s);
+    if (len > 1.0e-6) {
+        V3 axis = vscale(s, 1 / len);
+        Q4 r = qaxis_angle(axis, sign * len);
+        b->q = qnormalize(qmul(r, b->q));
+    }
+}
+
+/* ---- world creation -------------------------------------------------------- */
+oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb, const cc_shape_desc* shapes,
+                    int32_t ns, const cc_hull_data* hulls, const cc_point_constraint_desc* pcs, int32_t npc) {
+    oc_world* w = (oc_world*)calloc(1, sizeof(oc_world));
+    w->s = *s;
+    w->shapes = (cc_shape_desc*)xrealloc(NULL, sizeof(cc_shape_desc) * (size_t)ns);
+    memcpy(w->shapes, shapes, sizeof(cc_shape_desc) * (size_t)ns);
+    w->ns = ns;
+    if (hulls && hulls->total_points > 0) {
+        int P = hulls->total_points, F = hulls->total_faces, FV = hulls->total_face_vertices;
+        w->hullBuf = (double*)xrealloc(NULL, sizeof(double) * (size_t)(6 * P + 4 * F));
+        double* p = w->hullBuf;
+        memcpy(p, hulls->points, sizeof(double) * 3 * P); w->hulls.points = p; p += 3 * P;
+        memcpy(p, hulls->shrunk_points, sizeof(double) * 3 * P); w->hulls.shrunk_points = p; p += 3 * P;
+        memcpy(p, hulls->plane_normals, sizeof(double) * 3 * F); w->hulls.plane_normals = p; p += 3 * F;
+        memcpy(p, hulls->plane_constants, sizeof(double) * F); w->hulls.plane_constants = p;
+        w->hullIdx = (int32_t*)xrealloc(NULL, sizeof(int32_t) * (size_t)(2 * F + FV));
+        int32_t* q = w->hullIdx;
+        memcpy(q, hulls->face_first_vertex, sizeof(int32_t) * F); w->hulls.face_first_vertex = q; q += F;
+        memcpy(q, hulls->face_num_vertices, sizeof(int32_t) * F); w->hulls.face_num_vertices = q; q += F;
+        memcpy(q, hulls->face_vertex_indices, sizeof(int32_t) * FV); w->hulls.face_vertex_indices = q;
+        w->hulls.total_points = P; w->hulls.total_faces = F; w->hulls.total_face_vertices = FV;
+    }
+    w->sh.shapes = w->shapes;
+    w->sh.hulls = &w->hulls;
+    w->nb = nb;
+    w->b = (OBody*)calloc((size_t)(nb ? nb : 1), sizeof(OBody));
+    w->active = (int*)calloc((size_t)(nb ? nb : 1), sizeof(int));
+    w->bodyLinks = (int*)calloc((size_t)(nb ? nb : 1), sizeof(int));
+    w->bodyIslands = (int*)calloc((size_t)(nb ? nb : 1), sizeof(int));
+    for (int i = 0; i < nb; i++) {
+        const cc_body_desc* d = &bodies[i];
+        OBody* b = &w->b[i];
+        b->shape = d->shape; b->motion = d->motion_type; b->layer = d->object_layer;
+        b->groups = d->collision_groups; b->mask = d->collision_mask;
+        b->allowSleep = d->allow_sleeping; b->dof = d->allowed_dofs;
+        b->fmode = d->friction_combine; b->rmode = d->restitution_combine;
+        b->pos = vload(d->position);
+        b->q.x = d->quaternion[0]; b->q.y = d->quaternion[1]; b->q.z = d->quaternion[2]; b->q.w = d->quaternion[3];
+        b->v = vload(d->linear_velocity); b->w = vload(d->angular_velocity);
+        b->invMass = d->inv_mass; b->invI = vload(d->inv_inertia_diagonal);
+        b->inertiaRot.x = d->inertia_rotation[0]; b->inertiaRot.y = d->inertia_rotation[1];
+        b->inertiaRot.z = d->inertia_rotation[2]; b->inertiaRot.w = d->inertia_rotation[3];
+        b->friction = d->friction; b->restitution = d->restitution; b->gravityFactor = d->gravity_factor;
+        b->linDamp = d->linear_damping; b->angDamp = d->angular_damping;
+        b->maxLinV = d->max_linear_velocity; b->maxAngV = d->max_angular_velocity;
+        b->sleeping = d->sleeping;
+        b->activeIndex = INACTIVE;
+        b->islandIndex = -1;
+        set_transform(w, b); /* updateShape :677-712 */
+        if (b->motion == CC_MOTION_DYNAMIC) reset_sleep_timer(w, b);
+        if (b->motion != CC_MOTION_STATIC && !b->sleeping) add_active(w, b, i);
+    }
+    w->npc = npc;
+    w->pc = (OPoint*)calloc((size_t)(npc ? npc : 1), sizeof(OPoint));
+    for (int i = 0; i < npc; i++) {
+        w->pc[i].a = pcs[i].body_a; w->pc[i].b = pcs[i].body_b; w->pc[i].enabled = pcs[i].enabled;
+        w->pc[i].localA = vload(pcs[i].local_point_a); w->pc[i].localB = vload(pcs[i].local_point_b);
+    }
+    w->consLinks = (int*)calloc((size_t)(npc ? npc : 1), sizeof(int));
+    w->consIds = (int*)calloc((size_t)(npc ? npc : 1), sizeof(int));
+    w->prevDt = 0;
+    return w;
+}
+
+void oc_destroy(oc_world* w) {
+    if (!w) return;
+    free(w->shapes); free(w->hullBuf); free(w->hullIdx); free(w->b); free(w->active);
+    free(w->pairs); free(w->ct); free(w->htab); free(w->cc); free(w->pc);
+    free(w->bodyLinks); free(w->bodyIslands); free(w->contactLinks); free(w->consLinks); free(w->consIds);
+    for (int i = 0; i < w->cisl; i++) { free(w->isl[i].bodies); free(w->isl[i].contacts); free(w->isl[i].cons); }
+    free(w->isl);
+    free(w);
+}
+
+/* ---- contact cache (src/contacts.ts) as a hash map keyed by (a, b) ---------- */
+static uint64_t hkey(int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; }
+static uint32_t hhash(uint64_t k) {
+    k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+    return (uint32_t)k;
+}
+static void cache_rebuild(oc_world* w) {
+    int need = 16;
+    while (need < 2 * w->nct + 16) need <<= 1;
+    if (need != w->hcap) { w->htab = (int*)xrealloc(w->htab, sizeof(int) * (size_t)need); w->hcap = need; }
+    for (int i = 0; i < w->hcap; i++) w->htab[i] = -1;
+    for (int i = 0; i < w->nct; i++) {
+        uint32_t h = hhash(hkey(w->ct[i].a, w->ct[i].b)) & (uint32_t)(w->hcap - 1);
+        while (w->htab[h] >= 0) h = (h + 1) & (uint32_t)(w->hcap - 1);
+        w->htab[h] = i;
+    }
+}
+static int cache_find(oc_world* w, int a, int b) { /* findContact :481-513 */
+    if (!w->hcap) return -1;
+    uint32_t h = hhash(hkey(a, b)) & (uint32_t)(w->hcap - 1);
+    while (w->htab[h] >= 0) {
+        OContact* c = &w->ct[w->htab[h]];
+        if (c->a == a && c->b == b) return w->htab[h];
+        h = (h + 1) & (uint32_t)(w->hcap - 1);
+    }
+    return -1;
+}
+static int cache_create(oc_world* w, int a, int b) { /* createContact :235-285 + setContact :141-170 */
+    if (w->nct == w->cct) { w->cct = w->cct ? 2 * w->cct : 256; w->ct = (OContact*)xrealloc(w->ct, sizeof(OContact) * (size_t)w->cct); }
+    OContact* c = &w->ct[w->nct];
+    memset(c, 0, sizeof(*c));
+    c->a = a; c->b = b;
+    if (2 * (w->nct + 1) + 16 > w->hcap) { w->nct++; cache_rebuild(w); return w->nct - 1; }
+    uint32_t h = hhash(hkey(a, b)) & (uint32_t)(w->hcap - 1);
+    while (w->htab[h] >= 0) h = (h + 1) & (uint32_t)(w->hcap - 1);
+    w->htab[h] = w->nct;
+    return w->nct++;
+}
+/* ---- constraint parts (src/constraints/constraint-part/axis-constraint-part.ts) ---- */
+static void axis_props(OAxis* p, const OBody* A, const OBody* B, double invMA, double invMB, const M3* IA, const M3* IB,
+                       V3 r1, V3 r2, V3 axis, double bias) { /* calculateConstraintProperties :169-201 */
+    double inv = 0;
+    if (A->motion == CC_MOTION_DYNAMIC) {
+        p->r1x = vcross(r1, axis);
+        p->iI1r1x = m3_mul(IA, p->r1x);
+        inv += invMA + vdot(p->iI1r1x, p->r1x);
+    }
+    if (B->motion == CC_MOTION_DYNAMIC) {
+        p->r2x = vcross(r2, axis);
+        p->iI2r2x = m3_mul(IB, p->r2x);
+        inv += invMB + vdot(p->iI2r2x, p->r2x);
+    }
+    if (inv == 0) { p->effMass = 0; p->lambda = 0; }
+    else { p->effMass = 1 / inv; p->bias = bias; }
+}
+static int axis_active(const OAxis* p) { return p->effMass != 0; }
+static double axis_total_lambda(const OAxis* p, const OBody* A, const OBody* B, V3 axis) { /* getTotalLambda :578-625 */
+    int mA = A->motion != CC_MOTION_STATIC, mB = B->motion != CC_MOTION_STATIC;
+    double jv;
+    if (mA && mB) jv = vdot(axis, vsub(A->v, B->v));
+    else if (mA) jv = vdot(axis, A->v);
+    else if (mB) jv = -vdot(axis, B->v);
+    else jv = 0;
+    if (mA) jv += vdot(p->r1x, A->w);
+    if (mB) jv -= vdot(p->r2x, B->w);
+    double lam = p->effMass * (jv - p->bias);
+    return p->lambda + lam;
+}
+static void axis_apply(OAxis* p, OBody* A, OBody* B, double invMA, double invMB, V3 axis, double total) { /* applyLambda :640-707 */
+    double dl = total - p->lambda;
+    p->lambda = total;
+    if (dl == 0) return;
+    if (A->motion == CC_MOTION_DYNAMIC) {
+        double ls = dl * invMA;
+        A->v.x -= axis.x * ls; A->v.y -= axis.y * ls; A->v.z -= axis.z * ls;
+        A->v = dof_lin(A->v, A->dof);
+        A->w.x -= p->iI1r1x.x * dl; A->w.y -= p->iI1r1x.y * dl; A->w.z -= p->iI1r1x.z * dl;
+    }
+    if (B->motion == CC_MOTION_DYNAMIC) {
+        double ls = dl * invMB;
+        B->v.x += axis.x * ls; B->v.y += axis.y * ls; B->v.z += axis.z * ls;
+        B->v = dof_lin(B->v, B->dof);
+        B->w.x += p->iI2r2x.x * dl; B->w.y += p->iI2r2x.y * dl; B->w.z += p->iI2r2x.z * dl;
+    }
+}
+static void axis_warm(OAxis* p, OBody* A, OBody* B, double invMA, double invMB, V3 axis, double ratio) { /* warmStart :500-537 */
+    p->lambda *= ratio;
+    if (p->lambda == 0) return;
+    if (A->motion == CC_MOTION_DYNAMIC) {
+        A->v = dof_lin(vsub(A->v, vscale(axis, p->lambda * invMA)), A->dof);
+        A->w = vsub(A->w, vscale(p->iI1r1x, p->lambda));
+    }
+    if (B->motion == CC_MOTION_DYNAMIC) {
+        B->v = dof_lin(vadd(B->v, vscale(axis, p->lambda * invMB)), B->dof);
+        B->w = vadd(B->w, vscale(p->iI2r2x, p->lambda));
+    }
+}
+static int axis_solve_position(OAxis* p, OBody* A, OBody* B, double invMA, double invMB, V3 axis, double C, double beta) {
+    /* solvePositionConstraintWithMassOverride :790-847 */
+    if (C == 0 || !axis_active(p)) return 0;
+    double lam = -p->effMass * beta * C;
+    if (A->motion == CC_MOTION_DYNAMIC) {
+        V3 st = vscale(axis, lam * invMA);
+        A->com = dof_lin(vsub(A->com, st), A->dof); /* subPositionStep: applyTranslationDOFConstraint on the position */
+        add_rot_step(A, vscale(p->iI1r1x, lam), -1.0);
+    }
+    if (B->motion == CC_MOTION_DYNAMIC) {
+        V3 st = vscale(axis, lam * invMB);
+        B->com = dof_lin(vadd(B->com, st), B->dof); /* addPositionStep */
+        add_rot_step(B, vscale(p->iI2r2x, lam), 1.0);
+    }
+    return 1;
+}
+
+static double combine(double a, double b, int ma, int mb) { /* combineMaterial src/constraints/combine-material.ts:40-57 */
+    static const int prio[5] = {0, 1, 3, 4, 2}; /* AVERAGE, MULTIPLY, MIN, MAX, GEOMETRIC_MEAN */
+    int mode = prio[ma] >= prio[mb] ? ma : mb;
+    switch (mode) {
+    case CC_COMBINE_MULTIPLY: return a * b;
+    case CC_COMBINE_MIN: return dmin(a, b);
+    case CC_COMBINE_MAX: return dmax(a, b);
+    case CC_COMBINE_GEOMETRIC_MEAN: return sqrt(a * b);
+    default: return (a + b) / 2;
+    }
+}
+
+static uint32_t sort_key(uint32_t a, uint32_t b, uint32_t sa, uint32_t sb) { /* computeContactSortKey :1222-1231 */
+    uint32_t h = 0;
+    h = h * 31u + a; h = h * 31u + b; h = h * 31u + sa; h = h * 31u + sb;
+    return h;
+}
+
+/* calculateNormalVelocityBias src/constraints/contact-constraints.ts:288-423 */
+static double normal_bias(const oc_world* w, const OBody* A, const OBody* B, V3 pA, V3 pB, V3 n, V3 rA, V3 rB, double e,
+                          double dt) {
+    int mA = A->motion != CC_MOTION_STATIC, mB = B->motion != CC_MOTION_STATIC;
+    int dA = A->motion == CC_MOTION_DYNAMIC, dB = B->motion == CC_MOTION_DYNAMIC;
+    V3 rel;
+    if (mA && mB) rel = vsub(vadd(B->v, vcross(B->w, rB)), vadd(A->v, vcross(A->w, rA)));
+    else if (mA) rel = vneg(vadd(A->v, vcross(A->w, rA)));
+    else if (mB) rel = vadd(B->v, vcross(B->w, rB));
+    else return 0;
+    double nv = vdot(rel, n);
+    double pen = vdot(vsub(pA, pB), n);
+    double spec = dmax(0, -pen / dt);
+    V3 g = vload(w->s.gravity);
+    if (e > 0 && nv < -w->s.min_velocity_for_restitution) {
+        if (nv < -spec) {
+            V3 acc = v3(0, 0, 0);
+            if (mA && mB) acc = vadd(acc, vscale(g, B->gravityFactor - A->gravityFactor));
+            else if (mA) acc = vadd(acc, vscale(g, -A->gravityFactor));
+            else if (mB) acc = vadd(acc, vscale(g, B->gravityFactor));
+            if (dA) acc = vadd(acc, vscale(A->force, -A->invMass));
+            if (dB) acc = vadd(acc, vscale(B->force, B->invMass));
+            double fdv = dmin(0, vdot(acc, n) * dt);
+            return e * (nv - fdv);
+        }
+        return spec;
+    }
+    return spec;
+}
+
+/* addContactConstraint src/constraints/contact-constraints.ts:437-857 */
+static int add_contact_constraint(oc_world* w, int ia, int ib, OManifold* m, double dt) {
+    if (ia > ib) { /* swap so that id A < id B, manifold.swapShapes :86-107 */
+        int t = ia; ia = ib; ib = t;
+        m->normal = vneg(m->normal);
+        for (int i = 0; i < m->n; i++) { V3 x = m->relA[i]; m->relA[i] = m->relB[i]; m->relB[i] = x; }
+    }
+    OBody* A = &w->b[ia];
+    OBody* B = &w->b[ib];
+    int ci = cache_find(w, ia, ib);
+    int existing = ci >= 0;
+    if (!existing) ci = cache_create(w, ia, ib);
+    OContact* c = &w->ct[ci];
+    c->processed = 1;
+    M3 rotA = m3_from_quat(A->q), rotB = m3_from_quat(B->q);
+    c->normalB = vnormalize(m3_mul_t(&rotB, m->normal));
+    c->numPoints = m->n;
+    double fr = combine(A->friction, B->friction, A->fmode, B->fmode);
+    double re = combine(A->restitution, B->restitution, A->rmode, B->rmode);
+    if (existing) c->persisted = 1;
+    if ((A->motion == CC_MOTION_DYNAMIC && A->invMass != 0) || (B->motion == CC_MOTION_DYNAMIC && B->invMass != 0)) {
+        if (w->ncc == w->ccc) { w->ccc = w->ccc ? 2 * w->ccc : 256; w->cc = (OCC*)xrealloc(w->cc, sizeof(OCC) * (size_t)w->ccc); }
+        OCC* k = &w->cc[w->ncc++];
+        memset(k, 0, sizeof(*k));
+        k->a = ia; k->b = ib; k->contact = ci;
+        k->sortKey = sort_key((uint32_t)ia, (uint32_t)ib, EMPTY_SUB, EMPTY_SUB);
+        V3 n = m->normal;
+        k->normal = n;
+        if (fabs(n.x) > fabs(n.y)) { double tl = sqrt(n.x * n.x + n.z * n.z); k->t1 = v3(n.z / tl, 0, -n.x / tl); }
+        else { double tl = sqrt(n.y * n.y + n.z * n.z); k->t1 = v3(0, n.z / tl, -n.y / tl); }
+        k->t2 = vcross(n, k->t1);
+        k->friction = fr; k->restitution = re;
+        k->invMassA = A->invMass * 1.0; k->invMassB = B->invMass * 1.0;
+        if (A->motion == CC_MOTION_DYNAMIC) k->invIA = inv_inertia(A->invI, A->inertiaRot, &rotA, A->dof);
+        if (B->motion == CC_MOTION_DYNAMIC) k->invIB = inv_inertia(B->invI, B->inertiaRot, &rotB, B->dof);
+        k->np = 0;
+        for (int i = 0; i < m->n; i++) {
+            OCPoint* cp = &k->pts[k->np++];
+            cp->posA = vadd(m->base, m->relA[i]);
+            cp->posB = vadd(m->base, m->relB[i]);
+            cp->localA = m3_mul_t(&rotA, vsub(cp->posA, A->com));
+            cp->localB = m3_mul_t(&rotB, vsub(cp->posB, B->com));
+            int set = 0;
+            if (existing) { /* in-place cache: slots < i already hold this step's points (reference behaviour) */
+                for (int j = 0; j < c->numPoints; j++) {
+                    OCached* pt = &c->pts[j];
+                    double d1 = vlen2(vsub(cp->localA, pt->p1)), d2 = vlen2(vsub(cp->localB, pt->p2));
+                    double th = w->s.contact_point_preserve_lambda_max_dist_sq;
+                    if (d1 < th && d2 < th) {
+                        cp->n.lambda = pt->ln; cp->t1.lambda = pt->lt1; cp->t2.lambda = pt->lt2;
+                        set = 1;
+                        break;
+                    }
+                }
+            }
+            if (!set) cp->n.lambda = cp->t1.lambda = cp->t2.lambda = 0;
+            V3 mid = v3((cp->posA.x + cp->posB.x) * 0.5, (cp->posA.y + cp->posB.y) * 0.5, (cp->posA.z + cp->posB.z) * 0.5);
+            V3 rA = vsub(mid, A->com), rB = vsub(mid, B->com);
+            double bias = normal_bias(w, A, B, cp->posA, cp->posB, n, rA, rB, re, dt);
+            axis_props(&cp->n, A, B, k->invMassA, k->invMassB, &k->invIA, &k->invIB, rA, rB, n, bias);
+            if (fr > 0) {
+                axis_props(&cp->t1, A, B, k->invMassA, k->invMassB, &k->invIA, &k->invIB, rA, rB, k->t1, 0);
+                axis_props(&cp->t2, A, B, k->invMassA, k->invMassB, &k->invIA, &k->invIB, rA, rB, k->t2, 0);
+            } else {
+                cp->t1.effMass = 0; cp->t1.lambda = 0;
+                cp->t2.effMass = 0; cp->t2.lambda = 0;
+            }
+            OCached* st = &c->pts[i];
+            st->p1 = cp->localA; st->p2 = cp->localB;
+            st->ln = cp->n.lambda; st->lt1 = cp->t1.lambda; st->lt2 = cp->t2.lambda;
+        }
+        return 1;
+    }
+    for (int i = 0; i < m->n; i++) {
+        V3 pa = vadd(m->base, m->relA[i]), pb = vadd(m->base, m->relB[i]);
+        OCached* st = &c->pts[i];
+        st->p1 = m3_mul_t(&rotA, vsub(pa, A->com));
+        st->p2 = m3_mul_t(&rotB, vsub(pb, B->com));
+        st->ln = st->lt1 = st->lt2 = 0;
+    }
+    return 0;
+}
+
+/* ---- broadphase (src/broadphase/broadphase.ts:44-138, 217-277) --------------- */
+static int queries(const OBody* b) { return b->motion != CC_MOTION_STATIC && !b->sleeping; }
+static int64_t dedup(const OBody* b, int idx) { return queries(b) ? idx : (int64_t)INACTIVE; }
+static int overlap(V3 amn, V3 amx, V3 bmn, V3 bmx) { /* box3.intersectsBox3, inclusive */
+    return !(bmx.x < amn.x || bmn.x > amx.x || bmx.y < amn.y || bmn.y > amx.y || bmx.z < amn.z || bmn.z > amx.z);
+}
+static int pair_ok(const cc_settings* s, const OBody* a, int ia, const OBody* b, int ib) {
+    if (ia == ib) return 0;
+    if (!s->object_layer_pairs[a->layer * CC_MAX_OBJECT_LAYERS + b->layer]) return 0;
+    if (!((a->groups & b->mask) != 0 && (b->groups & a->mask) != 0)) return 0;
+    if (dedup(a, ia) >= dedup(b, ib)) return 0;
+    if (a->motion == CC_MOTION_STATIC && b->motion == CC_MOTION_STATIC) return 0;
+    if (a->motion != CC_MOTION_DYNAMIC && b->motion != CC_MOTION_DYNAMIC) return 0;
+    return 1;
+}
+static void push_pair(oc_world* w, int a, int b) {
+    if (w->npairs + 1 > w->cpairs) { w->cpairs = w->cpairs ? 2 * w->cpairs : 1024; w->pairs = (int*)xrealloc(w->pairs, sizeof(int) * 2 * (size_t)w->cpairs); }
+    w->pairs[2 * w->npairs] = a;
+    w->pairs[2 * w->npairs + 1] = b;
+    w->npairs++;
+}
+static void find_pairs(oc_world* w) {
+    double m = w->s.speculative_contact_distance;
+    w->npairs = 0;
+    for (int i = 0; i < w->nb; i++) {
+        OBody* a = &w->b[i];
+        if (!queries(a)) continue;
+        V3 mn = vsub(a->aabbMin, v3(m, m, m)), mx = vadd(a->aabbMax, v3(m, m, m));
+        for (int j = 0; j < w->nb; j++) {
+            OBody* b = &w->b[j];
+            if (!pair_ok(&w->s, a, i, b, j)) continue;
+            if (!overlap(mn, mx, b->aabbMin, b->aabbMax)) continue;
+            push_pair(w, i, j);
+        }
+    }
+}
+
+/* ---- narrowphase of one pair with the reduction collector (src/update.ts:333-563, 714-850) ---- */
+static int narrowphase_pair(oc_world* w, int ia, int ib, double dt) {
+    OBody* A = &w->b[ia];
+    OBody* B = &w->b[ib];
+    OHit h;
+    ocx_collide(&w->sh, A->shape, A->pos, A->q, B->shape, B->pos, B->q, w->s.speculative_contact_distance,
+                w->s.collision_tolerance, w->s.penetration_tolerance, &h);
+    if (!h.hit) return 0;
+    static OManifold m, tmp;
+    m.base = A->pos;
+    m.normal = vnormalize(h.axis);
+    m.depth = h.penetration;
+    m.n = 0;
+    tmp.base = A->pos;
+    tmp.n = 0;
+    double maxDist = w->s.speculative_contact_distance + w->s.manifold_tolerance;
+    if (h.faceA.n >= 2 && h.faceB.n >= 3) ocx_manifold_between_two_faces(&tmp, h.pointA, h.pointB, h.axis, maxDist, &h.faceA, &h.faceB);
+    else { tmp.relA[0] = vsub(h.pointA, tmp.base); tmp.relB[0] = vsub(h.pointB, tmp.base); tmp.n = 1; }
+    for (int i = 0; i < tmp.n && m.n < 64; i++) { m.relA[m.n] = tmp.relA[i]; m.relB[m.n] = tmp.relB[i]; m.n++; }
+    if (h.penetration > m.depth) m.depth = h.penetration;
+    if (m.n > 32) ocx_prune(&m, m.normal);
+    m.normal = vnormalize(m.normal);
+    if (m.n > 4) ocx_prune(&m, m.normal);
+    if (m.n > 0) return add_contact_constraint(w, ia, ib, &m, dt);
+    return 0;
+}
+
+/* ---- islands (src/islands.ts) ----------------------------------------------- */
+static int lowest(const oc_world* w, int i) { while (w->bodyLinks[i] != i) i = w->bodyLinks[i]; return i; }
+static void link_bodies(oc_world* w, int a, int b) { /* linkBodies :103-140 */
+    if (a >= w->nactive || b >= w->nactive) return;
+    int f = lowest(w, a), s = lowest(w, b);
+    if (f == s) return;
+    if (f < s) w->bodyLinks[s] = f; else w->bodyLinks[f] = s;
+    int lo = f < s ? f : s;
+    if (w->bodyLinks[a] > lo) w->bodyLinks[a] = lo;
+    if (w->bodyLinks[b] > lo) w->bodyLinks[b] = lo;
+}
+static void ipush(int** arr, int* n, int* cap, int v) {
+    if (*n == *cap) { *cap = *cap ? 2 * *cap : 8; *arr = (int*)xrealloc(*arr, sizeof(int) * (size_t)*cap); }
+    (*arr)[(*n)++] = v;
+}
+static int cmp_island(const void* x, const void* y) { /* compareIslandSize, stable via index */
+    const OIsland* a = (const OIsland*)x;
+    const OIsland* b = (const OIsland*)y;
+    int na = a->nc + a->nu, nb = b->nc + b->nu;
+    if (na != nb) return nb - na;
+    return 0;
+}
+static void build_islands(oc_world* w) {
+    int na = w->nactive;
+    for (int i = 0; i < w->nb; i++) w->b[i].islandIndex = -1;
+    for (int i = 0; i < na; i++) { w->bodyLinks[i] = i; w->bodyIslands[i] = 0; }
+    w->contactLinks = (int*)xrealloc(w->contactLinks, sizeof(int) * (size_t)(w->ncc ? w->ncc : 1));
+    for (int c = 0; c < w->ncc; c++) { /* linkContactConstraints :184-204 */
+        OBody* A = &w->b[w->cc[c].a];
+        OBody* B = &w->b[w->cc[c].b];
+        if (A->activeIndex != INACTIVE && B->activeIndex != INACTIVE) link_bodies(w, A->activeIndex, B->activeIndex);
+        w->contactLinks[c] = A->activeIndex < B->activeIndex ? A->activeIndex : B->activeIndex;
+    }
+    w->nconsIds = 0;
+    for (int i = 0; i < w->npc; i++) { /* linkUserConstraints :210-226 */
+        OPoint* p = &w->pc[i];
+        if (!p->enabled) continue;
+        OBody* A = &w->b[p->a];
+        OBody* B = &w->b[p->b];
+        if (A->activeIndex != INACTIVE || B->activeIndex != INACTIVE) {
+            if (A->activeIndex != INACTIVE && B->activeIndex != INACTIVE) link_bodies(w, A->activeIndex, B->activeIndex);
+            w->consIds[w->nconsIds] = i;
+            w->consLinks[w->nconsIds] = A->activeIndex < B->activeIndex ? A->activeIndex : B->activeIndex;
+            w->nconsIds++;
+        }
+    }
+    int nisl = 0; /* finalize :240-389 */
+    for (int i = 0; i < na; i++) {
+        int l = w->bodyLinks[i];
+        if (l != i) w->bodyIslands[i] = w->bodyIslands[l];
+        else w->bodyIslands[i] = nisl++;
+    }
+    if (nisl > w->cisl) {
+        w->isl = (OIsland*)xrealloc(w->isl, sizeof(OIsland) * (size_t)nisl);
+        memset(w->isl + w->cisl, 0, sizeof(OIsland) * (size_t)(nisl - w->cisl));
+        w->cisl = nisl;
+    }
+    w->nisl = nisl;
+    for (int i = 0; i < nisl; i++) { w->isl[i].nb = w->isl[i].nc = w->isl[i].nu = 0; }
+    for (int ai = 0; ai < na; ai++) {
+        int bi = w->active[ai];
+        int il = w->bodyIslands[ai];
+        ipush(&w->isl[il].bodies, &w->isl[il].nb, &w->isl[il].cb, ai);
+        w->b[bi].islandIndex = il;
+    }
+    for (int c = 0; c < w->ncc; c++) {
+        int l = w->contactLinks[c];
+        if (l >= INACTIVE) continue;
+        int il = w->bodyIslands[l];
+        ipush(&w->isl[il].contacts, &w->isl[il].nc, &w->isl[il].cc, c);
+    }
+    for (int k = 0; k < w->nconsIds; k++) {
+        int l = w->consLinks[k];
+        if (l >= INACTIVE) continue;
+        int il = w->bodyIslands[l];
+        ipush(&w->isl[il].cons, &w->isl[il].nu, &w->isl[il].cu, w->consIds[k]);
+    }
+    /* stable sort by constraint count, descending (Array.prototype.sort is stable) */
+    for (int i = 1; i < nisl; i++) { /* insertion sort keeps stability */
+        OIsland t = w->isl[i];
+        int j = i - 1;
+        while (j >= 0 && cmp_island(&w->isl[j], &t) > 0) { w->isl[j + 1] = w->isl[j]; j--; }
+        w->isl[j + 1] = t;
+    }
+    for (int i = 0; i < nisl; i++) {
+        OIsland* I = &w->isl[i];
+        for (int k = 0; k < I->nb; k++) w->b[w->active[I->bodies[k]]].islandIndex = i;
+        /* all overrides are 0 -> world defaults (islands.ts:344-386) */
+        I->velSteps = w->s.velocity_iterations;
+        I->posSteps = w->s.position_iterations;
+    }
+}
+
+static oc_world* g_sort_world;
+static int cmp_contact(const void* x, const void* y) { /* sortContactIndices :1240-1264 + stability */
+    const OCC* a = &g_sort_world->cc[*(const int*)x];
+    const OCC* b = &g_sort_world->cc[*(const int*)y];
+    if (a->sortKey != b->sortKey) return a->sortKey < b->sortKey ? -1 : 1;
+    if (a->a != b->a) return a->a < b->a ? -1 : 1;
+    if (a->b != b->b) return a->b < b->b ? -1 : 1;
+    return *(const int*)x < *(const int*)y ? -1 : 1;
+}
+static int cmp_int(const void* x, const void* y) { return *(const int*)x - *(const int*)y; }
+
+/* ---- point constraint (src/constraints/constraint-part/point-constraint-part.ts) ---- */
+static M3 cross_mat(V3 r) { M3 m = {{0, r.z, -r.y, -r.z, 0, r.x, r.y, -r.x, 0}}; return m; }
+static M3 m3_mm(const M3* a, const M3* b) {
+    M3 o;
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) o.m[c * 3 + r] = a->m[r] * b->m[c * 3] + a->m[3 + r] * b->m[c * 3 + 1] + a->m[6 + r] * b->m[c * 3 + 2];
+    return o;
+}
+static M3 m3_mm_rt(const M3* a, const M3* b) { /* a * b^T */
+    M3 o;
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) o.m[c * 3 + r] = a->m[r] * b->m[c] + a->m[3 + r] * b->m[3 + c] + a->m[6 + r] * b->m[6 + c];
+    return o;
+}
+static int m3_invert(const double* a, double* out) { /* mat4.invert3x3 on the 3x3 block, [15] = 1 when invertible */
+    double a00 = a[0], a01 = a[1], a02 = a[2], a10 = a[4], a11 = a[5], a12 = a[6], a20 = a[8], a21 = a[9], a22 = a[10];
+    double b01 = a22 * a11 - a12 * a21, b11 = -a22 * a10 + a12 * a20, b21 = a21 * a10 - a11 * a20;
+    double det = a00 * b01 + a01 * b11 + a02 * b21;
+    if (det == 0) return 0;
+    det = 1.0 / det;
+    memset(out, 0, 16 * sizeof(double));
+    out[0] = b01 * det; out[1] = (-a22 * a01 + a02 * a21) * det; out[2] = (a12 * a01 - a02 * a11) * det;
+    out[4] = b11 * det; out[5] = (a22 * a00 - a02 * a20) * det; out[6] = (-a12 * a00 + a02 * a10) * det;
+    out[8] = b21 * det; out[9] = (-a21 * a00 + a01 * a20) * det; out[10] = (a11 * a00 - a01 * a10) * det;
+    out[15] = 1;
+    return 1;
+}
+static void point_props(OPoint* p, OBody* A, OBody* B) { /* calculateConstraintProperties :77-165 */
+    M3 RA = m3_from_quat(A->q), RB = m3_from_quat(B->q);
+    p->r1 = m3_mul(&RA, p->localA);
+    p->r2 = m3_mul(&RB, p->localB);
+    double K[16];
+    memset(K, 0, sizeof(K));
+    double sumInv = 0;
+    memset(&p->iI1r1X, 0, sizeof(M3));
+    memset(&p->iI2r2X, 0, sizeof(M3));
+    if (A->motion == CC_MOTION_DYNAMIC) {
+        sumInv += A->invMass;
+        M3 I = inv_inertia(A->invI, A->inertiaRot, &RA, A->dof);
+        M3 X = cross_mat(p->r1);
+        p->iI1r1X = m3_mm(&I, &X);
+        M3 t1 = m3_mm(&X, &I);
+        M3 t2 = m3_mm_rt(&t1, &X);
+        for (int c = 0; c < 3; c++) for (int r = 0; r < 3; r++) K[c * 4 + r] += t2.m[c * 3 + r];
+    }
+    if (B->motion == CC_MOTION_DYNAMIC) {
+        sumInv += B->invMass;
+        M3 I = inv_inertia(B->invI, B->inertiaRot, &RB, B->dof);
+        M3 X = cross_mat(p->r2);
+        p->iI2r2X = m3_mm(&I, &X);
+        M3 t1 = m3_mm(&X, &I);
+        M3 t2 = m3_mm_rt(&t1, &X);
+        for (int c = 0; c < 3; c++) for (int r = 0; r < 3; r++) K[c * 4 + r] += t2.m[c * 3 + r];
+    }
+    K[0] += sumInv; K[5] += sumInv; K[10] += sumInv;
+    if (!m3_invert(K, p->eff)) { memset(p->eff, 0, sizeof(p->eff)); p->lambda = v3(0, 0, 0); }
+}
+static V3 eff_mul(const double* e, V3 v) {
+    return v3(e[0] * v.x + e[4] * v.y + e[8] * v.z, e[1] * v.x + e[5] * v.y + e[9] * v.z, e[2] * v.x + e[6] * v.y + e[10] * v.z);
+}
+static int point_apply(OPoint* p, OBody* A, OBody* B, V3 lam) { /* applyVelocityStep :232-262 */
+    if (vlen2(lam) == 0) return 0;
+    if (A->motion == CC_MOTION_DYNAMIC) {
+        A->v = vadd(A->v, vscale(lam, -A->invMass));
+        A->w = vsub(A->w, m3_mul(&p->iI1r1X, lam));
+    }
+    if (B->motion == CC_MOTION_DYNAMIC) {
+        B->v = vadd(B->v, vscale(lam, B->invMass));
+        B->w = vadd(B->w, m3_mul(&p->iI2r2X, lam));
+    }
+    return 1;
+}
+static void point_solve_velocity(OPoint* p, OBody* A, OBody* B) { /* solveVelocityConstraint :186-221 */
+    if (p->eff[15] == 0) return;
+    V3 jv = vsub(vsub(A->v, vcross(p->r1, A->w)), B->v);
+    jv = vadd(jv, vcross(p->r2, B->w));
+    V3 lam = eff_mul(p->eff, jv);
+    p->lambda = vadd(p->lambda, lam);
+    point_apply(p, A, B, lam);
+}
+static int point_solve_position(OPoint* p, OBody* A, OBody* B, double beta) { /* solvePositionConstraint :271-316 */
+    point_props(p, A, B);
+    if (p->eff[15] == 0) return 0;
+    V3 sep = vsub(vsub(vadd(B->com, p->r2), A->com), p->r1);
+    if (vlen2(sep) < 1e-12) return 0;
+    V3 lam = vscale(eff_mul(p->eff, sep), -beta);
+    if (A->motion == CC_MOTION_DYNAMIC) {
+        A->com = dof_lin(vsub(A->com, vscale(lam, A->invMass)), A->dof);
+        add_rot_step(A, m3_mul(&p->iI1r1X, lam), -1.0);
+    }
+    if (B->motion == CC_MOTION_DYNAMIC) {
+        B->com = dof_lin(vadd(B->com, vscale(lam, B->invMass)), B->dof);
+        add_rot_step(B, m3_mul(&p->iI2r2X, lam), 1.0);
+    }
+    return 1;
+}
+
+/* ---- contact solver (src/constraints/contact-constraints.ts:906-1216) ---------- */
+static void warm_start_contacts(oc_world* w, double ratio) {
+    for (int i = 0; i < w->ncc; i++) {
+        OCC* k = &w->cc[i];
+        OBody* A = &w->b[k->a];
+        OBody* B = &w->b[k->b];
+        for (int j = 0; j < k->np; j++) {
+            OCPoint* cp = &k->pts[j];
+            if (axis_active(&cp->t1) || axis_active(&cp->t2)) {
+                axis_warm(&cp->t1, A, B, k->invMassA, k->invMassB, k->t1, ratio);
+                axis_warm(&cp->t2, A, B, k->invMassA, k->invMassB, k->t2, ratio);
+            }
+            axis_warm(&cp->n, A, B, k->invMassA, k->invMassB, k->normal, ratio);
+        }
+    }
+}
+static void solve_velocity_contact(oc_world* w, OCC* k) {
+    OBody* A = &w->b[k->a];
+    OBody* B = &w->b[k->b];
+    if (A->motion != CC_MOTION_DYNAMIC && B->motion != CC_MOTION_DYNAMIC) return;
+    for (int j = 0; j < k->np; j++) {
+        OCPoint* cp = &k->pts[j];
+        if (axis_active(&cp->t1) || axis_active(&cp->t2)) {
+            double l1 = axis_total_lambda(&cp->t1, A, B, k->t1);
+            double l2 = axis_total_lambda(&cp->t2, A, B, k->t2);
+            double mf = k->friction * cp->n.lambda;
+            double fm2 = l1 * l1 + l2 * l2;
+            if (fm2 > mf * mf) { double sc = mf / sqrt(fm2); l1 *= sc; l2 *= sc; }
+            axis_apply(&cp->t1, A, B, k->invMassA, k->invMassB, k->t1, l1);
+            axis_apply(&cp->t2, A, B, k->invMassA, k->invMassB, k->t2, l2);
+        }
+    }
+    for (int j = 0; j < k->np; j++) {
+        OCPoint* cp = &k->pts[j];
+        double tl = axis_total_lambda(&cp->n, A, B, k->normal);
+        axis_apply(&cp->n, A, B, k->invMassA, k->invMassB, k->normal, dmax(0, tl));
+    }
+}
+static int solve_position_contact(oc_world* w, OCC* k) {
+    OBody* A = &w->b[k->a];
+    OBody* B = &w->b[k->b];
+    if (A->motion != CC_MOTION_DYNAMIC && B->motion != CC_MOTION_DYNAMIC) return 0;
+    int any = 0;
+    M3 RA = m3_from_quat(A->q), RB = m3_from_quat(B->q);
+    M3 IA, IB;
+    memset(&IA, 0, sizeof(IA));
+    memset(&IB, 0, sizeof(IB));
+    if (A->motion == CC_MOTION_DYNAMIC) IA = inv_inertia(A->invI, A->inertiaRot, &RA, A->dof);
+    if (B->motion == CC_MOTION_DYNAMIC) IB = inv_inertia(B->invI, B->inertiaRot, &RB, B->dof);
+    for (int j = 0; j < k->np; j++) {
+        OCPoint* cp = &k->pts[j];
+        V3 pA = vadd(A->com, m3_mul(&RA, cp->localA));
+        V3 pB = vadd(B->com, m3_mul(&RB, cp->localB));
+        double sep = vdot(vsub(pB, pA), k->normal) + w->s.penetration_slop;
+        sep = dmax(sep, -w->s.max_penetration_distance);
+        if (sep >= 0) continue;
+        V3 mid = v3((pA.x + pB.x) * 0.5, (pA.y + pB.y) * 0.5, (pA.z + pB.z) * 0.5);
+        V3 rA = vsub(mid, A->com), rB = vsub(mid, B->com);
+        axis_props(&cp->n, A, B, k->invMassA, k->invMassB, &IA, &IB, rA, rB, k->normal, 0);
+        if (axis_solve_position(&cp->n, A, B, k->invMassA, k->invMassB, k->normal, sep, w->s.baumgarte_factor)) any = 1;
+    }
+    return any;
+}
+
+/* ---- the step (src/update.ts:45-239) ---------------------------------------- */
+void oc_step(oc_world* w, double dt) {
+    memset(&w->ctr, 0, sizeof(w->ctr));
+    int64_t epa0 = ocx_epa_calls;
+    w->ncc = 0;
+    for (int i = 0; i < w->nct; i++) w->ct[i].processed = 0;
+    /* accelerationIntegrationUpdate :245-319 */
+    for (int i = 0; i < w->nb; i++) {
+        OBody* b = &w->b[i];
+        if (b->motion != CC_MOTION_DYNAMIC || b->sleeping) continue;
+        double gs = w->s.gravity_enabled ? b->gravityFactor : 0;
+        V3 g = vload(w->s.gravity);
+        b->v.x += (b->force.x * b->invMass + g.x * gs) * dt;
+        b->v.y += (b->force.y * b->invMass + g.y * gs) * dt;
+        b->v.z += (b->force.z * b->invMass + g.z * gs) * dt;
+        double ld = dmax(0, 1 - b->linDamp * dt);
+        b->v = vscale(b->v, ld);
+        double s2 = vlen2(b->v);
+        if (s2 > b->maxLinV * b->maxLinV) b->v = vscale(b->v, b->maxLinV / sqrt(s2));
+        b->v = dof_lin(b->v, b->dof);
+        M3 R = m3_from_quat(b->q);
+        M3 I = inv_inertia(b->invI, b->inertiaRot, &R, b->dof);
+        b->w = vadd(b->w, vscale(m3_mul(&I, b->torque), dt));
+        double ad = dmax(0, 1 - b->angDamp * dt);
+        b->w = vscale(b->w, ad);
+        double a2 = vlen2(b->w);
+        if (a2 > b->maxAngV * b->maxAngV) b->w = vscale(b->w, b->maxAngV / sqrt(a2));
+        b->w = dof_lin(b->w, b->dof >> 3);
+    }
+    find_pairs(w);
+    w->ctr.num_pairs = w->npairs;
+    for (int p = 0; p < w->npairs; p++) {
+        int ia = w->pairs[2 * p], ib = w->pairs[2 * p + 1];
+        OBody* A = &w->b[ia];
+        OBody* B = &w->b[ib];
+        if (A->motion > B->motion || (A->motion == B->motion && ib < ia)) { int t = ia; ia = ib; ib = t; }
+        int made = narrowphase_pair(w, ia, ib, dt);
+        if (made) {
+            if (w->b[ia].motion == CC_MOTION_DYNAMIC && w->b[ia].sleeping) body_wake(w, &w->b[ia], ia);
+            if (w->b[ib].motion == CC_MOTION_DYNAMIC && w->b[ib].sleeping) body_wake(w, &w->b[ib], ib);
+        }
+    }
+    /* destroyUnprocessedContacts: the constraint list refers to cache slots, remap them */
+    {
+        int* remap = (int*)xrealloc(NULL, sizeof(int) * (size_t)(w->nct ? w->nct : 1));
+        int k = 0;
+        for (int i = 0; i < w->nct; i++) {
+            if (w->ct[i].processed) { remap[i] = k; w->ct[k++] = w->ct[i]; }
+            else remap[i] = -1;
+        }
+        w->nct = k;
+        for (int i = 0; i < w->ncc; i++) w->cc[i].contact = remap[w->cc[i].contact];
+        free(remap);
+        cache_rebuild(w);
+    }
+    w->ctr.num_contacts = w->ncc;
+    for (int i = 0; i < w->ncc; i++) w->ctr.num_points += w->cc[i].np;
+    if (dt > 0) {
+        for (int i = 0; i < w->npc; i++) { /* wakeBodiesInUserConstraints :863-890 */
+            OPoint* p = &w->pc[i];
+            if (!p->enabled) continue;
+            OBody* A = &w->b[p->a];
+            OBody* B = &w->b[p->b];
+            int act = !A->sleeping || !B->sleeping;
+            int dyn = A->motion == CC_MOTION_DYNAMIC || B->motion == CC_MOTION_DYNAMIC;
+            if (act && dyn) {
+                if (A->sleeping && A->motion == CC_MOTION_DYNAMIC) body_wake(w, A, p->a);
+                if (B->sleeping && B->motion == CC_MOTION_DYNAMIC) body_wake(w, B, p->b);
+            }
+        }
+        build_islands(w);
+        for (int i = 0; i < w->npc; i++) w->pc[i].sleeping = w->b[w->pc[i].a].sleeping && w->b[w->pc[i].b].sleeping;
+        for (int i = 0; i < w->npc; i++)
+            if (w->pc[i].enabled && !w->pc[i].sleeping) point_props(&w->pc[i], &w->b[w->pc[i].a], &w->b[w->pc[i].b]);
+        if (w->s.warm_starting) {
+            double ratio = (w->s.warm_starting && w->prevDt > 0) ? dt / w->prevDt : 0.0;
+            warm_start_contacts(w, ratio);
+            for (int i = 0; i < w->npc; i++) {
+                OPoint* p = &w->pc[i];
+                if (!p->enabled || p->sleeping || p->eff[15] == 0) continue;
+                p->lambda = vscale(p->lambda, ratio);
+                point_apply(p, &w->b[p->a], &w->b[p->b], p->lambda);
+            }
+        }
+        w->prevDt = dt;
+        g_sort_world = w;
+        for (int il = 0; il < w->nisl; il++) {
+            OIsland* I = &w->isl[il];
+            if (I->nc == 0 && I->nu == 0) continue;
+            qsort(I->cons, (size_t)I->nu, sizeof(int), cmp_int); /* sortConstraintIds: priority 0, index */
+            qsort(I->contacts, (size_t)I->nc, sizeof(int), cmp_contact);
+            for (int it = 0; it < I->velSteps; it++) {
+                for (int k = 0; k < I->nc; k++) solve_velocity_contact(w, &w->cc[I->contacts[k]]);
+                for (int k = 0; k < I->nu; k++) {
+                    OPoint* p = &w->pc[I->cons[k]];
+                    point_solve_velocity(p, &w->b[p->a], &w->b[p->b]);
+                }
+            }
+        }
+        for (int i = 0; i < w->ncc; i++) { /* storeAppliedImpulses :1061-1076 */
+            OCC* k = &w->cc[i];
+            OContact* c = &w->ct[k->contact];
+            for (int j = 0; j < k->np; j++) {
+                c->pts[j].ln = k->pts[j].n.lambda;
+                c->pts[j].lt1 = k->pts[j].t1.lambda;
+                c->pts[j].lt2 = k->pts[j].t2.lambda;
+            }
+        }
+        for (int i = 0; i < w->nb; i++) { /* velocityIntegrationUpdate :896-976 */
+            OBody* b = &w->b[i];
+            if (b->motion == CC_MOTION_STATIC || b->sleeping) continue;
+            if (b->motion == CC_MOTION_DYNAMIC) { clamp_lin(b); clamp_ang(b); }
+            V3 rv = vscale(b->w, dt);
+            double ang = vlen(rv);
+            if (ang > 1e-6) {
+                V3 axis = vscale(rv, 1 / ang);
+                b->q = qnormalize(qmul(qaxis_angle(axis, ang), b->q));
+            }
+            V3 disp = vscale(b->v, dt);
+            b->com = vadd(b->com, disp);
+            pos_from_com(w, b);
+            set_transform(w, b);
+        }
+        int maxPos = 0;
+        for (int il = 0; il < w->nisl; il++) {
+            OIsland* I = &w->isl[il];
+            if (I->nc == 0 && I->nu == 0) continue;
+            for (int it = 0; it < I->posSteps; it++) {
+                int applied = 0;
+                if (I->nc > 0)
+                    for (int k = 0; k < I->nc; k++)
+                        if (solve_position_contact(w, &w->cc[I->contacts[k]])) applied = 1;
+                if (I->nu > 0) {
+                    int ua = 0;
+                    for (int k = 0; k < I->nu; k++) {
+                        OPoint* p = &w->pc[I->cons[k]];
+                        if (point_solve_position(p, &w->b[p->a], &w->b[p->b], w->s.baumgarte_factor)) ua = 1;
+                    }
+                    applied = ua || applied;
+                }
+                if (it + 1 > maxPos) maxPos = it + 1;
+                if (!applied) break;
+            }
+        }
+        w->ctr.position_sweeps = maxPos;
+        for (int i = 0; i < w->nb; i++) { /* updateBodyPositions :322-330 */
+            OBody* b = &w->b[i];
+            if (b->motion == CC_MOTION_STATIC || b->sleeping) continue;
+            pos_from_com(w, b);
+            set_transform(w, b);
+        }
+        if (w->s.allow_sleeping) { /* checkIslandSleep :392-423 */
+            double maxMove = w->s.point_velocity_sleep_threshold * w->s.time_before_sleep;
+            for (int il = 0; il < w->nisl; il++) {
+                OIsland* I = &w->isl[il];
+                int all = 1;
+                for (int k = 0; k < I->nb; k++) {
+                    OBody* b = &w->b[w->active[I->bodies[k]]];
+                    if (b->motion != CC_MOTION_DYNAMIC) continue;
+                    int can = update_sleep_state(w, b, dt, maxMove, w->s.time_before_sleep);
+                    all = all && can;
+                }
+                if (all) {
+                    for (int k = 0; k < I->nb; k++) {
+                        int bi = w->active[I->bodies[k]];
+                        OBody* b = &w->b[bi];
+                        if (b->motion == CC_MOTION_DYNAMIC) body_sleep(w, b);
+                    }
+                }
+            }
+        }
+    }
+    for (int i = 0; i < w->nb; i++) { /* clear forces */
+        OBody* b = &w->b[i];
+        if (b->motion == CC_MOTION_STATIC || b->sleeping) continue;
+        b->force = v3(0, 0, 0);
+        b->torque = v3(0, 0, 0);
+    }
+    w->ctr.num_islands = w->nisl;
+    w->ctr.num_epa = ocx_epa_calls - epa0;
+    for (int i = 0; i < w->nb; i++) {
+        if (w->b[i].motion == CC_MOTION_DYNAMIC) {
+            w->ctr.num_bodies++;
+            if (!w->b[i].sleeping) w->ctr.num_awake++;
+        }
+    }
+}
+
+/* ---- accessors ------------------------------------------------------------- */
+int32_t oc_num_bodies(const oc_world* w) { return w->nb; }
+
+void oc_get_bodies(const oc_world* w, cc_body_state* out) {
+    /* island label = lowest body index among the island's bodies */
+    int* lab = (int*)calloc((size_t)(w->nisl ? w->nisl : 1), sizeof(int));
+    for (int i = 0; i < w->nisl; i++) lab[i] = INT32_MAX;
+    for (int i = 0; i < w->nb; i++) if (w->b[i].islandIndex >= 0 && i < lab[w->b[i].islandIndex]) lab[w->b[i].islandIndex] = i;
+    for (int i = 0; i < w->nb; i++) {
+        const OBody* b = &w->b[i];
+        cc_body_state* o = &out[i];
+        vstore(o->position, b->pos);
+        o->quaternion[0] = b->q.x; o->quaternion[1] = b->q.y; o->quaternion[2] = b->q.z; o->quaternion[3] = b->q.w;
+        vstore(o->center_of_mass_position, b->com);
+        vstore(o->linear_velocity, b->v);
+        vstore(o->angular_velocity, b->w);
+        vstore(o->aabb_min, b->aabbMin);
+        vstore(o->aabb_max, b->aabbMax);
+        o->sleeping = b->sleeping;
+        o->island = b->islandIndex >= 0 ? lab[b->islandIndex] : -1;
+    }
+    free(lab);
+}
+
+void oc_set_bodies(oc_world* w, const cc_body_state* in) {
+    for (int i = 0; i < w->nb; i++) {
+        OBody* b = &w->b[i];
+        b->pos = vload(in[i].position);
+        b->q.x = in[i].quaternion[0]; b->q.y = in[i].quaternion[1]; b->q.z = in[i].quaternion[2]; b->q.w = in[i].quaternion[3];
+        b->v = vload(in[i].linear_velocity);
+        b->w = vload(in[i].angular_velocity);
+        set_transform(w, b);
+    }
+}
+
+static int cmp_pair(const void* x, const void* y) {
+    const int* a = (const int*)x;
+    const int* b = (const int*)y;
+    if (a[0] != b[0]) return a[0] - b[0];
+    return a[1] - b[1];
+}
+int32_t oc_get_pairs(const oc_world* w, int32_t* out, int32_t cap) {
+    int n = w->npairs < cap ? w->npairs : cap;
+    for (int i = 0; i < n; i++) {
+        int a = w->pairs[2 * i], b = w->pairs[2 * i + 1];
+        out[2 * i] = a < b ? a : b;
+        out[2 * i + 1] = a < b ? b : a;
+    }
+    qsort(out, (size_t)n, 2 * sizeof(int32_t), cmp_pair);
+    return w->npairs;
+}
+
+static int cmp_cout(const void* x, const void* y) {
+    const cc_contact_out* a = (const cc_contact_out*)x;
+    const cc_contact_out* b = (const cc_contact_out*)y;
+    if (a->body_a != b->body_a) return a->body_a - b->body_a;
+    return a->body_b - b->body_b;
+}
+int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap) {
+    int n = w->ncc < cap ? w->ncc : cap;
+    for (int i = 0; i < n; i++) {
+        const OCC* k = &w->cc[i];
+        cc_contact_out* o = &out[i];
+        memset(o, 0, sizeof(*o));
+        o->body_a = k->a; o->body_b = k->b; o->num_points = k->np;
+        vstore(o->normal, k->normal);
+        for (int j = 0; j < k->np; j++) {
+            vstore(o->position_a[j], k->pts[j].posA);
+            vstore(o->position_b[j], k->pts[j].posB);
+            o->lambda_n[j] = k->pts[j].n.lambda;
+            o->lambda_t1[j] = k->pts[j].t1.lambda;
+            o->lambda_t2[j] = k->pts[j].t2.lambda;
+        }
+    }
+    qsort(out, (size_t)n, sizeof(cc_contact_out), cmp_cout);
+    return w->ncc;
+}
+
+void oc_get_counters(const oc_world* w, cc_step_counters* out) { *out = w->ctr; }
+
+int32_t oc_broadphase_aabbs(const float* mn, const float* mx, const int32_t* motion, const int32_t* sleeping,
+                            const int32_t* layer, int32_t n, double margin, const cc_settings* s, int32_t* out,
+                            int32_t cap) {
+    OBody* b = (OBody*)calloc((size_t)(n ? n : 1), sizeof(OBody));
+    for (int i = 0; i < n; i++) {
+        b[i].aabbMin = v3(mn[3 * i], mn[3 * i + 1], mn[3 * i + 2]);
+        b[i].aabbMax = v3(mx[3 * i], mx[3 * i + 1], mx[3 * i + 2]);
+        b[i].motion = motion[i]; b[i].sleeping = sleeping[i]; b[i].layer = layer[i];
+        b[i].groups = b[i].mask = 0xffffffffu;
+    }
+    int cnt = 0;
+    for (int i = 0; i < n; i++) {
+        if (!queries(&b[i])) continue;
+        V3 amn = vsub(b[i].aabbMin, v3(margin, margin, margin)), amx = vadd(b[i].aabbMax, v3(margin, margin, margin));
+        for (int j = 0; j < n; j++) {
+            if (!pair_ok(s, &b[i], i, &b[j], j)) continue;
+            if (!overlap(amn, amx, b[j].aabbMin, b[j].aabbMax)) continue;
+            if (cnt < cap) { out[2 * cnt] = i < j ? i : j; out[2 * cnt + 1] = i < j ? j : i; }
+            cnt++;
+        }
+    }
+    free(b);
+    qsort(out, (size_t)(cnt < cap ? cnt : cap), 2 * sizeof(int32_t), cmp_pair);
+    return cnt;
+}
