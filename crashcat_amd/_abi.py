@@ -162,6 +162,19 @@ class StepCounters(C.Structure):
     ]
 
 
+class Totals(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("pairs", C.c_int64),
+        ("contacts", C.c_int64),
+        ("points", C.c_int64),
+        ("epa", C.c_int64),
+        ("awake_body_steps", C.c_int64),
+        ("solver_bytes", C.c_int64),
+        ("reserved", C.c_int64 * 9),
+    ]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("precision", C.c_int32),

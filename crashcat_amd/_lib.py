@@ -14,7 +14,7 @@ _LIB = None
 SYMBOLS = [
     "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
-    "cc_get_phase_times",
+    "cc_get_phase_times", "cc_get_totals",
 ]
 
 
@@ -48,6 +48,7 @@ def lib():
     fp = C.POINTER(C.c_float)
     L.cc_broadphase_aabbs.argtypes = [vp, fp, fp, ip, ip, ip, C.c_int32, C.c_double, ip, C.c_int32, ip]
     L.cc_get_phase_times.argtypes = [vp, C.POINTER(C.c_double), C.c_int32, ip]
+    L.cc_get_totals.argtypes = [vp, C.POINTER(_abi.Totals)]
     for s in SYMBOLS:
         getattr(L, s).restype = getattr(L, s).restype if s == "cc_last_error" else C.c_int
     _LIB = L
@@ -147,6 +148,11 @@ class Context:
         cnt = C.c_int32()
         self._check(lib().cc_get_phase_times(self.h, buf, 16, C.byref(cnt)))
         return {PHASES[i] if i < len(PHASES) else str(i): buf[i] for i in range(cnt.value)}
+
+    def totals(self):
+        t = _abi.Totals()
+        self._check(lib().cc_get_totals(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _abi.Totals._fields_ if k != "reserved"}
 
     def broadphase_aabbs(self, mn, mx, motion, sleeping, layer, margin):
         mn = np.ascontiguousarray(mn, dtype=np.float32)

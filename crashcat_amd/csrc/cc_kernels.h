@@ -14,38 +14,41 @@ struct Dev {
     int N;              // bodies (all worlds)
     int W;              // worlds
     const int* wStart;  // [W+1] body ranges
-    int pairCap, conCap, epaCap, hashCap;
+    int pairCap, conCap, epaCap, epaWorkers, hashCap;
     // broadphase
     int* bpCount;       // [N+1]
     int* bpOff;         // [N+1] exclusive scan
     int* pairA;         // [pairCap]
     int* pairB;
     int* counters;      // device counters, see CNT_*
-    // constraints per pair slot
-    Con<T>* con;        // [pairCap]
-    ConTap<T>* tap;     // [pairCap]
-    int* conValid;      // [pairCap + 1]
-    int* conScan;       // [pairCap + 1]
-    int* conIdx;        // [conCap] compacted pair slots, world contiguous
-    CacheRec<T>* rec[2];          // [pairCap] per parity
+    // contact constraints (slots allocated in creation order by an atomic counter)
+    Con<T>* con;        // [conCap]
+    ConTap<T>* tap;     // [conCap]
+    int* wConCnt;       // [W + 1] per-world constraint counts
+    int* wConStart;     // [W + 1] exclusive scan of wConCnt
+    int* wFill;         // [W]
+    int* conIdx;        // [conCap] constraint slots grouped by world
+    CacheRec<T>* rec[2];          // [conCap] per parity
     unsigned long long* hkeys[2]; // [hashCap]
     int* hvals[2];
-    // EPA jobs
+    // EPA jobs and the per-worker polytope workspace
     int* epaPair;       // [epaCap]
     Simplex<T>* epaSimplex;
-    EpaState<T>* epaWork;
+    EpaState<T>* epaWork;  // [epaWorkers]
     // point constraints
     PointCon<T>* pc;
     int NPC;
     const int* pcStart; // [W+1]
-    // per-world solver workspace
-    unsigned long long* skeys;  // [2 * conCap]
+    // per-world solver workspace (sequence = world's contacts then its point constraints)
+    unsigned long long* skeys;  // [2 * (conCap + NPC)] solve-order keys (hi, lo)
     int* seq;           // [conCap + NPC]
     int* seqA;
     int* seqB;
     int* prevA;
     int* prevB;
-    int* level;
+    int* nextA;
+    int* nextB;
+    int* level;         // in-degree during the level construction
     int* sched;
     int* lvlStart;      // [conCap + NPC + W + 1]
     int* blist;         // [2 * (conCap + NPC)]
@@ -56,7 +59,13 @@ struct Dev {
     int* stats;         // per world: [levels, posSweeps]
     T* prevDt;          // [1]
     int* parity;        // [1]
+    unsigned long long* acc;  // [ACC_N] running sums over steps, see ACC_*
 };
+
+// running sums (64-bit) over all steps since upload: counts and the algorithmic bytes of
+// the byte model (per-item compulsory HBM traffic of each phase, 4-byte words)
+enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
+       ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_N = 16 };
 
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_N = 8 };
 
@@ -240,9 +249,11 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<
     int fa = B.flags[ia], fb = B.flags[ib];
     bool dynA = fl_motion(fa) == MOTION_DYNAMIC, dynB = fl_motion(fb) == MOTION_DYNAMIC;
     if (!((dynA && B.invMass[ia] != T(0)) || (dynB && B.invMass[ib] != T(0)))) return;
-    Con<T>& k = d.con[slot];
-    ConTap<T>& tp = d.tap[slot];
-    CacheRec<T>& rc = d.rec[par][slot];
+    int cs = atomicAdd(&d.counters[CNT_CONS], 1);
+    if (cs >= d.conCap) { atomicOr(&d.counters[CNT_OVERFLOW], 2); return; }
+    Con<T>& k = d.con[cs];
+    ConTap<T>& tp = d.tap[cs];
+    CacheRec<T>& rc = d.rec[par][cs];
     // previous record of this contact (findContact src/contacts.ts:481-513)
     unsigned long long key = pair_key(ia, ib);
     int prev = -1;
@@ -324,7 +335,6 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<
         rc.lt1[i] = cp.t1.lambda;
         rc.lt2[i] = cp.t2.lambda;
     }
-    d.conValid[slot] = 1;
     if (dynA && B.sleeping[ia]) B.wake[ia] = 1;
     if (dynB && B.sleeping[ib]) B.wake[ib] = 1;
 }
@@ -369,7 +379,6 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev<T> d, T dt) {
     int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
     int par = *d.parity;
     for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < npairs; slot += gridDim.x * blockDim.x) {
-        d.conValid[slot] = 0;
         PairBodies<T> pb = order_pair(d, d.pairA[slot], d.pairB[slot]);
         int A = pb.a, Bi = pb.b;
         const Bodies<T>& B = d.B;
@@ -418,7 +427,9 @@ template <typename T>
 __global__ void __launch_bounds__(64) k_epa(Dev<T> d, T dt) {
     int nj = min(d.counters[CNT_EPA], d.epaCap);
     int par = *d.parity;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nj; j += gridDim.x * blockDim.x) {
+    int worker = blockIdx.x * blockDim.x + threadIdx.x;
+    if (worker >= d.epaWorkers) return;
+    for (int j = worker; j < nj; j += gridDim.x * blockDim.x) {
         int slot = d.epaPair[j];
         PairBodies<T> pb = order_pair(d, d.pairA[slot], d.pairB[slot]);
         int A = pb.a, Bi = pb.b;
@@ -436,7 +447,7 @@ __global__ void __launch_bounds__(64) k_epa(Dev<T> d, T dt) {
         SB.t = c.relPos;
         PenDepth<T> pd;
         Simplex<T> simplex = d.epaSimplex[j];
-        if (!pd_step_epa(pd, SA, SB, d.s.penTol, simplex, d.epaWork[j])) continue;
+        if (!pd_step_epa(pd, SA, SB, d.s.penTol, simplex, d.epaWork[worker])) continue;
         NpScratch<T> S;
         finish_convex(d, slot, A, Bi, c, pd, S, dt, par);
     }
@@ -454,13 +465,33 @@ __global__ void k_wake(Dev<T> d) {
     }
 }
 
-// compaction of constraint slots (world contiguous because pairs are)
+// group constraint slots by world (counting sort); the order inside a world is irrelevant
+// because the solver orders constraints by their keys
 template <typename T>
-__global__ void k_compact(Dev<T> d) {
-    int np = min(d.counters[CNT_PAIRS], d.pairCap);
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < np; s += gridDim.x * blockDim.x)
-        if (d.conValid[s]) d.conIdx[d.conScan[s]] = s;
-    if (blockIdx.x == 0 && threadIdx.x == 0) d.counters[CNT_CONS] = d.conScan[np];
+__global__ void k_con_world_count(Dev<T> d) {
+    int nc = min(d.counters[CNT_CONS], d.conCap);
+    unsigned long long pts = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        atomicAdd(&d.wConCnt[d.B.world[d.con[k].a]], 1);
+        pts += (unsigned long long)d.con[k].np;
+    }
+    for (int o = 32; o > 0; o >>= 1) pts += __shfl_xor(pts, o, 64);
+    if ((threadIdx.x & 63) == 0 && pts) atomicAdd(&d.acc[ACC_POINTS], pts);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&d.acc[ACC_PAIRS], (unsigned long long)min(d.counters[CNT_PAIRS], d.pairCap));
+        atomicAdd(&d.acc[ACC_CONS], (unsigned long long)nc);
+        atomicAdd(&d.acc[ACC_EPA], (unsigned long long)min(d.counters[CNT_EPA], d.epaCap));
+        atomicAdd(&d.acc[ACC_STEPS], 1ull);
+    }
+}
+
+template <typename T>
+__global__ void k_con_world_scatter(Dev<T> d) {
+    int nc = min(d.counters[CNT_CONS], d.conCap);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        int w = d.B.world[d.con[k].a];
+        d.conIdx[d.wConStart[w] + atomicAdd(&d.wFill[w], 1)] = k;
+    }
 }
 
 // ----------------------------------------------------------------- islands
@@ -474,13 +505,17 @@ __global__ void k_uf_init(Dev<T> d) {
     d.B.label[i] = act ? i : -1;
 }
 
+// Parents only ever point to smaller indices and only roots are hooked (by CAS), so a
+// non-root x may be re-pointed at any ancestor: path halving stores the grandparent.
 __device__ inline int uf_find(int* p, int x) {
-    int y = __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    while (y != x) {
-        x = y;
-        y = __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        int y = __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (y == x) return x;
+        int z = __hip_atomic_load(&p[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (z == y) return y;
+        __hip_atomic_store(&p[x], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = z;
     }
-    return x;
 }
 __device__ inline void uf_union(int* p, int a, int b) {
     for (;;) {
@@ -497,7 +532,7 @@ __global__ void k_uf_link(Dev<T> d) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc + d.NPC; k += gridDim.x * blockDim.x) {
         int a, b;
         if (k < nc) {
-            const Con<T>& c = d.con[d.conIdx[k]];
+            const Con<T>& c = d.con[k];
             a = c.a; b = c.b;
         } else {
             const PointCon<T>& pc = d.pc[k - nc];
@@ -525,7 +560,7 @@ __global__ void k_hash_insert(Dev<T> d) {
     unsigned long long* hk = d.hkeys[par];
     int* hv = d.hvals[par];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
-        int slot = d.conIdx[k];
+        int slot = k;
         const Con<T>& c = d.con[slot];
         unsigned long long key = pair_key(c.a, c.b);
         unsigned h = hash64(key) & (unsigned)(d.hashCap - 1);

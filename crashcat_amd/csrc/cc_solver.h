@@ -271,95 +271,71 @@ __device__ inline int island_of(const Dev<T>& d, int a, int b) {
     return la >= 0 ? la : d.B.label[b];
 }
 
+// solve-order key of sequence element k: contacts first, ordered by (sortKey, idA, idB)
+// (sortContactIndices :1240-1264), then point constraints by index (sortConstraintIds
+// constraints.ts:248-262, priority 0)
+__device__ inline bool key_less(const unsigned long long* K, int x, int y) {
+    unsigned long long hx = K[2 * x], hy = K[2 * y];
+    if (hx != hy) return hx < hy;
+    return K[2 * x + 1] < K[2 * y + 1];
+}
+
 template <typename T, int BS>
 __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __shared__ int sh[BS / 64 + 1];
-    __shared__ int sD, sAny;
+    __shared__ int sTail;
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
     const Bodies<T>& B = d.B;
     const int b0 = d.wStart[w], b1 = d.wStart[w + 1];
     const int nb = b1 - b0;
-    int pp0 = min(d.bpOff[b0], d.pairCap), pp1 = min(d.bpOff[b1], d.pairCap);
-    const int c0 = d.conScan[pp0], c1 = d.conScan[pp1];
+    const int c0 = d.wConStart[w], c1 = d.wConStart[w + 1];
     const int u0 = d.pcStart[w], u1 = d.pcStart[w + 1];
     const int m = c1 - c0, u = u1 - u0, L = m + u;
     const int so = c0 + u0;
     const int par = *d.parity;
     const T prevDt = *d.prevDt;
     const T ratio = (d.s.warmStarting && prevDt > T(0)) ? dt / prevDt : T(0);
+    unsigned long long* K = d.skeys + 2 * (size_t)so;
 
-    // ---- 1. order the world's contact constraints by (sortKey, idA, idB)
-    int P2 = 0;
-    if (m > 0) { P2 = 1; while (P2 < m) P2 <<= 1; }  // K[0..P2) lies inside [2*c0, 2*c1)
-    unsigned long long* K = d.skeys + 2 * (size_t)c0;
-    for (int k = tid; k < P2; k += BS) {
-        if (k < m) K[k] = ((unsigned long long)d.con[d.conIdx[c0 + k]].sortKey << 32) | (unsigned)k;
-        else K[k] = 0xffffffffffffffffull;
-    }
-    __syncthreads();
-    for (int size = 2; size <= P2; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = tid; t < P2 / 2; t += BS) {
-                int lo = 2 * t - (t & (stride - 1));
-                int hi = lo + stride;
-                bool up = ((lo & size) == 0);
-                unsigned long long x = K[lo], y = K[hi];
-                if ((x > y) == up) { K[lo] = y; K[hi] = x; }
-            }
-            __syncthreads();
-        }
-    }
-    // ties on the 32-bit key: order by (idA, idB) (sortContactIndices :1240-1264)
-    for (int k = tid; k < m; k += BS) {
-        unsigned kk = (unsigned)(K[k] >> 32);
-        if (k > 0 && (unsigned)(K[k - 1] >> 32) == kk) continue;
-        int e = k + 1;
-        while (e < m && (unsigned)(K[e] >> 32) == kk) e++;
-        for (int i = k + 1; i < e; i++) {
-            unsigned long long x = K[i];
-            const Con<T>& cx = d.con[d.conIdx[c0 + (unsigned)x]];
-            int j = i - 1;
-            while (j >= k) {
-                const Con<T>& cy = d.con[d.conIdx[c0 + (unsigned)K[j]]];
-                if (cy.a < cx.a || (cy.a == cx.a && cy.b <= cx.b)) break;
-                K[j + 1] = K[j];
-                j--;
-            }
-            K[j + 1] = x;
-        }
-    }
-    __syncthreads();
-    // ---- 2. sequence: contacts in key order, then point constraints by index
+    // ---- 1. sequence: the world's contact constraints, then its point constraints
     for (int k = tid; k < L; k += BS) {
         int s, ea = -1, eb = -1;
+        unsigned long long hi, lo;
         if (k < m) {
-            s = d.conIdx[c0 + (unsigned)K[k]];
+            s = d.conIdx[c0 + k];
             const Con<T>& c = d.con[s];
             if (c.dynMask & 1) ea = c.a;
             if (c.dynMask & 2) eb = c.b;
+            hi = (unsigned long long)c.sortKey;
+            lo = ((unsigned long long)(unsigned)c.a << 32) | (unsigned)c.b;
         } else {
             int pi = u0 + (k - m);
             s = -1 - pi;
             const PointCon<T>& pc = d.pc[pi];
-            bool act = pc.enabled && (d.B.label[pc.a] >= 0 || d.B.label[pc.b] >= 0);
+            bool act = pc.enabled && (B.label[pc.a] >= 0 || B.label[pc.b] >= 0);
             if (act) {
                 if (fl_motion(B.flags[pc.a]) == MOTION_DYNAMIC) ea = pc.a;
                 if (fl_motion(B.flags[pc.b]) == MOTION_DYNAMIC) eb = pc.b;
             } else {
                 s = 0x40000000;  // inactive: skipped by every phase
             }
+            hi = 1ull << 32;
+            lo = (unsigned long long)pi;
         }
+        K[2 * k] = hi;
+        K[2 * k + 1] = lo;
         d.seq[so + k] = s;
         d.seqA[so + k] = ea;
         d.seqB[so + k] = eb;
         d.prevA[so + k] = -1;
         d.prevB[so + k] = -1;
-        d.level[so + k] = 0;
+        d.nextA[so + k] = -1;
+        d.nextB[so + k] = -1;
     }
     for (int i = tid; i < nb; i += BS) d.bcnt[b0 + i] = 0;
     __syncthreads();
-    // ---- 3. per-body lists of sequence positions -> previous constraint of each body
+    // ---- 2. per-body lists of sequence elements (counting sort by dynamic body)
     for (int k = tid; k < L; k += BS) {
         int ea = d.seqA[so + k], eb = d.seqB[so + k];
         if (ea >= 0) atomicAdd(&d.bcnt[ea], 1);
@@ -378,56 +354,54 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         if (eb >= 0) BL[d.boff[eb] + atomicAdd(&d.bcnt[eb], 1)] = k;
     }
     __syncthreads();
+    // ---- 3. each body's list in solve order -> predecessor / successor on that body
     for (int i = tid; i < nb; i += BS) {
         int o = d.boff[b0 + i], c = d.bcnt[b0 + i];
-        for (int x = o + 1; x < o + c; x++) {  // insertion sort
+        for (int x = o + 1; x < o + c; x++) {  // insertion sort (lists are short)
             int v = BL[x];
             int y = x - 1;
-            while (y >= o && BL[y] > v) { BL[y + 1] = BL[y]; y--; }
+            while (y >= o && key_less(K, v, BL[y])) { BL[y + 1] = BL[y]; y--; }
             BL[y + 1] = v;
         }
         for (int x = o; x < o + c; x++) {
             int k = BL[x];
             int pv = x > o ? BL[x - 1] : -1;
-            if (d.seqA[so + k] == b0 + i) d.prevA[so + k] = pv;
-            else d.prevB[so + k] = pv;
+            int nx = x + 1 < o + c ? BL[x + 1] : -1;
+            if (d.seqA[so + k] == b0 + i) { d.prevA[so + k] = pv; d.nextA[so + k] = nx; }
+            else { d.prevB[so + k] = pv; d.nextB[so + k] = nx; }
         }
     }
     __syncthreads();
-    // ---- 4. levels = longest chain of shared-body predecessors (monotone relaxation)
-    for (;;) {
-        int changed = 0;
-        for (int k = tid; k < L; k += BS) {
-            int pa = d.prevA[so + k], pb = d.prevB[so + k];
-            int l = 0;
-            if (pa >= 0) l = max(l, d.level[so + pa] + 1);
-            if (pb >= 0) l = max(l, d.level[so + pb] + 1);
-            if (l > d.level[so + k]) { d.level[so + k] = l; changed = 1; }
-        }
-        if (!__syncthreads_or(changed)) break;
-    }
-    int mx = 0;
-    for (int k = tid; k < L; k += BS) mx = max(mx, d.level[so + k]);
-    int D = L > 0 ? block_reduce_max<BS>(mx, sh) + 1 : 0;
+    // ---- 4. levels: Kahn's algorithm on the predecessor DAG; level l holds the elements whose
+    // longest predecessor chain has length l, so elements of one level share no dynamic body
     int* LS = d.lvlStart + so + w;
-    for (int l = tid; l <= D; l += BS) LS[l] = 0;
+    if (tid == 0) sTail = 0;
     __syncthreads();
-    for (int k = tid; k < L; k += BS) atomicAdd(&LS[d.level[so + k]], 1);
-    __syncthreads();
-    block_scan_array<BS>(LS, D + 1, sh);
-    for (int i = tid; i < nb; i += BS) d.bcnt[b0 + i] = 0;  // reuse as per-level fill counters below
-    __syncthreads();
-    {
-        int* fill = d.lvlStart + (d.conCap + d.NPC + d.W + 1) + so + w;  // second half of the allocation
-        for (int l = tid; l <= D; l += BS) fill[l] = 0;
-        __syncthreads();
-        for (int k = tid; k < L; k += BS) {
-            int l = d.level[so + k];
-            d.sched[so + LS[l] + atomicAdd(&fill[l], 1)] = k;
-        }
+    for (int k = tid; k < L; k += BS) {
+        int deg = (d.prevA[so + k] >= 0 ? 1 : 0) + (d.prevB[so + k] >= 0 ? 1 : 0);
+        d.level[so + k] = deg;
+        if (deg == 0) d.sched[so + atomicAdd(&sTail, 1)] = k;
     }
     __syncthreads();
-    if (tid == 0) d.stats[2 * w] = D;
+    int lo = 0, hi = sTail;
+    int D = 0;
+    __syncthreads();  // every lane has read the first frontier before any lane extends it
+    while (lo < hi) {
+        if (tid == 0) LS[D] = lo;
+        for (int x = lo + tid; x < hi; x += BS) {
+            int e = d.sched[so + x];
+            int na = d.nextA[so + e], nbx = d.nextB[so + e];
+            if (na >= 0 && atomicSub(&d.level[so + na], 1) == 1) d.sched[so + atomicAdd(&sTail, 1)] = na;
+            if (nbx >= 0 && atomicSub(&d.level[so + nbx], 1) == 1) d.sched[so + atomicAdd(&sTail, 1)] = nbx;
+        }
+        __syncthreads();
+        lo = hi;
+        hi = sTail;
+        D++;
+        __syncthreads();
+    }
+    if (tid == 0) { LS[D] = lo; d.stats[2 * w] = D; }
+    __syncthreads();
 
     // ---- 5. user-constraint setup + warm start (contact-constraints.ts:906-955, constraints.ts:264-301)
     for (int k = tid; k < u; k += BS) {
@@ -529,6 +503,19 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         __syncthreads();
     }
     if (tid == 0) d.stats[2 * w + 1] = sweeps;
+    {  // byte model: per sweep 160 B/manifold + 192 B/point, per position sweep 240 B/manifold + 24 B/point
+        int pts = 0, awake = 0;
+        for (int k = tid; k < m; k += BS) pts += d.con[d.conIdx[c0 + k]].np;
+        for (int i = b0 + tid; i < b1; i += BS) awake += (fl_motion(B.flags[i]) == MOTION_DYNAMIC && !B.sleeping[i]) ? 1 : 0;
+        for (int o = 32; o > 0; o >>= 1) { pts += __shfl_xor(pts, o, 64); awake += __shfl_xor(awake, o, 64); }
+        if ((tid & 63) == 0) {
+            unsigned long long iv = (unsigned long long)d.s.velIters + 1;
+            unsigned long long by = iv * (160ull * (tid == 0 ? m : 0) + 192ull * pts) +
+                                    (unsigned long long)sweeps * (240ull * (tid == 0 ? m : 0) + 24ull * pts);
+            atomicAdd(&d.acc[ACC_BYTES_SOLVE], by);
+            atomicAdd(&d.acc[ACC_AWAKE], (unsigned long long)awake);
+        }
+    }
     // ---- 10. positions from the centre of mass + AABBs (update.ts:322-330)
     for (int i = b0 + tid; i < b1; i += BS) {
         int mo = fl_motion(B.flags[i]);

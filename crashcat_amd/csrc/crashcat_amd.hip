@@ -80,6 +80,7 @@ __global__ void k_set_pair_count(Dev<T> d) {
         int n = d.bpOff[d.N];
         d.counters[CNT_PAIRS] = n;
         d.counters[CNT_EPA] = 0;
+        d.counters[CNT_CONS] = 0;
         if (n > d.pairCap) atomicOr(&d.counters[CNT_OVERFLOW], 1);
     }
 }
@@ -135,15 +136,13 @@ struct CtxT : Impl {
     Dev<T> d;
     Alloc mem;
     int* scanSums = nullptr;
-    int N = 0, W = 0, NPC = 0, numShapes = 0;
+    int N = 0, W = 0, NPC = 0, numShapes = 0, maxWorld = 0;
     std::vector<int> hostWStart;
-    hipEvent_t ev[12];
-    bool events = false;
+    std::vector<hipEvent_t> ev;  // 7 per step of the last cc_step call
     int solverBS = 256;
     ~CtxT() {
         mem.release();
-        if (events)
-            for (auto& e : ev) (void)hipEventDestroy(e);
+        for (auto& e : ev) (void)hipEventDestroy(e);
     }
 };
 
@@ -370,34 +369,40 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     long long pairCap = (long long)c->cfg.max_pairs_per_body * std::max(nb, 1);
     if (pairCap > (1ll << 24) - 16) pairCap = (1ll << 24) - 16;  // two-level scan limit (4096 * 4096)
     d.pairCap = (int)pairCap;
-    d.conCap = d.pairCap;
-    d.epaCap = std::min(d.pairCap, std::max(4096, c->cfg.max_contacts_per_body * 512));
-    d.hashCap = next_pow2(2ll * d.pairCap);
+    d.conCap = (int)std::min<long long>((long long)c->cfg.max_contacts_per_body * std::max(nb, 1), pairCap);
+    d.epaCap = std::max(1024, d.pairCap / 4);
+    d.epaWorkers = std::min(d.epaCap, 65536);
+    d.hashCap = next_pow2(2ll * d.conCap);
+    int maxWorld = 0;
+    for (int w = 0; w < nw; w++) maxWorld = std::max(maxWorld, woff[w + 1] - woff[w]);
     d.bpCount = A.get<int>(nb + 1);
     d.bpOff = A.get<int>(nb + 1);
     d.pairA = A.get<int>(d.pairCap);
     d.pairB = A.get<int>(d.pairCap);
     d.counters = A.get<int>(CNT_N);
-    d.con = A.get<Con<T>>(d.pairCap);
-    d.tap = A.get<ConTap<T>>(d.pairCap);
-    d.conValid = A.get<int>(d.pairCap + 1);
-    d.conScan = A.get<int>(d.pairCap + 1);
+    d.con = A.get<Con<T>>(d.conCap);
+    d.tap = A.get<ConTap<T>>(d.conCap);
+    d.wConCnt = A.get<int>(nw + 1);
+    d.wConStart = A.get<int>(nw + 1);
+    d.wFill = A.get<int>(nw + 1);
     d.conIdx = A.get<int>(d.conCap);
     for (int k = 0; k < 2; k++) {
-        d.rec[k] = A.get<CacheRec<T>>(d.pairCap);
+        d.rec[k] = A.get<CacheRec<T>>(d.conCap);
         d.hkeys[k] = A.get<unsigned long long>(d.hashCap);
         d.hvals[k] = A.get<int>(d.hashCap);
     }
     d.epaPair = A.get<int>(d.epaCap);
     d.epaSimplex = A.get<Simplex<T>>(d.epaCap);
-    d.epaWork = A.get<EpaState<T>>(d.epaCap);
+    d.epaWork = A.get<EpaState<T>>(d.epaWorkers);
     size_t L = (size_t)d.conCap + npc;
-    d.skeys = A.get<unsigned long long>(2 * (size_t)d.conCap);
+    d.skeys = A.get<unsigned long long>(2 * L);
     d.seq = A.get<int>(L);
     d.seqA = A.get<int>(L);
     d.seqB = A.get<int>(L);
     d.prevA = A.get<int>(L);
     d.prevB = A.get<int>(L);
+    d.nextA = A.get<int>(L);
+    d.nextB = A.get<int>(L);
     d.level = A.get<int>(L);
     d.sched = A.get<int>(L);
     d.lvlStart = A.get<int>(2 * (L + nw + 1));
@@ -409,24 +414,24 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.stats = A.get<int>(2 * (size_t)nw);
     d.prevDt = A.get<T>(1);
     d.parity = A.get<int>(1);
+    d.acc = A.get<unsigned long long>(ACC_N);
     X->scanSums = A.get<int>(4096);
+    X->maxWorld = maxWorld;
     if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (workspace)");
     HIPCHK(c, hipMemset(d.counters, 0, CNT_N * sizeof(int)));
     HIPCHK(c, hipMemset(d.prevDt, 0, sizeof(T)));
+    HIPCHK(c, hipMemset(d.acc, 0, ACC_N * sizeof(unsigned long long)));
     HIPCHK(c, hipMemset(d.parity, 0, sizeof(int)));
     for (int k = 0; k < 2; k++) HIPCHK(c, hipMemset(d.hkeys[k], 0xff, d.hashCap * sizeof(unsigned long long)));
-    HIPCHK(c, hipMemset(d.conValid, 0, (d.pairCap + 1) * sizeof(int)));
+
     HIPCHK(c, hipMemset(d.stats, 0, 2 * nw * sizeof(int)));
     int blocks = (nb + 255) / 256;
     if (nb > 0) hipLaunchKernelGGL(k_init_bodies<T>, dim3(blocks), dim3(256), 0, c->stream, d);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (!X->events) {
-        for (auto& e : X->ev) HIPCHK(c, hipEventCreate(&e));
-        X->events = true;
-    }
     int bs = c->cfg.block_solver;
-    X->solverBS = (bs == 64 || bs == 128 || bs == 256 || bs == 512 || bs == 1024) ? bs : 256;
+    if (bs == 64 || bs == 128 || bs == 256 || bs == 512 || bs == 1024) X->solverBS = bs;
+    else X->solverBS = maxWorld > 4096 ? 1024 : (maxWorld > 1024 ? 512 : 256);
     return CC_OK;
 }
 
@@ -472,8 +477,9 @@ static void launch_world(CtxT<T>* X, hipStream_t st, T dt) {
     }
 }
 
+#define CC_PHASES 6
 template <typename T>
-static int step_impl(cc_ctx* c, double dtd, int nsteps, bool timed) {
+static int step_impl(cc_ctx* c, double dtd, int nsteps) {
     auto* X = static_cast<CtxT<T>*>(c->impl);
     Dev<T>& d = X->d;
     hipStream_t st = c->stream;
@@ -481,10 +487,16 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps, bool timed) {
     int N = d.N;
     int nbB = (N + 255) / 256;
     const double margin = (double)d.s.spec;
+    size_t needEv = (size_t)(CC_PHASES + 1) * nsteps;
+    while (X->ev.size() < needEv) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        X->ev.push_back(e);
+    }
     for (int s = 0; s < nsteps; s++) {
+        hipEvent_t* E = X->ev.data() + (size_t)(CC_PHASES + 1) * s;
         int e = 0;
-        bool rec = timed && s == nsteps - 1;
-        auto mark = [&]() { if (rec) (void)hipEventRecord(X->ev[e++], st); };
+        auto mark = [&]() { (void)hipEventRecord(E[e++], st); };
         mark();
         hipLaunchKernelGGL(k_accel<T>, dim3(nbB), dim3(256), 0, st, d, dt);
         hipLaunchKernelGGL((k_bp<T, false>), dim3(nbB), dim3(256), 0, st, d, margin);
@@ -492,15 +504,17 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps, bool timed) {
         hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
         hipLaunchKernelGGL((k_bp<T, true>), dim3(nbB), dim3(256), 0, st, d, margin);
         mark();  // broadphase
-        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(d.pairCap + 1, 256, 2048)), dim3(256), 0, st, d.conValid, 0, d.pairCap + 1);
         hipLaunchKernelGGL(k_narrowphase<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d, dt);
         mark();  // narrowphase (GJK + manifold + constraint setup)
-        hipLaunchKernelGGL(k_epa<T>, dim3(grid_for(d.epaCap, 64, 4096)), dim3(64), 0, st, d, dt);
+        hipLaunchKernelGGL(k_epa<T>, dim3((d.epaWorkers + 63) / 64), dim3(64), 0, st, d, dt);
         hipLaunchKernelGGL(k_count_epa<T>, dim3(1), dim3(64), 0, st, d);
         mark();  // EPA
         hipLaunchKernelGGL(k_wake<T>, dim3(nbB), dim3(256), 0, st, d);
-        scan_launch(st, d.conValid, d.conScan, d.pairCap, X->scanSums);
-        hipLaunchKernelGGL(k_compact<T>, dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(2 * (d.W + 1), 256, 64)), dim3(256), 0, st, d.wConCnt, 0, d.W + 1);
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(d.W + 1, 256, 64)), dim3(256), 0, st, d.wFill, 0, d.W + 1);
+        hipLaunchKernelGGL(k_con_world_count<T>, dim3(grid_for(d.conCap, 256, 2048)), dim3(256), 0, st, d);
+        scan_launch(st, d.wConCnt, d.wConStart, d.W, X->scanSums);
+        hipLaunchKernelGGL(k_con_world_scatter<T>, dim3(grid_for(d.conCap, 256, 2048)), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_uf_init<T>, dim3(nbB), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_uf_link<T>, dim3(grid_for(d.conCap + d.NPC, 256, 2048)), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_uf_compress<T>, dim3(nbB), dim3(256), 0, st, d);
@@ -511,10 +525,10 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps, bool timed) {
         hipLaunchKernelGGL(k_hash_insert<T>, dim3(grid_for(d.conCap, 256, 2048)), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_end_step<T>, dim3(1), dim3(64), 0, st, d, dt);
         mark();  // cache
-        if (rec) c->nPhase = e;
     }
     HIPCHK(c, hipGetLastError());
     c->steps += nsteps;
+    c->nPhase = nsteps;  // number of steps recorded
     return CC_OK;
 }
 
@@ -522,8 +536,8 @@ extern "C" int cc_step(cc_ctx* c, double dt, int32_t nsteps) {
     if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
     if (nsteps < 0 || !(dt >= 0)) return fail(c, CC_E_ARG, "invalid dt or nsteps");
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    if (c->cfg.precision == CC_PRECISION_FP64) return step_impl<double>(c, dt, nsteps, true);
-    return step_impl<float>(c, dt, nsteps, true);
+    if (c->cfg.precision == CC_PRECISION_FP64) return step_impl<double>(c, dt, nsteps);
+    return step_impl<float>(c, dt, nsteps);
 }
 
 template <typename T>
@@ -653,14 +667,18 @@ static int contacts_impl(cc_ctx* c, cc_contact_out* out, int cap, int* count) {
     *count = n;
     if (!out || cap <= 0) return CC_OK;
     std::vector<int> idx(n);
-    if (n > 0) HIPCHK(c, hipMemcpy(idx.data(), d.conIdx, n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; k++) idx[k] = k;
     std::vector<cc_contact_out> res;
     res.reserve(n);
+    std::vector<Con<T>> hcs(n);
+    std::vector<ConTap<T>> hts(n);
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(hcs.data(), d.con, n * sizeof(Con<T>), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(hts.data(), d.tap, n * sizeof(ConTap<T>), hipMemcpyDeviceToHost));
+    }
     for (int k = 0; k < n; k++) {
-        Con<T> h;
-        ConTap<T> t;
-        HIPCHK(c, hipMemcpy(&h, d.con + idx[k], sizeof(h), hipMemcpyDeviceToHost));
-        HIPCHK(c, hipMemcpy(&t, d.tap + idx[k], sizeof(t), hipMemcpyDeviceToHost));
+        const Con<T>& h = hcs[idx[k]];
+        const ConTap<T>& t = hts[idx[k]];
         cc_contact_out o;
         memset(&o, 0, sizeof(o));
         o.body_a = h.a; o.body_b = h.b; o.num_points = h.np;
@@ -720,13 +738,9 @@ static int counters_impl(cc_ctx* c, cc_step_counters* o) {
     }
     int nc = (int)o->num_contacts;
     if (nc > 0) {
-        std::vector<int> idx(nc);
-        HIPCHK(c, hipMemcpy(idx.data(), d.conIdx, nc * sizeof(int), hipMemcpyDeviceToHost));
-        for (int k = 0; k < nc; k++) {
-            int np;
-            HIPCHK(c, hipMemcpy(&np, &d.con[idx[k]].np, sizeof(int), hipMemcpyDeviceToHost));
-            o->num_points += np;
-        }
+        std::vector<Con<T>> hc(nc);
+        HIPCHK(c, hipMemcpy(hc.data(), d.con, nc * sizeof(Con<T>), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nc; k++) o->num_points += hc[k].np;
     }
     return CC_OK;
 }
@@ -738,21 +752,53 @@ extern "C" int cc_get_counters(cc_ctx* c, cc_step_counters* o) {
     return counters_impl<float>(c, o);
 }
 
+template <typename T>
+static int phase_impl(cc_ctx* c, double* out, int cap, int* count) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double sum[CC_PHASES] = {0};
+    for (int s = 0; s < c->nPhase; s++) {
+        hipEvent_t* E = X->ev.data() + (size_t)(CC_PHASES + 1) * s;
+        for (int p = 0; p < CC_PHASES; p++) {
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, E[p], E[p + 1]));
+            sum[p] += ms;
+        }
+    }
+    *count = c->nPhase > 0 ? CC_PHASES : 0;
+    for (int p = 0; p < *count && p < cap; p++) out[p] = sum[p];
+    return CC_OK;
+}
+
 extern "C" int cc_get_phase_times(cc_ctx* c, double* out, int32_t cap, int32_t* count) {
     if (!c || !c->impl || !count) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return phase_impl<double>(c, out, cap, count);
+    return phase_impl<float>(c, out, cap, count);
+}
+
+template <typename T>
+static int totals_impl(cc_ctx* c, cc_totals* o) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    Impl* im = c->impl;
-    hipEvent_t* ev;
-    if (c->cfg.precision == CC_PRECISION_FP64) ev = static_cast<CtxT<double>*>(im)->ev;
-    else ev = static_cast<CtxT<float>*>(im)->ev;
-    int n = c->nPhase > 1 ? c->nPhase - 1 : 0;
-    *count = n;
-    for (int i = 0; i < n && i < cap; i++) {
-        float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-        out[i] = ms;
-    }
+    unsigned long long a[ACC_N];
+    HIPCHK(c, hipMemcpy(a, X->d.acc, sizeof(a), hipMemcpyDeviceToHost));
+    memset(o, 0, sizeof(*o));
+    o->steps = (int64_t)a[ACC_STEPS];
+    o->pairs = (int64_t)a[ACC_PAIRS];
+    o->contacts = (int64_t)a[ACC_CONS];
+    o->points = (int64_t)a[ACC_POINTS];
+    o->epa = (int64_t)a[ACC_EPA];
+    o->awake_body_steps = (int64_t)a[ACC_AWAKE];
+    o->solver_bytes = (int64_t)a[ACC_BYTES_SOLVE];
     return CC_OK;
+}
+
+extern "C" int cc_get_totals(cc_ctx* c, cc_totals* o) {
+    if (!c || !c->impl || !o) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return totals_impl<double>(c, o);
+    return totals_impl<float>(c, o);
 }
 
 // Broadphase-only tap on caller-supplied AABBs (one world).

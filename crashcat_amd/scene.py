@@ -402,3 +402,37 @@ def replicate(builder, copies, seed0, **kw):
     for c in range(copies):
         builder(seed=seed0 + c, scene=sc, **kw)
     return sc
+
+
+def replicate_world(scene, copies, seed, extent=(60.0, 40.0, 60.0)):
+    """A batch scene of `copies` worlds that reuse world 0's bodies and shapes with positions,
+    orientations and velocities redrawn per world (config-2 sampling: positions uniform in the
+    extent above the floor, v ~ N(0,1), w ~ N(0,0.5)); static bodies are copied unchanged."""
+    base = scene.bodies()[: scene.offsets()[1]]
+    out = Scene(scene.settings)
+    out.shapes = scene.shapes
+    out._shape_index = scene._shape_index
+    out.point_constraints = []
+    n = len(base)
+    arrs = []
+    for c in range(copies):
+        b = base.copy()
+        if c > 0:
+            rng = np.random.Generator(np.random.PCG64(seed * 1000003 + c))
+            dyn = b["motion_type"] != _abi.CC_MOTION_STATIC
+            k = int(dyn.sum())
+            b["position"][dyn, 0] = rng.uniform(-extent[0] / 2, extent[0] / 2, k)
+            b["position"][dyn, 1] = rng.uniform(1.0, 1.0 + extent[1], k)
+            b["position"][dyn, 2] = rng.uniform(-extent[2] / 2, extent[2] / 2, k)
+            u = rng.uniform(size=(k, 3))
+            a = np.sqrt(1 - u[:, 0])
+            bb = np.sqrt(u[:, 0])
+            b["quaternion"][dyn] = np.stack([a * np.sin(2 * np.pi * u[:, 1]), a * np.cos(2 * np.pi * u[:, 1]),
+                                              bb * np.sin(2 * np.pi * u[:, 2]), bb * np.cos(2 * np.pi * u[:, 2])], 1)
+            b["linear_velocity"][dyn] = rng.normal(0, 1, (k, 3))
+            b["angular_velocity"][dyn] = rng.normal(0, 0.5, (k, 3))
+        arrs.append(b)
+    out._bodies = np.concatenate(arrs) if arrs else np.zeros(0, dtype=BODY_DTYPE)
+    out.rows = [None] * len(out._bodies)
+    out.world_offsets = [i * n for i in range(copies + 1)]
+    return out

@@ -180,39 +180,23 @@ def convex_hull(positions, convex_radius=DEFAULT_CONVEX_RADIUS, density=DEFAULT_
     fc = []
     for f in faces_idx:
         v = pts_in[f]
-        nrm = np.zeros(3)
-        for k in range(1, len(f) - 1):
-            nrm += np.cross(v[k] - v[0], v[k + 1] - v[0])
-        fc.append((v.mean(axis=0), nrm))
-    # getCenterOfMassAndVolume (convex-hull-builder.ts:492-550)
-    v4 = np.zeros(3)
-    for c, _ in fc:
-        v4 += c
-    v4 = v4 / len(fc)
-    vol = 0.0
-    com = np.zeros(3)
-    for f in faces_idx:
-        v1 = pts_in[f[0]]
-        v2 = pts_in[f[1]]
-        for k in range(2, len(f)):
-            v3 = pts_in[f[k]]
-            vt = float(np.dot(v1 - v4, np.cross(v2 - v4, v3 - v4)))
-            vol += vt
-            com += vt * (v1 + v2 + v3 + v4)
-            v2 = v3
-    com = com / (4.0 * vol)
+        e1 = v[1:-1] - v[0]
+        e2 = v[2:] - v[0]
+        fc.append((v.mean(axis=0), np.cross(e1, e2).sum(axis=0)))
+    # getCenterOfMassAndVolume (convex-hull-builder.ts:492-550): tetrahedra against the
+    # mean of the face centroids, one triangle fan per face
+    v4 = np.sum([c for c, _ in fc], axis=0) / len(fc)
+    tri = np.array([(f[0], f[k - 1], f[k]) for f in faces_idx for k in range(2, len(f))])
+    t1, t2, t3 = pts_in[tri[:, 0]], pts_in[tri[:, 1]], pts_in[tri[:, 2]]
+    vt = np.einsum("ij,ij->i", t1 - v4, np.cross(t2 - v4, t3 - v4))
+    vol = float(vt.sum())
+    com = (vt[:, None] * (t1 + t2 + t3 + v4)).sum(axis=0) / (4.0 * vol)
     volume = vol / 6.0
     # covariance inertia (convex-hull.ts:137-193)
     canon = np.array([[1 / 60, 1 / 120, 1 / 120], [1 / 120, 1 / 60, 1 / 120], [1 / 120, 1 / 120, 1 / 60]])
-    cov = np.zeros((3, 3))
-    for f in faces_idx:
-        t1 = pts_in[f[0]] - com
-        t2 = pts_in[f[1]] - com
-        for k in range(2, len(f)):
-            t3 = pts_in[f[k]] - com
-            A = np.column_stack([t1, t2, t3])
-            cov += np.linalg.det(A) * (A @ canon @ A.T)
-            t2 = t3
+    A = np.stack([t1 - com, t2 - com, t3 - com], axis=2)  # columns are the tetrahedron vertices
+    det = np.linalg.det(A)
+    cov = np.einsum("t,tij,jk,tlk->il", det, A, canon, A)
     inertia = np.eye(3) * np.trace(cov) - cov
     # runtime representation: points renumbered in face order, relative to COM
     vmap = {}
@@ -227,36 +211,40 @@ def convex_hull(positions, convex_radius=DEFAULT_CONVEX_RADIUS, density=DEFAULT_
             loop.append(vmap[oi])
         faces.append(loop)
     points = np.array(points)
-    normals = []
-    consts = []
-    for (c, nrm) in fc:
-        nn = nrm / np.linalg.norm(nrm)
-        normals.append(nn)
-        consts.append(-float(np.dot(nn, c - com)))
-    normals = np.array(normals)
-    consts = np.array(consts)
+    nrm = np.array([n for _, n in fc])
+    normals = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    cent = np.array([c for c, _ in fc])
+    consts = -np.einsum("ij,ij->i", normals, cent - com)
     # per-point supporting planes (convex-hull.ts:272-330)
+    vface = [[] for _ in range(len(points))]
+    for fi, f in enumerate(faces):
+        for p in f:
+            if fi not in vface[p]:
+                vface[p].append(fi)
     pfaces = []
     cos1 = math.cos(math.pi / 180.0)
+    nl = [tuple(float(x) for x in n) for n in normals]
     for p in range(len(points)):
-        nb = [fi for fi, f in enumerate(faces) if p in f]
+        nb = vface[p]
         best3 = None
         biggest = 0.05
         best2 = None
         smallest = cos1
-        for i1 in range(len(nb)):
-            n1 = normals[nb[i1]]
-            for i2 in range(i1 + 1, len(nb)):
-                n2 = normals[nb[i2]]
-                cr = np.cross(n1, n2)
-                d = float(np.dot(n1, n2))
-                if d < smallest:
-                    smallest = d
+        k = len(nb)
+        for i1 in range(k):
+            a = nl[nb[i1]]
+            for i2 in range(i1 + 1, k):
+                b = nl[nb[i2]]
+                cx, cy, cz = a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]
+                dt_ = a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+                if dt_ < smallest:
+                    smallest = dt_
                     best2 = (nb[i1], nb[i2])
-                for i3 in range(i2 + 1, len(nb)):
-                    v = abs(float(np.dot(cr, normals[nb[i3]])))
-                    if v > biggest:
-                        biggest = v
+                for i3 in range(i2 + 1, k):
+                    c = nl[nb[i3]]
+                    vv = abs(cx * c[0] + cy * c[1] + cz * c[2])
+                    if vv > biggest:
+                        biggest = vv
                         best3 = (nb[i1], nb[i2], nb[i3])
         if best3 is not None:
             pfaces.append(best3)
@@ -267,62 +255,40 @@ def convex_hull(positions, convex_radius=DEFAULT_CONVEX_RADIUS, density=DEFAULT_
     # convex radius reduction (convex-hull.ts:333-429)
     cr = float(convex_radius)
     if cr > 0:
-        min_size = float("inf")
-        for n, c in zip(normals, consts):
-            md = 0.0
-            for pt in points:
-                dist = -(float(np.dot(n, pt)) + c)
-                md = max(md, dist)
-            min_size = min(min_size, md)
-        cr = min(cr, 0.5 * min_size)
+        dist = -(points @ normals.T + consts)
+        cr = min(cr, 0.5 * float(np.maximum(dist, 0.0).max(axis=0).min()))
+    P = len(points)
+    n_f = np.array([len(pf) for pf in pfaces])
+    c1 = normals[[pf[0] for pf in pfaces]]
+    c2 = normals[[pf[1] if len(pf) > 1 else pf[0] for pf in pfaces]]
+    perp = np.cross(c1, c2)
+    pl = np.linalg.norm(perp, axis=1, keepdims=True)
+    perp = np.divide(perp, pl, out=np.zeros_like(perp), where=pl > 0)
+    c3 = np.where((n_f == 3)[:, None], normals[[pf[2] if len(pf) > 2 else pf[0] for pf in pfaces]], perp)
     if cr > 0:
-        for pi, pf in enumerate(pfaces):
-            if len(pf) == 1:
-                continue
-            n1 = normals[pf[0]]
-            n2 = normals[pf[1]]
-            if len(pf) == 3:
-                n3 = normals[pf[2]]
-                mask = np.array([1.0, 1.0, 1.0])
-            else:
-                n3 = np.cross(n1, n2)
-                n3 = n3 / np.linalg.norm(n3)
-                mask = np.array([1.0, 1.0, 0.0])
-            N = np.array([n1, n2, n3])  # rows
-            det = float(np.linalg.det(N))
-            if abs(det) < 1e-12:
+        m = n_f > 1
+        if m.any():
+            N = np.stack([c1[m], c2[m], c3[m]], axis=1)  # rows are plane normals
+            dets = np.linalg.det(N)
+            if np.any(np.abs(dets) < 1e-12):
                 cr = 0.0
-                break
-            # adjoint(N) * mask with a perspective divide by det, then divided by det again
-            adj = np.linalg.inv(N) * det
-            shift = (adj @ mask) / det / det
-            off = float(np.linalg.norm(shift))
-            if off > 1:
-                cr = min(cr, max_error_convex_radius / (off - 1))
-    # shrunk support points (convex-hull.ts:732-786)
-    shrunk = np.zeros_like(points)
-    for pi, pf in enumerate(pfaces):
-        pt = points[pi]
-        if len(pf) == 1:
-            shrunk[pi] = pt - normals[pf[0]] * cr
-            continue
-        pl = []
-        for k in range(2):
-            n = normals[pf[k]]
-            pl.append((n, -float(np.dot(n, pt)) - (-cr)))  # fromNormalAndPoint then offset(-r)
-        if len(pf) == 3:
-            n = normals[pf[2]]
-            pl.append((n, -float(np.dot(n, pt)) + cr))
-        else:
-            n = np.cross(pl[0][0], pl[1][0])
-            n = n / np.linalg.norm(n)
-            pl.append((n, -float(np.dot(n, pt))))
-        (c1, w1), (c2, w2), (c3, w3) = pl
-        den = float(np.dot(c1, np.cross(c2, c3)))
-        if den == 0:
-            shrunk[pi] = pt - c1 * cr
-        else:
-            shrunk[pi] = (-w1 * np.cross(c2, c3) - w2 * np.cross(c3, c1) - w3 * np.cross(c1, c2)) / den
+            else:
+                mask = np.where((n_f[m] == 3)[:, None], 1.0, np.array([1.0, 1.0, 0.0]))
+                # adjoint(N) * mask with a perspective divide by det, then divided by det again
+                shift = np.einsum("pij,pj->pi", np.linalg.inv(N), mask) / dets[:, None]
+                off = np.linalg.norm(shift, axis=1)
+                big = off > 1
+                if big.any():
+                    cr = min(cr, float((max_error_convex_radius / (off[big] - 1)).min()))
+    # shrunk support points (convex-hull.ts:732-786): 3-plane intersection of the offset planes
+    w1 = -np.einsum("ij,ij->i", c1, points) + cr
+    w2 = -np.einsum("ij,ij->i", c2, points) + cr
+    w3 = -np.einsum("ij,ij->i", c3, points) + np.where(n_f == 3, cr, 0.0)
+    c23, c31, c12 = np.cross(c2, c3), np.cross(c3, c1), np.cross(c1, c2)
+    den = np.einsum("ij,ij->i", c1, c23)
+    inter = -(w1[:, None] * c23 + w2[:, None] * c31 + w3[:, None] * c12) / np.where(den == 0, 1.0, den)[:, None]
+    fallback = points - c1 * cr
+    shrunk = np.where(((n_f == 1) | (den == 0))[:, None], fallback, inter)
     mn = points.min(axis=0)
     mx = points.max(axis=0)
     return Shape(CONVEX_HULL, convex_radius=cr, density=density, center_of_mass=tuple(com),
@@ -404,6 +370,23 @@ def _jacobi(inertia):
     return converged, r2, d2
 
 
+def _sorted_diagonal(a):
+    """The exit path of decomposePrincipalMomentsOfInertia for a diagonal tensor: eigenvalues
+    sorted descending, identity columns permuted, made right handed."""
+    d = np.array([a[0, 0], a[1, 1], a[2, 2]])
+    idx = [0, 1, 2]
+    if d[idx[0]] < d[idx[1]]:
+        idx[0], idx[1] = idx[1], idx[0]
+    if d[idx[1]] < d[idx[2]]:
+        idx[1], idx[2] = idx[2], idx[1]
+    if d[idx[0]] < d[idx[1]]:
+        idx[0], idx[1] = idx[1], idx[0]
+    r2 = np.eye(3)[:, idx].copy()
+    if np.dot(np.cross(r2[:, 0], r2[:, 1]), r2[:, 2]) < 0:
+        r2[:, 2] = -r2[:, 2]
+    return True, r2, d[idx]
+
+
 def _quat_from_mat3(r):
     """quat.fromMat3 (gl-matrix) on a column-major 3x3."""
     m = [r[0, 0], r[1, 0], r[2, 0], r[0, 1], r[1, 1], r[2, 1], r[0, 2], r[1, 2], r[2, 2]]
@@ -442,7 +425,12 @@ def motion_mass(mass, inertia, allowed_dofs=0b111111):
     inv_mass = 0.0 if trans == 0 else 1.0 / mass
     if rot_axes == 0:
         return inv_mass, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)
-    ok, r, d = _jacobi(inertia)
+    I = np.asarray(inertia, dtype=np.float64)
+    if I[0, 1] == 0 and I[0, 2] == 0 and I[1, 2] == 0:
+        # off-diagonal sum is zero: the Jacobi loop exits before its first rotation
+        ok, r, d = _sorted_diagonal(I)
+    else:
+        ok, r, d = _jacobi(I)
     if ok:
         if d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > 1e-20:
             inv = tuple((1.0 / x) if x != 0 else 0.0 for x in d)

@@ -199,6 +199,18 @@ typedef struct cc_step_counters {
     int64_t overflow;           /* non-zero if a capacity was exceeded              */
 } cc_step_counters;
 
+/* Running totals since upload (all worlds, all steps); feed the algorithmic byte model. */
+typedef struct cc_totals {
+    int64_t steps;
+    int64_t pairs;              /* sum over steps of broadphase pairs                */
+    int64_t contacts;           /* sum over steps of contact constraints             */
+    int64_t points;             /* sum over steps of contact points                  */
+    int64_t epa;                /* EPA invocations                                   */
+    int64_t awake_body_steps;   /* sum over steps of awake dynamic bodies            */
+    int64_t solver_bytes;       /* byte-model bytes of the velocity + position sweeps */
+    int64_t reserved[9];
+} cc_totals;
+
 typedef struct cc_config {
     int32_t precision;          /* CC_PRECISION_*                                   */
     int32_t device;             /* HIP device ordinal                               */
@@ -252,8 +264,12 @@ int cc_broadphase_aabbs(cc_ctx* ctx, const float* aabb_min, const float* aabb_ma
                         const int32_t* motion_type, const int32_t* sleeping, const int32_t* object_layer,
                         int32_t n, double margin, int32_t* out_pairs, int32_t capacity, int32_t* out_count);
 
-/* Per-phase timing of the last cc_step (milliseconds, HIP events on the context stream). */
+/* Per-phase device time summed over every step of the last cc_step call (milliseconds,
+ * HIP events on the context stream).  Phases: broadphase, narrowphase (GJK, manifold,
+ * constraint setup), EPA, islands, solver (sort, levels, warm start, velocity and
+ * position sweeps, integration, sleep), contact-cache hash. */
 int cc_get_phase_times(cc_ctx* ctx, double* out_ms, int32_t capacity, int32_t* out_count);
+int cc_get_totals(cc_ctx* ctx, cc_totals* out);
 
 #ifdef __cplusplus
 }

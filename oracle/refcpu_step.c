@@ -164,6 +164,16 @@ static M3 inv_inertia(V3 invDiag, Q4 iq, const M3* bodyR, int dof) {
     return o;
 }
 
+void oc_inverse_inertia(const double* inv_diag, const double* inertia_rot, const double* body_quat, int32_t dofs,
+                        double* out9) {
+    V3 d = {inv_diag[0], inv_diag[1], inv_diag[2]};
+    Q4 iq = {inertia_rot[0], inertia_rot[1], inertia_rot[2], inertia_rot[3]};
+    Q4 bq = {body_quat[0], body_quat[1], body_quat[2], body_quat[3]};
+    M3 R = m3_from_quat(bq);
+    M3 o = inv_inertia(d, iq, &R, dofs);
+    for (int k = 0; k < 9; k++) out9[k] = o.m[k];
+}
+
 static void sleep_points(oc_world* w, OBody* b, V3 out[3]) { /* getSleepTestPoints src/body/sleep.ts:22-76 */
     const cc_shape_desc* s = body_shape(w, b);
     out[0] = b->com;
@@ -583,26 +593,59 @@ static int pair_ok(const cc_settings* s, const OBody* a, int ia, const OBody* b,
     if (a->motion != CC_MOTION_DYNAMIC && b->motion != CC_MOTION_DYNAMIC) return 0;
     return 1;
 }
+static int cmp_pair_qo(const void* x, const void* y) {
+    const int* a = (const int*)x;
+    const int* b = (const int*)y;
+    if (a[0] != b[0]) return a[0] - b[0];
+    return a[1] - b[1];
+}
 static void push_pair(oc_world* w, int a, int b) {
     if (w->npairs + 1 > w->cpairs) { w->cpairs = w->cpairs ? 2 * w->cpairs : 1024; w->pairs = (int*)xrealloc(w->pairs, sizeof(int) * 2 * (size_t)w->cpairs); }
     w->pairs[2 * w->npairs] = a;
     w->pairs[2 * w->npairs + 1] = b;
     w->npairs++;
 }
+/* Pair enumeration: sort-and-sweep on x over the expanded AABBs, then pairs ordered by
+ * (querying body, other body) — the same list a brute-force double loop produces. */
+static double* g_sap_key;
+static int cmp_sap(const void* x, const void* y) {
+    int a = *(const int*)x, b = *(const int*)y;
+    if (g_sap_key[a] != g_sap_key[b]) return g_sap_key[a] < g_sap_key[b] ? -1 : 1;
+    return a - b;
+}
 static void find_pairs(oc_world* w) {
     double m = w->s.speculative_contact_distance;
     w->npairs = 0;
-    for (int i = 0; i < w->nb; i++) {
+    int n = w->nb;
+    int* ord = (int*)xrealloc(NULL, sizeof(int) * (size_t)(n ? n : 1));
+    double* key = (double*)xrealloc(NULL, sizeof(double) * (size_t)(n ? n : 1));
+    for (int i = 0; i < n; i++) {
+        ord[i] = i;
+        key[i] = w->b[i].aabbMin.x - (queries(&w->b[i]) ? m : 0.0);
+    }
+    g_sap_key = key;
+    qsort(ord, (size_t)n, sizeof(int), cmp_sap);
+    for (int oi = 0; oi < n; oi++) {
+        int i = ord[oi];
         OBody* a = &w->b[i];
-        if (!queries(a)) continue;
-        V3 mn = vsub(a->aabbMin, v3(m, m, m)), mx = vadd(a->aabbMax, v3(m, m, m));
-        for (int j = 0; j < w->nb; j++) {
+        double ahi = a->aabbMax.x + (queries(a) ? m : 0.0);
+        for (int oj = oi + 1; oj < n; oj++) {
+            int j = ord[oj];
+            if (key[j] > ahi) break;
             OBody* b = &w->b[j];
-            if (!pair_ok(&w->s, a, i, b, j)) continue;
-            if (!overlap(mn, mx, b->aabbMin, b->aabbMax)) continue;
-            push_pair(w, i, j);
+            int q = -1, o = -1;
+            if (pair_ok(&w->s, a, i, b, j)) { q = i; o = j; }
+            else if (pair_ok(&w->s, b, j, a, i)) { q = j; o = i; }
+            if (q < 0) continue;
+            OBody* Q = &w->b[q];
+            V3 mn = vsub(Q->aabbMin, v3(m, m, m)), mx = vadd(Q->aabbMax, v3(m, m, m));
+            if (!overlap(mn, mx, w->b[o].aabbMin, w->b[o].aabbMax)) continue;
+            push_pair(w, q, o);
         }
     }
+    free(ord);
+    free(key);
+    qsort(w->pairs, (size_t)w->npairs, 2 * sizeof(int), cmp_pair_qo);
 }
 
 /* ---- narrowphase of one pair with the reduction collector (src/update.ts:333-563, 714-850) ---- */
