@@ -279,6 +279,7 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<
     T fr = combine(B.friction[ia], B.friction[ib], fl_fmode(fa), fl_fmode(fb));
     T re = combine(B.restitution[ia], B.restitution[ib], fl_rmode(fa), fl_rmode(fb));
     k.a = ia; k.b = ib;
+    k.pairIdx = slot;
     k.dynMask = (dynA ? 1 : 0) | (dynB ? 2 : 0);
     k.movMask = (fl_motion(fa) != MOTION_STATIC ? 1 : 0) | (fl_motion(fb) != MOTION_STATIC ? 2 : 0);
     k.sortKey = sort_key((unsigned)ia, (unsigned)ib, 0xffffffffu, 0xffffffffu);

@@ -89,10 +89,60 @@ template <typename T> CC_HD Q4<T> qnormalize(Q4<T> a) {
     if (l > T(0)) { l = T(1) / sqrt(l); a.x *= l; a.y *= l; a.z *= l; a.w *= l; }
     return a;
 }
+// FP64 sine and cosine by the fdlibm method (Sun Microsystems, 1993): minimax polynomial
+// kernels on [-pi/4, pi/4] and a two-part Cody-Waite reduction by pi/2 outside it.  Only
+// IEEE add/mul/div are used, so with contraction off the results are the same bits on the
+// GPU and on any host; JavaScript engines descend from the same kernels.
+CC_HD double fd_ksin(double x, double y, int iy) {
+    const double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03,
+                 S3 = -1.98412698298579493134e-04, S4 = 2.75573137070700676789e-06,
+                 S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
+    double z = x * x, v = z * x;
+    double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+    if (iy == 0) return x + v * (S1 + z * r);
+    return x - ((z * (0.5 * y - v * r) - y) - v * S1);
+}
+CC_HD double fd_kcos(double x, double y) {
+    const double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03,
+                 C3 = 2.48015872894767294178e-05, C4 = -2.75573143513906633035e-07,
+                 C5 = 2.08757232129817482790e-09, C6 = -1.13596475577881948265e-11;
+    union { double d; unsigned long long u; } b;
+    b.d = x;
+    unsigned ix = (unsigned)(b.u >> 32) & 0x7fffffffu;
+    double z = x * x;
+    double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+    if (ix < 0x3FD33333u) return 1.0 - (0.5 * z - (z * r - x * y));  // |x| < 0.3
+    double qx;
+    if (ix > 0x3fe90000u) qx = 0.28125;                                // |x| > 0.78125
+    else { b.u = (unsigned long long)(ix - 0x00200000u) << 32; qx = b.d; }  // x/4, low word cleared
+    double hz = 0.5 * z - qx, a = 1.0 - qx;
+    return a - (hz - (z * r - x * y));
+}
+CC_HD void fd_sincos(double x, double& s, double& c) {
+    const double pio4 = 7.85398163397448278999e-01;
+    if (fabs(x) <= pio4) { s = fd_ksin(x, 0.0, 0); c = fd_kcos(x, 0.0); return; }
+    const double invpio2 = 6.36619772367581382433e-01, pio2_1 = 1.57079632673412561417e+00,
+                 pio2_1t = 6.07710050650619224932e-11;
+    double fn = (double)(long long)(x * invpio2 + (x < 0 ? -0.5 : 0.5));
+    long long n = (long long)fn;
+    double r = x - fn * pio2_1, w = fn * pio2_1t;
+    double y0 = r - w, y1 = (r - y0) - w;
+    double ks = fd_ksin(y0, y1, 1), kc = fd_kcos(y0, y1);
+    switch (n & 3) {
+        case 0: s = ks; c = kc; break;
+        case 1: s = kc; c = -ks; break;
+        case 2: s = -ks; c = -kc; break;
+        default: s = -kc; c = ks; break;
+    }
+}
+CC_HD void sin_cos(double x, double& s, double& c) { fd_sincos(x, s, c); }
+CC_HD void sin_cos(float x, float& s, float& c) { s = sinf(x); c = cosf(x); }
+
 template <typename T> CC_HD Q4<T> qaxis_angle(V3<T> axis, T rad) {
     T h = rad * T(0.5);
-    T s = sin(h);
-    Q4<T> r; r.x = s * axis.x; r.y = s * axis.y; r.z = s * axis.z; r.w = cos(h);
+    T s, c;
+    sin_cos(h, s, c);
+    Q4<T> r; r.x = s * axis.x; r.y = s * axis.y; r.z = s * axis.z; r.w = c;
     return r;
 }
 template <typename T> CC_HD Q4<T> qconj(Q4<T> a) { Q4<T> r; r.x = -a.x; r.y = -a.y; r.z = -a.z; r.w = a.w; return r; }

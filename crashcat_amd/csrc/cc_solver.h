@@ -281,6 +281,101 @@ __device__ inline bool key_less(const unsigned long long* K, int x, int y) {
 }
 
 template <typename T, int BS>
+__device__ inline void warm_start_levels(const Dev<T>& d, int so, int D, const int* LS, T ratio) {
+    for (int l = 0; l < D; l++) {
+        for (int x = LS[l] + threadIdx.x; x < LS[l + 1]; x += BS) {
+            int s = d.seq[so + d.sched[so + x]];
+            if (s >= 0x40000000) continue;
+            if (s >= 0) warm_start_contact(d, s, ratio);
+            else {
+                PointCon<T>& pc = d.pc[-1 - s];
+                if (pc.enabled && !pc.sleeping && pc.active) {
+                    pc.lambda = pc.lambda * ratio;
+                    point_apply(d, pc, pc.lambda);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// creation-order key of sequence element k: contacts in the order they were created (their
+// broadphase pair order), then point constraints by index; the warm start runs in this order
+// (src/update.ts:168-175 warm-starts the constraint arrays front to back)
+template <typename T>
+struct CreationLess {
+    const Dev<T>* d;
+    int so;
+    __device__ unsigned long long key(int k) const {
+        int s = d->seq[so + k];
+        if (s >= 0) return (unsigned long long)(unsigned)d->con[s].pairIdx;
+        return (1ull << 32) | (unsigned)(-1 - s);
+    }
+    __device__ bool operator()(int x, int y) const { return key(x) < key(y); }
+};
+struct SolveLess {
+    const unsigned long long* K;
+    __device__ bool operator()(int x, int y) const { return key_less(K, x, y); }
+};
+
+// each body's list of sequence elements in the given order -> predecessor / successor on that body
+template <typename T, int BS, typename Less>
+__device__ inline void link_body_lists(const Dev<T>& d, int b0, int nb, int so, int* BL, Less less) {
+    for (int i = threadIdx.x; i < nb; i += BS) {
+        int o = d.boff[b0 + i], c = d.bcnt[b0 + i];
+        for (int x = o + 1; x < o + c; x++) {  // insertion sort (lists are short)
+            int v = BL[x];
+            int y = x - 1;
+            while (y >= o && less(v, BL[y])) { BL[y + 1] = BL[y]; y--; }
+            BL[y + 1] = v;
+        }
+        for (int x = o; x < o + c; x++) {
+            int k = BL[x];
+            int pv = x > o ? BL[x - 1] : -1;
+            int nx = x + 1 < o + c ? BL[x + 1] : -1;
+            if (d.seqA[so + k] == b0 + i) { d.prevA[so + k] = pv; d.nextA[so + k] = nx; }
+            else { d.prevB[so + k] = pv; d.nextB[so + k] = nx; }
+        }
+    }
+    __syncthreads();
+}
+
+// Kahn's algorithm on the predecessor DAG: level l holds the elements whose longest predecessor
+// chain has length l, so elements of one level share no dynamic body; returns the level count
+template <typename T, int BS>
+__device__ inline int build_levels(const Dev<T>& d, int so, int L, int* LS, int& sTail) {
+    const int tid = threadIdx.x;
+    if (tid == 0) sTail = 0;
+    __syncthreads();
+    for (int k = tid; k < L; k += BS) {
+        int deg = (d.prevA[so + k] >= 0 ? 1 : 0) + (d.prevB[so + k] >= 0 ? 1 : 0);
+        d.level[so + k] = deg;
+        if (deg == 0) d.sched[so + atomicAdd(&sTail, 1)] = k;
+    }
+    __syncthreads();
+    int lo = 0, hi = sTail;
+    int D = 0;
+    __syncthreads();  // every lane has read the first frontier before any lane extends it
+    while (lo < hi) {
+        if (tid == 0) LS[D] = lo;
+        for (int x = lo + tid; x < hi; x += BS) {
+            int e = d.sched[so + x];
+            int na = d.nextA[so + e], nbx = d.nextB[so + e];
+            if (na >= 0 && atomicSub(&d.level[so + na], 1) == 1) d.sched[so + atomicAdd(&sTail, 1)] = na;
+            if (nbx >= 0 && atomicSub(&d.level[so + nbx], 1) == 1) d.sched[so + atomicAdd(&sTail, 1)] = nbx;
+        }
+        __syncthreads();
+        lo = hi;
+        hi = sTail;
+        D++;
+        __syncthreads();
+    }
+    if (tid == 0) LS[D] = lo;
+    __syncthreads();
+    return D;
+}
+
+template <typename T, int BS>
 __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __shared__ int sh[BS / 64 + 1];
     __shared__ int sTail;
@@ -354,79 +449,28 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         if (eb >= 0) BL[d.boff[eb] + atomicAdd(&d.bcnt[eb], 1)] = k;
     }
     __syncthreads();
-    // ---- 3. each body's list in solve order -> predecessor / successor on that body
-    for (int i = tid; i < nb; i += BS) {
-        int o = d.boff[b0 + i], c = d.bcnt[b0 + i];
-        for (int x = o + 1; x < o + c; x++) {  // insertion sort (lists are short)
-            int v = BL[x];
-            int y = x - 1;
-            while (y >= o && key_less(K, v, BL[y])) { BL[y + 1] = BL[y]; y--; }
-            BL[y + 1] = v;
-        }
-        for (int x = o; x < o + c; x++) {
-            int k = BL[x];
-            int pv = x > o ? BL[x - 1] : -1;
-            int nx = x + 1 < o + c ? BL[x + 1] : -1;
-            if (d.seqA[so + k] == b0 + i) { d.prevA[so + k] = pv; d.nextA[so + k] = nx; }
-            else { d.prevB[so + k] = pv; d.nextB[so + k] = nx; }
-        }
-    }
-    __syncthreads();
-    // ---- 4. levels: Kahn's algorithm on the predecessor DAG; level l holds the elements whose
-    // longest predecessor chain has length l, so elements of one level share no dynamic body
+    // ---- 3-5. warm start (contact-constraints.ts:906-955, constraints.ts:264-301).  The FP64
+    // path applies it in creation order, as the reference does, so that every body sums its
+    // warm-start impulses in the same order; the FP32 path uses the solve order, which
+    // changes only the rounding of those sums.
     int* LS = d.lvlStart + so + w;
-    if (tid == 0) sTail = 0;
-    __syncthreads();
-    for (int k = tid; k < L; k += BS) {
-        int deg = (d.prevA[so + k] >= 0 ? 1 : 0) + (d.prevB[so + k] >= 0 ? 1 : 0);
-        d.level[so + k] = deg;
-        if (deg == 0) d.sched[so + atomicAdd(&sTail, 1)] = k;
-    }
-    __syncthreads();
-    int lo = 0, hi = sTail;
-    int D = 0;
-    __syncthreads();  // every lane has read the first frontier before any lane extends it
-    while (lo < hi) {
-        if (tid == 0) LS[D] = lo;
-        for (int x = lo + tid; x < hi; x += BS) {
-            int e = d.sched[so + x];
-            int na = d.nextA[so + e], nbx = d.nextB[so + e];
-            if (na >= 0 && atomicSub(&d.level[so + na], 1) == 1) d.sched[so + atomicAdd(&sTail, 1)] = na;
-            if (nbx >= 0 && atomicSub(&d.level[so + nbx], 1) == 1) d.sched[so + atomicAdd(&sTail, 1)] = nbx;
-        }
-        __syncthreads();
-        lo = hi;
-        hi = sTail;
-        D++;
-        __syncthreads();
-    }
-    if (tid == 0) { LS[D] = lo; d.stats[2 * w] = D; }
-    __syncthreads();
-
-    // ---- 5. user-constraint setup + warm start (contact-constraints.ts:906-955, constraints.ts:264-301)
     for (int k = tid; k < u; k += BS) {
         PointCon<T>& pc = d.pc[u0 + k];
         pc.sleeping = B.sleeping[pc.a] && B.sleeping[pc.b];
         if (pc.enabled && !pc.sleeping) point_props(d, pc);
     }
     __syncthreads();
-    if (d.s.warmStarting) {
-        for (int l = 0; l < D; l++) {
-            for (int x = LS[l] + tid; x < LS[l + 1]; x += BS) {
-                int s = d.seq[so + d.sched[so + x]];
-                if (s >= 0x40000000) continue;
-                if (s >= 0) warm_start_contact(d, s, ratio);
-                else {
-                    PointCon<T>& pc = d.pc[-1 - s];
-                    if (pc.enabled && !pc.sleeping && pc.active) {
-                        pc.lambda = pc.lambda * ratio;
-                        point_apply(d, pc, pc.lambda);
-                    }
-                }
-            }
-            __syncthreads();
-        }
+    constexpr bool kCreationOrderWarmStart = sizeof(T) == 8;
+    int D = 0;
+    if (kCreationOrderWarmStart && d.s.warmStarting) {
+        link_body_lists<T, BS>(d, b0, nb, so, BL, CreationLess<T>{&d, so});
+        D = build_levels<T, BS>(d, so, L, LS, sTail);
+        warm_start_levels<T, BS>(d, so, D, LS, ratio);
     }
+    link_body_lists<T, BS>(d, b0, nb, so, BL, SolveLess{K});
+    D = build_levels<T, BS>(d, so, L, LS, sTail);
+    if (tid == 0) d.stats[2 * w] = D;
+    if (!kCreationOrderWarmStart && d.s.warmStarting) warm_start_levels<T, BS>(d, so, D, LS, ratio);
     // ---- 6. velocity iterations
     for (int it = 0; it < d.s.velIters; it++) {
         for (int l = 0; l < D; l++) {

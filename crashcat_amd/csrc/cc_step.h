@@ -70,6 +70,8 @@ struct Con {
     int a, b, np, dynMask;  // dynMask bit0: A dynamic, bit1: B dynamic
     unsigned sortKey;
     int movMask;            // bit0: A not static, bit1: B not static
+    int pairIdx;            // broadphase pair index: pairs are numbered in (query, other) order,
+                            // which is the order constraints are created in
     V3<T> normal, t1, t2;
     T friction, invMassA, invMassB;
     ConPoint<T> p[4];

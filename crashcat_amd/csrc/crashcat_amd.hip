@@ -106,6 +106,13 @@ __global__ void k_refresh_bodies(Dev<T> d) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.N) return;
     set_transform(d.B, d.st, i);
+    // d.B.wake holds "requested awake" here: wake a sleeping dynamic body at once, as
+    // wake (src/body/sleep.ts:188-203) does when a caller touches a sleeping body
+    if (d.B.wake[i] && d.B.sleeping[i] && fl_motion(d.B.flags[i]) == MOTION_DYNAMIC) {
+        d.B.sleeping[i] = 0;
+        reset_sleep_timer(d.B, d.st, i);
+    }
+    d.B.wake[i] = 0;
 }
 
 // ---------------------------------------------------------------- context
@@ -128,6 +135,7 @@ struct Alloc {
 };
 
 struct Impl {
+    int* counters = nullptr;  // device step counters (CNT_*), shared by both precisions
     virtual ~Impl() {}
 };
 
@@ -380,6 +388,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.pairA = A.get<int>(d.pairCap);
     d.pairB = A.get<int>(d.pairCap);
     d.counters = A.get<int>(CNT_N);
+    X->counters = d.counters;
     d.con = A.get<Con<T>>(d.conCap);
     d.tap = A.get<ConTap<T>>(d.conCap);
     d.wConCnt = A.get<int>(nw + 1);
@@ -485,6 +494,11 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
     hipStream_t st = c->stream;
     T dt = T(dtd);
     int N = d.N;
+    if (N == 0) {  // nothing to step; keep the step count for island reporting
+        c->steps += nsteps;
+        c->nPhase = 0;
+        return CC_OK;
+    }
     int nbB = (N + 255) / 256;
     const double margin = (double)d.s.spec;
     size_t needEv = (size_t)(CC_PHASES + 1) * nsteps;
@@ -546,6 +560,19 @@ static int counters_impl(cc_ctx* c, cc_step_counters* o);
 extern "C" int cc_sync(cc_ctx* c) {
     if (!c) return CC_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->impl && c->impl->counters) {
+        // a capacity overflow drops pairs, constraints or EPA jobs for the rest of that step,
+        // so the state is no longer the reference's: report it (it stays set until re-upload)
+        int ovf = 0;
+        HIPCHK(c, hipMemcpy(&ovf, c->impl->counters + CNT_OVERFLOW, sizeof(int), hipMemcpyDeviceToHost));
+        if (ovf) {
+            std::string m = "capacity exceeded:";
+            if (ovf & 1) m += " pairs (raise max_pairs_per_body)";
+            if (ovf & 2) m += " contact constraints (raise max_contacts_per_body)";
+            if (ovf & 4) m += " EPA jobs";
+            return fail(c, CC_E_CAPACITY, m);
+        }
+    }
     return CC_OK;
 }
 
@@ -612,6 +639,9 @@ static int set_bodies_impl(cc_ctx* c, const cc_body_state* in, int n) {
         HIPCHK(c, hipMemcpy(d.B.v, v.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(d.B.w, w.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(d.B.q, q.data(), n * sizeof(Q4<T>), hipMemcpyHostToDevice));
+        std::vector<int> awake(n);
+        for (int i = 0; i < n; i++) awake[i] = in[i].sleeping ? 0 : 1;
+        HIPCHK(c, hipMemcpy(d.B.wake, awake.data(), n * sizeof(int), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_refresh_bodies<T>, dim3((n + 255) / 256), dim3(256), 0, c->stream, d);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -805,6 +835,10 @@ extern "C" int cc_get_totals(cc_ctx* c, cc_totals* o) {
 template <typename T>
 static int bp_impl(cc_ctx* c, const float* mn, const float* mx, const int32_t* mo, const int32_t* sl, const int32_t* la,
                    int n, double margin, int32_t* out, int cap, int32_t* count) {
+    if (n == 0) {
+        *count = 0;
+        return CC_OK;
+    }
     // a one-world scene of unit spheres whose bounds are overwritten with the given AABBs
     cc_settings s;
     memset(&s, 0, sizeof(s));

@@ -161,6 +161,20 @@ class Scene:
             self._bodies = arr
         return self._bodies
 
+    def world(self, w):
+        """World w as a one-world scene (same shapes and settings, constraints renumbered)."""
+        offs = self.offsets()
+        b0, b1 = offs[w], offs[w + 1]
+        out = Scene(self.settings)
+        out.layer_moving, out.layer_static = self.layer_moving, self.layer_static
+        out.shapes, out._shape_index = self.shapes, self._shape_index
+        out._bodies = self.bodies()[b0:b1].copy()
+        out.rows = [None] * (b1 - b0)
+        out.world_offsets = [0]
+        out.point_constraints = [(a - b0, b - b0, la, lb) for (a, b, la, lb) in self.point_constraints
+                                 if b0 <= a < b1]
+        return out
+
     def c_bodies(self):
         b = self.bodies()
         return b.ctypes.data_as(C.POINTER(_abi.BodyDesc)), b

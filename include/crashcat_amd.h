@@ -247,7 +247,10 @@ int cc_sync(cc_ctx* ctx);
 
 /* Read back body state (all worlds, global body order). */
 int cc_download_bodies(cc_ctx* ctx, cc_body_state* out, int32_t num_bodies);
-/* Overwrite body poses/velocities (all worlds), e.g. to restart from an oracle state. */
+/* Overwrite body poses/velocities (all worlds), e.g. to restart from an oracle state.  A
+ * sleeping dynamic body whose `sleeping` field is 0 here is woken at once (sleep timer reset,
+ * back in the active set), as wake (src/body/sleep.ts:188-203) does after addImpulse
+ * (src/body/rigid-body.ts:887-892); `sleeping` = 1 never puts an awake body to sleep. */
 int cc_set_bodies(cc_ctx* ctx, const cc_body_state* in, int32_t num_bodies);
 
 /* Parity taps of the last step. */

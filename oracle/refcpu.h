@@ -57,6 +57,10 @@ typedef struct oc_hit {
 void oc_collide(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t sa, const double* pa, const double* qa,
                 int32_t sb, const double* pb, const double* qb, double max_separation, const cc_settings* s,
                 oc_hit* out);
+/* the convex GJK/EPA path for any pair (spheres included) */
+void oc_collide_convex(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t sa, const double* pa,
+                       const double* qa, int32_t sb, const double* pb, const double* qb, double max_separation,
+                       const cc_settings* s, oc_hit* out);
 /* manifold of one hit, pruned to <= 4 points (update.ts:385-563 for a single hit);
  * out_a/out_b are world points; returns the point count */
 int32_t oc_manifold_from_hit(const oc_hit* hit, double max_contact_distance, double* normal_out,
@@ -66,6 +70,7 @@ int32_t oc_clip_poly_vs_poly(const double* poly, int32_t n, const double* clip, 
                              double* out);
 int32_t oc_clip_poly_vs_edge(const double* poly, int32_t n, const double* e1, const double* e2, const double* normal,
                              double* out);
+int32_t oc_clip_poly_vs_plane(const double* poly, int32_t n, const double* origin, const double* normal, double* out);
 int32_t oc_manifold_between_two_faces(const double* p1, const double* p2, const double* axis, double max_dist,
                                       const double* face1, int32_t n1, const double* face2, int32_t n2,
                                       double* out_a, double* out_b);

@@ -44,6 +44,26 @@ def lib():
         L.oc_broadphase_aabbs.restype = C.c_int32
         L.oc_broadphase_aabbs.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), ip, ip, ip, C.c_int32,
                                           C.c_double, C.POINTER(_abi.Settings), ip, C.c_int32]
+        vp = C.c_void_p
+        for fn in ("oc_collide", "oc_collide_convex"):
+            getattr(L, fn).argtypes = [vp, vp, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double, vp, vp]
+        L.oc_manifold_from_hit.restype = C.c_int32
+        L.oc_manifold_from_hit.argtypes = [vp, C.c_double, dp, dp, dp]
+        L.oc_clip_poly_vs_plane.restype = C.c_int32
+        L.oc_clip_poly_vs_plane.argtypes = [dp, C.c_int32, dp, dp, dp]
+        L.oc_clip_poly_vs_poly.restype = C.c_int32
+        L.oc_clip_poly_vs_poly.argtypes = [dp, C.c_int32, dp, C.c_int32, dp, dp]
+        L.oc_clip_poly_vs_edge.restype = C.c_int32
+        L.oc_clip_poly_vs_edge.argtypes = [dp, C.c_int32, dp, dp, dp, dp]
+        L.oc_manifold_between_two_faces.restype = C.c_int32
+        L.oc_manifold_between_two_faces.argtypes = [dp, dp, dp, C.c_double, dp, C.c_int32, dp, C.c_int32, dp, dp]
+        L.oc_prune.restype = C.c_int32
+        L.oc_prune.argtypes = [dp, dp, C.c_int32, dp]
+        L.oc_barycentric2.argtypes = [dp, dp, C.c_double, dp]
+        L.oc_barycentric3.argtypes = [dp, dp, dp, C.c_double, dp]
+        L.oc_inverse_inertia.argtypes = [dp, dp, dp, C.c_int32, dp]
+        L.oc_supporting_face.restype = C.c_int32
+        L.oc_supporting_face.argtypes = [vp, vp, C.c_int32, dp, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -129,3 +149,138 @@ def broadphase_aabbs(mn, mx, motion, sleeping, layer, margin, settings):
                           sl.ctypes.data_as(ip), la.ctypes.data_as(ip), n, margin, C.byref(settings),
                           out.ctypes.data_as(ip), cnt)
     return out[:cnt]
+
+
+# ---------------------------------------------------------------- unit entry points
+
+class Hit(C.Structure):
+    _fields_ = [
+        ("hit", C.c_int32),
+        ("status", C.c_int32),
+        ("point_a", C.c_double * 3),
+        ("point_b", C.c_double * 3),
+        ("axis", C.c_double * 3),
+        ("penetration", C.c_double),
+        ("face_a_n", C.c_int32),
+        ("face_b_n", C.c_int32),
+        ("face_a", C.c_double * 96),
+        ("face_b", C.c_double * 96),
+    ]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _vec(v, n=3):
+    a = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+    assert a.size == n
+    return a
+
+
+def _poly(v):
+    a = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    return a, len(a)
+
+
+def collide(scene, sa, pa, qa, sb, pb, qb, max_separation=0.0, convex_path=False):
+    """collideShapeVsShape for two shapes of `scene` at world poses.  Returns a dict or None."""
+    L = lib()
+    shapes, hulls, keep = scene.c_shapes()
+    h = Hit()
+    fn = L.oc_collide_convex if convex_path else L.oc_collide
+    pa, qa, pb, qb = _vec(pa), _vec(qa, 4), _vec(pb), _vec(qb, 4)
+    fn(shapes, C.byref(hulls), sa, _dp(pa), _dp(qa), sb, _dp(pb), _dp(qb), C.c_double(max_separation),
+       C.byref(scene.settings), C.byref(h))
+    if not h.hit:
+        return None
+    return {
+        "hit": h, "status": h.status, "point_a": np.array(h.point_a[:]), "point_b": np.array(h.point_b[:]),
+        "axis": np.array(h.axis[:]), "penetration": h.penetration,
+        "face_a": np.array(h.face_a[:3 * h.face_a_n]).reshape(-1, 3),
+        "face_b": np.array(h.face_b[:3 * h.face_b_n]).reshape(-1, 3),
+    }
+
+
+def manifold_from_hit(hit, max_contact_distance):
+    L = lib()
+    nrm = np.zeros(3)
+    oa = np.zeros((64, 3))
+    ob = np.zeros((64, 3))
+    n = L.oc_manifold_from_hit(C.byref(hit["hit"]), C.c_double(max_contact_distance), _dp(nrm), _dp(oa), _dp(ob))
+    return nrm, oa[:n], ob[:n]
+
+
+def clip_poly_vs_plane(poly, origin, normal):
+    p, n = _poly(poly)
+    o, nn = _vec(origin), _vec(normal)
+    out = np.zeros((64, 3))
+    k = lib().oc_clip_poly_vs_plane(_dp(p), n, _dp(o), _dp(nn), _dp(out))
+    return out[:k]
+
+
+def clip_poly_vs_poly(poly, clip, normal):
+    p, n = _poly(poly)
+    c, m = _poly(clip)
+    nn = _vec(normal)
+    out = np.zeros((64, 3))
+    k = lib().oc_clip_poly_vs_poly(_dp(p), n, _dp(c), m, _dp(nn), _dp(out))
+    return out[:k]
+
+
+def clip_poly_vs_edge(poly, e1, e2, normal):
+    p, n = _poly(poly)
+    a, b, nn = _vec(e1), _vec(e2), _vec(normal)
+    out = np.zeros((64, 3))
+    k = lib().oc_clip_poly_vs_edge(_dp(p), n, _dp(a), _dp(b), _dp(nn), _dp(out))
+    return out[:k]
+
+
+def manifold_between_two_faces(p1, p2, axis, max_dist, face1, face2):
+    a, b, ax = _vec(p1), _vec(p2), _vec(axis)
+    f1, n1 = _poly(face1) if len(face1) else (np.zeros((1, 3)), 0)
+    f2, n2 = _poly(face2) if len(face2) else (np.zeros((1, 3)), 0)
+    oa = np.zeros((64, 3))
+    ob = np.zeros((64, 3))
+    k = lib().oc_manifold_between_two_faces(_dp(a), _dp(b), _dp(ax), C.c_double(max_dist), _dp(f1), n1, _dp(f2),
+                                            n2, _dp(oa), _dp(ob))
+    return oa[:k], ob[:k]
+
+
+def prune(rel_a, rel_b, axis):
+    a = np.ascontiguousarray(rel_a, dtype=np.float64).reshape(-1, 3).copy()
+    b = np.ascontiguousarray(rel_b, dtype=np.float64).reshape(-1, 3).copy()
+    assert len(a) > 4, "pruning needs more than four points"
+    ax = _vec(axis)
+    k = lib().oc_prune(_dp(a), _dp(b), len(a), _dp(ax))
+    return a[:k], b[:k]
+
+
+def barycentric2(a, b, tol=1e-12):
+    r = np.zeros(4)
+    A, B = _vec(a), _vec(b)
+    lib().oc_barycentric2(_dp(A), _dp(B), C.c_double(tol), _dp(r))
+    return r[0], r[1], bool(r[3])
+
+
+def barycentric3(a, b, c, tol=1e-12):
+    r = np.zeros(4)
+    A, B, Cc = _vec(a), _vec(b), _vec(c)
+    lib().oc_barycentric3(_dp(A), _dp(B), _dp(Cc), C.c_double(tol), _dp(r))
+    return r[0], r[1], r[2], bool(r[3])
+
+
+def inverse_inertia(inv_diag, inertia_rot, body_quat, dofs=0b111111):
+    """World-space inverse inertia as a 3x3 matrix (row, column)."""
+    d, iq, bq = _vec(inv_diag), _vec(inertia_rot, 4), _vec(body_quat, 4)
+    out = np.zeros(9)
+    lib().oc_inverse_inertia(_dp(d), _dp(iq), _dp(bq), dofs, _dp(out))
+    return out.reshape(3, 3).T  # stored column-major
+
+
+def supporting_face(scene, s, direction, pos=(0, 0, 0), quat=(0, 0, 0, 1)):
+    shapes, hulls, keep = scene.c_shapes()
+    d, p, q = _vec(direction), _vec(pos), _vec(quat, 4)
+    out = np.zeros((64, 3))
+    k = lib().oc_supporting_face(shapes, C.byref(hulls), s, _dp(d), _dp(p), _dp(q), _dp(out))
+    return out[:k]

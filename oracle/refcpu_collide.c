@@ -1202,6 +1202,32 @@ void oc_collide(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t 
     for (int i = 0; i < h.faceB.n; i++) vstore(out->face_b + 3 * i, h.faceB.v[i]);
 }
 
+/* the convex GJK/EPA path for any pair, spheres included (a sphere wrapped in a transformed
+ * shape takes this path in src/collision/narrowphase.ts:172-238) */
+void oc_collide_convex(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t sa, const double* pa,
+                       const double* qa, int32_t sb, const double* pb, const double* qb, double maxSep,
+                       const cc_settings* s, oc_hit* out) {
+    OShapes sh = {shapes, hulls};
+    OHit h;
+    Q4 QA = {qa[0], qa[1], qa[2], qa[3]}, QB = {qb[0], qb[1], qb[2], qb[3]};
+    h.hit = 0;
+    h.faceA.n = h.faceB.n = 0;
+    collide_convex_convex(&sh, sa, vload(pa), QA, sb, vload(pb), QB, maxSep, s->collision_tolerance,
+                          s->penetration_tolerance, &h);
+    memset(out, 0, sizeof(*out));
+    out->hit = h.hit;
+    out->status = h.hit ? h.status : -1;
+    if (!h.hit) return;
+    vstore(out->point_a, h.pointA);
+    vstore(out->point_b, h.pointB);
+    vstore(out->axis, h.axis);
+    out->penetration = h.penetration;
+    out->face_a_n = h.faceA.n;
+    out->face_b_n = h.faceB.n;
+    for (int i = 0; i < h.faceA.n; i++) vstore(out->face_a + 3 * i, h.faceA.v[i]);
+    for (int i = 0; i < h.faceB.n; i++) vstore(out->face_b + 3 * i, h.faceB.v[i]);
+}
+
 int32_t oc_manifold_from_hit(const oc_hit* hit, double maxDist, double* normal_out, double* out_a, double* out_b) {
     OManifold m;
     m.base = v3(0, 0, 0);
@@ -1223,6 +1249,13 @@ int32_t oc_clip_poly_vs_poly(const double* poly, int32_t n, const double* clip, 
                              double* out) {
     OFace a = face_from(poly, n), b = face_from(clip, m), o;
     clip_poly_vs_poly(&o, &a, &b, vload(normal));
+    for (int i = 0; i < o.n; i++) vstore(out + 3 * i, o.v[i]);
+    return o.n;
+}
+
+int32_t oc_clip_poly_vs_plane(const double* poly, int32_t n, const double* origin, const double* normal, double* out) {
+    OFace a = face_from(poly, n), o;
+    clip_poly_vs_plane(&o, &a, vload(origin), vload(normal));
     for (int i = 0; i < o.n; i++) vstore(out + 3 * i, o.v[i]);
     return o.n;
 }

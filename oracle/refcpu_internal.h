@@ -71,9 +71,53 @@ static inline Q4 qnormalize(Q4 a) {
     if (l > 0) { l = 1.0 / sqrt(l); a.x *= l; a.y *= l; a.z *= l; a.w *= l; }
     return a;
 }
+/* sine/cosine: the fdlibm method (Sun Microsystems, 1993; k_sin.c, k_cos.c, and the
+ * two-part Cody-Waite reduction of e_rem_pio2.c), which JavaScript's Math.sin/Math.cos
+ * descend from; the reference calls them through quat.setAxisAngle (mathcat).  The
+ * polynomial coefficients are fdlibm's published minimax constants. */
+static inline double o_ksin(double x, double y, int iy) {
+    const double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03,
+                 S3 = -1.98412698298579493134e-04, S4 = 2.75573137070700676789e-06,
+                 S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
+    double z = x * x, v = z * x, r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+    if (iy == 0) return x + v * (S1 + z * r);
+    return x - ((z * (0.5 * y - v * r) - y) - v * S1);
+}
+static inline double o_kcos(double x, double y) {
+    const double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03,
+                 C3 = 2.48015872894767294178e-05, C4 = -2.75573143513906633035e-07,
+                 C5 = 2.08757232129817482790e-09, C6 = -1.13596475577881948265e-11;
+    uint64_t u;
+    memcpy(&u, &x, 8);
+    uint32_t ix = (uint32_t)(u >> 32) & 0x7fffffffu;
+    double z = x * x, r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+    if (ix < 0x3FD33333u) return 1.0 - (0.5 * z - (z * r - x * y));
+    double qx;
+    if (ix > 0x3fe90000u) qx = 0.28125;
+    else { uint64_t q = (uint64_t)(ix - 0x00200000u) << 32; memcpy(&qx, &q, 8); }
+    double hz = 0.5 * z - qx, a = 1.0 - qx;
+    return a - (hz - (z * r - x * y));
+}
+static inline void o_sincos(double x, double* s, double* c) {
+    const double pio4 = 7.85398163397448278999e-01;
+    if (fabs(x) <= pio4) { *s = o_ksin(x, 0.0, 0); *c = o_kcos(x, 0.0); return; }
+    const double invpio2 = 6.36619772367581382433e-01, pio2_1 = 1.57079632673412561417e+00,
+                 pio2_1t = 6.07710050650619224932e-11;
+    double fn = (double)(long long)(x * invpio2 + (x < 0 ? -0.5 : 0.5));
+    long long n = (long long)fn;
+    double r = x - fn * pio2_1, w = fn * pio2_1t, y0 = r - w, y1 = (r - y0) - w;
+    double ks = o_ksin(y0, y1, 1), kc = o_kcos(y0, y1);
+    switch (n & 3) {
+        case 0: *s = ks; *c = kc; break;
+        case 1: *s = kc; *c = -ks; break;
+        case 2: *s = -ks; *c = -kc; break;
+        default: *s = -kc; *c = ks; break;
+    }
+}
 static inline Q4 qaxis_angle(V3 axis, double rad) {
-    double h = rad * 0.5, s = sin(h);
-    Q4 r = {s * axis.x, s * axis.y, s * axis.z, cos(h)};
+    double h = rad * 0.5, s, c;
+    o_sincos(h, &s, &c);
+    Q4 r = {s * axis.x, s * axis.y, s * axis.z, c};
     return r;
 }
 

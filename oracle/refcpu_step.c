@@ -1160,6 +1160,7 @@ void oc_set_bodies(oc_world* w, const cc_body_state* in) {
         b->v = vload(in[i].linear_velocity);
         b->w = vload(in[i].angular_velocity);
         set_transform(w, b);
+        if (!in[i].sleeping && b->sleeping && b->motion == CC_MOTION_DYNAMIC) body_wake(w, b, i);
     }
 }
 
