@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--bodies", type=int, default=20000)
     ap.add_argument("--block", type=int, default=0, help="solver workgroup size (0: by world size)")
     ap.add_argument("--pairs-per-body", type=int, default=24, help="pair buffer capacity per body")
+    ap.add_argument("--contacts-per-body", type=int, default=6, help="contact constraint capacity per body")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-accuracy", action="store_true")
@@ -72,7 +73,8 @@ def main():
     nbody = args.bodies * args.worlds
 
     if args.profile_steps:
-        ctx = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block)
+        ctx = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
+                      max_contacts_per_body=args.contacts_per_body)
         ctx.upload(batch)
         ctx.step(DT, args.profile_steps)
         ctx.sync()
@@ -80,7 +82,8 @@ def main():
         return
 
     # ---- FP32 batched
-    ctx = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block)
+    ctx = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
+                      max_contacts_per_body=args.contacts_per_body)
     ctx.upload(batch)
     log(f"fp32 batch uploaded: {args.worlds} worlds, {nbody} dynamic bodies")
     for w0 in range(0, args.warmup, 10):
@@ -108,7 +111,8 @@ def main():
     fp64 = None
     if not args.skip_fp64:
         one = S.replicate_world(base, 1, 1002)
-        c64 = Context("fp64", max_pairs_per_body=args.pairs_per_body, block_solver=args.block)
+        c64 = Context("fp64", max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
+                      max_contacts_per_body=args.contacts_per_body)
         c64.upload(one)
         w64 = max(1, min(args.warmup, 20))
         c64.step(DT, w64)
@@ -129,7 +133,8 @@ def main():
         from oracle.refcpu import OracleWorld
         one = S.replicate_world(base, 1, 1002)
         orc = OracleWorld(one)
-        g = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block)
+        g = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
+                      max_contacts_per_body=args.contacts_per_body)
         g.upload(one)
         t = time.perf_counter()
         orc.step(DT)

@@ -15,19 +15,40 @@ struct Dev {
     int W;              // worlds
     const int* wStart;  // [W+1] body ranges
     int pairCap, conCap, epaCap, epaWorkers, hashCap;
-    // broadphase
-    int* bpCount;       // [N+1]
-    int* bpOff;         // [N+1] exclusive scan
+    // broadphase: uniform grid per world, occupied cells in an open-addressing hash table
+    double cellWidth;   // widest box a body may have to be binned (wider ones are "large")
+    double cellSize, cellInv;  // cellWidth + 2 * margin, so overlapping small bodies are in adjacent cells
+    int cellTabCap;     // power of two, at least 2 * N
+    unsigned long long* cellKey;  // [cellTabCap] cell key per table slot (CC_EMPTY_KEY if free)
+    int* cellCount;     // [cellTabCap + 1] bodies per slot
+    int* cellStart;     // [cellTabCap + 1] exclusive scan of cellCount
+    int* cellFill;      // [cellTabCap]
+    int* cellList;      // [N] occupied slots (CNT_CELLS of them)
+    int* bodySlot;      // [N] table slot of a small body, -1 for a large one
+    int* cellBody;      // [N] small bodies grouped by cell, with their bounds
+    V3<T>* cellMin;
+    V3<T>* cellMax;
+    int* gLargeFlag;    // [N + 1]
+    int* gLargeOff;     // [N + 1] exclusive scan of gLargeFlag (bodies are world-ordered)
+    int* gLarge;        // [N] large bodies, grouped by world
+    int* bpCount;       // [N+1] pairs found by each querying body
+    int* bpOff;         // [N+1] exclusive scan of bpCount: pairs are stored in body order
+    int* bpSlots;       // [N * BP_SLOTS] first BP_SLOTS partners of each querying body
+    int* spillA;        // [spillCap] (body, partner, rank within the body) beyond a body's BP_SLOTS
+    int* spillB;
+    int* spillK;
+    int spillCap;
+    int* pairCon;       // [pairCap] constraint slot made from each pair, -1 if none
+    int* conFlag;       // [pairCap + 1]
+    int* conPos;        // [pairCap + 1] exclusive scan of conFlag
     int* pairA;         // [pairCap]
     int* pairB;
     int* counters;      // device counters, see CNT_*
     // contact constraints (slots allocated in creation order by an atomic counter)
     Con<T>* con;        // [conCap]
     ConTap<T>* tap;     // [conCap]
-    int* wConCnt;       // [W + 1] per-world constraint counts
-    int* wConStart;     // [W + 1] exclusive scan of wConCnt
-    int* wFill;         // [W]
-    int* conIdx;        // [conCap] constraint slots grouped by world
+    int* wConStart;     // [W + 1] range of each world in conIdx
+    int* conIdx;        // [conCap] constraint slots in pair order (so grouped by world)
     CacheRec<T>* rec[2];          // [conCap] per parity
     unsigned long long* hkeys[2]; // [hashCap]
     int* hvals[2];
@@ -54,6 +75,22 @@ struct Dev {
     int* blist;         // [2 * (conCap + NPC)]
     int* bcnt;          // [N]
     int* boff;          // [N]
+    // the world's contact rows copied in level order for the sweeps (element so + x; point j of
+    // element e at j * svCap + e), so a level's rows are read with coalesced loads
+    int svCap;          // conCap + NPC
+    int* svSlot;        // constraint slot (or -1 - point constraint, or 0x40000000 inactive)
+    int* svA;
+    int* svB;
+    int* svMeta;        // dynMask | movMask << 2 | np << 4 | dofA << 8 | dofB << 14
+    T* svIMA;
+    T* svIMB;
+    T* svFr;
+    V3<T>* svN;
+    V3<T>* svT1;
+    V3<T>* svT2;
+    RowSoA<T> rN, rT1, rT2;  // [4 * svCap]
+    V3<T>* svLA;        // [4 * svCap] contact points in body space
+    V3<T>* svLB;
     int* isl0;          // [N] island flags (by label)
     int* isl1;          // [N]
     int* stats;         // per world: [levels, posSweeps]
@@ -67,7 +104,29 @@ struct Dev {
 enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
        ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_N = 16 };
 
-enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_N = 8 };
+enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
+       CNT_N = 8 };
+// CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list
+constexpr int BP_SLOTS = 32;
+
+// orders LDS accesses between the lanes of one wavefront
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One atomic per wave for "give every active lane a slot": lanes are ranked by lane id.
+__device__ inline int wave_alloc(int* counter) {
+    unsigned long long act = __ballot(1);
+    int lane = (int)__lane_id();
+    int leader = __ffsll((long long)act) - 1;
+    int rank = __popcll(act & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(act));
+    base = __shfl(base, leader, 64);
+    return base + rank;
+}
 
 // ------------------------------------------------------------ acceleration
 // accelerationIntegrationUpdate src/update.ts:245-319 (external forces are zero on this path)
@@ -129,51 +188,206 @@ __device__ inline bool aabb_overlap(const Dev<T>& d, int a, int b, double m) {
     return !(bmx.x < amn.x || bmn.x > amx.x || bmx.y < amn.y || bmn.y > amx.y || bmx.z < amn.z || bmn.z > amx.z);
 }
 
-// One querying body per lane; the world's bodies are streamed through LDS tiles.
-template <typename T, bool WRITE>
-__global__ void k_bp(Dev<T> d, double margin) {
-    __shared__ V3<T> smin[256], smax[256];
-    __shared__ int sflag[256];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = i < d.N && queries(d.B, i);
-    // all lanes of a block belong to consecutive bodies; the block covers the union of their worlds
-    int iFirst = blockIdx.x * blockDim.x;
-    int iLast = min(d.N - 1, iFirst + (int)blockDim.x - 1);
-    int w0 = d.B.world[iFirst], w1 = d.B.world[iLast];
-    int jb = d.wStart[w0], je = d.wStart[w1 + 1];
-    int myW = i < d.N ? d.B.world[i] : -1;
-    int jw0 = myW >= 0 ? d.wStart[myW] : 0, jw1 = myW >= 0 ? d.wStart[myW + 1] : 0;
-    V3<double> amn, amx;
-    if (active) {
-        amn = cvt<double>(d.B.amin[i]) - mk3<double>(margin, margin, margin);
-        amx = cvt<double>(d.B.amax[i]) + mk3<double>(margin, margin, margin);
+// The pair set does not depend on how candidates are found (the reference walks one dynamic
+// AABB tree per broadphase layer; the set is the same for any tree).  Here candidates come
+// from a uniform grid per world.  A body whose box is at most cellWidth wide on every axis is
+// binned by the cell of its box centre; cells are cellWidth + 2 * margin wide, so two small
+// bodies whose boxes overlap after growing one by the margin sit in the same or adjacent
+// cells.  Wider bodies (floors, big static shapes) go to a per-world list.  Every candidate
+// pair is then decided by the exact FP64 predicate (aabb test + pair_ok).
+__device__ inline int grid_coord(double x, double inv) {
+    double f = floor(x * inv);
+    f = fmin(fmax(f, -8190.0), 8190.0);  // 14-bit cell coordinates; far cells merge, which is safe
+    return (int)f;
+}
+__device__ inline unsigned long long cell_key(int w, int cx, int cy, int cz) {
+    return ((unsigned long long)(unsigned)w << 42) | ((unsigned long long)(unsigned)(cx + 8192) << 28) |
+           ((unsigned long long)(unsigned)(cy + 8192) << 14) | (unsigned long long)(unsigned)(cz + 8192);
+}
+template <typename T>
+__device__ inline int cell_find(const Dev<T>& d, unsigned long long key) {
+    unsigned mask = (unsigned)(d.cellTabCap - 1);
+    unsigned h = hash64(key) & mask;
+    for (int probe = 0; probe < d.cellTabCap; probe++) {
+        unsigned long long k = d.cellKey[h];
+        if (k == key) return (int)h;
+        if (k == CC_EMPTY_KEY) return -1;
+        h = (h + 1) & mask;
     }
-    int cnt = 0;
-    int out = 0;
-    if (WRITE && active) out = d.bpOff[i];
-    for (int t0 = jb; t0 < je; t0 += blockDim.x) {
-        int j = t0 + threadIdx.x;
-        __syncthreads();
-        if (j < je) { smin[threadIdx.x] = d.B.amin[j]; smax[threadIdx.x] = d.B.amax[j]; sflag[threadIdx.x] = 1; }
-        __syncthreads();
-        if (active) {
-            int tn = min((int)blockDim.x, je - t0);
-            for (int k = 0; k < tn; k++) {
-                int jj = t0 + k;
-                if (jj < jw0 || jj >= jw1) continue;
-                V3<double> bmn = cvt<double>(smin[k]), bmx = cvt<double>(smax[k]);
-                if (bmx.x < amn.x || bmn.x > amx.x || bmx.y < amn.y || bmn.y > amx.y || bmx.z < amn.z || bmn.z > amx.z) continue;
-                if (!pair_ok(d, i, jj)) continue;
-                if (WRITE) {
-                    if (out < d.pairCap) { d.pairA[out] = i; d.pairB[out] = jj; }
-                    out++;
-                } else {
-                    cnt++;
+    return -1;
+}
+
+template <typename T>
+__global__ void k_cells_clear(Dev<T> d) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) { d.counters[CNT_SPILL] = 0; d.counters[CNT_CELLS] = 0; d.cellCount[d.cellTabCap] = 0; }
+    for (int h = t; h < d.cellTabCap; h += gridDim.x * blockDim.x) {
+        d.cellKey[h] = CC_EMPTY_KEY;
+        d.cellCount[h] = 0;
+        d.cellFill[h] = 0;
+    }
+    for (int i = t; i < d.N; i += gridDim.x * blockDim.x) d.bpCount[i] = 0;
+}
+
+template <typename T>
+__global__ void k_cells_insert(Dev<T> d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N) return;
+    V3<double> mn = cvt<double>(d.B.amin[i]), mx = cvt<double>(d.B.amax[i]);
+    double ex = fmax(mx.x - mn.x, fmax(mx.y - mn.y, mx.z - mn.z));
+    bool large = !(ex <= d.cellWidth);
+    d.gLargeFlag[i] = large ? 1 : 0;
+    if (large) { d.bodySlot[i] = -1; return; }
+    unsigned long long key = cell_key(d.B.world[i], grid_coord(0.5 * (mn.x + mx.x), d.cellInv),
+                                      grid_coord(0.5 * (mn.y + mx.y), d.cellInv), grid_coord(0.5 * (mn.z + mx.z), d.cellInv));
+    unsigned mask = (unsigned)(d.cellTabCap - 1);
+    unsigned h = hash64(key) & mask;
+    for (;;) {  // the table has at least twice as many slots as bodies, so a slot is always found
+        unsigned long long prev = atomicCAS(&d.cellKey[h], CC_EMPTY_KEY, key);
+        if (prev == CC_EMPTY_KEY || prev == key) break;
+        h = (h + 1) & mask;
+    }
+    d.bodySlot[i] = (int)h;
+    atomicAdd(&d.cellCount[h], 1);
+}
+
+template <typename T>
+__global__ void k_cells_scatter(Dev<T> d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N) return;
+    int h = d.bodySlot[i];
+    if (h < 0) { d.gLarge[d.gLargeOff[i]] = i; return; }
+    int k = atomicAdd(&d.cellFill[h], 1);
+    int p = d.cellStart[h] + k;
+    d.cellBody[p] = i;
+    d.cellMin[p] = d.B.amin[i];
+    d.cellMax[p] = d.B.amax[i];
+    if (k == 0) d.cellList[atomicAdd(&d.counters[CNT_CELLS], 1)] = h;
+}
+
+// record pair (query i, other j): the first BP_SLOTS of a body in its own slots, the rest in
+// the spill list (k_bp_compact lays them out in body order)
+template <typename T>
+__device__ inline void bp_emit(const Dev<T>& d, int i, int j) {
+    int k = atomicAdd(&d.bpCount[i], 1);
+    if (k < BP_SLOTS) {
+        d.bpSlots[(size_t)i * BP_SLOTS + k] = j;
+    } else {
+        int sp = atomicAdd(&d.counters[CNT_SPILL], 1);
+        if (sp < d.spillCap) { d.spillA[sp] = i; d.spillB[sp] = j; d.spillK[sp] = k; }
+        else atomicOr(&d.counters[CNT_OVERFLOW], 8);
+    }
+}
+
+template <typename T>
+__device__ inline bool box_hit(V3<double> amn, V3<double> amx, V3<double> bmn, V3<double> bmx) {
+    return !(bmx.x < amn.x || bmn.x > amx.x || bmx.y < amn.y || bmn.y > amx.y || bmx.z < amn.z || bmn.z > amx.z);
+}
+
+// Small-vs-small pairs, one wavefront per occupied cell: the bodies of the 27 cells around it
+// are staged through LDS in chunks and every querying body of the cell tests every staged one.
+constexpr int BP_CHUNK = 256;
+template <typename T>
+__global__ void __launch_bounds__(256) k_bp_cells(Dev<T> d, double margin) {
+    __shared__ V3<T> smn[4][BP_CHUNK], smx[4][BP_CHUNK];
+    __shared__ int sbody[4][BP_CHUNK];
+    __shared__ int nStart[4][28], nPre[4][28];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ncells = d.counters[CNT_CELLS];
+    for (int ci = blockIdx.x * 4 + wv; ci < ncells; ci += gridDim.x * 4) {
+        const int h = d.cellList[ci];
+        const int h0 = d.cellStart[h], nh = d.cellCount[h];
+        const int w = d.B.world[d.cellBody[h0]];
+        // the home cell's coordinates, from its first body's centre
+        V3<double> c0 = (cvt<double>(d.cellMin[h0]) + cvt<double>(d.cellMax[h0])) * 0.5;
+        int hx = grid_coord(c0.x, d.cellInv), hy = grid_coord(c0.y, d.cellInv), hz = grid_coord(c0.z, d.cellInv);
+        int cnt = 0, st = 0;
+        if (lane < 27) {
+            int nx = hx + lane % 3 - 1, ny = hy + (lane / 3) % 3 - 1, nz = hz + lane / 9 - 1;
+            int s = cell_find(d, cell_key(w, nx, ny, nz));
+            if (s >= 0) { cnt = d.cellCount[s]; st = d.cellStart[s]; }
+        }
+        // inclusive prefix of the 27 counts across the wave
+        int pre = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += v;
+        }
+        if (lane < 27) { nStart[wv][lane] = st; nPre[wv][lane + 1] = pre; }
+        if (lane == 0) nPre[wv][0] = 0;
+        const int total = __shfl(pre, 26, 64);
+        wave_sync();
+        for (int cb = 0; cb < total; cb += BP_CHUNK) {
+            const int nc = min(BP_CHUNK, total - cb);
+            for (int x = lane; x < nc; x += 64) {
+                int f = cb + x;
+                int lo = 0, hi = 26;  // the neighbour cell holding flat index f
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (nPre[wv][mid] <= f) lo = mid; else hi = mid - 1;
+                }
+                int p = nStart[wv][lo] + (f - nPre[wv][lo]);
+                sbody[wv][x] = d.cellBody[p];
+                smn[wv][x] = d.cellMin[p];
+                smx[wv][x] = d.cellMax[p];
+            }
+            wave_sync();
+            for (int hb = 0; hb < nh; hb++) {
+                const int i = d.cellBody[h0 + hb];
+                if (!queries(d.B, i)) continue;
+                const V3<double> amn = cvt<double>(d.cellMin[h0 + hb]) - mk3<double>(margin, margin, margin);
+                const V3<double> amx = cvt<double>(d.cellMax[h0 + hb]) + mk3<double>(margin, margin, margin);
+                for (int x = lane; x < nc; x += 64) {
+                    if (!box_hit<T>(amn, amx, cvt<double>(smn[wv][x]), cvt<double>(smx[wv][x]))) continue;
+                    int j = sbody[wv][x];
+                    if (pair_ok(d, i, j)) bp_emit(d, i, j);
                 }
             }
+            wave_sync();
         }
     }
-    if (!WRITE && i < d.N) d.bpCount[i] = cnt;
+}
+
+// Pairs with a large body: every querying body tests its world's large bodies; a large
+// querying body also tests the world's small bodies directly.
+template <typename T>
+__global__ void k_bp_large(Dev<T> d, double margin) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N || !queries(d.B, i)) return;
+    const V3<double> amn = cvt<double>(d.B.amin[i]) - mk3<double>(margin, margin, margin);
+    const V3<double> amx = cvt<double>(d.B.amax[i]) + mk3<double>(margin, margin, margin);
+    const int w = d.B.world[i];
+    const int wb0 = d.wStart[w], wb1 = d.wStart[w + 1];
+    for (int x = d.gLargeOff[wb0]; x < d.gLargeOff[wb1]; x++) {
+        int j = d.gLarge[x];
+        if (box_hit<T>(amn, amx, cvt<double>(d.B.amin[j]), cvt<double>(d.B.amax[j])) && pair_ok(d, i, j)) bp_emit(d, i, j);
+    }
+    if (d.bodySlot[i] >= 0) return;
+    for (int j = wb0; j < wb1; j++)
+        if (d.bodySlot[j] >= 0 && box_hit<T>(amn, amx, cvt<double>(d.B.amin[j]), cvt<double>(d.B.amax[j])) && pair_ok(d, i, j))
+            bp_emit(d, i, j);
+}
+
+template <typename T>
+__global__ void k_bp_compact(Dev<T> d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N) return;
+    int n = min(d.bpCount[i], BP_SLOTS);
+    int o = d.bpOff[i];
+    const int* slots = d.bpSlots + (size_t)i * BP_SLOTS;
+    for (int k = 0; k < n; k++)
+        if (o + k < d.pairCap) { d.pairA[o + k] = i; d.pairB[o + k] = slots[k]; }
+}
+
+// spilled pairs go to their rank within their body's range
+template <typename T>
+__global__ void k_bp_spill(Dev<T> d) {
+    int ns = min(d.counters[CNT_SPILL], d.spillCap);
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+        int i = d.spillA[s];
+        int p = d.bpOff[i] + d.spillK[s];
+        if (p < d.pairCap) { d.pairA[p] = i; d.pairB[p] = d.spillB[s]; }
+    }
 }
 
 // --------------------------------------------------------------- scans
@@ -249,8 +463,9 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<
     int fa = B.flags[ia], fb = B.flags[ib];
     bool dynA = fl_motion(fa) == MOTION_DYNAMIC, dynB = fl_motion(fb) == MOTION_DYNAMIC;
     if (!((dynA && B.invMass[ia] != T(0)) || (dynB && B.invMass[ib] != T(0)))) return;
-    int cs = atomicAdd(&d.counters[CNT_CONS], 1);
+    int cs = wave_alloc(&d.counters[CNT_CONS]);
     if (cs >= d.conCap) { atomicOr(&d.counters[CNT_OVERFLOW], 2); return; }
+    d.pairCon[slot] = cs;
     Con<T>& k = d.con[cs];
     ConTap<T>& tp = d.tap[cs];
     CacheRec<T>& rc = d.rec[par][cs];
@@ -279,7 +494,8 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<
     T fr = combine(B.friction[ia], B.friction[ib], fl_fmode(fa), fl_fmode(fb));
     T re = combine(B.restitution[ia], B.restitution[ib], fl_rmode(fa), fl_rmode(fb));
     k.a = ia; k.b = ib;
-    k.pairIdx = slot;
+    k.cq = d.pairA[slot];  // (query, other): the creation order of constraints
+    k.co = d.pairB[slot];
     k.dynMask = (dynA ? 1 : 0) | (dynB ? 2 : 0);
     k.movMask = (fl_motion(fa) != MOTION_STATIC ? 1 : 0) | (fl_motion(fb) != MOTION_STATIC ? 2 : 0);
     k.sortKey = sort_key((unsigned)ia, (unsigned)ib, 0xffffffffu, 0xffffffffu);
@@ -380,6 +596,7 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev<T> d, T dt) {
     int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
     int par = *d.parity;
     for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < npairs; slot += gridDim.x * blockDim.x) {
+        d.pairCon[slot] = -1;
         PairBodies<T> pb = order_pair(d, d.pairA[slot], d.pairB[slot]);
         int A = pb.a, Bi = pb.b;
         const Bodies<T>& B = d.B;
@@ -466,32 +683,35 @@ __global__ void k_wake(Dev<T> d) {
     }
 }
 
-// group constraint slots by world (counting sort); the order inside a world is irrelevant
-// because the solver orders constraints by their keys
+// Constraints in pair order: pairs are stored in body order and bodies in world order, so
+// compacting the pair list's constraint slots groups them by world with no sort.
 template <typename T>
-__global__ void k_con_world_count(Dev<T> d) {
-    int nc = min(d.counters[CNT_CONS], d.conCap);
-    unsigned long long pts = 0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
-        atomicAdd(&d.wConCnt[d.B.world[d.con[k].a]], 1);
-        pts += (unsigned long long)d.con[k].np;
-    }
-    for (int o = 32; o > 0; o >>= 1) pts += __shfl_xor(pts, o, 64);
-    if ((threadIdx.x & 63) == 0 && pts) atomicAdd(&d.acc[ACC_POINTS], pts);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(&d.acc[ACC_PAIRS], (unsigned long long)min(d.counters[CNT_PAIRS], d.pairCap));
-        atomicAdd(&d.acc[ACC_CONS], (unsigned long long)nc);
-        atomicAdd(&d.acc[ACC_EPA], (unsigned long long)min(d.counters[CNT_EPA], d.epaCap));
-        atomicAdd(&d.acc[ACC_STEPS], 1ull);
-    }
+__global__ void k_con_flags(Dev<T> d) {
+    int np = min(d.counters[CNT_PAIRS], d.pairCap);
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < d.pairCap; p += gridDim.x * blockDim.x)
+        d.conFlag[p] = (p < np && d.pairCon[p] >= 0) ? 1 : 0;
 }
 
 template <typename T>
-__global__ void k_con_world_scatter(Dev<T> d) {
-    int nc = min(d.counters[CNT_CONS], d.conCap);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
-        int w = d.B.world[d.con[k].a];
-        d.conIdx[d.wConStart[w] + atomicAdd(&d.wFill[w], 1)] = k;
+__global__ void k_con_compact(Dev<T> d) {
+    int np = min(d.counters[CNT_PAIRS], d.pairCap);
+    unsigned long long pts = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        int c = d.pairCon[p];
+        if (c >= 0) {
+            d.conIdx[d.conPos[p]] = c;
+            pts += (unsigned long long)d.con[c].np;
+        }
+    }
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w <= d.W; w += gridDim.x * blockDim.x)
+        d.wConStart[w] = d.conPos[min(d.bpOff[d.wStart[w]], np)];
+    for (int o = 32; o > 0; o >>= 1) pts += __shfl_xor(pts, o, 64);
+    if ((threadIdx.x & 63) == 0 && pts) atomicAdd(&d.acc[ACC_POINTS], pts);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&d.acc[ACC_PAIRS], (unsigned long long)np);
+        atomicAdd(&d.acc[ACC_CONS], (unsigned long long)min(d.counters[CNT_CONS], d.conCap));
+        atomicAdd(&d.acc[ACC_EPA], (unsigned long long)min(d.counters[CNT_EPA], d.epaCap));
+        atomicAdd(&d.acc[ACC_STEPS], 1ull);
     }
 }
 

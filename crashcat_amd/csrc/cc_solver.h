@@ -164,6 +164,120 @@ __device__ inline bool solve_position_contact(const Dev<T>& d, int slot) {  // :
     return any;
 }
 
+// ------------------------------------------------- the same three solves on the level-ordered rows
+template <typename T>
+__device__ inline void solve_velocity_rows(const Dev<T>& d, int e) {  // :963-1053 for one constraint
+    const int meta = d.svMeta[e];
+    const int dyn = meta & 3;
+    if (!dyn) return;
+    const Bodies<T>& B = d.B;
+    const int a = d.svA[e], b = d.svB[e];
+    V3<T> vA = B.v[a], wA = B.w[a], vB = B.v[b], wB = B.w[b];
+    const int mov = (meta >> 2) & 3, np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
+    const T iA = d.svIMA[e], iB = d.svIMB[e], fr = d.svFr[e];
+    const V3<T> n = d.svN[e], t1 = d.svT1[e], t2 = d.svT2[e];
+    const size_t cap = (size_t)d.svCap;
+    for (int j = 0; j < np; j++) {
+        const size_t k = j * cap + e;
+        Row<T> r1 = row_load(d.rT1, k), r2 = row_load(d.rT2, k);
+        if (r1.eff != T(0) || r2.eff != T(0)) {
+            T l1 = axis_total_lambda(r1, mov, vA, wA, vB, wB, t1);
+            T l2 = axis_total_lambda(r2, mov, vA, wA, vB, wB, t2);
+            T mf = fr * d.rN.lambda[k];
+            T fm2 = l1 * l1 + l2 * l2;
+            if (fm2 > mf * mf) { T sc = mf / sqrt(fm2); l1 *= sc; l2 *= sc; }
+            axis_apply(r1, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, t1, l1);
+            axis_apply(r2, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, t2, l2);
+            d.rT1.lambda[k] = r1.lambda;
+            d.rT2.lambda[k] = r2.lambda;
+        }
+    }
+    for (int j = 0; j < np; j++) {
+        const size_t k = j * cap + e;
+        Row<T> rn = row_load(d.rN, k);
+        T tl = axis_total_lambda(rn, mov, vA, wA, vB, wB, n);
+        axis_apply(rn, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, n, tmax<T>(T(0), tl));
+        d.rN.lambda[k] = rn.lambda;
+    }
+    if (dyn & 1) { B.v[a] = vA; B.w[a] = wA; }
+    if (dyn & 2) { B.v[b] = vB; B.w[b] = wB; }
+}
+
+template <typename T>
+__device__ inline void warm_start_rows(const Dev<T>& d, int e, T ratio) {  // :906-955 for one constraint
+    const int meta = d.svMeta[e];
+    const int dyn = meta & 3;
+    const Bodies<T>& B = d.B;
+    const int a = d.svA[e], b = d.svB[e];
+    V3<T> vA = B.v[a], wA = B.w[a], vB = B.v[b], wB = B.w[b];
+    const int np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
+    const T iA = d.svIMA[e], iB = d.svIMB[e];
+    const V3<T> n = d.svN[e], t1 = d.svT1[e], t2 = d.svT2[e];
+    const size_t cap = (size_t)d.svCap;
+    for (int j = 0; j < np; j++) {
+        const size_t k = j * cap + e;
+        Row<T> r1 = row_load(d.rT1, k), r2 = row_load(d.rT2, k), rn = row_load(d.rN, k);
+        if (r1.eff != T(0) || r2.eff != T(0)) {
+            axis_warm(r1, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, t1, ratio);
+            axis_warm(r2, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, t2, ratio);
+            d.rT1.lambda[k] = r1.lambda;
+            d.rT2.lambda[k] = r2.lambda;
+        }
+        axis_warm(rn, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, n, ratio);
+        d.rN.lambda[k] = rn.lambda;
+    }
+    if (dyn & 1) { B.v[a] = vA; B.w[a] = wA; }
+    if (dyn & 2) { B.v[b] = vB; B.w[b] = wB; }
+}
+
+template <typename T>
+__device__ inline bool solve_position_rows(const Dev<T>& d, int e) {  // :1098-1216 for one constraint
+    const int meta = d.svMeta[e];
+    const int dyn = meta & 3;
+    if (!dyn) return false;
+    const Bodies<T>& B = d.B;
+    const int a = d.svA[e], b = d.svB[e];
+    const int np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
+    const T iMA = d.svIMA[e], iMB = d.svIMB[e];
+    V3<T> cA = B.com[a], cB = B.com[b];
+    Q4<T> qA = B.q[a], qB = B.q[b];
+    M3<T> RA = m3_from_quat(qA), RB = m3_from_quat(qB);
+    M3<T> IA, IB;
+    for (int j = 0; j < 9; j++) { IA.m[j] = T(0); IB.m[j] = T(0); }
+    if (dyn & 1) IA = inv_inertia(B.invI[a], B.iRot[a], qA, dofA);
+    if (dyn & 2) IB = inv_inertia(B.invI[b], B.iRot[b], qB, dofB);
+    bool any = false;
+    const V3<T> n = d.svN[e];
+    const size_t cap = (size_t)d.svCap;
+    for (int j = 0; j < np; j++) {
+        const size_t k = j * cap + e;
+        V3<T> pA = cA + mul(RA, d.svLA[k]);
+        V3<T> pB = cB + mul(RB, d.svLB[k]);
+        T sep = dot(pB - pA, n) + d.s.slop;
+        sep = tmax<T>(sep, -d.s.maxPen);
+        if (sep >= T(0)) continue;
+        V3<T> mid = mk3<T>((pA.x + pB.x) * T(0.5), (pA.y + pB.y) * T(0.5), (pA.z + pB.z) * T(0.5));
+        V3<T> rA = mid - cA, rB = mid - cB;
+        Row<T> rn;
+        axis_props(rn, (dyn & 1) != 0, (dyn & 2) != 0, iMA, iMB, IA, IB, rA, rB, n, T(0));
+        // solvePositionConstraintWithMassOverride axis-constraint-part.ts:790-847
+        if (sep == T(0) || rn.eff == T(0)) continue;
+        T lam = -rn.eff * d.s.beta * sep;
+        if (dyn & 1) {
+            cA = dof_lin(cA - n * (lam * iMA), dofA);
+            add_rot_step(qA, rn.i1 * lam, T(-1));
+        }
+        if (dyn & 2) {
+            cB = dof_lin(cB + n * (lam * iMB), dofB);
+            add_rot_step(qB, rn.i2 * lam, T(1));
+        }
+        any = true;
+    }
+    if (dyn & 1) { B.com[a] = cA; B.q[a] = qA; }
+    if (dyn & 2) { B.com[b] = cB; B.q[b] = qB; }
+    return any;
+}
+
 // ---------------------------------------------------------------- point constraint
 template <typename T>
 __device__ inline void point_props(const Dev<T>& d, PointCon<T>& p) {  // point-constraint-part.ts:77-165
@@ -280,13 +394,16 @@ __device__ inline bool key_less(const unsigned long long* K, int x, int y) {
     return K[2 * x + 1] < K[2 * y + 1];
 }
 
-template <typename T, int BS>
+template <typename T, int BS, bool ROWS>
 __device__ inline void warm_start_levels(const Dev<T>& d, int so, int D, const int* LS, T ratio) {
     for (int l = 0; l < D; l++) {
         for (int x = LS[l] + threadIdx.x; x < LS[l + 1]; x += BS) {
-            int s = d.seq[so + d.sched[so + x]];
+            int s = ROWS ? d.svSlot[so + x] : d.seq[so + d.sched[so + x]];
             if (s >= 0x40000000) continue;
-            if (s >= 0) warm_start_contact(d, s, ratio);
+            if (s >= 0) {
+                if (ROWS) warm_start_rows(d, so + x, ratio);
+                else warm_start_contact(d, s, ratio);
+            }
             else {
                 PointCon<T>& pc = d.pc[-1 - s];
                 if (pc.enabled && !pc.sleeping && pc.active) {
@@ -299,8 +416,9 @@ __device__ inline void warm_start_levels(const Dev<T>& d, int so, int D, const i
     }
 }
 
-// creation-order key of sequence element k: contacts in the order they were created (their
-// broadphase pair order), then point constraints by index; the warm start runs in this order
+// creation-order key of sequence element k: contacts in the order they were created
+// (ascending (querying body, other body), the order of the pair list), then point constraints
+// by index; the warm start runs in this order
 // (src/update.ts:168-175 warm-starts the constraint arrays front to back)
 template <typename T>
 struct CreationLess {
@@ -308,8 +426,8 @@ struct CreationLess {
     int so;
     __device__ unsigned long long key(int k) const {
         int s = d->seq[so + k];
-        if (s >= 0) return (unsigned long long)(unsigned)d->con[s].pairIdx;
-        return (1ull << 32) | (unsigned)(-1 - s);
+        if (s >= 0) return ((unsigned long long)(unsigned)d->con[s].cq << 32) | (unsigned)d->con[s].co;
+        return (1ull << 63) | (unsigned)(-1 - s);
     }
     __device__ bool operator()(int x, int y) const { return key(x) < key(y); }
 };
@@ -465,30 +583,66 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     if (kCreationOrderWarmStart && d.s.warmStarting) {
         link_body_lists<T, BS>(d, b0, nb, so, BL, CreationLess<T>{&d, so});
         D = build_levels<T, BS>(d, so, L, LS, sTail);
-        warm_start_levels<T, BS>(d, so, D, LS, ratio);
+        warm_start_levels<T, BS, false>(d, so, D, LS, ratio);
     }
     link_body_lists<T, BS>(d, b0, nb, so, BL, SolveLess{K});
     D = build_levels<T, BS>(d, so, L, LS, sTail);
     if (tid == 0) d.stats[2 * w] = D;
-    if (!kCreationOrderWarmStart && d.s.warmStarting) warm_start_levels<T, BS>(d, so, D, LS, ratio);
+    // ---- stage the contact rows in level order
+    {
+        const size_t cap = (size_t)d.svCap;
+        for (int x = tid; x < L; x += BS) {
+            const int e = so + x;
+            const int s = d.seq[so + d.sched[so + x]];
+            d.svSlot[e] = s;
+            if (s < 0 || s >= 0x40000000) continue;
+            const Con<T>& k = d.con[s];
+            d.svA[e] = k.a;
+            d.svB[e] = k.b;
+            d.svMeta[e] = k.dynMask | (k.movMask << 2) | (k.np << 4) | (fl_dof(B.flags[k.a]) << 8) | (fl_dof(B.flags[k.b]) << 14);
+            d.svIMA[e] = k.invMassA;
+            d.svIMB[e] = k.invMassB;
+            d.svFr[e] = k.friction;
+            d.svN[e] = k.normal;
+            d.svT1[e] = k.t1;
+            d.svT2[e] = k.t2;
+            for (int j = 0; j < k.np; j++) {
+                const size_t q = j * cap + e;
+                row_store(d.rN, q, k.p[j].n);
+                row_store(d.rT1, q, k.p[j].t1);
+                row_store(d.rT2, q, k.p[j].t2);
+                d.svLA[q] = k.p[j].localA;
+                d.svLB[q] = k.p[j].localB;
+            }
+        }
+        __syncthreads();
+    }
+    if (!kCreationOrderWarmStart && d.s.warmStarting) warm_start_levels<T, BS, true>(d, so, D, LS, ratio);
     // ---- 6. velocity iterations
     for (int it = 0; it < d.s.velIters; it++) {
         for (int l = 0; l < D; l++) {
             for (int x = LS[l] + tid; x < LS[l + 1]; x += BS) {
-                int s = d.seq[so + d.sched[so + x]];
+                int s = d.svSlot[so + x];
                 if (s >= 0x40000000) continue;
-                if (s >= 0) solve_velocity_contact(d, s);
+                if (s >= 0) solve_velocity_rows(d, so + x);
                 else point_solve_velocity(d, d.pc[-1 - s]);
             }
             __syncthreads();
         }
     }
     // ---- 7. store impulses into this step's cache records (storeAppliedImpulses :1061-1076)
-    for (int k = tid; k < m; k += BS) {
-        int s = d.conIdx[c0 + k];
-        const Con<T>& c = d.con[s];
+    for (int x = tid; x < L; x += BS) {
+        const int e = so + x;
+        const int s = d.svSlot[e];
+        if (s < 0 || s >= 0x40000000) continue;
         CacheRec<T>& r = d.rec[par][s];
-        for (int j = 0; j < c.np; j++) { r.ln[j] = c.p[j].n.lambda; r.lt1[j] = c.p[j].t1.lambda; r.lt2[j] = c.p[j].t2.lambda; }
+        const int np = (d.svMeta[e] >> 4) & 7;
+        for (int j = 0; j < np; j++) {
+            const size_t q = j * (size_t)d.svCap + e;
+            r.ln[j] = d.rN.lambda[q];
+            r.lt1[j] = d.rT1.lambda[q];
+            r.lt2[j] = d.rT2.lambda[q];
+        }
     }
     // ---- 8. velocity integration (update.ts:896-976)
     for (int i = b0 + tid; i < b1; i += BS) {
@@ -521,16 +675,15 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         int anyActive = 0;
         for (int l = 0; l < D; l++) {
             for (int x = LS[l] + tid; x < LS[l + 1]; x += BS) {
-                int s = d.seq[so + d.sched[so + x]];
+                int s = d.svSlot[so + x];
                 if (s >= 0x40000000) continue;
                 int isl;
                 bool applied;
                 if (s >= 0) {
-                    const Con<T>& c = d.con[s];
-                    isl = island_of(d, c.a, c.b);
+                    isl = island_of(d, d.svA[so + x], d.svB[so + x]);
                     if (isl < 0 || !d.isl0[isl]) continue;
                     anyActive = 1;
-                    applied = solve_position_contact(d, s);
+                    applied = solve_position_rows(d, so + x);
                 } else {
                     PointCon<T>& pc = d.pc[-1 - s];
                     isl = island_of(d, pc.a, pc.b);

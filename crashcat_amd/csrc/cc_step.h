@@ -60,6 +60,29 @@ struct Row {
     V3<T> r1x, r2x, i1, i2;  // r1 x axis, r2 x axis, I1^-1 (r1 x axis), I2^-1 (r2 x axis)
     T eff, lambda, bias;
 };
+// a row set in structure-of-arrays form: element k of every array belongs to one row
+template <typename T>
+struct RowSoA {
+    V3<T>* r1x;
+    V3<T>* r2x;
+    V3<T>* i1;
+    V3<T>* i2;
+    T* eff;
+    T* lambda;
+    T* bias;
+};
+template <typename T>
+__device__ inline Row<T> row_load(const RowSoA<T>& R, size_t k) {
+    Row<T> r;
+    r.r1x = R.r1x[k]; r.r2x = R.r2x[k]; r.i1 = R.i1[k]; r.i2 = R.i2[k];
+    r.eff = R.eff[k]; r.lambda = R.lambda[k]; r.bias = R.bias[k];
+    return r;
+}
+template <typename T>
+__device__ inline void row_store(const RowSoA<T>& R, size_t k, const Row<T>& r) {
+    R.r1x[k] = r.r1x; R.r2x[k] = r.r2x; R.i1[k] = r.i1; R.i2[k] = r.i2;
+    R.eff[k] = r.eff; R.lambda[k] = r.lambda; R.bias[k] = r.bias;
+}
 template <typename T>
 struct ConPoint {
     Row<T> n, t1, t2;
@@ -70,8 +93,8 @@ struct Con {
     int a, b, np, dynMask;  // dynMask bit0: A dynamic, bit1: B dynamic
     unsigned sortKey;
     int movMask;            // bit0: A not static, bit1: B not static
-    int pairIdx;            // broadphase pair index: pairs are numbered in (query, other) order,
-                            // which is the order constraints are created in
+    int cq, co;             // the broadphase pair (querying body, other body); constraints are
+                            // created in ascending (cq, co) order
     V3<T> normal, t1, t2;
     T friction, invMassA, invMassB;
     ConPoint<T> p[4];

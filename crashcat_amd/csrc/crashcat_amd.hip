@@ -64,6 +64,8 @@ __global__ void cc::k_hash_clear(unsigned long long* keys, int cap) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) keys[i] = CC_EMPTY_KEY;
 }
 
+__global__ void k_copy_i32(int* dst, const int* src) { *dst = *src; }
+
 __global__ void k_fill_i32(int* p, int v, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -214,6 +216,32 @@ static int next_pow2(long long x) {
     long long p = 1;
     while (p < x) p <<= 1;
     return (int)p;
+}
+
+// Grid cell width.  A body's box is the bounds of its rotated local box (the 8 corners,
+// src/body/rigid-body.ts:527-553), so its width never exceeds the local box's diagonal.  The
+// width is the largest such diagonal over non-static bodies, capped at four times the median
+// so that one very large dynamic body does not coarsen the grid for all (it then joins the
+// per-world list of large bodies, as floors do).
+static double grid_cell_width(const cc_body_desc* bodies, int nb, const cc_shape_desc* shapes) {
+    std::vector<double> w;
+    for (int i = 0; i < nb; i++) {
+        if (bodies[i].motion_type == CC_MOTION_STATIC) continue;
+        const cc_shape_desc& q = shapes[bodies[i].shape];
+        double dx = q.aabb_max[0] - q.aabb_min[0], dy = q.aabb_max[1] - q.aabb_min[1], dz = q.aabb_max[2] - q.aabb_min[2];
+        w.push_back(std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    if (w.empty()) return 1.0;
+    std::sort(w.begin(), w.end());
+    double cs = std::min(w.back(), 4.0 * w[w.size() / 2]);
+    return cs > 1e-6 ? cs : 1.0;
+}
+// Cells are the width plus twice the margin (and a relative 1e-6): two small bodies whose boxes
+// overlap once one is grown by the margin then have centres less than a cell apart per axis.
+template <typename T>
+static void set_cell_size(Dev<T>& d, double margin) {
+    d.cellSize = (d.cellWidth + 2.0 * std::max(margin, 0.0)) * (1.0 + 1e-6);
+    d.cellInv = 1.0 / d.cellSize;
 }
 
 template <typename T>
@@ -375,25 +403,47 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.NPC = npc;
     // capacities
     long long pairCap = (long long)c->cfg.max_pairs_per_body * std::max(nb, 1);
-    if (pairCap > (1ll << 24) - 16) pairCap = (1ll << 24) - 16;  // two-level scan limit (4096 * 4096)
+    if (pairCap > (1ll << 30)) pairCap = 1ll << 30;
     d.pairCap = (int)pairCap;
     d.conCap = (int)std::min<long long>((long long)c->cfg.max_contacts_per_body * std::max(nb, 1), pairCap);
-    d.epaCap = std::max(1024, d.pairCap / 4);
+    d.epaCap = std::max(4096, d.pairCap / 16);  // EPA jobs: pairs whose cores overlap
     d.epaWorkers = std::min(d.epaCap, 65536);
     d.hashCap = next_pow2(2ll * d.conCap);
     int maxWorld = 0;
     for (int w = 0; w < nw; w++) maxWorld = std::max(maxWorld, woff[w + 1] - woff[w]);
+    if (nb > (1 << 23)) return fail(c, CC_E_CAPACITY, "more than 8388608 bodies in one context");
+    d.cellWidth = grid_cell_width(bodies, nb, shapes);
+    set_cell_size(d, (double)s->speculative_contact_distance);
+    d.cellTabCap = (int)next_pow2(std::max<long long>(2ll * nb, 64));
+    d.cellKey = A.get<unsigned long long>(d.cellTabCap);
+    d.cellCount = A.get<int>((size_t)d.cellTabCap + 1);
+    d.cellStart = A.get<int>((size_t)d.cellTabCap + 1);
+    d.cellFill = A.get<int>(d.cellTabCap);
+    d.cellList = A.get<int>(nb);
+    d.bodySlot = A.get<int>(nb);
+    d.cellBody = A.get<int>(nb);
+    d.cellMin = A.get<V3<T>>(nb);
+    d.cellMax = A.get<V3<T>>(nb);
+    d.gLargeFlag = A.get<int>(nb + 1);
+    d.gLargeOff = A.get<int>(nb + 1);
+    d.gLarge = A.get<int>(nb);
     d.bpCount = A.get<int>(nb + 1);
     d.bpOff = A.get<int>(nb + 1);
+    d.bpSlots = A.get<int>((size_t)nb * BP_SLOTS);
+    d.spillCap = std::max(nb, 1024);
+    d.spillA = A.get<int>(d.spillCap);
+    d.spillB = A.get<int>(d.spillCap);
+    d.spillK = A.get<int>(d.spillCap);
+    d.pairCon = A.get<int>(d.pairCap);
+    d.conFlag = A.get<int>((size_t)d.pairCap + 1);
+    d.conPos = A.get<int>((size_t)d.pairCap + 1);
     d.pairA = A.get<int>(d.pairCap);
     d.pairB = A.get<int>(d.pairCap);
     d.counters = A.get<int>(CNT_N);
     X->counters = d.counters;
     d.con = A.get<Con<T>>(d.conCap);
     d.tap = A.get<ConTap<T>>(d.conCap);
-    d.wConCnt = A.get<int>(nw + 1);
     d.wConStart = A.get<int>(nw + 1);
-    d.wFill = A.get<int>(nw + 1);
     d.conIdx = A.get<int>(d.conCap);
     for (int k = 0; k < 2; k++) {
         d.rec[k] = A.get<CacheRec<T>>(d.conCap);
@@ -416,6 +466,28 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.sched = A.get<int>(L);
     d.lvlStart = A.get<int>(2 * (L + nw + 1));
     d.blist = A.get<int>(2 * L);
+    d.svCap = (int)L;
+    d.svSlot = A.get<int>(L);
+    d.svA = A.get<int>(L);
+    d.svB = A.get<int>(L);
+    d.svMeta = A.get<int>(L);
+    d.svIMA = A.get<T>(L);
+    d.svIMB = A.get<T>(L);
+    d.svFr = A.get<T>(L);
+    d.svN = A.get<V3<T>>(L);
+    d.svT1 = A.get<V3<T>>(L);
+    d.svT2 = A.get<V3<T>>(L);
+    for (RowSoA<T>* R : {&d.rN, &d.rT1, &d.rT2}) {
+        R->r1x = A.get<V3<T>>(4 * L);
+        R->r2x = A.get<V3<T>>(4 * L);
+        R->i1 = A.get<V3<T>>(4 * L);
+        R->i2 = A.get<V3<T>>(4 * L);
+        R->eff = A.get<T>(4 * L);
+        R->lambda = A.get<T>(4 * L);
+        R->bias = A.get<T>(4 * L);
+    }
+    d.svLA = A.get<V3<T>>(4 * L);
+    d.svLB = A.get<V3<T>>(4 * L);
     d.bcnt = A.get<int>(nb);
     d.boff = A.get<int>(nb);
     d.isl0 = A.get<int>(nb);
@@ -424,7 +496,10 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.prevDt = A.get<T>(1);
     d.parity = A.get<int>(1);
     d.acc = A.get<unsigned long long>(ACC_N);
-    X->scanSums = A.get<int>(4096);
+    {  // scan workspace for the longest scan (pair flags, grid buckets or bodies)
+        long long n = std::max<long long>({(long long)d.pairCap + 1, (long long)d.cellTabCap + 1, (long long)nb + 1});
+        X->scanSums = A.get<int>((size_t)(n / 4096 + n / (4096ll * 4096) + 64));
+    }
     X->maxWorld = maxWorld;
     if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (workspace)");
     HIPCHK(c, hipMemset(d.counters, 0, CNT_N * sizeof(int)));
@@ -466,12 +541,37 @@ static int grid_for(long long n, int bs, int maxBlocks) {
     return (int)g;
 }
 
-static void scan_launch(hipStream_t st, const int* in, int* out, int n, int* sums) {
-    int nb = (n + 4095) / 4096;
-    if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, in, out, sums, n);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, nb, out + n);
-    hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1), dim3(256), 0, st, out, sums, n);
+// exclusive scan of n ints into out[0..n), total in out[n]; work holds the per-block sums of
+// every level (about n / 4000 ints)
+static void scan_launch(hipStream_t st, const int* in, int* out, int n, int* work) {
+    int nb = std::max(1, (n + 4095) / 4096);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, in, out, work, n);
+    if (nb <= 4096) {
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, work, nb, out + n);
+    } else {
+        scan_launch(st, work, work, nb, work + nb + 1);  // in place; its total lands in work[nb]
+        hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(1), 0, st, out + n, work + nb);
+    }
+    hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1), dim3(256), 0, st, out, work, n);
+}
+
+// broadphase of one step: bin the bodies into cells, find the pairs, lay them out in body order
+template <typename T>
+static void launch_broadphase(CtxT<T>* X, hipStream_t st, double margin) {
+    Dev<T>& d = X->d;
+    const int N = d.N;
+    const int nbB = (N + 255) / 256;
+    hipLaunchKernelGGL(k_cells_clear<T>, dim3(grid_for(std::max(d.cellTabCap, N), 256, 4096)), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_cells_insert<T>, dim3(nbB), dim3(256), 0, st, d);
+    scan_launch(st, d.cellCount, d.cellStart, d.cellTabCap, X->scanSums);
+    scan_launch(st, d.gLargeFlag, d.gLargeOff, N, X->scanSums);
+    hipLaunchKernelGGL(k_cells_scatter<T>, dim3(nbB), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_bp_cells<T>, dim3(grid_for((N + 3) / 4, 1, 8192)), dim3(256), 0, st, d, margin);
+    hipLaunchKernelGGL(k_bp_large<T>, dim3(nbB), dim3(256), 0, st, d, margin);
+    scan_launch(st, d.bpCount, d.bpOff, N, X->scanSums);
+    hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
+    hipLaunchKernelGGL(k_bp_compact<T>, dim3(nbB), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_bp_spill<T>, dim3(grid_for(d.spillCap, 256, 1024)), dim3(256), 0, st, d);
 }
 
 template <typename T>
@@ -513,10 +613,7 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
         auto mark = [&]() { (void)hipEventRecord(E[e++], st); };
         mark();
         hipLaunchKernelGGL(k_accel<T>, dim3(nbB), dim3(256), 0, st, d, dt);
-        hipLaunchKernelGGL((k_bp<T, false>), dim3(nbB), dim3(256), 0, st, d, margin);
-        scan_launch(st, d.bpCount, d.bpOff, N, X->scanSums);
-        hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
-        hipLaunchKernelGGL((k_bp<T, true>), dim3(nbB), dim3(256), 0, st, d, margin);
+        launch_broadphase<T>(X, st, margin);
         mark();  // broadphase
         hipLaunchKernelGGL(k_narrowphase<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d, dt);
         mark();  // narrowphase (GJK + manifold + constraint setup)
@@ -524,11 +621,10 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
         hipLaunchKernelGGL(k_count_epa<T>, dim3(1), dim3(64), 0, st, d);
         mark();  // EPA
         hipLaunchKernelGGL(k_wake<T>, dim3(nbB), dim3(256), 0, st, d);
-        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(2 * (d.W + 1), 256, 64)), dim3(256), 0, st, d.wConCnt, 0, d.W + 1);
-        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(d.W + 1, 256, 64)), dim3(256), 0, st, d.wFill, 0, d.W + 1);
-        hipLaunchKernelGGL(k_con_world_count<T>, dim3(grid_for(d.conCap, 256, 2048)), dim3(256), 0, st, d);
-        scan_launch(st, d.wConCnt, d.wConStart, d.W, X->scanSums);
-        hipLaunchKernelGGL(k_con_world_scatter<T>, dim3(grid_for(d.conCap, 256, 2048)), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_con_flags<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
+        scan_launch(st, d.conFlag, d.conPos, d.pairCap, X->scanSums);
+        hipLaunchKernelGGL(k_con_compact<T>, dim3(grid_for(std::max<long long>(d.pairCap, d.W + 1), 256, 4096)), dim3(256), 0,
+                           st, d);
         hipLaunchKernelGGL(k_uf_init<T>, dim3(nbB), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_uf_link<T>, dim3(grid_for(d.conCap + d.NPC, 256, 2048)), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_uf_compress<T>, dim3(nbB), dim3(256), 0, st, d);
@@ -702,13 +798,19 @@ static int contacts_impl(cc_ctx* c, cc_contact_out* out, int cap, int* count) {
     res.reserve(n);
     std::vector<Con<T>> hcs(n);
     std::vector<ConTap<T>> hts(n);
+    std::vector<CacheRec<T>> hrs(n);
     if (n > 0) {
+        int par = 0;
+        HIPCHK(c, hipMemcpy(&par, d.parity, sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(hcs.data(), d.con, n * sizeof(Con<T>), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(hts.data(), d.tap, n * sizeof(ConTap<T>), hipMemcpyDeviceToHost));
+        // the last step wrote its impulses to the records of the parity it ran with
+        HIPCHK(c, hipMemcpy(hrs.data(), d.rec[par ^ 1], n * sizeof(CacheRec<T>), hipMemcpyDeviceToHost));
     }
     for (int k = 0; k < n; k++) {
         const Con<T>& h = hcs[idx[k]];
         const ConTap<T>& t = hts[idx[k]];
+        const CacheRec<T>& r = hrs[idx[k]];
         cc_contact_out o;
         memset(&o, 0, sizeof(o));
         o.body_a = h.a; o.body_b = h.b; o.num_points = h.np;
@@ -716,7 +818,7 @@ static int contacts_impl(cc_ctx* c, cc_contact_out* out, int cap, int* count) {
         for (int j = 0; j < h.np && j < 4; j++) {
             o.position_a[j][0] = t.pa[j].x; o.position_a[j][1] = t.pa[j].y; o.position_a[j][2] = t.pa[j].z;
             o.position_b[j][0] = t.pb[j].x; o.position_b[j][1] = t.pb[j].y; o.position_b[j][2] = t.pb[j].z;
-            o.lambda_n[j] = h.p[j].n.lambda; o.lambda_t1[j] = h.p[j].t1.lambda; o.lambda_t2[j] = h.p[j].t2.lambda;
+            o.lambda_n[j] = r.ln[j]; o.lambda_t1[j] = r.lt1[j]; o.lambda_t2[j] = r.lt2[j];
         }
         res.push_back(o);
     }
@@ -874,14 +976,15 @@ static int bp_impl(cc_ctx* c, const float* mn, const float* mx, const int32_t* m
         HIPCHK(c, hipMemcpy(d.B.amin, a.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(d.B.amax, b.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
     }
-    hipStream_t st = c->stream;
-    int nbB = (n + 255) / 256;
-    if (n > 0) {
-        hipLaunchKernelGGL((k_bp<T, false>), dim3(nbB), dim3(256), 0, st, d, margin);
-        scan_launch(st, d.bpCount, d.bpOff, n, X->scanSums);
-        hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
-        hipLaunchKernelGGL((k_bp<T, true>), dim3(nbB), dim3(256), 0, st, d, margin);
-    }
+    // cell width from the given boxes: the widest, capped at four times the median width
+    std::vector<double> wd(n);
+    for (int i = 0; i < n; i++)
+        wd[i] = std::max({(double)mx[3 * i] - mn[3 * i], (double)mx[3 * i + 1] - mn[3 * i + 1], (double)mx[3 * i + 2] - mn[3 * i + 2]});
+    std::sort(wd.begin(), wd.end());
+    double cw = std::min(wd.back(), 4.0 * wd[wd.size() / 2]);
+    d.cellWidth = cw > 1e-6 ? cw : 1.0;
+    set_cell_size(d, margin);
+    launch_broadphase<T>(X, c->stream, margin);
     HIPCHK(c, hipGetLastError());
     return pairs_impl<T>(c, out, cap, count);
 }
