@@ -1,17 +1,23 @@
 """Benchmark of the crashcat step hot path on one MI355X.
 
-Workload (default): the mixed convex pile — 20,000 bodies drawn from spheres, boxes,
-capsules and 16-point convex hulls above a static floor (seed 1002) — stepped in FP32 as a
-batch of independent worlds (world 0 is the seeded scene, worlds 1.. redraw positions,
-orientations and velocities over the same shapes), next to the plain single-world FP64
-path on the same GPU and the FP64 CPU oracle on the host.
+Workload: the mixed convex pile -- 20,000 bodies drawn from spheres, boxes, capsules and
+16-point convex hulls above a static floor (seed 1002), 200 steps of 1/60 s -- stepped in FP32
+as a batch of independent worlds (world 0 is the seeded scene, worlds 1.. redraw positions,
+orientations and velocities over the same shapes).  Warmup steps are the first steps of the
+same trajectory, the timed steps the rest of it (20 + 180 = the 200 steps by default).
+
+Beside it: the plain single-world FP64 path on the same GPU, the FP32 path's error against
+the FP64 oracle (one step from identical state, at the start and in the settled pile; drift
+over 60 steps), and the oracle timed on one host core over a bounded sample.
 
 Prints one JSON line.  `value` is the whole-batch body-steps/s of the FP32 path with the
 scene resident in device memory (upload excluded).
 """
 import argparse
+import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -28,25 +34,60 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def byte_model(tot, body_bytes=540):
-    """Algorithmic bytes of the v1 byte model, split by phase.  Per item: body 540 B,
-    pair 16 B, manifold 876 B, point 228 B; the velocity/position sweeps are accumulated on
-    the device (160 B/manifold + 192 B/point per velocity sweep, 240 B/manifold + 24 B/point
-    per executed position sweep)."""
+def phase_bytes(t):
+    """Algorithmic bytes per phase of the v1 byte model (4-byte words, each phase reads its
+    inputs once and writes its outputs once) from the device's running totals:
+    accel 144 B/awake body; manifold kernel (narrowphase output, contact cache, constraint
+    setup) 876 B/manifold + 228 B/point; narrowphase (pair read) 16 B/pair; solver kernel (velocity integration, final
+    pose, sleep 396 B/awake body, plus the sweeps accumulated on the device:
+    (I_v + 1) (160 B/manifold + 192 B/point) + I_p (240 B/manifold + 24 B/point))."""
     return {
-        "body": body_bytes * tot["awake_body_steps"],
-        "pairs": 16 * tot["pairs"],
-        "setup": 876 * tot["contacts"] + 228 * tot["points"],
-        "solve": tot["solver_bytes"],
+        "accel": 144 * t["awake_body_steps"],
+        "narrowphase": 16 * t["pairs"],
+        "manifold": 876 * t["contacts"] + 228 * t["points"],
+        "solver": 396 * t["awake_body_steps"] + t["solver_bytes"],
     }
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE +
+    WRITE_SIZE runs, calibrated against a copy of known size), or None."""
+    path = os.path.join(HERE, "profiles", f"{kernel}_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    txt = open(path).read()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    w = re.search(r"workload:\s*(.+)", txt)
+    return (float(m.group(1)) if m else None), (w.group(1).strip() if w else None)
+
+
+def calibrate_copy(nbytes):
+    """A device-to-device copy of known size, for counter calibration runs."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copied {nbytes} bytes device to device 3 times")
+
+
+def pos_err(ob, gb, dyn):
+    e = np.linalg.norm(ob["position"] - gb["position"], axis=1)[dyn]
+    return {"max_m": float(e.max()), "p99_m": float(np.quantile(e, 0.99)), "median_m": float(np.median(e)),
+            "frac_le_1e-4": float((e <= 1e-4).mean())}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=180)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--worlds", type=int, default=32, help="worlds in the FP32 batch")
+    ap.add_argument("--worlds", type=int, default=256, help="worlds in the FP32 batch")
     ap.add_argument("--bodies", type=int, default=20000)
     ap.add_argument("--block", type=int, default=0, help="solver workgroup size (0: by world size)")
     ap.add_argument("--pairs-per-body", type=int, default=24, help="pair buffer capacity per body")
@@ -54,44 +95,51 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-accuracy", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=60, help="oracle steps timed from the settled pile")
     ap.add_argument("--profile-steps", type=int, default=0,
-                    help="only run the FP32 batch for this many steps (for rocprofv3)")
+                    help="only run the FP32 batch for warmup + this many steps (for rocprofv3)")
+    ap.add_argument("--calibrate", type=int, default=0, help="only copy this many bytes device to device")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); worlds shard across GPUs by "
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); worlds would shard across GPUs by "
               "contiguous world ranges, which this build does not launch.", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate:
+        calibrate_copy(args.calibrate)
+        return
 
     from crashcat_amd import scene as S
     from crashcat_amd._lib import Context
+
+    def ctx_for(prec):
+        return Context(prec, max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
+                       max_contacts_per_body=args.contacts_per_body)
 
     t0 = time.time()
     base = S.mixed_pile(seed=1002, n=args.bodies)
     log(f"scene: {base.num_bodies} bodies, {len(base.shapes)} shapes, built in {time.time() - t0:.1f}s")
     batch = S.replicate_world(base, args.worlds, 1002)
     nbody = args.bodies * args.worlds
+    log(f"batch: {args.worlds} worlds, {nbody} dynamic bodies, built in {time.time() - t0:.1f}s")
 
     if args.profile_steps:
-        ctx = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
-                      max_contacts_per_body=args.contacts_per_body)
+        ctx = ctx_for("fp32")
         ctx.upload(batch)
+        ctx.step(DT, args.warmup)
+        ctx.sync()
         ctx.step(DT, args.profile_steps)
         ctx.sync()
         log("profile run done", ctx.counters())
         return
 
-    # ---- FP32 batched
-    ctx = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
-                      max_contacts_per_body=args.contacts_per_body)
+    # ---- FP32 batch: the timed region
+    ctx = ctx_for("fp32")
     ctx.upload(batch)
-    log(f"fp32 batch uploaded: {args.worlds} worlds, {nbody} dynamic bodies")
     for w0 in range(0, args.warmup, 10):
         ctx.step(DT, min(10, args.warmup - w0))
         ctx.sync()
         log(f"  warmup {w0 + min(10, args.warmup - w0)}/{args.warmup}")
     tot0 = ctx.totals()
-    ctx.sync()
     t = time.perf_counter()
     ctx.step(DT, args.steps)
     ctx.sync()
@@ -102,22 +150,28 @@ def main():
     value = nbody * args.steps / el
     log(f"fp32 batch: {args.steps} steps in {el:.3f}s -> {value:.4g} body-steps/s; phases {phases}; {cnt}")
     dtot = {k: tot1[k] - tot0[k] for k in tot1}
-    bm = byte_model(dtot)
-    solver_ms = phases.get("solver", 0.0)
-    achieved = bm["solve"] / (solver_ms * 1e-3) / 1e9 if solver_ms > 0 else 0.0
+    pb = phase_bytes(dtot)
     ctx.close()
+
+    # dominant kernel: the phase with the largest measured time among the single-kernel phases
+    kernels = {"solver": "k_world", "manifold": "k_np_manifold", "narrowphase": "k_np_gjk"}
+    dom = max(kernels, key=lambda k: phases.get(k, 0.0))
+    dom_ms = phases[dom] / args.steps
+    achieved = pb[dom] / args.steps / (dom_ms * 1e-3) / 1e9
+    traffic, traffic_workload = read_traffic(kernels[dom])
+    per_phase = {k: {"ms_per_step": phases[k] / args.steps,
+                     "model_GBps": (pb[k] / args.steps / (phases[k] / args.steps * 1e-3) / 1e9) if k in pb and phases[k] > 0
+                     else None} for k in phases}
 
     # ---- FP64 plain single world
     fp64 = None
     if not args.skip_fp64:
         one = S.replicate_world(base, 1, 1002)
-        c64 = Context("fp64", max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
-                      max_contacts_per_body=args.contacts_per_body)
+        c64 = ctx_for("fp64")
         c64.upload(one)
-        w64 = max(1, min(args.warmup, 20))
-        c64.step(DT, w64)
+        c64.step(DT, args.warmup)
         c64.sync()
-        k64 = max(1, min(args.steps, 40))
+        k64 = args.steps  # the same steps of the trajectory as the timed FP32 region
         t = time.perf_counter()
         c64.step(DT, k64)
         c64.sync()
@@ -126,46 +180,59 @@ def main():
         log(f"fp64 plain: {k64} steps in {e64:.3f}s -> {fp64:.4g} body-steps/s; phases {c64.phase_times()}")
         c64.close()
 
-    # ---- accuracy vs the FP64 oracle and CPU baseline
+    # ---- accuracy vs the FP64 oracle, and the oracle's own speed on the host
     acc = {}
     cpu = None
+    cpu_sample = None
     if not (args.skip_accuracy and args.skip_cpu):
         from oracle.refcpu import OracleWorld
         one = S.replicate_world(base, 1, 1002)
-        orc = OracleWorld(one)
-        g = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
-                      max_contacts_per_body=args.contacts_per_body)
+        dyn = one.bodies()["motion_type"] == 2
+        if not args.skip_accuracy:
+            orc = OracleWorld(one)
+            g = ctx_for("fp32")
+            g.upload(one)
+            orc.step(DT)
+            g.step(DT, 1)
+            acc["step_from_initial_state"] = pos_err(orc.bodies(), g.bodies(), dyn)
+            for s in range(2, 61):
+                orc.step(DT)
+                g.step(DT, 1)
+                if s % 20 == 0:
+                    log(f"  oracle/fp32 lockstep {s} steps")
+            e = pos_err(orc.bodies(), g.bodies(), dyn)
+            acc["drift_60_steps"] = e
+            g.close()
+        # the settled pile: FP32 state after 100 steps, handed to the oracle and to a fresh
+        # FP32 context; one step of each from that identical state, then the oracle is timed
+        g = ctx_for("fp32")
         g.upload(one)
+        g.step(DT, 100)
+        st = g.bodies()
+        g.close()
+        orc = OracleWorld(one)
+        orc.set_bodies(st)
+        g = ctx_for("fp32")
+        g.upload(one)
+        g.set_bodies(st)
         t = time.perf_counter()
         orc.step(DT)
-        t_first = time.perf_counter() - t
+        t1 = time.perf_counter() - t
         g.step(DT, 1)
-        ob, gb = orc.bodies(), g.bodies()
-        dyn = one.bodies()["motion_type"] == 2
-        acc["step1_max_pos_err_m"] = float(np.abs(ob["position"] - gb["position"])[dyn].max())
-        nsteps = 1
-        cpu_steps = max(1, args.cpu_steps)
-        drift_steps = 0 if args.skip_accuracy else 60
-        tcpu = t_first
-        while nsteps < max(drift_steps, cpu_steps):
-            t = time.perf_counter()
-            orc.step(DT)
-            dtc = time.perf_counter() - t
-            if nsteps < cpu_steps:
-                tcpu += dtc
-            g.step(DT, 1)
-            nsteps += 1
-            if nsteps % 10 == 0:
-                log(f"  oracle/fp32 lockstep {nsteps} steps")
-        if drift_steps:
-            ob, gb = orc.bodies(), g.bodies()
-            err = np.linalg.norm(ob["position"] - gb["position"], axis=1)[dyn]
-            acc["drift_60_max_m"] = float(err.max())
-            acc["drift_60_median_m"] = float(np.median(err))
-            acc["drift_60_frac_le_1mm"] = float((err <= 1e-3).mean())
-        cpu = args.bodies * cpu_steps / tcpu
-        log(f"oracle: {cpu_steps} steps in {tcpu:.2f}s -> {cpu:.4g} body-steps/s; accuracy {acc}")
+        if not args.skip_accuracy:
+            acc["step_from_settled_pile"] = pos_err(orc.bodies(), g.bodies(), dyn)
         g.close()
+        if not args.skip_cpu:
+            n = max(1, args.cpu_steps)
+            tc = t1
+            for _ in range(n - 1):
+                t = time.perf_counter()
+                orc.step(DT)
+                tc += time.perf_counter() - t
+            cpu = args.bodies * n / tc
+            cpu_sample = (f"FP64 oracle (C restatement of crashcat's step), {n} steps of one {args.bodies}-body world "
+                          f"from the pile state at step 100 (contacts cold), single thread, {tc:.1f} s")
+        log(f"accuracy {acc}; cpu {cpu}")
 
     out = {
         "metric": "body-steps/s batched FP32 vs plain FP64 GPU at <=1e-4 m err; % HBM peak",
@@ -179,35 +246,36 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded scene generator)",
+        "data": "synthetic (seeded scene generator, seed 1002)",
         "config": {
-            "workload": "mixed convex pile: 20k spheres/boxes/capsules/16-pt hulls over a static floor, dt 1/60, "
-                        "10 velocity + 2 position iterations, batched as independent worlds",
+            "workload": "mixed convex pile: 20k spheres/boxes/capsules/16-point hulls in 60x40x60 m over a static "
+                        "floor, dt 1/60, 10 velocity + 2 position iterations, 200 steps (warmup + timed), batched "
+                        "as independent worlds",
             "worlds": args.worlds,
             "bodies_per_world": args.bodies,
-            "warmup_steps_before_timing": args.warmup,
         },
+        "awake_fraction": cnt["num_awake"] / max(1, cnt["num_bodies"]),
         "fp64_plain_body_steps_per_s": fp64,
         "batched_fp32_over_plain_fp64": (value / fp64) if fp64 else None,
         "accuracy_vs_fp64_oracle": acc,
-        "phase_ms_total": phases,
+        "phases": per_phase,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_world (sort, levels, warm start, velocity/position sweeps, integration, sleep)",
+            "kernel": kernels[dom],
             "achieved": achieved,
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_workload": traffic_workload,
         },
-        "model_bytes_per_body_step": sum(bm.values()) / max(1, nbody * args.steps),
+        "model_bytes_per_body_step": sum(pb.values()) / max(1, nbody * args.steps),
         "cpu_baseline": None if cpu is None else {
             "value": cpu,
             "unit": "body-steps/s",
             "cores": 1,
             "kind": "port",
-            "sample": f"FP64 oracle (C restatement of crashcat's step), first {args.cpu_steps} steps of one "
-                      f"{args.bodies}-body world of the same workload, single thread",
+            "sample": cpu_sample,
         },
     }
     print(json.dumps(out), flush=True)

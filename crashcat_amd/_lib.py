@@ -57,7 +57,7 @@ def lib():
 
 BODY_STATE_DTYPE = np.dtype(_abi.BodyState)
 CONTACT_DTYPE = np.dtype(_abi.ContactOut)
-PHASES = ["broadphase", "narrowphase", "epa", "islands", "solver", "cache"]
+PHASES = ["broadphase", "narrowphase", "epa", "manifold", "islands", "solver", "cache"]
 
 
 class Context:

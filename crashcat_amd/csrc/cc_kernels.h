@@ -52,6 +52,14 @@ struct Dev {
     CacheRec<T>* rec[2];          // [conCap] per parity
     unsigned long long* hkeys[2]; // [hashCap]
     int* hvals[2];
+    // narrowphase hits awaiting their manifold: pair slot, kind (HIT_*), and the closest points
+    // and axis (in body A's frame for convex pairs, in world space for sphere pairs)
+    int hitCap;
+    int* hitSlot;
+    int* hitKind;
+    V3<T>* hitPA;
+    V3<T>* hitPB;
+    V3<T>* hitAxis;
     // EPA jobs and the per-worker polytope workspace
     int* epaPair;       // [epaCap]
     Simplex<T>* epaSimplex;
@@ -105,8 +113,8 @@ enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
        ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_N = 16 };
 
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
-       CNT_N = 8 };
-// CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list
+       CNT_HITS = 7, CNT_N = 8 };
+// CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list, 16 hits
 constexpr int BP_SLOTS = 32;
 
 // orders LDS accesses between the lanes of one wavefront
@@ -589,12 +597,26 @@ __device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, c
     finish_pair(d, slot, A, Bi, wA, wB, wAxis, S.fA, S.fB, S.tmp, S.m, dt, par);
 }
 
-// One pair per lane: sphere-sphere closed form or the GJK step of collideConvexVsConvexLocal;
-// pairs whose cores overlap are queued for the EPA kernel.
+enum { HIT_SPHERES = 0, HIT_GJK = 1, HIT_EPA = 2 };
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_narrowphase(Dev<T> d, T dt) {
+__device__ inline void push_hit(const Dev<T>& d, int slot, int kind, V3<T> pa, V3<T> pb, V3<T> axis) {
+    int h = wave_alloc(&d.counters[CNT_HITS]);
+    if (h >= d.hitCap) { atomicOr(&d.counters[CNT_OVERFLOW], 16); return; }
+    d.hitSlot[h] = slot;
+    d.hitKind[h] = kind;
+    d.hitPA[h] = pa;
+    d.hitPB[h] = pb;
+    d.hitAxis[h] = axis;
+}
+
+// One pair per lane: sphere-sphere closed form or the GJK step of collideConvexVsConvexLocal
+// (convex.ts:257-641).  Hits go to the hit list for k_np_manifold; pairs whose cores overlap go
+// to the EPA queue.  Pairs whose bounding spheres are further apart than the speculative
+// distance cannot be within it, and GJK would report them separated at once: they stop here.
+template <typename T>
+__global__ void __launch_bounds__(256) k_np_gjk(Dev<T> d, T dt) {
     int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
-    int par = *d.parity;
     for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < npairs; slot += gridDim.x * blockDim.x) {
         d.pairCon[slot] = -1;
         PairBodies<T> pb = order_pair(d, d.pairA[slot], d.pairB[slot]);
@@ -602,7 +624,6 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev<T> d, T dt) {
         const Bodies<T>& B = d.B;
         const ShapeD<T>& sa = d.st.shapes[B.shape[A]];
         const ShapeD<T>& sb = d.st.shapes[B.shape[Bi]];
-        NpScratch<T> S;
         if (sa.type == SHAPE_SPHERE && sb.type == SHAPE_SPHERE) {  // collideSphereVsSphere sphere.ts:319-410
             V3<T> pa = B.pos[A], pbp = B.pos[Bi];
             T ra = sa.radius, rb = sb.radius;
@@ -611,10 +632,12 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev<T> d, T dt) {
             if (d2 >= rs * rs) continue;
             T dist = sqrt(d2);
             V3<T> n = dist > T(0) ? dd * (T(1) / dist) : mk3<T>(0, 1, 0);
-            S.fA.n = 0;
-            S.fB.n = 0;
-            finish_pair(d, slot, A, Bi, pa + n * ra, pbp - n * rb, n, S.fA, S.fB, S.tmp, S.m, dt, par);
+            push_hit(d, slot, HIT_SPHERES, pa + n * ra, pbp - n * rb, n);
             continue;
+        }
+        {
+            T reach = sa.brad + sb.brad + d.s.spec;
+            if (len2(B.pos[Bi] - B.pos[A]) > reach * reach) continue;
         }
         ConvexCtx<T> c;
         convex_setup(c, B.pos[A], B.q[A], B.pos[Bi], B.q[Bi], d.s.spec);
@@ -637,14 +660,13 @@ __global__ void __launch_bounds__(256) k_narrowphase(Dev<T> d, T dt) {
             else atomicOr(&d.counters[CNT_OVERFLOW], 4);
             continue;
         }
-        finish_convex(d, slot, A, Bi, c, pd, S, dt, par);
+        push_hit(d, slot, HIT_GJK, pd.pointA, pd.pointB, pd.axis);
     }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(64) k_epa(Dev<T> d, T dt) {
     int nj = min(d.counters[CNT_EPA], d.epaCap);
-    int par = *d.parity;
     int worker = blockIdx.x * blockDim.x + threadIdx.x;
     if (worker >= d.epaWorkers) return;
     for (int j = worker; j < nj; j += gridDim.x * blockDim.x) {
@@ -666,7 +688,37 @@ __global__ void __launch_bounds__(64) k_epa(Dev<T> d, T dt) {
         PenDepth<T> pd;
         Simplex<T> simplex = d.epaSimplex[j];
         if (!pd_step_epa(pd, SA, SB, d.s.penTol, simplex, d.epaWork[worker])) continue;
+        push_hit(d, slot, HIT_EPA, pd.pointA, pd.pointB, pd.axis);
+    }
+}
+
+// One hit per lane: world-space contact, supporting faces, manifold, contact cache and the
+// contact constraint (convex.ts:489-637, src/update.ts:385-563).
+template <typename T>
+__global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
+    int nh = min(d.counters[CNT_HITS], d.hitCap);
+    int par = *d.parity;
+    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+        int slot = d.hitSlot[h];
+        PairBodies<T> pb = order_pair(d, d.pairA[slot], d.pairB[slot]);
+        int A = pb.a, Bi = pb.b;
+        int kind = d.hitKind[h];
         NpScratch<T> S;
+        if (kind == HIT_SPHERES) {
+            S.fA.n = 0;
+            S.fB.n = 0;
+            finish_pair(d, slot, A, Bi, d.hitPA[h], d.hitPB[h], d.hitAxis[h], S.fA, S.fB, S.tmp, S.m, dt, par);
+            continue;
+        }
+        const Bodies<T>& B = d.B;
+        ConvexCtx<T> c;
+        convex_setup(c, B.pos[A], B.q[A], B.pos[Bi], B.q[Bi], d.s.spec);
+        if (kind == HIT_EPA) c.maxSepUse = tmin<T>(c.maxSepUse, T(1));
+        PenDepth<T> pd;
+        pd.status = 0;
+        pd.pointA = d.hitPA[h];
+        pd.pointB = d.hitPB[h];
+        pd.axis = d.hitAxis[h];
         finish_convex(d, slot, A, Bi, c, pd, S, dt, par);
     }
 }

@@ -14,6 +14,7 @@ struct ShapeD {
     int type, hfp, hnp, hff, hnf, pad;
     T radius, hh, cr;
     T he[3], com[3], amin[3], amax[3];
+    T brad;  // radius of a sphere about the shape origin holding the shape (core + convex radius)
 };
 
 template <typename T>

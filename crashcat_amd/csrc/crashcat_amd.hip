@@ -83,6 +83,7 @@ __global__ void k_set_pair_count(Dev<T> d) {
         d.counters[CNT_PAIRS] = n;
         d.counters[CNT_EPA] = 0;
         d.counters[CNT_CONS] = 0;
+        d.counters[CNT_HITS] = 0;
         if (n > d.pairCap) atomicOr(&d.counters[CNT_OVERFLOW], 1);
     }
 }
@@ -289,6 +290,24 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
             o.he[k] = T(q.half_extents[k]); o.com[k] = T(q.center_of_mass[k]);
             o.amin[k] = T(q.aabb_min[k]); o.amax[k] = T(q.aabb_max[k]);
         }
+        // bounding radius, generous (any support point of either mode lies inside it)
+        double br = 0.0;
+        double he2 = q.half_extents[0] * q.half_extents[0] + q.half_extents[1] * q.half_extents[1] + q.half_extents[2] * q.half_extents[2];
+        double com = std::sqrt(q.center_of_mass[0] * q.center_of_mass[0] + q.center_of_mass[1] * q.center_of_mass[1] +
+                               q.center_of_mass[2] * q.center_of_mass[2]);
+        if (q.type == CC_SHAPE_SPHERE) br = q.radius;
+        else if (q.type == CC_SHAPE_BOX) br = std::sqrt(he2) + q.convex_radius;
+        else if (q.type == CC_SHAPE_CAPSULE) br = q.half_height + q.radius;
+        else if (hulls && hulls->points && hulls->shrunk_points) {
+            for (int k = 0; k < q.hull_num_points && q.hull_first_point + k < hulls->total_points; k++) {
+                const double* pp = hulls->points + 3 * (q.hull_first_point + k);
+                const double* ss = hulls->shrunk_points + 3 * (q.hull_first_point + k);
+                br = std::max(br, std::sqrt(pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]));
+                br = std::max(br, std::sqrt(ss[0] * ss[0] + ss[1] * ss[1] + ss[2] * ss[2]) + q.convex_radius);
+            }
+            br += com;
+        } else br = 1e30;
+        o.brad = T(br * (1.0 + 1e-5) + 1e-6);
     }
     int P = hulls ? hulls->total_points : 0, F = hulls ? hulls->total_faces : 0, FV = hulls ? hulls->total_face_vertices : 0;
     for (int i = 0; i < ns; i++)
@@ -450,6 +469,12 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         d.hkeys[k] = A.get<unsigned long long>(d.hashCap);
         d.hvals[k] = A.get<int>(d.hashCap);
     }
+    d.hitCap = d.conCap + d.conCap / 8 + 1024;
+    d.hitSlot = A.get<int>(d.hitCap);
+    d.hitKind = A.get<int>(d.hitCap);
+    d.hitPA = A.get<V3<T>>(d.hitCap);
+    d.hitPB = A.get<V3<T>>(d.hitCap);
+    d.hitAxis = A.get<V3<T>>(d.hitCap);
     d.epaPair = A.get<int>(d.epaCap);
     d.epaSimplex = A.get<Simplex<T>>(d.epaCap);
     d.epaWork = A.get<EpaState<T>>(d.epaWorkers);
@@ -586,7 +611,7 @@ static void launch_world(CtxT<T>* X, hipStream_t st, T dt) {
     }
 }
 
-#define CC_PHASES 6
+#define CC_PHASES 7
 template <typename T>
 static int step_impl(cc_ctx* c, double dtd, int nsteps) {
     auto* X = static_cast<CtxT<T>*>(c->impl);
@@ -615,11 +640,13 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
         hipLaunchKernelGGL(k_accel<T>, dim3(nbB), dim3(256), 0, st, d, dt);
         launch_broadphase<T>(X, st, margin);
         mark();  // broadphase
-        hipLaunchKernelGGL(k_narrowphase<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d, dt);
-        mark();  // narrowphase (GJK + manifold + constraint setup)
+        hipLaunchKernelGGL(k_np_gjk<T>, dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
+        mark();  // narrowphase (sphere pairs, GJK)
         hipLaunchKernelGGL(k_epa<T>, dim3((d.epaWorkers + 63) / 64), dim3(64), 0, st, d, dt);
         hipLaunchKernelGGL(k_count_epa<T>, dim3(1), dim3(64), 0, st, d);
         mark();  // EPA
+        hipLaunchKernelGGL(k_np_manifold<T>, dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
+        mark();  // manifolds, contact cache, constraint setup
         hipLaunchKernelGGL(k_wake<T>, dim3(nbB), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_con_flags<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
         scan_launch(st, d.conFlag, d.conPos, d.pairCap, X->scanSums);

@@ -268,9 +268,10 @@ int cc_broadphase_aabbs(cc_ctx* ctx, const float* aabb_min, const float* aabb_ma
                         int32_t n, double margin, int32_t* out_pairs, int32_t capacity, int32_t* out_count);
 
 /* Per-phase device time summed over every step of the last cc_step call (milliseconds,
- * HIP events on the context stream).  Phases: broadphase, narrowphase (GJK, manifold,
- * constraint setup), EPA, islands, solver (sort, levels, warm start, velocity and
- * position sweeps, integration, sleep), contact-cache hash. */
+ * HIP events on the context stream).  Phases, in order: broadphase (with the acceleration
+ * integration), narrowphase (sphere pairs and GJK), EPA, manifold (supporting faces,
+ * clipping, contact cache, constraint setup), islands, solver (levels, warm start,
+ * velocity and position sweeps, integration, sleep), contact-cache hash. */
 int cc_get_phase_times(cc_ctx* ctx, double* out_ms, int32_t capacity, int32_t* out_count);
 int cc_get_totals(cc_ctx* ctx, cc_totals* out);
 
