@@ -292,30 +292,48 @@ __device__ inline bool box_hit(V3<double> amn, V3<double> amx, V3<double> bmn, V
     return !(bmx.x < amn.x || bmn.x > amx.x || bmx.y < amn.y || bmn.y > amx.y || bmx.z < amn.z || bmn.z > amx.z);
 }
 
-// Small-vs-small pairs, one wavefront per occupied cell: the bodies of the 27 cells around it
-// are staged through LDS in chunks and every querying body of the cell tests every staged one.
-constexpr int BP_CHUNK = 256;
+// pair_ok from staged per-body words: fl = flags, gr = groups, mk = mask, q = queries
+__device__ inline bool pair_ok_words(const unsigned char* layerPairs, int a, int fa, unsigned ga, unsigned ma, bool qa, int b,
+                                     int fb, unsigned gb, unsigned mb, bool qb) {
+    if (a == b) return false;
+    if (!layerPairs[fl_layer(fa) * 8 + fl_layer(fb)]) return false;
+    if (!((ga & mb) != 0 && (gb & ma) != 0)) return false;
+    long long da = qa ? a : 0x7fffffffll, db = qb ? b : 0x7fffffffll;
+    if (da >= db) return false;
+    return fl_motion(fa) == MOTION_DYNAMIC || fl_motion(fb) == MOTION_DYNAMIC;
+}
+
+// Small-vs-small pairs, one wavefront per occupied cell.  The cell's querying bodies (up to
+// 64 at a time) and the bodies of the 27 cells around it (in chunks) are staged in LDS with
+// everything the predicate needs; lanes then sweep the (home, candidate) grid.  A body is the
+// home of exactly one cell, so its pair counter lives in LDS for the whole kernel; k_bp_large
+// runs first and its pairs are counted in.
+constexpr int BP_CHUNK = 128;
+constexpr int BP_HOME = 64;
 template <typename T>
 __global__ void __launch_bounds__(256) k_bp_cells(Dev<T> d, double margin) {
     __shared__ V3<T> smn[4][BP_CHUNK], smx[4][BP_CHUNK];
-    __shared__ int sbody[4][BP_CHUNK];
+    __shared__ int sbody[4][BP_CHUNK], sfl[4][BP_CHUNK];
+    __shared__ unsigned sgr[4][BP_CHUNK], smk[4][BP_CHUNK];
+    __shared__ V3<double> hmn[4][BP_HOME], hmx[4][BP_HOME];
+    __shared__ int hbody[4][BP_HOME], hfl[4][BP_HOME], hcnt[4][BP_HOME];
+    __shared__ unsigned hgr[4][BP_HOME], hmk[4][BP_HOME];
     __shared__ int nStart[4][28], nPre[4][28];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ncells = d.counters[CNT_CELLS];
+    const unsigned char* lp = d.s.layerPairs;
     for (int ci = blockIdx.x * 4 + wv; ci < ncells; ci += gridDim.x * 4) {
         const int h = d.cellList[ci];
         const int h0 = d.cellStart[h], nh = d.cellCount[h];
         const int w = d.B.world[d.cellBody[h0]];
-        // the home cell's coordinates, from its first body's centre
         V3<double> c0 = (cvt<double>(d.cellMin[h0]) + cvt<double>(d.cellMax[h0])) * 0.5;
         int hx = grid_coord(c0.x, d.cellInv), hy = grid_coord(c0.y, d.cellInv), hz = grid_coord(c0.z, d.cellInv);
         int cnt = 0, st = 0;
         if (lane < 27) {
             int nx = hx + lane % 3 - 1, ny = hy + (lane / 3) % 3 - 1, nz = hz + lane / 9 - 1;
-            int s = cell_find(d, cell_key(w, nx, ny, nz));
-            if (s >= 0) { cnt = d.cellCount[s]; st = d.cellStart[s]; }
+            int sl = cell_find(d, cell_key(w, nx, ny, nz));
+            if (sl >= 0) { cnt = d.cellCount[sl]; st = d.cellStart[sl]; }
         }
-        // inclusive prefix of the 27 counts across the wave
         int pre = cnt;
         for (int o = 1; o < 64; o <<= 1) {
             int v = __shfl_up(pre, o, 64);
@@ -324,33 +342,72 @@ __global__ void __launch_bounds__(256) k_bp_cells(Dev<T> d, double margin) {
         if (lane < 27) { nStart[wv][lane] = st; nPre[wv][lane + 1] = pre; }
         if (lane == 0) nPre[wv][0] = 0;
         const int total = __shfl(pre, 26, 64);
-        wave_sync();
-        for (int cb = 0; cb < total; cb += BP_CHUNK) {
-            const int nc = min(BP_CHUNK, total - cb);
-            for (int x = lane; x < nc; x += 64) {
-                int f = cb + x;
-                int lo = 0, hi = 26;  // the neighbour cell holding flat index f
-                while (lo < hi) {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (nPre[wv][mid] <= f) lo = mid; else hi = mid - 1;
-                }
-                int p = nStart[wv][lo] + (f - nPre[wv][lo]);
-                sbody[wv][x] = d.cellBody[p];
-                smn[wv][x] = d.cellMin[p];
-                smx[wv][x] = d.cellMax[p];
+        for (int hb0 = 0; hb0 < nh; hb0 += BP_HOME) {
+            // stage this batch's querying home bodies, compacted
+            int i = -1;
+            bool q = false;
+            if (hb0 + lane < nh) {
+                i = d.cellBody[h0 + hb0 + lane];
+                q = queries(d.B, i);
+            }
+            unsigned long long qm = __ballot(q);
+            const int nq = __popcll(qm);
+            if (q) {
+                int r = __popcll(qm & ((1ull << lane) - 1ull));
+                hbody[wv][r] = i;
+                hmn[wv][r] = cvt<double>(d.cellMin[h0 + hb0 + lane]) - mk3<double>(margin, margin, margin);
+                hmx[wv][r] = cvt<double>(d.cellMax[h0 + hb0 + lane]) + mk3<double>(margin, margin, margin);
+                hfl[wv][r] = d.B.flags[i];
+                hgr[wv][r] = d.B.groups[i];
+                hmk[wv][r] = d.B.mask[i];
+                hcnt[wv][r] = d.bpCount[i];
             }
             wave_sync();
-            for (int hb = 0; hb < nh; hb++) {
-                const int i = d.cellBody[h0 + hb];
-                if (!queries(d.B, i)) continue;
-                const V3<double> amn = cvt<double>(d.cellMin[h0 + hb]) - mk3<double>(margin, margin, margin);
-                const V3<double> amx = cvt<double>(d.cellMax[h0 + hb]) + mk3<double>(margin, margin, margin);
+            if (nq == 0) continue;
+            for (int cb = 0; cb < total; cb += BP_CHUNK) {
+                const int nc = min(BP_CHUNK, total - cb);
                 for (int x = lane; x < nc; x += 64) {
-                    if (!box_hit<T>(amn, amx, cvt<double>(smn[wv][x]), cvt<double>(smx[wv][x]))) continue;
-                    int j = sbody[wv][x];
-                    if (pair_ok(d, i, j)) bp_emit(d, i, j);
+                    int f = cb + x;
+                    int lo = 0, hi = 26;  // the neighbour cell holding flat index f
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (nPre[wv][mid] <= f) lo = mid; else hi = mid - 1;
+                    }
+                    int p = nStart[wv][lo] + (f - nPre[wv][lo]);
+                    int j = d.cellBody[p];
+                    sbody[wv][x] = j;
+                    smn[wv][x] = d.cellMin[p];
+                    smx[wv][x] = d.cellMax[p];
+                    // a candidate that queries sets bit 30 of its staged flags (flags use bits 0-23)
+                    sfl[wv][x] = d.B.flags[j] | (queries(d.B, j) ? (1 << 30) : 0);
+                    sgr[wv][x] = d.B.groups[j];
+                    smk[wv][x] = d.B.mask[j];
                 }
+                wave_sync();
+                const int work = nq * nc;
+                int hh = lane / nc, x = lane % nc;  // (home, candidate) of this lane's first cell of work
+                for (int t = lane; t < work; t += 64) {
+                    const int j = sbody[wv][x];
+                    const int fj = sfl[wv][x];
+                    if (box_hit<T>(hmn[wv][hh], hmx[wv][hh], cvt<double>(smn[wv][x]), cvt<double>(smx[wv][x])) &&
+                        pair_ok_words(lp, hbody[wv][hh], hfl[wv][hh], hgr[wv][hh], hmk[wv][hh], true, j, fj & 0x3fffffff,
+                                      sgr[wv][x], smk[wv][x], (fj >> 30) & 1)) {
+                        const int ib = hbody[wv][hh];
+                        int k = atomicAdd(&hcnt[wv][hh], 1);
+                        if (k < BP_SLOTS) {
+                            d.bpSlots[(size_t)ib * BP_SLOTS + k] = j;
+                        } else {
+                            int sp = atomicAdd(&d.counters[CNT_SPILL], 1);
+                            if (sp < d.spillCap) { d.spillA[sp] = ib; d.spillB[sp] = j; d.spillK[sp] = k; }
+                            else atomicOr(&d.counters[CNT_OVERFLOW], 8);
+                        }
+                    }
+                    x += 64;
+                    while (x >= nc) { x -= nc; hh++; }
+                }
+                wave_sync();
             }
+            if (lane < nq) d.bpCount[hbody[wv][lane]] = hcnt[wv][lane];
             wave_sync();
         }
     }

@@ -591,8 +591,8 @@ static void launch_broadphase(CtxT<T>* X, hipStream_t st, double margin) {
     scan_launch(st, d.cellCount, d.cellStart, d.cellTabCap, X->scanSums);
     scan_launch(st, d.gLargeFlag, d.gLargeOff, N, X->scanSums);
     hipLaunchKernelGGL(k_cells_scatter<T>, dim3(nbB), dim3(256), 0, st, d);
-    hipLaunchKernelGGL(k_bp_cells<T>, dim3(grid_for((N + 3) / 4, 1, 8192)), dim3(256), 0, st, d, margin);
     hipLaunchKernelGGL(k_bp_large<T>, dim3(nbB), dim3(256), 0, st, d, margin);
+    hipLaunchKernelGGL(k_bp_cells<T>, dim3(grid_for((N + 3) / 4, 1, 8192)), dim3(256), 0, st, d, margin);
     scan_launch(st, d.bpCount, d.bpOff, N, X->scanSums);
     hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
     hipLaunchKernelGGL(k_bp_compact<T>, dim3(nbB), dim3(256), 0, st, d);
