@@ -25,9 +25,12 @@ struct Dev {
     int* cellFill;      // [cellTabCap]
     int* cellList;      // [N] occupied slots (CNT_CELLS of them)
     int* bodySlot;      // [N] table slot of a small body, -1 for a large one
-    int* cellBody;      // [N] small bodies grouped by cell, with their bounds
-    V3<T>* cellMin;
+    int* cellBody;      // [N] small bodies grouped by cell, with their bounds and the words of
+    V3<T>* cellMin;     //     the pair predicate (flags with bit 30 = querying, groups, mask)
     V3<T>* cellMax;
+    int* cellFl;
+    unsigned* cellGr;
+    unsigned* cellMk;
     int* gLargeFlag;    // [N + 1]
     int* gLargeOff;     // [N + 1] exclusive scan of gLargeFlag (bodies are world-ordered)
     int* gLarge;        // [N] large bodies, grouped by world
@@ -99,6 +102,8 @@ struct Dev {
     RowSoA<T> rN, rT1, rT2;  // [4 * svCap]
     V3<T>* svLA;        // [4 * svCap] contact points in body space
     V3<T>* svLB;
+    T* vw;              // [8 * N] linear and angular velocity of each body packed in one record,
+                        // used by the solver's sweeps so that a body costs one memory line
     int* isl0;          // [N] island flags (by label)
     int* isl1;          // [N]
     int* stats;         // per world: [levels, posSweeps]
@@ -270,6 +275,9 @@ __global__ void k_cells_scatter(Dev<T> d) {
     d.cellBody[p] = i;
     d.cellMin[p] = d.B.amin[i];
     d.cellMax[p] = d.B.amax[i];
+    d.cellFl[p] = d.B.flags[i] | (queries(d.B, i) ? (1 << 30) : 0);
+    d.cellGr[p] = d.B.groups[i];
+    d.cellMk[p] = d.B.mask[i];
     if (k == 0) d.cellList[atomicAdd(&d.counters[CNT_CELLS], 1)] = h;
 }
 
@@ -344,11 +352,12 @@ __global__ void __launch_bounds__(256) k_bp_cells(Dev<T> d, double margin) {
         const int total = __shfl(pre, 26, 64);
         for (int hb0 = 0; hb0 < nh; hb0 += BP_HOME) {
             // stage this batch's querying home bodies, compacted
-            int i = -1;
+            int i = -1, fl = 0;
             bool q = false;
             if (hb0 + lane < nh) {
                 i = d.cellBody[h0 + hb0 + lane];
-                q = queries(d.B, i);
+                fl = d.cellFl[h0 + hb0 + lane];
+                q = (fl >> 30) & 1;
             }
             unsigned long long qm = __ballot(q);
             const int nq = __popcll(qm);
@@ -357,9 +366,9 @@ __global__ void __launch_bounds__(256) k_bp_cells(Dev<T> d, double margin) {
                 hbody[wv][r] = i;
                 hmn[wv][r] = cvt<double>(d.cellMin[h0 + hb0 + lane]) - mk3<double>(margin, margin, margin);
                 hmx[wv][r] = cvt<double>(d.cellMax[h0 + hb0 + lane]) + mk3<double>(margin, margin, margin);
-                hfl[wv][r] = d.B.flags[i];
-                hgr[wv][r] = d.B.groups[i];
-                hmk[wv][r] = d.B.mask[i];
+                hfl[wv][r] = fl & 0x3fffffff;
+                hgr[wv][r] = d.cellGr[h0 + hb0 + lane];
+                hmk[wv][r] = d.cellMk[h0 + hb0 + lane];
                 hcnt[wv][r] = d.bpCount[i];
             }
             wave_sync();
@@ -374,14 +383,12 @@ __global__ void __launch_bounds__(256) k_bp_cells(Dev<T> d, double margin) {
                         if (nPre[wv][mid] <= f) lo = mid; else hi = mid - 1;
                     }
                     int p = nStart[wv][lo] + (f - nPre[wv][lo]);
-                    int j = d.cellBody[p];
-                    sbody[wv][x] = j;
+                    sbody[wv][x] = d.cellBody[p];
                     smn[wv][x] = d.cellMin[p];
                     smx[wv][x] = d.cellMax[p];
-                    // a candidate that queries sets bit 30 of its staged flags (flags use bits 0-23)
-                    sfl[wv][x] = d.B.flags[j] | (queries(d.B, j) ? (1 << 30) : 0);
-                    sgr[wv][x] = d.B.groups[j];
-                    smk[wv][x] = d.B.mask[j];
+                    sfl[wv][x] = d.cellFl[p];
+                    sgr[wv][x] = d.cellGr[p];
+                    smk[wv][x] = d.cellMk[p];
                 }
                 wave_sync();
                 const int work = nq * nc;

@@ -362,9 +362,18 @@ __device__ inline void gjk_closest_points(GjkOut<T>& out, const Sup<T>& A, const
             out.pointA = out.pointB = mk3<T>(0, 0, 0);
             return;
         }
-        s.pt[s.size].y = w;
-        s.pt[s.size].p = p;
-        s.pt[s.size].q = q;
+        {  // append with constant indices only, so the simplex stays in registers
+            SPt<T> np_;
+            np_.y = w;
+            np_.p = p;
+            np_.q = q;
+            switch (s.size) {
+                case 0: s.pt[0] = np_; break;
+                case 1: s.pt[1] = np_; break;
+                case 2: s.pt[2] = np_; break;
+                default: s.pt[3] = np_; break;
+            }
+        }
         s.size++;
         CPRes<T> cp;
         switch (s.size) {
@@ -379,17 +388,23 @@ __device__ inline void gjk_closest_points(GjkOut<T>& out, const Sup<T>& A, const
         cd2 = d2;
         cset = cp.set;
         if (cset == 0xf) { cpt = mk3<T>(0, 0, 0); cd2 = T(0); break; }
-        int ns = 0;
-        for (int i = 0; i < s.size; i++) {
-            if (cset & (1 << i)) {
-                if (ns != i) s.pt[ns] = s.pt[i];
-                ns++;
-            }
+        {  // keep the points named by cset, in order (slot k <- k-th set bit), by selects
+            const int keep = cset & ((1 << s.size) - 1);
+            const SPt<T> t0 = s.pt[0], t1 = s.pt[1], t2 = s.pt[2], t3 = s.pt[3];
+            auto pick = [&](int c) -> SPt<T> { return (c & 1) ? t0 : ((c & 2) ? t1 : ((c & 4) ? t2 : t3)); };
+            const int c1 = keep & (keep - 1), c2 = c1 & (c1 - 1), c3 = c2 & (c2 - 1);
+            s.pt[0] = pick(keep);
+            s.pt[1] = pick(c1);
+            s.pt[2] = pick(c2);
+            s.pt[3] = pick(c3);
+            s.size = __popc((unsigned)keep);
         }
-        s.size = ns;
         if (cd2 <= tol2) { cpt = mk3<T>(0, 0, 0); cd2 = T(0); break; }
         T ymax = T(0);
-        for (int i = 0; i < s.size; i++) ymax = tmax<T>(ymax, len2(s.pt[i].y));
+        if (s.size > 0) ymax = tmax<T>(ymax, len2(s.pt[0].y));
+        if (s.size > 1) ymax = tmax<T>(ymax, len2(s.pt[1].y));
+        if (s.size > 2) ymax = tmax<T>(ymax, len2(s.pt[2].y));
+        if (s.size > 3) ymax = tmax<T>(ymax, len2(s.pt[3].y));
         if (cd2 <= GJK_TOL * ymax) { cpt = mk3<T>(0, 0, 0); cd2 = T(0); break; }
         cpt = -cpt;
         if (prevDist2 - cd2 <= GJK_TOL * prevDist2) break;

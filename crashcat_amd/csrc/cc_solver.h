@@ -64,6 +64,19 @@ __device__ inline int block_scan_array(int* arr, int n, int* sh) {
     return carry;
 }
 
+// packed velocity record of body i: (vx, vy, vz, wx, wy, wz, -, -)
+template <typename T>
+__device__ inline void vw_load(const Dev<T>& d, int i, V3<T>& v, V3<T>& w) {
+    const T* p = d.vw + 8 * (size_t)i;
+    v = mk3<T>(p[0], p[1], p[2]);
+    w = mk3<T>(p[3], p[4], p[5]);
+}
+template <typename T>
+__device__ inline void vw_store(const Dev<T>& d, int i, V3<T> v, V3<T> w) {
+    T* p = d.vw + 8 * (size_t)i;
+    p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = w.x; p[4] = w.y; p[5] = w.z;
+}
+
 // --------------------------------------------------------------- contact rows
 template <typename T>
 __device__ inline void solve_velocity_contact(const Dev<T>& d, int slot) {  // :963-1053 for one constraint
@@ -172,7 +185,9 @@ __device__ inline void solve_velocity_rows(const Dev<T>& d, int e) {  // :963-10
     if (!dyn) return;
     const Bodies<T>& B = d.B;
     const int a = d.svA[e], b = d.svB[e];
-    V3<T> vA = B.v[a], wA = B.w[a], vB = B.v[b], wB = B.w[b];
+    V3<T> vA, wA, vB, wB;
+    vw_load(d, a, vA, wA);
+    vw_load(d, b, vB, wB);
     const int mov = (meta >> 2) & 3, np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
     const T iA = d.svIMA[e], iB = d.svIMB[e], fr = d.svFr[e];
     const V3<T> n = d.svN[e], t1 = d.svT1[e], t2 = d.svT2[e];
@@ -199,8 +214,8 @@ __device__ inline void solve_velocity_rows(const Dev<T>& d, int e) {  // :963-10
         axis_apply(rn, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, n, tmax<T>(T(0), tl));
         d.rN.lambda[k] = rn.lambda;
     }
-    if (dyn & 1) { B.v[a] = vA; B.w[a] = wA; }
-    if (dyn & 2) { B.v[b] = vB; B.w[b] = wB; }
+    if (dyn & 1) vw_store(d, a, vA, wA);
+    if (dyn & 2) vw_store(d, b, vB, wB);
 }
 
 template <typename T>
@@ -209,7 +224,9 @@ __device__ inline void warm_start_rows(const Dev<T>& d, int e, T ratio) {  // :9
     const int dyn = meta & 3;
     const Bodies<T>& B = d.B;
     const int a = d.svA[e], b = d.svB[e];
-    V3<T> vA = B.v[a], wA = B.w[a], vB = B.v[b], wB = B.w[b];
+    V3<T> vA, wA, vB, wB;
+    vw_load(d, a, vA, wA);
+    vw_load(d, b, vB, wB);
     const int np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
     const T iA = d.svIMA[e], iB = d.svIMB[e];
     const V3<T> n = d.svN[e], t1 = d.svT1[e], t2 = d.svT2[e];
@@ -226,8 +243,8 @@ __device__ inline void warm_start_rows(const Dev<T>& d, int e, T ratio) {  // :9
         axis_warm(rn, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, n, ratio);
         d.rN.lambda[k] = rn.lambda;
     }
-    if (dyn & 1) { B.v[a] = vA; B.w[a] = wA; }
-    if (dyn & 2) { B.v[b] = vB; B.w[b] = wB; }
+    if (dyn & 1) vw_store(d, a, vA, wA);
+    if (dyn & 2) vw_store(d, b, vB, wB);
 }
 
 template <typename T>
@@ -344,6 +361,34 @@ __device__ inline void point_apply(const Dev<T>& d, PointCon<T>& p, V3<T> lam) {
     }
 }
 
+// the same two on the packed velocity records
+template <typename T>
+__device__ inline void point_apply_vw(const Dev<T>& d, PointCon<T>& p, V3<T> lam) {
+    if (len2(lam) == T(0)) return;
+    const Bodies<T>& B = d.B;
+    V3<T> v, w;
+    if (fl_motion(B.flags[p.a]) == MOTION_DYNAMIC) {
+        vw_load(d, p.a, v, w);
+        vw_store(d, p.a, v + lam * (-B.invMass[p.a]), w - mul(p.i1X, lam));
+    }
+    if (fl_motion(B.flags[p.b]) == MOTION_DYNAMIC) {
+        vw_load(d, p.b, v, w);
+        vw_store(d, p.b, v + lam * B.invMass[p.b], w + mul(p.i2X, lam));
+    }
+}
+template <typename T>
+__device__ inline void point_solve_velocity_vw(const Dev<T>& d, PointCon<T>& p) {
+    if (!p.active) return;
+    V3<T> va, wa, vb, wb;
+    vw_load(d, p.a, va, wa);
+    vw_load(d, p.b, vb, wb);
+    V3<T> jv = va - cross(p.r1, wa) - vb;
+    jv = jv + cross(p.r2, wb);
+    V3<T> lam = mul(p.eff, jv);
+    p.lambda = p.lambda + lam;
+    point_apply_vw(d, p, lam);
+}
+
 template <typename T>
 __device__ inline void point_solve_velocity(const Dev<T>& d, PointCon<T>& p) {  // :186-221
     if (!p.active) return;
@@ -408,7 +453,8 @@ __device__ inline void warm_start_levels(const Dev<T>& d, int so, int D, const i
                 PointCon<T>& pc = d.pc[-1 - s];
                 if (pc.enabled && !pc.sleeping && pc.active) {
                     pc.lambda = pc.lambda * ratio;
-                    point_apply(d, pc, pc.lambda);
+                    if (ROWS) point_apply_vw(d, pc, pc.lambda);
+                    else point_apply(d, pc, pc.lambda);
                 }
             }
         }
@@ -617,6 +663,8 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         }
         __syncthreads();
     }
+    for (int i = b0 + tid; i < b1; i += BS) vw_store(d, i, B.v[i], B.w[i]);
+    __syncthreads();
     if (!kCreationOrderWarmStart && d.s.warmStarting) warm_start_levels<T, BS, true>(d, so, D, LS, ratio);
     // ---- 6. velocity iterations
     for (int it = 0; it < d.s.velIters; it++) {
@@ -625,11 +673,18 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
                 int s = d.svSlot[so + x];
                 if (s >= 0x40000000) continue;
                 if (s >= 0) solve_velocity_rows(d, so + x);
-                else point_solve_velocity(d, d.pc[-1 - s]);
+                else point_solve_velocity_vw(d, d.pc[-1 - s]);
             }
             __syncthreads();
         }
     }
+    for (int i = b0 + tid; i < b1; i += BS) {
+        V3<T> v, w;
+        vw_load(d, i, v, w);
+        B.v[i] = v;
+        B.w[i] = w;
+    }
+    __syncthreads();
     // ---- 7. store impulses into this step's cache records (storeAppliedImpulses :1061-1076)
     for (int x = tid; x < L; x += BS) {
         const int e = so + x;

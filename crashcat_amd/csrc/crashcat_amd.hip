@@ -443,6 +443,9 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.cellBody = A.get<int>(nb);
     d.cellMin = A.get<V3<T>>(nb);
     d.cellMax = A.get<V3<T>>(nb);
+    d.cellFl = A.get<int>(nb);
+    d.cellGr = A.get<unsigned>(nb);
+    d.cellMk = A.get<unsigned>(nb);
     d.gLargeFlag = A.get<int>(nb + 1);
     d.gLargeOff = A.get<int>(nb + 1);
     d.gLarge = A.get<int>(nb);
@@ -511,6 +514,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         R->lambda = A.get<T>(4 * L);
         R->bias = A.get<T>(4 * L);
     }
+    d.vw = A.get<T>(8 * (size_t)std::max(nb, 1));
     d.svLA = A.get<V3<T>>(4 * L);
     d.svLB = A.get<V3<T>>(4 * L);
     d.bcnt = A.get<int>(nb);
