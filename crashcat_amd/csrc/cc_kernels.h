@@ -87,12 +87,13 @@ struct Dev {
     int* bcnt;          // [N]
     int* boff;          // [N]
     // the world's contact rows copied in level order for the sweeps (element so + x; point j of
-    // element e at j * svCap + e), so a level's rows are read with coalesced loads
+    // element e at svPt[e] + j), so a level's rows are read with coalesced loads
     int svCap;          // conCap + NPC
     int* svSlot;        // constraint slot (or -1 - point constraint, or 0x40000000 inactive)
     int* svA;
     int* svB;
     int* svMeta;        // dynMask | movMask << 2 | np << 4 | dofA << 8 | dofB << 14
+    int* svPt;          // index of the element's first point row (points of a world are dense)
     T* svIMA;
     T* svIMB;
     T* svFr;

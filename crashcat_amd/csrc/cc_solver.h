@@ -191,9 +191,9 @@ __device__ inline void solve_velocity_rows(const Dev<T>& d, int e) {  // :963-10
     const int mov = (meta >> 2) & 3, np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
     const T iA = d.svIMA[e], iB = d.svIMB[e], fr = d.svFr[e];
     const V3<T> n = d.svN[e], t1 = d.svT1[e], t2 = d.svT2[e];
-    const size_t cap = (size_t)d.svCap;
+    const size_t pb = (size_t)d.svPt[e];
     for (int j = 0; j < np; j++) {
-        const size_t k = j * cap + e;
+        const size_t k = pb + j;
         Row<T> r1 = row_load(d.rT1, k), r2 = row_load(d.rT2, k);
         if (r1.eff != T(0) || r2.eff != T(0)) {
             T l1 = axis_total_lambda(r1, mov, vA, wA, vB, wB, t1);
@@ -208,7 +208,7 @@ __device__ inline void solve_velocity_rows(const Dev<T>& d, int e) {  // :963-10
         }
     }
     for (int j = 0; j < np; j++) {
-        const size_t k = j * cap + e;
+        const size_t k = pb + j;
         Row<T> rn = row_load(d.rN, k);
         T tl = axis_total_lambda(rn, mov, vA, wA, vB, wB, n);
         axis_apply(rn, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, n, tmax<T>(T(0), tl));
@@ -230,9 +230,9 @@ __device__ inline void warm_start_rows(const Dev<T>& d, int e, T ratio) {  // :9
     const int np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
     const T iA = d.svIMA[e], iB = d.svIMB[e];
     const V3<T> n = d.svN[e], t1 = d.svT1[e], t2 = d.svT2[e];
-    const size_t cap = (size_t)d.svCap;
+    const size_t pb = (size_t)d.svPt[e];
     for (int j = 0; j < np; j++) {
-        const size_t k = j * cap + e;
+        const size_t k = pb + j;
         Row<T> r1 = row_load(d.rT1, k), r2 = row_load(d.rT2, k), rn = row_load(d.rN, k);
         if (r1.eff != T(0) || r2.eff != T(0)) {
             axis_warm(r1, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, t1, ratio);
@@ -265,9 +265,9 @@ __device__ inline bool solve_position_rows(const Dev<T>& d, int e) {  // :1098-1
     if (dyn & 2) IB = inv_inertia(B.invI[b], B.iRot[b], qB, dofB);
     bool any = false;
     const V3<T> n = d.svN[e];
-    const size_t cap = (size_t)d.svCap;
+    const size_t pb = (size_t)d.svPt[e];
     for (int j = 0; j < np; j++) {
-        const size_t k = j * cap + e;
+        const size_t k = pb + j;
         V3<T> pA = cA + mul(RA, d.svLA[k]);
         V3<T> pB = cB + mul(RB, d.svLB[k]);
         T sep = dot(pB - pA, n) + d.s.slop;
@@ -634,13 +634,21 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     link_body_lists<T, BS>(d, b0, nb, so, BL, SolveLess{K});
     D = build_levels<T, BS>(d, so, L, LS, sTail);
     if (tid == 0) d.stats[2 * w] = D;
-    // ---- stage the contact rows in level order
+    // ---- stage the contact rows in level order; a constraint's points follow each other
+    // (point rows of element e at svPt[e] + j, dense over the level order)
     {
-        const size_t cap = (size_t)d.svCap;
+        for (int x = tid; x < L; x += BS) {
+            const int s = d.seq[so + d.sched[so + x]];
+            d.svPt[so + x] = (s >= 0 && s < 0x40000000) ? d.con[s].np : 0;
+        }
+        __syncthreads();
+        block_scan_array<BS>(d.svPt + so, L, sh);
         for (int x = tid; x < L; x += BS) {
             const int e = so + x;
             const int s = d.seq[so + d.sched[so + x]];
             d.svSlot[e] = s;
+            const int pbase = 4 * so + d.svPt[e];
+            d.svPt[e] = pbase;
             if (s < 0 || s >= 0x40000000) continue;
             const Con<T>& k = d.con[s];
             d.svA[e] = k.a;
@@ -653,7 +661,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
             d.svT1[e] = k.t1;
             d.svT2[e] = k.t2;
             for (int j = 0; j < k.np; j++) {
-                const size_t q = j * cap + e;
+                const size_t q = (size_t)pbase + j;
                 row_store(d.rN, q, k.p[j].n);
                 row_store(d.rT1, q, k.p[j].t1);
                 row_store(d.rT2, q, k.p[j].t2);
@@ -693,7 +701,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         CacheRec<T>& r = d.rec[par][s];
         const int np = (d.svMeta[e] >> 4) & 7;
         for (int j = 0; j < np; j++) {
-            const size_t q = j * (size_t)d.svCap + e;
+            const size_t q = (size_t)d.svPt[e] + j;
             r.ln[j] = d.rN.lambda[q];
             r.lt1[j] = d.rT1.lambda[q];
             r.lt2[j] = d.rT2.lambda[q];

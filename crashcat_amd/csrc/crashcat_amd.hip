@@ -426,7 +426,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.pairCap = (int)pairCap;
     d.conCap = (int)std::min<long long>((long long)c->cfg.max_contacts_per_body * std::max(nb, 1), pairCap);
     d.epaCap = std::max(4096, d.pairCap / 16);  // EPA jobs: pairs whose cores overlap
-    d.epaWorkers = std::min(d.epaCap, 65536);
+    d.epaWorkers = std::min(d.epaCap, 131072);
     d.hashCap = next_pow2(2ll * d.conCap);
     int maxWorld = 0;
     for (int w = 0; w < nw; w++) maxWorld = std::max(maxWorld, woff[w + 1] - woff[w]);
@@ -499,6 +499,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.svA = A.get<int>(L);
     d.svB = A.get<int>(L);
     d.svMeta = A.get<int>(L);
+    d.svPt = A.get<int>(L);
     d.svIMA = A.get<T>(L);
     d.svIMB = A.get<T>(L);
     d.svFr = A.get<T>(L);
