@@ -6,6 +6,14 @@
 
 namespace cc {
 
+// what the narrowphase reads of a body, in one 64-byte record (one memory line)
+template <typename T>
+struct alignas(64) NpBody {
+    V3<T> pos;
+    Q4<T> q;
+    int shape, flags;
+};
+
 template <typename T>
 struct Dev {
     SettingsD<T> s;
@@ -34,6 +42,7 @@ struct Dev {
     int* gLargeFlag;    // [N + 1]
     int* gLargeOff;     // [N + 1] exclusive scan of gLargeFlag (bodies are world-ordered)
     int* gLarge;        // [N] large bodies, grouped by world
+    NpBody<T>* npb;     // [N] pose, shape and flags of each body in one record, for the narrowphase
     int* bpCount;       // [N+1] pairs found by each querying body
     int* bpOff;         // [N+1] exclusive scan of bpCount: pairs are stored in body order
     int* bpSlots;       // [N * BP_SLOTS] first BP_SLOTS partners of each querying body
@@ -86,25 +95,17 @@ struct Dev {
     int* blist;         // [2 * (conCap + NPC)]
     int* bcnt;          // [N]
     int* boff;          // [N]
-    // the world's contact rows copied in level order for the sweeps (element so + x; point j of
-    // element e at svPt[e] + j), so a level's rows are read with coalesced loads
+    // the world's constraints copied in level order for the sweeps (element so + x; its points
+    // at ve[e].pt + j, dense over the level order), so a level is read with coalesced loads
     int svCap;          // conCap + NPC
     int* svSlot;        // constraint slot (or -1 - point constraint, or 0x40000000 inactive)
-    int* svA;
-    int* svB;
-    int* svMeta;        // dynMask | movMask << 2 | np << 4 | dofA << 8 | dofB << 14
-    int* svPt;          // index of the element's first point row (points of a world are dense)
-    T* svIMA;
-    T* svIMB;
-    T* svFr;
-    V3<T>* svN;
-    V3<T>* svT1;
-    V3<T>* svT2;
-    RowSoA<T> rN, rT1, rT2;  // [4 * svCap]
+    VElem<T>* ve;       // [svCap]
+    VPoint<T>* vp;      // [4 * svCap]
+    int* svPt;          // [svCap] point offsets during staging
     V3<T>* svLA;        // [4 * svCap] contact points in body space
     V3<T>* svLB;
-    T* vw;              // [8 * N] linear and angular velocity of each body packed in one record,
-                        // used by the solver's sweeps so that a body costs one memory line
+    VBody<T>* vb;       // [N] velocities, inverse inertia and inverse mass for the velocity sweeps
+    PBody<T>* pb;       // [N] poses for the position sweeps, one record per body
     int* isl0;          // [N] island flags (by label)
     int* isl1;          // [N]
     int* stats;         // per world: [levels, posSweeps]
@@ -150,6 +151,14 @@ __global__ void k_accel(Dev<T> d, T dt) {
     if (i >= d.N) return;
     const Bodies<T>& B = d.B;
     int f = B.flags[i];
+    {  // poses are final for this step's collision detection
+        NpBody<T> r;
+        r.pos = B.pos[i];
+        r.q = B.q[i];
+        r.shape = B.shape[i];
+        r.flags = f;
+        d.npb[i] = r;
+    }
     if (fl_motion(f) != MOTION_DYNAMIC || B.sleeping[i]) return;
     T gs = d.s.gravityEnabled ? B.gravityFactor[i] : T(0);
     V3<T> v = B.v[i];
@@ -476,6 +485,15 @@ struct PairBodies {
 };
 
 template <typename T>
+__device__ inline PairBodies<T> order_pair_f(int a, int fa, int b, int fb) {
+    int ma = fl_motion(fa), mb = fl_motion(fb);
+    PairBodies<T> r;
+    if (ma > mb || (ma == mb && b < a)) { r.a = b; r.b = a; }
+    else { r.a = a; r.b = b; }
+    return r;
+}
+
+template <typename T>
 __device__ inline PairBodies<T> order_pair(const Dev<T>& d, int a, int b) {
     int ma = fl_motion(d.B.flags[a]), mb = fl_motion(d.B.flags[b]);
     PairBodies<T> r;
@@ -516,10 +534,11 @@ __device__ inline T normal_bias(const Dev<T>& d, int A, int Bi, V3<T> pA, V3<T> 
 // Finish one hit: manifold (update.ts:385-563 for the single hit of a convex pair), then
 // addContactConstraint (contact-constraints.ts:437-857) into the pair slot.
 template <typename T>
-__device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<T> pointA, V3<T> pointB, V3<T> axis,
-                                   const Face<T>& fA, const Face<T>& fB, Face<T>* scratch3, Manifold<T>& m, T dt, int par) {
+__device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, const NpBody<T>& RA, const NpBody<T>& RB,
+                                   V3<T> pointA, V3<T> pointB, V3<T> axis, const Face<T>& fA, const Face<T>& fB,
+                                   Face<T>* scratch3, Manifold<T>& m, T dt, int par) {
     const Bodies<T>& B = d.B;
-    m.base = B.pos[A];
+    m.base = RA.pos;
     m.normal = normalize(axis);
     m.n = 0;
     T maxDist = d.s.spec + d.s.manifoldTol;
@@ -533,7 +552,9 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<
     V3<T> n = m.normal;
     bool sw = ia > ib;
     if (sw) { int t = ia; ia = ib; ib = t; n = -n; }
-    int fa = B.flags[ia], fb = B.flags[ib];
+    const NpBody<T>& Ria = sw ? RB : RA;
+    const NpBody<T>& Rib = sw ? RA : RB;
+    int fa = Ria.flags, fb = Rib.flags;
     bool dynA = fl_motion(fa) == MOTION_DYNAMIC, dynB = fl_motion(fb) == MOTION_DYNAMIC;
     if (!((dynA && B.invMass[ia] != T(0)) || (dynB && B.invMass[ib] != T(0)))) return;
     int cs = wave_alloc(&d.counters[CNT_CONS]);
@@ -562,7 +583,7 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<
     }
     rc.a = ia; rc.b = ib;
     rc.np = m.n;
-    Q4<T> qa = B.q[ia], qb = B.q[ib];
+    Q4<T> qa = Ria.q, qb = Rib.q;
     M3<T> rotA = m3_from_quat(qa), rotB = m3_from_quat(qb);
     T fr = combine(B.friction[ia], B.friction[ib], fl_fmode(fa), fl_fmode(fb));
     T re = combine(B.restitution[ia], B.restitution[ib], fl_rmode(fa), fl_rmode(fb));
@@ -573,8 +594,7 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<
     k.movMask = (fl_motion(fa) != MOTION_STATIC ? 1 : 0) | (fl_motion(fb) != MOTION_STATIC ? 2 : 0);
     k.sortKey = sort_key((unsigned)ia, (unsigned)ib, 0xffffffffu, 0xffffffffu);
     k.normal = n;
-    if (fabs(n.x) > fabs(n.y)) { T tl = sqrt(n.x * n.x + n.z * n.z); k.t1 = mk3<T>(n.z / tl, 0, -n.x / tl); }
-    else { T tl = sqrt(n.y * n.y + n.z * n.z); k.t1 = mk3<T>(0, n.z / tl, -n.y / tl); }
+    k.t1 = contact_t1(n);
     k.t2 = cross(n, k.t1);
     k.friction = fr;
     k.invMassA = B.invMass[ia];
@@ -599,31 +619,36 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, V3<
             for (int j = 0; j < rc.np; j++) {
                 T d1 = len2(cp.localA - rc.p1[j]), d2 = len2(cp.localB - rc.p2[j]);
                 if (d1 < d.s.preserveDist2 && d2 < d.s.preserveDist2) {
-                    cp.n.lambda = rc.ln[j]; cp.t1.lambda = rc.lt1[j]; cp.t2.lambda = rc.lt2[j];
+                    cp.lN = rc.ln[j]; cp.lT1 = rc.lt1[j]; cp.lT2 = rc.lt2[j];
                     set = true;
                     break;
                 }
             }
         }
-        if (!set) { cp.n.lambda = cp.t1.lambda = cp.t2.lambda = T(0); }
+        if (!set) { cp.lN = cp.lT1 = cp.lT2 = T(0); }
         V3<T> mid = mk3<T>((pA.x + pB.x) * T(0.5), (pA.y + pB.y) * T(0.5), (pA.z + pB.z) * T(0.5));
         V3<T> rA = mid - comA, rB = mid - comB;
         T bias = normal_bias(d, ia, ib, pA, pB, n, rA, rB, re, dt);
-        axis_props(cp.n, dynA, dynB, k.invMassA, k.invMassB, IA, IB, rA, rB, n, bias);
+        cp.r1 = rA;
+        cp.r2 = rB;
+        // an axis without mass drops its impulse and bias (axis-constraint-part.ts:87-201)
+        cp.effN = axis_eff(dynA, dynB, k.invMassA, k.invMassB, IA, IB, rA, rB, n);
+        cp.biasN = cp.effN == T(0) ? T(0) : bias;
+        if (cp.effN == T(0)) cp.lN = T(0);
         if (fr > T(0)) {
-            axis_props(cp.t1, dynA, dynB, k.invMassA, k.invMassB, IA, IB, rA, rB, k.t1, T(0));
-            axis_props(cp.t2, dynA, dynB, k.invMassA, k.invMassB, IA, IB, rA, rB, k.t2, T(0));
+            cp.effT1 = axis_eff(dynA, dynB, k.invMassA, k.invMassB, IA, IB, rA, rB, k.t1);
+            cp.effT2 = axis_eff(dynA, dynB, k.invMassA, k.invMassB, IA, IB, rA, rB, k.t2);
+            if (cp.effT1 == T(0)) cp.lT1 = T(0);
+            if (cp.effT2 == T(0)) cp.lT2 = T(0);
         } else {
-            cp.t1.eff = T(0); cp.t1.lambda = T(0); cp.t1.bias = T(0);
-            cp.t2.eff = T(0); cp.t2.lambda = T(0); cp.t2.bias = T(0);
-            cp.t1.r1x = cp.t1.r2x = cp.t1.i1 = cp.t1.i2 = mk3<T>(0, 0, 0);
-            cp.t2.r1x = cp.t2.r2x = cp.t2.i1 = cp.t2.i2 = mk3<T>(0, 0, 0);
+            cp.effT1 = cp.effT2 = T(0);
+            cp.lT1 = cp.lT2 = T(0);
         }
         rc.p1[i] = cp.localA;
         rc.p2[i] = cp.localB;
-        rc.ln[i] = cp.n.lambda;
-        rc.lt1[i] = cp.t1.lambda;
-        rc.lt2[i] = cp.t2.lambda;
+        rc.ln[i] = cp.lN;
+        rc.lt1[i] = cp.lT1;
+        rc.lt2[i] = cp.lT2;
     }
     if (dynA && B.sleeping[ia]) B.wake[ia] = 1;
     if (dynB && B.sleeping[ib]) B.wake[ib] = 1;
@@ -637,11 +662,11 @@ struct NpScratch {
 
 // world-space hit + supporting faces after GJK/EPA (convex.ts:489-637)
 template <typename T>
-__device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, const ConvexCtx<T>& c, PenDepth<T> pd,
-                                     NpScratch<T>& S, T dt, int par) {
-    const Bodies<T>& B = d.B;
-    V3<T> pa = B.pos[A];
-    Q4<T> qa = B.q[A];
+__device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, const NpBody<T>& RA,
+                                     const NpBody<T>& RB, const ConvexCtx<T>& c, PenDepth<T> pd, NpScratch<T>& S, T dt,
+                                     int par) {
+    V3<T> pa = RA.pos;
+    Q4<T> qa = RA.q;
     T ms = c.maxSepUse;
     // penetration = |pA - pB| - maxSep; the early-out fraction is +inf for the step collector
     T al = sqrt(len2(pd.axis));
@@ -649,7 +674,7 @@ __device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, c
     V3<T> wA = qrot2(qa, pd.pointA) + pa;
     V3<T> wB = qrot2(qa, pd.pointB) + pa;
     V3<T> wAxis = qrot2(qa, pd.axis);
-    supporting_face(S.fA, d.st, B.shape[A], -pd.axis, pa, qa);
+    supporting_face(S.fA, d.st, RA.shape, -pd.axis, pa, qa);
     Q4<T> crel = qconj(c.rel);
     V3<T> dirB = qrot2(crel, pd.axis);
     V3<T> posB = qrot2(qa, c.relPos) + pa;
@@ -658,8 +683,8 @@ __device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, c
     qB.y = qa.w * c.rel.y - qa.x * c.rel.z + qa.y * c.rel.w + qa.z * c.rel.x;
     qB.z = qa.w * c.rel.z + qa.x * c.rel.y - qa.y * c.rel.x + qa.z * c.rel.w;
     qB.w = qa.w * c.rel.w - qa.x * c.rel.x - qa.y * c.rel.y - qa.z * c.rel.z;
-    supporting_face(S.fB, d.st, B.shape[Bi], dirB, posB, qB);
-    finish_pair(d, slot, A, Bi, wA, wB, wAxis, S.fA, S.fB, S.tmp, S.m, dt, par);
+    supporting_face(S.fB, d.st, RB.shape, dirB, posB, qB);
+    finish_pair(d, slot, A, Bi, RA, RB, wA, wB, wAxis, S.fA, S.fB, S.tmp, S.m, dt, par);
 }
 
 enum { HIT_SPHERES = 0, HIT_GJK = 1, HIT_EPA = 2 };
@@ -680,17 +705,20 @@ __device__ inline void push_hit(const Dev<T>& d, int slot, int kind, V3<T> pa, V
 // to the EPA queue.  Pairs whose bounding spheres are further apart than the speculative
 // distance cannot be within it, and GJK would report them separated at once: they stop here.
 template <typename T>
-__global__ void __launch_bounds__(256) k_np_gjk(Dev<T> d, T dt) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2) k_np_gjk(Dev<T> d, T dt) {
     int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
     for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < npairs; slot += gridDim.x * blockDim.x) {
         d.pairCon[slot] = -1;
-        PairBodies<T> pb = order_pair(d, d.pairA[slot], d.pairB[slot]);
-        int A = pb.a, Bi = pb.b;
-        const Bodies<T>& B = d.B;
-        const ShapeD<T>& sa = d.st.shapes[B.shape[A]];
-        const ShapeD<T>& sb = d.st.shapes[B.shape[Bi]];
+        const int i0 = d.pairA[slot], i1 = d.pairB[slot];
+        const NpBody<T> r0 = d.npb[i0], r1 = d.npb[i1];
+        PairBodies<T> pb = order_pair_f<T>(i0, r0.flags, i1, r1.flags);
+        const bool sw = pb.a != i0;
+        const NpBody<T>& RA = sw ? r1 : r0;
+        const NpBody<T>& RB = sw ? r0 : r1;
+        const ShapeD<T>& sa = d.st.shapes[RA.shape];
+        const ShapeD<T>& sb = d.st.shapes[RB.shape];
         if (sa.type == SHAPE_SPHERE && sb.type == SHAPE_SPHERE) {  // collideSphereVsSphere sphere.ts:319-410
-            V3<T> pa = B.pos[A], pbp = B.pos[Bi];
+            V3<T> pa = RA.pos, pbp = RB.pos;
             T ra = sa.radius, rb = sb.radius;
             V3<T> dd = pbp - pa;
             T d2 = len2(dd), rs = ra + rb;
@@ -702,13 +730,13 @@ __global__ void __launch_bounds__(256) k_np_gjk(Dev<T> d, T dt) {
         }
         {
             T reach = sa.brad + sb.brad + d.s.spec;
-            if (len2(B.pos[Bi] - B.pos[A]) > reach * reach) continue;
+            if (len2(RB.pos - RA.pos) > reach * reach) continue;
         }
         ConvexCtx<T> c;
-        convex_setup(c, B.pos[A], B.q[A], B.pos[Bi], B.q[Bi], d.s.spec);
+        convex_setup(c, RA.pos, RA.q, RB.pos, RB.q, d.s.spec);
         Sup<T> SA, SB;
-        sup_for_shape(SA, d.st, B.shape[A], false);
-        sup_for_shape(SB, d.st, B.shape[Bi], false);
+        sup_for_shape(SA, d.st, RA.shape, false);
+        sup_for_shape(SB, d.st, RB.shape, false);
         SB.transformed = true;
         SB.R = m3_from_quat(c.rel);
         SB.t = c.relPos;
@@ -736,15 +764,18 @@ __global__ void __launch_bounds__(64) k_epa(Dev<T> d, T dt) {
     if (worker >= d.epaWorkers) return;
     for (int j = worker; j < nj; j += gridDim.x * blockDim.x) {
         int slot = d.epaPair[j];
-        PairBodies<T> pb = order_pair(d, d.pairA[slot], d.pairB[slot]);
-        int A = pb.a, Bi = pb.b;
-        const Bodies<T>& B = d.B;
+        const int i0 = d.pairA[slot], i1 = d.pairB[slot];
+        const NpBody<T> r0 = d.npb[i0], r1 = d.npb[i1];
+        PairBodies<T> pb = order_pair_f<T>(i0, r0.flags, i1, r1.flags);
+        const bool sw = pb.a != i0;
+        const NpBody<T>& RA = sw ? r1 : r0;
+        const NpBody<T>& RB = sw ? r0 : r1;
         ConvexCtx<T> c;
-        convex_setup(c, B.pos[A], B.q[A], B.pos[Bi], B.q[Bi], d.s.spec);
+        convex_setup(c, RA.pos, RA.q, RB.pos, RB.q, d.s.spec);
         c.maxSepUse = tmin<T>(c.maxSepUse, T(1));
         Sup<T> SA, SB;
-        sup_for_shape(SA, d.st, B.shape[A], true);
-        sup_for_shape(SB, d.st, B.shape[Bi], true);
+        sup_for_shape(SA, d.st, RA.shape, true);
+        sup_for_shape(SB, d.st, RB.shape, true);
         SA.addRadius = true;
         SA.addR = c.maxSepUse;
         SB.transformed = true;
@@ -765,26 +796,30 @@ __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
     int par = *d.parity;
     for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
         int slot = d.hitSlot[h];
-        PairBodies<T> pb = order_pair(d, d.pairA[slot], d.pairB[slot]);
+        const int i0 = d.pairA[slot], i1 = d.pairB[slot];
+        const NpBody<T> r0 = d.npb[i0], r1 = d.npb[i1];
+        PairBodies<T> pb = order_pair_f<T>(i0, r0.flags, i1, r1.flags);
+        const bool sw = pb.a != i0;
+        const NpBody<T>& RA = sw ? r1 : r0;
+        const NpBody<T>& RB = sw ? r0 : r1;
         int A = pb.a, Bi = pb.b;
         int kind = d.hitKind[h];
         NpScratch<T> S;
         if (kind == HIT_SPHERES) {
             S.fA.n = 0;
             S.fB.n = 0;
-            finish_pair(d, slot, A, Bi, d.hitPA[h], d.hitPB[h], d.hitAxis[h], S.fA, S.fB, S.tmp, S.m, dt, par);
+            finish_pair(d, slot, A, Bi, RA, RB, d.hitPA[h], d.hitPB[h], d.hitAxis[h], S.fA, S.fB, S.tmp, S.m, dt, par);
             continue;
         }
-        const Bodies<T>& B = d.B;
         ConvexCtx<T> c;
-        convex_setup(c, B.pos[A], B.q[A], B.pos[Bi], B.q[Bi], d.s.spec);
+        convex_setup(c, RA.pos, RA.q, RB.pos, RB.q, d.s.spec);
         if (kind == HIT_EPA) c.maxSepUse = tmin<T>(c.maxSepUse, T(1));
         PenDepth<T> pd;
         pd.status = 0;
         pd.pointA = d.hitPA[h];
         pd.pointB = d.hitPB[h];
         pd.axis = d.hitAxis[h];
-        finish_convex(d, slot, A, Bi, c, pd, S, dt, par);
+        finish_convex(d, slot, A, Bi, RA, RB, c, pd, S, dt, par);
     }
 }
 

@@ -64,212 +64,191 @@ __device__ inline int block_scan_array(int* arr, int n, int* sh) {
     return carry;
 }
 
-// packed velocity record of body i: (vx, vy, vz, wx, wy, wz, -, -)
+// velocities of body i in its velocity record
 template <typename T>
 __device__ inline void vw_load(const Dev<T>& d, int i, V3<T>& v, V3<T>& w) {
-    const T* p = d.vw + 8 * (size_t)i;
-    v = mk3<T>(p[0], p[1], p[2]);
-    w = mk3<T>(p[3], p[4], p[5]);
+    const VBody<T>& r = d.vb[i];
+    v = r.v;
+    w = r.w;
 }
 template <typename T>
 __device__ inline void vw_store(const Dev<T>& d, int i, V3<T> v, V3<T> w) {
-    T* p = d.vw + 8 * (size_t)i;
-    p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = w.x; p[4] = w.y; p[5] = w.z;
+    VBody<T>& r = d.vb[i];
+    r.v = v;
+    r.w = w;
 }
 
-// --------------------------------------------------------------- contact rows
+// --------------------------------------------------------------- contact constraints
+// One contact constraint of the sweeps (solveVelocityConstraint contact-constraints.ts:963-1053):
+// friction of every point, then the normal of every point.  All loads are issued before the
+// first update, so a constraint costs two dependent memory round trips (its element record,
+// then bodies and points).  In the last sweep (rec set) the impulses go to this step's cache
+// record (storeAppliedImpulses :1061-1076) instead of the point records.
 template <typename T>
-__device__ inline void solve_velocity_contact(const Dev<T>& d, int slot) {  // :963-1053 for one constraint
-    Con<T>& k = d.con[slot];
-    int dyn = k.dynMask;
-    if (!dyn) return;
-    const Bodies<T>& B = d.B;
-    int a = k.a, b = k.b;
-    V3<T> vA = B.v[a], wA = B.w[a], vB = B.v[b], wB = B.w[b];
-    int dofA = fl_dof(B.flags[a]), dofB = fl_dof(B.flags[b]);
-    int mov = k.movMask;
-    T iA = k.invMassA, iB = k.invMassB;
-    int np = k.np;
-    for (int j = 0; j < np; j++) {
-        ConPoint<T>& cp = k.p[j];
-        if (cp.t1.eff != T(0) || cp.t2.eff != T(0)) {
-            T l1 = axis_total_lambda(cp.t1, mov, vA, wA, vB, wB, k.t1);
-            T l2 = axis_total_lambda(cp.t2, mov, vA, wA, vB, wB, k.t2);
-            T mf = k.friction * cp.n.lambda;
-            T fm2 = l1 * l1 + l2 * l2;
-            if (fm2 > mf * mf) { T sc = mf / sqrt(fm2); l1 *= sc; l2 *= sc; }
-            axis_apply(cp.t1, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, k.t1, l1);
-            axis_apply(cp.t2, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, k.t2, l2);
-        }
-    }
-    for (int j = 0; j < np; j++) {
-        ConPoint<T>& cp = k.p[j];
-        T tl = axis_total_lambda(cp.n, mov, vA, wA, vB, wB, k.normal);
-        axis_apply(cp.n, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, k.normal, tmax<T>(T(0), tl));
-    }
-    if (dyn & 1) { B.v[a] = vA; B.w[a] = wA; }
-    if (dyn & 2) { B.v[b] = vB; B.w[b] = wB; }
-}
-
-template <typename T>
-__device__ inline void warm_start_contact(const Dev<T>& d, int slot, T ratio) {  // :906-955 for one constraint
-    Con<T>& k = d.con[slot];
-    int dyn = k.dynMask;
-    const Bodies<T>& B = d.B;
-    int a = k.a, b = k.b;
-    V3<T> vA = B.v[a], wA = B.w[a], vB = B.v[b], wB = B.w[b];
-    int dofA = fl_dof(B.flags[a]), dofB = fl_dof(B.flags[b]);
-    T iA = k.invMassA, iB = k.invMassB;
-    for (int j = 0; j < k.np; j++) {
-        ConPoint<T>& cp = k.p[j];
-        if (cp.t1.eff != T(0) || cp.t2.eff != T(0)) {
-            axis_warm(cp.t1, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, k.t1, ratio);
-            axis_warm(cp.t2, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, k.t2, ratio);
-        }
-        axis_warm(cp.n, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, k.normal, ratio);
-    }
-    if (dyn & 1) { B.v[a] = vA; B.w[a] = wA; }
-    if (dyn & 2) { B.v[b] = vB; B.w[b] = wB; }
-}
-
-template <typename T>
-__device__ inline bool solve_position_contact(const Dev<T>& d, int slot) {  // :1098-1216 for one constraint
-    Con<T>& k = d.con[slot];
-    int dyn = k.dynMask;
-    if (!dyn) return false;
-    const Bodies<T>& B = d.B;
-    int a = k.a, b = k.b;
-    int fa = B.flags[a], fb = B.flags[b];
-    V3<T> cA = B.com[a], cB = B.com[b];
-    Q4<T> qA = B.q[a], qB = B.q[b];
-    M3<T> RA = m3_from_quat(qA), RB = m3_from_quat(qB);
-    M3<T> IA, IB;
-    for (int j = 0; j < 9; j++) { IA.m[j] = T(0); IB.m[j] = T(0); }
-    if (dyn & 1) IA = inv_inertia(B.invI[a], B.iRot[a], qA, fl_dof(fa));
-    if (dyn & 2) IB = inv_inertia(B.invI[b], B.iRot[b], qB, fl_dof(fb));
-    bool any = false;
-    V3<T> n = k.normal;
-    for (int j = 0; j < k.np; j++) {
-        ConPoint<T>& cp = k.p[j];
-        V3<T> pA = cA + mul(RA, cp.localA);
-        V3<T> pB = cB + mul(RB, cp.localB);
-        T sep = dot(pB - pA, n) + d.s.slop;
-        sep = tmax<T>(sep, -d.s.maxPen);
-        if (sep >= T(0)) continue;
-        V3<T> mid = mk3<T>((pA.x + pB.x) * T(0.5), (pA.y + pB.y) * T(0.5), (pA.z + pB.z) * T(0.5));
-        V3<T> rA = mid - cA, rB = mid - cB;
-        axis_props(cp.n, (dyn & 1) != 0, (dyn & 2) != 0, k.invMassA, k.invMassB, IA, IB, rA, rB, n, T(0));
-        // solvePositionConstraintWithMassOverride axis-constraint-part.ts:790-847
-        if (sep == T(0) || cp.n.eff == T(0)) continue;
-        T lam = -cp.n.eff * d.s.beta * sep;
-        if (dyn & 1) {
-            cA = dof_lin(cA - n * (lam * k.invMassA), fl_dof(fa));
-            add_rot_step(qA, cp.n.i1 * lam, T(-1));
-        }
-        if (dyn & 2) {
-            cB = dof_lin(cB + n * (lam * k.invMassB), fl_dof(fb));
-            add_rot_step(qB, cp.n.i2 * lam, T(1));
-        }
-        any = true;
-    }
-    if (dyn & 1) { B.com[a] = cA; B.q[a] = qA; }
-    if (dyn & 2) { B.com[b] = cB; B.q[b] = qB; }
-    return any;
-}
-
-// ------------------------------------------------- the same three solves on the level-ordered rows
-template <typename T>
-__device__ inline void solve_velocity_rows(const Dev<T>& d, int e) {  // :963-1053 for one constraint
-    const int meta = d.svMeta[e];
+__device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, CacheRec<T>* rec) {
+    const int meta = E.meta;
     const int dyn = meta & 3;
-    if (!dyn) return;
-    const Bodies<T>& B = d.B;
-    const int a = d.svA[e], b = d.svB[e];
-    V3<T> vA, wA, vB, wB;
-    vw_load(d, a, vA, wA);
-    vw_load(d, b, vB, wB);
     const int mov = (meta >> 2) & 3, np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
-    const T iA = d.svIMA[e], iB = d.svIMB[e], fr = d.svFr[e];
-    const V3<T> n = d.svN[e], t1 = d.svT1[e], t2 = d.svT2[e];
-    const size_t pb = (size_t)d.svPt[e];
-    for (int j = 0; j < np; j++) {
-        const size_t k = pb + j;
-        Row<T> r1 = row_load(d.rT1, k), r2 = row_load(d.rT2, k);
-        if (r1.eff != T(0) || r2.eff != T(0)) {
-            T l1 = axis_total_lambda(r1, mov, vA, wA, vB, wB, t1);
-            T l2 = axis_total_lambda(r2, mov, vA, wA, vB, wB, t2);
-            T mf = fr * d.rN.lambda[k];
+    const VBody<T> IA = d.vb[E.a], IB = d.vb[E.b];
+    VPoint<T> P[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (j < np) P[j] = d.vp[E.pt + j];
+    VPoint<T>* Lm = P;
+    V3<T> vA = IA.v, wA = IA.w, vB = IB.v, wB = IB.w;
+    const V3<T> n = E.n, t1 = contact_t1(n), t2 = cross(n, t1);
+    const T iA = IA.invMass, iB = IB.invMass, fr = E.fr;
+    const int npS = dyn ? np : 0;  // a constraint without a dynamic body is not solved
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (j >= npS) continue;
+        const VPoint<T>& p = P[j];
+        VPoint<T>& l = Lm[j];
+        if (p.effT1 != T(0) || p.effT2 != T(0)) {
+            T l1 = l.lT1 + p.effT1 * axis_jv(dyn, mov, p.r1, p.r2, vA, wA, vB, wB, t1);
+            T l2 = l.lT2 + p.effT2 * axis_jv(dyn, mov, p.r1, p.r2, vA, wA, vB, wB, t2);
+            T mf = fr * l.lN;
             T fm2 = l1 * l1 + l2 * l2;
             if (fm2 > mf * mf) { T sc = mf / sqrt(fm2); l1 *= sc; l2 *= sc; }
-            axis_apply(r1, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, t1, l1);
-            axis_apply(r2, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, t2, l2);
-            d.rT1.lambda[k] = r1.lambda;
-            d.rT2.lambda[k] = r2.lambda;
+            T d1 = l1 - l.lT1;
+            l.lT1 = l1;
+            if (d1 != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, IA.I, IB.I, p.r1, p.r2, vA, wA, vB, wB, t1, d1);
+            T d2 = l2 - l.lT2;
+            l.lT2 = l2;
+            if (d2 != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, IA.I, IB.I, p.r1, p.r2, vA, wA, vB, wB, t2, d2);
         }
     }
-    for (int j = 0; j < np; j++) {
-        const size_t k = pb + j;
-        Row<T> rn = row_load(d.rN, k);
-        T tl = axis_total_lambda(rn, mov, vA, wA, vB, wB, n);
-        axis_apply(rn, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, n, tmax<T>(T(0), tl));
-        d.rN.lambda[k] = rn.lambda;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (j >= npS) continue;
+        const VPoint<T>& p = P[j];
+        VPoint<T>& l = Lm[j];
+        T tl = l.lN + p.effN * (axis_jv(dyn, mov, p.r1, p.r2, vA, wA, vB, wB, n) - p.biasN);
+        tl = tmax<T>(T(0), tl);
+        T dn = tl - l.lN;
+        l.lN = tl;
+        if (dn != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, IA.I, IB.I, p.r1, p.r2, vA, wA, vB, wB, n, dn);
     }
-    if (dyn & 1) vw_store(d, a, vA, wA);
-    if (dyn & 2) vw_store(d, b, vB, wB);
+    if (rec) {
+        CacheRec<T>& r = rec[E.slot];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j >= np) continue;
+            r.ln[j] = Lm[j].lN;
+            r.lt1[j] = Lm[j].lT1;
+            r.lt2[j] = Lm[j].lT2;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j >= np) continue;
+            VPoint<T>& q = d.vp[E.pt + j];
+            q.lN = Lm[j].lN;
+            q.lT1 = Lm[j].lT1;
+            q.lT2 = Lm[j].lT2;
+        }
+    }
+    if (dyn & 1) vw_store(d, E.a, vA, wA);
+    if (dyn & 2) vw_store(d, E.b, vB, wB);
+}
+
+// warm start of one constraint's points (warmStartVelocity contact-constraints.ts:906-955)
+template <typename T>
+__device__ inline void warm_points(int dyn, int dofA, int dofB, T iA, T iB, const M3<T>& IA, const M3<T>& IB, V3<T> n,
+                                   V3<T> t1, V3<T> t2, int np, VPoint<T>* P, V3<T>& vA, V3<T>& wA, V3<T>& vB,
+                                   V3<T>& wB, T ratio) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (j >= np) continue;
+        VPoint<T>& p = P[j];
+        VPoint<T>& l = P[j];
+        if (p.effT1 != T(0) || p.effT2 != T(0)) {
+            axis_warm(dyn, dofA, dofB, iA, iB, IA, IB, p.r1, p.r2, vA, wA, vB, wB, t1, l.lT1, ratio);
+            axis_warm(dyn, dofA, dofB, iA, iB, IA, IB, p.r1, p.r2, vA, wA, vB, wB, t2, l.lT2, ratio);
+        }
+        axis_warm(dyn, dofA, dofB, iA, iB, IA, IB, p.r1, p.r2, vA, wA, vB, wB, n, l.lN, ratio);
+    }
 }
 
 template <typename T>
-__device__ inline void warm_start_rows(const Dev<T>& d, int e, T ratio) {  // :906-955 for one constraint
-    const int meta = d.svMeta[e];
+__device__ inline void warm_start_elem(const Dev<T>& d, const VElem<T>& E, T ratio) {
+    const int meta = E.meta;
     const int dyn = meta & 3;
-    const Bodies<T>& B = d.B;
-    const int a = d.svA[e], b = d.svB[e];
-    V3<T> vA, wA, vB, wB;
-    vw_load(d, a, vA, wA);
-    vw_load(d, b, vB, wB);
     const int np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
-    const T iA = d.svIMA[e], iB = d.svIMB[e];
-    const V3<T> n = d.svN[e], t1 = d.svT1[e], t2 = d.svT2[e];
-    const size_t pb = (size_t)d.svPt[e];
-    for (int j = 0; j < np; j++) {
-        const size_t k = pb + j;
-        Row<T> r1 = row_load(d.rT1, k), r2 = row_load(d.rT2, k), rn = row_load(d.rN, k);
-        if (r1.eff != T(0) || r2.eff != T(0)) {
-            axis_warm(r1, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, t1, ratio);
-            axis_warm(r2, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, t2, ratio);
-            d.rT1.lambda[k] = r1.lambda;
-            d.rT2.lambda[k] = r2.lambda;
-        }
-        axis_warm(rn, dyn, dofA, dofB, iA, iB, vA, wA, vB, wB, n, ratio);
-        d.rN.lambda[k] = rn.lambda;
+    const VBody<T> IA = d.vb[E.a], IB = d.vb[E.b];
+    VPoint<T> P[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (j < np) P[j] = d.vp[E.pt + j];
+    VPoint<T>* Lm = P;
+    V3<T> vA = IA.v, wA = IA.w, vB = IB.v, wB = IB.w;
+    const V3<T> t1 = contact_t1(E.n);
+    warm_points(dyn, dofA, dofB, IA.invMass, IB.invMass, IA.I, IB.I, E.n, t1, cross(E.n, t1), np, P, vA, wA, vB, wB,
+                ratio);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (j >= np) continue;
+        VPoint<T>& q = d.vp[E.pt + j];
+        q.lN = Lm[j].lN;
+        q.lT1 = Lm[j].lT1;
+        q.lT2 = Lm[j].lT2;
     }
-    if (dyn & 1) vw_store(d, a, vA, wA);
-    if (dyn & 2) vw_store(d, b, vB, wB);
+    if (dyn & 1) vw_store(d, E.a, vA, wA);
+    if (dyn & 2) vw_store(d, E.b, vB, wB);
 }
 
+// the same warm start on a constraint record, in creation order (before the rows are staged)
 template <typename T>
-__device__ inline bool solve_position_rows(const Dev<T>& d, int e) {  // :1098-1216 for one constraint
-    const int meta = d.svMeta[e];
+__device__ inline void warm_start_con(const Dev<T>& d, int slot, T ratio) {
+    Con<T>& k = d.con[slot];
+    const int dyn = k.dynMask;
+    const int dofA = fl_dof(d.B.flags[k.a]), dofB = fl_dof(d.B.flags[k.b]);
+    const VBody<T> IA = d.vb[k.a], IB = d.vb[k.b];
+    const int np = k.np;
+    VPoint<T> P[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (j >= np) continue;
+        const ConPoint<T>& c = k.p[j];
+        P[j].r1 = c.r1; P[j].r2 = c.r2;
+        P[j].effN = c.effN; P[j].effT1 = c.effT1; P[j].effT2 = c.effT2;
+        P[j].lN = c.lN; P[j].lT1 = c.lT1; P[j].lT2 = c.lT2;
+    }
+    V3<T> vA = IA.v, wA = IA.w, vB = IB.v, wB = IB.w;
+    warm_points(dyn, dofA, dofB, k.invMassA, k.invMassB, IA.I, IB.I, k.normal, k.t1, k.t2, np, P, vA, wA, vB, wB, ratio);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (j >= np) continue;
+        k.p[j].lN = P[j].lN; k.p[j].lT1 = P[j].lT1; k.p[j].lT2 = P[j].lT2;
+    }
+    if (dyn & 1) vw_store(d, k.a, vA, wA);
+    if (dyn & 2) vw_store(d, k.b, vB, wB);
+}
+
+// one contact constraint of a position sweep (solvePositionConstraint :1098-1216) on the pose
+// records PA, PB of its bodies and its body-space points LA, LB
+template <typename T>
+__device__ inline bool solve_position_elem(const Dev<T>& d, const VElem<T>& E, const PBody<T>& PA, const PBody<T>& PB,
+                                           const V3<T>* LA, const V3<T>* LB) {
+    const int meta = E.meta;
     const int dyn = meta & 3;
     if (!dyn) return false;
-    const Bodies<T>& B = d.B;
-    const int a = d.svA[e], b = d.svB[e];
     const int np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
-    const T iMA = d.svIMA[e], iMB = d.svIMB[e];
-    V3<T> cA = B.com[a], cB = B.com[b];
-    Q4<T> qA = B.q[a], qB = B.q[b];
+    const T iMA = PA.invMass, iMB = PB.invMass;
+    V3<T> cA = PA.com, cB = PB.com;
+    Q4<T> qA = PA.q, qB = PB.q;
     M3<T> RA = m3_from_quat(qA), RB = m3_from_quat(qB);
     M3<T> IA, IB;
     for (int j = 0; j < 9; j++) { IA.m[j] = T(0); IB.m[j] = T(0); }
-    if (dyn & 1) IA = inv_inertia(B.invI[a], B.iRot[a], qA, dofA);
-    if (dyn & 2) IB = inv_inertia(B.invI[b], B.iRot[b], qB, dofB);
+    if (dyn & 1) IA = inv_inertia(PA.invI, PA.iRot, qA, dofA);
+    if (dyn & 2) IB = inv_inertia(PB.invI, PB.iRot, qB, dofB);
     bool any = false;
-    const V3<T> n = d.svN[e];
-    const size_t pb = (size_t)d.svPt[e];
-    for (int j = 0; j < np; j++) {
-        const size_t k = pb + j;
-        V3<T> pA = cA + mul(RA, d.svLA[k]);
-        V3<T> pB = cB + mul(RB, d.svLB[k]);
+    const V3<T> n = E.n;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (j >= np) continue;
+        V3<T> pA = cA + mul(RA, LA[j]);
+        V3<T> pB = cB + mul(RB, LB[j]);
         T sep = dot(pB - pA, n) + d.s.slop;
         sep = tmax<T>(sep, -d.s.maxPen);
         if (sep >= T(0)) continue;
@@ -290,17 +269,16 @@ __device__ inline bool solve_position_rows(const Dev<T>& d, int e) {  // :1098-1
         }
         any = true;
     }
-    if (dyn & 1) { B.com[a] = cA; B.q[a] = qA; }
-    if (dyn & 2) { B.com[b] = cB; B.q[b] = qB; }
+    if (dyn & 1) { PBody<T>& r = d.pb[E.a]; r.com = cA; r.q = qA; }
+    if (dyn & 2) { PBody<T>& r = d.pb[E.b]; r.com = cB; r.q = qB; }
     return any;
 }
 
 // ---------------------------------------------------------------- point constraint
 template <typename T>
-__device__ inline void point_props(const Dev<T>& d, PointCon<T>& p) {  // point-constraint-part.ts:77-165
+__device__ inline void point_props_q(const Dev<T>& d, PointCon<T>& p, Q4<T> qa, Q4<T> qb) {  // point-constraint-part.ts:77-165
     const Bodies<T>& B = d.B;
     int a = p.a, b = p.b;
-    Q4<T> qa = B.q[a], qb = B.q[b];
     M3<double> RA = m3_from_quat(cvtq<double>(qa)), RB = m3_from_quat(cvtq<double>(qb));
     V3<double> r1 = mul(RA, cvt<double>(p.localA)), r2 = mul(RB, cvt<double>(p.localB));
     p.r1 = cvt<T>(r1);
@@ -347,21 +325,13 @@ __device__ inline void point_props(const Dev<T>& d, PointCon<T>& p) {  // point-
     p.active = 1;
 }
 
+// applyVelocityStep point-constraint-part.ts:232-262 and solveVelocityConstraint :186-221 on the
+// velocity records
 template <typename T>
-__device__ inline void point_apply(const Dev<T>& d, PointCon<T>& p, V3<T> lam) {  // applyVelocityStep :232-262
-    if (len2(lam) == T(0)) return;
-    const Bodies<T>& B = d.B;
-    if (fl_motion(B.flags[p.a]) == MOTION_DYNAMIC) {
-        B.v[p.a] = B.v[p.a] + lam * (-B.invMass[p.a]);
-        B.w[p.a] = B.w[p.a] - mul(p.i1X, lam);
-    }
-    if (fl_motion(B.flags[p.b]) == MOTION_DYNAMIC) {
-        B.v[p.b] = B.v[p.b] + lam * B.invMass[p.b];
-        B.w[p.b] = B.w[p.b] + mul(p.i2X, lam);
-    }
+__device__ inline void point_props(const Dev<T>& d, PointCon<T>& p) {
+    point_props_q(d, p, d.B.q[p.a], d.B.q[p.b]);
 }
 
-// the same two on the packed velocity records
 template <typename T>
 __device__ inline void point_apply_vw(const Dev<T>& d, PointCon<T>& p, V3<T> lam) {
     if (len2(lam) == T(0)) return;
@@ -389,72 +359,60 @@ __device__ inline void point_solve_velocity_vw(const Dev<T>& d, PointCon<T>& p) 
     point_apply_vw(d, p, lam);
 }
 
+// solvePositionConstraint point-constraint.ts:211-230 (part :271-316) on the pose records
 template <typename T>
-__device__ inline void point_solve_velocity(const Dev<T>& d, PointCon<T>& p) {  // :186-221
-    if (!p.active) return;
-    const Bodies<T>& B = d.B;
-    V3<T> jv = B.v[p.a] - cross(p.r1, B.w[p.a]) - B.v[p.b];
-    jv = jv + cross(p.r2, B.w[p.b]);
-    V3<T> lam = mul(p.eff, jv);
-    p.lambda = p.lambda + lam;
-    point_apply(d, p, lam);
-}
-
-template <typename T>
-__device__ inline bool point_solve_position(const Dev<T>& d, PointCon<T>& p) {  // point-constraint.ts:211-230, part :271-316
-    point_props(d, p);
+__device__ inline bool point_solve_position(const Dev<T>& d, PointCon<T>& p, PBody<T> PA, PBody<T> PB) {
+    point_props_q(d, p, PA.q, PB.q);
     if (!p.active) return false;
     const Bodies<T>& B = d.B;
-    V3<T> sep = B.com[p.b] + p.r2 - B.com[p.a] - p.r1;
+    V3<T> sep = PB.com + p.r2 - PA.com - p.r1;
     if (len2(sep) < T(1e-12)) return false;
     V3<T> lam = mul(p.eff, sep) * (-d.s.beta);
     if (fl_motion(B.flags[p.a]) == MOTION_DYNAMIC) {
-        B.com[p.a] = dof_lin(B.com[p.a] - lam * B.invMass[p.a], fl_dof(B.flags[p.a]));
-        Q4<T> q = B.q[p.a];
+        PBody<T>& r = d.pb[p.a];
+        r.com = dof_lin(PA.com - lam * B.invMass[p.a], fl_dof(B.flags[p.a]));
+        Q4<T> q = PA.q;
         add_rot_step(q, mul(p.i1X, lam), T(-1));
-        B.q[p.a] = q;
+        r.q = q;
     }
     if (fl_motion(B.flags[p.b]) == MOTION_DYNAMIC) {
-        B.com[p.b] = dof_lin(B.com[p.b] + lam * B.invMass[p.b], fl_dof(B.flags[p.b]));
-        Q4<T> q = B.q[p.b];
+        PBody<T>& r = d.pb[p.b];
+        r.com = dof_lin(PB.com + lam * B.invMass[p.b], fl_dof(B.flags[p.b]));
+        Q4<T> q = PB.q;
         add_rot_step(q, mul(p.i2X, lam), T(1));
-        B.q[p.b] = q;
+        r.q = q;
     }
     return true;
 }
 
 // ------------------------------------------------------------------- the kernel
-template <typename T>
-__device__ inline int island_of(const Dev<T>& d, int a, int b) {
-    int la = d.B.label[a];
-    return la >= 0 ? la : d.B.label[b];
-}
 
-// solve-order key of sequence element k: contacts first, ordered by (sortKey, idA, idB)
-// (sortContactIndices :1240-1264), then point constraints by index (sortConstraintIds
-// constraints.ts:248-262, priority 0)
-__device__ inline bool key_less(const unsigned long long* K, int x, int y) {
-    unsigned long long hx = K[2 * x], hy = K[2 * y];
-    if (hx != hy) return hx < hy;
-    return K[2 * x + 1] < K[2 * y + 1];
-}
 
 template <typename T, int BS, bool ROWS>
 __device__ inline void warm_start_levels(const Dev<T>& d, int so, int D, const int* LS, T ratio) {
     for (int l = 0; l < D; l++) {
-        for (int x = LS[l] + threadIdx.x; x < LS[l + 1]; x += BS) {
-            int s = ROWS ? d.svSlot[so + x] : d.seq[so + d.sched[so + x]];
-            if (s >= 0x40000000) continue;
-            if (s >= 0) {
-                if (ROWS) warm_start_rows(d, so + x, ratio);
-                else warm_start_contact(d, s, ratio);
+        const int lo = LS[l], hi = LS[l + 1];
+        for (int x = lo + threadIdx.x; x < hi; x += BS) {
+            int kind, s;
+            VElem<T> E;
+            if (ROWS) {
+                E = d.ve[so + x];
+                kind = E.meta >> 28;
+                s = E.a;
+            } else {
+                s = d.seq[so + d.sched[so + x]];
+                kind = s >= 0x40000000 ? VK_INACTIVE : (s >= 0 ? VK_CONTACT : VK_POINT);
+                if (s < 0) s = -1 - s;
             }
-            else {
-                PointCon<T>& pc = d.pc[-1 - s];
+            if (kind == VK_INACTIVE) continue;
+            if (kind == VK_CONTACT) {
+                if (ROWS) warm_start_elem(d, E, ratio);
+                else warm_start_con(d, s, ratio);
+            } else {
+                PointCon<T>& pc = d.pc[s];
                 if (pc.enabled && !pc.sleeping && pc.active) {
                     pc.lambda = pc.lambda * ratio;
-                    if (ROWS) point_apply_vw(d, pc, pc.lambda);
-                    else point_apply(d, pc, pc.lambda);
+                    point_apply_vw(d, pc, pc.lambda);
                 }
             }
         }
@@ -470,35 +428,86 @@ template <typename T>
 struct CreationLess {
     const Dev<T>* d;
     int so;
-    __device__ unsigned long long key(int k) const {
+    __device__ void key(int k, unsigned long long& hi, unsigned long long& lo) const {
         int s = d->seq[so + k];
-        if (s >= 0) return ((unsigned long long)(unsigned)d->con[s].cq << 32) | (unsigned)d->con[s].co;
-        return (1ull << 63) | (unsigned)(-1 - s);
+        hi = s >= 0 ? ((unsigned long long)(unsigned)d->con[s].cq << 32) | (unsigned)d->con[s].co
+                    : (1ull << 63) | (unsigned)(-1 - s);
+        lo = 0;
     }
-    __device__ bool operator()(int x, int y) const { return key(x) < key(y); }
 };
+// solve-order key of sequence element k: contacts first, ordered by (sortKey, idA, idB)
+// (sortContactIndices :1240-1264), then point constraints by index (sortConstraintIds
+// constraints.ts:248-262, priority 0)
 struct SolveLess {
     const unsigned long long* K;
-    __device__ bool operator()(int x, int y) const { return key_less(K, x, y); }
+    __device__ void key(int k, unsigned long long& hi, unsigned long long& lo) const {
+        hi = K[2 * k];
+        lo = K[2 * k + 1];
+    }
 };
 
-// each body's list of sequence elements in the given order -> predecessor / successor on that body
+__device__ inline bool key_lt(unsigned long long h0, unsigned long long l0, unsigned long long h1, unsigned long long l1) {
+    return h0 < h1 || (h0 == h1 && l0 < l1);
+}
+
+// Each body's list of sequence elements in the given order -> predecessor / successor on that
+// body.  A list entry is k << 1 | side (side 1: the body is the element's B body).  Keys are
+// unique, so the order is the same whichever way it is sorted: lists of up to eight entries
+// are sorted in registers by a 19-comparator network, longer ones by insertion sort.
 template <typename T, int BS, typename Less>
 __device__ inline void link_body_lists(const Dev<T>& d, int b0, int nb, int so, int* BL, Less less) {
     for (int i = threadIdx.x; i < nb; i += BS) {
-        int o = d.boff[b0 + i], c = d.bcnt[b0 + i];
-        for (int x = o + 1; x < o + c; x++) {  // insertion sort (lists are short)
-            int v = BL[x];
-            int y = x - 1;
-            while (y >= o && less(v, BL[y])) { BL[y + 1] = BL[y]; y--; }
-            BL[y + 1] = v;
-        }
-        for (int x = o; x < o + c; x++) {
-            int k = BL[x];
-            int pv = x > o ? BL[x - 1] : -1;
-            int nx = x + 1 < o + c ? BL[x + 1] : -1;
-            if (d.seqA[so + k] == b0 + i) { d.prevA[so + k] = pv; d.nextA[so + k] = nx; }
-            else { d.prevB[so + k] = pv; d.nextB[so + k] = nx; }
+        const int o = d.boff[b0 + i], c = d.bcnt[b0 + i];
+        auto link = [&](int ent, int pv, int nx) {
+            const int k = ent >> 1;
+            if (ent & 1) { d.prevB[so + k] = pv; d.nextB[so + k] = nx; }
+            else { d.prevA[so + k] = pv; d.nextA[so + k] = nx; }
+        };
+        if (c <= 8) {
+            int v[8];
+            unsigned long long kh[8], kl[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = j < c ? BL[o + j] : -1;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if (j < c) less.key(v[j] >> 1, kh[j], kl[j]);
+                else kh[j] = kl[j] = ~0ull;
+            }
+#define CC_CX(p, q)                                                              \
+    {                                                                            \
+        const bool sw = key_lt(kh[q], kl[q], kh[p], kl[p]);                      \
+        const unsigned long long th = kh[p], tl = kl[p];                         \
+        const int tv = v[p];                                                     \
+        kh[p] = sw ? kh[q] : kh[p]; kl[p] = sw ? kl[q] : kl[p]; v[p] = sw ? v[q] : v[p]; \
+        kh[q] = sw ? th : kh[q]; kl[q] = sw ? tl : kl[q]; v[q] = sw ? tv : v[q];  \
+    }
+            CC_CX(0, 2) CC_CX(1, 3) CC_CX(4, 6) CC_CX(5, 7)
+            CC_CX(0, 4) CC_CX(1, 5) CC_CX(2, 6) CC_CX(3, 7)
+            CC_CX(0, 1) CC_CX(2, 3) CC_CX(4, 5) CC_CX(6, 7)
+            CC_CX(2, 4) CC_CX(3, 5)
+            CC_CX(1, 4) CC_CX(3, 6)
+            CC_CX(1, 2) CC_CX(3, 4) CC_CX(5, 6)
+#undef CC_CX
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (j < c) link(v[j], j > 0 ? v[j - 1] >> 1 : -1, j + 1 < c ? v[j + 1] >> 1 : -1);
+        } else {
+            for (int x = o + 1; x < o + c; x++) {  // insertion sort
+                const int ent = BL[x];
+                unsigned long long h, l;
+                less.key(ent >> 1, h, l);
+                int y = x - 1;
+                while (y >= o) {
+                    unsigned long long hy, ly;
+                    less.key(BL[y] >> 1, hy, ly);
+                    if (!key_lt(h, l, hy, ly)) break;
+                    BL[y + 1] = BL[y];
+                    y--;
+                }
+                BL[y + 1] = ent;
+            }
+            for (int x = o; x < o + c; x++)
+                link(BL[x], x > o ? BL[x - 1] >> 1 : -1, x + 1 < o + c ? BL[x + 1] >> 1 : -1);
         }
     }
     __syncthreads();
@@ -539,9 +548,12 @@ __device__ inline int build_levels(const Dev<T>& d, int so, int L, int* LS, int&
     return D;
 }
 
+constexpr int LS_LDS = 4096;  // level starts kept in LDS (16 KB)
+
 template <typename T, int BS>
 __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __shared__ int sh[BS / 64 + 1];
+    __shared__ int sLS[LS_LDS];
     __shared__ int sTail;
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
@@ -609,8 +621,8 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     int* BL = d.blist + 2 * (size_t)so;
     for (int k = tid; k < L; k += BS) {
         int ea = d.seqA[so + k], eb = d.seqB[so + k];
-        if (ea >= 0) BL[d.boff[ea] + atomicAdd(&d.bcnt[ea], 1)] = k;
-        if (eb >= 0) BL[d.boff[eb] + atomicAdd(&d.bcnt[eb], 1)] = k;
+        if (ea >= 0) BL[d.boff[ea] + atomicAdd(&d.bcnt[ea], 1)] = k << 1;
+        if (eb >= 0) BL[d.boff[eb] + atomicAdd(&d.bcnt[eb], 1)] = (k << 1) | 1;
     }
     __syncthreads();
     // ---- 3-5. warm start (contact-constraints.ts:906-955, constraints.ts:264-301).  The FP64
@@ -623,6 +635,17 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         pc.sleeping = B.sleeping[pc.a] && B.sleeping[pc.b];
         if (pc.enabled && !pc.sleeping) point_props(d, pc);
     }
+    // the velocity records: the inverse inertia is that of the constraint set-up (same pose)
+    for (int i = b0 + tid; i < b1; i += BS) {
+        VBody<T> r;
+        r.v = B.v[i];
+        r.w = B.w[i];
+        for (int j = 0; j < 9; j++) r.I.m[j] = T(0);
+        const int f = B.flags[i];
+        if (fl_motion(f) == MOTION_DYNAMIC) r.I = inv_inertia(B.invI[i], B.iRot[i], B.q[i], fl_dof(f));
+        r.invMass = B.invMass[i];
+        d.vb[i] = r;
+    }
     __syncthreads();
     constexpr bool kCreationOrderWarmStart = sizeof(T) == 8;
     int D = 0;
@@ -634,54 +657,77 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     link_body_lists<T, BS>(d, b0, nb, so, BL, SolveLess{K});
     D = build_levels<T, BS>(d, so, L, LS, sTail);
     if (tid == 0) d.stats[2 * w] = D;
-    // ---- stage the contact rows in level order; a constraint's points follow each other
-    // (point rows of element e at svPt[e] + j, dense over the level order)
+    // level starts in LDS when they fit
+    const bool lsLds = D + 1 <= LS_LDS;
+    if (lsLds)
+        for (int l = tid; l <= D; l += BS) sLS[l] = LS[l];
+    // ---- stage the constraints in level order; a constraint's points follow each other
+    // (dense over the level order)
+    int totPts = 0;
     {
         for (int x = tid; x < L; x += BS) {
             const int s = d.seq[so + d.sched[so + x]];
             d.svPt[so + x] = (s >= 0 && s < 0x40000000) ? d.con[s].np : 0;
         }
         __syncthreads();
-        block_scan_array<BS>(d.svPt + so, L, sh);
+        totPts = block_scan_array<BS>(d.svPt + so, L, sh);
         for (int x = tid; x < L; x += BS) {
             const int e = so + x;
             const int s = d.seq[so + d.sched[so + x]];
             d.svSlot[e] = s;
             const int pbase = 4 * so + d.svPt[e];
-            d.svPt[e] = pbase;
-            if (s < 0 || s >= 0x40000000) continue;
+            VElem<T> E;
+            E.pt = pbase;
+            E.slot = s;
+            E.fr = T(0);
+            E.n = mk3<T>(0, 0, 0);
+            if (s >= 0x40000000) {
+                E.a = E.b = 0;
+                E.meta = VK_INACTIVE << 28;
+                d.ve[e] = E;
+                continue;
+            }
+            if (s < 0) {
+                E.a = -1 - s;
+                E.b = 0;
+                E.meta = VK_POINT << 28;
+                d.ve[e] = E;
+                continue;
+            }
             const Con<T>& k = d.con[s];
-            d.svA[e] = k.a;
-            d.svB[e] = k.b;
-            d.svMeta[e] = k.dynMask | (k.movMask << 2) | (k.np << 4) | (fl_dof(B.flags[k.a]) << 8) | (fl_dof(B.flags[k.b]) << 14);
-            d.svIMA[e] = k.invMassA;
-            d.svIMB[e] = k.invMassB;
-            d.svFr[e] = k.friction;
-            d.svN[e] = k.normal;
-            d.svT1[e] = k.t1;
-            d.svT2[e] = k.t2;
+            E.a = k.a;
+            E.b = k.b;
+            E.meta = k.dynMask | (k.movMask << 2) | (k.np << 4) | (fl_dof(B.flags[k.a]) << 8) | (fl_dof(B.flags[k.b]) << 14);
+            E.n = k.normal;
+            E.fr = k.friction;
+            d.ve[e] = E;
             for (int j = 0; j < k.np; j++) {
                 const size_t q = (size_t)pbase + j;
-                row_store(d.rN, q, k.p[j].n);
-                row_store(d.rT1, q, k.p[j].t1);
-                row_store(d.rT2, q, k.p[j].t2);
-                d.svLA[q] = k.p[j].localA;
-                d.svLB[q] = k.p[j].localB;
+                const ConPoint<T>& c = k.p[j];
+                VPoint<T> P;
+                P.r1 = c.r1; P.r2 = c.r2;
+                P.effN = c.effN; P.effT1 = c.effT1; P.effT2 = c.effT2; P.biasN = c.biasN;
+                P.lN = c.lN; P.lT1 = c.lT1; P.lT2 = c.lT2;
+                d.vp[q] = P;
+                d.svLA[q] = c.localA;
+                d.svLB[q] = c.localB;
             }
         }
         __syncthreads();
     }
-    for (int i = b0 + tid; i < b1; i += BS) vw_store(d, i, B.v[i], B.w[i]);
-    __syncthreads();
-    if (!kCreationOrderWarmStart && d.s.warmStarting) warm_start_levels<T, BS, true>(d, so, D, LS, ratio);
-    // ---- 6. velocity iterations
-    for (int it = 0; it < d.s.velIters; it++) {
+    const int* LSp = lsLds ? sLS : LS;
+    if (!kCreationOrderWarmStart && d.s.warmStarting) warm_start_levels<T, BS, true>(d, so, D, LSp, ratio);
+    // ---- 6. velocity iterations; the last one also stores the impulses (step 7)
+    const int velIters = d.s.velIters;
+    for (int it = 0; it < velIters; it++) {
+        CacheRec<T>* rec = it == velIters - 1 ? d.rec[par] : nullptr;
         for (int l = 0; l < D; l++) {
-            for (int x = LS[l] + tid; x < LS[l + 1]; x += BS) {
-                int s = d.svSlot[so + x];
-                if (s >= 0x40000000) continue;
-                if (s >= 0) solve_velocity_rows(d, so + x);
-                else point_solve_velocity_vw(d, d.pc[-1 - s]);
+            const int lo = LSp[l], hi = LSp[l + 1];
+            for (int x = lo + tid; x < hi; x += BS) {
+                const VElem<T> E = d.ve[so + x];
+                const int kind = E.meta >> 28;
+                if (kind == VK_CONTACT) solve_velocity_elem(d, E, rec);
+                else if (kind == VK_POINT) point_solve_velocity_vw(d, d.pc[E.a]);
             }
             __syncthreads();
         }
@@ -694,17 +740,19 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     }
     __syncthreads();
     // ---- 7. store impulses into this step's cache records (storeAppliedImpulses :1061-1076)
-    for (int x = tid; x < L; x += BS) {
+    // when there was no velocity iteration to do it
+    for (int x = tid; x < (velIters > 0 ? 0 : L); x += BS) {
         const int e = so + x;
         const int s = d.svSlot[e];
         if (s < 0 || s >= 0x40000000) continue;
         CacheRec<T>& r = d.rec[par][s];
-        const int np = (d.svMeta[e] >> 4) & 7;
+        const VElem<T>& E = d.ve[e];
+        const int np = (E.meta >> 4) & 7;
         for (int j = 0; j < np; j++) {
-            const size_t q = (size_t)d.svPt[e] + j;
-            r.ln[j] = d.rN.lambda[q];
-            r.lt1[j] = d.rT1.lambda[q];
-            r.lt2[j] = d.rT2.lambda[q];
+            const VPoint<T>& P = d.vp[(size_t)E.pt + j];
+            r.ln[j] = P.lN;
+            r.lt1[j] = P.lT1;
+            r.lt2[j] = P.lT2;
         }
     }
     // ---- 8. velocity integration (update.ts:896-976)
@@ -730,30 +778,42 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         pos_from_com(B, d.st, i);
         set_transform(B, d.st, i);
     }
-    for (int i = b0 + tid; i < b1; i += BS) { d.isl0[i] = 1; d.isl1[i] = 0; }
+    for (int i = b0 + tid; i < b1; i += BS) {
+        d.isl0[i] = 1;
+        d.isl1[i] = 0;
+        PBody<T> r;
+        r.com = B.com[i];
+        r.q = B.q[i];
+        r.invI = B.invI[i];
+        r.iRot = B.iRot[i];
+        r.invMass = B.invMass[i];
+        r.label = B.label[i];
+        d.pb[i] = r;
+    }
     __syncthreads();
     // ---- 9. position iterations with the per-island early out (update.ts:187-223)
     int sweeps = 0;
     for (int it = 0; it < d.s.posIters; it++) {
         int anyActive = 0;
         for (int l = 0; l < D; l++) {
-            for (int x = LS[l] + tid; x < LS[l + 1]; x += BS) {
-                int s = d.svSlot[so + x];
-                if (s >= 0x40000000) continue;
-                int isl;
-                bool applied;
-                if (s >= 0) {
-                    isl = island_of(d, d.svA[so + x], d.svB[so + x]);
-                    if (isl < 0 || !d.isl0[isl]) continue;
-                    anyActive = 1;
-                    applied = solve_position_rows(d, so + x);
-                } else {
-                    PointCon<T>& pc = d.pc[-1 - s];
-                    isl = island_of(d, pc.a, pc.b);
-                    if (isl < 0 || !d.isl0[isl]) continue;
-                    anyActive = 1;
-                    applied = point_solve_position(d, pc);
-                }
+            const int lo = LSp[l], hi = LSp[l + 1];
+            for (int x = lo + tid; x < hi; x += BS) {
+                const VElem<T> E = d.ve[so + x];
+                const int kind = E.meta >> 28;
+                if (kind == VK_INACTIVE) continue;
+                const bool con = kind == VK_CONTACT;
+                int ia = E.a, ib = E.b;
+                if (!con) { ia = d.pc[E.a].a; ib = d.pc[E.a].b; }
+                const PBody<T> PA = d.pb[ia], PB = d.pb[ib];
+                V3<T> LA[4], LB[4];
+                const int np = con ? (E.meta >> 4) & 7 : 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (j < np) { LA[j] = d.svLA[E.pt + j]; LB[j] = d.svLB[E.pt + j]; }
+                const int isl = PA.label >= 0 ? PA.label : PB.label;
+                if (isl < 0 || !d.isl0[isl]) continue;
+                anyActive = 1;
+                const bool applied = con ? solve_position_elem(d, E, PA, PB, LA, LB) : point_solve_position(d, d.pc[E.a], PA, PB);
                 if (applied) d.isl1[isl] = 1;
             }
             __syncthreads();
@@ -762,18 +822,22 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         for (int i = b0 + tid; i < b1; i += BS) { d.isl0[i] = d.isl1[i]; d.isl1[i] = 0; }
         __syncthreads();
     }
+    for (int i = b0 + tid; i < b1; i += BS) {
+        if (fl_motion(B.flags[i]) != MOTION_DYNAMIC) continue;
+        const PBody<T>& r = d.pb[i];
+        B.com[i] = r.com;
+        B.q[i] = r.q;
+    }
     if (tid == 0) d.stats[2 * w + 1] = sweeps;
     {  // byte model: per sweep 160 B/manifold + 192 B/point, per position sweep 240 B/manifold + 24 B/point
-        int pts = 0, awake = 0;
-        for (int k = tid; k < m; k += BS) pts += d.con[d.conIdx[c0 + k]].np;
+        int awake = 0;
         for (int i = b0 + tid; i < b1; i += BS) awake += (fl_motion(B.flags[i]) == MOTION_DYNAMIC && !B.sleeping[i]) ? 1 : 0;
-        for (int o = 32; o > 0; o >>= 1) { pts += __shfl_xor(pts, o, 64); awake += __shfl_xor(awake, o, 64); }
-        if ((tid & 63) == 0) {
+        for (int o = 32; o > 0; o >>= 1) awake += __shfl_xor(awake, o, 64);
+        if ((tid & 63) == 0) atomicAdd(&d.acc[ACC_AWAKE], (unsigned long long)awake);
+        if (tid == 0) {
             unsigned long long iv = (unsigned long long)d.s.velIters + 1;
-            unsigned long long by = iv * (160ull * (tid == 0 ? m : 0) + 192ull * pts) +
-                                    (unsigned long long)sweeps * (240ull * (tid == 0 ? m : 0) + 24ull * pts);
+            unsigned long long by = iv * (160ull * m + 192ull * totPts) + (unsigned long long)sweeps * (240ull * m + 24ull * totPts);
             atomicAdd(&d.acc[ACC_BYTES_SOLVE], by);
-            atomicAdd(&d.acc[ACC_AWAKE], (unsigned long long)awake);
         }
     }
     // ---- 10. positions from the centre of mass + AABBs (update.ts:322-330)

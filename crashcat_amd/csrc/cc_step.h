@@ -54,38 +54,18 @@ struct Bodies {
     int* wake;     // set when a new constraint wakes the body
 };
 
-// Contact constraint record (src/constraints/contact-constraints.ts:31-95) for one pair slot.
+// One axis with its cached terms, as the position pass builds it for a contact point.
 template <typename T>
 struct Row {
     V3<T> r1x, r2x, i1, i2;  // r1 x axis, r2 x axis, I1^-1 (r1 x axis), I2^-1 (r2 x axis)
     T eff, lambda, bias;
 };
-// a row set in structure-of-arrays form: element k of every array belongs to one row
-template <typename T>
-struct RowSoA {
-    V3<T>* r1x;
-    V3<T>* r2x;
-    V3<T>* i1;
-    V3<T>* i2;
-    T* eff;
-    T* lambda;
-    T* bias;
-};
-template <typename T>
-__device__ inline Row<T> row_load(const RowSoA<T>& R, size_t k) {
-    Row<T> r;
-    r.r1x = R.r1x[k]; r.r2x = R.r2x[k]; r.i1 = R.i1[k]; r.i2 = R.i2[k];
-    r.eff = R.eff[k]; r.lambda = R.lambda[k]; r.bias = R.bias[k];
-    return r;
-}
-template <typename T>
-__device__ inline void row_store(const RowSoA<T>& R, size_t k, const Row<T>& r) {
-    R.r1x[k] = r.r1x; R.r2x[k] = r.r2x; R.i1[k] = r.i1; R.i2[k] = r.i2;
-    R.eff[k] = r.eff; R.lambda[k] = r.lambda; R.bias[k] = r.bias;
-}
+// Contact constraint record (src/constraints/contact-constraints.ts:31-95) for one pair slot.
 template <typename T>
 struct ConPoint {
-    Row<T> n, t1, t2;
+    V3<T> r1, r2;                 // lever arms from the centres of mass to the contact point
+    T effN, effT1, effT2, biasN;  // effective masses of the three axes, normal bias
+    T lN, lT1, lT2;               // accumulated impulses
     V3<T> localA, localB;
 };
 template <typename T>
@@ -105,6 +85,45 @@ struct CacheRec {
     int a, b, np, pad;
     V3<T> p1[4], p2[4];
     T ln[4], lt1[4], lt2[4];
+};
+// What the sweeps read: one record per body, per constraint and per contact point, each read
+// with one-line loads (a random body costs one memory line rather than one per field).
+// A body: its velocities, its world-space inverse inertia for this step (fixed while the
+// velocities are iterated: the pose changes only afterwards) and its inverse mass.
+template <typename T>
+struct alignas(16) VBody {
+    V3<T> v, w;
+    M3<T> I;
+    T invMass;
+};
+// A constraint in level order.  meta: dynMask | movMask << 2 | np << 4 | dofA << 8 |
+// dofB << 14 | kind << 28 (0 contact, 1 point constraint with index a, 2 inactive); pt: its
+// first point record; slot: its constraint slot.
+enum { VK_CONTACT = 0, VK_POINT = 1, VK_INACTIVE = 2 };
+template <typename T>
+struct alignas(16) VElem {
+    int a, b, meta, pt;
+    int slot;
+    V3<T> n;
+    T fr;
+};
+// A contact point: lever arms, effective masses of the three axes, normal bias, impulses.
+template <typename T>
+struct alignas(16) VPoint {
+    V3<T> r1, r2;
+    T effN, effT1, effT2, biasN;
+    T lN, lT1, lT2;
+};
+// A body as the position sweeps read it: centre of mass and rotation (updated by the sweeps),
+// local inverse inertia and its rotation, inverse mass, island label.
+template <typename T>
+struct alignas(16) PBody {
+    V3<T> com;
+    Q4<T> q;
+    V3<T> invI;
+    Q4<T> iRot;
+    T invMass;
+    int label;
 };
 // World-space points of a constraint, kept only for the parity taps.
 template <typename T>
@@ -234,6 +253,15 @@ __device__ inline void add_rot_step(Q4<T>& q, V3<T> s, T sign) {  // add/subRota
     }
 }
 
+// first friction axis of a contact normal (contact-constraints.ts getPerpendicular use in
+// addContactConstraint); the set-up and the sweeps compute it with the same operations
+template <typename T>
+__device__ inline V3<T> contact_t1(V3<T> n) {
+    if (fabs(n.x) > fabs(n.y)) { T tl = sqrt(n.x * n.x + n.z * n.z); return mk3<T>(n.z / tl, 0, -n.x / tl); }
+    T tl = sqrt(n.y * n.y + n.z * n.z);
+    return mk3<T>(0, n.z / tl, -n.y / tl);
+}
+
 // ---------------------------------------------------------- constraint parts
 template <typename T>
 __device__ inline void axis_props(Row<T>& p, bool dynA, bool dynB, T invMA, T invMB, const M3<T>& IA, const M3<T>& IB,
@@ -246,45 +274,61 @@ __device__ inline void axis_props(Row<T>& p, bool dynA, bool dynB, T invMA, T in
     else { p.eff = T(1) / inv; p.bias = bias; }
 }
 
+// effective mass of one axis from the lever arms (calculateConstraintProperties
+// axis-constraint-part.ts:87-201); 0 when the axis has no mass
 template <typename T>
-__device__ inline T axis_total_lambda(const Row<T>& p, int mov, V3<T> vA, V3<T> wA, V3<T> vB, V3<T> wB, V3<T> axis) {
-    // getTotalLambda :578-625
+__device__ inline T axis_eff(bool dynA, bool dynB, T invMA, T invMB, const M3<T>& IA, const M3<T>& IB, V3<T> r1,
+                             V3<T> r2, V3<T> axis) {
+    T inv = T(0);
+    if (dynA) { V3<T> r1x = cross(r1, axis); V3<T> i1 = mul(IA, r1x); inv += invMA + dot(i1, r1x); }
+    if (dynB) { V3<T> r2x = cross(r2, axis); V3<T> i2 = mul(IB, r2x); inv += invMB + dot(i2, r2x); }
+    return inv == T(0) ? T(0) : T(1) / inv;
+}
+
+// J v of one axis (getTotalLambda :578-625 before the lambda and bias terms).  r x axis is
+// recomputed from the lever arm exactly as the constraint set-up computes it, and is zero for
+// a body that is not dynamic.
+template <typename T>
+__device__ inline T axis_jv(int dyn, int mov, V3<T> r1, V3<T> r2, V3<T> vA, V3<T> wA, V3<T> vB, V3<T> wB, V3<T> axis) {
     T jv;
     bool mA = mov & 1, mB = mov & 2;
     if (mA && mB) jv = dot(axis, vA - vB);
     else if (mA) jv = dot(axis, vA);
     else if (mB) jv = -dot(axis, vB);
     else jv = T(0);
-    if (mA) jv += dot(p.r1x, wA);
-    if (mB) jv -= dot(p.r2x, wB);
-    return p.lambda + p.eff * (jv - p.bias);
+    const V3<T> z = mk3<T>(0, 0, 0);
+    if (mA) jv += dot((dyn & 1) ? cross(r1, axis) : z, wA);
+    if (mB) jv -= dot((dyn & 2) ? cross(r2, axis) : z, wB);
+    return jv;
 }
 
+// velocity change of an impulse dl along one axis (applyLambda :640-707)
 template <typename T>
-__device__ inline void axis_apply(Row<T>& p, int dyn, int dofA, int dofB, T invMA, T invMB, V3<T>& vA, V3<T>& wA,
-                                  V3<T>& vB, V3<T>& wB, V3<T> axis, T total) {  // applyLambda :640-707
-    T dl = total - p.lambda;
-    p.lambda = total;
-    if (dl == T(0)) return;
+__device__ inline void axis_impulse(int dyn, int dofA, int dofB, T invMA, T invMB, const M3<T>& IA, const M3<T>& IB,
+                                    V3<T> r1, V3<T> r2, V3<T>& vA, V3<T>& wA, V3<T>& vB, V3<T>& wB, V3<T> axis, T dl) {
     if (dyn & 1) {
         T ls = dl * invMA;
         vA = dof_lin(mk3<T>(vA.x - axis.x * ls, vA.y - axis.y * ls, vA.z - axis.z * ls), dofA);
-        wA = mk3<T>(wA.x - p.i1.x * dl, wA.y - p.i1.y * dl, wA.z - p.i1.z * dl);
+        V3<T> i1 = mul(IA, cross(r1, axis));
+        wA = mk3<T>(wA.x - i1.x * dl, wA.y - i1.y * dl, wA.z - i1.z * dl);
     }
     if (dyn & 2) {
         T ls = dl * invMB;
         vB = dof_lin(mk3<T>(vB.x + axis.x * ls, vB.y + axis.y * ls, vB.z + axis.z * ls), dofB);
-        wB = mk3<T>(wB.x + p.i2.x * dl, wB.y + p.i2.y * dl, wB.z + p.i2.z * dl);
+        V3<T> i2 = mul(IB, cross(r2, axis));
+        wB = mk3<T>(wB.x + i2.x * dl, wB.y + i2.y * dl, wB.z + i2.z * dl);
     }
 }
 
+// warm start of one axis (warmStart :500-537)
 template <typename T>
-__device__ inline void axis_warm(Row<T>& p, int dyn, int dofA, int dofB, T invMA, T invMB, V3<T>& vA, V3<T>& wA,
-                                 V3<T>& vB, V3<T>& wB, V3<T> axis, T ratio) {  // warmStart :500-537
-    p.lambda *= ratio;
-    if (p.lambda == T(0)) return;
-    if (dyn & 1) { vA = dof_lin(vA - axis * (p.lambda * invMA), dofA); wA = wA - p.i1 * p.lambda; }
-    if (dyn & 2) { vB = dof_lin(vB + axis * (p.lambda * invMB), dofB); wB = wB + p.i2 * p.lambda; }
+__device__ inline void axis_warm(int dyn, int dofA, int dofB, T invMA, T invMB, const M3<T>& IA, const M3<T>& IB,
+                                 V3<T> r1, V3<T> r2, V3<T>& vA, V3<T>& wA, V3<T>& vB, V3<T>& wB, V3<T> axis, T& lambda,
+                                 T ratio) {
+    lambda *= ratio;
+    if (lambda == T(0)) return;
+    if (dyn & 1) { vA = dof_lin(vA - axis * (lambda * invMA), dofA); wA = wA - mul(IA, cross(r1, axis)) * lambda; }
+    if (dyn & 2) { vB = dof_lin(vB + axis * (lambda * invMB), dofB); wB = wB + mul(IB, cross(r2, axis)) * lambda; }
 }
 
 }  // namespace cc

@@ -449,6 +449,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.gLargeFlag = A.get<int>(nb + 1);
     d.gLargeOff = A.get<int>(nb + 1);
     d.gLarge = A.get<int>(nb);
+    d.npb = A.get<NpBody<T>>(nb);
     d.bpCount = A.get<int>(nb + 1);
     d.bpOff = A.get<int>(nb + 1);
     d.bpSlots = A.get<int>((size_t)nb * BP_SLOTS);
@@ -496,28 +497,13 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.blist = A.get<int>(2 * L);
     d.svCap = (int)L;
     d.svSlot = A.get<int>(L);
-    d.svA = A.get<int>(L);
-    d.svB = A.get<int>(L);
-    d.svMeta = A.get<int>(L);
+    d.ve = A.get<VElem<T>>(L);
+    d.vp = A.get<VPoint<T>>(4 * L);
     d.svPt = A.get<int>(L);
-    d.svIMA = A.get<T>(L);
-    d.svIMB = A.get<T>(L);
-    d.svFr = A.get<T>(L);
-    d.svN = A.get<V3<T>>(L);
-    d.svT1 = A.get<V3<T>>(L);
-    d.svT2 = A.get<V3<T>>(L);
-    for (RowSoA<T>* R : {&d.rN, &d.rT1, &d.rT2}) {
-        R->r1x = A.get<V3<T>>(4 * L);
-        R->r2x = A.get<V3<T>>(4 * L);
-        R->i1 = A.get<V3<T>>(4 * L);
-        R->i2 = A.get<V3<T>>(4 * L);
-        R->eff = A.get<T>(4 * L);
-        R->lambda = A.get<T>(4 * L);
-        R->bias = A.get<T>(4 * L);
-    }
-    d.vw = A.get<T>(8 * (size_t)std::max(nb, 1));
     d.svLA = A.get<V3<T>>(4 * L);
     d.svLB = A.get<V3<T>>(4 * L);
+    d.vb = A.get<VBody<T>>(std::max(nb, 1));
+    d.pb = A.get<PBody<T>>(std::max(nb, 1));
     d.bcnt = A.get<int>(nb);
     d.boff = A.get<int>(nb);
     d.isl0 = A.get<int>(nb);
@@ -545,7 +531,11 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int bs = c->cfg.block_solver;
     if (bs == 64 || bs == 128 || bs == 256 || bs == 512 || bs == 1024) X->solverBS = bs;
-    else X->solverBS = maxWorld > 4096 ? 1024 : (maxWorld > 1024 ? 512 : 256);
+    else {
+        X->solverBS = maxWorld > 4096 ? 1024 : (maxWorld > 1024 ? 512 : 256);
+        // FP64 records need twice the registers: 1024 lanes would spill
+        if (sizeof(T) == 8) X->solverBS = std::min(X->solverBS, 512);
+    }
     return CC_OK;
 }
 
