@@ -6,12 +6,28 @@
 
 namespace cc {
 
-// what the narrowphase reads of a body, in one 64-byte record (one memory line)
+// What the narrowphase reads of a body, in two records of one memory line each: the pose part
+// (GJK, EPA and manifolds) and the part only the contact set-up reads.  Written once per step
+// before the broadphase, when the poses and velocities are those of the collision phase.
+// tinfo: shape type | (most vertices of a supporting face) << 4; brad: bounding radius of the
+// shape about the body position.
 template <typename T>
 struct alignas(64) NpBody {
     V3<T> pos;
     Q4<T> q;
     int shape, flags;
+    T brad;
+    int tinfo;
+    V3<T> com;
+    T invMass;
+};
+template <typename T>
+struct alignas(64) NpDyn {
+    V3<T> v, w;
+    V3<T> invI;
+    Q4<T> iRot;
+    T friction, restitution;
+    int sleeping;
 };
 
 template <typename T>
@@ -42,7 +58,10 @@ struct Dev {
     int* gLargeFlag;    // [N + 1]
     int* gLargeOff;     // [N + 1] exclusive scan of gLargeFlag (bodies are world-ordered)
     int* gLarge;        // [N] large bodies, grouped by world
-    NpBody<T>* npb;     // [N] pose, shape and flags of each body in one record, for the narrowphase
+    NpBody<T>* npb;     // [N] narrowphase records of each body
+    NpDyn<T>* npd;      // [N]
+    T* bBrad;           // [N] bounding radius of each body's shape (fixed at upload)
+    int* bTinfo;        // [N] shape type and face size of each body's shape (fixed at upload)
     int* bpCount;       // [N+1] pairs found by each querying body
     int* bpOff;         // [N+1] exclusive scan of bpCount: pairs are stored in body order
     int* bpSlots;       // [N * BP_SLOTS] first BP_SLOTS partners of each querying body
@@ -75,7 +94,7 @@ struct Dev {
     // EPA jobs and the per-worker polytope workspace
     int* epaPair;       // [epaCap]
     Simplex<T>* epaSimplex;
-    EpaState<T>* epaWork;  // [epaWorkers]
+    char* epaWork;         // [ceil(epaWorkers / 64) * EpaWave<T>::bytes] polytopes, one block per wavefront
     // point constraints
     PointCon<T>* pc;
     int NPC;
@@ -120,7 +139,7 @@ enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
        ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_N = 16 };
 
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
-       CNT_HITS = 7, CNT_N = 8 };
+       CNT_HITS = 7, CNT_HITS_L = 8, CNT_N = 9 };
 // CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list, 16 hits
 constexpr int BP_SLOTS = 32;
 
@@ -151,15 +170,31 @@ __global__ void k_accel(Dev<T> d, T dt) {
     if (i >= d.N) return;
     const Bodies<T>& B = d.B;
     int f = B.flags[i];
+    const int sl = B.sleeping[i];
     {  // poses are final for this step's collision detection
         NpBody<T> r;
         r.pos = B.pos[i];
         r.q = B.q[i];
         r.shape = B.shape[i];
         r.flags = f;
+        r.brad = d.bBrad[i];
+        r.tinfo = d.bTinfo[i];
+        r.com = B.com[i];
+        r.invMass = B.invMass[i];
         d.npb[i] = r;
     }
-    if (fl_motion(f) != MOTION_DYNAMIC || B.sleeping[i]) return;
+    NpDyn<T> rd;
+    rd.v = B.v[i];
+    rd.w = B.w[i];
+    rd.invI = B.invI[i];
+    rd.iRot = B.iRot[i];
+    rd.friction = B.friction[i];
+    rd.restitution = B.restitution[i];
+    rd.sleeping = sl;
+    if (fl_motion(f) != MOTION_DYNAMIC || sl) {
+        d.npd[i] = rd;
+        return;
+    }
     T gs = d.s.gravityEnabled ? B.gravityFactor[i] : T(0);
     V3<T> v = B.v[i];
     v.x += (T(0) * B.invMass[i] + d.s.gravity[0] * gs) * dt;
@@ -179,6 +214,9 @@ __global__ void k_accel(Dev<T> d, T dt) {
     if (a2 > ma * ma) w = w * (ma / sqrt(a2));
     w = dof_lin(w, dof >> 3);
     B.w[i] = w;
+    rd.v = v;
+    rd.w = w;
+    d.npd[i] = rd;
 }
 
 // ---------------------------------------------------------------- broadphase
@@ -504,14 +542,15 @@ __device__ inline PairBodies<T> order_pair(const Dev<T>& d, int a, int b) {
 
 // normal velocity bias, src/constraints/contact-constraints.ts:288-423 (no external forces)
 template <typename T>
-__device__ inline T normal_bias(const Dev<T>& d, int A, int Bi, V3<T> pA, V3<T> pB, V3<T> n, V3<T> rA, V3<T> rB, T e, T dt) {
+__device__ inline T normal_bias(const Dev<T>& d, int A, int Bi, int fa, int fb, const NpDyn<T>& DA, const NpDyn<T>& DB,
+                                V3<T> pA, V3<T> pB, V3<T> n, V3<T> rA, V3<T> rB, T e, T dt) {
     const Bodies<T>& B = d.B;
-    int ma = fl_motion(B.flags[A]), mb = fl_motion(B.flags[Bi]);
+    int ma = fl_motion(fa), mb = fl_motion(fb);
     bool mA = ma != MOTION_STATIC, mB = mb != MOTION_STATIC;
     V3<T> rel;
-    if (mA && mB) rel = (B.v[Bi] + cross(B.w[Bi], rB)) - (B.v[A] + cross(B.w[A], rA));
-    else if (mA) rel = -(B.v[A] + cross(B.w[A], rA));
-    else if (mB) rel = B.v[Bi] + cross(B.w[Bi], rB);
+    if (mA && mB) rel = (DB.v + cross(DB.w, rB)) - (DA.v + cross(DA.w, rA));
+    else if (mA) rel = -(DA.v + cross(DA.w, rA));
+    else if (mB) rel = DB.v + cross(DB.w, rB);
     else return T(0);
     T nv = dot(rel, n);
     T pen = dot(pA - pB, n);
@@ -533,10 +572,10 @@ __device__ inline T normal_bias(const Dev<T>& d, int A, int Bi, V3<T> pA, V3<T> 
 
 // Finish one hit: manifold (update.ts:385-563 for the single hit of a convex pair), then
 // addContactConstraint (contact-constraints.ts:437-857) into the pair slot.
-template <typename T>
+template <typename T, int N>
 __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, const NpBody<T>& RA, const NpBody<T>& RB,
-                                   V3<T> pointA, V3<T> pointB, V3<T> axis, const Face<T>& fA, const Face<T>& fB,
-                                   Face<T>* scratch3, Manifold<T>& m, T dt, int par) {
+                                   V3<T> pointA, V3<T> pointB, V3<T> axis, const Face<T, N>& fA, const Face<T, N>& fB,
+                                   Face<T, N>* scratch3, Manifold<T, N>& m, T dt, int par) {
     const Bodies<T>& B = d.B;
     m.base = RA.pos;
     m.normal = normalize(axis);
@@ -556,7 +595,8 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
     const NpBody<T>& Rib = sw ? RA : RB;
     int fa = Ria.flags, fb = Rib.flags;
     bool dynA = fl_motion(fa) == MOTION_DYNAMIC, dynB = fl_motion(fb) == MOTION_DYNAMIC;
-    if (!((dynA && B.invMass[ia] != T(0)) || (dynB && B.invMass[ib] != T(0)))) return;
+    if (!((dynA && Ria.invMass != T(0)) || (dynB && Rib.invMass != T(0)))) return;
+    const NpDyn<T> DA = d.npd[ia], DB = d.npd[ib];
     int cs = wave_alloc(&d.counters[CNT_CONS]);
     if (cs >= d.conCap) { atomicOr(&d.counters[CNT_OVERFLOW], 2); return; }
     d.pairCon[slot] = cs;
@@ -585,8 +625,8 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
     rc.np = m.n;
     Q4<T> qa = Ria.q, qb = Rib.q;
     M3<T> rotA = m3_from_quat(qa), rotB = m3_from_quat(qb);
-    T fr = combine(B.friction[ia], B.friction[ib], fl_fmode(fa), fl_fmode(fb));
-    T re = combine(B.restitution[ia], B.restitution[ib], fl_rmode(fa), fl_rmode(fb));
+    T fr = combine(DA.friction, DB.friction, fl_fmode(fa), fl_fmode(fb));
+    T re = combine(DA.restitution, DB.restitution, fl_rmode(fa), fl_rmode(fb));
     k.a = ia; k.b = ib;
     k.cq = d.pairA[slot];  // (query, other): the creation order of constraints
     k.co = d.pairB[slot];
@@ -597,13 +637,13 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
     k.t1 = contact_t1(n);
     k.t2 = cross(n, k.t1);
     k.friction = fr;
-    k.invMassA = B.invMass[ia];
-    k.invMassB = B.invMass[ib];
+    k.invMassA = Ria.invMass;
+    k.invMassB = Rib.invMass;
     M3<T> IA, IB;
     for (int j = 0; j < 9; j++) { IA.m[j] = T(0); IB.m[j] = T(0); }
-    if (dynA) IA = inv_inertia(B.invI[ia], B.iRot[ia], qa, fl_dof(fa));
-    if (dynB) IB = inv_inertia(B.invI[ib], B.iRot[ib], qb, fl_dof(fb));
-    V3<T> comA = B.com[ia], comB = B.com[ib];
+    if (dynA) IA = inv_inertia(DA.invI, DA.iRot, qa, fl_dof(fa));
+    if (dynB) IB = inv_inertia(DB.invI, DB.iRot, qb, fl_dof(fb));
+    V3<T> comA = Ria.com, comB = Rib.com;
     k.np = m.n;
     for (int i = 0; i < m.n; i++) {
         ConPoint<T>& cp = k.p[i];
@@ -628,7 +668,7 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
         if (!set) { cp.lN = cp.lT1 = cp.lT2 = T(0); }
         V3<T> mid = mk3<T>((pA.x + pB.x) * T(0.5), (pA.y + pB.y) * T(0.5), (pA.z + pB.z) * T(0.5));
         V3<T> rA = mid - comA, rB = mid - comB;
-        T bias = normal_bias(d, ia, ib, pA, pB, n, rA, rB, re, dt);
+        T bias = normal_bias(d, ia, ib, fa, fb, DA, DB, pA, pB, n, rA, rB, re, dt);
         cp.r1 = rA;
         cp.r2 = rB;
         // an axis without mass drops its impulse and bias (axis-constraint-part.ts:87-201)
@@ -650,20 +690,20 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
         rc.lt1[i] = cp.lT1;
         rc.lt2[i] = cp.lT2;
     }
-    if (dynA && B.sleeping[ia]) B.wake[ia] = 1;
-    if (dynB && B.sleeping[ib]) B.wake[ib] = 1;
+    if (dynA && DA.sleeping) B.wake[ia] = 1;
+    if (dynB && DB.sleeping) B.wake[ib] = 1;
 }
 
-template <typename T>
+template <typename T, int N>
 struct NpScratch {
-    Face<T> fA, fB, tmp[3];
-    Manifold<T> m;
+    Face<T, N> fA, fB, tmp[3];
+    Manifold<T, N> m;
 };
 
 // world-space hit + supporting faces after GJK/EPA (convex.ts:489-637)
-template <typename T>
+template <typename T, int N>
 __device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, const NpBody<T>& RA,
-                                     const NpBody<T>& RB, const ConvexCtx<T>& c, PenDepth<T> pd, NpScratch<T>& S, T dt,
+                                     const NpBody<T>& RB, const ConvexCtx<T>& c, PenDepth<T> pd, NpScratch<T, N>& S, T dt,
                                      int par) {
     V3<T> pa = RA.pos;
     Q4<T> qa = RA.q;
@@ -689,10 +729,14 @@ __device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, c
 
 enum { HIT_SPHERES = 0, HIT_GJK = 1, HIT_EPA = 2 };
 
+// Hits whose two shapes have supporting faces of at most four vertices go to the front of the
+// hit list (manifolds of capacity 8), the others to its back (capacity 32).
 template <typename T>
-__device__ inline void push_hit(const Dev<T>& d, int slot, int kind, V3<T> pa, V3<T> pb, V3<T> axis) {
-    int h = wave_alloc(&d.counters[CNT_HITS]);
-    if (h >= d.hitCap) { atomicOr(&d.counters[CNT_OVERFLOW], 16); return; }
+__device__ inline void push_hit(const Dev<T>& d, int slot, int kind, V3<T> pa, V3<T> pb, V3<T> axis, bool small) {
+    int h;
+    if (small) h = wave_alloc(&d.counters[CNT_HITS]);
+    else h = d.hitCap - 1 - wave_alloc(&d.counters[CNT_HITS_L]);
+    if (h < 0 || h >= d.hitCap) { atomicOr(&d.counters[CNT_OVERFLOW], 16); return; }
     d.hitSlot[h] = slot;
     d.hitKind[h] = kind;
     d.hitPA[h] = pa;
@@ -700,36 +744,41 @@ __device__ inline void push_hit(const Dev<T>& d, int slot, int kind, V3<T> pa, V
     d.hitAxis[h] = axis;
 }
 
-// One pair per lane: sphere-sphere closed form or the GJK step of collideConvexVsConvexLocal
-// (convex.ts:257-641).  Hits go to the hit list for k_np_manifold; pairs whose cores overlap go
-// to the EPA queue.  Pairs whose bounding spheres are further apart than the speculative
-// distance cannot be within it, and GJK would report them separated at once: they stop here.
+// The two narrowphase records of a pair slot, after the motion-type swap (src/update.ts:74-78).
+template <typename T>
+__device__ inline void np_pair(const Dev<T>& d, int slot, NpBody<T>& RA, NpBody<T>& RB) {
+    const int i0 = d.pairA[slot], i1 = d.pairB[slot];
+    const NpBody<T> r0 = d.npb[i0], r1 = d.npb[i1];
+    PairBodies<T> pb = order_pair_f<T>(i0, r0.flags, i1, r1.flags);
+    const bool sw = pb.a != i0;
+    RA = sw ? r1 : r0;
+    RB = sw ? r0 : r1;
+}
+
+// One pair per lane: sphere pairs in closed form; pairs whose bounding spheres are further
+// apart than the speculative distance stop at once (GJK would report them separated); the rest
+// run the GJK step of collideConvexVsConvexLocal (convex.ts:257-641): a hit for k_np_manifold,
+// or, when the cores overlap, a job for EPA.
 template <typename T>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2) k_np_gjk(Dev<T> d, T dt) {
     int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
     for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < npairs; slot += gridDim.x * blockDim.x) {
         d.pairCon[slot] = -1;
-        const int i0 = d.pairA[slot], i1 = d.pairB[slot];
-        const NpBody<T> r0 = d.npb[i0], r1 = d.npb[i1];
-        PairBodies<T> pb = order_pair_f<T>(i0, r0.flags, i1, r1.flags);
-        const bool sw = pb.a != i0;
-        const NpBody<T>& RA = sw ? r1 : r0;
-        const NpBody<T>& RB = sw ? r0 : r1;
-        const ShapeD<T>& sa = d.st.shapes[RA.shape];
-        const ShapeD<T>& sb = d.st.shapes[RB.shape];
-        if (sa.type == SHAPE_SPHERE && sb.type == SHAPE_SPHERE) {  // collideSphereVsSphere sphere.ts:319-410
+        NpBody<T> RA, RB;
+        np_pair(d, slot, RA, RB);
+        if ((RA.tinfo & 15) == SHAPE_SPHERE && (RB.tinfo & 15) == SHAPE_SPHERE) {  // collideSphereVsSphere sphere.ts:319-410
             V3<T> pa = RA.pos, pbp = RB.pos;
-            T ra = sa.radius, rb = sb.radius;
+            T ra = d.st.shapes[RA.shape].radius, rb = d.st.shapes[RB.shape].radius;
             V3<T> dd = pbp - pa;
             T d2 = len2(dd), rs = ra + rb;
             if (d2 >= rs * rs) continue;
             T dist = sqrt(d2);
             V3<T> n = dist > T(0) ? dd * (T(1) / dist) : mk3<T>(0, 1, 0);
-            push_hit(d, slot, HIT_SPHERES, pa + n * ra, pbp - n * rb, n);
+            push_hit(d, slot, HIT_SPHERES, pa + n * ra, pbp - n * rb, n, true);
             continue;
         }
         {
-            T reach = sa.brad + sb.brad + d.s.spec;
+            T reach = RA.brad + RB.brad + d.s.spec;
             if (len2(RB.pos - RA.pos) > reach * reach) continue;
         }
         ConvexCtx<T> c;
@@ -748,17 +797,17 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2) k_np_gjk(Dev<T> d
         pd_step_gjk(pd, simplex, SA, SB, SA.cr + c.maxSepUse, SB.cr, axis0, d.s.colTol);
         if (pd.status == PD_NOT_COLLIDING) continue;
         if (pd.status == PD_INDETERMINATE) {
-            int j = atomicAdd(&d.counters[CNT_EPA], 1);
+            int j = wave_alloc(&d.counters[CNT_EPA]);
             if (j < d.epaCap) { d.epaPair[j] = slot; d.epaSimplex[j] = simplex; }
             else atomicOr(&d.counters[CNT_OVERFLOW], 4);
             continue;
         }
-        push_hit(d, slot, HIT_GJK, pd.pointA, pd.pointB, pd.axis);
+        push_hit(d, slot, HIT_GJK, pd.pointA, pd.pointB, pd.axis, (RA.tinfo >> 4) <= 4 && (RB.tinfo >> 4) <= 4);
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(64) k_epa(Dev<T> d, T dt) {
+__global__ void __launch_bounds__(64, 2) k_epa(Dev<T> d, T dt) {
     int nj = min(d.counters[CNT_EPA], d.epaCap);
     int worker = blockIdx.x * blockDim.x + threadIdx.x;
     if (worker >= d.epaWorkers) return;
@@ -783,18 +832,25 @@ __global__ void __launch_bounds__(64) k_epa(Dev<T> d, T dt) {
         SB.t = c.relPos;
         PenDepth<T> pd;
         Simplex<T> simplex = d.epaSimplex[j];
-        if (!pd_step_epa(pd, SA, SB, d.s.penTol, simplex, d.epaWork[worker])) continue;
-        push_hit(d, slot, HIT_EPA, pd.pointA, pd.pointB, pd.axis);
+        EpaH<T> H;
+        H.b = d.epaWork + (size_t)(worker / EPA_LANES) * EpaWave<T>::bytes;
+        H.lane = worker % EPA_LANES;
+        if (!pd_step_epa(pd, SA, SB, d.s.penTol, simplex, H)) continue;
+        push_hit(d, slot, HIT_EPA, pd.pointA, pd.pointB, pd.axis, (RA.tinfo >> 4) <= 4 && (RB.tinfo >> 4) <= 4);
     }
 }
 
 // One hit per lane: world-space contact, supporting faces, manifold, contact cache and the
 // contact constraint (convex.ts:489-637, src/update.ts:385-563).
-template <typename T>
+// N: face and manifold capacity; the small variant takes the front of the hit list, the
+// general one its back
+template <typename T, int N>
 __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
-    int nh = min(d.counters[CNT_HITS], d.hitCap);
+    const bool small = N == CC_SMALL_FACE;
+    const int h0 = small ? 0 : d.hitCap - min(d.counters[CNT_HITS_L], d.hitCap);
+    const int h1 = small ? min(d.counters[CNT_HITS], d.hitCap) : d.hitCap;
     int par = *d.parity;
-    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+    for (int h = h0 + blockIdx.x * blockDim.x + threadIdx.x; h < h1; h += gridDim.x * blockDim.x) {
         int slot = d.hitSlot[h];
         const int i0 = d.pairA[slot], i1 = d.pairB[slot];
         const NpBody<T> r0 = d.npb[i0], r1 = d.npb[i1];
@@ -804,7 +860,7 @@ __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
         const NpBody<T>& RB = sw ? r0 : r1;
         int A = pb.a, Bi = pb.b;
         int kind = d.hitKind[h];
-        NpScratch<T> S;
+        NpScratch<T, N> S;
         if (kind == HIT_SPHERES) {
             S.fA.n = 0;
             S.fB.n = 0;

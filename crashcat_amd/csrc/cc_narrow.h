@@ -11,7 +11,8 @@ enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3 };
 
 template <typename T>
 struct ShapeD {
-    int type, hfp, hnp, hff, hnf, pad;
+    int type, hfp, hnp, hff, hnf;
+    int mfv;  // most vertices a supporting face of the shape can have
     T radius, hh, cr;
     T he[3], com[3], amin[3], amax[3];
     T brad;  // radius of a sphere about the shape origin holding the shape (core + convex radius)
@@ -464,217 +465,258 @@ __device__ inline void pd_step_gjk(PenDepth<T>& out, Simplex<T>& outSimplex, con
 #define EPA_POINT_CAP 160
 #define EPA_MAX_POINTS_TO_INCLUDE_ORIGIN_IN_HULL 32
 #define EPA_MAX_POINTS 128
+#define EPA_LANES 64
 
 struct EEdge { short nbT; unsigned char nbE; unsigned char start; };
-struct ETri {
-    double nx, ny, nz, cx, cy, cz;
-    double cls;
-    double l0, l1;
-    EEdge e[3];
-    unsigned char rel0, interior, removed, inQueue;
-};
 struct EStack { short tri; short edge; short iter; short pad; };
+enum { TNX, TNY, TNZ, TCX, TCY, TCZ, TCLS, TL0, TL1, TNF };  // the FP64 fields of a triangle
+enum { TF_REL0, TF_INTERIOR, TF_REMOVED, TF_INQUEUE };     // its flag bytes
+
+// The polytopes of the EPAs of one wavefront, one per lane, interleaved by lane: element k
+// of an array of lane l sits at k * 64 + l.  The lanes' polytopes grow alike (the same first
+// triangles and point slots), so a wavefront's accesses share memory lines.
+template <typename T>
+struct EpaWave {  // byte offsets of the arrays in one wavefront's block
+    static constexpr size_t L = EPA_LANES;
+    static constexpr size_t oY = 0;
+    static constexpr size_t oP = oY + sizeof(V3<T>) * EPA_POINT_CAP * L;
+    static constexpr size_t oQ = oP + sizeof(V3<T>) * EPA_POINT_CAP * L;
+    static constexpr size_t oTD = oQ + sizeof(V3<T>) * EPA_POINT_CAP * L;
+    static constexpr size_t oTE = oTD + sizeof(double) * TNF * EPA_MAX_TRIANGLES * L;
+    static constexpr size_t oTF = oTE + sizeof(EEdge) * 3 * EPA_MAX_TRIANGLES * L;
+    static constexpr size_t oFL = oTF + 4 * EPA_MAX_TRIANGLES * L;
+    static constexpr size_t oQU = oFL + sizeof(short) * 2 * EPA_MAX_TRIANGLES * L;
+    static constexpr size_t oST = oQU + sizeof(short) * 2 * EPA_MAX_TRIANGLES * L;
+    static constexpr size_t oED = oST + sizeof(EStack) * EPA_MAX_EDGE_LENGTH * L;
+    static constexpr size_t oNT = oED + sizeof(EEdge) * EPA_MAX_EDGE_LENGTH * L;
+    static constexpr size_t bytes = (oNT + sizeof(short) * EPA_MAX_EDGE_LENGTH * L + 255) / 256 * 256;
+};
+
+// One lane's polytope: accessors into the wavefront's block, counters in registers.
+// Triangles come from a bump allocator and a stack of freed ones (freed first), which hands
+// out the same indices as one free list holding 0..255 in pop order.
+template <typename T>
+struct EpaH {
+    char* b;
+    int lane;
+    int n, bump, nrec, nq, nedges, nnew;
+    typedef EpaWave<T> W;
+    __device__ V3<T>& y(int i) const { return ((V3<T>*)(b + W::oY))[i * EPA_LANES + lane]; }
+    __device__ V3<T>& p(int i) const { return ((V3<T>*)(b + W::oP))[i * EPA_LANES + lane]; }
+    __device__ V3<T>& q(int i) const { return ((V3<T>*)(b + W::oQ))[i * EPA_LANES + lane]; }
+    __device__ double& td(int f, int t) const { return ((double*)(b + W::oTD))[(f * EPA_MAX_TRIANGLES + t) * EPA_LANES + lane]; }
+    __device__ EEdge& te(int t, int e) const { return ((EEdge*)(b + W::oTE))[(e * EPA_MAX_TRIANGLES + t) * EPA_LANES + lane]; }
+    __device__ unsigned char& tf(int t, int f) const { return ((unsigned char*)(b + W::oTF))[((t * EPA_LANES + lane) << 2) + f]; }
+    __device__ short& fl(int i) const { return ((short*)(b + W::oFL))[i * EPA_LANES + lane]; }
+    __device__ short& qu(int i) const { return ((short*)(b + W::oQU))[i * EPA_LANES + lane]; }
+    __device__ EStack& st(int i) const { return ((EStack*)(b + W::oST))[i * EPA_LANES + lane]; }
+    __device__ EEdge& ed(int i) const { return ((EEdge*)(b + W::oED))[i * EPA_LANES + lane]; }
+    __device__ short& nt(int i) const { return ((short*)(b + W::oNT))[i * EPA_LANES + lane]; }
+    __device__ double cls(int t) const { return td(TCLS, t); }
+};
 
 template <typename T>
-struct EpaState {
-    V3<T> y[EPA_POINT_CAP], p[EPA_POINT_CAP], q[EPA_POINT_CAP];
-    ETri tris[EPA_MAX_TRIANGLES];
-    short freeList[EPA_MAX_TRIANGLES * 2];
-    short queue[EPA_MAX_TRIANGLES * 2];
-    EStack stack[EPA_MAX_EDGE_LENGTH];
-    EEdge edges[EPA_MAX_EDGE_LENGTH];
-    short newT[EPA_MAX_EDGE_LENGTH];
-    int n, nfree, nq, nedges, nnew;
-};
-
-__device__ inline bool tri_facing(const ETri& t, double px, double py, double pz) {
-    return t.nx * (px - t.cx) + t.ny * (py - t.cy) + t.nz * (pz - t.cz) > 0.0;
+__device__ inline bool tri_facing(const EpaH<T>& h, int t, double px, double py, double pz) {
+    return h.td(TNX, t) * (px - h.td(TCX, t)) + h.td(TNY, t) * (py - h.td(TCY, t)) + h.td(TNZ, t) * (pz - h.td(TCZ, t)) > 0.0;
 }
 
 template <typename T>
-__device__ inline void heap_push(EpaState<T>& h, int ti) {
+__device__ inline void heap_push(EpaH<T>& h, int ti) {
     if (h.nq >= EPA_MAX_TRIANGLES * 2) return;
-    h.queue[h.nq++] = (short)ti;
-    h.tris[ti].inQueue = 1;
+    h.qu(h.nq++) = (short)ti;
+    h.tf(ti, TF_INQUEUE) = 1;
     int cur = h.nq - 1;
     while (cur > 0) {
         int par = (cur - 1) >> 1;
-        if (h.tris[h.queue[par]].cls > h.tris[h.queue[cur]].cls) {
-            short t = h.queue[par]; h.queue[par] = h.queue[cur]; h.queue[cur] = t;
+        short qp = h.qu(par), qc = h.qu(cur);
+        if (h.cls(qp) > h.cls(qc)) {
+            h.qu(par) = qc; h.qu(cur) = qp;
             cur = par;
         } else break;
     }
 }
 
 template <typename T>
-__device__ inline int heap_pop(EpaState<T>& h) {
+__device__ inline int heap_pop(EpaH<T>& h) {
     if (h.nq == 0) return -1;
-    short* q = h.queue;
-    short tmp = q[h.nq - 1]; q[h.nq - 1] = q[0]; q[0] = tmp;
+    short tmp = h.qu(h.nq - 1); h.qu(h.nq - 1) = h.qu(0); h.qu(0) = tmp;
     int count = h.nq - 1;
     int largest = 0;
     for (;;) {
         int child = (largest << 1) + 1;
         if (child >= count) break;
         int prev = largest;
-        if (h.tris[q[largest]].cls > h.tris[q[child]].cls) largest = child;
+        if (h.cls(h.qu(largest)) > h.cls(h.qu(child))) largest = child;
         ++child;
-        if (child < count && h.tris[q[largest]].cls > h.tris[q[child]].cls) largest = child;
+        if (child < count && h.cls(h.qu(largest)) > h.cls(h.qu(child))) largest = child;
         if (prev == largest) break;
-        short t = q[prev]; q[prev] = q[largest]; q[largest] = t;
+        short t = h.qu(prev); h.qu(prev) = h.qu(largest); h.qu(largest) = t;
     }
-    return q[--h.nq];
+    return h.qu(--h.nq);
 }
 
 template <typename T>
-__device__ inline void free_tri(EpaState<T>& h, int ti) {
-    if (h.nfree < EPA_MAX_TRIANGLES * 2) h.freeList[h.nfree++] = (short)ti;
+__device__ inline void free_tri(EpaH<T>& h, int ti) {  // a free list of 512 entries, as (256 - bump) + nrec
+    if ((EPA_MAX_TRIANGLES - h.bump) + h.nrec < EPA_MAX_TRIANGLES * 2) h.fl(h.nrec++) = (short)ti;
 }
 
 template <typename T>
-__device__ inline int create_tri(EpaState<T>& h, int i1, int i2, int i3) {  // createTriangle :179-345
-    if (h.nfree == 0) return -1;
-    int ti = h.freeList[--h.nfree];
-    ETri& t = h.tris[ti];
-    t.cls = INFINITY;
-    t.l0 = t.l1 = 0.0;
-    t.rel0 = 0; t.interior = 0; t.removed = 0; t.inQueue = 0;
-    t.e[0].start = (unsigned char)i1; t.e[1].start = (unsigned char)i2; t.e[2].start = (unsigned char)i3;
-    t.e[0].nbT = t.e[1].nbT = t.e[2].nbT = -1;
-    V3<double> y0 = cvt<double>(h.y[i1]), y1 = cvt<double>(h.y[i2]), y2 = cvt<double>(h.y[i3]);
-    t.cx = (y0.x + y1.x + y2.x) / 3.0; t.cy = (y0.y + y1.y + y2.y) / 3.0; t.cz = (y0.z + y1.z + y2.z) / 3.0;
-    V3<double> cen = mk3<double>(t.cx, t.cy, t.cz);
+__device__ inline int create_tri(EpaH<T>& h, int i1, int i2, int i3) {  // createTriangle :179-345
+    int ti;
+    if (h.nrec > 0) ti = h.fl(--h.nrec);
+    else if (h.bump < EPA_MAX_TRIANGLES) ti = h.bump++;
+    else return -1;
+    double cls = INFINITY, l0 = 0.0, l1 = 0.0;
+    unsigned char rel0 = 0, interior = 0;
+    V3<double> y0 = cvt<double>(h.y(i1)), y1 = cvt<double>(h.y(i2)), y2 = cvt<double>(h.y(i3));
+    double cx = (y0.x + y1.x + y2.x) / 3.0, cy = (y0.y + y1.y + y2.y) / 3.0, cz = (y0.z + y1.z + y2.z) / 3.0;
+    V3<double> cen = mk3<double>(cx, cy, cz);
     V3<double> y10 = y1 - y0, y20 = y2 - y0, y21 = y2 - y1;
     double y20d = len2(y20), y21d = len2(y21);
+    V3<double> n;
     if (y20d < y21d) {
-        V3<double> n = cross(y10, y20);
-        t.nx = n.x; t.ny = n.y; t.nz = n.z;
+        n = cross(y10, y20);
         double nl = len2(n);
         if (nl > 1e-10) {
             double cdn = dot(cen, n);
-            t.cls = fabs(cdn) * cdn / nl;
+            cls = fabs(cdn) * cdn / nl;
             double y10d = len2(y10), y10y20 = dot(y10, y20);
             double det = y10d * y20d - y10y20 * y10y20;
             if (det > 0.0) {
                 double y0y10 = dot(y0, y10), y0y20 = dot(y0, y20);
-                double l0 = (y10y20 * y0y20 - y20d * y0y10) / det;
-                double l1 = (y10y20 * y0y10 - y10d * y0y20) / det;
-                t.l0 = l0; t.l1 = l1; t.rel0 = 1;
-                if (l0 > -1e-3 && l1 > -1e-3 && l0 + l1 < 1.0 + 1e-3) t.interior = 1;
+                l0 = (y10y20 * y0y20 - y20d * y0y10) / det;
+                l1 = (y10y20 * y0y10 - y10d * y0y20) / det;
+                rel0 = 1;
+                if (l0 > -1e-3 && l1 > -1e-3 && l0 + l1 < 1.0 + 1e-3) interior = 1;
             }
         }
     } else {
-        V3<double> n = cross(y10, y21);
-        t.nx = n.x; t.ny = n.y; t.nz = n.z;
+        n = cross(y10, y21);
         double nl = len2(n);
         if (nl > 1e-10) {
             double cdn = dot(cen, n);
-            t.cls = fabs(cdn) * cdn / nl;
+            cls = fabs(cdn) * cdn / nl;
             double y10d = len2(y10), y10y21 = dot(y10, y21);
             double det = y10d * y21d - y10y21 * y10y21;
             if (det > 0.0) {
                 double y1y10 = dot(y1, y10), y1y21 = dot(y1, y21);
-                double l0 = (y21d * y1y10 - y10y21 * y1y21) / det;
-                double l1 = (y10y21 * y1y10 - y10d * y1y21) / det;
-                t.l0 = l0; t.l1 = l1; t.rel0 = 0;
-                if (l0 > -1e-3 && l1 > -1e-3 && l0 + l1 < 1.0 + 1e-3) t.interior = 1;
+                l0 = (y21d * y1y10 - y10y21 * y1y21) / det;
+                l1 = (y10y21 * y1y10 - y10d * y1y21) / det;
+                rel0 = 0;
+                if (l0 > -1e-3 && l1 > -1e-3 && l0 + l1 < 1.0 + 1e-3) interior = 1;
             }
         }
     }
+    h.td(TNX, ti) = n.x; h.td(TNY, ti) = n.y; h.td(TNZ, ti) = n.z;
+    h.td(TCX, ti) = cx; h.td(TCY, ti) = cy; h.td(TCZ, ti) = cz;
+    h.td(TCLS, ti) = cls; h.td(TL0, ti) = l0; h.td(TL1, ti) = l1;
+    h.tf(ti, TF_REL0) = rel0; h.tf(ti, TF_INTERIOR) = interior; h.tf(ti, TF_REMOVED) = 0; h.tf(ti, TF_INQUEUE) = 0;
+    EEdge e;
+    e.nbT = -1; e.nbE = 0;
+    e.start = (unsigned char)i1; h.te(ti, 0) = e;
+    e.start = (unsigned char)i2; h.te(ti, 1) = e;
+    e.start = (unsigned char)i3; h.te(ti, 2) = e;
     return ti;
 }
 
 template <typename T>
-__device__ inline void link_tri(EpaState<T>& h, int t1, int e1, int t2, int e2) {
-    h.tris[t1].e[e1].nbT = (short)t2; h.tris[t1].e[e1].nbE = (unsigned char)e2;
-    h.tris[t2].e[e2].nbT = (short)t1; h.tris[t2].e[e2].nbE = (unsigned char)e1;
+__device__ inline void link_tri(EpaH<T>& h, int t1, int e1, int t2, int e2) {
+    EEdge& a = h.te(t1, e1);
+    a.nbT = (short)t2; a.nbE = (unsigned char)e2;
+    EEdge& b = h.te(t2, e2);
+    b.nbT = (short)t1; b.nbE = (unsigned char)e1;
 }
 
 template <typename T>
-__device__ inline void unlink_tri(EpaState<T>& h, int ti) {  // unlinkTriangle :451-466
-    ETri& t = h.tris[ti];
+__device__ inline void unlink_tri(EpaH<T>& h, int ti) {  // unlinkTriangle :451-466
     for (int i = 0; i < 3; i++) {
-        if (t.e[i].nbT >= 0) {
-            h.tris[t.e[i].nbT].e[t.e[i].nbE].nbT = -1;
-            t.e[i].nbT = -1;
+        EEdge e = h.te(ti, i);
+        if (e.nbT >= 0) {
+            h.te(e.nbT, e.nbE).nbT = -1;
+            h.te(ti, i).nbT = -1;
         }
     }
-    if (!t.inQueue) free_tri(h, ti);
+    if (!h.tf(ti, TF_INQUEUE)) free_tri(h, ti);
 }
 
 template <typename T>
-__device__ inline bool find_edge(EpaState<T>& h, int facing, double px, double py, double pz) {  // findEdge :468-539
+__device__ inline bool find_edge(EpaH<T>& h, int facing, double px, double py, double pz) {  // findEdge :468-539
     h.nedges = 0;
-    h.tris[facing].removed = 1;
+    h.tf(facing, TF_REMOVED) = 1;
     int sp = 0;
-    h.stack[0].tri = (short)facing; h.stack[0].edge = 0; h.stack[0].iter = -1;
+    EStack s0;
+    s0.tri = (short)facing; s0.edge = 0; s0.iter = -1; s0.pad = 0;
+    h.st(0) = s0;
     int nextExpected = -1;
     for (;;) {
-        EStack& cur = h.stack[sp];
+        EStack cur = h.st(sp);
         if (++cur.iter >= 3) {
             unlink_tri(h, cur.tri);
             if (--sp < 0) break;
         } else {
-            EEdge e = h.tris[cur.tri].e[(cur.edge + cur.iter) % 3];
+            h.st(sp).iter = cur.iter;
+            EEdge e = h.te(cur.tri, (cur.edge + cur.iter) % 3);
             int n = e.nbT;
-            if (n >= 0 && !h.tris[n].removed) {
-                if (tri_facing(h.tris[n], px, py, pz)) {
-                    h.tris[n].removed = 1;
+            if (n >= 0 && !h.tf(n, TF_REMOVED)) {
+                if (tri_facing(h, n, px, py, pz)) {
+                    h.tf(n, TF_REMOVED) = 1;
                     if (sp + 1 >= EPA_MAX_EDGE_LENGTH) return false;
                     sp++;
-                    h.stack[sp].tri = (short)n; h.stack[sp].edge = e.nbE; h.stack[sp].iter = 0;
+                    EStack s1;
+                    s1.tri = (short)n; s1.edge = e.nbE; s1.iter = 0; s1.pad = 0;
+                    h.st(sp) = s1;
                 } else {
                     if (e.start != nextExpected && nextExpected != -1) return false;
-                    nextExpected = h.tris[n].e[e.nbE].start;
+                    nextExpected = h.te(n, e.nbE).start;
                     if (h.nedges >= EPA_MAX_EDGE_LENGTH) return false;
-                    h.edges[h.nedges++] = e;
+                    h.ed(h.nedges++) = e;
                 }
             }
         }
     }
-    if (h.nedges > 0 && h.edges[0].start != nextExpected) return false;
+    if (h.nedges > 0 && h.ed(0).start != nextExpected) return false;
     return h.nedges >= 3;
 }
 
 template <typename T>
-__device__ inline bool add_point(EpaState<T>& h, int facing, int idx, double closestDist2) {  // addPoint :541-593
-    V3<double> p = cvt<double>(h.y[idx]);
+__device__ inline bool add_point(EpaH<T>& h, int facing, int idx, double closestDist2) {  // addPoint :541-593
+    V3<double> p = cvt<double>(h.y(idx));
     h.nnew = 0;
     if (!find_edge(h, facing, p.x, p.y, p.z)) return false;
     int ne = h.nedges;
     for (int i = 0; i < ne; i++) {
         int in = (i + 1) % ne;
-        int nt = create_tri(h, h.edges[i].start, h.edges[in].start, idx);
+        int nt = create_tri(h, h.ed(i).start, h.ed(in).start, idx);
         if (nt < 0) return false;
-        h.newT[h.nnew++] = (short)nt;
-        ETri& t = h.tris[nt];
-        if ((t.interior && t.cls < closestDist2) || t.cls < 0.0) heap_push(h, nt);
+        h.nt(h.nnew++) = (short)nt;
+        if ((h.tf(nt, TF_INTERIOR) && h.cls(nt) < closestDist2) || h.cls(nt) < 0.0) heap_push(h, nt);
     }
     for (int i = 0; i < ne; i++) {
         int in = (i + 1) % ne;
-        link_tri(h, h.newT[i], 0, h.edges[i].nbT, h.edges[i].nbE);
-        link_tri(h, h.newT[i], 1, h.newT[in], 2);
+        EEdge e = h.ed(i);
+        link_tri(h, h.nt(i), 0, e.nbT, e.nbE);
+        link_tri(h, h.nt(i), 1, h.nt(in), 2);
     }
     return true;
 }
 
 template <typename T>
-__device__ inline int epa_add(EpaState<T>& P, const Sup<T>& A, const Sup<T>& B, V3<T> d) {
+__device__ inline int epa_add(EpaH<T>& P, const Sup<T>& A, const Sup<T>& B, V3<T> d) {
     V3<T> p = sup_get(A, d);
     V3<T> q = sup_get(B, -d);
     int i = P.n++;
-    P.y[i] = p - q;
-    P.p[i] = p;
-    P.q[i] = q;
+    P.y(i) = p - q;
+    P.p(i) = p;
+    P.q(i) = q;
     return i;
 }
 
-// penetrationDepthStepEPA penetration.ts:214-614; the state lives in device memory.
+// penetrationDepthStepEPA penetration.ts:214-614; the polytope lives in device memory.
 template <typename T>
 __device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const Sup<T>& B, T tol, const Simplex<T>& s,
-                                   EpaState<T>& H) {
+                                   EpaH<T>& H) {
     H.n = s.size;
-    for (int i = 0; i < s.size; i++) { H.y[i] = s.pt[i].y; H.p[i] = s.pt[i].p; H.q[i] = s.pt[i].q; }
+    for (int i = 0; i < s.size; i++) { H.y(i) = s.pt[i].y; H.p(i) = s.pt[i].p; H.q(i) = s.pt[i].q; }
     if (H.n == 1) {
         H.n--;
         epa_add(H, A, B, mk3<T>(0, 1, 0));
@@ -682,7 +724,7 @@ __device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const Sup<
         epa_add(H, A, B, mk3<T>(1, -1, -1));
         epa_add(H, A, B, mk3<T>(0, -1, 1));
     } else if (H.n == 2) {
-        V3<T> axis = H.y[1] - H.y[0];
+        V3<T> axis = H.y(1) - H.y(0);
         T al = vlen(axis);
         V3<T> an = mk3<T>(axis.x / al, axis.y / al, axis.z / al);
         V3<T> d1;
@@ -699,8 +741,8 @@ __device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const Sup<
         epa_add(H, A, B, d3);
     }
     // initialize :347-371
-    H.nfree = EPA_MAX_TRIANGLES;
-    for (int i = 0; i < EPA_MAX_TRIANGLES; i++) H.freeList[i] = (short)(EPA_MAX_TRIANGLES - 1 - i);
+    H.bump = 0;
+    H.nrec = 0;
     H.nq = 0;
     {
         int t1 = create_tri(H, 0, 1, 2);
@@ -715,14 +757,15 @@ __device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const Sup<
         // findFacingTriangle :420-445
         int best = -1;
         double bestD = 0.0;
-        V3<double> p = cvt<double>(H.y[i]);
+        V3<double> p = cvt<double>(H.y(i));
         for (int k = 0; k < H.nq; k++) {
-            const ETri& t = H.tris[H.queue[k]];
-            if (t.removed) continue;
-            double dt = t.nx * (p.x - t.cx) + t.ny * (p.y - t.cy) + t.nz * (p.z - t.cz);
+            const int t = H.qu(k);
+            if (H.tf(t, TF_REMOVED)) continue;
+            const double nx = H.td(TNX, t), ny = H.td(TNY, t), nz = H.td(TNZ, t);
+            double dt = nx * (p.x - H.td(TCX, t)) + ny * (p.y - H.td(TCY, t)) + nz * (p.z - H.td(TCZ, t));
             if (dt > 0.0) {
-                double d2 = dt * dt / (t.nx * t.nx + t.ny * t.ny + t.nz * t.nz);
-                if (d2 > bestD) { best = H.queue[k]; bestD = d2; }
+                double d2 = dt * dt / (nx * nx + ny * ny + nz * nz);
+                if (d2 > bestD) { best = t; bestD = d2; }
             }
         }
         if (best >= 0) {
@@ -731,20 +774,20 @@ __device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const Sup<
     }
     for (;;) {
         if (H.nq == 0) { out.status = PD_NOT_COLLIDING; return false; }
-        int t = H.queue[0];
-        if (H.tris[t].removed) {
+        int t = H.qu(0);
+        if (H.tf(t, TF_REMOVED)) {
             heap_pop(H);
             if (H.nq == 0) { out.status = PD_NOT_COLLIDING; return false; }
             free_tri(H, t);
             continue;
         }
-        if (H.tris[t].cls >= 0.0) break;
+        if (H.cls(t) >= 0.0) break;
         heap_pop(H);
-        V3<T> nrm = mk3<T>(T(H.tris[t].nx), T(H.tris[t].ny), T(H.tris[t].nz));
+        V3<T> nrm = mk3<T>(T(H.td(TNX, t)), T(H.td(TNY, t)), T(H.td(TNZ, t)));
         if (H.n >= EPA_POINT_CAP) { out.status = PD_NOT_COLLIDING; return false; }
         int ni = epa_add(H, A, B, nrm);
-        V3<double> w = cvt<double>(H.y[ni]);
-        if (!tri_facing(H.tris[t], w.x, w.y, w.z) || !add_point(H, t, ni, 1.7976931348623157e308)) {
+        V3<double> w = cvt<double>(H.y(ni));
+        if (!tri_facing(H, t, w.x, w.y, w.z) || !add_point(H, t, ni, 1.7976931348623157e308)) {
             out.status = PD_NOT_COLLIDING;
             return false;
         }
@@ -757,52 +800,53 @@ __device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const Sup<
     do {
         int t = heap_pop(H);
         if (t < 0) { out.status = PD_NOT_COLLIDING; return false; }
-        ETri& TT = H.tris[t];
-        if (TT.removed) { free_tri(H, t); continue; }
-        if (TT.cls >= closestDist2) break;
+        if (H.tf(t, TF_REMOVED)) { free_tri(H, t); continue; }
+        const double tcls = H.cls(t);
+        if (tcls >= closestDist2) break;
         if (last >= 0) free_tri(H, last);
         last = t;
-        V3<T> nrm = mk3<T>(T(TT.nx), T(TT.ny), T(TT.nz));
+        const double nx = H.td(TNX, t), ny = H.td(TNY, t), nz = H.td(TNZ, t);
+        V3<T> nrm = mk3<T>(T(nx), T(ny), T(nz));
         if (H.n >= EPA_POINT_CAP) break;
         int ni = epa_add(H, A, B, nrm);
-        V3<double> w = cvt<double>(H.y[ni]);
-        double dt = TT.nx * w.x + TT.ny * w.y + TT.nz * w.z;
+        V3<double> w = cvt<double>(H.y(ni));
+        double dt = nx * w.x + ny * w.y + nz * w.z;
         if (dt < 0.0) { out.status = PD_NOT_COLLIDING; return false; }
-        double d2 = dt * dt / (TT.nx * TT.nx + TT.ny * TT.ny + TT.nz * TT.nz);
-        if (d2 - TT.cls < TT.cls * double(tol)) break;
+        double d2 = dt * dt / (nx * nx + ny * ny + nz * nz);
+        if (d2 - tcls < tcls * double(tol)) break;
         closestDist2 = d2 < closestDist2 ? d2 : closestDist2;
-        if (!tri_facing(TT, w.x, w.y, w.z)) break;
+        if (!tri_facing(H, t, w.x, w.y, w.z)) break;
         if (!add_point(H, t, ni, closestDist2)) break;
         bool defect = false;
         for (int i = 0; i < H.nnew; i++) {
-            const ETri& nt = H.tris[H.newT[i]];
-            if (nt.nx * nt.cx + nt.ny * nt.cy + nt.nz * nt.cz < 0.0) { defect = true; break; }
+            const int q = H.nt(i);
+            if (H.td(TNX, q) * H.td(TCX, q) + H.td(TNY, q) * H.td(TCY, q) + H.td(TNZ, q) * H.td(TCZ, q) < 0.0) { defect = true; break; }
         }
         if (defect) {
             V3<T> p2 = sup_get(A, -nrm);
             V3<T> q2 = sup_get(B, nrm);
             V3<double> w2 = cvt<double>(p2 - q2);
-            double dot2 = -(TT.nx * w2.x + TT.ny * w2.y + TT.nz * w2.z);
+            double dot2 = -(H.td(TNX, t) * w2.x + H.td(TNY, t) * w2.y + H.td(TNZ, t) * w2.z);
             if (dot2 < dt) flip = true;
             break;
         }
     } while (H.nq > 0 && H.n < EPA_MAX_POINTS);
     if (last < 0) { out.status = PD_NOT_COLLIDING; return false; }
-    const ETri& L = H.tris[last];
-    double nl2 = L.nx * L.nx + L.ny * L.ny + L.nz * L.nz;
-    double cdn = L.cx * L.nx + L.cy * L.ny + L.cz * L.nz;
+    const double Lnx = H.td(TNX, last), Lny = H.td(TNY, last), Lnz = H.td(TNZ, last);
+    double nl2 = Lnx * Lnx + Lny * Lny + Lnz * Lnz;
+    double cdn = H.td(TCX, last) * Lnx + H.td(TCY, last) * Lny + H.td(TCZ, last) * Lnz;
     double sc = cdn / nl2;
-    V3<double> pn = mk3<double>(L.nx * sc, L.ny * sc, L.nz * sc);
+    V3<double> pn = mk3<double>(Lnx * sc, Lny * sc, Lnz * sc);
     if (len2(pn) < 1e-10) { out.status = PD_NOT_COLLIDING; return false; }
     if (flip) pn = -pn;
-    int i0 = L.e[0].start, i1 = L.e[1].start, i2 = L.e[2].start;
-    T l0 = T(L.l0), l1 = T(L.l1);
-    if (L.rel0) {
-        out.pointA = H.p[i0] + (H.p[i1] - H.p[i0]) * l0 + (H.p[i2] - H.p[i0]) * l1;
-        out.pointB = H.q[i0] + (H.q[i1] - H.q[i0]) * l0 + (H.q[i2] - H.q[i0]) * l1;
+    int i0 = H.te(last, 0).start, i1 = H.te(last, 1).start, i2 = H.te(last, 2).start;
+    T l0 = T(H.td(TL0, last)), l1 = T(H.td(TL1, last));
+    if (H.tf(last, TF_REL0)) {
+        out.pointA = H.p(i0) + (H.p(i1) - H.p(i0)) * l0 + (H.p(i2) - H.p(i0)) * l1;
+        out.pointB = H.q(i0) + (H.q(i1) - H.q(i0)) * l0 + (H.q(i2) - H.q(i0)) * l1;
     } else {
-        out.pointA = H.p[i1] + (H.p[i0] - H.p[i1]) * l0 + (H.p[i2] - H.p[i1]) * l1;
-        out.pointB = H.q[i1] + (H.q[i0] - H.q[i1]) * l0 + (H.q[i2] - H.q[i1]) * l1;
+        out.pointA = H.p(i1) + (H.p(i0) - H.p(i1)) * l0 + (H.p(i2) - H.p(i1)) * l1;
+        out.pointB = H.q(i1) + (H.q(i0) - H.q(i1)) * l0 + (H.q(i2) - H.q(i1)) * l1;
     }
     out.axis = cvt<T>(pn);
     out.status = PD_COLLIDING;
@@ -810,17 +854,21 @@ __device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const Sup<
 }
 
 // ------------------------------------------------------------------ faces
+// A face or clipped polygon of at most N vertices.  A supporting face has at most 16
+// vertices and clipping two faces of n1 and n2 vertices gives at most n1 + n2, so the general
+// capacity is 32; pairs whose shapes' faces have at most four vertices use capacity 8.
 #define CC_MAX_FACE 32
-template <typename T> struct Face { int n; V3<T> v[CC_MAX_FACE]; };
+#define CC_SMALL_FACE 8
+template <typename T, int N = CC_MAX_FACE> struct Face { int n; V3<T> v[N]; };
 
-template <typename T>
-__device__ inline void face_transform(Face<T>& f, V3<T> pos, Q4<T> q) {  // transformFace face.ts:30-84
+template <typename T, int N>
+__device__ inline void face_transform(Face<T, N>& f, V3<T> pos, Q4<T> q) {  // transformFace face.ts:30-84
     M3<T> R = m3_from_quat(q);
     for (int i = 0; i < f.n; i++) f.v[i] = mul(R, f.v[i]) + pos;
 }
 
-template <typename T>
-__device__ inline void supporting_face(Face<T>& f, const ShapeTable<T>& st, int si, V3<T> d, V3<T> pos, Q4<T> q) {
+template <typename T, int N>
+__device__ inline void supporting_face(Face<T, N>& f, const ShapeTable<T>& st, int si, V3<T> d, V3<T> pos, Q4<T> q) {
     const ShapeD<T>& s = st.shapes[si];
     if (s.type == SHAPE_SPHERE) { f.n = 0; return; }  // sphere.ts:143-146
     if (s.type == SHAPE_BOX) {  // box.ts:129-332
@@ -858,7 +906,7 @@ __device__ inline void supporting_face(Face<T>& f, const ShapeTable<T>& st, int 
         int first = st.ffv[f0 + best], nv = st.fnv[f0 + best];
         int delta = (nv + 16 - 1) / 16;
         f.n = 0;
-        for (int i = 0; i < nv && f.n < CC_MAX_FACE; i += delta) {
+        for (int i = 0; i < nv && f.n < N; i += delta) {
             int vi = st.fvi[first + i];
             const T* pp = st.pts + 3 * (s.hfp + vi);
             f.v[f.n++] = mk3<T>(pp[0], pp[1], pp[2]);
@@ -868,8 +916,8 @@ __device__ inline void supporting_face(Face<T>& f, const ShapeTable<T>& st, int 
 }
 
 // ---------------------------------------------------------------- clipping (clip.ts)
-template <typename T>
-__device__ inline void clip_poly_vs_plane(Face<T>& out, const Face<T>& poly, V3<T> origin, V3<T> normal) {  // :13-92
+template <typename T, int N>
+__device__ inline void clip_poly_vs_plane(Face<T, N>& out, const Face<T, N>& poly, V3<T> origin, V3<T> normal) {  // :13-92
     out.n = 0;
     if (poly.n < 2) return;
     V3<T> e1 = poly.v[poly.n - 1];
@@ -884,35 +932,35 @@ __device__ inline void clip_poly_vs_plane(Face<T>& out, const Face<T>& poly, V3<
             T den = dot(e12, normal);
             if (den != T(0)) {
                 T t = prevNum / den;
-                if (out.n < CC_MAX_FACE) out.v[out.n++] = e1 + e12 * t;
+                if (out.n < N) out.v[out.n++] = e1 + e12 * t;
             } else curIn = prevIn;
         }
-        if (curIn && out.n < CC_MAX_FACE) out.v[out.n++] = e2;
+        if (curIn && out.n < N) out.v[out.n++] = e2;
         prevNum = num;
         prevIn = curIn;
         e1 = e2;
     }
 }
 
-template <typename T>
-__device__ inline void clip_poly_vs_poly(Face<T>& out, const Face<T>& toClip, const Face<T>& clip, V3<T> normal,
-                                         Face<T>* tmp) {  // :108-175
+template <typename T, int N>
+__device__ inline void clip_poly_vs_poly(Face<T, N>& out, const Face<T, N>& toClip, const Face<T, N>& clip, V3<T> normal,
+                                         Face<T, N>* tmp) {  // :108-175
     out.n = 0;
     if (toClip.n < 2 || clip.n < 3) return;
     int tmpIdx = 0;
     for (int i = 0; i < clip.n; i++) {
         V3<T> c1 = clip.v[i], c2 = clip.v[(i + 1) % clip.n];
         V3<T> cn = cross(normal, c2 - c1);
-        const Face<T>& src = (i == 0) ? toClip : tmp[tmpIdx];
+        const Face<T, N>& src = (i == 0) ? toClip : tmp[tmpIdx];
         tmpIdx ^= 1;
-        Face<T>& dst = (i == clip.n - 1) ? out : tmp[tmpIdx];
+        Face<T, N>& dst = (i == clip.n - 1) ? out : tmp[tmpIdx];
         clip_poly_vs_plane(dst, src, c1, cn);
         if (dst.n < 3) { out.n = 0; break; }
     }
 }
 
-template <typename T>
-__device__ inline void clip_poly_vs_edge(Face<T>& out, const Face<T>& poly, V3<T> ev1, V3<T> ev2, V3<T> clipNormal) {  // :190-363
+template <typename T, int N>
+__device__ inline void clip_poly_vs_edge(Face<T, N>& out, const Face<T, N>& poly, V3<T> ev1, V3<T> ev2, V3<T> clipNormal) {  // :190-363
     out.n = 0;
     if (poly.n < 3) return;
     V3<T> en = cross(clipNormal, ev2 - ev1);
@@ -936,7 +984,7 @@ __device__ inline void clip_poly_vs_edge(Face<T>& out, const Face<T>& poly, V3<T
             V3<T> cl = (den != T(0)) ? e1 + e12 * (prevNum / den) : e1;
             T pr = dot(cl - v1, v12);
             V3<T> o = pr < T(0) ? v1 : (pr > v12l ? v2 : cl);
-            if (out.n < CC_MAX_FACE) out.v[out.n++] = o;
+            if (out.n < N) out.v[out.n++] = o;
         }
         prevNum = num;
         prevIn = curIn;
@@ -945,24 +993,23 @@ __device__ inline void clip_poly_vs_edge(Face<T>& out, const Face<T>& poly, V3<T
 }
 
 // ------------------------------------------------------------- manifold
-#define CC_MANIFOLD_CAP 32
-template <typename T>
+template <typename T, int N = CC_MAX_FACE>
 struct Manifold {
     V3<T> base, normal;
     int n;
-    V3<T> relA[CC_MANIFOLD_CAP], relB[CC_MANIFOLD_CAP];
+    V3<T> relA[N], relB[N];
 };
 
 // manifoldBetweenTwoFaces manifold.ts:345-446
-template <typename T>
-__device__ inline void manifold_between_two_faces(Manifold<T>& out, V3<T> p1, V3<T> p2, V3<T> axis, T maxDist,
-                                                  const Face<T>& f1, const Face<T>& f2, Face<T>* scratch3) {
+template <typename T, int N>
+__device__ inline void manifold_between_two_faces(Manifold<T, N>& out, V3<T> p1, V3<T> p2, V3<T> axis, T maxDist,
+                                                  const Face<T, N>& f1, const Face<T, N>& f2, Face<T, N>* scratch3) {
     out.n = 0;
     if (f1.n < 2 || f2.n < 3) {
         out.relA[0] = p1 - out.base; out.relB[0] = p2 - out.base; out.n = 1;
         return;
     }
-    Face<T>& clipped = scratch3[2];
+    Face<T, N>& clipped = scratch3[2];
     clipped.n = 0;
     if (f1.n >= 3) clip_poly_vs_poly(clipped, f2, f1, axis, scratch3);
     else if (f1.n == 2) clip_poly_vs_edge(clipped, f2, f1.v[0], f1.v[1], axis);
@@ -980,7 +1027,7 @@ __device__ inline void manifold_between_two_faces(Manifold<T>& out, V3<T> p1, V3
                 out.relA[out.n] = q1 - out.base;
                 out.relB[out.n] = q2 - out.base;
                 out.n++;
-                if (out.n >= CC_MANIFOLD_CAP) break;
+                if (out.n >= N) break;
             }
         }
     }
@@ -988,12 +1035,12 @@ __device__ inline void manifold_between_two_faces(Manifold<T>& out, V3<T> p1, V3
 }
 
 // pruneContactPoints manifold.ts:192-313
-template <typename T>
-__device__ inline void prune(Manifold<T>& m, V3<T> axis) {
+template <typename T, int N>
+__device__ inline void prune(Manifold<T, N>& m, V3<T> axis) {
     if (m.n <= 4) return;
     const T MIN_D2 = T(1.0e-6);
-    T pd2[CC_MANIFOLD_CAP];
-    V3<T> proj[CC_MANIFOLD_CAP];
+    T pd2[N];
+    V3<T> proj[N];
     for (int i = 0; i < m.n; i++) {
         V3<T> v1 = m.relA[i], v2 = m.relB[i];
         T dp = dot(v1, axis);

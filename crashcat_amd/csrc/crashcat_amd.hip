@@ -84,13 +84,18 @@ __global__ void k_set_pair_count(Dev<T> d) {
         d.counters[CNT_EPA] = 0;
         d.counters[CNT_CONS] = 0;
         d.counters[CNT_HITS] = 0;
+        d.counters[CNT_HITS_L] = 0;
         if (n > d.pairCap) atomicOr(&d.counters[CNT_OVERFLOW], 1);
     }
 }
 
 template <typename T>
 __global__ void k_count_epa(Dev<T> d) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) d.counters[CNT_EPA_TOTAL] += d.counters[CNT_EPA];
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        d.counters[CNT_EPA_TOTAL] += d.counters[CNT_EPA];
+        // the two hit classes fill the list from either end
+        if ((long long)d.counters[CNT_HITS] + d.counters[CNT_HITS_L] > d.hitCap) atomicOr(&d.counters[CNT_OVERFLOW], 16);
+    }
 }
 
 // body initialisation at upload: COM, AABB, sleep test spheres (rigid-body.ts:390-434)
@@ -100,6 +105,11 @@ __global__ void k_init_bodies(Dev<T> d) {
     if (i >= d.N) return;
     set_transform(d.B, d.st, i);
     reset_sleep_timer(d.B, d.st, i);
+    {
+        const ShapeD<T>& s = d.st.shapes[d.B.shape[i]];
+        d.bBrad[i] = s.brad;
+        d.bTinfo[i] = s.type | (s.mfv << 4);
+    }
     d.B.wake[i] = 0;
     d.B.label[i] = -1;
 }
@@ -284,7 +294,8 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         if (q.type < 0 || q.type > 3) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
         ShapeD<T>& o = sh[i];
         o.type = q.type; o.hfp = q.hull_first_point; o.hnp = q.hull_num_points; o.hff = q.hull_first_face;
-        o.hnf = q.hull_num_faces; o.pad = 0;
+        o.hnf = q.hull_num_faces;
+        o.mfv = q.type == CC_SHAPE_BOX ? 4 : (q.type == CC_SHAPE_CAPSULE ? 2 : 0);
         o.radius = T(q.radius); o.hh = T(q.half_height); o.cr = T(q.convex_radius);
         for (int k = 0; k < 3; k++) {
             o.he[k] = T(q.half_extents[k]); o.com[k] = T(q.center_of_mass[k]);
@@ -313,8 +324,18 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     for (int i = 0; i < ns; i++)
         if (shapes[i].type == CC_SHAPE_CONVEX_HULL &&
             (shapes[i].hull_first_point + shapes[i].hull_num_points > P || shapes[i].hull_first_face + shapes[i].hull_num_faces > F ||
-             shapes[i].hull_num_points > 255))
+             shapes[i].hull_num_points > 255 || shapes[i].hull_first_face < 0 || shapes[i].hull_num_faces < 0))
             return fail(c, CC_E_ARG, "hull data out of range");
+    for (int i = 0; i < ns; i++) {  // a hull's supporting face takes at most 16 of a face's vertices
+        if (shapes[i].type != CC_SHAPE_CONVEX_HULL) continue;
+        int mfv = 0;
+        for (int f = shapes[i].hull_first_face; f < shapes[i].hull_first_face + shapes[i].hull_num_faces; f++) {
+            int nv = hulls->face_num_vertices[f];
+            int delta = (nv + 15) / 16;
+            mfv = std::max(mfv, delta > 0 ? (nv + delta - 1) / delta : 0);
+        }
+        sh[i].mfv = mfv;
+    }
     std::vector<T> hp(3 * std::max(P, 1)), hs(3 * std::max(P, 1)), hn(3 * std::max(F, 1));
     for (int i = 0; i < 3 * P; i++) { hp[i] = T(hulls->points[i]); hs[i] = T(hulls->shrunk_points[i]); }
     for (int i = 0; i < 3 * F; i++) hn[i] = T(hulls->plane_normals[i]);
@@ -450,6 +471,9 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.gLargeOff = A.get<int>(nb + 1);
     d.gLarge = A.get<int>(nb);
     d.npb = A.get<NpBody<T>>(nb);
+    d.npd = A.get<NpDyn<T>>(nb);
+    d.bBrad = A.get<T>(nb);
+    d.bTinfo = A.get<int>(nb);
     d.bpCount = A.get<int>(nb + 1);
     d.bpOff = A.get<int>(nb + 1);
     d.bpSlots = A.get<int>((size_t)nb * BP_SLOTS);
@@ -481,7 +505,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.hitAxis = A.get<V3<T>>(d.hitCap);
     d.epaPair = A.get<int>(d.epaCap);
     d.epaSimplex = A.get<Simplex<T>>(d.epaCap);
-    d.epaWork = A.get<EpaState<T>>(d.epaWorkers);
+    d.epaWork = A.get<char>((size_t)((d.epaWorkers + EPA_LANES - 1) / EPA_LANES) * EpaWave<T>::bytes);
     size_t L = (size_t)d.conCap + npc;
     d.skeys = A.get<unsigned long long>(2 * L);
     d.seq = A.get<int>(L);
@@ -640,7 +664,8 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
         hipLaunchKernelGGL(k_epa<T>, dim3((d.epaWorkers + 63) / 64), dim3(64), 0, st, d, dt);
         hipLaunchKernelGGL(k_count_epa<T>, dim3(1), dim3(64), 0, st, d);
         mark();  // EPA
-        hipLaunchKernelGGL(k_np_manifold<T>, dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
+        hipLaunchKernelGGL((k_np_manifold<T, CC_SMALL_FACE>), dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
+        hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
         mark();  // manifolds, contact cache, constraint setup
         hipLaunchKernelGGL(k_wake<T>, dim3(nbB), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_con_flags<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
