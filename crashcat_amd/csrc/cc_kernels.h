@@ -58,6 +58,10 @@ struct Dev {
     int* gLargeFlag;    // [N + 1]
     int* gLargeOff;     // [N + 1] exclusive scan of gLargeFlag (bodies are world-ordered)
     int* gLarge;        // [N] large bodies, grouped by world
+    int* gjkList;       // [pairCap] pair slots that passed the filter
+    int* gjkCombo;      // [pairCap] their (shape A type, shape B type) as typeA * 4 + typeB
+    int* gjkSorted;     // [pairCap] the same slots ordered by type pair, for GJK
+    int* gjkHist;       // [2 * 16] per type pair: count, then next free position
     NpBody<T>* npb;     // [N] narrowphase records of each body
     NpDyn<T>* npd;      // [N]
     T* bBrad;           // [N] bounding radius of each body's shape (fixed at upload)
@@ -139,7 +143,7 @@ enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
        ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_N = 16 };
 
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
-       CNT_HITS = 7, CNT_HITS_L = 8, CNT_N = 9 };
+       CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_N = 10 };
 // CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list, 16 hits
 constexpr int BP_SLOTS = 32;
 
@@ -755,32 +759,113 @@ __device__ inline void np_pair(const Dev<T>& d, int slot, NpBody<T>& RA, NpBody<
     RB = sw ? r0 : r1;
 }
 
-// One pair per lane: sphere pairs in closed form; pairs whose bounding spheres are further
-// apart than the speculative distance stop at once (GJK would report them separated); the rest
-// run the GJK step of collideConvexVsConvexLocal (convex.ts:257-641): a hit for k_np_manifold,
-// or, when the cores overlap, a job for EPA.
+// Position of this thread's entry when the flagged threads of the block append one entry each
+// to a list whose length is *counter: one global atomic per block.  Every thread of the block
+// calls it; sh holds BS / 64 + 1 ints.
+template <int BS>
+__device__ inline int block_append(int* counter, bool flag, int* sh) {
+    const unsigned long long m = __ballot(flag);
+    const int lane = (int)__lane_id(), wave = threadIdx.x >> 6;
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) sh[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < BS / 64; w++) { int c = sh[w]; sh[w] = tot; tot += c; }
+        sh[BS / 64] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    const int r = flag ? sh[BS / 64] + sh[wave] + rank : -1;
+    __syncthreads();
+    return r;
+}
+
+// The pair filter: sphere pairs in closed form (collideSphereVsSphere sphere.ts:319-410); pairs
+// whose bounding spheres are further apart than the speculative distance stop here (GJK would
+// report them separated at once); the rest are listed for GJK with their shape-type pair.
+template <typename T>
+__global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
+    __shared__ int sh[256 / 64 + 1];
+    const int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
+    for (int base = blockIdx.x * 256; base < npairs; base += gridDim.x * 256) {  // the same trip count in every thread
+        const int slot = base + threadIdx.x;
+        bool gjk = false;
+        int combo = 0;
+        if (slot < npairs) {
+            d.pairCon[slot] = -1;
+            NpBody<T> RA, RB;
+            np_pair(d, slot, RA, RB);
+            const int ta = RA.tinfo & 15, tb = RB.tinfo & 15;
+            if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) {
+                V3<T> pa = RA.pos, pbp = RB.pos;
+                T ra = d.st.shapes[RA.shape].radius, rb = d.st.shapes[RB.shape].radius;
+                V3<T> dd = pbp - pa;
+                T d2 = len2(dd), rs = ra + rb;
+                if (d2 < rs * rs) {
+                    T dist = sqrt(d2);
+                    V3<T> n = dist > T(0) ? dd * (T(1) / dist) : mk3<T>(0, 1, 0);
+                    push_hit(d, slot, HIT_SPHERES, pa + n * ra, pbp - n * rb, n, true);
+                }
+            } else {
+                T reach = RA.brad + RB.brad + d.s.spec;
+                gjk = !(len2(RB.pos - RA.pos) > reach * reach);
+                combo = ta * 4 + tb;
+            }
+        }
+        const int idx = block_append<256>(&d.counters[CNT_GJK], gjk, sh);
+        if (gjk) {
+            d.gjkList[idx] = slot;
+            d.gjkCombo[idx] = combo;
+        }
+    }
+}
+
+// counting sort of the GJK list by type pair: counts, then positions
+template <typename T>
+__global__ void __launch_bounds__(256) k_gjk_count(Dev<T> d) {
+    __shared__ int h[16];
+    if (threadIdx.x < 16) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int n = min(d.counters[CNT_GJK], d.pairCap);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) atomicAdd(&h[d.gjkCombo[k]], 1);
+    __syncthreads();
+    if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&d.gjkHist[threadIdx.x], h[threadIdx.x]);
+}
+template <typename T>
+__global__ void k_gjk_offsets(Dev<T> d) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int acc = 0;
+        for (int c = 0; c < 16; c++) { int v = d.gjkHist[c]; d.gjkHist[16 + c] = acc; acc += v; }
+    }
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_gjk_scatter(Dev<T> d) {
+    __shared__ int h[16], b[16];
+    const int n = min(d.counters[CNT_GJK], d.pairCap);
+    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {  // the same trip count in every thread
+        if (threadIdx.x < 16) h[threadIdx.x] = 0;
+        __syncthreads();
+        const int k = base + threadIdx.x;
+        const int c = k < n ? d.gjkCombo[k] : -1;
+        const int r = c >= 0 ? atomicAdd(&h[c], 1) : 0;
+        __syncthreads();
+        if (threadIdx.x < 16) b[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&d.gjkHist[16 + threadIdx.x], h[threadIdx.x]) : 0;
+        __syncthreads();
+        if (c >= 0) d.gjkSorted[b[c] + r] = d.gjkList[k];
+        __syncthreads();
+    }
+}
+
+// GJK of each listed pair, in type-pair order so that a wavefront's lanes run the same support
+// functions: the GJK step of collideConvexVsConvexLocal (convex.ts:257-641), giving a hit for
+// k_np_manifold or, when the cores overlap, a job for EPA.
 template <typename T>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2) k_np_gjk(Dev<T> d, T dt) {
-    int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
-    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < npairs; slot += gridDim.x * blockDim.x) {
-        d.pairCon[slot] = -1;
+    const int n = min(d.counters[CNT_GJK], d.pairCap);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int slot = d.gjkSorted[k];
         NpBody<T> RA, RB;
         np_pair(d, slot, RA, RB);
-        if ((RA.tinfo & 15) == SHAPE_SPHERE && (RB.tinfo & 15) == SHAPE_SPHERE) {  // collideSphereVsSphere sphere.ts:319-410
-            V3<T> pa = RA.pos, pbp = RB.pos;
-            T ra = d.st.shapes[RA.shape].radius, rb = d.st.shapes[RB.shape].radius;
-            V3<T> dd = pbp - pa;
-            T d2 = len2(dd), rs = ra + rb;
-            if (d2 >= rs * rs) continue;
-            T dist = sqrt(d2);
-            V3<T> n = dist > T(0) ? dd * (T(1) / dist) : mk3<T>(0, 1, 0);
-            push_hit(d, slot, HIT_SPHERES, pa + n * ra, pbp - n * rb, n, true);
-            continue;
-        }
-        {
-            T reach = RA.brad + RB.brad + d.s.spec;
-            if (len2(RB.pos - RA.pos) > reach * reach) continue;
-        }
         ConvexCtx<T> c;
         convex_setup(c, RA.pos, RA.q, RB.pos, RB.q, d.s.spec);
         Sup<T> SA, SB;

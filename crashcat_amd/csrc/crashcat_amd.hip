@@ -85,6 +85,8 @@ __global__ void k_set_pair_count(Dev<T> d) {
         d.counters[CNT_CONS] = 0;
         d.counters[CNT_HITS] = 0;
         d.counters[CNT_HITS_L] = 0;
+        d.counters[CNT_GJK] = 0;
+        for (int c = 0; c < 32; c++) d.gjkHist[c] = 0;
         if (n > d.pairCap) atomicOr(&d.counters[CNT_OVERFLOW], 1);
     }
 }
@@ -497,6 +499,10 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         d.hkeys[k] = A.get<unsigned long long>(d.hashCap);
         d.hvals[k] = A.get<int>(d.hashCap);
     }
+    d.gjkList = A.get<int>(d.pairCap);
+    d.gjkCombo = A.get<int>(d.pairCap);
+    d.gjkSorted = A.get<int>(d.pairCap);
+    d.gjkHist = A.get<int>(32);
     d.hitCap = d.conCap + d.conCap / 8 + 1024;
     d.hitSlot = A.get<int>(d.hitCap);
     d.hitKind = A.get<int>(d.hitCap);
@@ -659,6 +665,10 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
         hipLaunchKernelGGL(k_accel<T>, dim3(nbB), dim3(256), 0, st, d, dt);
         launch_broadphase<T>(X, st, margin);
         mark();  // broadphase
+        hipLaunchKernelGGL(k_np_filter<T>, dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
+        hipLaunchKernelGGL(k_gjk_count<T>, dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_gjk_offsets<T>, dim3(1), dim3(64), 0, st, d);
+        hipLaunchKernelGGL(k_gjk_scatter<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_np_gjk<T>, dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
         mark();  // narrowphase (sphere pairs, GJK)
         hipLaunchKernelGGL(k_epa<T>, dim3((d.epaWorkers + 63) / 64), dim3(64), 0, st, d, dt);
