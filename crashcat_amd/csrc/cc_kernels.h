@@ -116,6 +116,8 @@ struct Dev {
     int* sched;
     int* lvlStart;      // [conCap + NPC + W + 1]
     int* blist;         // [2 * (conCap + NPC)]
+    unsigned long long* blkey;  // [4 * (conCap + NPC)] solve-order key (hi, lo) of each blist entry
+    int worldLdsInts;   // ints of dynamic LDS each world's workgroup has
     int* bcnt;          // [N]
     int* boff;          // [N]
     // the world's constraints copied in level order for the sweeps (element so + x; its points

@@ -445,19 +445,34 @@ struct SolveLess {
         lo = K[2 * k + 1];
     }
 };
+// the same keys, read from the copy kept beside each list entry (x: the entry's position)
+struct SolveLessAt {
+    const unsigned long long* BK;
+    __device__ void key_at(int x, unsigned long long& hi, unsigned long long& lo) const {
+        hi = BK[2 * x];
+        lo = BK[2 * x + 1];
+    }
+};
 
 __device__ inline bool key_lt(unsigned long long h0, unsigned long long l0, unsigned long long h1, unsigned long long l1) {
     return h0 < h1 || (h0 == h1 && l0 < l1);
 }
+
+// key of the list entry at position x holding element k (the solve order reads the copy
+// stored beside the entry)
+template <typename Less>
+__device__ inline void list_key(const Less& less, int x, int k, unsigned long long& h, unsigned long long& l) { less.key(k, h, l); }
+__device__ inline void list_key(const SolveLessAt& less, int x, int k, unsigned long long& h, unsigned long long& l) { less.key_at(x, h, l); }
 
 // Each body's list of sequence elements in the given order -> predecessor / successor on that
 // body.  A list entry is k << 1 | side (side 1: the body is the element's B body).  Keys are
 // unique, so the order is the same whichever way it is sorted: lists of up to eight entries
 // are sorted in registers by a 19-comparator network, longer ones by insertion sort.
 template <typename T, int BS, typename Less>
-__device__ inline void link_body_lists(const Dev<T>& d, int b0, int nb, int so, int* BL, Less less) {
+__device__ inline void link_body_lists(const Dev<T>& d, int b0, int nb, int so, int* BL, unsigned long long* BK, const int* BC,
+                                       Less less) {
     for (int i = threadIdx.x; i < nb; i += BS) {
-        const int o = d.boff[b0 + i], c = d.bcnt[b0 + i];
+        const int o = d.boff[b0 + i], c = BC[i];
         auto link = [&](int ent, int pv, int nx) {
             const int k = ent >> 1;
             if (ent & 1) { d.prevB[so + k] = pv; d.nextB[so + k] = nx; }
@@ -470,7 +485,7 @@ __device__ inline void link_body_lists(const Dev<T>& d, int b0, int nb, int so, 
             for (int j = 0; j < 8; j++) v[j] = j < c ? BL[o + j] : -1;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                if (j < c) less.key(v[j] >> 1, kh[j], kl[j]);
+                if (j < c) list_key(less, o + j, v[j] >> 1, kh[j], kl[j]);
                 else kh[j] = kl[j] = ~0ull;
             }
 #define CC_CX(p, q)                                                              \
@@ -492,19 +507,24 @@ __device__ inline void link_body_lists(const Dev<T>& d, int b0, int nb, int so, 
             for (int j = 0; j < 8; j++)
                 if (j < c) link(v[j], j > 0 ? v[j - 1] >> 1 : -1, j + 1 < c ? v[j + 1] >> 1 : -1);
         } else {
-            for (int x = o + 1; x < o + c; x++) {  // insertion sort
+            for (int x = o + 1; x < o + c; x++) {  // insertion sort; the stored keys move with the entries
                 const int ent = BL[x];
+                const unsigned long long bh = BK[2 * x], bl = BK[2 * x + 1];
                 unsigned long long h, l;
-                less.key(ent >> 1, h, l);
+                list_key(less, x, ent >> 1, h, l);
                 int y = x - 1;
                 while (y >= o) {
                     unsigned long long hy, ly;
-                    less.key(BL[y] >> 1, hy, ly);
+                    list_key(less, y, BL[y] >> 1, hy, ly);
                     if (!key_lt(h, l, hy, ly)) break;
                     BL[y + 1] = BL[y];
+                    BK[2 * (y + 1)] = BK[2 * y];
+                    BK[2 * (y + 1) + 1] = BK[2 * y + 1];
                     y--;
                 }
                 BL[y + 1] = ent;
+                BK[2 * (y + 1)] = bh;
+                BK[2 * (y + 1) + 1] = bl;
             }
             for (int x = o; x < o + c; x++)
                 link(BL[x], x > o ? BL[x - 1] >> 1 : -1, x + 1 < o + c ? BL[x + 1] >> 1 : -1);
@@ -515,17 +535,41 @@ __device__ inline void link_body_lists(const Dev<T>& d, int b0, int nb, int so, 
 
 // Kahn's algorithm on the predecessor DAG: level l holds the elements whose longest predecessor
 // chain has length l, so elements of one level share no dynamic body; returns the level count
+// When DEG is set (LDS), the in-degrees (at most 2) are packed 2 bits each, 16 per word; a
+// decrement never borrows from a neighbouring field because no field is decremented below 0.
 template <typename T, int BS>
-__device__ inline int build_levels(const Dev<T>& d, int so, int L, int* LS, int& sTail) {
+__device__ inline int build_levels(const Dev<T>& d, int so, int L, int* LS, int& sTail, unsigned* DEG) {
     const int tid = threadIdx.x;
     if (tid == 0) sTail = 0;
     __syncthreads();
-    for (int k = tid; k < L; k += BS) {
-        int deg = (d.prevA[so + k] >= 0 ? 1 : 0) + (d.prevB[so + k] >= 0 ? 1 : 0);
-        d.level[so + k] = deg;
-        if (deg == 0) d.sched[so + atomicAdd(&sTail, 1)] = k;
+    if (DEG) {
+        for (int wd = tid; wd < (L + 15) >> 4; wd += BS) {
+            unsigned word = 0;
+            for (int j = 0; j < 16; j++) {
+                const int k = (wd << 4) + j;
+                if (k >= L) break;
+                const int deg = (d.prevA[so + k] >= 0 ? 1 : 0) + (d.prevB[so + k] >= 0 ? 1 : 0);
+                word |= (unsigned)deg << (2 * j);
+                if (deg == 0) d.sched[so + atomicAdd(&sTail, 1)] = k;
+            }
+            DEG[wd] = word;
+        }
+    } else {
+        for (int k = tid; k < L; k += BS) {
+            int deg = (d.prevA[so + k] >= 0 ? 1 : 0) + (d.prevB[so + k] >= 0 ? 1 : 0);
+            d.level[so + k] = deg;
+            if (deg == 0) d.sched[so + atomicAdd(&sTail, 1)] = k;
+        }
     }
     __syncthreads();
+    auto ready = [&](int e) -> bool {  // one predecessor of e done: was it the last?
+        if (DEG) {
+            const unsigned sh = 2u * (unsigned)(e & 15);
+            const unsigned old = atomicSub(&DEG[e >> 4], 1u << sh);
+            return ((old >> sh) & 3u) == 1u;
+        }
+        return atomicSub(&d.level[so + e], 1) == 1;
+    };
     int lo = 0, hi = sTail;
     int D = 0;
     __syncthreads();  // every lane has read the first frontier before any lane extends it
@@ -534,8 +578,8 @@ __device__ inline int build_levels(const Dev<T>& d, int so, int L, int* LS, int&
         for (int x = lo + tid; x < hi; x += BS) {
             int e = d.sched[so + x];
             int na = d.nextA[so + e], nbx = d.nextB[so + e];
-            if (na >= 0 && atomicSub(&d.level[so + na], 1) == 1) d.sched[so + atomicAdd(&sTail, 1)] = na;
-            if (nbx >= 0 && atomicSub(&d.level[so + nbx], 1) == 1) d.sched[so + atomicAdd(&sTail, 1)] = nbx;
+            if (na >= 0 && ready(na)) d.sched[so + atomicAdd(&sTail, 1)] = na;
+            if (nbx >= 0 && ready(nbx)) d.sched[so + atomicAdd(&sTail, 1)] = nbx;
         }
         __syncthreads();
         lo = hi;
@@ -554,6 +598,7 @@ template <typename T, int BS>
 __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __shared__ int sh[BS / 64 + 1];
     __shared__ int sLS[LS_LDS];
+    extern __shared__ int dynLds[];  // d.worldLdsInts ints: per-body counts, then packed in-degrees
     __shared__ int sTail;
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
@@ -604,25 +649,43 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         d.nextA[so + k] = -1;
         d.nextB[so + k] = -1;
     }
-    for (int i = tid; i < nb; i += BS) d.bcnt[b0 + i] = 0;
+    // per-body counts in LDS when the world's bodies fit, then the packed in-degrees
+    const bool cntLds = nb <= d.worldLdsInts;
+    int* BC = cntLds ? dynLds : d.bcnt + b0;
+    const int degOff = cntLds ? ((nb + 3) & ~3) : 0;
+    unsigned* DEG = degOff + ((L + 15) >> 4) <= d.worldLdsInts ? (unsigned*)(dynLds + degOff) : nullptr;
+    for (int i = tid; i < nb; i += BS) BC[i] = 0;
     __syncthreads();
-    // ---- 2. per-body lists of sequence elements (counting sort by dynamic body)
+    // ---- 2. per-body lists of sequence elements (counting sort by dynamic body); each entry
+    // keeps a copy of its element's solve-order key
     for (int k = tid; k < L; k += BS) {
         int ea = d.seqA[so + k], eb = d.seqB[so + k];
-        if (ea >= 0) atomicAdd(&d.bcnt[ea], 1);
-        if (eb >= 0) atomicAdd(&d.bcnt[eb], 1);
+        if (ea >= 0) atomicAdd(&BC[ea - b0], 1);
+        if (eb >= 0) atomicAdd(&BC[eb - b0], 1);
     }
     __syncthreads();
-    for (int i = tid; i < nb; i += BS) d.boff[b0 + i] = d.bcnt[b0 + i];
+    for (int i = tid; i < nb; i += BS) d.boff[b0 + i] = BC[i];
     __syncthreads();
     block_scan_array<BS>(d.boff + b0, nb, sh);
-    for (int i = tid; i < nb; i += BS) d.bcnt[b0 + i] = 0;
+    for (int i = tid; i < nb; i += BS) BC[i] = 0;
     __syncthreads();
     int* BL = d.blist + 2 * (size_t)so;
+    unsigned long long* BK = d.blkey + 4 * (size_t)so;
     for (int k = tid; k < L; k += BS) {
         int ea = d.seqA[so + k], eb = d.seqB[so + k];
-        if (ea >= 0) BL[d.boff[ea] + atomicAdd(&d.bcnt[ea], 1)] = k << 1;
-        if (eb >= 0) BL[d.boff[eb] + atomicAdd(&d.bcnt[eb], 1)] = (k << 1) | 1;
+        const unsigned long long kh = K[2 * k], kl = K[2 * k + 1];
+        if (ea >= 0) {
+            const int x = d.boff[ea] + atomicAdd(&BC[ea - b0], 1);
+            BL[x] = k << 1;
+            BK[2 * x] = kh;
+            BK[2 * x + 1] = kl;
+        }
+        if (eb >= 0) {
+            const int x = d.boff[eb] + atomicAdd(&BC[eb - b0], 1);
+            BL[x] = (k << 1) | 1;
+            BK[2 * x] = kh;
+            BK[2 * x + 1] = kl;
+        }
     }
     __syncthreads();
     // ---- 3-5. warm start (contact-constraints.ts:906-955, constraints.ts:264-301).  The FP64
@@ -650,12 +713,12 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     constexpr bool kCreationOrderWarmStart = sizeof(T) == 8;
     int D = 0;
     if (kCreationOrderWarmStart && d.s.warmStarting) {
-        link_body_lists<T, BS>(d, b0, nb, so, BL, CreationLess<T>{&d, so});
-        D = build_levels<T, BS>(d, so, L, LS, sTail);
+        link_body_lists<T, BS>(d, b0, nb, so, BL, BK, BC, CreationLess<T>{&d, so});
+        D = build_levels<T, BS>(d, so, L, LS, sTail, DEG);
         warm_start_levels<T, BS, false>(d, so, D, LS, ratio);
     }
-    link_body_lists<T, BS>(d, b0, nb, so, BL, SolveLess{K});
-    D = build_levels<T, BS>(d, so, L, LS, sTail);
+    link_body_lists<T, BS>(d, b0, nb, so, BL, BK, BC, SolveLessAt{BK});
+    D = build_levels<T, BS>(d, so, L, LS, sTail, DEG);
     if (tid == 0) d.stats[2 * w] = D;
     // level starts in LDS when they fit
     const bool lsLds = D + 1 <= LS_LDS;

@@ -163,6 +163,10 @@ struct CtxT : Impl {
     std::vector<int> hostWStart;
     std::vector<hipEvent_t> ev;  // 7 per step of the last cc_step call
     int solverBS = 256;
+    int device = 0;
+    size_t worldLdsNeed = 0;   // ints of LDS the largest world can use
+    size_t worldLdsBytes = 0;  // dynamic LDS given to the world kernel
+    int worldLdsBS = 0;        // block size worldLdsBytes was sized for
     ~CtxT() {
         mem.release();
         for (auto& e : ev) (void)hipEventDestroy(e);
@@ -525,6 +529,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.sched = A.get<int>(L);
     d.lvlStart = A.get<int>(2 * (L + nw + 1));
     d.blist = A.get<int>(2 * L);
+    d.blkey = A.get<unsigned long long>(4 * L);
     d.svCap = (int)L;
     d.svSlot = A.get<int>(L);
     d.ve = A.get<VElem<T>>(L);
@@ -547,6 +552,14 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         X->scanSums = A.get<int>((size_t)(n / 4096 + n / (4096ll * 4096) + 64));
     }
     X->maxWorld = maxWorld;
+    X->device = c->cfg.device;
+    X->worldLdsBS = 0;
+    X->worldLdsNeed = 0;
+    for (int w = 0; w < nw; w++) {  // per-body counts + in-degrees packed 16 per int
+        size_t bw = (size_t)(woff[w + 1] - woff[w]);
+        size_t lw = (size_t)c->cfg.max_contacts_per_body * bw + (size_t)(pcStart[w + 1] - pcStart[w]);
+        X->worldLdsNeed = std::max(X->worldLdsNeed, ((bw + 3) & ~(size_t)3) + (lw + 15) / 16);
+    }
     if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (workspace)");
     HIPCHK(c, hipMemset(d.counters, 0, CNT_N * sizeof(int)));
     HIPCHK(c, hipMemset(d.prevDt, 0, sizeof(T)));
@@ -624,15 +637,38 @@ static void launch_broadphase(CtxT<T>* X, hipStream_t st, double margin) {
     hipLaunchKernelGGL(k_bp_spill<T>, dim3(grid_for(d.spillCap, 256, 1024)), dim3(256), 0, st, d);
 }
 
+// dynamic LDS of the world kernel: what the largest world can use (its per-body counts and
+// packed in-degrees), within what the device leaves beside the kernel's static LDS
+template <typename T, int BS>
+static void launch_world_bs(CtxT<T>* X, hipStream_t st, T dt) {
+    Dev<T>& d = X->d;
+    if (X->worldLdsBS != BS) {
+        hipFuncAttributes fa;
+        int maxShared = 0;
+        size_t dyn = 0;
+        if (hipFuncGetAttributes(&fa, (const void*)k_world<T, BS>) == hipSuccess &&
+            hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, X->device) == hipSuccess &&
+            (size_t)maxShared > fa.sharedSizeBytes) {
+            size_t avail = ((size_t)maxShared - fa.sharedSizeBytes) / 4;
+            dyn = std::min(avail, X->worldLdsNeed) * 4;
+        }
+        if (dyn > 0 && hipFuncSetAttribute((const void*)k_world<T, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
+            dyn = 0;
+        X->worldLdsBytes = dyn;
+        X->worldLdsBS = BS;
+    }
+    d.worldLdsInts = (int)(X->worldLdsBytes / 4);
+    hipLaunchKernelGGL((k_world<T, BS>), dim3(d.W), dim3(BS), X->worldLdsBytes, st, d, dt);
+}
+
 template <typename T>
 static void launch_world(CtxT<T>* X, hipStream_t st, T dt) {
-    Dev<T>& d = X->d;
     switch (X->solverBS) {
-        case 64: hipLaunchKernelGGL((k_world<T, 64>), dim3(d.W), dim3(64), 0, st, d, dt); break;
-        case 128: hipLaunchKernelGGL((k_world<T, 128>), dim3(d.W), dim3(128), 0, st, d, dt); break;
-        case 512: hipLaunchKernelGGL((k_world<T, 512>), dim3(d.W), dim3(512), 0, st, d, dt); break;
-        case 1024: hipLaunchKernelGGL((k_world<T, 1024>), dim3(d.W), dim3(1024), 0, st, d, dt); break;
-        default: hipLaunchKernelGGL((k_world<T, 256>), dim3(d.W), dim3(256), 0, st, d, dt); break;
+        case 64: launch_world_bs<T, 64>(X, st, dt); break;
+        case 128: launch_world_bs<T, 128>(X, st, dt); break;
+        case 512: launch_world_bs<T, 512>(X, st, dt); break;
+        case 1024: launch_world_bs<T, 1024>(X, st, dt); break;
+        default: launch_world_bs<T, 256>(X, st, dt); break;
     }
 }
 
