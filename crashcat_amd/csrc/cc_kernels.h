@@ -81,6 +81,7 @@ struct Dev {
     int* counters;      // device counters, see CNT_*
     // contact constraints (slots allocated in creation order by an atomic counter)
     Con<T>* con;        // [conCap]
+    ConHead* conHead;   // [conCap]
     ConTap<T>* tap;     // [conCap]
     int* wConStart;     // [W + 1] range of each world in conIdx
     int* conIdx;        // [conCap] constraint slots in pair order (so grouped by world)
@@ -114,6 +115,7 @@ struct Dev {
     int* nextB;
     int* level;         // in-degree during the level construction
     int* sched;
+    int* elemNp;        // [conCap + NPC] points of each sequence element (0: not a contact)
     int* lvlStart;      // [conCap + NPC + W + 1]
     int* blist;         // [2 * (conCap + NPC)]
     unsigned long long* blkey;  // [4 * (conCap + NPC)] solve-order key (hi, lo) of each blist entry
@@ -696,6 +698,12 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
         rc.lt1[i] = cp.lT1;
         rc.lt2[i] = cp.lT2;
     }
+    ConHead hd;
+    hd.a = ia;
+    hd.b = ib;
+    hd.sortKey = k.sortKey;
+    hd.bits = k.dynMask | (k.movMask << 2) | (k.np << 4);
+    d.conHead[cs] = hd;
     if (dynA && DA.sleeping) B.wake[ia] = 1;
     if (dynB && DB.sleeping) B.wake[ib] = 1;
 }
@@ -995,7 +1003,7 @@ __global__ void k_con_compact(Dev<T> d) {
         int c = d.pairCon[p];
         if (c >= 0) {
             d.conIdx[d.conPos[p]] = c;
-            pts += (unsigned long long)d.con[c].np;
+            pts += (unsigned long long)(d.conHead[c].bits >> 4);
         }
     }
     for (int w = blockIdx.x * blockDim.x + threadIdx.x; w <= d.W; w += gridDim.x * blockDim.x)
@@ -1048,7 +1056,7 @@ __global__ void k_uf_link(Dev<T> d) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc + d.NPC; k += gridDim.x * blockDim.x) {
         int a, b;
         if (k < nc) {
-            const Con<T>& c = d.con[k];
+            const ConHead c = d.conHead[k];
             a = c.a; b = c.b;
         } else {
             const PointCon<T>& pc = d.pc[k - nc];
@@ -1077,7 +1085,7 @@ __global__ void k_hash_insert(Dev<T> d) {
     int* hv = d.hvals[par];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         int slot = k;
-        const Con<T>& c = d.con[slot];
+        const ConHead c = d.conHead[slot];
         unsigned long long key = pair_key(c.a, c.b);
         unsigned h = hash64(key) & (unsigned)(d.hashCap - 1);
         for (int probe = 0; probe < d.hashCap; probe++) {

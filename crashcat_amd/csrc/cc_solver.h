@@ -464,6 +464,25 @@ template <typename Less>
 __device__ inline void list_key(const Less& less, int x, int k, unsigned long long& h, unsigned long long& l) { less.key(k, h, l); }
 __device__ inline void list_key(const SolveLessAt& less, int x, int k, unsigned long long& h, unsigned long long& l) { less.key_at(x, h, l); }
 
+// The same order as one 64-bit key inside the list of one body.  Creation keys are 64-bit
+// already.  A solve key (hi, lo = idA << 32 | idB with idA < idB) of a contact of this body
+// orders like (hi, the other body): the other body is below this body when this body is B and
+// above it when this body is A.  Point constraints (hi = 2^32) follow every contact.
+template <typename Less>
+__device__ inline unsigned long long list_key64(const Less& less, int x, int k, int body) {
+    unsigned long long h, l;
+    less.key(k, h, l);
+    return h;
+}
+__device__ inline unsigned long long list_key64(const SolveLessAt& less, int x, int k, int body) {
+    unsigned long long h, l;
+    less.key_at(x, h, l);
+    if (h >> 32) return (1ull << 63) | (l & 0x7fffffffull);
+    const unsigned a = (unsigned)(l >> 32), b = (unsigned)l;
+    const unsigned other = a == (unsigned)body ? b : a;
+    return (h << 31) | other;
+}
+
 // Each body's list of sequence elements in the given order -> predecessor / successor on that
 // body.  A list entry is k << 1 | side (side 1: the body is the element's B body).  Keys are
 // unique, so the order is the same whichever way it is sorted: lists of up to eight entries
@@ -478,33 +497,39 @@ __device__ inline void link_body_lists(const Dev<T>& d, int b0, int nb, int so, 
             if (ent & 1) { d.prevB[so + k] = pv; d.nextB[so + k] = nx; }
             else { d.prevA[so + k] = pv; d.nextA[so + k] = nx; }
         };
-        if (c <= 8) {
-            int v[8];
-            unsigned long long kh[8], kl[8];
+        if (c <= 16) {
+            int v[16];
+            unsigned long long kk[16];
 #pragma unroll
-            for (int j = 0; j < 8; j++) v[j] = j < c ? BL[o + j] : -1;
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                if (j < c) list_key(less, o + j, v[j] >> 1, kh[j], kl[j]);
-                else kh[j] = kl[j] = ~0ull;
+            for (int j = 0; j < 16; j++) {
+                v[j] = -1;
+                kk[j] = ~0ull;
+                if (j < c) {
+                    v[j] = BL[o + j];
+                    kk[j] = list_key64(less, o + j, v[j] >> 1, b0 + i);
+                }
             }
-#define CC_CX(p, q)                                                              \
-    {                                                                            \
-        const bool sw = key_lt(kh[q], kl[q], kh[p], kl[p]);                      \
-        const unsigned long long th = kh[p], tl = kl[p];                         \
-        const int tv = v[p];                                                     \
-        kh[p] = sw ? kh[q] : kh[p]; kl[p] = sw ? kl[q] : kl[p]; v[p] = sw ? v[q] : v[p]; \
-        kh[q] = sw ? th : kh[q]; kl[q] = sw ? tl : kl[q]; v[q] = sw ? tv : v[q];  \
+#define CC_CX(p, q)                                      \
+    {                                                    \
+        const bool sw = kk[q] < kk[p];                   \
+        const unsigned long long tk = kk[p];             \
+        const int tv = v[p];                             \
+        kk[p] = sw ? kk[q] : kk[p]; v[p] = sw ? v[q] : v[p]; \
+        kk[q] = sw ? tk : kk[q]; v[q] = sw ? tv : v[q];  \
     }
-            CC_CX(0, 2) CC_CX(1, 3) CC_CX(4, 6) CC_CX(5, 7)
-            CC_CX(0, 4) CC_CX(1, 5) CC_CX(2, 6) CC_CX(3, 7)
-            CC_CX(0, 1) CC_CX(2, 3) CC_CX(4, 5) CC_CX(6, 7)
-            CC_CX(2, 4) CC_CX(3, 5)
-            CC_CX(1, 4) CC_CX(3, 6)
-            CC_CX(1, 2) CC_CX(3, 4) CC_CX(5, 6)
+            if (c <= 8) {  // 19 comparators
+                CC_CX(0, 2) CC_CX(1, 3) CC_CX(4, 6) CC_CX(5, 7)
+                CC_CX(0, 4) CC_CX(1, 5) CC_CX(2, 6) CC_CX(3, 7)
+                CC_CX(0, 1) CC_CX(2, 3) CC_CX(4, 5) CC_CX(6, 7)
+                CC_CX(2, 4) CC_CX(3, 5)
+                CC_CX(1, 4) CC_CX(3, 6)
+                CC_CX(1, 2) CC_CX(3, 4) CC_CX(5, 6)
+            } else {  // Batcher's odd-even merge sort, 63 comparators
+                CC_CX(0, 1) CC_CX(2, 3) CC_CX(4, 5) CC_CX(6, 7) CC_CX(8, 9) CC_CX(10, 11) CC_CX(12, 13) CC_CX(14, 15) CC_CX(0, 2) CC_CX(1, 3) CC_CX(4, 6) CC_CX(5, 7) CC_CX(8, 10) CC_CX(9, 11) CC_CX(12, 14) CC_CX(13, 15) CC_CX(1, 2) CC_CX(5, 6) CC_CX(9, 10) CC_CX(13, 14) CC_CX(0, 4) CC_CX(1, 5) CC_CX(2, 6) CC_CX(3, 7) CC_CX(8, 12) CC_CX(9, 13) CC_CX(10, 14) CC_CX(11, 15) CC_CX(2, 4) CC_CX(3, 5) CC_CX(10, 12) CC_CX(11, 13) CC_CX(1, 2) CC_CX(3, 4) CC_CX(5, 6) CC_CX(9, 10) CC_CX(11, 12) CC_CX(13, 14) CC_CX(0, 8) CC_CX(1, 9) CC_CX(2, 10) CC_CX(3, 11) CC_CX(4, 12) CC_CX(5, 13) CC_CX(6, 14) CC_CX(7, 15) CC_CX(4, 8) CC_CX(5, 9) CC_CX(6, 10) CC_CX(7, 11) CC_CX(2, 4) CC_CX(3, 5) CC_CX(6, 8) CC_CX(7, 9) CC_CX(10, 12) CC_CX(11, 13) CC_CX(1, 2) CC_CX(3, 4) CC_CX(5, 6) CC_CX(7, 8) CC_CX(9, 10) CC_CX(11, 12) CC_CX(13, 14)
+            }
 #undef CC_CX
 #pragma unroll
-            for (int j = 0; j < 8; j++)
+            for (int j = 0; j < 16; j++)
                 if (j < c) link(v[j], j > 0 ? v[j - 1] >> 1 : -1, j + 1 < c ? v[j + 1] >> 1 : -1);
         } else {
             for (int x = o + 1; x < o + c; x++) {  // insertion sort; the stored keys move with the entries
@@ -616,13 +641,14 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
 
     // ---- 1. sequence: the world's contact constraints, then its point constraints
     for (int k = tid; k < L; k += BS) {
-        int s, ea = -1, eb = -1;
+        int s, ea = -1, eb = -1, npk = 0;
         unsigned long long hi, lo;
         if (k < m) {
             s = d.conIdx[c0 + k];
-            const Con<T>& c = d.con[s];
-            if (c.dynMask & 1) ea = c.a;
-            if (c.dynMask & 2) eb = c.b;
+            const ConHead c = d.conHead[s];
+            npk = c.bits >> 4;
+            if (c.bits & 1) ea = c.a;
+            if (c.bits & 2) eb = c.b;
             hi = (unsigned long long)c.sortKey;
             lo = ((unsigned long long)(unsigned)c.a << 32) | (unsigned)c.b;
         } else {
@@ -641,6 +667,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         }
         K[2 * k] = hi;
         K[2 * k + 1] = lo;
+        d.elemNp[so + k] = npk;
         d.seq[so + k] = s;
         d.seqA[so + k] = ea;
         d.seqB[so + k] = eb;
@@ -728,16 +755,16 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     // (dense over the level order)
     int totPts = 0;
     {
-        for (int x = tid; x < L; x += BS) {
-            const int s = d.seq[so + d.sched[so + x]];
-            d.svPt[so + x] = (s >= 0 && s < 0x40000000) ? d.con[s].np : 0;
+        for (int x = tid; x < L; x += BS) {  // slot and point count of each element, in level order
+            const int k = d.sched[so + x];
+            d.svSlot[so + x] = d.seq[so + k];
+            d.svPt[so + x] = d.elemNp[so + k];
         }
         __syncthreads();
         totPts = block_scan_array<BS>(d.svPt + so, L, sh);
         for (int x = tid; x < L; x += BS) {
             const int e = so + x;
-            const int s = d.seq[so + d.sched[so + x]];
-            d.svSlot[e] = s;
+            const int s = d.svSlot[e];
             const int pbase = 4 * so + d.svPt[e];
             VElem<T> E;
             E.pt = pbase;

@@ -79,6 +79,13 @@ struct Con {
     T friction, invMassA, invMassB;
     ConPoint<T> p[4];
 };
+// What the island, cache and ordering passes read of a constraint: bodies, sort key,
+// bits = dynMask | movMask << 2 | np << 4.
+struct ConHead {
+    int a, b;
+    unsigned sortKey;
+    int bits;
+};
 // Persistent contact cache entry (src/contacts.ts:9-58): four in-place slots.
 template <typename T>
 struct CacheRec {

@@ -495,6 +495,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.counters = A.get<int>(CNT_N);
     X->counters = d.counters;
     d.con = A.get<Con<T>>(d.conCap);
+    d.conHead = A.get<ConHead>(d.conCap);
     d.tap = A.get<ConTap<T>>(d.conCap);
     d.wConStart = A.get<int>(nw + 1);
     d.conIdx = A.get<int>(d.conCap);
@@ -527,6 +528,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.nextB = A.get<int>(L);
     d.level = A.get<int>(L);
     d.sched = A.get<int>(L);
+    d.elemNp = A.get<int>(L);
     d.lvlStart = A.get<int>(2 * (L + nw + 1));
     d.blist = A.get<int>(2 * L);
     d.blkey = A.get<unsigned long long>(4 * L);
