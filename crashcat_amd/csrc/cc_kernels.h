@@ -30,6 +30,15 @@ struct alignas(64) NpDyn {
     int sleeping;
 };
 
+// a body in the cell-sorted broadphase arrays: one record, so that staging a neighbour cell
+// reads its bodies' lines once
+template <typename T>
+struct alignas(8) CellRec {
+    V3<T> mn, mx;
+    int body, fl;
+    unsigned gr, mk;
+};
+
 template <typename T>
 struct Dev {
     SettingsD<T> s;
@@ -49,12 +58,8 @@ struct Dev {
     int* cellFill;      // [cellTabCap]
     int* cellList;      // [N] occupied slots (CNT_CELLS of them)
     int* bodySlot;      // [N] table slot of a small body, -1 for a large one
-    int* cellBody;      // [N] small bodies grouped by cell, with their bounds and the words of
-    V3<T>* cellMin;     //     the pair predicate (flags with bit 30 = querying, groups, mask)
-    V3<T>* cellMax;
-    int* cellFl;
-    unsigned* cellGr;
-    unsigned* cellMk;
+    CellRec<T>* cellRec;  // [N] small bodies grouped by cell: body, bounds and the words of the
+                          // pair predicate (flags with bit 30 = querying, groups, mask)
     int* gLargeFlag;    // [N + 1]
     int* gLargeOff;     // [N + 1] exclusive scan of gLargeFlag (bodies are world-ordered)
     int* gLarge;        // [N] large bodies, grouped by world
@@ -328,12 +333,14 @@ __global__ void k_cells_scatter(Dev<T> d) {
     if (h < 0) { d.gLarge[d.gLargeOff[i]] = i; return; }
     int k = atomicAdd(&d.cellFill[h], 1);
     int p = d.cellStart[h] + k;
-    d.cellBody[p] = i;
-    d.cellMin[p] = d.B.amin[i];
-    d.cellMax[p] = d.B.amax[i];
-    d.cellFl[p] = d.B.flags[i] | (queries(d.B, i) ? (1 << 30) : 0);
-    d.cellGr[p] = d.B.groups[i];
-    d.cellMk[p] = d.B.mask[i];
+    CellRec<T> r;
+    r.body = i;
+    r.mn = d.B.amin[i];
+    r.mx = d.B.amax[i];
+    r.fl = d.B.flags[i] | (queries(d.B, i) ? (1 << 30) : 0);
+    r.gr = d.B.groups[i];
+    r.mk = d.B.mask[i];
+    d.cellRec[p] = r;
     if (k == 0) d.cellList[atomicAdd(&d.counters[CNT_CELLS], 1)] = h;
 }
 
@@ -389,8 +396,9 @@ __global__ void __launch_bounds__(256) k_bp_cells(Dev<T> d, double margin) {
     for (int ci = blockIdx.x * 4 + wv; ci < ncells; ci += gridDim.x * 4) {
         const int h = d.cellList[ci];
         const int h0 = d.cellStart[h], nh = d.cellCount[h];
-        const int w = d.B.world[d.cellBody[h0]];
-        V3<double> c0 = (cvt<double>(d.cellMin[h0]) + cvt<double>(d.cellMax[h0])) * 0.5;
+        const CellRec<T> r0 = d.cellRec[h0];
+        const int w = d.B.world[r0.body];
+        V3<double> c0 = (cvt<double>(r0.mn) + cvt<double>(r0.mx)) * 0.5;
         int hx = grid_coord(c0.x, d.cellInv), hy = grid_coord(c0.y, d.cellInv), hz = grid_coord(c0.z, d.cellInv);
         int cnt = 0, st = 0;
         if (lane < 27) {
@@ -408,24 +416,23 @@ __global__ void __launch_bounds__(256) k_bp_cells(Dev<T> d, double margin) {
         const int total = __shfl(pre, 26, 64);
         for (int hb0 = 0; hb0 < nh; hb0 += BP_HOME) {
             // stage this batch's querying home bodies, compacted
-            int i = -1, fl = 0;
+            CellRec<T> hr;
             bool q = false;
             if (hb0 + lane < nh) {
-                i = d.cellBody[h0 + hb0 + lane];
-                fl = d.cellFl[h0 + hb0 + lane];
-                q = (fl >> 30) & 1;
+                hr = d.cellRec[h0 + hb0 + lane];
+                q = (hr.fl >> 30) & 1;
             }
             unsigned long long qm = __ballot(q);
             const int nq = __popcll(qm);
             if (q) {
                 int r = __popcll(qm & ((1ull << lane) - 1ull));
-                hbody[wv][r] = i;
-                hmn[wv][r] = cvt<double>(d.cellMin[h0 + hb0 + lane]) - mk3<double>(margin, margin, margin);
-                hmx[wv][r] = cvt<double>(d.cellMax[h0 + hb0 + lane]) + mk3<double>(margin, margin, margin);
-                hfl[wv][r] = fl & 0x3fffffff;
-                hgr[wv][r] = d.cellGr[h0 + hb0 + lane];
-                hmk[wv][r] = d.cellMk[h0 + hb0 + lane];
-                hcnt[wv][r] = d.bpCount[i];
+                hbody[wv][r] = hr.body;
+                hmn[wv][r] = cvt<double>(hr.mn) - mk3<double>(margin, margin, margin);
+                hmx[wv][r] = cvt<double>(hr.mx) + mk3<double>(margin, margin, margin);
+                hfl[wv][r] = hr.fl & 0x3fffffff;
+                hgr[wv][r] = hr.gr;
+                hmk[wv][r] = hr.mk;
+                hcnt[wv][r] = d.bpCount[hr.body];
             }
             wave_sync();
             if (nq == 0) continue;
@@ -439,12 +446,13 @@ __global__ void __launch_bounds__(256) k_bp_cells(Dev<T> d, double margin) {
                         if (nPre[wv][mid] <= f) lo = mid; else hi = mid - 1;
                     }
                     int p = nStart[wv][lo] + (f - nPre[wv][lo]);
-                    sbody[wv][x] = d.cellBody[p];
-                    smn[wv][x] = d.cellMin[p];
-                    smx[wv][x] = d.cellMax[p];
-                    sfl[wv][x] = d.cellFl[p];
-                    sgr[wv][x] = d.cellGr[p];
-                    smk[wv][x] = d.cellMk[p];
+                    const CellRec<T> cr = d.cellRec[p];
+                    sbody[wv][x] = cr.body;
+                    smn[wv][x] = cr.mn;
+                    smx[wv][x] = cr.mx;
+                    sfl[wv][x] = cr.fl;
+                    sgr[wv][x] = cr.gr;
+                    smk[wv][x] = cr.mk;
                 }
                 wave_sync();
                 const int work = nq * nc;

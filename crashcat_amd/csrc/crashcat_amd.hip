@@ -467,12 +467,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.cellFill = A.get<int>(d.cellTabCap);
     d.cellList = A.get<int>(nb);
     d.bodySlot = A.get<int>(nb);
-    d.cellBody = A.get<int>(nb);
-    d.cellMin = A.get<V3<T>>(nb);
-    d.cellMax = A.get<V3<T>>(nb);
-    d.cellFl = A.get<int>(nb);
-    d.cellGr = A.get<unsigned>(nb);
-    d.cellMk = A.get<unsigned>(nb);
+    d.cellRec = A.get<CellRec<T>>(nb);
     d.gLargeFlag = A.get<int>(nb + 1);
     d.gLargeOff = A.get<int>(nb + 1);
     d.gLarge = A.get<int>(nb);
