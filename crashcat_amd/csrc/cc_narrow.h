@@ -486,7 +486,8 @@ struct EpaWave {  // byte offsets of the arrays in one wavefront's block
     static constexpr size_t oTF = oTE + sizeof(EEdge) * 3 * EPA_MAX_TRIANGLES * L;
     static constexpr size_t oFL = oTF + 4 * EPA_MAX_TRIANGLES * L;
     static constexpr size_t oQU = oFL + sizeof(short) * 2 * EPA_MAX_TRIANGLES * L;
-    static constexpr size_t oST = oQU + sizeof(short) * 2 * EPA_MAX_TRIANGLES * L;
+    static constexpr size_t oHC = oQU + sizeof(short) * 2 * EPA_MAX_TRIANGLES * L;
+    static constexpr size_t oST = oHC + sizeof(double) * 2 * EPA_MAX_TRIANGLES * L;
     static constexpr size_t oED = oST + sizeof(EStack) * EPA_MAX_EDGE_LENGTH * L;
     static constexpr size_t oNT = oED + sizeof(EEdge) * EPA_MAX_EDGE_LENGTH * L;
     static constexpr size_t bytes = (oNT + sizeof(short) * EPA_MAX_EDGE_LENGTH * L + 255) / 256 * 256;
@@ -509,6 +510,9 @@ struct EpaH {
     __device__ unsigned char& tf(int t, int f) const { return ((unsigned char*)(b + W::oTF))[((t * EPA_LANES + lane) << 2) + f]; }
     __device__ short& fl(int i) const { return ((short*)(b + W::oFL))[i * EPA_LANES + lane]; }
     __device__ short& qu(int i) const { return ((short*)(b + W::oQU))[i * EPA_LANES + lane]; }
+    // the closestLengthSq of the triangle in heap slot i, kept beside the slot: a triangle in the
+    // heap is never freed or rewritten, so the copy stays equal to its cls
+    __device__ double& hc(int i) const { return ((double*)(b + W::oHC))[i * EPA_LANES + lane]; }
     __device__ EStack& st(int i) const { return ((EStack*)(b + W::oST))[i * EPA_LANES + lane]; }
     __device__ EEdge& ed(int i) const { return ((EEdge*)(b + W::oED))[i * EPA_LANES + lane]; }
     __device__ short& nt(int i) const { return ((short*)(b + W::oNT))[i * EPA_LANES + lane]; }
@@ -523,36 +527,55 @@ __device__ inline bool tri_facing(const EpaH<T>& h, int t, double px, double py,
 template <typename T>
 __device__ inline void heap_push(EpaH<T>& h, int ti) {
     if (h.nq >= EPA_MAX_TRIANGLES * 2) return;
-    h.qu(h.nq++) = (short)ti;
+    const double c = h.cls(ti);
+    int cur = h.nq++;
     h.tf(ti, TF_INQUEUE) = 1;
-    int cur = h.nq - 1;
-    while (cur > 0) {
+    while (cur > 0) {  // sift up; the new entry is written once, at its final slot
         int par = (cur - 1) >> 1;
-        short qp = h.qu(par), qc = h.qu(cur);
-        if (h.cls(qp) > h.cls(qc)) {
-            h.qu(par) = qc; h.qu(cur) = qp;
+        const double cp = h.hc(par);
+        if (cp > c) {
+            h.qu(cur) = h.qu(par);
+            h.hc(cur) = cp;
             cur = par;
         } else break;
     }
+    h.qu(cur) = (short)ti;
+    h.hc(cur) = c;
 }
 
 template <typename T>
 __device__ inline int heap_pop(EpaH<T>& h) {
     if (h.nq == 0) return -1;
-    short tmp = h.qu(h.nq - 1); h.qu(h.nq - 1) = h.qu(0); h.qu(0) = tmp;
-    int count = h.nq - 1;
-    int largest = 0;
+    const short top = h.qu(0);
+    const double topc = h.hc(0);
+    const int count = h.nq - 1;
+    // the last entry moves to the root and sifts down (the reference swaps it there first)
+    const short mv = h.qu(count);
+    const double mc = h.hc(count);
+    int cur = 0;
     for (;;) {
-        int child = (largest << 1) + 1;
+        int child = (cur << 1) + 1;
         if (child >= count) break;
-        int prev = largest;
-        if (h.cls(h.qu(largest)) > h.cls(h.qu(child))) largest = child;
+        int largest = cur;
+        double lc = mc;
+        const double c1 = h.hc(child);
+        if (lc > c1) { largest = child; lc = c1; }
         ++child;
-        if (child < count && h.cls(h.qu(largest)) > h.cls(h.qu(child))) largest = child;
-        if (prev == largest) break;
-        short t = h.qu(prev); h.qu(prev) = h.qu(largest); h.qu(largest) = t;
+        if (child < count) {
+            const double c2 = h.hc(child);
+            if (lc > c2) { largest = child; lc = c2; }
+        }
+        if (largest == cur) break;
+        h.qu(cur) = h.qu(largest);
+        h.hc(cur) = lc;
+        cur = largest;
     }
-    return h.qu(--h.nq);
+    h.qu(cur) = mv;
+    h.hc(cur) = mc;
+    h.qu(count) = top;
+    h.hc(count) = topc;
+    h.nq = count;
+    return top;
 }
 
 template <typename T>
