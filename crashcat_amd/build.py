@@ -33,7 +33,33 @@ def build(verbose=False):
     return OUT
 
 
+NODE_DIR = os.path.join(HERE, "node")
+NODE_SRC = os.path.join(NODE_DIR, "cc_node.c")
+NODE_OUT = os.path.join(NODE_DIR, "crashcat_amd.node")
+NODE_INCLUDE = "/usr/include/node"
+
+
+def node_available():
+    """The Node N-API headers are installed (the addon is plain C over node_api.h)."""
+    return os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h"))
+
+
+def build_node(verbose=False):
+    """Builds the N-API addon crashcat_amd/node/crashcat_amd.node against the library."""
+    cmd = ["gcc", "-O2", "-Wall", "-shared", "-fPIC", "-DNODE_GYP_MODULE_NAME=crashcat_amd", "-I" + NODE_INCLUDE,
+           "-I" + os.path.join(HERE, "..", "include"), NODE_SRC, "-L" + HERE, "-lcrashcat_amd",
+           "-Wl,-rpath,$ORIGIN/..", "-o", NODE_OUT + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(NODE_OUT + ".tmp", NODE_OUT)
+    return NODE_OUT
+
+
 if __name__ == "__main__":
     build(verbose=True)
     print("built", OUT)
+    if node_available():
+        build_node(verbose=True)
+        print("built", NODE_OUT)
     sys.exit(0)

@@ -1,0 +1,156 @@
+'use strict';
+// JavaScript host of the GPU step.  A world description (the fields of the C ABI's structs,
+// by name) is packed into the structs whose layouts the addon reads from the C header, uploaded
+// once, and stepped by updateWorld, which takes the place of crashcat's
+// updateWorld(world, listener, timeStep) (src/update.ts:45) for the worlds this path supports.
+// Node 12 syntax (no ?. or ??).
+const path = require('path');
+
+const addon = require(path.join(__dirname, 'crashcat_amd.node'));
+const LAYOUT = addon.layout();
+const PRECISION = { fp32: 0, fp64: 1 };
+
+function writeField(dv, base, spec, value) {
+  const off = base + spec[0];
+  const type = spec[1];
+  const count = spec[2];
+  for (let i = 0; i < count; i++) {
+    let v = count === 1 ? value : value[i];
+    if (v === undefined || v === null) v = 0;
+    switch (type) {
+      case 'f64': dv.setFloat64(off + 8 * i, v, true); break;
+      case 'i32': dv.setInt32(off + 4 * i, v, true); break;
+      case 'u32': dv.setUint32(off + 4 * i, v >>> 0, true); break;
+      case 'u8': dv.setUint8(off + i, v); break;
+      case 'i64': dv.setBigInt64(off + 8 * i, BigInt(v), true); break;
+      default: throw new Error('crashcat_amd: unknown field type ' + type);
+    }
+  }
+}
+
+function readField(dv, base, spec) {
+  const off = base + spec[0];
+  const type = spec[1];
+  const count = spec[2];
+  const out = [];
+  for (let i = 0; i < count; i++) {
+    switch (type) {
+      case 'f64': out.push(dv.getFloat64(off + 8 * i, true)); break;
+      case 'i32': out.push(dv.getInt32(off + 4 * i, true)); break;
+      case 'u32': out.push(dv.getUint32(off + 4 * i, true)); break;
+      case 'u8': out.push(dv.getUint8(off + i)); break;
+      case 'i64': out.push(Number(dv.getBigInt64(off + 8 * i, true))); break;
+      default: throw new Error('crashcat_amd: unknown field type ' + type);
+    }
+  }
+  return count === 1 ? out[0] : out;
+}
+
+// records (objects keyed by the C field names) -> ArrayBuffer of `structName`; a field a
+// record does not name is zero
+function pack(structName, records) {
+  const st = LAYOUT[structName];
+  if (!st) throw new Error('crashcat_amd: no struct ' + structName);
+  const buf = new ArrayBuffer(st.size * records.length);
+  const dv = new DataView(buf);
+  const names = Object.keys(st.fields);
+  for (let k = 0; k < records.length; k++) {
+    const r = records[k];
+    for (const name of Object.keys(r)) {
+      if (names.indexOf(name) < 0) throw new Error('crashcat_amd: ' + structName + ' has no field ' + name);
+    }
+    for (const name of names) {
+      if (r[name] !== undefined) writeField(dv, k * st.size, st.fields[name], r[name]);
+    }
+  }
+  return buf;
+}
+
+function unpack(structName, buf) {
+  const st = LAYOUT[structName];
+  const dv = new DataView(buf);
+  const n = buf.byteLength / st.size;
+  const out = [];
+  for (let k = 0; k < n; k++) {
+    const r = {};
+    for (const name of Object.keys(st.fields)) r[name] = readField(dv, k * st.size, st.fields[name]);
+    out.push(r);
+  }
+  return out;
+}
+
+function hasCallbacks(listener) {
+  if (listener === undefined || listener === null) return false;
+  for (const k of Object.keys(listener)) if (typeof listener[k] === 'function') return true;
+  return false;
+}
+
+// One GPU context holding a batch of independent worlds.
+class GpuWorld {
+  // desc: { settings, worldBodyOffsets, shapes, hull, bodies, pointConstraints }
+  // opts: { precision: 'fp32' | 'fp64', maxPairsPerBody, maxContactsPerBody, device }
+  constructor(desc, opts) {
+    const o = opts || {};
+    const prec = PRECISION[o.precision || 'fp32'];
+    if (prec === undefined) throw new Error('crashcat_amd: precision must be fp32 or fp64');
+    this.handle = addon.create(prec, o.maxPairsPerBody || 24, o.maxContactsPerBody || 8, o.device || 0);
+    const hull = desc.hull || {};
+    const hullBuffers = {
+      points: new Float64Array(hull.points || []),
+      shrunkPoints: new Float64Array(hull.shrunkPoints || []),
+      faceFirstVertex: new Int32Array(hull.faceFirstVertex || []),
+      faceNumVertices: new Int32Array(hull.faceNumVertices || []),
+      faceVertexIndices: new Int32Array(hull.faceVertexIndices || []),
+      planeNormals: new Float64Array(hull.planeNormals || []),
+      planeConstants: new Float64Array(hull.planeConstants || []),
+      totalPoints: (hull.points || []).length / 3,
+      totalFaces: (hull.faceNumVertices || []).length,
+      totalFaceVertices: (hull.faceVertexIndices || []).length,
+    };
+    const bodies = desc.bodies || [];
+    const offsets = desc.worldBodyOffsets || [0, bodies.length];
+    addon.upload(this.handle, pack('cc_settings', [desc.settings]), new Int32Array(offsets),
+                 pack('cc_body_desc', bodies), pack('cc_shape_desc', desc.shapes || []), hullBuffers,
+                 pack('cc_point_constraint_desc', desc.pointConstraints || []));
+    this.numBodies = bodies.length;
+    this.previousTimeStep = 0;
+  }
+
+  step(timeStep, n) {
+    addon.step(this.handle, timeStep, n === undefined ? 1 : n);
+    this.previousTimeStep = timeStep;
+  }
+
+  sync() { addon.sync(this.handle); }
+
+  // body states in the cc_body_state layout (ArrayBuffer), or as objects
+  bodyStateBuffer() {
+    const buf = new ArrayBuffer(LAYOUT.cc_body_state.size * this.numBodies);
+    addon.downloadBodies(this.handle, buf);
+    return buf;
+  }
+  bodies() { return unpack('cc_body_state', this.bodyStateBuffer()); }
+  setBodies(states) {
+    const buf = states instanceof ArrayBuffer ? states : pack('cc_body_state', states);
+    addon.setBodies(this.handle, buf);
+  }
+  pairs() { return addon.downloadPairs(this.handle); }
+  counters() {
+    const buf = new ArrayBuffer(LAYOUT.cc_step_counters.size);
+    addon.counters(this.handle, buf);
+    return unpack('cc_step_counters', buf)[0];
+  }
+}
+
+function createWorld(desc, opts) { return new GpuWorld(desc, opts); }
+
+// updateWorld(world, listener, timeStep) (src/update.ts:45): one step of every world of the
+// context.  Listener callbacks cannot run inside a GPU step, so a listener with callbacks is
+// refused, as the C ABI documents.
+function updateWorld(world, listener, timeStep) {
+  if (!(world instanceof GpuWorld)) throw new Error('crashcat_amd: updateWorld needs a world made by createWorld');
+  if (hasCallbacks(listener)) throw new Error('crashcat_amd: listener callbacks are not available on the GPU step');
+  world.step(timeStep, 1);
+}
+
+module.exports = { createWorld, updateWorld, GpuWorld, pack, unpack, layout: LAYOUT, addon };

@@ -250,6 +250,47 @@ class Scene:
         offs = np.asarray(self.offsets(), dtype=np.int32)
         return offs
 
+    def desc(self):
+        """The scene as the JavaScript host's world description (crashcat_amd/node/index.js):
+        plain numbers and lists keyed by the C ABI's field names, JSON-serialisable."""
+        def struct_dict(st):
+            out = {}
+            for name, ctype in st._fields_:
+                if name.endswith("_"):
+                    continue
+                v = getattr(st, name)
+                out[name] = list(v) if hasattr(v, "__len__") else v
+            return out
+        arr, hull, keep = self.c_shapes()
+        shapes = [struct_dict(arr[i]) for i in range(len(self.shapes))]
+        bodies = []
+        b = self.bodies()
+        for row in b:
+            d = {}
+            for name in b.dtype.names:
+                if name.endswith("_"):
+                    continue
+                v = row[name]
+                d[name] = v.tolist() if hasattr(v, "tolist") else v
+            bodies.append(d)
+        pcs, npc = self.c_point_constraints()
+        return {
+            "settings": struct_dict(self.settings),
+            "worldBodyOffsets": [int(x) for x in self.c_offsets()],
+            "shapes": shapes,
+            "hull": {
+                "points": keep["pts"].reshape(-1)[: 3 * hull.total_points].tolist(),
+                "shrunkPoints": keep["shr"].reshape(-1)[: 3 * hull.total_points].tolist(),
+                "faceFirstVertex": keep["ffv"][: hull.total_faces].tolist(),
+                "faceNumVertices": keep["fnv"][: hull.total_faces].tolist(),
+                "faceVertexIndices": keep["fvi"][: hull.total_face_vertices].tolist(),
+                "planeNormals": keep["pn"].reshape(-1)[: 3 * hull.total_faces].tolist(),
+                "planeConstants": keep["pc"][: hull.total_faces].tolist(),
+            },
+            "bodies": bodies,
+            "pointConstraints": [struct_dict(pcs[i]) for i in range(npc)],
+        }
+
 
 # ------------------------------------------------------------------------ PRNG
 _M64 = (1 << 64) - 1

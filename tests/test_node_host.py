@@ -1,0 +1,113 @@
+"""The JavaScript host (Node N-API addon + crashcat_amd/node/index.js) over the same C ABI.
+
+CPU: the struct layouts the addon reads from the C header equal the ctypes mirror, packing
+round-trips, unknown fields and a missing GPU fail loudly.  GPU: the cube heap stepped from
+JavaScript with updateWorld is bit-identical to the FP64 oracle (bodies and pair set).
+"""
+import ctypes as C
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from crashcat_amd import _abi
+from crashcat_amd import build as B
+from crashcat_amd import scene as SC
+from crashcat_amd._lib import BODY_STATE_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ADDON = os.path.join(HERE, "..", "crashcat_amd", "node", "crashcat_amd.node")
+STEP = os.path.join(HERE, "node", "step_world.js")
+
+needs_node = pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+
+
+def _addon():
+    if not os.path.exists(ADDON):
+        if not B.node_available():
+            pytest.skip("Node N-API headers are not installed")
+        B.build_node()
+    return ADDON
+
+
+def _node(script):
+    _addon()
+    r = subprocess.run(["node", "-e", script], capture_output=True, text=True, timeout=60,
+                       cwd=os.path.join(HERE, "..", "crashcat_amd", "node"))
+    return r
+
+
+@needs_node
+def test_addon_layouts_match_the_ctypes_mirror():
+    r = _node("process.stdout.write(JSON.stringify(require('./index.js').layout))")
+    assert r.returncode == 0, r.stderr
+    L = json.loads(r.stdout)
+    mirror = {"cc_settings": _abi.Settings, "cc_shape_desc": _abi.ShapeDesc, "cc_body_desc": _abi.BodyDesc,
+              "cc_body_state": _abi.BodyState, "cc_point_constraint_desc": _abi.PointConstraintDesc,
+              "cc_step_counters": _abi.StepCounters}
+    for name, st in mirror.items():
+        assert L[name]["size"] == C.sizeof(st), name
+        names = [f for f, _ in st._fields_ if not f.endswith("_")]
+        assert sorted(L[name]["fields"]) == sorted(names), name
+        for f in names:
+            assert L[name]["fields"][f][0] == getattr(st, f).offset, (name, f)
+
+
+@needs_node
+def test_pack_round_trip_and_unknown_field():
+    r = _node("""
+        const cc = require('./index.js');
+        const b = cc.pack('cc_body_desc', [{shape: 3, position: [1.5, -2, 3], quaternion: [0, 0, 0.6, 0.8],
+                                            inv_mass: 0.25, collision_mask: 4294967295}]);
+        const r = cc.unpack('cc_body_desc', b)[0];
+        let threw = false;
+        try { cc.pack('cc_body_desc', [{no_such_field: 1}]); } catch (e) { threw = /no field/.test(e.message); }
+        process.stdout.write(JSON.stringify({shape: r.shape, position: r.position, q: r.quaternion,
+                                             m: r.inv_mass, mask: r.collision_mask, threw: threw}));
+    """)
+    assert r.returncode == 0, r.stderr
+    v = json.loads(r.stdout)
+    assert v == {"shape": 3, "position": [1.5, -2, 3], "q": [0, 0, 0.6, 0.8], "m": 0.25, "mask": 4294967295,
+                 "threw": True}
+
+
+@needs_node
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_create_without_gpu_throws():
+    r = _node("""
+        const cc = require('./index.js');
+        try { cc.createWorld({settings: {}, bodies: []}); process.stdout.write('no error'); }
+        catch (e) { process.stdout.write(e.message); }
+    """)
+    assert r.returncode == 0, r.stderr
+    assert "no usable HIP device" in r.stdout
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_update_world_cube_heap_bitwise(tmp_path):
+    from oracle import refcpu as O
+    sc = SC.cube_heap(seed=1001)
+    desc = tmp_path / "world.json"
+    desc.write_text(json.dumps(sc.desc()))
+    out, pairs = tmp_path / "bodies.bin", tmp_path / "pairs.bin"
+    steps = 30
+    _addon()
+    r = subprocess.run(["node", STEP, str(desc), "fp64", str(steps), str(out), str(pairs)], capture_output=True,
+                       text=True, timeout=180)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, dtype=BODY_STATE_DTYPE)
+    orc = O.OracleWorld(sc)
+    for _ in range(steps):
+        orc.step(1.0 / 60.0)
+    ref = orc.bodies()
+    assert len(got) == len(ref)
+    for f in ["position", "quaternion", "center_of_mass_position", "linear_velocity", "angular_velocity", "sleeping"]:
+        assert np.array_equal(got[f], ref[f]), f
+    gp = np.fromfile(pairs, dtype=np.int32).reshape(-1, 2)
+    op = np.asarray(orc.pairs(), dtype=np.int32).reshape(-1, 2)
+    assert np.array_equal(gp, op)
+    assert json.loads(r.stdout.strip().splitlines()[-1])["overflow"] == 0
