@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run the FP32 batch for warmup + this many steps (for rocprofv3)")
     ap.add_argument("--calibrate", type=int, default=0, help="only copy this many bytes device to device")
+    ap.add_argument("--batched-worlds", type=int, default=4096,
+                    help="worlds in the batched uneven-worlds measurement (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); worlds would shard across GPUs by "
@@ -180,6 +182,35 @@ def main():
         log(f"fp64 plain: {k64} steps in {e64:.3f}s -> {fp64:.4g} body-steps/s; phases {c64.phase_times()}")
         c64.close()
 
+    # ---- batched uneven worlds: thousands of piles of 8-512 bodies with capsule chains, 60 steps
+    batched = None
+    if args.batched_worlds:
+        t0 = time.time()
+        bw = S.batched_worlds(seed=1003, num_worlds=args.batched_worlds)
+        log(f"batched worlds: {bw.num_bodies} bodies in {bw.num_worlds} worlds, "
+            f"{len(bw.point_constraints)} point constraints, built in {time.time() - t0:.1f}s")
+        batched = {"workload": "batched uneven worlds: body count per world log-uniform in [8,512], boxes and "
+                               "spheres on a 1.1 m grid over a floor plus 2 chains of 8 capsules joined by point "
+                               "constraints, seed 1003, dt 1/60, 5 warmup + 60 timed steps",
+                   "worlds": bw.num_worlds, "bodies": bw.num_bodies,
+                   "point_constraints": len(bw.point_constraints)}
+        for prec in ("fp32", "fp64"):
+            c = Context(prec, max_pairs_per_body=16, max_contacts_per_body=8, block_solver=args.block)
+            c.upload(bw)
+            c.step(DT, 5)
+            c.sync()
+            t = time.perf_counter()
+            c.step(DT, 60)
+            c.sync()
+            e = time.perf_counter() - t
+            ph = c.phase_times()
+            batched[f"{prec}_body_steps_per_s"] = bw.num_bodies * 60 / e
+            batched[f"{prec}_phase_ms_per_step"] = {k: v / 60 for k, v in ph.items()}
+            log(f"batched worlds {prec}: 60 steps in {e:.3f}s -> {bw.num_bodies * 60 / e:.4g} body-steps/s; "
+                f"phases {ph}")
+            c.close()
+        del bw
+
     # ---- accuracy vs the FP64 oracle, and the oracle's own speed on the host
     acc = {}
     cpu = None
@@ -258,6 +289,7 @@ def main():
         "fp64_plain_body_steps_per_s": fp64,
         "batched_fp32_over_plain_fp64": (value / fp64) if fp64 else None,
         "accuracy_vs_fp64_oracle": acc,
+        "batched_uneven_worlds": batched,
         "phases": per_phase,
         "roofline": {
             "bound": "hbm",

@@ -47,6 +47,7 @@ struct Dev {
     int N;              // bodies (all worlds)
     int W;              // worlds
     const int* wStart;  // [W+1] body ranges
+    const int* worldOrder;  // [W] worlds by descending body count: the solver's workgroup order
     int pairCap, conCap, epaCap, epaWorkers, hashCap;
     // broadphase: uniform grid per world, occupied cells in an open-addressing hash table
     double cellWidth;   // widest box a body may have to be binned (wider ones are "large")

@@ -625,7 +625,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __shared__ int sLS[LS_LDS];
     extern __shared__ int dynLds[];  // d.worldLdsInts ints: per-body counts, then packed in-degrees
     __shared__ int sTail;
-    const int w = blockIdx.x;
+    const int w = d.worldOrder[blockIdx.x];
     const int tid = threadIdx.x;
     const Bodies<T>& B = d.B;
     const int b0 = d.wStart[w], b1 = d.wStart[w + 1];

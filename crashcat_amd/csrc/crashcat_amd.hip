@@ -419,6 +419,16 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed");
     HIPCHK(c, hipMemcpy(dW, woff, (nw + 1) * sizeof(int), hipMemcpyHostToDevice));
     d.wStart = dW;
+    {  // the world solver's workgroups start on the largest worlds (longest solves first)
+        std::vector<int> order(nw);
+        for (int w = 0; w < nw; w++) order[w] = w;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return woff[x + 1] - woff[x] > woff[y + 1] - woff[y]; });
+        int* dO = A.get<int>(nw);
+        if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed");
+        HIPCHK(c, hipMemcpy(dO, order.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+        d.worldOrder = dO;
+    }
     // point constraints, grouped by world in index order
     std::vector<std::pair<int, int>> pcw;
     for (int i = 0; i < npc; i++) {
