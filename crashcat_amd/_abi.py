@@ -5,6 +5,7 @@ CC_SHAPE_SPHERE = 0
 CC_SHAPE_BOX = 1
 CC_SHAPE_CAPSULE = 2
 CC_SHAPE_CONVEX_HULL = 3
+CC_SHAPE_TRIANGLE_MESH = 4
 
 CC_MOTION_STATIC = 0
 CC_MOTION_KINEMATIC = 1
@@ -41,6 +42,8 @@ class Settings(C.Structure):
         ("allow_sleeping", C.c_int32),
         ("num_object_layers", C.c_int32),
         ("object_layer_pairs", C.c_uint8 * (CC_MAX_OBJECT_LAYERS * CC_MAX_OBJECT_LAYERS)),
+        ("collide_with_backfaces", C.c_int32),
+        ("collide_only_with_active_edges", C.c_int32),
     ]
 
 
@@ -59,6 +62,22 @@ class ShapeDesc(C.Structure):
         ("center_of_mass", D3),
         ("aabb_min", D3),
         ("aabb_max", D3),
+        ("mesh_first_triangle", C.c_int32),
+        ("mesh_num_triangles", C.c_int32),
+        ("mesh_first_node", C.c_int32),
+        ("mesh_num_nodes", C.c_int32),
+    ]
+
+
+class MeshData(C.Structure):
+    _fields_ = [
+        ("positions", C.POINTER(C.c_double)),
+        ("triangles", C.POINTER(C.c_int32)),
+        ("nodes", C.POINTER(C.c_double)),
+        ("total_vertices", C.c_int32),
+        ("total_triangles", C.c_int32),
+        ("total_nodes", C.c_int32),
+        ("pad_", C.c_int32),
     ]
 
 
@@ -138,6 +157,8 @@ class ContactOut(C.Structure):
         ("body_b", C.c_int32),
         ("num_points", C.c_int32),
         ("pad_", C.c_int32),
+        ("sub_shape_a", C.c_uint32),
+        ("sub_shape_b", C.c_uint32),
         ("normal", D3),
         ("position_a", D3 * 4),
         ("position_b", D3 * 4),

@@ -14,7 +14,7 @@ _LIB = None
 SYMBOLS = [
     "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
-    "cc_get_phase_times", "cc_get_totals",
+    "cc_get_phase_times", "cc_get_totals", "cc_build_triangle_mesh",
 ]
 
 
@@ -37,7 +37,9 @@ def lib():
     L.cc_last_error.restype = C.c_char_p
     L.cc_upload_scene.argtypes = [vp, C.POINTER(_abi.Settings), C.c_int32, ip, C.POINTER(_abi.BodyDesc), C.c_int32,
                                   C.POINTER(_abi.ShapeDesc), C.c_int32, C.POINTER(_abi.HullData),
-                                  C.POINTER(_abi.PointConstraintDesc), C.c_int32]
+                                  C.POINTER(_abi.PointConstraintDesc), C.c_int32, C.POINTER(_abi.MeshData)]
+    dp = C.POINTER(C.c_double)
+    L.cc_build_triangle_mesh.argtypes = [dp, C.c_int32, ip, C.c_int32, C.c_double, C.c_double, C.c_int32, dp, ip, dp, ip]
     L.cc_step.argtypes = [vp, C.c_double, C.c_int32]
     L.cc_sync.argtypes = [vp]
     L.cc_download_bodies.argtypes = [vp, C.POINTER(_abi.BodyState), C.c_int32]
@@ -98,6 +100,7 @@ class Context:
         L = lib()
         bodies = scene.bodies()
         shapes, hulls, keep = scene.c_shapes()
+        meshes = keep["meshes"]
         pcs, npc = scene.c_point_constraints()
         offs = scene.c_offsets()
         self._keep = (bodies, keep, pcs, offs)
@@ -106,7 +109,7 @@ class Context:
         self._check(L.cc_upload_scene(self.h, C.byref(scene.settings), len(offs) - 1,
                                       offs.ctypes.data_as(C.POINTER(C.c_int32)),
                                       bodies.ctypes.data_as(C.POINTER(_abi.BodyDesc)), self.n, shapes,
-                                      len(scene.shapes), C.byref(hulls), pcs, npc))
+                                      len(scene.shapes), C.byref(hulls), pcs, npc, C.byref(meshes)))
 
     def step(self, dt=1.0 / 60.0, n=1):
         self._check(lib().cc_step(self.h, dt, n))
@@ -171,3 +174,25 @@ class Context:
                                               out.ctypes.data_as(ip), cap, C.byref(cnt)))
         self.n = n
         return out[:min(cnt.value, cap)]
+
+
+def build_triangle_mesh(positions, indices, degenerate_tolerance=1e-6, active_edge_cos_threshold=0.996195,
+                        max_leaf_triangles=8):
+    """buildTriangleMesh (src/shapes/utils/triangle-mesh-builder.ts:62-220) on the host, through the
+    library: returns (positions [V,3] float64, triangles [T,4] int32 = a, b, c, active-edge bits,
+    nodes [N,8] float64 = the BVH node buffer)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 3)
+    nv, nt = len(pos), len(idx)
+    op = np.zeros((max(nv, 1), 3), dtype=np.float64)
+    ot = np.zeros((max(nt, 1), 4), dtype=np.int32)
+    on = np.zeros((max(2 * nt, 1), 8), dtype=np.float64)
+    cnt = np.zeros(3, dtype=np.int32)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    rc = lib().cc_build_triangle_mesh(pos.ctypes.data_as(dp), nv, idx.ctypes.data_as(ip), nt, degenerate_tolerance,
+                                      active_edge_cos_threshold, max_leaf_triangles, op.ctypes.data_as(dp),
+                                      ot.ctypes.data_as(ip), on.ctypes.data_as(dp), cnt.ctypes.data_as(ip))
+    if rc != 0:
+        raise CrashcatError(f"cc_build_triangle_mesh failed (code {rc})")
+    return op[:cnt[0]].copy(), ot[:cnt[1]].copy(), on[:cnt[2]].copy()

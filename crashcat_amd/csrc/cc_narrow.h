@@ -7,11 +7,20 @@
 
 namespace cc {
 
-enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3 };
+enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3, SHAPE_MESH = 4 };
+
+// A BVH node of a triangle mesh: bounds, then for an internal node the right child (node
+// index within the mesh; the left child is the next node) and the split axis, for a leaf the
+// first triangle (index within the mesh) and -(count + 1).
+template <typename T>
+struct MNode {
+    T mn[3], mx[3];
+    int a, b;
+};
 
 template <typename T>
 struct ShapeD {
-    int type, hfp, hnp, hff, hnf;
+    int type, hfp, hnp, hff, hnf;  // meshes: first triangle, triangles, first node, nodes
     int mfv;  // most vertices a supporting face of the shape can have
     T radius, hh, cr;
     T he[3], com[3], amin[3], amax[3];
@@ -27,6 +36,9 @@ struct ShapeTable {
     const int* fnv;   // face vertex count
     const int* fvi;   // face vertex indices (relative to the hull's first point)
     const T* pn;      // face plane normals
+    const T* mpos;    // mesh vertices, xyz
+    const int4* mtri; // mesh triangles: vertex a, b, c (indices into mpos), active-edge bits
+    const MNode<T>* mnode;
 };
 
 // ---------------------------------------------------------------- supports

@@ -20,6 +20,9 @@ struct SettingsD {
     T spec, manifoldTol, colTol, penTol, preserveDist2, minVelRest, slop, beta, maxPen, tSleep, sleepThresh;
     int velIters, posIters, gravityEnabled, warmStarting, allowSleeping, pad;
     unsigned char layerPairs[64];
+    T normalCos;           // manifolds merge hits whose normals are within this cosine
+    int backfaces;         // triangle meshes: collide with back faces
+    int activeEdges;       // triangle meshes: correct normals at inactive edges
 };
 
 template <typename T>

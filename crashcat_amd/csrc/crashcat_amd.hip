@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/crashcat_amd.h"
+#include "cc_mesh_build.h"
 #include "cc_solver.h"
 
 using namespace cc;
@@ -164,6 +165,7 @@ struct CtxT : Impl {
     std::vector<hipEvent_t> ev;  // 7 per step of the last cc_step call
     int solverBS = 256;
     int device = 0;
+    bool hasMesh = false;
     size_t worldLdsNeed = 0;   // ints of LDS the largest world can use
     size_t worldLdsBytes = 0;  // dynamic LDS given to the world kernel
     int worldLdsBS = 0;        // block size worldLdsBytes was sized for
@@ -264,7 +266,7 @@ static void set_cell_size(Dev<T>& d, double margin) {
 template <typename T>
 static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* woff, const cc_body_desc* bodies, int32_t nb,
                   const cc_shape_desc* shapes, int32_t ns, const cc_hull_data* hulls, const cc_point_constraint_desc* pcs,
-                  int32_t npc) {
+                  int32_t npc, const cc_mesh_data* meshes) {
     auto* X = new CtxT<T>();
     delete c->impl;
     c->impl = X;
@@ -293,15 +295,30 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.s.warmStarting = s->warm_starting;
     d.s.allowSleeping = s->allow_sleeping;
     memcpy(d.s.layerPairs, s->object_layer_pairs, 64);
+    d.s.normalCos = T(s->normal_cos_max_delta_rotation);
+    d.s.backfaces = s->collide_with_backfaces;
+    d.s.activeEdges = s->collide_only_with_active_edges;
     // shapes
     std::vector<ShapeD<T>> sh(std::max(ns, 1));
     for (int i = 0; i < ns; i++) {
         const cc_shape_desc& q = shapes[i];
-        if (q.type < 0 || q.type > 3) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
+        if (q.type < 0 || q.type > 4) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
+        if (q.type == CC_SHAPE_TRIANGLE_MESH) {
+            const int MT = meshes ? meshes->total_triangles : 0, MN = meshes ? meshes->total_nodes : 0;
+            if (q.mesh_first_triangle < 0 || q.mesh_num_triangles < 0 || q.mesh_first_node < 0 || q.mesh_num_nodes < 0 ||
+                q.mesh_first_triangle + q.mesh_num_triangles > MT || q.mesh_first_node + q.mesh_num_nodes > MN ||
+                (q.mesh_num_nodes > 0 && (!meshes->nodes || !meshes->triangles || !meshes->positions)))
+                return fail(c, CC_E_ARG, "mesh data out of range");
+        }
         ShapeD<T>& o = sh[i];
         o.type = q.type; o.hfp = q.hull_first_point; o.hnp = q.hull_num_points; o.hff = q.hull_first_face;
         o.hnf = q.hull_num_faces;
         o.mfv = q.type == CC_SHAPE_BOX ? 4 : (q.type == CC_SHAPE_CAPSULE ? 2 : 0);
+        if (q.type == CC_SHAPE_TRIANGLE_MESH) {
+            o.hfp = q.mesh_first_triangle; o.hnp = q.mesh_num_triangles;
+            o.hff = q.mesh_first_node; o.hnf = q.mesh_num_nodes;
+            o.mfv = 3;
+        }
         o.radius = T(q.radius); o.hh = T(q.half_height); o.cr = T(q.convex_radius);
         for (int k = 0; k < 3; k++) {
             o.he[k] = T(q.half_extents[k]); o.com[k] = T(q.center_of_mass[k]);
@@ -362,7 +379,57 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         HIPCHK(c, hipMemcpy(dFnv, hulls->face_num_vertices, F * sizeof(int), hipMemcpyHostToDevice));
     }
     if (FV > 0) HIPCHK(c, hipMemcpy(dFvi, hulls->face_vertex_indices, FV * sizeof(int), hipMemcpyHostToDevice));
-    d.st = ShapeTable<T>{dShapes, dPts, dShr, dFfv, dFnv, dFvi, dPn};
+    d.st = ShapeTable<T>{dShapes, dPts, dShr, dFfv, dFnv, dFvi, dPn, nullptr, nullptr, nullptr};
+    // triangle meshes: vertices and triangles as given, BVH nodes decoded from the node buffer
+    X->hasMesh = false;
+    for (int i = 0; i < ns; i++) X->hasMesh = X->hasMesh || shapes[i].type == CC_SHAPE_TRIANGLE_MESH;
+    if (X->hasMesh) {
+        const int MV = meshes->total_vertices, MT = meshes->total_triangles, MN = meshes->total_nodes;
+        std::vector<T> mp(3 * (size_t)std::max(MV, 1));
+        for (size_t k = 0; k < 3 * (size_t)MV; k++) mp[k] = T(meshes->positions[k]);
+        std::vector<int4> mt((size_t)std::max(MT, 1));
+        for (int t = 0; t < MT; t++) {
+            const int32_t* r = meshes->triangles + 4 * (size_t)t;
+            for (int k = 0; k < 3; k++)
+                if (r[k] < 0 || r[k] >= MV) return fail(c, CC_E_ARG, "mesh triangle vertex out of range");
+            mt[t] = make_int4(r[0], r[1], r[2], r[3] & 7);
+        }
+        std::vector<MNode<T>> mn((size_t)std::max(MN, 1));
+        for (int i = 0; i < ns; i++) {  // decode each mesh's nodes (offsets count doubles from its first node)
+            const cc_shape_desc& q = shapes[i];
+            if (q.type != CC_SHAPE_TRIANGLE_MESH) continue;
+            for (int j = 0; j < q.mesh_num_nodes; j++) {
+                const double* nd = meshes->nodes + 8 * (size_t)(q.mesh_first_node + j);
+                MNode<T>& o = mn[(size_t)(q.mesh_first_node + j)];
+                for (int k = 0; k < 3; k++) {
+                    T lo = T(nd[k]), hi = T(nd[3 + k]);
+                    if ((double)lo > nd[k]) lo = std::nextafter(lo, T(-INFINITY));  // FP32: round outwards
+                    if ((double)hi < nd[3 + k]) hi = std::nextafter(hi, T(INFINITY));
+                    o.mn[k] = lo; o.mx[k] = hi;
+                }
+                if (nd[7] < 0) {
+                    const int st = (int)nd[6], cnt = -((int)nd[7] + 1);
+                    if (st < 0 || cnt < 0 || st + cnt > q.mesh_num_triangles) return fail(c, CC_E_ARG, "mesh BVH leaf out of range");
+                    o.a = st; o.b = (int)nd[7];
+                } else {
+                    const long long off = (long long)nd[6];
+                    if (off % 8 != 0 || off / 8 <= j || off / 8 >= q.mesh_num_nodes || j + 1 >= q.mesh_num_nodes)
+                        return fail(c, CC_E_ARG, "mesh BVH child offset out of range");
+                    o.a = (int)(off / 8); o.b = (int)nd[7];
+                }
+            }
+        }
+        T* dMp = A.get<T>(mp.size());
+        int4* dMt = A.get<int4>(mt.size());
+        MNode<T>* dMn = A.get<MNode<T>>(mn.size());
+        if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (meshes)");
+        HIPCHK(c, hipMemcpy(dMp, mp.data(), mp.size() * sizeof(T), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dMt, mt.data(), mt.size() * sizeof(int4), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dMn, mn.data(), mn.size() * sizeof(MNode<T>), hipMemcpyHostToDevice));
+        d.st.mpos = dMp;
+        d.st.mtri = dMt;
+        d.st.mnode = dMn;
+    }
     // bodies
     X->hostWStart.assign(woff, woff + nw + 1);
     std::vector<int> worldOf(std::max(nb, 1));
@@ -378,6 +445,10 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         if (b.shape < 0 || b.shape >= ns) return fail(c, CC_E_ARG, "body shape index out of range");
         if (b.motion_type < 0 || b.motion_type > 2) return fail(c, CC_E_ARG, "invalid motion type");
         if (b.object_layer < 0 || b.object_layer >= CC_MAX_OBJECT_LAYERS) return fail(c, CC_E_ARG, "invalid object layer");
+        if (X->hasMesh && shapes[b.shape].type == CC_SHAPE_TRIANGLE_MESH && b.motion_type != CC_MOTION_STATIC)
+            return fail(c, CC_E_UNSUPPORTED, "triangle-mesh bodies must be static");
+        if (X->hasMesh && shapes[b.shape].type == CC_SHAPE_SPHERE)
+            return fail(c, CC_E_UNSUPPORTED, "sphere bodies in a scene with a triangle mesh (sphere vs mesh is not on the GPU path)");
         pos[i] = mk3<T>(T(b.position[0]), T(b.position[1]), T(b.position[2]));
         q[i].x = T(b.quaternion[0]); q[i].y = T(b.quaternion[1]); q[i].z = T(b.quaternion[2]); q[i].w = T(b.quaternion[3]);
         v[i] = mk3<T>(T(b.linear_velocity[0]), T(b.linear_velocity[1]), T(b.linear_velocity[2]));
@@ -591,7 +662,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
 
 extern "C" int cc_upload_scene(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* woff, const cc_body_desc* bodies,
                                int32_t nb, const cc_shape_desc* shapes, int32_t ns, const cc_hull_data* hulls,
-                               const cc_point_constraint_desc* pcs, int32_t npc) {
+                               const cc_point_constraint_desc* pcs, int32_t npc, const cc_mesh_data* meshes) {
     if (!c || !s || nw <= 0 || !woff || nb < 0 || (nb > 0 && !bodies) || ns < 0 || (ns > 0 && !shapes))
         return fail(c, CC_E_ARG, "invalid arguments to cc_upload_scene");
     if (woff[0] != 0 || woff[nw] != nb) return fail(c, CC_E_ARG, "world offsets must start at 0 and end at num_bodies");
@@ -600,8 +671,36 @@ extern "C" int cc_upload_scene(cc_ctx* c, const cc_settings* s, int32_t nw, cons
     if (npc > 0 && !pcs) return fail(c, CC_E_ARG, "point constraints pointer is null");
     HIPCHK(c, hipSetDevice(c->cfg.device));
     c->steps = 0;
-    if (c->cfg.precision == CC_PRECISION_FP64) return upload<double>(c, s, nw, woff, bodies, nb, shapes, ns, hulls, pcs, npc);
-    return upload<float>(c, s, nw, woff, bodies, nb, shapes, ns, hulls, pcs, npc);
+    if (c->cfg.precision == CC_PRECISION_FP64)
+        return upload<double>(c, s, nw, woff, bodies, nb, shapes, ns, hulls, pcs, npc, meshes);
+    return upload<float>(c, s, nw, woff, bodies, nb, shapes, ns, hulls, pcs, npc, meshes);
+}
+
+extern "C" int cc_build_triangle_mesh(const double* positions, int32_t nv, const int32_t* indices, int32_t nt,
+                                      double degenerate_tolerance, double active_edge_cos_threshold, int32_t max_leaf,
+                                      double* out_positions, int32_t* out_triangles, double* out_nodes,
+                                      int32_t* out_counts) {
+    if (nv < 0 || nt < 0 || (nv > 0 && !positions) || (nt > 0 && !indices) || max_leaf < 1 || !out_counts ||
+        (nv > 0 && !out_positions) || (nt > 0 && (!out_triangles || !out_nodes)))
+        return CC_E_ARG;
+    ccmesh::Builder B;
+    if (ccmesh::build_triangle_mesh(positions, nv, indices, nt, degenerate_tolerance, active_edge_cos_threshold, max_leaf,
+                                    B) != 0)
+        return CC_E_ARG;
+    const size_t V = B.pos.size() / 3, T = B.tri.size(), N = B.nodes.size() / 8;
+    if (V > (size_t)nv || T > (size_t)nt || N > 2 * (size_t)nt) return CC_E_ARG;
+    if (V) memcpy(out_positions, B.pos.data(), sizeof(double) * 3 * V);
+    for (size_t t = 0; t < T; t++) {
+        out_triangles[4 * t] = B.tri[t].a;
+        out_triangles[4 * t + 1] = B.tri[t].b;
+        out_triangles[4 * t + 2] = B.tri[t].c;
+        out_triangles[4 * t + 3] = B.tri[t].act;
+    }
+    if (N) memcpy(out_nodes, B.nodes.data(), sizeof(double) * 8 * N);
+    out_counts[0] = (int32_t)V;
+    out_counts[1] = (int32_t)T;
+    out_counts[2] = (int32_t)N;
+    return CC_OK;
 }
 
 static int grid_for(long long n, int bs, int maxBlocks) {
@@ -1063,7 +1162,7 @@ static int bp_impl(cc_ctx* c, const float* mn, const float* mx, const int32_t* m
         bd[i].allowed_dofs = 63;
     }
     int woff[2] = {0, n};
-    int r = upload<T>(c, &s, 1, woff, bd.data(), n, &sh, 1, nullptr, nullptr, 0);
+    int r = upload<T>(c, &s, 1, woff, bd.data(), n, &sh, 1, nullptr, nullptr, 0, nullptr);
     if (r) return r;
     auto* X = static_cast<CtxT<T>*>(c->impl);
     Dev<T>& d = X->d;

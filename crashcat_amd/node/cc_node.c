@@ -108,6 +108,7 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_settings, gravity_enabled, "i32", 1); F(cc_settings, warm_starting, "i32", 1);
     F(cc_settings, allow_sleeping, "i32", 1); F(cc_settings, num_object_layers, "i32", 1);
     F(cc_settings, object_layer_pairs, "u8", CC_MAX_OBJECT_LAYERS * CC_MAX_OBJECT_LAYERS);
+    F(cc_settings, collide_with_backfaces, "i32", 1); F(cc_settings, collide_only_with_active_edges, "i32", 1);
     add_struct(env, out, "cc_settings", sizeof(cc_settings), fields);
     napi_create_object(env, &fields);
     F(cc_shape_desc, type, "i32", 1); F(cc_shape_desc, hull_first_point, "i32", 1);
@@ -116,6 +117,8 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_shape_desc, half_height, "f64", 1); F(cc_shape_desc, half_extents, "f64", 3);
     F(cc_shape_desc, convex_radius, "f64", 1); F(cc_shape_desc, center_of_mass, "f64", 3);
     F(cc_shape_desc, aabb_min, "f64", 3); F(cc_shape_desc, aabb_max, "f64", 3);
+    F(cc_shape_desc, mesh_first_triangle, "i32", 1); F(cc_shape_desc, mesh_num_triangles, "i32", 1);
+    F(cc_shape_desc, mesh_first_node, "i32", 1); F(cc_shape_desc, mesh_num_nodes, "i32", 1);
     add_struct(env, out, "cc_shape_desc", sizeof(cc_shape_desc), fields);
     napi_create_object(env, &fields);
     F(cc_body_desc, shape, "i32", 1); F(cc_body_desc, motion_type, "i32", 1);
@@ -224,11 +227,28 @@ static napi_value upload(napi_env env, napi_callback_info info) {
     if (napi_get_named_property(env, h, "totalFaces", &v) == napi_ok) napi_get_value_int32(env, v, &hull.total_faces);
     if (napi_get_named_property(env, h, "totalFaceVertices", &v) == napi_ok)
         napi_get_value_int32(env, v, &hull.total_face_vertices);
+    /* triangle meshes ride on the same object: meshPositions (Float64Array), meshTriangles
+     * (Int32Array, 4 per triangle), meshNodes (Float64Array, the BVH node buffer) */
+    cc_mesh_data mesh;
+    memset(&mesh, 0, sizeof mesh);
+    size_t mvb = 0, mtb = 0, mnb = 0;
+    if (napi_has_named_property(env, h, "meshPositions", &has) == napi_ok && has) {
+        napi_get_named_property(env, h, "meshPositions", &v);
+        mesh.positions = (const double*)buffer_data(env, v, &mvb);
+        napi_get_named_property(env, h, "meshTriangles", &v);
+        mesh.triangles = (const int32_t*)buffer_data(env, v, &mtb);
+        napi_get_named_property(env, h, "meshNodes", &v);
+        mesh.nodes = (const double*)buffer_data(env, v, &mnb);
+        if (!mesh.positions || !mesh.triangles || !mesh.nodes) return NULL;
+        mesh.total_vertices = (int32_t)(mvb / (3 * sizeof(double)));
+        mesh.total_triangles = (int32_t)(mtb / (4 * sizeof(int32_t)));
+        mesh.total_nodes = (int32_t)(mnb / (8 * sizeof(double)));
+    }
     int32_t nw = (int32_t)(ob / sizeof(int32_t)) - 1;
     int rc = cc_upload_scene(c, (const cc_settings*)s, nw, off, (const cc_body_desc*)bodies,
                              (int32_t)(bb / sizeof(cc_body_desc)), (const cc_shape_desc*)shapes,
                              (int32_t)(shb / sizeof(cc_shape_desc)), &hull, (const cc_point_constraint_desc*)pcs,
-                             (int32_t)(pb / sizeof(cc_point_constraint_desc)));
+                             (int32_t)(pb / sizeof(cc_point_constraint_desc)), mesh.positions ? &mesh : NULL);
     if (rc != CC_OK) return throw_rc(env, c, rc);
     return NULL;
 }

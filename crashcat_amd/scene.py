@@ -38,6 +38,8 @@ def create_world_settings():
     s.warm_starting = 1
     s.allow_sleeping = 1
     s.num_object_layers = 0
+    s.collide_with_backfaces = 0
+    s.collide_only_with_active_edges = 1
     return s
 
 
@@ -185,6 +187,8 @@ class Scene:
         arr = (_abi.ShapeDesc * max(n, 1))()
         pts, shr, ffv, fnv, fvi, pn, pc = [], [], [], [], [], [], []
         np_, nf, nfv = 0, 0, 0
+        mpos, mtri, mnode = [], [], []
+        nmv, nmt, nmn = 0, 0, 0
         for i, s in enumerate(self.shapes):
             d = arr[i]
             d.type = s.type
@@ -211,6 +215,19 @@ class Scene:
                     pc.append(c)
                 np_ += len(s.points)
                 nf += len(s.faces)
+            if s.type == S.TRIANGLE_MESH:
+                d.mesh_first_triangle = nmt
+                d.mesh_num_triangles = len(s.mesh_triangles)
+                d.mesh_first_node = nmn
+                d.mesh_num_nodes = len(s.mesh_nodes)
+                t = np.array(s.mesh_triangles, dtype=np.int32)
+                t[:, :3] += nmv
+                mpos.append(np.asarray(s.mesh_positions, dtype=np.float64))
+                mtri.append(t)
+                mnode.append(np.asarray(s.mesh_nodes, dtype=np.float64))
+                nmv += len(s.mesh_positions)
+                nmt += len(t)
+                nmn += len(s.mesh_nodes)
         keep = {}
         keep["pts"] = np.ascontiguousarray(np.concatenate(pts) if pts else np.zeros((1, 3))).astype(np.float64)
         keep["shr"] = np.ascontiguousarray(np.concatenate(shr) if shr else np.zeros((1, 3))).astype(np.float64)
@@ -232,6 +249,17 @@ class Scene:
         h.total_points = np_
         h.total_faces = nf
         h.total_face_vertices = nfv
+        keep["mpos"] = np.ascontiguousarray(np.concatenate(mpos) if mpos else np.zeros((1, 3)), dtype=np.float64)
+        keep["mtri"] = np.ascontiguousarray(np.concatenate(mtri) if mtri else np.zeros((1, 4)), dtype=np.int32)
+        keep["mnode"] = np.ascontiguousarray(np.concatenate(mnode) if mnode else np.zeros((1, 8)), dtype=np.float64)
+        m = _abi.MeshData()
+        m.positions = keep["mpos"].ctypes.data_as(dp)
+        m.triangles = keep["mtri"].ctypes.data_as(ip)
+        m.nodes = keep["mnode"].ctypes.data_as(dp)
+        m.total_vertices = nmv
+        m.total_triangles = nmt
+        m.total_nodes = nmn
+        keep["meshes"] = m
         keep["arr"] = arr
         return arr, h, keep
 

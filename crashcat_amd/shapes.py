@@ -27,6 +27,7 @@ SPHERE = _abi.CC_SHAPE_SPHERE
 BOX = _abi.CC_SHAPE_BOX
 CAPSULE = _abi.CC_SHAPE_CAPSULE
 CONVEX_HULL = _abi.CC_SHAPE_CONVEX_HULL
+TRIANGLE_MESH = _abi.CC_SHAPE_TRIANGLE_MESH
 
 
 class Shape:
@@ -51,10 +52,14 @@ class Shape:
         self.plane_normals = kw.get("plane_normals")
         self.plane_constants = kw.get("plane_constants")
         self.volume = kw.get("volume", 0.0)
+        # triangle-mesh data (built: welded vertices, triangles in BVH leaf order, BVH nodes)
+        self.mesh_positions = kw.get("mesh_positions")
+        self.mesh_triangles = kw.get("mesh_triangles")
+        self.mesh_nodes = kw.get("mesh_nodes")
 
     def key(self):
         """Hashable identity used to deduplicate identical shapes at upload."""
-        if self.type == CONVEX_HULL:
+        if self.type in (CONVEX_HULL, TRIANGLE_MESH):
             return (self.type, id(self))
         return (self.type, self.radius, self.half_height, self.half_extents, self.convex_radius, self.density)
 
@@ -437,3 +442,40 @@ def motion_mass(mass, inertia, allowed_dofs=0b111111):
             return inv_mass, inv, _quat_from_mat3(r)
     s = 2.5 * inv_mass
     return inv_mass, (s, s, s), (0.0, 0.0, 0.0, 1.0)
+
+
+# ------------------------------------------------------------------ triangle mesh
+def triangle_mesh(positions, indices, degenerate_tolerance=1e-6, active_edge_cos_threshold=0.996195,
+                  max_leaf_triangles=8):
+    """triangleMesh.create — src/shapes/triangle-mesh.ts:114-147 with DEFAULT_TRIANGLE_MESH_OPTIONS
+    (:79-84): the mesh is built on the host by the library (weld, clean, active edges, CENTER BVH);
+    its bounds are the BVH root's, its centre of mass the origin, its mass zero (static geometry)."""
+    from ._lib import build_triangle_mesh
+    pos, tri, nodes = build_triangle_mesh(positions, indices, degenerate_tolerance, active_edge_cos_threshold,
+                                          max_leaf_triangles)
+    if len(nodes):
+        mn, mx = tuple(nodes[0, 0:3]), tuple(nodes[0, 3:6])
+    else:
+        mn = mx = (0.0, 0.0, 0.0)
+    return Shape(TRIANGLE_MESH, aabb_min=mn, aabb_max=mx, mass=0.0, inertia=np.zeros((3, 3)), mesh_positions=pos,
+                 mesh_triangles=tri, mesh_nodes=nodes)
+
+
+def height_field_mesh(nx, nz, spacing, height, origin=(0.0, 0.0)):
+    """Positions and indices of a regular grid of nx x nz vertices at `spacing`, with
+    height(x, z) giving y; two triangles per cell, wound so that their normals point up."""
+    xs = origin[0] + spacing * np.arange(nx)
+    zs = origin[1] + spacing * np.arange(nz)
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    Y = height(X, Z)
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    i = np.arange(nx - 1)[:, None]
+    k = np.arange(nz - 1)[None, :]
+    v00 = (i * nz + k).ravel()
+    v10 = ((i + 1) * nz + k).ravel()
+    v01 = (i * nz + k + 1).ravel()
+    v11 = ((i + 1) * nz + k + 1).ravel()
+    tris = np.empty((2 * len(v00), 3), dtype=np.int32)
+    tris[0::2] = np.stack([v00, v01, v10], axis=1)
+    tris[1::2] = np.stack([v10, v01, v11], axis=1)
+    return pos, tris

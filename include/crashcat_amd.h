@@ -4,7 +4,8 @@
  * The library replaces the body of crashcat's
  *     updateWorld(world, listener, timeStep)          src/update.ts:45
  * for worlds whose bodies use the convex shapes {sphere, box, capsule,
- * convex hull} and whose listener has no callbacks.  Everything that
+ * convex hull} and static triangle meshes (collided with boxes, capsules and
+ * hulls), and whose listener has no callbacks.  Everything that
  * crosses this boundary is plain data: the host (JavaScript over N-API,
  * Python over ctypes, or C/C++) owns every buffer it passes in, calls copy
  * synchronously and keep no alias after they return; device memory is owned
@@ -36,6 +37,7 @@ extern "C" {
 #define CC_SHAPE_BOX 1
 #define CC_SHAPE_CAPSULE 2
 #define CC_SHAPE_CONVEX_HULL 3
+#define CC_SHAPE_TRIANGLE_MESH 4
 
 /* ---- motion types: src/body/motion-type.ts ------------------------------ */
 #define CC_MOTION_STATIC 0
@@ -79,10 +81,14 @@ typedef struct cc_settings {
     /* object_layer_pairs[a * CC_MAX_OBJECT_LAYERS + b] != 0 when object layers a and b
      * collide (src/layers.ts:88-112, enableCollision). */
     uint8_t object_layer_pairs[CC_MAX_OBJECT_LAYERS * CC_MAX_OBJECT_LAYERS];
+    int32_t collide_with_backfaces;         /* narrowphase.collideWithBackfaces (false)       */
+    int32_t collide_only_with_active_edges; /* narrowphase.collideOnlyWithActiveEdges (true)  */
 } cc_settings;
 
-/* One shape; mirrors the fields of the Sphere/Box/Capsule/ConvexHull shape records
- * (src/shapes/sphere.ts:40-56, box.ts:42-52, capsule.ts:30-48, convex-hull.ts:37-62). */
+/* One shape; mirrors the fields of the Sphere/Box/Capsule/ConvexHull/TriangleMesh shape
+ * records (src/shapes/sphere.ts:40-56, box.ts:42-52, capsule.ts:30-48, convex-hull.ts:37-62,
+ * triangle-mesh.ts:88-95).  A triangle mesh's aabb is its BVH root's bounds and its centre
+ * of mass is the origin (triangle-mesh.ts:114-147). */
 typedef struct cc_shape_desc {
     int32_t type;               /* CC_SHAPE_* */
     int32_t hull_first_point;   /* hulls: offset into cc_hull_data.points / shrunk_points */
@@ -97,6 +103,10 @@ typedef struct cc_shape_desc {
     double center_of_mass[3];   /* shape.centerOfMass (hull: the hull's COM)        */
     double aabb_min[3];         /* shape.aabb (local bounds)                        */
     double aabb_max[3];
+    int32_t mesh_first_triangle; /* meshes: offset into cc_mesh_data.triangles (in triangles) */
+    int32_t mesh_num_triangles;
+    int32_t mesh_first_node;     /* meshes: offset into cc_mesh_data.nodes (in nodes)          */
+    int32_t mesh_num_nodes;
 } cc_shape_desc;
 
 /* Flattened convex-hull data shared by all hull shapes.  Points are relative to the
@@ -115,6 +125,27 @@ typedef struct cc_hull_data {
     int32_t total_face_vertices;
     int32_t pad_;
 } cc_hull_data;
+
+/* Flattened triangle-mesh data shared by all mesh shapes: the TriangleMeshData and
+ * TriangleMeshBVH a mesh shape holds after buildTriangleMesh
+ * (src/shapes/utils/triangle-mesh-builder.ts:62-220).
+ *   triangles: 4 ints per triangle — vertex a, b, c (absolute indices into positions) and
+ *     the active-edge bits (bit 0 edge a-b, bit 1 b-c, bit 2 c-a; triangle-mesh-data.ts:4-14);
+ *     a mesh's triangles are in BVH leaf order, so a leaf covers a contiguous range.
+ *   nodes: 8 doubles per node, the reference's node buffer (triangle-mesh-bvh.ts:15-25):
+ *     min xyz, max xyz, then for an internal node the right child's offset and the split
+ *     axis, for a leaf the first triangle and -(count + 1).  Offsets count doubles from the
+ *     mesh's first node (the left child of the node at offset o is at o + 8); triangle
+ *     indices count from the mesh's first triangle. */
+typedef struct cc_mesh_data {
+    const double* positions;          /* [3 * total_vertices] */
+    const int32_t* triangles;         /* [4 * total_triangles] */
+    const double* nodes;              /* [8 * total_nodes]     */
+    int32_t total_vertices;
+    int32_t total_triangles;
+    int32_t total_nodes;
+    int32_t pad_;
+} cc_mesh_data;
 
 /* One body at upload; mirrors RigidBody + MotionProperties (src/body/rigid-body.ts:142-252,
  * src/body/motion-properties.ts:25-69).  Mass properties are already decomposed into
@@ -177,6 +208,8 @@ typedef struct cc_contact_out {
     int32_t body_b;
     int32_t num_points;
     int32_t pad_;
+    uint32_t sub_shape_a;       /* sub-shape ids (src/body/sub-shape.ts); 0xffffffff when empty */
+    uint32_t sub_shape_b;
     double normal[3];           /* world space, A -> B                              */
     double position_a[4][3];    /* world-space contact points on A and B at creation */
     double position_b[4][3];
@@ -232,13 +265,31 @@ const char* cc_last_error(const cc_ctx* ctx);
  * entries; bodies of world w are [offsets[w], offsets[w+1]) and keep creation order
  * (src/body/rigid-body.ts:398-407).  Point constraints refer to global body indices
  * and must join bodies of the same world.  Replaces rigidBody.create (src/body/rigid-body.ts:390-434)
- * + broadphase.addBody for all bodies at once. */
+ * + broadphase.addBody for all bodies at once.  meshes may be NULL when no shape is a
+ * triangle mesh.  Bodies with a mesh shape must be static, and a scene with a mesh body
+ * has no sphere bodies (the sphere-vs-mesh path, triangle-mesh.ts:1418-1780, is not on the
+ * GPU): both are rejected with CC_E_UNSUPPORTED. */
 int cc_upload_scene(cc_ctx* ctx, const cc_settings* settings,
                     int32_t num_worlds, const int32_t* world_body_offsets,
                     const cc_body_desc* bodies, int32_t num_bodies,
                     const cc_shape_desc* shapes, int32_t num_shapes,
                     const cc_hull_data* hulls,
-                    const cc_point_constraint_desc* point_constraints, int32_t num_point_constraints);
+                    const cc_point_constraint_desc* point_constraints, int32_t num_point_constraints,
+                    const cc_mesh_data* meshes);
+
+/* Build one triangle mesh's data from a triangle soup, on the host: weld vertices on a
+ * 1e-8 grid, drop triangles whose edge cross product is shorter than degenerate_tolerance
+ * and duplicates, mark active edges by the dihedral-angle cosine threshold, and build the
+ * BVH with the CENTER split strategy and max_leaf_triangles per leaf; replaces
+ * buildTriangleMesh (src/shapes/utils/triangle-mesh-builder.ts:62-220) with
+ * triangleMeshBvh.build (src/shapes/utils/triangle-mesh-bvh.ts:368-576).  Outputs use the
+ * cc_mesh_data layout with vertex indices from 0; capacities: out_positions 3 * num_vertices
+ * doubles, out_triangles 4 * num_triangles ints, out_nodes 8 * (2 * num_triangles) doubles.
+ * out_counts receives (vertices, triangles, nodes). */
+int cc_build_triangle_mesh(const double* positions, int32_t num_vertices, const int32_t* indices,
+                           int32_t num_triangles, double degenerate_tolerance, double active_edge_cos_threshold,
+                           int32_t max_leaf_triangles, double* out_positions, int32_t* out_triangles,
+                           double* out_nodes, int32_t* out_counts);
 
 /* Advance every world by nsteps steps of dt; replaces updateWorld (src/update.ts:45-239).
  * Work is enqueued on the context's stream; call cc_sync to wait. */

@@ -29,7 +29,7 @@ typedef struct oc_world oc_world;
 /* world / step */
 oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb,
                     const cc_shape_desc* shapes, int32_t ns, const cc_hull_data* hulls,
-                    const cc_point_constraint_desc* pcs, int32_t npc);
+                    const cc_point_constraint_desc* pcs, int32_t npc, const cc_mesh_data* meshes);
 void oc_destroy(oc_world* w);
 void oc_step(oc_world* w, double dt);
 int32_t oc_num_bodies(const oc_world* w);
@@ -37,7 +37,7 @@ void oc_get_bodies(const oc_world* w, cc_body_state* out);
 void oc_set_bodies(oc_world* w, const cc_body_state* in);
 /* pairs of the last step as (min,max) index pairs, sorted */
 int32_t oc_get_pairs(const oc_world* w, int32_t* out, int32_t cap);
-/* contact constraints of the last step, sorted by (a,b) */
+/* contact constraints of the last step, sorted by (a, b, sub-shape a, sub-shape b) */
 int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap);
 void oc_get_counters(const oc_world* w, cc_step_counters* out);
 /* broadphase only, on given AABBs (widened from float) */
@@ -61,6 +61,13 @@ void oc_collide(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t 
 void oc_collide_convex(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t sa, const double* pa,
                        const double* qa, int32_t sb, const double* pb, const double* qb, double max_separation,
                        const cc_settings* s, oc_hit* out);
+/* collideConvexVsTriangleMesh (src/shapes/triangle-mesh.ts:1026-1379): every hit of convex shape sa
+ * against mesh shape sm, in traversal order, with the triangle's sub-shape id in sub_b[i];
+ * s supplies the tolerances and the back-face / active-edge switches; returns the hit count */
+int32_t oc_collide_mesh(const cc_shape_desc* shapes, const cc_hull_data* hulls, const cc_mesh_data* meshes, int32_t sa,
+                        const double* pa, const double* qa, int32_t sm, const double* pm, const double* qm,
+                        double max_separation, const double* move_dir, const cc_settings* s, oc_hit* out,
+                        uint32_t* sub_b, int32_t cap);
 /* manifold of one hit, pruned to <= 4 points (update.ts:385-563 for a single hit);
  * out_a/out_b are world points; returns the point count */
 int32_t oc_manifold_from_hit(const oc_hit* hit, double max_contact_distance, double* normal_out,

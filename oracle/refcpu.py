@@ -31,7 +31,7 @@ def lib():
         L.oc_create.restype = C.c_void_p
         L.oc_create.argtypes = [C.POINTER(_abi.Settings), C.POINTER(_abi.BodyDesc), C.c_int32,
                                 C.POINTER(_abi.ShapeDesc), C.c_int32, C.POINTER(_abi.HullData),
-                                C.POINTER(_abi.PointConstraintDesc), C.c_int32]
+                                C.POINTER(_abi.PointConstraintDesc), C.c_int32, C.POINTER(_abi.MeshData)]
         L.oc_destroy.argtypes = [C.c_void_p]
         L.oc_step.argtypes = [C.c_void_p, C.c_double]
         L.oc_get_bodies.argtypes = [C.c_void_p, C.POINTER(_abi.BodyState)]
@@ -47,6 +47,9 @@ def lib():
         vp = C.c_void_p
         for fn in ("oc_collide", "oc_collide_convex"):
             getattr(L, fn).argtypes = [vp, vp, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double, vp, vp]
+        L.oc_collide_mesh.restype = C.c_int32
+        L.oc_collide_mesh.argtypes = [vp, vp, vp, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double, dp, vp, vp,
+                                      C.POINTER(C.c_uint32), C.c_int32]
         L.oc_manifold_from_hit.restype = C.c_int32
         L.oc_manifold_from_hit.argtypes = [vp, C.c_double, dp, dp, dp]
         L.oc_clip_poly_vs_plane.restype = C.c_int32
@@ -93,7 +96,7 @@ class OracleWorld:
         self._settings = scene.settings
         self.n = b1 - b0
         self.h = L.oc_create(C.byref(scene.settings), bodies.ctypes.data_as(C.POINTER(_abi.BodyDesc)), self.n,
-                             shapes, len(scene.shapes), C.byref(hulls), pa, len(pcs))
+                             shapes, len(scene.shapes), C.byref(hulls), pa, len(pcs), C.byref(self._keep["meshes"]))
 
     def __del__(self):
         try:
@@ -284,3 +287,23 @@ def supporting_face(scene, s, direction, pos=(0, 0, 0), quat=(0, 0, 0, 1)):
     out = np.zeros((64, 3))
     k = lib().oc_supporting_face(shapes, C.byref(hulls), s, _dp(d), _dp(p), _dp(q), _dp(out))
     return out[:k]
+
+
+def collide_mesh(scene, sa, pa, qa, sm, pm=(0, 0, 0), qm=(0, 0, 0, 1), max_separation=0.0, move_dir=(0, 0, 0),
+                 backfaces=True, active_edges=True, cap=256):
+    """Hits of convex shape sa against mesh shape sm (collideConvexVsTriangleMesh) with the
+    default collide settings (createDefaultCollideShapeSettings, collide-shape-vs-shape.ts:203-213)
+    unless given: list of (Hit, sub-shape id of the triangle)."""
+    L = lib()
+    shapes, hulls, keep = scene.c_shapes()
+    st = _abi.Settings()
+    C.pointer(st)[0] = scene.settings
+    st.collide_with_backfaces = 1 if backfaces else 0
+    st.collide_only_with_active_edges = 1 if active_edges else 0
+    out = (Hit * cap)()
+    sub = (C.c_uint32 * cap)()
+    v = [_vec(pa), _vec(qa, 4), _vec(pm), _vec(qm, 4), _vec(move_dir)]
+    n = L.oc_collide_mesh(C.cast(shapes, C.c_void_p), C.cast(C.pointer(hulls), C.c_void_p),
+                          C.cast(C.pointer(keep["meshes"]), C.c_void_p), sa, _dp(v[0]), _dp(v[1]), sm, _dp(v[2]),
+                          _dp(v[3]), max_separation, _dp(v[4]), C.cast(C.pointer(st), C.c_void_p), out, sub, cap)
+    return [(out[i], sub[i]) for i in range(min(n, cap))]

@@ -14,9 +14,10 @@ int64_t ocx_epa_calls = 0;
 /* ======================================================================== */
 /* support functions (src/collision/support.ts and the shape files)       */
 /* ======================================================================== */
-enum { SK_ZERO, SK_SPHERE_R, SK_BOX, SK_CAP_NC, SK_CAP_R, SK_HULL };
+enum { SK_ZERO, SK_SPHERE_R, SK_BOX, SK_CAP_NC, SK_CAP_R, SK_HULL, SK_TRI };
 typedef struct {
     int kind;
+    V3 ta, tb, tc; /* SK_TRI: the triangle's vertices */
     double r, hh;
     V3 he;
     const double* pts;
@@ -105,6 +106,13 @@ static V3 sup_base(const OSup* s, V3 d) {
             if (dt > best) { best = dt; out = p; }
         }
         return out;
+    }
+    case SK_TRI: { /* triangleGetSupport support.ts:83-118 */
+        double d1 = s->ta.x * d.x + s->ta.y * d.y + s->ta.z * d.z;
+        double d2 = s->tb.x * d.x + s->tb.y * d.y + s->tb.z * d.z;
+        double d3 = s->tc.x * d.x + s->tc.y * d.y + s->tc.z * d.z;
+        if (d1 > d2) return d1 > d3 ? s->ta : s->tc;
+        return d2 > d3 ? s->tb : s->tc;
     }
     }
     return v3(0, 0, 0);
@@ -993,6 +1001,157 @@ static void collide_convex_convex(const OShapes* sh, int sa, V3 pa, Q4 qa, int s
     supporting_face(&out->faceB, sh, sb, dirB, posB, qB);
 }
 
+/* fixNormal src/collision/active-edges.ts:74-153: the triangle normal replaces the GJK/EPA
+ * normal when the contact point is on an inactive edge or inside the triangle */
+static V3 fix_normal(V3 v0, V3 v1, V3 v2, V3 tn, int act, V3 p, V3 n, V3 md) {
+    if (act == 7) return n;
+    double nl = vlen(n), tl = vlen(tn);
+    if (vdot(md, n) * tl < vdot(md, tn) * nl) return n;
+    if (act == 0) return tn;
+    if (vdot(tn, n) > 0.999848 * nl * tl) return n;
+    Bary b = bary3(vsub(v0, p), vsub(v1, p), vsub(v2, p), 1e-10);
+    int edge;
+    if (b.u > 1.0 - 1.0e-4) edge = 5;
+    else if (b.v > 1.0 - 1.0e-4) edge = 3;
+    else if (b.w > 1.0 - 1.0e-4) edge = 6;
+    else if (b.u < 1.0e-4) edge = 2;
+    else if (b.v < 1.0e-4) edge = 4;
+    else if (b.w < 1.0e-4) edge = 1;
+    else return tn;
+    return (act & edge) ? n : tn;
+}
+
+static int g_mesh_calls;
+int ocx_mesh_calls(void) { return g_mesh_calls; }
+
+/* collideConvexVsTriangleMesh src/shapes/triangle-mesh.ts:1026-1379 at unit scale: the convex
+ * shape A's bounds in mesh space cull the BVH (closer child first), each triangle is moved to
+ * A's space and collided by GJK (EPA when the cores overlap) against A's support, the normal is
+ * corrected at inactive edges, and each hit goes to fn in traversal order. */
+void ocx_collide_mesh(const OShapes* sh, int sa, V3 pa, Q4 qa, int sm, V3 pm, Q4 qm, const cc_settings* s,
+                      double maxSep0, V3 moveDir, ocx_hit_fn fn, void* ctx) {
+    const cc_shape_desc* A = &sh->shapes[sa];
+    const cc_shape_desc* M = &sh->shapes[sm];
+    const cc_mesh_data* md = sh->meshes;
+    g_mesh_calls++;
+    Q4 iqa = {-qa.x, -qa.y, -qa.z, qa.w};
+    Q4 t21q = qmul(iqa, qm);
+    V3 t21p = qrotate(iqa, vsub(pm, pa));
+    Q4 iqm = {-qm.x, -qm.y, -qm.z, qm.w};
+    Q4 qAinB = qmul(iqm, qa);
+    V3 pAinB = qrotate(iqm, vsub(pa, pm));
+    V3 e = v3(maxSep0, maxSep0, maxSep0);
+    V3 b1mn = vsub(vload(A->aabb_min), e), b1mx = vadd(vload(A->aabb_max), e);
+    M3 R = m3_from_quat(qAinB);
+    V3 q2mn = v3(0, 0, 0), q2mx = v3(0, 0, 0);
+    for (int c = 0; c < 8; c++) { /* box3.transformMat4 of the local bounds */
+        V3 p = v3((c & 1) ? A->aabb_max[0] : A->aabb_min[0], (c & 2) ? A->aabb_max[1] : A->aabb_min[1],
+                  (c & 4) ? A->aabb_max[2] : A->aabb_min[2]);
+        V3 wp = vadd(m3_mul(&R, p), pAinB);
+        if (c == 0) { q2mn = wp; q2mx = wp; }
+        else {
+            q2mn = v3(dmin(q2mn.x, wp.x), dmin(q2mn.y, wp.y), dmin(q2mn.z, wp.z));
+            q2mx = v3(dmax(q2mx.x, wp.x), dmax(q2mx.y, wp.y), dmax(q2mx.z, wp.z));
+        }
+    }
+    q2mn = vsub(q2mn, e);
+    q2mx = vadd(q2mx, e);
+    OSup SA;
+    sup_for_shape(&SA, sh, sa, MODE_EXCLUDE);
+    if (M->mesh_num_nodes == 0) return;
+    const double* nodes = md->nodes + 8 * (size_t)M->mesh_first_node;
+    const int32_t* tris = md->triangles + 4 * (size_t)M->mesh_first_triangle;
+    const int ntri = M->mesh_num_triangles;
+    const int nbits = ntri > 1 ? 32 - __builtin_clz((unsigned)(ntri - 1)) : 0; /* pushIndex sub-shape.ts:60-64 */
+    const uint32_t mask = nbits >= 32 ? 0xffffffffu : ((1u << nbits) - 1u);
+    V3 qc = v3((q2mn.x + q2mx.x) * 0.5, (q2mn.y + q2mx.y) * 0.5, (q2mn.z + q2mx.z) * 0.5);
+    V3 mdl = qrotate(iqa, moveDir);
+    static int* stack = NULL;
+    static int scap = 0;
+    int ns = 0;
+    if (!scap) { scap = 256; stack = (int*)malloc(sizeof(int) * (size_t)scap); }
+    stack[ns++] = 0;
+    while (ns > 0) {
+        const int o = stack[--ns];
+        const double* nd = nodes + o;
+        if (!(nd[0] <= q2mx.x && nd[3] >= q2mn.x && nd[1] <= q2mx.y && nd[4] >= q2mn.y && nd[2] <= q2mx.z &&
+              nd[5] >= q2mn.z))
+            continue;
+        if (nd[7] < 0) {
+            const int t0 = (int)nd[6], cnt = -((int)nd[7] + 1);
+            for (int i = 0; i < cnt; i++) {
+                const int ti = t0 + i;
+                const int32_t* tr = tris + 4 * (size_t)ti;
+                V3 a = vadd(qrotate(t21q, vload(md->positions + 3 * (size_t)tr[0])), t21p);
+                V3 b = vadd(qrotate(t21q, vload(md->positions + 3 * (size_t)tr[1])), t21p);
+                V3 c = vadd(qrotate(t21q, vload(md->positions + 3 * (size_t)tr[2])), t21p);
+                V3 tmn = v3(dmin(dmin(a.x, b.x), c.x), dmin(dmin(a.y, b.y), c.y), dmin(dmin(a.z, b.z), c.z));
+                V3 tmx = v3(dmax(dmax(a.x, b.x), c.x), dmax(dmax(a.y, b.y), c.y), dmax(dmax(a.z, b.z), c.z));
+                if (tmx.x < b1mn.x || tmn.x > b1mx.x || tmx.y < b1mn.y || tmn.y > b1mx.y || tmx.z < b1mn.z ||
+                    tmn.z > b1mx.z)
+                    continue;
+                V3 nrm = vcross(vsub(b, a), vsub(c, a));
+                int back = vdot(nrm, a) > 0.0;
+                if (!s->collide_with_backfaces && back) continue;
+                V3 axis = vneg(nrm);
+                if (vlen2(axis) < 1e-10) axis = v3(1, 0, 0);
+                else axis = vnormalize(axis);
+                double maxSep = maxSep0;
+                OSup T;
+                memset(&T, 0, sizeof(T));
+                T.kind = SK_TRI;
+                T.ta = a; T.tb = b; T.tc = c;
+                PenDepth pd;
+                Simplex simplex;
+                simplex.size = 0;
+                pd_step_gjk(&pd, &simplex, &SA, &T, SA.convexRadius + maxSep, 0.0, axis, s->collision_tolerance);
+                if (pd.status == PD_NOT_COLLIDING) continue;
+                int status = 0;
+                if (pd.status == PD_INDETERMINATE) {
+                    maxSep = dmin(maxSep, 1.0);
+                    OSup SA2;
+                    sup_for_shape(&SA2, sh, sa, MODE_INCLUDE);
+                    SA2.addRadius = 1;
+                    SA2.addR = maxSep;
+                    if (!pd_step_epa(&pd, &SA2, &T, s->penetration_tolerance, &simplex)) continue;
+                    status = 1;
+                }
+                double pen = sqrt(vlen2(vsub(pd.pointA, pd.pointB))) - maxSep;
+                double al = sqrt(vlen2(pd.axis));
+                if (al > 0.0) {
+                    double corr = maxSep / al;
+                    pd.pointA = vadd(pd.pointA, vscale(pd.axis, -corr));
+                }
+                const int act = tr[3];
+                if (s->collide_only_with_active_edges && act != 7)
+                    pd.axis = fix_normal(a, b, c, back ? nrm : vneg(nrm), act, pd.pointB, pd.axis, mdl);
+                OHit h;
+                h.hit = 1;
+                h.status = status;
+                h.pointA = vadd(qrotate(qa, pd.pointA), pa);
+                h.pointB = vadd(qrotate(qa, pd.pointB), pa);
+                h.axis = qrotate(qa, pd.axis);
+                h.penetration = pen;
+                supporting_face(&h.faceA, sh, sa, vneg(pd.axis), pa, qa);
+                h.faceB.n = 3;
+                h.faceB.v[0] = a; h.faceB.v[1] = b; h.faceB.v[2] = c;
+                face_transform(&h.faceB, pa, qa);
+                fn(ctx, &h, (0xffffffffu & ~mask) | (uint32_t)ti);
+            }
+        } else {
+            const int l = o + 8, r = (int)nd[6];
+            const double* ln = nodes + l;
+            const double* rn = nodes + r;
+            V3 lc = v3((ln[0] + ln[3]) * 0.5, (ln[1] + ln[4]) * 0.5, (ln[2] + ln[5]) * 0.5);
+            V3 rc = v3((rn[0] + rn[3]) * 0.5, (rn[1] + rn[4]) * 0.5, (rn[2] + rn[5]) * 0.5);
+            double ld = vlen2(vsub(lc, qc)), rd = vlen2(vsub(rc, qc));
+            if (ns + 2 > scap) { scap *= 2; stack = (int*)realloc(stack, sizeof(int) * (size_t)scap); }
+            if (ld <= rd) { stack[ns++] = r; stack[ns++] = l; }
+            else { stack[ns++] = l; stack[ns++] = r; }
+        }
+    }
+}
+
 /* collideShapeVsShape dispatch (src/collision/narrowphase.ts:172-238, src/shapes/sphere.ts:154-167) */
 void ocx_collide(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, double maxSep, double colTol,
                  double penTol, OHit* out) {
@@ -1184,7 +1343,7 @@ static OFace face_from(const double* v, int n) {
 
 void oc_collide(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t sa, const double* pa, const double* qa,
                 int32_t sb, const double* pb, const double* qb, double maxSep, const cc_settings* s, oc_hit* out) {
-    OShapes sh = {shapes, hulls};
+    OShapes sh = {shapes, hulls, NULL};
     OHit h;
     Q4 QA = {qa[0], qa[1], qa[2], qa[3]}, QB = {qb[0], qb[1], qb[2], qb[3]};
     ocx_collide(&sh, sa, vload(pa), QA, sb, vload(pb), QB, maxSep, s->collision_tolerance, s->penetration_tolerance, &h);
@@ -1207,7 +1366,7 @@ void oc_collide(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t 
 void oc_collide_convex(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t sa, const double* pa,
                        const double* qa, int32_t sb, const double* pb, const double* qb, double maxSep,
                        const cc_settings* s, oc_hit* out) {
-    OShapes sh = {shapes, hulls};
+    OShapes sh = {shapes, hulls, NULL};
     OHit h;
     Q4 QA = {qa[0], qa[1], qa[2], qa[3]}, QB = {qb[0], qb[1], qb[2], qb[3]};
     h.hit = 0;
@@ -1301,10 +1460,38 @@ void oc_barycentric3(const double* a, const double* b, const double* c, double t
 
 int32_t oc_supporting_face(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t s, const double* dir,
                            const double* pos, const double* quat, double* out) {
-    OShapes sh = {shapes, hulls};
+    OShapes sh = {shapes, hulls, NULL};
     OFace f;
     Q4 q = {quat[0], quat[1], quat[2], quat[3]};
     supporting_face(&f, &sh, s, vload(dir), vload(pos), q);
     for (int i = 0; i < f.n; i++) vstore(out + 3 * i, f.v[i]);
     return f.n;
+}
+
+typedef struct { oc_hit* out; uint32_t* sub; int n, cap; } MeshHits;
+static void mesh_hits_cb(void* ctx, const OHit* h, uint32_t sub) {
+    MeshHits* m = (MeshHits*)ctx;
+    if (m->n < m->cap) {
+        oc_hit* o = &m->out[m->n];
+        memset(o, 0, sizeof(*o));
+        o->hit = 1;
+        o->status = h->status;
+        vstore(o->point_a, h->pointA); vstore(o->point_b, h->pointB); vstore(o->axis, h->axis);
+        o->penetration = h->penetration;
+        o->face_a_n = h->faceA.n; o->face_b_n = h->faceB.n;
+        for (int i = 0; i < h->faceA.n; i++) vstore(o->face_a + 3 * i, h->faceA.v[i]);
+        for (int i = 0; i < h->faceB.n; i++) vstore(o->face_b + 3 * i, h->faceB.v[i]);
+        m->sub[m->n] = sub;
+    }
+    m->n++;
+}
+int32_t oc_collide_mesh(const cc_shape_desc* shapes, const cc_hull_data* hulls, const cc_mesh_data* meshes, int32_t sa,
+                        const double* pa, const double* qa, int32_t sm, const double* pm, const double* qm,
+                        double maxSep, const double* move_dir, const cc_settings* s, oc_hit* out, uint32_t* sub_b,
+                        int32_t cap) {
+    OShapes sh = {shapes, hulls, meshes};
+    Q4 QA = {qa[0], qa[1], qa[2], qa[3]}, QM = {qm[0], qm[1], qm[2], qm[3]};
+    MeshHits m = {out, sub_b, 0, cap};
+    ocx_collide_mesh(&sh, sa, vload(pa), QA, sm, vload(pm), QM, s, maxSep, vload(move_dir), mesh_hits_cb, &m);
+    return m.n;
 }

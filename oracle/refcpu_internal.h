@@ -125,6 +125,7 @@ static inline Q4 qaxis_angle(V3 axis, double rad) {
 typedef struct {
     const cc_shape_desc* shapes;
     const cc_hull_data* hulls;
+    const cc_mesh_data* meshes;
 } OShapes;
 
 #define OC_MAX_FACE 32
@@ -152,6 +153,12 @@ void ocx_collide(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, 
 void ocx_manifold_between_two_faces(OManifold* out, V3 p1, V3 p2, V3 axis, double maxDist, const OFace* f1,
                                     const OFace* f2);
 void ocx_prune(OManifold* m, V3 axis);
+/* one hit of a convex shape (A) against a triangle of a mesh (B), in traversal order; the
+ * triangle's sub-shape id rides along */
+typedef void (*ocx_hit_fn)(void* ctx, const OHit* h, uint32_t sub_b);
+void ocx_collide_mesh(const OShapes* sh, int sa, V3 pa, Q4 qa, int sm, V3 pm, Q4 qm, const cc_settings* s,
+                      double maxSep, V3 moveDir, ocx_hit_fn fn, void* ctx);
+int ocx_mesh_calls(void);
 extern int64_t ocx_epa_calls;
 
 #endif

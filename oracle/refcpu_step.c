@@ -36,6 +36,7 @@ typedef struct {
 typedef struct { V3 p1, p2; double ln, lt1, lt2; } OCached;
 typedef struct {
     int a, b;
+    uint32_t sa, sb; /* sub-shape ids */
     int numPoints;
     V3 normalB;
     OCached pts[4];
@@ -47,6 +48,7 @@ typedef struct { V3 r1x, r2x, iI1r1x, iI2r2x; double effMass, lambda, bias; } OA
 typedef struct { V3 posA, posB, localA, localB; OAxis n, t1, t2; } OCPoint;
 typedef struct {
     int a, b, contact;
+    uint32_t subA, subB;
     V3 normal, t1, t2;
     double friction, restitution, invMassA, invMassB;
     M3 invIA, invIB;
@@ -79,6 +81,9 @@ struct oc_world {
     cc_hull_data hulls;
     double* hullBuf;
     int32_t* hullIdx;
+    cc_mesh_data meshes;
+    double* meshBuf;
+    int32_t* meshIdx;
     OBody* b;
     int nb;
     int* active;
@@ -260,7 +265,8 @@ static void add_rot_step(OBody* b, V3 s, double sign) { /* add/subRotationStep s
 
 /* ---- world creation -------------------------------------------------------- */
 oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb, const cc_shape_desc* shapes,
-                    int32_t ns, const cc_hull_data* hulls, const cc_point_constraint_desc* pcs, int32_t npc) {
+                    int32_t ns, const cc_hull_data* hulls, const cc_point_constraint_desc* pcs, int32_t npc,
+                    const cc_mesh_data* meshes) {
     oc_world* w = (oc_world*)calloc(1, sizeof(oc_world));
     w->s = *s;
     w->shapes = (cc_shape_desc*)xrealloc(NULL, sizeof(cc_shape_desc) * (size_t)ns);
@@ -281,8 +287,21 @@ oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb
         memcpy(q, hulls->face_vertex_indices, sizeof(int32_t) * FV); w->hulls.face_vertex_indices = q;
         w->hulls.total_points = P; w->hulls.total_faces = F; w->hulls.total_face_vertices = FV;
     }
+    if (meshes && meshes->total_triangles > 0) {
+        int V = meshes->total_vertices, T = meshes->total_triangles, N = meshes->total_nodes;
+        w->meshBuf = (double*)xrealloc(NULL, sizeof(double) * (size_t)(3 * V + 8 * N));
+        memcpy(w->meshBuf, meshes->positions, sizeof(double) * 3 * (size_t)V);
+        memcpy(w->meshBuf + 3 * (size_t)V, meshes->nodes, sizeof(double) * 8 * (size_t)N);
+        w->meshIdx = (int32_t*)xrealloc(NULL, sizeof(int32_t) * 4 * (size_t)T);
+        memcpy(w->meshIdx, meshes->triangles, sizeof(int32_t) * 4 * (size_t)T);
+        w->meshes.positions = w->meshBuf;
+        w->meshes.nodes = w->meshBuf + 3 * (size_t)V;
+        w->meshes.triangles = w->meshIdx;
+        w->meshes.total_vertices = V; w->meshes.total_triangles = T; w->meshes.total_nodes = N;
+    }
     w->sh.shapes = w->shapes;
     w->sh.hulls = &w->hulls;
+    w->sh.meshes = &w->meshes;
     w->nb = nb;
     w->b = (OBody*)calloc((size_t)(nb ? nb : 1), sizeof(OBody));
     w->active = (int*)calloc((size_t)(nb ? nb : 1), sizeof(int));
@@ -325,7 +344,7 @@ oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb
 
 void oc_destroy(oc_world* w) {
     if (!w) return;
-    free(w->shapes); free(w->hullBuf); free(w->hullIdx); free(w->b); free(w->active);
+    free(w->shapes); free(w->hullBuf); free(w->hullIdx); free(w->meshBuf); free(w->meshIdx); free(w->b); free(w->active);
     free(w->pairs); free(w->ct); free(w->htab); free(w->cc); free(w->pc);
     free(w->bodyLinks); free(w->bodyIslands); free(w->contactLinks); free(w->consLinks); free(w->consIds);
     for (int i = 0; i < w->cisl; i++) { free(w->isl[i].bodies); free(w->isl[i].contacts); free(w->isl[i].cons); }
@@ -333,11 +352,14 @@ void oc_destroy(oc_world* w) {
     free(w);
 }
 
-/* ---- contact cache (src/contacts.ts) as a hash map keyed by (a, b) ---------- */
+/* ---- contact cache (src/contacts.ts) as a hash map keyed by (a, b, subA, subB) ---------- */
 static uint64_t hkey(int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; }
 static uint32_t hhash(uint64_t k) {
     k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
     return (uint32_t)k;
+}
+static uint32_t ckey(int a, int b, uint32_t sa, uint32_t sb) {
+    return hhash(hkey(a, b)) ^ hhash(((uint64_t)sa << 32) | sb);
 }
 static void cache_rebuild(oc_world* w) {
     int need = 16;
@@ -345,28 +367,28 @@ static void cache_rebuild(oc_world* w) {
     if (need != w->hcap) { w->htab = (int*)xrealloc(w->htab, sizeof(int) * (size_t)need); w->hcap = need; }
     for (int i = 0; i < w->hcap; i++) w->htab[i] = -1;
     for (int i = 0; i < w->nct; i++) {
-        uint32_t h = hhash(hkey(w->ct[i].a, w->ct[i].b)) & (uint32_t)(w->hcap - 1);
+        uint32_t h = ckey(w->ct[i].a, w->ct[i].b, w->ct[i].sa, w->ct[i].sb) & (uint32_t)(w->hcap - 1);
         while (w->htab[h] >= 0) h = (h + 1) & (uint32_t)(w->hcap - 1);
         w->htab[h] = i;
     }
 }
-static int cache_find(oc_world* w, int a, int b) { /* findContact :481-513 */
+static int cache_find(oc_world* w, int a, int b, uint32_t sa, uint32_t sb) { /* findContact :481-513 */
     if (!w->hcap) return -1;
-    uint32_t h = hhash(hkey(a, b)) & (uint32_t)(w->hcap - 1);
+    uint32_t h = ckey(a, b, sa, sb) & (uint32_t)(w->hcap - 1);
     while (w->htab[h] >= 0) {
         OContact* c = &w->ct[w->htab[h]];
-        if (c->a == a && c->b == b) return w->htab[h];
+        if (c->a == a && c->b == b && c->sa == sa && c->sb == sb) return w->htab[h];
         h = (h + 1) & (uint32_t)(w->hcap - 1);
     }
     return -1;
 }
-static int cache_create(oc_world* w, int a, int b) { /* createContact :235-285 + setContact :141-170 */
+static int cache_create(oc_world* w, int a, int b, uint32_t sa, uint32_t sb) { /* createContact :235-285 + setContact :141-170 */
     if (w->nct == w->cct) { w->cct = w->cct ? 2 * w->cct : 256; w->ct = (OContact*)xrealloc(w->ct, sizeof(OContact) * (size_t)w->cct); }
     OContact* c = &w->ct[w->nct];
     memset(c, 0, sizeof(*c));
-    c->a = a; c->b = b;
+    c->a = a; c->b = b; c->sa = sa; c->sb = sb;
     if (2 * (w->nct + 1) + 16 > w->hcap) { w->nct++; cache_rebuild(w); return w->nct - 1; }
-    uint32_t h = hhash(hkey(a, b)) & (uint32_t)(w->hcap - 1);
+    uint32_t h = ckey(a, b, sa, sb) & (uint32_t)(w->hcap - 1);
     while (w->htab[h] >= 0) h = (h + 1) & (uint32_t)(w->hcap - 1);
     w->htab[h] = w->nct;
     return w->nct++;
@@ -496,17 +518,18 @@ static double normal_bias(const oc_world* w, const OBody* A, const OBody* B, V3 
 }
 
 /* addContactConstraint src/constraints/contact-constraints.ts:437-857 */
-static int add_contact_constraint(oc_world* w, int ia, int ib, OManifold* m, double dt) {
+static int add_contact_constraint(oc_world* w, int ia, int ib, OManifold* m, uint32_t subA, uint32_t subB, double dt) {
     if (ia > ib) { /* swap so that id A < id B, manifold.swapShapes :86-107 */
         int t = ia; ia = ib; ib = t;
+        uint32_t u = subA; subA = subB; subB = u;
         m->normal = vneg(m->normal);
         for (int i = 0; i < m->n; i++) { V3 x = m->relA[i]; m->relA[i] = m->relB[i]; m->relB[i] = x; }
     }
     OBody* A = &w->b[ia];
     OBody* B = &w->b[ib];
-    int ci = cache_find(w, ia, ib);
+    int ci = cache_find(w, ia, ib, subA, subB);
     int existing = ci >= 0;
-    if (!existing) ci = cache_create(w, ia, ib);
+    if (!existing) ci = cache_create(w, ia, ib, subA, subB);
     OContact* c = &w->ct[ci];
     c->processed = 1;
     M3 rotA = m3_from_quat(A->q), rotB = m3_from_quat(B->q);
@@ -520,7 +543,8 @@ static int add_contact_constraint(oc_world* w, int ia, int ib, OManifold* m, dou
         OCC* k = &w->cc[w->ncc++];
         memset(k, 0, sizeof(*k));
         k->a = ia; k->b = ib; k->contact = ci;
-        k->sortKey = sort_key((uint32_t)ia, (uint32_t)ib, EMPTY_SUB, EMPTY_SUB);
+        k->subA = subA; k->subB = subB;
+        k->sortKey = sort_key((uint32_t)ia, (uint32_t)ib, subA, subB);
         V3 n = m->normal;
         k->normal = n;
         if (fabs(n.x) > fabs(n.y)) { double tl = sqrt(n.x * n.x + n.z * n.z); k->t1 = v3(n.z / tl, 0, -n.x / tl); }
@@ -649,30 +673,107 @@ static void find_pairs(oc_world* w) {
 }
 
 /* ---- narrowphase of one pair with the reduction collector (src/update.ts:333-563, 714-850) ---- */
+#define OC_MAX_MANIFOLDS 32
+typedef struct {
+    const oc_world* w;
+    V3 base;                       /* bodyA.position */
+    int reversed;                  /* the mesh is body A: hits come back inverted (collide-shape-vs-shape.ts:228-271) */
+    int n;
+    OManifold m[OC_MAX_MANIFOLDS];
+    uint32_t subA[OC_MAX_MANIFOLDS], subB[OC_MAX_MANIFOLDS];
+} OCollector;
+static OCollector g_col;
+static OManifold g_tmp;
+
+/* addHit :385-520: merge by normal, else a new manifold (the shallowest one is replaced when
+ * all 32 are in use), then append the hit's clipped points */
+static void collector_add(OCollector* C, const OHit* h, uint32_t subA, uint32_t subB) {
+    V3 normal = vnormalize(h->axis);
+    int f = -1;
+    for (int i = 0; i < C->n; i++)
+        if (vdot(normal, C->m[i].normal) >= C->w->s.normal_cos_max_delta_rotation) { f = i; break; }
+    if (f < 0) {
+        if (C->n >= OC_MAX_MANIFOLDS) {
+            int sh = 0;
+            for (int i = 1; i < C->n; i++)
+                if (C->m[i].depth < C->m[sh].depth) sh = i;
+            if (h->penetration < C->m[sh].depth) return;
+            f = sh;
+        } else {
+            f = C->n++;
+        }
+        OManifold* m = &C->m[f];
+        m->n = 0;
+        m->base = C->base;
+        m->normal = normal;
+        m->depth = h->penetration;
+        C->subA[f] = subA;
+        C->subB[f] = subB;
+    } else {
+        C->m[f].normal = vadd(C->m[f].normal, normal);
+    }
+    OManifold* m = &C->m[f];
+    g_tmp.base = C->base;
+    g_tmp.n = 0;
+    double maxDist = C->w->s.speculative_contact_distance + C->w->s.manifold_tolerance;
+    if (h->faceA.n >= 2 && h->faceB.n >= 3)
+        ocx_manifold_between_two_faces(&g_tmp, h->pointA, h->pointB, h->axis, maxDist, &h->faceA, &h->faceB);
+    else { g_tmp.relA[0] = vsub(h->pointA, g_tmp.base); g_tmp.relB[0] = vsub(h->pointB, g_tmp.base); g_tmp.n = 1; }
+    for (int i = 0; i < g_tmp.n && m->n < 64; i++) { m->relA[m->n] = g_tmp.relA[i]; m->relB[m->n] = g_tmp.relB[i]; m->n++; }
+    if (h->penetration > m->depth) m->depth = h->penetration;
+    if (m->n > 32) ocx_prune(m, m->normal);
+}
+/* a convex-vs-triangle hit, through the inverted collector when the mesh is body A */
+static void mesh_hit(void* ctx, const OHit* h, uint32_t subTri) {
+    OCollector* C = (OCollector*)ctx;
+    if (!C->reversed) { collector_add(C, h, EMPTY_SUB, subTri); return; }
+    OHit r;
+    r.hit = h->hit; r.status = h->status;
+    r.pointA = h->pointB; r.pointB = h->pointA;
+    r.axis = vneg(h->axis);
+    r.penetration = h->penetration;
+    r.faceA = h->faceB; r.faceB = h->faceA;
+    collector_add(C, &r, subTri, EMPTY_SUB);
+}
 static int narrowphase_pair(oc_world* w, int ia, int ib, double dt) {
     OBody* A = &w->b[ia];
     OBody* B = &w->b[ib];
-    OHit h;
-    ocx_collide(&w->sh, A->shape, A->pos, A->q, B->shape, B->pos, B->q, w->s.speculative_contact_distance,
-                w->s.collision_tolerance, w->s.penetration_tolerance, &h);
-    if (!h.hit) return 0;
-    static OManifold m, tmp;
-    m.base = A->pos;
-    m.normal = vnormalize(h.axis);
-    m.depth = h.penetration;
-    m.n = 0;
-    tmp.base = A->pos;
-    tmp.n = 0;
-    double maxDist = w->s.speculative_contact_distance + w->s.manifold_tolerance;
-    if (h.faceA.n >= 2 && h.faceB.n >= 3) ocx_manifold_between_two_faces(&tmp, h.pointA, h.pointB, h.axis, maxDist, &h.faceA, &h.faceB);
-    else { tmp.relA[0] = vsub(h.pointA, tmp.base); tmp.relB[0] = vsub(h.pointB, tmp.base); tmp.n = 1; }
-    for (int i = 0; i < tmp.n && m.n < 64; i++) { m.relA[m.n] = tmp.relA[i]; m.relB[m.n] = tmp.relB[i]; m.n++; }
-    if (h.penetration > m.depth) m.depth = h.penetration;
-    if (m.n > 32) ocx_prune(&m, m.normal);
-    m.normal = vnormalize(m.normal);
-    if (m.n > 4) ocx_prune(&m, m.normal);
-    if (m.n > 0) return add_contact_constraint(w, ia, ib, &m, dt);
-    return 0;
+    OCollector* C = &g_col;
+    C->w = w;
+    C->base = A->pos;
+    C->n = 0;
+    int ta = w->shapes[A->shape].type, tb = w->shapes[B->shape].type;
+    if (ta == CC_SHAPE_TRIANGLE_MESH || tb == CC_SHAPE_TRIANGLE_MESH) {
+        if (ta == CC_SHAPE_TRIANGLE_MESH && tb == CC_SHAPE_TRIANGLE_MESH) return 0; /* no mesh-vs-mesh function */
+        if (ta == CC_SHAPE_SPHERE || tb == CC_SHAPE_SPHERE) {
+            fprintf(stderr, "refcpu: sphere vs triangle mesh is not restated\n");
+            abort();
+        }
+        V3 move = vsub(A->v, B->v); /* activeEdgeMovementDirection :730-735 */
+        double maxSep = w->s.speculative_contact_distance;
+        if (tb == CC_SHAPE_TRIANGLE_MESH) {
+            C->reversed = 0;
+            ocx_collide_mesh(&w->sh, A->shape, A->pos, A->q, B->shape, B->pos, B->q, &w->s, maxSep, move, mesh_hit, C);
+        } else {
+            C->reversed = 1;
+            ocx_collide_mesh(&w->sh, B->shape, B->pos, B->q, A->shape, A->pos, A->q, &w->s, maxSep, move, mesh_hit, C);
+        }
+    } else {
+        OHit h;
+        ocx_collide(&w->sh, A->shape, A->pos, A->q, B->shape, B->pos, B->q, w->s.speculative_contact_distance,
+                    w->s.collision_tolerance, w->s.penetration_tolerance, &h);
+        if (!h.hit) return 0;
+        collector_add(C, &h, EMPTY_SUB, EMPTY_SUB);
+    }
+    /* finalizeAndCreateConstraints :522-555 */
+    int made = 0;
+    for (int i = 0; i < C->n; i++) {
+        OManifold* m = &C->m[i];
+        m->normal = vnormalize(m->normal);
+        if (m->n > 4) ocx_prune(m, m->normal);
+        if (m->n > 0) made |= add_contact_constraint(w, ia, ib, m, C->subA[i], C->subB[i], dt);
+    }
+    return made;
 }
 
 /* ---- islands (src/islands.ts) ----------------------------------------------- */
@@ -1185,7 +1286,10 @@ static int cmp_cout(const void* x, const void* y) {
     const cc_contact_out* a = (const cc_contact_out*)x;
     const cc_contact_out* b = (const cc_contact_out*)y;
     if (a->body_a != b->body_a) return a->body_a - b->body_a;
-    return a->body_b - b->body_b;
+    if (a->body_b != b->body_b) return a->body_b - b->body_b;
+    if (a->sub_shape_a != b->sub_shape_a) return a->sub_shape_a < b->sub_shape_a ? -1 : 1;
+    if (a->sub_shape_b != b->sub_shape_b) return a->sub_shape_b < b->sub_shape_b ? -1 : 1;
+    return 0;
 }
 int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap) {
     int n = w->ncc < cap ? w->ncc : cap;
@@ -1194,6 +1298,7 @@ int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap) {
         cc_contact_out* o = &out[i];
         memset(o, 0, sizeof(*o));
         o->body_a = k->a; o->body_b = k->b; o->num_points = k->np;
+        o->sub_shape_a = k->subA; o->sub_shape_b = k->subB;
         vstore(o->normal, k->normal);
         for (int j = 0; j < k->np; j++) {
             vstore(o->position_a[j], k->pts[j].posA);
