@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--calibrate", type=int, default=0, help="only copy this many bytes device to device")
     ap.add_argument("--batched-worlds", type=int, default=4096,
                     help="worlds in the batched uneven-worlds measurement (0 skips it)")
+    ap.add_argument("--terrain-bodies", type=int, default=50000,
+                    help="bodies dropped on the 512 x 512 terrain mesh in its measurement (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); worlds would shard across GPUs by "
@@ -211,6 +213,37 @@ def main():
             c.close()
         del bw
 
+    # ---- convex bodies dropped on a triangle-mesh terrain (522,242 triangles)
+    terrain = None
+    if args.terrain_bodies:
+        t0 = time.time()
+        tw = S.terrain_drop(seed=1004, n=args.terrain_bodies)
+        log(f"terrain: {tw.num_bodies} bodies, built in {time.time() - t0:.1f}s")
+        terrain = {"workload": "terrain drop: static mesh on a 512 x 512 vertex grid at 0.5 m (522,242 triangles, "
+                               "heights from 4 seeded sinusoids, amplitudes 0.5-3 m, wavelengths 8-64 m), boxes "
+                               "(he 0.5), capsules (r 0.25, hh 0.25) and 16-point hulls (r 0.5) dropped from 5-30 m, "
+                               "seed 1004, dt 1/60; 60 steps after a 60-step warmup (bodies reach the terrain)",
+                   "bodies": args.terrain_bodies}
+        for prec in ("fp32", "fp64"):
+            c = Context(prec, max_pairs_per_body=16, max_contacts_per_body=8, block_solver=args.block)
+            c.upload(tw)
+            for w0 in range(0, 60, 20):
+                c.step(DT, 20)
+                c.sync()
+            t = time.perf_counter()
+            c.step(DT, 60)
+            c.sync()
+            e = time.perf_counter() - t
+            ph = c.phase_times()
+            cn = c.counters()
+            terrain[f"{prec}_body_steps_per_s"] = args.terrain_bodies * 60 / e
+            terrain[f"{prec}_phase_ms_per_step"] = {k: v / 60 for k, v in ph.items()}
+            terrain[f"{prec}_contacts_last_step"] = cn["num_contacts"]
+            log(f"terrain {prec}: 60 steps in {e:.3f}s -> {args.terrain_bodies * 60 / e:.4g} body-steps/s; "
+                f"phases {ph}; {cn}")
+            c.close()
+        del tw
+
     # ---- accuracy vs the FP64 oracle, and the oracle's own speed on the host
     acc = {}
     cpu = None
@@ -290,6 +323,7 @@ def main():
         "batched_fp32_over_plain_fp64": (value / fp64) if fp64 else None,
         "accuracy_vs_fp64_oracle": acc,
         "batched_uneven_worlds": batched,
+        "terrain_mesh": terrain,
         "phases": per_phase,
         "roofline": {
             "bound": "hbm",

@@ -79,7 +79,12 @@ struct Dev {
     int* spillB;
     int* spillK;
     int spillCap;
-    int* pairCon;       // [pairCap] constraint slot made from each pair, -1 if none
+    int* pairCon;       // [pairCap] first constraint slot made from each pair (a run of consecutive
+                        // slots, its length in the first head's bits), -1 if none
+    int hasMesh;        // a shape is a triangle mesh: pairs may make several constraints
+    int* meshList;      // [pairCap] pair slots of convex-vs-mesh pairs
+    int meshWorkers;    // threads of the mesh kernel (each owns a collector and an EPA polytope)
+    char* meshWork;     // [meshWorkers * sizeof(MeshColl<T>)] collectors
     int* conFlag;       // [pairCap + 1]
     int* conPos;        // [pairCap + 1] exclusive scan of conFlag
     int* pairA;         // [pairCap]
@@ -153,8 +158,9 @@ enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
        ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_N = 16 };
 
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
-       CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_N = 10 };
-// CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list, 16 hits
+       CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_MESH = 10, CNT_N = 11 };
+// CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list, 16 hits,
+// 32 mesh collector (manifolds or BVH stack)
 constexpr int BP_SLOTS = 32;
 
 // orders LDS accesses between the lanes of one wavefront
@@ -603,24 +609,36 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
     m.normal = normalize(m.normal);
     if (m.n > 4) prune(m, m.normal);
     if (m.n <= 0) return;
+    make_contact(d, slot, A, Bi, RA, RB, m, 0xffffffffu, 0xffffffffu, -1, 0, 1, dt, par);
+}
+
+// addContactConstraint (contact-constraints.ts:437-857) for one finished manifold of the pair in
+// `slot`: cs is its constraint slot, or -1 to allocate one; the pair's constraints form a run of
+// runCnt consecutive slots and this is number runIdx of it.
+template <typename T, typename M>
+__device__ inline void make_contact(const Dev<T>& d, int slot, int A, int Bi, const NpBody<T>& RA, const NpBody<T>& RB,
+                                    const M& m, unsigned subA, unsigned subB, int cs, int runIdx, int runCnt, T dt,
+                                    int par) {
+    const Bodies<T>& B = d.B;
     // swap so that id A < id B (manifold.swapShapes :86-107)
     int ia = A, ib = Bi;
     V3<T> n = m.normal;
     bool sw = ia > ib;
-    if (sw) { int t = ia; ia = ib; ib = t; n = -n; }
+    if (sw) { int t = ia; ia = ib; ib = t; n = -n; unsigned u = subA; subA = subB; subB = u; }
     const NpBody<T>& Ria = sw ? RB : RA;
     const NpBody<T>& Rib = sw ? RA : RB;
     int fa = Ria.flags, fb = Rib.flags;
     bool dynA = fl_motion(fa) == MOTION_DYNAMIC, dynB = fl_motion(fb) == MOTION_DYNAMIC;
     if (!((dynA && Ria.invMass != T(0)) || (dynB && Rib.invMass != T(0)))) return;
     const NpDyn<T> DA = d.npd[ia], DB = d.npd[ib];
-    int cs = wave_alloc(&d.counters[CNT_CONS]);
+    if (cs < 0) cs = wave_alloc(&d.counters[CNT_CONS]);
     if (cs >= d.conCap) { atomicOr(&d.counters[CNT_OVERFLOW], 2); return; }
-    d.pairCon[slot] = cs;
+    if (runIdx == 0) d.pairCon[slot] = cs;
     Con<T>& k = d.con[cs];
     ConTap<T>& tp = d.tap[cs];
     CacheRec<T>& rc = d.rec[par][cs];
-    // previous record of this contact (findContact src/contacts.ts:481-513)
+    // previous record of this contact (findContact src/contacts.ts:481-513): the table maps the
+    // body pair to the first record of its run, the run is searched for the sub-shape ids
     unsigned long long key = pair_key(ia, ib);
     int prev = -1;
     {
@@ -629,7 +647,15 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
         for (int probe = 0; probe < d.hashCap; probe++) {
             unsigned long long kk = hk[h];
             if (kk == CC_EMPTY_KEY) break;
-            if (kk == key) { prev = d.hvals[par ^ 1][h]; break; }
+            if (kk == key) {
+                const int p0 = d.hvals[par ^ 1][h];
+                const int run = d.rec[par ^ 1][p0].run;
+                for (int j = 0; j < run; j++) {
+                    const CacheRec<T>& r = d.rec[par ^ 1][p0 + j];
+                    if (r.subA == subA && r.subB == subB) { prev = p0 + j; break; }
+                }
+                break;
+            }
             h = (h + 1) & (unsigned)(d.hashCap - 1);
         }
     }
@@ -639,17 +665,19 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
         for (int j = 0; j < 4; j++) { rc.p1[j] = rc.p2[j] = mk3<T>(0, 0, 0); rc.ln[j] = rc.lt1[j] = rc.lt2[j] = T(0); }
     }
     rc.a = ia; rc.b = ib;
+    rc.subA = subA; rc.subB = subB;
+    rc.run = runIdx == 0 ? runCnt : 0;
     rc.np = m.n;
     Q4<T> qa = Ria.q, qb = Rib.q;
     M3<T> rotA = m3_from_quat(qa), rotB = m3_from_quat(qb);
     T fr = combine(DA.friction, DB.friction, fl_fmode(fa), fl_fmode(fb));
     T re = combine(DA.restitution, DB.restitution, fl_rmode(fa), fl_rmode(fb));
     k.a = ia; k.b = ib;
-    k.cq = d.pairA[slot];  // (query, other): the creation order of constraints
-    k.co = d.pairB[slot];
+    k.cq = d.pairA[slot];  // (query, other, place in the pair's run): the creation order of constraints
+    k.co = d.pairB[slot] | (runIdx << 24);
     k.dynMask = (dynA ? 1 : 0) | (dynB ? 2 : 0);
     k.movMask = (fl_motion(fa) != MOTION_STATIC ? 1 : 0) | (fl_motion(fb) != MOTION_STATIC ? 2 : 0);
-    k.sortKey = sort_key((unsigned)ia, (unsigned)ib, 0xffffffffu, 0xffffffffu);
+    k.sortKey = sort_key((unsigned)ia, (unsigned)ib, subA, subB);
     k.normal = n;
     k.t1 = contact_t1(n);
     k.t2 = cross(n, k.t1);
@@ -711,7 +739,7 @@ __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, con
     hd.a = ia;
     hd.b = ib;
     hd.sortKey = k.sortKey;
-    hd.bits = k.dynMask | (k.movMask << 2) | (k.np << 4);
+    hd.bits = k.dynMask | (k.movMask << 2) | (k.np << 4) | (runIdx > 0 ? 128 : ((runCnt - 1) << 8));
     d.conHead[cs] = hd;
     if (dynA && DA.sleeping) B.wake[ia] = 1;
     if (dynB && DB.sleeping) B.wake[ib] = 1;
@@ -808,7 +836,7 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
     const int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
     for (int base = blockIdx.x * 256; base < npairs; base += gridDim.x * 256) {  // the same trip count in every thread
         const int slot = base + threadIdx.x;
-        bool gjk = false;
+        bool gjk = false, mesh = false;
         int combo = 0;
         if (slot < npairs) {
             d.pairCon[slot] = -1;
@@ -825,6 +853,8 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
                     V3<T> n = dist > T(0) ? dd * (T(1) / dist) : mk3<T>(0, 1, 0);
                     push_hit(d, slot, HIT_SPHERES, pa + n * ra, pbp - n * rb, n, true);
                 }
+            } else if (ta == SHAPE_MESH || tb == SHAPE_MESH) {
+                mesh = ta != tb;  // no mesh-vs-mesh function
             } else {
                 T reach = RA.brad + RB.brad + d.s.spec;
                 gjk = !(len2(RB.pos - RA.pos) > reach * reach);
@@ -835,6 +865,11 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
         if (gjk) {
             d.gjkList[idx] = slot;
             d.gjkCombo[idx] = combo;
+        }
+        if (d.hasMesh) {
+            __syncthreads();
+            const int mi = block_append<256>(&d.counters[CNT_MESH], mesh, sh);
+            if (mesh) d.meshList[mi] = slot;
         }
     }
 }
@@ -1001,7 +1036,7 @@ template <typename T>
 __global__ void k_con_flags(Dev<T> d) {
     int np = min(d.counters[CNT_PAIRS], d.pairCap);
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < d.pairCap; p += gridDim.x * blockDim.x)
-        d.conFlag[p] = (p < np && d.pairCon[p] >= 0) ? 1 : 0;
+        d.conFlag[p] = (p < np && d.pairCon[p] >= 0) ? (d.hasMesh ? 1 + ((d.conHead[d.pairCon[p]].bits >> 8) & 7) : 1) : 0;
 }
 
 template <typename T>
@@ -1011,8 +1046,12 @@ __global__ void k_con_compact(Dev<T> d) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
         int c = d.pairCon[p];
         if (c >= 0) {
-            d.conIdx[d.conPos[p]] = c;
-            pts += (unsigned long long)(d.conHead[c].bits >> 4);
+            const int bits = d.conHead[c].bits;
+            const int run = d.hasMesh ? 1 + ((bits >> 8) & 7) : 1;
+            for (int t = 0; t < run; t++) {
+                d.conIdx[d.conPos[p] + t] = c + t;
+                pts += (unsigned long long)((t ? d.conHead[c + t].bits : bits) >> 4 & 7);
+            }
         }
     }
     for (int w = blockIdx.x * blockDim.x + threadIdx.x; w <= d.W; w += gridDim.x * blockDim.x)
@@ -1095,6 +1134,7 @@ __global__ void k_hash_insert(Dev<T> d) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         int slot = k;
         const ConHead c = d.conHead[slot];
+        if (c.bits & 128) continue;  // not the first constraint of its pair's run
         unsigned long long key = pair_key(c.a, c.b);
         unsigned h = hash64(key) & (unsigned)(d.hashCap - 1);
         for (int probe = 0; probe < d.hashCap; probe++) {

@@ -130,6 +130,20 @@ __device__ inline V3<T> sup_base(const Sup<T>& s, V3<T> d) {
     }
 }
 
+// a triangle's support (triangleGetSupport support.ts:83-118): the vertex of largest projection
+template <typename T>
+struct TriSup {
+    V3<T> a, b, c;
+};
+template <typename T>
+__device__ inline V3<T> sup_get(const TriSup<T>& s, V3<T> d) {
+    const T d1 = s.a.x * d.x + s.a.y * d.y + s.a.z * d.z;
+    const T d2 = s.b.x * d.x + s.b.y * d.y + s.b.z * d.z;
+    const T d3 = s.c.x * d.x + s.c.y * d.y + s.c.z * d.z;
+    if (d1 > d2) return d1 > d3 ? s.a : s.c;
+    return d2 > d3 ? s.b : s.c;
+}
+
 // TransformedSupport (support.ts:236-261) + AddConvexRadiusSupport (support.ts:288-303)
 template <typename T>
 __device__ inline V3<T> sup_get(const Sup<T>& s, V3<T> d) {
@@ -351,8 +365,8 @@ __device__ inline CPRes<T> cp_tetra(V3<T> a, V3<T> b, V3<T> c, V3<T> d, bool mus
 template <typename T> struct GjkOut { V3<T> pointA, pointB, axis; T dist2; Simplex<T> simplex; };
 
 // gjkClosestPoints gjk.ts:1294-1546
-template <typename T>
-__device__ inline void gjk_closest_points(GjkOut<T>& out, const Sup<T>& A, const Sup<T>& B, T tolerance, V3<T> direction,
+template <typename T, typename SB>
+__device__ inline void gjk_closest_points(GjkOut<T>& out, const Sup<T>& A, const SB& B, T tolerance, V3<T> direction,
                                           T maxDist2) {
     const T GJK_TOL = T(1e-5);
     T tol2 = tolerance * tolerance;
@@ -443,8 +457,8 @@ enum { PD_NOT_COLLIDING = 0, PD_COLLIDING = 1, PD_INDETERMINATE = 2 };
 template <typename T> struct PenDepth { int status; V3<T> axis, pointA, pointB; };
 
 // penetrationDepthStepGJK penetration.ts:35-110
-template <typename T>
-__device__ inline void pd_step_gjk(PenDepth<T>& out, Simplex<T>& outSimplex, const Sup<T>& A, const Sup<T>& B, T crA, T crB,
+template <typename T, typename SB>
+__device__ inline void pd_step_gjk(PenDepth<T>& out, Simplex<T>& outSimplex, const Sup<T>& A, const SB& B, T crA, T crB,
                                    V3<T> dir, T tol) {
     T cr = crA + crB, cr2 = cr * cr;
     GjkOut<T> g;
@@ -735,8 +749,8 @@ __device__ inline bool add_point(EpaH<T>& h, int facing, int idx, double closest
     return true;
 }
 
-template <typename T>
-__device__ inline int epa_add(EpaH<T>& P, const Sup<T>& A, const Sup<T>& B, V3<T> d) {
+template <typename T, typename SB>
+__device__ inline int epa_add(EpaH<T>& P, const Sup<T>& A, const SB& B, V3<T> d) {
     V3<T> p = sup_get(A, d);
     V3<T> q = sup_get(B, -d);
     int i = P.n++;
@@ -747,8 +761,8 @@ __device__ inline int epa_add(EpaH<T>& P, const Sup<T>& A, const Sup<T>& B, V3<T
 }
 
 // penetrationDepthStepEPA penetration.ts:214-614; the polytope lives in device memory.
-template <typename T>
-__device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const Sup<T>& B, T tol, const Simplex<T>& s,
+template <typename T, typename SB>
+__device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const SB& B, T tol, const Simplex<T>& s,
                                    EpaH<T>& H) {
     H.n = s.size;
     for (int i = 0; i < s.size; i++) { H.y(i) = s.pt[i].y; H.p(i) = s.pt[i].p; H.q(i) = s.pt[i].q; }

@@ -421,17 +421,22 @@ __device__ inline void warm_start_levels(const Dev<T>& d, int so, int D, const i
 }
 
 // creation-order key of sequence element k: contacts in the order they were created
-// (ascending (querying body, other body), the order of the pair list), then point constraints
-// by index; the warm start runs in this order
+// (ascending (querying body, other body), then the manifolds of one pair in collector order),
+// then point constraints by index; the warm start runs in this order
 // (src/update.ts:168-175 warm-starts the constraint arrays front to back)
 template <typename T>
 struct CreationLess {
     const Dev<T>* d;
     int so;
     __device__ void key(int k, unsigned long long& hi, unsigned long long& lo) const {
+        // bodies are below 2^23, a run holds at most 8 constraints (co carries the place in bits 24+)
         int s = d->seq[so + k];
-        hi = s >= 0 ? ((unsigned long long)(unsigned)d->con[s].cq << 32) | (unsigned)d->con[s].co
-                    : (1ull << 63) | (unsigned)(-1 - s);
+        if (s >= 0) {
+            const unsigned co = (unsigned)d->con[s].co;
+            hi = ((((unsigned long long)(unsigned)d->con[s].cq << 26) | (co & 0xffffffu)) << 3) | (co >> 24);
+        } else {
+            hi = (1ull << 63) | (unsigned)(-1 - s);
+        }
         lo = 0;
     }
 };
@@ -646,7 +651,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         if (k < m) {
             s = d.conIdx[c0 + k];
             const ConHead c = d.conHead[s];
-            npk = c.bits >> 4;
+            npk = (c.bits >> 4) & 7;
             if (c.bits & 1) ea = c.a;
             if (c.bits & 2) eb = c.b;
             hi = (unsigned long long)c.sortKey;

@@ -76,8 +76,9 @@ struct Con {
     int a, b, np, dynMask;  // dynMask bit0: A dynamic, bit1: B dynamic
     unsigned sortKey;
     int movMask;            // bit0: A not static, bit1: B not static
-    int cq, co;             // the broadphase pair (querying body, other body); constraints are
-                            // created in ascending (cq, co) order
+    int cq, co;             // the broadphase pair (querying body, other body; bits 24+ of co: the
+                            // constraint's place in the pair's run); constraints are created in
+                            // ascending (cq, co, place) order
     V3<T> normal, t1, t2;
     T friction, invMassA, invMassB;
     ConPoint<T> p[4];
@@ -92,7 +93,8 @@ struct ConHead {
 // Persistent contact cache entry (src/contacts.ts:9-58): four in-place slots.
 template <typename T>
 struct CacheRec {
-    int a, b, np, pad;
+    int a, b, np, run;     // run: constraints the pair made this step (in its first record; 0 in the others)
+    unsigned subA, subB;   // sub-shape ids (0xffffffff when empty)
     V3<T> p1[4], p2[4];
     T ln[4], lt1[4], lt2[4];
 };

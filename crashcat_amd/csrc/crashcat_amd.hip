@@ -11,6 +11,7 @@
 
 #include "../../include/crashcat_amd.h"
 #include "cc_mesh_build.h"
+#include "cc_mesh.h"
 #include "cc_solver.h"
 
 using namespace cc;
@@ -87,6 +88,7 @@ __global__ void k_set_pair_count(Dev<T> d) {
         d.counters[CNT_HITS] = 0;
         d.counters[CNT_HITS_L] = 0;
         d.counters[CNT_GJK] = 0;
+        d.counters[CNT_MESH] = 0;
         for (int c = 0; c < 32; c++) d.gjkHist[c] = 0;
         if (n > d.pairCap) atomicOr(&d.counters[CNT_OVERFLOW], 1);
     }
@@ -552,6 +554,12 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.gLargeFlag = A.get<int>(nb + 1);
     d.gLargeOff = A.get<int>(nb + 1);
     d.gLarge = A.get<int>(nb);
+    d.hasMesh = X->hasMesh ? 1 : 0;
+    if (X->hasMesh) {  // one collector per mesh thread; the EPA polytopes are the EPA kernel's
+        d.meshList = A.get<int>(d.pairCap);
+        d.meshWorkers = std::min(((d.epaWorkers + EPA_LANES - 1) / EPA_LANES) * EPA_LANES, 16384);
+        d.meshWork = A.get<char>((size_t)d.meshWorkers * sizeof(MeshColl<T>));
+    }
     d.npb = A.get<NpBody<T>>(nb);
     d.npd = A.get<NpDyn<T>>(nb);
     d.bBrad = A.get<T>(nb);
@@ -818,7 +826,8 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
         mark();  // EPA
         hipLaunchKernelGGL((k_np_manifold<T, CC_SMALL_FACE>), dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
         hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
-        mark();  // manifolds, contact cache, constraint setup
+        if (d.hasMesh) hipLaunchKernelGGL(k_np_mesh<T>, dim3(d.meshWorkers / MESH_LANES), dim3(MESH_LANES), 0, st, d, dt);
+        mark();  // manifolds (and convex-vs-mesh pairs), contact cache, constraint setup
         hipLaunchKernelGGL(k_wake<T>, dim3(nbB), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_con_flags<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
         scan_launch(st, d.conFlag, d.conPos, d.pairCap, X->scanSums);
@@ -1013,6 +1022,7 @@ static int contacts_impl(cc_ctx* c, cc_contact_out* out, int cap, int* count) {
         cc_contact_out o;
         memset(&o, 0, sizeof(o));
         o.body_a = h.a; o.body_b = h.b; o.num_points = h.np;
+        o.sub_shape_a = r.subA; o.sub_shape_b = r.subB;
         o.normal[0] = h.normal.x; o.normal[1] = h.normal.y; o.normal[2] = h.normal.z;
         for (int j = 0; j < h.np && j < 4; j++) {
             o.position_a[j][0] = t.pa[j].x; o.position_a[j][1] = t.pa[j].y; o.position_a[j][2] = t.pa[j].z;
@@ -1022,7 +1032,10 @@ static int contacts_impl(cc_ctx* c, cc_contact_out* out, int cap, int* count) {
         res.push_back(o);
     }
     std::sort(res.begin(), res.end(), [](const cc_contact_out& x, const cc_contact_out& y) {
-        return x.body_a != y.body_a ? x.body_a < y.body_a : x.body_b < y.body_b;
+        if (x.body_a != y.body_a) return x.body_a < y.body_a;
+        if (x.body_b != y.body_b) return x.body_b < y.body_b;
+        if (x.sub_shape_a != y.sub_shape_a) return x.sub_shape_a < y.sub_shape_a;
+        return x.sub_shape_b < y.sub_shape_b;
     });
     for (int k = 0; k < n && k < cap; k++) out[k] = res[k];
     return CC_OK;

@@ -479,6 +479,52 @@ def batched_worlds(seed=1003, num_worlds=4096, min_bodies=8, max_bodies=512, cha
     return sc
 
 
+def terrain_heights(rng, amps=(0.5, 3.0), wavelengths=(8.0, 64.0), terms=4):
+    """height(x, z) = sum of `terms` seeded sinusoids: amplitude U[amps], wavelength U[wavelengths],
+    direction uniform in the plane, phase U[0, 2 pi)."""
+    par = []
+    for _ in range(terms):
+        a = rng.uniform(*amps)
+        lam = rng.uniform(*wavelengths)
+        th = rng.uniform(0, 2 * math.pi)
+        ph = rng.uniform(0, 2 * math.pi)
+        par.append((a, lam, math.cos(th), math.sin(th), ph))
+
+    def h(x, z):
+        y = 0.0 * x
+        for a, lam, dx, dz, ph in par:
+            y = y + a * np.sin(2 * math.pi * (dx * x + dz * z) / lam + ph)
+        return y
+    return h
+
+
+def terrain_drop(seed=1004, vertices=512, spacing=0.5, n=50000, drop=(5.0, 30.0), hull_variants=64, scene=None):
+    """Config 4: a static triangle mesh over a `vertices` x `vertices` grid at `spacing` (511^2 * 2
+    = 522,242 triangles at the full size), heights from terrain_heights; n boxes (he 0.5),
+    capsules (r 0.25, hh 0.25) and convex hulls (16 points on a sphere of radius 0.5; one of
+    `hull_variants` seeded hulls), a third each, dropped from U[drop] m above the terrain at
+    positions uniform over it (1 m in from its edges), random orientation, at rest."""
+    sc = scene if scene is not None else Scene()
+    rng = Rng(seed)
+    h = terrain_heights(rng)
+    ext = (vertices - 1) * spacing
+    pos, idx = S.height_field_mesh(vertices, vertices, spacing, h, origin=(-ext / 2, -ext / 2))
+    sc.add_body(S.triangle_mesh(pos, idx), (0.0, 0.0, 0.0), motion_type=_abi.CC_MOTION_STATIC, friction=0.5)
+    box = S.box((0.5, 0.5, 0.5))
+    cap = S.capsule(0.25, 0.25)
+    hulls = [S.convex_hull(_unit_sphere_points(rng, 16) * 0.5) for _ in range(hull_variants)]
+    half = ext / 2 - 1.0
+    for i in range(n):
+        kind = i % 3
+        shape = box if kind == 0 else (cap if kind == 1 else hulls[int(rng.uniform(0, hull_variants)) % hull_variants])
+        x = rng.uniform(-half, half)
+        z = rng.uniform(-half, half)
+        y = float(h(np.float64(x), np.float64(z))) + rng.uniform(*drop)
+        sc.add_body(shape, (x, y, z), rng.quaternion(), friction=0.5)
+    sc.end_world()
+    return sc
+
+
 def replicate(builder, copies, seed0, **kw):
     """A batch of `copies` independent worlds made by `builder` with seeds seed0, seed0+1, ..."""
     sc = Scene()
