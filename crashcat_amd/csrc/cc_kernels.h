@@ -160,7 +160,7 @@ enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
        CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_MESH = 10, CNT_N = 11 };
 // CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list, 16 hits,
-// 32 mesh collector (manifolds or BVH stack)
+// 32 mesh BVH traversal stack
 constexpr int BP_SLOTS = 32;
 
 // orders LDS accesses between the lanes of one wavefront
@@ -1036,7 +1036,7 @@ template <typename T>
 __global__ void k_con_flags(Dev<T> d) {
     int np = min(d.counters[CNT_PAIRS], d.pairCap);
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < d.pairCap; p += gridDim.x * blockDim.x)
-        d.conFlag[p] = (p < np && d.pairCon[p] >= 0) ? (d.hasMesh ? 1 + ((d.conHead[d.pairCon[p]].bits >> 8) & 7) : 1) : 0;
+        d.conFlag[p] = (p < np && d.pairCon[p] >= 0) ? (d.hasMesh ? 1 + ((d.conHead[d.pairCon[p]].bits >> 8) & 31) : 1) : 0;
 }
 
 template <typename T>
@@ -1047,7 +1047,7 @@ __global__ void k_con_compact(Dev<T> d) {
         int c = d.pairCon[p];
         if (c >= 0) {
             const int bits = d.conHead[c].bits;
-            const int run = d.hasMesh ? 1 + ((bits >> 8) & 7) : 1;
+            const int run = d.hasMesh ? 1 + ((bits >> 8) & 31) : 1;
             for (int t = 0; t < run; t++) {
                 d.conIdx[d.conPos[p] + t] = c + t;
                 pts += (unsigned long long)((t ? d.conHead[c + t].bits : bits) >> 4 & 7);

@@ -11,7 +11,7 @@
 
 namespace cc {
 
-constexpr int MESH_MAX_MANIFOLDS = 8;  // the reference's pool holds 32; more than 8 normals is an overflow
+constexpr int MESH_MAX_MANIFOLDS = 32;  // the collector's manifold pool (update.ts:358-360)
 constexpr int MESH_MAX_POINTS = 64;    // points a manifold collects before it is pruned (update.ts:500-518)
 constexpr int MESH_STACK = 64;         // BVH traversal stack (deeper trees overflow)
 constexpr int MESH_LANES = 64;
@@ -43,9 +43,9 @@ __device__ inline V3<T> fix_normal(V3<T> v0, V3<T> v1, V3<T> v2, V3<T> tn, int a
     return (act & edge) ? n : tn;
 }
 
-// addHit of the reduction collector (src/update.ts:385-520); returns false on overflow
+// addHit of the reduction collector (src/update.ts:385-520)
 template <typename T>
-__device__ inline bool mesh_add_hit(const Dev<T>& d, MeshColl<T>& C, int& nm, V3<T> base, V3<T> pA, V3<T> pB, V3<T> axis,
+__device__ inline void mesh_add_hit(const Dev<T>& d, MeshColl<T>& C, int& nm, V3<T> base, V3<T> pA, V3<T> pB, V3<T> axis,
                                     T pen, unsigned subA, unsigned subB, const Face<T, CC_MAX_FACE>& fA,
                                     const Face<T, CC_MAX_FACE>& fB, NpScratch<T, CC_MAX_FACE>& S) {
     const V3<T> normal = normalize(axis);
@@ -53,8 +53,15 @@ __device__ inline bool mesh_add_hit(const Dev<T>& d, MeshColl<T>& C, int& nm, V3
     for (int i = 0; i < nm; i++)
         if (dot(normal, C.m[i].normal) >= d.s.normalCos) { f = i; break; }
     if (f < 0) {
-        if (nm >= MESH_MAX_MANIFOLDS) return false;
-        f = nm++;
+        if (nm >= MESH_MAX_MANIFOLDS) {  // pool full: a deeper hit replaces the shallowest manifold
+            int sh = 0;
+            for (int i = 1; i < nm; i++)
+                if (C.depth[i] < C.depth[sh]) sh = i;
+            if (pen < C.depth[sh]) return;
+            f = sh;
+        } else {
+            f = nm++;
+        }
         C.m[f].n = 0;
         C.m[f].base = base;
         C.m[f].normal = normal;
@@ -78,7 +85,6 @@ __device__ inline bool mesh_add_hit(const Dev<T>& d, MeshColl<T>& C, int& nm, V3
     }
     if (pen > C.depth[f]) C.depth[f] = pen;
     if (m.n > 32) prune(m, m.normal);
-    return true;
 }
 
 template <typename T>
@@ -208,8 +214,8 @@ __global__ void __launch_bounds__(64) k_np_mesh(Dev<T> d, T dt) {
                     S.fB.n = 3;
                     S.fB.v[0] = a; S.fB.v[1] = b; S.fB.v[2] = c;
                     face_transform(S.fB, pa, qa);
-                    if (!rev) ok = mesh_add_hit(d, C, nm, RA.pos, wA, wB, wAxis, pen, 0xffffffffu, sub, S.fA, S.fB, S);
-                    else ok = mesh_add_hit(d, C, nm, RA.pos, wB, wA, -wAxis, pen, sub, 0xffffffffu, S.fB, S.fA, S);
+                    if (!rev) mesh_add_hit(d, C, nm, RA.pos, wA, wB, wAxis, pen, 0xffffffffu, sub, S.fA, S.fB, S);
+                    else mesh_add_hit(d, C, nm, RA.pos, wB, wA, -wAxis, pen, sub, 0xffffffffu, S.fB, S.fA, S);
                 }
             } else {
                 const int l = o + 1, r = nd.a;
@@ -222,7 +228,7 @@ __global__ void __launch_bounds__(64) k_np_mesh(Dev<T> d, T dt) {
                 else { stack[ns++] = l; stack[ns++] = r; }
             }
         }
-        if (!ok) { atomicOr(&d.counters[CNT_OVERFLOW], 32); continue; }
+        if (!ok) { atomicOr(&d.counters[CNT_OVERFLOW], 32); continue; }  // BVH deeper than the stack
         // finalizeAndCreateConstraints (update.ts:522-555): the pair's manifolds become a run of
         // consecutive constraint slots
         int run = 0;

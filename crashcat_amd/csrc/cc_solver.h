@@ -429,11 +429,11 @@ struct CreationLess {
     const Dev<T>* d;
     int so;
     __device__ void key(int k, unsigned long long& hi, unsigned long long& lo) const {
-        // bodies are below 2^23, a run holds at most 8 constraints (co carries the place in bits 24+)
+        // bodies are below 2^23, a run holds at most 32 constraints (co carries the place in bits 24+)
         int s = d->seq[so + k];
         if (s >= 0) {
             const unsigned co = (unsigned)d->con[s].co;
-            hi = ((((unsigned long long)(unsigned)d->con[s].cq << 26) | (co & 0xffffffu)) << 3) | (co >> 24);
+            hi = ((((unsigned long long)(unsigned)d->con[s].cq << 24) | (co & 0xffffffu)) << 5) | (co >> 24);
         } else {
             hi = (1ull << 63) | (unsigned)(-1 - s);
         }

@@ -88,7 +88,8 @@ struct Con {
 struct ConHead {
     int a, b;
     unsigned sortKey;
-    int bits;
+    int bits;  // bits 0-1 dynamic A/B, 2-3 non-static A/B, 4-6 points, 7 not the first of its pair's
+               // run, 8-12 (first only) the run's length - 1
 };
 // Persistent contact cache entry (src/contacts.ts:9-58): four in-place slots.
 template <typename T>

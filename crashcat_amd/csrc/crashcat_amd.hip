@@ -557,7 +557,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.hasMesh = X->hasMesh ? 1 : 0;
     if (X->hasMesh) {  // one collector per mesh thread; the EPA polytopes are the EPA kernel's
         d.meshList = A.get<int>(d.pairCap);
-        d.meshWorkers = std::min(((d.epaWorkers + EPA_LANES - 1) / EPA_LANES) * EPA_LANES, 16384);
+        d.meshWorkers = ((d.epaWorkers + EPA_LANES - 1) / EPA_LANES) * EPA_LANES;
         d.meshWork = A.get<char>((size_t)d.meshWorkers * sizeof(MeshColl<T>));
     }
     d.npb = A.get<NpBody<T>>(nb);
@@ -874,6 +874,9 @@ extern "C" int cc_sync(cc_ctx* c) {
             if (ovf & 1) m += " pairs (raise max_pairs_per_body)";
             if (ovf & 2) m += " contact constraints (raise max_contacts_per_body)";
             if (ovf & 4) m += " EPA jobs";
+            if (ovf & 8) m += " pair spill list";
+            if (ovf & 16) m += " narrowphase hits";
+            if (ovf & 32) m += " mesh BVH traversal stack (a BVH deeper than 63 levels)";
             return fail(c, CC_E_CAPACITY, m);
         }
     }
