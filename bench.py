@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60, help="oracle steps timed from the settled pile")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run the FP32 batch for warmup + this many steps (for rocprofv3)")
+    ap.add_argument("--profile-workload", choices=["pile", "terrain"], default="pile",
+                    help="workload of --profile-steps (terrain: the 50,000-body drop, 60 warmup steps)")
     ap.add_argument("--calibrate", type=int, default=0, help="only copy this many bytes device to device")
     ap.add_argument("--batched-worlds", type=int, default=4096,
                     help="worlds in the batched uneven-worlds measurement (0 skips it)")
@@ -118,6 +120,18 @@ def main():
     def ctx_for(prec):
         return Context(prec, max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
                        max_contacts_per_body=args.contacts_per_body)
+
+    if args.profile_steps and args.profile_workload == "terrain":
+        tw = S.terrain_drop(seed=1004, n=args.terrain_bodies or 50000)
+        ctx = Context("fp32", max_pairs_per_body=16, max_contacts_per_body=8, block_solver=args.block)
+        ctx.upload(tw)
+        for _ in range(3):
+            ctx.step(DT, 20)
+            ctx.sync()
+        ctx.step(DT, args.profile_steps)
+        ctx.sync()
+        log("terrain profile run done", ctx.counters())
+        return
 
     t0 = time.time()
     base = S.mixed_pile(seed=1002, n=args.bodies)

@@ -87,7 +87,10 @@ function hasCallbacks(listener) {
 
 // One GPU context holding a batch of independent worlds.
 class GpuWorld {
-  // desc: { settings, worldBodyOffsets, shapes, hull, bodies, pointConstraints }
+  // desc: { settings, worldBodyOffsets, shapes, hull, mesh, bodies, pointConstraints }
+  // mesh (optional): { positions, triangles, nodes } as a TriangleMeshShape holds them after
+  // buildTriangleMesh: data.positions, the triangle buffer as 4 ints per triangle (a, b, c,
+  // active-edge bits) and bvh.buffer, concatenated over the scene's meshes
   // opts: { precision: 'fp32' | 'fp64', maxPairsPerBody, maxContactsPerBody, device }
   constructor(desc, opts) {
     const o = opts || {};
@@ -107,6 +110,11 @@ class GpuWorld {
       totalFaces: (hull.faceNumVertices || []).length,
       totalFaceVertices: (hull.faceVertexIndices || []).length,
     };
+    if (desc.mesh && desc.mesh.triangles && desc.mesh.triangles.length) {
+      hullBuffers.meshPositions = new Float64Array(desc.mesh.positions);
+      hullBuffers.meshTriangles = new Int32Array(desc.mesh.triangles);
+      hullBuffers.meshNodes = new Float64Array(desc.mesh.nodes);
+    }
     const bodies = desc.bodies || [];
     const offsets = desc.worldBodyOffsets || [0, bodies.length];
     addon.upload(this.handle, pack('cc_settings', [desc.settings]), new Int32Array(offsets),

@@ -315,6 +315,11 @@ class Scene:
                 "planeNormals": keep["pn"].reshape(-1)[: 3 * hull.total_faces].tolist(),
                 "planeConstants": keep["pc"][: hull.total_faces].tolist(),
             },
+            "mesh": {
+                "positions": keep["mpos"].reshape(-1)[: 3 * keep["meshes"].total_vertices].tolist(),
+                "triangles": keep["mtri"].reshape(-1)[: 4 * keep["meshes"].total_triangles].tolist(),
+                "nodes": keep["mnode"].reshape(-1)[: 8 * keep["meshes"].total_nodes].tolist(),
+            },
             "bodies": bodies,
             "pointConstraints": [struct_dict(pcs[i]) for i in range(npc)],
         }

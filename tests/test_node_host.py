@@ -88,9 +88,10 @@ def test_create_without_gpu_throws():
 
 @needs_node
 @pytest.mark.gpu
-def test_node_update_world_cube_heap_bitwise(tmp_path):
+@pytest.mark.parametrize("which", ["cube_heap", "terrain"])
+def test_node_update_world_cube_heap_bitwise(tmp_path, which):
     from oracle import refcpu as O
-    sc = SC.cube_heap(seed=1001)
+    sc = SC.cube_heap(seed=1001) if which == "cube_heap" else SC.terrain_drop(vertices=24, n=40, drop=(0.5, 3.0))
     desc = tmp_path / "world.json"
     desc.write_text(json.dumps(sc.desc()))
     out, pairs = tmp_path / "bodies.bin", tmp_path / "pairs.bin"
