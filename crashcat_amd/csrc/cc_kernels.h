@@ -85,6 +85,12 @@ struct Dev {
     int* meshList;      // [pairCap] pair slots of convex-vs-mesh pairs
     int meshWorkers;    // threads of the mesh kernel (each owns a collector and an EPA polytope)
     char* meshWork;     // [meshWorkers * sizeof(MeshColl<T>)] collectors
+    int* meshCnt;       // [pairCap + 1] candidate triangles of each mesh pair
+    int* meshOff;       // [pairCap + 1] exclusive scan of meshCnt
+    int candCap;
+    int* candTri;       // [candCap] candidate triangle (index within its mesh)
+    int* candPair;      // [candCap] its mesh pair (index into meshList)
+    void* candHit;      // [candCap] MeshHit<T>: the candidate's GJK/EPA result
     int* conFlag;       // [pairCap + 1]
     int* conPos;        // [pairCap + 1] exclusive scan of conFlag
     int* pairA;         // [pairCap]
@@ -160,7 +166,7 @@ enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
        CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_MESH = 10, CNT_N = 11 };
 // CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list, 16 hits,
-// 32 mesh BVH traversal stack
+// 32 mesh BVH traversal stack, 64 mesh candidate triangles
 constexpr int BP_SLOTS = 32;
 
 // orders LDS accesses between the lanes of one wavefront

@@ -1,11 +1,13 @@
 // Convex shapes against static triangle meshes: collideConvexVsTriangleMesh
 // (src/shapes/triangle-mesh.ts:1026-1379) with the reduction collector of the step
-// (src/update.ts:333-563).  One thread per convex-vs-mesh pair walks the mesh BVH from the
-// root (closer child first), collides every triangle that survives the bounds tests by GJK
-// (EPA when the cores overlap) against the convex shape, corrects the normal at inactive
-// edges, merges the hits into manifolds by normal and creates the pair's run of contact
-// constraints.  The convex shape's support mode, the triangle support and the EPA polytope
-// are the convex path's; the polytope workspace is the EPA kernel's, free again by then.
+// (src/update.ts:333-563), in four passes: each convex-vs-mesh pair walks the mesh BVH from
+// the root (closer child first) and counts the triangles that survive the bounds and
+// back-face tests; a scan gives each pair a range, a second walk lists them in walk order;
+// every listed triangle is collided by GJK (EPA when the cores overlap) on its own lane, its
+// normal corrected at inactive edges; then one thread per pair merges its hits into
+// manifolds in walk order and creates the pair's run of contact constraints.  The convex
+// shape's support modes, the triangle support and the EPA polytope are the convex path's; the
+// polytope workspace is the EPA kernel's, free again by then.
 #pragma once
 #include "cc_kernels.h"
 
@@ -15,6 +17,13 @@ constexpr int MESH_MAX_MANIFOLDS = 32;  // the collector's manifold pool (update
 constexpr int MESH_MAX_POINTS = 64;    // points a manifold collects before it is pruned (update.ts:500-518)
 constexpr int MESH_STACK = 64;         // BVH traversal stack (deeper trees overflow)
 constexpr int MESH_LANES = 64;
+
+template <typename T>
+struct MeshHit {  // a candidate triangle's GJK/EPA result in the convex shape's frame
+    int hit;
+    T pen;
+    V3<T> pA, pB, axis;
+};
 
 template <typename T>
 struct MeshColl {
@@ -87,148 +96,257 @@ __device__ inline void mesh_add_hit(const Dev<T>& d, MeshColl<T>& C, int& nm, V3
     if (m.n > 32) prune(m, m.normal);
 }
 
+// One convex-vs-mesh pair, as collideTriangleMeshVsConvex / collideConvexVsTriangleMesh set it up
+// (triangle-mesh.ts:1058-1131): the convex shape is A of the mesh function; `rev` when the mesh
+// is the narrowphase's body A (hits are then inverted, collide-shape-vs-shape.ts:228-330).
 template <typename T>
-__global__ void __launch_bounds__(64) k_np_mesh(Dev<T> d, T dt) {
+struct MeshPair {
+    int slot, A, Bi;
+    bool rev;
+    NpBody<T> RA, RB;
+    int cshape;
+    V3<T> pa;
+    Q4<T> qa;
+    Q4<T> t21q;
+    V3<T> t21p, b1mn, b1mx, q2mn, q2mx, qc, mdl;
+    int mfn, mnn, mft;
+    unsigned mask;
+};
+
+template <typename T>
+__device__ inline void mesh_pair(const Dev<T>& d, int slot, MeshPair<T>& P) {
+    P.slot = slot;
+    const int i0 = d.pairA[slot], i1 = d.pairB[slot];
+    const NpBody<T> r0 = d.npb[i0], r1 = d.npb[i1];
+    const PairBodies<T> pb = order_pair_f<T>(i0, r0.flags, i1, r1.flags);
+    const bool sw = pb.a != i0;
+    P.A = sw ? i1 : i0;
+    P.Bi = sw ? i0 : i1;
+    P.RA = sw ? r1 : r0;
+    P.RB = sw ? r0 : r1;
+    P.rev = (P.RA.tinfo & 15) == SHAPE_MESH;
+    const NpBody<T>& RC = P.rev ? P.RB : P.RA;
+    const NpBody<T>& RM = P.rev ? P.RA : P.RB;
+    const V3<T> move = d.npd[P.A].v - d.npd[P.Bi].v;  // activeEdgeMovementDirection (update.ts:730-735)
+    const ShapeD<T>& sc = d.st.shapes[RC.shape];
+    const ShapeD<T>& sm = d.st.shapes[RM.shape];
+    P.cshape = RC.shape;
+    P.pa = RC.pos;
+    P.qa = RC.q;
+    const V3<T> pm = RM.pos;
+    const Q4<T> qm = RM.q;
+    const Q4<T> iqa = qconj(P.qa);
+    P.t21q = qmul(iqa, qm);
+    P.t21p = qrotate(iqa, pm - P.pa);
+    const Q4<T> iqm = qconj(qm);
+    const Q4<T> qAinB = qmul(iqm, P.qa);
+    const V3<T> pAinB = qrotate(iqm, P.pa - pm);
+    const T ms = d.s.spec;
+    const V3<T> e = mk3<T>(ms, ms, ms);
+    P.b1mn = mk3<T>(sc.amin[0], sc.amin[1], sc.amin[2]) - e;
+    P.b1mx = mk3<T>(sc.amax[0], sc.amax[1], sc.amax[2]) + e;
+    const M3<T> R = m3_from_quat(qAinB);
+    V3<T> q2mn, q2mx;
+    for (int c = 0; c < 8; c++) {  // box3.transformMat4 of the local bounds
+        const V3<T> p = mk3<T>((c & 1) ? sc.amax[0] : sc.amin[0], (c & 2) ? sc.amax[1] : sc.amin[1],
+                               (c & 4) ? sc.amax[2] : sc.amin[2]);
+        const V3<T> wp = mul(R, p) + pAinB;
+        if (c == 0) { q2mn = wp; q2mx = wp; }
+        else {
+            q2mn = mk3<T>(tmin<T>(q2mn.x, wp.x), tmin<T>(q2mn.y, wp.y), tmin<T>(q2mn.z, wp.z));
+            q2mx = mk3<T>(tmax<T>(q2mx.x, wp.x), tmax<T>(q2mx.y, wp.y), tmax<T>(q2mx.z, wp.z));
+        }
+    }
+    P.q2mn = q2mn - e;
+    P.q2mx = q2mx + e;
+    P.qc = mk3<T>((P.q2mn.x + P.q2mx.x) * T(0.5), (P.q2mn.y + P.q2mx.y) * T(0.5), (P.q2mn.z + P.q2mx.z) * T(0.5));
+    P.mdl = qrotate(iqa, move);
+    P.mfn = sm.hff;
+    P.mnn = sm.hnf;
+    P.mft = sm.hfp;
+    const int ntri = sm.hnp;
+    const int nbits = ntri > 1 ? 32 - __clz(ntri - 1) : 0;  // pushIndex sub-shape.ts:60-64
+    P.mask = nbits >= 32 ? 0xffffffffu : ((1u << nbits) - 1u);
+}
+
+// a triangle of the pair's mesh in the convex shape's frame
+template <typename T>
+__device__ inline void mesh_tri(const Dev<T>& d, const MeshPair<T>& P, int ti, V3<T>& a, V3<T>& b, V3<T>& c, int& act) {
+    const int4 tr = d.st.mtri[P.mft + ti];
+    const T* X = d.st.mpos;
+    a = qrotate(P.t21q, mk3<T>(X[3 * tr.x], X[3 * tr.x + 1], X[3 * tr.x + 2])) + P.t21p;
+    b = qrotate(P.t21q, mk3<T>(X[3 * tr.y], X[3 * tr.y + 1], X[3 * tr.y + 2])) + P.t21p;
+    c = qrotate(P.t21q, mk3<T>(X[3 * tr.z], X[3 * tr.z + 1], X[3 * tr.z + 2])) + P.t21p;
+    act = tr.w;
+}
+
+// The BVH walk (closer child first) calling f(triangle) for each triangle that passes the
+// bounds and back-face tests, in the reference's order; false if the stack overflowed.
+template <typename T, typename F>
+__device__ inline bool mesh_walk(const Dev<T>& d, const MeshPair<T>& P, F&& f) {
+    int stack[MESH_STACK];
+    int ns = 0;
+    const MNode<T>* nodes = d.st.mnode + P.mfn;
+    if (P.mnn > 0) stack[ns++] = 0;
+    while (ns > 0) {
+        const int o = stack[--ns];
+        const MNode<T> nd = nodes[o];
+        if (!(nd.mn[0] <= P.q2mx.x && nd.mx[0] >= P.q2mn.x && nd.mn[1] <= P.q2mx.y && nd.mx[1] >= P.q2mn.y &&
+              nd.mn[2] <= P.q2mx.z && nd.mx[2] >= P.q2mn.z))
+            continue;
+        if (nd.b < 0) {
+            const int t0 = nd.a, cnt = -(nd.b + 1);
+            for (int i = 0; i < cnt; i++) {
+                V3<T> a, b, c;
+                int act;
+                mesh_tri(d, P, t0 + i, a, b, c, act);
+                const V3<T> tmn = mk3<T>(tmin<T>(tmin<T>(a.x, b.x), c.x), tmin<T>(tmin<T>(a.y, b.y), c.y),
+                                         tmin<T>(tmin<T>(a.z, b.z), c.z));
+                const V3<T> tmx = mk3<T>(tmax<T>(tmax<T>(a.x, b.x), c.x), tmax<T>(tmax<T>(a.y, b.y), c.y),
+                                         tmax<T>(tmax<T>(a.z, b.z), c.z));
+                if (tmx.x < P.b1mn.x || tmn.x > P.b1mx.x || tmx.y < P.b1mn.y || tmn.y > P.b1mx.y || tmx.z < P.b1mn.z ||
+                    tmn.z > P.b1mx.z)
+                    continue;
+                const V3<T> nrm = cross(b - a, c - a);
+                if (!d.s.backfaces && dot(nrm, a) > T(0)) continue;
+                f(t0 + i);
+            }
+        } else {
+            const int l = o + 1, r = nd.a;
+            const MNode<T> ln = nodes[l], rn = nodes[r];
+            const V3<T> lc = mk3<T>((ln.mn[0] + ln.mx[0]) * T(0.5), (ln.mn[1] + ln.mx[1]) * T(0.5), (ln.mn[2] + ln.mx[2]) * T(0.5));
+            const V3<T> rc = mk3<T>((rn.mn[0] + rn.mx[0]) * T(0.5), (rn.mn[1] + rn.mx[1]) * T(0.5), (rn.mn[2] + rn.mx[2]) * T(0.5));
+            const T ld = len2(lc - P.qc), rd = len2(rc - P.qc);
+            if (ns + 2 > MESH_STACK) return false;
+            if (ld <= rd) { stack[ns++] = r; stack[ns++] = l; }
+            else { stack[ns++] = l; stack[ns++] = r; }
+        }
+    }
+    return true;
+}
+
+// 1. candidates per pair (a count, then the scan gives each pair its range)
+template <typename T>
+__global__ void __launch_bounds__(256) k_mesh_count(Dev<T> d) {
+    const int nmesh = min(d.counters[CNT_MESH], d.pairCap);
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.pairCap; j += gridDim.x * blockDim.x) {
+        int n = 0;
+        if (j < nmesh) {
+            MeshPair<T> P;
+            mesh_pair(d, d.meshList[j], P);
+            if (!mesh_walk(d, P, [&](int) { n++; })) { atomicOr(&d.counters[CNT_OVERFLOW], 32); n = 0; }
+        }
+        d.meshCnt[j] = n;
+    }
+}
+
+// 2. the candidate triangles of each pair, in walk order
+template <typename T>
+__global__ void __launch_bounds__(256) k_mesh_emit(Dev<T> d) {
+    const int nmesh = min(d.counters[CNT_MESH], d.pairCap);
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nmesh; j += gridDim.x * blockDim.x) {
+        const int n = d.meshCnt[j];
+        if (n == 0) continue;
+        const int o = d.meshOff[j];
+        if (o + n > d.candCap) { atomicOr(&d.counters[CNT_OVERFLOW], 64); continue; }
+        MeshPair<T> P;
+        mesh_pair(d, d.meshList[j], P);
+        int k = 0;
+        mesh_walk(d, P, [&](int ti) {
+            if (k < n) { d.candTri[o + k] = ti; d.candPair[o + k] = j; }
+            k++;
+        });
+    }
+}
+
+// 3. GJK (EPA when the cores overlap) of every candidate, one lane each (triangle-mesh.ts:1190-1297)
+template <typename T>
+__global__ void __launch_bounds__(64) k_mesh_tri(Dev<T> d) {
+    const int nmesh = min(d.counters[CNT_MESH], d.pairCap);
+    const int nc = min(d.meshOff[d.pairCap], d.candCap);
+    const int worker = blockIdx.x * blockDim.x + threadIdx.x;
+    if (worker >= d.meshWorkers || nmesh == 0) return;
+    EpaH<T> H;
+    H.b = d.epaWork + (size_t)(worker / EPA_LANES) * EpaWave<T>::bytes;
+    H.lane = worker % EPA_LANES;
+    for (int ci = worker; ci < nc; ci += d.meshWorkers) {
+        const int j = d.candPair[ci], ti = d.candTri[ci];
+        MeshPair<T> P;
+        mesh_pair(d, d.meshList[j], P);
+        V3<T> a, b, c;
+        int act;
+        mesh_tri(d, P, ti, a, b, c, act);
+        MeshHit<T>& out = ((MeshHit<T>*)d.candHit)[ci];
+        out.hit = 0;
+        const V3<T> nrm = cross(b - a, c - a);
+        const bool back = dot(nrm, a) > T(0);
+        V3<T> axis = -nrm;
+        if (len2(axis) < T(1e-10)) axis = mk3<T>(1, 0, 0);
+        else axis = normalize(axis);
+        T maxSep = d.s.spec;
+        Sup<T> SA;
+        sup_for_shape(SA, d.st, P.cshape, false);
+        TriSup<T> TS;
+        TS.a = a; TS.b = b; TS.c = c;
+        PenDepth<T> pd;
+        Simplex<T> simplex;
+        simplex.size = 0;
+        pd_step_gjk(pd, simplex, SA, TS, SA.cr + maxSep, T(0), axis, d.s.colTol);
+        if (pd.status == PD_NOT_COLLIDING) continue;
+        if (pd.status == PD_INDETERMINATE) {
+            maxSep = tmin<T>(maxSep, T(1));
+            Sup<T> SA2;
+            sup_for_shape(SA2, d.st, P.cshape, true);
+            SA2.addRadius = true;
+            SA2.addR = maxSep;
+            if (!pd_step_epa(pd, SA2, TS, d.s.penTol, simplex, H)) continue;
+        }
+        const T pen = sqrt(len2(pd.pointA - pd.pointB)) - maxSep;
+        const T al = sqrt(len2(pd.axis));
+        if (al > T(0)) { T corr = maxSep / al; pd.pointA = pd.pointA + pd.axis * (-corr); }
+        if (d.s.activeEdges && act != 7) pd.axis = fix_normal(a, b, c, back ? nrm : -nrm, act, pd.pointB, pd.axis, P.mdl);
+        out.hit = 1;
+        out.pen = pen;
+        out.pA = pd.pointA;
+        out.pB = pd.pointB;
+        out.axis = pd.axis;
+    }
+}
+
+// 4. the reduction collector over each pair's hits in walk order, then its run of constraints
+template <typename T>
+__global__ void __launch_bounds__(64) k_mesh_collect(Dev<T> d, T dt) {
     const int nmesh = min(d.counters[CNT_MESH], d.pairCap);
     const int worker = blockIdx.x * blockDim.x + threadIdx.x;
     if (worker >= d.meshWorkers) return;
     const int par = *d.parity;
     MeshColl<T>& C = ((MeshColl<T>*)d.meshWork)[worker];
-    EpaH<T> H;
-    H.b = d.epaWork + (size_t)(worker / EPA_LANES) * EpaWave<T>::bytes;
-    H.lane = worker % EPA_LANES;
     NpScratch<T, CC_MAX_FACE> S;
-    int stack[MESH_STACK];
     for (int j = worker; j < nmesh; j += d.meshWorkers) {
-        const int slot = d.meshList[j];
-        const int i0 = d.pairA[slot], i1 = d.pairB[slot];
-        const NpBody<T> r0 = d.npb[i0], r1 = d.npb[i1];
-        const PairBodies<T> pb = order_pair_f<T>(i0, r0.flags, i1, r1.flags);
-        const bool sw = pb.a != i0;
-        const int A = sw ? i1 : i0, Bi = sw ? i0 : i1;
-        const NpBody<T>& RA = sw ? r1 : r0;
-        const NpBody<T>& RB = sw ? r0 : r1;
-        // collideTriangleMeshVsConvex is the convex function with the shapes swapped and the
-        // hits inverted (collide-shape-vs-shape.ts:228-330)
-        const bool rev = (RA.tinfo & 15) == SHAPE_MESH;
-        const NpBody<T>& RC = rev ? RB : RA;
-        const NpBody<T>& RM = rev ? RA : RB;
-        const V3<T> move = d.npd[A].v - d.npd[Bi].v;  // activeEdgeMovementDirection (update.ts:730-735)
-        const ShapeD<T>& sc = d.st.shapes[RC.shape];
-        const ShapeD<T>& sm = d.st.shapes[RM.shape];
-        const V3<T> pa = RC.pos, pm = RM.pos;
-        const Q4<T> qa = RC.q, qm = RM.q;
-        const Q4<T> iqa = qconj(qa);
-        const Q4<T> t21q = qmul(iqa, qm);
-        const V3<T> t21p = qrotate(iqa, pm - pa);
-        const Q4<T> iqm = qconj(qm);
-        const Q4<T> qAinB = qmul(iqm, qa);
-        const V3<T> pAinB = qrotate(iqm, pa - pm);
-        const T maxSep0 = d.s.spec;
-        const V3<T> e = mk3<T>(maxSep0, maxSep0, maxSep0);
-        const V3<T> b1mn = mk3<T>(sc.amin[0], sc.amin[1], sc.amin[2]) - e;
-        const V3<T> b1mx = mk3<T>(sc.amax[0], sc.amax[1], sc.amax[2]) + e;
-        V3<T> q2mn, q2mx;
-        {
-            const M3<T> R = m3_from_quat(qAinB);
-            for (int c = 0; c < 8; c++) {  // box3.transformMat4 of the local bounds
-                const V3<T> p = mk3<T>((c & 1) ? sc.amax[0] : sc.amin[0], (c & 2) ? sc.amax[1] : sc.amin[1],
-                                       (c & 4) ? sc.amax[2] : sc.amin[2]);
-                const V3<T> wp = mul(R, p) + pAinB;
-                if (c == 0) { q2mn = wp; q2mx = wp; }
-                else {
-                    q2mn = mk3<T>(tmin<T>(q2mn.x, wp.x), tmin<T>(q2mn.y, wp.y), tmin<T>(q2mn.z, wp.z));
-                    q2mx = mk3<T>(tmax<T>(q2mx.x, wp.x), tmax<T>(q2mx.y, wp.y), tmax<T>(q2mx.z, wp.z));
-                }
-            }
-            q2mn = q2mn - e;
-            q2mx = q2mx + e;
-        }
-        Sup<T> SA;
-        sup_for_shape(SA, d.st, RC.shape, false);
-        const int ntri = sm.hnp;
-        const int nbits = ntri > 1 ? 32 - __clz(ntri - 1) : 0;  // pushIndex sub-shape.ts:60-64
-        const unsigned mask = nbits >= 32 ? 0xffffffffu : ((1u << nbits) - 1u);
-        const V3<T> qc = mk3<T>((q2mn.x + q2mx.x) * T(0.5), (q2mn.y + q2mx.y) * T(0.5), (q2mn.z + q2mx.z) * T(0.5));
-        const V3<T> mdl = qrotate(iqa, move);
-        const MNode<T>* nodes = d.st.mnode + sm.hff;
-        const int4* tris = d.st.mtri + sm.hfp;
+        const int n = d.meshCnt[j];
+        const int o = d.meshOff[j];
+        if (n == 0 || o + n > d.candCap) continue;
+        MeshPair<T> P;
+        mesh_pair(d, d.meshList[j], P);
         int nm = 0;
-        bool ok = true;
-        int ns = 0;
-        if (sm.hnf > 0) stack[ns++] = 0;
-        while (ns > 0 && ok) {
-            const int o = stack[--ns];
-            const MNode<T> nd = nodes[o];
-            if (!(nd.mn[0] <= q2mx.x && nd.mx[0] >= q2mn.x && nd.mn[1] <= q2mx.y && nd.mx[1] >= q2mn.y && nd.mn[2] <= q2mx.z &&
-                  nd.mx[2] >= q2mn.z))
-                continue;
-            if (nd.b < 0) {
-                const int t0 = nd.a, cnt = -(nd.b + 1);
-                for (int i = 0; i < cnt && ok; i++) {
-                    const int ti = t0 + i;
-                    const int4 tr = tris[ti];
-                    const T* P = d.st.mpos;
-                    const V3<T> a = qrotate(t21q, mk3<T>(P[3 * tr.x], P[3 * tr.x + 1], P[3 * tr.x + 2])) + t21p;
-                    const V3<T> b = qrotate(t21q, mk3<T>(P[3 * tr.y], P[3 * tr.y + 1], P[3 * tr.y + 2])) + t21p;
-                    const V3<T> c = qrotate(t21q, mk3<T>(P[3 * tr.z], P[3 * tr.z + 1], P[3 * tr.z + 2])) + t21p;
-                    const V3<T> tmn = mk3<T>(tmin<T>(tmin<T>(a.x, b.x), c.x), tmin<T>(tmin<T>(a.y, b.y), c.y),
-                                             tmin<T>(tmin<T>(a.z, b.z), c.z));
-                    const V3<T> tmx = mk3<T>(tmax<T>(tmax<T>(a.x, b.x), c.x), tmax<T>(tmax<T>(a.y, b.y), c.y),
-                                             tmax<T>(tmax<T>(a.z, b.z), c.z));
-                    if (tmx.x < b1mn.x || tmn.x > b1mx.x || tmx.y < b1mn.y || tmn.y > b1mx.y || tmx.z < b1mn.z ||
-                        tmn.z > b1mx.z)
-                        continue;
-                    const V3<T> nrm = cross(b - a, c - a);
-                    const bool back = dot(nrm, a) > T(0);
-                    if (!d.s.backfaces && back) continue;
-                    V3<T> axis = -nrm;
-                    if (len2(axis) < T(1e-10)) axis = mk3<T>(1, 0, 0);
-                    else axis = normalize(axis);
-                    T maxSep = maxSep0;
-                    TriSup<T> TS;
-                    TS.a = a; TS.b = b; TS.c = c;
-                    PenDepth<T> pd;
-                    Simplex<T> simplex;
-                    simplex.size = 0;
-                    pd_step_gjk(pd, simplex, SA, TS, SA.cr + maxSep, T(0), axis, d.s.colTol);
-                    if (pd.status == PD_NOT_COLLIDING) continue;
-                    if (pd.status == PD_INDETERMINATE) {
-                        maxSep = tmin<T>(maxSep, T(1));
-                        Sup<T> SA2;
-                        sup_for_shape(SA2, d.st, RC.shape, true);
-                        SA2.addRadius = true;
-                        SA2.addR = maxSep;
-                        if (!pd_step_epa(pd, SA2, TS, d.s.penTol, simplex, H)) continue;
-                    }
-                    const T pen = sqrt(len2(pd.pointA - pd.pointB)) - maxSep;
-                    const T al = sqrt(len2(pd.axis));
-                    if (al > T(0)) { T corr = maxSep / al; pd.pointA = pd.pointA + pd.axis * (-corr); }
-                    if (d.s.activeEdges && tr.w != 7)
-                        pd.axis = fix_normal(a, b, c, back ? nrm : -nrm, tr.w, pd.pointB, pd.axis, mdl);
-                    const V3<T> wA = qrotate(qa, pd.pointA) + pa;
-                    const V3<T> wB = qrotate(qa, pd.pointB) + pa;
-                    const V3<T> wAxis = qrotate(qa, pd.axis);
-                    const unsigned sub = (0xffffffffu & ~mask) | (unsigned)ti;
-                    supporting_face(S.fA, d.st, RC.shape, -pd.axis, pa, qa);
-                    S.fB.n = 3;
-                    S.fB.v[0] = a; S.fB.v[1] = b; S.fB.v[2] = c;
-                    face_transform(S.fB, pa, qa);
-                    if (!rev) mesh_add_hit(d, C, nm, RA.pos, wA, wB, wAxis, pen, 0xffffffffu, sub, S.fA, S.fB, S);
-                    else mesh_add_hit(d, C, nm, RA.pos, wB, wA, -wAxis, pen, sub, 0xffffffffu, S.fB, S.fA, S);
-                }
-            } else {
-                const int l = o + 1, r = nd.a;
-                const MNode<T> ln = nodes[l], rn = nodes[r];
-                const V3<T> lc = mk3<T>((ln.mn[0] + ln.mx[0]) * T(0.5), (ln.mn[1] + ln.mx[1]) * T(0.5), (ln.mn[2] + ln.mx[2]) * T(0.5));
-                const V3<T> rc = mk3<T>((rn.mn[0] + rn.mx[0]) * T(0.5), (rn.mn[1] + rn.mx[1]) * T(0.5), (rn.mn[2] + rn.mx[2]) * T(0.5));
-                const T ld = len2(lc - qc), rd = len2(rc - qc);
-                if (ns + 2 > MESH_STACK) { ok = false; break; }
-                if (ld <= rd) { stack[ns++] = r; stack[ns++] = l; }
-                else { stack[ns++] = l; stack[ns++] = r; }
-            }
+        for (int k = 0; k < n; k++) {
+            const MeshHit<T> h = ((const MeshHit<T>*)d.candHit)[o + k];
+            if (!h.hit) continue;
+            const int ti = d.candTri[o + k];
+            V3<T> a, b, c;
+            int act;
+            mesh_tri(d, P, ti, a, b, c, act);
+            const V3<T> wA = qrotate(P.qa, h.pA) + P.pa;
+            const V3<T> wB = qrotate(P.qa, h.pB) + P.pa;
+            const V3<T> wAxis = qrotate(P.qa, h.axis);
+            const unsigned sub = (0xffffffffu & ~P.mask) | (unsigned)ti;
+            supporting_face(S.fA, d.st, P.cshape, -h.axis, P.pa, P.qa);
+            S.fB.n = 3;
+            S.fB.v[0] = a; S.fB.v[1] = b; S.fB.v[2] = c;
+            face_transform(S.fB, P.pa, P.qa);
+            if (!P.rev) mesh_add_hit(d, C, nm, P.RA.pos, wA, wB, wAxis, h.pen, 0xffffffffu, sub, S.fA, S.fB, S);
+            else mesh_add_hit(d, C, nm, P.RA.pos, wB, wA, -wAxis, h.pen, sub, 0xffffffffu, S.fB, S.fA, S);
         }
-        if (!ok) { atomicOr(&d.counters[CNT_OVERFLOW], 32); continue; }  // BVH deeper than the stack
         // finalizeAndCreateConstraints (update.ts:522-555): the pair's manifolds become a run of
         // consecutive constraint slots
         int run = 0;
@@ -239,14 +357,14 @@ __global__ void __launch_bounds__(64) k_np_mesh(Dev<T> d, T dt) {
             if (m.n > 0) run++;
         }
         if (run == 0) continue;
-        const bool dynA = fl_motion(RA.flags) == MOTION_DYNAMIC, dynB = fl_motion(RB.flags) == MOTION_DYNAMIC;
-        if (!((dynA && RA.invMass != T(0)) || (dynB && RB.invMass != T(0)))) continue;
+        const bool dynA = fl_motion(P.RA.flags) == MOTION_DYNAMIC, dynB = fl_motion(P.RB.flags) == MOTION_DYNAMIC;
+        if (!((dynA && P.RA.invMass != T(0)) || (dynB && P.RB.invMass != T(0)))) continue;
         const int cs0 = atomicAdd(&d.counters[CNT_CONS], run);
         if (cs0 + run > d.conCap) { atomicOr(&d.counters[CNT_OVERFLOW], 2); continue; }
         int k = 0;
         for (int i = 0; i < nm; i++) {
             if (C.m[i].n <= 0) continue;
-            make_contact(d, slot, A, Bi, RA, RB, C.m[i], C.subA[i], C.subB[i], cs0 + k, k, run, dt, par);
+            make_contact(d, P.slot, P.A, P.Bi, P.RA, P.RB, C.m[i], C.subA[i], C.subB[i], cs0 + k, k, run, dt, par);
             k++;
         }
     }

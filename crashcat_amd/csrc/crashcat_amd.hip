@@ -559,6 +559,12 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         d.meshList = A.get<int>(d.pairCap);
         d.meshWorkers = ((d.epaWorkers + EPA_LANES - 1) / EPA_LANES) * EPA_LANES;
         d.meshWork = A.get<char>((size_t)d.meshWorkers * sizeof(MeshColl<T>));
+        d.meshCnt = A.get<int>((size_t)d.pairCap + 1);
+        d.meshOff = A.get<int>((size_t)d.pairCap + 1);
+        d.candCap = (int)std::min<long long>(64ll * std::max(nb, 1), 1ll << 30);
+        d.candTri = A.get<int>(d.candCap);
+        d.candPair = A.get<int>(d.candCap);
+        d.candHit = A.get<MeshHit<T>>(d.candCap);
     }
     d.npb = A.get<NpBody<T>>(nb);
     d.npd = A.get<NpDyn<T>>(nb);
@@ -826,7 +832,13 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
         mark();  // EPA
         hipLaunchKernelGGL((k_np_manifold<T, CC_SMALL_FACE>), dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
         hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
-        if (d.hasMesh) hipLaunchKernelGGL(k_np_mesh<T>, dim3(d.meshWorkers / MESH_LANES), dim3(MESH_LANES), 0, st, d, dt);
+        if (d.hasMesh) {
+            hipLaunchKernelGGL(k_mesh_count<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
+            scan_launch(st, d.meshCnt, d.meshOff, d.pairCap, X->scanSums);
+            hipLaunchKernelGGL(k_mesh_emit<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
+            hipLaunchKernelGGL(k_mesh_tri<T>, dim3(d.meshWorkers / MESH_LANES), dim3(MESH_LANES), 0, st, d);
+            hipLaunchKernelGGL(k_mesh_collect<T>, dim3(d.meshWorkers / MESH_LANES), dim3(MESH_LANES), 0, st, d, dt);
+        }
         mark();  // manifolds (and convex-vs-mesh pairs), contact cache, constraint setup
         hipLaunchKernelGGL(k_wake<T>, dim3(nbB), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_con_flags<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
@@ -877,6 +889,7 @@ extern "C" int cc_sync(cc_ctx* c) {
             if (ovf & 8) m += " pair spill list";
             if (ovf & 16) m += " narrowphase hits";
             if (ovf & 32) m += " mesh BVH traversal stack (a BVH deeper than 63 levels)";
+            if (ovf & 64) m += " mesh candidate triangles (64 per body)";
             return fail(c, CC_E_CAPACITY, m);
         }
     }
