@@ -204,4 +204,5 @@ class Config(C.Structure):
         ("max_contacts_per_body", C.c_int32),
         ("block_solver", C.c_int32),
         ("block_bodies", C.c_int32),
+        ("island_groups", C.c_int32),
     ]

@@ -65,7 +65,8 @@ PHASES = ["broadphase", "narrowphase", "epa", "manifold", "islands", "solver", "
 class Context:
     """One device context (a batch of worlds) behind the C ABI."""
 
-    def __init__(self, precision="fp32", device=0, max_pairs_per_body=16, block_solver=0, max_contacts_per_body=8):
+    def __init__(self, precision="fp32", device=0, max_pairs_per_body=16, block_solver=0, max_contacts_per_body=8,
+                 island_groups=0):
         L = lib()
         cfg = _abi.Config()
         cfg.precision = _abi.CC_PRECISION_FP64 if precision == "fp64" else _abi.CC_PRECISION_FP32
@@ -73,6 +74,7 @@ class Context:
         cfg.max_pairs_per_body = max_pairs_per_body
         cfg.max_contacts_per_body = max_contacts_per_body
         cfg.block_solver = block_solver
+        cfg.island_groups = island_groups
         h = C.c_void_p()
         rc = L.cc_create(C.byref(h), C.byref(cfg))
         if rc != 0:

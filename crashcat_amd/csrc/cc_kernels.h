@@ -47,7 +47,14 @@ struct Dev {
     int N;              // bodies (all worlds)
     int W;              // worlds
     const int* wStart;  // [W+1] body ranges
-    const int* worldOrder;  // [W] worlds by descending body count: the solver's workgroup order
+    // the solver's workgroups: (world, island group, groups of the world, body scratch base),
+    // largest worlds first.  A world of few worlds is split into island groups (islands never
+    // share a body, so each group is solved alone, in the reference order inside each island)
+    const int4* vTab;   // [V]
+    int V;
+    const int* worldG;      // [W] island groups of each world (1: not split)
+    const int* worldVBase;  // [W] index of the world's first group (groups are numbered world by world)
+    int* vCnt;              // [2 * V] contacts, active point constraints of each group
     int pairCap, conCap, epaCap, epaWorkers, hashCap;
     // broadphase: uniform grid per world, occupied cells in an open-addressing hash table
     double cellWidth;   // widest box a body may have to be binned (wider ones are "large")
@@ -1033,6 +1040,49 @@ __global__ void k_wake(Dev<T> d) {
         d.B.wake[i] = 0;
         d.B.sleeping[i] = 0;
         reset_sleep_timer(d.B, d.st, i);
+    }
+}
+
+// The island group of a body of a split world: by its island label; bodies outside every
+// island (static, sleeping) go to group 0, which alone writes their solver records.
+__device__ inline int island_group(int label, int G) { return label >= 0 ? label % G : 0; }
+
+// the body whose island places a contact constraint: a dynamic one (every constraint has one)
+__device__ inline int con_island_body(const ConHead& c) { return (c.bits & 1) ? c.a : c.b; }
+
+// a point constraint takes part in the step when enabled and one of its bodies is in an island
+template <typename T>
+__device__ inline bool pc_active(const Dev<T>& d, const PointCon<T>& pc) {
+    return pc.enabled && (d.B.label[pc.a] >= 0 || d.B.label[pc.b] >= 0);
+}
+
+// group sizes of the split worlds: constraints counted by the island group of their body
+template <typename T>
+__global__ void k_group_zero(Dev<T> d) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < d.V; v += gridDim.x * blockDim.x) {
+        d.vCnt[2 * v] = 0;
+        d.vCnt[2 * v + 1] = 0;
+    }
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < d.W; w += gridDim.x * blockDim.x)
+        if (d.worldG[w] > 1) { d.stats[2 * w] = 0; d.stats[2 * w + 1] = 0; }
+}
+template <typename T>
+__global__ void k_group_count(Dev<T> d) {
+    const int nc = min(d.counters[CNT_CONS], d.conCap);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc + d.NPC; k += gridDim.x * blockDim.x) {
+        int body, kind;
+        if (k < nc) {
+            body = con_island_body(d.conHead[d.conIdx[k]]);
+            kind = 0;
+        } else {
+            const PointCon<T>& pc = d.pc[k - nc];
+            if (!pc_active(d, pc)) continue;
+            body = d.B.label[pc.a] >= 0 ? pc.a : pc.b;
+            kind = 1;
+        }
+        const int w = d.B.world[body], G = d.worldG[w];
+        if (G == 1) continue;
+        atomicAdd(&d.vCnt[2 * (d.worldVBase[w] + island_group(d.B.label[body], G)) + kind], 1);
     }
 }
 

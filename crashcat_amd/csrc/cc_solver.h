@@ -493,10 +493,10 @@ __device__ inline unsigned long long list_key64(const SolveLessAt& less, int x, 
 // unique, so the order is the same whichever way it is sorted: lists of up to eight entries
 // are sorted in registers by a 19-comparator network, longer ones by insertion sort.
 template <typename T, int BS, typename Less>
-__device__ inline void link_body_lists(const Dev<T>& d, int b0, int nb, int so, int* BL, unsigned long long* BK, const int* BC,
-                                       Less less) {
+__device__ inline void link_body_lists(const Dev<T>& d, int b0, int bb, int nb, int so, int* BL, unsigned long long* BK,
+                                       const int* BC, Less less) {
     for (int i = threadIdx.x; i < nb; i += BS) {
-        const int o = d.boff[b0 + i], c = BC[i];
+        const int o = d.boff[bb + i], c = BC[i];
         auto link = [&](int ent, int pv, int nx) {
             const int k = ent >> 1;
             if (ent & 1) { d.prevB[so + k] = pv; d.nextB[so + k] = nx; }
@@ -630,26 +630,69 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __shared__ int sLS[LS_LDS];
     extern __shared__ int dynLds[];  // d.worldLdsInts ints: per-body counts, then packed in-degrees
     __shared__ int sTail;
-    const int w = d.worldOrder[blockIdx.x];
+    const int4 vt = d.vTab[blockIdx.x];
+    const int w = vt.x, grp = vt.y, G = vt.z;
     const int tid = threadIdx.x;
     const Bodies<T>& B = d.B;
     const int b0 = d.wStart[w], b1 = d.wStart[w + 1];
     const int nb = b1 - b0;
+    const int bb = G > 1 ? vt.w : b0;  // this group's per-body scratch (list offsets, counts)
     const int c0 = d.wConStart[w], c1 = d.wConStart[w + 1];
     const int u0 = d.pcStart[w], u1 = d.pcStart[w + 1];
-    const int m = c1 - c0, u = u1 - u0, L = m + u;
-    const int so = c0 + u0;
+    // a split world's group takes the constraints of its islands, packed after the groups before it
+    int m = c1 - c0, u = u1 - u0;
+    int so = c0 + u0;
+    const int vid = d.worldVBase[w] + grp;
+    if (G > 1) {
+        for (int g2 = d.worldVBase[w]; g2 < vid; g2++) so += d.vCnt[2 * g2] + d.vCnt[2 * g2 + 1];
+        m = d.vCnt[2 * vid];
+        u = d.vCnt[2 * vid + 1];
+    }
+    const int L = m + u;
+    auto mine = [&](int i) { return G == 1 || island_group(B.label[i], G) == grp; };
     const int par = *d.parity;
     const T prevDt = *d.prevDt;
     const T ratio = (d.s.warmStarting && prevDt > T(0)) ? dt / prevDt : T(0);
     unsigned long long* K = d.skeys + 2 * (size_t)so;
 
-    // ---- 1. sequence: the world's contact constraints, then its point constraints
+    // ---- 1. sequence: the world's contact constraints, then its point constraints (a split
+    // world's group: those of its islands, in the same order)
+    if (G > 1) {
+        int run = 0;
+        for (int base = 0; base < c1 - c0; base += BS) {
+            const int k = base + tid;
+            int s = -1;
+            bool f = false;
+            if (k < c1 - c0) {
+                s = d.conIdx[c0 + k];
+                f = island_group(B.label[con_island_body(d.conHead[s])], G) == grp;
+            }
+            int tot;
+            const int r = block_excl_scan<BS>(f ? 1 : 0, sh, tot);
+            if (f) d.seq[so + run + r] = s;
+            run += tot;
+            __syncthreads();
+        }
+        run = 0;
+        for (int base = 0; base < u1 - u0; base += BS) {
+            const int k = base + tid;
+            bool f = false;
+            if (k < u1 - u0) {
+                const PointCon<T>& pc = d.pc[u0 + k];
+                f = pc_active(d, pc) && island_group(B.label[B.label[pc.a] >= 0 ? pc.a : pc.b], G) == grp;
+            }
+            int tot;
+            const int r = block_excl_scan<BS>(f ? 1 : 0, sh, tot);
+            if (f) d.seq[so + m + run + r] = -1 - (u0 + k);
+            run += tot;
+            __syncthreads();
+        }
+    }
     for (int k = tid; k < L; k += BS) {
         int s, ea = -1, eb = -1, npk = 0;
         unsigned long long hi, lo;
         if (k < m) {
-            s = d.conIdx[c0 + k];
+            s = G > 1 ? d.seq[so + k] : d.conIdx[c0 + k];
             const ConHead c = d.conHead[s];
             npk = (c.bits >> 4) & 7;
             if (c.bits & 1) ea = c.a;
@@ -657,7 +700,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
             hi = (unsigned long long)c.sortKey;
             lo = ((unsigned long long)(unsigned)c.a << 32) | (unsigned)c.b;
         } else {
-            int pi = u0 + (k - m);
+            int pi = G > 1 ? -1 - d.seq[so + k] : u0 + (k - m);
             s = -1 - pi;
             const PointCon<T>& pc = d.pc[pi];
             bool act = pc.enabled && (B.label[pc.a] >= 0 || B.label[pc.b] >= 0);
@@ -683,7 +726,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     }
     // per-body counts in LDS when the world's bodies fit, then the packed in-degrees
     const bool cntLds = nb <= d.worldLdsInts;
-    int* BC = cntLds ? dynLds : d.bcnt + b0;
+    int* BC = cntLds ? dynLds : d.bcnt + bb;
     const int degOff = cntLds ? ((nb + 3) & ~3) : 0;
     unsigned* DEG = degOff + ((L + 15) >> 4) <= d.worldLdsInts ? (unsigned*)(dynLds + degOff) : nullptr;
     for (int i = tid; i < nb; i += BS) BC[i] = 0;
@@ -696,9 +739,9 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         if (eb >= 0) atomicAdd(&BC[eb - b0], 1);
     }
     __syncthreads();
-    for (int i = tid; i < nb; i += BS) d.boff[b0 + i] = BC[i];
+    for (int i = tid; i < nb; i += BS) d.boff[bb + i] = BC[i];
     __syncthreads();
-    block_scan_array<BS>(d.boff + b0, nb, sh);
+    block_scan_array<BS>(d.boff + bb, nb, sh);
     for (int i = tid; i < nb; i += BS) BC[i] = 0;
     __syncthreads();
     int* BL = d.blist + 2 * (size_t)so;
@@ -707,13 +750,13 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         int ea = d.seqA[so + k], eb = d.seqB[so + k];
         const unsigned long long kh = K[2 * k], kl = K[2 * k + 1];
         if (ea >= 0) {
-            const int x = d.boff[ea] + atomicAdd(&BC[ea - b0], 1);
+            const int x = d.boff[bb + ea - b0] + atomicAdd(&BC[ea - b0], 1);
             BL[x] = k << 1;
             BK[2 * x] = kh;
             BK[2 * x + 1] = kl;
         }
         if (eb >= 0) {
-            const int x = d.boff[eb] + atomicAdd(&BC[eb - b0], 1);
+            const int x = d.boff[bb + eb - b0] + atomicAdd(&BC[eb - b0], 1);
             BL[x] = (k << 1) | 1;
             BK[2 * x] = kh;
             BK[2 * x + 1] = kl;
@@ -724,14 +767,16 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     // path applies it in creation order, as the reference does, so that every body sums its
     // warm-start impulses in the same order; the FP32 path uses the solve order, which
     // changes only the rounding of those sums.
-    int* LS = d.lvlStart + so + w;
-    for (int k = tid; k < u; k += BS) {
+    int* LS = d.lvlStart + so + vid;
+    for (int k = tid; k < u1 - u0; k += BS) {
         PointCon<T>& pc = d.pc[u0 + k];
+        if (G > 1 && island_group(pc_active(d, pc) ? B.label[B.label[pc.a] >= 0 ? pc.a : pc.b] : -1, G) != grp) continue;
         pc.sleeping = B.sleeping[pc.a] && B.sleeping[pc.b];
         if (pc.enabled && !pc.sleeping) point_props(d, pc);
     }
     // the velocity records: the inverse inertia is that of the constraint set-up (same pose)
     for (int i = b0 + tid; i < b1; i += BS) {
+        if (!mine(i)) continue;
         VBody<T> r;
         r.v = B.v[i];
         r.w = B.w[i];
@@ -745,13 +790,16 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     constexpr bool kCreationOrderWarmStart = sizeof(T) == 8;
     int D = 0;
     if (kCreationOrderWarmStart && d.s.warmStarting) {
-        link_body_lists<T, BS>(d, b0, nb, so, BL, BK, BC, CreationLess<T>{&d, so});
+        link_body_lists<T, BS>(d, b0, bb, nb, so, BL, BK, BC, CreationLess<T>{&d, so});
         D = build_levels<T, BS>(d, so, L, LS, sTail, DEG);
         warm_start_levels<T, BS, false>(d, so, D, LS, ratio);
     }
-    link_body_lists<T, BS>(d, b0, nb, so, BL, BK, BC, SolveLessAt{BK});
+    link_body_lists<T, BS>(d, b0, bb, nb, so, BL, BK, BC, SolveLessAt{BK});
     D = build_levels<T, BS>(d, so, L, LS, sTail, DEG);
-    if (tid == 0) d.stats[2 * w] = D;
+    if (tid == 0) {
+        if (G > 1) atomicMax(&d.stats[2 * w], D);
+        else d.stats[2 * w] = D;
+    }
     // level starts in LDS when they fit
     const bool lsLds = D + 1 <= LS_LDS;
     if (lsLds)
@@ -828,6 +876,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         }
     }
     for (int i = b0 + tid; i < b1; i += BS) {
+        if (!mine(i)) continue;
         V3<T> v, w;
         vw_load(d, i, v, w);
         B.v[i] = v;
@@ -854,7 +903,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     for (int i = b0 + tid; i < b1; i += BS) {
         int f = B.flags[i];
         int mo = fl_motion(f);
-        if (mo == MOTION_STATIC || B.sleeping[i]) continue;
+        if (mo == MOTION_STATIC || B.sleeping[i] || !mine(i)) continue;
         V3<T> v = B.v[i], wv = B.w[i];
         if (mo == MOTION_DYNAMIC) {
             T ml = B.maxLinV[i], ma = B.maxAngV[i];
@@ -874,6 +923,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         set_transform(B, d.st, i);
     }
     for (int i = b0 + tid; i < b1; i += BS) {
+        if (!mine(i)) continue;
         d.isl0[i] = 1;
         d.isl1[i] = 0;
         PBody<T> r;
@@ -914,19 +964,24 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
             __syncthreads();
         }
         if (__syncthreads_or(anyActive)) sweeps = it + 1;
-        for (int i = b0 + tid; i < b1; i += BS) { d.isl0[i] = d.isl1[i]; d.isl1[i] = 0; }
+        for (int i = b0 + tid; i < b1; i += BS)
+            if (mine(i)) { d.isl0[i] = d.isl1[i]; d.isl1[i] = 0; }
         __syncthreads();
     }
     for (int i = b0 + tid; i < b1; i += BS) {
-        if (fl_motion(B.flags[i]) != MOTION_DYNAMIC) continue;
+        if (fl_motion(B.flags[i]) != MOTION_DYNAMIC || !mine(i)) continue;
         const PBody<T>& r = d.pb[i];
         B.com[i] = r.com;
         B.q[i] = r.q;
     }
-    if (tid == 0) d.stats[2 * w + 1] = sweeps;
+    if (tid == 0) {
+        if (G > 1) atomicMax(&d.stats[2 * w + 1], sweeps);
+        else d.stats[2 * w + 1] = sweeps;
+    }
     {  // byte model: per sweep 160 B/manifold + 192 B/point, per position sweep 240 B/manifold + 24 B/point
         int awake = 0;
-        for (int i = b0 + tid; i < b1; i += BS) awake += (fl_motion(B.flags[i]) == MOTION_DYNAMIC && !B.sleeping[i]) ? 1 : 0;
+        for (int i = b0 + tid; i < b1; i += BS)
+            awake += (fl_motion(B.flags[i]) == MOTION_DYNAMIC && !B.sleeping[i] && mine(i)) ? 1 : 0;
         for (int o = 32; o > 0; o >>= 1) awake += __shfl_xor(awake, o, 64);
         if ((tid & 63) == 0) atomicAdd(&d.acc[ACC_AWAKE], (unsigned long long)awake);
         if (tid == 0) {
@@ -938,7 +993,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     // ---- 10. positions from the centre of mass + AABBs (update.ts:322-330)
     for (int i = b0 + tid; i < b1; i += BS) {
         int mo = fl_motion(B.flags[i]);
-        if (mo == MOTION_STATIC || B.sleeping[i]) continue;
+        if (mo == MOTION_STATIC || B.sleeping[i] || !mine(i)) continue;
         pos_from_com(B, d.st, i);
         set_transform(B, d.st, i);
         d.isl0[i] = 1;  // island sleep flags
@@ -949,14 +1004,14 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         T maxMove = d.s.sleepThresh * d.s.tSleep;
         for (int i = b0 + tid; i < b1; i += BS) {
             int l = B.label[i];
-            if (l < 0 || fl_motion(B.flags[i]) != MOTION_DYNAMIC) continue;
+            if (l < 0 || fl_motion(B.flags[i]) != MOTION_DYNAMIC || !mine(i)) continue;
             bool can = update_sleep_state(B, d.st, i, dt, maxMove, d.s.tSleep);
             if (!can) d.isl0[l] = 0;
         }
         __syncthreads();
         for (int i = b0 + tid; i < b1; i += BS) {
             int l = B.label[i];
-            if (l < 0 || fl_motion(B.flags[i]) != MOTION_DYNAMIC) continue;
+            if (l < 0 || fl_motion(B.flags[i]) != MOTION_DYNAMIC || !mine(i)) continue;
             if (d.isl0[l]) {
                 B.sleeping[i] = 1;
                 B.v[i] = mk3<T>(0, 0, 0);

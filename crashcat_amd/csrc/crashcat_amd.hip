@@ -103,6 +103,29 @@ __global__ void k_count_epa(Dev<T> d) {
     }
 }
 
+// the solver records of a body as the world solve packs them (its island group writes them each
+// step; those of bodies outside every island, read by any group, are kept valid from here)
+template <typename T>
+__device__ inline void init_solver_records(Dev<T>& d, int i) {
+    const Bodies<T>& B = d.B;
+    VBody<T> r;
+    r.v = B.v[i];
+    r.w = B.w[i];
+    for (int j = 0; j < 9; j++) r.I.m[j] = T(0);
+    const int f = B.flags[i];
+    if (fl_motion(f) == MOTION_DYNAMIC) r.I = inv_inertia(B.invI[i], B.iRot[i], B.q[i], fl_dof(f));
+    r.invMass = B.invMass[i];
+    d.vb[i] = r;
+    PBody<T> p;
+    p.com = B.com[i];
+    p.q = B.q[i];
+    p.invI = B.invI[i];
+    p.iRot = B.iRot[i];
+    p.invMass = B.invMass[i];
+    p.label = B.label[i];
+    d.pb[i] = p;
+}
+
 // body initialisation at upload: COM, AABB, sleep test spheres (rigid-body.ts:390-434)
 template <typename T>
 __global__ void k_init_bodies(Dev<T> d) {
@@ -117,6 +140,7 @@ __global__ void k_init_bodies(Dev<T> d) {
     }
     d.B.wake[i] = 0;
     d.B.label[i] = -1;
+    init_solver_records(d, i);
 }
 
 template <typename T>
@@ -131,6 +155,7 @@ __global__ void k_refresh_bodies(Dev<T> d) {
         reset_sleep_timer(d.B, d.st, i);
     }
     d.B.wake[i] = 0;
+    if (fl_motion(d.B.flags[i]) == MOTION_STATIC) init_solver_records(d, i);
 }
 
 // ---------------------------------------------------------------- context
@@ -168,6 +193,7 @@ struct CtxT : Impl {
     int solverBS = 256;
     int device = 0;
     bool hasMesh = false;
+    bool anySplit = false;  // a world is solved as several island groups
     size_t worldLdsNeed = 0;   // ints of LDS the largest world can use
     size_t worldLdsBytes = 0;  // dynamic LDS given to the world kernel
     int worldLdsBS = 0;        // block size worldLdsBytes was sized for
@@ -200,7 +226,7 @@ static int fail(cc_ctx* c, int code, const std::string& msg) {
 extern "C" int cc_create(cc_ctx** out, const cc_config* cfg) {
     if (!out) return CC_E_ARG;
     cc_ctx* c = new cc_ctx();
-    cc_config def = {CC_PRECISION_FP32, 0, 16, 8, 0, 0};
+    cc_config def = {CC_PRECISION_FP32, 0, 16, 8, 0, 0, 0};
     c->cfg = cfg ? *cfg : def;
     if (c->cfg.max_pairs_per_body <= 0) c->cfg.max_pairs_per_body = 16;
     if (c->cfg.max_contacts_per_body <= 0) c->cfg.max_contacts_per_body = 8;
@@ -492,15 +518,46 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed");
     HIPCHK(c, hipMemcpy(dW, woff, (nw + 1) * sizeof(int), hipMemcpyHostToDevice));
     d.wStart = dW;
-    {  // the world solver's workgroups start on the largest worlds (longest solves first)
+    // the world solver's workgroups: worlds largest first (longest solves first); in a batch of
+    // fewer than 64 worlds each world is split into island groups of about 64 bodies, at most
+    // 256 / worlds of them, so that its independent islands fill more compute units
+    size_t splitExtra = 0;
+    {
+        std::vector<int> G(nw), vbase(nw), xbase(nw);
+        int V = 0;
+        for (int w = 0; w < nw; w++) {
+            const int n = woff[w + 1] - woff[w];
+            int g = 1;
+            if (nw < 64) g = std::max(1, std::min((n + 63) / 64, std::max(1, 256 / nw)));
+            if (c->cfg.island_groups > 0) g = std::min(g, (int)c->cfg.island_groups);
+            G[w] = g;
+            vbase[w] = V;
+            V += g;
+            xbase[w] = nb + (int)splitExtra;
+            if (g > 1) splitExtra += (size_t)(g - 1) * n;
+        }
         std::vector<int> order(nw);
         for (int w = 0; w < nw; w++) order[w] = w;
         std::stable_sort(order.begin(), order.end(),
                          [&](int x, int y) { return woff[x + 1] - woff[x] > woff[y + 1] - woff[y]; });
-        int* dO = A.get<int>(nw);
+        std::vector<int4> vt;
+        vt.reserve(V);
+        for (int w : order)
+            for (int g = 0; g < G[w]; g++)
+                vt.push_back(make_int4(w, g, G[w], g == 0 ? woff[w] : xbase[w] + (g - 1) * (woff[w + 1] - woff[w])));
+        int4* dV = A.get<int4>(std::max(V, 1));
+        int* dG = A.get<int>(nw);
+        int* dB = A.get<int>(nw);
+        d.vCnt = A.get<int>(2 * (size_t)std::max(V, 1));
         if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed");
-        HIPCHK(c, hipMemcpy(dO, order.data(), nw * sizeof(int), hipMemcpyHostToDevice));
-        d.worldOrder = dO;
+        HIPCHK(c, hipMemcpy(dV, vt.data(), V * sizeof(int4), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dG, G.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dB, vbase.data(), nw * sizeof(int), hipMemcpyHostToDevice));
+        d.vTab = dV;
+        d.V = V;
+        d.worldG = dG;
+        d.worldVBase = dB;
+        X->anySplit = V > nw;
     }
     // point constraints, grouped by world in index order
     std::vector<std::pair<int, int>> pcw;
@@ -619,7 +676,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.level = A.get<int>(L);
     d.sched = A.get<int>(L);
     d.elemNp = A.get<int>(L);
-    d.lvlStart = A.get<int>(2 * (L + nw + 1));
+    d.lvlStart = A.get<int>(2 * (L + d.V + 1));
     d.blist = A.get<int>(2 * L);
     d.blkey = A.get<unsigned long long>(4 * L);
     d.svCap = (int)L;
@@ -631,8 +688,8 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.svLB = A.get<V3<T>>(4 * L);
     d.vb = A.get<VBody<T>>(std::max(nb, 1));
     d.pb = A.get<PBody<T>>(std::max(nb, 1));
-    d.bcnt = A.get<int>(nb);
-    d.boff = A.get<int>(nb);
+    d.bcnt = A.get<int>((size_t)nb + splitExtra);
+    d.boff = A.get<int>((size_t)nb + splitExtra);
     d.isl0 = A.get<int>(nb);
     d.isl1 = A.get<int>(nb);
     d.stats = A.get<int>(2 * (size_t)nw);
@@ -778,7 +835,11 @@ static void launch_world_bs(CtxT<T>* X, hipStream_t st, T dt) {
         X->worldLdsBS = BS;
     }
     d.worldLdsInts = (int)(X->worldLdsBytes / 4);
-    hipLaunchKernelGGL((k_world<T, BS>), dim3(d.W), dim3(BS), X->worldLdsBytes, st, d, dt);
+    if (X->anySplit) {
+        hipLaunchKernelGGL(k_group_zero<T>, dim3(grid_for(std::max(d.V, d.W), 256, 1024)), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_group_count<T>, dim3(grid_for((long long)d.conCap + d.NPC, 256, 4096)), dim3(256), 0, st, d);
+    }
+    hipLaunchKernelGGL((k_world<T, BS>), dim3(d.V), dim3(BS), X->worldLdsBytes, st, d, dt);
 }
 
 template <typename T>

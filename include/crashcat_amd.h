@@ -251,6 +251,7 @@ typedef struct cc_config {
     int32_t max_contacts_per_body; /* capacity factor for contact constraints (default 12) */
     int32_t block_solver;       /* threads per solver workgroup (0 = autotuned)     */
     int32_t block_bodies;       /* threads per body-parallel workgroup (0 = autotuned) */
+    int32_t island_groups;      /* most island groups a world is solved as (0 = by size, 1 = never split) */
 } cc_config;
 
 typedef struct cc_ctx cc_ctx;
