@@ -1171,11 +1171,25 @@ __global__ void k_uf_link(Dev<T> d) {
     }
 }
 
+// the root of x without writing any parent: when every thread stores its own body's root at
+// once, a path-halving store of another thread could land after that store and leave an
+// intermediate ancestor as the final label (the island would then carry two labels)
+__device__ inline int uf_root(const int* p, int x) {
+    for (;;) {
+        const int y = __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (y == x) return x;
+        x = y;
+    }
+}
+
 template <typename T>
 __global__ void k_uf_compress(Dev<T> d) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.N) return;
-    if (d.B.label[i] >= 0) d.B.label[i] = uf_find(d.B.label, i);
+    if (d.B.label[i] >= 0) {
+        const int r = uf_root(d.B.label, i);
+        __hip_atomic_store(&d.B.label[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // --------------------------------------------------------- contact cache hash

@@ -395,3 +395,30 @@ def test_island_groups_change_nothing(prec):
     _assert_bodies_equal(b1, b2, prec)
     assert np.array_equal(p1, p2)
     _assert_contacts_equal(c1, c2, prec)
+
+
+def test_island_labels_are_roots_and_runs_repeat():
+    """Every island label is its island's lowest body index (label[label[i]] == label[i] and no
+    body of the island is lower), step after step, and two contexts stepping the same 20,000-body
+    world as island groups stay bit-identical.  Regression test: a concurrent path-halving store
+    once left an intermediate ancestor as a body's final label, splitting an island in two."""
+    sc = SC.replicate_world(SC.mixed_pile(seed=1002, n=20000), 1, 1002)
+    a = Context("fp64", max_pairs_per_body=24, max_contacts_per_body=6)
+    b = Context("fp64", max_pairs_per_body=24, max_contacts_per_body=6)
+    a.upload(sc)
+    b.upload(sc)
+    try:
+        for s in range(1, 31):
+            a.step(DT, 1)
+            b.step(DT, 1)
+            x, y = a.bodies(), b.bodies()
+            lab = x["island"]
+            act = lab >= 0
+            assert np.all(lab[lab[act]] == lab[act]), f"step {s}: a label is not a root"
+            idx = np.arange(len(lab))
+            assert np.all(lab[act] <= idx[act]), f"step {s}: a label above its body"
+            _assert_bodies_equal(x, y, f"step {s}")
+            assert np.array_equal(x["island"], y["island"])
+    finally:
+        a.close()
+        b.close()
