@@ -348,6 +348,9 @@ def main():
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": traffic,
             "traffic_workload": traffic_workload,
+            # the same launch by the calibrated FETCH_SIZE + WRITE_SIZE counters (bytes moved)
+            "traffic_GBps": (traffic / (dom_ms * 1e-3) / 1e9) if traffic else None,
+            "traffic_frac": (traffic / (dom_ms * 1e-3) / 1e9 / PEAK_HBM_GBS) if traffic else None,
         },
         "model_bytes_per_body_step": sum(pb.values()) / max(1, nbody * args.steps),
         "cpu_baseline": None if cpu is None else {
