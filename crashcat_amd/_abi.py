@@ -156,7 +156,7 @@ class ContactOut(C.Structure):
         ("body_a", C.c_int32),
         ("body_b", C.c_int32),
         ("num_points", C.c_int32),
-        ("pad_", C.c_int32),
+        ("persisted", C.c_int32),
         ("sub_shape_a", C.c_uint32),
         ("sub_shape_b", C.c_uint32),
         ("normal", D3),
@@ -165,6 +165,15 @@ class ContactOut(C.Structure):
         ("lambda_n", D4),
         ("lambda_t1", D4),
         ("lambda_t2", D4),
+    ]
+
+
+class ContactRemoved(C.Structure):
+    _fields_ = [
+        ("body_a", C.c_int32),
+        ("body_b", C.c_int32),
+        ("sub_shape_a", C.c_uint32),
+        ("sub_shape_b", C.c_uint32),
     ]
 
 

@@ -14,7 +14,7 @@ _LIB = None
 SYMBOLS = [
     "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
-    "cc_get_phase_times", "cc_get_totals", "cc_build_triangle_mesh",
+    "cc_get_phase_times", "cc_get_totals", "cc_build_triangle_mesh", "cc_download_removed_contacts",
 ]
 
 
@@ -46,6 +46,7 @@ def lib():
     L.cc_set_bodies.argtypes = [vp, C.POINTER(_abi.BodyState), C.c_int32]
     L.cc_download_pairs.argtypes = [vp, ip, C.c_int32, ip]
     L.cc_download_contacts.argtypes = [vp, C.POINTER(_abi.ContactOut), C.c_int32, ip]
+    L.cc_download_removed_contacts.argtypes = [vp, C.POINTER(_abi.ContactRemoved), C.c_int32, ip]
     L.cc_get_counters.argtypes = [vp, C.POINTER(_abi.StepCounters)]
     fp = C.POINTER(C.c_float)
     L.cc_broadphase_aabbs.argtypes = [vp, fp, fp, ip, ip, ip, C.c_int32, C.c_double, ip, C.c_int32, ip]
@@ -59,6 +60,7 @@ def lib():
 
 BODY_STATE_DTYPE = np.dtype(_abi.BodyState)
 CONTACT_DTYPE = np.dtype(_abi.ContactOut)
+REMOVED_DTYPE = np.dtype(_abi.ContactRemoved)
 PHASES = ["broadphase", "narrowphase", "epa", "manifold", "islands", "solver", "cache"]
 
 
@@ -141,6 +143,16 @@ class Context:
         out = np.zeros(max(cnt.value, 1), dtype=CONTACT_DTYPE)
         self._check(lib().cc_download_contacts(self.h, out.ctypes.data_as(C.POINTER(_abi.ContactOut)), cnt.value,
                                                C.byref(cnt)))
+        return out[:cnt.value]
+
+    def removed_contacts(self):
+        """Contacts of the step before the last that the last step did not make again
+        (onContactRemoved, src/contacts.ts:297-307), sorted by (body_a, body_b, sub ids)."""
+        cnt = C.c_int32()
+        self._check(lib().cc_download_removed_contacts(self.h, None, 0, C.byref(cnt)))
+        out = np.zeros(max(cnt.value, 1), dtype=REMOVED_DTYPE)
+        self._check(lib().cc_download_removed_contacts(self.h, out.ctypes.data_as(C.POINTER(_abi.ContactRemoved)),
+                                                       cnt.value, C.byref(cnt)))
         return out[:cnt.value]
 
     def counters(self):

@@ -110,6 +110,8 @@ struct Dev {
     int* wConStart;     // [W + 1] range of each world in conIdx
     int* conIdx;        // [conCap] constraint slots in pair order (so grouped by world)
     CacheRec<T>* rec[2];          // [conCap] per parity
+    int* recHit[2];               // [conCap] per parity: 0 when written, 1 once the next step matched it
+    int* recCnt;                  // [2] records written per parity
     unsigned long long* hkeys[2]; // [hashCap]
     int* hvals[2];
     // narrowphase hits awaiting their manifold: pair slot, kind (HIT_*), and the closest points
@@ -672,8 +674,11 @@ __device__ inline void make_contact(const Dev<T>& d, int slot, int A, int Bi, co
             h = (h + 1) & (unsigned)(d.hashCap - 1);
         }
     }
-    if (prev >= 0) rc = d.rec[par ^ 1][prev];
-    else {
+    d.recHit[par][cs] = 0;
+    if (prev >= 0) {
+        rc = d.rec[par ^ 1][prev];
+        d.recHit[par ^ 1][prev] = 1;  // processedThisFrame (src/contacts.ts:468)
+    } else {
         rc.np = 0;
         for (int j = 0; j < 4; j++) { rc.p1[j] = rc.p2[j] = mk3<T>(0, 0, 0); rc.ln[j] = rc.lt1[j] = rc.lt2[j] = T(0); }
     }
@@ -752,7 +757,8 @@ __device__ inline void make_contact(const Dev<T>& d, int slot, int A, int Bi, co
     hd.a = ia;
     hd.b = ib;
     hd.sortKey = k.sortKey;
-    hd.bits = k.dynMask | (k.movMask << 2) | (k.np << 4) | (runIdx > 0 ? 128 : ((runCnt - 1) << 8));
+    hd.bits = k.dynMask | (k.movMask << 2) | (k.np << 4) | (runIdx > 0 ? 128 : ((runCnt - 1) << 8)) |
+              (prev >= 0 ? 8192 : 0);
     d.conHead[cs] = hd;
     if (dynA && DA.sleeping) B.wake[ia] = 1;
     if (dynB && DB.sleeping) B.wake[ib] = 1;
@@ -1201,6 +1207,7 @@ __global__ void k_hash_insert(Dev<T> d) {
     int par = *d.parity;
     unsigned long long* hk = d.hkeys[par];
     int* hv = d.hvals[par];
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.recCnt[par] = nc;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         int slot = k;
         const ConHead c = d.conHead[slot];
@@ -1212,6 +1219,20 @@ __global__ void k_hash_insert(Dev<T> d) {
             if (old == CC_EMPTY_KEY || old == key) { hv[h] = slot; break; }
             h = (h + 1) & (unsigned)(d.hashCap - 1);
         }
+    }
+}
+
+// The previous step's records that the last step did not match (destroyUnprocessedContacts,
+// src/contacts.ts:531-555), as (a, b, subA, subB) at an atomic position; q is the parity that
+// holds them.  The host sorts the list.
+template <typename T>
+__global__ void k_removed_contacts(Dev<T> d, int q, int4* out, int cap, int* count) {
+    const int n = d.recCnt[q];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (d.recHit[q][i]) continue;
+        const CacheRec<T>& r = d.rec[q][i];
+        const int k = atomicAdd(count, 1);
+        if (k < cap) out[k] = make_int4(r.a, r.b, (int)r.subA, (int)r.subB);
     }
 }
 

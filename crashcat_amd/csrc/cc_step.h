@@ -89,7 +89,7 @@ struct ConHead {
     int a, b;
     unsigned sortKey;
     int bits;  // bits 0-1 dynamic A/B, 2-3 non-static A/B, 4-6 points, 7 not the first of its pair's
-               // run, 8-12 (first only) the run's length - 1
+               // run, 8-12 (first only) the run's length - 1, 13 found in the previous step's cache
 };
 // Persistent contact cache entry (src/contacts.ts:9-58): four in-place slots.
 template <typename T>

@@ -648,9 +648,11 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.conIdx = A.get<int>(d.conCap);
     for (int k = 0; k < 2; k++) {
         d.rec[k] = A.get<CacheRec<T>>(d.conCap);
+        d.recHit[k] = A.get<int>(d.conCap);
         d.hkeys[k] = A.get<unsigned long long>(d.hashCap);
         d.hvals[k] = A.get<int>(d.hashCap);
     }
+    d.recCnt = A.get<int>(2);
     d.gjkList = A.get<int>(d.pairCap);
     d.gjkCombo = A.get<int>(d.pairCap);
     d.gjkSorted = A.get<int>(d.pairCap);
@@ -715,6 +717,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     HIPCHK(c, hipMemset(d.acc, 0, ACC_N * sizeof(unsigned long long)));
     HIPCHK(c, hipMemset(d.parity, 0, sizeof(int)));
     for (int k = 0; k < 2; k++) HIPCHK(c, hipMemset(d.hkeys[k], 0xff, d.hashCap * sizeof(unsigned long long)));
+    HIPCHK(c, hipMemset(d.recCnt, 0, 2 * sizeof(int)));
 
     HIPCHK(c, hipMemset(d.stats, 0, 2 * nw * sizeof(int)));
     int blocks = (nb + 255) / 256;
@@ -1084,10 +1087,12 @@ static int contacts_impl(cc_ctx* c, cc_contact_out* out, int cap, int* count) {
     std::vector<Con<T>> hcs(n);
     std::vector<ConTap<T>> hts(n);
     std::vector<CacheRec<T>> hrs(n);
+    std::vector<ConHead> hhs(n);
     if (n > 0) {
         int par = 0;
         HIPCHK(c, hipMemcpy(&par, d.parity, sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(hcs.data(), d.con, n * sizeof(Con<T>), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(hhs.data(), d.conHead, n * sizeof(ConHead), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(hts.data(), d.tap, n * sizeof(ConTap<T>), hipMemcpyDeviceToHost));
         // the last step wrote its impulses to the records of the parity it ran with
         HIPCHK(c, hipMemcpy(hrs.data(), d.rec[par ^ 1], n * sizeof(CacheRec<T>), hipMemcpyDeviceToHost));
@@ -1099,6 +1104,7 @@ static int contacts_impl(cc_ctx* c, cc_contact_out* out, int cap, int* count) {
         cc_contact_out o;
         memset(&o, 0, sizeof(o));
         o.body_a = h.a; o.body_b = h.b; o.num_points = h.np;
+        o.persisted = (hhs[idx[k]].bits >> 13) & 1;
         o.sub_shape_a = r.subA; o.sub_shape_b = r.subB;
         o.normal[0] = h.normal.x; o.normal[1] = h.normal.y; o.normal[2] = h.normal.z;
         for (int j = 0; j < h.np && j < 4; j++) {
@@ -1116,6 +1122,61 @@ static int contacts_impl(cc_ctx* c, cc_contact_out* out, int cap, int* count) {
     });
     for (int k = 0; k < n && k < cap; k++) out[k] = res[k];
     return CC_OK;
+}
+
+template <typename T>
+static int removed_impl(cc_ctx* c, cc_contact_removed* out, int cap, int* count) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int q = 0, rc[2] = {0, 0};
+    HIPCHK(c, hipMemcpy(&q, d.parity, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(rc, d.recCnt, sizeof(rc), hipMemcpyDeviceToHost));
+    // the step before the last wrote parity q (the last step wrote q ^ 1 and flipped to q)
+    const int n = std::min(rc[q], d.conCap);
+    *count = 0;
+    if (n <= 0) return CC_OK;
+    int4* buf = nullptr;
+    int* cnt = nullptr;
+    HIPCHK(c, hipMalloc(&buf, (size_t)n * sizeof(int4)));
+    if (hipMalloc(&cnt, sizeof(int)) != hipSuccess) { (void)hipFree(buf); return fail(c, CC_E_HIP, "hipMalloc failed"); }
+    int m = 0;
+    std::vector<int4> h;
+    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(int), c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_removed_contacts<T>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->stream, d, q, buf, n,
+                           cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&m, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && m > 0) {
+        h.resize(m);
+        e = hipMemcpy(h.data(), buf, (size_t)m * sizeof(int4), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(buf);
+    (void)hipFree(cnt);
+    if (e != hipSuccess) return fail(c, CC_E_HIP, hipGetErrorString(e));
+    std::sort(h.begin(), h.end(), [](const int4& x, const int4& y) {
+        if (x.x != y.x) return x.x < y.x;
+        if (x.y != y.y) return x.y < y.y;
+        if ((unsigned)x.z != (unsigned)y.z) return (unsigned)x.z < (unsigned)y.z;
+        return (unsigned)x.w < (unsigned)y.w;
+    });
+    *count = m;
+    for (int i = 0; i < m && i < cap && out; i++) {
+        out[i].body_a = h[i].x; out[i].body_b = h[i].y;
+        out[i].sub_shape_a = (uint32_t)h[i].z; out[i].sub_shape_b = (uint32_t)h[i].w;
+    }
+    return CC_OK;
+}
+
+extern "C" int cc_download_removed_contacts(cc_ctx* c, cc_contact_removed* out, int32_t cap, int32_t* count) {
+    if (!c || !c->impl || !count) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (cap < 0) return fail(c, CC_E_ARG, "negative capacity");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return removed_impl<double>(c, out, cap, count);
+    return removed_impl<float>(c, out, cap, count);
 }
 
 extern "C" int cc_download_contacts(cc_ctx* c, cc_contact_out* out, int32_t cap, int32_t* count) {

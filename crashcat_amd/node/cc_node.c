@@ -153,6 +153,18 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_step_counters, num_islands, "i64", 1); F(cc_step_counters, max_levels, "i64", 1);
     F(cc_step_counters, position_sweeps, "i64", 1); F(cc_step_counters, overflow, "i64", 1);
     add_struct(env, out, "cc_step_counters", sizeof(cc_step_counters), fields);
+    napi_create_object(env, &fields);
+    F(cc_contact_out, body_a, "i32", 1); F(cc_contact_out, body_b, "i32", 1);
+    F(cc_contact_out, num_points, "i32", 1); F(cc_contact_out, persisted, "i32", 1);
+    F(cc_contact_out, sub_shape_a, "u32", 1); F(cc_contact_out, sub_shape_b, "u32", 1);
+    F(cc_contact_out, normal, "f64", 3); F(cc_contact_out, position_a, "f64", 12);
+    F(cc_contact_out, position_b, "f64", 12); F(cc_contact_out, lambda_n, "f64", 4);
+    F(cc_contact_out, lambda_t1, "f64", 4); F(cc_contact_out, lambda_t2, "f64", 4);
+    add_struct(env, out, "cc_contact_out", sizeof(cc_contact_out), fields);
+    napi_create_object(env, &fields);
+    F(cc_contact_removed, body_a, "i32", 1); F(cc_contact_removed, body_b, "i32", 1);
+    F(cc_contact_removed, sub_shape_a, "u32", 1); F(cc_contact_removed, sub_shape_b, "u32", 1);
+    add_struct(env, out, "cc_contact_removed", sizeof(cc_contact_removed), fields);
     return out;
 }
 #undef F
@@ -341,6 +353,31 @@ static napi_value download_pairs(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+/* ---- downloadContacts(handle) / downloadRemovedContacts(handle) -> ArrayBuffer of records --- */
+static napi_value download_records(napi_env env, napi_callback_info info, int removed) {
+    size_t argc = 1;
+    napi_value a[1];
+    CHECK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+    cc_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    int32_t count = 0;
+    int rc = removed ? cc_download_removed_contacts(c, NULL, 0, &count) : cc_download_contacts(c, NULL, 0, &count);
+    if (rc != CC_OK) return throw_rc(env, c, rc);
+    const size_t rec = removed ? sizeof(cc_contact_removed) : sizeof(cc_contact_out);
+    void* data;
+    napi_value ab;
+    CHECK(env, napi_create_arraybuffer(env, (size_t)count * rec, &data, &ab));
+    if (count > 0) {
+        int32_t got = 0;
+        rc = removed ? cc_download_removed_contacts(c, (cc_contact_removed*)data, count, &got)
+                     : cc_download_contacts(c, (cc_contact_out*)data, count, &got);
+        if (rc != CC_OK) return throw_rc(env, c, rc);
+    }
+    return ab;
+}
+static napi_value download_contacts(napi_env env, napi_callback_info info) { return download_records(env, info, 0); }
+static napi_value download_removed(napi_env env, napi_callback_info info) { return download_records(env, info, 1); }
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor d[] = {
         {"layout", 0, layout, 0, 0, 0, napi_default, 0},
@@ -352,6 +389,8 @@ static napi_value init(napi_env env, napi_value exports) {
         {"setBodies", 0, set_bodies, 0, 0, 0, napi_default, 0},
         {"counters", 0, counters, 0, 0, 0, napi_default, 0},
         {"downloadPairs", 0, download_pairs, 0, 0, 0, napi_default, 0},
+        {"downloadContacts", 0, download_contacts, 0, 0, 0, napi_default, 0},
+        {"downloadRemovedContacts", 0, download_removed, 0, 0, 0, napi_default, 0},
     };
     napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
     return exports;

@@ -79,10 +79,29 @@ function unpack(structName, buf) {
   return out;
 }
 
-function hasCallbacks(listener) {
-  if (listener === undefined || listener === null) return false;
-  for (const k of Object.keys(listener)) if (typeof listener[k] === 'function') return true;
-  return false;
+// Listener callbacks (src/listener.ts:23-99).  The validate callbacks decide what the step
+// does and cannot run inside a GPU step; the contact callbacks are delivered after the step
+// from the event records the step leaves (cc_download_contacts, cc_download_removed_contacts).
+const STEP_CALLBACKS = ['onBodyPairValidate', 'onContactValidate'];
+const EVENT_CALLBACKS = ['onContactAdded', 'onContactPersisted', 'onContactRemoved'];
+
+function callbackNames(listener) {
+  if (listener === undefined || listener === null) return [];
+  return Object.keys(listener).filter((k) => typeof listener[k] === 'function');
+}
+
+// The manifold handed to onContactAdded / onContactPersisted: world-space normal (A -> B),
+// contact points on A and B in world space, sub-shape ids; bodies ordered id A < id B.
+function eventManifold(c) {
+  const n = c.num_points;
+  return {
+    worldSpaceNormal: c.normal.slice(),
+    numContactPoints: n,
+    contactPointsOnA: c.position_a.slice(0, 3 * n),
+    contactPointsOnB: c.position_b.slice(0, 3 * n),
+    subShapeIdA: c.sub_shape_a,
+    subShapeIdB: c.sub_shape_b,
+  };
 }
 
 // One GPU context holding a batch of independent worlds.
@@ -143,6 +162,27 @@ class GpuWorld {
     addon.setBodies(this.handle, buf);
   }
   pairs() { return addon.downloadPairs(this.handle); }
+  // contacts of the last step (cc_contact_out records, `persisted` set for those the step
+  // before also had) and the contacts the last step removed (cc_contact_removed records)
+  contacts() { return unpack('cc_contact_out', addon.downloadContacts(this.handle)); }
+  removedContacts() { return unpack('cc_contact_removed', addon.downloadRemovedContacts(this.handle)); }
+
+  // Deliver the last step's contact events to a listener: onContactAdded / onContactPersisted
+  // (src/constraints/contact-constraints.ts:493-499) per contact in (body a, body b, sub-shape)
+  // order, then onContactRemoved (src/contacts.ts:305-307) per removed contact.  Bodies are
+  // passed as their indices; the step has already run, so the settings object is informative.
+  dispatchContactEvents(listener) {
+    const added = listener.onContactAdded, persisted = listener.onContactPersisted;
+    if (typeof added === 'function' || typeof persisted === 'function') {
+      for (const c of this.contacts()) {
+        const cb = c.persisted ? persisted : added;
+        if (typeof cb === 'function') cb(c.body_a, c.body_b, eventManifold(c), { isSensor: false });
+      }
+    }
+    if (typeof listener.onContactRemoved === 'function') {
+      for (const r of this.removedContacts()) listener.onContactRemoved(r.body_a, r.body_b, r.sub_shape_a, r.sub_shape_b);
+    }
+  }
   counters() {
     const buf = new ArrayBuffer(LAYOUT.cc_step_counters.size);
     addon.counters(this.handle, buf);
@@ -153,12 +193,15 @@ class GpuWorld {
 function createWorld(desc, opts) { return new GpuWorld(desc, opts); }
 
 // updateWorld(world, listener, timeStep) (src/update.ts:45): one step of every world of the
-// context.  Listener callbacks cannot run inside a GPU step, so a listener with callbacks is
-// refused, as the C ABI documents.
+// context.  The contact callbacks of the listener are called after the step from its event
+// records; a validate callback, which would have to run inside the GPU step, is refused.
 function updateWorld(world, listener, timeStep) {
   if (!(world instanceof GpuWorld)) throw new Error('crashcat_amd: updateWorld needs a world made by createWorld');
-  if (hasCallbacks(listener)) throw new Error('crashcat_amd: listener callbacks are not available on the GPU step');
+  const names = callbackNames(listener);
+  const refused = names.filter((k) => STEP_CALLBACKS.includes(k) || !EVENT_CALLBACKS.includes(k));
+  if (refused.length) throw new Error(`crashcat_amd: listener callbacks not available on the GPU step: ${refused.join(', ')}`);
   world.step(timeStep, 1);
+  if (names.length) world.dispatchContactEvents(listener);
 }
 
 module.exports = { createWorld, updateWorld, GpuWorld, pack, unpack, layout: LAYOUT, addon };

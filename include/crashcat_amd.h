@@ -207,7 +207,9 @@ typedef struct cc_contact_out {
     int32_t body_a;             /* lower id of the pair (src/constraints/contact-constraints.ts:448-453) */
     int32_t body_b;
     int32_t num_points;
-    int32_t pad_;
+    int32_t persisted;          /* 1: the step before had this (a, b, sub a, sub b) contact, the
+                                   onContactPersisted case; 0: a new contact, onContactAdded
+                                   (src/constraints/contact-constraints.ts:493-499) */
     uint32_t sub_shape_a;       /* sub-shape ids (src/body/sub-shape.ts); 0xffffffff when empty */
     uint32_t sub_shape_b;
     double normal[3];           /* world space, A -> B                              */
@@ -217,6 +219,15 @@ typedef struct cc_contact_out {
     double lambda_t1[4];
     double lambda_t2[4];
 } cc_contact_out;
+
+/* A contact of the step before the last that the last step did not make again: the
+ * arguments of onContactRemoved (src/listener.ts:84-98), ids sorted body_a < body_b. */
+typedef struct cc_contact_removed {
+    int32_t body_a;
+    int32_t body_b;
+    uint32_t sub_shape_a;
+    uint32_t sub_shape_b;
+} cc_contact_removed;
 
 /* Per-step counters used by the byte model. */
 typedef struct cc_step_counters {
@@ -309,8 +320,16 @@ int cc_set_bodies(cc_ctx* ctx, const cc_body_state* in, int32_t num_bodies);
 /* Broadphase pairs as (a, b) global body indices, a < b, sorted ascending;
  * replaces broadphase.findCollidingPairs (src/broadphase/broadphase.ts:217-277). */
 int cc_download_pairs(cc_ctx* ctx, int32_t* out_pairs, int32_t capacity, int32_t* out_count);
-/* Contact constraints sorted by (body_a, body_b). */
+/* Contact constraints sorted by (body_a, body_b, sub_shape_a, sub_shape_b).  With `persisted`
+ * they are the post-step form of onContactAdded / onContactPersisted (called from
+ * addContactConstraint, src/constraints/contact-constraints.ts:493-499). */
 int cc_download_contacts(cc_ctx* ctx, cc_contact_out* out, int32_t capacity, int32_t* out_count);
+/* Contacts removed by the last step, sorted by (body_a, body_b, sub_shape_a, sub_shape_b):
+ * the post-step form of onContactRemoved, which destroyContact (src/contacts.ts:297-307) calls
+ * for every contact that destroyStaleContactsBetweenBodies (:404-430) and
+ * destroyUnprocessedContacts (:531-555) find unprocessed.  *out_count is the number removed;
+ * at most `capacity` are written.  Call cc_step with one step to see every step's events. */
+int cc_download_removed_contacts(cc_ctx* ctx, cc_contact_removed* out, int32_t capacity, int32_t* out_count);
 int cc_get_counters(cc_ctx* ctx, cc_step_counters* out);
 
 /* Broadphase pair set on caller-supplied AABBs (one world), without stepping; the

@@ -39,6 +39,8 @@ void oc_set_bodies(oc_world* w, const cc_body_state* in);
 int32_t oc_get_pairs(const oc_world* w, int32_t* out, int32_t cap);
 /* contact constraints of the last step, sorted by (a, b, sub-shape a, sub-shape b) */
 int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap);
+/* contacts the last step removed (destroyUnprocessedContacts), sorted like the contacts */
+int32_t oc_get_removed_contacts(const oc_world* w, cc_contact_removed* out, int32_t cap);
 void oc_get_counters(const oc_world* w, cc_step_counters* out);
 /* broadphase only, on given AABBs (widened from float) */
 int32_t oc_broadphase_aabbs(const float* mn, const float* mx, const int32_t* motion, const int32_t* sleeping,

@@ -40,6 +40,8 @@ def lib():
         L.oc_get_pairs.argtypes = [C.c_void_p, ip, C.c_int32]
         L.oc_get_contacts.restype = C.c_int32
         L.oc_get_contacts.argtypes = [C.c_void_p, C.POINTER(_abi.ContactOut), C.c_int32]
+        L.oc_get_removed_contacts.restype = C.c_int32
+        L.oc_get_removed_contacts.argtypes = [C.c_void_p, C.POINTER(_abi.ContactRemoved), C.c_int32]
         L.oc_get_counters.argtypes = [C.c_void_p, C.POINTER(_abi.StepCounters)]
         L.oc_broadphase_aabbs.restype = C.c_int32
         L.oc_broadphase_aabbs.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), ip, ip, ip, C.c_int32,
@@ -127,6 +129,12 @@ class OracleWorld:
         n = lib().oc_get_contacts(self.h, None, 0)
         out = np.zeros(max(n, 1), dtype=CONTACT_DTYPE)
         lib().oc_get_contacts(self.h, out.ctypes.data_as(C.POINTER(_abi.ContactOut)), n)
+        return out[:n]
+
+    def removed_contacts(self):
+        n = lib().oc_get_removed_contacts(self.h, None, 0)
+        out = np.zeros(max(n, 1), dtype=np.dtype(_abi.ContactRemoved))
+        lib().oc_get_removed_contacts(self.h, out.ctypes.data_as(C.POINTER(_abi.ContactRemoved)), n)
         return out[:n]
 
     def counters(self):

@@ -55,6 +55,7 @@ typedef struct {
     int np;
     OCPoint pts[4];
     uint32_t sortKey;
+    int persisted; /* the contact existed before this step (onContactPersisted, else onContactAdded) */
 } OCC;
 
 typedef struct {
@@ -94,6 +95,8 @@ struct oc_world {
     /* contacts cache */
     OContact* ct; int nct, cct;
     int* htab; int hcap;
+    /* contacts removed by the last step: (a, b, subA, subB) each */
+    cc_contact_removed* rem; int nrem, crem;
     /* constraints */
     OCC* cc; int ncc, ccc;
     /* user constraints */
@@ -345,7 +348,7 @@ oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb
 void oc_destroy(oc_world* w) {
     if (!w) return;
     free(w->shapes); free(w->hullBuf); free(w->hullIdx); free(w->meshBuf); free(w->meshIdx); free(w->b); free(w->active);
-    free(w->pairs); free(w->ct); free(w->htab); free(w->cc); free(w->pc);
+    free(w->pairs); free(w->ct); free(w->htab); free(w->rem); free(w->cc); free(w->pc);
     free(w->bodyLinks); free(w->bodyIslands); free(w->contactLinks); free(w->consLinks); free(w->consIds);
     for (int i = 0; i < w->cisl; i++) { free(w->isl[i].bodies); free(w->isl[i].contacts); free(w->isl[i].cons); }
     free(w->isl);
@@ -544,6 +547,7 @@ static int add_contact_constraint(oc_world* w, int ia, int ib, OManifold* m, uin
         memset(k, 0, sizeof(*k));
         k->a = ia; k->b = ib; k->contact = ci;
         k->subA = subA; k->subB = subB;
+        k->persisted = existing;
         k->sortKey = sort_key((uint32_t)ia, (uint32_t)ib, subA, subB);
         V3 n = m->normal;
         k->normal = n;
@@ -1087,9 +1091,19 @@ void oc_step(oc_world* w, double dt) {
     {
         int* remap = (int*)xrealloc(NULL, sizeof(int) * (size_t)(w->nct ? w->nct : 1));
         int k = 0;
+        w->nrem = 0;
         for (int i = 0; i < w->nct; i++) {
             if (w->ct[i].processed) { remap[i] = k; w->ct[k++] = w->ct[i]; }
-            else remap[i] = -1;
+            else { /* destroyContact :297-307 reports it to onContactRemoved */
+                remap[i] = -1;
+                if (w->nrem == w->crem) {
+                    w->crem = w->crem ? 2 * w->crem : 256;
+                    w->rem = (cc_contact_removed*)xrealloc(w->rem, sizeof(cc_contact_removed) * (size_t)w->crem);
+                }
+                cc_contact_removed* r = &w->rem[w->nrem++];
+                r->body_a = w->ct[i].a; r->body_b = w->ct[i].b;
+                r->sub_shape_a = w->ct[i].sa; r->sub_shape_b = w->ct[i].sb;
+            }
         }
         w->nct = k;
         for (int i = 0; i < w->ncc; i++) w->cc[i].contact = remap[w->cc[i].contact];
@@ -1298,6 +1312,7 @@ int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap) {
         cc_contact_out* o = &out[i];
         memset(o, 0, sizeof(*o));
         o->body_a = k->a; o->body_b = k->b; o->num_points = k->np;
+        o->persisted = k->persisted;
         o->sub_shape_a = k->subA; o->sub_shape_b = k->subB;
         vstore(o->normal, k->normal);
         for (int j = 0; j < k->np; j++) {
@@ -1310,6 +1325,28 @@ int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap) {
     }
     qsort(out, (size_t)n, sizeof(cc_contact_out), cmp_cout);
     return w->ncc;
+}
+
+static int cmp_removed(const void* x, const void* y) {
+    const cc_contact_removed* a = (const cc_contact_removed*)x;
+    const cc_contact_removed* b = (const cc_contact_removed*)y;
+    if (a->body_a != b->body_a) return a->body_a < b->body_a ? -1 : 1;
+    if (a->body_b != b->body_b) return a->body_b < b->body_b ? -1 : 1;
+    if (a->sub_shape_a != b->sub_shape_a) return a->sub_shape_a < b->sub_shape_a ? -1 : 1;
+    if (a->sub_shape_b != b->sub_shape_b) return a->sub_shape_b < b->sub_shape_b ? -1 : 1;
+    return 0;
+}
+
+int32_t oc_get_removed_contacts(const oc_world* w, cc_contact_removed* out, int32_t cap) {
+    int n = w->nrem < cap ? w->nrem : cap;
+    if (n > 0) {
+        cc_contact_removed* t = (cc_contact_removed*)xrealloc(NULL, sizeof(cc_contact_removed) * (size_t)w->nrem);
+        memcpy(t, w->rem, sizeof(cc_contact_removed) * (size_t)w->nrem);
+        qsort(t, (size_t)w->nrem, sizeof(cc_contact_removed), cmp_removed);
+        memcpy(out, t, sizeof(cc_contact_removed) * (size_t)n);
+        free(t);
+    }
+    return w->nrem;
 }
 
 void oc_get_counters(const oc_world* w, cc_step_counters* out) { *out = w->ctr; }

@@ -21,6 +21,7 @@ from crashcat_amd._lib import BODY_STATE_DTYPE
 HERE = os.path.dirname(os.path.abspath(__file__))
 ADDON = os.path.join(HERE, "..", "crashcat_amd", "node", "crashcat_amd.node")
 STEP = os.path.join(HERE, "node", "step_world.js")
+EVENTS = os.path.join(HERE, "node", "events_world.js")
 
 needs_node = pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
 
@@ -47,7 +48,8 @@ def test_addon_layouts_match_the_ctypes_mirror():
     L = json.loads(r.stdout)
     mirror = {"cc_settings": _abi.Settings, "cc_shape_desc": _abi.ShapeDesc, "cc_body_desc": _abi.BodyDesc,
               "cc_body_state": _abi.BodyState, "cc_point_constraint_desc": _abi.PointConstraintDesc,
-              "cc_step_counters": _abi.StepCounters}
+              "cc_step_counters": _abi.StepCounters, "cc_contact_out": _abi.ContactOut,
+              "cc_contact_removed": _abi.ContactRemoved}
     for name, st in mirror.items():
         assert L[name]["size"] == C.sizeof(st), name
         names = [f for f, _ in st._fields_ if not f.endswith("_")]
@@ -84,6 +86,58 @@ def test_create_without_gpu_throws():
     """)
     assert r.returncode == 0, r.stderr
     assert "no usable HIP device" in r.stdout
+
+
+@needs_node
+def test_update_world_refuses_validate_callbacks():
+    """onBodyPairValidate / onContactValidate decide what a step does and cannot run inside the
+    GPU step: updateWorld refuses them before stepping (the contact callbacks are delivered
+    after the step instead)."""
+    r = _node("""
+const cc = require('./index.js');
+const w = Object.create(cc.GpuWorld.prototype);
+for (const k of ['onBodyPairValidate', 'onContactValidate']) {
+  try { cc.updateWorld(w, { [k]() { return true; } }, 1 / 60); console.log('stepped'); }
+  catch (e) { console.log(e.message); }
+}""")
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert len(lines) == 2
+    assert "not available on the GPU step: onBodyPairValidate" in lines[0]
+    assert "not available on the GPU step: onContactValidate" in lines[1]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_listener_receives_the_oracle_contact_events(tmp_path):
+    """updateWorld with onContactAdded / onContactPersisted / onContactRemoved from JavaScript:
+    per step, the same events as the FP64 oracle (keys and point counts), on a heap that lands,
+    settles and sleeps."""
+    from oracle import refcpu as O
+    sc = SC.cube_heap(seed=1001, nx=3, ny=3, nz=3)
+    desc = tmp_path / "world.json"
+    desc.write_text(json.dumps(sc.desc()))
+    steps = 200
+    _addon()
+    r = subprocess.run(["node", EVENTS, str(desc), "fp64", str(steps)], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr
+    got = [json.loads(x) for x in r.stdout.strip().splitlines()]
+    assert len(got) == steps
+    orc = O.OracleWorld(sc)
+    totals = [0, 0, 0]
+    for s in range(steps):
+        orc.step(1.0 / 60.0)
+        c, rm = orc.contacts(), orc.removed_contacts()
+        rows = [[int(k["body_a"]), int(k["body_b"]), int(k["sub_shape_a"]), int(k["sub_shape_b"]),
+                 int(k["num_points"])] for k in c]
+        pers = [x for x, k in zip(rows, c) if k["persisted"]]
+        added = [x for x, k in zip(rows, c) if not k["persisted"]]
+        rem = [[int(k["body_a"]), int(k["body_b"]), int(k["sub_shape_a"]), int(k["sub_shape_b"])] for k in rm]
+        assert got[s]["added"] == added, f"step {s + 1}: added"
+        assert got[s]["persisted"] == pers, f"step {s + 1}: persisted"
+        assert got[s]["removed"] == rem, f"step {s + 1}: removed"
+        totals = [totals[0] + len(added), totals[1] + len(pers), totals[2] + len(rem)]
+    assert min(totals) > 0, totals
 
 
 @needs_node
