@@ -7,7 +7,10 @@
 // normal corrected at inactive edges; then one thread per pair merges its hits into
 // manifolds in walk order and creates the pair's run of contact constraints.  The convex
 // shape's support modes, the triangle support and the EPA polytope are the convex path's; the
-// polytope workspace is the EPA kernel's, free again by then.
+// polytope workspace is the EPA kernel's, free again by then.  A sphere takes
+// collideSphereVsTriangleMesh (triangle-mesh.ts:1418-1720) through the same passes: the walk is
+// queried from the sphere's centre in the mesh's frame, and a candidate's hit is the closest
+// point of its triangle.
 #pragma once
 #include "cc_kernels.h"
 
@@ -99,16 +102,23 @@ __device__ inline void mesh_add_hit(const Dev<T>& d, MeshColl<T>& C, int& nm, V3
 // One convex-vs-mesh pair, as collideTriangleMeshVsConvex / collideConvexVsTriangleMesh set it up
 // (triangle-mesh.ts:1058-1131): the convex shape is A of the mesh function; `rev` when the mesh
 // is the narrowphase's body A (hits are then inverted, collide-shape-vs-shape.ts:228-330).
+// Triangles are read in the convex shape's frame (t21q, t21p); for a sphere (`sphere`,
+// triangle-mesh.ts:1448-1500) in the mesh's frame relative to the sphere's centre sc, with the
+// walk's query box sc -+ (radius + max separation).  Hits are stored in the frame (hq, hp).
 template <typename T>
 struct MeshPair {
     int slot, A, Bi;
-    bool rev;
+    bool rev, sphere;
     NpBody<T> RA, RB;
     int cshape;
     V3<T> pa;
     Q4<T> qa;
     Q4<T> t21q;
     V3<T> t21p, b1mn, b1mx, q2mn, q2mx, qc, mdl;
+    V3<T> sc;
+    T rad;
+    Q4<T> hq;
+    V3<T> hp;
     int mfn, mnn, mft;
     unsigned mask;
 };
@@ -135,6 +145,28 @@ __device__ inline void mesh_pair(const Dev<T>& d, int slot, MeshPair<T>& P) {
     P.qa = RC.q;
     const V3<T> pm = RM.pos;
     const Q4<T> qm = RM.q;
+    P.mfn = sm.hff;
+    P.mnn = sm.hnf;
+    P.mft = sm.hfp;
+    const int ntri = sm.hnp;
+    const int nbits = ntri > 1 ? 32 - __clz(ntri - 1) : 0;  // pushIndex sub-shape.ts:60-64
+    P.mask = nbits >= 32 ? 0xffffffffu : ((1u << nbits) - 1u);
+    P.sphere = sc.type == SHAPE_SPHERE;
+    if (P.sphere) {  // triangle-mesh.ts:1452-1480
+        const Q4<T> iqm = qconj(qm);
+        P.rad = sc.radius;
+        P.sc = qrotate(iqm, P.pa - pm);
+        const T er = P.rad + d.s.spec;
+        P.q2mn = mk3<T>(P.sc.x - er, P.sc.y - er, P.sc.z - er);
+        P.q2mx = mk3<T>(P.sc.x + er, P.sc.y + er, P.sc.z + er);
+        P.qc = P.sc;
+        P.mdl = qrotate(iqm, move);  // triangle-mesh.ts:1607-1611
+        P.hq = qm;
+        P.hp = pm;
+        return;
+    }
+    P.hq = P.qa;
+    P.hp = P.pa;
     const Q4<T> iqa = qconj(P.qa);
     P.t21q = qmul(iqa, qm);
     P.t21p = qrotate(iqa, pm - P.pa);
@@ -161,12 +193,6 @@ __device__ inline void mesh_pair(const Dev<T>& d, int slot, MeshPair<T>& P) {
     P.q2mx = q2mx + e;
     P.qc = mk3<T>((P.q2mn.x + P.q2mx.x) * T(0.5), (P.q2mn.y + P.q2mx.y) * T(0.5), (P.q2mn.z + P.q2mx.z) * T(0.5));
     P.mdl = qrotate(iqa, move);
-    P.mfn = sm.hff;
-    P.mnn = sm.hnf;
-    P.mft = sm.hfp;
-    const int ntri = sm.hnp;
-    const int nbits = ntri > 1 ? 32 - __clz(ntri - 1) : 0;  // pushIndex sub-shape.ts:60-64
-    P.mask = nbits >= 32 ? 0xffffffffu : ((1u << nbits) - 1u);
 }
 
 // a triangle of the pair's mesh in the convex shape's frame
@@ -174,10 +200,123 @@ template <typename T>
 __device__ inline void mesh_tri(const Dev<T>& d, const MeshPair<T>& P, int ti, V3<T>& a, V3<T>& b, V3<T>& c, int& act) {
     const int4 tr = d.st.mtri[P.mft + ti];
     const T* X = d.st.mpos;
+    act = tr.w;
+    if (P.sphere) {  // rv0..rv2 (triangle-mesh.ts:1540-1545)
+        a = mk3<T>(X[3 * tr.x], X[3 * tr.x + 1], X[3 * tr.x + 2]) - P.sc;
+        b = mk3<T>(X[3 * tr.y], X[3 * tr.y + 1], X[3 * tr.y + 2]) - P.sc;
+        c = mk3<T>(X[3 * tr.z], X[3 * tr.z + 1], X[3 * tr.z + 2]) - P.sc;
+        return;
+    }
     a = qrotate(P.t21q, mk3<T>(X[3 * tr.x], X[3 * tr.x + 1], X[3 * tr.x + 2])) + P.t21p;
     b = qrotate(P.t21q, mk3<T>(X[3 * tr.y], X[3 * tr.y + 1], X[3 * tr.y + 2])) + P.t21p;
     c = qrotate(P.t21q, mk3<T>(X[3 * tr.z], X[3 * tr.z + 1], X[3 * tr.z + 2])) + P.t21p;
-    act = tr.w;
+}
+
+// getClosestPointOnTriangle (src/collision/triangle.ts:62-198, degenerate triangles :199-286):
+// the point of triangle abc closest to the origin, its squared distance, and the feature it
+// lies on (bits of the vertices a, b, c that span it; 7 the interior).
+template <typename T>
+__device__ inline int closest_on_triangle(V3<T> a, V3<T> b, V3<T> c, V3<T>& p, T& dist2) {
+    const V3<T> ab = b - a, ac = c - a;
+    const V3<T> n = cross(ab, ac);
+    const T nl = len2(n);
+    if (nl < T(1e-10)) {
+        int f = 4;
+        p = c;
+        T best = len2(c);
+        const T al = len2(a);
+        if (al < best) { f = 1; p = a; best = al; }
+        const T bl = len2(b);
+        if (bl < best) { f = 2; p = b; best = bl; }
+        const T eps2 = T(1.1920929e-7 * 1.1920929e-7);
+        const T acl = len2(ac);
+        if (acl > eps2) {
+            const T v = tmax<T>(T(0), tmin<T>(T(1), -dot(a, ac) / acl));
+            const V3<T> q = mk3<T>(a.x + v * ac.x, a.y + v * ac.y, a.z + v * ac.z);
+            const T dq = len2(q);
+            if (dq < best) { f = 5; p = q; best = dq; }
+        }
+        const V3<T> bc = c - b;
+        const T bcl = len2(bc);
+        if (bcl > eps2) {
+            const T v = tmax<T>(T(0), tmin<T>(T(1), -dot(b, bc) / bcl));
+            const V3<T> q = mk3<T>(b.x + v * bc.x, b.y + v * bc.y, b.z + v * bc.z);
+            const T dq = len2(q);
+            if (dq < best) { f = 6; p = q; best = dq; }
+        }
+        const T abl = len2(ab);
+        if (abl > eps2) {
+            const T v = tmax<T>(T(0), tmin<T>(T(1), -dot(a, ab) / abl));
+            const V3<T> q = mk3<T>(a.x + v * ab.x, a.y + v * ab.y, a.z + v * ab.z);
+            const T dq = len2(q);
+            if (dq < best) { f = 3; p = q; best = dq; }
+        }
+        dist2 = best;
+        return f;
+    }
+    const V3<T> ap = -a;
+    const T d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= T(0) && d2 <= T(0)) { p = a; dist2 = len2(a); return 1; }
+    const V3<T> bp = -b;
+    const T d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= T(0) && d4 <= d3) { p = b; dist2 = len2(b); return 2; }
+    if (d1 * d4 <= d3 * d2 && d1 >= T(0) && d3 <= T(0)) {
+        const T v = d1 / (d1 - d3);
+        p = mk3<T>(a.x + v * ab.x, a.y + v * ab.y, a.z + v * ab.z);
+        dist2 = len2(p);
+        return 3;
+    }
+    const V3<T> cp = -c;
+    const T d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= T(0) && d5 <= d6) { p = c; dist2 = len2(c); return 4; }
+    if (d5 * d2 <= d1 * d6 && d2 >= T(0) && d6 <= T(0)) {
+        const T w = d2 / (d2 - d6);
+        p = mk3<T>(a.x + w * ac.x, a.y + w * ac.y, a.z + w * ac.z);
+        dist2 = len2(p);
+        return 5;
+    }
+    const T d43 = d4 - d3, d56 = d5 - d6;
+    if (d3 * d6 <= d5 * d4 && d43 >= T(0) && d56 >= T(0)) {
+        const T w = d43 / (d43 + d56);
+        const V3<T> bc = c - b;
+        p = mk3<T>(b.x + w * bc.x, b.y + w * bc.y, b.z + w * bc.z);
+        dist2 = len2(p);
+        return 6;
+    }
+    const T cd = (a.x + b.x + c.x) * n.x + (a.y + b.y + c.y) * n.y + (a.z + b.z + c.z) * n.z;
+    const T sc = cd / (T(3) * nl);
+    p = mk3<T>(n.x * sc, n.y * sc, n.z * sc);
+    dist2 = len2(p);
+    return 7;
+}
+
+// FEATURE_TO_ACTIVE_EDGES (src/collision/sphere-triangle.ts:218-227), 4 bits per feature: the
+// edges, one of which must be active for a hit on that feature to keep its own normal
+__device__ inline int feature_edges(int f) { return (0x02461350 >> (4 * f)) & 7; }
+
+// One candidate of a sphere pair (triangle-mesh.ts:1546-1640): the hit in the mesh's frame, or false.
+template <typename T>
+__device__ inline bool sphere_tri_hit(const Dev<T>& d, const MeshPair<T>& P, V3<T> a, V3<T> b, V3<T> c, int act,
+                                      MeshHit<T>& out) {
+    const V3<T> nrm = cross(b - a, c - a);
+    const bool back = dot(nrm, a) > T(0);
+    V3<T> p2;
+    T dd;
+    const int f = closest_on_triangle(a, b, c, p2, dd);
+    const T ms = P.rad + d.s.spec;
+    if (dd > ms * ms) return false;
+    const T pen = P.rad - sqrt(dd);
+    V3<T> axis = dd > T(0) ? normalize(p2) : mk3<T>(0, 1, 0);
+    const V3<T> p1 = mk3<T>(axis.x * P.rad, axis.y * P.rad, axis.z * P.rad);
+    if (d.s.activeEdges && f != 7 && act != 7 && (act & feature_edges(f)) == 0) {
+        const V3<T> na = back ? nrm : -nrm;
+        if (dot(P.mdl, axis) * vlen(na) >= dot(P.mdl, na)) axis = na;
+    }
+    out.pen = pen;
+    out.pA = P.sc + p1;
+    out.pB = P.sc + p2;
+    out.axis = axis;
+    return true;
 }
 
 // The BVH walk (closer child first) calling f(triangle) for each triangle that passes the
@@ -204,8 +343,8 @@ __device__ inline bool mesh_walk(const Dev<T>& d, const MeshPair<T>& P, F&& f) {
                                          tmin<T>(tmin<T>(a.z, b.z), c.z));
                 const V3<T> tmx = mk3<T>(tmax<T>(tmax<T>(a.x, b.x), c.x), tmax<T>(tmax<T>(a.y, b.y), c.y),
                                          tmax<T>(tmax<T>(a.z, b.z), c.z));
-                if (tmx.x < P.b1mn.x || tmn.x > P.b1mx.x || tmx.y < P.b1mn.y || tmn.y > P.b1mx.y || tmx.z < P.b1mn.z ||
-                    tmn.z > P.b1mx.z)
+                if (!P.sphere && (tmx.x < P.b1mn.x || tmn.x > P.b1mx.x || tmx.y < P.b1mn.y || tmn.y > P.b1mx.y || tmx.z < P.b1mn.z ||
+                    tmn.z > P.b1mx.z))
                     continue;
                 const V3<T> nrm = cross(b - a, c - a);
                 if (!d.s.backfaces && dot(nrm, a) > T(0)) continue;
@@ -278,6 +417,10 @@ __global__ void __launch_bounds__(64) k_mesh_tri(Dev<T> d) {
         mesh_tri(d, P, ti, a, b, c, act);
         MeshHit<T>& out = ((MeshHit<T>*)d.candHit)[ci];
         out.hit = 0;
+        if (P.sphere) {
+            if (sphere_tri_hit(d, P, a, b, c, act, out)) out.hit = 1;
+            continue;
+        }
         const V3<T> nrm = cross(b - a, c - a);
         const bool back = dot(nrm, a) > T(0);
         V3<T> axis = -nrm;
@@ -336,14 +479,19 @@ __global__ void __launch_bounds__(64) k_mesh_collect(Dev<T> d, T dt) {
             V3<T> a, b, c;
             int act;
             mesh_tri(d, P, ti, a, b, c, act);
-            const V3<T> wA = qrotate(P.qa, h.pA) + P.pa;
-            const V3<T> wB = qrotate(P.qa, h.pB) + P.pa;
-            const V3<T> wAxis = qrotate(P.qa, h.axis);
+            const V3<T> wA = qrotate(P.hq, h.pA) + P.hp;
+            const V3<T> wB = qrotate(P.hq, h.pB) + P.hp;
+            const V3<T> wAxis = qrotate(P.hq, h.axis);
             const unsigned sub = (0xffffffffu & ~P.mask) | (unsigned)ti;
-            supporting_face(S.fA, d.st, P.cshape, -h.axis, P.pa, P.qa);
-            S.fB.n = 3;
-            S.fB.v[0] = a; S.fB.v[1] = b; S.fB.v[2] = c;
-            face_transform(S.fB, P.pa, P.qa);
+            if (P.sphere) {  // a sphere has no supporting face: the collector keeps the single point
+                S.fA.n = 0;
+                S.fB.n = 0;
+            } else {
+                supporting_face(S.fA, d.st, P.cshape, -h.axis, P.pa, P.qa);
+                S.fB.n = 3;
+                S.fB.v[0] = a; S.fB.v[1] = b; S.fB.v[2] = c;
+                face_transform(S.fB, P.pa, P.qa);
+            }
             if (!P.rev) mesh_add_hit(d, C, nm, P.RA.pos, wA, wB, wAxis, h.pen, 0xffffffffu, sub, S.fA, S.fB, S);
             else mesh_add_hit(d, C, nm, P.RA.pos, wB, wA, -wAxis, h.pen, sub, 0xffffffffu, S.fB, S.fA, S);
         }

@@ -475,8 +475,6 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         if (b.object_layer < 0 || b.object_layer >= CC_MAX_OBJECT_LAYERS) return fail(c, CC_E_ARG, "invalid object layer");
         if (X->hasMesh && shapes[b.shape].type == CC_SHAPE_TRIANGLE_MESH && b.motion_type != CC_MOTION_STATIC)
             return fail(c, CC_E_UNSUPPORTED, "triangle-mesh bodies must be static");
-        if (X->hasMesh && shapes[b.shape].type == CC_SHAPE_SPHERE)
-            return fail(c, CC_E_UNSUPPORTED, "sphere bodies in a scene with a triangle mesh (sphere vs mesh is not on the GPU path)");
         pos[i] = mk3<T>(T(b.position[0]), T(b.position[1]), T(b.position[2]));
         q[i].x = T(b.quaternion[0]); q[i].y = T(b.quaternion[1]); q[i].z = T(b.quaternion[2]); q[i].w = T(b.quaternion[3]);
         v[i] = mk3<T>(T(b.linear_velocity[0]), T(b.linear_velocity[1]), T(b.linear_velocity[2]));

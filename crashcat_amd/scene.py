@@ -503,12 +503,14 @@ def terrain_heights(rng, amps=(0.5, 3.0), wavelengths=(8.0, 64.0), terms=4):
     return h
 
 
-def terrain_drop(seed=1004, vertices=512, spacing=0.5, n=50000, drop=(5.0, 30.0), hull_variants=64, scene=None):
+def terrain_drop(seed=1004, vertices=512, spacing=0.5, n=50000, drop=(5.0, 30.0), hull_variants=64, scene=None,
+                 spheres=False):
     """Config 4: a static triangle mesh over a `vertices` x `vertices` grid at `spacing` (511^2 * 2
     = 522,242 triangles at the full size), heights from terrain_heights; n boxes (he 0.5),
     capsules (r 0.25, hh 0.25) and convex hulls (16 points on a sphere of radius 0.5; one of
     `hull_variants` seeded hulls), a third each, dropped from U[drop] m above the terrain at
-    positions uniform over it (1 m in from its edges), random orientation, at rest."""
+    positions uniform over it (1 m in from its edges), random orientation, at rest.  With
+    `spheres`, every fourth body is a sphere of radius 0.5 instead (a quarter of each kind)."""
     sc = scene if scene is not None else Scene()
     rng = Rng(seed)
     h = terrain_heights(rng)
@@ -518,10 +520,14 @@ def terrain_drop(seed=1004, vertices=512, spacing=0.5, n=50000, drop=(5.0, 30.0)
     box = S.box((0.5, 0.5, 0.5))
     cap = S.capsule(0.25, 0.25)
     hulls = [S.convex_hull(_unit_sphere_points(rng, 16) * 0.5) for _ in range(hull_variants)]
+    ball = S.sphere(0.5)
     half = ext / 2 - 1.0
     for i in range(n):
-        kind = i % 3
-        shape = box if kind == 0 else (cap if kind == 1 else hulls[int(rng.uniform(0, hull_variants)) % hull_variants])
+        kind = i % (4 if spheres else 3)
+        if kind == 3:
+            shape = ball
+        else:
+            shape = box if kind == 0 else (cap if kind == 1 else hulls[int(rng.uniform(0, hull_variants)) % hull_variants])
         x = rng.uniform(-half, half)
         z = rng.uniform(-half, half)
         y = float(h(np.float64(x), np.float64(z))) + rng.uniform(*drop)

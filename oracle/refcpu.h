@@ -63,13 +63,16 @@ void oc_collide(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t 
 void oc_collide_convex(const cc_shape_desc* shapes, const cc_hull_data* hulls, int32_t sa, const double* pa,
                        const double* qa, int32_t sb, const double* pb, const double* qb, double max_separation,
                        const cc_settings* s, oc_hit* out);
-/* collideConvexVsTriangleMesh (src/shapes/triangle-mesh.ts:1026-1379): every hit of convex shape sa
- * against mesh shape sm, in traversal order, with the triangle's sub-shape id in sub_b[i];
+/* collideConvexVsTriangleMesh (src/shapes/triangle-mesh.ts:1026-1379), or collideSphereVsTriangleMesh
+ * (:1418-1720) when sa is a sphere: every hit of shape sa against mesh shape sm, in traversal order, with the triangle's sub-shape id in sub_b[i];
  * s supplies the tolerances and the back-face / active-edge switches; returns the hit count */
 int32_t oc_collide_mesh(const cc_shape_desc* shapes, const cc_hull_data* hulls, const cc_mesh_data* meshes, int32_t sa,
                         const double* pa, const double* qa, int32_t sm, const double* pm, const double* qm,
                         double max_separation, const double* move_dir, const cc_settings* s, oc_hit* out,
                         uint32_t* sub_b, int32_t cap);
+/* getClosestPointOnTriangle (src/collision/triangle.ts:62-286): closest point of triangle abc to
+ * the origin and its squared distance; returns the TriangleFeature bits */
+int32_t oc_closest_on_triangle(const double* a, const double* b, const double* c, double* point, double* dist2);
 /* manifold of one hit, pruned to <= 4 points (update.ts:385-563 for a single hit);
  * out_a/out_b are world points; returns the point count */
 int32_t oc_manifold_from_hit(const oc_hit* hit, double max_contact_distance, double* normal_out,

@@ -52,6 +52,8 @@ def lib():
         L.oc_collide_mesh.restype = C.c_int32
         L.oc_collide_mesh.argtypes = [vp, vp, vp, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double, dp, vp, vp,
                                       C.POINTER(C.c_uint32), C.c_int32]
+        L.oc_closest_on_triangle.restype = C.c_int32
+        L.oc_closest_on_triangle.argtypes = [dp, dp, dp, dp, dp]
         L.oc_manifold_from_hit.restype = C.c_int32
         L.oc_manifold_from_hit.argtypes = [vp, C.c_double, dp, dp, dp]
         L.oc_clip_poly_vs_plane.restype = C.c_int32
@@ -295,6 +297,15 @@ def supporting_face(scene, s, direction, pos=(0, 0, 0), quat=(0, 0, 0, 1)):
     out = np.zeros((64, 3))
     k = lib().oc_supporting_face(shapes, C.byref(hulls), s, _dp(d), _dp(p), _dp(q), _dp(out))
     return out[:k]
+
+
+def closest_on_triangle(a, b, c):
+    """getClosestPointOnTriangle: (point, squared distance, feature bits) of triangle abc to the origin."""
+    p = _vec((0, 0, 0))
+    d2 = C.c_double()
+    v = [_vec(a), _vec(b), _vec(c)]
+    f = lib().oc_closest_on_triangle(_dp(v[0]), _dp(v[1]), _dp(v[2]), _dp(p), C.byref(d2))
+    return np.array(p[:]), d2.value, f
 
 
 def collide_mesh(scene, sa, pa, qa, sm, pm=(0, 0, 0), qm=(0, 0, 0, 1), max_separation=0.0, move_dir=(0, 0, 0),

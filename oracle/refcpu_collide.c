@@ -1152,6 +1152,175 @@ void ocx_collide_mesh(const OShapes* sh, int sa, V3 pa, Q4 qa, int sm, V3 pm, Q4
     }
 }
 
+/* getClosestPointOnTriangle (src/collision/triangle.ts:62-198; degenerate triangles,
+ * handleDegenerateTriangle :199-286): the point of triangle abc closest to the origin, its
+ * squared distance, and the TriangleFeature bits (:9-24) of the feature it lies on. */
+static int closest_on_triangle(V3 a, V3 b, V3 c, V3* p, double* dist2) {
+    V3 ab = vsub(b, a), ac = vsub(c, a);
+    V3 n = vcross(ab, ac);
+    double nl = vlen2(n);
+    if (nl < 1e-10) {
+        int f = 4;
+        *p = c;
+        double best = vlen2(c);
+        double al = vlen2(a);
+        if (al < best) { f = 1; *p = a; best = al; }
+        double bl = vlen2(b);
+        if (bl < best) { f = 2; *p = b; best = bl; }
+        const double eps2 = 1.1920929e-7 * 1.1920929e-7;
+        double acl = vlen2(ac);
+        if (acl > eps2) {
+            double v = dmax(0.0, dmin(1.0, -vdot(a, ac) / acl));
+            V3 q = v3(a.x + v * ac.x, a.y + v * ac.y, a.z + v * ac.z);
+            double dq = vlen2(q);
+            if (dq < best) { f = 5; *p = q; best = dq; }
+        }
+        V3 bc = vsub(c, b);
+        double bcl = vlen2(bc);
+        if (bcl > eps2) {
+            double v = dmax(0.0, dmin(1.0, -vdot(b, bc) / bcl));
+            V3 q = v3(b.x + v * bc.x, b.y + v * bc.y, b.z + v * bc.z);
+            double dq = vlen2(q);
+            if (dq < best) { f = 6; *p = q; best = dq; }
+        }
+        double abl = vlen2(ab);
+        if (abl > eps2) {
+            double v = dmax(0.0, dmin(1.0, -vdot(a, ab) / abl));
+            V3 q = v3(a.x + v * ab.x, a.y + v * ab.y, a.z + v * ab.z);
+            double dq = vlen2(q);
+            if (dq < best) { f = 3; *p = q; best = dq; }
+        }
+        *dist2 = best;
+        return f;
+    }
+    V3 ap = vneg(a);
+    double d1 = vdot(ab, ap), d2 = vdot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) { *p = a; *dist2 = vlen2(a); return 1; }
+    V3 bp = vneg(b);
+    double d3 = vdot(ab, bp), d4 = vdot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) { *p = b; *dist2 = vlen2(b); return 2; }
+    if (d1 * d4 <= d3 * d2 && d1 >= 0 && d3 <= 0) {
+        double v = d1 / (d1 - d3);
+        *p = v3(a.x + v * ab.x, a.y + v * ab.y, a.z + v * ab.z);
+        *dist2 = vlen2(*p);
+        return 3;
+    }
+    V3 cp = vneg(c);
+    double d5 = vdot(ab, cp), d6 = vdot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) { *p = c; *dist2 = vlen2(c); return 4; }
+    if (d5 * d2 <= d1 * d6 && d2 >= 0 && d6 <= 0) {
+        double w = d2 / (d2 - d6);
+        *p = v3(a.x + w * ac.x, a.y + w * ac.y, a.z + w * ac.z);
+        *dist2 = vlen2(*p);
+        return 5;
+    }
+    double d43 = d4 - d3, d56 = d5 - d6;
+    if (d3 * d6 <= d5 * d4 && d43 >= 0 && d56 >= 0) {
+        double w = d43 / (d43 + d56);
+        V3 bc = vsub(c, b);
+        *p = v3(b.x + w * bc.x, b.y + w * bc.y, b.z + w * bc.z);
+        *dist2 = vlen2(*p);
+        return 6;
+    }
+    double cd = (a.x + b.x + c.x) * n.x + (a.y + b.y + c.y) * n.y + (a.z + b.z + c.z) * n.z;
+    double sc = cd / (3 * nl);
+    *p = v3(n.x * sc, n.y * sc, n.z * sc);
+    *dist2 = vlen2(*p);
+    return 7;
+}
+
+int32_t oc_closest_on_triangle(const double* a, const double* b, const double* c, double* point, double* dist2) {
+    V3 p;
+    int f = closest_on_triangle(vload(a), vload(b), vload(c), &p, dist2);
+    vstore(point, p);
+    return f;
+}
+
+/* FEATURE_TO_ACTIVE_EDGES src/collision/sphere-triangle.ts:218-227 */
+static const int k_feature_edges[8] = {0, 5, 3, 1, 6, 4, 2, 0};
+
+/* collideSphereVsTriangleMesh src/shapes/triangle-mesh.ts:1418-1720 (sphere A of radius r at
+ * pa, mesh at pm/qm, unit scales): the hits against each triangle the BVH walk reaches, in the
+ * walk's order, in world space. */
+void ocx_collide_sphere_mesh(const OShapes* sh, int sa, V3 pa, int sm, V3 pm, Q4 qm, const cc_settings* s,
+                             double maxSep, V3 moveDir, ocx_hit_fn fn, void* ctx) {
+    const cc_shape_desc* A = &sh->shapes[sa];
+    const cc_shape_desc* M = &sh->shapes[sm];
+    const cc_mesh_data* md = sh->meshes;
+    g_mesh_calls++;
+    const double r = A->radius;
+    const double ms2 = (r + maxSep) * (r + maxSep);
+    Q4 iqm = {-qm.x, -qm.y, -qm.z, qm.w};
+    V3 c = qrotate(iqm, vsub(pa, pm));
+    const double er = r + maxSep;
+    V3 bmn = v3(c.x - er, c.y - er, c.z - er), bmx = v3(c.x + er, c.y + er, c.z + er);
+    if (M->mesh_num_nodes == 0) return;
+    const double* nodes = md->nodes + 8 * (size_t)M->mesh_first_node;
+    const int32_t* tris = md->triangles + 4 * (size_t)M->mesh_first_triangle;
+    const int ntri = M->mesh_num_triangles;
+    const int nbits = ntri > 1 ? 32 - __builtin_clz((unsigned)(ntri - 1)) : 0;
+    const uint32_t mask = nbits >= 32 ? 0xffffffffu : ((1u << nbits) - 1u);
+    V3 mdm = qrotate(iqm, moveDir);
+    static int* stack = NULL;
+    static int scap = 0;
+    int ns = 0;
+    if (!scap) { scap = 256; stack = (int*)malloc(sizeof(int) * (size_t)scap); }
+    stack[ns++] = 0;
+    while (ns > 0) {
+        const int o = stack[--ns];
+        const double* nd = nodes + o;
+        if (!(nd[0] <= bmx.x && nd[3] >= bmn.x && nd[1] <= bmx.y && nd[4] >= bmn.y && nd[2] <= bmx.z && nd[5] >= bmn.z))
+            continue;
+        if (nd[7] < 0) {
+            const int t0 = (int)nd[6], cnt = -((int)nd[7] + 1);
+            for (int i = 0; i < cnt; i++) {
+                const int ti = t0 + i;
+                const int32_t* tr = tris + 4 * (size_t)ti;
+                V3 v0 = vload(md->positions + 3 * (size_t)tr[0]);
+                V3 v1 = vload(md->positions + 3 * (size_t)tr[1]);
+                V3 v2 = vload(md->positions + 3 * (size_t)tr[2]);
+                V3 a = vsub(v0, c), b = vsub(v1, c), cc = vsub(v2, c);
+                V3 nrm = vcross(vsub(b, a), vsub(cc, a));
+                int back = vdot(nrm, a) > 0.0;
+                if (!s->collide_with_backfaces && back) continue;
+                V3 p2;
+                double d2;
+                int f = closest_on_triangle(a, b, cc, &p2, &d2);
+                if (d2 > ms2) continue;
+                double pen = r - sqrt(d2);
+                V3 axis = d2 > 0 ? vnormalize(p2) : v3(0, 1, 0);
+                V3 p1 = v3(axis.x * r, axis.y * r, axis.z * r);
+                const int act = tr[3];
+                if (s->collide_only_with_active_edges && f != 7 && act != 7 && (act & k_feature_edges[f]) == 0) {
+                    V3 na = back ? nrm : vneg(nrm);
+                    if (vdot(mdm, axis) * vlen(na) >= vdot(mdm, na)) axis = na;
+                }
+                OHit h;
+                memset(&h, 0, sizeof(h));
+                h.hit = 1;
+                h.status = 0;
+                h.pointA = vadd(qrotate(qm, vadd(c, p1)), pm);
+                h.pointB = vadd(qrotate(qm, vadd(c, p2)), pm);
+                h.axis = qrotate(qm, axis);
+                h.penetration = pen;
+                h.faceA.n = 0; /* a sphere has no supporting face; the triangle's is not used then */
+                h.faceB.n = 0;
+                fn(ctx, &h, (0xffffffffu & ~mask) | (uint32_t)ti);
+            }
+        } else {
+            const int l = o + 8, rr = (int)nd[6];
+            const double* ln = nodes + l;
+            const double* rn = nodes + rr;
+            V3 lc = v3((ln[0] + ln[3]) * 0.5, (ln[1] + ln[4]) * 0.5, (ln[2] + ln[5]) * 0.5);
+            V3 rc = v3((rn[0] + rn[3]) * 0.5, (rn[1] + rn[4]) * 0.5, (rn[2] + rn[5]) * 0.5);
+            double ld = vlen2(vsub(lc, c)), rd = vlen2(vsub(rc, c));
+            if (ns + 2 > scap) { scap *= 2; stack = (int*)realloc(stack, sizeof(int) * (size_t)scap); }
+            if (ld <= rd) { stack[ns++] = rr; stack[ns++] = l; }
+            else { stack[ns++] = l; stack[ns++] = rr; }
+        }
+    }
+}
+
 /* collideShapeVsShape dispatch (src/collision/narrowphase.ts:172-238, src/shapes/sphere.ts:154-167) */
 void ocx_collide(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, double maxSep, double colTol,
                  double penTol, OHit* out) {
@@ -1492,6 +1661,9 @@ int32_t oc_collide_mesh(const cc_shape_desc* shapes, const cc_hull_data* hulls, 
     OShapes sh = {shapes, hulls, meshes};
     Q4 QA = {qa[0], qa[1], qa[2], qa[3]}, QM = {qm[0], qm[1], qm[2], qm[3]};
     MeshHits m = {out, sub_b, 0, cap};
-    ocx_collide_mesh(&sh, sa, vload(pa), QA, sm, vload(pm), QM, s, maxSep, vload(move_dir), mesh_hits_cb, &m);
+    if (shapes[sa].type == CC_SHAPE_SPHERE)
+        ocx_collide_sphere_mesh(&sh, sa, vload(pa), sm, vload(pm), QM, s, maxSep, vload(move_dir), mesh_hits_cb, &m);
+    else
+        ocx_collide_mesh(&sh, sa, vload(pa), QA, sm, vload(pm), QM, s, maxSep, vload(move_dir), mesh_hits_cb, &m);
     return m.n;
 }

@@ -158,6 +158,8 @@ void ocx_prune(OManifold* m, V3 axis);
 typedef void (*ocx_hit_fn)(void* ctx, const OHit* h, uint32_t sub_b);
 void ocx_collide_mesh(const OShapes* sh, int sa, V3 pa, Q4 qa, int sm, V3 pm, Q4 qm, const cc_settings* s,
                       double maxSep, V3 moveDir, ocx_hit_fn fn, void* ctx);
+void ocx_collide_sphere_mesh(const OShapes* sh, int sa, V3 pa, int sm, V3 pm, Q4 qm, const cc_settings* s,
+                             double maxSep, V3 moveDir, ocx_hit_fn fn, void* ctx);
 int ocx_mesh_calls(void);
 extern int64_t ocx_epa_calls;
 

@@ -749,13 +749,15 @@ static int narrowphase_pair(oc_world* w, int ia, int ib, double dt) {
     int ta = w->shapes[A->shape].type, tb = w->shapes[B->shape].type;
     if (ta == CC_SHAPE_TRIANGLE_MESH || tb == CC_SHAPE_TRIANGLE_MESH) {
         if (ta == CC_SHAPE_TRIANGLE_MESH && tb == CC_SHAPE_TRIANGLE_MESH) return 0; /* no mesh-vs-mesh function */
-        if (ta == CC_SHAPE_SPHERE || tb == CC_SHAPE_SPHERE) {
-            fprintf(stderr, "refcpu: sphere vs triangle mesh is not restated\n");
-            abort();
-        }
         V3 move = vsub(A->v, B->v); /* activeEdgeMovementDirection :730-735 */
         double maxSep = w->s.speculative_contact_distance;
-        if (tb == CC_SHAPE_TRIANGLE_MESH) {
+        if (ta == CC_SHAPE_SPHERE || tb == CC_SHAPE_SPHERE) { /* collideSphereVsTriangleMesh and its reverse */
+            C->reversed = ta == CC_SHAPE_TRIANGLE_MESH;
+            if (!C->reversed)
+                ocx_collide_sphere_mesh(&w->sh, A->shape, A->pos, B->shape, B->pos, B->q, &w->s, maxSep, move, mesh_hit, C);
+            else
+                ocx_collide_sphere_mesh(&w->sh, B->shape, B->pos, A->shape, A->pos, A->q, &w->s, maxSep, move, mesh_hit, C);
+        } else if (tb == CC_SHAPE_TRIANGLE_MESH) {
             C->reversed = 0;
             ocx_collide_mesh(&w->sh, A->shape, A->pos, A->q, B->shape, B->pos, B->q, &w->s, maxSep, move, mesh_hit, C);
         } else {

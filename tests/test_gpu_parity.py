@@ -269,20 +269,23 @@ def test_empty_and_static_only_worlds():
 
 
 # ---- convex shapes on static triangle meshes (collideConvexVsTriangleMesh + reduction collector)
-def _valley(n=6, seed=7):
-    """Boxes, capsules and hulls dropped into a V-shaped valley (two planes at 19 degrees to the
-    horizontal meeting along x = 0, plus a ripple): bodies resting on both flanks make two
-    manifolds for one body pair, and those on the ripple merge several triangles' hits."""
+def _valley(n=6, seed=7, spheres=False):
+    """Boxes, capsules and hulls (and with `spheres` spheres of radius 0.35) dropped into a
+    V-shaped valley (two planes at 19 degrees to the horizontal meeting along x = 0, plus a
+    ripple): bodies resting on both flanks make two manifolds for one body pair, and those on
+    the ripple merge several triangles' hits."""
     from crashcat_amd import shapes as S
     sc = SC.Scene()
     rng = SC.Rng(seed)
     pos, idx = S.height_field_mesh(17, 17, 0.5, lambda x, z: 0.35 * np.abs(x) + 0.05 * np.sin(2.1 * z), origin=(-4, -4))
     sc.add_body(S.triangle_mesh(pos, idx), (0, 0, 0), motion_type=0, friction=0.5)
     shapes = [S.box((0.4, 0.3, 0.5)), S.capsule(0.2, 0.3), S.convex_hull(SC._unit_sphere_points(rng, 16) * 0.45)]
+    if spheres:
+        shapes.append(S.sphere(0.35))
     for i in range(n):
         x = rng.uniform(-1.5, 1.5)
         z = rng.uniform(-2.5, 2.5)
-        sc.add_body(shapes[i % 3], (x, 0.35 * abs(x) + 0.8 + 0.6 * i, z), rng.quaternion(), friction=0.5)
+        sc.add_body(shapes[i % len(shapes)], (x, 0.35 * abs(x) + 0.8 + 0.6 * i, z), rng.quaternion(), friction=0.5)
     sc.end_world()
     return sc
 
@@ -294,6 +297,13 @@ def test_fp64_mesh_valley_bitwise():
 def test_fp64_terrain_drop_bitwise():
     sc = SC.terrain_drop(vertices=24, n=40, drop=(0.5, 3.0))
     _lockstep_fp64(sc, 120, check_every=4)
+
+
+def test_fp64_spheres_on_meshes_bitwise():
+    """collideSphereVsTriangleMesh (triangle-mesh.ts:1418-1720) with the other shapes: spheres
+    in the valley (resting against both flanks) and a quarter of a terrain drop's bodies."""
+    _lockstep_fp64(_valley(n=8, spheres=True), 90, check_every=1)
+    _lockstep_fp64(SC.terrain_drop(vertices=24, n=48, drop=(0.5, 3.0), spheres=True), 120, check_every=4)
 
 
 def test_mesh_runs_of_several_manifolds_match():
@@ -321,14 +331,16 @@ def test_mesh_runs_of_several_manifolds_match():
         g.close()
 
 
-def test_fp32_terrain_step_tolerance():
+@pytest.mark.parametrize("spheres", [False, True])
+def test_fp32_terrain_step_tolerance(spheres):
     """FP32 against the FP64 oracle on a terrain (400 bodies on a 48 x 48 grid): an oracle runs
     40 steps, then a fresh oracle and a fresh FP32 context take one step from its state.  Mesh vertices, BVH bounds (rounded
     outwards) and the GJK/EPA arithmetic are FP32.  Tolerances: 99% of the bodies within 1e-4 m
     (measured 99.75%), 99th percentile under 2e-5 m (measured 1.9e-6 m), every body within 2e-3 m
     (measured 3.8e-4 m: a near-tie in clipping or in the merge of two triangles' normals picks
-    other contact points for that body)."""
-    sc = SC.terrain_drop(vertices=48, n=400, drop=(0.5, 4.0))
+    other contact points for that body).  The same bounds hold with spheres as a quarter of the
+    bodies."""
+    sc = SC.terrain_drop(vertices=48, n=400, drop=(0.5, 4.0), spheres=spheres)
     run = O.OracleWorld(sc)
     for _ in range(40):
         run.step(DT)
@@ -359,11 +371,11 @@ def test_mesh_uploads_reject_unsupported_cases():
     from crashcat_amd import shapes as S
     pos, idx = S.height_field_mesh(3, 3, 1.0, lambda x, z: 0.0 * x)
     mesh = S.triangle_mesh(pos, idx)
-    for motion, other in [(1, S.box((0.5, 0.5, 0.5))), (0, S.sphere(0.5))]:
+    for motion, other in [(1, S.box((0.5, 0.5, 0.5))), (2, S.sphere(0.5))]:
         sc = SC.Scene()
         sc.add_body(mesh, (0, 0, 0), motion_type=0)
         row = list(sc.rows[0])
-        row[1] = motion  # a kinematic mesh body
+        row[1] = motion  # a kinematic, then a dynamic mesh body (with a sphere beside it)
         sc.rows[0] = tuple(row)
         sc.add_body(other, (0, 2, 0))
         sc.end_world()
