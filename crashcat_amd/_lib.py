@@ -15,6 +15,7 @@ SYMBOLS = [
     "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
     "cc_get_phase_times", "cc_get_totals", "cc_build_triangle_mesh", "cc_download_removed_contacts",
+    "cc_snapshot", "cc_restore",
 ]
 
 
@@ -47,6 +48,8 @@ def lib():
     L.cc_download_pairs.argtypes = [vp, ip, C.c_int32, ip]
     L.cc_download_contacts.argtypes = [vp, C.POINTER(_abi.ContactOut), C.c_int32, ip]
     L.cc_download_removed_contacts.argtypes = [vp, C.POINTER(_abi.ContactRemoved), C.c_int32, ip]
+    L.cc_snapshot.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.cc_restore.argtypes = [vp, vp, C.c_int64]
     L.cc_get_counters.argtypes = [vp, C.POINTER(_abi.StepCounters)]
     fp = C.POINTER(C.c_float)
     L.cc_broadphase_aabbs.argtypes = [vp, fp, fp, ip, ip, ip, C.c_int32, C.c_double, ip, C.c_int32, ip]
@@ -144,6 +147,19 @@ class Context:
         self._check(lib().cc_download_contacts(self.h, out.ctypes.data_as(C.POINTER(_abi.ContactOut)), cnt.value,
                                                C.byref(cnt)))
         return out[:cnt.value]
+
+    def snapshot(self):
+        """The state one step hands to the next, as bytes (cc_snapshot)."""
+        n = C.c_int64()
+        self._check(lib().cc_snapshot(self.h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._check(lib().cc_snapshot(self.h, buf, n.value, C.byref(n)))
+        return buf.raw[:n.value]
+
+    def restore(self, blob):
+        """Load a snapshot taken from a context holding the same upload (cc_restore)."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        self._check(lib().cc_restore(self.h, buf, len(blob)))
 
     def removed_contacts(self):
         """Contacts of the step before the last that the last step did not make again

@@ -1222,6 +1222,25 @@ __global__ void k_hash_insert(Dev<T> d) {
     }
 }
 
+// The hash of parity q rebuilt from its n records (a restored snapshot): the first record of
+// each pair's run (run > 0) under the pair's key, as k_hash_insert made it.
+template <typename T>
+__global__ void k_hash_rebuild(Dev<T> d, int q, int n) {
+    unsigned long long* hk = d.hkeys[q];
+    int* hv = d.hvals[q];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const CacheRec<T>& r = d.rec[q][k];
+        if (r.run <= 0) continue;
+        unsigned long long key = pair_key(r.a, r.b);
+        unsigned h = hash64(key) & (unsigned)(d.hashCap - 1);
+        for (int probe = 0; probe < d.hashCap; probe++) {
+            unsigned long long old = atomicCAS(&hk[h], CC_EMPTY_KEY, key);
+            if (old == CC_EMPTY_KEY || old == key) { hv[h] = k; break; }
+            h = (h + 1) & (unsigned)(d.hashCap - 1);
+        }
+    }
+}
+
 // The previous step's records that the last step did not match (destroyUnprocessedContacts,
 // src/contacts.ts:531-555), as (a, b, subA, subB) at an atomic position; q is the parity that
 // holds them.  The host sorts the list.

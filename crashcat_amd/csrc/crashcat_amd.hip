@@ -1039,6 +1039,126 @@ extern "C" int cc_set_bodies(cc_ctx* c, const cc_body_state* in, int32_t n) {
     return set_bodies_impl<float>(c, in, n);
 }
 
+// ---------------------------------------------------------------- snapshot / restore
+// Blob layout: SnapHeader, then the device arrays of snap_segments in their order, then the
+// last step's contact-cache records.  Only a context holding the same upload reads it back.
+struct SnapHeader {
+    char magic[8];  // "CCSNAP01"
+    int32_t version, tsize, n, npc, w, ncache, parity, pad;
+    double prevDt;
+    int64_t bytes;  // the whole blob
+};
+
+template <typename T>
+static std::vector<std::pair<void*, size_t>> snap_segments(Dev<T>& d) {
+    const size_t n = (size_t)d.N;
+    const Bodies<T>& B = d.B;
+    return {{B.pos, n * sizeof(V3<T>)},     {B.q, n * sizeof(Q4<T>)},         {B.com, n * sizeof(V3<T>)},
+            {B.v, n * sizeof(V3<T>)},       {B.w, n * sizeof(V3<T>)},         {B.amin, n * sizeof(V3<T>)},
+            {B.amax, n * sizeof(V3<T>)},    {B.sleeping, n * sizeof(int)},    {B.sc, 3 * n * sizeof(V3<T>)},
+            {B.sr, 3 * n * sizeof(T)},      {B.stimer, n * sizeof(T)},        {B.label, n * sizeof(int)},
+            {B.wake, n * sizeof(int)},      {B.flags, n * sizeof(int)},       {d.vb, n * sizeof(VBody<T>)},
+            {d.pb, n * sizeof(PBody<T>)},   {d.pc, (size_t)d.NPC * sizeof(PointCon<T>)}};
+}
+
+template <typename T>
+static int64_t snap_bytes(Dev<T>& d, int ncache) {
+    int64_t total = sizeof(SnapHeader);
+    for (auto& sg : snap_segments(d)) total += (int64_t)sg.second;
+    return total + (int64_t)ncache * (int64_t)sizeof(CacheRec<T>);
+}
+
+template <typename T>
+static int snapshot_impl(cc_ctx* c, void* out, int64_t cap, int64_t* size) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int par = 0, rc[2] = {0, 0};
+    HIPCHK(c, hipMemcpy(&par, d.parity, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(rc, d.recCnt, sizeof(rc), hipMemcpyDeviceToHost));
+    const int q = par ^ 1;  // the parity the last step wrote, which the next step reads
+    const int nr = std::min(rc[q], d.conCap);
+    const int64_t total = snap_bytes(d, nr);
+    *size = total;
+    if (!out || cap < total) return CC_OK;
+    SnapHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, "CCSNAP01", 8);
+    h.version = 1;
+    h.tsize = (int32_t)sizeof(T);
+    h.n = d.N; h.npc = d.NPC; h.w = d.W;
+    h.ncache = nr;
+    h.parity = par;
+    T pdt;
+    HIPCHK(c, hipMemcpy(&pdt, d.prevDt, sizeof(T), hipMemcpyDeviceToHost));
+    h.prevDt = (double)pdt;
+    h.bytes = total;
+    char* p = static_cast<char*>(out);
+    memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    for (auto& sg : snap_segments(d)) {
+        if (sg.second) HIPCHK(c, hipMemcpy(p, sg.first, sg.second, hipMemcpyDeviceToHost));
+        p += sg.second;
+    }
+    if (nr > 0) HIPCHK(c, hipMemcpy(p, d.rec[q], (size_t)nr * sizeof(CacheRec<T>), hipMemcpyDeviceToHost));
+    return CC_OK;
+}
+
+template <typename T>
+static int restore_impl(cc_ctx* c, const void* in, int64_t size) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    SnapHeader h;
+    if (size < (int64_t)sizeof(h)) return fail(c, CC_E_ARG, "snapshot too short");
+    memcpy(&h, in, sizeof(h));
+    if (memcmp(h.magic, "CCSNAP01", 8) != 0 || h.version != 1) return fail(c, CC_E_ARG, "not a crashcat_amd snapshot");
+    if (h.tsize != (int32_t)sizeof(T) || h.n != d.N || h.npc != d.NPC || h.w != d.W)
+        return fail(c, CC_E_ARG, "snapshot of another scene or precision");
+    if (h.ncache < 0 || h.ncache > d.conCap || (h.parity != 0 && h.parity != 1))
+        return fail(c, CC_E_ARG, "corrupt snapshot header");
+    const int64_t total = snap_bytes(d, h.ncache);
+    if (h.bytes != total || size != total) return fail(c, CC_E_ARG, "snapshot size mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const char* p = static_cast<const char*>(in) + sizeof(h);
+    for (auto& sg : snap_segments(d)) {
+        if (sg.second) HIPCHK(c, hipMemcpy(sg.first, p, sg.second, hipMemcpyHostToDevice));
+        p += sg.second;
+    }
+    const int q = h.parity ^ 1;
+    const int n = h.ncache;
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(d.rec[q], p, (size_t)n * sizeof(CacheRec<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemset(d.recHit[q], 0, (size_t)n * sizeof(int)));
+    }
+    int rc[2] = {0, 0};
+    rc[q] = n;
+    HIPCHK(c, hipMemcpy(d.recCnt, rc, sizeof(rc), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.parity, &h.parity, sizeof(int), hipMemcpyHostToDevice));
+    const T pdt = T(h.prevDt);
+    HIPCHK(c, hipMemcpy(d.prevDt, &pdt, sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(d.counters, 0, CNT_N * sizeof(int)));
+    HIPCHK(c, hipMemset(d.hkeys[q], 0xff, (size_t)d.hashCap * sizeof(unsigned long long)));
+    if (n > 0) hipLaunchKernelGGL(k_hash_rebuild<T>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, c->stream, d, q, n);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CC_OK;
+}
+
+extern "C" int cc_snapshot(cc_ctx* c, void* out, int64_t cap, int64_t* size) {
+    if (!c || !c->impl || !size) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return snapshot_impl<double>(c, out, cap, size);
+    return snapshot_impl<float>(c, out, cap, size);
+}
+
+extern "C" int cc_restore(cc_ctx* c, const void* in, int64_t size) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (!in) return fail(c, CC_E_ARG, "null snapshot");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return restore_impl<double>(c, in, size);
+    return restore_impl<float>(c, in, size);
+}
+
 template <typename T>
 static int pairs_impl(cc_ctx* c, int32_t* out, int cap, int* count) {
     auto* X = static_cast<CtxT<T>*>(c->impl);

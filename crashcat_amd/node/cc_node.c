@@ -378,6 +378,38 @@ static napi_value download_records(napi_env env, napi_callback_info info, int re
 static napi_value download_contacts(napi_env env, napi_callback_info info) { return download_records(env, info, 0); }
 static napi_value download_removed(napi_env env, napi_callback_info info) { return download_records(env, info, 1); }
 
+/* ---- snapshot(ctx) -> ArrayBuffer; restore(ctx, buffer) ---------------------------------- */
+static napi_value snapshot(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value a[1];
+    CHECK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+    cc_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    int64_t size = 0;
+    int rc = cc_snapshot(c, NULL, 0, &size);
+    if (rc != CC_OK) return throw_rc(env, c, rc);
+    void* data;
+    napi_value ab;
+    CHECK(env, napi_create_arraybuffer(env, (size_t)size, &data, &ab));
+    rc = cc_snapshot(c, data, size, &size);
+    if (rc != CC_OK) return throw_rc(env, c, rc);
+    return ab;
+}
+
+static napi_value restore(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value a[2];
+    CHECK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+    cc_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    size_t bytes;
+    void* p = buffer_data(env, a[1], &bytes);
+    if (!p) return NULL;
+    int rc = cc_restore(c, p, (int64_t)bytes);
+    if (rc != CC_OK) return throw_rc(env, c, rc);
+    return NULL;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor d[] = {
         {"layout", 0, layout, 0, 0, 0, napi_default, 0},
@@ -391,6 +423,8 @@ static napi_value init(napi_env env, napi_value exports) {
         {"downloadPairs", 0, download_pairs, 0, 0, 0, napi_default, 0},
         {"downloadContacts", 0, download_contacts, 0, 0, 0, napi_default, 0},
         {"downloadRemovedContacts", 0, download_removed, 0, 0, 0, napi_default, 0},
+        {"snapshot", 0, snapshot, 0, 0, 0, napi_default, 0},
+        {"restore", 0, restore, 0, 0, 0, napi_default, 0},
     };
     napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
     return exports;

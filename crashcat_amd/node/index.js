@@ -167,6 +167,15 @@ class GpuWorld {
   contacts() { return unpack('cc_contact_out', addon.downloadContacts(this.handle)); }
   removedContacts() { return unpack('cc_contact_removed', addon.downloadRemovedContacts(this.handle)); }
 
+  // the state one step hands to the next as an ArrayBuffer (cc_snapshot), and back into a
+  // world made from the same description (cc_restore): the steps after it repeat bit for bit
+  snapshot() { return addon.snapshot(this.handle); }
+  restore(buf) {
+    addon.restore(this.handle, buf);
+    const dv = buf instanceof ArrayBuffer ? new DataView(buf) : new DataView(buf.buffer, buf.byteOffset, buf.byteLength);
+    this.previousTimeStep = dv.getFloat64(40, true);  // the blob header's previous time step
+  }
+
   // Deliver the last step's contact events to a listener: onContactAdded / onContactPersisted
   // (src/constraints/contact-constraints.ts:493-499) per contact in (body a, body b, sub-shape)
   // order, then onContactRemoved (src/contacts.ts:305-307) per removed contact.  Bodies are

@@ -344,6 +344,19 @@ int cc_broadphase_aabbs(cc_ctx* ctx, const float* aabb_min, const float* aabb_ma
  * clipping, contact cache, constraint setup), islands, solver (levels, warm start,
  * velocity and position sweeps, integration, sleep), contact-cache hash. */
 int cc_get_phase_times(cc_ctx* ctx, double* out_ms, int32_t capacity, int32_t* out_count);
+
+/* World snapshot: the state one step hands to the next, as one binary blob in place of the
+ * reference's JSON-serialisable world (README.md:2296-2298; warm-start state in world.contacts,
+ * src/contacts.ts:9-58, and world.previousTimeStep, src/world.ts:38): every body's pose,
+ * velocities, bounds and sleep state, the solver's body records, the point constraints with
+ * their impulses, the last step's contact cache (keys, sub-shape ids, contact points and
+ * impulses) and the previous time step.  cc_snapshot writes the blob when `capacity` is at
+ * least *out_size (always set; call with capacity 0 for the size).  cc_restore loads one into
+ * a context holding the same upload (same precision, bodies, shapes and point constraints;
+ * CC_E_ARG otherwise): the steps that follow are bit-identical to those after the snapshot.
+ * The parity taps (pairs, contacts, removed contacts) are empty until the next step. */
+int cc_snapshot(cc_ctx* ctx, void* out, int64_t capacity, int64_t* out_size);
+int cc_restore(cc_ctx* ctx, const void* in, int64_t size);
 int cc_get_totals(cc_ctx* ctx, cc_totals* out);
 
 #ifdef __cplusplus

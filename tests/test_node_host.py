@@ -142,6 +142,40 @@ def test_node_listener_receives_the_oracle_contact_events(tmp_path):
 
 @needs_node
 @pytest.mark.gpu
+def test_node_snapshot_restore_repeats_the_run(tmp_path):
+    """world.snapshot() after 20 steps, restored into a second world made from the same
+    description: the next 20 steps of both are bit-identical (cc_snapshot / cc_restore)."""
+    sc = SC.terrain_drop(vertices=16, n=24, drop=(0.5, 2.5), spheres=True)
+    desc = tmp_path / "world.json"
+    desc.write_text(json.dumps(sc.desc()))
+    _addon()
+    script = """
+const fs = require('fs');
+const cc = require('./index.js');
+const desc = JSON.parse(fs.readFileSync(process.argv[1], 'utf8'));
+const a = cc.createWorld(desc, { precision: 'fp32', maxPairsPerBody: 16 });
+for (let i = 0; i < 20; i++) cc.updateWorld(a, undefined, 1 / 60);
+const snap = a.snapshot();
+for (let i = 0; i < 20; i++) cc.updateWorld(a, undefined, 1 / 60);
+const b = cc.createWorld(desc, { precision: 'fp32', maxPairsPerBody: 16 });
+b.restore(snap);
+const resumed = b.previousTimeStep;
+for (let i = 0; i < 20; i++) cc.updateWorld(b, undefined, 1 / 60);
+const x = new Uint8Array(a.bodyStateBuffer()), y = new Uint8Array(b.bodyStateBuffer());
+let same = x.length === y.length;
+for (let i = 0; same && i < x.length; i++) same = x[i] === y[i];
+console.log(JSON.stringify({ same, bytes: snap.byteLength, resumed }));
+"""
+    r = subprocess.run(["node", "-e", script, str(desc)], capture_output=True, text=True, timeout=120,
+                       cwd=os.path.join(HERE, "..", "crashcat_amd", "node"))
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["same"] and out["bytes"] > 0
+    assert out["resumed"] == pytest.approx(1 / 60)
+
+
+@needs_node
+@pytest.mark.gpu
 @pytest.mark.parametrize("which", ["cube_heap", "terrain"])
 def test_node_update_world_cube_heap_bitwise(tmp_path, which):
     from oracle import refcpu as O
