@@ -72,7 +72,7 @@ struct Dev {
     int* gLargeOff;     // [N + 1] exclusive scan of gLargeFlag (bodies are world-ordered)
     int* gLarge;        // [N] large bodies, grouped by world
     int* gjkList;       // [pairCap] pair slots that passed the filter
-    int* gjkCombo;      // [pairCap] their (shape A type, shape B type) as typeA * 4 + typeB
+    int* gjkCombo;      // [pairCap] their (shape A kind, shape B kind) as kindA * 4 + kindB, cylinders with hulls
     int* gjkSorted;     // [pairCap] the same slots ordered by type pair, for GJK
     int* gjkHist;       // [2 * 16] per type pair: count, then next free position
     NpBody<T>* npb;     // [N] narrowphase records of each body
@@ -877,7 +877,7 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
             } else {
                 T reach = RA.brad + RB.brad + d.s.spec;
                 gjk = !(len2(RB.pos - RA.pos) > reach * reach);
-                combo = ta * 4 + tb;
+                combo = min(ta, 3) * 4 + min(tb, 3);  // groups pairs by support kind, cylinders with hulls
             }
         }
         const int idx = block_append<256>(&d.counters[CNT_GJK], gjk, sh);

@@ -7,7 +7,7 @@
 
 namespace cc {
 
-enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3, SHAPE_MESH = 4 };
+enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3, SHAPE_MESH = 4, SHAPE_CYLINDER = 5 };
 
 // A BVH node of a triangle mesh: bounds, then for an internal node the right child (node
 // index within the mesh; the left child is the next node) and the split axis, for a leaf the
@@ -42,7 +42,7 @@ struct ShapeTable {
 };
 
 // ---------------------------------------------------------------- supports
-enum { SK_ZERO, SK_SPHERE_R, SK_BOX, SK_CAP_NC, SK_CAP_R, SK_HULL };
+enum { SK_ZERO, SK_SPHERE_R, SK_BOX, SK_CAP_NC, SK_CAP_R, SK_HULL, SK_CYL };
 
 template <typename T>
 struct Sup {
@@ -58,7 +58,7 @@ struct Sup {
 };
 
 // getShapeSupportFunction for unit scale (sphere.ts:246-256, support.ts:336-364,
-// capsule.ts:392-404, convex-hull.ts:979-996)
+// capsule.ts:392-404, convex-hull.ts:979-996, cylinder.ts:406-419)
 template <typename T>
 __device__ inline void sup_for_shape(Sup<T>& s, const ShapeTable<T>& st, int si, bool include) {
     const ShapeD<T>& d = st.shapes[si];
@@ -88,6 +88,11 @@ __device__ inline void sup_for_shape(Sup<T>& s, const ShapeTable<T>& st, int si,
             if (include) { s.kind = SK_CAP_R; s.r = d.radius; }
             else { s.kind = SK_CAP_NC; s.cr = d.radius; }
             break;
+        case SHAPE_CYLINDER:  // setCylinderWithConvexSupport / setCylinderNoConvexSupport (cylinder.ts:323-397)
+            s.kind = SK_CYL;
+            if (include) { s.hh = d.hh; s.r = d.radius; }
+            else { s.hh = d.hh - d.cr; s.r = d.radius - d.cr; s.cr = d.cr; }
+            break;
         default:
             s.kind = SK_HULL;
             s.np = d.hnp;
@@ -109,6 +114,12 @@ __device__ inline V3<T> sup_base(const Sup<T>& s, V3<T> d) {
         case SK_BOX:
             return mk3<T>(d.x >= T(0) ? s.he.x : -s.he.x, d.y >= T(0) ? s.he.y : -s.he.y, d.z >= T(0) ? s.he.z : -s.he.z);
         case SK_CAP_NC: return mk3<T>(0, d.y > T(0) ? s.hh : -s.hh, 0);
+        case SK_CYL: {  // cylinder.ts:295-311
+            const T hl = sqrt(d.x * d.x + d.z * d.z);
+            V3<T> o = mk3<T>(0, d.y >= T(0) ? s.hh : -s.hh, 0);
+            if (hl > T(0)) { const T sc = s.r / hl; o.x = d.x * sc; o.z = d.z * sc; }
+            return o;
+        }
         case SK_CAP_R: {
             T l2 = len2(d);
             if (l2 > T(0)) {
@@ -942,6 +953,34 @@ __device__ inline void supporting_face(Face<T, N>& f, const ShapeTable<T>& st, i
         T pt = dot(top, d), pb = dot(bot, d);
         if (fabs(pt - pb) < T(0.02) * vlen(d)) { f.n = 2; f.v[0] = top; f.v[1] = bot; }
         else { f.n = 0; return; }
+    } else if (s.type == SHAPE_CYLINDER) {  // cylinder.ts:167-240: a side edge, or the cap octagon
+        const T xz = d.x * d.x + d.z * d.z, y2 = d.y * d.y;
+        if (xz > y2) {
+            const T hl = sqrt(xz);
+            const T ff = -s.radius / hl;
+            const T vx = d.x * ff, vz = d.z * ff;
+            f.n = 2;
+            f.v[0] = mk3<T>(vx, s.hh, vz);
+            f.v[1] = mk3<T>(vx, -s.hh, vz);
+        } else {
+            const T sx = d.y < T(0) ? s.radius : -s.radius;
+            const T sy = d.y < T(0) ? s.hh : -s.hh;
+            const T sz = s.radius;
+            T rc = T(1), rs = T(0);
+            if (xz > T(0.00765427) * y2) {  // more than 5 degrees from the axis: a vertex toward d
+                const T hl = sqrt(xz);
+                rc = d.x / hl;
+                rs = d.z / hl;
+            }
+            const T c45 = T(0.707106769);
+            const T ox[8] = {T(0), c45, T(1), c45, T(0), -c45, T(-1), -c45};
+            const T oz[8] = {T(1), c45, T(0), -c45, T(-1), -c45, T(0), c45};
+            f.n = 8;
+            for (int i = 0; i < 8; i++) {
+                const T px = sx * ox[i], py = sy * T(1), pz = sz * oz[i];
+                f.v[i] = mk3<T>(px * rc - pz * rs, py, px * rs + pz * rc);
+            }
+        }
     } else {  // convex-hull.ts:479-548
         int f0 = s.hff;
         V3<T> n0 = mk3<T>(st.pn[3 * f0], st.pn[3 * f0 + 1], st.pn[3 * f0 + 2]);

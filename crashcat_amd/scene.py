@@ -415,18 +415,22 @@ def _unit_sphere_points(rng, n):
     return np.array(pts)
 
 
-def mixed_pile(seed=1002, n=20000, extent=(60.0, 40.0, 60.0), scene=None):
+def mixed_pile(seed=1002, n=20000, extent=(60.0, 40.0, 60.0), scene=None, cylinders=False):
     """Config 2: n bodies drawn uniformly from sphere (r U[0.3,0.8]), box (he U[0.2,0.6]^3),
     capsule (r U[0.2,0.4], hh U[0.2,0.6]) and convex hull of 16 points on the unit sphere
     scaled U[0.4,1.0]; positions uniform in a 60x40x60 m box above a static floor;
-    v ~ N(0,1) per axis, w ~ N(0,0.5) per axis."""
+    v ~ N(0,1) per axis, w ~ N(0,0.5) per axis.  With `cylinders`, a fifth kind: cylinders of
+    r U[0.2,0.6], hh U[0.2,0.6] (convex radius 0.05)."""
     sc = scene if scene is not None else Scene()
     rng = Rng(seed)
     half = (max(50.0, extent[0] / 2 + 5), 0.5, max(50.0, extent[2] / 2 + 5))
     _floor(sc, half=half)
+    kinds = 5 if cylinders else 4
     for _ in range(n):
-        kind = min(3, int(rng.uniform() * 4))
-        if kind == 0:
+        kind = min(kinds - 1, int(rng.uniform() * kinds))
+        if kind == 4:
+            shape = S.cylinder(rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6))
+        elif kind == 0:
             shape = S.sphere(rng.uniform(0.3, 0.8))
         elif kind == 1:
             shape = S.box((rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6)))

@@ -28,6 +28,7 @@ BOX = _abi.CC_SHAPE_BOX
 CAPSULE = _abi.CC_SHAPE_CAPSULE
 CONVEX_HULL = _abi.CC_SHAPE_CONVEX_HULL
 TRIANGLE_MESH = _abi.CC_SHAPE_TRIANGLE_MESH
+CYLINDER = _abi.CC_SHAPE_CYLINDER
 
 
 class Shape:
@@ -108,6 +109,27 @@ def capsule(radius, half_height_of_cylinder, density=DEFAULT_SHAPE_DENSITY):
     t = h + r
     return Shape(CAPSULE, radius=r, half_height=h, density=density, aabb_min=(-r, -t, -r), aabb_max=(r, t, r),
                  mass=mass, inertia=np.diag([ixz + hxz, iy + hy, ixz + hxz]), volume=volume)
+
+
+# ---------------------------------------------------------------------- cylinder
+def cylinder(radius, half_height, convex_radius=DEFAULT_CONVEX_RADIUS, density=DEFAULT_SHAPE_DENSITY):
+    """cylinder.create({halfHeight, radius, convexRadius, density}) — src/shapes/cylinder.ts:56-105
+    (axis y), mass properties :117-150."""
+    r = float(radius)
+    h = float(half_height)
+    cr = float(convex_radius)
+    if h < cr:
+        raise ValueError("Cylinder halfHeight must be >= convexRadius")
+    if r < cr:
+        raise ValueError("Cylinder radius must be >= convexRadius")
+    if cr < 0:
+        raise ValueError("Cylinder convexRadius must be >= 0")
+    volume = math.pi * r * r * 2 * h
+    mass = volume * density
+    iyy = 0.5 * mass * r * r
+    ixx = 0.5 * iyy + (mass * 4 * h * h) / 12
+    return Shape(CYLINDER, radius=r, half_height=h, convex_radius=cr, density=density, aabb_min=(-r, -h, -r),
+                 aabb_max=(r, h, r), mass=mass, inertia=np.diag([ixx, iyy, ixx]), volume=volume)
 
 
 # ------------------------------------------------------------------- convex hull

@@ -38,6 +38,7 @@ extern "C" {
 #define CC_SHAPE_CAPSULE 2
 #define CC_SHAPE_CONVEX_HULL 3
 #define CC_SHAPE_TRIANGLE_MESH 4
+#define CC_SHAPE_CYLINDER 5      /* src/shapes/cylinder.ts: radius, half_height, convex_radius; axis y */
 
 /* ---- motion types: src/body/motion-type.ts ------------------------------ */
 #define CC_MOTION_STATIC 0

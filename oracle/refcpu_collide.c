@@ -14,7 +14,7 @@ int64_t ocx_epa_calls = 0;
 /* ======================================================================== */
 /* support functions (src/collision/support.ts and the shape files)       */
 /* ======================================================================== */
-enum { SK_ZERO, SK_SPHERE_R, SK_BOX, SK_CAP_NC, SK_CAP_R, SK_HULL, SK_TRI };
+enum { SK_ZERO, SK_SPHERE_R, SK_BOX, SK_CAP_NC, SK_CAP_R, SK_HULL, SK_TRI, SK_CYL };
 typedef struct {
     int kind;
     V3 ta, tb, tc; /* SK_TRI: the triangle's vertices */
@@ -34,7 +34,7 @@ typedef struct {
 #define MODE_EXCLUDE 1
 
 /* getShapeSupportFunction for unit scale: sphere.ts:246-256, support.ts:336-364 (box),
- * capsule.ts:392-404, convex-hull.ts:979-996 */
+ * capsule.ts:392-404, convex-hull.ts:979-996, cylinder.ts:406-419 */
 static void sup_for_shape(OSup* s, const OShapes* sh, int si, int mode) {
     const cc_shape_desc* d = &sh->shapes[si];
     memset(s, 0, sizeof(*s));
@@ -58,6 +58,15 @@ static void sup_for_shape(OSup* s, const OShapes* sh, int si, int mode) {
         s->hh = d->half_height;
         if (mode == MODE_INCLUDE) { s->kind = SK_CAP_R; s->r = d->radius; s->convexRadius = 0; }
         else { s->kind = SK_CAP_NC; s->convexRadius = d->radius; }
+        break;
+    case CC_SHAPE_CYLINDER: /* setCylinderWithConvexSupport :378-397, setCylinderNoConvexSupport :323-340 */
+        s->kind = SK_CYL;
+        if (mode == MODE_INCLUDE) { s->hh = d->half_height; s->r = d->radius; s->convexRadius = 0; }
+        else {
+            s->hh = d->half_height - d->convex_radius;
+            s->r = d->radius - d->convex_radius;
+            s->convexRadius = d->convex_radius;
+        }
         break;
     case CC_SHAPE_CONVEX_HULL:
         s->kind = SK_HULL;
@@ -89,6 +98,12 @@ static V3 sup_base(const OSup* s, V3 d) {
         return v3(d.x >= 0 ? s->he.x : -s->he.x, d.y >= 0 ? s->he.y : -s->he.y, d.z >= 0 ? s->he.z : -s->he.z);
     case SK_CAP_NC: /* capsule.ts:291-297 */
         return v3(0, d.y > 0 ? s->hh : -s->hh, 0);
+    case SK_CYL: { /* cylinderNoConvexGetSupport / cylinderWithConvexGetSupport cylinder.ts:295-311, 353-367 */
+        double hl = sqrt(d.x * d.x + d.z * d.z);
+        V3 o = v3(0, d.y >= 0 ? s->hh : -s->hh, 0);
+        if (hl > 0) { double sc = s->r / hl; o.x = d.x * sc; o.z = d.z * sc; }
+        return o;
+    }
     case SK_CAP_R: { /* capsule.ts:330-345 */
         double l2 = vlen2(d);
         if (l2 > 0) {
@@ -890,6 +905,37 @@ static void face_hull(OFace* f, const OShapes* sh, const cc_shape_desc* s, V3 d)
 }
 
 /* getShapeSupportingFace (src/shapes/shapes.ts:489-509) */
+/* getSupportingFace of a cylinder (cylinder.ts:167-240, cap template :43-53): the vertical edge
+ * of the side facing away from dir, or the cap's octagon, turned so that a vertex points along
+ * dir's horizontal part when dir is more than 5 degrees from the axis */
+static void face_cylinder(OFace* f, V3 d, double hh, double r) {
+    static const double c45 = 0.707106769;
+    static const double ox[8] = {0.0, c45, 1.0, c45, 0.0, -c45, -1.0, -c45};
+    static const double oz[8] = {1.0, c45, 0.0, -c45, -1.0, -c45, 0.0, c45};
+    double xz = d.x * d.x + d.z * d.z, y2 = d.y * d.y;
+    if (xz > y2) {
+        double hl = sqrt(xz);
+        double ff = -r / hl;
+        double vx = d.x * ff, vz = d.z * ff;
+        f->n = 2;
+        f->v[0] = v3(vx, hh, vz);
+        f->v[1] = v3(vx, -hh, vz);
+        return;
+    }
+    double sx = d.y < 0 ? r : -r, sy = d.y < 0 ? hh : -hh, sz = r;
+    double rc = 1, rs = 0;
+    if (xz > 0.00765427 * y2) {
+        double hl = sqrt(xz);
+        rc = d.x / hl;
+        rs = d.z / hl;
+    }
+    f->n = 8;
+    for (int i = 0; i < 8; i++) {
+        double px = sx * ox[i], py = sy * 1.0, pz = sz * oz[i];
+        f->v[i] = v3(px * rc - pz * rs, py, px * rs + pz * rc);
+    }
+}
+
 static void supporting_face(OFace* f, const OShapes* sh, int si, V3 dir, V3 pos, Q4 q) {
     const cc_shape_desc* s = &sh->shapes[si];
     switch (s->type) {
@@ -897,6 +943,7 @@ static void supporting_face(OFace* f, const OShapes* sh, int si, V3 dir, V3 pos,
     case CC_SHAPE_BOX: face_box(f, dir, vload(s->half_extents)); break;
     case CC_SHAPE_CAPSULE: face_capsule(f, dir, s->half_height, s->radius); if (f->n == 0) return; break;
     case CC_SHAPE_CONVEX_HULL: face_hull(f, sh, s, dir); break;
+    case CC_SHAPE_CYLINDER: face_cylinder(f, dir, s->half_height, s->radius); break;
     default: f->n = 0; return;
     }
     face_transform(f, pos, q);

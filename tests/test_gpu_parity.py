@@ -269,7 +269,7 @@ def test_empty_and_static_only_worlds():
 
 
 # ---- convex shapes on static triangle meshes (collideConvexVsTriangleMesh + reduction collector)
-def _valley(n=6, seed=7, spheres=False):
+def _valley(n=6, seed=7, spheres=False, cylinders=False):
     """Boxes, capsules and hulls (and with `spheres` spheres of radius 0.35) dropped into a
     V-shaped valley (two planes at 19 degrees to the horizontal meeting along x = 0, plus a
     ripple): bodies resting on both flanks make two manifolds for one body pair, and those on
@@ -282,6 +282,8 @@ def _valley(n=6, seed=7, spheres=False):
     shapes = [S.box((0.4, 0.3, 0.5)), S.capsule(0.2, 0.3), S.convex_hull(SC._unit_sphere_points(rng, 16) * 0.45)]
     if spheres:
         shapes.append(S.sphere(0.35))
+    if cylinders:
+        shapes.append(S.cylinder(0.3, 0.35))
     for i in range(n):
         x = rng.uniform(-1.5, 1.5)
         z = rng.uniform(-2.5, 2.5)
@@ -297,6 +299,51 @@ def test_fp64_mesh_valley_bitwise():
 def test_fp64_terrain_drop_bitwise():
     sc = SC.terrain_drop(vertices=24, n=40, drop=(0.5, 3.0))
     _lockstep_fp64(sc, 120, check_every=4)
+
+
+def test_fp64_cylinders_bitwise():
+    """Cylinders (cylinder.ts: support in both radius modes, side edge and cap octagon faces)
+    among the other convex shapes on a floor, and against the valley's triangles."""
+    _lockstep_fp64(SC.mixed_pile(seed=1005, n=300, extent=(10.0, 8.0, 10.0), cylinders=True), 60, check_every=3)
+    _lockstep_fp64(_valley(n=8, cylinders=True), 90, check_every=1)
+
+
+@pytest.mark.parametrize("seed", [1006, 1007])
+def test_fp32_cylinder_pile_step_tolerance(seed):
+    """FP32 against the FP64 oracle, one step from a settling pile with cylinders (500 bodies,
+    40 oracle steps in), fresh caches on both sides.  The cylinder adds discrete choices that
+    the two precisions can take apart near their thresholds: side edge or cap octagon for a
+    direction near 45 degrees to the axis, the octagon turned or not near 5 degrees
+    (cylinder.ts:175-212), and EPA stopping within its tolerance on the curved side; an island
+    where one such choice differs spreads the difference over its bodies.  Tolerances: 97% of
+    the bodies within 1e-4 m, the 99th percentile under 5e-4 m, every body within 2e-3 m
+    (measured, seed 1006: 98.6%, 1.4e-4 m, 5.9e-4 m, about 7 bodies of one island; seed 1007:
+    100%, 4.3e-6 m, 4.8e-5 m; without cylinders both seeds agree to 1.2e-6 m)."""
+    sc = SC.mixed_pile(seed=seed, n=500, extent=(12.0, 8.0, 12.0), cylinders=True)
+    run = O.OracleWorld(sc)
+    for _ in range(40):
+        run.step(DT)
+    st = run.bodies()
+    orc = O.OracleWorld(sc)
+    orc.set_bodies(st)
+    g = Context("fp32", max_pairs_per_body=16)
+    g.upload(sc)
+    try:
+        g.set_bodies(st)
+        orc.step(DT)
+        g.step(DT, 1)
+        g.sync()
+        assert np.array_equal(orc.pairs(), g.pairs())
+        ob, gb = orc.bodies(), g.bodies()
+        dyn = sc.bodies()["motion_type"] == 2
+        err = np.linalg.norm(ob["position"] - gb["position"], axis=1)[dyn]
+        print(f"fp32 cylinder pile step: p99 {np.quantile(err, 0.99):.3g} max {err.max():.3g} "
+              f"frac<=1e-4 {np.mean(err <= 1e-4):.4f}")
+        assert np.mean(err <= 1e-4) >= 0.97
+        assert np.quantile(err, 0.99) <= 5e-4
+        assert err.max() <= 2e-3
+    finally:
+        g.close()
 
 
 def test_fp64_spheres_on_meshes_bitwise():
