@@ -7,6 +7,7 @@ CC_SHAPE_CAPSULE = 2
 CC_SHAPE_CONVEX_HULL = 3
 CC_SHAPE_TRIANGLE_MESH = 4
 CC_SHAPE_CYLINDER = 5
+CC_SHAPE_PLANE = 6
 
 CC_MOTION_STATIC = 0
 CC_MOTION_KINEMATIC = 1

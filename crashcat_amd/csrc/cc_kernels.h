@@ -797,7 +797,7 @@ __device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, c
     finish_pair(d, slot, A, Bi, RA, RB, wA, wB, wAxis, S.fA, S.fB, S.tmp, S.m, dt, par);
 }
 
-enum { HIT_SPHERES = 0, HIT_GJK = 1, HIT_EPA = 2 };
+enum { HIT_SPHERES = 0, HIT_GJK = 1, HIT_EPA = 2, HIT_PLANE = 3 };
 
 // Hits whose two shapes have supporting faces of at most four vertices go to the front of the
 // hit list (manifolds of capacity 8), the others to its back (capacity 32).
@@ -871,6 +871,20 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
                     T dist = sqrt(d2);
                     V3<T> n = dist > T(0) ? dd * (T(1) / dist) : mk3<T>(0, 1, 0);
                     push_hit(d, slot, HIT_SPHERES, pa + n * ra, pbp - n * rb, n, true);
+                }
+            } else if (ta == SHAPE_PLANE || tb == SHAPE_PLANE) {
+                // collideConvexVsPlane and its reverse (plane.ts:297-310); nothing against a
+                // plane or a mesh.  The hit is inverted when the plane is the narrowphase's body A.
+                if (ta != tb && ta != SHAPE_MESH && tb != SHAPE_MESH) {
+                    const bool rev = ta == SHAPE_PLANE;
+                    const NpBody<T>& RC = rev ? RB : RA;
+                    const NpBody<T>& RP = rev ? RA : RB;
+                    PlaneHit<T> ph;
+                    if (plane_hit(d.st, RC.shape, RC.pos, RC.q, RP.shape, RP.pos, RP.q, d.s.spec, ph)) {
+                        const bool sm = (RC.tinfo >> 4) <= 4;
+                        if (!rev) push_hit(d, slot, HIT_PLANE, ph.pA, ph.pB, ph.axis, sm);
+                        else push_hit(d, slot, HIT_PLANE, ph.pB, ph.pA, -ph.axis, sm);
+                    }
                 }
             } else if (ta == SHAPE_MESH || tb == SHAPE_MESH) {
                 mesh = ta != tb;  // no mesh-vs-mesh function
@@ -1022,6 +1036,20 @@ __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
         if (kind == HIT_SPHERES) {
             S.fA.n = 0;
             S.fB.n = 0;
+            finish_pair(d, slot, A, Bi, RA, RB, d.hitPA[h], d.hitPB[h], d.hitAxis[h], S.fA, S.fB, S.tmp, S.m, dt, par);
+            continue;
+        }
+        if (kind == HIT_PLANE) {  // faces of collideConvexVsPlane (plane.ts:550-575), swapped when reversed
+            const bool rev = (RA.tinfo & 15) == SHAPE_PLANE;
+            const NpBody<T>& RC = rev ? RB : RA;
+            const NpBody<T>& RP = rev ? RA : RB;
+            PlaneHit<T> ph;
+            plane_hit(d.st, RC.shape, RC.pos, RC.q, RP.shape, RP.pos, RP.q, T(INFINITY), ph);
+            Face<T, N>& fC = rev ? S.fB : S.fA;
+            Face<T, N>& fP = rev ? S.fA : S.fB;
+            supporting_face(fC, d.st, RC.shape, ph.nl, RC.pos, RC.q);
+            if (fC.n > 0) plane_face(fP, d.st, RC.shape, RC.pos, RP.shape, RP.pos, RP.q);
+            else fP.n = 0;
             finish_pair(d, slot, A, Bi, RA, RB, d.hitPA[h], d.hitPB[h], d.hitAxis[h], S.fA, S.fB, S.tmp, S.m, dt, par);
             continue;
         }

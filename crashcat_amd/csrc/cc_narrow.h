@@ -7,7 +7,7 @@
 
 namespace cc {
 
-enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3, SHAPE_MESH = 4, SHAPE_CYLINDER = 5 };
+enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3, SHAPE_MESH = 4, SHAPE_CYLINDER = 5, SHAPE_PLANE = 6 };
 
 // A BVH node of a triangle mesh: bounds, then for an internal node the right child (node
 // index within the mesh; the left child is the next node) and the split axis, for a leaf the
@@ -1193,4 +1193,148 @@ __device__ inline void convex_setup(ConvexCtx<T>& c, V3<T> pa, Q4<T> qa, V3<T> p
     c.maxSepUse = maxSep;
 }
 
+}  // namespace cc
+
+// ---------------------------------------------------------------- planes (src/shapes/plane.ts)
+// A plane shape: he = unit normal, radius = constant (points p with n.p + constant = 0; the
+// solid side is n.p + constant < 0), hh = half extent of its bounded form.  The transforms
+// below follow mathcat's mat4 (column-major) as collideConvexVsPlane uses it.
+namespace cc {
+template <typename T> struct M4 { T m[16]; };
+
+template <typename T>
+__device__ inline M4<T> m4_from_rt(Q4<T> q, V3<T> v) {  // mat4.fromRotationTranslation
+    const T x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
+    const T xx = q.x * x2, xy = q.x * y2, xz = q.x * z2, yy = q.y * y2, yz = q.y * z2, zz = q.z * z2;
+    const T wx = q.w * x2, wy = q.w * y2, wz = q.w * z2;
+    M4<T> o;
+    o.m[0] = T(1) - (yy + zz); o.m[1] = xy + wz; o.m[2] = xz - wy; o.m[3] = T(0);
+    o.m[4] = xy - wz; o.m[5] = T(1) - (xx + zz); o.m[6] = yz + wx; o.m[7] = T(0);
+    o.m[8] = xz + wy; o.m[9] = yz - wx; o.m[10] = T(1) - (xx + yy); o.m[11] = T(0);
+    o.m[12] = v.x; o.m[13] = v.y; o.m[14] = v.z; o.m[15] = T(1);
+    return o;
+}
+
+template <typename T>
+__device__ inline M4<T> m4_invert(const M4<T>& A) {  // mat4.invert (cofactors over the determinant)
+    const T* a = A.m;
+    const T a00 = a[0], a01 = a[1], a02 = a[2], a03 = a[3], a10 = a[4], a11 = a[5], a12 = a[6], a13 = a[7];
+    const T a20 = a[8], a21 = a[9], a22 = a[10], a23 = a[11], a30 = a[12], a31 = a[13], a32 = a[14], a33 = a[15];
+    const T b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10, b02 = a00 * a13 - a03 * a10;
+    const T b03 = a01 * a12 - a02 * a11, b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+    const T b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30, b08 = a20 * a33 - a23 * a30;
+    const T b09 = a21 * a32 - a22 * a31, b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+    T det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+    M4<T> o;
+    det = T(1) / det;
+    o.m[0] = (a11 * b11 - a12 * b10 + a13 * b09) * det;
+    o.m[1] = (a02 * b10 - a01 * b11 - a03 * b09) * det;
+    o.m[2] = (a31 * b05 - a32 * b04 + a33 * b03) * det;
+    o.m[3] = (a22 * b04 - a21 * b05 - a23 * b03) * det;
+    o.m[4] = (a12 * b08 - a10 * b11 - a13 * b07) * det;
+    o.m[5] = (a00 * b11 - a02 * b08 + a03 * b07) * det;
+    o.m[6] = (a32 * b02 - a30 * b05 - a33 * b01) * det;
+    o.m[7] = (a20 * b05 - a22 * b02 + a23 * b01) * det;
+    o.m[8] = (a10 * b10 - a11 * b08 + a13 * b06) * det;
+    o.m[9] = (a01 * b08 - a00 * b10 - a03 * b06) * det;
+    o.m[10] = (a30 * b04 - a31 * b02 + a33 * b00) * det;
+    o.m[11] = (a21 * b02 - a20 * b04 - a23 * b00) * det;
+    o.m[12] = (a11 * b07 - a10 * b09 - a12 * b06) * det;
+    o.m[13] = (a00 * b09 - a01 * b07 + a02 * b06) * det;
+    o.m[14] = (a31 * b01 - a30 * b03 - a32 * b00) * det;
+    o.m[15] = (a20 * b03 - a21 * b01 + a22 * b00) * det;
+    return o;
+}
+
+template <typename T>
+__device__ inline M4<T> m4_mul(const M4<T>& A, const M4<T>& B) {  // mat4.multiply: A * B
+    M4<T> o;
+    for (int j = 0; j < 4; j++) {
+        const T b0 = B.m[4 * j], b1 = B.m[4 * j + 1], b2 = B.m[4 * j + 2], b3 = B.m[4 * j + 3];
+        for (int i = 0; i < 4; i++) o.m[4 * j + i] = b0 * A.m[i] + b1 * A.m[4 + i] + b2 * A.m[8 + i] + b3 * A.m[12 + i];
+    }
+    return o;
+}
+
+template <typename T>
+__device__ inline V3<T> m4_point(const M4<T>& M, V3<T> p) {  // vec3.transformMat4
+    const T* m = M.m;
+    T w = m[3] * p.x + m[7] * p.y + m[11] * p.z + m[15];
+    if (w == T(0)) w = T(1);
+    return mk3<T>((m[0] * p.x + m[4] * p.y + m[8] * p.z + m[12]) / w, (m[1] * p.x + m[5] * p.y + m[9] * p.z + m[13]) / w,
+                  (m[2] * p.x + m[6] * p.y + m[10] * p.z + m[14]) / w);
+}
+
+// transformPlane (plane.ts:196-208) of the unit-scale plane (scalePlane :184-194 renormalises it)
+template <typename T>
+__device__ inline void plane_in(const ShapeD<T>& P, const M4<T>& M, V3<T>& n, T& c) {
+    const V3<T> n0 = mk3<T>(P.he[0], P.he[1], P.he[2]);
+    const T l = vlen(n0);
+    const V3<T> sn = n0 * (T(1) / l);
+    const T sc = P.radius / l;
+    const T* m = M.m;
+    n = mk3<T>(m[0] * sn.x + m[4] * sn.y + m[8] * sn.z, m[1] * sn.x + m[5] * sn.y + m[9] * sn.z,
+               m[2] * sn.x + m[6] * sn.y + m[10] * sn.z);
+    c = sc - (m[12] * n.x + m[13] * n.y + m[14] * n.z);
+}
+
+template <typename T>
+struct PlaneHit { V3<T> pA, pB, axis, nl; T pen; };
+
+// collideConvexVsPlane (plane.ts:474-590): convex shape sc at (pc, qc) against plane shape sp at
+// (pp, qp), with the convex shape's DEFAULT support (radius included for boxes and cylinders,
+// excluded and added back for the others, support.ts:340-368); false when separated by more than
+// maxSep.  Points and the axis (from the convex shape toward the plane) in world space; nl is
+// the plane normal in the convex shape's frame.
+template <typename T>
+__device__ inline bool plane_hit(const ShapeTable<T>& st, int sc, V3<T> pc, Q4<T> qc, int sp, V3<T> pp, Q4<T> qp,
+                                 T maxSep, PlaneHit<T>& h) {
+    const M4<T> M = m4_mul(m4_invert(m4_from_rt(qc, pc)), m4_from_rt(qp, pp));
+    V3<T> n;
+    T c;
+    plane_in(st.shapes[sp], M, n, c);
+    const int ty = st.shapes[sc].type;
+    Sup<T> S;
+    sup_for_shape(S, st, sc, ty == SHAPE_BOX || ty == SHAPE_CYLINDER);
+    const V3<T> s = sup_base(S, -n);
+    const T dist = dot(n, s) + c;
+    const T cr = S.cr;
+    const T pen = -dist + cr;
+    if (!(pen > -maxSep)) return false;
+    const V3<T> p1 = s - mk3<T>(n.x * cr, n.y * cr, n.z * cr);
+    const V3<T> p2 = s - mk3<T>(n.x * dist, n.y * dist, n.z * dist);
+    const M4<T> TA = m4_from_rt(qc, pc);
+    h.pA = m4_point(TA, p1);
+    h.pB = m4_point(TA, p2);
+    h.axis = qrotate(qc, -n);
+    h.pen = pen;
+    h.nl = n;
+    return true;
+}
+
+// getAdaptivePlaneSupportingFace (plane.ts:631-725): a quad on the plane about the convex
+// shape's position, its half size the diagonal of the convex shape's local bounds
+template <typename T, int N>
+__device__ inline void plane_face(Face<T, N>& f, const ShapeTable<T>& st, int sc, V3<T> pc, int sp, V3<T> pp, Q4<T> qp) {
+    V3<T> n;
+    T c;
+    plane_in(st.shapes[sp], m4_from_rt(qp, pp), n, c);
+    const T dd = dot(n, pc) + c;  // plane3.projectPoint
+    const V3<T> ctr = pc - mk3<T>(n.x * dd, n.y * dd, n.z * dd);
+    V3<T> p1 = cross(n, mk3<T>(0, 1, 0));  // getOrthogonalBasis (plane.ts:210-233)
+    const T l2 = len2(p1);
+    if (l2 < T(1e-6)) p1 = mk3<T>(1, 0, 0);
+    else { const T k = T(1) / sqrt(l2); p1 = mk3<T>(p1.x * k, p1.y * k, p1.z * k); }
+    V3<T> p2 = normalize(cross(p1, n));
+    p1 = normalize(cross(n, p2));
+    const ShapeD<T>& S = st.shapes[sc];
+    const T size = vlen(mk3<T>(S.amax[0] - S.amin[0], S.amax[1] - S.amin[1], S.amax[2] - S.amin[2]));
+    p1 = mk3<T>(p1.x * size, p1.y * size, p1.z * size);
+    p2 = mk3<T>(p2.x * size, p2.y * size, p2.z * size);
+    f.n = 4;
+    f.v[0] = ctr + (p1 + p2);
+    f.v[1] = ctr + (p1 - p2);
+    f.v[2] = ctr - (p1 + p2);
+    f.v[3] = ctr + (-p1 + p2);
+}
 }  // namespace cc

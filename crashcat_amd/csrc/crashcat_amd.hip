@@ -330,7 +330,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     std::vector<ShapeD<T>> sh(std::max(ns, 1));
     for (int i = 0; i < ns; i++) {
         const cc_shape_desc& q = shapes[i];
-        if (q.type < 0 || q.type > 5) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
+        if (q.type < 0 || q.type > 6) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
         if (q.type == CC_SHAPE_TRIANGLE_MESH) {
             const int MT = meshes ? meshes->total_triangles : 0, MN = meshes ? meshes->total_nodes : 0;
             if (q.mesh_first_triangle < 0 || q.mesh_num_triangles < 0 || q.mesh_first_node < 0 || q.mesh_num_nodes < 0 ||
@@ -341,7 +341,8 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         ShapeD<T>& o = sh[i];
         o.type = q.type; o.hfp = q.hull_first_point; o.hnp = q.hull_num_points; o.hff = q.hull_first_face;
         o.hnf = q.hull_num_faces;
-        o.mfv = q.type == CC_SHAPE_BOX ? 4 : (q.type == CC_SHAPE_CAPSULE ? 2 : (q.type == CC_SHAPE_CYLINDER ? 8 : 0));
+        o.mfv = q.type == CC_SHAPE_BOX || q.type == CC_SHAPE_PLANE ? 4
+                : (q.type == CC_SHAPE_CAPSULE ? 2 : (q.type == CC_SHAPE_CYLINDER ? 8 : 0));
         if (q.type == CC_SHAPE_TRIANGLE_MESH) {
             o.hfp = q.mesh_first_triangle; o.hnp = q.mesh_num_triangles;
             o.hff = q.mesh_first_node; o.hnf = q.mesh_num_nodes;
@@ -361,6 +362,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         else if (q.type == CC_SHAPE_BOX) br = std::sqrt(he2) + q.convex_radius;
         else if (q.type == CC_SHAPE_CAPSULE) br = q.half_height + q.radius;
         else if (q.type == CC_SHAPE_CYLINDER) br = std::sqrt(q.radius * q.radius + q.half_height * q.half_height);
+        else if (q.type == CC_SHAPE_PLANE) br = 1e30;
         else if (hulls && hulls->points && hulls->shrunk_points) {
             for (int k = 0; k < q.hull_num_points && q.hull_first_point + k < hulls->total_points; k++) {
                 const double* pp = hulls->points + 3 * (q.hull_first_point + k);
@@ -476,6 +478,8 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         if (b.object_layer < 0 || b.object_layer >= CC_MAX_OBJECT_LAYERS) return fail(c, CC_E_ARG, "invalid object layer");
         if (X->hasMesh && shapes[b.shape].type == CC_SHAPE_TRIANGLE_MESH && b.motion_type != CC_MOTION_STATIC)
             return fail(c, CC_E_UNSUPPORTED, "triangle-mesh bodies must be static");
+        if (shapes[b.shape].type == CC_SHAPE_PLANE && b.motion_type != CC_MOTION_STATIC)
+            return fail(c, CC_E_UNSUPPORTED, "plane bodies must be static (plane.ts:53-57)");
         pos[i] = mk3<T>(T(b.position[0]), T(b.position[1]), T(b.position[2]));
         q[i].x = T(b.quaternion[0]); q[i].y = T(b.quaternion[1]); q[i].z = T(b.quaternion[2]); q[i].w = T(b.quaternion[3]);
         v[i] = mk3<T>(T(b.linear_velocity[0]), T(b.linear_velocity[1]), T(b.linear_velocity[2]));

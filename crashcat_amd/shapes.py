@@ -29,6 +29,8 @@ CAPSULE = _abi.CC_SHAPE_CAPSULE
 CONVEX_HULL = _abi.CC_SHAPE_CONVEX_HULL
 TRIANGLE_MESH = _abi.CC_SHAPE_TRIANGLE_MESH
 CYLINDER = _abi.CC_SHAPE_CYLINDER
+PLANE = _abi.CC_SHAPE_PLANE
+DEFAULT_PLANE_HALF_EXTENT = 1000.0  # src/shapes/plane.ts
 
 
 class Shape:
@@ -130,6 +132,39 @@ def cylinder(radius, half_height, convex_radius=DEFAULT_CONVEX_RADIUS, density=D
     ixx = 0.5 * iyy + (mass * 4 * h * h) / 12
     return Shape(CYLINDER, radius=r, half_height=h, convex_radius=cr, density=density, aabb_min=(-r, -h, -r),
                  aabb_max=(r, h, r), mass=mass, inertia=np.diag([ixx, iyy, ixx]), volume=volume)
+
+
+# ------------------------------------------------------------------------- plane
+def _plane_basis(n):
+    """getOrthogonalBasis (src/shapes/plane.ts:210-233)."""
+    p1 = np.cross(n, (0.0, 1.0, 0.0))
+    l2 = float(p1 @ p1)
+    p1 = np.array([1.0, 0.0, 0.0]) if l2 < 1e-6 else p1 * (1 / math.sqrt(l2))
+    p2 = np.cross(p1, n)
+    p2 = p2 * (1 / math.sqrt(float(p2 @ p2)))
+    p1 = np.cross(n, p2)
+    p1 = p1 * (1 / math.sqrt(float(p1 @ p1)))
+    return p1, p2
+
+
+def plane(normal, constant=0.0, half_extent=DEFAULT_PLANE_HALF_EXTENT):
+    """plane.create({plane: {normal, constant}, halfExtent}) — src/shapes/plane.ts:77-118: the points p
+    with normal . p + constant = 0, solid below; static bodies only.  Local bounds: the bounded
+    quad and the same quad half_extent behind it (computePlaneLocalBounds :273-295)."""
+    n = np.asarray(normal, dtype=np.float64)
+    l = float(n @ n)
+    if l > 0:
+        n = n * (1 / math.sqrt(l))
+    c = float(constant)
+    h = float(half_extent)
+    p1, p2 = _plane_basis(n)
+    p1, p2 = p1 * h, p2 * h
+    pt = n * -c
+    verts = [pt + (p1 + p2), pt + (p1 - p2), pt - (p1 + p2), pt + (-p1 + p2)]
+    pts = np.array(verts + [v - n * h for v in verts])
+    return Shape(PLANE, half_extents=tuple(float(x) for x in n), radius=c, half_height=h, convex_radius=0.0,
+                 aabb_min=tuple(pts.min(axis=0)), aabb_max=tuple(pts.max(axis=0)), mass=0.0,
+                 inertia=np.zeros((3, 3)), volume=0.0)
 
 
 # ------------------------------------------------------------------- convex hull

@@ -39,6 +39,8 @@ extern "C" {
 #define CC_SHAPE_CONVEX_HULL 3
 #define CC_SHAPE_TRIANGLE_MESH 4
 #define CC_SHAPE_CYLINDER 5      /* src/shapes/cylinder.ts: radius, half_height, convex_radius; axis y */
+#define CC_SHAPE_PLANE 6         /* src/shapes/plane.ts (static bodies only): half_extents = unit normal,
+                                    radius = constant, half_height = half extent of the bounded form */
 
 /* ---- motion types: src/body/motion-type.ts ------------------------------ */
 #define CC_MOTION_STATIC 0

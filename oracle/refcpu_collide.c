@@ -1368,11 +1368,132 @@ void ocx_collide_sphere_mesh(const OShapes* sh, int sa, V3 pa, int sm, V3 pm, Q4
     }
 }
 
+/* ---- planes (src/shapes/plane.ts), with mathcat's column-major mat4 ----------------------- */
+typedef struct { double m[16]; } OM4;
+static OM4 m4_from_rt(Q4 q, V3 v) { /* mat4.fromRotationTranslation */
+    double x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
+    double xx = q.x * x2, xy = q.x * y2, xz = q.x * z2, yy = q.y * y2, yz = q.y * z2, zz = q.z * z2;
+    double wx = q.w * x2, wy = q.w * y2, wz = q.w * z2;
+    OM4 o = {{1 - (yy + zz), xy + wz, xz - wy, 0, xy - wz, 1 - (xx + zz), yz + wx, 0, xz + wy, yz - wx, 1 - (xx + yy), 0,
+              v.x, v.y, v.z, 1}};
+    return o;
+}
+static OM4 m4_invert(const OM4* A) { /* mat4.invert */
+    const double* a = A->m;
+    double a00 = a[0], a01 = a[1], a02 = a[2], a03 = a[3], a10 = a[4], a11 = a[5], a12 = a[6], a13 = a[7];
+    double a20 = a[8], a21 = a[9], a22 = a[10], a23 = a[11], a30 = a[12], a31 = a[13], a32 = a[14], a33 = a[15];
+    double b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10, b02 = a00 * a13 - a03 * a10;
+    double b03 = a01 * a12 - a02 * a11, b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+    double b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30, b08 = a20 * a33 - a23 * a30;
+    double b09 = a21 * a32 - a22 * a31, b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+    double det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+    det = 1.0 / det;
+    OM4 o = {{(a11 * b11 - a12 * b10 + a13 * b09) * det, (a02 * b10 - a01 * b11 - a03 * b09) * det,
+              (a31 * b05 - a32 * b04 + a33 * b03) * det, (a22 * b04 - a21 * b05 - a23 * b03) * det,
+              (a12 * b08 - a10 * b11 - a13 * b07) * det, (a00 * b11 - a02 * b08 + a03 * b07) * det,
+              (a32 * b02 - a30 * b05 - a33 * b01) * det, (a20 * b05 - a22 * b02 + a23 * b01) * det,
+              (a10 * b10 - a11 * b08 + a13 * b06) * det, (a01 * b08 - a00 * b10 - a03 * b06) * det,
+              (a30 * b04 - a31 * b02 + a33 * b00) * det, (a21 * b02 - a20 * b04 - a23 * b00) * det,
+              (a11 * b07 - a10 * b09 - a12 * b06) * det, (a00 * b09 - a01 * b07 + a02 * b06) * det,
+              (a31 * b01 - a30 * b03 - a32 * b00) * det, (a20 * b03 - a21 * b01 + a22 * b00) * det}};
+    return o;
+}
+static OM4 m4_mul(const OM4* A, const OM4* B) { /* mat4.multiply: A * B */
+    OM4 o;
+    for (int j = 0; j < 4; j++) {
+        double b0 = B->m[4 * j], b1 = B->m[4 * j + 1], b2 = B->m[4 * j + 2], b3 = B->m[4 * j + 3];
+        for (int i = 0; i < 4; i++) o.m[4 * j + i] = b0 * A->m[i] + b1 * A->m[4 + i] + b2 * A->m[8 + i] + b3 * A->m[12 + i];
+    }
+    return o;
+}
+static V3 m4_point(const OM4* M, V3 p) { /* vec3.transformMat4 */
+    const double* m = M->m;
+    double w = m[3] * p.x + m[7] * p.y + m[11] * p.z + m[15];
+    if (w == 0) w = 1;
+    return v3((m[0] * p.x + m[4] * p.y + m[8] * p.z + m[12]) / w, (m[1] * p.x + m[5] * p.y + m[9] * p.z + m[13]) / w,
+              (m[2] * p.x + m[6] * p.y + m[10] * p.z + m[14]) / w);
+}
+/* scalePlane at unit scale (:184-194), then transformPlane (:196-208) */
+static void plane_in(const cc_shape_desc* P, const OM4* M, V3* n, double* c) {
+    V3 n0 = vload(P->half_extents);
+    double l = vlen(n0);
+    V3 sn = vscale(n0, 1 / l);
+    double sc = P->radius / l;
+    const double* m = M->m;
+    *n = v3(m[0] * sn.x + m[4] * sn.y + m[8] * sn.z, m[1] * sn.x + m[5] * sn.y + m[9] * sn.z,
+            m[2] * sn.x + m[6] * sn.y + m[10] * sn.z);
+    *c = sc - (m[12] * n->x + m[13] * n->y + m[14] * n->z);
+}
+/* collideConvexVsPlane src/shapes/plane.ts:474-590, with the adaptive plane face :631-725 */
+static void collide_convex_plane(const OShapes* sh, int sc, V3 pc, Q4 qc, int sp, V3 pp, Q4 qp, double maxSep,
+                                 OHit* out) {
+    OM4 TA = m4_from_rt(qc, pc), TB = m4_from_rt(qp, pp);
+    OM4 iA = m4_invert(&TA);
+    OM4 M = m4_mul(&iA, &TB);
+    V3 n;
+    double c;
+    plane_in(&sh->shapes[sp], &M, &n, &c);
+    int ty = sh->shapes[sc].type;
+    OSup S;
+    sup_for_shape(&S, sh, sc, (ty == CC_SHAPE_BOX || ty == CC_SHAPE_CYLINDER) ? MODE_INCLUDE : MODE_EXCLUDE);
+    V3 s = sup_base(&S, vneg(n));
+    double dist = vdot(n, s) + c;
+    double cr = S.convexRadius;
+    double pen = -dist + cr;
+    out->hit = 0;
+    out->faceA.n = out->faceB.n = 0;
+    if (!(pen > -maxSep)) return;
+    V3 p1 = vsub(s, v3(n.x * cr, n.y * cr, n.z * cr));
+    V3 p2 = vsub(s, v3(n.x * dist, n.y * dist, n.z * dist));
+    out->hit = 1;
+    out->status = 0;
+    out->pointA = m4_point(&TA, p1);
+    out->pointB = m4_point(&TA, p2);
+    out->axis = qrotate(qc, vneg(n));
+    out->penetration = pen;
+    supporting_face(&out->faceA, sh, sc, n, pc, qc);
+    if (out->faceA.n > 0) {
+        V3 wn;
+        double wc;
+        plane_in(&sh->shapes[sp], &TB, &wn, &wc);
+        double dd = vdot(wn, pc) + wc; /* plane3.projectPoint */
+        V3 ctr = vsub(pc, v3(wn.x * dd, wn.y * dd, wn.z * dd));
+        V3 b1 = vcross(wn, v3(0, 1, 0)); /* getOrthogonalBasis :210-233 */
+        double l2 = vlen2(b1);
+        if (l2 < 1e-6) b1 = v3(1, 0, 0);
+        else { double k = 1 / sqrt(l2); b1 = v3(b1.x * k, b1.y * k, b1.z * k); }
+        V3 b2 = vnormalize(vcross(b1, wn));
+        b1 = vnormalize(vcross(wn, b2));
+        const cc_shape_desc* C = &sh->shapes[sc];
+        double size = vlen(v3(C->aabb_max[0] - C->aabb_min[0], C->aabb_max[1] - C->aabb_min[1], C->aabb_max[2] - C->aabb_min[2]));
+        b1 = v3(b1.x * size, b1.y * size, b1.z * size);
+        b2 = v3(b2.x * size, b2.y * size, b2.z * size);
+        out->faceB.n = 4;
+        out->faceB.v[0] = vadd(ctr, vadd(b1, b2));
+        out->faceB.v[1] = vadd(ctr, vsub(b1, b2));
+        out->faceB.v[2] = vsub(ctr, vadd(b1, b2));
+        out->faceB.v[3] = vadd(ctr, vadd(vneg(b1), b2));
+    }
+}
+
 /* collideShapeVsShape dispatch (src/collision/narrowphase.ts:172-238, src/shapes/sphere.ts:154-167) */
 void ocx_collide(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, double maxSep, double colTol,
                  double penTol, OHit* out) {
     out->faceA.n = out->faceB.n = 0;
-    if (sh->shapes[sa].type == CC_SHAPE_SPHERE && sh->shapes[sb].type == CC_SHAPE_SPHERE)
+    int ta = sh->shapes[sa].type, tb = sh->shapes[sb].type;
+    if (ta == CC_SHAPE_PLANE || tb == CC_SHAPE_PLANE) { /* plane.ts:297-310 */
+        out->hit = 0;
+        if (ta == tb) return;
+        if (tb == CC_SHAPE_PLANE) { collide_convex_plane(sh, sa, pa, qa, sb, pb, qb, maxSep, out); return; }
+        OHit h;
+        collide_convex_plane(sh, sb, pb, qb, sa, pa, qa, maxSep, &h); /* reversedCollideShapeVsShape */
+        *out = h;
+        out->pointA = h.pointB; out->pointB = h.pointA;
+        out->axis = vneg(h.axis);
+        out->faceA = h.faceB; out->faceB = h.faceA;
+        return;
+    }
+    if (ta == CC_SHAPE_SPHERE && tb == CC_SHAPE_SPHERE)
         collide_sphere_sphere(sh, sa, pa, sb, pb, out);
     else
         collide_convex_convex(sh, sa, pa, qa, sb, pb, qb, maxSep, colTol, penTol, out);

@@ -747,6 +747,8 @@ static int narrowphase_pair(oc_world* w, int ia, int ib, double dt) {
     C->base = A->pos;
     C->n = 0;
     int ta = w->shapes[A->shape].type, tb = w->shapes[B->shape].type;
+    if ((ta == CC_SHAPE_PLANE || tb == CC_SHAPE_PLANE) && (ta == CC_SHAPE_TRIANGLE_MESH || tb == CC_SHAPE_TRIANGLE_MESH))
+        return 0; /* no plane-vs-mesh function */
     if (ta == CC_SHAPE_TRIANGLE_MESH || tb == CC_SHAPE_TRIANGLE_MESH) {
         if (ta == CC_SHAPE_TRIANGLE_MESH && tb == CC_SHAPE_TRIANGLE_MESH) return 0; /* no mesh-vs-mesh function */
         V3 move = vsub(A->v, B->v); /* activeEdgeMovementDirection :730-735 */

@@ -4,6 +4,8 @@ FP64 plain path: bit-identical bodies, pairs, contacts (points, normals, impulse
 same island partition as the oracle, step after step.  FP32 batched path: the same pair set
 from the same boxes, and per-step body error within the tolerances stated below.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -341,6 +343,60 @@ def test_fp32_cylinder_pile_step_tolerance(seed):
               f"frac<=1e-4 {np.mean(err <= 1e-4):.4f}")
         assert np.mean(err <= 1e-4) >= 0.97
         assert np.quantile(err, 0.99) <= 5e-4
+        assert err.max() <= 2e-3
+    finally:
+        g.close()
+
+
+def _on_plane(sc, quat=(0.0, 0.0, 0.0, 1.0)):
+    """The scene with its box floor (body 0, top face at y = 0) replaced by a plane body through
+    the origin with normal +y, turned by `quat`."""
+    from crashcat_amd import shapes as S
+    row = list(sc.rows[0])
+    row[0] = sc.add_shape(S.plane((0.0, 1.0, 0.0)))
+    row[10] = (0.0, 0.0, 0.0)
+    row[11] = tuple(quat)
+    sc.rows[0] = tuple(row)
+    sc._bodies = None
+    return sc
+
+
+def test_fp64_plane_floors_bitwise():
+    """collideConvexVsPlane and the adaptive plane face (plane.ts:474-725): a pile of every convex
+    shape on a plane floor, and a heap on a plane tilted 10 degrees (plane body rotated)."""
+    _lockstep_fp64(_on_plane(SC.mixed_pile(seed=1005, n=300, extent=(10.0, 8.0, 10.0), cylinders=True)), 60,
+                   check_every=3)
+    a = math.radians(10)
+    _lockstep_fp64(_on_plane(SC.cube_heap(seed=1001, nx=3, ny=3, nz=3), (math.sin(a / 2), 0.0, 0.0, math.cos(a / 2))),
+                   90, check_every=3)
+
+
+def test_fp32_plane_floor_step_tolerance():
+    """FP32 against the FP64 oracle, one step from a settling 500-body pile on a plane floor, 40
+    oracle steps in, fresh caches: 99% of the bodies within 1e-4 m, p99 under 2e-5 m, every
+    body within 2e-3 m."""
+    sc = _on_plane(SC.mixed_pile(seed=1008, n=500, extent=(12.0, 8.0, 12.0)))
+    run = O.OracleWorld(sc)
+    for _ in range(40):
+        run.step(DT)
+    st = run.bodies()
+    orc = O.OracleWorld(sc)
+    orc.set_bodies(st)
+    g = Context("fp32", max_pairs_per_body=16)
+    g.upload(sc)
+    try:
+        g.set_bodies(st)
+        orc.step(DT)
+        g.step(DT, 1)
+        g.sync()
+        assert np.array_equal(orc.pairs(), g.pairs())
+        ob, gb = orc.bodies(), g.bodies()
+        dyn = sc.bodies()["motion_type"] == 2
+        err = np.linalg.norm(ob["position"] - gb["position"], axis=1)[dyn]
+        print(f"fp32 plane floor step: p99 {np.quantile(err, 0.99):.3g} max {err.max():.3g} "
+              f"frac<=1e-4 {np.mean(err <= 1e-4):.4f}")
+        assert np.mean(err <= 1e-4) >= 0.99
+        assert np.quantile(err, 0.99) <= 2e-5
         assert err.max() <= 2e-3
     finally:
         g.close()
