@@ -103,6 +103,8 @@ struct Dev {
     int* pairA;         // [pairCap]
     int* pairB;
     int* counters;      // device counters, see CNT_*
+    int hasPlane;       // a plane shape is uploaded
+    int* planeList;     // [pairCap] pair slots of convex-vs-plane pairs (when hasPlane)
     // contact constraints (slots allocated in creation order by an atomic counter)
     Con<T>* con;        // [conCap]
     ConHead* conHead;   // [conCap]
@@ -173,7 +175,7 @@ enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
        ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_N = 16 };
 
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
-       CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_MESH = 10, CNT_N = 11 };
+       CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_MESH = 10, CNT_PLANE = 11, CNT_N = 12 };
 // CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list, 16 hits,
 // 32 mesh BVH traversal stack, 64 mesh candidate triangles
 constexpr int BP_SLOTS = 32;
@@ -797,7 +799,7 @@ __device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, c
     finish_pair(d, slot, A, Bi, RA, RB, wA, wB, wAxis, S.fA, S.fB, S.tmp, S.m, dt, par);
 }
 
-enum { HIT_SPHERES = 0, HIT_GJK = 1, HIT_EPA = 2, HIT_PLANE = 3 };
+enum { HIT_SPHERES = 0, HIT_GJK = 1, HIT_EPA = 2 };
 
 // Hits whose two shapes have supporting faces of at most four vertices go to the front of the
 // hit list (manifolds of capacity 8), the others to its back (capacity 32).
@@ -855,7 +857,7 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
     const int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
     for (int base = blockIdx.x * 256; base < npairs; base += gridDim.x * 256) {  // the same trip count in every thread
         const int slot = base + threadIdx.x;
-        bool gjk = false, mesh = false;
+        bool gjk = false, mesh = false, plane = false;
         int combo = 0;
         if (slot < npairs) {
             d.pairCon[slot] = -1;
@@ -873,19 +875,7 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
                     push_hit(d, slot, HIT_SPHERES, pa + n * ra, pbp - n * rb, n, true);
                 }
             } else if (ta == SHAPE_PLANE || tb == SHAPE_PLANE) {
-                // collideConvexVsPlane and its reverse (plane.ts:297-310); nothing against a
-                // plane or a mesh.  The hit is inverted when the plane is the narrowphase's body A.
-                if (ta != tb && ta != SHAPE_MESH && tb != SHAPE_MESH) {
-                    const bool rev = ta == SHAPE_PLANE;
-                    const NpBody<T>& RC = rev ? RB : RA;
-                    const NpBody<T>& RP = rev ? RA : RB;
-                    PlaneHit<T> ph;
-                    if (plane_hit(d.st, RC.shape, RC.pos, RC.q, RP.shape, RP.pos, RP.q, d.s.spec, ph)) {
-                        const bool sm = (RC.tinfo >> 4) <= 4;
-                        if (!rev) push_hit(d, slot, HIT_PLANE, ph.pA, ph.pB, ph.axis, sm);
-                        else push_hit(d, slot, HIT_PLANE, ph.pB, ph.pA, -ph.axis, sm);
-                    }
-                }
+                plane = ta != tb && ta != SHAPE_MESH && tb != SHAPE_MESH;  // k_np_plane; none vs a plane or a mesh
             } else if (ta == SHAPE_MESH || tb == SHAPE_MESH) {
                 mesh = ta != tb;  // no mesh-vs-mesh function
             } else {
@@ -903,6 +893,11 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
             __syncthreads();
             const int mi = block_append<256>(&d.counters[CNT_MESH], mesh, sh);
             if (mesh) d.meshList[mi] = slot;
+        }
+        if (d.hasPlane) {
+            __syncthreads();
+            const int pi = block_append<256>(&d.counters[CNT_PLANE], plane, sh);
+            if (plane) d.planeList[pi] = slot;
         }
     }
 }
@@ -1039,20 +1034,6 @@ __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
             finish_pair(d, slot, A, Bi, RA, RB, d.hitPA[h], d.hitPB[h], d.hitAxis[h], S.fA, S.fB, S.tmp, S.m, dt, par);
             continue;
         }
-        if (kind == HIT_PLANE) {  // faces of collideConvexVsPlane (plane.ts:550-575), swapped when reversed
-            const bool rev = (RA.tinfo & 15) == SHAPE_PLANE;
-            const NpBody<T>& RC = rev ? RB : RA;
-            const NpBody<T>& RP = rev ? RA : RB;
-            PlaneHit<T> ph;
-            plane_hit(d.st, RC.shape, RC.pos, RC.q, RP.shape, RP.pos, RP.q, T(INFINITY), ph);
-            Face<T, N>& fC = rev ? S.fB : S.fA;
-            Face<T, N>& fP = rev ? S.fA : S.fB;
-            supporting_face(fC, d.st, RC.shape, ph.nl, RC.pos, RC.q);
-            if (fC.n > 0) plane_face(fP, d.st, RC.shape, RC.pos, RP.shape, RP.pos, RP.q);
-            else fP.n = 0;
-            finish_pair(d, slot, A, Bi, RA, RB, d.hitPA[h], d.hitPB[h], d.hitAxis[h], S.fA, S.fB, S.tmp, S.m, dt, par);
-            continue;
-        }
         ConvexCtx<T> c;
         convex_setup(c, RA.pos, RA.q, RB.pos, RB.q, d.s.spec);
         if (kind == HIT_EPA) c.maxSepUse = tmin<T>(c.maxSepUse, T(1));
@@ -1062,6 +1043,38 @@ __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
         pd.pointB = d.hitPB[h];
         pd.axis = d.hitAxis[h];
         finish_convex(d, slot, A, Bi, RA, RB, c, pd, S, dt, par);
+    }
+}
+
+// Convex-vs-plane pairs (collideConvexVsPlane plane.ts:474-590 and its reverse :297-310): the hit,
+// the convex shape's supporting face and the adaptive plane face (:550-575), then the pair's
+// manifold and constraint, one thread per listed pair.  Kept out of the filter and the manifold
+// kernels so that the convex pairs' kernels keep their registers.
+template <typename T>
+__global__ void __launch_bounds__(256) k_np_plane(Dev<T> d, T dt) {
+    const int n = min(d.counters[CNT_PLANE], d.pairCap);
+    const int par = *d.parity;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int slot = d.planeList[k];
+        const int i0 = d.pairA[slot], i1 = d.pairB[slot];
+        const NpBody<T> r0 = d.npb[i0], r1 = d.npb[i1];
+        const PairBodies<T> pb = order_pair_f<T>(i0, r0.flags, i1, r1.flags);
+        const bool sw = pb.a != i0;
+        const NpBody<T>& RA = sw ? r1 : r0;
+        const NpBody<T>& RB = sw ? r0 : r1;
+        const bool rev = (RA.tinfo & 15) == SHAPE_PLANE;  // the plane is the narrowphase's body A
+        const NpBody<T>& RC = rev ? RB : RA;
+        const NpBody<T>& RP = rev ? RA : RB;
+        PlaneHit<T> ph;
+        if (!plane_hit(d.st, RC.shape, RC.pos, RC.q, RP.shape, RP.pos, RP.q, d.s.spec, ph)) continue;
+        NpScratch<T, CC_MAX_FACE> S;
+        Face<T, CC_MAX_FACE>& fC = rev ? S.fB : S.fA;
+        Face<T, CC_MAX_FACE>& fP = rev ? S.fA : S.fB;
+        supporting_face(fC, d.st, RC.shape, ph.nl, RC.pos, RC.q);
+        if (fC.n > 0) plane_face(fP, d.st, RC.shape, RC.pos, RP.shape, RP.pos, RP.q);
+        else fP.n = 0;
+        if (!rev) finish_pair(d, slot, pb.a, pb.b, RA, RB, ph.pA, ph.pB, ph.axis, S.fA, S.fB, S.tmp, S.m, dt, par);
+        else finish_pair(d, slot, pb.a, pb.b, RA, RB, ph.pB, ph.pA, -ph.axis, S.fA, S.fB, S.tmp, S.m, dt, par);
     }
 }
 

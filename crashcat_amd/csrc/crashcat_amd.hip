@@ -89,6 +89,7 @@ __global__ void k_set_pair_count(Dev<T> d) {
         d.counters[CNT_HITS_L] = 0;
         d.counters[CNT_GJK] = 0;
         d.counters[CNT_MESH] = 0;
+        d.counters[CNT_PLANE] = 0;
         for (int c = 0; c < 32; c++) d.gjkHist[c] = 0;
         if (n > d.pairCap) atomicOr(&d.counters[CNT_OVERFLOW], 1);
     }
@@ -615,6 +616,9 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.gLargeOff = A.get<int>(nb + 1);
     d.gLarge = A.get<int>(nb);
     d.hasMesh = X->hasMesh ? 1 : 0;
+    d.hasPlane = 0;
+    for (int i = 0; i < ns; i++) d.hasPlane = d.hasPlane || shapes[i].type == CC_SHAPE_PLANE;
+    if (d.hasPlane) d.planeList = A.get<int>(d.pairCap);
     if (X->hasMesh) {  // one collector per mesh thread; the EPA polytopes are the EPA kernel's
         d.meshList = A.get<int>(d.pairCap);
         d.meshWorkers = ((d.epaWorkers + EPA_LANES - 1) / EPA_LANES) * EPA_LANES;
@@ -899,6 +903,8 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
         mark();  // EPA
         hipLaunchKernelGGL((k_np_manifold<T, CC_SMALL_FACE>), dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
         hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
+        if (d.hasPlane)
+            hipLaunchKernelGGL(k_np_plane<T>, dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d, dt);
         if (d.hasMesh) {
             hipLaunchKernelGGL(k_mesh_count<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
             scan_launch(st, d.meshCnt, d.meshOff, d.pairCap, X->scanSums);
