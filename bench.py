@@ -108,7 +108,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); worlds would shard across GPUs by "
-              "contiguous world ranges, which this build does not launch.", file=sys.stderr)
+              "contiguous world ranges, which this program does not launch.", file=sys.stderr)
         sys.exit(2)
     if args.calibrate:
         calibrate_copy(args.calibrate)

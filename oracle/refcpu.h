@@ -1,8 +1,9 @@
 /*
  * refcpu — FP64 CPU restatement of crashcat's per-step collision-and-contact
  * hot path.  TEST INFRASTRUCTURE (the oracle): only tests/, the smoke check and
- * bench.py's CPU-baseline leg load it, and only as the checker or the timed CPU
- * baseline; the product library never links or calls it.
+ * bench.py's CPU-baseline leg load it, and only as the results the GPU path is
+ * compared with or as the timed CPU baseline; the product library never links or
+ * calls it.
  *
  * It follows the reference files op for op (citations beside each function in
  * refcpu_collide.c / refcpu_step.c) in double precision, on one thread, with

@@ -1,7 +1,8 @@
 """ctypes binding of the FP64 CPU oracle (librefcpu.so).
 
 TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
-bench.py's CPU-baseline leg, as the checker or the timed CPU baseline.
+bench.py's CPU-baseline leg, as the results the GPU path is compared with or as the timed
+CPU baseline.
 """
 import ctypes as C
 import os
