@@ -10,6 +10,12 @@ CC_SHAPE_CYLINDER = 5
 CC_SHAPE_PLANE = 6
 
 CC_MOTION_STATIC = 0
+CC_OK = 0
+CC_E_ARG = -1
+CC_E_HIP = -2
+CC_E_CAPACITY = -3
+CC_E_STATE = -4
+CC_E_UNSUPPORTED = -5
 CC_MOTION_KINEMATIC = 1
 CC_MOTION_DYNAMIC = 2
 
@@ -125,6 +131,12 @@ class BodyDesc(C.Structure):
         ("angular_damping", C.c_double),
         ("max_linear_velocity", C.c_double),
         ("max_angular_velocity", C.c_double),
+        ("force", D3),
+        ("torque", D3),
+        ("num_velocity_steps_override", C.c_int32),
+        ("num_position_steps_override", C.c_int32),
+        ("sensor", C.c_int32),
+        ("motion_quality", C.c_int32),
     ]
 
 
@@ -150,6 +162,8 @@ class PointConstraintDesc(C.Structure):
         ("pad_", C.c_int32),
         ("local_point_a", D3),
         ("local_point_b", D3),
+        ("num_velocity_steps_override", C.c_int32),
+        ("num_position_steps_override", C.c_int32),
     ]
 
 

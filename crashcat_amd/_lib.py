@@ -15,7 +15,7 @@ SYMBOLS = [
     "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
     "cc_get_phase_times", "cc_get_totals", "cc_build_triangle_mesh", "cc_download_removed_contacts",
-    "cc_snapshot", "cc_restore",
+    "cc_snapshot", "cc_restore", "cc_add_forces",
 ]
 
 
@@ -55,6 +55,7 @@ def lib():
     L.cc_broadphase_aabbs.argtypes = [vp, fp, fp, ip, ip, ip, C.c_int32, C.c_double, ip, C.c_int32, ip]
     L.cc_get_phase_times.argtypes = [vp, C.POINTER(C.c_double), C.c_int32, ip]
     L.cc_get_totals.argtypes = [vp, C.POINTER(_abi.Totals)]
+    L.cc_add_forces.argtypes = [vp, dp, dp, C.c_int32, C.c_int32]
     for s in SYMBOLS:
         getattr(L, s).restype = getattr(L, s).restype if s == "cc_last_error" else C.c_int
     _LIB = L
@@ -132,6 +133,14 @@ class Context:
     def set_bodies(self, states):
         st = np.ascontiguousarray(states, dtype=BODY_STATE_DTYPE)
         self._check(lib().cc_set_bodies(self.h, st.ctypes.data_as(C.POINTER(_abi.BodyState)), self.n))
+
+    def add_forces(self, force=None, torque=None, wake=True):
+        """addForce / addTorque on every body: (num_bodies, 3) arrays, either may be None."""
+        dp = C.POINTER(C.c_double)
+        f = None if force is None else np.ascontiguousarray(force, dtype=np.float64).reshape(self.n, 3)
+        t = None if torque is None else np.ascontiguousarray(torque, dtype=np.float64).reshape(self.n, 3)
+        self._check(lib().cc_add_forces(self.h, None if f is None else f.ctypes.data_as(dp),
+                                        None if t is None else t.ctypes.data_as(dp), self.n, int(bool(wake))))
 
     def pairs(self):
         cnt = C.c_int32()

@@ -164,6 +164,9 @@ struct Dev {
     int* isl0;          // [N] island flags (by label)
     int* isl1;          // [N]
     int* stats;         // per world: [levels, posSweeps]
+    int* islIt;         // [3 * N] per island root when a body or point constraint has iteration
+                        // overrides (else null): max velocity and position overrides, default flags
+    int hasForces;      // a body was given a force or torque: integrate and clear them
     T* prevDt;          // [1]
     int* parity;        // [1]
     unsigned long long* acc;  // [ACC_N] running sums over steps, see ACC_*
@@ -200,7 +203,8 @@ __device__ inline int wave_alloc(int* counter) {
 }
 
 // ------------------------------------------------------------ acceleration
-// accelerationIntegrationUpdate src/update.ts:245-319 (external forces are zero on this path)
+// accelerationIntegrationUpdate src/update.ts:245-319: a = F / m + g, then damping, the
+// clamp and the translation DOFs; omega += I_w^-1 tau dt (I_w^-1 of the current rotation, FP64)
 template <typename T>
 __global__ void k_accel(Dev<T> d, T dt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -234,9 +238,11 @@ __global__ void k_accel(Dev<T> d, T dt) {
     }
     T gs = d.s.gravityEnabled ? B.gravityFactor[i] : T(0);
     V3<T> v = B.v[i];
-    v.x += (T(0) * B.invMass[i] + d.s.gravity[0] * gs) * dt;
-    v.y += (T(0) * B.invMass[i] + d.s.gravity[1] * gs) * dt;
-    v.z += (T(0) * B.invMass[i] + d.s.gravity[2] * gs) * dt;
+    const V3<T> z3 = mk3<T>(0, 0, 0);
+    const V3<T> F = d.hasForces ? B.force[i] : z3, tau = d.hasForces ? B.torque[i] : z3;
+    v.x += (F.x * B.invMass[i] + d.s.gravity[0] * gs) * dt;
+    v.y += (F.y * B.invMass[i] + d.s.gravity[1] * gs) * dt;
+    v.z += (F.z * B.invMass[i] + d.s.gravity[2] * gs) * dt;
     T ld = tmax<T>(T(0), T(1) - B.linDamp[i] * dt);
     v = v * ld;
     T s2 = len2(v), ml = B.maxLinV[i];
@@ -245,6 +251,10 @@ __global__ void k_accel(Dev<T> d, T dt) {
     v = dof_lin(v, dof);
     B.v[i] = v;
     V3<T> w = B.w[i];
+    if (tau.x != T(0) || tau.y != T(0) || tau.z != T(0)) {  // a zero torque adds exactly zero
+        const M3<T> I = inv_inertia(B.invI[i], B.iRot[i], B.q[i], dof);
+        w = w + mul(I, tau) * dt;
+    }
     T ad = tmax<T>(T(0), T(1) - B.angDamp[i] * dt);
     w = w * ad;
     T a2 = len2(w), ma = B.maxAngV[i];
@@ -580,7 +590,7 @@ __device__ inline PairBodies<T> order_pair(const Dev<T>& d, int a, int b) {
     return r;
 }
 
-// normal velocity bias, src/constraints/contact-constraints.ts:288-423 (no external forces)
+// normal velocity bias, src/constraints/contact-constraints.ts:288-423
 template <typename T>
 __device__ inline T normal_bias(const Dev<T>& d, int A, int Bi, int fa, int fb, const NpDyn<T>& DA, const NpDyn<T>& DB,
                                 V3<T> pA, V3<T> pB, V3<T> n, V3<T> rA, V3<T> rB, T e, T dt) {
@@ -602,6 +612,8 @@ __device__ inline T normal_bias(const Dev<T>& d, int A, int Bi, int fa, int fb, 
             if (mA && mB) acc = g * (B.gravityFactor[Bi] - B.gravityFactor[A]);
             else if (mA) acc = g * (-B.gravityFactor[A]);
             else if (mB) acc = g * B.gravityFactor[Bi];
+            if (d.hasForces && ma == MOTION_DYNAMIC) acc = acc + B.force[A] * (-B.invMass[A]);
+            if (d.hasForces && mb == MOTION_DYNAMIC) acc = acc + B.force[Bi] * B.invMass[Bi];
             T fdv = tmin<T>(T(0), dot(acc, n) * dt);
             return e * (nv - fdv);
         }

@@ -624,6 +624,27 @@ __device__ inline int build_levels(const Dev<T>& d, int so, int L, int* LS, int&
 
 constexpr int LS_LDS = 4096;  // level starts kept in LDS (16 KB)
 
+// One dynamic body's or point constraint's overrides (velocity | position << 8) into its
+// island's record: the two maxima and a flag per kind for "uses the default" (override 0).
+__device__ inline void isl_add_ovr(int* r, int o) {
+    const int v = o & 255, p = (o >> 8) & 255;
+    atomicMax(&r[0], v);
+    atomicMax(&r[1], p);
+    const int f = (v == 0 ? 1 : 0) | (p == 0 ? 2 : 0);
+    if (f) atomicOr(&r[2], f);
+}
+// the island's velocity | position << 16 iteration counts (islands.ts:344-386)
+template <typename T>
+__device__ inline int isl_iters(const Dev<T>& d, int label) {
+    if (!d.islIt || label < 0) return d.s.velIters | (d.s.posIters << 16);
+    const int* r = d.islIt + 3 * (size_t)label;
+    int v = r[0], p = r[1];
+    const int f = r[2];
+    if (f & 1) v = max(v, d.s.velIters);
+    if (f & 2) p = max(p, d.s.posIters);
+    return v | (p << 16);
+}
+
 template <typename T, int BS>
 __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __shared__ int sh[BS / 64 + 1];
@@ -774,6 +795,28 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         pc.sleeping = B.sleeping[pc.a] && B.sleeping[pc.b];
         if (pc.enabled && !pc.sleeping) point_props(d, pc);
     }
+    // island iteration counts from the overrides of its dynamic bodies and point constraints
+    if (d.islIt) {
+        for (int i = b0 + tid; i < b1; i += BS)
+            if (mine(i) && B.label[i] == i) {
+                int* r = d.islIt + 3 * (size_t)i;
+                r[0] = 0; r[1] = 0; r[2] = 0;
+            }
+        __syncthreads();
+        for (int i = b0 + tid; i < b1; i += BS) {
+            const int l = B.label[i];
+            if (l < 0 || !mine(i) || fl_motion(B.flags[i]) != MOTION_DYNAMIC) continue;
+            isl_add_ovr(d.islIt + 3 * (size_t)l, B.ovr[i]);
+        }
+        for (int k = tid; k < u1 - u0; k += BS) {
+            const PointCon<T>& pc = d.pc[u0 + k];
+            if (!pc_active(d, pc)) continue;
+            const int l = B.label[B.label[pc.a] >= 0 ? pc.a : pc.b];
+            if (G > 1 && island_group(l, G) != grp) continue;
+            isl_add_ovr(d.islIt + 3 * (size_t)l, pc.ovr);
+        }
+        __syncthreads();
+    }
     // the velocity records: the inverse inertia is that of the constraint set-up (same pose)
     for (int i = b0 + tid; i < b1; i += BS) {
         if (!mine(i)) continue;
@@ -807,7 +850,9 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     // ---- stage the constraints in level order; a constraint's points follow each other
     // (dense over the level order)
     int totPts = 0;
+    int maxVel = d.s.velIters, maxPos = d.s.posIters;
     {
+        int mv = 0, mp = 0;
         for (int x = tid; x < L; x += BS) {  // slot and point count of each element, in level order
             const int k = d.sched[so + x];
             d.svSlot[so + x] = d.seq[so + k];
@@ -824,6 +869,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
             E.slot = s;
             E.fr = T(0);
             E.n = mk3<T>(0, 0, 0);
+            E.iters = 0;
             if (s >= 0x40000000) {
                 E.a = E.b = 0;
                 E.meta = VK_INACTIVE << 28;
@@ -834,10 +880,17 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
                 E.a = -1 - s;
                 E.b = 0;
                 E.meta = VK_POINT << 28;
+                const PointCon<T>& pc = d.pc[E.a];
+                E.iters = isl_iters(d, B.label[B.label[pc.a] >= 0 ? pc.a : pc.b]);
+                mv = max(mv, E.iters & 0xffff);
+                mp = max(mp, E.iters >> 16);
                 d.ve[e] = E;
                 continue;
             }
             const Con<T>& k = d.con[s];
+            E.iters = isl_iters(d, B.label[k.a] >= 0 ? B.label[k.a] : B.label[k.b]);
+            mv = max(mv, E.iters & 0xffff);
+            mp = max(mp, E.iters >> 16);
             E.a = k.a;
             E.b = k.b;
             E.meta = k.dynMask | (k.movMask << 2) | (k.np << 4) | (fl_dof(B.flags[k.a]) << 8) | (fl_dof(B.flags[k.b]) << 14);
@@ -857,18 +910,23 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
             }
         }
         __syncthreads();
+        if (d.islIt) {  // the sweeps run to the largest island count; each element stops at its own
+            maxVel = block_reduce_max<BS>(mv, sh);
+            maxPos = block_reduce_max<BS>(mp, sh);
+        }
     }
     const int* LSp = lsLds ? sLS : LS;
     if (!kCreationOrderWarmStart && d.s.warmStarting) warm_start_levels<T, BS, true>(d, so, D, LSp, ratio);
     // ---- 6. velocity iterations; the last one also stores the impulses (step 7)
-    const int velIters = d.s.velIters;
-    for (int it = 0; it < velIters; it++) {
-        CacheRec<T>* rec = it == velIters - 1 ? d.rec[par] : nullptr;
+    for (int it = 0; it < maxVel; it++) {
         for (int l = 0; l < D; l++) {
             const int lo = LSp[l], hi = LSp[l + 1];
             for (int x = lo + tid; x < hi; x += BS) {
                 const VElem<T> E = d.ve[so + x];
                 const int kind = E.meta >> 28;
+                const int nv = E.iters & 0xffff;
+                if (it >= nv) continue;
+                CacheRec<T>* rec = it == nv - 1 ? d.rec[par] : nullptr;
                 if (kind == VK_CONTACT) solve_velocity_elem(d, E, rec);
                 else if (kind == VK_POINT) point_solve_velocity_vw(d, d.pc[E.a]);
             }
@@ -884,13 +942,14 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     }
     __syncthreads();
     // ---- 7. store impulses into this step's cache records (storeAppliedImpulses :1061-1076)
-    // when there was no velocity iteration to do it
-    for (int x = tid; x < (velIters > 0 ? 0 : L); x += BS) {
+    // of the islands that ran no velocity iteration to do it
+    for (int x = tid; x < (d.s.velIters > 0 && !d.islIt ? 0 : L); x += BS) {
         const int e = so + x;
         const int s = d.svSlot[e];
         if (s < 0 || s >= 0x40000000) continue;
         CacheRec<T>& r = d.rec[par][s];
         const VElem<T>& E = d.ve[e];
+        if ((E.iters & 0xffff) != 0) continue;
         const int np = (E.meta >> 4) & 7;
         for (int j = 0; j < np; j++) {
             const VPoint<T>& P = d.vp[(size_t)E.pt + j];
@@ -938,14 +997,14 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __syncthreads();
     // ---- 9. position iterations with the per-island early out (update.ts:187-223)
     int sweeps = 0;
-    for (int it = 0; it < d.s.posIters; it++) {
+    for (int it = 0; it < maxPos; it++) {
         int anyActive = 0;
         for (int l = 0; l < D; l++) {
             const int lo = LSp[l], hi = LSp[l + 1];
             for (int x = lo + tid; x < hi; x += BS) {
                 const VElem<T> E = d.ve[so + x];
                 const int kind = E.meta >> 28;
-                if (kind == VK_INACTIVE) continue;
+                if (kind == VK_INACTIVE || it >= (E.iters >> 16)) continue;
                 const bool con = kind == VK_CONTACT;
                 int ia = E.a, ib = E.b;
                 if (!con) { ia = d.pc[E.a].a; ib = d.pc[E.a].b; }
@@ -1017,6 +1076,15 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
                 B.v[i] = mk3<T>(0, 0, 0);
                 B.w[i] = mk3<T>(0, 0, 0);
             }
+        }
+    }
+    // ---- 12. clear the forces of the bodies still awake (update.ts:234-238)
+    if (d.hasForces) {
+        __syncthreads();
+        for (int i = b0 + tid; i < b1; i += BS) {
+            if (!mine(i) || fl_motion(B.flags[i]) == MOTION_STATIC || B.sleeping[i]) continue;
+            B.force[i] = mk3<T>(0, 0, 0);
+            B.torque[i] = mk3<T>(0, 0, 0);
         }
     }
 }

@@ -55,6 +55,9 @@ struct Bodies {
     T* stimer;
     int* label;    // union-find parent / island label (-1 when not in an island)
     int* wake;     // set when a new constraint wakes the body
+    V3<T>* force;  // accumulated force and torque, integrated by the next step and then cleared
+    V3<T>* torque;
+    int* ovr;      // iteration overrides: velocity | position << 8 (0 = world default)
 };
 
 // One axis with its cached terms, as the position pass builds it for a contact point.
@@ -113,10 +116,11 @@ struct alignas(16) VBody {
 // dofB << 14 | kind << 28 (0 contact, 1 point constraint with index a, 2 inactive); pt: its
 // first point record; slot: its constraint slot.
 enum { VK_CONTACT = 0, VK_POINT = 1, VK_INACTIVE = 2 };
+// iters: the velocity | position << 16 iterations of the element's island
 template <typename T>
 struct alignas(16) VElem {
     int a, b, meta, pt;
-    int slot;
+    int slot, iters;
     V3<T> n;
     T fr;
 };
@@ -153,6 +157,7 @@ struct PointCon {
     M3<T> eff;        // K^-1 (3x3), computed in FP64
     int active;
     V3<T> lambda;
+    int ovr;          // iteration overrides: velocity | position << 8 (0 = world default)
 };
 
 #define CC_EMPTY_KEY 0xffffffffffffffffull

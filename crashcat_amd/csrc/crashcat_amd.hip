@@ -292,13 +292,67 @@ static void set_cell_size(Dev<T>& d, double margin) {
     d.cellInv = 1.0 / d.cellSize;
 }
 
+// Checks of the hull data a scene's hull shapes index, made before anything is allocated: every
+// point, face and face-vertex range inside the flattened arrays, and every face vertex inside its
+// hull's points (the kernels index with these values unchecked).
+static int check_hulls(cc_ctx* c, const cc_shape_desc* shapes, int32_t ns, const cc_hull_data* hulls) {
+    const int P = hulls ? hulls->total_points : 0, F = hulls ? hulls->total_faces : 0;
+    const int FV = hulls ? hulls->total_face_vertices : 0;
+    if (P < 0 || F < 0 || FV < 0) return fail(c, CC_E_ARG, "negative hull totals");
+    if ((P > 0 && (!hulls->points || !hulls->shrunk_points)) ||
+        (F > 0 && (!hulls->face_first_vertex || !hulls->face_num_vertices || !hulls->plane_normals)) ||
+        (FV > 0 && !hulls->face_vertex_indices))
+        return fail(c, CC_E_ARG, "hull data pointer is null");
+    for (int i = 0; i < ns; i++) {
+        const cc_shape_desc& q = shapes[i];
+        if (q.type != CC_SHAPE_CONVEX_HULL) continue;
+        if (q.hull_first_point < 0 || q.hull_num_points < 1 || q.hull_num_points > 255 ||
+            (long long)q.hull_first_point + q.hull_num_points > P || q.hull_first_face < 0 || q.hull_num_faces < 0 ||
+            (long long)q.hull_first_face + q.hull_num_faces > F)
+            return fail(c, CC_E_ARG, "hull data out of range");
+        for (int f = q.hull_first_face; f < q.hull_first_face + q.hull_num_faces; f++) {
+            const int fv = hulls->face_first_vertex[f], nv = hulls->face_num_vertices[f];
+            if (fv < 0 || nv < 0 || (long long)fv + nv > FV) return fail(c, CC_E_ARG, "hull face vertices out of range");
+            for (int k = fv; k < fv + nv; k++)
+                if (hulls->face_vertex_indices[k] < 0 || hulls->face_vertex_indices[k] >= q.hull_num_points)
+                    return fail(c, CC_E_ARG, "hull face vertex index out of range");
+        }
+    }
+    return CC_OK;
+}
+
+template <typename T>
+static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, const int32_t* woff,
+                       const cc_body_desc* bodies, int32_t nb, const cc_shape_desc* shapes, int32_t ns,
+                       const cc_hull_data* hulls, const cc_point_constraint_desc* pcs, int32_t npc,
+                       const cc_mesh_data* meshes);
+
+// A scene is built into a fresh context and installed only when the whole upload succeeds; a
+// failed upload leaves the context with no scene (later calls report CC_E_STATE).
 template <typename T>
 static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* woff, const cc_body_desc* bodies, int32_t nb,
                   const cc_shape_desc* shapes, int32_t ns, const cc_hull_data* hulls, const cc_point_constraint_desc* pcs,
                   int32_t npc, const cc_mesh_data* meshes) {
-    auto* X = new CtxT<T>();
     delete c->impl;
+    c->impl = nullptr;
+    const int hc = check_hulls(c, shapes, ns, hulls);
+    if (hc != CC_OK) return hc;
+    auto* X = new CtxT<T>();
+    const int rc = upload_into<T>(c, X, s, nw, woff, bodies, nb, shapes, ns, hulls, pcs, npc, meshes);
+    if (rc != CC_OK) {
+        (void)hipStreamSynchronize(c->stream);
+        delete X;
+        return rc;
+    }
     c->impl = X;
+    return CC_OK;
+}
+
+template <typename T>
+static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, const int32_t* woff,
+                       const cc_body_desc* bodies, int32_t nb, const cc_shape_desc* shapes, int32_t ns,
+                       const cc_hull_data* hulls, const cc_point_constraint_desc* pcs, int32_t npc,
+                       const cc_mesh_data* meshes) {
     Dev<T>& d = X->d;
     memset(&d, 0, sizeof(d));
     Alloc& A = X->mem;
@@ -376,11 +430,6 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         o.brad = T(br * (1.0 + 1e-5) + 1e-6);
     }
     int P = hulls ? hulls->total_points : 0, F = hulls ? hulls->total_faces : 0, FV = hulls ? hulls->total_face_vertices : 0;
-    for (int i = 0; i < ns; i++)
-        if (shapes[i].type == CC_SHAPE_CONVEX_HULL &&
-            (shapes[i].hull_first_point + shapes[i].hull_num_points > P || shapes[i].hull_first_face + shapes[i].hull_num_faces > F ||
-             shapes[i].hull_num_points > 255 || shapes[i].hull_first_face < 0 || shapes[i].hull_num_faces < 0))
-            return fail(c, CC_E_ARG, "hull data out of range");
     for (int i = 0; i < ns; i++) {  // a hull's supporting face takes at most 16 of a face's vertices
         if (shapes[i].type != CC_SHAPE_CONVEX_HULL) continue;
         int mfv = 0;
@@ -470,8 +519,10 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     std::vector<V3<T>> pos(nb), com(nb), v(nb), wv(nb), invI(nb);
     std::vector<Q4<T>> q(nb), ir(nb);
     std::vector<T> im(nb), fr(nb), re(nb), gf(nb), ld(nb), ad(nb), ml(nb), ma(nb);
-    std::vector<int> shp(nb), flags(nb), slp(nb);
+    std::vector<int> shp(nb), flags(nb), slp(nb), ovr(nb);
     std::vector<unsigned> grp(nb), msk(nb);
+    std::vector<V3<T>> frc(nb), trq(nb);
+    bool anyOvr = false, anyForce = false;
     for (int i = 0; i < nb; i++) {
         const cc_body_desc& b = bodies[i];
         if (b.shape < 0 || b.shape >= ns) return fail(c, CC_E_ARG, "body shape index out of range");
@@ -481,6 +532,22 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
             return fail(c, CC_E_UNSUPPORTED, "triangle-mesh bodies must be static");
         if (shapes[b.shape].type == CC_SHAPE_PLANE && b.motion_type != CC_MOTION_STATIC)
             return fail(c, CC_E_UNSUPPORTED, "plane bodies must be static (plane.ts:53-57)");
+        if (b.sensor) return fail(c, CC_E_UNSUPPORTED, "sensor bodies are not on the GPU path (body " + std::to_string(i) + ")");
+        if (b.motion_quality != CC_MOTION_QUALITY_DISCRETE)
+            return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is not on the GPU path (body " + std::to_string(i) + ")");
+        if (b.num_velocity_steps_override < 0 || b.num_velocity_steps_override > 255 || b.num_position_steps_override < 0 ||
+            b.num_position_steps_override > 255)
+            return fail(c, CC_E_ARG, "iteration overrides must be in [0, 255]");
+        anyOvr = anyOvr || (b.motion_type == CC_MOTION_DYNAMIC &&
+                            (b.num_velocity_steps_override != 0 || b.num_position_steps_override != 0));
+        ovr[i] = b.num_velocity_steps_override | (b.num_position_steps_override << 8);
+        frc[i] = mk3<T>(0, 0, 0);
+        trq[i] = mk3<T>(0, 0, 0);
+        if (b.motion_type == CC_MOTION_DYNAMIC) {  // addForce / addTorque touch dynamic bodies only
+            frc[i] = mk3<T>(T(b.force[0]), T(b.force[1]), T(b.force[2]));
+            trq[i] = mk3<T>(T(b.torque[0]), T(b.torque[1]), T(b.torque[2]));
+            for (int k = 0; k < 3; k++) anyForce = anyForce || b.force[k] != 0.0 || b.torque[k] != 0.0;
+        }
         pos[i] = mk3<T>(T(b.position[0]), T(b.position[1]), T(b.position[2]));
         q[i].x = T(b.quaternion[0]); q[i].y = T(b.quaternion[1]); q[i].z = T(b.quaternion[2]); q[i].w = T(b.quaternion[3]);
         v[i] = mk3<T>(T(b.linear_velocity[0]), T(b.linear_velocity[1]), T(b.linear_velocity[2]));
@@ -508,6 +575,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     UP(invI, invI, V3<T>) UP(iRot, ir, Q4<T>) UP(friction, fr, T) UP(restitution, re, T) UP(gravityFactor, gf, T)
     UP(linDamp, ld, T) UP(angDamp, ad, T) UP(maxLinV, ml, T) UP(maxAngV, ma, T) UP(shape, shp, int) UP(flags, flags, int)
     UP(groups, grp, unsigned) UP(mask, msk, unsigned) UP(world, worldOf, int) UP(sleeping, slp, int)
+    UP(force, frc, V3<T>) UP(torque, trq, V3<T>) UP(ovr, ovr, int)
 #undef UP
     B.amin = A.get<V3<T>>(nb);
     B.amax = A.get<V3<T>>(nb);
@@ -569,6 +637,9 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         int a = pcs[i].body_a, b = pcs[i].body_b;
         if (a < 0 || a >= nb || b < 0 || b >= nb || worldOf[a] != worldOf[b] || a == b)
             return fail(c, CC_E_ARG, "point constraint bodies invalid or in different worlds");
+        const int ov = pcs[i].num_velocity_steps_override, op = pcs[i].num_position_steps_override;
+        if (ov < 0 || ov > 255 || op < 0 || op > 255) return fail(c, CC_E_ARG, "iteration overrides must be in [0, 255]");
+        anyOvr = anyOvr || ov != 0 || op != 0;
         pcw.push_back({worldOf[a], i});
     }
     std::stable_sort(pcw.begin(), pcw.end(), [](auto& x, auto& y) { return x.first < y.first; });
@@ -581,6 +652,7 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
         o.a = p.body_a; o.b = p.body_b; o.enabled = p.enabled;
         o.localA = mk3<T>(T(p.local_point_a[0]), T(p.local_point_a[1]), T(p.local_point_a[2]));
         o.localB = mk3<T>(T(p.local_point_b[0]), T(p.local_point_b[1]), T(p.local_point_b[2]));
+        o.ovr = p.num_velocity_steps_override | (p.num_position_steps_override << 8);
         pcStart[pcw[k].first + 1]++;
     }
     for (int w = 0; w < nw; w++) pcStart[w + 1] += pcStart[w];
@@ -702,6 +774,8 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     d.isl0 = A.get<int>(nb);
     d.isl1 = A.get<int>(nb);
     d.stats = A.get<int>(2 * (size_t)nw);
+    d.islIt = anyOvr ? A.get<int>(3 * (size_t)std::max(nb, 1)) : nullptr;
+    d.hasForces = anyForce ? 1 : 0;
     d.prevDt = A.get<T>(1);
     d.parity = A.get<int>(1);
     d.acc = A.get<unsigned long long>(ACC_N);
@@ -1050,12 +1124,70 @@ extern "C" int cc_set_bodies(cc_ctx* c, const cc_body_state* in, int32_t n) {
     return set_bodies_impl<float>(c, in, n);
 }
 
+// wake the sleeping dynamic bodies flagged in B.wake (sleep.ts:188-203), between steps
+template <typename T>
+__global__ void k_wake_requested(Dev<T> d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N) return;
+    if (d.B.wake[i] && d.B.sleeping[i] && fl_motion(d.B.flags[i]) == MOTION_DYNAMIC) {
+        d.B.sleeping[i] = 0;
+        reset_sleep_timer(d.B, d.st, i);
+    }
+    d.B.wake[i] = 0;
+}
+
+template <typename T>
+static int add_forces_impl(cc_ctx* c, const double* f, const double* t, int n, int wake) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    if (n != d.N) return fail(c, CC_E_ARG, "num_bodies mismatch");
+    if (n == 0 || (!f && !t)) return CC_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<V3<T>> F(n), Tq(n);
+    std::vector<int> fl(n), wk(n, 0);
+    HIPCHK(c, hipMemcpy(F.data(), d.B.force, n * sizeof(V3<T>), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(Tq.data(), d.B.torque, n * sizeof(V3<T>), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(fl.data(), d.B.flags, n * sizeof(int), hipMemcpyDeviceToHost));
+    bool anyWake = false;
+    for (int i = 0; i < n; i++) {
+        if (fl_motion(fl[i]) != MOTION_DYNAMIC) continue;  // addForce / addTorque: dynamic bodies only
+        bool nz = false;
+        if (f) {
+            F[i] = F[i] + mk3<T>(T(f[3 * i]), T(f[3 * i + 1]), T(f[3 * i + 2]));
+            nz = nz || f[3 * i] != 0.0 || f[3 * i + 1] != 0.0 || f[3 * i + 2] != 0.0;
+        }
+        if (t) {
+            Tq[i] = Tq[i] + mk3<T>(T(t[3 * i]), T(t[3 * i + 1]), T(t[3 * i + 2]));
+            nz = nz || t[3 * i] != 0.0 || t[3 * i + 1] != 0.0 || t[3 * i + 2] != 0.0;
+        }
+        if (nz) d.hasForces = 1;
+        if (nz && wake) { wk[i] = 1; anyWake = true; }
+    }
+    HIPCHK(c, hipMemcpy(d.B.force, F.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.B.torque, Tq.data(), n * sizeof(V3<T>), hipMemcpyHostToDevice));
+    if (anyWake) {
+        HIPCHK(c, hipMemcpy(d.B.wake, wk.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_wake_requested<T>, dim3((n + 255) / 256), dim3(256), 0, c->stream, d);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return CC_OK;
+}
+
+extern "C" int cc_add_forces(cc_ctx* c, const double* force, const double* torque, int32_t n, int32_t wake) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (n < 0) return fail(c, CC_E_ARG, "negative body count");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return add_forces_impl<double>(c, force, torque, n, wake);
+    return add_forces_impl<float>(c, force, torque, n, wake);
+}
+
 // ---------------------------------------------------------------- snapshot / restore
 // Blob layout: SnapHeader, then the device arrays of snap_segments in their order, then the
 // last step's contact-cache records.  Only a context holding the same upload reads it back.
 struct SnapHeader {
-    char magic[8];  // "CCSNAP01"
-    int32_t version, tsize, n, npc, w, ncache, parity, pad;
+    char magic[8];  // "CCSNAP02"
+    int32_t version, tsize, n, npc, w, ncache, parity, hasForces;
     double prevDt;
     int64_t bytes;  // the whole blob
 };
@@ -1069,7 +1201,8 @@ static std::vector<std::pair<void*, size_t>> snap_segments(Dev<T>& d) {
             {B.amax, n * sizeof(V3<T>)},    {B.sleeping, n * sizeof(int)},    {B.sc, 3 * n * sizeof(V3<T>)},
             {B.sr, 3 * n * sizeof(T)},      {B.stimer, n * sizeof(T)},        {B.label, n * sizeof(int)},
             {B.wake, n * sizeof(int)},      {B.flags, n * sizeof(int)},       {d.vb, n * sizeof(VBody<T>)},
-            {d.pb, n * sizeof(PBody<T>)},   {d.pc, (size_t)d.NPC * sizeof(PointCon<T>)}};
+            {d.pb, n * sizeof(PBody<T>)},   {d.pc, (size_t)d.NPC * sizeof(PointCon<T>)},
+            {B.force, n * sizeof(V3<T>)},   {B.torque, n * sizeof(V3<T>)}};
 }
 
 template <typename T>
@@ -1094,12 +1227,13 @@ static int snapshot_impl(cc_ctx* c, void* out, int64_t cap, int64_t* size) {
     if (!out || cap < total) return CC_OK;
     SnapHeader h;
     memset(&h, 0, sizeof(h));
-    memcpy(h.magic, "CCSNAP01", 8);
-    h.version = 1;
+    memcpy(h.magic, "CCSNAP02", 8);
+    h.version = 2;
     h.tsize = (int32_t)sizeof(T);
     h.n = d.N; h.npc = d.NPC; h.w = d.W;
     h.ncache = nr;
     h.parity = par;
+    h.hasForces = d.hasForces;
     T pdt;
     HIPCHK(c, hipMemcpy(&pdt, d.prevDt, sizeof(T), hipMemcpyDeviceToHost));
     h.prevDt = (double)pdt;
@@ -1122,7 +1256,7 @@ static int restore_impl(cc_ctx* c, const void* in, int64_t size) {
     SnapHeader h;
     if (size < (int64_t)sizeof(h)) return fail(c, CC_E_ARG, "snapshot too short");
     memcpy(&h, in, sizeof(h));
-    if (memcmp(h.magic, "CCSNAP01", 8) != 0 || h.version != 1) return fail(c, CC_E_ARG, "not a crashcat_amd snapshot");
+    if (memcmp(h.magic, "CCSNAP02", 8) != 0 || h.version != 2) return fail(c, CC_E_ARG, "not a crashcat_amd snapshot");
     if (h.tsize != (int32_t)sizeof(T) || h.n != d.N || h.npc != d.NPC || h.w != d.W)
         return fail(c, CC_E_ARG, "snapshot of another scene or precision");
     if (h.ncache < 0 || h.ncache > d.conCap || (h.parity != 0 && h.parity != 1))
@@ -1145,6 +1279,7 @@ static int restore_impl(cc_ctx* c, const void* in, int64_t size) {
     rc[q] = n;
     HIPCHK(c, hipMemcpy(d.recCnt, rc, sizeof(rc), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.parity, &h.parity, sizeof(int), hipMemcpyHostToDevice));
+    d.hasForces = d.hasForces || h.hasForces;
     const T pdt = T(h.prevDt);
     HIPCHK(c, hipMemcpy(d.prevDt, &pdt, sizeof(T), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(d.counters, 0, CNT_N * sizeof(int)));

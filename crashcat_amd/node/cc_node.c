@@ -133,6 +133,9 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_body_desc, restitution, "f64", 1); F(cc_body_desc, gravity_factor, "f64", 1);
     F(cc_body_desc, linear_damping, "f64", 1); F(cc_body_desc, angular_damping, "f64", 1);
     F(cc_body_desc, max_linear_velocity, "f64", 1); F(cc_body_desc, max_angular_velocity, "f64", 1);
+    F(cc_body_desc, force, "f64", 3); F(cc_body_desc, torque, "f64", 3);
+    F(cc_body_desc, num_velocity_steps_override, "i32", 1); F(cc_body_desc, num_position_steps_override, "i32", 1);
+    F(cc_body_desc, sensor, "i32", 1); F(cc_body_desc, motion_quality, "i32", 1);
     add_struct(env, out, "cc_body_desc", sizeof(cc_body_desc), fields);
     napi_create_object(env, &fields);
     F(cc_body_state, position, "f64", 3); F(cc_body_state, quaternion, "f64", 4);
@@ -145,6 +148,8 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_point_constraint_desc, body_a, "i32", 1); F(cc_point_constraint_desc, body_b, "i32", 1);
     F(cc_point_constraint_desc, enabled, "i32", 1); F(cc_point_constraint_desc, local_point_a, "f64", 3);
     F(cc_point_constraint_desc, local_point_b, "f64", 3);
+    F(cc_point_constraint_desc, num_velocity_steps_override, "i32", 1);
+    F(cc_point_constraint_desc, num_position_steps_override, "i32", 1);
     add_struct(env, out, "cc_point_constraint_desc", sizeof(cc_point_constraint_desc), fields);
     napi_create_object(env, &fields);
     F(cc_step_counters, num_bodies, "i64", 1); F(cc_step_counters, num_awake, "i64", 1);
@@ -224,21 +229,35 @@ static napi_value upload(napi_env env, napi_callback_info info) {
     memset(&hull, 0, sizeof hull);
     napi_value h = a[5], v;
     bool has;
-    size_t nb;
-#define HULL(field, prop)                                                          \
+    /* bytes of each hull buffer (0 when absent): the totals below must fit in them */
+    size_t bPts = 0, bShr = 0, bFfv = 0, bFnv = 0, bFvi = 0, bPn = 0, bPc = 0;
+#define HULL(field, prop, bytes)                                                   \
     if (napi_has_named_property(env, h, prop, &has) == napi_ok && has) {           \
         napi_get_named_property(env, h, prop, &v);                                 \
-        hull.field = buffer_data(env, v, &nb);                                     \
+        hull.field = buffer_data(env, v, &bytes);                                  \
         if (!hull.field) return NULL;                                              \
     }
-    HULL(points, "points") HULL(shrunk_points, "shrunkPoints") HULL(face_first_vertex, "faceFirstVertex")
-    HULL(face_num_vertices, "faceNumVertices") HULL(face_vertex_indices, "faceVertexIndices")
-    HULL(plane_normals, "planeNormals") HULL(plane_constants, "planeConstants")
+    HULL(points, "points", bPts) HULL(shrunk_points, "shrunkPoints", bShr)
+    HULL(face_first_vertex, "faceFirstVertex", bFfv) HULL(face_num_vertices, "faceNumVertices", bFnv)
+    HULL(face_vertex_indices, "faceVertexIndices", bFvi) HULL(plane_normals, "planeNormals", bPn)
+    HULL(plane_constants, "planeConstants", bPc)
 #undef HULL
     if (napi_get_named_property(env, h, "totalPoints", &v) == napi_ok) napi_get_value_int32(env, v, &hull.total_points);
     if (napi_get_named_property(env, h, "totalFaces", &v) == napi_ok) napi_get_value_int32(env, v, &hull.total_faces);
     if (napi_get_named_property(env, h, "totalFaceVertices", &v) == napi_ok)
         napi_get_value_int32(env, v, &hull.total_face_vertices);
+    {
+        const size_t P = (size_t)(hull.total_points > 0 ? hull.total_points : 0);
+        const size_t Fc = (size_t)(hull.total_faces > 0 ? hull.total_faces : 0);
+        const size_t FV = (size_t)(hull.total_face_vertices > 0 ? hull.total_face_vertices : 0);
+        if (hull.total_points < 0 || hull.total_faces < 0 || hull.total_face_vertices < 0 ||
+            bPts < 3 * P * sizeof(double) || bShr < 3 * P * sizeof(double) || bFfv < Fc * sizeof(int32_t) ||
+            bFnv < Fc * sizeof(int32_t) || bPn < 3 * Fc * sizeof(double) || bPc < Fc * sizeof(double) ||
+            bFvi < FV * sizeof(int32_t)) {
+            napi_throw_range_error(env, NULL, "crashcat_amd: hull totals exceed the hull buffers");
+            return NULL;
+        }
+    }
     /* triangle meshes ride on the same object: meshPositions (Float64Array), meshTriangles
      * (Int32Array, 4 per triangle), meshNodes (Float64Array, the BVH node buffer) */
     cc_mesh_data mesh;
@@ -378,6 +397,35 @@ static napi_value download_records(napi_env env, napi_callback_info info, int re
 static napi_value download_contacts(napi_env env, napi_callback_info info) { return download_records(env, info, 0); }
 static napi_value download_removed(napi_env env, napi_callback_info info) { return download_records(env, info, 1); }
 
+/* ---- addForces(ctx, force, torque, wake): Float64Array (3 per body) or null each ------- */
+static napi_value add_forces(napi_env env, napi_callback_info info) {
+    size_t argc = 4;
+    napi_value a[4];
+    CHECK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+    if (argc < 3) { napi_throw_type_error(env, NULL, "crashcat_amd: addForces needs 3 arguments"); return NULL; }
+    cc_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    const double* fv[2] = {NULL, NULL};
+    size_t by[2] = {0, 0};
+    for (int k = 0; k < 2; k++) {
+        napi_valuetype t;
+        CHECK(env, napi_typeof(env, a[1 + k], &t));
+        if (t == napi_null || t == napi_undefined) continue;
+        fv[k] = (const double*)buffer_data(env, a[1 + k], &by[k]);
+        if (!fv[k]) return NULL;
+    }
+    int32_t wake = 1;
+    if (argc > 3) napi_get_value_int32(env, a[3], &wake);
+    size_t n = fv[0] ? by[0] : by[1];
+    if (n % (3 * sizeof(double)) || (fv[0] && fv[1] && by[0] != by[1])) {
+        napi_throw_range_error(env, NULL, "crashcat_amd: force and torque need 3 doubles per body");
+        return NULL;
+    }
+    int rc = cc_add_forces(c, fv[0], fv[1], (int32_t)(n / (3 * sizeof(double))), wake);
+    if (rc != CC_OK) return throw_rc(env, c, rc);
+    return NULL;
+}
+
 /* ---- snapshot(ctx) -> ArrayBuffer; restore(ctx, buffer) ---------------------------------- */
 static napi_value snapshot(napi_env env, napi_callback_info info) {
     size_t argc = 1;
@@ -425,6 +473,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"downloadRemovedContacts", 0, download_removed, 0, 0, 0, napi_default, 0},
         {"snapshot", 0, snapshot, 0, 0, 0, napi_default, 0},
         {"restore", 0, restore, 0, 0, 0, napi_default, 0},
+        {"addForces", 0, add_forces, 0, 0, 0, napi_default, 0},
     };
     napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
     return exports;

@@ -106,8 +106,10 @@ class Scene:
                  friction=0.2, restitution=0.0, gravity_factor=1.0, linear_damping=0.05, angular_damping=0.05,
                  max_linear_velocity=500.0, max_angular_velocity=0.25 * math.pi * 60.0, allow_sleeping=True,
                  allowed_dofs=0b111111, friction_combine=4, restitution_combine=3, collision_groups=0xFFFFFFFF,
-                 collision_mask=0xFFFFFFFF):
-        """rigidBody.create — src/body/rigid-body.ts:390-434, defaults :120-139."""
+                 collision_mask=0xFFFFFFFF, force=(0.0, 0.0, 0.0), torque=(0.0, 0.0, 0.0),
+                 num_velocity_steps_override=0, num_position_steps_override=0, sensor=False, motion_quality=0):
+        """rigidBody.create — src/body/rigid-body.ts:390-434, defaults :120-139; force / torque are
+        the accumulators addForce / addTorque fill before the first step."""
         si = self.add_shape(shape)
         if object_layer is None:
             object_layer = self.layer_static if motion_type == _abi.CC_MOTION_STATIC else self.layer_moving
@@ -119,7 +121,8 @@ class Scene:
                           allowed_dofs, friction_combine, restitution_combine, 0, tuple(position), tuple(quaternion),
                           tuple(linear_velocity), tuple(angular_velocity), inv_mass, inv_i, irot, friction,
                           restitution, gravity_factor, linear_damping, angular_damping, max_linear_velocity,
-                          max_angular_velocity))
+                          max_angular_velocity, tuple(force), tuple(torque), int(num_velocity_steps_override),
+                          int(num_position_steps_override), int(bool(sensor)), int(motion_quality)))
         self._bodies = None
         return len(self.rows) - 1
 
@@ -128,8 +131,10 @@ class Scene:
         if self.world_offsets[-1] != len(self.rows):
             self.world_offsets.append(len(self.rows))
 
-    def add_point_constraint(self, a, b, local_a, local_b):
-        self.point_constraints.append((a, b, tuple(local_a), tuple(local_b)))
+    def add_point_constraint(self, a, b, local_a, local_b, num_velocity_steps_override=0,
+                             num_position_steps_override=0):
+        self.point_constraints.append((a, b, tuple(local_a), tuple(local_b), int(num_velocity_steps_override),
+                                       int(num_position_steps_override)))
 
     # -------------------------------------------------------------- C views
     @property
@@ -157,7 +162,9 @@ class Scene:
                          "allow_sleeping", "allowed_dofs", "friction_combine", "restitution_combine", "sleeping",
                          "position", "quaternion", "linear_velocity", "angular_velocity", "inv_mass",
                          "inv_inertia_diagonal", "inertia_rotation", "friction", "restitution", "gravity_factor",
-                         "linear_damping", "angular_damping", "max_linear_velocity", "max_angular_velocity"]
+                         "linear_damping", "angular_damping", "max_linear_velocity", "max_angular_velocity",
+                         "force", "torque", "num_velocity_steps_override", "num_position_steps_override", "sensor",
+                         "motion_quality"]
                 for n, c in zip(names, cols):
                     arr[n] = np.asarray(c)
             self._bodies = arr
@@ -173,7 +180,7 @@ class Scene:
         out._bodies = self.bodies()[b0:b1].copy()
         out.rows = [None] * (b1 - b0)
         out.world_offsets = [0]
-        out.point_constraints = [(a - b0, b - b0, la, lb) for (a, b, la, lb) in self.point_constraints
+        out.point_constraints = [(a - b0, b - b0, la, lb, ov, op) for (a, b, la, lb, ov, op) in self.point_constraints
                                  if b0 <= a < b1]
         return out
 
@@ -266,12 +273,14 @@ class Scene:
     def c_point_constraints(self):
         n = len(self.point_constraints)
         arr = (_abi.PointConstraintDesc * max(n, 1))()
-        for i, (a, b, la, lb) in enumerate(self.point_constraints):
+        for i, (a, b, la, lb, ov, op) in enumerate(self.point_constraints):
             arr[i].body_a = a
             arr[i].body_b = b
             arr[i].enabled = 1
             arr[i].local_point_a[:] = la
             arr[i].local_point_b[:] = lb
+            arr[i].num_velocity_steps_override = ov
+            arr[i].num_position_steps_override = op
         return arr, n
 
     def c_offsets(self):

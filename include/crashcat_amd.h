@@ -47,6 +47,10 @@ extern "C" {
 #define CC_MOTION_KINEMATIC 1
 #define CC_MOTION_DYNAMIC 2
 
+/* ---- motion quality: MotionQuality, src/body/motion-properties.ts:8-22 ---- */
+#define CC_MOTION_QUALITY_DISCRETE 0
+#define CC_MOTION_QUALITY_LINEAR_CAST 1
+
 /* ---- material combine modes: src/constraints/combine-material.ts:1-8 ---- */
 #define CC_COMBINE_AVERAGE 0
 #define CC_COMBINE_MULTIPLY 1
@@ -178,6 +182,21 @@ typedef struct cc_body_desc {
     double angular_damping;
     double max_linear_velocity;
     double max_angular_velocity;
+    /* accumulated force and torque (world space; motionProperties.force / .torque,
+     * src/body/motion-properties.ts) that the next step integrates (src/update.ts:253-292) and
+     * then clears (:234-238); cc_add_forces adds more between steps */
+    double force[3];
+    double torque[3];
+    /* numVelocityStepsOverride / numPositionStepsOverride (src/body/motion-properties.ts:59):
+     * 0 = the world default; an island solves max over its dynamic bodies and point constraints,
+     * and at least the world default when any of them is 0 (src/islands.ts:344-386); at most 255 */
+    int32_t num_velocity_steps_override;
+    int32_t num_position_steps_override;
+    /* body.sensor (src/body/rigid-body.ts:207-208) and motionProperties.motionQuality: sensors and
+     * MotionQuality.LINEAR_CAST (CCD, src/update.ts:932-934) are not on the GPU path; a body with
+     * either is rejected with CC_E_UNSUPPORTED rather than stepped differently */
+    int32_t sensor;
+    int32_t motion_quality;     /* CC_MOTION_QUALITY_* */
 } cc_body_desc;
 
 /* Body state read back after stepping. */
@@ -202,6 +221,11 @@ typedef struct cc_point_constraint_desc {
     int32_t pad_;
     double local_point_a[3];
     double local_point_b[3];
+    /* the constraint's iteration overrides (ConstraintBase numVelocityStepsOverride /
+     * numPositionStepsOverride, read by getConstraintIterationOverrides, src/islands.ts:366-374);
+     * 0 = world default; at most 255 */
+    int32_t num_velocity_steps_override;
+    int32_t num_position_steps_override;
 } cc_point_constraint_desc;
 
 /* One contact constraint of the last step, for parity taps
@@ -310,6 +334,13 @@ int cc_build_triangle_mesh(const double* positions, int32_t num_vertices, const 
  * Work is enqueued on the context's stream; call cc_sync to wait. */
 int cc_step(cc_ctx* ctx, double dt, int32_t nsteps);
 int cc_sync(cc_ctx* ctx);
+
+/* Add force and torque to every dynamic body (world space, accumulated until the next step
+ * integrates and clears them): addForce / addTorque (src/body/rigid-body.ts:840-869) for all
+ * bodies at once.  force and torque are 3 * num_bodies doubles each (either may be NULL);
+ * entries of non-dynamic bodies are ignored, as the reference ignores them.  With wake != 0 a
+ * sleeping body that receives a non-zero force or torque is woken (sleep.ts:188-203). */
+int cc_add_forces(cc_ctx* ctx, const double* force, const double* torque, int32_t num_bodies, int32_t wake);
 
 /* Read back body state (all worlds, global body order). */
 int cc_download_bodies(cc_ctx* ctx, cc_body_state* out, int32_t num_bodies);

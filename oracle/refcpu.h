@@ -36,6 +36,18 @@ void oc_step(oc_world* w, double dt);
 int32_t oc_num_bodies(const oc_world* w);
 void oc_get_bodies(const oc_world* w, cc_body_state* out);
 void oc_set_bodies(oc_world* w, const cc_body_state* in);
+/* addForce / addTorque for every body (3 doubles each; NULL for none); wake: wake a sleeping
+ * body that receives a non-zero entry */
+void oc_add_forces(oc_world* w, const double* force, const double* torque, int32_t wake);
+/* replace the world settings between steps (tests isolate one solver stage this way) */
+void oc_set_settings(oc_world* w, const cc_settings* s);
+/* the island of a body after the last step: its velocity and position iteration counts
+ * (island.numVelocitySteps / numPositionSteps, src/islands.ts:344-386); 0 when not in an island */
+int32_t oc_island_iterations(const oc_world* w, int32_t body, int32_t* vel, int32_t* pos);
+/* computeContactSortKey (src/constraints/contact-constraints.ts:1222-1231) */
+uint32_t oc_sort_key(uint32_t a, uint32_t b, uint32_t sub_a, uint32_t sub_b);
+/* the contact tangent basis of addContactConstraint (:536-556) */
+void oc_contact_tangents(const double* normal, double* t1, double* t2);
 /* pairs of the last step as (min,max) index pairs, sorted */
 int32_t oc_get_pairs(const oc_world* w, int32_t* out, int32_t cap);
 /* contact constraints of the last step, sorted by (a, b, sub-shape a, sub-shape b) */

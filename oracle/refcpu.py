@@ -44,6 +44,13 @@ def lib():
         L.oc_get_removed_contacts.restype = C.c_int32
         L.oc_get_removed_contacts.argtypes = [C.c_void_p, C.POINTER(_abi.ContactRemoved), C.c_int32]
         L.oc_get_counters.argtypes = [C.c_void_p, C.POINTER(_abi.StepCounters)]
+        L.oc_add_forces.argtypes = [C.c_void_p, dp, dp, C.c_int32]
+        L.oc_set_settings.argtypes = [C.c_void_p, C.POINTER(_abi.Settings)]
+        L.oc_sort_key.restype = C.c_uint32
+        L.oc_sort_key.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.oc_contact_tangents.argtypes = [dp, dp, dp]
+        L.oc_island_iterations.restype = C.c_int32
+        L.oc_island_iterations.argtypes = [C.c_void_p, C.c_int32, ip, ip]
         L.oc_broadphase_aabbs.restype = C.c_int32
         L.oc_broadphase_aabbs.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), ip, ip, ip, C.c_int32,
                                           C.c_double, C.POINTER(_abi.Settings), ip, C.c_int32]
@@ -91,12 +98,14 @@ class OracleWorld:
         bodies = scene.bodies()[b0:b1].copy()
         self._bodies = bodies
         shapes, hulls, self._keep = scene.c_shapes()
-        pcs = [(a - b0, b - b0, la, lb) for (a, b, la, lb) in scene.point_constraints if b0 <= a < b1]
+        pcs = [(a - b0, b - b0, la, lb, ov, op) for (a, b, la, lb, ov, op) in scene.point_constraints if b0 <= a < b1]
         pa = (_abi.PointConstraintDesc * max(len(pcs), 1))()
-        for i, (a, b, la, lb) in enumerate(pcs):
+        for i, (a, b, la, lb, ov, op) in enumerate(pcs):
             pa[i].body_a, pa[i].body_b, pa[i].enabled = a, b, 1
             pa[i].local_point_a[:] = la
             pa[i].local_point_b[:] = lb
+            pa[i].num_velocity_steps_override = ov
+            pa[i].num_position_steps_override = op
         self._pcs = pa
         self._settings = scene.settings
         self.n = b1 - b0
@@ -121,6 +130,26 @@ class OracleWorld:
     def set_bodies(self, states):
         st = np.ascontiguousarray(states, dtype=BODY_STATE_DTYPE)
         lib().oc_set_bodies(self.h, st.ctypes.data_as(C.POINTER(_abi.BodyState)))
+
+    def add_forces(self, force=None, torque=None, wake=True):
+        """addForce / addTorque on every body: (n, 3) arrays, either may be None."""
+        dp = C.POINTER(C.c_double)
+        f = None if force is None else np.ascontiguousarray(force, dtype=np.float64).reshape(self.n, 3)
+        t = None if torque is None else np.ascontiguousarray(torque, dtype=np.float64).reshape(self.n, 3)
+        lib().oc_add_forces(self.h, None if f is None else f.ctypes.data_as(dp), None if t is None else t.ctypes.data_as(dp),
+                            int(bool(wake)))
+
+    def set_settings(self, settings):
+        """Replace the world settings between steps (a copy is kept)."""
+        self._settings = settings
+        lib().oc_set_settings(self.h, C.byref(settings))
+
+    def island_iterations(self, body):
+        """(velocity, position) iteration counts of the body's island after the last step, or None."""
+        v, p = C.c_int32(), C.c_int32()
+        if not lib().oc_island_iterations(self.h, body, C.byref(v), C.byref(p)):
+            return None
+        return v.value, p.value
 
     def pairs(self):
         n = lib().oc_get_pairs(self.h, None, 0)
@@ -327,3 +356,17 @@ def collide_mesh(scene, sa, pa, qa, sm, pm=(0, 0, 0), qm=(0, 0, 0, 1), max_separ
                           C.cast(C.pointer(keep["meshes"]), C.c_void_p), sa, _dp(v[0]), _dp(v[1]), sm, _dp(v[2]),
                           _dp(v[3]), max_separation, _dp(v[4]), C.cast(C.pointer(st), C.c_void_p), out, sub, cap)
     return [(out[i], sub[i]) for i in range(min(n, cap))]
+
+
+def sort_key(a, b, sub_a, sub_b):
+    """computeContactSortKey as the oracle computes it."""
+    return int(lib().oc_sort_key(a, b, sub_a, sub_b))
+
+
+def contact_tangents(n):
+    """The oracle's tangent basis (t1, t2) of a contact normal."""
+    nn = _vec(n)
+    t1 = np.zeros(3)
+    t2 = np.zeros(3)
+    lib().oc_contact_tangents(_dp(nn), _dp(t1), _dp(t2))
+    return t1, t2

@@ -25,6 +25,7 @@ typedef struct {
     Q4 inertiaRot;
     double friction, restitution, gravityFactor, linDamp, angDamp, maxLinV, maxAngV;
     int sleeping;
+    int ovV, ovP; /* numVelocityStepsOverride / numPositionStepsOverride (motion-properties.ts:59) */
     int activeIndex;
     int islandIndex;
     V3 aabbMin, aabbMax;
@@ -65,6 +66,7 @@ typedef struct {
     M3 iI1r1X, iI2r2X;
     double eff[16]; /* mat4 (3x3 part used, [15] marks active) */
     V3 lambda;
+    int ovV, ovP; /* the constraint's iteration overrides */
 } OPoint;
 
 typedef struct {
@@ -327,6 +329,12 @@ oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb
         b->linDamp = d->linear_damping; b->angDamp = d->angular_damping;
         b->maxLinV = d->max_linear_velocity; b->maxAngV = d->max_angular_velocity;
         b->sleeping = d->sleeping;
+        b->ovV = d->num_velocity_steps_override;
+        b->ovP = d->num_position_steps_override;
+        if (b->motion == CC_MOTION_DYNAMIC) { /* motionProperties.force / torque (dynamic bodies only) */
+            b->force = vload(d->force);
+            b->torque = vload(d->torque);
+        }
         b->activeIndex = INACTIVE;
         b->islandIndex = -1;
         set_transform(w, b); /* updateShape :677-712 */
@@ -338,6 +346,7 @@ oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb
     for (int i = 0; i < npc; i++) {
         w->pc[i].a = pcs[i].body_a; w->pc[i].b = pcs[i].body_b; w->pc[i].enabled = pcs[i].enabled;
         w->pc[i].localA = vload(pcs[i].local_point_a); w->pc[i].localB = vload(pcs[i].local_point_b);
+        w->pc[i].ovV = pcs[i].num_velocity_steps_override; w->pc[i].ovP = pcs[i].num_position_steps_override;
     }
     w->consLinks = (int*)calloc((size_t)(npc ? npc : 1), sizeof(int));
     w->consIds = (int*)calloc((size_t)(npc ? npc : 1), sizeof(int));
@@ -490,6 +499,14 @@ static uint32_t sort_key(uint32_t a, uint32_t b, uint32_t sa, uint32_t sb) { /* 
     return h;
 }
 
+/* tangent basis of a contact normal, addContactConstraint :536-556: t1 = (nz, 0, -nx)/|.| when
+ * |nx| > |ny|, else (0, nz, -ny)/|.|; t2 = n x t1 */
+static void contact_tangents(V3 n, V3* t1, V3* t2) {
+    if (fabs(n.x) > fabs(n.y)) { double tl = sqrt(n.x * n.x + n.z * n.z); *t1 = v3(n.z / tl, 0, -n.x / tl); }
+    else { double tl = sqrt(n.y * n.y + n.z * n.z); *t1 = v3(0, n.z / tl, -n.y / tl); }
+    *t2 = vcross(n, *t1);
+}
+
 /* calculateNormalVelocityBias src/constraints/contact-constraints.ts:288-423 */
 static double normal_bias(const oc_world* w, const OBody* A, const OBody* B, V3 pA, V3 pB, V3 n, V3 rA, V3 rB, double e,
                           double dt) {
@@ -551,9 +568,7 @@ static int add_contact_constraint(oc_world* w, int ia, int ib, OManifold* m, uin
         k->sortKey = sort_key((uint32_t)ia, (uint32_t)ib, subA, subB);
         V3 n = m->normal;
         k->normal = n;
-        if (fabs(n.x) > fabs(n.y)) { double tl = sqrt(n.x * n.x + n.z * n.z); k->t1 = v3(n.z / tl, 0, -n.x / tl); }
-        else { double tl = sqrt(n.y * n.y + n.z * n.z); k->t1 = v3(0, n.z / tl, -n.y / tl); }
-        k->t2 = vcross(n, k->t1);
+        contact_tangents(n, &k->t1, &k->t2);
         k->friction = fr; k->restitution = re;
         k->invMassA = A->invMass * 1.0; k->invMassB = B->invMass * 1.0;
         if (A->motion == CC_MOTION_DYNAMIC) k->invIA = inv_inertia(A->invI, A->inertiaRot, &rotA, A->dof);
@@ -871,9 +886,28 @@ static void build_islands(oc_world* w) {
     for (int i = 0; i < nisl; i++) {
         OIsland* I = &w->isl[i];
         for (int k = 0; k < I->nb; k++) w->b[w->active[I->bodies[k]]].islandIndex = i;
-        /* all overrides are 0 -> world defaults (islands.ts:344-386) */
-        I->velSteps = w->s.velocity_iterations;
-        I->posSteps = w->s.position_iterations;
+        /* solver iterations of the island (islands.ts:344-386): the largest override of its dynamic
+         * bodies and user constraints, and at least the world default if any of them uses 0 */
+        int nv = 0, np = 0, defV = 0, defP = 0;
+        for (int k = 0; k < I->nb; k++) {
+            const OBody* b = &w->b[w->active[I->bodies[k]]];
+            if (b->motion != CC_MOTION_DYNAMIC) continue;
+            if (b->ovV > nv) nv = b->ovV;
+            defV = defV || b->ovV == 0;
+            if (b->ovP > np) np = b->ovP;
+            defP = defP || b->ovP == 0;
+        }
+        for (int k = 0; k < I->nu; k++) {
+            const OPoint* p = &w->pc[I->cons[k]];
+            if (p->ovV > nv) nv = p->ovV;
+            defV = defV || p->ovV == 0;
+            if (p->ovP > np) np = p->ovP;
+            defP = defP || p->ovP == 0;
+        }
+        if (defV && w->s.velocity_iterations > nv) nv = w->s.velocity_iterations;
+        if (defP && w->s.position_iterations > np) np = w->s.position_iterations;
+        I->velSteps = nv;
+        I->posSteps = np;
     }
 }
 
@@ -1281,6 +1315,40 @@ void oc_set_bodies(oc_world* w, const cc_body_state* in) {
         set_transform(w, b);
         if (!in[i].sleeping && b->sleeping && b->motion == CC_MOTION_DYNAMIC) body_wake(w, b, i);
     }
+}
+
+void oc_add_forces(oc_world* w, const double* force, const double* torque, int32_t wake) { /* addForce/addTorque rigid-body.ts:840-869 */
+    for (int i = 0; i < w->nb; i++) {
+        OBody* b = &w->b[i];
+        if (b->motion != CC_MOTION_DYNAMIC) continue;
+        int nz = 0;
+        if (force) {
+            b->force = vadd(b->force, vload(force + 3 * i));
+            nz = nz || force[3 * i] != 0 || force[3 * i + 1] != 0 || force[3 * i + 2] != 0;
+        }
+        if (torque) {
+            b->torque = vadd(b->torque, vload(torque + 3 * i));
+            nz = nz || torque[3 * i] != 0 || torque[3 * i + 1] != 0 || torque[3 * i + 2] != 0;
+        }
+        if (wake && nz && b->sleeping) body_wake(w, b, i);
+    }
+}
+
+/* unit entry points of the known-answer tests */
+void oc_set_settings(oc_world* w, const cc_settings* s) { w->s = *s; }
+int32_t oc_island_iterations(const oc_world* w, int32_t body, int32_t* vel, int32_t* pos) {
+    if (body < 0 || body >= w->nb || w->b[body].islandIndex < 0) return 0;
+    const OIsland* I = &w->isl[w->b[body].islandIndex];
+    *vel = I->velSteps;
+    *pos = I->posSteps;
+    return 1;
+}
+uint32_t oc_sort_key(uint32_t a, uint32_t b, uint32_t sa, uint32_t sb) { return sort_key(a, b, sa, sb); }
+void oc_contact_tangents(const double* n, double* t1, double* t2) {
+    V3 a, b;
+    contact_tangents(vload(n), &a, &b);
+    vstore(t1, a);
+    vstore(t2, b);
 }
 
 static int cmp_pair(const void* x, const void* y) {

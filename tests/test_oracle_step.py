@@ -295,7 +295,7 @@ def test_sleep_timer_needs_half_a_second():
 
 def _chain_gaps(sc, bs):
     gaps = []
-    for a, b, la, lb in sc.point_constraints:
+    for a, b, la, lb, *_ in sc.point_constraints:
         def world_pt(i, lp):
             x, y, z, ww = bs["quaternion"][i]
             r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * ww), 2 * (x * z + y * ww)],

@@ -70,7 +70,7 @@ def test_batched_worlds_layout():
     # floor + n in [8, 512] + 2 chains of 8
     assert np.all(counts >= 1 + 8 + 16) and np.all(counts <= 1 + 512 + 16)
     assert len(sc.point_constraints) == 16 * 2 * 7
-    for a, b, _, _ in sc.point_constraints:
+    for a, b, *_ in sc.point_constraints:
         w = np.searchsorted(offs, a, side="right") - 1
         assert offs[w] <= b < offs[w + 1]
 
