@@ -15,7 +15,7 @@ SYMBOLS = [
     "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
     "cc_get_phase_times", "cc_get_totals", "cc_build_triangle_mesh", "cc_download_removed_contacts",
-    "cc_snapshot", "cc_restore", "cc_add_forces",
+    "cc_snapshot", "cc_restore", "cc_add_forces", "cc_download_removed_contacts_noalloc",
 ]
 
 
@@ -48,6 +48,7 @@ def lib():
     L.cc_download_pairs.argtypes = [vp, ip, C.c_int32, ip]
     L.cc_download_contacts.argtypes = [vp, C.POINTER(_abi.ContactOut), C.c_int32, ip]
     L.cc_download_removed_contacts.argtypes = [vp, C.POINTER(_abi.ContactRemoved), C.c_int32, ip]
+    L.cc_download_removed_contacts_noalloc.argtypes = [vp, C.POINTER(_abi.ContactRemoved), C.c_int32, ip]
     L.cc_snapshot.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.cc_restore.argtypes = [vp, vp, C.c_int64]
     L.cc_get_counters.argtypes = [vp, C.POINTER(_abi.StepCounters)]

@@ -224,6 +224,28 @@ static int fail(cc_ctx* c, int code, const std::string& msg) {
         if (e_ != hipSuccess) return fail(ctx, CC_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// ---------------------------------------------------------------- allocation-free scratch
+// Device memory of the entry points that allocate nothing (cc_download_removed_contacts_noalloc):
+// one array per GPU, held until the process ends.  Host code never passes the array's name to a
+// kernel or a copy (on the host it is only a stand-in symbol); it uses the device address that
+// hipGetSymbolAddress returned for the device, resolved once per device in cc_create.
+#define CC_SCRATCH_RECORDS CC_NOALLOC_MAX_RECORDS  // removed-contact candidates (16 B each) one call stages
+#define CC_SCRATCH_BYTES ((size_t)CC_SCRATCH_RECORDS * 16 + 256)
+static __device__ __align__(256) unsigned char scratch[CC_SCRATCH_BYTES];
+#define CC_MAX_DEVICES 64
+static void* g_scratch[CC_MAX_DEVICES];  // device address of `scratch` on each device (null: not resolved)
+
+// resolve the scratch address of the current device (called with that device current)
+static hipError_t resolve_scratch(int device) {
+    if (device < 0 || device >= CC_MAX_DEVICES) return hipErrorInvalidDevice;
+    if (__atomic_load_n(&g_scratch[device], __ATOMIC_ACQUIRE)) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    __atomic_store_n(&g_scratch[device], p, __ATOMIC_RELEASE);  // the same address if two threads race
+    return hipSuccess;
+}
+
 extern "C" int cc_create(cc_ctx** out, const cc_config* cfg) {
     if (!out) return CC_E_ARG;
     cc_ctx* c = new cc_ctx();
@@ -240,7 +262,8 @@ extern "C" int cc_create(cc_ctx** out, const cc_config* cfg) {
         delete c;
         return CC_E_HIP;
     }
-    if (hipSetDevice(c->cfg.device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    if (hipSetDevice(c->cfg.device) != hipSuccess || resolve_scratch(c->cfg.device) != hipSuccess ||
+        hipStreamCreate(&c->stream) != hipSuccess) {
         delete c;
         return CC_E_HIP;
     }
@@ -1441,6 +1464,57 @@ extern "C" int cc_download_removed_contacts(cc_ctx* c, cc_contact_removed* out, 
     HIPCHK(c, hipSetDevice(c->cfg.device));
     if (c->cfg.precision == CC_PRECISION_FP64) return removed_impl<double>(c, out, cap, count);
     return removed_impl<float>(c, out, cap, count);
+}
+
+// The removed contacts staged in the device's scratch array instead of per-call allocations; the
+// host sorts them in the caller's buffer.
+template <typename T>
+static int removed_noalloc_impl(cc_ctx* c, cc_contact_removed* out, int cap, int* count) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    char* base = static_cast<char*>(g_scratch[c->cfg.device]);
+    if (!base) return fail(c, CC_E_STATE, "scratch memory of this device is not resolved");
+    int4* buf = reinterpret_cast<int4*>(base + 256);
+    int* cnt = reinterpret_cast<int*>(base);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int q = 0, rc[2] = {0, 0};
+    HIPCHK(c, hipMemcpy(&q, d.parity, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(rc, d.recCnt, sizeof(rc), hipMemcpyDeviceToHost));
+    const int n = std::min(rc[q], d.conCap);
+    *count = 0;
+    if (n <= 0) return CC_OK;
+    if (n > CC_SCRATCH_RECORDS)
+        return fail(c, CC_E_CAPACITY, "the previous step kept " + std::to_string(n) + " contacts; the allocation-free " +
+                                          "download stages at most " + std::to_string(CC_SCRATCH_RECORDS));
+    // the counter starts at zero on every call (a fill kernel, so that the call can be captured)
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, cnt, 0, 1);
+    hipLaunchKernelGGL(k_removed_contacts<T>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->stream, d, q, buf, n, cnt);
+    HIPCHK(c, hipGetLastError());
+    int m = 0;
+    HIPCHK(c, hipMemcpyAsync(&m, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *count = m;
+    if (!out || cap <= 0 || m == 0) return CC_OK;
+    if (cap < m)
+        return fail(c, CC_E_ARG, "capacity " + std::to_string(cap) + " is below the " + std::to_string(m) +
+                                     " removed contacts (query the count with capacity 0)");
+    static_assert(sizeof(cc_contact_removed) == sizeof(int4), "cc_contact_removed is four 32-bit fields");
+    HIPCHK(c, hipMemcpy(out, buf, (size_t)m * sizeof(int4), hipMemcpyDeviceToHost));
+    std::sort(out, out + m, [](const cc_contact_removed& x, const cc_contact_removed& y) {
+        if (x.body_a != y.body_a) return x.body_a < y.body_a;
+        if (x.body_b != y.body_b) return x.body_b < y.body_b;
+        if (x.sub_shape_a != y.sub_shape_a) return x.sub_shape_a < y.sub_shape_a;
+        return x.sub_shape_b < y.sub_shape_b;
+    });
+    return CC_OK;
+}
+
+extern "C" int cc_download_removed_contacts_noalloc(cc_ctx* c, cc_contact_removed* out, int32_t cap, int32_t* count) {
+    if (!c || !c->impl || !count) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (cap < 0) return fail(c, CC_E_ARG, "negative capacity");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return removed_noalloc_impl<double>(c, out, cap, count);
+    return removed_noalloc_impl<float>(c, out, cap, count);
 }
 
 extern "C" int cc_download_contacts(cc_ctx* c, cc_contact_out* out, int32_t cap, int32_t* count) {

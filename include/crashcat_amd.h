@@ -364,6 +364,17 @@ int cc_download_contacts(cc_ctx* ctx, cc_contact_out* out, int32_t capacity, int
  * destroyUnprocessedContacts (:531-555) find unprocessed.  *out_count is the number removed;
  * at most `capacity` are written.  Call cc_step with one step to see every step's events. */
 int cc_download_removed_contacts(cc_ctx* ctx, cc_contact_removed* out, int32_t capacity, int32_t* out_count);
+/* cc_download_removed_contacts without any device allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory once they run, such as a
+ * serving loop that reads every step's removed contacts (the plain entry point allocates and
+ * frees a staging list per call, and a free synchronizes the whole device).  The staging list and
+ * its counter live in one array per GPU, held until the process ends, of CC_NOALLOC_MAX_RECORDS
+ * records: a previous step that kept more contacts is rejected with CC_E_CAPACITY.  Results are
+ * those of cc_download_removed_contacts; with out != NULL and 0 < capacity < *out_count it fails
+ * with CC_E_ARG (query the count with capacity 0 first).  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (several host threads, several contexts) race. */
+#define CC_NOALLOC_MAX_RECORDS (1 << 20)
+int cc_download_removed_contacts_noalloc(cc_ctx* ctx, cc_contact_removed* out, int32_t capacity, int32_t* out_count);
 int cc_get_counters(cc_ctx* ctx, cc_step_counters* out);
 
 /* Broadphase pair set on caller-supplied AABBs (one world), without stepping; the
