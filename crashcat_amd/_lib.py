@@ -15,7 +15,7 @@ SYMBOLS = [
     "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
     "cc_get_phase_times", "cc_get_totals", "cc_build_triangle_mesh", "cc_download_removed_contacts",
-    "cc_snapshot", "cc_restore", "cc_add_forces", "cc_download_removed_contacts_noalloc",
+    "cc_snapshot", "cc_restore", "cc_add_forces", "cc_download_removed_contacts_noalloc", "cc_fwd", "cc_bwd",
 ]
 
 
@@ -57,6 +57,8 @@ def lib():
     L.cc_get_phase_times.argtypes = [vp, C.POINTER(C.c_double), C.c_int32, ip]
     L.cc_get_totals.argtypes = [vp, C.POINTER(_abi.Totals)]
     L.cc_add_forces.argtypes = [vp, dp, dp, C.c_int32, C.c_int32]
+    L.cc_fwd.argtypes = [vp, C.c_double, C.c_int32, C.c_int32]
+    L.cc_bwd.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     for s in SYMBOLS:
         getattr(L, s).restype = getattr(L, s).restype if s == "cc_last_error" else C.c_int
     _LIB = L
@@ -142,6 +144,23 @@ class Context:
         t = None if torque is None else np.ascontiguousarray(torque, dtype=np.float64).reshape(self.n, 3)
         self._check(lib().cc_add_forces(self.h, None if f is None else f.ctypes.data_as(dp),
                                         None if t is None else t.ctypes.data_as(dp), self.n, int(bool(wake))))
+
+    def fwd(self, dt, nsteps, checkpoint_every=10):
+        """Differentiable forward: nsteps steps of dt with a checkpoint every checkpoint_every."""
+        self._check(lib().cc_fwd(self.h, dt, nsteps, checkpoint_every))
+
+    def bwd(self, dl_dcom, dl_dvel=None):
+        """Adjoint of the last fwd: returns dL/d(initial v, w, com) as (n, 3) arrays and dL/d(friction
+        of each body) as (n,), given dL/d(final com) (and optionally dL/d(final v))."""
+        dp = C.POINTER(C.c_double)
+        g = np.ascontiguousarray(dl_dcom, dtype=np.float64).reshape(self.n, 3)
+        gv = None if dl_dvel is None else np.ascontiguousarray(dl_dvel, dtype=np.float64).reshape(self.n, 3)
+        out = {k: np.zeros((self.n, 3)) for k in ("v0", "w0", "com0")}
+        out["friction"] = np.zeros(self.n)
+        self._check(lib().cc_bwd(self.h, g.ctypes.data_as(dp), None if gv is None else gv.ctypes.data_as(dp),
+                                 out["v0"].ctypes.data_as(dp), out["w0"].ctypes.data_as(dp),
+                                 out["com0"].ctypes.data_as(dp), out["friction"].ctypes.data_as(dp)))
+        return out
 
     def pairs(self):
         cnt = C.c_int32()

@@ -167,6 +167,13 @@ struct Dev {
     int* islIt;         // [3 * N] per island root when a body or point constraint has iteration
                         // overrides (else null): max velocity and position overrides, default flags
     int hasForces;      // a body was given a force or torque: integrate and clear them
+    // differentiable stepping (cc_fwd / cc_bwd): constraints record their pair-order rank and the
+    // warm-start source of each point; when tapeLam is set the solver also records its sweeps
+    int diff;
+    LamRec<T>* tapeLam;  // [(velIters + 1) * ptsCap]: sweep s at (s + 1) * ptsCap, the warm start at 0
+    int* tapeSrc;        // [ptsCap] warm-start source of each staged point
+    int* tapeRank;       // [svCap] pair-order rank of each staged element
+    int ptsCap;          // 4 * svCap
     T* prevDt;          // [1]
     int* parity;        // [1]
     unsigned long long* acc;  // [ACC_N] running sums over steps, see ACC_*
@@ -560,6 +567,26 @@ __global__ void k_bp_spill(Dev<T> d) {
     }
 }
 
+// Differentiable stepping: each querying body's partners in ascending order, so that the pair
+// order (and with it the constraints' pair-order ranks) does not depend on which thread found a
+// pair first.  Nothing else reads the pair order: constraint creation and solve orders come from
+// their own keys.
+template <typename T>
+__global__ void k_bp_sort(Dev<T> d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N) return;
+    const int lo = d.bpOff[i], hi = min(d.bpOff[i + 1], d.pairCap);
+    for (int p = lo + 1; p < hi; p++) {
+        const int v = d.pairB[p];
+        int q = p - 1;
+        while (q >= lo && d.pairB[q] > v) {
+            d.pairB[q + 1] = d.pairB[q];
+            q--;
+        }
+        d.pairB[q + 1] = v;
+    }
+}
+
 // --------------------------------------------------------------- scans
 // Exclusive scan of n ints (n <= 4096 * 4096) in three launches.
 __global__ void k_scan_blocks(const int* in, int* out, int* blockSums, int n);
@@ -689,9 +716,14 @@ __device__ inline void make_contact(const Dev<T>& d, int slot, int A, int Bi, co
         }
     }
     d.recHit[par][cs] = 0;
+    // differentiable stepping: the source of each in-place slot's impulses (rank * 4 + point of
+    // the previous step's constraint; slots past its point count hold older values: no source)
+    int srcOf[4] = {-1, -1, -1, -1};
     if (prev >= 0) {
         rc = d.rec[par ^ 1][prev];
         d.recHit[par ^ 1][prev] = 1;  // processedThisFrame (src/contacts.ts:468)
+        if (d.diff)
+            for (int j = 0; j < 4; j++) srcOf[j] = j < rc.np ? rc.rank * 4 + j : -1;
     } else {
         rc.np = 0;
         for (int j = 0; j < 4; j++) { rc.p1[j] = rc.p2[j] = mk3<T>(0, 0, 0); rc.ln[j] = rc.lt1[j] = rc.lt2[j] = T(0); }
@@ -732,15 +764,21 @@ __device__ inline void make_contact(const Dev<T>& d, int slot, int A, int Bi, co
         cp.localA = mul_t(rotA, pA - comA);
         cp.localB = mul_t(rotB, pB - comB);
         bool set = false;
+        int srcI = -1;
         if (prev >= 0) {  // first match over the in-place slots (slots < i already hold this step's points)
             for (int j = 0; j < rc.np; j++) {
                 T d1 = len2(cp.localA - rc.p1[j]), d2 = len2(cp.localB - rc.p2[j]);
                 if (d1 < d.s.preserveDist2 && d2 < d.s.preserveDist2) {
                     cp.lN = rc.ln[j]; cp.lT1 = rc.lt1[j]; cp.lT2 = rc.lt2[j];
+                    srcI = srcOf[j];
                     set = true;
                     break;
                 }
             }
+        }
+        if (d.diff) {
+            tp.src[i] = srcI;
+            srcOf[i] = srcI;  // slot i now holds this point and its impulses
         }
         if (!set) { cp.lN = cp.lT1 = cp.lT2 = T(0); }
         V3<T> mid = mk3<T>((pA.x + pB.x) * T(0.5), (pA.y + pB.y) * T(0.5), (pA.z + pB.z) * T(0.5));

@@ -84,8 +84,10 @@ __device__ inline void vw_store(const Dev<T>& d, int i, V3<T> v, V3<T> w) {
 // first update, so a constraint costs two dependent memory round trips (its element record,
 // then bodies and points).  In the last sweep (rec set) the impulses go to this step's cache
 // record (storeAppliedImpulses :1061-1076) instead of the point records.
-template <typename T>
-__device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, CacheRec<T>* rec) {
+// TAPE: also record each point's impulses of this sweep for the adjoint (tl: the sweep's records)
+template <typename T, bool TAPE = false>
+__device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, CacheRec<T>* rec,
+                                           LamRec<T>* tl = nullptr) {
     const int meta = E.meta;
     const int dyn = meta & 3;
     const int mov = (meta >> 2) & 3, np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
@@ -99,8 +101,10 @@ __device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, C
     const V3<T> n = E.n, t1 = contact_t1(n), t2 = cross(n, t1);
     const T iA = IA.invMass, iB = IB.invMass, fr = E.fr;
     const int npS = dyn ? np : 0;  // a constraint without a dynamic body is not solved
+    T pre1[4], pre2[4], clp[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
+        pre1[j] = pre2[j] = clp[j] = T(0);
         if (j >= npS) continue;
         const VPoint<T>& p = P[j];
         VPoint<T>& l = Lm[j];
@@ -109,7 +113,13 @@ __device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, C
             T l2 = l.lT2 + p.effT2 * axis_jv(dyn, mov, p.r1, p.r2, vA, wA, vB, wB, t2);
             T mf = fr * l.lN;
             T fm2 = l1 * l1 + l2 * l2;
-            if (fm2 > mf * mf) { T sc = mf / sqrt(fm2); l1 *= sc; l2 *= sc; }
+            if (TAPE) { pre1[j] = l1; pre2[j] = l2; }
+            if (fm2 > mf * mf) {
+                T sc = mf / sqrt(fm2);
+                l1 *= sc;
+                l2 *= sc;
+                if (TAPE) clp[j] = T(1);
+            }
             T d1 = l1 - l.lT1;
             l.lT1 = l1;
             if (d1 != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, IA.I, IB.I, p.r1, p.r2, vA, wA, vB, wB, t1, d1);
@@ -128,6 +138,18 @@ __device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, C
         T dn = tl - l.lN;
         l.lN = tl;
         if (dn != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, IA.I, IB.I, p.r1, p.r2, vA, wA, vB, wB, n, dn);
+    }
+    if (TAPE) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j >= np) continue;
+            LamRec<T> r;
+            r.n = Lm[j].lN;
+            r.t1 = pre1[j];
+            r.t2 = pre2[j];
+            r.clamp = clp[j];
+            tl[E.pt + j] = r;
+        }
     }
     if (rec) {
         CacheRec<T>& r = rec[E.slot];
@@ -645,7 +667,8 @@ __device__ inline int isl_iters(const Dev<T>& d, int label) {
     return v | (p << 16);
 }
 
-template <typename T, int BS>
+// TAPE: differentiable stepping records the staged constraints' sweeps for the adjoint
+template <typename T, int BS, bool TAPE = false>
 __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __shared__ int sh[BS / 64 + 1];
     __shared__ int sLS[LS_LDS];
@@ -888,6 +911,14 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
                 continue;
             }
             const Con<T>& k = d.con[s];
+            if (d.diff) {  // pair-order rank: the constraint's place in the world's sequence (G == 1)
+                const int rank = c0 + d.sched[so + x];
+                d.rec[par][s].rank = rank;
+                if (TAPE) {
+                    d.tapeRank[e] = rank;
+                    for (int j = 0; j < k.np; j++) d.tapeSrc[(size_t)pbase + j] = d.tap[s].src[j];
+                }
+            }
             E.iters = isl_iters(d, B.label[k.a] >= 0 ? B.label[k.a] : B.label[k.b]);
             mv = max(mv, E.iters & 0xffff);
             mp = max(mp, E.iters >> 16);
@@ -917,6 +948,24 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     }
     const int* LSp = lsLds ? sLS : LS;
     if (!kCreationOrderWarmStart && d.s.warmStarting) warm_start_levels<T, BS, true>(d, so, D, LSp, ratio);
+    if (TAPE) {  // the warm-started impulses, sweep -1 of the record
+        __syncthreads();
+        for (int x = tid; x < L; x += BS) {
+            const VElem<T>& E = d.ve[so + x];
+            if ((E.meta >> 28) != VK_CONTACT) continue;
+            const int np = (E.meta >> 4) & 7;
+            for (int j = 0; j < np; j++) {
+                const VPoint<T>& P = d.vp[(size_t)E.pt + j];
+                LamRec<T> r;
+                r.n = P.lN;
+                r.t1 = P.lT1;
+                r.t2 = P.lT2;
+                r.clamp = T(0);
+                d.tapeLam[(size_t)E.pt + j] = r;
+            }
+        }
+        __syncthreads();
+    }
     // ---- 6. velocity iterations; the last one also stores the impulses (step 7)
     for (int it = 0; it < maxVel; it++) {
         for (int l = 0; l < D; l++) {
@@ -927,7 +976,8 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
                 const int nv = E.iters & 0xffff;
                 if (it >= nv) continue;
                 CacheRec<T>* rec = it == nv - 1 ? d.rec[par] : nullptr;
-                if (kind == VK_CONTACT) solve_velocity_elem(d, E, rec);
+                if (TAPE && kind == VK_CONTACT) solve_velocity_elem<T, true>(d, E, rec, d.tapeLam + (size_t)(it + 1) * d.ptsCap);
+                else if (kind == VK_CONTACT) solve_velocity_elem(d, E, rec);
                 else if (kind == VK_POINT) point_solve_velocity_vw(d, d.pc[E.a]);
             }
             __syncthreads();

@@ -101,6 +101,7 @@ struct CacheRec {
     unsigned subA, subB;   // sub-shape ids (0xffffffff when empty)
     V3<T> p1[4], p2[4];
     T ln[4], lt1[4], lt2[4];
+    int rank;              // differentiable stepping: the constraint's place in pair order this step
 };
 // What the sweeps read: one record per body, per constraint and per contact point, each read
 // with one-line loads (a random body costs one memory line rather than one per field).
@@ -146,6 +147,16 @@ struct alignas(16) PBody {
 template <typename T>
 struct ConTap {
     V3<T> pa[4], pb[4];
+    int src[4];  // differentiable stepping: where each point's warm-start impulses came from
+                 // (rank * 4 + point of the previous step's constraint), -1 for none
+};
+
+// One contact point after one velocity sweep, recorded for the adjoint: its normal impulse, its
+// friction impulses before the disc projection, and 1 when the projection clamped them (the
+// recorded sweep -1 holds the warm-started impulses).
+template <typename T>
+struct alignas(16) LamRec {
+    T n, t1, t2, clamp;
 };
 
 // Ball joint (src/constraints/point-constraint.ts, constraint-part/point-constraint-part.ts).

@@ -12,7 +12,7 @@
 #include "../../include/crashcat_amd.h"
 #include "cc_mesh_build.h"
 #include "cc_mesh.h"
-#include "cc_solver.h"
+#include "cc_adjoint.h"
 
 using namespace cc;
 
@@ -180,6 +180,7 @@ struct Alloc {
 
 struct Impl {
     int* counters = nullptr;  // device step counters (CNT_*), shared by both precisions
+    virtual void state_changed() {}  // a step, body change or restore: a forward record no longer applies
     virtual ~Impl() {}
 };
 
@@ -198,7 +199,48 @@ struct CtxT : Impl {
     size_t worldLdsNeed = 0;   // ints of LDS the largest world can use
     size_t worldLdsBytes = 0;  // dynamic LDS given to the world kernel
     int worldLdsBS = 0;        // block size worldLdsBytes was sized for
+    size_t tapeLdsBytes = 0;   // the same for the recording solver (block size 256)
+    bool tapeLdsSet = false;
+    // differentiable stepping: checkpoints of the forward run, one step record per step of a
+    // segment, the adjoint state
+    struct Diff {
+        bool valid = false;        // cc_fwd ran and nothing changed the state since
+        int n = 0, K = 0;          // steps, checkpoint interval
+        double dt = 0;
+        std::vector<T> ratio;      // warm-start ratio of each step
+        std::vector<char*> ckpt;   // checkpoints at steps 0, K, 2K, ... and after the last step
+        size_t ckptBytes = 0;
+        int slots = 0;             // step records allocated
+        VElem<T>* ve = nullptr;
+        VPoint<T>* vp = nullptr;
+        V3<T>* la = nullptr;
+        V3<T>* lb = nullptr;
+        int* lvl = nullptr;
+        LamRec<T>* lam = nullptr;
+        int* src = nullptr;
+        int* rank = nullptr;
+        int* stats = nullptr;
+        int* wcs = nullptr;
+        VBody<T>* vbody = nullptr;
+        NpBody<T>* npb = nullptr;
+        size_t lvlInts = 0;
+        V3<T>* xb = nullptr;       // adjoints
+        V3<T>* qb = nullptr;
+        V3<T>* vbar = nullptr;
+        V3<T>* wbar = nullptr;
+        T* mub = nullptr;
+        T* lamBar = nullptr;
+        T* bBar = nullptr;
+        V3<T>* rBar = nullptr;
+        V3<T>* vf = nullptr;
+        V3<T>* wf = nullptr;
+        T* seed[2] = {nullptr, nullptr};
+        Alloc mem;                 // everything above
+    } diff;
+    void state_changed() override { diff.valid = false; }
     ~CtxT() {
+        diff.mem.release();
+        for (char* p : diff.ckpt) (void)hipFree(p);
         mem.release();
         for (auto& e : ev) (void)hipEventDestroy(e);
     }
@@ -919,6 +961,7 @@ static void launch_broadphase(CtxT<T>* X, hipStream_t st, double margin) {
     hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
     hipLaunchKernelGGL(k_bp_compact<T>, dim3(nbB), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_bp_spill<T>, dim3(grid_for(d.spillCap, 256, 1024)), dim3(256), 0, st, d);
+    if (d.diff) hipLaunchKernelGGL(k_bp_sort<T>, dim3(nbB), dim3(256), 0, st, d);
 }
 
 // dynamic LDS of the world kernel: what the largest world can use (its per-body counts and
@@ -949,8 +992,38 @@ static void launch_world_bs(CtxT<T>* X, hipStream_t st, T dt) {
     hipLaunchKernelGGL((k_world<T, BS>), dim3(d.V), dim3(BS), X->worldLdsBytes, st, d, dt);
 }
 
+// the recording solver of differentiable stepping (block size 256; worlds are never split there)
 template <typename T>
-static void launch_world(CtxT<T>* X, hipStream_t st, T dt) {
+static void launch_world_tape(CtxT<T>* X, hipStream_t st, T dt) {
+    Dev<T>& d = X->d;
+    if (!X->tapeLdsSet) {
+        hipFuncAttributes fa;
+        int maxShared = 0;
+        size_t dyn = 0;
+        if (hipFuncGetAttributes(&fa, (const void*)k_world<T, 256, true>) == hipSuccess &&
+            hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, X->device) == hipSuccess &&
+            (size_t)maxShared > fa.sharedSizeBytes) {
+            size_t avail = ((size_t)maxShared - fa.sharedSizeBytes) / 4;
+            dyn = std::min(avail, X->worldLdsNeed) * 4;
+        }
+        if (dyn > 0 &&
+            hipFuncSetAttribute((const void*)k_world<T, 256, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) !=
+                hipSuccess)
+            dyn = 0;
+        X->tapeLdsBytes = dyn;
+        X->tapeLdsSet = true;
+    }
+    d.worldLdsInts = (int)(X->tapeLdsBytes / 4);
+    hipLaunchKernelGGL((k_world<T, 256, true>), dim3(d.V), dim3(256), X->tapeLdsBytes, st, d, dt);
+    d.worldLdsInts = (int)(X->worldLdsBytes / 4);
+}
+
+template <typename T>
+static void launch_world(CtxT<T>* X, hipStream_t st, T dt, bool tape = false) {
+    if (tape) {
+        launch_world_tape<T>(X, st, dt);
+        return;
+    }
     switch (X->solverBS) {
         case 64: launch_world_bs<T, 64>(X, st, dt); break;
         case 128: launch_world_bs<T, 128>(X, st, dt); break;
@@ -962,7 +1035,7 @@ static void launch_world(CtxT<T>* X, hipStream_t st, T dt) {
 
 #define CC_PHASES 7
 template <typename T>
-static int step_impl(cc_ctx* c, double dtd, int nsteps) {
+static int step_impl(cc_ctx* c, double dtd, int nsteps, bool tape = false) {
     auto* X = static_cast<CtxT<T>*>(c->impl);
     Dev<T>& d = X->d;
     hipStream_t st = c->stream;
@@ -1019,7 +1092,7 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps) {
         hipLaunchKernelGGL(k_uf_link<T>, dim3(grid_for(d.conCap + d.NPC, 256, 2048)), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_uf_compress<T>, dim3(nbB), dim3(256), 0, st, d);
         mark();  // islands
-        launch_world<T>(X, st, dt);
+        launch_world<T>(X, st, dt, tape);
         mark();  // solver + integration + sleep
         hipLaunchKernelGGL(k_hash_clear_cur<T>, dim3(grid_for(d.hashCap, 256, 2048)), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_hash_insert<T>, dim3(grid_for(d.conCap, 256, 2048)), dim3(256), 0, st, d);
@@ -1036,6 +1109,7 @@ extern "C" int cc_step(cc_ctx* c, double dt, int32_t nsteps) {
     if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
     if (nsteps < 0 || !(dt >= 0)) return fail(c, CC_E_ARG, "invalid dt or nsteps");
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->impl->state_changed();
     if (c->cfg.precision == CC_PRECISION_FP64) return step_impl<double>(c, dt, nsteps);
     return step_impl<float>(c, dt, nsteps);
 }
@@ -1143,6 +1217,7 @@ extern "C" int cc_set_bodies(cc_ctx* c, const cc_body_state* in, int32_t n) {
     if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
     if (n > 0 && !in) return fail(c, CC_E_ARG, "null input");
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->impl->state_changed();
     if (c->cfg.precision == CC_PRECISION_FP64) return set_bodies_impl<double>(c, in, n);
     return set_bodies_impl<float>(c, in, n);
 }
@@ -1201,6 +1276,7 @@ extern "C" int cc_add_forces(cc_ctx* c, const double* force, const double* torqu
     if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
     if (n < 0) return fail(c, CC_E_ARG, "negative body count");
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->impl->state_changed();
     if (c->cfg.precision == CC_PRECISION_FP64) return add_forces_impl<double>(c, force, torque, n, wake);
     return add_forces_impl<float>(c, force, torque, n, wake);
 }
@@ -1324,6 +1400,7 @@ extern "C" int cc_restore(cc_ctx* c, const void* in, int64_t size) {
     if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
     if (!in) return fail(c, CC_E_ARG, "null snapshot");
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->impl->state_changed();
     if (c->cfg.precision == CC_PRECISION_FP64) return restore_impl<double>(c, in, size);
     return restore_impl<float>(c, in, size);
 }
@@ -1683,4 +1760,252 @@ extern "C" int cc_broadphase_aabbs(cc_ctx* c, const float* mn, const float* mx, 
     HIPCHK(c, hipSetDevice(c->cfg.device));
     if (c->cfg.precision == CC_PRECISION_FP64) return bp_impl<double>(c, mn, mx, mo, sl, la, n, margin, out, cap, count);
     return bp_impl<float>(c, mn, mx, mo, sl, la, n, margin, out, cap, count);
+}
+
+// ---------------------------------------------------------------- differentiable stepping
+// cc_fwd steps like cc_step and keeps a checkpoint of the state every K steps; cc_bwd walks the
+// segments backwards: it restores a segment's checkpoint, steps through the segment again with
+// the solver recording (bit for bit the same steps), and runs the adjoint of each recorded step
+// from the last to the first (cc_adjoint.h).
+template <typename T>
+static std::vector<std::pair<void*, size_t>> ckpt_segments(Dev<T>& d) {
+    auto v = snap_segments(d);
+    for (int k = 0; k < 2; k++) {
+        v.push_back({d.rec[k], (size_t)d.conCap * sizeof(CacheRec<T>)});
+        v.push_back({d.recHit[k], (size_t)d.conCap * sizeof(int)});
+        v.push_back({d.hkeys[k], (size_t)d.hashCap * sizeof(unsigned long long)});
+        v.push_back({d.hvals[k], (size_t)d.hashCap * sizeof(int)});
+    }
+    v.push_back({d.recCnt, 2 * sizeof(int)});
+    v.push_back({d.parity, sizeof(int)});
+    v.push_back({d.prevDt, sizeof(T)});
+    v.push_back({d.counters, CNT_N * sizeof(int)});
+    return v;
+}
+
+template <typename T>
+static hipError_t ckpt_copy(CtxT<T>* X, hipStream_t st, char* buf, bool save) {
+    size_t off = 0;
+    for (auto& sg : ckpt_segments(X->d)) {
+        if (sg.second) {
+            const hipError_t e = save ? hipMemcpyAsync(buf + off, sg.first, sg.second, hipMemcpyDeviceToDevice, st)
+                                      : hipMemcpyAsync(sg.first, buf + off, sg.second, hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) return e;
+        }
+        off += sg.second;
+    }
+    return hipSuccess;
+}
+
+template <typename T>
+static int fwd_impl(cc_ctx* c, double dt, int n, int K) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    auto& D = X->diff;
+    D.valid = false;
+    if (X->anySplit)
+        return fail(c, CC_E_UNSUPPORTED, "differentiable stepping solves each world on one workgroup: use "
+                                         "island_groups = 1 or a batch of at least 64 worlds");
+    if (d.NPC > 0) return fail(c, CC_E_UNSUPPORTED, "differentiable stepping has no point constraints");
+    if (d.islIt) return fail(c, CC_E_UNSUPPORTED, "differentiable stepping has no iteration overrides");
+    if (d.s.allowSleeping)
+        return fail(c, CC_E_UNSUPPORTED, "differentiable stepping needs sleeping off (allow_sleeping = 0): a body "
+                                         "put to sleep has its velocities zeroed");
+    X->solverBS = 256;  // the recording solver's block size (results do not depend on it)
+    d.diff = 1;
+    hipStream_t st = c->stream;
+    const auto segs = ckpt_segments(d);
+    size_t bytes = 0;
+    for (auto& sg : segs) bytes += sg.second;
+    const int nck = (n + K - 1) / K + 1;
+    if (D.ckptBytes != bytes) {
+        for (char* p : D.ckpt) (void)hipFree(p);
+        D.ckpt.clear();
+        D.ckptBytes = bytes;
+    }
+    while ((int)D.ckpt.size() < nck) {
+        char* p = nullptr;
+        HIPCHK(c, hipMalloc(&p, bytes));
+        D.ckpt.push_back(p);
+    }
+    T pdt;
+    HIPCHK(c, hipMemcpy(&pdt, d.prevDt, sizeof(T), hipMemcpyDeviceToHost));
+    const T dtT = T(dt);
+    D.ratio.assign(n, T(0));
+    for (int s = 0; s < n; s++) {
+        const T prev = s == 0 ? pdt : dtT;
+        D.ratio[s] = (d.s.warmStarting && prev > T(0)) ? dtT / prev : T(0);
+    }
+    for (int s = 0; s < n; s++) {
+        if (s % K == 0) HIPCHK(c, ckpt_copy(X, st, D.ckpt[s / K], true));
+        const int rc = step_impl<T>(c, dt, 1);
+        if (rc != CC_OK) return rc;
+    }
+    HIPCHK(c, ckpt_copy(X, st, D.ckpt[nck - 1], true));
+    HIPCHK(c, hipStreamSynchronize(st));
+    D.n = n;
+    D.K = K;
+    D.dt = dt;
+    D.valid = true;
+    return CC_OK;
+}
+
+template <typename T>
+static int bwd_alloc(cc_ctx* c, CtxT<T>* X) {
+    Dev<T>& d = X->d;
+    auto& D = X->diff;
+    if (D.slots >= D.K && D.xb) return CC_OK;
+    D.mem.release();
+    const size_t S = (size_t)D.K, N = (size_t)std::max(d.N, 1), sv = (size_t)d.svCap, pts = 4 * sv;
+    const size_t W = (size_t)d.W;
+    D.lvlInts = 2 * (sv + (size_t)d.V + 1);
+    Alloc& A = D.mem;
+    D.ve = A.get<VElem<T>>(S * sv);
+    D.vp = A.get<VPoint<T>>(S * pts);
+    D.la = A.get<V3<T>>(S * pts);
+    D.lb = A.get<V3<T>>(S * pts);
+    D.lvl = A.get<int>(S * D.lvlInts);
+    D.lam = A.get<LamRec<T>>(S * pts * (size_t)(d.s.velIters + 1));
+    D.src = A.get<int>(S * pts);
+    D.rank = A.get<int>(S * sv);
+    D.stats = A.get<int>(S * 2 * W);
+    D.wcs = A.get<int>(S * (W + 1));
+    D.vbody = A.get<VBody<T>>(S * N);
+    D.npb = A.get<NpBody<T>>(S * N);
+    D.xb = A.get<V3<T>>(N);
+    D.qb = A.get<V3<T>>(N);
+    D.vbar = A.get<V3<T>>(N);
+    D.wbar = A.get<V3<T>>(N);
+    D.mub = A.get<T>(N);
+    D.lamBar = A.get<T>(3 * pts);
+    D.bBar = A.get<T>(pts);
+    D.rBar = A.get<V3<T>>(2 * pts);
+    D.vf = A.get<V3<T>>(N);
+    D.wf = A.get<V3<T>>(N);
+    for (int k = 0; k < 2; k++) D.seed[k] = A.get<T>(3 * 4 * (size_t)d.conCap);
+    if (A.err) {
+        D.mem.release();
+        D.xb = nullptr;
+        return fail(c, CC_E_HIP, "hipMalloc failed (step records of the adjoint)");
+    }
+    D.slots = D.K;
+    d.ptsCap = (int)pts;
+    return CC_OK;
+}
+
+template <typename T>
+static int bwd_impl(cc_ctx* c, const double* gcom, const double* gvel, double* gv0, double* gw0, double* gcom0,
+                    double* gfr) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    auto& D = X->diff;
+    if (!D.valid) return fail(c, CC_E_STATE, "cc_bwd follows a cc_fwd with no step, body change or restore after it");
+    const int rc = bwd_alloc(c, X);
+    if (rc != CC_OK) return rc;
+    hipStream_t st = c->stream;
+    const int N = d.N, W = d.W, n = D.n, K = D.K;
+    const size_t sv = (size_t)d.svCap, pts = 4 * sv;
+    {  // the adjoints of the final state
+        std::vector<V3<T>> x(N), v(N), z(N);
+        for (int i = 0; i < N; i++) {
+            x[i] = mk3<T>(T(gcom[3 * i]), T(gcom[3 * i + 1]), T(gcom[3 * i + 2]));
+            v[i] = gvel ? mk3<T>(T(gvel[3 * i]), T(gvel[3 * i + 1]), T(gvel[3 * i + 2])) : mk3<T>(0, 0, 0);
+            z[i] = mk3<T>(0, 0, 0);
+        }
+        HIPCHK(c, hipMemcpy(D.xb, x.data(), N * sizeof(V3<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(D.vbar, v.data(), N * sizeof(V3<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(D.qb, z.data(), N * sizeof(V3<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(D.wbar, z.data(), N * sizeof(V3<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemsetAsync(D.mub, 0, N * sizeof(T), st));
+        for (int k = 0; k < 2; k++) HIPCHK(c, hipMemsetAsync(D.seed[k], 0, 3 * 4 * (size_t)d.conCap * sizeof(T), st));
+    }
+    const Dev<T> keep = d;  // the solver's own staging arrays, restored after each recorded step
+    const int nseg = (n + K - 1) / K;
+    int rcode = CC_OK;
+    for (int seg = nseg - 1; seg >= 0 && rcode == CC_OK; seg--) {
+        HIPCHK(c, ckpt_copy(X, st, D.ckpt[seg], false));
+        const int s0 = seg * K, s1 = std::min(n, s0 + K);
+        for (int s = s0; s < s1; s++) {  // the segment again, recording each step
+            const size_t k = (size_t)(s - s0);
+            d.ve = D.ve + k * sv;
+            d.vp = D.vp + k * pts;
+            d.svLA = D.la + k * pts;
+            d.svLB = D.lb + k * pts;
+            d.lvlStart = D.lvl + k * D.lvlInts;
+            d.tapeLam = D.lam + k * pts * (size_t)(d.s.velIters + 1);
+            d.tapeSrc = D.src + k * pts;
+            d.tapeRank = D.rank + k * sv;
+            d.ptsCap = (int)pts;
+            rcode = step_impl<T>(c, D.dt, 1, true);
+            d.ve = keep.ve; d.vp = keep.vp; d.svLA = keep.svLA; d.svLB = keep.svLB; d.lvlStart = keep.lvlStart;
+            d.tapeLam = nullptr; d.tapeSrc = nullptr; d.tapeRank = nullptr;
+            if (rcode != CC_OK) break;
+            HIPCHK(c, hipMemcpyAsync(D.vbody + k * N, d.vb, N * sizeof(VBody<T>), hipMemcpyDeviceToDevice, st));
+            HIPCHK(c, hipMemcpyAsync(D.npb + k * N, d.npb, N * sizeof(NpBody<T>), hipMemcpyDeviceToDevice, st));
+            HIPCHK(c, hipMemcpyAsync(D.stats + k * 2 * W, d.stats, 2 * W * sizeof(int), hipMemcpyDeviceToDevice, st));
+            HIPCHK(c, hipMemcpyAsync(D.wcs + k * (W + 1), d.wConStart, (W + 1) * sizeof(int), hipMemcpyDeviceToDevice, st));
+        }
+        if (rcode != CC_OK) break;
+        for (int s = s1 - 1; s >= s0; s--) {  // the adjoint of each recorded step, last first
+            const size_t k = (size_t)(s - s0);
+            Adj<T> a;
+            a.xb = D.xb; a.qb = D.qb; a.vb = D.vbar; a.wb = D.wbar; a.mub = D.mub;
+            a.lamBar = D.lamBar; a.bBar = D.bBar; a.rBar = D.rBar; a.vf = D.vf; a.wf = D.wf;
+            a.seedIn = D.seed[s & 1];
+            a.seedOut = D.seed[(s + 1) & 1];
+            a.ve = D.ve + k * sv; a.vp = D.vp + k * pts; a.la = D.la + k * pts; a.lb = D.lb + k * pts;
+            a.lam = D.lam + k * pts * (size_t)(d.s.velIters + 1);
+            a.src = D.src + k * pts; a.rank = D.rank + k * sv; a.lvl = D.lvl + k * D.lvlInts;
+            a.stats = D.stats + k * 2 * W; a.wcs = D.wcs + k * (W + 1);
+            a.vbody = D.vbody + k * N; a.npb = D.npb + k * N;
+            a.ptsCap = (int)pts;
+            a.sweeps = d.s.velIters;
+            a.dt = T(D.dt);
+            a.ratio = D.ratio[s];
+            hipLaunchKernelGGL((k_adj_world<T, 256>), dim3(d.V), dim3(256), 0, st, d, a);
+        }
+        HIPCHK(c, hipGetLastError());
+    }
+    // leave the context in the state cc_fwd left it in
+    HIPCHK(c, ckpt_copy(X, st, D.ckpt[(n + K - 1) / K], false));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (rcode != CC_OK) return rcode;
+    std::vector<V3<T>> h(N);
+    auto out3 = [&](const V3<T>* src, double* dst) -> int {
+        if (!dst) return CC_OK;
+        HIPCHK(c, hipMemcpy(h.data(), src, N * sizeof(V3<T>), hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; i++) {
+            dst[3 * i] = (double)h[i].x;
+            dst[3 * i + 1] = (double)h[i].y;
+            dst[3 * i + 2] = (double)h[i].z;
+        }
+        return CC_OK;
+    };
+    int r = out3(D.vbar, gv0);
+    if (r == CC_OK) r = out3(D.wbar, gw0);
+    if (r == CC_OK) r = out3(D.xb, gcom0);
+    if (r != CC_OK) return r;
+    if (gfr) {
+        std::vector<T> m(N);
+        HIPCHK(c, hipMemcpy(m.data(), D.mub, N * sizeof(T), hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; i++) gfr[i] = (double)m[i];
+    }
+    return CC_OK;
+}
+
+extern "C" int cc_fwd(cc_ctx* c, double dt, int32_t nsteps, int32_t checkpoint_every) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (nsteps < 1 || checkpoint_every < 1 || !(dt > 0)) return fail(c, CC_E_ARG, "invalid dt, nsteps or checkpoint interval");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return fwd_impl<double>(c, dt, nsteps, checkpoint_every);
+    return fwd_impl<float>(c, dt, nsteps, checkpoint_every);
+}
+
+extern "C" int cc_bwd(cc_ctx* c, const double* dl_dcom, const double* dl_dvel, double* dl_dv0, double* dl_dw0,
+                      double* dl_dcom0, double* dl_dfriction) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (!dl_dcom) return fail(c, CC_E_ARG, "dl_dcom is required");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return bwd_impl<double>(c, dl_dcom, dl_dvel, dl_dv0, dl_dw0, dl_dcom0, dl_dfriction);
+    return bwd_impl<float>(c, dl_dcom, dl_dvel, dl_dv0, dl_dw0, dl_dcom0, dl_dfriction);
 }

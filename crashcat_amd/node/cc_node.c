@@ -426,6 +426,58 @@ static napi_value add_forces(napi_env env, napi_callback_info info) {
     return NULL;
 }
 
+/* ---- fwd(ctx, dt, nsteps, checkpointEvery); bwd(ctx, dLdcom, dLdvel|null) -> ArrayBuffer --
+ * bwd returns one ArrayBuffer of 10 * n doubles: dL/dv0 (3n), dL/dw0 (3n), dL/dcom0 (3n) and
+ * dL/dfriction (n), the layout the JavaScript host slices */
+static napi_value fwd(napi_env env, napi_callback_info info) {
+    size_t argc = 4;
+    napi_value a[4];
+    CHECK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+    cc_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    double dt = 1.0 / 60.0;
+    int32_t n = 1, k = 10;
+    if (argc > 1) napi_get_value_double(env, a[1], &dt);
+    if (argc > 2) napi_get_value_int32(env, a[2], &n);
+    if (argc > 3) napi_get_value_int32(env, a[3], &k);
+    int rc = cc_fwd(c, dt, n, k);
+    if (rc != CC_OK) return throw_rc(env, c, rc);
+    return NULL;
+}
+
+static napi_value bwd(napi_env env, napi_callback_info info) {
+    size_t argc = 3;
+    napi_value a[3];
+    CHECK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+    if (argc < 2) { napi_throw_type_error(env, NULL, "crashcat_amd: bwd needs a context and dL/dcom"); return NULL; }
+    cc_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    size_t gb = 0, vb = 0;
+    const double* g = (const double*)buffer_data(env, a[1], &gb);
+    if (!g) return NULL;
+    const double* gv = NULL;
+    if (argc > 2) {
+        napi_valuetype t;
+        CHECK(env, napi_typeof(env, a[2], &t));
+        if (t != napi_null && t != napi_undefined) {
+            gv = (const double*)buffer_data(env, a[2], &vb);
+            if (!gv) return NULL;
+        }
+    }
+    if (gb % (3 * sizeof(double)) || (gv && vb != gb)) {
+        napi_throw_range_error(env, NULL, "crashcat_amd: dL/dcom and dL/dvel need 3 doubles per body");
+        return NULL;
+    }
+    const size_t n = gb / (3 * sizeof(double));
+    void* data;
+    napi_value ab;
+    CHECK(env, napi_create_arraybuffer(env, 10 * n * sizeof(double), &data, &ab));
+    double* o = (double*)data;
+    int rc = cc_bwd(c, g, gv, o, o + 3 * n, o + 6 * n, o + 9 * n);
+    if (rc != CC_OK) return throw_rc(env, c, rc);
+    return ab;
+}
+
 /* ---- snapshot(ctx) -> ArrayBuffer; restore(ctx, buffer) ---------------------------------- */
 static napi_value snapshot(napi_env env, napi_callback_info info) {
     size_t argc = 1;
@@ -474,6 +526,8 @@ static napi_value init(napi_env env, napi_value exports) {
         {"snapshot", 0, snapshot, 0, 0, 0, napi_default, 0},
         {"restore", 0, restore, 0, 0, 0, napi_default, 0},
         {"addForces", 0, add_forces, 0, 0, 0, napi_default, 0},
+        {"fwd", 0, fwd, 0, 0, 0, napi_default, 0},
+        {"bwd", 0, bwd, 0, 0, 0, napi_default, 0},
     };
     napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
     return exports;

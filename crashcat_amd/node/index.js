@@ -197,6 +197,45 @@ class GpuWorld {
     addon.counters(this.handle, buf);
     return unpack('cc_step_counters', buf)[0];
   }
+
+  // addForce / addTorque (src/body/rigid-body.ts:840-869) on every body: Float64Arrays of 3 per
+  // body (either may be null); they act on the next step, which then clears them
+  addForces(force, torque, wake) {
+    addon.addForces(this.handle, force || null, torque || null, wake === undefined ? 1 : (wake ? 1 : 0));
+  }
+
+  // differentiable stepping (cc_fwd / cc_bwd): nSteps steps with a checkpoint every
+  // checkpointEvery, then the adjoint given dL/d(final centres of mass) (Float64Array, 3 per
+  // body) and optionally dL/d(final linear velocities)
+  fwd(timeStep, nSteps, checkpointEvery) {
+    addon.fwd(this.handle, timeStep, nSteps, checkpointEvery || 10);
+    this.previousTimeStep = timeStep;
+  }
+  bwd(dLdCom, dLdVel) {
+    const n = this.numBodies;
+    const all = new Float64Array(addon.bwd(this.handle, dLdCom, dLdVel || null));
+    return {
+      linearVelocity: all.subarray(0, 3 * n),
+      angularVelocity: all.subarray(3 * n, 6 * n),
+      centerOfMass: all.subarray(6 * n, 9 * n),
+      friction: all.subarray(9 * n, 10 * n),
+    };
+  }
+}
+
+// A run of steps as an autograd Function: forward(ctx, world, timeStep, nSteps, checkpointEvery)
+// steps the world from its uploaded state and returns the final centres of mass (Float64Array,
+// 3 per body); backward(ctx, dLdCom) returns the gradients with respect to the initial linear
+// and angular velocities, centres of mass and each body's friction coefficient.
+class WorldRun {
+  static forward(ctx, world, timeStep, nSteps, checkpointEvery) {
+    world.fwd(timeStep, nSteps, checkpointEvery);
+    ctx.world = world;
+    const com = new Float64Array(3 * world.numBodies);
+    world.bodies().forEach((b, i) => com.set(b.center_of_mass_position, 3 * i));
+    return com;
+  }
+  static backward(ctx, dLdCom) { return ctx.world.bwd(dLdCom); }
 }
 
 function createWorld(desc, opts) { return new GpuWorld(desc, opts); }
@@ -213,4 +252,4 @@ function updateWorld(world, listener, timeStep) {
   if (names.length) world.dispatchContactEvents(listener);
 }
 
-module.exports = { createWorld, updateWorld, GpuWorld, pack, unpack, layout: LAYOUT, addon };
+module.exports = { createWorld, updateWorld, GpuWorld, WorldRun, pack, unpack, layout: LAYOUT, addon };

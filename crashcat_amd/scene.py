@@ -497,6 +497,41 @@ def batched_worlds(seed=1003, num_worlds=4096, min_bodies=8, max_bodies=512, cha
     return sc
 
 
+def throw_worlds(seed=1005, num_worlds=1024, friction=0.5, scene=None):
+    """Config 5: independent worlds, each a 4x4x4 stack of boxes of half-extent 0.25 (density
+    1000) resting on a static floor and a projectile sphere (r 0.4, 10 kg) launched at the stack
+    from 4 m away with v0 ~ N((12, 2, 0), 1) m/s; one shared friction coefficient on every body
+    (geometric-mean combine, so each contact's friction equals it); sleeping off.  The loss of
+    world w is |centroid of its stack - target_w|^2 after the run, with a seeded target a little
+    past the stack.  Returns the scene with `throw` = {stack: (W, 64) body indices, projectile:
+    (W,) body indices, target: (W, 3)}."""
+    sc = scene if scene is not None else Scene()
+    sc.settings.allow_sleeping = 0
+    box = S.box((0.25, 0.25, 0.25))
+    r = 0.4
+    ball = S.sphere(r, density=10.0 / (4.0 / 3.0 * math.pi * r ** 3))
+    stack, proj, target = [], [], []
+    for w in range(num_worlds):
+        rng = Rng((seed << 32) + w)
+        _floor(sc, friction=friction)
+        ids = []
+        for j in range(4):
+            for i in range(4):
+                for k in range(4):
+                    x = (i - 1.5) * 0.505
+                    y = 0.25 + 0.001 + j * 0.502
+                    z = (k - 1.5) * 0.505
+                    ids.append(sc.add_body(box, (x, y, z), friction=friction))
+        v0 = (rng.normal(12.0, 1.0), rng.normal(2.0, 1.0), rng.normal(0.0, 1.0))
+        proj.append(sc.add_body(ball, (-4.0, 1.0, 0.0), linear_velocity=v0, friction=friction))
+        stack.append(ids)
+        target.append((rng.uniform(0.2, 0.8), rng.uniform(0.8, 1.0), rng.uniform(-0.3, 0.3)))
+        sc.end_world()
+    sc.throw = {"stack": np.array(stack, dtype=np.int64), "projectile": np.array(proj, dtype=np.int64),
+                "target": np.array(target)}
+    return sc
+
+
 def terrain_heights(rng, amps=(0.5, 3.0), wavelengths=(8.0, 64.0), terms=4):
     """height(x, z) = sum of `terms` seeded sinusoids: amplitude U[amps], wavelength U[wavelengths],
     direction uniform in the plane, phase U[0, 2 pi)."""

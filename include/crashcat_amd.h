@@ -401,6 +401,30 @@ int cc_get_phase_times(cc_ctx* ctx, double* out_ms, int32_t capacity, int32_t* o
  * CC_E_ARG otherwise): the steps that follow are bit-identical to those after the snapshot.
  * The parity taps (pairs, contacts, removed contacts) are empty until the next step. */
 int cc_snapshot(cc_ctx* ctx, void* out, int64_t capacity, int64_t* out_size);
+
+/* Differentiable stepping: a forward/backward pair over nsteps steps of dt, the adjoint of the
+ * velocity-level contact solve and the integrator (reverse mode, per step in reverse order):
+ * the friction-disc and normal-clamp projections with their active sets
+ * (src/constraints/contact-constraints.ts:963-1053, axis-constraint-part.ts:578-707), the warm
+ * start and the impulses it carries from step to step (:906-955, 651-686), the speculative
+ * normal bias (:288-423), the acceleration and velocity integration (src/update.ts:245-319,
+ * 896-976) and the combined friction (combine-material.ts:40-57).  Contact normals, lever arms,
+ * effective masses and inverse inertias are held constant within a step, and the position
+ * (Baumgarte) sweeps, the restitution bias and the velocity clamps are not differentiated.
+ * cc_fwd steps like cc_step and keeps a checkpoint of the state every checkpoint_every steps;
+ * cc_bwd recomputes each segment from its checkpoint with the solver recording, runs the
+ * adjoint backwards through it, and leaves the context in the state cc_fwd left it in.  Worlds
+ * must be solved unsplit (island_groups = 1, or at least 64 worlds), with sleeping off and no
+ * point constraints or iteration overrides (CC_E_UNSUPPORTED otherwise).  There is no such
+ * pair in the reference; it is the forward/backward of an autograd Function over updateWorld.
+ * cc_bwd takes dL/d(centre of mass) of every body after the last step (3 * num_bodies doubles;
+ * dl_dvel, the same for the final linear velocities, may be NULL) and writes dL/d(initial linear
+ * and angular velocities), dL/d(initial centres of mass) and dL/d(friction coefficient of each
+ * body) into the outputs that are not NULL.  It must follow cc_fwd with no cc_step, cc_set_bodies,
+ * cc_add_forces or cc_restore in between (CC_E_STATE). */
+int cc_fwd(cc_ctx* ctx, double dt, int32_t nsteps, int32_t checkpoint_every);
+int cc_bwd(cc_ctx* ctx, const double* dl_dcom, const double* dl_dvel, double* dl_dv0, double* dl_dw0,
+           double* dl_dcom0, double* dl_dfriction);
 int cc_restore(cc_ctx* ctx, const void* in, int64_t size);
 int cc_get_totals(cc_ctx* ctx, cc_totals* out);
 
