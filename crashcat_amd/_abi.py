@@ -230,4 +230,5 @@ class Config(C.Structure):
         ("block_solver", C.c_int32),
         ("block_bodies", C.c_int32),
         ("island_groups", C.c_int32),
+        ("disable_graphs", C.c_int32),
     ]

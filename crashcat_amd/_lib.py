@@ -68,14 +68,19 @@ def lib():
 BODY_STATE_DTYPE = np.dtype(_abi.BodyState)
 CONTACT_DTYPE = np.dtype(_abi.ContactOut)
 REMOVED_DTYPE = np.dtype(_abi.ContactRemoved)
-PHASES = ["broadphase", "narrowphase", "epa", "manifold", "islands", "solver", "cache"]
+# the step's phases, in the order cc_get_phase_times reports them
+PHASES = ["accel", "grid", "cell_pairs", "pair_layout", "filter", "gjk", "epa", "manifold", "manifold_large", "islands",
+          "solver", "cache"]
+# phases that time one kernel on its own (with the launches of tiny bookkeeping kernels)
+PHASE_KERNEL = {"accel": "k_accel", "cell_pairs": "k_bp_cells", "gjk": "k_np_gjk", "epa": "k_epa",
+                "manifold": "k_np_manifold", "solver": "k_world"}
 
 
 class Context:
     """One device context (a batch of worlds) behind the C ABI."""
 
     def __init__(self, precision="fp32", device=0, max_pairs_per_body=16, block_solver=0, max_contacts_per_body=8,
-                 island_groups=0):
+                 island_groups=0, block_bodies=0, graphs=True):
         L = lib()
         cfg = _abi.Config()
         cfg.precision = _abi.CC_PRECISION_FP64 if precision == "fp64" else _abi.CC_PRECISION_FP32
@@ -84,6 +89,8 @@ class Context:
         cfg.max_contacts_per_body = max_contacts_per_body
         cfg.block_solver = block_solver
         cfg.island_groups = island_groups
+        cfg.block_bodies = block_bodies
+        cfg.disable_graphs = 0 if graphs else 1
         h = C.c_void_p()
         rc = L.cc_create(C.byref(h), C.byref(cfg))
         if rc != 0:

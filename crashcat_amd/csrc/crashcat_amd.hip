@@ -8,6 +8,9 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "../../include/crashcat_amd.h"
 #include "cc_mesh_build.h"
@@ -201,6 +204,15 @@ struct CtxT : Impl {
     int worldLdsBS = 0;        // block size worldLdsBytes was sized for
     size_t tapeLdsBytes = 0;   // the same for the recording solver (block size 256)
     bool tapeLdsSet = false;
+    int bodyBS = 256;          // block size of the one-thread-per-body kernels
+    bool tuned = false;        // the autotuner has chosen the block sizes
+    int tuneTries = 0;         // steps the autotuner waited for contacts
+    // the step's phases as captured graphs, and the arguments they were captured with
+    std::vector<hipGraphExec_t> graphs;
+    bool graphsValid = false;
+    Dev<T> graphDev;
+    T graphDt = T(0);
+    int graphBS = 0, graphBB = 0;
     // differentiable stepping: checkpoints of the forward run, one step record per step of a
     // segment, the adjoint state
     struct Diff {
@@ -239,6 +251,8 @@ struct CtxT : Impl {
     } diff;
     void state_changed() override { diff.valid = false; }
     ~CtxT() {
+        for (auto& g : graphs)
+            if (g) (void)hipGraphExecDestroy(g);
         diff.mem.release();
         for (char* p : diff.ckpt) (void)hipFree(p);
         mem.release();
@@ -291,7 +305,7 @@ static hipError_t resolve_scratch(int device) {
 extern "C" int cc_create(cc_ctx** out, const cc_config* cfg) {
     if (!out) return CC_E_ARG;
     cc_ctx* c = new cc_ctx();
-    cc_config def = {CC_PRECISION_FP32, 0, 16, 8, 0, 0, 0};
+    cc_config def = {CC_PRECISION_FP32, 0, 16, 8, 0, 0, 0, 0};
     c->cfg = cfg ? *cfg : def;
     if (c->cfg.max_pairs_per_body <= 0) c->cfg.max_pairs_per_body = 16;
     if (c->cfg.max_contacts_per_body <= 0) c->cfg.max_contacts_per_body = 8;
@@ -871,6 +885,10 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int bs = c->cfg.block_solver;
+    const int bbq = c->cfg.block_bodies;
+    X->bodyBS = (bbq == 64 || bbq == 128 || bbq == 256 || bbq == 512 || bbq == 1024) ? bbq : 256;
+    X->tuned = false;
+    X->tuneTries = 0;
     if (bs == 64 || bs == 128 || bs == 256 || bs == 512 || bs == 1024) X->solverBS = bs;
     else {
         X->solverBS = maxWorld > 4096 ? 1024 : (maxWorld > 1024 ? 512 : 256);
@@ -944,162 +962,342 @@ static void scan_launch(hipStream_t st, const int* in, int* out, int n, int* wor
     hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256 > 0 ? (n + 255) / 256 : 1), dim3(256), 0, st, out, work, n);
 }
 
-// broadphase of one step: bin the bodies into cells, find the pairs, lay them out in body order
-template <typename T>
-static void launch_broadphase(CtxT<T>* X, hipStream_t st, double margin) {
-    Dev<T>& d = X->d;
-    const int N = d.N;
-    const int nbB = (N + 255) / 256;
-    hipLaunchKernelGGL(k_cells_clear<T>, dim3(grid_for(std::max(d.cellTabCap, N), 256, 4096)), dim3(256), 0, st, d);
-    hipLaunchKernelGGL(k_cells_insert<T>, dim3(nbB), dim3(256), 0, st, d);
-    scan_launch(st, d.cellCount, d.cellStart, d.cellTabCap, X->scanSums);
-    scan_launch(st, d.gLargeFlag, d.gLargeOff, N, X->scanSums);
-    hipLaunchKernelGGL(k_cells_scatter<T>, dim3(nbB), dim3(256), 0, st, d);
-    hipLaunchKernelGGL(k_bp_large<T>, dim3(nbB), dim3(256), 0, st, d, margin);
-    hipLaunchKernelGGL(k_bp_cells<T>, dim3(grid_for((N + 3) / 4, 1, 8192)), dim3(256), 0, st, d, margin);
-    scan_launch(st, d.bpCount, d.bpOff, N, X->scanSums);
-    hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
-    hipLaunchKernelGGL(k_bp_compact<T>, dim3(nbB), dim3(256), 0, st, d);
-    hipLaunchKernelGGL(k_bp_spill<T>, dim3(grid_for(d.spillCap, 256, 1024)), dim3(256), 0, st, d);
-    if (d.diff) hipLaunchKernelGGL(k_bp_sort<T>, dim3(nbB), dim3(256), 0, st, d);
+// ---------------------------------------------------------------- the step's phases
+// A step is a fixed sequence of launches with no host round trip, in phases that each end at a
+// HIP event (the phase timers) and are each replayed as one captured hipGraph: the big kernels
+// have a phase of their own, so the timers name the dominant kernel.
+enum {
+    PH_ACCEL = 0,  // k_accel
+    PH_GRID,       // cells: clear, insert, scans, scatter; pairs with large bodies
+    PH_BPCELLS,    // k_bp_cells
+    PH_PAIRS,      // pair layout: scan, compaction, spill
+    PH_FILTER,     // k_np_filter and the GJK list's counting sort
+    PH_GJK,        // k_np_gjk
+    PH_EPA,        // k_epa
+    PH_MANIFOLD,   // k_np_manifold<T, 8>
+    PH_MANIFOLD_L, // k_np_manifold<T, 32>, planes, triangle meshes
+    PH_ISLANDS,    // wake, constraint compaction, union-find
+    PH_SOLVER,     // k_world (levels, warm start, sweeps, integration, sleep)
+    PH_CACHE,      // contact-cache hash, end of step
+    PH_N
+};
+#define CC_PHASES PH_N
+
+// the world kernel's dynamic LDS: what the largest world can use (its per-body counts and packed
+// in-degrees), within what the device leaves beside the kernel's static LDS
+template <typename T, int BS, bool TAPE>
+static size_t world_lds_bytes(CtxT<T>* X) {
+    hipFuncAttributes fa;
+    int maxShared = 0;
+    size_t dyn = 0;
+    if (hipFuncGetAttributes(&fa, (const void*)k_world<T, BS, TAPE>) == hipSuccess &&
+        hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, X->device) == hipSuccess &&
+        (size_t)maxShared > fa.sharedSizeBytes) {
+        size_t avail = ((size_t)maxShared - fa.sharedSizeBytes) / 4;
+        dyn = std::min(avail, X->worldLdsNeed) * 4;
+    }
+    if (dyn > 0 && hipFuncSetAttribute((const void*)k_world<T, BS, TAPE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)dyn) != hipSuccess)
+        dyn = 0;
+    return dyn;
 }
 
-// dynamic LDS of the world kernel: what the largest world can use (its per-body counts and
-// packed in-degrees), within what the device leaves beside the kernel's static LDS
-template <typename T, int BS>
-static void launch_world_bs(CtxT<T>* X, hipStream_t st, T dt) {
-    Dev<T>& d = X->d;
-    if (X->worldLdsBS != BS) {
-        hipFuncAttributes fa;
-        int maxShared = 0;
-        size_t dyn = 0;
-        if (hipFuncGetAttributes(&fa, (const void*)k_world<T, BS>) == hipSuccess &&
-            hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, X->device) == hipSuccess &&
-            (size_t)maxShared > fa.sharedSizeBytes) {
-            size_t avail = ((size_t)maxShared - fa.sharedSizeBytes) / 4;
-            dyn = std::min(avail, X->worldLdsNeed) * 4;
+// set the solver's kernel attributes for the current block size (outside any graph capture)
+template <typename T>
+static void prepare_world(CtxT<T>* X, bool tape) {
+    if (tape) {
+        if (!X->tapeLdsSet) {
+            X->tapeLdsBytes = world_lds_bytes<T, 256, true>(X);
+            X->tapeLdsSet = true;
         }
-        if (dyn > 0 && hipFuncSetAttribute((const void*)k_world<T, BS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
-            dyn = 0;
-        X->worldLdsBytes = dyn;
-        X->worldLdsBS = BS;
+        X->d.worldLdsInts = (int)(X->tapeLdsBytes / 4);
+        return;
     }
-    d.worldLdsInts = (int)(X->worldLdsBytes / 4);
+    if (X->worldLdsBS != X->solverBS) {
+        switch (X->solverBS) {
+            case 64: X->worldLdsBytes = world_lds_bytes<T, 64, false>(X); break;
+            case 128: X->worldLdsBytes = world_lds_bytes<T, 128, false>(X); break;
+            case 512: X->worldLdsBytes = world_lds_bytes<T, 512, false>(X); break;
+            case 1024: X->worldLdsBytes = world_lds_bytes<T, 1024, false>(X); break;
+            default: X->worldLdsBytes = world_lds_bytes<T, 256, false>(X); break;
+        }
+        X->worldLdsBS = X->solverBS;
+    }
+    X->d.worldLdsInts = (int)(X->worldLdsBytes / 4);
+}
+
+template <typename T>
+static void launch_world(CtxT<T>* X, hipStream_t st, T dt, bool tape) {
+    Dev<T>& d = X->d;
     if (X->anySplit) {
         hipLaunchKernelGGL(k_group_zero<T>, dim3(grid_for(std::max(d.V, d.W), 256, 1024)), dim3(256), 0, st, d);
         hipLaunchKernelGGL(k_group_count<T>, dim3(grid_for((long long)d.conCap + d.NPC, 256, 4096)), dim3(256), 0, st, d);
     }
-    hipLaunchKernelGGL((k_world<T, BS>), dim3(d.V), dim3(BS), X->worldLdsBytes, st, d, dt);
-}
-
-// the recording solver of differentiable stepping (block size 256; worlds are never split there)
-template <typename T>
-static void launch_world_tape(CtxT<T>* X, hipStream_t st, T dt) {
-    Dev<T>& d = X->d;
-    if (!X->tapeLdsSet) {
-        hipFuncAttributes fa;
-        int maxShared = 0;
-        size_t dyn = 0;
-        if (hipFuncGetAttributes(&fa, (const void*)k_world<T, 256, true>) == hipSuccess &&
-            hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, X->device) == hipSuccess &&
-            (size_t)maxShared > fa.sharedSizeBytes) {
-            size_t avail = ((size_t)maxShared - fa.sharedSizeBytes) / 4;
-            dyn = std::min(avail, X->worldLdsNeed) * 4;
-        }
-        if (dyn > 0 &&
-            hipFuncSetAttribute((const void*)k_world<T, 256, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) !=
-                hipSuccess)
-            dyn = 0;
-        X->tapeLdsBytes = dyn;
-        X->tapeLdsSet = true;
-    }
-    d.worldLdsInts = (int)(X->tapeLdsBytes / 4);
-    hipLaunchKernelGGL((k_world<T, 256, true>), dim3(d.V), dim3(256), X->tapeLdsBytes, st, d, dt);
-    d.worldLdsInts = (int)(X->worldLdsBytes / 4);
-}
-
-template <typename T>
-static void launch_world(CtxT<T>* X, hipStream_t st, T dt, bool tape = false) {
-    if (tape) {
-        launch_world_tape<T>(X, st, dt);
+    if (tape) {  // the recording solver of differentiable stepping (worlds are never split there)
+        hipLaunchKernelGGL((k_world<T, 256, true>), dim3(d.V), dim3(256), X->tapeLdsBytes, st, d, dt);
         return;
     }
+    const size_t lds = X->worldLdsBytes;
     switch (X->solverBS) {
-        case 64: launch_world_bs<T, 64>(X, st, dt); break;
-        case 128: launch_world_bs<T, 128>(X, st, dt); break;
-        case 512: launch_world_bs<T, 512>(X, st, dt); break;
-        case 1024: launch_world_bs<T, 1024>(X, st, dt); break;
-        default: launch_world_bs<T, 256>(X, st, dt); break;
+        case 64: hipLaunchKernelGGL((k_world<T, 64>), dim3(d.V), dim3(64), lds, st, d, dt); break;
+        case 128: hipLaunchKernelGGL((k_world<T, 128>), dim3(d.V), dim3(128), lds, st, d, dt); break;
+        case 512: hipLaunchKernelGGL((k_world<T, 512>), dim3(d.V), dim3(512), lds, st, d, dt); break;
+        case 1024: hipLaunchKernelGGL((k_world<T, 1024>), dim3(d.V), dim3(1024), lds, st, d, dt); break;
+        default: hipLaunchKernelGGL((k_world<T, 256>), dim3(d.V), dim3(256), lds, st, d, dt); break;
     }
 }
 
-#define CC_PHASES 7
+// the launches of one phase; BB is the block size of the one-thread-per-body kernels
 template <typename T>
-static int step_impl(cc_ctx* c, double dtd, int nsteps, bool tape = false) {
-    auto* X = static_cast<CtxT<T>*>(c->impl);
+static void enqueue_phase(CtxT<T>* X, hipStream_t st, int p, T dt, double margin, bool tape) {
     Dev<T>& d = X->d;
-    hipStream_t st = c->stream;
-    T dt = T(dtd);
-    int N = d.N;
-    if (N == 0) {  // nothing to step; keep the step count for island reporting
-        c->steps += nsteps;
-        c->nPhase = 0;
-        return CC_OK;
+    const int N = d.N;
+    const int BB = X->bodyBS;
+    const int nbB = (N + BB - 1) / BB;
+    switch (p) {
+        case PH_ACCEL:
+            hipLaunchKernelGGL(k_accel<T>, dim3(nbB), dim3(BB), 0, st, d, dt);
+            break;
+        case PH_GRID:
+            hipLaunchKernelGGL(k_cells_clear<T>, dim3(grid_for(std::max(d.cellTabCap, N), 256, 4096)), dim3(256), 0, st, d);
+            hipLaunchKernelGGL(k_cells_insert<T>, dim3(nbB), dim3(BB), 0, st, d);
+            scan_launch(st, d.cellCount, d.cellStart, d.cellTabCap, X->scanSums);
+            scan_launch(st, d.gLargeFlag, d.gLargeOff, N, X->scanSums);
+            hipLaunchKernelGGL(k_cells_scatter<T>, dim3(nbB), dim3(BB), 0, st, d);
+            hipLaunchKernelGGL(k_bp_large<T>, dim3(nbB), dim3(BB), 0, st, d, margin);
+            break;
+        case PH_BPCELLS:
+            hipLaunchKernelGGL(k_bp_cells<T>, dim3(grid_for((N + 3) / 4, 1, 8192)), dim3(256), 0, st, d, margin);
+            break;
+        case PH_PAIRS:
+            scan_launch(st, d.bpCount, d.bpOff, N, X->scanSums);
+            hipLaunchKernelGGL(k_set_pair_count<T>, dim3(1), dim3(64), 0, st, d);
+            hipLaunchKernelGGL(k_bp_compact<T>, dim3(nbB), dim3(BB), 0, st, d);
+            hipLaunchKernelGGL(k_bp_spill<T>, dim3(grid_for(d.spillCap, 256, 1024)), dim3(256), 0, st, d);
+            if (d.diff) hipLaunchKernelGGL(k_bp_sort<T>, dim3(nbB), dim3(BB), 0, st, d);
+            break;
+        case PH_FILTER:
+            hipLaunchKernelGGL(k_np_filter<T>, dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
+            hipLaunchKernelGGL(k_gjk_count<T>, dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d);
+            hipLaunchKernelGGL(k_gjk_offsets<T>, dim3(1), dim3(64), 0, st, d);
+            hipLaunchKernelGGL(k_gjk_scatter<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
+            break;
+        case PH_GJK:
+            hipLaunchKernelGGL(k_np_gjk<T>, dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
+            break;
+        case PH_EPA:
+            hipLaunchKernelGGL(k_epa<T>, dim3((d.epaWorkers + 63) / 64), dim3(64), 0, st, d, dt);
+            hipLaunchKernelGGL(k_count_epa<T>, dim3(1), dim3(64), 0, st, d);
+            break;
+        case PH_MANIFOLD:
+            hipLaunchKernelGGL((k_np_manifold<T, CC_SMALL_FACE>), dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
+            break;
+        case PH_MANIFOLD_L:
+            hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
+            if (d.hasPlane)
+                hipLaunchKernelGGL(k_np_plane<T>, dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d, dt);
+            if (d.hasMesh) {
+                hipLaunchKernelGGL(k_mesh_count<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
+                scan_launch(st, d.meshCnt, d.meshOff, d.pairCap, X->scanSums);
+                hipLaunchKernelGGL(k_mesh_emit<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
+                hipLaunchKernelGGL(k_mesh_tri<T>, dim3(d.meshWorkers / MESH_LANES), dim3(MESH_LANES), 0, st, d);
+                hipLaunchKernelGGL(k_mesh_collect<T>, dim3(d.meshWorkers / MESH_LANES), dim3(MESH_LANES), 0, st, d, dt);
+            }
+            break;
+        case PH_ISLANDS:
+            hipLaunchKernelGGL(k_wake<T>, dim3(nbB), dim3(BB), 0, st, d);
+            hipLaunchKernelGGL(k_con_flags<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
+            scan_launch(st, d.conFlag, d.conPos, d.pairCap, X->scanSums);
+            hipLaunchKernelGGL(k_con_compact<T>, dim3(grid_for(std::max<long long>(d.pairCap, d.W + 1), 256, 4096)),
+                               dim3(256), 0, st, d);
+            hipLaunchKernelGGL(k_uf_init<T>, dim3(nbB), dim3(BB), 0, st, d);
+            hipLaunchKernelGGL(k_uf_link<T>, dim3(grid_for(d.conCap + d.NPC, 256, 2048)), dim3(256), 0, st, d);
+            hipLaunchKernelGGL(k_uf_compress<T>, dim3(nbB), dim3(BB), 0, st, d);
+            break;
+        case PH_SOLVER:
+            launch_world<T>(X, st, dt, tape);
+            break;
+        case PH_CACHE:
+            hipLaunchKernelGGL(k_hash_clear_cur<T>, dim3(grid_for(d.hashCap, 256, 2048)), dim3(256), 0, st, d);
+            hipLaunchKernelGGL(k_hash_insert<T>, dim3(grid_for(d.conCap, 256, 2048)), dim3(256), 0, st, d);
+            hipLaunchKernelGGL(k_end_step<T>, dim3(1), dim3(64), 0, st, d, dt);
+            break;
     }
-    int nbB = (N + 255) / 256;
-    const double margin = (double)d.s.spec;
-    size_t needEv = (size_t)(CC_PHASES + 1) * nsteps;
+}
+
+// broadphase of one step on its own (cc_broadphase_aabbs): cells, pairs, layout
+template <typename T>
+static void launch_broadphase(CtxT<T>* X, hipStream_t st, double margin) {
+    for (int p = PH_GRID; p <= PH_PAIRS; p++) enqueue_phase<T>(X, st, p, T(0), margin, false);
+}
+
+// The phases as graphs, captured again whenever a kernel argument changes (dt, a device
+// pointer or count, a block size).  Returns false when capture is not available (the phases
+// are then launched directly).
+template <typename T>
+static bool phase_graphs(CtxT<T>* X, hipStream_t st, T dt, double margin) {
+    const bool same = X->graphsValid && X->graphDt == dt && X->graphBS == X->solverBS && X->graphBB == X->bodyBS &&
+                      memcmp(&X->graphDev, &X->d, sizeof(Dev<T>)) == 0;
+    if (same) return true;
+    for (auto& g : X->graphs)
+        if (g) (void)hipGraphExecDestroy(g);
+    X->graphs.assign(PH_N, nullptr);
+    X->graphsValid = false;
+    for (int p = 0; p < PH_N; p++) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+        enqueue_phase<T>(X, st, p, dt, margin, false);
+        if (hipStreamEndCapture(st, &g) != hipSuccess || !g) return false;
+        const hipError_t e = hipGraphInstantiate(&X->graphs[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return false;
+    }
+    X->graphDev = X->d;
+    X->graphDt = dt;
+    X->graphBS = X->solverBS;
+    X->graphBB = X->bodyBS;
+    X->graphsValid = true;
+    return true;
+}
+
+// enqueue nsteps steps (graphs when allowed), recording the phase events of each
+template <typename T>
+static int enqueue_steps(cc_ctx* c, CtxT<T>* X, T dt, int nsteps, bool tape, bool graphs) {
+    hipStream_t st = c->stream;
+    const double margin = (double)X->d.s.spec;
+    prepare_world<T>(X, tape);
+    const bool useGraphs = graphs && !tape && phase_graphs<T>(X, st, dt, margin);
+    size_t needEv = (size_t)(PH_N + 1) * nsteps;
     while (X->ev.size() < needEv) {
         hipEvent_t e;
         HIPCHK(c, hipEventCreate(&e));
         X->ev.push_back(e);
     }
     for (int s = 0; s < nsteps; s++) {
-        hipEvent_t* E = X->ev.data() + (size_t)(CC_PHASES + 1) * s;
-        int e = 0;
-        auto mark = [&]() { (void)hipEventRecord(E[e++], st); };
-        mark();
-        hipLaunchKernelGGL(k_accel<T>, dim3(nbB), dim3(256), 0, st, d, dt);
-        launch_broadphase<T>(X, st, margin);
-        mark();  // broadphase
-        hipLaunchKernelGGL(k_np_filter<T>, dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
-        hipLaunchKernelGGL(k_gjk_count<T>, dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d);
-        hipLaunchKernelGGL(k_gjk_offsets<T>, dim3(1), dim3(64), 0, st, d);
-        hipLaunchKernelGGL(k_gjk_scatter<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
-        hipLaunchKernelGGL(k_np_gjk<T>, dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
-        mark();  // narrowphase (sphere pairs, GJK)
-        hipLaunchKernelGGL(k_epa<T>, dim3((d.epaWorkers + 63) / 64), dim3(64), 0, st, d, dt);
-        hipLaunchKernelGGL(k_count_epa<T>, dim3(1), dim3(64), 0, st, d);
-        mark();  // EPA
-        hipLaunchKernelGGL((k_np_manifold<T, CC_SMALL_FACE>), dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
-        hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
-        if (d.hasPlane)
-            hipLaunchKernelGGL(k_np_plane<T>, dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d, dt);
-        if (d.hasMesh) {
-            hipLaunchKernelGGL(k_mesh_count<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
-            scan_launch(st, d.meshCnt, d.meshOff, d.pairCap, X->scanSums);
-            hipLaunchKernelGGL(k_mesh_emit<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
-            hipLaunchKernelGGL(k_mesh_tri<T>, dim3(d.meshWorkers / MESH_LANES), dim3(MESH_LANES), 0, st, d);
-            hipLaunchKernelGGL(k_mesh_collect<T>, dim3(d.meshWorkers / MESH_LANES), dim3(MESH_LANES), 0, st, d, dt);
+        hipEvent_t* E = X->ev.data() + (size_t)(PH_N + 1) * s;
+        HIPCHK(c, hipEventRecord(E[0], st));
+        for (int p = 0; p < PH_N; p++) {
+            if (useGraphs) HIPCHK(c, hipGraphLaunch(X->graphs[p], st));
+            else enqueue_phase<T>(X, st, p, dt, margin, tape);
+            HIPCHK(c, hipEventRecord(E[p + 1], st));
         }
-        mark();  // manifolds (and convex-vs-mesh pairs), contact cache, constraint setup
-        hipLaunchKernelGGL(k_wake<T>, dim3(nbB), dim3(256), 0, st, d);
-        hipLaunchKernelGGL(k_con_flags<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
-        scan_launch(st, d.conFlag, d.conPos, d.pairCap, X->scanSums);
-        hipLaunchKernelGGL(k_con_compact<T>, dim3(grid_for(std::max<long long>(d.pairCap, d.W + 1), 256, 4096)), dim3(256), 0,
-                           st, d);
-        hipLaunchKernelGGL(k_uf_init<T>, dim3(nbB), dim3(256), 0, st, d);
-        hipLaunchKernelGGL(k_uf_link<T>, dim3(grid_for(d.conCap + d.NPC, 256, 2048)), dim3(256), 0, st, d);
-        hipLaunchKernelGGL(k_uf_compress<T>, dim3(nbB), dim3(256), 0, st, d);
-        mark();  // islands
-        launch_world<T>(X, st, dt, tape);
-        mark();  // solver + integration + sleep
-        hipLaunchKernelGGL(k_hash_clear_cur<T>, dim3(grid_for(d.hashCap, 256, 2048)), dim3(256), 0, st, d);
-        hipLaunchKernelGGL(k_hash_insert<T>, dim3(grid_for(d.conCap, 256, 2048)), dim3(256), 0, st, d);
-        hipLaunchKernelGGL(k_end_step<T>, dim3(1), dim3(64), 0, st, d, dt);
-        mark();  // cache
     }
     HIPCHK(c, hipGetLastError());
+    return CC_OK;
+}
+
+// The autotuner: the solver's block size and that of the one-thread-per-body kernels, chosen by
+// timing trial steps from a checkpoint of the state (restored after each: results do not depend
+// on block sizes), once per workload (cached by precision, bodies, worlds, largest world, device).
+// It runs at the first cc_step after upload whose previous step made constraints, so that the
+// trials see contacts.
+struct TuneKey {
+    int prec, n, w, maxWorld, device;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(prec, n, w, maxWorld, device) < std::tie(o.prec, o.n, o.w, o.maxWorld, o.device);
+    }
+};
+static std::map<TuneKey, std::pair<int, int>> g_tuned;
+static std::mutex g_tunedLock;
+
+template <typename T>
+static std::vector<std::pair<void*, size_t>> ckpt_segments(Dev<T>& d);
+template <typename T>
+static hipError_t ckpt_copy(CtxT<T>* X, hipStream_t st, char* buf, bool save);
+
+template <typename T>
+static int autotune(cc_ctx* c, CtxT<T>* X, T dt) {
+    Dev<T>& d = X->d;
+    const TuneKey key{c->cfg.precision, d.N, d.W, X->maxWorld, X->device};
+    {
+        std::lock_guard<std::mutex> lk(g_tunedLock);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            if (c->cfg.block_solver == 0) X->solverBS = it->second.first;
+            if (c->cfg.block_bodies == 0) X->bodyBS = it->second.second;
+            X->tuned = true;
+            return CC_OK;
+        }
+    }
+    hipStream_t st = c->stream;
+    int cnt[CNT_N];
+    HIPCHK(c, hipMemcpy(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (cnt[CNT_CONS] <= 0) {  // not yet: the trials would see no contacts
+        if (++X->tuneTries >= 8) X->tuned = true;  // a scene without contacts keeps the rule-based sizes
+        return CC_OK;
+    }
+    size_t bytes = 0;
+    for (auto& sg : ckpt_segments(d)) bytes += sg.second;
+    char* buf = nullptr;
+    if (hipMalloc(&buf, bytes) != hipSuccess) {  // no room for the trials: keep the rule-based sizes
+        (void)hipGetLastError();
+        X->tuned = true;
+        return CC_OK;
+    }
+    HIPCHK(c, ckpt_copy(X, st, buf, true));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    auto trial = [&](int bs, int bb) -> float {
+        X->solverBS = bs;
+        X->bodyBS = bb;
+        (void)ckpt_copy(X, st, buf, false);
+        prepare_world<T>(X, false);
+        (void)hipEventRecord(e0, st);
+        for (int p = 0; p < PH_N; p++) enqueue_phase<T>(X, st, p, dt, (double)d.s.spec, false);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
+    std::vector<int> sCand;
+    if (c->cfg.block_solver != 0) sCand = {X->solverBS};
+    else if (sizeof(T) == 8) sCand = X->maxWorld > 1024 ? std::vector<int>{256, 512} : std::vector<int>{128, 256, 512};
+    else sCand = X->maxWorld > 1024 ? std::vector<int>{256, 512, 1024} : std::vector<int>{128, 256, 512};
+    std::vector<int> bCand = c->cfg.block_bodies != 0 ? std::vector<int>{X->bodyBS} : std::vector<int>{128, 256, 512};
+    const int bb0 = c->cfg.block_bodies != 0 ? X->bodyBS : 256;
+    (void)trial(sCand[0], bb0);  // warm-up (first launches)
+    int bestS = sCand[0], bestB = bb0;
+    float best = 1e30f;
+    for (int bs : sCand) {
+        const float t = trial(bs, bb0);
+        if (t < best) { best = t; bestS = bs; }
+    }
+    for (int bb : bCand) {
+        if (bb == bb0) continue;
+        const float t = trial(bestS, bb);
+        if (t < best) { best = t; bestB = bb; }
+    }
+    X->solverBS = bestS;
+    X->bodyBS = bestB;
+    HIPCHK(c, ckpt_copy(X, st, buf, false));
+    HIPCHK(c, hipStreamSynchronize(st));
+    (void)hipFree(buf);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    X->tuned = true;
+    std::lock_guard<std::mutex> lk(g_tunedLock);
+    g_tuned[key] = {bestS, bestB};
+    return CC_OK;
+}
+
+template <typename T>
+static int step_impl(cc_ctx* c, double dtd, int nsteps, bool tape = false) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    const T dt = T(dtd);
+    if (d.N == 0) {  // nothing to step; keep the step count for island reporting
+        c->steps += nsteps;
+        c->nPhase = 0;
+        return CC_OK;
+    }
+    if (!tape && !d.diff && !X->tuned && nsteps > 0 && (c->cfg.block_solver == 0 || c->cfg.block_bodies == 0)) {
+        const int rc = autotune<T>(c, X, dt);
+        if (rc != CC_OK) return rc;
+    }
+    const int rc = enqueue_steps<T>(c, X, dt, nsteps, tape, c->cfg.disable_graphs == 0);
+    if (rc != CC_OK) return rc;
     c->steps += nsteps;
     c->nPhase = nsteps;  // number of steps recorded
     return CC_OK;

@@ -287,9 +287,13 @@ typedef struct cc_config {
     int32_t device;             /* HIP device ordinal                               */
     int32_t max_pairs_per_body; /* capacity factor for broadphase pairs (default 24) */
     int32_t max_contacts_per_body; /* capacity factor for contact constraints (default 12) */
-    int32_t block_solver;       /* threads per solver workgroup (0 = autotuned)     */
-    int32_t block_bodies;       /* threads per body-parallel workgroup (0 = autotuned) */
+    /* block sizes (64, 128, 256, 512 or 1024; 0 = autotuned: at the first cc_step after an
+     * upload whose previous step made constraints, trial steps from a checkpoint of the state
+     * time the candidates and the fastest is kept, cached per workload for the process) */
+    int32_t block_solver;       /* threads per solver workgroup                     */
+    int32_t block_bodies;       /* threads per workgroup of the one-thread-per-body kernels */
     int32_t island_groups;      /* most island groups a world is solved as (0 = by size, 1 = never split) */
+    int32_t disable_graphs;     /* 0: each step phase is replayed as a captured hipGraph; 1: launched directly */
 } cc_config;
 
 typedef struct cc_ctx cc_ctx;
@@ -384,10 +388,12 @@ int cc_broadphase_aabbs(cc_ctx* ctx, const float* aabb_min, const float* aabb_ma
                         int32_t n, double margin, int32_t* out_pairs, int32_t capacity, int32_t* out_count);
 
 /* Per-phase device time summed over every step of the last cc_step call (milliseconds,
- * HIP events on the context stream).  Phases, in order: broadphase (with the acceleration
- * integration), narrowphase (sphere pairs and GJK), EPA, manifold (supporting faces,
- * clipping, contact cache, constraint setup), islands, solver (levels, warm start,
- * velocity and position sweeps, integration, sleep), contact-cache hash. */
+ * HIP events on the context stream).  Phases, in order (12): acceleration integration
+ * (k_accel), grid (cells and pairs with large bodies), cell pairs (k_bp_cells), pair layout,
+ * pair filter and GJK list sort, GJK (k_np_gjk), EPA (k_epa), manifolds with faces of at most
+ * 4 vertices (k_np_manifold, FP32: <float, 8>), the other manifolds with planes and triangle
+ * meshes, islands, solver (k_world: levels, warm start, velocity and position sweeps,
+ * integration, sleep), contact-cache hash. */
 int cc_get_phase_times(cc_ctx* ctx, double* out_ms, int32_t capacity, int32_t* out_count);
 
 /* World snapshot: the state one step hands to the next, as one binary blob in place of the

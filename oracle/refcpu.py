@@ -23,7 +23,7 @@ def build():
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "librefcpu.so")
+        path = os.environ.get("CC_ORACLE_LIB") or os.path.join(_HERE, "librefcpu.so")
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)

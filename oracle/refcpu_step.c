@@ -110,6 +110,12 @@ struct oc_world {
     cc_step_counters ctr;
 };
 
+/* qsort of n elements; a zero-length (possibly null) array is left alone, as the C library
+ * requires a valid pointer even for n = 0 */
+static void sort_some(void* base, size_t n, size_t size, int (*cmp)(const void*, const void*)) {
+    if (n > 1) qsort(base, n, size, cmp);
+}
+
 static void* xrealloc(void* p, size_t n) {
     void* r = realloc(p, n ? n : 1);
     if (!r) { fprintf(stderr, "refcpu: out of memory\n"); abort(); }
@@ -667,7 +673,7 @@ static void find_pairs(oc_world* w) {
         key[i] = w->b[i].aabbMin.x - (queries(&w->b[i]) ? m : 0.0);
     }
     g_sap_key = key;
-    qsort(ord, (size_t)n, sizeof(int), cmp_sap);
+    sort_some(ord, (size_t)n, sizeof(int), cmp_sap);
     for (int oi = 0; oi < n; oi++) {
         int i = ord[oi];
         OBody* a = &w->b[i];
@@ -688,7 +694,7 @@ static void find_pairs(oc_world* w) {
     }
     free(ord);
     free(key);
-    qsort(w->pairs, (size_t)w->npairs, 2 * sizeof(int), cmp_pair_qo);
+    sort_some(w->pairs, (size_t)w->npairs, 2 * sizeof(int), cmp_pair_qo);
 }
 
 /* ---- narrowphase of one pair with the reduction collector (src/update.ts:333-563, 714-850) ---- */
@@ -1182,8 +1188,8 @@ void oc_step(oc_world* w, double dt) {
         for (int il = 0; il < w->nisl; il++) {
             OIsland* I = &w->isl[il];
             if (I->nc == 0 && I->nu == 0) continue;
-            qsort(I->cons, (size_t)I->nu, sizeof(int), cmp_int); /* sortConstraintIds: priority 0, index */
-            qsort(I->contacts, (size_t)I->nc, sizeof(int), cmp_contact);
+            sort_some(I->cons, (size_t)I->nu, sizeof(int), cmp_int); /* sortConstraintIds: priority 0, index */
+            sort_some(I->contacts, (size_t)I->nc, sizeof(int), cmp_contact);
             for (int it = 0; it < I->velSteps; it++) {
                 for (int k = 0; k < I->nc; k++) solve_velocity_contact(w, &w->cc[I->contacts[k]]);
                 for (int k = 0; k < I->nu; k++) {
@@ -1364,7 +1370,7 @@ int32_t oc_get_pairs(const oc_world* w, int32_t* out, int32_t cap) {
         out[2 * i] = a < b ? a : b;
         out[2 * i + 1] = a < b ? b : a;
     }
-    qsort(out, (size_t)n, 2 * sizeof(int32_t), cmp_pair);
+    sort_some(out, (size_t)n, 2 * sizeof(int32_t), cmp_pair);
     return w->npairs;
 }
 
@@ -1395,7 +1401,7 @@ int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap) {
             o->lambda_t2[j] = k->pts[j].t2.lambda;
         }
     }
-    qsort(out, (size_t)n, sizeof(cc_contact_out), cmp_cout);
+    sort_some(out, (size_t)n, sizeof(cc_contact_out), cmp_cout);
     return w->ncc;
 }
 
@@ -1414,7 +1420,7 @@ int32_t oc_get_removed_contacts(const oc_world* w, cc_contact_removed* out, int3
     if (n > 0) {
         cc_contact_removed* t = (cc_contact_removed*)xrealloc(NULL, sizeof(cc_contact_removed) * (size_t)w->nrem);
         memcpy(t, w->rem, sizeof(cc_contact_removed) * (size_t)w->nrem);
-        qsort(t, (size_t)w->nrem, sizeof(cc_contact_removed), cmp_removed);
+        sort_some(t, (size_t)w->nrem, sizeof(cc_contact_removed), cmp_removed);
         memcpy(out, t, sizeof(cc_contact_removed) * (size_t)n);
         free(t);
     }
@@ -1445,6 +1451,6 @@ int32_t oc_broadphase_aabbs(const float* mn, const float* mx, const int32_t* mot
         }
     }
     free(b);
-    qsort(out, (size_t)(cnt < cap ? cnt : cap), 2 * sizeof(int32_t), cmp_pair);
+    sort_some(out, (size_t)(cnt < cap ? cnt : cap), 2 * sizeof(int32_t), cmp_pair);
     return cnt;
 }
