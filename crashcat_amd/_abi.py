@@ -16,6 +16,8 @@ CC_E_HIP = -2
 CC_E_CAPACITY = -3
 CC_E_STATE = -4
 CC_E_UNSUPPORTED = -5
+CC_ORDER_REFERENCE = 0
+CC_ORDER_COLORED = 1
 CC_MOTION_KINEMATIC = 1
 CC_MOTION_DYNAMIC = 2
 
@@ -217,7 +219,9 @@ class Totals(C.Structure):
         ("epa", C.c_int64),
         ("awake_body_steps", C.c_int64),
         ("solver_bytes", C.c_int64),
-        ("reserved", C.c_int64 * 9),
+        ("gjk", C.c_int64),
+        ("body_steps", C.c_int64),
+        ("reserved", C.c_int64 * 7),
     ]
 
 
@@ -231,4 +235,6 @@ class Config(C.Structure):
         ("block_bodies", C.c_int32),
         ("island_groups", C.c_int32),
         ("disable_graphs", C.c_int32),
+        ("solver_order", C.c_int32),
+        ("measure_residual", C.c_int32),
     ]

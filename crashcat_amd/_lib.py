@@ -14,7 +14,7 @@ _LIB = None
 SYMBOLS = [
     "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
-    "cc_get_phase_times", "cc_get_totals", "cc_build_triangle_mesh", "cc_download_removed_contacts",
+    "cc_get_phase_times", "cc_get_totals", "cc_get_solver_residual", "cc_build_triangle_mesh", "cc_download_removed_contacts",
     "cc_snapshot", "cc_restore", "cc_add_forces", "cc_download_removed_contacts_noalloc", "cc_fwd", "cc_bwd",
 ]
 
@@ -57,6 +57,7 @@ def lib():
     L.cc_get_phase_times.argtypes = [vp, C.POINTER(C.c_double), C.c_int32, ip]
     L.cc_get_totals.argtypes = [vp, C.POINTER(_abi.Totals)]
     L.cc_add_forces.argtypes = [vp, dp, dp, C.c_int32, C.c_int32]
+    L.cc_get_solver_residual.argtypes = [vp, C.POINTER(C.c_double)]
     L.cc_fwd.argtypes = [vp, C.c_double, C.c_int32, C.c_int32]
     L.cc_bwd.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     for s in SYMBOLS:
@@ -80,7 +81,7 @@ class Context:
     """One device context (a batch of worlds) behind the C ABI."""
 
     def __init__(self, precision="fp32", device=0, max_pairs_per_body=16, block_solver=0, max_contacts_per_body=8,
-                 island_groups=0, block_bodies=0, graphs=True):
+                 island_groups=0, block_bodies=0, graphs=True, solver_order="reference", measure_residual=False):
         L = lib()
         cfg = _abi.Config()
         cfg.precision = _abi.CC_PRECISION_FP64 if precision == "fp64" else _abi.CC_PRECISION_FP32
@@ -91,6 +92,10 @@ class Context:
         cfg.island_groups = island_groups
         cfg.block_bodies = block_bodies
         cfg.disable_graphs = 0 if graphs else 1
+        if solver_order not in ("reference", "colored"):
+            raise ValueError("solver_order is 'reference' or 'colored'")
+        cfg.solver_order = _abi.CC_ORDER_COLORED if solver_order == "colored" else _abi.CC_ORDER_REFERENCE
+        cfg.measure_residual = 1 if measure_residual else 0
         h = C.c_void_p()
         rc = L.cc_create(C.byref(h), C.byref(cfg))
         if rc != 0:
@@ -217,6 +222,13 @@ class Context:
         cnt = C.c_int32()
         self._check(lib().cc_get_phase_times(self.h, buf, 16, C.byref(cnt)))
         return {PHASES[i] if i < len(PHASES) else str(i): buf[i] for i in range(cnt.value)}
+
+    def solver_residual(self):
+        """The last step's velocity residual (cc_get_solver_residual): sum of |dlambda|, sum of
+        normal impulses, largest |dlambda|, contact points."""
+        out = (C.c_double * 4)()
+        self._check(lib().cc_get_solver_residual(self.h, out))
+        return {"sum": out[0], "lambda_n_sum": out[1], "max": out[2], "points": int(out[3])}
 
     def totals(self):
         t = _abi.Totals()

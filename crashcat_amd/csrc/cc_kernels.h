@@ -167,6 +167,9 @@ struct Dev {
     int* islIt;         // [3 * N] per island root when a body or point constraint has iteration
                         // overrides (else null): max velocity and position overrides, default flags
     int hasForces;      // a body was given a force or torque: integrate and clear them
+    int colored;        // solve order: 0 crashcat's (levels of the order's DAG), 1 graph-coloured
+    double* resid;      // [4] when set, the last step's velocity residual: sum |dlambda|, sum lambda_n,
+                        // max |dlambda| (bits of a non-negative double), points
     // differentiable stepping (cc_fwd / cc_bwd): constraints record their pair-order rank and the
     // warm-start source of each point; when tapeLam is set the solver also records its sweeps
     int diff;
@@ -182,7 +185,7 @@ struct Dev {
 // running sums (64-bit) over all steps since upload: counts and the algorithmic bytes of
 // the byte model (per-item compulsory HBM traffic of each phase, 4-byte words)
 enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
-       ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_N = 16 };
+       ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_GJK, ACC_N = 16 };
 
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
        CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_MESH = 10, CNT_PLANE = 11, CNT_N = 12 };
@@ -214,6 +217,7 @@ __device__ inline int wave_alloc(int* counter) {
 // clamp and the translation DOFs; omega += I_w^-1 tau dt (I_w^-1 of the current rotation, FP64)
 template <typename T>
 __global__ void k_accel(Dev<T> d, T dt) {
+    if (d.resid && blockIdx.x == 0 && threadIdx.x < 4) d.resid[threadIdx.x] = 0.0;  // this step's residual
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.N) return;
     const Bodies<T>& B = d.B;
@@ -1215,6 +1219,7 @@ __global__ void k_con_compact(Dev<T> d) {
         atomicAdd(&d.acc[ACC_PAIRS], (unsigned long long)np);
         atomicAdd(&d.acc[ACC_CONS], (unsigned long long)min(d.counters[CNT_CONS], d.conCap));
         atomicAdd(&d.acc[ACC_EPA], (unsigned long long)min(d.counters[CNT_EPA], d.epaCap));
+        atomicAdd(&d.acc[ACC_GJK], (unsigned long long)min(d.counters[CNT_GJK], d.pairCap));
         atomicAdd(&d.acc[ACC_STEPS], 1ull);
     }
 }

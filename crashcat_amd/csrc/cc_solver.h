@@ -66,13 +66,13 @@ __device__ inline int block_scan_array(int* arr, int n, int* sh) {
 
 // velocities of body i in its velocity record
 template <typename T>
-__device__ inline void vw_load(const Dev<T>& d, int i, V3<T>& v, V3<T>& w) {
+__device__ __forceinline__ void vw_load(const Dev<T>& d, int i, V3<T>& v, V3<T>& w) {
     const VBody<T>& r = d.vb[i];
     v = r.v;
     w = r.w;
 }
 template <typename T>
-__device__ inline void vw_store(const Dev<T>& d, int i, V3<T> v, V3<T> w) {
+__device__ __forceinline__ void vw_store(const Dev<T>& d, int i, V3<T> v, V3<T> w) {
     VBody<T>& r = d.vb[i];
     r.v = v;
     r.w = w;
@@ -85,21 +85,39 @@ __device__ inline void vw_store(const Dev<T>& d, int i, V3<T> v, V3<T> w) {
 // then bodies and points).  In the last sweep (rec set) the impulses go to this step's cache
 // record (storeAppliedImpulses :1061-1076) instead of the point records.
 // TAPE: also record each point's impulses of this sweep for the adjoint (tl: the sweep's records)
+// An element's inputs that do not change during a sweep: its points, the bodies' inverse inertia
+// and inverse mass.
+template <typename T>
+struct VIn {
+    VPoint<T> P[4];
+    M3<T> IA, IB;
+    T iA, iB;
+};
+template <typename T>
+__device__ __forceinline__ void vin_load(const Dev<T>& d, const VElem<T>& E, VIn<T>& in) {
+    const int np = (E.meta >> 4) & 7;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (j < np) in.P[j] = d.vp[E.pt + j];
+    in.IA = d.vb[E.a].I;
+    in.IB = d.vb[E.b].I;
+    in.iA = d.vb[E.a].invMass;
+    in.iB = d.vb[E.b].invMass;
+}
+
 template <typename T, bool TAPE = false>
-__device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, CacheRec<T>* rec,
-                                           LamRec<T>* tl = nullptr) {
+__device__ __forceinline__ void solve_velocity_core(const Dev<T>& d, const VElem<T>& E, VIn<T>& in, CacheRec<T>* rec,
+                                                    LamRec<T>* tl) {
     const int meta = E.meta;
     const int dyn = meta & 3;
     const int mov = (meta >> 2) & 3, np = (meta >> 4) & 7, dofA = (meta >> 8) & 63, dofB = (meta >> 14) & 63;
-    const VBody<T> IA = d.vb[E.a], IB = d.vb[E.b];
-    VPoint<T> P[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-        if (j < np) P[j] = d.vp[E.pt + j];
+    VPoint<T>* P = in.P;
     VPoint<T>* Lm = P;
-    V3<T> vA = IA.v, wA = IA.w, vB = IB.v, wB = IB.w;
+    V3<T> vA, wA, vB, wB;
+    vw_load(d, E.a, vA, wA);
+    vw_load(d, E.b, vB, wB);
     const V3<T> n = E.n, t1 = contact_t1(n), t2 = cross(n, t1);
-    const T iA = IA.invMass, iB = IB.invMass, fr = E.fr;
+    const T iA = in.iA, iB = in.iB, fr = E.fr;
     const int npS = dyn ? np : 0;  // a constraint without a dynamic body is not solved
     T pre1[4], pre2[4], clp[4];
 #pragma unroll
@@ -122,10 +140,10 @@ __device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, C
             }
             T d1 = l1 - l.lT1;
             l.lT1 = l1;
-            if (d1 != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, IA.I, IB.I, p.r1, p.r2, vA, wA, vB, wB, t1, d1);
+            if (d1 != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, in.IA, in.IB, p.r1, p.r2, vA, wA, vB, wB, t1, d1);
             T d2 = l2 - l.lT2;
             l.lT2 = l2;
-            if (d2 != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, IA.I, IB.I, p.r1, p.r2, vA, wA, vB, wB, t2, d2);
+            if (d2 != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, in.IA, in.IB, p.r1, p.r2, vA, wA, vB, wB, t2, d2);
         }
     }
 #pragma unroll
@@ -137,7 +155,7 @@ __device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, C
         tl = tmax<T>(T(0), tl);
         T dn = tl - l.lN;
         l.lN = tl;
-        if (dn != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, IA.I, IB.I, p.r1, p.r2, vA, wA, vB, wB, n, dn);
+        if (dn != T(0)) axis_impulse(dyn, dofA, dofB, iA, iB, in.IA, in.IB, p.r1, p.r2, vA, wA, vB, wB, n, dn);
     }
     if (TAPE) {
 #pragma unroll
@@ -172,6 +190,13 @@ __device__ inline void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, C
     }
     if (dyn & 1) vw_store(d, E.a, vA, wA);
     if (dyn & 2) vw_store(d, E.b, vB, wB);
+}
+template <typename T, bool TAPE = false>
+__device__ __forceinline__ void solve_velocity_elem(const Dev<T>& d, const VElem<T>& E, CacheRec<T>* rec,
+                                                    LamRec<T>* tl = nullptr) {
+    VIn<T> in;
+    vin_load(d, E, in);
+    solve_velocity_core<T, TAPE>(d, E, in, rec, tl);
 }
 
 // warm start of one constraint's points (warmStartVelocity contact-constraints.ts:906-955)
@@ -355,7 +380,7 @@ __device__ inline void point_props(const Dev<T>& d, PointCon<T>& p) {
 }
 
 template <typename T>
-__device__ inline void point_apply_vw(const Dev<T>& d, PointCon<T>& p, V3<T> lam) {
+__device__ __forceinline__ void point_apply_vw(const Dev<T>& d, PointCon<T>& p, V3<T> lam) {
     if (len2(lam) == T(0)) return;
     const Bodies<T>& B = d.B;
     V3<T> v, w;
@@ -369,7 +394,7 @@ __device__ inline void point_apply_vw(const Dev<T>& d, PointCon<T>& p, V3<T> lam
     }
 }
 template <typename T>
-__device__ inline void point_solve_velocity_vw(const Dev<T>& d, PointCon<T>& p) {
+__device__ __forceinline__ void point_solve_velocity_vw(const Dev<T>& d, PointCon<T>& p) {
     if (!p.active) return;
     V3<T> va, wa, vb, wb;
     vw_load(d, p.a, va, wa);
@@ -480,6 +505,29 @@ struct SolveLessAt {
         lo = BK[2 * x + 1];
     }
 };
+// The graph-coloured solve order (the optional solver mode): each body's list is ordered by a
+// hash of the elements' solve keys (splitmix64's finaliser; unique keys, the same in every run,
+// unlike element indices) instead of by the keys.  The levels of that random orientation are
+// the rounds of a Jones-Plassmann colouring of the conflict graph (elements conflict when they
+// share a dynamic body): an element's level is one more than the highest level among its
+// neighbours of higher priority, so elements of a level share no dynamic body, and the longest
+// chain is short (about 15-20 levels where crashcat's order needs 50-120).
+__device__ inline unsigned long long jp_prio(unsigned long long h, unsigned long long l) {
+    unsigned long long x = h * 0x9E3779B97F4A7C15ull ^ l;
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return x;
+}
+struct ColorLessAt {
+    const unsigned long long* BK;
+    __device__ void key_at(int x, unsigned long long& hi, unsigned long long& lo) const {
+        hi = jp_prio(BK[2 * x], BK[2 * x + 1]);
+        lo = BK[2 * x];
+    }
+};
 
 __device__ inline bool key_lt(unsigned long long h0, unsigned long long l0, unsigned long long h1, unsigned long long l1) {
     return h0 < h1 || (h0 == h1 && l0 < l1);
@@ -490,6 +538,7 @@ __device__ inline bool key_lt(unsigned long long h0, unsigned long long l0, unsi
 template <typename Less>
 __device__ inline void list_key(const Less& less, int x, int k, unsigned long long& h, unsigned long long& l) { less.key(k, h, l); }
 __device__ inline void list_key(const SolveLessAt& less, int x, int k, unsigned long long& h, unsigned long long& l) { less.key_at(x, h, l); }
+__device__ inline void list_key(const ColorLessAt& less, int x, int k, unsigned long long& h, unsigned long long& l) { less.key_at(x, h, l); }
 
 // The same order as one 64-bit key inside the list of one body.  Creation keys are 64-bit
 // already.  A solve key (hi, lo = idA << 32 | idB with idA < idB) of a contact of this body
@@ -499,6 +548,11 @@ template <typename Less>
 __device__ inline unsigned long long list_key64(const Less& less, int x, int k, int body) {
     unsigned long long h, l;
     less.key(k, h, l);
+    return h;
+}
+__device__ inline unsigned long long list_key64(const ColorLessAt& less, int x, int k, int body) {
+    unsigned long long h, l;
+    less.key_at(x, h, l);
     return h;
 }
 __device__ inline unsigned long long list_key64(const SolveLessAt& less, int x, int k, int body) {
@@ -642,6 +696,59 @@ __device__ inline int build_levels(const Dev<T>& d, int so, int L, int* LS, int&
     if (tid == 0) LS[D] = lo;
     __syncthreads();
     return D;
+}
+
+// The velocity residual of the solved constraints: for every contact point, the change a further
+// sweep would make to its normal impulse and, after the friction-disc projection, to its friction
+// impulse, from the final velocities and this step's impulses (in the cache records).
+template <typename T, int BS>
+__device__ inline void velocity_residual(const Dev<T>& d, int so, int L, const CacheRec<T>* rec) {
+    double sD = 0.0, sL = 0.0, mx = 0.0, np = 0.0;
+    for (int x = threadIdx.x; x < L; x += BS) {
+        const VElem<T> E = d.ve[so + x];
+        if ((E.meta >> 28) != VK_CONTACT || (E.iters & 0xffff) == 0) continue;
+        const int meta = E.meta, dyn = meta & 3, mov = (meta >> 2) & 3, n_ = (meta >> 4) & 7;
+        if (!dyn) continue;
+        V3<T> vA, wA, vB, wB;
+        vw_load(d, E.a, vA, wA);
+        vw_load(d, E.b, vB, wB);
+        const V3<T> n = E.n, t1 = contact_t1(n), t2 = cross(n, t1);
+        const CacheRec<T>& r = rec[E.slot];
+        for (int j = 0; j < n_; j++) {
+            const VPoint<T> p = d.vp[E.pt + j];
+            const T ln = r.ln[j], lt1 = r.lt1[j], lt2 = r.lt2[j];
+            const T tn = tmax<T>(T(0), ln + p.effN * (axis_jv(dyn, mov, p.r1, p.r2, vA, wA, vB, wB, n) - p.biasN));
+            T l1 = lt1, l2 = lt2;
+            if (p.effT1 != T(0) || p.effT2 != T(0)) {
+                l1 = lt1 + p.effT1 * axis_jv(dyn, mov, p.r1, p.r2, vA, wA, vB, wB, t1);
+                l2 = lt2 + p.effT2 * axis_jv(dyn, mov, p.r1, p.r2, vA, wA, vB, wB, t2);
+                const T mf = E.fr * ln, fm2 = l1 * l1 + l2 * l2;
+                if (fm2 > mf * mf) {
+                    const T sc = mf / sqrt(fm2);
+                    l1 *= sc;
+                    l2 *= sc;
+                }
+            }
+            const double dl = fabs((double)tn - (double)ln) + sqrt(((double)l1 - lt1) * ((double)l1 - lt1) +
+                                                                   ((double)l2 - lt2) * ((double)l2 - lt2));
+            sD += dl;
+            sL += (double)ln;
+            mx = fmax(mx, dl);
+            np += 1.0;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sD += __shfl_xor(sD, o, 64);
+        sL += __shfl_xor(sL, o, 64);
+        mx = fmax(mx, __shfl_xor(mx, o, 64));
+        np += __shfl_xor(np, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && np > 0.0) {
+        atomicAdd(&d.resid[0], sD);
+        atomicAdd(&d.resid[1], sL);
+        atomicMax(reinterpret_cast<unsigned long long*>(&d.resid[2]), (unsigned long long)__double_as_longlong(mx));
+        atomicAdd(&d.resid[3], np);
+    }
 }
 
 constexpr int LS_LDS = 4096;  // level starts kept in LDS (16 KB)
@@ -855,12 +962,17 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __syncthreads();
     constexpr bool kCreationOrderWarmStart = sizeof(T) == 8;
     int D = 0;
-    if (kCreationOrderWarmStart && d.s.warmStarting) {
-        link_body_lists<T, BS>(d, b0, bb, nb, so, BL, BK, BC, CreationLess<T>{&d, so});
-        D = build_levels<T, BS>(d, so, L, LS, sTail, DEG);
-        warm_start_levels<T, BS, false>(d, so, D, LS, ratio);
+    const bool colored = d.colored && !TAPE;
+    if (colored) {
+        link_body_lists<T, BS>(d, b0, bb, nb, so, BL, BK, BC, ColorLessAt{BK});
+    } else {
+        if (kCreationOrderWarmStart && d.s.warmStarting) {
+            link_body_lists<T, BS>(d, b0, bb, nb, so, BL, BK, BC, CreationLess<T>{&d, so});
+            D = build_levels<T, BS>(d, so, L, LS, sTail, DEG);
+            warm_start_levels<T, BS, false>(d, so, D, LS, ratio);
+        }
+        link_body_lists<T, BS>(d, b0, bb, nb, so, BL, BK, BC, SolveLessAt{BK});
     }
-    link_body_lists<T, BS>(d, b0, bb, nb, so, BL, BK, BC, SolveLessAt{BK});
     D = build_levels<T, BS>(d, so, L, LS, sTail, DEG);
     if (tid == 0) {
         if (G > 1) atomicMax(&d.stats[2 * w], D);
@@ -947,7 +1059,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         }
     }
     const int* LSp = lsLds ? sLS : LS;
-    if (!kCreationOrderWarmStart && d.s.warmStarting) warm_start_levels<T, BS, true>(d, so, D, LSp, ratio);
+    if ((!kCreationOrderWarmStart || colored) && d.s.warmStarting) warm_start_levels<T, BS, true>(d, so, D, LSp, ratio);
     if (TAPE) {  // the warm-started impulses, sweep -1 of the record
         __syncthreads();
         for (int x = tid; x < L; x += BS) {
@@ -983,6 +1095,7 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
             __syncthreads();
         }
     }
+    if (d.resid && !TAPE) velocity_residual<T, BS>(d, so, L, d.rec[par]);
     for (int i = b0 + tid; i < b1; i += BS) {
         if (!mine(i)) continue;
         V3<T> v, w;

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <array>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -305,7 +306,7 @@ static hipError_t resolve_scratch(int device) {
 extern "C" int cc_create(cc_ctx** out, const cc_config* cfg) {
     if (!out) return CC_E_ARG;
     cc_ctx* c = new cc_ctx();
-    cc_config def = {CC_PRECISION_FP32, 0, 16, 8, 0, 0, 0, 0};
+    cc_config def = {CC_PRECISION_FP32, 0, 16, 8, 0, 0, 0, 0, 0, 0};
     c->cfg = cfg ? *cfg : def;
     if (c->cfg.max_pairs_per_body <= 0) c->cfg.max_pairs_per_body = 16;
     if (c->cfg.max_contacts_per_body <= 0) c->cfg.max_contacts_per_body = 8;
@@ -858,6 +859,8 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     d.prevDt = A.get<T>(1);
     d.parity = A.get<int>(1);
     d.acc = A.get<unsigned long long>(ACC_N);
+    d.colored = c->cfg.solver_order == CC_ORDER_COLORED ? 1 : 0;
+    d.resid = c->cfg.measure_residual ? A.get<double>(4) : nullptr;
     {  // scan workspace for the longest scan (pair flags, grid buckets or bodies)
         long long n = std::max<long long>({(long long)d.pairCap + 1, (long long)d.cellTabCap + 1, (long long)nb + 1});
         X->scanSums = A.get<int>((size_t)(n / 4096 + n / (4096ll * 4096) + 64));
@@ -875,6 +878,7 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     HIPCHK(c, hipMemset(d.counters, 0, CNT_N * sizeof(int)));
     HIPCHK(c, hipMemset(d.prevDt, 0, sizeof(T)));
     HIPCHK(c, hipMemset(d.acc, 0, ACC_N * sizeof(unsigned long long)));
+    if (d.resid) HIPCHK(c, hipMemset(d.resid, 0, 4 * sizeof(double)));
     HIPCHK(c, hipMemset(d.parity, 0, sizeof(int)));
     for (int k = 0; k < 2; k++) HIPCHK(c, hipMemset(d.hkeys[k], 0xff, d.hashCap * sizeof(unsigned long long)));
     HIPCHK(c, hipMemset(d.recCnt, 0, 2 * sizeof(int)));
@@ -986,7 +990,7 @@ enum {
 // the world kernel's dynamic LDS: what the largest world can use (its per-body counts and packed
 // in-degrees), within what the device leaves beside the kernel's static LDS
 template <typename T, int BS, bool TAPE>
-static size_t world_lds_bytes(CtxT<T>* X) {
+static size_t world_lds_bytes(CtxT<T>* X, size_t needInts) {
     hipFuncAttributes fa;
     int maxShared = 0;
     size_t dyn = 0;
@@ -994,20 +998,21 @@ static size_t world_lds_bytes(CtxT<T>* X) {
         hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, X->device) == hipSuccess &&
         (size_t)maxShared > fa.sharedSizeBytes) {
         size_t avail = ((size_t)maxShared - fa.sharedSizeBytes) / 4;
-        dyn = std::min(avail, X->worldLdsNeed) * 4;
+        dyn = std::min(avail, needInts) * 4;
     }
-    if (dyn > 0 && hipFuncSetAttribute((const void*)k_world<T, BS, TAPE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)dyn) != hipSuccess)
+    if (dyn > 0 && hipFuncSetAttribute((const void*)k_world<T, BS, TAPE>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
         dyn = 0;
     return dyn;
 }
+
 
 // set the solver's kernel attributes for the current block size (outside any graph capture)
 template <typename T>
 static void prepare_world(CtxT<T>* X, bool tape) {
     if (tape) {
         if (!X->tapeLdsSet) {
-            X->tapeLdsBytes = world_lds_bytes<T, 256, true>(X);
+            X->tapeLdsBytes = world_lds_bytes<T, 256, true>(X, X->worldLdsNeed);
             X->tapeLdsSet = true;
         }
         X->d.worldLdsInts = (int)(X->tapeLdsBytes / 4);
@@ -1015,11 +1020,11 @@ static void prepare_world(CtxT<T>* X, bool tape) {
     }
     if (X->worldLdsBS != X->solverBS) {
         switch (X->solverBS) {
-            case 64: X->worldLdsBytes = world_lds_bytes<T, 64, false>(X); break;
-            case 128: X->worldLdsBytes = world_lds_bytes<T, 128, false>(X); break;
-            case 512: X->worldLdsBytes = world_lds_bytes<T, 512, false>(X); break;
-            case 1024: X->worldLdsBytes = world_lds_bytes<T, 1024, false>(X); break;
-            default: X->worldLdsBytes = world_lds_bytes<T, 256, false>(X); break;
+            case 64: X->worldLdsBytes = world_lds_bytes<T, 64, false>(X, X->worldLdsNeed); break;
+            case 128: X->worldLdsBytes = world_lds_bytes<T, 128, false>(X, X->worldLdsNeed); break;
+            case 512: X->worldLdsBytes = world_lds_bytes<T, 512, false>(X, X->worldLdsNeed); break;
+            case 1024: X->worldLdsBytes = world_lds_bytes<T, 1024, false>(X, X->worldLdsNeed); break;
+            default: X->worldLdsBytes = world_lds_bytes<T, 256, false>(X, X->worldLdsNeed); break;
         }
         X->worldLdsBS = X->solverBS;
     }
@@ -1192,12 +1197,13 @@ static int enqueue_steps(cc_ctx* c, CtxT<T>* X, T dt, int nsteps, bool tape, boo
 // It runs at the first cc_step after upload whose previous step made constraints, so that the
 // trials see contacts.
 struct TuneKey {
-    int prec, n, w, maxWorld, device;
+    int prec, n, w, maxWorld, device, fixedSolver, fixedBodies;
     bool operator<(const TuneKey& o) const {
-        return std::tie(prec, n, w, maxWorld, device) < std::tie(o.prec, o.n, o.w, o.maxWorld, o.device);
+        return std::tie(prec, n, w, maxWorld, device, fixedSolver, fixedBodies) <
+               std::tie(o.prec, o.n, o.w, o.maxWorld, o.device, o.fixedSolver, o.fixedBodies);
     }
 };
-static std::map<TuneKey, std::pair<int, int>> g_tuned;
+static std::map<TuneKey, std::array<int, 2>> g_tuned;  // solver block, body block
 static std::mutex g_tunedLock;
 
 template <typename T>
@@ -1208,13 +1214,13 @@ static hipError_t ckpt_copy(CtxT<T>* X, hipStream_t st, char* buf, bool save);
 template <typename T>
 static int autotune(cc_ctx* c, CtxT<T>* X, T dt) {
     Dev<T>& d = X->d;
-    const TuneKey key{c->cfg.precision, d.N, d.W, X->maxWorld, X->device};
+    const TuneKey key{c->cfg.precision, d.N, d.W, X->maxWorld, X->device, c->cfg.block_solver, c->cfg.block_bodies};
     {
         std::lock_guard<std::mutex> lk(g_tunedLock);
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) {
-            if (c->cfg.block_solver == 0) X->solverBS = it->second.first;
-            if (c->cfg.block_bodies == 0) X->bodyBS = it->second.second;
+            if (c->cfg.block_solver == 0) X->solverBS = it->second[0];
+            if (c->cfg.block_bodies == 0) X->bodyBS = it->second[1];
             X->tuned = true;
             return CC_OK;
         }
@@ -1238,17 +1244,21 @@ static int autotune(cc_ctx* c, CtxT<T>* X, T dt) {
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
+    // one timed step from the checkpoint per candidate, after an untimed one (a kernel's first
+    // launch loads its code)
     auto trial = [&](int bs, int bb) -> float {
         X->solverBS = bs;
         X->bodyBS = bb;
-        (void)ckpt_copy(X, st, buf, false);
         prepare_world<T>(X, false);
-        (void)hipEventRecord(e0, st);
-        for (int p = 0; p < PH_N; p++) enqueue_phase<T>(X, st, p, dt, (double)d.s.spec, false);
-        (void)hipEventRecord(e1, st);
-        (void)hipEventSynchronize(e1);
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
+        for (int r = 0; r < 2; r++) {
+            (void)ckpt_copy(X, st, buf, false);
+            (void)hipEventRecord(e0, st);
+            for (int p = 0; p < PH_N; p++) enqueue_phase<T>(X, st, p, dt, (double)d.s.spec, false);
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+        }
         return ms;
     };
     std::vector<int> sCand;
@@ -1257,7 +1267,6 @@ static int autotune(cc_ctx* c, CtxT<T>* X, T dt) {
     else sCand = X->maxWorld > 1024 ? std::vector<int>{256, 512, 1024} : std::vector<int>{128, 256, 512};
     std::vector<int> bCand = c->cfg.block_bodies != 0 ? std::vector<int>{X->bodyBS} : std::vector<int>{128, 256, 512};
     const int bb0 = c->cfg.block_bodies != 0 ? X->bodyBS : 256;
-    (void)trial(sCand[0], bb0);  // warm-up (first launches)
     int bestS = sCand[0], bestB = bb0;
     float best = 1e30f;
     for (int bs : sCand) {
@@ -1271,6 +1280,7 @@ static int autotune(cc_ctx* c, CtxT<T>* X, T dt) {
     }
     X->solverBS = bestS;
     X->bodyBS = bestB;
+    prepare_world<T>(X, false);
     HIPCHK(c, ckpt_copy(X, st, buf, false));
     HIPCHK(c, hipStreamSynchronize(st));
     (void)hipFree(buf);
@@ -1886,7 +1896,25 @@ static int totals_impl(cc_ctx* c, cc_totals* o) {
     o->epa = (int64_t)a[ACC_EPA];
     o->awake_body_steps = (int64_t)a[ACC_AWAKE];
     o->solver_bytes = (int64_t)a[ACC_BYTES_SOLVE];
+    o->gjk = (int64_t)a[ACC_GJK];
+    o->body_steps = (int64_t)a[ACC_STEPS] * X->d.N;
     return CC_OK;
+}
+
+template <typename T>
+static int residual_impl(cc_ctx* c, double* out) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    if (!X->d.resid) return fail(c, CC_E_STATE, "the context does not measure residuals (cc_config.measure_residual)");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, X->d.resid, 4 * sizeof(double), hipMemcpyDeviceToHost));
+    return CC_OK;
+}
+
+extern "C" int cc_get_solver_residual(cc_ctx* c, double out[4]) {
+    if (!c || !c->impl || !out) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return residual_impl<double>(c, out);
+    return residual_impl<float>(c, out);
 }
 
 extern "C" int cc_get_totals(cc_ctx* c, cc_totals* o) {
@@ -2005,6 +2033,7 @@ static int fwd_impl(cc_ctx* c, double dt, int n, int K) {
         return fail(c, CC_E_UNSUPPORTED, "differentiable stepping solves each world on one workgroup: use "
                                          "island_groups = 1 or a batch of at least 64 worlds");
     if (d.NPC > 0) return fail(c, CC_E_UNSUPPORTED, "differentiable stepping has no point constraints");
+    if (d.colored) return fail(c, CC_E_UNSUPPORTED, "differentiable stepping solves in crashcat's order (CC_ORDER_REFERENCE)");
     if (d.islIt) return fail(c, CC_E_UNSUPPORTED, "differentiable stepping has no iteration overrides");
     if (d.s.allowSleeping)
         return fail(c, CC_E_UNSUPPORTED, "differentiable stepping needs sleeping off (allow_sleeping = 0): a body "

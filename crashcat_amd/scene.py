@@ -189,7 +189,12 @@ class Scene:
         return b.ctypes.data_as(C.POINTER(_abi.BodyDesc)), b
 
     def c_shapes(self):
-        """(shape array, HullData, keepalive) in the C layout."""
+        """(shape array, HullData, keepalive) in the C layout; built once per shape list (shapes
+        are only ever appended)."""
+        key = (id(self.shapes), len(self.shapes))
+        cached = getattr(self, "_cshapes", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
         n = len(self.shapes)
         arr = (_abi.ShapeDesc * max(n, 1))()
         pts, shr, ffv, fnv, fvi, pn, pc = [], [], [], [], [], [], []
@@ -268,6 +273,7 @@ class Scene:
         m.total_nodes = nmn
         keep["meshes"] = m
         keep["arr"] = arr
+        self._cshapes = (key, (arr, h, keep))
         return arr, h, keep
 
     def c_point_constraints(self):

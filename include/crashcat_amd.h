@@ -279,7 +279,9 @@ typedef struct cc_totals {
     int64_t epa;                /* EPA invocations                                   */
     int64_t awake_body_steps;   /* sum over steps of awake dynamic bodies            */
     int64_t solver_bytes;       /* byte-model bytes of the velocity + position sweeps */
-    int64_t reserved[9];
+    int64_t gjk;                /* pairs handed to GJK (past the bounding-sphere filter) */
+    int64_t body_steps;         /* sum over steps of all bodies, static ones included */
+    int64_t reserved[7];
 } cc_totals;
 
 typedef struct cc_config {
@@ -294,7 +296,16 @@ typedef struct cc_config {
     int32_t block_bodies;       /* threads per workgroup of the one-thread-per-body kernels */
     int32_t island_groups;      /* most island groups a world is solved as (0 = by size, 1 = never split) */
     int32_t disable_graphs;     /* 0: each step phase is replayed as a captured hipGraph; 1: launched directly */
+    /* CC_ORDER_REFERENCE: each island's constraints are solved in crashcat's order (bit-identical
+     * to the FP64 oracle on the FP64 path); CC_ORDER_COLORED: in the colours of a graph colouring
+     * (no two constraints of a colour share a dynamic body), far fewer sequential stages on large
+     * islands, a different Gauss-Seidel order judged by its residual (cc_get_solver_residual) */
+    int32_t solver_order;
+    int32_t measure_residual;   /* 1: every step measures its velocity residual (one extra pass) */
 } cc_config;
+
+#define CC_ORDER_REFERENCE 0
+#define CC_ORDER_COLORED 1
 
 typedef struct cc_ctx cc_ctx;
 
@@ -408,6 +419,13 @@ int cc_get_phase_times(cc_ctx* ctx, double* out_ms, int32_t capacity, int32_t* o
  * The parity taps (pairs, contacts, removed contacts) are empty until the next step. */
 int cc_snapshot(cc_ctx* ctx, void* out, int64_t capacity, int64_t* out_size);
 
+/* The velocity residual of the last step (needs cc_config.measure_residual): out[0] the sum over
+ * every solved contact point of the impulse change a further velocity sweep would make (normal,
+ * plus the friction pair after its disc projection), out[1] the sum of the normal impulses,
+ * out[2] the largest single change, out[3] the number of points.  No counterpart in the
+ * reference, which has no residual; it judges CC_ORDER_COLORED against CC_ORDER_REFERENCE. */
+int cc_get_solver_residual(cc_ctx* ctx, double out[4]);
+
 /* Differentiable stepping: a forward/backward pair over nsteps steps of dt, the adjoint of the
  * velocity-level contact solve and the integrator (reverse mode, per step in reverse order):
  * the friction-disc and normal-clamp projections with their active sets
@@ -420,8 +438,9 @@ int cc_snapshot(cc_ctx* ctx, void* out, int64_t capacity, int64_t* out_size);
  * cc_fwd steps like cc_step and keeps a checkpoint of the state every checkpoint_every steps;
  * cc_bwd recomputes each segment from its checkpoint with the solver recording, runs the
  * adjoint backwards through it, and leaves the context in the state cc_fwd left it in.  Worlds
- * must be solved unsplit (island_groups = 1, or at least 64 worlds), with sleeping off and no
- * point constraints or iteration overrides (CC_E_UNSUPPORTED otherwise).  There is no such
+ * must be solved unsplit (island_groups = 1, or at least 64 worlds) in crashcat's order
+ * (CC_ORDER_REFERENCE), with sleeping off and no point constraints or iteration overrides
+ * (CC_E_UNSUPPORTED otherwise).  There is no such
  * pair in the reference; it is the forward/backward of an autograd Function over updateWorld.
  * cc_bwd takes dL/d(centre of mass) of every body after the last step (3 * num_bodies doubles;
  * dl_dvel, the same for the final linear velocities, may be NULL) and writes dL/d(initial linear

@@ -6,8 +6,9 @@
  * calls it.
  *
  * It follows the reference files op for op (citations beside each function in
- * refcpu_collide.c / refcpu_step.c) in double precision, on one thread, with
- * crashcat's own orderings: active list in wake/creation order, contact
+ * refcpu_collide.c / refcpu_step.c) in double precision, one world on one
+ * thread (its scratch is thread-local, so separate worlds may step on separate
+ * threads at once), with crashcat's own orderings: active list in wake/creation order, contact
  * constraints in pair order, warm start in creation order, islands numbered by
  * lowest active index and sorted by size, contacts key-sorted per island.
  * Deliberate differences, none of which changes a pair set or an island

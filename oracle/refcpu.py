@@ -90,12 +90,14 @@ CONTACT_DTYPE = np.dtype(_abi.ContactOut)
 class OracleWorld:
     """One FP64 oracle world built from a Scene (one world of it)."""
 
-    def __init__(self, scene, world=0):
+    def __init__(self, scene, world=0, bodies=None):
+        """bodies: the world's body descriptors to use instead of the scene's (same count)."""
         L = lib()
         offs = scene.offsets()
         b0, b1 = offs[world], offs[world + 1]
         self.b0 = b0
-        bodies = scene.bodies()[b0:b1].copy()
+        bodies = scene.bodies()[b0:b1].copy() if bodies is None else np.array(bodies, copy=True)
+        assert len(bodies) == b1 - b0
         self._bodies = bodies
         shapes, hulls, self._keep = scene.c_shapes()
         pcs = [(a - b0, b - b0, la, lb, ov, op) for (a, b, la, lb, ov, op) in scene.point_constraints if b0 <= a < b1]

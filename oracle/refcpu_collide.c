@@ -725,8 +725,8 @@ static int epa_add(EPts* P, const OSup* A, const OSup* B, V3 d) { /* addEpaSuppo
 
 /* penetrationDepthStepEPA penetration.ts:214-614 */
 static int pd_step_epa(PenDepth* out, const OSup* A, const OSup* B, double tol, const Simplex* s) {
-    static EHull H; /* single-threaded oracle */
-    static EPts P;
+    static __thread EHull H; /* per thread: worlds may step on several threads */
+    static __thread EPts P;
     ocx_epa_calls++;
     P.n = 0;
     for (int i = 0; i < s->size; i++) { P.y[i] = s->pt[i].y; P.p[i] = s->pt[i].p; P.q[i] = s->pt[i].q; }
@@ -1068,7 +1068,7 @@ static V3 fix_normal(V3 v0, V3 v1, V3 v2, V3 tn, int act, V3 p, V3 n, V3 md) {
     return (act & edge) ? n : tn;
 }
 
-static int g_mesh_calls;
+static __thread int g_mesh_calls;
 int ocx_mesh_calls(void) { return g_mesh_calls; }
 
 /* collideConvexVsTriangleMesh src/shapes/triangle-mesh.ts:1026-1379 at unit scale: the convex
@@ -1113,8 +1113,8 @@ void ocx_collide_mesh(const OShapes* sh, int sa, V3 pa, Q4 qa, int sm, V3 pm, Q4
     const uint32_t mask = nbits >= 32 ? 0xffffffffu : ((1u << nbits) - 1u);
     V3 qc = v3((q2mn.x + q2mx.x) * 0.5, (q2mn.y + q2mx.y) * 0.5, (q2mn.z + q2mx.z) * 0.5);
     V3 mdl = qrotate(iqa, moveDir);
-    static int* stack = NULL;
-    static int scap = 0;
+    static __thread int* stack = NULL;
+    static __thread int scap = 0;
     int ns = 0;
     if (!scap) { scap = 256; stack = (int*)malloc(sizeof(int) * (size_t)scap); }
     stack[ns++] = 0;
@@ -1308,8 +1308,8 @@ void ocx_collide_sphere_mesh(const OShapes* sh, int sa, V3 pa, int sm, V3 pm, Q4
     const int nbits = ntri > 1 ? 32 - __builtin_clz((unsigned)(ntri - 1)) : 0;
     const uint32_t mask = nbits >= 32 ? 0xffffffffu : ((1u << nbits) - 1u);
     V3 mdm = qrotate(iqm, moveDir);
-    static int* stack = NULL;
-    static int scap = 0;
+    static __thread int* stack = NULL;
+    static __thread int scap = 0;
     int ns = 0;
     if (!scap) { scap = 256; stack = (int*)malloc(sizeof(int) * (size_t)scap); }
     stack[ns++] = 0;
@@ -1528,7 +1528,7 @@ static void clip_poly_vs_plane(OFace* out, const OFace* poly, V3 origin, V3 norm
 }
 
 static void clip_poly_vs_poly(OFace* out, const OFace* toClip, const OFace* clip, V3 normal) { /* :108-175 */
-    static OFace tmp[2];
+    static __thread OFace tmp[2];
     out->n = 0;
     if (toClip->n < 2 || clip->n < 3) return;
     int tmpIdx = 0;

@@ -656,7 +656,7 @@ static void push_pair(oc_world* w, int a, int b) {
 }
 /* Pair enumeration: sort-and-sweep on x over the expanded AABBs, then pairs ordered by
  * (querying body, other body) — the same list a brute-force double loop produces. */
-static double* g_sap_key;
+static __thread double* g_sap_key;
 static int cmp_sap(const void* x, const void* y) {
     int a = *(const int*)x, b = *(const int*)y;
     if (g_sap_key[a] != g_sap_key[b]) return g_sap_key[a] < g_sap_key[b] ? -1 : 1;
@@ -707,8 +707,8 @@ typedef struct {
     OManifold m[OC_MAX_MANIFOLDS];
     uint32_t subA[OC_MAX_MANIFOLDS], subB[OC_MAX_MANIFOLDS];
 } OCollector;
-static OCollector g_col;
-static OManifold g_tmp;
+static __thread OCollector g_col;
+static __thread OManifold g_tmp;
 
 /* addHit :385-520: merge by normal, else a new manifold (the shallowest one is replaced when
  * all 32 are in use), then append the hit's clipped points */
@@ -917,7 +917,7 @@ static void build_islands(oc_world* w) {
     }
 }
 
-static oc_world* g_sort_world;
+static __thread oc_world* g_sort_world;
 static int cmp_contact(const void* x, const void* y) { /* sortContactIndices :1240-1264 + stability */
     const OCC* a = &g_sort_world->cc[*(const int*)x];
     const OCC* b = &g_sort_world->cc[*(const int*)y];

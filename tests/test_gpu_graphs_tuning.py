@@ -8,9 +8,11 @@ import numpy as np
 import pytest
 
 from crashcat_amd import scene as SC
+from crashcat_amd import shapes as SH
 from crashcat_amd._lib import PHASES, Context
 from oracle import refcpu as O
 from tests.test_gpu_parity import _assert_bodies_equal, _assert_contacts_equal
+from tests.test_gpu_step_inputs import _pile
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +21,7 @@ DTS = [DT] * 6 + [DT / 2] * 3 + [DT] * 6
 
 
 @pytest.mark.parametrize("block_solver,block_bodies,graphs", [(128, 128, True), (512, 512, True), (256, 256, False),
-                                                              (0, 0, True)])
+                                                              (0, 0, True), (64, 512, False)])
 def test_fp64_bitwise_at_every_block_size_with_and_without_graphs(block_solver, block_bodies, graphs):
     sc = SC.cube_heap(seed=1001, nx=6, ny=6, nz=6)
     orc = O.OracleWorld(sc)
@@ -72,3 +74,32 @@ def test_single_large_world_fp32_at_1024_threads():
     for a, b in zip(ref, got):
         assert np.array_equal(a["position"], b["position"])
         assert np.array_equal(a["linear_velocity"], b["linear_velocity"])
+
+
+@pytest.mark.parametrize("groups,block_solver", [(0, 0), (1, 0), (1, 128), (0, 512)])
+def test_fp64_chains_and_kinematics_at_tuned_and_fixed_block_sizes(groups, block_solver):
+    """A pile with a capsule chain on point constraints and a moving kinematic body, solved as
+    island groups and as one workgroup, at autotuned and fixed solver block sizes, FP64 bit for
+    bit against the oracle."""
+    sc = _pile(21, n=120, kinematic=1)
+    cap = SH.capsule(0.1, 0.3)
+    prev = None
+    for k in range(8):
+        i = sc.add_body(cap, (2.0, 6.0 - 0.8 * k, 1.0), quaternion=(0.0, 0.0, 0.0, 1.0))
+        if prev is not None:
+            sc.add_point_constraint(prev, i, (0, -0.4, 0), (0, 0.4, 0))
+        prev = i
+    assert sc.num_worlds == 1
+    orc = O.OracleWorld(sc)
+    g = Context("fp64", island_groups=groups, block_solver=block_solver)
+    g.upload(sc)
+    try:
+        for s_ in range(60):
+            orc.step(DT)
+            g.step(DT, 1)
+            if s_ % 10 == 9:
+                g.sync()
+                _assert_bodies_equal(orc.bodies(), g.bodies(), f"step {s_}")
+                _assert_contacts_equal(orc.contacts(), g.contacts(), f"step {s_}")
+    finally:
+        g.close()

@@ -338,3 +338,23 @@ def test_point_constraint_chain_on_a_pile():
     w = O.OracleWorld(sc)
     w.step(DT, 60)
     assert _chain_gaps(sc, w.bodies()).max() < 0.1
+
+
+def test_worlds_step_on_threads_as_alone():
+    """Separate oracle worlds stepped on separate threads at once (the CPU baseline's form)
+    give exactly what each gives stepped alone: the oracle's scratch is per thread."""
+    from concurrent.futures import ThreadPoolExecutor
+    base = SC.mixed_pile(seed=1002, n=400, extent=(8.0, 6.0, 8.0), cylinders=True)
+    batch = SC.replicate_world(base, 6, 1002, extent=(8.0, 6.0, 8.0))
+
+    def run(w):
+        o = O.OracleWorld(batch, w)
+        o.step(DT, 40)
+        return o.bodies()
+
+    alone = [run(w) for w in range(6)]
+    with ThreadPoolExecutor(6) as ex:
+        threaded = list(ex.map(run, range(6)))
+    for a, b in zip(alone, threaded):
+        for f in ("position", "quaternion", "linear_velocity", "angular_velocity"):
+            assert np.array_equal(a[f], b[f])
