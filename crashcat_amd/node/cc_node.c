@@ -174,22 +174,24 @@ static napi_value layout(napi_env env, napi_callback_info info) {
 }
 #undef F
 
-/* ---- create(precision, maxPairsPerBody, maxContactsPerBody, device) -> handle ------------- */
+/* ---- create(precision, maxPairsPerBody, maxContactsPerBody, device, solverOrder) -> handle - */
 static napi_value create(napi_env env, napi_callback_info info) {
-    size_t argc = 4;
-    napi_value a[4];
+    size_t argc = 5;
+    napi_value a[5];
     CHECK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
-    int32_t prec = CC_PRECISION_FP32, pairs = 24, cons = 8, dev = 0;
+    int32_t prec = CC_PRECISION_FP32, pairs = 24, cons = 8, dev = 0, order = CC_ORDER_REFERENCE;
     if (argc > 0) napi_get_value_int32(env, a[0], &prec);
     if (argc > 1) napi_get_value_int32(env, a[1], &pairs);
     if (argc > 2) napi_get_value_int32(env, a[2], &cons);
     if (argc > 3) napi_get_value_int32(env, a[3], &dev);
+    if (argc > 4) napi_get_value_int32(env, a[4], &order);
     cc_config cfg;
     memset(&cfg, 0, sizeof cfg);
     cfg.precision = prec;
     cfg.device = dev;
     cfg.max_pairs_per_body = pairs;
     cfg.max_contacts_per_body = cons;
+    cfg.solver_order = order;
     cc_ctx* c = NULL;
     int rc = cc_create(&c, &cfg);
     if (rc != CC_OK) {

@@ -110,12 +110,15 @@ class GpuWorld {
   // mesh (optional): { positions, triangles, nodes } as a TriangleMeshShape holds them after
   // buildTriangleMesh: data.positions, the triangle buffer as 4 ints per triangle (a, b, c,
   // active-edge bits) and bvh.buffer, concatenated over the scene's meshes
-  // opts: { precision: 'fp32' | 'fp64', maxPairsPerBody, maxContactsPerBody, device }
+  // opts: { precision: 'fp32' | 'fp64', maxPairsPerBody, maxContactsPerBody, device,
+  //         solverOrder: 'reference' (crashcat's order) | 'colored' (graph-coloured) }
   constructor(desc, opts) {
     const o = opts || {};
     const prec = PRECISION[o.precision || 'fp32'];
     if (prec === undefined) throw new Error('crashcat_amd: precision must be fp32 or fp64');
-    this.handle = addon.create(prec, o.maxPairsPerBody || 24, o.maxContactsPerBody || 8, o.device || 0);
+    const order = { reference: 0, colored: 1 }[o.solverOrder || 'reference'];
+    if (order === undefined) throw new Error('crashcat_amd: solverOrder must be reference or colored');
+    this.handle = addon.create(prec, o.maxPairsPerBody || 24, o.maxContactsPerBody || 8, o.device || 0, order);
     const hull = desc.hull || {};
     const hullBuffers = {
       points: new Float64Array(hull.points || []),
@@ -240,16 +243,31 @@ class WorldRun {
 
 function createWorld(desc, opts) { return new GpuWorld(desc, opts); }
 
+// the listener's callbacks, refusing those that would have to run inside the GPU step
+function checkListener(listener) {
+  const names = callbackNames(listener);
+  const refused = names.filter((k) => STEP_CALLBACKS.includes(k) || !EVENT_CALLBACKS.includes(k));
+  if (refused.length) throw new Error(`crashcat_amd: listener callbacks not available on the GPU step: ${refused.join(', ')}`);
+  return names;
+}
+
+// bind a crashcat World to the GPU (crashcat_world.js); updateWorld binds on first use
+function fromCrashcatWorld(world, opts) { return require('./crashcat_world.js').fromCrashcatWorld(world, opts); }
+
 // updateWorld(world, listener, timeStep) (src/update.ts:45): one step of every world of the
 // context.  The contact callbacks of the listener are called after the step from its event
 // records; a validate callback, which would have to run inside the GPU step, is refused.
 function updateWorld(world, listener, timeStep) {
-  if (!(world instanceof GpuWorld)) throw new Error('crashcat_amd: updateWorld needs a world made by createWorld');
-  const names = callbackNames(listener);
-  const refused = names.filter((k) => STEP_CALLBACKS.includes(k) || !EVENT_CALLBACKS.includes(k));
-  if (refused.length) throw new Error(`crashcat_amd: listener callbacks not available on the GPU step: ${refused.join(', ')}`);
+  if (!(world instanceof GpuWorld)) {
+    const cw = require('./crashcat_world.js');
+    if (!cw.isCrashcatWorld(world)) throw new Error('crashcat_amd: updateWorld needs a world made by createWorld or a crashcat World');
+    checkListener(listener);
+    cw.updateCrashcatWorld(world, listener, timeStep);
+    return;
+  }
+  const names = checkListener(listener);
   world.step(timeStep, 1);
   if (names.length) world.dispatchContactEvents(listener);
 }
 
-module.exports = { createWorld, updateWorld, GpuWorld, WorldRun, pack, unpack, layout: LAYOUT, addon };
+module.exports = { createWorld, updateWorld, fromCrashcatWorld, GpuWorld, WorldRun, pack, unpack, layout: LAYOUT, addon };

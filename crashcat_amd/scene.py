@@ -340,6 +340,124 @@ class Scene:
         }
 
 
+    def crashcat_world(self):
+        """The scene (one world) as crashcat's own objects, JSON-serialisable: a World's settings
+        (src/world-settings.ts:4-200), bodies.pool of RigidBody records (src/body/rigid-body.ts:
+        142-252) with motionProperties, shapes in crashcat's records (sphere.ts:41-55, box.ts:
+        43-52, capsule.ts:32-49, convex-hull.ts:38-86, cylinder.ts:30-40, plane.ts:59-74,
+        triangle-mesh.ts:105-112) and a point-constraint pool (point-constraint.ts:21-25).  A
+        body's `shape` is an index into the returned `shapes` list (JSON has no shared objects;
+        a loader puts the shape objects in place).  Bodies were created in index order, so body
+        k has sequence k and id k + k * 2^32 (src/body/body-id.ts)."""
+        assert self.num_worlds == 1, "one world"
+        st = self.settings
+        n = st.num_object_layers
+        pairs = [int(st.object_layer_pairs[a * 8 + b]) for a in range(n) for b in range(n)]
+        settings = {
+            "layers": {"broadphaseLayers": 1, "objectLayers": n, "objectLayerToBroadphaseLayer": [0] * n,
+                       "objectLayerPairs": pairs, "objectVsBroadphase": [1] * n, "broadphasePairs": [1]},
+            "gravity": list(st.gravity), "gravityEnabled": bool(st.gravity_enabled),
+            "contacts": {"contactPointPreserveLambdaMaxDistSq": st.contact_point_preserve_lambda_max_dist_sq},
+            "narrowphase": {"collisionTolerance": st.collision_tolerance, "penetrationTolerance": st.penetration_tolerance,
+                            "returnDeepestPoint": False,
+                            "collideOnlyWithActiveEdges": bool(st.collide_only_with_active_edges),
+                            "speculativeContactDistance": st.speculative_contact_distance,
+                            "manifoldTolerance": st.manifold_tolerance,
+                            "collideWithBackfaces": bool(st.collide_with_backfaces), "useManifoldReduction": True,
+                            "normalCosMaxDeltaRotation": st.normal_cos_max_delta_rotation},
+            "solver": {"minVelocityForRestitution": st.min_velocity_for_restitution,
+                       "velocityIterations": st.velocity_iterations, "positionIterations": st.position_iterations,
+                       "warmStarting": bool(st.warm_starting), "penetrationSlop": st.penetration_slop,
+                       "baumgarteFactor": st.baumgarte_factor, "warmStartImpulseRatio": 1.0,
+                       "maxPenetrationDistance": st.max_penetration_distance},
+            "sleeping": {"allowSleeping": bool(st.allow_sleeping), "timeBeforeSleep": st.time_before_sleep,
+                         "pointVelocitySleepThreshold": st.point_velocity_sleep_threshold},
+            "ccd": {"linearCastThreshold": 0.05, "linearCastMaxPenetration": 0.25, "minVelocityForRestitution": 1.0},
+        }
+
+        def aabb(sh):
+            return [list(sh.aabb_min), list(sh.aabb_max)]
+
+        shapes = []
+        for sh in self.shapes:
+            base = {"aabb": aabb(sh), "centerOfMass": list(sh.center_of_mass), "density": sh.density,
+                    "volume": sh.volume, "materialId": -1}
+            if sh.type == S.SPHERE:
+                base.update(type=0, radius=sh.radius)
+            elif sh.type == S.BOX:
+                base.update(type=1, halfExtents=list(sh.half_extents), convexRadius=sh.convex_radius)
+            elif sh.type == S.CAPSULE:
+                base.update(type=2, radius=sh.radius, halfHeightOfCylinder=sh.half_height)
+            elif sh.type == S.CYLINDER:
+                base.update(type=9, radius=sh.radius, halfHeight=sh.half_height, convexRadius=sh.convex_radius)
+            elif sh.type == S.PLANE:
+                base.update(type=11, plane={"normal": list(sh.half_extents), "constant": sh.radius},
+                            halfExtent=sh.half_height)
+            elif sh.type == S.CONVEX_HULL:
+                vi, faces = [], []
+                for f in sh.faces:
+                    faces.append({"firstVertex": len(vi), "numVertices": len(f)})
+                    vi.extend(int(v) for v in f)
+                pts = []
+                for p_, pf in zip(sh.points, sh.point_faces):
+                    fs = [int(x) for x in pf] + [0] * (3 - len(pf))
+                    pts.append({"position": [float(x) for x in p_], "numFaces": len(pf), "faces": fs})
+                base.update(type=3, points=pts, faces=faces, vertexIndices=vi, convexRadius=sh.convex_radius,
+                            planes=[{"normal": [float(x) for x in nrm], "constant": float(c)}
+                                    for nrm, c in zip(sh.plane_normals, sh.plane_constants)])
+            elif sh.type == S.TRIANGLE_MESH:
+                pos = np.asarray(sh.mesh_positions, dtype=np.float64)
+                tri = np.asarray(sh.mesh_triangles, dtype=np.int64)
+                buf = []
+                for a, b, c, act in tri:  # stride 8: a, b, c, normal, active edges, material
+                    nrm = np.cross(pos[b] - pos[a], pos[c] - pos[a])
+                    nrm = nrm / max(np.linalg.norm(nrm), 1e-300)
+                    buf.extend([int(a), int(b), int(c), float(nrm[0]), float(nrm[1]), float(nrm[2]), int(act), -1])
+                base.update(type=4, data={"positions": pos.reshape(-1).tolist(), "triangleBuffer": buf,
+                                          "triangleCount": len(tri)},
+                            bvh={"buffer": np.asarray(sh.mesh_nodes, dtype=np.float64).reshape(-1).tolist()})
+            else:
+                raise ValueError(f"shape type {sh.type}")
+            shapes.append(base)
+        b = self.bodies()
+        pool = []
+        for k, r in enumerate(b):
+            pool.append({
+                "_pooled": False, "id": k + k * 2 ** 32, "index": k, "sequence": k, "userData": None,
+                "shape": int(r["shape"]), "position": r["position"].tolist(), "quaternion": r["quaternion"].tolist(),
+                "centerOfMassPosition": [0.0, 0.0, 0.0], "motionType": int(r["motion_type"]),
+                "motionProperties": {
+                    "linearVelocity": r["linear_velocity"].tolist(), "angularVelocity": r["angular_velocity"].tolist(),
+                    "invInertiaDiagonal": r["inv_inertia_diagonal"].tolist(),
+                    "inertiaRotation": r["inertia_rotation"].tolist(),
+                    "force": r["force"].tolist(), "torque": r["torque"].tolist(), "invMass": float(r["inv_mass"]),
+                    "linearDamping": float(r["linear_damping"]), "angularDamping": float(r["angular_damping"]),
+                    "maxLinearVelocity": float(r["max_linear_velocity"]),
+                    "maxAngularVelocity": float(r["max_angular_velocity"]), "gravityFactor": float(r["gravity_factor"]),
+                    "motionQuality": int(r["motion_quality"]), "allowedDegreesOfFreedom": int(r["allowed_dofs"]),
+                    "numVelocityStepsOverride": int(r["num_velocity_steps_override"]),
+                    "numPositionStepsOverride": int(r["num_position_steps_override"]),
+                    "sleepTestSpheres": [], "allowSleeping": bool(r["allow_sleeping"]), "sleepTestTimer": 0},
+                "friction": float(r["friction"]), "restitution": float(r["restitution"]),
+                "frictionCombineMode": int(r["friction_combine"]), "restitutionCombineMode": int(r["restitution_combine"]),
+                "collisionGroups": int(r["collision_groups"]), "collisionMask": int(r["collision_mask"]),
+                "useManifoldReduction": True, "sensor": bool(r["sensor"]), "enhancedInternalEdgeRemoval": False,
+                "collideKinematicVsNonDynamic": False, "sleeping": bool(r["sleeping"]),
+                "aabb": [[0.0, 0.0, 0.0], [0.0, 0.0, 0.0]], "objectLayer": int(r["object_layer"]),
+            })
+        cons = []
+        for k, (a, bb, la, lb, ov, op) in enumerate(self.point_constraints):
+            cons.append({"_pooled": False, "_sleeping": False, "id": k, "index": k, "sequence": 0, "enabled": True,
+                         "constraintPriority": 0, "numVelocityStepsOverride": ov, "numPositionStepsOverride": op,
+                         "bodyIndexA": a, "bodyIndexB": bb, "localSpacePosition1": list(la),
+                         "localSpacePosition2": list(lb)})
+        return {"settings": settings, "shapes": shapes,
+                "bodies": {"pool": pool, "freeIndices": [], "nextSequence": len(pool)},
+                "constraints": {"pools": {"0": {"type": 0, "constraints": cons, "freeIndices": [],
+                                                "nextSequence": len(cons)}} if cons else {}},
+                "previousTimeStep": 0}
+
+
 # ------------------------------------------------------------------------ PRNG
 _M64 = (1 << 64) - 1
 

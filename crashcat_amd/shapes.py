@@ -55,6 +55,9 @@ class Shape:
         self.plane_normals = kw.get("plane_normals")
         self.plane_constants = kw.get("plane_constants")
         self.volume = kw.get("volume", 0.0)
+        # per point, the 1-3 faces whose planes shift it inward by the convex radius
+        # (ConvexHullPoint.faces / numFaces, src/shapes/convex-hull.ts:64-71)
+        self.point_faces = kw.get("point_faces")
         # triangle-mesh data (built: welded vertices, triangles in BVH leaf order, BVH nodes)
         self.mesh_positions = kw.get("mesh_positions")
         self.mesh_triangles = kw.get("mesh_triangles")
@@ -233,6 +236,41 @@ def _hull_faces(pts, tol):
     return faces
 
 
+def _dot(a, b):
+    return (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
+
+
+def _cross(a, b):
+    return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1], a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                     a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], axis=1)
+
+
+def shrunk_points(points, normals, point_faces, cr):
+    """The convex-radius-excluded support points (ConvexHullNoConvexSupport,
+    src/shapes/convex-hull.ts:732-786): a point with one supporting face moves in by cr along
+    its normal; with two or three, it is the intersection of its faces' planes moved in by cr (the
+    third plane, for two faces, through the point at right angles to both), or the one-face form
+    when the planes do not meet in a point.  Written component by component, in this order, so
+    that the JavaScript host (crashcat_amd/node/crashcat_world.js) computes the same bits."""
+    points = np.asarray(points, dtype=np.float64)
+    normals = np.asarray(normals, dtype=np.float64)
+    n_f = np.array([len(pf) for pf in point_faces])
+    c1 = normals[[pf[0] for pf in point_faces]]
+    c2 = normals[[pf[1] if len(pf) > 1 else pf[0] for pf in point_faces]]
+    perp = _cross(c1, c2)
+    pl = np.sqrt(_dot(perp, perp))[:, None]
+    perp = np.divide(perp, pl, out=np.zeros_like(perp), where=pl > 0)
+    c3 = np.where((n_f == 3)[:, None], normals[[pf[2] if len(pf) > 2 else pf[0] for pf in point_faces]], perp)
+    w1 = -_dot(c1, points) + cr
+    w2 = -_dot(c2, points) + cr
+    w3 = -_dot(c3, points) + np.where(n_f == 3, cr, 0.0)
+    c23, c31, c12 = _cross(c2, c3), _cross(c3, c1), _cross(c1, c2)
+    den = _dot(c1, c23)
+    inter = -((w1[:, None] * c23 + w2[:, None] * c31) + w3[:, None] * c12) / np.where(den == 0, 1.0, den)[:, None]
+    fallback = points - c1 * cr
+    return np.where(((n_f == 1) | (den == 0))[:, None], fallback, inter)
+
+
 def convex_hull(positions, convex_radius=DEFAULT_CONVEX_RADIUS, density=DEFAULT_SHAPE_DENSITY,
                 hull_tolerance=1e-3, max_error_convex_radius=0.05):
     """convexHull.create({positions, convexRadius}) — src/shapes/convex-hull.ts:106-447."""
@@ -342,21 +380,13 @@ def convex_hull(positions, convex_radius=DEFAULT_CONVEX_RADIUS, density=DEFAULT_
                 big = off > 1
                 if big.any():
                     cr = min(cr, float((max_error_convex_radius / (off[big] - 1)).min()))
-    # shrunk support points (convex-hull.ts:732-786): 3-plane intersection of the offset planes
-    w1 = -np.einsum("ij,ij->i", c1, points) + cr
-    w2 = -np.einsum("ij,ij->i", c2, points) + cr
-    w3 = -np.einsum("ij,ij->i", c3, points) + np.where(n_f == 3, cr, 0.0)
-    c23, c31, c12 = np.cross(c2, c3), np.cross(c3, c1), np.cross(c1, c2)
-    den = np.einsum("ij,ij->i", c1, c23)
-    inter = -(w1[:, None] * c23 + w2[:, None] * c31 + w3[:, None] * c12) / np.where(den == 0, 1.0, den)[:, None]
-    fallback = points - c1 * cr
-    shrunk = np.where(((n_f == 1) | (den == 0))[:, None], fallback, inter)
+    shrunk = shrunk_points(points, normals, pfaces, cr)
     mn = points.min(axis=0)
     mx = points.max(axis=0)
     return Shape(CONVEX_HULL, convex_radius=cr, density=density, center_of_mass=tuple(com),
                  aabb_min=tuple(mn), aabb_max=tuple(mx), mass=volume * density, inertia=inertia * density,
                  points=points, shrunk_points=shrunk, faces=faces, plane_normals=normals, plane_constants=consts,
-                 volume=volume)
+                 volume=volume, point_faces=pfaces)
 
 
 # ------------------------------------------------------------------ mass -> motion

@@ -200,3 +200,156 @@ def test_node_update_world_cube_heap_bitwise(tmp_path, which):
     op = np.asarray(orc.pairs(), dtype=np.int32).reshape(-1, 2)
     assert np.array_equal(gp, op)
     assert json.loads(r.stdout.strip().splitlines()[-1])["overflow"] == 0
+
+
+# ------------------------------------------------------------------ crashcat World adapter
+RUN_CRASHCAT = os.path.join(HERE, "node", "crashcat_world_run.js")
+
+
+def _crashcat_scenes():
+    from crashcat_amd import shapes as SH
+    chains = SC.Scene()
+    chains.add_body(SH.box((20, 0.5, 20)), (0, -0.5, 0), motion_type=_abi.CC_MOTION_STATIC, friction=0.5)
+    for k in range(6):
+        chains.add_body(SH.box((0.3, 0.3, 0.3)), (0.7 * k - 2.0, 0.4 + 0.1 * k, 0.2 * k), friction=0.5)
+    cap = SH.capsule(0.1, 0.3)
+    prev = None
+    for k in range(5):
+        i = chains.add_body(cap, (4.0, 5.0 - 0.8 * k, 0.0), quaternion=(0.0, 0.0, 0.0, 1.0))
+        if prev is not None:
+            chains.add_point_constraint(prev, i, (0, -0.4, 0), (0, 0.4, 0))
+        prev = i
+    return {"cube_heap": SC.cube_heap(seed=1001, nx=3, ny=3, nz=3),
+            "mixed": SC.mixed_pile(seed=1002, n=60, extent=(6.0, 6.0, 6.0), cylinders=True),
+            "chains": chains,
+            "terrain": SC.terrain_drop(vertices=16, n=24, drop=(0.5, 2.5))}
+
+
+@needs_node
+@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain"])
+def test_crashcat_world_upload_equals_the_scene_description(tmp_path, which):
+    """The adapter reads crashcat's own objects (World settings, RigidBody pool with
+    motionProperties, shape records, point-constraint pool) and produces exactly the upload the
+    scene's C-struct description holds, hull support points (computed in JavaScript from the
+    hull's points, planes and per-point faces) bit for bit included."""
+    sc = _crashcat_scenes()[which]
+    f = tmp_path / "world.json"
+    f.write_text(json.dumps(sc.crashcat_world()))
+    r = _node(f"""
+const fs = require('fs');
+const cw = require('./crashcat_world.js');
+const d = JSON.parse(fs.readFileSync({json.dumps(str(f))}, 'utf8'));
+for (const b of d.bodies.pool) b.shape = d.shapes[b.shape];
+const world = {{ settings: d.settings, bodies: d.bodies, constraints: d.constraints, previousTimeStep: 0 }};
+process.stdout.write(JSON.stringify(cw.describe(world)));
+""")
+    assert r.returncode == 0, r.stderr
+    got, ref = json.loads(r.stdout), sc.desc()
+    assert got["worldBodyOffsets"] == ref["worldBodyOffsets"]
+    for k in ref["hull"]:
+        assert got["hull"][k] == ref["hull"][k], k
+    for k in ("positions", "triangles", "nodes"):
+        assert got["mesh"][k] == ref["mesh"][k], k
+    assert len(got["shapes"]) == len(ref["shapes"])
+    for a, b in zip(got["shapes"], ref["shapes"]):
+        for k in b:
+            assert a[k] == b[k], k
+    assert len(got["bodies"]) == len(ref["bodies"])
+    for a, b in zip(got["bodies"], ref["bodies"]):
+        for k in b:
+            assert a[k] == b[k], k
+    for k in ref["settings"]:
+        assert got["settings"][k] == pytest.approx(ref["settings"][k], rel=0, abs=0), k
+    assert got["pointConstraints"] == ref["pointConstraints"]
+
+
+@needs_node
+@pytest.mark.parametrize("what,needle", [
+    ("sensor", "is a sensor"), ("ccd", "LINEAR_CAST"), ("compound", "COMPOUND"), ("hinge", "constraint type 2"),
+    ("layers", "object layers"), ("reduction", "manifold reduction"), ("ids", "not in index order"),
+    ("moving_mesh", "triangle meshes must be static")])
+def test_crashcat_world_unsupported_features_are_refused(tmp_path, what, needle):
+    """Features the GPU path does not have are refused with an Error before anything runs
+    (so the host never steps such a world differently without knowing)."""
+    sc = SC.terrain_drop(vertices=8, n=4, drop=(0.5, 2.5)) if what == "moving_mesh" else SC.cube_heap(seed=1001, nx=2, ny=2, nz=2)
+    d = sc.crashcat_world()
+    b = d["bodies"]["pool"]
+    if what == "sensor":
+        b[1]["sensor"] = True
+    elif what == "ccd":
+        b[1]["motionProperties"]["motionQuality"] = 1
+    elif what == "compound":
+        d["shapes"].append({"type": 5, "aabb": [[0, 0, 0], [1, 1, 1]], "centerOfMass": [0, 0, 0]})
+        b[1]["shape"] = len(d["shapes"]) - 1
+    elif what == "hinge":
+        d["constraints"]["pools"]["2"] = {"type": 2, "constraints": [{"_pooled": False}]}
+    elif what == "layers":
+        d["settings"]["layers"]["objectLayers"] = 9
+    elif what == "reduction":
+        b[2]["useManifoldReduction"] = False
+    elif what == "ids":
+        b[1]["id"], b[2]["id"] = b[2]["id"], b[1]["id"]
+    elif what == "moving_mesh":
+        mesh = next(i for i, s in enumerate(d["shapes"]) if s["type"] == 4)
+        k = next(i for i, x in enumerate(b) if x["shape"] == mesh)
+        b[k]["motionType"] = 2
+    f = tmp_path / "world.json"
+    f.write_text(json.dumps(d))
+    r = _node(f"""
+const fs = require('fs');
+const cc = require('./index.js');
+const d = JSON.parse(fs.readFileSync({json.dumps(str(f))}, 'utf8'));
+for (const b of d.bodies.pool) b.shape = d.shapes[b.shape];
+const world = {{ settings: d.settings, bodies: d.bodies, constraints: d.constraints, previousTimeStep: 0 }};
+try {{ cc.fromCrashcatWorld(world); process.stdout.write('accepted'); }} catch (e) {{ process.stdout.write(e.message); }}
+""")
+    assert r.returncode == 0, r.stderr
+    assert needle in r.stdout, r.stdout
+
+
+@needs_node
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain"])
+def test_crashcat_world_steps_bitwise_with_events_and_forces(tmp_path, which):
+    """updateWorld on a crashcat-shaped World: FP64, bit for bit against the oracle in the state
+    written back into the RigidBody objects; per step the listener's events (with the body
+    objects) are the oracle's; a force and torque put on a body's motionProperties before step 10
+    are integrated and cleared as crashcat does."""
+    from oracle import refcpu as O
+    sc = _crashcat_scenes()[which]
+    f = tmp_path / "world.json"
+    f.write_text(json.dumps(sc.crashcat_world()))
+    steps, fstep, fbody = 40, 10, sc.num_bodies - 1
+    _addon()
+    r = subprocess.run(["node", RUN_CRASHCAT, str(f), "fp64", str(steps), str(fstep), str(fbody)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = [json.loads(x) for x in r.stdout.strip().splitlines()]
+    events, final = lines[:-1], lines[-1]
+    assert len(events) == steps and final["previousTimeStep"] == pytest.approx(1 / 60)
+    orc = O.OracleWorld(sc)
+    n = sc.num_bodies
+    for s in range(steps):
+        if s == fstep:
+            fo, to = np.zeros((n, 3)), np.zeros((n, 3))
+            fo[fbody] = (400, 900, 0)
+            to[fbody] = (0, 0, 30)
+            orc.add_forces(fo, to, wake=False)
+        orc.step(1.0 / 60.0)
+        c, rm = orc.contacts(), orc.removed_contacts()
+        rows = [[int(k["body_a"]), int(k["body_b"]), int(k["sub_shape_a"]), int(k["sub_shape_b"]), int(k["num_points"])]
+                for k in c]
+        assert events[s]["added"] == [x for x, k in zip(rows, c) if not k["persisted"]], f"step {s + 1}"
+        assert events[s]["persisted"] == [x for x, k in zip(rows, c) if k["persisted"]], f"step {s + 1}"
+        assert events[s]["removed"] == [[int(k["body_a"]), int(k["body_b"]), int(k["sub_shape_a"]),
+                                         int(k["sub_shape_b"])] for k in rm], f"step {s + 1}"
+    assert events[fstep]["forceCleared"]
+    ref = orc.bodies()
+    got = final["bodies"]
+    for i in range(n):
+        assert got[i]["position"] == ref["position"][i].tolist(), i
+        assert got[i]["quaternion"] == ref["quaternion"][i].tolist(), i
+        assert got[i]["com"] == ref["center_of_mass_position"][i].tolist(), i
+        assert got[i]["v"] == ref["linear_velocity"][i].tolist(), i
+        assert got[i]["w"] == ref["angular_velocity"][i].tolist(), i
+        assert got[i]["sleeping"] == bool(ref["sleeping"][i]), i
