@@ -210,6 +210,30 @@ class StepCounters(C.Structure):
     ]
 
 
+class CachedContact(C.Structure):
+    _fields_ = [
+        ("body_a", C.c_int32),
+        ("body_b", C.c_int32),
+        ("sub_shape_a", C.c_uint32),
+        ("sub_shape_b", C.c_uint32),
+        ("num_points", C.c_int32),
+        ("pad_", C.c_int32),
+        ("local_a", (C.c_double * 3) * 4),
+        ("local_b", (C.c_double * 3) * 4),
+        ("lambda_n", C.c_double * 4),
+        ("lambda_t1", C.c_double * 4),
+        ("lambda_t2", C.c_double * 4),
+    ]
+
+
+class BodySleep(C.Structure):
+    _fields_ = [
+        ("timer", C.c_double),
+        ("center", (C.c_double * 3) * 3),
+        ("radius", C.c_double * 3),
+    ]
+
+
 class Totals(C.Structure):
     _fields_ = [
         ("steps", C.c_int64),

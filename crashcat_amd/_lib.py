@@ -16,6 +16,7 @@ SYMBOLS = [
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
     "cc_get_phase_times", "cc_get_totals", "cc_get_solver_residual", "cc_build_triangle_mesh", "cc_download_removed_contacts",
     "cc_snapshot", "cc_restore", "cc_add_forces", "cc_download_removed_contacts_noalloc", "cc_fwd", "cc_bwd",
+    "cc_set_contact_cache", "cc_download_contact_cache", "cc_set_sleep_state", "cc_set_point_impulses",
 ]
 
 
@@ -58,6 +59,10 @@ def lib():
     L.cc_get_totals.argtypes = [vp, C.POINTER(_abi.Totals)]
     L.cc_add_forces.argtypes = [vp, dp, dp, C.c_int32, C.c_int32]
     L.cc_get_solver_residual.argtypes = [vp, C.POINTER(C.c_double)]
+    L.cc_set_contact_cache.argtypes = [vp, C.POINTER(_abi.CachedContact), C.c_int32, C.c_double]
+    L.cc_download_contact_cache.argtypes = [vp, C.POINTER(_abi.CachedContact), C.c_int32, ip, dp]
+    L.cc_set_sleep_state.argtypes = [vp, C.POINTER(_abi.BodySleep), C.c_int32]
+    L.cc_set_point_impulses.argtypes = [vp, dp, C.c_int32]
     L.cc_fwd.argtypes = [vp, C.c_double, C.c_int32, C.c_int32]
     L.cc_bwd.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     for s in SYMBOLS:
@@ -222,6 +227,29 @@ class Context:
         cnt = C.c_int32()
         self._check(lib().cc_get_phase_times(self.h, buf, 16, C.byref(cnt)))
         return {PHASES[i] if i < len(PHASES) else str(i): buf[i] for i in range(cnt.value)}
+
+    def set_contact_cache(self, records, previous_time_step):
+        """Replace the cache the next step warm-starts from (cc_set_contact_cache)."""
+        r = np.ascontiguousarray(records, dtype=np.dtype(_abi.CachedContact))
+        self._check(lib().cc_set_contact_cache(self.h, r.ctypes.data_as(C.POINTER(_abi.CachedContact)), len(r),
+                                               C.c_double(previous_time_step)))
+
+    def contact_cache(self):
+        """(records sorted by key, previous time step) of the last step (cc_download_contact_cache)."""
+        n, pdt = C.c_int32(), C.c_double()
+        self._check(lib().cc_download_contact_cache(self.h, None, 0, C.byref(n), C.byref(pdt)))
+        out = np.zeros(max(n.value, 1), dtype=np.dtype(_abi.CachedContact))
+        self._check(lib().cc_download_contact_cache(self.h, out.ctypes.data_as(C.POINTER(_abi.CachedContact)),
+                                                    n.value, C.byref(n), None))
+        return out[:n.value], pdt.value
+
+    def set_sleep_state(self, sleep):
+        s = np.ascontiguousarray(sleep, dtype=np.dtype(_abi.BodySleep))
+        self._check(lib().cc_set_sleep_state(self.h, s.ctypes.data_as(C.POINTER(_abi.BodySleep)), len(s)))
+
+    def set_point_impulses(self, lam):
+        a = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1, 3)
+        self._check(lib().cc_set_point_impulses(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), len(a)))
 
     def solver_residual(self):
         """The last step's velocity residual (cc_get_solver_residual): sum of |dlambda|, sum of

@@ -1597,6 +1597,178 @@ static int restore_impl(cc_ctx* c, const void* in, int64_t size) {
     return CC_OK;
 }
 
+// ---------------------------------------------------------------- warm state in plain records
+static bool cached_less(const cc_cached_contact& x, const cc_cached_contact& y) {
+    return std::tie(x.body_a, x.body_b, x.sub_shape_a, x.sub_shape_b) <
+           std::tie(y.body_a, y.body_b, y.sub_shape_a, y.sub_shape_b);
+}
+
+template <typename T>
+static int set_cache_impl(cc_ctx* c, const cc_cached_contact* in, int n, double prevDt) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    if (n < 0 || (n > 0 && !in)) return fail(c, CC_E_ARG, "contact cache: bad count or null records");
+    if (n > d.conCap) return fail(c, CC_E_CAPACITY, "contact cache: more records than the cache holds");
+    std::vector<cc_cached_contact> v(in, in + n);
+    for (const auto& r : v)
+        if (r.body_a < 0 || r.body_b >= d.N || r.body_a >= r.body_b || r.num_points < 1 || r.num_points > 4)
+            return fail(c, CC_E_ARG, "contact cache: record with bodies out of range or unordered, or 0 or >4 points");
+    std::sort(v.begin(), v.end(), cached_less);
+    for (size_t k = 1; k < v.size(); k++)
+        if (!cached_less(v[k - 1], v[k])) return fail(c, CC_E_ARG, "contact cache: two records with the same key");
+    // records of one body pair form a run; its first record holds the run length (the hash
+    // points at it and the lookup scans the run for the sub-shape ids)
+    std::vector<CacheRec<T>> rec(v.size());
+    for (size_t k = 0; k < v.size(); k++) {
+        const auto& r = v[k];
+        CacheRec<T> o;
+        memset(&o, 0, sizeof(o));
+        o.a = r.body_a;
+        o.b = r.body_b;
+        o.np = r.num_points;
+        o.subA = r.sub_shape_a;
+        o.subB = r.sub_shape_b;
+        for (int j = 0; j < 4; j++) {
+            o.p1[j] = mk3<T>(T(r.local_a[j][0]), T(r.local_a[j][1]), T(r.local_a[j][2]));
+            o.p2[j] = mk3<T>(T(r.local_b[j][0]), T(r.local_b[j][1]), T(r.local_b[j][2]));
+            o.ln[j] = T(r.lambda_n[j]);
+            o.lt1[j] = T(r.lambda_t1[j]);
+            o.lt2[j] = T(r.lambda_t2[j]);
+        }
+        if (k == 0 || v[k - 1].body_a != r.body_a || v[k - 1].body_b != r.body_b) {
+            size_t e = k;
+            while (e < v.size() && v[e].body_a == r.body_a && v[e].body_b == r.body_b) e++;
+            o.run = (int)(e - k);
+        }
+        rec[k] = o;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int par = 0;
+    HIPCHK(c, hipMemcpy(&par, d.parity, sizeof(int), hipMemcpyDeviceToHost));
+    const int q = par ^ 1;  // the parity the next step reads
+    if (n > 0) {
+        HIPCHK(c, hipMemcpy(d.rec[q], rec.data(), (size_t)n * sizeof(CacheRec<T>), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemset(d.recHit[q], 0, (size_t)n * sizeof(int)));
+    }
+    int rc[2] = {0, 0};
+    rc[q] = n;
+    HIPCHK(c, hipMemcpy(d.recCnt, rc, sizeof(rc), hipMemcpyHostToDevice));
+    const T pdt = T(prevDt);
+    HIPCHK(c, hipMemcpy(d.prevDt, &pdt, sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(d.counters, 0, CNT_N * sizeof(int)));
+    HIPCHK(c, hipMemset(d.hkeys[q], 0xff, (size_t)d.hashCap * sizeof(unsigned long long)));
+    if (n > 0) hipLaunchKernelGGL(k_hash_rebuild<T>, dim3(grid_for(n, 256, 2048)), dim3(256), 0, c->stream, d, q, n);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CC_OK;
+}
+
+template <typename T>
+static int get_cache_impl(cc_ctx* c, cc_cached_contact* out, int cap, int* count, double* prevDt) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int par = 0, rc[2] = {0, 0};
+    HIPCHK(c, hipMemcpy(&par, d.parity, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(rc, d.recCnt, sizeof(rc), hipMemcpyDeviceToHost));
+    const int q = par ^ 1;
+    const int n = std::min(rc[q], d.conCap);
+    *count = n;
+    if (prevDt) {
+        T pdt;
+        HIPCHK(c, hipMemcpy(&pdt, d.prevDt, sizeof(T), hipMemcpyDeviceToHost));
+        *prevDt = (double)pdt;
+    }
+    if (!out || cap <= 0) return CC_OK;
+    std::vector<CacheRec<T>> rec(n);
+    if (n > 0) HIPCHK(c, hipMemcpy(rec.data(), d.rec[q], (size_t)n * sizeof(CacheRec<T>), hipMemcpyDeviceToHost));
+    std::vector<cc_cached_contact> v(n);
+    for (int k = 0; k < n; k++) {
+        const auto& r = rec[k];
+        cc_cached_contact o;
+        memset(&o, 0, sizeof(o));
+        o.body_a = r.a;
+        o.body_b = r.b;
+        o.sub_shape_a = r.subA;
+        o.sub_shape_b = r.subB;
+        o.num_points = r.np;
+        for (int j = 0; j < r.np && j < 4; j++) {
+            o.local_a[j][0] = r.p1[j].x; o.local_a[j][1] = r.p1[j].y; o.local_a[j][2] = r.p1[j].z;
+            o.local_b[j][0] = r.p2[j].x; o.local_b[j][1] = r.p2[j].y; o.local_b[j][2] = r.p2[j].z;
+            o.lambda_n[j] = r.ln[j];
+            o.lambda_t1[j] = r.lt1[j];
+            o.lambda_t2[j] = r.lt2[j];
+        }
+        v[k] = o;
+    }
+    std::sort(v.begin(), v.end(), cached_less);
+    for (int k = 0; k < n && k < cap; k++) out[k] = v[k];
+    return CC_OK;
+}
+
+template <typename T>
+static int set_sleep_impl(cc_ctx* c, const cc_body_sleep* in, int n) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    if (!in || n != d.N) return fail(c, CC_E_ARG, "sleep state: one record per body");
+    std::vector<T> timer(n), radius(3 * (size_t)n);
+    std::vector<V3<T>> center(3 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        timer[i] = T(in[i].timer);
+        for (int k = 0; k < 3; k++) {
+            center[3 * (size_t)i + k] = mk3<T>(T(in[i].center[k][0]), T(in[i].center[k][1]), T(in[i].center[k][2]));
+            radius[3 * (size_t)i + k] = T(in[i].radius[k]);
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(d.B.stimer, timer.data(), n * sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.B.sc, center.data(), 3 * (size_t)n * sizeof(V3<T>), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.B.sr, radius.data(), 3 * (size_t)n * sizeof(T), hipMemcpyHostToDevice));
+    return CC_OK;
+}
+
+template <typename T>
+static int set_point_impulses_impl(cc_ctx* c, const double* lam, int n) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    if (n != d.NPC || (n > 0 && !lam)) return fail(c, CC_E_ARG, "point impulses: three per point constraint");
+    if (n == 0) return CC_OK;
+    std::vector<PointCon<T>> pc(n);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(pc.data(), d.pc, n * sizeof(PointCon<T>), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; k++) pc[k].lambda = mk3<T>(T(lam[3 * k]), T(lam[3 * k + 1]), T(lam[3 * k + 2]));
+    HIPCHK(c, hipMemcpy(d.pc, pc.data(), n * sizeof(PointCon<T>), hipMemcpyHostToDevice));
+    return CC_OK;
+}
+
+#define CC_DISPATCH(call) (c->cfg.precision == CC_PRECISION_FP64 ? call<double> : call<float>)
+
+extern "C" int cc_set_contact_cache(cc_ctx* c, const cc_cached_contact* in, int32_t n, double prevDt) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->impl->state_changed();
+    return CC_DISPATCH(set_cache_impl)(c, in, n, prevDt);
+}
+extern "C" int cc_download_contact_cache(cc_ctx* c, cc_cached_contact* out, int32_t cap, int32_t* count,
+                                         double* prevDt) {
+    if (!c || !c->impl || !count) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    return CC_DISPATCH(get_cache_impl)(c, out, cap, count, prevDt);
+}
+extern "C" int cc_set_sleep_state(cc_ctx* c, const cc_body_sleep* in, int32_t n) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->impl->state_changed();
+    return CC_DISPATCH(set_sleep_impl)(c, in, n);
+}
+extern "C" int cc_set_point_impulses(cc_ctx* c, const double* lam, int32_t n) {
+    if (!c || !c->impl) return fail(c, CC_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->impl->state_changed();
+    return CC_DISPATCH(set_point_impulses_impl)(c, lam, n);
+}
+#undef CC_DISPATCH
+
 extern "C" int cc_snapshot(cc_ctx* c, void* out, int64_t cap, int64_t* size) {
     if (!c || !c->impl || !size) return fail(c, CC_E_STATE, "no scene uploaded");
     HIPCHK(c, hipSetDevice(c->cfg.device));

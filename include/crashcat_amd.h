@@ -256,6 +256,33 @@ typedef struct cc_contact_removed {
     uint32_t sub_shape_b;
 } cc_contact_removed;
 
+/* One manifold of the contact cache the next step warm-starts from (CachedManifold in
+ * world.contacts, src/contacts.ts:9-58; matched by (body pair, sub-shape ids), its points by
+ * the lambda-preserve distance, contact-constraints.ts:651-686).  Points are in each body's
+ * centre-of-mass frame (rotated into the body), body_a < body_b. */
+typedef struct cc_cached_contact {
+    int32_t body_a;
+    int32_t body_b;
+    uint32_t sub_shape_a;
+    uint32_t sub_shape_b;
+    int32_t num_points;         /* 1..4 */
+    int32_t pad_;
+    double local_a[4][3];
+    double local_b[4][3];
+    double lambda_n[4];
+    double lambda_t1[4];
+    double lambda_t2[4];
+} cc_cached_contact;
+
+/* A body's sleep test (motionProperties.sleepTestSpheres / sleepTestTimer,
+ * src/body/motion-properties.ts:70-75, src/body/sleep.ts): three sphere centres and radii and the
+ * time it has been at rest. */
+typedef struct cc_body_sleep {
+    double timer;
+    double center[3][3];
+    double radius[3];
+} cc_body_sleep;
+
 /* Per-step counters used by the byte model. */
 typedef struct cc_step_counters {
     int64_t num_bodies;         /* dynamic bodies                                   */
@@ -418,6 +445,21 @@ int cc_get_phase_times(cc_ctx* ctx, double* out_ms, int32_t capacity, int32_t* o
  * CC_E_ARG otherwise): the steps that follow are bit-identical to those after the snapshot.
  * The parity taps (pairs, contacts, removed contacts) are empty until the next step. */
 int cc_snapshot(cc_ctx* ctx, void* out, int64_t capacity, int64_t* out_size);
+
+/* The state one step hands to the next, in plain records rather than the snapshot blob, so that a
+ * host can bring it from elsewhere (another engine's world mid-run, the oracle): the contact
+ * cache with the previous time step (world.contacts, world.previousTimeStep, src/world.ts:38;
+ * it replaces the cache of the last step, which the next step reads), the bodies' sleep tests,
+ * and the point constraints' accumulated impulses (3 per constraint, PointConstraintPart
+ * totalLambda, src/constraints/point-constraint-part.ts).  Records are validated (CC_E_ARG:
+ * bodies out of range or unordered, point counts outside 1..4, more records than the cache
+ * holds, CC_E_CAPACITY).  cc_download_contact_cache returns the last step's cache, sorted by
+ * (body_a, body_b, sub_shape_a, sub_shape_b), with the time step it was made with. */
+int cc_set_contact_cache(cc_ctx* ctx, const cc_cached_contact* contacts, int32_t count, double previous_time_step);
+int cc_download_contact_cache(cc_ctx* ctx, cc_cached_contact* out, int32_t capacity, int32_t* out_count,
+                              double* out_previous_time_step);
+int cc_set_sleep_state(cc_ctx* ctx, const cc_body_sleep* in, int32_t num_bodies);
+int cc_set_point_impulses(cc_ctx* ctx, const double* lambda, int32_t num_point_constraints);
 
 /* The velocity residual of the last step (needs cc_config.measure_residual): out[0] the sum over
  * every solved contact point of the impulse change a further velocity sweep would make (normal,

@@ -55,6 +55,12 @@ int32_t oc_get_pairs(const oc_world* w, int32_t* out, int32_t cap);
 int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap);
 /* contacts the last step removed (destroyUnprocessedContacts), sorted like the contacts */
 int32_t oc_get_removed_contacts(const oc_world* w, cc_contact_removed* out, int32_t cap);
+/* the state the next step starts from, as the C ABI's plain records: the contact cache sorted by
+ * (body_a, body_b, sub ids) (returns the count; with the previous time step), the bodies' sleep
+ * tests, the point constraints' accumulated impulses (3 each) */
+int32_t oc_get_contact_cache(const oc_world* w, cc_cached_contact* out, int32_t cap, double* prev_dt);
+void oc_get_sleep_state(const oc_world* w, cc_body_sleep* out);
+void oc_get_point_impulses(const oc_world* w, double* out);
 void oc_get_counters(const oc_world* w, cc_step_counters* out);
 /* broadphase only, on given AABBs (widened from float) */
 int32_t oc_broadphase_aabbs(const float* mn, const float* mx, const int32_t* motion, const int32_t* sleeping,

@@ -41,6 +41,10 @@ def lib():
         L.oc_get_pairs.argtypes = [C.c_void_p, ip, C.c_int32]
         L.oc_get_contacts.restype = C.c_int32
         L.oc_get_contacts.argtypes = [C.c_void_p, C.POINTER(_abi.ContactOut), C.c_int32]
+        L.oc_get_contact_cache.restype = C.c_int32
+        L.oc_get_contact_cache.argtypes = [C.c_void_p, C.POINTER(_abi.CachedContact), C.c_int32, dp]
+        L.oc_get_sleep_state.argtypes = [C.c_void_p, C.POINTER(_abi.BodySleep)]
+        L.oc_get_point_impulses.argtypes = [C.c_void_p, dp]
         L.oc_get_removed_contacts.restype = C.c_int32
         L.oc_get_removed_contacts.argtypes = [C.c_void_p, C.POINTER(_abi.ContactRemoved), C.c_int32]
         L.oc_get_counters.argtypes = [C.c_void_p, C.POINTER(_abi.StepCounters)]
@@ -109,6 +113,7 @@ class OracleWorld:
             pa[i].num_velocity_steps_override = ov
             pa[i].num_position_steps_override = op
         self._pcs = pa
+        self.npc = len(pcs)
         self._settings = scene.settings
         self.n = b1 - b0
         self.h = L.oc_create(C.byref(scene.settings), bodies.ctypes.data_as(C.POINTER(_abi.BodyDesc)), self.n,
@@ -164,6 +169,24 @@ class OracleWorld:
         out = np.zeros(max(n, 1), dtype=CONTACT_DTYPE)
         lib().oc_get_contacts(self.h, out.ctypes.data_as(C.POINTER(_abi.ContactOut)), n)
         return out[:n]
+
+    def contact_cache(self):
+        """(records, previous time step): the cache the next step matches, sorted by key."""
+        pdt = C.c_double()
+        n = lib().oc_get_contact_cache(self.h, None, 0, C.byref(pdt))
+        out = np.zeros(max(n, 1), dtype=np.dtype(_abi.CachedContact))
+        lib().oc_get_contact_cache(self.h, out.ctypes.data_as(C.POINTER(_abi.CachedContact)), n, None)
+        return out[:n], pdt.value
+
+    def sleep_state(self):
+        out = np.zeros(self.n, dtype=np.dtype(_abi.BodySleep))
+        lib().oc_get_sleep_state(self.h, out.ctypes.data_as(C.POINTER(_abi.BodySleep)))
+        return out
+
+    def point_impulses(self):
+        out = np.zeros((max(len(self._pcs), 1), 3))
+        lib().oc_get_point_impulses(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[:self.npc]
 
     def removed_contacts(self):
         n = lib().oc_get_removed_contacts(self.h, None, 0)

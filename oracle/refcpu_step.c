@@ -1427,6 +1427,69 @@ int32_t oc_get_removed_contacts(const oc_world* w, cc_contact_removed* out, int3
     return w->nrem;
 }
 
+static int cmp_cached(const void* x, const void* y) {
+    const cc_cached_contact* a = (const cc_cached_contact*)x;
+    const cc_cached_contact* b = (const cc_cached_contact*)y;
+    if (a->body_a != b->body_a) return a->body_a < b->body_a ? -1 : 1;
+    if (a->body_b != b->body_b) return a->body_b < b->body_b ? -1 : 1;
+    if (a->sub_shape_a != b->sub_shape_a) return a->sub_shape_a < b->sub_shape_a ? -1 : 1;
+    if (a->sub_shape_b != b->sub_shape_b) return a->sub_shape_b < b->sub_shape_b ? -1 : 1;
+    return 0;
+}
+
+/* world.contacts as plain records (src/contacts.ts:9-58): the cache the next step matches */
+int32_t oc_get_contact_cache(const oc_world* w, cc_cached_contact* out, int32_t cap, double* prev_dt) {
+    if (prev_dt) *prev_dt = w->prevDt;
+    if (!out || cap <= 0) return w->nct;
+    cc_cached_contact* t = (cc_cached_contact*)xrealloc(NULL, sizeof(cc_cached_contact) * (size_t)(w->nct ? w->nct : 1));
+    for (int k = 0; k < w->nct; k++) {
+        const OContact* c = &w->ct[k];
+        cc_cached_contact* o = &t[k];
+        memset(o, 0, sizeof(*o));
+        o->body_a = c->a;
+        o->body_b = c->b;
+        o->sub_shape_a = c->sa;
+        o->sub_shape_b = c->sb;
+        o->num_points = c->numPoints;
+        for (int j = 0; j < c->numPoints && j < 4; j++) {
+            const OCached* p = &c->pts[j];
+            o->local_a[j][0] = p->p1.x; o->local_a[j][1] = p->p1.y; o->local_a[j][2] = p->p1.z;
+            o->local_b[j][0] = p->p2.x; o->local_b[j][1] = p->p2.y; o->local_b[j][2] = p->p2.z;
+            o->lambda_n[j] = p->ln;
+            o->lambda_t1[j] = p->lt1;
+            o->lambda_t2[j] = p->lt2;
+        }
+    }
+    sort_some(t, (size_t)w->nct, sizeof(cc_cached_contact), cmp_cached);
+    const int n = w->nct < cap ? w->nct : cap;
+    if (n > 0) memcpy(out, t, sizeof(cc_cached_contact) * (size_t)n);
+    free(t);
+    return w->nct;
+}
+
+/* motionProperties.sleepTestSpheres / sleepTestTimer of every body */
+void oc_get_sleep_state(const oc_world* w, cc_body_sleep* out) {
+    for (int i = 0; i < w->nb; i++) {
+        const OBody* b = &w->b[i];
+        out[i].timer = b->sleepTimer;
+        for (int k = 0; k < 3; k++) {
+            out[i].center[k][0] = b->sleepCenter[k].x;
+            out[i].center[k][1] = b->sleepCenter[k].y;
+            out[i].center[k][2] = b->sleepCenter[k].z;
+            out[i].radius[k] = b->sleepRadius[k];
+        }
+    }
+}
+
+/* PointConstraintPart totalLambda of every point constraint */
+void oc_get_point_impulses(const oc_world* w, double* out) {
+    for (int k = 0; k < w->npc; k++) {
+        out[3 * k] = w->pc[k].lambda.x;
+        out[3 * k + 1] = w->pc[k].lambda.y;
+        out[3 * k + 2] = w->pc[k].lambda.z;
+    }
+}
+
 void oc_get_counters(const oc_world* w, cc_step_counters* out) { *out = w->ctr; }
 
 int32_t oc_broadphase_aabbs(const float* mn, const float* mx, const int32_t* motion, const int32_t* sleeping,
