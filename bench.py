@@ -31,9 +31,9 @@ Beside it on the same JSON line:
   config4           50,000 convex bodies on a 522k-triangle terrain, 300 steps, FP32 and FP64
   config5           1,024 throw worlds, 120 steps: forward + adjoint throughput; gradients
                     against the FP64 oracle: central differences before the impact (12 steps,
-                    where the loss is smooth), median slopes of 41 samples over the 120 steps
-                    (where central differences measure the loss's small jumps), and the raw
-                    central differences by step size
+                    where the loss is smooth), and over the 120 steps (where it is not) the raw
+                    central differences by step size and a line fit of 41 samples with the
+                    loss's scatter about it
   cpu_baseline      the FP64 oracle on the host, one world per thread, over a bounded sample of
                     the headline workload (worlds of the batch from the GPU state at the start of
                     the timed window)
@@ -537,10 +537,12 @@ def config5(args, S, Context):
                        "rel_err_fp32": abs(adj(early["fp32"], w)[3] - ref) / max(abs(ref), 1e-12),
                        "rel_err_fp64": abs(adj(early["fp64"], w)[3] - ref) / max(abs(ref), 1e-12)})
     out["gradients_12_steps_vs_central_fd"] = smooth
-    # (2) 120 steps: the loss has small jumps (contact-set and manifold switches) on top of a
-    # smooth part; central differences measure the jumps (they grow like 1/h), so the smooth
-    # derivative -- what the adjoint computes, the contact set held fixed -- is estimated as the
-    # median slope between 41 consecutive samples over [-2e-3, 2e-3]
+    # (2) 120 steps: after the impact the loss of the oracle is not smooth at finite-difference
+    # scales (central differences grow like 1/h: contact sets, manifold choices and warm-start
+    # matches of the settling stack switch under perturbations of that size).  Reported: the
+    # central differences by step size, and for each parameter of two worlds a least-squares line
+    # through 41 samples over [-2e-3, 2e-3] with the loss's scatter about it, beside the
+    # adjoint's slope times the same range (what a smooth loss would move by)
     sweep = {}
     for h in (1e-2, 1e-3, 1e-4, 1e-5, 1e-6):
         a, b = pool.map(lambda x: loss(0, steps, dmu=x), (h, -h))
@@ -549,19 +551,17 @@ def config5(args, S, Context):
     pw = []
     xs = np.linspace(-2e-3, 2e-3, 41)
     for w in range(min(args.fd_worlds, 2)):
-        est = []
+        rows = []
         for k in range(4):
             ys = np.array(list(pool.map(lambda x: perturbed(w, steps, k, x), xs)))
-            sl = np.diff(ys) / np.diff(xs)
-            est.append(float(np.median(sl)))
-        est = np.array(est)
-        scale = max(np.abs(est).max(), 1e-12)
-        pw.append({"world": w, "median_slope": est.tolist(), "adjoint_fp32": adj(grads["fp32"], w).tolist(),
-                   "adjoint_fp64": adj(grads["fp64"], w).tolist(),
-                   "rel_err_fp32": float(np.abs(adj(grads["fp32"], w) - est).max() / scale),
-                   "rel_err_fp64": float(np.abs(adj(grads["fp64"], w) - est).max() / scale)})
+            slope, icpt = np.polyfit(xs, ys, 1)
+            rows.append({"param": ["v0x", "v0y", "v0z", "mu"][k], "fit_slope": float(slope),
+                         "scatter_std": float(np.std(ys - (slope * xs + icpt))),
+                         "adjoint_fp64": float(adj(grads["fp64"], w)[k]), "adjoint_fp32": float(adj(grads["fp32"], w)[k]),
+                         "adjoint_change_over_range": float(abs(adj(grads["fp64"], w)[k]) * 4e-3)})
+        pw.append({"world": w, "params": rows})
     pool.shutdown()
-    out["gradients_120_steps_vs_median_slope"] = pw
+    out["loss_120_steps_line_fit"] = pw
     log(f"config5 gradients: 12 steps {smooth}; 120 steps {pw}; central FD by h {sweep}")
     return out
 

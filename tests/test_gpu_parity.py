@@ -170,18 +170,25 @@ def test_broadphase_pairs_bit_exact(seed, n):
         g.close()
 
 
+def _point_set_differs(oc, gc, k):
+    """The FP32 and FP64 manifolds of contact k chose different points (a different count, or a
+    point more than 1 mm from its counterpart): a discrete choice inside the reference's
+    tolerances (clipping and reduction near-ties, manifold.ts:192-313; EPA's stop,
+    penetration.ts:430-478) went the other way."""
+    n1, n2 = oc["num_points"][k], gc["num_points"][k]
+    if n1 != n2:
+        return True
+    return not np.allclose(oc["position_a"][k][:n1], gc["position_a"][k][:n2], atol=1e-3)
+
+
 def test_fp32_first_step_tolerance():
     """FP32 batched path, one step from the same initial state as the FP64 oracle (20,000-body
-    mixed pile, bodies placed at random and so often deeply overlapping).  The pair set is
-    identical.  Two effects separate FP32 from FP64, both discrete choices inside tolerances of
-    the reference's own algorithm: (1) near-ties in clipping and point reduction
-    (manifold.ts:192-313) pick different contact points, after which the solver spreads
-    different impulses over that island; (2) EPA stops within its relative tolerance
-    (penetration.ts:430-478) at a slightly different face, which moves the witness point of a
-    deep contact by up to about a millimetre.  Tolerances: islands with a point-selection
-    difference (1) stay under 1% of the bodies; outside them 99.9% of bodies agree to 5e-5 m
-    (measured 2.7e-5 m) and every body to 2e-4 m (measured 1.03e-4 m, one deep capsule
-    contact of kind (2)); over all bodies the 99th percentile is under 2e-5 m."""
+    mixed pile, placed at random and so often deeply overlapping), every dynamic body counted.
+    The pair set is identical.  Measured: 99.835 % of bodies within 1e-4 m (33 over), 99th
+    percentile 6.8e-6 m, median 1.2e-6 m.  Every body over 1e-4 m is in an island where some
+    contact's point set differs between the precisions (a discrete choice at a near-tie, after
+    which the solver spreads different impulses over the island); the test asserts that cause
+    for each of them, at most 0.25 % of bodies over the bound, and the percentile figures."""
     base = SC.mixed_pile(seed=1002)
     orc = O.OracleWorld(base)
     g = Context("fp32", max_pairs_per_body=24)
@@ -199,19 +206,56 @@ def test_fp32_first_step_tolerance():
         assert np.array_equal(oc["body_a"], gc["body_a"]) and np.array_equal(oc["body_b"], gc["body_b"])
         flipped = set()
         for k in range(len(oc)):
-            n1, n2 = oc["num_points"][k], gc["num_points"][k]
-            same = n1 == n2 and np.allclose(oc["position_a"][k][:n1], gc["position_a"][k][:n2], atol=1e-3)
-            if not same:
-                flipped.add(int(ob["island"][oc["body_a"][k]]) if ob["island"][oc["body_a"][k]] >= 0 else
-                            int(ob["island"][oc["body_b"][k]]))
-        in_flip = np.isin(ob["island"], list(flipped)) & (ob["island"] >= 0)
-        assert in_flip.sum() < 0.01 * dyn.sum()
-        rest = dyn & ~in_flip
-        assert np.quantile(err[rest], 0.999) <= 5e-5
-        assert err[rest].max() <= 2e-4
+            if _point_set_differs(oc, gc, k):
+                isl = ob["island"][oc["body_a"][k]]
+                flipped.add(int(isl if isl >= 0 else ob["island"][oc["body_b"][k]]))
+        over = np.nonzero(dyn & (err > 1e-4))[0]
+        unexplained = [int(i) for i in over if int(ob["island"][i]) not in flipped]
+        assert not unexplained, (unexplained, err[unexplained])
+        assert len(over) <= 0.0025 * dyn.sum(), len(over)
         assert np.quantile(err[dyn], 0.99) <= 2e-5
+        assert np.median(err[dyn]) <= 5e-6
     finally:
         g.close()
+
+
+def test_fp32_batched_chains_against_the_oracle():
+    """Config 3's form in FP32 (worlds of boxes and spheres with capsule chains on point
+    constraints) against the FP64 oracle, one step from the same initial state, per world: at
+    least 99.5 % of the dynamic bodies within 1e-4 m, every one over it in an island with a
+    differing contact point set."""
+    sc = SC.batched_worlds(seed=1003, num_worlds=24, min_bodies=8, max_bodies=200)
+    offs = sc.offsets()
+    g = Context("fp32", max_pairs_per_body=16, max_contacts_per_body=8)
+    g.upload(sc)
+    try:
+        g.step(DT, 1)
+        g.sync()
+        gb_all, gc_all = g.bodies(), g.contacts()
+    finally:
+        g.close()
+    total = over_total = 0
+    for w in range(sc.num_worlds):
+        orc = O.OracleWorld(sc, w)
+        orc.step(DT)
+        ob = orc.bodies()
+        gb = gb_all[offs[w]:offs[w + 1]]
+        dyn = sc.bodies()["motion_type"][offs[w]:offs[w + 1]] == 2
+        err = np.linalg.norm(ob["position"] - gb["position"], axis=1)
+        err[~dyn] = 0
+        oc = orc.contacts()
+        sel = (gc_all["body_a"] >= offs[w]) & (gc_all["body_a"] < offs[w + 1])
+        gc = gc_all[sel].copy()
+        gc["body_a"] -= offs[w]
+        gc["body_b"] -= offs[w]
+        assert np.array_equal(oc["body_a"], gc["body_a"]) and np.array_equal(oc["body_b"], gc["body_b"]), w
+        flipped = {int(ob["island"][oc["body_a"][k]] if ob["island"][oc["body_a"][k]] >= 0 else
+                       ob["island"][oc["body_b"][k]]) for k in range(len(oc)) if _point_set_differs(oc, gc, k)}
+        over = np.nonzero(dyn & (err > 1e-4))[0]
+        assert all(int(ob["island"][i]) in flipped for i in over), (w, over, err[over])
+        total += int(dyn.sum())
+        over_total += len(over)
+    assert over_total <= 0.005 * total, (over_total, total)
 
 
 def test_fp32_batched_worlds_match_single_world_runs():
