@@ -8,7 +8,8 @@ seeded scene, worlds 1.. redraw positions, orientations and velocities over the 
 `--warmup` untimed steps; the steps before the warmup are advanced untimed too, so that the
 window is the same steps of the same trajectory whatever the flags (with --steps 20 --warmup 5:
 steps 176-180 warm up, 181-200 are timed).  `value` = bodies x timed steps / timed seconds,
-with the scene resident in device memory.
+with the scene resident in device memory.  Bulk state, broadphase boxes and the solver are FP32;
+GJK, EPA and the convex manifolds run in FP64 from the FP32 poses (the context's default).
 
 Beside it on the same JSON line:
   trajectory        body-steps/s over all steps of the trajectory (the autotuner's one-time
@@ -19,8 +20,13 @@ Beside it on the same JSON line:
                     byte model over its HIP-event time; `traffic` (HBM bytes per launch from
                     calibrated FETCH_SIZE/WRITE_SIZE counters) only when the committed counter
                     summary covers this exact window and the figure stays under peak
-  accuracy          FP32 against the FP64 oracle: one step from the initial state, one step from
-                    the settled pile, and the drift over 60 steps, against 1e-4 m and 1e-3 m
+  accuracy          FP32 against the FP64 oracle from one state (the FP32 context's rounding of
+                    it, given to both): one step from the initial state, one step from the
+                    settled pile, and the drift over 60 steps, against 1e-4 m and 1e-3 m; the
+                    oracle's own sensitivity to that rounding; the same steps with GJK, EPA and
+                    the manifolds in FP32
+  narrowphase_fp32_variant
+                    the headline window with GJK, EPA and the manifolds in FP32
   fp64_plain_colored_order_body_steps_per_s
                     the same single world in the graph-coloured solve order
   config1           cube heap, 1,000 boxes, 300 steps: FP64 plain path (bit-for-bit check
@@ -242,10 +248,10 @@ def advance(ctx, n, label, chunk=10):
     return time.perf_counter() - t
 
 
-def headline(args, S, Context, batch, nbody, profile=False):
+def headline(args, S, Context, batch, nbody, profile=False, narrowphase=None):
     traj, pre = headline_window(args.steps, args.warmup)
     ctx = Context("fp32", max_pairs_per_body=args.pairs_per_body, block_solver=args.block,
-                  max_contacts_per_body=args.contacts_per_body)
+                  max_contacts_per_body=args.contacts_per_body, narrowphase=narrowphase or args.narrowphase)
     ctx.upload(batch)
     t_pre = advance(ctx, pre, "untimed advance") if pre else 0.0
     t_warm = advance(ctx, args.warmup, "warmup", chunk=max(1, args.warmup))
@@ -290,22 +296,41 @@ def fp64_plain(args, S, Context, base, order="reference"):
 
 
 def accuracy(args, S, Context, base):
+    """FP32 against the FP64 oracle from one state: the scene's (or the settled pile's) poses and
+    velocities as the FP32 context holds them, given to both.  The oracle's own sensitivity to that
+    rounding (the oracle from the exact FP64 scene against the oracle from its FP32 rounding, after
+    one step and after 60) is reported beside it: no program that stores the state in FP32 can sit
+    closer to the unrounded run than that, and the pile is chaotic enough that two FP64 runs one
+    FP32 rounding apart drift as far in 60 steps as the FP32 path does."""
     from oracle.refcpu import OracleWorld
     one = S.replicate_world(base, 1, 1002)
     dyn = one.bodies()["motion_type"] == 2
-    acc = {}
-    orc = OracleWorld(one)
+    start = S.as_fp32(one.bodies())
+    acc = {"state": "the scene's FP32 rounding (positions, orientations, velocities) given to both sides"}
+    exact = OracleWorld(one)
+    exact.step(DT)
+    orc = OracleWorld(one, bodies=start)
+    orc.step(DT)
+    ob1 = orc.bodies()
+    acc["oracle_input_rounding_floor"] = pos_err(exact.bodies(), ob1, dyn, 1e-4)
     g = Context("fp32")
     g.upload(one)
-    orc.step(DT)
     g.step(DT, 1)
-    acc["step_from_initial_state"] = pos_err(orc.bodies(), g.bodies(), dyn, 1e-4)
+    acc["step_from_initial_state"] = pos_err(ob1, g.bodies(), dyn, 1e-4)
     for s in range(2, 61):
         orc.step(DT)
+        exact.step(DT)
         g.step(DT, 1)
         if s % 20 == 0:
             log(f"  oracle/fp32 lockstep {s} steps")
     acc["drift_60_steps"] = pos_err(orc.bodies(), g.bodies(), dyn, 1e-3)
+    acc["oracle_input_rounding_floor_60_steps"] = pos_err(exact.bodies(), orc.bodies(), dyn, 1e-3)
+    del exact
+    g.close()
+    g = Context("fp32", narrowphase="fp32")
+    g.upload(one)
+    g.step(DT, 1)
+    acc["narrowphase_fp32_step_from_initial_state"] = pos_err(ob1, g.bodies(), dyn, 1e-4)
     g.close()
     # the settled pile: FP32 state after 100 steps handed to the oracle and to a fresh FP32
     # context; one step of each from that identical state
@@ -316,13 +341,16 @@ def accuracy(args, S, Context, base):
     g.close()
     orc = OracleWorld(one)
     orc.set_bodies(st)
-    g = Context("fp32")
-    g.upload(one)
-    g.set_bodies(st)
     orc.step(DT)
-    g.step(DT, 1)
-    acc["step_from_settled_pile"] = pos_err(orc.bodies(), g.bodies(), dyn, 1e-4)
-    g.close()
+    ob = orc.bodies()
+    for mode in ("fp64", "fp32"):
+        g = Context("fp32", narrowphase=mode)
+        g.upload(one)
+        g.set_bodies(st)
+        g.step(DT, 1)
+        key = "step_from_settled_pile" if mode == "fp64" else "narrowphase_fp32_step_from_settled_pile"
+        acc[key] = pos_err(ob, g.bodies(), dyn, 1e-4)
+        g.close()
     log(f"accuracy {acc}")
     return acc
 
@@ -579,6 +607,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-accuracy", action="store_true")
+    ap.add_argument("--skip-np32-variant", action="store_true", help="skip the FP32-narrowphase headline run")
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps per world in the CPU baseline")
     ap.add_argument("--cpu-worlds", type=int, default=16, help="worlds (one per thread) in the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -588,6 +617,8 @@ def main():
     ap.add_argument("--fd-worlds", type=int, default=4, help="config-5 worlds compared with finite differences")
     ap.add_argument("--skip-config1", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the headline trajectory (for rocprofv3)")
+    ap.add_argument("--narrowphase", default="fp64", choices=("fp64", "fp32"),
+                    help="arithmetic of GJK and EPA in the FP32 batch")
     ap.add_argument("--calibrate", type=int, default=0, help="only copy this many bytes device to device")
     ap.add_argument("--summarize-trace", default=None, help="rocprofv3 --kernel-trace output directory")
     ap.add_argument("--summarize-counters", nargs=5, default=None,
@@ -648,6 +679,12 @@ def main():
         roofline["traffic_GBps"] = traffic / (dom_ms * 1e-3) / 1e9
         roofline["traffic_over_algorithmic"] = traffic / (pb[dom] / K)
 
+    np32 = None
+    if not args.skip_np32_variant:
+        V = headline(args, S, Context, batch, nbody, profile=True, narrowphase="fp32")
+        np32 = {"body_steps_per_s": nbody * K / V["el"], "ms_per_step": 1000.0 * V["el"] / K,
+                "phase_ms_per_step": {k: x / K for k, x in V["phases"].items()},
+                "note": "the same window with GJK, EPA and the manifolds in FP32 (EPA faces FP64)"}
     fp64, fp64_phases = (None, None) if args.skip_fp64 else fp64_plain(args, S, Context, base)
     fp64c = None if args.skip_fp64 else fp64_plain(args, S, Context, base, "colored")[0]
     acc = None if args.skip_accuracy else accuracy(args, S, Context, base)
@@ -687,6 +724,7 @@ def main():
         "fp64_plain_colored_order_body_steps_per_s": fp64c,
         "batched_fp32_over_plain_fp64": (value / fp64) if fp64 else None,
         "accuracy_vs_fp64_oracle": acc,
+        "narrowphase_fp32_variant": np32,
         "phases": per_phase,
         "roofline": roofline,
         "model_bytes_per_body_step": sum(pb.values()) / max(1, nbody * K),

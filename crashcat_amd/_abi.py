@@ -261,4 +261,5 @@ class Config(C.Structure):
         ("disable_graphs", C.c_int32),
         ("solver_order", C.c_int32),
         ("measure_residual", C.c_int32),
+        ("narrowphase_fp32", C.c_int32),
     ]

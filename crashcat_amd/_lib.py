@@ -86,7 +86,10 @@ class Context:
     """One device context (a batch of worlds) behind the C ABI."""
 
     def __init__(self, precision="fp32", device=0, max_pairs_per_body=16, block_solver=0, max_contacts_per_body=8,
-                 island_groups=0, block_bodies=0, graphs=True, solver_order="reference", measure_residual=False):
+                 island_groups=0, block_bodies=0, graphs=True, solver_order="reference", measure_residual=False,
+                 narrowphase="fp64"):
+        """narrowphase: arithmetic of GJK and EPA in an FP32 context, "fp64" (from the FP32 poses,
+        with FP64 shape data) or "fp32"; FP64 contexts always use FP64."""
         L = lib()
         cfg = _abi.Config()
         cfg.precision = _abi.CC_PRECISION_FP64 if precision == "fp64" else _abi.CC_PRECISION_FP32
@@ -101,6 +104,9 @@ class Context:
             raise ValueError("solver_order is 'reference' or 'colored'")
         cfg.solver_order = _abi.CC_ORDER_COLORED if solver_order == "colored" else _abi.CC_ORDER_REFERENCE
         cfg.measure_residual = 1 if measure_residual else 0
+        if narrowphase not in ("fp64", "fp32"):
+            raise ValueError("narrowphase is 'fp64' or 'fp32'")
+        cfg.narrowphase_fp32 = 1 if narrowphase == "fp32" else 0
         h = C.c_void_p()
         rc = L.cc_create(C.byref(h), C.byref(cfg))
         if rc != 0:

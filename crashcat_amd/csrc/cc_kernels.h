@@ -2,6 +2,8 @@
 // grid-stride loop bounded by a count read from device memory, so a step launches a
 // fixed sequence of kernels with no host round trip (and can be captured in a graph).
 #pragma once
+#include <type_traits>
+
 #include "cc_step.h"
 
 namespace cc {
@@ -117,17 +119,28 @@ struct Dev {
     unsigned long long* hkeys[2]; // [hashCap]
     int* hvals[2];
     // narrowphase hits awaiting their manifold: pair slot, kind (HIT_*), and the closest points
-    // and axis (in body A's frame for convex pairs, in world space for sphere pairs)
+    // and axis (in body A's frame for convex pairs, in world space for sphere pairs), kept in FP64
+    // so that an FP64 manifold of an FP32 context starts from GJK's and EPA's unrounded results
     int hitCap;
     int* hitSlot;
     int* hitKind;
-    V3<T>* hitPA;
-    V3<T>* hitPB;
-    V3<T>* hitAxis;
+    V3<double>* hitPA;
+    V3<double>* hitPB;
+    V3<double>* hitAxis;
     // EPA jobs and the per-worker polytope workspace
     int* epaPair;       // [epaCap]
     Simplex<T>* epaSimplex;
-    char* epaWork;         // [ceil(epaWorkers / 64) * EpaWave<T>::bytes] polytopes, one block per wavefront
+    char* epaWork;         // [ceil(epaWorkers / 64) * max(EpaWave<T>, EpaWave<double>)::bytes] polytopes,
+                           // one block per wavefront
+    // FP32 contexts with npWide = 1 run GJK and EPA in FP64 from the FP32 poses, with the shapes
+    // and tolerances in FP64 beside the FP32 ones and the EPA jobs' simplices in FP64: their
+    // discrete choices (EPA's last face, GJK's closest points between parallel features) then go
+    // as the FP64 path's from the same poses; the manifolds of convex pairs (supporting faces,
+    // clipping, pruning) are then computed in FP64 too
+    int npWide;
+    ShapeTable<double> st64;
+    double spec64, colTol64, penTol64, manifoldTol64;
+    Simplex<double>* epaSimplex64;  // [epaCap] when npWide
     // point constraints
     PointCon<T>* pc;
     int NPC;
@@ -621,6 +634,25 @@ __device__ inline PairBodies<T> order_pair(const Dev<T>& d, int a, int b) {
     return r;
 }
 
+// The arithmetic type C of GJK, EPA and the convex manifolds: T, or FP64 in an FP32 context with
+// npWide (the shape table, tolerances and EPA simplices in FP64 are then the context's FP64 copies).
+template <typename T, typename C>
+__device__ inline const ShapeTable<C>& np_shapes(const Dev<T>& d) {
+    if constexpr (std::is_same<T, C>::value) return d.st;
+    else return d.st64;
+}
+template <typename C> struct NpTol { C spec, colTol, penTol, manifoldTol; };
+template <typename T, typename C>
+__device__ inline NpTol<C> np_tol(const Dev<T>& d) {
+    if constexpr (std::is_same<T, C>::value) return NpTol<C>{d.s.spec, d.s.colTol, d.s.penTol, d.s.manifoldTol};
+    else return NpTol<C>{C(d.spec64), C(d.colTol64), C(d.penTol64), C(d.manifoldTol64)};
+}
+template <typename T, typename C>
+__device__ inline Simplex<C>* np_simplices(const Dev<T>& d) {
+    if constexpr (std::is_same<T, C>::value) return d.epaSimplex;
+    else return d.epaSimplex64;
+}
+
 // normal velocity bias, src/constraints/contact-constraints.ts:288-423
 template <typename T>
 __device__ inline T normal_bias(const Dev<T>& d, int A, int Bi, int fa, int fb, const NpDyn<T>& DA, const NpDyn<T>& DB,
@@ -655,15 +687,15 @@ __device__ inline T normal_bias(const Dev<T>& d, int A, int Bi, int fa, int fb, 
 
 // Finish one hit: manifold (update.ts:385-563 for the single hit of a convex pair), then
 // addContactConstraint (contact-constraints.ts:437-857) into the pair slot.
-template <typename T, int N>
+template <typename T, typename C, int N>
 __device__ inline void finish_pair(const Dev<T>& d, int slot, int A, int Bi, const NpBody<T>& RA, const NpBody<T>& RB,
-                                   V3<T> pointA, V3<T> pointB, V3<T> axis, const Face<T, N>& fA, const Face<T, N>& fB,
-                                   Face<T, N>* scratch3, Manifold<T, N>& m, T dt, int par) {
-    const Bodies<T>& B = d.B;
-    m.base = RA.pos;
+                                   V3<C> pointA, V3<C> pointB, V3<C> axis, const Face<C, N>& fA, const Face<C, N>& fB,
+                                   Face<C, N>* scratch3, Manifold<C, N>& m, T dt, int par) {
+    const NpTol<C> tol = np_tol<T, C>(d);
+    m.base = cvt<C>(RA.pos);
     m.normal = normalize(axis);
     m.n = 0;
-    T maxDist = d.s.spec + d.s.manifoldTol;
+    C maxDist = tol.spec + tol.manifoldTol;
     if (fA.n >= 2 && fB.n >= 3) manifold_between_two_faces(m, pointA, pointB, axis, maxDist, fA, fB, scratch3);
     else { m.relA[0] = pointA - m.base; m.relB[0] = pointB - m.base; m.n = 1; }
     m.normal = normalize(m.normal);
@@ -682,7 +714,7 @@ __device__ inline void make_contact(const Dev<T>& d, int slot, int A, int Bi, co
     const Bodies<T>& B = d.B;
     // swap so that id A < id B (manifold.swapShapes :86-107)
     int ia = A, ib = Bi;
-    V3<T> n = m.normal;
+    V3<T> n = cvt<T>(m.normal);
     bool sw = ia > ib;
     if (sw) { int t = ia; ia = ib; ib = t; n = -n; unsigned u = subA; subA = subB; subB = u; }
     const NpBody<T>& Ria = sw ? RB : RA;
@@ -760,9 +792,9 @@ __device__ inline void make_contact(const Dev<T>& d, int slot, int A, int Bi, co
     k.np = m.n;
     for (int i = 0; i < m.n; i++) {
         ConPoint<T>& cp = k.p[i];
-        V3<T> ra_ = sw ? m.relB[i] : m.relA[i];
-        V3<T> rb_ = sw ? m.relA[i] : m.relB[i];
-        V3<T> pA = m.base + ra_, pB = m.base + rb_;
+        V3<T> ra_ = cvt<T>(sw ? m.relB[i] : m.relA[i]);
+        V3<T> rb_ = cvt<T>(sw ? m.relA[i] : m.relB[i]);
+        V3<T> pA = cvt<T>(m.base) + ra_, pB = cvt<T>(m.base) + rb_;
         tp.pa[i] = pA;
         tp.pb[i] = pB;
         cp.localA = mul_t(rotA, pA - comA);
@@ -827,29 +859,30 @@ struct NpScratch {
 };
 
 // world-space hit + supporting faces after GJK/EPA (convex.ts:489-637)
-template <typename T, int N>
+template <typename T, typename C, int N>
 __device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, const NpBody<T>& RA,
-                                     const NpBody<T>& RB, const ConvexCtx<T>& c, PenDepth<T> pd, NpScratch<T, N>& S, T dt,
+                                     const NpBody<T>& RB, const ConvexCtx<C>& c, PenDepth<C> pd, NpScratch<C, N>& S, T dt,
                                      int par) {
-    V3<T> pa = RA.pos;
-    Q4<T> qa = RA.q;
-    T ms = c.maxSepUse;
+    const ShapeTable<C>& stc = np_shapes<T, C>(d);
+    V3<C> pa = cvt<C>(RA.pos);
+    Q4<C> qa = cvtq<C>(RA.q);
+    C ms = c.maxSepUse;
     // penetration = |pA - pB| - maxSep; the early-out fraction is +inf for the step collector
-    T al = sqrt(len2(pd.axis));
-    if (al > T(0)) { T corr = ms / al; pd.pointA = pd.pointA + pd.axis * (-corr); }
-    V3<T> wA = qrot2(qa, pd.pointA) + pa;
-    V3<T> wB = qrot2(qa, pd.pointB) + pa;
-    V3<T> wAxis = qrot2(qa, pd.axis);
-    supporting_face(S.fA, d.st, RA.shape, -pd.axis, pa, qa);
-    Q4<T> crel = qconj(c.rel);
-    V3<T> dirB = qrot2(crel, pd.axis);
-    V3<T> posB = qrot2(qa, c.relPos) + pa;
-    Q4<T> qB;
+    C al = sqrt(len2(pd.axis));
+    if (al > C(0)) { C corr = ms / al; pd.pointA = pd.pointA + pd.axis * (-corr); }
+    V3<C> wA = qrot2(qa, pd.pointA) + pa;
+    V3<C> wB = qrot2(qa, pd.pointB) + pa;
+    V3<C> wAxis = qrot2(qa, pd.axis);
+    supporting_face(S.fA, stc, RA.shape, -pd.axis, pa, qa);
+    Q4<C> crel = qconj(c.rel);
+    V3<C> dirB = qrot2(crel, pd.axis);
+    V3<C> posB = qrot2(qa, c.relPos) + pa;
+    Q4<C> qB;
     qB.x = qa.w * c.rel.x + qa.x * c.rel.w + qa.y * c.rel.z - qa.z * c.rel.y;
     qB.y = qa.w * c.rel.y - qa.x * c.rel.z + qa.y * c.rel.w + qa.z * c.rel.x;
     qB.z = qa.w * c.rel.z + qa.x * c.rel.y - qa.y * c.rel.x + qa.z * c.rel.w;
     qB.w = qa.w * c.rel.w - qa.x * c.rel.x - qa.y * c.rel.y - qa.z * c.rel.z;
-    supporting_face(S.fB, d.st, RB.shape, dirB, posB, qB);
+    supporting_face(S.fB, stc, RB.shape, dirB, posB, qB);
     finish_pair(d, slot, A, Bi, RA, RB, wA, wB, wAxis, S.fA, S.fB, S.tmp, S.m, dt, par);
 }
 
@@ -857,17 +890,17 @@ enum { HIT_SPHERES = 0, HIT_GJK = 1, HIT_EPA = 2 };
 
 // Hits whose two shapes have supporting faces of at most four vertices go to the front of the
 // hit list (manifolds of capacity 8), the others to its back (capacity 32).
-template <typename T>
-__device__ inline void push_hit(const Dev<T>& d, int slot, int kind, V3<T> pa, V3<T> pb, V3<T> axis, bool small) {
+template <typename T, typename C>
+__device__ inline void push_hit(const Dev<T>& d, int slot, int kind, V3<C> pa, V3<C> pb, V3<C> axis, bool small) {
     int h;
     if (small) h = wave_alloc(&d.counters[CNT_HITS]);
     else h = d.hitCap - 1 - wave_alloc(&d.counters[CNT_HITS_L]);
     if (h < 0 || h >= d.hitCap) { atomicOr(&d.counters[CNT_OVERFLOW], 16); return; }
     d.hitSlot[h] = slot;
     d.hitKind[h] = kind;
-    d.hitPA[h] = pa;
-    d.hitPB[h] = pb;
-    d.hitAxis[h] = axis;
+    d.hitPA[h] = cvt<double>(pa);
+    d.hitPB[h] = cvt<double>(pb);
+    d.hitAxis[h] = cvt<double>(axis);
 }
 
 // The two narrowphase records of a pair slot, after the motion-type swap (src/update.ts:74-78).
@@ -994,44 +1027,51 @@ __global__ void __launch_bounds__(256) k_gjk_scatter(Dev<T> d) {
 
 // GJK of each listed pair, in type-pair order so that a wavefront's lanes run the same support
 // functions: the GJK step of collideConvexVsConvexLocal (convex.ts:257-641), giving a hit for
-// k_np_manifold or, when the cores overlap, a job for EPA.
-template <typename T>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2) k_np_gjk(Dev<T> d, T dt) {
+// k_np_manifold or, when the cores overlap, a job for EPA.  C: arithmetic type (above).
+template <typename T, typename C>
+__global__ void __launch_bounds__(256, sizeof(C) == 4 ? 3 : 2) k_np_gjk(Dev<T> d, T dt) {
     const int n = min(d.counters[CNT_GJK], d.pairCap);
+    const ShapeTable<C>& stc = np_shapes<T, C>(d);
+    const NpTol<C> tol = np_tol<T, C>(d);
+    Simplex<C>* simplices = np_simplices<T, C>(d);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int slot = d.gjkSorted[k];
         NpBody<T> RA, RB;
         np_pair(d, slot, RA, RB);
-        ConvexCtx<T> c;
-        convex_setup(c, RA.pos, RA.q, RB.pos, RB.q, d.s.spec);
-        Sup<T> SA, SB;
-        sup_for_shape(SA, d.st, RA.shape, false);
-        sup_for_shape(SB, d.st, RB.shape, false);
+        ConvexCtx<C> c;
+        convex_setup(c, cvt<C>(RA.pos), cvtq<C>(RA.q), cvt<C>(RB.pos), cvtq<C>(RB.q), tol.spec);
+        Sup<C> SA, SB;
+        sup_for_shape(SA, stc, RA.shape, false);
+        sup_for_shape(SB, stc, RB.shape, false);
         SB.transformed = true;
         SB.R = m3_from_quat(c.rel);
         SB.t = c.relPos;
-        V3<T> axis0 = c.relPos;
-        if (len2(axis0) <= T(1e-12)) axis0 = mk3<T>(1, 0, 0);
-        PenDepth<T> pd;
-        Simplex<T> simplex;
+        V3<C> axis0 = c.relPos;
+        if (len2(axis0) <= C(1e-12)) axis0 = mk3<C>(1, 0, 0);
+        PenDepth<C> pd;
+        Simplex<C> simplex;
         simplex.size = 0;
-        pd_step_gjk(pd, simplex, SA, SB, SA.cr + c.maxSepUse, SB.cr, axis0, d.s.colTol);
+        pd_step_gjk(pd, simplex, SA, SB, SA.cr + c.maxSepUse, SB.cr, axis0, tol.colTol);
         if (pd.status == PD_NOT_COLLIDING) continue;
         if (pd.status == PD_INDETERMINATE) {
             int j = wave_alloc(&d.counters[CNT_EPA]);
-            if (j < d.epaCap) { d.epaPair[j] = slot; d.epaSimplex[j] = simplex; }
+            if (j < d.epaCap) { d.epaPair[j] = slot; simplices[j] = simplex; }
             else atomicOr(&d.counters[CNT_OVERFLOW], 4);
             continue;
         }
-        push_hit(d, slot, HIT_GJK, pd.pointA, pd.pointB, pd.axis, (RA.tinfo >> 4) <= 4 && (RB.tinfo >> 4) <= 4);
+        push_hit(d, slot, HIT_GJK, pd.pointA, pd.pointB, pd.axis,
+                 (RA.tinfo >> 4) <= 4 && (RB.tinfo >> 4) <= 4);
     }
 }
 
-template <typename T>
+template <typename T, typename C>
 __global__ void __launch_bounds__(64, 2) k_epa(Dev<T> d, T dt) {
     int nj = min(d.counters[CNT_EPA], d.epaCap);
     int worker = blockIdx.x * blockDim.x + threadIdx.x;
     if (worker >= d.epaWorkers) return;
+    const ShapeTable<C>& stc = np_shapes<T, C>(d);
+    const NpTol<C> tol = np_tol<T, C>(d);
+    const Simplex<C>* simplices = np_simplices<T, C>(d);
     for (int j = worker; j < nj; j += gridDim.x * blockDim.x) {
         int slot = d.epaPair[j];
         const int i0 = d.pairA[slot], i1 = d.pairB[slot];
@@ -1040,32 +1080,33 @@ __global__ void __launch_bounds__(64, 2) k_epa(Dev<T> d, T dt) {
         const bool sw = pb.a != i0;
         const NpBody<T>& RA = sw ? r1 : r0;
         const NpBody<T>& RB = sw ? r0 : r1;
-        ConvexCtx<T> c;
-        convex_setup(c, RA.pos, RA.q, RB.pos, RB.q, d.s.spec);
-        c.maxSepUse = tmin<T>(c.maxSepUse, T(1));
-        Sup<T> SA, SB;
-        sup_for_shape(SA, d.st, RA.shape, true);
-        sup_for_shape(SB, d.st, RB.shape, true);
+        ConvexCtx<C> c;
+        convex_setup(c, cvt<C>(RA.pos), cvtq<C>(RA.q), cvt<C>(RB.pos), cvtq<C>(RB.q), tol.spec);
+        c.maxSepUse = tmin<C>(c.maxSepUse, C(1));
+        Sup<C> SA, SB;
+        sup_for_shape(SA, stc, RA.shape, true);
+        sup_for_shape(SB, stc, RB.shape, true);
         SA.addRadius = true;
         SA.addR = c.maxSepUse;
         SB.transformed = true;
         SB.R = m3_from_quat(c.rel);
         SB.t = c.relPos;
-        PenDepth<T> pd;
-        Simplex<T> simplex = d.epaSimplex[j];
-        EpaH<T> H;
-        H.b = d.epaWork + (size_t)(worker / EPA_LANES) * EpaWave<T>::bytes;
+        PenDepth<C> pd;
+        Simplex<C> simplex = simplices[j];
+        EpaH<C> H;
+        H.b = d.epaWork + (size_t)(worker / EPA_LANES) * EpaWave<C>::bytes;
         H.lane = worker % EPA_LANES;
-        if (!pd_step_epa(pd, SA, SB, d.s.penTol, simplex, H)) continue;
-        push_hit(d, slot, HIT_EPA, pd.pointA, pd.pointB, pd.axis, (RA.tinfo >> 4) <= 4 && (RB.tinfo >> 4) <= 4);
+        if (!pd_step_epa(pd, SA, SB, tol.penTol, simplex, H)) continue;
+        push_hit(d, slot, HIT_EPA, pd.pointA, pd.pointB, pd.axis,
+                 (RA.tinfo >> 4) <= 4 && (RB.tinfo >> 4) <= 4);
     }
 }
 
 // One hit per lane: world-space contact, supporting faces, manifold, contact cache and the
 // contact constraint (convex.ts:489-637, src/update.ts:385-563).
 // N: face and manifold capacity; the small variant takes the front of the hit list, the
-// general one its back
-template <typename T, int N>
+// general one its back.  C: arithmetic type (np_shapes above).
+template <typename T, int N, typename C>
 __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
     const bool small = N == CC_SMALL_FACE;
     const int h0 = small ? 0 : d.hitCap - min(d.counters[CNT_HITS_L], d.hitCap);
@@ -1081,21 +1122,22 @@ __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
         const NpBody<T>& RB = sw ? r0 : r1;
         int A = pb.a, Bi = pb.b;
         int kind = d.hitKind[h];
-        NpScratch<T, N> S;
+        NpScratch<C, N> S;
         if (kind == HIT_SPHERES) {
             S.fA.n = 0;
             S.fB.n = 0;
-            finish_pair(d, slot, A, Bi, RA, RB, d.hitPA[h], d.hitPB[h], d.hitAxis[h], S.fA, S.fB, S.tmp, S.m, dt, par);
+            finish_pair(d, slot, A, Bi, RA, RB, cvt<C>(d.hitPA[h]), cvt<C>(d.hitPB[h]), cvt<C>(d.hitAxis[h]), S.fA, S.fB,
+                        S.tmp, S.m, dt, par);
             continue;
         }
-        ConvexCtx<T> c;
-        convex_setup(c, RA.pos, RA.q, RB.pos, RB.q, d.s.spec);
-        if (kind == HIT_EPA) c.maxSepUse = tmin<T>(c.maxSepUse, T(1));
-        PenDepth<T> pd;
+        ConvexCtx<C> c;
+        convex_setup(c, cvt<C>(RA.pos), cvtq<C>(RA.q), cvt<C>(RB.pos), cvtq<C>(RB.q), np_tol<T, C>(d).spec);
+        if (kind == HIT_EPA) c.maxSepUse = tmin<C>(c.maxSepUse, C(1));
+        PenDepth<C> pd;
         pd.status = 0;
-        pd.pointA = d.hitPA[h];
-        pd.pointB = d.hitPB[h];
-        pd.axis = d.hitAxis[h];
+        pd.pointA = cvt<C>(d.hitPA[h]);
+        pd.pointB = cvt<C>(d.hitPB[h]);
+        pd.axis = cvt<C>(d.hitAxis[h]);
         finish_convex(d, slot, A, Bi, RA, RB, c, pd, S, dt, par);
     }
 }
@@ -1103,11 +1145,12 @@ __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
 // Convex-vs-plane pairs (collideConvexVsPlane plane.ts:474-590 and its reverse :297-310): the hit,
 // the convex shape's supporting face and the adaptive plane face (:550-575), then the pair's
 // manifold and constraint, one thread per listed pair.  Kept out of the filter and the manifold
-// kernels so that the convex pairs' kernels keep their registers.
-template <typename T>
+// kernels so that the convex pairs' kernels keep their registers.  C: arithmetic type (np_shapes).
+template <typename T, typename C>
 __global__ void __launch_bounds__(256) k_np_plane(Dev<T> d, T dt) {
     const int n = min(d.counters[CNT_PLANE], d.pairCap);
     const int par = *d.parity;
+    const ShapeTable<C>& stc = np_shapes<T, C>(d);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int slot = d.planeList[k];
         const int i0 = d.pairA[slot], i1 = d.pairB[slot];
@@ -1119,13 +1162,15 @@ __global__ void __launch_bounds__(256) k_np_plane(Dev<T> d, T dt) {
         const bool rev = (RA.tinfo & 15) == SHAPE_PLANE;  // the plane is the narrowphase's body A
         const NpBody<T>& RC = rev ? RB : RA;
         const NpBody<T>& RP = rev ? RA : RB;
-        PlaneHit<T> ph;
-        if (!plane_hit(d.st, RC.shape, RC.pos, RC.q, RP.shape, RP.pos, RP.q, d.s.spec, ph)) continue;
-        NpScratch<T, CC_MAX_FACE> S;
-        Face<T, CC_MAX_FACE>& fC = rev ? S.fB : S.fA;
-        Face<T, CC_MAX_FACE>& fP = rev ? S.fA : S.fB;
-        supporting_face(fC, d.st, RC.shape, ph.nl, RC.pos, RC.q);
-        if (fC.n > 0) plane_face(fP, d.st, RC.shape, RC.pos, RP.shape, RP.pos, RP.q);
+        const V3<C> pc = cvt<C>(RC.pos), pp = cvt<C>(RP.pos);
+        const Q4<C> qc = cvtq<C>(RC.q), qp = cvtq<C>(RP.q);
+        PlaneHit<C> ph;
+        if (!plane_hit(stc, RC.shape, pc, qc, RP.shape, pp, qp, np_tol<T, C>(d).spec, ph)) continue;
+        NpScratch<C, CC_MAX_FACE> S;
+        Face<C, CC_MAX_FACE>& fC = rev ? S.fB : S.fA;
+        Face<C, CC_MAX_FACE>& fP = rev ? S.fA : S.fB;
+        supporting_face(fC, stc, RC.shape, ph.nl, pc, qc);
+        if (fC.n > 0) plane_face(fP, stc, RC.shape, pc, RP.shape, pp, qp);
         else fP.n = 0;
         if (!rev) finish_pair(d, slot, pb.a, pb.b, RA, RB, ph.pA, ph.pB, ph.axis, S.fA, S.fB, S.tmp, S.m, dt, par);
         else finish_pair(d, slot, pb.a, pb.b, RA, RB, ph.pB, ph.pA, -ph.axis, S.fA, S.fB, S.tmp, S.m, dt, par);

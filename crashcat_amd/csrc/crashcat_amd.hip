@@ -306,7 +306,7 @@ static hipError_t resolve_scratch(int device) {
 extern "C" int cc_create(cc_ctx** out, const cc_config* cfg) {
     if (!out) return CC_E_ARG;
     cc_ctx* c = new cc_ctx();
-    cc_config def = {CC_PRECISION_FP32, 0, 16, 8, 0, 0, 0, 0, 0, 0};
+    cc_config def = {CC_PRECISION_FP32, 0, 16, 8, 0, 0, 0, 0, 0, 0, 0};
     c->cfg = cfg ? *cfg : def;
     if (c->cfg.max_pairs_per_body <= 0) c->cfg.max_pairs_per_body = 16;
     if (c->cfg.max_contacts_per_body <= 0) c->cfg.max_contacts_per_body = 8;
@@ -428,6 +428,86 @@ static int upload(cc_ctx* c, const cc_settings* s, int32_t nw, const int32_t* wo
     return CC_OK;
 }
 
+// The shape records and hull arrays of a scene in arithmetic type U (the context's bulk type,
+// and FP64 beside it in an FP32 context whose GJK and EPA run in FP64).  Mesh fields are set by
+// the caller.
+template <typename U>
+static int shape_table(cc_ctx* c, Alloc& A, const cc_shape_desc* shapes, int32_t ns, const cc_hull_data* hulls,
+                       ShapeTable<U>& out) {
+    std::vector<ShapeD<U>> sh(std::max(ns, 1));
+    for (int i = 0; i < ns; i++) {
+        const cc_shape_desc& q = shapes[i];
+        ShapeD<U>& o = sh[i];
+        o.type = q.type; o.hfp = q.hull_first_point; o.hnp = q.hull_num_points; o.hff = q.hull_first_face;
+        o.hnf = q.hull_num_faces;
+        o.mfv = q.type == CC_SHAPE_BOX || q.type == CC_SHAPE_PLANE ? 4
+                : (q.type == CC_SHAPE_CAPSULE ? 2 : (q.type == CC_SHAPE_CYLINDER ? 8 : 0));
+        if (q.type == CC_SHAPE_TRIANGLE_MESH) {
+            o.hfp = q.mesh_first_triangle; o.hnp = q.mesh_num_triangles;
+            o.hff = q.mesh_first_node; o.hnf = q.mesh_num_nodes;
+            o.mfv = 3;
+        }
+        o.radius = U(q.radius); o.hh = U(q.half_height); o.cr = U(q.convex_radius);
+        for (int k = 0; k < 3; k++) {
+            o.he[k] = U(q.half_extents[k]); o.com[k] = U(q.center_of_mass[k]);
+            o.amin[k] = U(q.aabb_min[k]); o.amax[k] = U(q.aabb_max[k]);
+        }
+        // bounding radius, generous (any support point of either mode lies inside it)
+        double br = 0.0;
+        double he2 = q.half_extents[0] * q.half_extents[0] + q.half_extents[1] * q.half_extents[1] + q.half_extents[2] * q.half_extents[2];
+        double com = std::sqrt(q.center_of_mass[0] * q.center_of_mass[0] + q.center_of_mass[1] * q.center_of_mass[1] +
+                               q.center_of_mass[2] * q.center_of_mass[2]);
+        if (q.type == CC_SHAPE_SPHERE) br = q.radius;
+        else if (q.type == CC_SHAPE_BOX) br = std::sqrt(he2) + q.convex_radius;
+        else if (q.type == CC_SHAPE_CAPSULE) br = q.half_height + q.radius;
+        else if (q.type == CC_SHAPE_CYLINDER) br = std::sqrt(q.radius * q.radius + q.half_height * q.half_height);
+        else if (q.type == CC_SHAPE_PLANE) br = 1e30;
+        else if (hulls && hulls->points && hulls->shrunk_points) {
+            for (int k = 0; k < q.hull_num_points && q.hull_first_point + k < hulls->total_points; k++) {
+                const double* pp = hulls->points + 3 * (q.hull_first_point + k);
+                const double* ss = hulls->shrunk_points + 3 * (q.hull_first_point + k);
+                br = std::max(br, std::sqrt(pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]));
+                br = std::max(br, std::sqrt(ss[0] * ss[0] + ss[1] * ss[1] + ss[2] * ss[2]) + q.convex_radius);
+            }
+            br += com;
+        } else br = 1e30;
+        o.brad = U(br * (1.0 + 1e-5) + 1e-6);
+    }
+    int P = hulls ? hulls->total_points : 0, F = hulls ? hulls->total_faces : 0, FV = hulls ? hulls->total_face_vertices : 0;
+    for (int i = 0; i < ns; i++) {  // a hull's supporting face takes at most 16 of a face's vertices
+        if (shapes[i].type != CC_SHAPE_CONVEX_HULL) continue;
+        int mfv = 0;
+        for (int f = shapes[i].hull_first_face; f < shapes[i].hull_first_face + shapes[i].hull_num_faces; f++) {
+            int nv = hulls->face_num_vertices[f];
+            int delta = (nv + 15) / 16;
+            mfv = std::max(mfv, delta > 0 ? (nv + delta - 1) / delta : 0);
+        }
+        sh[i].mfv = mfv;
+    }
+    std::vector<U> hp(3 * std::max(P, 1)), hs(3 * std::max(P, 1)), hn(3 * std::max(F, 1));
+    for (int i = 0; i < 3 * P; i++) { hp[i] = U(hulls->points[i]); hs[i] = U(hulls->shrunk_points[i]); }
+    for (int i = 0; i < 3 * F; i++) hn[i] = U(hulls->plane_normals[i]);
+    ShapeD<U>* dShapes = A.get<ShapeD<U>>(sh.size());
+    U* dPts = A.get<U>(hp.size());
+    U* dShr = A.get<U>(hs.size());
+    U* dPn = A.get<U>(hn.size());
+    int* dFfv = A.get<int>(std::max(F, 1));
+    int* dFnv = A.get<int>(std::max(F, 1));
+    int* dFvi = A.get<int>(std::max(FV, 1));
+    if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (shapes)");
+    HIPCHK(c, hipMemcpy(dShapes, sh.data(), sh.size() * sizeof(ShapeD<U>), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dPts, hp.data(), hp.size() * sizeof(U), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dShr, hs.data(), hs.size() * sizeof(U), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dPn, hn.data(), hn.size() * sizeof(U), hipMemcpyHostToDevice));
+    if (F > 0) {
+        HIPCHK(c, hipMemcpy(dFfv, hulls->face_first_vertex, F * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dFnv, hulls->face_num_vertices, F * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (FV > 0) HIPCHK(c, hipMemcpy(dFvi, hulls->face_vertex_indices, FV * sizeof(int), hipMemcpyHostToDevice));
+    out = ShapeTable<U>{dShapes, dPts, dShr, dFfv, dFnv, dFvi, dPn, nullptr, nullptr, nullptr};
+    return CC_OK;
+}
+
 template <typename T>
 static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, const int32_t* woff,
                        const cc_body_desc* bodies, int32_t nb, const cc_shape_desc* shapes, int32_t ns,
@@ -462,7 +542,6 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     d.s.backfaces = s->collide_with_backfaces;
     d.s.activeEdges = s->collide_only_with_active_edges;
     // shapes
-    std::vector<ShapeD<T>> sh(std::max(ns, 1));
     for (int i = 0; i < ns; i++) {
         const cc_shape_desc& q = shapes[i];
         if (q.type < 0 || q.type > 6) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
@@ -473,74 +552,22 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                 (q.mesh_num_nodes > 0 && (!meshes->nodes || !meshes->triangles || !meshes->positions)))
                 return fail(c, CC_E_ARG, "mesh data out of range");
         }
-        ShapeD<T>& o = sh[i];
-        o.type = q.type; o.hfp = q.hull_first_point; o.hnp = q.hull_num_points; o.hff = q.hull_first_face;
-        o.hnf = q.hull_num_faces;
-        o.mfv = q.type == CC_SHAPE_BOX || q.type == CC_SHAPE_PLANE ? 4
-                : (q.type == CC_SHAPE_CAPSULE ? 2 : (q.type == CC_SHAPE_CYLINDER ? 8 : 0));
-        if (q.type == CC_SHAPE_TRIANGLE_MESH) {
-            o.hfp = q.mesh_first_triangle; o.hnp = q.mesh_num_triangles;
-            o.hff = q.mesh_first_node; o.hnf = q.mesh_num_nodes;
-            o.mfv = 3;
-        }
-        o.radius = T(q.radius); o.hh = T(q.half_height); o.cr = T(q.convex_radius);
-        for (int k = 0; k < 3; k++) {
-            o.he[k] = T(q.half_extents[k]); o.com[k] = T(q.center_of_mass[k]);
-            o.amin[k] = T(q.aabb_min[k]); o.amax[k] = T(q.aabb_max[k]);
-        }
-        // bounding radius, generous (any support point of either mode lies inside it)
-        double br = 0.0;
-        double he2 = q.half_extents[0] * q.half_extents[0] + q.half_extents[1] * q.half_extents[1] + q.half_extents[2] * q.half_extents[2];
-        double com = std::sqrt(q.center_of_mass[0] * q.center_of_mass[0] + q.center_of_mass[1] * q.center_of_mass[1] +
-                               q.center_of_mass[2] * q.center_of_mass[2]);
-        if (q.type == CC_SHAPE_SPHERE) br = q.radius;
-        else if (q.type == CC_SHAPE_BOX) br = std::sqrt(he2) + q.convex_radius;
-        else if (q.type == CC_SHAPE_CAPSULE) br = q.half_height + q.radius;
-        else if (q.type == CC_SHAPE_CYLINDER) br = std::sqrt(q.radius * q.radius + q.half_height * q.half_height);
-        else if (q.type == CC_SHAPE_PLANE) br = 1e30;
-        else if (hulls && hulls->points && hulls->shrunk_points) {
-            for (int k = 0; k < q.hull_num_points && q.hull_first_point + k < hulls->total_points; k++) {
-                const double* pp = hulls->points + 3 * (q.hull_first_point + k);
-                const double* ss = hulls->shrunk_points + 3 * (q.hull_first_point + k);
-                br = std::max(br, std::sqrt(pp[0] * pp[0] + pp[1] * pp[1] + pp[2] * pp[2]));
-                br = std::max(br, std::sqrt(ss[0] * ss[0] + ss[1] * ss[1] + ss[2] * ss[2]) + q.convex_radius);
-            }
-            br += com;
-        } else br = 1e30;
-        o.brad = T(br * (1.0 + 1e-5) + 1e-6);
     }
-    int P = hulls ? hulls->total_points : 0, F = hulls ? hulls->total_faces : 0, FV = hulls ? hulls->total_face_vertices : 0;
-    for (int i = 0; i < ns; i++) {  // a hull's supporting face takes at most 16 of a face's vertices
-        if (shapes[i].type != CC_SHAPE_CONVEX_HULL) continue;
-        int mfv = 0;
-        for (int f = shapes[i].hull_first_face; f < shapes[i].hull_first_face + shapes[i].hull_num_faces; f++) {
-            int nv = hulls->face_num_vertices[f];
-            int delta = (nv + 15) / 16;
-            mfv = std::max(mfv, delta > 0 ? (nv + delta - 1) / delta : 0);
-        }
-        sh[i].mfv = mfv;
+    {
+        const int rc = shape_table<T>(c, A, shapes, ns, hulls, d.st);
+        if (rc != CC_OK) return rc;
     }
-    std::vector<T> hp(3 * std::max(P, 1)), hs(3 * std::max(P, 1)), hn(3 * std::max(F, 1));
-    for (int i = 0; i < 3 * P; i++) { hp[i] = T(hulls->points[i]); hs[i] = T(hulls->shrunk_points[i]); }
-    for (int i = 0; i < 3 * F; i++) hn[i] = T(hulls->plane_normals[i]);
-    ShapeD<T>* dShapes = A.get<ShapeD<T>>(sh.size());
-    T* dPts = A.get<T>(hp.size());
-    T* dShr = A.get<T>(hs.size());
-    T* dPn = A.get<T>(hn.size());
-    int* dFfv = A.get<int>(std::max(F, 1));
-    int* dFnv = A.get<int>(std::max(F, 1));
-    int* dFvi = A.get<int>(std::max(FV, 1));
-    if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (shapes)");
-    HIPCHK(c, hipMemcpy(dShapes, sh.data(), sh.size() * sizeof(ShapeD<T>), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(dPts, hp.data(), hp.size() * sizeof(T), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(dShr, hs.data(), hs.size() * sizeof(T), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(dPn, hn.data(), hn.size() * sizeof(T), hipMemcpyHostToDevice));
-    if (F > 0) {
-        HIPCHK(c, hipMemcpy(dFfv, hulls->face_first_vertex, F * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(dFnv, hulls->face_num_vertices, F * sizeof(int), hipMemcpyHostToDevice));
+    // FP32 contexts run GJK and EPA in FP64 unless asked not to: FP64 copies of the shapes and
+    // tolerances (the EPA simplices are allocated with the other narrowphase buffers)
+    d.npWide = sizeof(T) == 4 && !c->cfg.narrowphase_fp32 ? 1 : 0;
+    d.spec64 = s->speculative_contact_distance;
+    d.colTol64 = s->collision_tolerance;
+    d.penTol64 = s->penetration_tolerance;
+    d.manifoldTol64 = s->manifold_tolerance;
+    if (d.npWide) {
+        const int rc = shape_table<double>(c, A, shapes, ns, hulls, d.st64);
+        if (rc != CC_OK) return rc;
     }
-    if (FV > 0) HIPCHK(c, hipMemcpy(dFvi, hulls->face_vertex_indices, FV * sizeof(int), hipMemcpyHostToDevice));
-    d.st = ShapeTable<T>{dShapes, dPts, dShr, dFfv, dFnv, dFvi, dPn, nullptr, nullptr, nullptr};
     // triangle meshes: vertices and triangles as given, BVH nodes decoded from the node buffer
     X->hasMesh = false;
     for (int i = 0; i < ns; i++) X->hasMesh = X->hasMesh || shapes[i].type == CC_SHAPE_TRIANGLE_MESH;
@@ -819,12 +846,14 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     d.hitCap = d.conCap + d.conCap / 8 + 1024;
     d.hitSlot = A.get<int>(d.hitCap);
     d.hitKind = A.get<int>(d.hitCap);
-    d.hitPA = A.get<V3<T>>(d.hitCap);
-    d.hitPB = A.get<V3<T>>(d.hitCap);
-    d.hitAxis = A.get<V3<T>>(d.hitCap);
+    d.hitPA = A.get<V3<double>>(d.hitCap);
+    d.hitPB = A.get<V3<double>>(d.hitCap);
+    d.hitAxis = A.get<V3<double>>(d.hitCap);
     d.epaPair = A.get<int>(d.epaCap);
     d.epaSimplex = A.get<Simplex<T>>(d.epaCap);
-    d.epaWork = A.get<char>((size_t)((d.epaWorkers + EPA_LANES - 1) / EPA_LANES) * EpaWave<T>::bytes);
+    if (d.npWide) d.epaSimplex64 = A.get<Simplex<double>>(d.epaCap);
+    d.epaWork = A.get<char>((size_t)((d.epaWorkers + EPA_LANES - 1) / EPA_LANES) *
+                            std::max(EpaWave<T>::bytes, EpaWave<double>::bytes));
     size_t L = (size_t)d.conCap + npc;
     d.skeys = A.get<unsigned long long>(2 * L);
     d.seq = A.get<int>(L);
@@ -1088,19 +1117,29 @@ static void enqueue_phase(CtxT<T>* X, hipStream_t st, int p, T dt, double margin
             hipLaunchKernelGGL(k_gjk_scatter<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
             break;
         case PH_GJK:
-            hipLaunchKernelGGL(k_np_gjk<T>, dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
+            if (d.npWide) hipLaunchKernelGGL((k_np_gjk<T, double>), dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
+            else hipLaunchKernelGGL((k_np_gjk<T, T>), dim3(grid_for(d.pairCap, 256, 8192)), dim3(256), 0, st, d, dt);
             break;
         case PH_EPA:
-            hipLaunchKernelGGL(k_epa<T>, dim3((d.epaWorkers + 63) / 64), dim3(64), 0, st, d, dt);
+            if (d.npWide) hipLaunchKernelGGL((k_epa<T, double>), dim3((d.epaWorkers + 63) / 64), dim3(64), 0, st, d, dt);
+            else hipLaunchKernelGGL((k_epa<T, T>), dim3((d.epaWorkers + 63) / 64), dim3(64), 0, st, d, dt);
             hipLaunchKernelGGL(k_count_epa<T>, dim3(1), dim3(64), 0, st, d);
             break;
         case PH_MANIFOLD:
-            hipLaunchKernelGGL((k_np_manifold<T, CC_SMALL_FACE>), dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
+            if (d.npWide)
+                hipLaunchKernelGGL((k_np_manifold<T, CC_SMALL_FACE, double>), dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
+            else
+                hipLaunchKernelGGL((k_np_manifold<T, CC_SMALL_FACE, T>), dim3(grid_for(d.hitCap, 256, 8192)), dim3(256), 0, st, d, dt);
             break;
         case PH_MANIFOLD_L:
-            hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
-            if (d.hasPlane)
-                hipLaunchKernelGGL(k_np_plane<T>, dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d, dt);
+            if (d.npWide)
+                hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE, double>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
+            else
+                hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE, T>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
+            if (d.hasPlane && d.npWide)
+                hipLaunchKernelGGL((k_np_plane<T, double>), dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d, dt);
+            else if (d.hasPlane)
+                hipLaunchKernelGGL((k_np_plane<T, T>), dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d, dt);
             if (d.hasMesh) {
                 hipLaunchKernelGGL(k_mesh_count<T>, dim3(grid_for(d.pairCap, 256, 4096)), dim3(256), 0, st, d);
                 scan_launch(st, d.meshCnt, d.meshOff, d.pairCap, X->scanSums);

@@ -716,6 +716,16 @@ def replicate(builder, copies, seed0, **kw):
     return sc
 
 
+def as_fp32(bodies):
+    """A copy of body descriptors or body states with the fields an FP32 context stores rounded to
+    FP32 (position, orientation, linear and angular velocity).  An FP64 oracle started from the
+    result and an FP32 context given the same array then hold one and the same state."""
+    out = np.array(bodies, copy=True)
+    for f in ("position", "quaternion", "linear_velocity", "angular_velocity"):
+        out[f] = out[f].astype(np.float32).astype(np.float64)
+    return out
+
+
 def replicate_world(scene, copies, seed, extent=(60.0, 40.0, 60.0)):
     """A batch scene of `copies` worlds that reuse world 0's bodies and shapes with positions,
     orientations and velocities redrawn per world (config-2 sampling: positions uniform in the

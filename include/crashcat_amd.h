@@ -59,7 +59,7 @@ extern "C" {
 #define CC_COMBINE_GEOMETRIC_MEAN 4
 
 /* ---- precision of a context ------------------------------------------- */
-#define CC_PRECISION_FP32 0 /* batched path: bulk FP32, FP64 in EPA faces and inertia */
+#define CC_PRECISION_FP32 0 /* batched path: bulk FP32; GJK, EPA and inertia in FP64     */
 #define CC_PRECISION_FP64 1 /* plain path: FP64 throughout                        */
 
 #define CC_MAX_OBJECT_LAYERS 8
@@ -329,6 +329,11 @@ typedef struct cc_config {
      * islands, a different Gauss-Seidel order judged by its residual (cc_get_solver_residual) */
     int32_t solver_order;
     int32_t measure_residual;   /* 1: every step measures its velocity residual (one extra pass) */
+    /* FP32 contexts: 0 (default) = GJK and EPA run in FP64 from the FP32 poses with FP64 shape
+     * data, so their discrete choices (EPA's last face, GJK's closest points between parallel
+     * features) are the FP64 path's; 1 = GJK and EPA in FP32 (EPA face normals, distances and
+     * barycentrics still FP64).  Ignored by FP64 contexts. */
+    int32_t narrowphase_fp32;
 } cc_config;
 
 #define CC_ORDER_REFERENCE 0
