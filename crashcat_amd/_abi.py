@@ -8,6 +8,9 @@ CC_SHAPE_CONVEX_HULL = 3
 CC_SHAPE_TRIANGLE_MESH = 4
 CC_SHAPE_CYLINDER = 5
 CC_SHAPE_PLANE = 6
+CC_DECO_SCALED = 1
+CC_DECO_TRANSFORMED = 2
+CC_DECO_OFFSET_COM = 4
 
 CC_MOTION_STATIC = 0
 CC_OK = 0
@@ -76,6 +79,11 @@ class ShapeDesc(C.Structure):
         ("mesh_num_triangles", C.c_int32),
         ("mesh_first_node", C.c_int32),
         ("mesh_num_nodes", C.c_int32),
+        ("decorated", C.c_int32),
+        ("pad2_", C.c_int32),
+        ("scale", D3),
+        ("local_position", D3),
+        ("local_quaternion", D4),
     ]
 
 

@@ -653,6 +653,19 @@ __device__ inline Simplex<C>* np_simplices(const Dev<T>& d) {
     else return d.epaSimplex64;
 }
 
+// The poses the two leaves of a pair are collided at, in type C (a transformed decorator
+// composes its body's pose with its local transform, leaf_pose).
+template <typename C, typename T>
+__device__ inline void leaf_poses(const ShapeTable<C>& st, const NpBody<T>& RA, const NpBody<T>& RB, V3<C>& pa,
+                                  Q4<C>& qa, V3<C>& pb, Q4<C>& qb) {
+    pa = cvt<C>(RA.pos);
+    qa = cvtq<C>(RA.q);
+    pb = cvt<C>(RB.pos);
+    qb = cvtq<C>(RB.q);
+    leaf_pose(st.shapes[RA.shape], pa, qa);
+    leaf_pose(st.shapes[RB.shape], pb, qb);
+}
+
 // normal velocity bias, src/constraints/contact-constraints.ts:288-423
 template <typename T>
 __device__ inline T normal_bias(const Dev<T>& d, int A, int Bi, int fa, int fb, const NpDyn<T>& DA, const NpDyn<T>& DB,
@@ -866,6 +879,7 @@ __device__ inline void finish_convex(const Dev<T>& d, int slot, int A, int Bi, c
     const ShapeTable<C>& stc = np_shapes<T, C>(d);
     V3<C> pa = cvt<C>(RA.pos);
     Q4<C> qa = cvtq<C>(RA.q);
+    leaf_pose(stc.shapes[RA.shape], pa, qa);
     C ms = c.maxSepUse;
     // penetration = |pA - pB| - maxSep; the early-out fraction is +inf for the step collector
     C al = sqrt(len2(pd.axis));
@@ -951,9 +965,13 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
             NpBody<T> RA, RB;
             np_pair(d, slot, RA, RB);
             const int ta = RA.tinfo & 15, tb = RB.tinfo & 15;
-            if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) {
-                V3<T> pa = RA.pos, pbp = RB.pos;
-                T ra = d.st.shapes[RA.shape].radius, rb = d.st.shapes[RB.shape].radius;
+            if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) {  // radii times |scale.x|, leaves at their poses
+                V3<T> pa, pbp;
+                Q4<T> qa, qb;
+                leaf_poses(d.st, RA, RB, pa, qa, pbp, qb);
+                const ShapeD<T>& SA = d.st.shapes[RA.shape];
+                const ShapeD<T>& SB = d.st.shapes[RB.shape];
+                T ra = SA.radius * fabs(shape_scale(SA).x), rb = SB.radius * fabs(shape_scale(SB).x);
                 V3<T> dd = pbp - pa;
                 T d2 = len2(dd), rs = ra + rb;
                 if (d2 < rs * rs) {
@@ -1038,8 +1056,11 @@ __global__ void __launch_bounds__(256, sizeof(C) == 4 ? 3 : 2) k_np_gjk(Dev<T> d
         const int slot = d.gjkSorted[k];
         NpBody<T> RA, RB;
         np_pair(d, slot, RA, RB);
+        V3<C> pa, pb;
+        Q4<C> qa, qb;
+        leaf_poses(stc, RA, RB, pa, qa, pb, qb);
         ConvexCtx<C> c;
-        convex_setup(c, cvt<C>(RA.pos), cvtq<C>(RA.q), cvt<C>(RB.pos), cvtq<C>(RB.q), tol.spec);
+        convex_setup(c, pa, qa, pb, qb, tol.spec);
         Sup<C> SA, SB;
         sup_for_shape(SA, stc, RA.shape, false);
         sup_for_shape(SB, stc, RB.shape, false);
@@ -1080,8 +1101,11 @@ __global__ void __launch_bounds__(64, 2) k_epa(Dev<T> d, T dt) {
         const bool sw = pb.a != i0;
         const NpBody<T>& RA = sw ? r1 : r0;
         const NpBody<T>& RB = sw ? r0 : r1;
+        V3<C> lpa, lpb;
+        Q4<C> lqa, lqb;
+        leaf_poses(stc, RA, RB, lpa, lqa, lpb, lqb);
         ConvexCtx<C> c;
-        convex_setup(c, cvt<C>(RA.pos), cvtq<C>(RA.q), cvt<C>(RB.pos), cvtq<C>(RB.q), tol.spec);
+        convex_setup(c, lpa, lqa, lpb, lqb, tol.spec);
         c.maxSepUse = tmin<C>(c.maxSepUse, C(1));
         Sup<C> SA, SB;
         sup_for_shape(SA, stc, RA.shape, true);
@@ -1130,8 +1154,11 @@ __global__ void __launch_bounds__(256) k_np_manifold(Dev<T> d, T dt) {
                         S.tmp, S.m, dt, par);
             continue;
         }
+        V3<C> lpa, lpb;
+        Q4<C> lqa, lqb;
+        leaf_poses(np_shapes<T, C>(d), RA, RB, lpa, lqa, lpb, lqb);
         ConvexCtx<C> c;
-        convex_setup(c, cvt<C>(RA.pos), cvtq<C>(RA.q), cvt<C>(RB.pos), cvtq<C>(RB.q), np_tol<T, C>(d).spec);
+        convex_setup(c, lpa, lqa, lpb, lqb, np_tol<T, C>(d).spec);
         if (kind == HIT_EPA) c.maxSepUse = tmin<C>(c.maxSepUse, C(1));
         PenDepth<C> pd;
         pd.status = 0;

@@ -490,7 +490,7 @@ __global__ void __launch_bounds__(64) k_mesh_collect(Dev<T> d, T dt) {
                 supporting_face(S.fA, d.st, P.cshape, -h.axis, P.pa, P.qa);
                 S.fB.n = 3;
                 S.fB.v[0] = a; S.fB.v[1] = b; S.fB.v[2] = c;
-                face_transform(S.fB, P.pa, P.qa);
+                face_transform(S.fB, P.pa, P.qa, mk3<T>(1, 1, 1));
             }
             if (!P.rev) mesh_add_hit(d, C, nm, P.RA.pos, wA, wB, wAxis, h.pen, 0xffffffffu, sub, S.fA, S.fB, S);
             else mesh_add_hit(d, C, nm, P.RA.pos, wB, wA, -wAxis, h.pen, sub, 0xffffffffu, S.fB, S.fA, S);

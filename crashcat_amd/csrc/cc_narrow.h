@@ -25,7 +25,30 @@ struct ShapeD {
     T radius, hh, cr;
     T he[3], com[3], amin[3], amax[3];
     T brad;  // radius of a sphere about the shape origin holding the shape (core + convex radius)
+    // decorators over the leaf (scaled.ts, transformed.ts, offset-center-of-mass.ts): deco holds
+    // the CC_DECO_* bits, sc the scale of the leaf, lp / lq the local transform
+    int deco;
+    T sc[3], lp[3], lq[4];
 };
+
+enum { DECO_SCALED = 1, DECO_TRANSFORMED = 2 };
+
+// The scale a shape's leaf is collided with (1 without a scaled decorator).
+template <typename T>
+__device__ inline V3<T> shape_scale(const ShapeD<T>& d) {
+    return (d.deco & DECO_SCALED) ? mk3<T>(d.sc[0], d.sc[1], d.sc[2]) : mk3<T>(1, 1, 1);
+}
+
+// The pose a decorated shape's leaf is collided at: a transformed decorator composes the body
+// pose with its own, p + rotate(q, t), q * tq (collideTransformedVsShape transformed.ts:370-420).
+template <typename T>
+__device__ inline void leaf_pose(const ShapeD<T>& d, V3<T>& p, Q4<T>& q) {
+    if (!(d.deco & DECO_TRANSFORMED)) return;
+    Q4<T> lq;
+    lq.x = d.lq[0]; lq.y = d.lq[1]; lq.z = d.lq[2]; lq.w = d.lq[3];
+    p = p + qrotate(q, mk3<T>(d.lp[0], d.lp[1], d.lp[2]));
+    q = qmul(q, lq);
+}
 
 template <typename T>
 struct ShapeTable {
@@ -52,52 +75,68 @@ struct Sup {
     V3<T> he;
     const T* pts;
     bool transformed, addRadius;
+    bool hscaled;  // a scaled hull's points, scaled as they are read
     T addR;
     M3<T> R;
     V3<T> t;
+    V3<T> hs;
 };
 
-// getShapeSupportFunction for unit scale (sphere.ts:246-256, support.ts:336-364,
-// capsule.ts:392-404, convex-hull.ts:979-996, cylinder.ts:406-419)
+// getShapeSupportFunction with the shape's scale (sphere.ts:195-263, uniform |scale.x|;
+// support.ts:336-368, box: |scale| per axis, the convex radius by the smallest |scale| clamped to
+// the default; capsule.ts:305-404 and cylinder.ts:323-419, uniform; convex-hull.ts:979-1010,
+// scaled points and the scaled shrunk points and radius).  A unit scale leaves every value as
+// the unscaled functions compute it.
 template <typename T>
 __device__ inline void sup_for_shape(Sup<T>& s, const ShapeTable<T>& st, int si, bool include) {
     const ShapeD<T>& d = st.shapes[si];
+    const V3<T> sc = shape_scale(d);
+    const T ax = fabs(sc.x), ay = fabs(sc.y), az = fabs(sc.z);
     s.transformed = false;
     s.addRadius = false;
+    s.hscaled = false;
     s.np = 0;
     s.pts = nullptr;
     s.r = s.hh = s.cr = T(0);
     s.he = mk3<T>(0, 0, 0);
     switch (d.type) {
         case SHAPE_SPHERE:
-            if (include) { s.kind = SK_SPHERE_R; s.r = d.radius; }
-            else { s.kind = SK_ZERO; s.cr = d.radius; }
+            if (include) { s.kind = SK_SPHERE_R; s.r = ax * d.radius; }
+            else { s.kind = SK_ZERO; s.cr = ax * d.radius; }
             break;
-        case SHAPE_BOX:
+        case SHAPE_BOX: {
             s.kind = SK_BOX;
+            const V3<T> he = mk3<T>(ax * d.he[0], ay * d.he[1], az * d.he[2]);
             if (!include) {
-                T cr = tmin<T>(d.cr, T(0.05));
-                s.he = mk3<T>(tmax<T>(T(0), d.he[0] - cr), tmax<T>(T(0), d.he[1] - cr), tmax<T>(T(0), d.he[2] - cr));
+                T cr = tmin<T>(d.cr * tmin<T>(ax, tmin<T>(ay, az)), T(0.05));
+                s.he = mk3<T>(tmax<T>(T(0), he.x - cr), tmax<T>(T(0), he.y - cr), tmax<T>(T(0), he.z - cr));
                 s.cr = cr;
             } else {
-                s.he = mk3<T>(d.he[0], d.he[1], d.he[2]);
+                s.he = he;
             }
             break;
+        }
         case SHAPE_CAPSULE:
-            s.hh = d.hh;
-            if (include) { s.kind = SK_CAP_R; s.r = d.radius; }
-            else { s.kind = SK_CAP_NC; s.cr = d.radius; }
+            s.hh = ax * d.hh;
+            if (include) { s.kind = SK_CAP_R; s.r = ax * d.radius; }
+            else { s.kind = SK_CAP_NC; s.cr = ax * d.radius; }
             break;
         case SHAPE_CYLINDER:  // setCylinderWithConvexSupport / setCylinderNoConvexSupport (cylinder.ts:323-397)
             s.kind = SK_CYL;
-            if (include) { s.hh = d.hh; s.r = d.radius; }
-            else { s.hh = d.hh - d.cr; s.r = d.radius - d.cr; s.cr = d.cr; }
+            if (include) { s.hh = ax * d.hh; s.r = ax * d.radius; }
+            else { const T cr = ax * d.cr; s.hh = ax * d.hh - cr; s.r = ax * d.radius - cr; s.cr = cr; }
             break;
         default:
             s.kind = SK_HULL;
             s.np = d.hnp;
-            if (include || d.cr == T(0)) { s.pts = st.pts + 3 * d.hfp; }
-            else { s.pts = st.shrunk + 3 * d.hfp; s.cr = d.cr; }
+            if (include || d.cr == T(0)) {
+                s.pts = st.pts + 3 * d.hfp;
+                s.hscaled = sc.x != T(1) || sc.y != T(1) || sc.z != T(1);
+                s.hs = sc;
+            } else {
+                s.pts = st.shrunk + 3 * d.hfp;
+                s.cr = d.cr * tmin<T>(ax, tmin<T>(ay, az));
+            }
             break;
     }
 }
@@ -133,6 +172,7 @@ __device__ inline V3<T> sup_base(const Sup<T>& s, V3<T> d) {
             V3<T> out = mk3<T>(0, 0, 0);
             for (int i = 0; i < s.np; i++) {
                 V3<T> p = mk3<T>(s.pts[3 * i], s.pts[3 * i + 1], s.pts[3 * i + 2]);
+                if (s.hscaled) p = mk3<T>(p.x * s.hs.x, p.y * s.hs.y, p.z * s.hs.z);
                 T dt = dot(p, d);
                 if (dt > best) { best = dt; out = p; }
             }
@@ -922,14 +962,17 @@ __device__ inline bool pd_step_epa(PenDepth<T>& out, const Sup<T>& A, const SB& 
 template <typename T, int N = CC_MAX_FACE> struct Face { int n; V3<T> v[N]; };
 
 template <typename T, int N>
-__device__ inline void face_transform(Face<T, N>& f, V3<T> pos, Q4<T> q) {  // transformFace face.ts:30-84
+__device__ inline void face_transform(Face<T, N>& f, V3<T> pos, Q4<T> q, V3<T> sc) {  // transformFace face.ts:30-84
     M3<T> R = m3_from_quat(q);
+    for (int k = 0; k < 3; k++) { R.m[k] *= sc.x; R.m[3 + k] *= sc.y; R.m[6 + k] *= sc.z; }
     for (int i = 0; i < f.n; i++) f.v[i] = mul(R, f.v[i]) + pos;
 }
 
 template <typename T, int N>
 __device__ inline void supporting_face(Face<T, N>& f, const ShapeTable<T>& st, int si, V3<T> d, V3<T> pos, Q4<T> q) {
     const ShapeD<T>& s = st.shapes[si];
+    const V3<T> sc = shape_scale(s);
+    const bool insideOut = sc.x * sc.y * sc.z < T(0);  // isScaleInsideOut face.ts:87-89
     if (s.type == SHAPE_SPHERE) { f.n = 0; return; }  // sphere.ts:143-146
     if (s.type == SHAPE_BOX) {  // box.ts:129-332
         T hx = s.he[0], hy = s.he[1], hz = s.he[2];
@@ -944,6 +987,10 @@ __device__ inline void supporting_face(Face<T, N>& f, const ShapeTable<T>& st, i
         } else {
             if (d.z < T(0)) { f.v[0] = mk3<T>(-hx, -hy, hz); f.v[1] = mk3<T>(hx, -hy, hz); f.v[2] = mk3<T>(hx, hy, hz); f.v[3] = mk3<T>(-hx, hy, hz); }
             else { f.v[0] = mk3<T>(-hx, -hy, -hz); f.v[1] = mk3<T>(-hx, hy, -hz); f.v[2] = mk3<T>(hx, hy, -hz); f.v[3] = mk3<T>(hx, -hy, -hz); }
+        }
+        if (insideOut) {  // the same face, listed in reverse
+            V3<T> t = f.v[0]; f.v[0] = f.v[3]; f.v[3] = t;
+            t = f.v[1]; f.v[1] = f.v[2]; f.v[2] = t;
         }
     } else if (s.type == SHAPE_CAPSULE) {  // capsule.ts:193-253
         T len = sqrt(d.x * d.x + d.z * d.z);
@@ -977,30 +1024,40 @@ __device__ inline void supporting_face(Face<T, N>& f, const ShapeTable<T>& st, i
             const T oz[8] = {T(1), c45, T(0), -c45, T(-1), -c45, T(0), c45};
             f.n = 8;
             for (int i = 0; i < 8; i++) {
-                const T px = sx * ox[i], py = sy * T(1), pz = sz * oz[i];
+                const int t = insideOut ? 7 - i : i;
+                const T px = sx * ox[t], py = sy * T(1), pz = sz * oz[t];
                 f.v[i] = mk3<T>(px * rc - pz * rs, py, px * rs + pz * rc);
             }
         }
-    } else {  // convex-hull.ts:479-548
+    } else {  // convex-hull.ts:479-548: normals through the inverse scale
+        const V3<T> inv = mk3<T>(T(1) / sc.x, T(1) / sc.y, T(1) / sc.z);
         int f0 = s.hff;
-        V3<T> n0 = mk3<T>(st.pn[3 * f0], st.pn[3 * f0 + 1], st.pn[3 * f0 + 2]);
+        V3<T> n0 = mk3<T>(inv.x * st.pn[3 * f0], inv.y * st.pn[3 * f0 + 1], inv.z * st.pn[3 * f0 + 2]);
         T bestDot = dot(n0, d) / vlen(n0);
         int best = 0;
         for (int i = 1; i < s.hnf; i++) {
-            V3<T> n = mk3<T>(st.pn[3 * (f0 + i)], st.pn[3 * (f0 + i) + 1], st.pn[3 * (f0 + i) + 2]);
+            V3<T> n = mk3<T>(inv.x * st.pn[3 * (f0 + i)], inv.y * st.pn[3 * (f0 + i) + 1], inv.z * st.pn[3 * (f0 + i) + 2]);
             T dt = dot(n, d) / vlen(n);
             if (dt < bestDot) { bestDot = dt; best = i; }
         }
         int first = st.ffv[f0 + best], nv = st.fnv[f0 + best];
         int delta = (nv + 16 - 1) / 16;
         f.n = 0;
-        for (int i = 0; i < nv && f.n < N; i += delta) {
-            int vi = st.fvi[first + i];
-            const T* pp = st.pts + 3 * (s.hfp + vi);
-            f.v[f.n++] = mk3<T>(pp[0], pp[1], pp[2]);
+        if (insideOut) {
+            for (int i = nv - 1; i >= 0 && f.n < N; i -= delta) {
+                int vi = st.fvi[first + i];
+                const T* pp = st.pts + 3 * (s.hfp + vi);
+                f.v[f.n++] = mk3<T>(pp[0], pp[1], pp[2]);
+            }
+        } else {
+            for (int i = 0; i < nv && f.n < N; i += delta) {
+                int vi = st.fvi[first + i];
+                const T* pp = st.pts + 3 * (s.hfp + vi);
+                f.v[f.n++] = mk3<T>(pp[0], pp[1], pp[2]);
+            }
         }
     }
-    face_transform(f, pos, q);
+    face_transform(f, pos, q, sc);
 }
 
 // ---------------------------------------------------------------- clipping (clip.ts)

@@ -448,6 +448,12 @@ static int shape_table(cc_ctx* c, Alloc& A, const cc_shape_desc* shapes, int32_t
             o.mfv = 3;
         }
         o.radius = U(q.radius); o.hh = U(q.half_height); o.cr = U(q.convex_radius);
+        o.deco = q.decorated;
+        for (int k = 0; k < 3; k++) {
+            o.sc[k] = q.decorated & CC_DECO_SCALED ? U(q.scale[k]) : U(1);
+            o.lp[k] = q.decorated & CC_DECO_TRANSFORMED ? U(q.local_position[k]) : U(0);
+        }
+        for (int k = 0; k < 4; k++) o.lq[k] = q.decorated & CC_DECO_TRANSFORMED ? U(q.local_quaternion[k]) : U(k == 3);
         for (int k = 0; k < 3; k++) {
             o.he[k] = U(q.half_extents[k]); o.com[k] = U(q.center_of_mass[k]);
             o.amin[k] = U(q.aabb_min[k]); o.amax[k] = U(q.aabb_max[k]);
@@ -471,6 +477,13 @@ static int shape_table(cc_ctx* c, Alloc& A, const cc_shape_desc* shapes, int32_t
             }
             br += com;
         } else br = 1e30;
+        if (q.decorated & CC_DECO_SCALED) {  // a bound of the scaled leaf (a scaled hull's shrunk points are scaled already)
+            const double ms = std::max(std::fabs(q.scale[0]), std::max(std::fabs(q.scale[1]), std::fabs(q.scale[2])));
+            br *= std::max(ms, 1.0);
+        }
+        if (q.decorated & CC_DECO_TRANSFORMED)  // about the body position, the leaf moved by its local translation
+            br += std::sqrt(q.local_position[0] * q.local_position[0] + q.local_position[1] * q.local_position[1] +
+                            q.local_position[2] * q.local_position[2]);
         o.brad = U(br * (1.0 + 1e-5) + 1e-6);
     }
     int P = hulls ? hulls->total_points : 0, F = hulls ? hulls->total_faces : 0, FV = hulls ? hulls->total_face_vertices : 0;
@@ -542,9 +555,20 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     d.s.backfaces = s->collide_with_backfaces;
     d.s.activeEdges = s->collide_only_with_active_edges;
     // shapes
+    bool anyDeco = false, anyMeshOrPlane = false;
     for (int i = 0; i < ns; i++) {
         const cc_shape_desc& q = shapes[i];
         if (q.type < 0 || q.type > 6) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
+        if (q.decorated & ~(CC_DECO_SCALED | CC_DECO_TRANSFORMED | CC_DECO_OFFSET_COM))
+            return fail(c, CC_E_ARG, "unknown decorator bits");
+        if (q.decorated) {
+            if (q.type == CC_SHAPE_TRIANGLE_MESH || q.type == CC_SHAPE_PLANE)
+                return fail(c, CC_E_UNSUPPORTED, "decorated triangle meshes and planes are not on the GPU path");
+            if ((q.decorated & CC_DECO_SCALED) && (q.scale[0] == 0 || q.scale[1] == 0 || q.scale[2] == 0))
+                return fail(c, CC_E_ARG, "a scale component is zero");
+            anyDeco = true;
+        }
+        anyMeshOrPlane = anyMeshOrPlane || q.type == CC_SHAPE_TRIANGLE_MESH || q.type == CC_SHAPE_PLANE;
         if (q.type == CC_SHAPE_TRIANGLE_MESH) {
             const int MT = meshes ? meshes->total_triangles : 0, MN = meshes ? meshes->total_nodes : 0;
             if (q.mesh_first_triangle < 0 || q.mesh_num_triangles < 0 || q.mesh_first_node < 0 || q.mesh_num_nodes < 0 ||
@@ -553,6 +577,8 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                 return fail(c, CC_E_ARG, "mesh data out of range");
         }
     }
+    if (anyDeco && anyMeshOrPlane)
+        return fail(c, CC_E_UNSUPPORTED, "decorated shapes in a scene with a triangle mesh or a plane are not on the GPU path");
     {
         const int rc = shape_table<T>(c, A, shapes, ns, hulls, d.st);
         if (rc != CC_OK) return rc;

@@ -119,6 +119,8 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_shape_desc, aabb_min, "f64", 3); F(cc_shape_desc, aabb_max, "f64", 3);
     F(cc_shape_desc, mesh_first_triangle, "i32", 1); F(cc_shape_desc, mesh_num_triangles, "i32", 1);
     F(cc_shape_desc, mesh_first_node, "i32", 1); F(cc_shape_desc, mesh_num_nodes, "i32", 1);
+    F(cc_shape_desc, decorated, "i32", 1); F(cc_shape_desc, scale, "f64", 3);
+    F(cc_shape_desc, local_position, "f64", 3); F(cc_shape_desc, local_quaternion, "f64", 4);
     add_struct(env, out, "cc_shape_desc", sizeof(cc_shape_desc), fields);
     napi_create_object(env, &fields);
     F(cc_body_desc, shape, "i32", 1); F(cc_body_desc, motion_type, "i32", 1);

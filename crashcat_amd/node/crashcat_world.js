@@ -147,7 +147,8 @@ function shapeTables(pool) {
     const d = { type: t, radius: 0, half_height: 0, half_extents: [0, 0, 0], convex_radius: 0,
                 center_of_mass: vec(sh.centerOfMass, 3), aabb_min: aabb.min, aabb_max: aabb.max,
                 hull_first_point: 0, hull_num_points: 0, hull_first_face: 0, hull_num_faces: 0,
-                mesh_first_triangle: 0, mesh_num_triangles: 0, mesh_first_node: 0, mesh_num_nodes: 0 };
+                mesh_first_triangle: 0, mesh_num_triangles: 0, mesh_first_node: 0, mesh_num_nodes: 0,
+                decorated: 0, scale: [1, 1, 1], local_position: [0, 0, 0], local_quaternion: [0, 0, 0, 1] };
     if (sh.type === 0) {
       d.radius = sh.radius;
     } else if (sh.type === 1) {

@@ -211,6 +211,10 @@ class Scene:
             d.center_of_mass[:] = s.center_of_mass
             d.aabb_min[:] = s.aabb_min
             d.aabb_max[:] = s.aabb_max
+            d.decorated = s.decorated
+            d.scale[:] = s.scale
+            d.local_position[:] = s.local_position
+            d.local_quaternion[:] = s.local_quaternion
             if s.type == S.CONVEX_HULL:
                 d.hull_first_point = np_
                 d.hull_num_points = len(s.points)
@@ -579,6 +583,65 @@ def mixed_pile(seed=1002, n=20000, extent=(60.0, 40.0, 60.0), scene=None, cylind
         v = (rng.normal(0, 1), rng.normal(0, 1), rng.normal(0, 1))
         w = (rng.normal(0, 0.5), rng.normal(0, 0.5), rng.normal(0, 0.5))
         sc.add_body(shape, (x, y, z), q, linear_velocity=v, angular_velocity=w)
+    sc.end_world()
+    return sc
+
+
+def decorated_pile(seed=1009, n=300, extent=(12.0, 8.0, 12.0), scene=None):
+    """A pile of decorated shapes over a static box floor, drawn uniformly from: scaled boxes
+    (per-axis scale in [0.5, 2], each axis mirrored with probability 1/4, so some are inside out),
+    uniformly scaled spheres, capsules and cylinders, scaled 16-point hulls (non-uniform), boxes,
+    capsules, cylinders and hulls in a transformed decorator (offset up to 0.3 m, random rotation),
+    and offset-centre-of-mass boxes and spheres; positions uniform in the extent above the floor,
+    v ~ N(0, 1), w ~ N(0, 0.5)."""
+    sc = scene if scene is not None else Scene()
+    rng = Rng(seed)
+    half = (max(50.0, extent[0] / 2 + 5), 0.5, max(50.0, extent[2] / 2 + 5))
+    _floor(sc, half=half)
+
+    def sgn():
+        return -1.0 if rng.uniform() < 0.25 else 1.0
+
+    for _ in range(n):
+        kind = min(9, int(rng.uniform() * 10))
+        if kind == 0:
+            shape = S.scaled(S.box((rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5))),
+                             (sgn() * rng.uniform(0.5, 2.0), sgn() * rng.uniform(0.5, 2.0), sgn() * rng.uniform(0.5, 2.0)))
+        elif kind == 1:
+            u = sgn() * rng.uniform(0.5, 2.0)
+            shape = S.scaled(S.sphere(rng.uniform(0.2, 0.4)), (u, u, u))
+        elif kind == 2:
+            u = rng.uniform(0.5, 2.0)
+            shape = S.scaled(S.capsule(rng.uniform(0.1, 0.25), rng.uniform(0.1, 0.3)), (u, u, u))
+        elif kind == 3:
+            u = rng.uniform(0.7, 1.5)
+            shape = S.scaled(S.cylinder(rng.uniform(0.2, 0.4), rng.uniform(0.2, 0.4)), (u, u, u))
+        elif kind == 4:
+            pts = _unit_sphere_points(rng, 16) * rng.uniform(0.3, 0.6)
+            shape = S.scaled(S.convex_hull(pts), (rng.uniform(0.6, 1.6), sgn() * rng.uniform(0.6, 1.6),
+                                                  rng.uniform(0.6, 1.6)))
+        elif kind in (5, 6, 7):
+            inner = [S.box((rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5))),
+                     S.capsule(rng.uniform(0.15, 0.3), rng.uniform(0.2, 0.5)),
+                     S.cylinder(rng.uniform(0.2, 0.4), rng.uniform(0.2, 0.4))][kind - 5]
+            if rng.uniform() < 0.5:
+                inner = S.scaled(inner, (1.5, 1.5, 1.5))
+            t = (rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3))
+            shape = S.transformed(inner, t, rng.quaternion())
+        elif kind == 8:
+            pts = _unit_sphere_points(rng, 16) * rng.uniform(0.3, 0.6)
+            shape = S.transformed(S.convex_hull(pts), (0.0, rng.uniform(-0.2, 0.2), 0.0), rng.quaternion())
+        else:
+            inner = S.box((rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5))) \
+                if rng.uniform() < 0.5 else S.sphere(rng.uniform(0.2, 0.4))
+            shape = S.offset_center_of_mass(inner, (rng.uniform(-0.1, 0.1), rng.uniform(-0.2, 0.0),
+                                                    rng.uniform(-0.1, 0.1)))
+        x = rng.uniform(-extent[0] / 2, extent[0] / 2)
+        y = rng.uniform(1.0, 1.0 + extent[1])
+        z = rng.uniform(-extent[2] / 2, extent[2] / 2)
+        v = (rng.normal(0, 1), rng.normal(0, 1), rng.normal(0, 1))
+        w = (rng.normal(0, 0.5), rng.normal(0, 0.5), rng.normal(0, 0.5))
+        sc.add_body(shape, (x, y, z), rng.quaternion(), linear_velocity=v, angular_velocity=w)
     sc.end_world()
     return sc
 

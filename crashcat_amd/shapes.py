@@ -62,10 +62,16 @@ class Shape:
         self.mesh_positions = kw.get("mesh_positions")
         self.mesh_triangles = kw.get("mesh_triangles")
         self.mesh_nodes = kw.get("mesh_nodes")
+        # decorators flattened onto this leaf (scaled / transformed / offset centre of mass)
+        self.decorated = kw.get("decorated", 0)  # CC_DECO_* bits
+        self.scale = tuple(kw.get("scale", (1.0, 1.0, 1.0)))
+        self.local_position = tuple(kw.get("local_position", (0.0, 0.0, 0.0)))
+        self.local_quaternion = tuple(kw.get("local_quaternion", (0.0, 0.0, 0.0, 1.0)))
+        self.transformed = kw.get("transformed", False)  # a transformed decorator is in the chain
 
     def key(self):
         """Hashable identity used to deduplicate identical shapes at upload."""
-        if self.type in (CONVEX_HULL, TRIANGLE_MESH):
+        if self.type in (CONVEX_HULL, TRIANGLE_MESH) or self.decorated:
             return (self.type, id(self))
         return (self.type, self.radius, self.half_height, self.half_extents, self.convex_radius, self.density)
 
@@ -387,6 +393,155 @@ def convex_hull(positions, convex_radius=DEFAULT_CONVEX_RADIUS, density=DEFAULT_
                  aabb_min=tuple(mn), aabb_max=tuple(mx), mass=volume * density, inertia=inertia * density,
                  points=points, shrunk_points=shrunk, faces=faces, plane_normals=normals, plane_constants=consts,
                  volume=volume, point_faces=pfaces)
+
+
+# ------------------------------------------------------------------ decorators
+# scaled.ts, transformed.ts and offset-center-of-mass.ts wrap a shape; in the narrowphase a
+# decorator only changes the pose (transformed: p + rotate(q, t), q * tq) or the scale (scaled:
+# the product of the chain's scales, applied to the leaf's support functions and faces) the
+# leaf is collided with, and consumes no sub-shape id bits.  So a chain of decorators over a
+# convex leaf is kept as that leaf with one local transform and one scale, and the decorated
+# shape's own centre of mass, bounds and mass properties.
+
+def _qrotate(q, v):
+    """vec3.transformQuat (gl-matrix form, as mathcat@0.0.10 computes it)."""
+    qx, qy, qz, qw = q
+    x, y, z = v
+    uvx, uvy, uvz = qy * z - qz * y, qz * x - qx * z, qx * y - qy * x
+    uuvx, uuvy, uuvz = qy * uvz - qz * uvy, qz * uvx - qx * uvz, qx * uvy - qy * uvx
+    w2 = qw * 2
+    return (x + uvx * w2 + uuvx * 2, y + uvy * w2 + uuvy * 2, z + uvz * w2 + uuvz * 2)
+
+
+def _mat3_from_quat(q):
+    """mat4.fromQuat (the 3x3 part), as rows."""
+    x, y, z, w = q
+    x2, y2, z2 = x + x, y + y, z + z
+    xx, yx, yy = x * x2, y * x2, y * y2
+    zx, zy, zz = z * x2, z * y2, z * z2
+    wx, wy, wz = w * x2, w * y2, w * z2
+    return np.array([[1 - yy - zz, yx - wz, zx + wy], [yx + wz, 1 - xx - zz, zy - wx], [zx - wy, zy + wx, 1 - xx - yy]])
+
+
+def _leaf_copy(shape, **over):
+    kw = dict(radius=shape.radius, half_height=shape.half_height, half_extents=shape.half_extents,
+              convex_radius=shape.convex_radius, density=shape.density, center_of_mass=shape.center_of_mass,
+              aabb_min=shape.aabb_min, aabb_max=shape.aabb_max, mass=shape.mass, inertia=shape.inertia,
+              points=shape.points, shrunk_points=shape.shrunk_points, faces=shape.faces,
+              plane_normals=shape.plane_normals, plane_constants=shape.plane_constants, volume=shape.volume,
+              point_faces=shape.point_faces, decorated=shape.decorated, scale=shape.scale,
+              local_position=shape.local_position, local_quaternion=shape.local_quaternion,
+              transformed=shape.transformed)
+    kw.update(over)
+    return Shape(shape.type, **kw)
+
+
+def _check_leaf(shape):
+    if shape.type in (TRIANGLE_MESH, PLANE):
+        raise ValueError("decorated shapes are supported over convex leaves (sphere, box, capsule, hull, cylinder)")
+
+
+def scaled(shape, scale):
+    """scaled.create({shape, scale}) — src/shapes/scaled.ts:39-56: bounds scaled by |scale|
+    (:66-79), mass properties by massProperties.scale (src/body/mass-properties.ts:59-114), and a
+    centre of mass of zero: computeScaledCenterOfMass (:81-85) scales the scaled shape's own
+    initial (zero) centre of mass, not the inner shape's.  A convex hull gets the shrunk support
+    points of the scaled hull (ConvexHullNoConvexSupportScaled, convex-hull.ts:842-935)."""
+    _check_leaf(shape)
+    sc = tuple(float(v) for v in scale)
+    if 0.0 in sc:
+        raise ValueError("scale components must be non-zero")
+    tot = tuple(a * b for a, b in zip(shape.scale, sc))
+    a = [abs(v) for v in sc]
+    mn = tuple(shape.aabb_min[k] * a[k] for k in range(3))
+    mx = tuple(shape.aabb_max[k] * a[k] for k in range(3))
+    # massProperties.scale
+    I = np.asarray(shape.inertia, dtype=np.float64)
+    dg = [I[0, 0], I[1, 1], I[2, 2]]
+    half = 0.5 * (dg[0] + dg[1] + dg[2])
+    xyz = [half - dg[0], half - dg[1], half - dg[2]]
+    sq = [sc[k] * sc[k] * xyz[k] for k in range(3)]
+    J = np.array(I)
+    J[0, 0], J[1, 1], J[2, 2] = sq[1] + sq[2], sq[0] + sq[2], sq[0] + sq[1]
+    J[0, 1] = J[1, 0] = sc[0] * sc[1] * I[0, 1]
+    J[0, 2] = J[2, 0] = sc[0] * sc[2] * I[0, 2]
+    J[1, 2] = J[2, 1] = sc[1] * sc[2] * I[1, 2]
+    ms = abs(sc[0] * sc[1] * sc[2])
+    J = J * ms
+    over = dict(scale=tot, aabb_min=mn, aabb_max=mx, mass=shape.mass * ms, inertia=J, center_of_mass=(0.0, 0.0, 0.0),
+                volume=shape.volume * abs(sc[0]) * abs(sc[1]) * abs(sc[2]),
+                decorated=shape.decorated | _abi.CC_DECO_SCALED)
+    if shape.type == CONVEX_HULL:
+        over["shrunk_points"] = scaled_shrunk_points(shape.points, shape.plane_normals, shape.point_faces,
+                                                      shape.convex_radius, tot)
+    return _leaf_copy(shape, **over)
+
+
+def transformed(shape, position, quaternion):
+    """transformedShape.create({shape, position, quaternion}) — src/shapes/transformed.ts:42-115:
+    bounds of the inner bounds' eight transformed corners, centre of mass t + rotate(q, com),
+    inertia rotated (massProperties.rotate, src/body/mass-properties.ts:121-161).  One transformed
+    decorator per chain (the narrowphase composes the body pose with it once)."""
+    _check_leaf(shape)
+    if shape.transformed:
+        raise ValueError("nested transformed shapes are not supported")
+    t = tuple(float(v) for v in position)
+    q = tuple(float(v) for v in quaternion)
+    lo, hi = shape.aabb_min, shape.aabb_max
+    corners = [_qrotate(q, (lo[0] if x == 0 else hi[0], lo[1] if y == 0 else hi[1], lo[2] if z == 0 else hi[2]))
+               for x in range(2) for y in range(2) for z in range(2)]
+    corners = [(c[0] + t[0], c[1] + t[1], c[2] + t[2]) for c in corners]
+    mn = tuple(min(c[k] for c in corners) for k in range(3))
+    mx = tuple(max(c[k] for c in corners) for k in range(3))
+    r = _qrotate(q, shape.center_of_mass)
+    com = (r[0] + t[0], r[1] + t[1], r[2] + t[2])
+    R = _mat3_from_quat(q)
+    inertia = (R @ np.asarray(shape.inertia, dtype=np.float64)) @ R.T
+    return _leaf_copy(shape, local_position=t, local_quaternion=q, aabb_min=mn, aabb_max=mx, center_of_mass=com,
+                      inertia=inertia, transformed=True, decorated=shape.decorated | _abi.CC_DECO_TRANSFORMED)
+
+
+def offset_center_of_mass(shape, offset):
+    """offsetCenterOfMass.create({shape, offset}) — src/shapes/offset-center-of-mass.ts:50-83:
+    the collision geometry is the inner shape's, the centre of mass moves by `offset`, and the
+    inertia moves with it (massProperties.translate, src/body/mass-properties.ts:168-211)."""
+    _check_leaf(shape)
+    o = np.array([float(v) for v in offset])
+    com = tuple(float(shape.center_of_mass[k] + o[k]) for k in range(3))
+    I = np.asarray(shape.inertia, dtype=np.float64) + shape.mass * (np.eye(3) * float(o @ o) - np.outer(o, o))
+    return _leaf_copy(shape, center_of_mass=com, inertia=I, decorated=shape.decorated | _abi.CC_DECO_OFFSET_COM)
+
+
+def scaled_shrunk_points(points, normals, point_faces, cr, scale):
+    """The shrunk support points of a scaled hull (setConvexHullNoConvexSupportScaled,
+    src/shapes/convex-hull.ts:842-935): each point scaled, its supporting planes' normals
+    multiplied by 1 / scale and normalised, the radius scaled by the smallest |scale| component,
+    then the planes' intersection moved in by that radius (the third plane, for two faces,
+    through the point at right angles to both), or the one-face form."""
+    sc = np.asarray(scale, dtype=np.float64)
+    inv = np.array([1.0 / sc[0], 1.0 / sc[1], 1.0 / sc[2]])
+    r = cr * min(abs(sc[0]), abs(sc[1]), abs(sc[2]))
+    pts = np.asarray(points, dtype=np.float64) * sc
+    nn = np.asarray(normals, dtype=np.float64) * inv
+
+    def unit(v):  # vec3.normalize: v * (1 / sqrt(len2)) when len2 > 0
+        l2 = _dot(v, v)
+        f = np.divide(1.0, np.sqrt(l2), out=np.zeros_like(l2), where=l2 > 0)
+        return v * f[:, None]
+
+    n_f = np.array([len(pf) for pf in point_faces])
+    c1 = unit(nn[[pf[0] for pf in point_faces]])
+    c2 = unit(nn[[pf[1] if len(pf) > 1 else pf[0] for pf in point_faces]])
+    perp = unit(_cross(c1, c2))
+    c3 = np.where((n_f == 3)[:, None], unit(nn[[pf[2] if len(pf) > 2 else pf[0] for pf in point_faces]]), perp)
+    w1 = -_dot(c1, pts) + r
+    w2 = -_dot(c2, pts) + r
+    w3 = -_dot(c3, pts) + np.where(n_f == 3, r, 0.0)
+    c23, c31, c12 = _cross(c2, c3), _cross(c3, c1), _cross(c1, c2)
+    den = _dot(c1, c23)
+    inter = -((w1[:, None] * c23 + w2[:, None] * c31) + w3[:, None] * c12) / np.where(den == 0, 1.0, den)[:, None]
+    fallback = pts + c1 * (-r)
+    return np.where(((n_f == 1) | (den == 0))[:, None], fallback, inter)
 
 
 # ------------------------------------------------------------------ mass -> motion

@@ -96,6 +96,10 @@ typedef struct cc_settings {
  * records (src/shapes/sphere.ts:40-56, box.ts:42-52, capsule.ts:30-48, convex-hull.ts:37-62,
  * triangle-mesh.ts:88-95).  A triangle mesh's aabb is its BVH root's bounds and its centre
  * of mass is the origin (triangle-mesh.ts:114-147). */
+#define CC_DECO_SCALED 1      /* the chain holds a scaled decorator: `scale` applies          */
+#define CC_DECO_TRANSFORMED 2 /* ... a transformed decorator: the local transform applies     */
+#define CC_DECO_OFFSET_COM 4  /* ... an offset centre of mass (already in center_of_mass)     */
+
 typedef struct cc_shape_desc {
     int32_t type;               /* CC_SHAPE_* */
     int32_t hull_first_point;   /* hulls: offset into cc_hull_data.points / shrunk_points */
@@ -114,6 +118,22 @@ typedef struct cc_shape_desc {
     int32_t mesh_num_triangles;
     int32_t mesh_first_node;     /* meshes: offset into cc_mesh_data.nodes (in nodes)          */
     int32_t mesh_num_nodes;
+    /* Decorators flattened onto the leaf shape above (scaled.ts, transformed.ts,
+     * offset-center-of-mass.ts; a decorator consumes no sub-shape id bits).  The leaf is collided
+     * at the body pose composed with the local transform, p + rotate(q, local_position) and
+     * q * local_quaternion (collideTransformedVsShape transformed.ts:370-420), with `scale`
+     * applied to its support functions and supporting faces (sphere, capsule and cylinder use
+     * |scale[0]|, scaled.ts:23-28; the product of nested scales).  The leaf's own fields above
+     * stay unscaled, except that a scaled convex hull's shrunk_points are the scaled ones
+     * (setConvexHullNoConvexSupportScaled convex-hull.ts:842-935).  center_of_mass, aabb_* and the
+     * body's mass properties are the decorated shape's (an offset centre of mass moves only
+     * those).  decorated: CC_DECO_* bits, 0 = scale 1 and no local transform.  Convex leaves
+     * only, and not in a scene with a triangle mesh or a plane. */
+    int32_t decorated;
+    int32_t pad2_;
+    double scale[3];
+    double local_position[3];
+    double local_quaternion[4];
 } cc_shape_desc;
 
 /* Flattened convex-hull data shared by all hull shapes.  Points are relative to the

@@ -28,44 +28,60 @@ typedef struct {
     V3 t;
     int addRadius;
     double addR;
+    int hscaled; /* a scaled hull's points, scaled as they are read (convexHullWithConvexScaledGetSupport) */
+    V3 hs;
 } OSup;
 
 #define MODE_INCLUDE 0
 #define MODE_EXCLUDE 1
 
-/* getShapeSupportFunction for unit scale: sphere.ts:246-256, support.ts:336-364 (box),
- * capsule.ts:392-404, convex-hull.ts:979-996, cylinder.ts:406-419 */
+/* The scale a decorated shape's leaf is collided with (scaled.ts:296-330 accumulates it down the
+ * chain); 1 for a shape without a scaled decorator. */
+static V3 shape_scale(const cc_shape_desc* d) {
+    return (d->decorated & CC_DECO_SCALED) ? vload(d->scale) : v3(1, 1, 1);
+}
+
+/* getShapeSupportFunction with the shape's scale: sphere.ts:195-263 (uniform, |scale.x|),
+ * support.ts:336-368 (box: |scale| per axis, convex radius by the smallest |scale| clamped to
+ * the default), capsule.ts:305-404 (uniform), convex-hull.ts:979-1010 (scaled points, the
+ * scaled shrunk points and radius), cylinder.ts:323-419 (uniform) */
 static void sup_for_shape(OSup* s, const OShapes* sh, int si, int mode) {
     const cc_shape_desc* d = &sh->shapes[si];
     memset(s, 0, sizeof(*s));
+    const V3 sc = shape_scale(d);
+    const double ax = fabs(sc.x), ay = fabs(sc.y), az = fabs(sc.z);
+    const double minScale = dmin(ax, dmin(ay, az));
     switch (d->type) {
     case CC_SHAPE_SPHERE:
-        if (mode == MODE_INCLUDE) { s->kind = SK_SPHERE_R; s->r = d->radius; s->convexRadius = 0; }
-        else { s->kind = SK_ZERO; s->convexRadius = d->radius; }
+        if (mode == MODE_INCLUDE) { s->kind = SK_SPHERE_R; s->r = ax * d->radius; s->convexRadius = 0; }
+        else { s->kind = SK_ZERO; s->convexRadius = ax * d->radius; }
         break;
-    case CC_SHAPE_BOX:
+    case CC_SHAPE_BOX: {
         s->kind = SK_BOX;
+        const V3 he = v3(ax * d->half_extents[0], ay * d->half_extents[1], az * d->half_extents[2]);
         if (mode == MODE_EXCLUDE) {
-            double cr = dmin(d->convex_radius * 1.0, 0.05); /* min(r * minScale, DEFAULT_CONVEX_RADIUS) */
-            s->he = v3(dmax(0, d->half_extents[0] - cr), dmax(0, d->half_extents[1] - cr), dmax(0, d->half_extents[2] - cr));
+            double cr = dmin(d->convex_radius * minScale, 0.05); /* clamped to DEFAULT_CONVEX_RADIUS */
+            s->he = v3(dmax(0, he.x - cr), dmax(0, he.y - cr), dmax(0, he.z - cr));
             s->convexRadius = cr;
         } else {
-            s->he = vload(d->half_extents);
+            s->he = he;
             s->convexRadius = 0;
         }
         break;
+    }
     case CC_SHAPE_CAPSULE:
-        s->hh = d->half_height;
-        if (mode == MODE_INCLUDE) { s->kind = SK_CAP_R; s->r = d->radius; s->convexRadius = 0; }
-        else { s->kind = SK_CAP_NC; s->convexRadius = d->radius; }
+        s->hh = ax * d->half_height;
+        if (mode == MODE_INCLUDE) { s->kind = SK_CAP_R; s->r = ax * d->radius; s->convexRadius = 0; }
+        else { s->kind = SK_CAP_NC; s->convexRadius = ax * d->radius; }
         break;
     case CC_SHAPE_CYLINDER: /* setCylinderWithConvexSupport :378-397, setCylinderNoConvexSupport :323-340 */
         s->kind = SK_CYL;
-        if (mode == MODE_INCLUDE) { s->hh = d->half_height; s->r = d->radius; s->convexRadius = 0; }
+        if (mode == MODE_INCLUDE) { s->hh = ax * d->half_height; s->r = ax * d->radius; s->convexRadius = 0; }
         else {
-            s->hh = d->half_height - d->convex_radius;
-            s->r = d->radius - d->convex_radius;
-            s->convexRadius = d->convex_radius;
+            const double cr = ax * d->convex_radius;
+            s->hh = ax * d->half_height - cr;
+            s->r = ax * d->radius - cr;
+            s->convexRadius = cr;
         }
         break;
     case CC_SHAPE_CONVEX_HULL:
@@ -74,9 +90,11 @@ static void sup_for_shape(OSup* s, const OShapes* sh, int si, int mode) {
         if (mode == MODE_INCLUDE || d->convex_radius == 0) {
             s->pts = sh->hulls->points + 3 * d->hull_first_point;
             s->convexRadius = 0;
+            s->hscaled = sc.x != 1 || sc.y != 1 || sc.z != 1;
+            s->hs = sc;
         } else {
             s->pts = sh->hulls->shrunk_points + 3 * d->hull_first_point;
-            s->convexRadius = d->convex_radius;
+            s->convexRadius = d->convex_radius * minScale;
         }
         break;
     default:
@@ -112,11 +130,12 @@ static V3 sup_base(const OSup* s, V3 d) {
         }
         return v3(0, s->hh, 0);
     }
-    case SK_HULL: { /* convex-hull.ts:588-608, 698-716 */
+    case SK_HULL: { /* convex-hull.ts:588-608, 639-658, 698-716 */
         double best = -INFINITY;
         V3 out = v3(0, 0, 0);
         for (int i = 0; i < s->np; i++) {
             V3 p = vload(s->pts + 3 * i);
+            if (s->hscaled) p = v3(p.x * s->hs.x, p.y * s->hs.y, p.z * s->hs.z);
             double dt = vdot(p, d);
             if (dt > best) { best = dt; out = p; }
         }
@@ -850,13 +869,22 @@ static int pd_step_epa(PenDepth* out, const OSup* A, const OSup* B, double tol, 
 /* ======================================================================== */
 /* supporting faces                                                          */
 /* ======================================================================== */
-/* transformFace (src/utils/face.ts:30-84) for unit scale */
-static void face_transform(OFace* f, V3 pos, Q4 q) {
+/* transformFace (src/utils/face.ts:30-84): rotation times scale, then the translation */
+static void face_transform(OFace* f, V3 pos, Q4 q, V3 sc) {
     M3 R = m3_from_quat(q);
+    for (int k = 0; k < 3; k++) { R.m[k] *= sc.x; R.m[3 + k] *= sc.y; R.m[6 + k] *= sc.z; }
     for (int i = 0; i < f->n; i++) f->v[i] = vadd(m3_mul(&R, f->v[i]), pos);
 }
 
-static void face_box(OFace* f, V3 d, V3 he) { /* box.ts:129-332, insideOut = false */
+/* isScaleInsideOut (src/utils/face.ts:87-89) */
+static int inside_out(V3 sc) { return sc.x * sc.y * sc.z < 0; }
+
+/* the vertices of a face in the opposite order */
+static void face_reverse(OFace* f) {
+    for (int i = 0, j = f->n - 1; i < j; i++, j--) { V3 t = f->v[i]; f->v[i] = f->v[j]; f->v[j] = t; }
+}
+
+static void face_box(OFace* f, V3 d, V3 he) { /* box.ts:129-332; an inside-out scale lists the same face reversed */
     double hx = he.x, hy = he.y, hz = he.z;
     double ax = fabs(d.x), ay = fabs(d.y), az = fabs(d.z);
     f->n = 4;
@@ -885,12 +913,16 @@ static void face_capsule(OFace* f, V3 d, double hh, double r) { /* capsule.ts:19
 
 static void face_hull(OFace* f, const OShapes* sh, const cc_shape_desc* s, V3 d) { /* convex-hull.ts:479-548 */
     const cc_hull_data* H = sh->hulls;
+    const V3 sc = shape_scale(s);
+    const V3 inv = v3(1 / sc.x, 1 / sc.y, 1 / sc.z); /* normals transform by the inverse scale */
     int f0 = s->hull_first_face;
     V3 n0 = vload(H->plane_normals + 3 * f0);
+    n0 = v3(inv.x * n0.x, inv.y * n0.y, inv.z * n0.z);
     double bestDot = vdot(n0, d) / vlen(n0);
     int best = 0;
     for (int i = 1; i < s->hull_num_faces; i++) {
         V3 n = vload(H->plane_normals + 3 * (f0 + i));
+        n = v3(inv.x * n.x, inv.y * n.y, inv.z * n.z);
         double dt = vdot(n, d) / vlen(n);
         if (dt < bestDot) { bestDot = dt; best = i; }
     }
@@ -898,9 +930,16 @@ static void face_hull(OFace* f, const OShapes* sh, const cc_shape_desc* s, V3 d)
     int maxv = 32 / 2;
     int delta = (nv + maxv - 1) / maxv;
     f->n = 0;
-    for (int i = 0; i < nv; i += delta) {
-        int vi = H->face_vertex_indices[first + i];
-        f->v[f->n++] = vload(H->points + 3 * (s->hull_first_point + vi));
+    if (inside_out(sc)) {
+        for (int i = nv - 1; i >= 0; i -= delta) {
+            int vi = H->face_vertex_indices[first + i];
+            f->v[f->n++] = vload(H->points + 3 * (s->hull_first_point + vi));
+        }
+    } else {
+        for (int i = 0; i < nv; i += delta) {
+            int vi = H->face_vertex_indices[first + i];
+            f->v[f->n++] = vload(H->points + 3 * (s->hull_first_point + vi));
+        }
     }
 }
 
@@ -908,7 +947,7 @@ static void face_hull(OFace* f, const OShapes* sh, const cc_shape_desc* s, V3 d)
 /* getSupportingFace of a cylinder (cylinder.ts:167-240, cap template :43-53): the vertical edge
  * of the side facing away from dir, or the cap's octagon, turned so that a vertex points along
  * dir's horizontal part when dir is more than 5 degrees from the axis */
-static void face_cylinder(OFace* f, V3 d, double hh, double r) {
+static void face_cylinder(OFace* f, V3 d, double hh, double r, int insideOut) {
     static const double c45 = 0.707106769;
     static const double ox[8] = {0.0, c45, 1.0, c45, 0.0, -c45, -1.0, -c45};
     static const double oz[8] = {1.0, c45, 0.0, -c45, -1.0, -c45, 0.0, c45};
@@ -931,22 +970,29 @@ static void face_cylinder(OFace* f, V3 d, double hh, double r) {
     }
     f->n = 8;
     for (int i = 0; i < 8; i++) {
-        double px = sx * ox[i], py = sy * 1.0, pz = sz * oz[i];
+        const int t = insideOut ? 7 - i : i;
+        double px = sx * ox[t], py = sy * 1.0, pz = sz * oz[t];
         f->v[i] = v3(px * rc - pz * rs, py, px * rs + pz * rc);
     }
 }
 
 static void supporting_face(OFace* f, const OShapes* sh, int si, V3 dir, V3 pos, Q4 q) {
     const cc_shape_desc* s = &sh->shapes[si];
+    const V3 sc = shape_scale(s);
     switch (s->type) {
     case CC_SHAPE_SPHERE: f->n = 0; return;
-    case CC_SHAPE_BOX: face_box(f, dir, vload(s->half_extents)); break;
+    case CC_SHAPE_BOX:
+        face_box(f, dir, vload(s->half_extents));
+        if (inside_out(sc)) face_reverse(f);
+        break;
     case CC_SHAPE_CAPSULE: face_capsule(f, dir, s->half_height, s->radius); if (f->n == 0) return; break;
     case CC_SHAPE_CONVEX_HULL: face_hull(f, sh, s, dir); break;
-    case CC_SHAPE_CYLINDER: face_cylinder(f, dir, s->half_height, s->radius); break;
+    case CC_SHAPE_CYLINDER:
+        face_cylinder(f, dir, s->half_height, s->radius, inside_out(sc));
+        break;
     default: f->n = 0; return;
     }
-    face_transform(f, pos, q);
+    face_transform(f, pos, q, sc);
 }
 
 /* ======================================================================== */
@@ -961,9 +1007,21 @@ static V3 qrot2(Q4 q, V3 v) {
     return v3(v.x + 2 * uuvx, v.y + 2 * uuvy, v.z + 2 * uuvz);
 }
 
-/* collideSphereVsSphere src/shapes/sphere.ts:319-410 */
+/* The pose a decorated shape's leaf is collided at: a transformed decorator composes the body pose
+ * with its own, p + rotate(q, t) and q * tq (collideTransformedVsShape transformed.ts:370-420);
+ * scaled and offset-centre-of-mass decorators leave it unchanged. */
+static void leaf_pose(const cc_shape_desc* d, V3* p, Q4* q) {
+    if (!(d->decorated & CC_DECO_TRANSFORMED)) return;
+    const V3 t = qrotate(*q, vload(d->local_position));
+    const Q4 lq = {d->local_quaternion[0], d->local_quaternion[1], d->local_quaternion[2], d->local_quaternion[3]};
+    *p = vadd(*p, t);
+    *q = qmul(*q, lq);
+}
+
+/* collideSphereVsSphere src/shapes/sphere.ts:319-410 (radii times |scale.x|) */
 static void collide_sphere_sphere(const OShapes* sh, int sa, V3 pa, int sb, V3 pb, OHit* out) {
-    double ra = sh->shapes[sa].radius, rb = sh->shapes[sb].radius;
+    double ra = sh->shapes[sa].radius * fabs(shape_scale(&sh->shapes[sa]).x);
+    double rb = sh->shapes[sb].radius * fabs(shape_scale(&sh->shapes[sb]).x);
     V3 d = vsub(pb, pa);
     double d2 = vlen2(d), rs = ra + rb;
     out->hit = 0;
@@ -1182,7 +1240,7 @@ void ocx_collide_mesh(const OShapes* sh, int sa, V3 pa, Q4 qa, int sm, V3 pm, Q4
                 supporting_face(&h.faceA, sh, sa, vneg(pd.axis), pa, qa);
                 h.faceB.n = 3;
                 h.faceB.v[0] = a; h.faceB.v[1] = b; h.faceB.v[2] = c;
-                face_transform(&h.faceB, pa, qa);
+                face_transform(&h.faceB, pa, qa, v3(1, 1, 1));
                 fn(ctx, &h, (0xffffffffu & ~mask) | (uint32_t)ti);
             }
         } else {
@@ -1481,6 +1539,8 @@ void ocx_collide(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, 
                  double penTol, OHit* out) {
     out->faceA.n = out->faceB.n = 0;
     int ta = sh->shapes[sa].type, tb = sh->shapes[sb].type;
+    leaf_pose(&sh->shapes[sa], &pa, &qa); /* collideTransformedVsShape, then collideShapeVsTransformed */
+    leaf_pose(&sh->shapes[sb], &pb, &qb);
     if (ta == CC_SHAPE_PLANE || tb == CC_SHAPE_PLANE) { /* plane.ts:297-310 */
         out->hit = 0;
         if (ta == tb) return;
