@@ -21,7 +21,9 @@ const cc = require('./index.js');
 
 // ShapeType (src/shapes/shapes.ts:37-62) -> the C ABI's shape codes
 const SHAPE_TO_CC = { 0: 0, 1: 1, 2: 2, 3: 3, 4: 4, 9: 5, 11: 6 };
-const SHAPE_NAMES = { 5: 'COMPOUND', 6: 'TRANSFORMED', 7: 'SCALED', 8: 'EMPTY', 10: 'OFFSET_CENTER_OF_MASS' };
+const SHAPE_NAMES = { 5: 'COMPOUND', 8: 'EMPTY' };
+// decorators (ShapeType.TRANSFORMED, SCALED, OFFSET_CENTER_OF_MASS) and the C ABI's CC_DECO_* bits
+const DECO_BIT = { 7: 1, 6: 2, 10: 4 };
 const MOTION_STATIC = 0;
 const POINT_CONSTRAINT = 0;  // ConstraintType.POINT (src/constraints/constraint-id.ts:25-33)
 const TRIANGLE_STRIDE = 8;   // src/shapes/utils/triangle-mesh-data.ts:4-14
@@ -84,6 +86,60 @@ function shrunkPoints(shape) {
   return out;
 }
 
+// The shrunk support points of a scaled hull (setConvexHullNoConvexSupportScaled,
+// src/shapes/convex-hull.ts:842-935), in the order of the scene builder's scaled_shrunk_points
+// (crashcat_amd/shapes.py): points scaled, plane normals times 1 / scale and normalised, the
+// radius times the smallest |scale|.
+function scaledShrunkPoints(shape, sc) {
+  const inv = [1 / sc[0], 1 / sc[1], 1 / sc[2]];
+  const r = shape.convexRadius * Math.min(Math.abs(sc[0]), Math.abs(sc[1]), Math.abs(sc[2]));
+  const out = [];
+  const dot = (a, b) => (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+  const cross = (a, b) => [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]];
+  const unit = (v) => {
+    const l2 = dot(v, v);
+    const f = l2 > 0 ? 1 / Math.sqrt(l2) : 0;
+    return [v[0] * f, v[1] * f, v[2] * f];
+  };
+  const nrm = (f) => { const n = shape.planes[f].normal; return unit([n[0] * inv[0], n[1] * inv[1], n[2] * inv[2]]); };
+  for (const pt of shape.points) {
+    const q = pt.position;
+    const p = [q[0] * sc[0], q[1] * sc[1], q[2] * sc[2]];
+    const nf = pt.numFaces;
+    const c1 = nrm(pt.faces[0]);
+    const c2 = nrm(pt.faces[nf > 1 ? 1 : 0]);
+    const c3 = nf === 3 ? nrm(pt.faces[2]) : unit(cross(c1, c2));
+    const w1 = -dot(c1, p) + r, w2 = -dot(c2, p) + r, w3 = -dot(c3, p) + (nf === 3 ? r : 0);
+    const c23 = cross(c2, c3), c31 = cross(c3, c1), c12 = cross(c1, c2);
+    const den = dot(c1, c23);
+    if (nf === 1 || den === 0) {
+      out.push(p[0] + c1[0] * -r, p[1] + c1[1] * -r, p[2] + c1[2] * -r);
+    } else {
+      for (let k = 0; k < 3; k++) out.push(-((w1 * c23[k] + w2 * c31[k]) + w3 * c12[k]) / den);
+    }
+  }
+  return out;
+}
+
+// A chain of decorators over a leaf, as one scale and one local transform (the product of the
+// chain's scales; at most one transformed decorator).
+function flattenDecorators(sh) {
+  let deco = 0, transforms = 0;
+  let scale = [1, 1, 1], pos = [0, 0, 0], quat = [0, 0, 0, 1];
+  let s = sh;
+  while (DECO_BIT[s.type] !== undefined) {
+    deco |= DECO_BIT[s.type];
+    if (s.type === 7) scale = [scale[0] * s.scale[0], scale[1] * s.scale[1], scale[2] * s.scale[2]];
+    else if (s.type === 6) {
+      if (transforms++) fail('nested transformed shapes are not on the GPU path');
+      pos = vec(s.position, 3);
+      quat = vec(s.quaternion, 4);
+    }
+    s = s.shape;
+  }
+  return { leaf: s, deco, scale, pos, quat };
+}
+
 // Settings in the C ABI's cc_settings field names.
 function settingsDesc(s) {
   const L = s.layers;
@@ -139,16 +195,20 @@ function shapeTables(pool) {
                  planeNormals: [], planeConstants: [] };
   const mesh = { positions: [], triangles: [], nodes: [] };
   let np = 0, nf = 0, nfv = 0, nmv = 0, nmt = 0, nmn = 0;
-  const add = (sh) => {
-    if (index.has(sh)) return index.get(sh);
+  const add = (outer) => {
+    if (index.has(outer)) return index.get(outer);
+    const F = flattenDecorators(outer);
+    const sh = F.leaf;
     const t = SHAPE_TO_CC[sh.type];
     if (t === undefined) fail(`shape type ${SHAPE_NAMES[sh.type] || sh.type} is not on the GPU path`);
-    const aabb = box3(sh.aabb);
+    if (F.deco && (t === 4 || t === 6)) fail('decorated triangle meshes and planes are not on the GPU path');
+    const aabb = box3(outer.aabb);
     const d = { type: t, radius: 0, half_height: 0, half_extents: [0, 0, 0], convex_radius: 0,
                 center_of_mass: vec(sh.centerOfMass, 3), aabb_min: aabb.min, aabb_max: aabb.max,
                 hull_first_point: 0, hull_num_points: 0, hull_first_face: 0, hull_num_faces: 0,
                 mesh_first_triangle: 0, mesh_num_triangles: 0, mesh_first_node: 0, mesh_num_nodes: 0,
-                decorated: 0, scale: [1, 1, 1], local_position: [0, 0, 0], local_quaternion: [0, 0, 0, 1] };
+                decorated: F.deco, scale: F.scale, local_position: F.pos, local_quaternion: F.quat };
+    d.center_of_mass = vec(outer.centerOfMass, 3);
     if (sh.type === 0) {
       d.radius = sh.radius;
     } else if (sh.type === 1) {
@@ -172,7 +232,7 @@ function shapeTables(pool) {
       d.hull_first_face = nf;
       d.hull_num_faces = sh.faces.length;
       for (const p of sh.points) hull.points.push(p.position[0], p.position[1], p.position[2]);
-      for (const v of shrunkPoints(sh)) hull.shrunkPoints.push(v);
+      for (const v of (F.deco & 1 ? scaledShrunkPoints(sh, F.scale) : shrunkPoints(sh))) hull.shrunkPoints.push(v);
       sh.faces.forEach((f, i) => {
         hull.faceFirstVertex.push(nfv);
         hull.faceNumVertices.push(f.numVertices);
@@ -201,7 +261,7 @@ function shapeTables(pool) {
       nmt += tc;
       nmn += nodes.length / 8;
     }
-    index.set(sh, shapes.length);
+    index.set(outer, shapes.length);
     shapes.push(d);
     return shapes.length - 1;
   };

@@ -382,10 +382,21 @@ class Scene:
         def aabb(sh):
             return [list(sh.aabb_min), list(sh.aabb_max)]
 
-        shapes = []
-        for sh in self.shapes:
+        def record(sh):
             base = {"aabb": aabb(sh), "centerOfMass": list(sh.center_of_mass), "density": sh.density,
                     "volume": sh.volume, "materialId": -1}
+            if sh.wrap is not None:  # scaled.ts:31-38, transformed.ts:31-39, offset-center-of-mass.ts:40-47
+                kind, par, inner = sh.wrap
+                base.pop("density")
+                base.pop("materialId")
+                base["shape"] = record(inner)
+                if kind == "scaled":
+                    base.update(type=7, scale=list(par))
+                elif kind == "transformed":
+                    base.update(type=6, position=list(par[0]), quaternion=list(par[1]))
+                else:
+                    base.update(type=10, offset=list(par))
+                return base
             if sh.type == S.SPHERE:
                 base.update(type=0, radius=sh.radius)
             elif sh.type == S.BOX:
@@ -422,7 +433,9 @@ class Scene:
                             bvh={"buffer": np.asarray(sh.mesh_nodes, dtype=np.float64).reshape(-1).tolist()})
             else:
                 raise ValueError(f"shape type {sh.type}")
-            shapes.append(base)
+            return base
+
+        shapes = [record(sh) for sh in self.shapes]
         b = self.bodies()
         pool = []
         for k, r in enumerate(b):

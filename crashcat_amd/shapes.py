@@ -68,6 +68,9 @@ class Shape:
         self.local_position = tuple(kw.get("local_position", (0.0, 0.0, 0.0)))
         self.local_quaternion = tuple(kw.get("local_quaternion", (0.0, 0.0, 0.0, 1.0)))
         self.transformed = kw.get("transformed", False)  # a transformed decorator is in the chain
+        # the decorator this shape is, as crashcat holds it: (kind, parameters, inner shape), None
+        # for an undecorated shape
+        self.wrap = kw.get("wrap")
 
     def key(self):
         """Hashable identity used to deduplicate identical shapes at upload."""
@@ -431,7 +434,7 @@ def _leaf_copy(shape, **over):
               plane_normals=shape.plane_normals, plane_constants=shape.plane_constants, volume=shape.volume,
               point_faces=shape.point_faces, decorated=shape.decorated, scale=shape.scale,
               local_position=shape.local_position, local_quaternion=shape.local_quaternion,
-              transformed=shape.transformed)
+              transformed=shape.transformed, wrap=None)
     kw.update(over)
     return Shape(shape.type, **kw)
 
@@ -470,7 +473,7 @@ def scaled(shape, scale):
     J = J * ms
     over = dict(scale=tot, aabb_min=mn, aabb_max=mx, mass=shape.mass * ms, inertia=J, center_of_mass=(0.0, 0.0, 0.0),
                 volume=shape.volume * abs(sc[0]) * abs(sc[1]) * abs(sc[2]),
-                decorated=shape.decorated | _abi.CC_DECO_SCALED)
+                decorated=shape.decorated | _abi.CC_DECO_SCALED, wrap=("scaled", sc, shape))
     if shape.type == CONVEX_HULL:
         over["shrunk_points"] = scaled_shrunk_points(shape.points, shape.plane_normals, shape.point_faces,
                                                       shape.convex_radius, tot)
@@ -498,7 +501,8 @@ def transformed(shape, position, quaternion):
     R = _mat3_from_quat(q)
     inertia = (R @ np.asarray(shape.inertia, dtype=np.float64)) @ R.T
     return _leaf_copy(shape, local_position=t, local_quaternion=q, aabb_min=mn, aabb_max=mx, center_of_mass=com,
-                      inertia=inertia, transformed=True, decorated=shape.decorated | _abi.CC_DECO_TRANSFORMED)
+                      inertia=inertia, transformed=True, decorated=shape.decorated | _abi.CC_DECO_TRANSFORMED,
+                      wrap=("transformed", (t, q), shape))
 
 
 def offset_center_of_mass(shape, offset):
@@ -509,7 +513,8 @@ def offset_center_of_mass(shape, offset):
     o = np.array([float(v) for v in offset])
     com = tuple(float(shape.center_of_mass[k] + o[k]) for k in range(3))
     I = np.asarray(shape.inertia, dtype=np.float64) + shape.mass * (np.eye(3) * float(o @ o) - np.outer(o, o))
-    return _leaf_copy(shape, center_of_mass=com, inertia=I, decorated=shape.decorated | _abi.CC_DECO_OFFSET_COM)
+    return _leaf_copy(shape, center_of_mass=com, inertia=I, decorated=shape.decorated | _abi.CC_DECO_OFFSET_COM,
+                      wrap=("offset", tuple(float(v) for v in o), shape))
 
 
 def scaled_shrunk_points(points, normals, point_faces, cr, scale):

@@ -222,11 +222,12 @@ def _crashcat_scenes():
     return {"cube_heap": SC.cube_heap(seed=1001, nx=3, ny=3, nz=3),
             "mixed": SC.mixed_pile(seed=1002, n=60, extent=(6.0, 6.0, 6.0), cylinders=True),
             "chains": chains,
-            "terrain": SC.terrain_drop(vertices=16, n=24, drop=(0.5, 2.5))}
+            "terrain": SC.terrain_drop(vertices=16, n=24, drop=(0.5, 2.5)),
+            "decorated": SC.decorated_pile(seed=1011, n=60, extent=(6.0, 4.0, 6.0))}
 
 
 @needs_node
-@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain"])
+@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated"])
 def test_crashcat_world_upload_equals_the_scene_description(tmp_path, which):
     """The adapter reads crashcat's own objects (World settings, RigidBody pool with
     motionProperties, shape records, point-constraint pool) and produces exactly the upload the
@@ -309,7 +310,7 @@ try {{ cc.fromCrashcatWorld(world); process.stdout.write('accepted'); }} catch (
 
 @needs_node
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain"])
+@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated"])
 def test_crashcat_world_steps_bitwise_with_events_and_forces(tmp_path, which):
     """updateWorld on a crashcat-shaped World: FP64, bit for bit against the oracle in the state
     written back into the RigidBody objects; per step the listener's events (with the body
