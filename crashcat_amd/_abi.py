@@ -8,6 +8,7 @@ CC_SHAPE_CONVEX_HULL = 3
 CC_SHAPE_TRIANGLE_MESH = 4
 CC_SHAPE_CYLINDER = 5
 CC_SHAPE_PLANE = 6
+CC_SHAPE_COMPOUND = 7
 CC_DECO_SCALED = 1
 CC_DECO_TRANSFORMED = 2
 CC_DECO_OFFSET_COM = 4
@@ -84,6 +85,8 @@ class ShapeDesc(C.Structure):
         ("scale", D3),
         ("local_position", D3),
         ("local_quaternion", D4),
+        ("compound_first_child", C.c_int32),
+        ("compound_num_children", C.c_int32),
     ]
 
 

@@ -143,7 +143,7 @@ class Context:
         self._check(L.cc_upload_scene(self.h, C.byref(scene.settings), len(offs) - 1,
                                       offs.ctypes.data_as(C.POINTER(C.c_int32)),
                                       bodies.ctypes.data_as(C.POINTER(_abi.BodyDesc)), self.n, shapes,
-                                      len(scene.shapes), C.byref(hulls), pcs, npc, C.byref(meshes)))
+                                      keep["num_shapes"], C.byref(hulls), pcs, npc, C.byref(meshes)))
 
     def step(self, dt=1.0 / 60.0, n=1):
         self._check(lib().cc_step(self.h, dt, n))

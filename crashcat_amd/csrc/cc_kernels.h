@@ -90,7 +90,7 @@ struct Dev {
     int spillCap;
     int* pairCon;       // [pairCap] first constraint slot made from each pair (a run of consecutive
                         // slots, its length in the first head's bits), -1 if none
-    int hasMesh;        // a shape is a triangle mesh: pairs may make several constraints
+    int hasMesh;        // a shape is a triangle mesh: pairs may make several constraints (so may compounds)
     int* meshList;      // [pairCap] pair slots of convex-vs-mesh pairs
     int meshWorkers;    // threads of the mesh kernel (each owns a collector and an EPA polytope)
     char* meshWork;     // [meshWorkers * sizeof(MeshColl<T>)] collectors
@@ -139,7 +139,14 @@ struct Dev {
     // clipping, pruning) are then computed in FP64 too
     int npWide;
     ShapeTable<double> st64;
-    double spec64, colTol64, penTol64, manifoldTol64;
+    double spec64, colTol64, penTol64, manifoldTol64, normalCos64;
+    // compound pairs (compound.ts): listed by the filter, each solved by one thread that owns a
+    // reduction collector (cmpColl) and an EPA polytope (cmpEpa, one block per 64 threads)
+    int hasCompound;
+    int* cmpList;       // [pairCap]
+    int cmpWorkers;
+    char* cmpColl;
+    char* cmpEpa;
     Simplex<double>* epaSimplex64;  // [epaCap] when npWide
     // point constraints
     PointCon<T>* pc;
@@ -201,7 +208,7 @@ enum { ACC_STEPS = 0, ACC_PAIRS, ACC_CONS, ACC_POINTS, ACC_EPA, ACC_AWAKE,
        ACC_BYTES_BODY, ACC_BYTES_BP, ACC_BYTES_NP, ACC_BYTES_SOLVE, ACC_GJK, ACC_N = 16 };
 
 enum { CNT_PAIRS = 0, CNT_CONS = 1, CNT_EPA = 2, CNT_OVERFLOW = 3, CNT_EPA_TOTAL = 4, CNT_SPILL = 5, CNT_CELLS = 6,
-       CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_MESH = 10, CNT_PLANE = 11, CNT_N = 12 };
+       CNT_HITS = 7, CNT_HITS_L = 8, CNT_GJK = 9, CNT_MESH = 10, CNT_PLANE = 11, CNT_COMPOUND = 12, CNT_N = 13 };
 // CNT_OVERFLOW bits: 1 pairs, 2 contact constraints, 4 EPA jobs, 8 pair spill list, 16 hits,
 // 32 mesh BVH traversal stack, 64 mesh candidate triangles
 constexpr int BP_SLOTS = 32;
@@ -641,11 +648,11 @@ __device__ inline const ShapeTable<C>& np_shapes(const Dev<T>& d) {
     if constexpr (std::is_same<T, C>::value) return d.st;
     else return d.st64;
 }
-template <typename C> struct NpTol { C spec, colTol, penTol, manifoldTol; };
+template <typename C> struct NpTol { C spec, colTol, penTol, manifoldTol, normalCos; };
 template <typename T, typename C>
 __device__ inline NpTol<C> np_tol(const Dev<T>& d) {
-    if constexpr (std::is_same<T, C>::value) return NpTol<C>{d.s.spec, d.s.colTol, d.s.penTol, d.s.manifoldTol};
-    else return NpTol<C>{C(d.spec64), C(d.colTol64), C(d.penTol64), C(d.manifoldTol64)};
+    if constexpr (std::is_same<T, C>::value) return NpTol<C>{d.s.spec, d.s.colTol, d.s.penTol, d.s.manifoldTol, d.s.normalCos};
+    else return NpTol<C>{C(d.spec64), C(d.colTol64), C(d.penTol64), C(d.manifoldTol64), C(d.normalCos64)};
 }
 template <typename T, typename C>
 __device__ inline Simplex<C>* np_simplices(const Dev<T>& d) {
@@ -958,7 +965,7 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
     const int npairs = min(d.counters[CNT_PAIRS], d.pairCap);
     for (int base = blockIdx.x * 256; base < npairs; base += gridDim.x * 256) {  // the same trip count in every thread
         const int slot = base + threadIdx.x;
-        bool gjk = false, mesh = false, plane = false;
+        bool gjk = false, mesh = false, plane = false, cmp = false;
         int combo = 0;
         if (slot < npairs) {
             d.pairCon[slot] = -1;
@@ -979,6 +986,9 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
                     V3<T> n = dist > T(0) ? dd * (T(1) / dist) : mk3<T>(0, 1, 0);
                     push_hit(d, slot, HIT_SPHERES, pa + n * ra, pbp - n * rb, n, true);
                 }
+            } else if (ta == SHAPE_COMPOUND || tb == SHAPE_COMPOUND) {  // k_compound
+                T reach = RA.brad + RB.brad + d.s.spec;
+                cmp = !(len2(RB.pos - RA.pos) > reach * reach);
             } else if (ta == SHAPE_PLANE || tb == SHAPE_PLANE) {
                 plane = ta != tb && ta != SHAPE_MESH && tb != SHAPE_MESH;  // k_np_plane; none vs a plane or a mesh
             } else if (ta == SHAPE_MESH || tb == SHAPE_MESH) {
@@ -1003,6 +1013,11 @@ __global__ void __launch_bounds__(256) k_np_filter(Dev<T> d, T dt) {
             __syncthreads();
             const int pi = block_append<256>(&d.counters[CNT_PLANE], plane, sh);
             if (plane) d.planeList[pi] = slot;
+        }
+        if (d.hasCompound) {
+            __syncthreads();
+            const int ci = block_append<256>(&d.counters[CNT_COMPOUND], cmp, sh);
+            if (cmp) d.cmpList[ci] = slot;
         }
     }
 }
@@ -1265,7 +1280,8 @@ template <typename T>
 __global__ void k_con_flags(Dev<T> d) {
     int np = min(d.counters[CNT_PAIRS], d.pairCap);
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < d.pairCap; p += gridDim.x * blockDim.x)
-        d.conFlag[p] = (p < np && d.pairCon[p] >= 0) ? (d.hasMesh ? 1 + ((d.conHead[d.pairCon[p]].bits >> 8) & 31) : 1) : 0;
+        d.conFlag[p] = (p < np && d.pairCon[p] >= 0)
+                           ? (d.hasMesh || d.hasCompound ? 1 + ((d.conHead[d.pairCon[p]].bits >> 8) & 31) : 1) : 0;
 }
 
 template <typename T>
@@ -1276,7 +1292,7 @@ __global__ void k_con_compact(Dev<T> d) {
         int c = d.pairCon[p];
         if (c >= 0) {
             const int bits = d.conHead[c].bits;
-            const int run = d.hasMesh ? 1 + ((bits >> 8) & 31) : 1;
+            const int run = d.hasMesh || d.hasCompound ? 1 + ((bits >> 8) & 31) : 1;
             for (int t = 0; t < run; t++) {
                 d.conIdx[d.conPos[p] + t] = c + t;
                 pts += (unsigned long long)((t ? d.conHead[c + t].bits : bits) >> 4 & 7);

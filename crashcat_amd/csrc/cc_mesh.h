@@ -56,14 +56,16 @@ __device__ inline V3<T> fix_normal(V3<T> v0, V3<T> v1, V3<T> v2, V3<T> tn, int a
 }
 
 // addHit of the reduction collector (src/update.ts:385-520)
-template <typename T>
-__device__ inline void mesh_add_hit(const Dev<T>& d, MeshColl<T>& C, int& nm, V3<T> base, V3<T> pA, V3<T> pB, V3<T> axis,
-                                    T pen, unsigned subA, unsigned subB, const Face<T, CC_MAX_FACE>& fA,
-                                    const Face<T, CC_MAX_FACE>& fB, NpScratch<T, CC_MAX_FACE>& S) {
-    const V3<T> normal = normalize(axis);
+// (collector of arithmetic type K: T, or FP64 for the FP64 narrowphase of an FP32 context)
+template <typename T, typename K>
+__device__ inline void mesh_add_hit(const Dev<T>& d, MeshColl<K>& C, int& nm, V3<K> base, V3<K> pA, V3<K> pB, V3<K> axis,
+                                    K pen, unsigned subA, unsigned subB, const Face<K, CC_MAX_FACE>& fA,
+                                    const Face<K, CC_MAX_FACE>& fB, NpScratch<K, CC_MAX_FACE>& S) {
+    const NpTol<K> tol = np_tol<T, K>(d);
+    const V3<K> normal = normalize(axis);
     int f = -1;
     for (int i = 0; i < nm; i++)
-        if (dot(normal, C.m[i].normal) >= d.s.normalCos) { f = i; break; }
+        if (dot(normal, C.m[i].normal) >= tol.normalCos) { f = i; break; }
     if (f < 0) {
         if (nm >= MESH_MAX_MANIFOLDS) {  // pool full: a deeper hit replaces the shallowest manifold
             int sh = 0;
@@ -83,11 +85,11 @@ __device__ inline void mesh_add_hit(const Dev<T>& d, MeshColl<T>& C, int& nm, V3
     } else {
         C.m[f].normal = C.m[f].normal + normal;
     }
-    Manifold<T, MESH_MAX_POINTS>& m = C.m[f];
-    Manifold<T, CC_MAX_FACE>& t = S.m;
+    Manifold<K, MESH_MAX_POINTS>& m = C.m[f];
+    Manifold<K, CC_MAX_FACE>& t = S.m;
     t.base = base;
     t.n = 0;
-    const T maxDist = d.s.spec + d.s.manifoldTol;
+    const K maxDist = tol.spec + tol.manifoldTol;
     if (fA.n >= 2 && fB.n >= 3) manifold_between_two_faces(t, pA, pB, axis, maxDist, fA, fB, S.tmp);
     else { t.relA[0] = pA - base; t.relB[0] = pB - base; t.n = 1; }
     for (int i = 0; i < t.n && m.n < MESH_MAX_POINTS; i++) {

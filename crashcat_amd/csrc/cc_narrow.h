@@ -7,7 +7,8 @@
 
 namespace cc {
 
-enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3, SHAPE_MESH = 4, SHAPE_CYLINDER = 5, SHAPE_PLANE = 6 };
+enum { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2, SHAPE_HULL = 3, SHAPE_MESH = 4, SHAPE_CYLINDER = 5, SHAPE_PLANE = 6,
+       SHAPE_COMPOUND = 7 };
 
 // A BVH node of a triangle mesh: bounds, then for an internal node the right child (node
 // index within the mesh; the left child is the next node) and the split axis, for a leaf the
@@ -29,6 +30,7 @@ struct ShapeD {
     // the CC_DECO_* bits, sc the scale of the leaf, lp / lq the local transform
     int deco;
     T sc[3], lp[3], lq[4];
+    int cfc, ncc;  // compounds: first child (a shape index) and number of children
 };
 
 enum { DECO_SCALED = 1, DECO_TRANSFORMED = 2 };

@@ -16,6 +16,7 @@
 #include "../../include/crashcat_amd.h"
 #include "cc_mesh_build.h"
 #include "cc_mesh.h"
+#include "cc_compound.h"
 #include "cc_adjoint.h"
 
 using namespace cc;
@@ -94,6 +95,7 @@ __global__ void k_set_pair_count(Dev<T> d) {
         d.counters[CNT_GJK] = 0;
         d.counters[CNT_MESH] = 0;
         d.counters[CNT_PLANE] = 0;
+        d.counters[CNT_COMPOUND] = 0;
         for (int c = 0; c < 32; c++) d.gjkHist[c] = 0;
         if (n > d.pairCap) atomicOr(&d.counters[CNT_OVERFLOW], 1);
     }
@@ -447,6 +449,9 @@ static int shape_table(cc_ctx* c, Alloc& A, const cc_shape_desc* shapes, int32_t
             o.hff = q.mesh_first_node; o.hnf = q.mesh_num_nodes;
             o.mfv = 3;
         }
+        o.cfc = q.type == CC_SHAPE_COMPOUND ? q.compound_first_child : 0;
+        o.ncc = q.type == CC_SHAPE_COMPOUND ? q.compound_num_children : 0;
+        if (q.type == CC_SHAPE_COMPOUND) o.mfv = CC_MAX_FACE;
         o.radius = U(q.radius); o.hh = U(q.half_height); o.cr = U(q.convex_radius);
         o.deco = q.decorated;
         for (int k = 0; k < 3; k++) {
@@ -485,6 +490,14 @@ static int shape_table(cc_ctx* c, Alloc& A, const cc_shape_desc* shapes, int32_t
             br += std::sqrt(q.local_position[0] * q.local_position[0] + q.local_position[1] * q.local_position[1] +
                             q.local_position[2] * q.local_position[2]);
         o.brad = U(br * (1.0 + 1e-5) + 1e-6);
+    }
+    for (int i = 0; i < ns; i++) {  // a compound holds its children's spheres about their positions
+        const cc_shape_desc& q = shapes[i];
+        if (q.type != CC_SHAPE_COMPOUND) continue;
+        double br = 0.0;
+        for (int k = q.compound_first_child; k < q.compound_first_child + q.compound_num_children; k++)
+            br = std::max(br, (double)sh[k].brad);  // a child's radius counts its local translation already
+        sh[i].brad = U(br * (1.0 + 1e-5) + 1e-6);
     }
     int P = hulls ? hulls->total_points : 0, F = hulls ? hulls->total_faces : 0, FV = hulls ? hulls->total_face_vertices : 0;
     for (int i = 0; i < ns; i++) {  // a hull's supporting face takes at most 16 of a face's vertices
@@ -558,7 +571,7 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     bool anyDeco = false, anyMeshOrPlane = false;
     for (int i = 0; i < ns; i++) {
         const cc_shape_desc& q = shapes[i];
-        if (q.type < 0 || q.type > 6) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
+        if (q.type < 0 || q.type > CC_SHAPE_COMPOUND) return fail(c, CC_E_UNSUPPORTED, "unsupported shape type " + std::to_string(q.type));
         if (q.decorated & ~(CC_DECO_SCALED | CC_DECO_TRANSFORMED | CC_DECO_OFFSET_COM))
             return fail(c, CC_E_ARG, "unknown decorator bits");
         if (q.decorated) {
@@ -569,6 +582,17 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
             anyDeco = true;
         }
         anyMeshOrPlane = anyMeshOrPlane || q.type == CC_SHAPE_TRIANGLE_MESH || q.type == CC_SHAPE_PLANE;
+        if (q.type == CC_SHAPE_COMPOUND) {  // children: convex leaves (scaled allowed) with their local transform
+            anyDeco = true;
+            if (q.compound_num_children < 1 || q.compound_first_child < 0 ||
+                (long long)q.compound_first_child + q.compound_num_children > ns)
+                return fail(c, CC_E_ARG, "compound children out of range");
+            for (int k = q.compound_first_child; k < q.compound_first_child + q.compound_num_children; k++) {
+                const int t = shapes[k].type;
+                if (t == CC_SHAPE_COMPOUND || t == CC_SHAPE_TRIANGLE_MESH || t == CC_SHAPE_PLANE)
+                    return fail(c, CC_E_UNSUPPORTED, "compound children must be convex shapes");
+            }
+        }
         if (q.type == CC_SHAPE_TRIANGLE_MESH) {
             const int MT = meshes ? meshes->total_triangles : 0, MN = meshes ? meshes->total_nodes : 0;
             if (q.mesh_first_triangle < 0 || q.mesh_num_triangles < 0 || q.mesh_first_node < 0 || q.mesh_num_nodes < 0 ||
@@ -578,7 +602,8 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
         }
     }
     if (anyDeco && anyMeshOrPlane)
-        return fail(c, CC_E_UNSUPPORTED, "decorated shapes in a scene with a triangle mesh or a plane are not on the GPU path");
+        return fail(c, CC_E_UNSUPPORTED,
+                    "decorated or compound shapes in a scene with a triangle mesh or a plane are not on the GPU path");
     {
         const int rc = shape_table<T>(c, A, shapes, ns, hulls, d.st);
         if (rc != CC_OK) return rc;
@@ -590,6 +615,7 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     d.colTol64 = s->collision_tolerance;
     d.penTol64 = s->penetration_tolerance;
     d.manifoldTol64 = s->manifold_tolerance;
+    d.normalCos64 = s->normal_cos_max_delta_rotation;
     if (d.npWide) {
         const int rc = shape_table<double>(c, A, shapes, ns, hulls, d.st64);
         if (rc != CC_OK) return rc;
@@ -824,6 +850,16 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     d.hasPlane = 0;
     for (int i = 0; i < ns; i++) d.hasPlane = d.hasPlane || shapes[i].type == CC_SHAPE_PLANE;
     if (d.hasPlane) d.planeList = A.get<int>(d.pairCap);
+    d.hasCompound = 0;
+    for (int i = 0; i < ns; i++) d.hasCompound = d.hasCompound || shapes[i].type == CC_SHAPE_COMPOUND;
+    if (d.hasCompound) {  // one collector and one EPA polytope per compound thread
+        d.cmpList = A.get<int>(d.pairCap);
+        d.cmpWorkers = (int)std::min<long long>(4096, ((long long)d.pairCap + 63) / 64 * 64);
+        const size_t coll = d.npWide ? sizeof(MeshColl<double>) : sizeof(MeshColl<T>);
+        const size_t wave = d.npWide ? EpaWave<double>::bytes : EpaWave<T>::bytes;
+        d.cmpColl = A.get<char>((size_t)d.cmpWorkers * coll);
+        d.cmpEpa = A.get<char>((size_t)((d.cmpWorkers + EPA_LANES - 1) / EPA_LANES) * wave);
+    }
     if (X->hasMesh) {  // one collector per mesh thread; the EPA polytopes are the EPA kernel's
         d.meshList = A.get<int>(d.pairCap);
         d.meshWorkers = ((d.epaWorkers + EPA_LANES - 1) / EPA_LANES) * EPA_LANES;
@@ -1162,6 +1198,10 @@ static void enqueue_phase(CtxT<T>* X, hipStream_t st, int p, T dt, double margin
                 hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE, double>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
             else
                 hipLaunchKernelGGL((k_np_manifold<T, CC_MAX_FACE, T>), dim3(grid_for(d.hitCap / 8, 256, 1024)), dim3(256), 0, st, d, dt);
+            if (d.hasCompound && d.npWide)
+                hipLaunchKernelGGL((k_compound<T, double>), dim3(d.cmpWorkers / 64), dim3(64), 0, st, d, dt);
+            else if (d.hasCompound)
+                hipLaunchKernelGGL((k_compound<T, T>), dim3(d.cmpWorkers / 64), dim3(64), 0, st, d, dt);
             if (d.hasPlane && d.npWide)
                 hipLaunchKernelGGL((k_np_plane<T, double>), dim3(grid_for(d.pairCap, 256, 2048)), dim3(256), 0, st, d, dt);
             else if (d.hasPlane)

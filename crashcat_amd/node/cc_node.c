@@ -121,6 +121,7 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_shape_desc, mesh_first_node, "i32", 1); F(cc_shape_desc, mesh_num_nodes, "i32", 1);
     F(cc_shape_desc, decorated, "i32", 1); F(cc_shape_desc, scale, "f64", 3);
     F(cc_shape_desc, local_position, "f64", 3); F(cc_shape_desc, local_quaternion, "f64", 4);
+    F(cc_shape_desc, compound_first_child, "i32", 1); F(cc_shape_desc, compound_num_children, "i32", 1);
     add_struct(env, out, "cc_shape_desc", sizeof(cc_shape_desc), fields);
     napi_create_object(env, &fields);
     F(cc_body_desc, shape, "i32", 1); F(cc_body_desc, motion_type, "i32", 1);

@@ -20,8 +20,8 @@
 const cc = require('./index.js');
 
 // ShapeType (src/shapes/shapes.ts:37-62) -> the C ABI's shape codes
-const SHAPE_TO_CC = { 0: 0, 1: 1, 2: 2, 3: 3, 4: 4, 9: 5, 11: 6 };
-const SHAPE_NAMES = { 5: 'COMPOUND', 8: 'EMPTY' };
+const SHAPE_TO_CC = { 0: 0, 1: 1, 2: 2, 3: 3, 4: 4, 5: 7, 9: 5, 11: 6 };
+const SHAPE_NAMES = { 8: 'EMPTY' };
 // decorators (ShapeType.TRANSFORMED, SCALED, OFFSET_CENTER_OF_MASS) and the C ABI's CC_DECO_* bits
 const DECO_BIT = { 7: 1, 6: 2, 10: 4 };
 const MOTION_STATIC = 0;
@@ -195,20 +195,32 @@ function shapeTables(pool) {
                  planeNormals: [], planeConstants: [] };
   const mesh = { positions: [], triangles: [], nodes: [] };
   let np = 0, nf = 0, nfv = 0, nmv = 0, nmt = 0, nmn = 0;
-  const add = (outer) => {
-    if (index.has(outer)) return index.get(outer);
+  // compounds whose children are still to be appended: the children follow every body's shape,
+  // each a leaf desc with the child's position and quaternion as its local transform
+  const pending = [];
+  const makeDesc = (outer, child) => {
     const F = flattenDecorators(outer);
     const sh = F.leaf;
     const t = SHAPE_TO_CC[sh.type];
     if (t === undefined) fail(`shape type ${SHAPE_NAMES[sh.type] || sh.type} is not on the GPU path`);
-    if (F.deco && (t === 4 || t === 6)) fail('decorated triangle meshes and planes are not on the GPU path');
+    if (F.deco && (t === 4 || t === 6 || t === 7)) fail('decorated triangle meshes, planes and compounds are not on the GPU path');
+    if (child) {
+      if (t === 4 || t === 6 || t === 7 || (F.deco & 2)) {
+        fail('compound children must be convex shapes (scaled allowed), not transformed, compound, meshes or planes');
+      }
+      F.deco |= 2;
+      F.pos = vec(child.position, 3);
+      F.quat = vec(child.quaternion, 4);
+    }
     const aabb = box3(outer.aabb);
     const d = { type: t, radius: 0, half_height: 0, half_extents: [0, 0, 0], convex_radius: 0,
                 center_of_mass: vec(sh.centerOfMass, 3), aabb_min: aabb.min, aabb_max: aabb.max,
                 hull_first_point: 0, hull_num_points: 0, hull_first_face: 0, hull_num_faces: 0,
                 mesh_first_triangle: 0, mesh_num_triangles: 0, mesh_first_node: 0, mesh_num_nodes: 0,
-                decorated: F.deco, scale: F.scale, local_position: F.pos, local_quaternion: F.quat };
+                decorated: F.deco, scale: F.scale, local_position: F.pos, local_quaternion: F.quat,
+                compound_first_child: 0, compound_num_children: 0 };
     d.center_of_mass = vec(outer.centerOfMass, 3);
+    if (t === 7) pending.push({ d, children: sh.children });
     if (sh.type === 0) {
       d.radius = sh.radius;
     } else if (sh.type === 1) {
@@ -261,11 +273,25 @@ function shapeTables(pool) {
       nmt += tc;
       nmn += nodes.length / 8;
     }
+    return d;
+  };
+  const add = (outer) => {
+    if (index.has(outer)) return index.get(outer);
+    const d = makeDesc(outer, null);
     index.set(outer, shapes.length);
     shapes.push(d);
     return shapes.length - 1;
   };
-  return { add, shapes, hull, mesh };
+  const finish = () => {
+    for (const { d, children } of pending) {
+      if (!children || !children.length) fail('a compound has no children');
+      d.compound_first_child = shapes.length;
+      d.compound_num_children = children.length;
+      for (const c of children) shapes.push(makeDesc(c.shape, c));
+    }
+    pending.length = 0;
+  };
+  return { add, finish, shapes, hull, mesh };
 }
 
 function bodyDesc(b, shapeIndex) {
@@ -321,6 +347,7 @@ function describe(world) {
     if (inert < 0) inert = T.add({ type: 0, radius: 1e-3, aabb: [[-1e-3, -1e-3, -1e-3], [1e-3, 1e-3, 1e-3]], centerOfMass: [0, 0, 0] });
     return placeholder(inert);
   });
+  T.finish();
   const pointConstraints = [];
   const pools = (world.constraints && world.constraints.pools) || {};
   for (const key of Object.keys(pools)) {

@@ -195,14 +195,33 @@ class Scene:
         cached = getattr(self, "_cshapes", None)
         if cached is not None and cached[0] == key:
             return cached[1]
-        n = len(self.shapes)
+        # a compound's children follow the scene's shapes, each a leaf desc whose local transform
+        # is the child's position and quaternion
+        descs = list(self.shapes)
+        child_range = {}
+        for i, s in enumerate(self.shapes):
+            if s.type == S.COMPOUND:
+                child_range[i] = (len(descs), len(s.children))
+                descs.extend((pos, q, ch) for pos, q, ch in s.children)
+        n = len(descs)
         arr = (_abi.ShapeDesc * max(n, 1))()
         pts, shr, ffv, fnv, fvi, pn, pc = [], [], [], [], [], [], []
         np_, nf, nfv = 0, 0, 0
         mpos, mtri, mnode = [], [], []
         nmv, nmt, nmn = 0, 0, 0
-        for i, s in enumerate(self.shapes):
+        for i, s in enumerate(descs):
             d = arr[i]
+            if isinstance(s, tuple):  # a compound's child
+                pos, q, s = s
+                d.local_position[:] = pos
+                d.local_quaternion[:] = q
+                d.decorated = s.decorated | _abi.CC_DECO_TRANSFORMED
+            else:
+                d.local_position[:] = s.local_position
+                d.local_quaternion[:] = s.local_quaternion
+                d.decorated = s.decorated
+            if i in child_range:
+                d.compound_first_child, d.compound_num_children = child_range[i]
             d.type = s.type
             d.radius = s.radius
             d.half_height = s.half_height
@@ -211,10 +230,7 @@ class Scene:
             d.center_of_mass[:] = s.center_of_mass
             d.aabb_min[:] = s.aabb_min
             d.aabb_max[:] = s.aabb_max
-            d.decorated = s.decorated
             d.scale[:] = s.scale
-            d.local_position[:] = s.local_position
-            d.local_quaternion[:] = s.local_quaternion
             if s.type == S.CONVEX_HULL:
                 d.hull_first_point = np_
                 d.hull_num_points = len(s.points)
@@ -277,6 +293,7 @@ class Scene:
         m.total_nodes = nmn
         keep["meshes"] = m
         keep["arr"] = arr
+        keep["num_shapes"] = n  # the scene's shapes and the compounds' children after them
         self._cshapes = (key, (arr, h, keep))
         return arr, h, keep
 
@@ -309,7 +326,7 @@ class Scene:
                 out[name] = list(v) if hasattr(v, "__len__") else v
             return out
         arr, hull, keep = self.c_shapes()
-        shapes = [struct_dict(arr[i]) for i in range(len(self.shapes))]
+        shapes = [struct_dict(arr[i]) for i in range(keep["num_shapes"])]
         bodies = []
         b = self.bodies()
         for row in b:
@@ -385,6 +402,12 @@ class Scene:
         def record(sh):
             base = {"aabb": aabb(sh), "centerOfMass": list(sh.center_of_mass), "density": sh.density,
                     "volume": sh.volume, "materialId": -1}
+            if sh.type == S.COMPOUND:  # compound.ts:15-27
+                base.pop("density")
+                base.pop("materialId")
+                base.update(type=5, children=[{"position": list(p), "quaternion": list(q), "shape": record(ch)}
+                                              for p, q, ch in sh.children])
+                return base
             if sh.wrap is not None:  # scaled.ts:31-38, transformed.ts:31-39, offset-center-of-mass.ts:40-47
                 kind, par, inner = sh.wrap
                 base.pop("density")
@@ -649,6 +672,52 @@ def decorated_pile(seed=1009, n=300, extent=(12.0, 8.0, 12.0), scene=None):
                 if rng.uniform() < 0.5 else S.sphere(rng.uniform(0.2, 0.4))
             shape = S.offset_center_of_mass(inner, (rng.uniform(-0.1, 0.1), rng.uniform(-0.2, 0.0),
                                                     rng.uniform(-0.1, 0.1)))
+        x = rng.uniform(-extent[0] / 2, extent[0] / 2)
+        y = rng.uniform(1.0, 1.0 + extent[1])
+        z = rng.uniform(-extent[2] / 2, extent[2] / 2)
+        v = (rng.normal(0, 1), rng.normal(0, 1), rng.normal(0, 1))
+        w = (rng.normal(0, 0.5), rng.normal(0, 0.5), rng.normal(0, 0.5))
+        sc.add_body(shape, (x, y, z), rng.quaternion(), linear_velocity=v, angular_velocity=w)
+    sc.end_world()
+    return sc
+
+
+def compound_pile(seed=1012, n=200, extent=(12.0, 8.0, 12.0), scene=None):
+    """A pile over a static box floor: two thirds compound bodies of 2-4 children (boxes,
+    spheres, capsules, cylinders and 12-point hulls, some scaled, at random positions within
+    0.4 m and random orientations), one third plain boxes and spheres; positions uniform in the
+    extent above the floor, v ~ N(0, 1), w ~ N(0, 0.5)."""
+    sc = scene if scene is not None else Scene()
+    rng = Rng(seed)
+    half = (max(50.0, extent[0] / 2 + 5), 0.5, max(50.0, extent[2] / 2 + 5))
+    _floor(sc, half=half)
+
+    def leaf():
+        k = min(4, int(rng.uniform() * 5))
+        if k == 0:
+            return S.box((rng.uniform(0.15, 0.35), rng.uniform(0.15, 0.35), rng.uniform(0.15, 0.35)))
+        if k == 1:
+            return S.sphere(rng.uniform(0.15, 0.3))
+        if k == 2:
+            return S.capsule(rng.uniform(0.1, 0.2), rng.uniform(0.1, 0.3))
+        if k == 3:
+            return S.cylinder(rng.uniform(0.15, 0.3), rng.uniform(0.15, 0.3))
+        return S.convex_hull(_unit_sphere_points(rng, 12) * rng.uniform(0.2, 0.4))
+
+    for _ in range(n):
+        if rng.uniform() < 2.0 / 3.0:
+            kids = []
+            for _c in range(2 + min(2, int(rng.uniform() * 3))):
+                ch = leaf()
+                if rng.uniform() < 0.25:
+                    ch = S.scaled(ch, (1.3, 1.3, 1.3) if ch.type in (S.SPHERE, S.CAPSULE, S.CYLINDER)
+                                  else (rng.uniform(0.7, 1.4), rng.uniform(0.7, 1.4), rng.uniform(0.7, 1.4)))
+                kids.append(((rng.uniform(-0.4, 0.4), rng.uniform(-0.4, 0.4), rng.uniform(-0.4, 0.4)), rng.quaternion(), ch))
+            shape = S.compound(kids)
+        elif rng.uniform() < 0.5:
+            shape = S.box((rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.5)))
+        else:
+            shape = S.sphere(rng.uniform(0.2, 0.5))
         x = rng.uniform(-extent[0] / 2, extent[0] / 2)
         y = rng.uniform(1.0, 1.0 + extent[1])
         z = rng.uniform(-extent[2] / 2, extent[2] / 2)

@@ -30,6 +30,7 @@ CONVEX_HULL = _abi.CC_SHAPE_CONVEX_HULL
 TRIANGLE_MESH = _abi.CC_SHAPE_TRIANGLE_MESH
 CYLINDER = _abi.CC_SHAPE_CYLINDER
 PLANE = _abi.CC_SHAPE_PLANE
+COMPOUND = _abi.CC_SHAPE_COMPOUND
 DEFAULT_PLANE_HALF_EXTENT = 1000.0  # src/shapes/plane.ts
 
 
@@ -71,10 +72,12 @@ class Shape:
         # the decorator this shape is, as crashcat holds it: (kind, parameters, inner shape), None
         # for an undecorated shape
         self.wrap = kw.get("wrap")
+        # a compound's children: (position, quaternion, shape)
+        self.children = kw.get("children")
 
     def key(self):
         """Hashable identity used to deduplicate identical shapes at upload."""
-        if self.type in (CONVEX_HULL, TRIANGLE_MESH) or self.decorated:
+        if self.type in (CONVEX_HULL, TRIANGLE_MESH, COMPOUND) or self.decorated:
             return (self.type, id(self))
         return (self.type, self.radius, self.half_height, self.half_extents, self.convex_radius, self.density)
 
@@ -515,6 +518,54 @@ def offset_center_of_mass(shape, offset):
     I = np.asarray(shape.inertia, dtype=np.float64) + shape.mass * (np.eye(3) * float(o @ o) - np.outer(o, o))
     return _leaf_copy(shape, center_of_mass=com, inertia=I, decorated=shape.decorated | _abi.CC_DECO_OFFSET_COM,
                       wrap=("offset", tuple(float(v) for v in o), shape))
+
+
+def compound(children):
+    """compound.create({children}) — src/shapes/compound.ts:29-150: `children` is a list of
+    (position, quaternion, shape).  Bounds of the children's transformed bounds' corners; centre
+    of mass the mass-weighted child centres (position + rotate(quaternion, child centre)); mass the
+    sum; inertia each child's rotated by its quaternion and moved to the compound's centre by the
+    parallel-axis theorem (:169-227).  Children are convex leaves, scaled or offset allowed, not
+    themselves transformed or compound (a child's transform is applied once)."""
+    if not children:
+        raise ValueError("CompoundShape must have at least one child")
+    kids = []
+    for pos, quat, sh in children:
+        _check_leaf(sh)
+        if sh.type == COMPOUND or sh.transformed:
+            raise ValueError("compound children are convex leaves (scaled or offset allowed), not transformed or compound")
+        kids.append((tuple(float(v) for v in pos), tuple(float(v) for v in quat), sh))
+    lo = [math.inf] * 3
+    hi = [-math.inf] * 3
+    for pos, q, sh in kids:
+        a, b = sh.aabb_min, sh.aabb_max
+        for x in range(2):
+            for y in range(2):
+                for z in range(2):
+                    c = _qrotate(q, (a[0] if x == 0 else b[0], a[1] if y == 0 else b[1], a[2] if z == 0 else b[2]))
+                    c = (c[0] + pos[0], c[1] + pos[1], c[2] + pos[2])
+                    for k in range(3):
+                        lo[k] = min(lo[k], c[k])
+                        hi[k] = max(hi[k], c[k])
+    com = [0.0, 0.0, 0.0]
+    tot = 0.0
+    for pos, q, sh in kids:
+        r = _qrotate(q, sh.center_of_mass)
+        w = (pos[0] + r[0], pos[1] + r[1], pos[2] + r[2])
+        for k in range(3):
+            com[k] += w[k] * sh.mass
+        tot += sh.mass
+    if tot > 0:
+        com = [c / tot for c in com]
+    I = np.zeros((3, 3))
+    for pos, q, sh in kids:
+        r = _qrotate(q, sh.center_of_mass)
+        rel = np.array([pos[0] + r[0] - com[0], pos[1] + r[1] - com[1], pos[2] + r[2] - com[2]])
+        R = _mat3_from_quat(q)
+        J = (R @ np.asarray(sh.inertia, dtype=np.float64)) @ R.T
+        I += J + sh.mass * (np.eye(3) * float(rel @ rel) - np.outer(rel, rel))
+    return Shape(COMPOUND, density=kids[0][2].density, center_of_mass=tuple(com), aabb_min=tuple(lo),
+                 aabb_max=tuple(hi), mass=tot, inertia=I, volume=sum(sh.volume for _, _, sh in kids), children=kids)
 
 
 def scaled_shrunk_points(points, normals, point_faces, cr, scale):

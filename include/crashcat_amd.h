@@ -41,6 +41,10 @@ extern "C" {
 #define CC_SHAPE_CYLINDER 5      /* src/shapes/cylinder.ts: radius, half_height, convex_radius; axis y */
 #define CC_SHAPE_PLANE 6         /* src/shapes/plane.ts (static bodies only): half_extents = unit normal,
                                     radius = constant, half_height = half extent of the bounded form */
+#define CC_SHAPE_COMPOUND 7      /* src/shapes/compound.ts: children are the shape descs
+                                    [compound_first_child, + compound_num_children), each a convex
+                                    leaf (scaled allowed) whose local transform (CC_DECO_TRANSFORMED)
+                                    is the child's position and quaternion */
 
 /* ---- motion types: src/body/motion-type.ts ------------------------------ */
 #define CC_MOTION_STATIC 0
@@ -134,6 +138,11 @@ typedef struct cc_shape_desc {
     double scale[3];
     double local_position[3];
     double local_quaternion[4];
+    /* compounds (CC_SHAPE_COMPOUND): the children are shapes [first, first + num) of this array,
+     * collided in index order with sub-shape ids pushIndex(child, num) (compound.ts:494-640);
+     * center_of_mass and aabb_* are the compound's (compound.ts:60-150) */
+    int32_t compound_first_child;
+    int32_t compound_num_children;
 } cc_shape_desc;
 
 /* Flattened convex-hull data shared by all hull shapes.  Points are relative to the

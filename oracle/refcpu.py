@@ -117,7 +117,7 @@ class OracleWorld:
         self._settings = scene.settings
         self.n = b1 - b0
         self.h = L.oc_create(C.byref(scene.settings), bodies.ctypes.data_as(C.POINTER(_abi.BodyDesc)), self.n,
-                             shapes, len(scene.shapes), C.byref(hulls), pa, len(pcs), C.byref(self._keep["meshes"]))
+                             shapes, self._keep["num_shapes"], C.byref(hulls), pa, len(pcs), C.byref(self._keep["meshes"]))
 
     def __del__(self):
         try:

@@ -760,6 +760,37 @@ static void mesh_hit(void* ctx, const OHit* h, uint32_t subTri) {
     r.faceA = h->faceB; r.faceB = h->faceA;
     collector_add(C, &r, subTri, EMPTY_SUB);
 }
+/* collideCompoundVsShape / collideShapeVsCompound (src/shapes/compound.ts:494-640): every child
+ * of body A's compound, then of body B's, in index order, its sub-shape id pushed with
+ * pushIndex(child, count) (src/body/sub-shape.ts:43-64); a child carries its position and
+ * quaternion as a local transform, which ocx_collide composes with the body pose as the
+ * compound does.  Each leaf pair's hit goes to the reduction collector. */
+static uint32_t sub_push(uint32_t value, int bit, int id, int nbits) {
+    const uint32_t mask = nbits ? (((1u << nbits) - 1u) << bit) : 0u;
+    return (value & ~mask) | ((uint32_t)id << bit);
+}
+static void collide_compound(oc_world* w, OCollector* C, int sa, uint32_t subA, int bitA, V3 pa, Q4 qa, int sb,
+                             uint32_t subB, int bitB, V3 pb, Q4 qb) {
+    const cc_shape_desc* A = &w->shapes[sa];
+    const cc_shape_desc* B = &w->shapes[sb];
+    if (A->type == CC_SHAPE_COMPOUND || B->type == CC_SHAPE_COMPOUND) {
+        const int onA = A->type == CC_SHAPE_COMPOUND;
+        const cc_shape_desc* K = onA ? A : B;
+        const int n = K->compound_num_children;
+        const int nbits = n > 1 ? 32 - __builtin_clz((unsigned)(n - 1)) : 0;
+        for (int i = 0; i < n; i++) {
+            const int child = K->compound_first_child + i;
+            if (onA) collide_compound(w, C, child, sub_push(subA, bitA, i, nbits), bitA + nbits, pa, qa, sb, subB, bitB, pb, qb);
+            else collide_compound(w, C, sa, subA, bitA, pa, qa, child, sub_push(subB, bitB, i, nbits), bitB + nbits, pb, qb);
+        }
+        return;
+    }
+    OHit h;
+    ocx_collide(&w->sh, sa, pa, qa, sb, pb, qb, w->s.speculative_contact_distance, w->s.collision_tolerance,
+                w->s.penetration_tolerance, &h);
+    if (h.hit) collector_add(C, &h, subA, subB);
+}
+
 static int narrowphase_pair(oc_world* w, int ia, int ib, double dt) {
     OBody* A = &w->b[ia];
     OBody* B = &w->b[ib];
@@ -787,6 +818,8 @@ static int narrowphase_pair(oc_world* w, int ia, int ib, double dt) {
             C->reversed = 1;
             ocx_collide_mesh(&w->sh, B->shape, B->pos, B->q, A->shape, A->pos, A->q, &w->s, maxSep, move, mesh_hit, C);
         }
+    } else if (ta == CC_SHAPE_COMPOUND || tb == CC_SHAPE_COMPOUND) {
+        collide_compound(w, C, A->shape, EMPTY_SUB, 0, A->pos, A->q, B->shape, EMPTY_SUB, 0, B->pos, B->q);
     } else {
         OHit h;
         ocx_collide(&w->sh, A->shape, A->pos, A->q, B->shape, B->pos, B->q, w->s.speculative_contact_distance,

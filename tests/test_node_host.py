@@ -223,11 +223,12 @@ def _crashcat_scenes():
             "mixed": SC.mixed_pile(seed=1002, n=60, extent=(6.0, 6.0, 6.0), cylinders=True),
             "chains": chains,
             "terrain": SC.terrain_drop(vertices=16, n=24, drop=(0.5, 2.5)),
-            "decorated": SC.decorated_pile(seed=1011, n=60, extent=(6.0, 4.0, 6.0))}
+            "decorated": SC.decorated_pile(seed=1011, n=60, extent=(6.0, 4.0, 6.0)),
+            "compound": SC.compound_pile(seed=1014, n=50, extent=(6.0, 4.0, 6.0))}
 
 
 @needs_node
-@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated"])
+@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated", "compound"])
 def test_crashcat_world_upload_equals_the_scene_description(tmp_path, which):
     """The adapter reads crashcat's own objects (World settings, RigidBody pool with
     motionProperties, shape records, point-constraint pool) and produces exactly the upload the
@@ -266,7 +267,8 @@ process.stdout.write(JSON.stringify(cw.describe(world)));
 
 @needs_node
 @pytest.mark.parametrize("what,needle", [
-    ("sensor", "is a sensor"), ("ccd", "LINEAR_CAST"), ("compound", "COMPOUND"), ("hinge", "constraint type 2"),
+    ("sensor", "is a sensor"), ("ccd", "LINEAR_CAST"), ("compound", "compound children must be convex"),
+    ("empty", "EMPTY"), ("hinge", "constraint type 2"),
     ("layers", "object layers"), ("reduction", "manifold reduction"), ("ids", "not in index order"),
     ("moving_mesh", "triangle meshes must be static")])
 def test_crashcat_world_unsupported_features_are_refused(tmp_path, what, needle):
@@ -279,8 +281,14 @@ def test_crashcat_world_unsupported_features_are_refused(tmp_path, what, needle)
         b[1]["sensor"] = True
     elif what == "ccd":
         b[1]["motionProperties"]["motionQuality"] = 1
-    elif what == "compound":
-        d["shapes"].append({"type": 5, "aabb": [[0, 0, 0], [1, 1, 1]], "centerOfMass": [0, 0, 0]})
+    elif what == "compound":  # a compound inside a compound
+        inner = {"type": 5, "aabb": [[0, 0, 0], [1, 1, 1]], "centerOfMass": [0, 0, 0],
+                 "children": [{"position": [0, 0, 0], "quaternion": [0, 0, 0, 1], "shape": d["shapes"][b[1]["shape"]]}]}
+        d["shapes"].append({"type": 5, "aabb": [[0, 0, 0], [1, 1, 1]], "centerOfMass": [0, 0, 0],
+                            "children": [{"position": [0, 0, 0], "quaternion": [0, 0, 0, 1], "shape": inner}]})
+        b[1]["shape"] = len(d["shapes"]) - 1
+    elif what == "empty":
+        d["shapes"].append({"type": 8, "aabb": [[0, 0, 0], [0, 0, 0]], "centerOfMass": [0, 0, 0]})
         b[1]["shape"] = len(d["shapes"]) - 1
     elif what == "hinge":
         d["constraints"]["pools"]["2"] = {"type": 2, "constraints": [{"_pooled": False}]}
@@ -310,7 +318,7 @@ try {{ cc.fromCrashcatWorld(world); process.stdout.write('accepted'); }} catch (
 
 @needs_node
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated"])
+@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated", "compound"])
 def test_crashcat_world_steps_bitwise_with_events_and_forces(tmp_path, which):
     """updateWorld on a crashcat-shaped World: FP64, bit for bit against the oracle in the state
     written back into the RigidBody objects; per step the listener's events (with the body
