@@ -24,6 +24,8 @@ CC_ORDER_REFERENCE = 0
 CC_ORDER_COLORED = 1
 CC_MOTION_KINEMATIC = 1
 CC_MOTION_DYNAMIC = 2
+CC_MOTION_QUALITY_DISCRETE = 0
+CC_MOTION_QUALITY_LINEAR_CAST = 1
 
 CC_PRECISION_FP32 = 0
 CC_PRECISION_FP64 = 1
@@ -58,6 +60,9 @@ class Settings(C.Structure):
         ("object_layer_pairs", C.c_uint8 * (CC_MAX_OBJECT_LAYERS * CC_MAX_OBJECT_LAYERS)),
         ("collide_with_backfaces", C.c_int32),
         ("collide_only_with_active_edges", C.c_int32),
+        ("ccd_linear_cast_threshold", C.c_double),
+        ("ccd_linear_cast_max_penetration", C.c_double),
+        ("ccd_min_velocity_for_restitution", C.c_double),
     ]
 
 
@@ -194,6 +199,22 @@ class ContactOut(C.Structure):
         ("lambda_n", D4),
         ("lambda_t1", D4),
         ("lambda_t2", D4),
+    ]
+
+
+class CcdBody(C.Structure):
+    _fields_ = [
+        ("body_index", C.c_int32),
+        ("hit_body_index", C.c_int32),
+        ("delta_position", D3),
+        ("contact_normal", D3),
+        ("contact_point", D3),
+        ("fraction", C.c_double),
+        ("fraction_plus_slop", C.c_double),
+        ("linear_cast_threshold_sq", C.c_double),
+        ("max_penetration", C.c_double),
+        ("combined_friction", C.c_double),
+        ("combined_restitution", C.c_double),
     ]
 
 

@@ -14,7 +14,7 @@ _LIB = None
 SYMBOLS = [
     "cc_create", "cc_destroy", "cc_last_error", "cc_upload_scene", "cc_step", "cc_sync", "cc_download_bodies",
     "cc_set_bodies", "cc_download_pairs", "cc_download_contacts", "cc_get_counters", "cc_broadphase_aabbs",
-    "cc_get_phase_times", "cc_get_totals", "cc_get_solver_residual", "cc_build_triangle_mesh", "cc_download_removed_contacts",
+    "cc_get_phase_times", "cc_get_totals", "cc_get_solver_residual", "cc_build_triangle_mesh", "cc_download_removed_contacts", "cc_download_ccd_bodies",
     "cc_snapshot", "cc_restore", "cc_add_forces", "cc_download_removed_contacts_noalloc", "cc_fwd", "cc_bwd",
     "cc_set_contact_cache", "cc_download_contact_cache", "cc_set_sleep_state", "cc_set_point_impulses",
 ]
@@ -49,6 +49,7 @@ def lib():
     L.cc_download_pairs.argtypes = [vp, ip, C.c_int32, ip]
     L.cc_download_contacts.argtypes = [vp, C.POINTER(_abi.ContactOut), C.c_int32, ip]
     L.cc_download_removed_contacts.argtypes = [vp, C.POINTER(_abi.ContactRemoved), C.c_int32, ip]
+    L.cc_download_ccd_bodies.argtypes = [vp, C.POINTER(_abi.CcdBody), C.c_int32, ip]
     L.cc_download_removed_contacts_noalloc.argtypes = [vp, C.POINTER(_abi.ContactRemoved), C.c_int32, ip]
     L.cc_snapshot.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.cc_restore.argtypes = [vp, vp, C.c_int64]
@@ -74,6 +75,7 @@ def lib():
 BODY_STATE_DTYPE = np.dtype(_abi.BodyState)
 CONTACT_DTYPE = np.dtype(_abi.ContactOut)
 REMOVED_DTYPE = np.dtype(_abi.ContactRemoved)
+CCD_DTYPE = np.dtype(_abi.CcdBody)
 # the step's phases, in the order cc_get_phase_times reports them
 PHASES = ["accel", "grid", "cell_pairs", "pair_layout", "filter", "gjk", "epa", "manifold", "manifold_large", "islands",
           "solver", "cache"]
@@ -221,6 +223,16 @@ class Context:
         out = np.zeros(max(cnt.value, 1), dtype=REMOVED_DTYPE)
         self._check(lib().cc_download_removed_contacts(self.h, out.ctypes.data_as(C.POINTER(_abi.ContactRemoved)),
                                                        cnt.value, C.byref(cnt)))
+        return out[:cnt.value]
+
+    def ccd_bodies(self):
+        """The last step's CCD records (world.ccd.ccdBodies after updateWorld, src/update.ts:131-140),
+        each world's in time-of-impact order."""
+        cnt = C.c_int32()
+        self._check(lib().cc_download_ccd_bodies(self.h, None, 0, C.byref(cnt)))
+        out = np.zeros(max(cnt.value, 1), dtype=CCD_DTYPE)
+        self._check(lib().cc_download_ccd_bodies(self.h, out.ctypes.data_as(C.POINTER(_abi.CcdBody)), cnt.value,
+                                                 C.byref(cnt)))
         return out[:cnt.value]
 
     def counters(self):

@@ -143,6 +143,18 @@ struct Dev {
     // compound pairs (compound.ts): listed by the filter, each solved by one thread that owns a
     // reduction collector (cmpColl) and an EPA polytope (cmpEpa, one block per 64 threads)
     int hasCompound;
+    // CCD (MotionQuality.LINEAR_CAST): per world the range of its records [ccdStart[w],
+    // ccdStart[w + 1]) (one per dynamic LINEAR_CAST body), the records in body order and their
+    // time-of-impact order, each body's record this step (-1 for none), the two constraint parts
+    // of each world and each body's shape inner radius
+    int hasCcd;
+    int* ccdStart;
+    int* ccdCount;  // per world: the records of the last step
+    CcdRec<T>* ccd;
+    int* ccdOrd;
+    int* ccdIdx;
+    CcdPart<T>* ccdPart;
+    T* ccdInner;
     int* cmpList;       // [pairCap]
     int cmpWorkers;
     char* cmpColl;

@@ -9,7 +9,7 @@
 // share no dynamic body, so a level is solved in parallel and levels run in order;
 // every body therefore sees exactly the serial sequence of updates.
 #pragma once
-#include "cc_kernels.h"
+#include "cc_ccd.h"
 
 namespace cc {
 
@@ -774,8 +774,57 @@ __device__ inline int isl_iters(const Dev<T>& d, int label) {
     return v | (p << 16);
 }
 
-// TAPE: differentiable stepping records the staged constraints' sweeps for the adjoint
-template <typename T, int BS, bool TAPE = false>
+// velocity integration of one body (update.ts:896-976); with `ccd`, a dynamic LINEAR_CAST body
+// whose displacement exceeds its threshold is only rotated (setQuaternion) and its record is
+// returned for the CCD pass instead of moving it (:932-967)
+template <typename T>
+__device__ inline bool integrate_body(const Dev<T>& d, int i, T dt, bool ccd, CcdRec<T>& rec) {
+    const Bodies<T>& B = d.B;
+    int f = B.flags[i];
+    int mo = fl_motion(f);
+    V3<T> v = B.v[i], wv = B.w[i];
+    if (mo == MOTION_DYNAMIC) {
+        T ml = B.maxLinV[i], ma = B.maxAngV[i];
+        if (len2(v) > ml * ml) { v = normalize(v) * ml; B.v[i] = v; }
+        if (len2(wv) > ma * ma) { wv = normalize(wv) * ma; B.w[i] = wv; }
+    }
+    V3<T> rv = wv * dt;
+    T ang = vlen(rv);
+    Q4<T> q = B.q[i];
+    if (ang > T(1e-6)) {
+        V3<T> axis = rv * (T(1) / ang);
+        q = qnormalize(qmul(qaxis_angle(axis, ang), q));
+        B.q[i] = q;
+    }
+    const V3<T> disp = v * dt;
+    if (ccd && (f & FL_LINEAR_CAST) && mo == MOTION_DYNAMIC) {
+        const T inner = d.ccdInner[i];
+        const T thr = d.s.ccdThr * inner;
+        if (len2(disp) > thr * thr) {
+            rec.body = i;
+            rec.hit = -1;
+            rec.delta = disp;
+            rec.normal = mk3<T>(0, 0, 0);
+            rec.point = mk3<T>(0, 0, 0);
+            rec.fraction = T(1);
+            rec.fps = T(1);
+            rec.thrSq = thr * thr;
+            rec.maxPen = tmin<T>(d.s.slop, d.s.ccdMaxPen * inner);
+            rec.friction = T(0);
+            rec.restitution = T(0);
+            set_transform(B, d.st, i);
+            return true;
+        }
+    }
+    B.com[i] = B.com[i] + disp;
+    pos_from_com(B, d.st, i);
+    set_transform(B, d.st, i);
+    return false;
+}
+
+// TAPE: differentiable stepping records the staged constraints' sweeps for the adjoint;
+// CCD: the world may hold LINEAR_CAST bodies (the CCD pass after integration)
+template <typename T, int BS, bool TAPE = false, bool CCD = false>
 __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
     __shared__ int sh[BS / 64 + 1];
     __shared__ int sLS[LS_LDS];
@@ -1122,27 +1171,48 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         }
     }
     // ---- 8. velocity integration (update.ts:896-976)
-    for (int i = b0 + tid; i < b1; i += BS) {
-        int f = B.flags[i];
-        int mo = fl_motion(f);
-        if (mo == MOTION_STATIC || B.sleeping[i] || !mine(i)) continue;
-        V3<T> v = B.v[i], wv = B.w[i];
-        if (mo == MOTION_DYNAMIC) {
-            T ml = B.maxLinV[i], ma = B.maxAngV[i];
-            if (len2(v) > ml * ml) { v = normalize(v) * ml; B.v[i] = v; }
-            if (len2(wv) > ma * ma) { wv = normalize(wv) * ma; B.w[i] = wv; }
+    bool ccdWorld = false;
+    if constexpr (CCD) ccdWorld = d.ccdStart[w + 1] > d.ccdStart[w];
+    if (!ccdWorld) {
+        for (int i = b0 + tid; i < b1; i += BS) {
+            int mo = fl_motion(B.flags[i]);
+            if (mo == MOTION_STATIC || B.sleeping[i] || !mine(i)) continue;
+            CcdRec<T> rec;
+            integrate_body(d, i, dt, false, rec);
         }
-        V3<T> rv = wv * dt;
-        T ang = vlen(rv);
-        Q4<T> q = B.q[i];
-        if (ang > T(1e-6)) {
-            V3<T> axis = rv * (T(1) / ang);
-            q = qnormalize(qmul(qaxis_angle(axis, ang), q));
-            B.q[i] = q;
+    } else if constexpr (CCD) {
+        // ---- 8b. CCD (update.ts:131-140): the records in body order (ccd.clear first, src/ccd.ts:107-123),
+        // then findCCDContacts, sortCCDBodies and resolveCCDContacts
+        const int cb0 = d.ccdStart[w];
+        int nccd = 0;
+        for (int base = b0; base < b1; base += BS) {
+            const int i = base + tid;
+            bool reg = false;
+            CcdRec<T> rec;
+            if (i < b1) {
+                const int mo = fl_motion(B.flags[i]);
+                if (mo != MOTION_STATIC && !B.sleeping[i]) reg = integrate_body(d, i, dt, true, rec);
+            }
+            int tot;
+            const int r = block_excl_scan<BS>(reg ? 1 : 0, sh, tot);
+            if (reg) {
+                d.ccd[cb0 + nccd + r] = rec;
+                d.ccdIdx[i] = nccd + r;
+            } else if (i < b1) {
+                d.ccdIdx[i] = -1;
+            }
+            nccd += tot;
         }
-        B.com[i] = B.com[i] + v * dt;
-        pos_from_com(B, d.st, i);
-        set_transform(B, d.st, i);
+        if (tid == 0) d.ccdCount[w] = nccd;
+        __syncthreads();
+        if (nccd > 0) {
+            using C = typename std::conditional<sizeof(T) == 4, double, T>::type;
+            for (int e = 0; e < nccd; e++) {
+                if (d.npWide) ccd_find<T, C, BS>(d, d.ccd, d.ccdIdx, cb0, e, b0, b1, dt);
+                else ccd_find<T, T, BS>(d, d.ccd, d.ccdIdx, cb0, e, b0, b1, dt);
+            }
+            ccd_resolve<T, BS>(d, d.ccd, d.ccdOrd, d.ccdIdx, d.ccdPart + 2 * (size_t)w, cb0, nccd);
+        }
     }
     for (int i = b0 + tid; i < b1; i += BS) {
         if (!mine(i)) continue;

@@ -14,6 +14,24 @@ CC_HD int fl_fmode(int f) { return (f >> 9) & 7; }
 CC_HD int fl_rmode(int f) { return (f >> 12) & 7; }
 CC_HD int fl_layer(int f) { return (f >> 16) & 255; }
 
+// ---------------------------------------------------------------- CCD records
+constexpr int FL_LINEAR_CAST = 1 << 24;  // Bodies::flags: motionQuality == LINEAR_CAST
+
+// CCDBody (src/ccd.ts:19-60)
+template <typename T>
+struct CcdRec {
+    int body, hit;
+    V3<T> delta, normal, point;
+    T fraction, fps, thrSq, maxPen, friction, restitution;
+};
+// the AxisConstraintPart applyCCD keeps from one call to the next (module scope in
+// src/update.ts:1453-1454): a zero record at upload, then whatever the last call left
+template <typename T>
+struct CcdPart {
+    V3<T> r1x, r2x, i1, i2;
+    T eff, lambda, bias;
+};
+
 template <typename T>
 struct SettingsD {
     T gravity[3];
@@ -23,6 +41,7 @@ struct SettingsD {
     T normalCos;           // manifolds merge hits whose normals are within this cosine
     int backfaces;         // triangle meshes: collide with back faces
     int activeEdges;       // triangle meshes: correct normals at inactive edges
+    T ccdThr, ccdMaxPen, ccdMinVelRest;  // ccd.linearCastThreshold, .linearCastMaxPenetration, .minVelocityForRestitution
 };
 
 template <typename T>

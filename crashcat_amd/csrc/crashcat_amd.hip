@@ -207,6 +207,8 @@ struct CtxT : Impl {
     int worldLdsBS = 0;        // block size worldLdsBytes was sized for
     size_t tapeLdsBytes = 0;   // the same for the recording solver (block size 256)
     bool tapeLdsSet = false;
+    size_t ccdLdsBytes = 0;    // the same for the CCD variant (block size 256)
+    bool ccdLdsSet = false;
     int bodyBS = 256;          // block size of the one-thread-per-body kernels
     bool tuned = false;        // the autotuner has chosen the block sizes
     int tuneTries = 0;         // steps the autotuner waited for contacts
@@ -567,6 +569,9 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     d.s.normalCos = T(s->normal_cos_max_delta_rotation);
     d.s.backfaces = s->collide_with_backfaces;
     d.s.activeEdges = s->collide_only_with_active_edges;
+    d.s.ccdThr = T(s->ccd_linear_cast_threshold);
+    d.s.ccdMaxPen = T(s->ccd_linear_cast_max_penetration);
+    d.s.ccdMinVelRest = T(s->ccd_min_velocity_for_restitution);
     // shapes
     bool anyDeco = false, anyMeshOrPlane = false;
     for (int i = 0; i < ns; i++) {
@@ -692,8 +697,8 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
         if (shapes[b.shape].type == CC_SHAPE_PLANE && b.motion_type != CC_MOTION_STATIC)
             return fail(c, CC_E_UNSUPPORTED, "plane bodies must be static (plane.ts:53-57)");
         if (b.sensor) return fail(c, CC_E_UNSUPPORTED, "sensor bodies are not on the GPU path (body " + std::to_string(i) + ")");
-        if (b.motion_quality != CC_MOTION_QUALITY_DISCRETE)
-            return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is not on the GPU path (body " + std::to_string(i) + ")");
+        if (b.motion_quality != CC_MOTION_QUALITY_DISCRETE && b.motion_quality != CC_MOTION_QUALITY_LINEAR_CAST)
+            return fail(c, CC_E_ARG, "invalid motion quality (body " + std::to_string(i) + ")");
         if (b.num_velocity_steps_override < 0 || b.num_velocity_steps_override > 255 || b.num_position_steps_override < 0 ||
             b.num_position_steps_override > 255)
             return fail(c, CC_E_ARG, "iteration overrides must be in [0, 255]");
@@ -719,7 +724,8 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
         ma[i] = T(b.max_angular_velocity);
         shp[i] = b.shape;
         flags[i] = (b.motion_type & 3) | ((b.allowed_dofs & 63) << 2) | ((b.allow_sleeping ? 1 : 0) << 8) |
-                   ((b.friction_combine & 7) << 9) | ((b.restitution_combine & 7) << 12) | ((b.object_layer & 255) << 16);
+                   ((b.friction_combine & 7) << 9) | ((b.restitution_combine & 7) << 12) | ((b.object_layer & 255) << 16) |
+                   (b.motion_quality == CC_MOTION_QUALITY_LINEAR_CAST ? FL_LINEAR_CAST : 0);
         slp[i] = b.motion_type == CC_MOTION_STATIC ? 0 : (b.sleeping ? 1 : 0);
         grp[i] = b.collision_groups;
         msk[i] = b.collision_mask;
@@ -743,6 +749,59 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
     B.stimer = A.get<T>(nb);
     B.label = A.get<int>(nb);
     B.wake = A.get<int>(nb);
+    // CCD: one record per dynamic LINEAR_CAST body, in a world of undecorated convex leaves (the
+    // shape cast of castConvexVsConvex, convex.ts:675-875)
+    std::vector<int> ccdStart(nw + 1, 0);
+    {
+        std::vector<T> inner(nb, T(0));
+        for (int w = 0; w < nw; w++) {
+            int n = 0;
+            for (int i = woff[w]; i < woff[w + 1]; i++)
+                n += (bodies[i].motion_quality == CC_MOTION_QUALITY_LINEAR_CAST && bodies[i].motion_type == CC_MOTION_DYNAMIC) ? 1 : 0;
+            ccdStart[w + 1] = ccdStart[w] + n;
+            if (n == 0) continue;
+            for (int i = woff[w]; i < woff[w + 1]; i++) {
+                const cc_shape_desc& q = shapes[bodies[i].shape];
+                const bool convex = q.type == CC_SHAPE_SPHERE || q.type == CC_SHAPE_BOX || q.type == CC_SHAPE_CAPSULE ||
+                                    q.type == CC_SHAPE_CONVEX_HULL || q.type == CC_SHAPE_CYLINDER;
+                if (!convex || q.decorated != 0)
+                    return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is on the GPU path in worlds of undecorated "
+                                                     "convex shapes only (world " + std::to_string(w) + ", body " + std::to_string(i) + ")");
+                // getShapeInnerRadius: sphere.ts:147-149, box.ts:333-335, capsule.ts:254-256,
+                // cylinder.ts:261-263, convex-hull.ts:550-558 (the nearest face plane)
+                double r = 0;
+                if (q.type == CC_SHAPE_SPHERE || q.type == CC_SHAPE_CAPSULE) r = q.radius;
+                else if (q.type == CC_SHAPE_BOX) r = std::min(q.half_extents[0], std::min(q.half_extents[1], q.half_extents[2]));
+                else if (q.type == CC_SHAPE_CYLINDER) r = std::min(q.half_height, q.radius);
+                else {
+                    r = 1.7976931348623157e308;
+                    for (int f = 0; f < q.hull_num_faces; f++) r = std::min(r, -hulls->plane_constants[q.hull_first_face + f]);
+                    r = std::max(0.0, r);
+                }
+                inner[i] = T(r);
+            }
+        }
+        d.hasCcd = ccdStart[nw] > 0 ? 1 : 0;
+        if (d.hasCcd) {
+            const int NC = ccdStart[nw];
+            d.ccdStart = A.get<int>(nw + 1);
+            d.ccdCount = A.get<int>(nw);
+            d.ccd = A.get<CcdRec<T>>(NC);
+            d.ccdOrd = A.get<int>(NC);
+            d.ccdIdx = A.get<int>(nb);
+            d.ccdPart = A.get<CcdPart<T>>(2 * (size_t)nw);
+            d.ccdInner = A.get<T>(nb);
+            if (A.err) return fail(c, CC_E_HIP, "hipMalloc failed (CCD)");
+            std::vector<CcdPart<T>> zp(2 * (size_t)nw);
+            std::memset((void*)zp.data(), 0, zp.size() * sizeof(CcdPart<T>));
+            std::vector<int> none(nb, -1);
+            HIPCHK(c, hipMemcpy(d.ccdStart, ccdStart.data(), (nw + 1) * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemset(d.ccdCount, 0, nw * sizeof(int)));
+            HIPCHK(c, hipMemcpy(d.ccdPart, zp.data(), zp.size() * sizeof(CcdPart<T>), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(d.ccdIdx, none.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(d.ccdInner, inner.data(), nb * sizeof(T), hipMemcpyHostToDevice));
+        }
+    }
     d.N = nb;
     d.W = nw;
     int* dW = A.get<int>(nw + 1);
@@ -761,6 +820,7 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
             int g = 1;
             if (nw < 64) g = std::max(1, std::min((n + 63) / 64, std::max(1, 256 / nw)));
             if (c->cfg.island_groups > 0) g = std::min(g, (int)c->cfg.island_groups);
+            if (ccdStart[w + 1] > ccdStart[w]) g = 1;  // the CCD pass needs the whole world in one workgroup
             G[w] = g;
             vbase[w] = V;
             V += g;
@@ -1080,18 +1140,18 @@ enum {
 
 // the world kernel's dynamic LDS: what the largest world can use (its per-body counts and packed
 // in-degrees), within what the device leaves beside the kernel's static LDS
-template <typename T, int BS, bool TAPE>
+template <typename T, int BS, bool TAPE, bool CCD = false>
 static size_t world_lds_bytes(CtxT<T>* X, size_t needInts) {
     hipFuncAttributes fa;
     int maxShared = 0;
     size_t dyn = 0;
-    if (hipFuncGetAttributes(&fa, (const void*)k_world<T, BS, TAPE>) == hipSuccess &&
+    if (hipFuncGetAttributes(&fa, (const void*)k_world<T, BS, TAPE, CCD>) == hipSuccess &&
         hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, X->device) == hipSuccess &&
         (size_t)maxShared > fa.sharedSizeBytes) {
         size_t avail = ((size_t)maxShared - fa.sharedSizeBytes) / 4;
         dyn = std::min(avail, needInts) * 4;
     }
-    if (dyn > 0 && hipFuncSetAttribute((const void*)k_world<T, BS, TAPE>,
+    if (dyn > 0 && hipFuncSetAttribute((const void*)k_world<T, BS, TAPE, CCD>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
         dyn = 0;
     return dyn;
@@ -1107,6 +1167,14 @@ static void prepare_world(CtxT<T>* X, bool tape) {
             X->tapeLdsSet = true;
         }
         X->d.worldLdsInts = (int)(X->tapeLdsBytes / 4);
+        return;
+    }
+    if (X->d.hasCcd) {  // worlds with LINEAR_CAST bodies: the CCD variant at 256 threads
+        if (!X->ccdLdsSet) {
+            X->ccdLdsBytes = world_lds_bytes<T, 256, false, true>(X, X->worldLdsNeed);
+            X->ccdLdsSet = true;
+        }
+        X->d.worldLdsInts = (int)(X->ccdLdsBytes / 4);
         return;
     }
     if (X->worldLdsBS != X->solverBS) {
@@ -1131,6 +1199,10 @@ static void launch_world(CtxT<T>* X, hipStream_t st, T dt, bool tape) {
     }
     if (tape) {  // the recording solver of differentiable stepping (worlds are never split there)
         hipLaunchKernelGGL((k_world<T, 256, true>), dim3(d.V), dim3(256), X->tapeLdsBytes, st, d, dt);
+        return;
+    }
+    if (d.hasCcd) {
+        hipLaunchKernelGGL((k_world<T, 256, false, true>), dim3(d.V), dim3(256), X->ccdLdsBytes, st, d, dt);
         return;
     }
     const size_t lds = X->worldLdsBytes;
@@ -1407,6 +1479,8 @@ static int step_impl(cc_ctx* c, double dtd, int nsteps, bool tape = false) {
         c->nPhase = 0;
         return CC_OK;
     }
+    if (tape && d.hasCcd)
+        return fail(c, CC_E_UNSUPPORTED, "differentiable stepping of worlds with LINEAR_CAST (CCD) bodies is not supported");
     if (!tape && !d.diff && !X->tuned && nsteps > 0 && (c->cfg.block_solver == 0 || c->cfg.block_bodies == 0)) {
         const int rc = autotune<T>(c, X, dt);
         if (rc != CC_OK) return rc;
@@ -1614,7 +1688,9 @@ static std::vector<std::pair<void*, size_t>> snap_segments(Dev<T>& d) {
             {B.sr, 3 * n * sizeof(T)},      {B.stimer, n * sizeof(T)},        {B.label, n * sizeof(int)},
             {B.wake, n * sizeof(int)},      {B.flags, n * sizeof(int)},       {d.vb, n * sizeof(VBody<T>)},
             {d.pb, n * sizeof(PBody<T>)},   {d.pc, (size_t)d.NPC * sizeof(PointCon<T>)},
-            {B.force, n * sizeof(V3<T>)},   {B.torque, n * sizeof(V3<T>)}};
+            {B.force, n * sizeof(V3<T>)},   {B.torque, n * sizeof(V3<T>)},
+            // the constraint parts applyCCD carries from one call to the next
+            {d.ccdPart, d.hasCcd ? 2 * (size_t)d.W * sizeof(CcdPart<T>) : 0}};
 }
 
 template <typename T>
@@ -2028,6 +2104,52 @@ extern "C" int cc_download_removed_contacts(cc_ctx* c, cc_contact_removed* out, 
     return removed_impl<float>(c, out, cap, count);
 }
 
+template <typename T>
+static int ccd_bodies_impl(cc_ctx* c, cc_ccd_body* out, int cap, int* count) {
+    auto* X = static_cast<CtxT<T>*>(c->impl);
+    Dev<T>& d = X->d;
+    *count = 0;
+    if (!d.hasCcd) return CC_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int nw = d.W;
+    std::vector<int> start(nw + 1), cnt(nw);
+    HIPCHK(c, hipMemcpy(start.data(), d.ccdStart, (nw + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(cnt.data(), d.ccdCount, nw * sizeof(int), hipMemcpyDeviceToHost));
+    const int NC = start[nw];
+    std::vector<CcdRec<T>> rec(NC);
+    std::vector<int> ord(NC);
+    HIPCHK(c, hipMemcpy(rec.data(), d.ccd, NC * sizeof(CcdRec<T>), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ord.data(), d.ccdOrd, NC * sizeof(int), hipMemcpyDeviceToHost));
+    int k = 0;
+    for (int w = 0; w < nw; w++)
+        for (int i = 0; i < cnt[w]; i++, k++) {
+            if (k >= cap || !out) continue;
+            const CcdRec<T>& r = rec[start[w] + ord[start[w] + i]];
+            cc_ccd_body& o = out[k];
+            o.body_index = r.body;
+            o.hit_body_index = r.hit;
+            const V3<T>* v[3] = {&r.delta, &r.normal, &r.point};
+            double* dst[3] = {o.delta_position, o.contact_normal, o.contact_point};
+            for (int a = 0; a < 3; a++) { dst[a][0] = v[a]->x; dst[a][1] = v[a]->y; dst[a][2] = v[a]->z; }
+            o.fraction = r.fraction;
+            o.fraction_plus_slop = r.fps;
+            o.linear_cast_threshold_sq = r.thrSq;
+            o.max_penetration = r.maxPen;
+            o.combined_friction = r.friction;
+            o.combined_restitution = r.restitution;
+        }
+    *count = k;
+    return CC_OK;
+}
+
+extern "C" int cc_download_ccd_bodies(cc_ctx* c, cc_ccd_body* out, int32_t cap, int32_t* count) {
+    if (!c || !c->impl || !count) return fail(c, CC_E_STATE, "no scene uploaded");
+    if (cap < 0) return fail(c, CC_E_ARG, "negative capacity");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == CC_PRECISION_FP64) return ccd_bodies_impl<double>(c, out, cap, count);
+    return ccd_bodies_impl<float>(c, out, cap, count);
+}
+
 // The removed contacts staged in the device's scratch array instead of per-call allocations; the
 // host sorts them in the caller's buffer.
 template <typename T>
@@ -2310,6 +2432,7 @@ static int fwd_impl(cc_ctx* c, double dt, int n, int K) {
         return fail(c, CC_E_UNSUPPORTED, "differentiable stepping solves each world on one workgroup: use "
                                          "island_groups = 1 or a batch of at least 64 worlds");
     if (d.NPC > 0) return fail(c, CC_E_UNSUPPORTED, "differentiable stepping has no point constraints");
+    if (d.hasCcd) return fail(c, CC_E_UNSUPPORTED, "differentiable stepping has no LINEAR_CAST (CCD) bodies");
     if (d.colored) return fail(c, CC_E_UNSUPPORTED, "differentiable stepping solves in crashcat's order (CC_ORDER_REFERENCE)");
     if (d.islIt) return fail(c, CC_E_UNSUPPORTED, "differentiable stepping has no iteration overrides");
     if (d.s.allowSleeping)

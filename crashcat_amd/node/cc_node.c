@@ -109,6 +109,8 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_settings, allow_sleeping, "i32", 1); F(cc_settings, num_object_layers, "i32", 1);
     F(cc_settings, object_layer_pairs, "u8", CC_MAX_OBJECT_LAYERS * CC_MAX_OBJECT_LAYERS);
     F(cc_settings, collide_with_backfaces, "i32", 1); F(cc_settings, collide_only_with_active_edges, "i32", 1);
+    F(cc_settings, ccd_linear_cast_threshold, "f64", 1); F(cc_settings, ccd_linear_cast_max_penetration, "f64", 1);
+    F(cc_settings, ccd_min_velocity_for_restitution, "f64", 1);
     add_struct(env, out, "cc_settings", sizeof(cc_settings), fields);
     napi_create_object(env, &fields);
     F(cc_shape_desc, type, "i32", 1); F(cc_shape_desc, hull_first_point, "i32", 1);

@@ -9,8 +9,9 @@
 // RigidBody objects.  Body indices on the GPU are the pool's indices (a removed body's slot
 // becomes an inert placeholder), so the sort keys, which crashcat computes from body indices
 // (contact-constraints.ts:1222-1231), are the same.  Features this path does not have are
-// refused with an Error before anything runs: sensors, CCD (MotionQuality.LINEAR_CAST),
-// compound/transformed/scaled/offset-centre-of-mass/empty/user shapes, moving triangle meshes,
+// refused with an Error before anything runs: sensors, CCD (MotionQuality.LINEAR_CAST) in a world
+// holding anything but undecorated convex shapes (the library refuses it), empty/user shapes,
+// compounds holding compounds, meshes or planes, moving triangle meshes,
 // constraint types other than points, enhanced internal-edge removal, manifold reduction turned
 // off, kinematic-vs-non-dynamic collision, returnDeepestPoint, a warm-start impulse ratio other
 // than 1, more than 8 object layers, and live bodies whose id order differs from their index
@@ -184,6 +185,9 @@ function settingsDesc(s) {
     object_layer_pairs: pairs,
     collide_with_backfaces: s.narrowphase.collideWithBackfaces ? 1 : 0,
     collide_only_with_active_edges: s.narrowphase.collideOnlyWithActiveEdges ? 1 : 0,
+    ccd_linear_cast_threshold: s.ccd.linearCastThreshold,
+    ccd_linear_cast_max_penetration: s.ccd.linearCastMaxPenetration,
+    ccd_min_velocity_for_restitution: s.ccd.minVelocityForRestitution,
   };
 }
 
@@ -297,7 +301,6 @@ function shapeTables(pool) {
 function bodyDesc(b, shapeIndex) {
   const m = b.motionProperties;
   if (b.sensor) fail(`body ${b.index} is a sensor; sensors are not on the GPU path`);
-  if (m.motionQuality === 1) fail(`body ${b.index} uses MotionQuality.LINEAR_CAST (CCD), which is not on the GPU path`);
   if (b.enhancedInternalEdgeRemoval) fail(`body ${b.index} asks for enhanced internal-edge removal, which is not on the GPU path`);
   if (b.useManifoldReduction === false) fail(`body ${b.index} turns manifold reduction off, which is not on the GPU path`);
   if (b.collideKinematicVsNonDynamic) fail(`body ${b.index} collides kinematic vs non-dynamic, which is not on the GPU path`);
@@ -317,7 +320,7 @@ function bodyDesc(b, shapeIndex) {
     force: vec(m.force, 3), torque: vec(m.torque, 3),
     num_velocity_steps_override: m.numVelocityStepsOverride || 0,
     num_position_steps_override: m.numPositionStepsOverride || 0,
-    sensor: 0, motion_quality: 0,
+    sensor: 0, motion_quality: m.motionQuality === 1 ? 1 : 0,
   };
 }
 

@@ -40,6 +40,9 @@ def create_world_settings():
     s.num_object_layers = 0
     s.collide_with_backfaces = 0
     s.collide_only_with_active_edges = 1
+    s.ccd_linear_cast_threshold = 0.05
+    s.ccd_linear_cast_max_penetration = 0.25
+    s.ccd_min_velocity_for_restitution = 1.0
     return s
 
 
@@ -107,9 +110,10 @@ class Scene:
                  max_linear_velocity=500.0, max_angular_velocity=0.25 * math.pi * 60.0, allow_sleeping=True,
                  allowed_dofs=0b111111, friction_combine=4, restitution_combine=3, collision_groups=0xFFFFFFFF,
                  collision_mask=0xFFFFFFFF, force=(0.0, 0.0, 0.0), torque=(0.0, 0.0, 0.0),
-                 num_velocity_steps_override=0, num_position_steps_override=0, sensor=False, motion_quality=0):
+                 num_velocity_steps_override=0, num_position_steps_override=0, sensor=False, motion_quality=0, sleeping=False):
         """rigidBody.create — src/body/rigid-body.ts:390-434, defaults :120-139; force / torque are
-        the accumulators addForce / addTorque fill before the first step."""
+        the accumulators addForce / addTorque fill before the first step; `sleeping` starts a
+        dynamic body asleep (rigidBody.sleep, src/body/sleep.ts:205-222)."""
         si = self.add_shape(shape)
         if object_layer is None:
             object_layer = self.layer_static if motion_type == _abi.CC_MOTION_STATIC else self.layer_moving
@@ -118,7 +122,8 @@ class Scene:
         else:
             inv_mass, inv_i, irot = S.motion_mass(shape.mass, shape.inertia, allowed_dofs)
         self.rows.append((si, motion_type, object_layer, collision_groups, collision_mask, int(allow_sleeping),
-                          allowed_dofs, friction_combine, restitution_combine, 0, tuple(position), tuple(quaternion),
+                          allowed_dofs, friction_combine, restitution_combine,
+                          int(bool(sleeping) and motion_type != _abi.CC_MOTION_STATIC), tuple(position), tuple(quaternion),
                           tuple(linear_velocity), tuple(angular_velocity), inv_mass, inv_i, irot, friction,
                           restitution, gravity_factor, linear_damping, angular_damping, max_linear_velocity,
                           max_angular_velocity, tuple(force), tuple(torque), int(num_velocity_steps_override),
@@ -393,7 +398,9 @@ class Scene:
                        "maxPenetrationDistance": st.max_penetration_distance},
             "sleeping": {"allowSleeping": bool(st.allow_sleeping), "timeBeforeSleep": st.time_before_sleep,
                          "pointVelocitySleepThreshold": st.point_velocity_sleep_threshold},
-            "ccd": {"linearCastThreshold": 0.05, "linearCastMaxPenetration": 0.25, "minVelocityForRestitution": 1.0},
+            "ccd": {"linearCastThreshold": st.ccd_linear_cast_threshold,
+                    "linearCastMaxPenetration": st.ccd_linear_cast_max_penetration,
+                    "minVelocityForRestitution": st.ccd_min_velocity_for_restitution},
         }
 
         def aabb(sh):
@@ -724,6 +731,75 @@ def compound_pile(seed=1012, n=200, extent=(12.0, 8.0, 12.0), scene=None):
         v = (rng.normal(0, 1), rng.normal(0, 1), rng.normal(0, 1))
         w = (rng.normal(0, 0.5), rng.normal(0, 0.5), rng.normal(0, 0.5))
         sc.add_body(shape, (x, y, z), rng.quaternion(), linear_velocity=v, angular_velocity=w)
+    sc.end_world()
+    return sc
+
+
+def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None):
+    """Fast LINEAR_CAST bodies (MotionQuality.LINEAR_CAST, src/update.ts:932-967) in a walled
+    range: a static floor, four thin static walls (0.1 m) around a 16 m square, one tilted thin
+    slab in the middle, and `targets` dynamic boxes, spheres, capsules, cylinders and 12-point hulls
+    resting on the floor (a third of them asleep).  The bullets (the same five shapes, 0.1-0.3 m)
+    start inside the walls at 30-150 m/s towards a random point on a wall or a target, with random
+    spin and restitution in [0, 0.6); pairs of bullets fly at each other, one kinematic LINEAR_CAST
+    paddle sweeps the range at 60 m/s, and a few slow LINEAR_CAST bodies stay below the cast
+    threshold."""
+    sc = scene if scene is not None else Scene()
+    rng = Rng(seed)
+    sc.add_body(S.box((12.0, 0.5, 12.0)), (0.0, -0.5, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+    for x, z, he in ((8.0, 0.0, (0.05, 3.0, 8.0)), (-8.0, 0.0, (0.05, 3.0, 8.0)),
+                     (0.0, 8.0, (8.0, 3.0, 0.05)), (0.0, -8.0, (8.0, 3.0, 0.05))):
+        sc.add_body(S.box(he), (x, 3.0, z), motion_type=_abi.CC_MOTION_STATIC)
+    sc.add_body(S.box((2.0, 1.5, 0.05)), (0.0, 1.5, 0.0), _quat_yaw(0.6), motion_type=_abi.CC_MOTION_STATIC)
+
+    def shape(lo, hi):
+        k = min(4, int(rng.uniform() * 5))
+        r = rng.uniform(lo, hi)
+        if k == 0:
+            return S.box((r, rng.uniform(lo, hi), rng.uniform(lo, hi)))
+        if k == 1:
+            return S.sphere(r)
+        if k == 2:
+            return S.capsule(rng.uniform(lo, hi), r * 0.6)
+        if k == 3:
+            return S.cylinder(rng.uniform(lo, hi), r)
+        return S.convex_hull(_unit_sphere_points(rng, 12) * r)
+
+    aims = []
+    for _ in range(targets):
+        p = (rng.uniform(-6.5, 6.5), rng.uniform(0.3, 0.6), rng.uniform(-6.5, 6.5))
+        if abs(p[0]) < 2.5 and abs(p[2]) < 1.5:
+            p = (p[0] + 5.0, p[1], p[2])
+        sc.add_body(shape(0.2, 0.4), p, rng.quaternion(), sleeping=rng.uniform() < 1.0 / 3.0)
+        aims.append(p)
+
+    def fire(pos, aim, speed, **kw):
+        d = np.array(aim, dtype=float) - np.array(pos, dtype=float)
+        d /= np.linalg.norm(d)
+        w = (rng.normal(0, 5), rng.normal(0, 5), rng.normal(0, 5))
+        return sc.add_body(shape(0.1, 0.3), pos, rng.quaternion(), linear_velocity=tuple(speed * d),
+                           angular_velocity=w, motion_quality=_abi.CC_MOTION_QUALITY_LINEAR_CAST,
+                           restitution=rng.uniform(0.0, 0.6), **kw)
+
+    for b in range(bullets):
+        pos = (rng.uniform(-6.0, 6.0), rng.uniform(0.5, 4.0), rng.uniform(-6.0, 6.0))
+        if rng.uniform() < 0.5:
+            aim = aims[int(rng.uniform() * len(aims))]
+        else:
+            side = int(rng.uniform() * 4)
+            t = rng.uniform(-6.0, 6.0)
+            aim = [(8.0, 1.5, t), (-8.0, 1.5, t), (t, 1.5, 8.0), (t, 1.5, -8.0)][side]
+        fire(pos, aim, rng.uniform(30.0, 150.0))
+    for k in range(3):  # head-on pairs
+        y = 3.0 + 0.4 * k
+        z = rng.uniform(-5.0, 5.0)
+        fire((-3.0, y, z), (3.0, y, z), 90.0, gravity_factor=0.0)
+        fire((3.0, y, z), (-3.0, y, z), 90.0, gravity_factor=0.0)
+    sc.add_body(S.box((0.1, 0.5, 1.0)), (-6.0, 4.5, 0.0), motion_type=_abi.CC_MOTION_KINEMATIC,
+                linear_velocity=(60.0, 0.0, 0.0), motion_quality=_abi.CC_MOTION_QUALITY_LINEAR_CAST)
+    for _ in range(4):  # below the threshold: 0.05 * inner radius per step
+        sc.add_body(S.sphere(0.3), (rng.uniform(-5.0, 5.0), 1.0, rng.uniform(-5.0, 5.0)),
+                    linear_velocity=(0.5, 0.0, 0.0), motion_quality=_abi.CC_MOTION_QUALITY_LINEAR_CAST)
     sc.end_world()
     return sc
 

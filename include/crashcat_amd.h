@@ -94,6 +94,9 @@ typedef struct cc_settings {
     uint8_t object_layer_pairs[CC_MAX_OBJECT_LAYERS * CC_MAX_OBJECT_LAYERS];
     int32_t collide_with_backfaces;         /* narrowphase.collideWithBackfaces (false)       */
     int32_t collide_only_with_active_edges; /* narrowphase.collideOnlyWithActiveEdges (true)  */
+    double ccd_linear_cast_threshold;       /* ccd.linearCastThreshold (0.05)                 */
+    double ccd_linear_cast_max_penetration; /* ccd.linearCastMaxPenetration (0.25)            */
+    double ccd_min_velocity_for_restitution; /* ccd.minVelocityForRestitution (1)             */
 } cc_settings;
 
 /* One shape; mirrors the fields of the Sphere/Box/Capsule/ConvexHull/TriangleMesh shape
@@ -285,6 +288,24 @@ typedef struct cc_contact_removed {
     uint32_t sub_shape_b;
 } cc_contact_removed;
 
+/* One CCD record of the last step: CCDBody (src/ccd.ts:19-60) of a dynamic LINEAR_CAST body
+ * whose displacement exceeded linearCastThreshold * innerRadius, as world.ccd.ccdBodies holds it
+ * after updateWorld: in time-of-impact order (sortCCDBodies, src/update.ts:1607-1621) and after
+ * resolveCCDContacts (a later collision invalidated by an earlier one has hit_body_index -1). */
+typedef struct cc_ccd_body {
+    int32_t body_index;
+    int32_t hit_body_index;             /* -1: no hit, the body moved its full displacement */
+    double delta_position[3];
+    double contact_normal[3];
+    double contact_point[3];
+    double fraction;
+    double fraction_plus_slop;
+    double linear_cast_threshold_sq;
+    double max_penetration;
+    double combined_friction;
+    double combined_restitution;
+} cc_ccd_body;
+
 /* One manifold of the contact cache the next step warm-starts from (CachedManifold in
  * world.contacts, src/contacts.ts:9-58; matched by (body pair, sub-shape ids), its points by
  * the lambda-preserve distance, contact-constraints.ts:651-686).  Points are in each body's
@@ -440,6 +461,10 @@ int cc_download_contacts(cc_ctx* ctx, cc_contact_out* out, int32_t capacity, int
  * destroyUnprocessedContacts (:531-555) find unprocessed.  *out_count is the number removed;
  * at most `capacity` are written.  Call cc_step with one step to see every step's events. */
 int cc_download_removed_contacts(cc_ctx* ctx, cc_contact_removed* out, int32_t capacity, int32_t* out_count);
+/* The CCD records of the last step (world.ccd.ccdBodies after updateWorld, src/update.ts:131-140),
+ * world by world, each world's in time-of-impact order.  *out_count is their number; at most
+ * `capacity` are written. */
+int cc_download_ccd_bodies(cc_ctx* ctx, cc_ccd_body* out, int32_t capacity, int32_t* out_count);
 /* cc_download_removed_contacts without any device allocation.  It has no counterpart in the
  * reference: it is for callers that cannot allocate device memory once they run, such as a
  * serving loop that reads every step's removed contacts (the plain entry point allocates and

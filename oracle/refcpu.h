@@ -55,6 +55,8 @@ int32_t oc_get_pairs(const oc_world* w, int32_t* out, int32_t cap);
 int32_t oc_get_contacts(const oc_world* w, cc_contact_out* out, int32_t cap);
 /* contacts the last step removed (destroyUnprocessedContacts), sorted like the contacts */
 int32_t oc_get_removed_contacts(const oc_world* w, cc_contact_removed* out, int32_t cap);
+/* the last step's CCD records in time-of-impact order (world.ccd.ccdBodies); returns their number */
+int32_t oc_get_ccd_bodies(const oc_world* w, cc_ccd_body* out, int32_t cap);
 /* the state the next step starts from, as the C ABI's plain records: the contact cache sorted by
  * (body_a, body_b, sub ids) (returns the count; with the previous time step), the bodies' sleep
  * tests, the point constraints' accumulated impulses (3 each) */

@@ -47,6 +47,8 @@ def lib():
         L.oc_get_point_impulses.argtypes = [C.c_void_p, dp]
         L.oc_get_removed_contacts.restype = C.c_int32
         L.oc_get_removed_contacts.argtypes = [C.c_void_p, C.POINTER(_abi.ContactRemoved), C.c_int32]
+        L.oc_get_ccd_bodies.restype = C.c_int32
+        L.oc_get_ccd_bodies.argtypes = [C.c_void_p, C.POINTER(_abi.CcdBody), C.c_int32]
         L.oc_get_counters.argtypes = [C.c_void_p, C.POINTER(_abi.StepCounters)]
         L.oc_add_forces.argtypes = [C.c_void_p, dp, dp, C.c_int32]
         L.oc_set_settings.argtypes = [C.c_void_p, C.POINTER(_abi.Settings)]
@@ -104,6 +106,12 @@ class OracleWorld:
         assert len(bodies) == b1 - b0
         self._bodies = bodies
         shapes, hulls, self._keep = scene.c_shapes()
+        if ((bodies["motion_quality"] == _abi.CC_MOTION_QUALITY_LINEAR_CAST)
+                & (bodies["motion_type"] == _abi.CC_MOTION_DYNAMIC)).any():
+            for si in np.unique(bodies["shape"]):  # the CCD cast is restated for convex leaves
+                if shapes[si].type not in (_abi.CC_SHAPE_SPHERE, _abi.CC_SHAPE_BOX, _abi.CC_SHAPE_CAPSULE,
+                                           _abi.CC_SHAPE_CONVEX_HULL, _abi.CC_SHAPE_CYLINDER) or shapes[si].decorated:
+                    raise ValueError("LINEAR_CAST bodies need a world of undecorated convex shapes")
         pcs = [(a - b0, b - b0, la, lb, ov, op) for (a, b, la, lb, ov, op) in scene.point_constraints if b0 <= a < b1]
         pa = (_abi.PointConstraintDesc * max(len(pcs), 1))()
         for i, (a, b, la, lb, ov, op) in enumerate(pcs):
@@ -192,6 +200,12 @@ class OracleWorld:
         n = lib().oc_get_removed_contacts(self.h, None, 0)
         out = np.zeros(max(n, 1), dtype=np.dtype(_abi.ContactRemoved))
         lib().oc_get_removed_contacts(self.h, out.ctypes.data_as(C.POINTER(_abi.ContactRemoved)), n)
+        return out[:n]
+
+    def ccd_bodies(self):
+        n = lib().oc_get_ccd_bodies(self.h, None, 0)
+        out = np.zeros(max(n, 1), dtype=np.dtype(_abi.CcdBody))
+        lib().oc_get_ccd_bodies(self.h, out.ctypes.data_as(C.POINTER(_abi.CcdBody)), n)
         return out[:n]
 
     def counters(self):

@@ -1534,6 +1534,136 @@ static void collide_convex_plane(const OShapes* sh, int sc, V3 pc, Q4 qc, int sp
     }
 }
 
+/* ======================================================================== */
+/* shape cast for CCD                                                        */
+/* ======================================================================== */
+/* gjkCastShape gjk.ts:853-1267: shape A (transformed by posAInB / quatAInB) moves by `disp` against
+ * B at rest, both without their convex radius; returns 0 for no hit, else the fraction, the contact
+ * point on A and the separating axis, all in B's space. */
+static int gjk_cast_shape(const OSup* A, const OSup* B, V3 disp, double tolerance, double crA, double crB,
+                          double maxLambda, double* lambdaOut, V3* pointA, V3* sepAxis) {
+    double tol2 = tolerance * tolerance;
+    const double sumR = crA + crB;
+    Simplex s;
+    s.size = 0;
+    double lambda = 0.0;
+    V3 x = v3(0, 0, 0);
+    V3 q = vneg(sup_get(B, v3(0, 0, 0)));
+    V3 p = sup_get(A, v3(0, 0, 0));
+    V3 v = vsub(q, p);
+    double vLen2 = DBL_MAX;
+    int allowRestart = 0;
+    V3 prevV = v3(0, 0, 0);
+    int it = 0;
+    while (it < GJK_MAX_ITERATIONS) {
+        it++;
+        p = sup_get(A, vneg(v));
+        q = sup_get(B, v);
+        V3 w = vsub(x, vsub(q, p));
+        const double vw = v.x * w.x + v.y * w.y + v.z * w.z - sumR * sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+        if (vw > 0.0) {
+            const double vr = v.x * disp.x + v.y * disp.y + v.z * disp.z;
+            if (vr >= -1.0e-18) return 0;
+            const double old = lambda;
+            lambda -= vw / vr;
+            if (old == lambda) break;
+            if (lambda >= maxLambda) return 0;
+            x = vscale(disp, lambda);
+            vLen2 = DBL_MAX;
+            tol2 = tolerance + sumR;
+            tol2 = tol2 * tol2;
+            allowRestart = 1;
+        }
+        s.pt[s.size].p = p;
+        s.pt[s.size].q = q;
+        s.size++;
+        for (int i = 0; i < s.size; i++) s.pt[i].y = vadd(vsub(x, s.pt[i].q), s.pt[i].p);
+        CPSRes c;
+        cp_simplex(&c, vLen2, 0, &s);
+        if (c.found) {
+            vLen2 = c.dist2;
+            v = c.point;
+        } else {
+            if (!allowRestart) break;
+            allowRestart = 0;
+            s.pt[0].p = p;
+            s.pt[0].q = q;
+            s.size = 1;
+            v = vsub(x, q);
+            vLen2 = DBL_MAX;
+            continue;
+        }
+        if (c.set == 0xf) break;
+        int ns = 0; /* updatePointSetPQ */
+        for (int i = 0; i < s.size; i++)
+            if (c.set & (1 << i)) {
+                if (ns != i) s.pt[ns] = s.pt[i];
+                ns++;
+            }
+        s.size = ns;
+        if (vLen2 <= tol2) break;
+        prevV = v;
+    }
+    for (int i = 0; i < s.size; i++) s.pt[i].y = vadd(vsub(x, s.pt[i].q), s.pt[i].p);
+    const double vl = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    V3 nv = v3(0, 0, 0);
+    if (vl > 0) nv = vscale(v, 1 / vl);
+    V3 pa = v3(0, 0, 0), pb = v3(0, 0, 0);
+    if (s.size == 1) {
+        pb = vadd(s.pt[0].q, vscale(nv, crB));
+        pa = lambda > 0.0 ? pb : vadd(s.pt[0].p, vscale(nv, -crA));
+    } else if (s.size == 2) {
+        Bary b = bary2(s.pt[0].y, s.pt[1].y, 1e-10);
+        pb = vadd(vadd(vadd(pb, vscale(s.pt[0].q, b.u)), vscale(s.pt[1].q, b.v)), vscale(nv, crB));
+        pa = lambda > 0.0 ? pb
+                          : vadd(vadd(vadd(pa, vscale(s.pt[0].p, b.u)), vscale(s.pt[1].p, b.v)), vscale(nv, -crA));
+    } else if (s.size >= 3) {
+        Bary b = bary3(s.pt[0].y, s.pt[1].y, s.pt[2].y, 1e-10);
+        pb = vadd(vadd(vadd(vadd(pb, vscale(s.pt[0].q, b.u)), vscale(s.pt[1].q, b.v)), vscale(s.pt[2].q, b.w)),
+                  vscale(nv, crB));
+        if (lambda > 0.0) pa = pb;
+        else
+            pa = vadd(vadd(vadd(vadd(pa, vscale(s.pt[0].p, b.u)), vscale(s.pt[1].p, b.v)), vscale(s.pt[2].p, b.w)),
+                      vscale(nv, -crA));
+    }
+    *lambdaOut = lambda;
+    *pointA = pa;
+    *sepAxis = sumR > 0.0 ? vneg(v) : vneg(prevV);
+    return 1;
+}
+
+/* castConvexVsConvex convex.ts:675-875 with penetrationCastShape penetration.ts:621-688 as the CCD
+ * visitor calls them (update.ts:1218-1237: unit scales, returnDeepestPoint false, so no EPA, and
+ * the supporting faces go unused).  A at (pa, qa) moves by `disp` (world space) against B at
+ * (pb, qb).  Returns 1 for a hit accepted below `earlyOut`, with the fraction, A's contact point
+ * and the (unnormalised) penetration axis in world space. */
+int ocx_cast_convex(const OShapes* sh, int sa, V3 pa, Q4 qa, V3 disp, int sb, V3 pb, Q4 qb, int backfaces,
+                    double earlyOut, double* fraction, V3* pointA, V3* penAxis) {
+    const Q4 ib = {-qb.x, -qb.y, -qb.z, qb.w};
+    const V3 posAInB = qrotate(ib, vsub(pa, pb));
+    const Q4 quatAInB = qmul(ib, qa);
+    const V3 dispInB = qrotate(ib, disp);
+    OSup A, B;
+    sup_for_shape(&A, sh, sa, MODE_EXCLUDE);
+    sup_for_shape(&B, sh, sb, MODE_EXCLUDE);
+    A.transformed = 1;
+    A.R = m3_from_quat(quatAInB);
+    A.t = posAInB;
+    const double tolerance = 1e-4;
+    double lambda;
+    V3 pA, sep;
+    if (!gjk_cast_shape(&A, &B, dispInB, tolerance, A.convexRadius, B.convexRadius, earlyOut, &lambda, &pA, &sep))
+        return 0;
+    if (vlen2(sep) < tolerance * tolerance) sep = dispInB; /* contactNormalInvalid */
+    if (lambda > 1.0) return 0;
+    if (!backfaces && vdot(sep, dispInB) <= 0) return 0;
+    if (lambda >= earlyOut) return 0;
+    *fraction = lambda;
+    *pointA = vadd(qrotate(qb, pA), pb);
+    *penAxis = qrotate(qb, sep);
+    return 1;
+}
+
 /* collideShapeVsShape dispatch (src/collision/narrowphase.ts:172-238, src/shapes/sphere.ts:154-167) */
 void ocx_collide(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, double maxSep, double colTol,
                  double penTol, OHit* out) {

@@ -32,6 +32,8 @@ typedef struct {
     V3 sleepCenter[3];
     double sleepRadius[3];
     double sleepTimer;
+    int quality;   /* motionProperties.motionQuality */
+    int ccdIndex;  /* body.ccdBodyIndex: the body's CCD record this step, or -1 */
 } OBody;
 
 typedef struct { V3 p1, p2; double ln, lt1, lt2; } OCached;
@@ -68,6 +70,13 @@ typedef struct {
     V3 lambda;
     int ovV, ovP; /* the constraint's iteration overrides */
 } OPoint;
+
+/* CCDBody (src/ccd.ts:19-60) */
+typedef struct {
+    int body, hit;
+    V3 delta, normal, point;
+    double fraction, fps, thrSq, maxPen, friction, restitution;
+} OCcd;
 
 typedef struct {
     int* bodies; int nb, cb;      /* active indices */
@@ -106,6 +115,11 @@ struct oc_world {
     /* islands */
     int* bodyLinks; int* bodyIslands; int* contactLinks; int* consLinks; int* consIds; int nconsIds;
     OIsland* isl; int nisl, cisl;
+    /* CCD: this step's records in body order, their time-of-impact order, and the two constraint
+     * parts applyCCD reuses from one call to the next (module scope in update.ts:1453-1454) */
+    OCcd* ccd; int nccd, cccd;
+    int* ccdOrder;
+    OAxis ccdN, ccdF;
     /* counters */
     cc_step_counters ctr;
 };
@@ -337,6 +351,8 @@ oc_world* oc_create(const cc_settings* s, const cc_body_desc* bodies, int32_t nb
         b->sleeping = d->sleeping;
         b->ovV = d->num_velocity_steps_override;
         b->ovP = d->num_position_steps_override;
+        b->quality = d->motion_quality;
+        b->ccdIndex = -1;
         if (b->motion == CC_MOTION_DYNAMIC) { /* motionProperties.force / torque (dynamic bodies only) */
             b->force = vload(d->force);
             b->torque = vload(d->torque);
@@ -367,6 +383,7 @@ void oc_destroy(oc_world* w) {
     free(w->bodyLinks); free(w->bodyIslands); free(w->contactLinks); free(w->consLinks); free(w->consIds);
     for (int i = 0; i < w->cisl; i++) { free(w->isl[i].bodies); free(w->isl[i].contacts); free(w->isl[i].cons); }
     free(w->isl);
+    free(w->ccd); free(w->ccdOrder);
     free(w);
 }
 
@@ -1122,11 +1139,173 @@ static int solve_position_contact(oc_world* w, OCC* k) {
     return any;
 }
 
+/* ---- continuous collision detection (src/update.ts:896-1677) ---------------- */
+/* getShapeInnerRadius (shapes.ts:522-525): sphere.ts:147-149, box.ts:333-335, capsule.ts:254-256,
+ * cylinder.ts:261-263, convex-hull.ts:550-558 (the nearest face plane), scaled.ts:127-130 */
+static double inner_radius(const oc_world* w, int si) {
+    const cc_shape_desc* d = &w->shapes[si];
+    double r = 0;
+    switch (d->type) {
+    case CC_SHAPE_SPHERE: case CC_SHAPE_CAPSULE: r = d->radius; break;
+    case CC_SHAPE_BOX: r = dmin(d->half_extents[0], dmin(d->half_extents[1], d->half_extents[2])); break;
+    case CC_SHAPE_CYLINDER: r = dmin(d->half_height, d->radius); break;
+    case CC_SHAPE_CONVEX_HULL: {
+        r = DBL_MAX;
+        for (int f = 0; f < d->hull_num_faces; f++) r = dmin(r, -w->hulls.plane_constants[d->hull_first_face + f]);
+        r = dmax(0.0, r);
+        break;
+    }
+    default: break;
+    }
+    if (d->decorated & CC_DECO_SCALED)
+        r = dmin(fabs(d->scale[0]), dmin(fabs(d->scale[1]), fabs(d->scale[2]))) * r;
+    return r;
+}
+
+/* the motion a LINEAR_CAST body that is dynamic or kinematic makes this step, zero for any other
+ * (calculateBodyMotion :985-996) */
+static V3 body_motion(const OBody* b, double dt) {
+    if (b->quality == CC_MOTION_QUALITY_LINEAR_CAST && b->motion != CC_MOTION_STATIC) return vscale(b->v, dt);
+    return v3(0, 0, 0);
+}
+
+/* findCCDContacts :1357-1437 with the visitor :1110-1240 and the collector :1003-1107.  The
+ * reference visits the bodies that a swept-AABB cast through its tree reaches; those early-outs
+ * are conservative, so every body passes to the filters and the cast here, in index order (the
+ * visit order decides only between hits of equal fraction-plus-slop).  The manifold block of
+ * :1376-1432 never runs: the collector's reset after every visit (:1238) clears hasHit, so the
+ * contact point stays the collector's. */
+static void ccd_find(oc_world* w, OCcd* c, double dt) {
+    const OBody* A = &w->b[c->body];
+    for (int j = 0; j < w->nb; j++) {
+        const OBody* B = &w->b[j];
+        if (j == c->body) continue;
+        if (!w->s.object_layer_pairs[A->layer * CC_MAX_OBJECT_LAYERS + B->layer]) continue;
+        if (!((A->groups & B->mask) != 0 && (B->groups & A->mask) != 0)) continue;
+        V3 rel = c->delta;
+        if (B->ccdIndex >= 0) {
+            const OCcd* cb = &w->ccd[B->ccdIndex];
+            if (c->body > cb->body) continue;
+            rel = vsub(rel, cb->delta);
+        }
+        if (vlen2(rel) < c->thrSq) continue;
+        double fraction;
+        V3 pA, pen;
+        if (!ocx_cast_convex(&w->sh, A->shape, A->pos, A->q, rel, B->shape, B->pos, B->q, w->s.collide_with_backfaces,
+                             c->fps, &fraction, &pA, &pen))
+            continue;
+        /* addHit :1045-1098 */
+        const V3 n = vnormalize(pen);
+        const double den = vdot(n, c->delta);
+        if (den <= c->maxPen) continue;
+        const double fps = fraction + c->maxPen / den;
+        if (fps >= c->fps) continue;
+        c->hit = j;
+        c->fraction = fraction;
+        c->fps = fps;
+        c->normal = n;
+        V3 mb = body_motion(B, dt);
+        if (vlen2(mb) > 1e-12) c->point = vadd(pA, vscale(mb, fraction));
+        else c->point = pA;
+        c->friction = combine(A->friction, B->friction, A->fmode, B->fmode);
+        c->restitution = combine(A->restitution, B->restitution, A->rmode, B->rmode);
+    }
+}
+
+/* compareCCDBodies :1596-1604 */
+static oc_world* g_ccd_world;
+static int cmp_ccd(const void* x, const void* y) {
+    const OCcd* a = &g_ccd_world->ccd[*(const int*)x];
+    const OCcd* b = &g_ccd_world->ccd[*(const int*)y];
+    if (a->fps != b->fps) return a->fps < b->fps ? -1 : 1;
+    return a->body - b->body;
+}
+
+/* applyCCD :1457-1593: one normal iteration and one friction iteration at the contact, through
+ * the two constraint parts kept from the previous call */
+static void apply_ccd(oc_world* w, const OCcd* c, OBody* A, OBody* B) {
+    if (A->motion != CC_MOTION_DYNAMIC) return;
+    const V3 atHit = vadd(A->com, vscale(c->delta, c->fraction));
+    const V3 r1 = vsub(c->point, atHit);
+    const V3 r2 = vsub(c->point, B->com);
+    const V3 v1 = vadd(A->v, vcross(A->w, r1));
+    const V3 v2 = B->motion != CC_MOTION_STATIC ? vadd(B->v, vcross(B->w, r2)) : v3(0, 0, 0);
+    const V3 rv = vsub(v2, v1);
+    const double nv = vdot(rv, c->normal);
+    if (nv >= 0) return;
+    double bias = 0;
+    if (c->restitution > 0 && nv < -w->s.ccd_min_velocity_for_restitution) bias = c->restitution * nv;
+    const double invMA = 1.0 * A->invMass;
+    const double invMB = B->motion == CC_MOTION_DYNAMIC ? 1.0 * B->invMass : 0;
+    M3 IA, IB;
+    memset(&IB, 0, sizeof(IB));
+    M3 RA = m3_from_quat(A->q);
+    IA = inv_inertia(A->invI, A->inertiaRot, &RA, A->dof);
+    if (B->motion == CC_MOTION_DYNAMIC) {
+        M3 RB = m3_from_quat(B->q);
+        IB = inv_inertia(B->invI, B->inertiaRot, &RB, B->dof);
+    }
+    axis_props(&w->ccdN, A, B, invMA, invMB, &IA, &IB, r1, r2, c->normal, bias);
+    double tot = axis_total_lambda(&w->ccdN, A, B, c->normal);
+    axis_apply(&w->ccdN, A, B, invMA, invMB, c->normal, tot);
+    if (c->friction > 0) {
+        const V3 tv = vsub(rv, vscale(c->normal, nv));
+        const double tl2 = vlen2(tv);
+        if (tl2 > 1e-12) {
+            const V3 dir = vscale(tv, 1.0 / sqrt(tl2));
+            const double maxF = c->friction * w->ccdN.lambda;
+            axis_props(&w->ccdF, A, B, invMA, invMB, &IA, &IB, r1, r2, dir, 0);
+            tot = axis_total_lambda(&w->ccdF, A, B, dir);
+            axis_apply(&w->ccdF, A, B, invMA, invMB, dir, dmax(-maxF, dmin(maxF, tot)));
+        }
+    }
+    clamp_lin(A); clamp_ang(A);
+    if (B->motion == CC_MOTION_DYNAMIC) { clamp_lin(B); clamp_ang(B); }
+}
+
+/* findCCDContacts, sortCCDBodies :1607-1621 and resolveCCDContacts :1626-1677.  The body's
+ * ccdBodyIndex still names its place before the sort, and resolveCCDContacts looks it up in the
+ * sorted array (:1647-1649); that lookup is kept as it is. */
+static void ccd_step(oc_world* w, double dt) {
+    for (int i = 0; i < w->nccd; i++) ccd_find(w, &w->ccd[i], dt);
+    w->ccdOrder = (int*)xrealloc(w->ccdOrder, sizeof(int) * (size_t)w->nccd);
+    for (int i = 0; i < w->nccd; i++) w->ccdOrder[i] = i;
+    g_ccd_world = w;
+    sort_some(w->ccdOrder, (size_t)w->nccd, sizeof(int), cmp_ccd);
+    for (int i = 0; i < w->nccd; i++) {
+        OCcd* c = &w->ccd[w->ccdOrder[i]];
+        OBody* A = &w->b[c->body];
+        if (c->hit == -1) {
+            if (A->motion == CC_MOTION_DYNAMIC) {
+                A->pos = vadd(A->pos, c->delta);
+                set_transform(w, A);
+            }
+            continue;
+        }
+        OBody* B = &w->b[c->hit];
+        if (B->ccdIndex >= 0) {
+            OCcd* cb = &w->ccd[w->ccdOrder[B->ccdIndex]];
+            if (cb->hit != -1 && cb->fraction > c->fraction) {
+                cb->hit = -1;
+                cb->fps = c->fraction;
+            }
+        }
+        apply_ccd(w, c, A, B);
+        if (A->motion == CC_MOTION_DYNAMIC) {
+            A->pos = vadd(A->pos, vscale(c->delta, c->fps));
+            set_transform(w, A);
+        }
+        if (B->sleeping) body_wake(w, B, c->hit);
+    }
+}
+
 /* ---- the step (src/update.ts:45-239) ---------------------------------------- */
 void oc_step(oc_world* w, double dt) {
     memset(&w->ctr, 0, sizeof(w->ctr));
     int64_t epa0 = ocx_epa_calls;
     w->ncc = 0;
+    w->nccd = 0; /* ccd.clear src/ccd.ts:107-123 */
+    for (int i = 0; i < w->nb; i++) w->b[i].ccdIndex = -1;
     for (int i = 0; i < w->nct; i++) w->ct[i].processed = 0;
     /* accelerationIntegrationUpdate :245-319 */
     for (int i = 0; i < w->nb; i++) {
@@ -1251,10 +1430,33 @@ void oc_step(oc_world* w, double dt) {
                 b->q = qnormalize(qmul(qaxis_angle(axis, ang), b->q));
             }
             V3 disp = vscale(b->v, dt);
+            if (b->quality == CC_MOTION_QUALITY_LINEAR_CAST && b->motion == CC_MOTION_DYNAMIC) { /* :932-967 */
+                const double inner = inner_radius(w, b->shape);
+                const double thr = w->s.ccd_linear_cast_threshold * inner;
+                if (vlen2(disp) > thr * thr) {
+                    if (w->nccd == w->cccd) {
+                        w->cccd = w->cccd ? 2 * w->cccd : 16;
+                        w->ccd = (OCcd*)xrealloc(w->ccd, sizeof(OCcd) * (size_t)w->cccd);
+                    }
+                    OCcd* c = &w->ccd[w->nccd];
+                    memset(c, 0, sizeof(*c));
+                    b->ccdIndex = w->nccd++;
+                    c->body = i;
+                    c->hit = -1;
+                    c->delta = disp;
+                    c->fraction = 1.0;
+                    c->fps = 1.0;
+                    c->thrSq = thr * thr;
+                    c->maxPen = dmin(w->s.penetration_slop, w->s.ccd_linear_cast_max_penetration * inner);
+                    set_transform(w, b); /* setQuaternion: the position waits for the CCD pass */
+                    continue;
+                }
+            }
             b->com = vadd(b->com, disp);
             pos_from_com(w, b);
             set_transform(w, b);
         }
+        if (w->nccd > 0) ccd_step(w, dt);
         int maxPos = 0;
         for (int il = 0; il < w->nisl; il++) {
             OIsland* I = &w->isl[il];
@@ -1321,6 +1523,24 @@ void oc_step(oc_world* w, double dt) {
 }
 
 /* ---- accessors ------------------------------------------------------------- */
+int32_t oc_get_ccd_bodies(const oc_world* w, cc_ccd_body* out, int32_t cap) {
+    for (int i = 0; i < w->nccd && i < cap; i++) {
+        const OCcd* c = &w->ccd[w->ccdOrder[i]];
+        cc_ccd_body* o = &out[i];
+        o->body_index = c->body;
+        o->hit_body_index = c->hit;
+        vstore(o->delta_position, c->delta);
+        vstore(o->contact_normal, c->normal);
+        vstore(o->contact_point, c->point);
+        o->fraction = c->fraction;
+        o->fraction_plus_slop = c->fps;
+        o->linear_cast_threshold_sq = c->thrSq;
+        o->max_penetration = c->maxPen;
+        o->combined_friction = c->friction;
+        o->combined_restitution = c->restitution;
+    }
+    return w->nccd;
+}
 int32_t oc_num_bodies(const oc_world* w) { return w->nb; }
 
 void oc_get_bodies(const oc_world* w, cc_body_state* out) {

@@ -1,0 +1,187 @@
+"""The oracle's continuous collision detection (MotionQuality.LINEAR_CAST, src/update.ts:896-1677)
+pinned by what the mathematics fixes: the stop position of a bullet at a thin wall (the time of
+impact from the shape cast plus the penetration allowance min(penetrationSlop,
+linearCastMaxPenetration * innerRadius), per shape type), the bounce of one normal iteration with
+restitution, the sphere-sphere time of impact, the Coulomb cone of the friction iteration,
+momentum conservation between two CCD bodies, the threshold below which a LINEAR_CAST body steps
+exactly as a discrete one, kinematic LINEAR_CAST bodies moving undelayed, and the wake of a
+sleeping body that is hit.  The reference's own tests (tst/ccd.test.ts) place the bullets too far
+to reach the wall in their one step, so they hold no vector for this path."""
+import math
+
+import numpy as np
+import pytest
+
+from crashcat_amd import _abi
+from crashcat_amd import scene as SC
+from crashcat_amd import shapes as S
+from oracle import refcpu as O
+
+DT = 1.0 / 60.0
+LC = _abi.CC_MOTION_QUALITY_LINEAR_CAST
+
+
+def _zero_g():
+    st, _, _ = SC.test_world_settings()
+    st.gravity[0] = st.gravity[1] = st.gravity[2] = 0.0
+    return st
+
+
+def _wall_scene(bullet, x0=-2.0, v=(200.0, 0.0, 0.0), quality=LC, q=(0.0, 0.0, 0.0, 1.0), **kw):
+    sc = SC.Scene(_zero_g())
+    sc.add_body(S.box((0.1, 5.0, 5.0)), (0.0, 0.0, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+    sc.add_body(bullet, (x0, 0.0, 0.0), q, linear_velocity=v, motion_quality=quality, linear_damping=0.0,
+                angular_damping=0.0, **kw)
+    sc.end_world()
+    return sc
+
+
+def _one_step(sc, steps=1):
+    o = O.OracleWorld(sc)
+    for _ in range(steps):
+        o.step(DT)
+    return o.bodies()
+
+
+@pytest.mark.parametrize("kind,front,inner", [
+    ("sphere", 0.2, 0.2),
+    ("box", 0.04, 0.04),          # inner radius = smallest half extent (box.ts:333-335); a convex
+                                  # radius below it, as a thinner box is widened to the radius
+                                  # (support.ts:351-362)
+    ("capsule", 0.15, 0.15),      # capsule along y: its radius faces the wall
+    ("cylinder", 0.25, 0.1),      # min(half height, radius) (cylinder.ts:261-263)
+    ("hull", 0.3, 0.03),          # the nearest face plane of a flat hull (convex-hull.ts:550-558)
+])
+def test_bullet_stops_at_thin_wall_with_the_allowed_penetration(kind, front, inner):
+    """A bullet at 200 m/s stops where the cast meets the wall plus
+    min(penetrationSlop, 0.25 * innerRadius) along its motion; the discrete twin passes through."""
+    shape = {"sphere": S.sphere(0.2), "box": S.box((0.04, 0.3, 0.3), convex_radius=0.02), "capsule": S.capsule(0.15, 0.3),
+             "cylinder": S.cylinder(0.25, 0.1),
+             "hull": S.convex_hull(np.array([[sx * 0.3, sy * 0.03, sz * 0.3] for sx in (-1, 1) for sy in (-1, 1)
+                                             for sz in (-1, 1)]))}[kind]
+    # the cylinder lies on its side (axis along z) so that its round face meets the wall
+    q = (0.7071067811865476, 0.0, 0.0, 0.7071067811865476) if kind == "cylinder" else (0.0, 0.0, 0.0, 1.0)
+    b = _one_step(_wall_scene(shape, q=q))
+    pen = b["position"][1][0] + front - (-0.1)
+    assert pen == pytest.approx(min(0.02, 0.25 * inner), abs=3e-4), (kind, pen)
+    # the one normal iteration takes out the approach velocity (exactly for a centred contact)
+    assert abs(b["linear_velocity"][1][0]) < 2e-4
+    d = _one_step(_wall_scene(shape, q=q, quality=0))
+    assert d["position"][1][0] > 0.2  # tunnelled
+
+
+def test_restitution_bounce_of_one_normal_iteration():
+    """One normal iteration with bias e * v_n (applyCCD :1478-1527): a sphere at 200 m/s with
+    restitution 0.5 (max-combined with the wall's 0) leaves at -100 m/s."""
+    b = _one_step(_wall_scene(S.sphere(0.2), restitution=0.5))
+    assert b["linear_velocity"][1][0] == pytest.approx(-100.0, abs=1e-9)
+    assert np.allclose(b["angular_velocity"][1], 0.0, atol=1e-12)
+
+
+def test_sphere_sphere_time_of_impact():
+    """A sphere (r 0.25) cast at a resting sphere (r 0.5) 3 m ahead: the fraction is
+    (3 - 0.75) / |delta| to the cast tolerance, and the mover ends there plus the slop."""
+    sc = SC.Scene(_zero_g())
+    sc.add_body(S.sphere(0.5), (3.0, 0.0, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+    sc.add_body(S.sphere(0.25), (0.0, 0.0, 0.0), linear_velocity=(240.0, 0.0, 0.0), motion_quality=LC,
+                linear_damping=0.0)
+    sc.end_world()
+    b = _one_step(sc)
+    delta = 240.0 * DT
+    frac = (3.0 - 0.75) / delta
+    slop = min(0.02, 0.25 * 0.25) / delta
+    assert b["position"][1][0] == pytest.approx(delta * (frac + slop), abs=2e-4)
+
+
+@pytest.mark.parametrize("friction", [0.0, 0.3, 1.0])
+def test_friction_iteration_stays_in_the_coulomb_cone(friction):
+    """A sphere hitting the wall at 45 degrees: the normal velocity goes to zero and the
+    tangential one drops by at most mu * (normal change), not at all without friction."""
+    v0 = np.array([150.0, 0.0, 150.0])
+    sc = SC.Scene(_zero_g())
+    sc.add_body(S.box((0.1, 5.0, 5.0)), (0.0, 0.0, 0.0), motion_type=_abi.CC_MOTION_STATIC, friction=friction)
+    sc.add_body(S.sphere(0.2), (-1.5, 0.0, -1.0), linear_velocity=tuple(v0), motion_quality=LC, friction=friction,
+                linear_damping=0.0, angular_damping=0.0, friction_combine=_abi_combine_min())
+    sc.end_world()
+    b = _one_step(sc)
+    v1 = b["linear_velocity"][1]
+    assert abs(v1[0]) < 1e-9
+    dt_ = v0[2] - v1[2]
+    if friction == 0.0:
+        assert dt_ == 0.0
+    else:
+        assert 0.0 < dt_ <= friction * v0[0] * (1 + 1e-12)
+
+
+def _abi_combine_min():
+    return 2  # CC_COMBINE_MIN: the wall's friction decides
+
+
+def test_two_ccd_bodies_conserve_momentum():
+    """Two LINEAR_CAST spheres flying at each other (no gravity, no damping): the lower index
+    casts against the other with the relative displacement and the impulse is equal and opposite."""
+    sc = SC.Scene(_zero_g())
+    m = []
+    for x, v, r in ((-1.0, 120.0, 0.2), (1.0, -60.0, 0.3)):
+        sc.add_body(S.sphere(r), (x, 0.0, 0.0), linear_velocity=(v, 0.0, 0.0), motion_quality=LC,
+                    linear_damping=0.0, angular_damping=0.0)
+    sc.end_world()
+    bd = sc.bodies()
+    mass = 1.0 / bd["inv_mass"]
+    p0 = (mass[:, None] * bd["linear_velocity"]).sum(0)
+    b = _one_step(sc)
+    p1 = (mass[:, None] * b["linear_velocity"]).sum(0)
+    assert np.allclose(p1, p0, rtol=0, atol=1e-9 * np.abs(p0).max())
+    assert b["linear_velocity"][0][0] < 120.0 and b["linear_velocity"][1][0] > -60.0
+
+
+def test_below_threshold_steps_as_discrete():
+    """|v dt| <= linearCastThreshold * innerRadius: the LINEAR_CAST body is never delayed, so a
+    pile with such bodies steps bit for bit as with discrete ones."""
+    def pile(quality):
+        sc = SC.Scene()
+        sc.add_body(S.box((10.0, 0.5, 10.0)), (0.0, -0.5, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+        for k in range(6):
+            sc.add_body(S.box((0.3, 0.3, 0.3)), (0.1 * k, 0.35 + 0.65 * k, 0.0), motion_quality=quality,
+                        linear_velocity=(0.5, 0.0, 0.0))
+        sc.end_world()
+        return sc
+    a, b = O.OracleWorld(pile(LC)), O.OracleWorld(pile(0))
+    for _ in range(60):
+        a.step(DT)
+        b.step(DT)
+    for f in ("position", "quaternion", "linear_velocity", "angular_velocity"):
+        assert np.array_equal(a.bodies()[f], b.bodies()[f]), f
+
+
+def test_kinematic_linear_cast_moves_undelayed():
+    """Kinematic bodies cannot be stopped (update.ts:930-934): a LINEAR_CAST kinematic body moves
+    by v dt through the wall exactly as a discrete one."""
+    sc = _wall_scene(S.box((0.2, 0.2, 0.2)), motion_type=_abi.CC_MOTION_KINEMATIC)
+    b = _one_step(sc)
+    assert b["position"][1][0] == pytest.approx(-2.0 + 200.0 * DT, abs=1e-12)
+
+
+def test_hit_wakes_a_sleeping_body():
+    """resolveCCDContacts wakes the body it hits (:1672-1675) after the impulse reaches it."""
+    sc = SC.Scene(_zero_g())
+    sc.add_body(S.box((0.5, 0.5, 0.5)), (0.0, 0.0, 0.0), sleeping=True)
+    sc.add_body(S.sphere(0.1), (-2.0, 0.0, 0.0), linear_velocity=(180.0, 0.0, 0.0), motion_quality=LC)
+    sc.end_world()
+    b = _one_step(sc)
+    assert b["sleeping"][0] == 0
+    assert b["linear_velocity"][0][0] > 0.0
+    assert b["position"][1][0] < -0.5
+
+
+def test_gallery_keeps_every_bullet_inside_or_above_the_walls():
+    """The gallery scene: no LINEAR_CAST body ends beyond a wall below the walls' top (6 m)."""
+    sc = SC.ccd_gallery()
+    bd = sc.bodies()
+    o = O.OracleWorld(sc)
+    lc = (bd["motion_quality"] == LC) & (bd["motion_type"] == _abi.CC_MOTION_DYNAMIC)
+    for _ in range(40):
+        o.step(DT)
+        p = o.bodies()["position"]
+        out = lc & ((np.abs(p[:, 0]) > 8.05) | (np.abs(p[:, 2]) > 8.05)) & (p[:, 1] < 5.5)
+        assert not out.any(), np.where(out)

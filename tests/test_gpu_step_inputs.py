@@ -2,7 +2,7 @@
 external forces and torques (update.ts:245-319, cleared at :234-238), per-island iteration
 overrides (islands.ts:344-386), moving kinematic bodies in worlds solved as one workgroup and
 as several island groups, the warm-start match after moving a body between steps and a
-changing dt; and the upload rejections (sensors, CCD, overrides out of range, a failed upload
+changing dt; and the upload rejections (sensors, an unknown motion quality, overrides out of range, a failed upload
 leaving no scene behind)."""
 import ctypes as C
 
@@ -159,7 +159,7 @@ def _one_body(**kw):
 
 @pytest.mark.parametrize("kw,code", [
     (dict(sensor=True), _abi.CC_E_UNSUPPORTED),
-    (dict(motion_quality=1), _abi.CC_E_UNSUPPORTED),
+    (dict(motion_quality=2), _abi.CC_E_ARG),
     (dict(num_velocity_steps_override=256), _abi.CC_E_ARG),
     (dict(num_position_steps_override=-1), _abi.CC_E_ARG),
 ])

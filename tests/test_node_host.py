@@ -224,11 +224,12 @@ def _crashcat_scenes():
             "chains": chains,
             "terrain": SC.terrain_drop(vertices=16, n=24, drop=(0.5, 2.5)),
             "decorated": SC.decorated_pile(seed=1011, n=60, extent=(6.0, 4.0, 6.0)),
-            "compound": SC.compound_pile(seed=1014, n=50, extent=(6.0, 4.0, 6.0))}
+            "compound": SC.compound_pile(seed=1014, n=50, extent=(6.0, 4.0, 6.0)),
+            "ccd": SC.ccd_gallery(seed=1015, bullets=10, targets=20)}
 
 
 @needs_node
-@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated", "compound"])
+@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated", "compound", "ccd"])
 def test_crashcat_world_upload_equals_the_scene_description(tmp_path, which):
     """The adapter reads crashcat's own objects (World settings, RigidBody pool with
     motionProperties, shape records, point-constraint pool) and produces exactly the upload the
@@ -267,7 +268,7 @@ process.stdout.write(JSON.stringify(cw.describe(world)));
 
 @needs_node
 @pytest.mark.parametrize("what,needle", [
-    ("sensor", "is a sensor"), ("ccd", "LINEAR_CAST"), ("compound", "compound children must be convex"),
+    ("sensor", "is a sensor"), ("compound", "compound children must be convex"),
     ("empty", "EMPTY"), ("hinge", "constraint type 2"),
     ("layers", "object layers"), ("reduction", "manifold reduction"), ("ids", "not in index order"),
     ("moving_mesh", "triangle meshes must be static")])
@@ -279,8 +280,6 @@ def test_crashcat_world_unsupported_features_are_refused(tmp_path, what, needle)
     b = d["bodies"]["pool"]
     if what == "sensor":
         b[1]["sensor"] = True
-    elif what == "ccd":
-        b[1]["motionProperties"]["motionQuality"] = 1
     elif what == "compound":  # a compound inside a compound
         inner = {"type": 5, "aabb": [[0, 0, 0], [1, 1, 1]], "centerOfMass": [0, 0, 0],
                  "children": [{"position": [0, 0, 0], "quaternion": [0, 0, 0, 1], "shape": d["shapes"][b[1]["shape"]]}]}
@@ -318,7 +317,7 @@ try {{ cc.fromCrashcatWorld(world); process.stdout.write('accepted'); }} catch (
 
 @needs_node
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated", "compound"])
+@pytest.mark.parametrize("which", ["cube_heap", "mixed", "chains", "terrain", "decorated", "compound", "ccd"])
 def test_crashcat_world_steps_bitwise_with_events_and_forces(tmp_path, which):
     """updateWorld on a crashcat-shaped World: FP64, bit for bit against the oracle in the state
     written back into the RigidBody objects; per step the listener's events (with the body

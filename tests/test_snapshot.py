@@ -22,6 +22,7 @@ SCENES = {
     "mixed_pile": lambda: SC.mixed_pile(seed=1002, n=400, extent=(12.0, 8.0, 12.0)),
     "terrain_spheres": lambda: SC.terrain_drop(vertices=24, n=48, drop=(0.5, 3.0), spheres=True),
     "batched_chains": lambda: SC.batched_worlds(seed=1003, num_worlds=8, min_bodies=8, max_bodies=64),
+    "ccd_gallery": lambda: SC.ccd_gallery(seed=1013),
 }
 FIELDS = ["position", "quaternion", "center_of_mass_position", "linear_velocity", "angular_velocity", "aabb_min",
           "aabb_max", "sleeping"]
