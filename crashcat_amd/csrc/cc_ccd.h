@@ -140,6 +140,52 @@ __device__ inline bool cast_convex(const ShapeTable<C>& st, int sa, V3<C> pa, Q4
     return true;
 }
 
+// castConvexVsPlane plane.ts:727-880 as the CCD visitor calls it (unit scales): the convex shape's
+// DEFAULT support point is taken to world space with A's transform and measured against the plane
+// in the plane shape's own frame; the contact point goes through the plane body's transform
+// (translated by the cast for a hit ahead) and the penetration axis is -normal through that same
+// matrix, as a point.  The hit is accepted below a fraction of 1 (see cast_convex).
+template <typename C>
+__device__ inline bool cast_convex_plane(const ShapeTable<C>& st, int sa, V3<C> pa, Q4<C> qa, V3<C> disp, int sp, V3<C> pb,
+                                         Q4<C> qb, bool backfaces, C& fraction, V3<C>& pointA, V3<C>& penAxis) {
+    const ShapeD<C>& P = st.shapes[sp];
+    const V3<C> sn0 = mk3<C>(P.he[0] / C(1), P.he[1] / C(1), P.he[2] / C(1));
+    const C len = vlen(sn0);
+    const V3<C> n = sn0 * (C(1) / len);
+    const C c = P.radius / len;
+    const M4<C> start = m4_from_rt(qa, pa);
+    const int ty = st.shapes[sa].type;
+    Sup<C> S;
+    sup_for_shape(S, st, sa, ty == SHAPE_BOX || ty == SHAPE_CYLINDER);
+    const V3<C> sw = m4_point(start, sup_get(S, -qrotate(qconj(qa), n)));
+    const C sd = dot(n, sw) + c;
+    const C cr = S.cr;
+    const C pen = -sd + cr;
+    const C dt = dot(disp, n);
+    M4<C> hitM = m4_from_rt(qb, pb);
+    C f;
+    if (pen > C(0)) {
+        if (!backfaces && dt > C(0)) return false;
+        if (pen <= C(-1)) return false;
+        f = C(0);
+    } else if (dt < C(0)) {
+        f = pen / dt;
+        if (f >= C(1)) return false;
+        const V3<C> o = disp * f;
+        const M4<C> p2w = hitM;  // mat4.translate
+        hitM.m[12] = p2w.m[0] * o.x + p2w.m[4] * o.y + p2w.m[8] * o.z + p2w.m[12];
+        hitM.m[13] = p2w.m[1] * o.x + p2w.m[5] * o.y + p2w.m[9] * o.z + p2w.m[13];
+        hitM.m[14] = p2w.m[2] * o.x + p2w.m[6] * o.y + p2w.m[10] * o.z + p2w.m[14];
+        hitM.m[15] = p2w.m[3] * o.x + p2w.m[7] * o.y + p2w.m[11] * o.z + p2w.m[15];
+    } else {
+        return false;
+    }
+    fraction = f;
+    pointA = m4_point(hitM, sw - n * cr);
+    penAxis = m4_point(hitM, -n);
+    return true;
+}
+
 // calculateConstraintPropertiesWithMassOverride axis-constraint-part.ts:222-264 (unit scales)
 template <typename T>
 __device__ inline void ccd_props(CcdPart<T>& p, bool dynA, bool dynB, T invMA, T invMB, const M3<T>& IA, const M3<T>& IB,
@@ -298,9 +344,12 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
         }
         C fraction;
         V3<C> pA, pen;
-        if (!cast_convex(stc, B.shape[a], cvt<C>(B.pos[a]), cvtq<C>(B.q[a]), cvt<C>(rel), B.shape[j], cvt<C>(B.pos[j]),
-                         cvtq<C>(B.q[j]), d.s.backfaces != 0, fraction, pA, pen))
-            continue;
+        const bool hit = stc.shapes[B.shape[j]].type == SHAPE_PLANE
+                             ? cast_convex_plane(stc, B.shape[a], cvt<C>(B.pos[a]), cvtq<C>(B.q[a]), cvt<C>(rel), B.shape[j],
+                                                 cvt<C>(B.pos[j]), cvtq<C>(B.q[j]), d.s.backfaces != 0, fraction, pA, pen)
+                             : cast_convex(stc, B.shape[a], cvt<C>(B.pos[a]), cvtq<C>(B.q[a]), cvt<C>(rel), B.shape[j],
+                                           cvt<C>(B.pos[j]), cvtq<C>(B.q[j]), d.s.backfaces != 0, fraction, pA, pen);
+        if (!hit) continue;
         // addHit :1045-1098
         const V3<T> n = cvt<T>(normalize(pen));
         const T den = dot(n, c.delta);

@@ -764,9 +764,10 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                 const cc_shape_desc& q = shapes[bodies[i].shape];
                 const bool convex = q.type == CC_SHAPE_SPHERE || q.type == CC_SHAPE_BOX || q.type == CC_SHAPE_CAPSULE ||
                                     q.type == CC_SHAPE_CONVEX_HULL || q.type == CC_SHAPE_CYLINDER;
-                if (!convex || q.decorated != 0)
+                if ((!convex && q.type != CC_SHAPE_PLANE) || q.decorated != 0)
                     return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is on the GPU path in worlds of undecorated "
-                                                     "convex shapes only (world " + std::to_string(w) + ", body " + std::to_string(i) + ")");
+                                                     "convex shapes and planes only (world " + std::to_string(w) + ", body " +
+                                                     std::to_string(i) + ")");
                 // getShapeInnerRadius: sphere.ts:147-149, box.ts:333-335, capsule.ts:254-256,
                 // cylinder.ts:261-263, convex-hull.ts:550-558 (the nearest face plane)
                 double r = 0;

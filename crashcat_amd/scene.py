@@ -735,7 +735,7 @@ def compound_pile(seed=1012, n=200, extent=(12.0, 8.0, 12.0), scene=None):
     return sc
 
 
-def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None):
+def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False):
     """Fast LINEAR_CAST bodies (MotionQuality.LINEAR_CAST, src/update.ts:932-967) in a walled
     range: a static floor, four thin static walls (0.1 m) around a 16 m square, one tilted thin
     slab in the middle, and `targets` dynamic boxes, spheres, capsules, cylinders and 12-point hulls
@@ -743,10 +743,13 @@ def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None):
     start inside the walls at 30-150 m/s towards a random point on a wall or a target, with random
     spin and restitution in [0, 0.6); pairs of bullets fly at each other, one kinematic LINEAR_CAST
     paddle sweeps the range at 60 m/s, and a few slow LINEAR_CAST bodies stay below the cast
-    threshold."""
+    threshold.  With `plane_floor` the floor is a static plane through the origin."""
     sc = scene if scene is not None else Scene()
     rng = Rng(seed)
-    sc.add_body(S.box((12.0, 0.5, 12.0)), (0.0, -0.5, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+    if plane_floor:
+        sc.add_body(S.plane((0.0, 1.0, 0.0)), (0.0, 0.0, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+    else:
+        sc.add_body(S.box((12.0, 0.5, 12.0)), (0.0, -0.5, 0.0), motion_type=_abi.CC_MOTION_STATIC)
     for x, z, he in ((8.0, 0.0, (0.05, 3.0, 8.0)), (-8.0, 0.0, (0.05, 3.0, 8.0)),
                      (0.0, 8.0, (8.0, 3.0, 0.05)), (0.0, -8.0, (8.0, 3.0, 0.05))):
         sc.add_body(S.box(he), (x, 3.0, z), motion_type=_abi.CC_MOTION_STATIC)

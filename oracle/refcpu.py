@@ -108,10 +108,11 @@ class OracleWorld:
         shapes, hulls, self._keep = scene.c_shapes()
         if ((bodies["motion_quality"] == _abi.CC_MOTION_QUALITY_LINEAR_CAST)
                 & (bodies["motion_type"] == _abi.CC_MOTION_DYNAMIC)).any():
-            for si in np.unique(bodies["shape"]):  # the CCD cast is restated for convex leaves
+            for si in np.unique(bodies["shape"]):  # the CCD casts are restated for convex leaves and planes
                 if shapes[si].type not in (_abi.CC_SHAPE_SPHERE, _abi.CC_SHAPE_BOX, _abi.CC_SHAPE_CAPSULE,
-                                           _abi.CC_SHAPE_CONVEX_HULL, _abi.CC_SHAPE_CYLINDER) or shapes[si].decorated:
-                    raise ValueError("LINEAR_CAST bodies need a world of undecorated convex shapes")
+                                           _abi.CC_SHAPE_CONVEX_HULL, _abi.CC_SHAPE_CYLINDER,
+                                           _abi.CC_SHAPE_PLANE) or shapes[si].decorated:
+                    raise ValueError("LINEAR_CAST bodies need a world of undecorated convex shapes and planes")
         pcs = [(a - b0, b - b0, la, lb, ov, op) for (a, b, la, lb, ov, op) in scene.point_constraints if b0 <= a < b1]
         pa = (_abi.PointConstraintDesc * max(len(pcs), 1))()
         for i, (a, b, la, lb, ov, op) in enumerate(pcs):

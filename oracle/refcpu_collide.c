@@ -1664,6 +1664,55 @@ int ocx_cast_convex(const OShapes* sh, int sa, V3 pa, Q4 qa, V3 disp, int sb, V3
     return 1;
 }
 
+/* castConvexVsPlane plane.ts:727-880 as the CCD visitor calls it (unit scales): the convex shape's
+ * DEFAULT support point against the plane is taken to world space with A's transform and
+ * measured against the plane in the plane shape's own frame; the contact point goes through the
+ * plane body's transform (translated by the cast for a hit ahead) and the penetration axis is
+ * -normal through that same matrix, as a point.  Returns 1 for a hit below `earlyOut`. */
+int ocx_cast_convex_plane(const OShapes* sh, int sa, V3 pa, Q4 qa, V3 disp, int sp, V3 pb, Q4 qb, int backfaces,
+                          double earlyOut, double* fraction, V3* pointA, V3* penAxis) {
+    const cc_shape_desc* P = &sh->shapes[sp];
+    const V3 sn0 = v3(P->half_extents[0] / 1.0, P->half_extents[1] / 1.0, P->half_extents[2] / 1.0);
+    const double len = vlen(sn0);
+    const V3 n = vscale(sn0, 1 / len);
+    const double c = P->radius / len;
+    const OM4 start = m4_from_rt(qa, pa);
+    const int ty = sh->shapes[sa].type;
+    OSup S;
+    sup_for_shape(&S, sh, sa, (ty == CC_SHAPE_BOX || ty == CC_SHAPE_CYLINDER) ? MODE_INCLUDE : MODE_EXCLUDE);
+    const Q4 iq = {-qa.x, -qa.y, -qa.z, qa.w};
+    const V3 dirS = vneg(qrotate(iq, n));
+    const V3 sw = m4_point(&start, sup_get(&S, dirS));
+    const double sd = vdot(n, sw) + c;
+    const double cr = S.convexRadius;
+    const double pen = -sd + cr;
+    const double dot = vdot(disp, n);
+    OM4 hitM;
+    double f;
+    if (pen > 0.0) {
+        if (!backfaces && dot > 0.0) return 0;
+        if (pen <= -earlyOut) return 0;
+        f = 0.0;
+        hitM = m4_from_rt(qb, pb);
+    } else if (dot < 0.0) {
+        f = pen / dot;
+        if (f >= earlyOut) return 0;
+        const OM4 p2w = m4_from_rt(qb, pb);
+        const V3 o = vscale(disp, f);
+        hitM = p2w; /* mat4.translate */
+        hitM.m[12] = p2w.m[0] * o.x + p2w.m[4] * o.y + p2w.m[8] * o.z + p2w.m[12];
+        hitM.m[13] = p2w.m[1] * o.x + p2w.m[5] * o.y + p2w.m[9] * o.z + p2w.m[13];
+        hitM.m[14] = p2w.m[2] * o.x + p2w.m[6] * o.y + p2w.m[10] * o.z + p2w.m[14];
+        hitM.m[15] = p2w.m[3] * o.x + p2w.m[7] * o.y + p2w.m[11] * o.z + p2w.m[15];
+    } else {
+        return 0;
+    }
+    *fraction = f;
+    *pointA = m4_point(&hitM, vsub(sw, vscale(n, cr)));
+    *penAxis = m4_point(&hitM, vneg(n));
+    return 1;
+}
+
 /* collideShapeVsShape dispatch (src/collision/narrowphase.ts:172-238, src/shapes/sphere.ts:154-167) */
 void ocx_collide(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, double maxSep, double colTol,
                  double penTol, OHit* out) {

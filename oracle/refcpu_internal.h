@@ -148,6 +148,8 @@ typedef struct {
     V3 relA[OC_MAX_MANIFOLD_POINTS], relB[OC_MAX_MANIFOLD_POINTS];
 } OManifold;
 
+int ocx_cast_convex_plane(const OShapes* sh, int sa, V3 pa, Q4 qa, V3 disp, int sp, V3 pb, Q4 qb, int backfaces,
+                          double earlyOut, double* fraction, V3* pointA, V3* penAxis);
 int ocx_cast_convex(const OShapes* sh, int sa, V3 pa, Q4 qa, V3 disp, int sb, V3 pb, Q4 qb, int backfaces,
                     double earlyOut, double* fraction, V3* pointA, V3* penAxis);
 void ocx_collide(const OShapes* sh, int sa, V3 pa, Q4 qa, int sb, V3 pb, Q4 qb, double maxSep,

@@ -1170,11 +1170,23 @@ static V3 body_motion(const OBody* b, double dt) {
 }
 
 /* findCCDContacts :1357-1437 with the visitor :1110-1240 and the collector :1003-1107.  The
- * reference visits the bodies that a swept-AABB cast through its tree reaches; those early-outs
- * are conservative, so every body passes to the filters and the cast here, in index order (the
- * visit order decides only between hits of equal fraction-plus-slop).  The manifold block of
- * :1376-1432 never runs: the collector's reset after every visit (:1238) clears hasHit, so the
- * contact point stays the collector's. */
+ * reference visits the bodies that a swept-AABB cast through its tree reaches (dbvt.ts:869-975)
+ * and whose bounds, widened by A's half size, a ray along the relative displacement meets
+ * (:1194-1206); here a body is visited when A's bounds swept over the relative displacement,
+ * widened by 1e-3 m and a relative 1e-5, overlap its bounds, in index order (the visit order
+ * decides only between hits of equal fraction-plus-slop).  The manifold block of :1376-1432
+ * never runs: the collector's reset after every visit (:1238) clears hasHit, so the contact
+ * point stays the collector's. */
+static int ccd_swept_overlap(const OBody* A, V3 rel, const OBody* B) {
+    const double lo[3] = {A->aabbMin.x + dmin(rel.x, 0.0), A->aabbMin.y + dmin(rel.y, 0.0), A->aabbMin.z + dmin(rel.z, 0.0)};
+    const double hi[3] = {A->aabbMax.x + dmax(rel.x, 0.0), A->aabbMax.y + dmax(rel.y, 0.0), A->aabbMax.z + dmax(rel.z, 0.0)};
+    const double blo[3] = {B->aabbMin.x, B->aabbMin.y, B->aabbMin.z}, bhi[3] = {B->aabbMax.x, B->aabbMax.y, B->aabbMax.z};
+    for (int k = 0; k < 3; k++) {
+        const double m = 1e-3 + 1e-5 * (fabs(lo[k]) + fabs(hi[k]) + fabs(blo[k]) + fabs(bhi[k]));
+        if (lo[k] - m > bhi[k] || blo[k] - m > hi[k]) return 0;
+    }
+    return 1;
+}
 static void ccd_find(oc_world* w, OCcd* c, double dt) {
     const OBody* A = &w->b[c->body];
     for (int j = 0; j < w->nb; j++) {
@@ -1189,11 +1201,15 @@ static void ccd_find(oc_world* w, OCcd* c, double dt) {
             rel = vsub(rel, cb->delta);
         }
         if (vlen2(rel) < c->thrSq) continue;
+        if (!ccd_swept_overlap(A, rel, B)) continue;
         double fraction;
         V3 pA, pen;
-        if (!ocx_cast_convex(&w->sh, A->shape, A->pos, A->q, rel, B->shape, B->pos, B->q, w->s.collide_with_backfaces,
-                             c->fps, &fraction, &pA, &pen))
-            continue;
+        const int hit = w->shapes[B->shape].type == CC_SHAPE_PLANE
+                            ? ocx_cast_convex_plane(&w->sh, A->shape, A->pos, A->q, rel, B->shape, B->pos, B->q,
+                                                    w->s.collide_with_backfaces, c->fps, &fraction, &pA, &pen)
+                            : ocx_cast_convex(&w->sh, A->shape, A->pos, A->q, rel, B->shape, B->pos, B->q,
+                                              w->s.collide_with_backfaces, c->fps, &fraction, &pA, &pen);
+        if (!hit) continue;
         /* addHit :1045-1098 */
         const V3 n = vnormalize(pen);
         const double den = vdot(n, c->delta);

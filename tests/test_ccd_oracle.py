@@ -185,3 +185,37 @@ def test_gallery_keeps_every_bullet_inside_or_above_the_walls():
         p = o.bodies()["position"]
         out = lc & ((np.abs(p[:, 0]) > 8.05) | (np.abs(p[:, 2]) > 8.05)) & (p[:, 1] < 5.5)
         assert not out.any(), np.where(out)
+
+
+@pytest.mark.parametrize("kind,r", [("sphere", 0.25), ("box", 0.3)])
+def test_bullet_stops_on_a_plane_with_the_allowed_penetration(kind, r):
+    """castConvexVsPlane (plane.ts:727-880): a body falling at 150 m/s onto a plane through the
+    origin stops at r - min(penetrationSlop, 0.25 * innerRadius); the discrete twin falls through."""
+    def scene(quality):
+        sc = SC.Scene(_zero_g())
+        sc.add_body(S.plane((0.0, 1.0, 0.0)), (0.0, 0.0, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+        shape = S.sphere(r) if kind == "sphere" else S.box((r, r, r))
+        sc.add_body(shape, (0.3, 1.5, -0.2), linear_velocity=(0.0, -150.0, 0.0), motion_quality=quality,
+                    linear_damping=0.0, angular_damping=0.0)
+        sc.end_world()
+        return sc
+    b = _one_step(scene(LC))
+    assert b["position"][1][1] == pytest.approx(r - min(0.02, 0.25 * r), abs=2e-4)
+    if kind == "sphere":  # centred contact: the normal iteration stops it
+        assert abs(b["linear_velocity"][1][1]) < 1e-9
+    else:  # the DEFAULT support is a corner: part of the impulse goes into spin
+        assert -150.0 < b["linear_velocity"][1][1] < 0.0 and np.abs(b["angular_velocity"][1]).max() > 1.0
+    assert _one_step(scene(0))["position"][1][1] < -0.5
+
+
+def test_plane_cast_measures_against_the_plane_shape_frame():
+    """castConvexVsPlane measures A's world support point against the plane in the plane shape's
+    own frame (plane.ts:784-797), so a plane body raised to y = 1 stops a falling sphere where a
+    plane through the origin would."""
+    sc = SC.Scene(_zero_g())
+    sc.add_body(S.plane((0.0, 1.0, 0.0)), (0.0, 1.0, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+    sc.add_body(S.sphere(0.25), (0.0, 2.0, 0.0), linear_velocity=(0.0, -150.0, 0.0), motion_quality=LC,
+                linear_damping=0.0, angular_damping=0.0)
+    sc.end_world()
+    b = _one_step(sc)
+    assert b["position"][1][1] == pytest.approx(0.25 - 0.02, abs=2e-4)  # not 1.23
