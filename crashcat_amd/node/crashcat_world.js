@@ -10,7 +10,7 @@
 // becomes an inert placeholder), so the sort keys, which crashcat computes from body indices
 // (contact-constraints.ts:1222-1231), are the same.  Features this path does not have are
 // refused with an Error before anything runs: sensors, CCD (MotionQuality.LINEAR_CAST) in a world
-// holding anything but undecorated convex shapes (the library refuses it), empty/user shapes,
+// holding anything but undecorated convex shapes and planes (the library refuses it), empty/user shapes,
 // compounds holding compounds, meshes or planes, moving triangle meshes,
 // constraint types other than points, enhanced internal-edge removal, manifold reduction turned
 // off, kinematic-vs-non-dynamic collision, returnDeepestPoint, a warm-start impulse ratio other
