@@ -491,6 +491,48 @@ def config4(args, S, Context):
     return out
 
 
+def ccd_batch(args, S, Context):
+    """CCD (MotionQuality.LINEAR_CAST): a batch of gallery worlds (bullets of every convex shape at
+    30-150 m/s into thin walls, resting and sleeping targets and each other) through the first
+    second, where the CCD records are; FP32 and FP64, and the same worlds with the bullets
+    discrete for the cost of the CCD pass."""
+    sc = S.Scene()
+    for w in range(args.ccd_worlds):
+        S.ccd_gallery(seed=5000 + w, scene=sc)
+    nb = sc.num_bodies
+    out = {"workload": f"{args.ccd_worlds} CCD gallery worlds ({nb // args.ccd_worlds} bodies each: 24 LINEAR_CAST "
+                       "bullets, 6 head-on, a kinematic paddle, 60 targets, 4 slow LINEAR_CAST bodies, walls), seeds "
+                       "5000+, dt 1/60, steps 1-60 timed in chunks of 10", "bodies": nb}
+    for prec in ("fp32", "fp64"):
+        c = Context(prec, max_pairs_per_body=16)
+        c.upload(sc)
+        c.step(1.0 / 60.0, 1)  # kernels loaded, autotuned
+        c.sync()
+        c.upload(sc)
+        e = advance(c, 60, f"ccd {prec}", chunk=10)
+        recs = c.ccd_bodies()
+        out[f"{prec}_body_steps_per_s"] = nb * 60 / e
+        out[f"{prec}_ms_per_step"] = e / 60 * 1e3
+        out[f"{prec}_ccd_records_last_step"] = int(len(recs))
+        log(f"ccd {prec}: 60 steps in {e:.3f}s -> {nb * 60 / e:.4g} body-steps/s, {len(recs)} CCD records in the last step")
+        c.close()
+    d = S.Scene()
+    for w in range(args.ccd_worlds):
+        S.ccd_gallery(seed=5000 + w, scene=d)
+    bodies = d.bodies().copy()  # the same worlds with every body discrete
+    bodies["motion_quality"] = 0
+    d._bodies = bodies
+    c = Context("fp32", max_pairs_per_body=16)
+    c.upload(d)
+    c.step(1.0 / 60.0, 1)
+    c.sync()
+    c.upload(d)
+    e = advance(c, 60, "ccd fp32 discrete", chunk=10)
+    out["fp32_discrete_ms_per_step"] = e / 60 * 1e3
+    c.close()
+    return out
+
+
 def config5(args, S, Context):
     from oracle.refcpu import OracleWorld
     nw, steps, K = args.throw_worlds, 120, 10
@@ -616,6 +658,7 @@ def main():
     ap.add_argument("--throw-worlds", type=int, default=1024, help="worlds of config 5 (0 skips it)")
     ap.add_argument("--fd-worlds", type=int, default=4, help="config-5 worlds compared with finite differences")
     ap.add_argument("--skip-config1", action="store_true")
+    ap.add_argument("--ccd-worlds", type=int, default=512, help="CCD gallery worlds (0 skips them)")
     ap.add_argument("--profile", action="store_true", help="only the headline trajectory (for rocprofv3)")
     ap.add_argument("--narrowphase", default="fp64", choices=("fp64", "fp32"),
                     help="arithmetic of GJK and EPA in the FP32 batch")
@@ -692,6 +735,7 @@ def main():
     c3 = config3(args, S, Context) if args.batched_worlds else None
     c4 = config4(args, S, Context) if args.terrain_bodies else None
     c5 = config5(args, S, Context) if args.throw_worlds else None
+    ccd = ccd_batch(args, S, Context) if args.ccd_worlds else None
     cpu = None if args.skip_cpu else cpu_baseline(args, batch, H["start_state"], H["traj"] - K + 1)
 
     out = {
@@ -733,6 +777,7 @@ def main():
         "config3": c3,
         "config4": c4,
         "config5": c5,
+        "ccd": ccd,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
