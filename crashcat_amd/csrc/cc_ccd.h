@@ -296,16 +296,19 @@ __device__ inline void apply_ccd(const Dev<T>& d, CcdPart<T>* parts, const CcdRe
     if (dynB) ccd_clamp(B, b);
 }
 
-// One CCD body's search (findCCDContacts update.ts:1357-1437 with the visitor :1110-1240 and the
-// collector :1003-1107): every body of the world [b0, b1) passes the visitor's filters and the
-// cast; the reference's tree and ray early-outs are conservative, so the swept-AABB overlap that
-// stands in for them here (A's bounds over its relative displacement, widened by 1e-3 m and a
-// relative term for the bounds' rounding) keeps every body the cast could hit.  The earliest
+// One CCD body's search by one wavefront (findCCDContacts update.ts:1357-1437 with the visitor
+// :1110-1240 and the collector :1003-1107): the lanes take the world's bodies [b0, b1) in turn
+// through the visitor's filters, a swept-bounds test (A's bounds over its relative displacement,
+// widened by 1e-3 m and a relative term for the bounds' rounding, against the body's bounds; it
+// stands in for the reference's tree and ray early-outs) and the cast.  The earliest
 // fraction-plus-slop wins, equal ones by the lower body index (the reference decides those by its
 // tree's visit order).  The manifold block of :1376-1432 never runs there (the collector's reset
-// after each visit, :1238, clears hasHit), so the contact point is the collector's.
-template <typename T, typename C, int BS>
+// after each visit, :1238, clears hasHit), so the contact point is the collector's.  The searches
+// read only the state before the CCD pass, so the workgroup's wavefronts run different records
+// at once.
+template <typename T, typename C>
 __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccdIdx, int cb0, int e, int b0, int b1, T dt) {
+    const int lane = threadIdx.x & 63;
     const Bodies<T>& B = d.B;
     const CcdRec<T> c = recs[cb0 + e];
     const int a = c.body;
@@ -316,7 +319,7 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
     int bestJ = 0x7fffffff;
     T bFrac = T(0);
     V3<T> bNormal = mk3<T>(0, 0, 0), bPoint = mk3<T>(0, 0, 0);
-    for (int j = b0 + threadIdx.x; j < b1; j += BS) {
+    for (int j = b0 + lane; j < b1; j += 64) {
         if (j == a) continue;
         const int fb = B.flags[j];
         if (!d.s.layerPairs[fl_layer(fa) * 8 + fl_layer(fb)]) continue;
@@ -363,9 +366,7 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
         const V3<T> mb = ((fb & FL_LINEAR_CAST) && fl_motion(fb) != MOTION_STATIC) ? B.v[j] * dt : mk3<T>(0, 0, 0);
         bPoint = len2(mb) > T(1e-12) ? cvt<T>(pA) + mb * T(fraction) : cvt<T>(pA);
     }
-    // the workgroup's earliest (fraction-plus-slop, body index)
-    __shared__ T sBest[BS / 64];
-    __shared__ int sJ[BS / 64];
+    // the wavefront's earliest (fraction-plus-slop, body index)
     T m = best;
     int mj = bestJ;
     for (int o = 32; o > 0; o >>= 1) {
@@ -373,12 +374,6 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
         const int oj = __shfl_xor(mj, o, 64);
         if (om < m || (om == m && oj < mj)) { m = om; mj = oj; }
     }
-    if ((threadIdx.x & 63) == 0) { sBest[threadIdx.x >> 6] = m; sJ[threadIdx.x >> 6] = mj; }
-    __syncthreads();
-    m = sBest[0];
-    mj = sJ[0];
-    for (int k = 1; k < BS / 64; k++)
-        if (sBest[k] < m || (sBest[k] == m && sJ[k] < mj)) { m = sBest[k]; mj = sJ[k]; }
     if (bestJ != 0x7fffffff && bestJ == mj) {
         CcdRec<T>& r = recs[cb0 + e];
         r.hit = bestJ;
@@ -389,7 +384,6 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
         r.friction = combine(B.friction[a], B.friction[bestJ], fl_fmode(fa), fl_fmode(B.flags[bestJ]));
         r.restitution = combine(B.restitution[a], B.restitution[bestJ], fl_rmode(fa), fl_rmode(B.flags[bestJ]));
     }
-    __syncthreads();
 }
 
 // sortCCDBodies :1607-1621 (by fraction-plus-slop, then body index) and resolveCCDContacts

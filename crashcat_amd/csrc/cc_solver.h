@@ -1207,10 +1207,11 @@ __global__ void __launch_bounds__(BS) k_world(Dev<T> d, T dt) {
         __syncthreads();
         if (nccd > 0) {
             using C = typename std::conditional<sizeof(T) == 4, double, T>::type;
-            for (int e = 0; e < nccd; e++) {
-                if (d.npWide) ccd_find<T, C, BS>(d, d.ccd, d.ccdIdx, cb0, e, b0, b1, dt);
-                else ccd_find<T, T, BS>(d, d.ccd, d.ccdIdx, cb0, e, b0, b1, dt);
+            for (int e = tid >> 6; e < nccd; e += BS / 64) {  // one record per wavefront
+                if (d.npWide) ccd_find<T, C>(d, d.ccd, d.ccdIdx, cb0, e, b0, b1, dt);
+                else ccd_find<T, T>(d, d.ccd, d.ccdIdx, cb0, e, b0, b1, dt);
             }
+            __syncthreads();
             ccd_resolve<T, BS>(d, d.ccd, d.ccdOrd, d.ccdIdx, d.ccdPart + 2 * (size_t)w, cb0, nccd);
         }
     }
