@@ -1,10 +1,10 @@
 // Continuous collision detection for MotionQuality.LINEAR_CAST bodies (src/update.ts:896-1677,
 // src/ccd.ts), run by the world's workgroup between velocity integration and the position
 // iterations: the integration pass delays the move of every dynamic LINEAR_CAST body whose step
-// displacement exceeds its threshold and registers it here in body order; the workgroup then
-// casts each registered body against the world's bodies (one thread per candidate, the earliest
-// hit by a workgroup reduction), ranks the records by time of impact and resolves them in that
-// order on one thread, as the reference does.
+// displacement exceeds its threshold and registers it here in body order; each wavefront then
+// casts one registered body at a time against the world's bodies (one lane per candidate, the
+// earliest hit by a shuffle reduction), and the workgroup ranks the records by time of impact and
+// resolves them in that order on one thread, as the reference does.
 #pragma once
 #include "cc_kernels.h"
 
