@@ -175,6 +175,14 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_contact_removed, body_a, "i32", 1); F(cc_contact_removed, body_b, "i32", 1);
     F(cc_contact_removed, sub_shape_a, "u32", 1); F(cc_contact_removed, sub_shape_b, "u32", 1);
     add_struct(env, out, "cc_contact_removed", sizeof(cc_contact_removed), fields);
+    napi_create_object(env, &fields);
+    F(cc_ccd_body, body_index, "i32", 1); F(cc_ccd_body, hit_body_index, "i32", 1);
+    F(cc_ccd_body, delta_position, "f64", 3); F(cc_ccd_body, contact_normal, "f64", 3);
+    F(cc_ccd_body, contact_point, "f64", 3); F(cc_ccd_body, fraction, "f64", 1);
+    F(cc_ccd_body, fraction_plus_slop, "f64", 1); F(cc_ccd_body, linear_cast_threshold_sq, "f64", 1);
+    F(cc_ccd_body, max_penetration, "f64", 1); F(cc_ccd_body, combined_friction, "f64", 1);
+    F(cc_ccd_body, combined_restitution, "f64", 1);
+    add_struct(env, out, "cc_ccd_body", sizeof(cc_ccd_body), fields);
     return out;
 }
 #undef F
@@ -404,6 +412,27 @@ static napi_value download_records(napi_env env, napi_callback_info info, int re
 static napi_value download_contacts(napi_env env, napi_callback_info info) { return download_records(env, info, 0); }
 static napi_value download_removed(napi_env env, napi_callback_info info) { return download_records(env, info, 1); }
 
+/* ---- downloadCcdBodies(handle) -> ArrayBuffer of cc_ccd_body records (world.ccd.ccdBodies) --- */
+static napi_value download_ccd(napi_env env, napi_callback_info info) {
+    size_t argc = 1;
+    napi_value a[1];
+    CHECK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+    cc_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    int32_t count = 0;
+    int rc = cc_download_ccd_bodies(c, NULL, 0, &count);
+    if (rc != CC_OK) return throw_rc(env, c, rc);
+    void* data;
+    napi_value ab;
+    CHECK(env, napi_create_arraybuffer(env, (size_t)count * sizeof(cc_ccd_body), &data, &ab));
+    if (count > 0) {
+        int32_t got = 0;
+        rc = cc_download_ccd_bodies(c, (cc_ccd_body*)data, count, &got);
+        if (rc != CC_OK) return throw_rc(env, c, rc);
+    }
+    return ab;
+}
+
 /* ---- addForces(ctx, force, torque, wake): Float64Array (3 per body) or null each ------- */
 static napi_value add_forces(napi_env env, napi_callback_info info) {
     size_t argc = 4;
@@ -530,6 +559,7 @@ static napi_value init(napi_env env, napi_value exports) {
         {"downloadPairs", 0, download_pairs, 0, 0, 0, napi_default, 0},
         {"downloadContacts", 0, download_contacts, 0, 0, 0, napi_default, 0},
         {"downloadRemovedContacts", 0, download_removed, 0, 0, 0, napi_default, 0},
+        {"downloadCcdBodies", 0, download_ccd, 0, 0, 0, napi_default, 0},
         {"snapshot", 0, snapshot, 0, 0, 0, napi_default, 0},
         {"restore", 0, restore, 0, 0, 0, napi_default, 0},
         {"addForces", 0, add_forces, 0, 0, 0, napi_default, 0},

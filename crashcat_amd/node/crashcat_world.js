@@ -456,6 +456,30 @@ class CrashcatGpuWorld {
     this.written = states;
   }
 
+  // world.ccd.ccdBodies as updateWorld leaves it (src/ccd.ts:19-60, src/update.ts:131-140): the
+  // step's CCD records in time-of-impact order, and each CCD body's ccdBodyIndex, its place in
+  // body order (before the sort).  A hit's sub-shape ids are the empty id (convex shapes and
+  // planes); a record without a hit keeps the reset values.
+  writeCcd() {
+    if (!this.world.ccd) this.world.ccd = { ccdBodies: [] };
+    const pool = this.world.bodies.pool;
+    for (const b of pool) if (b && !b._pooled) b.ccdBodyIndex = -1;
+    const recs = this.gpu.ccdBodies();
+    const byBody = recs.map((r) => r.body_index).sort((x, y) => x - y);
+    byBody.forEach((bi, k) => { if (pool[bi]) pool[bi].ccdBodyIndex = k; });
+    this.world.ccd.ccdBodies = recs.map((r) => ({
+      bodyIndex: r.body_index, hitBodyIndex: r.hit_body_index,
+      deltaPosition: r.delta_position.slice(), contactNormal: r.contact_normal.slice(),
+      contactPoint: r.contact_point.slice(), fraction: r.fraction, fractionPlusSlop: r.fraction_plus_slop,
+      linearCastThresholdSq: r.linear_cast_threshold_sq, maxPenetration: r.max_penetration,
+      subShapeId1: r.hit_body_index >= 0 ? 0xffffffff : 0, subShapeId2: r.hit_body_index >= 0 ? 0xffffffff : 0,
+      contactSettings: {
+        combinedFriction: r.combined_friction, combinedRestitution: r.combined_restitution, isSensor: false,
+        invMassScale1: 1, invMassScale2: 1, invInertiaScale1: 1, invInertiaScale2: 1,
+      },
+    }));
+  }
+
   // The listener's contact callbacks with the RigidBody objects (src/listener.ts:23-99): the
   // manifold in crashcat's form (base offset at the origin, so the relative points are world
   // points), the contact settings with the combined friction and restitution.
@@ -497,6 +521,7 @@ class CrashcatGpuWorld {
     this.pushForces();
     this.gpu.step(timeStep, 1);
     this.writeBack();
+    this.writeCcd();
     this.world.previousTimeStep = timeStep;
     if (listener) this.dispatch(listener);
   }

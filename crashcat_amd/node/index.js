@@ -169,6 +169,8 @@ class GpuWorld {
   // before also had) and the contacts the last step removed (cc_contact_removed records)
   contacts() { return unpack('cc_contact_out', addon.downloadContacts(this.handle)); }
   removedContacts() { return unpack('cc_contact_removed', addon.downloadRemovedContacts(this.handle)); }
+  // the last step's CCD records in time-of-impact order (world.ccd.ccdBodies after updateWorld)
+  ccdBodies() { return unpack('cc_ccd_body', addon.downloadCcdBodies(this.handle)); }
 
   // the state one step hands to the next as an ArrayBuffer (cc_snapshot), and back into a
   // world made from the same description (cc_restore): the steps after it repeat bit for bit

@@ -31,6 +31,8 @@ for (let s = 0; s < steps; s++) {
     const m = world.bodies.pool[forceBody].motionProperties;
     ev.forceCleared = m.force.every((x) => x === 0) && m.torque.every((x) => x === 0);
   }
+  ev.ccd = world.ccd.ccdBodies.map((c) => [c.bodyIndex, c.hitBodyIndex, c.fraction, c.fractionPlusSlop,
+                                           c.contactPoint[0], c.contactPoint[1], c.contactPoint[2]]);
   console.log(JSON.stringify(ev));
 }
 const out = world.bodies.pool.map((b) => ({

@@ -351,6 +351,10 @@ def test_crashcat_world_steps_bitwise_with_events_and_forces(tmp_path, which):
         assert events[s]["persisted"] == [x for x, k in zip(rows, c) if k["persisted"]], f"step {s + 1}"
         assert events[s]["removed"] == [[int(k["body_a"]), int(k["body_b"]), int(k["sub_shape_a"]),
                                          int(k["sub_shape_b"])] for k in rm], f"step {s + 1}"
+        cc = orc.ccd_bodies()  # world.ccd.ccdBodies written back by the adapter
+        assert events[s]["ccd"] == [[int(k["body_index"]), int(k["hit_body_index"]), float(k["fraction"]),
+                                     float(k["fraction_plus_slop"])] + [float(x) for x in k["contact_point"]]
+                                    for k in cc], f"step {s + 1}"
     assert events[fstep]["forceCleared"]
     ref = orc.bodies()
     got = final["bodies"]
