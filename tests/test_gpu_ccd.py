@@ -147,3 +147,30 @@ def test_differentiable_stepping_refuses_ccd_worlds():
             g.fwd(DT, 2, 1)
     finally:
         g.close()
+
+
+def test_fp32_narrowphase_mode_ccd_keeps_bullets_in_the_range():
+    """With the narrowphase in FP32 (cc_config.narrowphase_fp32) the shape casts run in FP32 too:
+    one step from the oracle's state stays within 1e-3 m at the 99th percentile, and over 40 steps
+    no LINEAR_CAST body leaves the walled range below the walls' top."""
+    sc = SC.ccd_gallery(seed=1013)
+    run = O.OracleWorld(sc)
+    for _ in range(3):
+        run.step(DT)
+    ob, gb, oc, gc = _fp32_one_step(sc, states=SC.as_fp32(run.bodies()), pairs=16, narrowphase="fp32",
+                                    same_contacts=False)
+    err, dyn = _step_error(sc, ob, gb)
+    print(f"fp32-narrowphase ccd gallery step: p99 {np.quantile(err[dyn], 0.99):.3g} max {err.max():.3g}")
+    assert np.quantile(err[dyn], 0.99) <= 1e-3
+    bd = sc.bodies()
+    lc = (bd["motion_quality"] == _abi.CC_MOTION_QUALITY_LINEAR_CAST) & (bd["motion_type"] == _abi.CC_MOTION_DYNAMIC)
+    g = Context("fp32", max_pairs_per_body=16, narrowphase="fp32")
+    g.upload(sc)
+    try:
+        for _ in range(40):
+            g.step(DT, 1)
+            p = g.bodies()["position"]
+            out = lc & ((np.abs(p[:, 0]) > 8.05) | (np.abs(p[:, 2]) > 8.05)) & (p[:, 1] < 5.5)
+            assert not out.any(), np.where(out)
+    finally:
+        g.close()
