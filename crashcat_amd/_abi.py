@@ -215,6 +215,8 @@ class CcdBody(C.Structure):
         ("max_penetration", C.c_double),
         ("combined_friction", C.c_double),
         ("combined_restitution", C.c_double),
+        ("sub_shape_id_1", C.c_uint32),
+        ("sub_shape_id_2", C.c_uint32),
     ]
 
 

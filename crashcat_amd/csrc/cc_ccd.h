@@ -6,7 +6,7 @@
 // earliest hit by a shuffle reduction), and the workgroup ranks the records by time of impact and
 // resolves them in that order on one thread, as the reference does.
 #pragma once
-#include "cc_kernels.h"
+#include "cc_compound.h"
 
 namespace cc {
 
@@ -317,6 +317,7 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
     const V3<double> amn = cvt<double>(B.amin[a]), amx = cvt<double>(B.amax[a]);
     T best = c.fps;
     int bestJ = 0x7fffffff;
+    unsigned bSub = 0xffffffffu;
     T bFrac = T(0);
     V3<T> bNormal = mk3<T>(0, 0, 0), bPoint = mk3<T>(0, 0, 0);
     for (int j = b0 + lane; j < b1; j += 64) {
@@ -345,26 +346,51 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
             }
             if (sep) continue;
         }
-        C fraction;
-        V3<C> pA, pen;
-        const bool hit = stc.shapes[B.shape[j]].type == SHAPE_PLANE
-                             ? cast_convex_plane(stc, B.shape[a], cvt<C>(B.pos[a]), cvtq<C>(B.q[a]), cvt<C>(rel), B.shape[j],
-                                                 cvt<C>(B.pos[j]), cvtq<C>(B.q[j]), d.s.backfaces != 0, fraction, pA, pen)
-                             : cast_convex(stc, B.shape[a], cvt<C>(B.pos[a]), cvtq<C>(B.q[a]), cvt<C>(rel), B.shape[j],
-                                           cvt<C>(B.pos[j]), cvtq<C>(B.q[j]), d.s.backfaces != 0, fraction, pA, pen);
-        if (!hit) continue;
-        // addHit :1045-1098
-        const V3<T> n = cvt<T>(normalize(pen));
-        const T den = dot(n, c.delta);
-        if (den <= c.maxPen) continue;
-        const T fps = T(fraction) + c.maxPen / den;
-        if (!(fps < best)) continue;
-        best = fps;
-        bestJ = j;
-        bFrac = T(fraction);
-        bNormal = n;
-        const V3<T> mb = ((fb & FL_LINEAR_CAST) && fl_motion(fb) != MOTION_STATIC) ? B.v[j] * dt : mk3<T>(0, 0, 0);
-        bPoint = len2(mb) > T(1e-12) ? cvt<T>(pA) + mb * T(fraction) : cvt<T>(pA);
+        // a leaf, or every child of a compound at the body pose composed with the child's transform
+        // with its sub-shape id pushed (castShapeVsCompound compound.ts)
+        const int sj = B.shape[j];
+        const ShapeD<C>& DJ = stc.shapes[sj];
+        const bool cmp = DJ.type == SHAPE_COMPOUND;
+        const int nk = cmp ? DJ.ncc : 1;
+        const int nbits = cmp ? sub_bits(nk) : 0;
+        const V3<C> pj = cvt<C>(B.pos[j]);
+        const Q4<C> qj = cvtq<C>(B.q[j]);
+        for (int k = 0; k < nk; k++) {
+            int sk = sj;
+            V3<C> pk = pj;
+            Q4<C> qk = qj;
+            unsigned subk = 0xffffffffu;
+            if (cmp) {
+                sk = DJ.cfc + k;
+                const ShapeD<C>& K = stc.shapes[sk];
+                pk = pj + qrotate(qj, mk3<C>(K.lp[0], K.lp[1], K.lp[2]));
+                Q4<C> lq;
+                lq.x = K.lq[0]; lq.y = K.lq[1]; lq.z = K.lq[2]; lq.w = K.lq[3];
+                qk = qmul(qj, lq);
+                subk = sub_push(0xffffffffu, 0, k, nbits);
+            }
+            C fraction;
+            V3<C> pA, pen;
+            const bool hit = stc.shapes[sk].type == SHAPE_PLANE
+                                 ? cast_convex_plane(stc, B.shape[a], cvt<C>(B.pos[a]), cvtq<C>(B.q[a]), cvt<C>(rel), sk, pk,
+                                                     qk, d.s.backfaces != 0, fraction, pA, pen)
+                                 : cast_convex(stc, B.shape[a], cvt<C>(B.pos[a]), cvtq<C>(B.q[a]), cvt<C>(rel), sk, pk, qk,
+                                               d.s.backfaces != 0, fraction, pA, pen);
+            if (!hit) continue;
+            // addHit :1045-1098
+            const V3<T> n = cvt<T>(normalize(pen));
+            const T den = dot(n, c.delta);
+            if (den <= c.maxPen) continue;
+            const T fps = T(fraction) + c.maxPen / den;
+            if (!(fps < best)) continue;
+            best = fps;
+            bestJ = j;
+            bSub = subk;
+            bFrac = T(fraction);
+            bNormal = n;
+            const V3<T> mb = ((fb & FL_LINEAR_CAST) && fl_motion(fb) != MOTION_STATIC) ? B.v[j] * dt : mk3<T>(0, 0, 0);
+            bPoint = len2(mb) > T(1e-12) ? cvt<T>(pA) + mb * T(fraction) : cvt<T>(pA);
+        }
     }
     // the wavefront's earliest (fraction-plus-slop, body index)
     T m = best;
@@ -377,6 +403,7 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
     if (bestJ != 0x7fffffff && bestJ == mj) {
         CcdRec<T>& r = recs[cb0 + e];
         r.hit = bestJ;
+        r.sub2 = bSub;
         r.fraction = bFrac;
         r.fps = best;
         r.normal = bNormal;

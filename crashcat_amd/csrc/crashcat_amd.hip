@@ -764,9 +764,16 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                 const cc_shape_desc& q = shapes[bodies[i].shape];
                 const bool convex = q.type == CC_SHAPE_SPHERE || q.type == CC_SHAPE_BOX || q.type == CC_SHAPE_CAPSULE ||
                                     q.type == CC_SHAPE_CONVEX_HULL || q.type == CC_SHAPE_CYLINDER;
-                if ((!convex && q.type != CC_SHAPE_PLANE) || q.decorated != 0)
+                bool ok = (convex || q.type == CC_SHAPE_PLANE) && q.decorated == 0;
+                if (q.type == CC_SHAPE_COMPOUND) {  // a compound of unscaled convex children, hit but never cast
+                    ok = !(bodies[i].motion_quality == CC_MOTION_QUALITY_LINEAR_CAST && bodies[i].motion_type == CC_MOTION_DYNAMIC);
+                    for (int k = 0; k < q.compound_num_children && ok; k++)
+                        ok = (shapes[q.compound_first_child + k].decorated & ~CC_DECO_TRANSFORMED) == 0;
+                }
+                if (!ok)
                     return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is on the GPU path in worlds of undecorated "
-                                                     "convex shapes and planes only (world " + std::to_string(w) + ", body " +
+                                                     "convex shapes, planes and compounds of unscaled convex children only, the "
+                                                     "LINEAR_CAST bodies convex (world " + std::to_string(w) + ", body " +
                                                      std::to_string(i) + ")");
                 // getShapeInnerRadius: sphere.ts:147-149, box.ts:333-335, capsule.ts:254-256,
                 // cylinder.ts:261-263, convex-hull.ts:550-558 (the nearest face plane)
@@ -774,7 +781,7 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                 if (q.type == CC_SHAPE_SPHERE || q.type == CC_SHAPE_CAPSULE) r = q.radius;
                 else if (q.type == CC_SHAPE_BOX) r = std::min(q.half_extents[0], std::min(q.half_extents[1], q.half_extents[2]));
                 else if (q.type == CC_SHAPE_CYLINDER) r = std::min(q.half_height, q.radius);
-                else {
+                else if (q.type == CC_SHAPE_CONVEX_HULL) {
                     r = 1.7976931348623157e308;
                     for (int f = 0; f < q.hull_num_faces; f++) r = std::min(r, -hulls->plane_constants[q.hull_first_face + f]);
                     r = std::max(0.0, r);
@@ -2138,6 +2145,8 @@ static int ccd_bodies_impl(cc_ctx* c, cc_ccd_body* out, int cap, int* count) {
             o.max_penetration = r.maxPen;
             o.combined_friction = r.friction;
             o.combined_restitution = r.restitution;
+            o.sub_shape_id_1 = r.hit >= 0 ? 0xffffffffu : 0u;
+            o.sub_shape_id_2 = r.hit >= 0 ? r.sub2 : 0u;
         }
     *count = k;
     return CC_OK;

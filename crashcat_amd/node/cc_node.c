@@ -182,6 +182,7 @@ static napi_value layout(napi_env env, napi_callback_info info) {
     F(cc_ccd_body, fraction_plus_slop, "f64", 1); F(cc_ccd_body, linear_cast_threshold_sq, "f64", 1);
     F(cc_ccd_body, max_penetration, "f64", 1); F(cc_ccd_body, combined_friction, "f64", 1);
     F(cc_ccd_body, combined_restitution, "f64", 1);
+    F(cc_ccd_body, sub_shape_id_1, "u32", 1); F(cc_ccd_body, sub_shape_id_2, "u32", 1);
     add_struct(env, out, "cc_ccd_body", sizeof(cc_ccd_body), fields);
     return out;
 }

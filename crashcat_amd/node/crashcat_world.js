@@ -10,7 +10,7 @@
 // becomes an inert placeholder), so the sort keys, which crashcat computes from body indices
 // (contact-constraints.ts:1222-1231), are the same.  Features this path does not have are
 // refused with an Error before anything runs: sensors, CCD (MotionQuality.LINEAR_CAST) in a world
-// holding anything but undecorated convex shapes and planes (the library refuses it), empty/user shapes,
+// it cannot cast in (the library refuses it), empty/user shapes,
 // compounds holding compounds, meshes or planes, moving triangle meshes,
 // constraint types other than points, enhanced internal-edge removal, manifold reduction turned
 // off, kinematic-vs-non-dynamic collision, returnDeepestPoint, a warm-start impulse ratio other
@@ -458,8 +458,7 @@ class CrashcatGpuWorld {
 
   // world.ccd.ccdBodies as updateWorld leaves it (src/ccd.ts:19-60, src/update.ts:131-140): the
   // step's CCD records in time-of-impact order, and each CCD body's ccdBodyIndex, its place in
-  // body order (before the sort).  A hit's sub-shape ids are the empty id (convex shapes and
-  // planes); a record without a hit keeps the reset values.
+  // body order (before the sort).  A record without a hit keeps the reset sub-shape ids (0).
   writeCcd() {
     if (!this.world.ccd) this.world.ccd = { ccdBodies: [] };
     const pool = this.world.bodies.pool;
@@ -472,7 +471,7 @@ class CrashcatGpuWorld {
       deltaPosition: r.delta_position.slice(), contactNormal: r.contact_normal.slice(),
       contactPoint: r.contact_point.slice(), fraction: r.fraction, fractionPlusSlop: r.fraction_plus_slop,
       linearCastThresholdSq: r.linear_cast_threshold_sq, maxPenetration: r.max_penetration,
-      subShapeId1: r.hit_body_index >= 0 ? 0xffffffff : 0, subShapeId2: r.hit_body_index >= 0 ? 0xffffffff : 0,
+      subShapeId1: r.sub_shape_id_1, subShapeId2: r.sub_shape_id_2,
       contactSettings: {
         combinedFriction: r.combined_friction, combinedRestitution: r.combined_restitution, isSensor: false,
         invMassScale1: 1, invMassScale2: 1, invInertiaScale1: 1, invInertiaScale2: 1,

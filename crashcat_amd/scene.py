@@ -735,7 +735,7 @@ def compound_pile(seed=1012, n=200, extent=(12.0, 8.0, 12.0), scene=None):
     return sc
 
 
-def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False):
+def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False, compound_targets=False):
     """Fast LINEAR_CAST bodies (MotionQuality.LINEAR_CAST, src/update.ts:932-967) in a walled
     range: a static floor, four thin static walls (0.1 m) around a 16 m square, one tilted thin
     slab in the middle, and `targets` dynamic boxes, spheres, capsules, cylinders and 12-point hulls
@@ -743,7 +743,8 @@ def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False
     start inside the walls at 30-150 m/s towards a random point on a wall or a target, with random
     spin and restitution in [0, 0.6); pairs of bullets fly at each other, one kinematic LINEAR_CAST
     paddle sweeps the range at 60 m/s, and a few slow LINEAR_CAST bodies stay below the cast
-    threshold.  With `plane_floor` the floor is a static plane through the origin."""
+    threshold.  With `plane_floor` the floor is a static plane through the origin; with
+    `compound_targets` every third target is a compound of two or three unscaled leaves."""
     sc = scene if scene is not None else Scene()
     rng = Rng(seed)
     if plane_floor:
@@ -773,7 +774,11 @@ def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False
         p = (rng.uniform(-6.5, 6.5), rng.uniform(0.3, 0.6), rng.uniform(-6.5, 6.5))
         if abs(p[0]) < 2.5 and abs(p[2]) < 1.5:
             p = (p[0] + 5.0, p[1], p[2])
-        sc.add_body(shape(0.2, 0.4), p, rng.quaternion(), sleeping=rng.uniform() < 1.0 / 3.0)
+        tshape = shape(0.2, 0.4)
+        if compound_targets and len(aims) % 3 == 0:
+            tshape = S.compound([((rng.uniform(-0.3, 0.3), rng.uniform(-0.2, 0.2), rng.uniform(-0.3, 0.3)), rng.quaternion(),
+                                  shape(0.12, 0.25)) for _ in range(2 + int(rng.uniform() * 2))])
+        sc.add_body(tshape, p, rng.quaternion(), sleeping=rng.uniform() < 1.0 / 3.0)
         aims.append(p)
 
     def fire(pos, aim, speed, **kw):

@@ -304,6 +304,8 @@ typedef struct cc_ccd_body {
     double max_penetration;
     double combined_friction;
     double combined_restitution;
+    uint32_t sub_shape_id_1;            /* the hit's sub-shape ids (EMPTY 0xffffffff on a leaf; */
+    uint32_t sub_shape_id_2;            /* a compound child's pushed index); 0 without a hit   */
 } cc_ccd_body;
 
 /* One manifold of the contact cache the next step warm-starts from (CachedManifold in

@@ -108,11 +108,21 @@ class OracleWorld:
         shapes, hulls, self._keep = scene.c_shapes()
         if ((bodies["motion_quality"] == _abi.CC_MOTION_QUALITY_LINEAR_CAST)
                 & (bodies["motion_type"] == _abi.CC_MOTION_DYNAMIC)).any():
-            for si in np.unique(bodies["shape"]):  # the CCD casts are restated for convex leaves and planes
-                if shapes[si].type not in (_abi.CC_SHAPE_SPHERE, _abi.CC_SHAPE_BOX, _abi.CC_SHAPE_CAPSULE,
-                                           _abi.CC_SHAPE_CONVEX_HULL, _abi.CC_SHAPE_CYLINDER,
-                                           _abi.CC_SHAPE_PLANE) or shapes[si].decorated:
-                    raise ValueError("LINEAR_CAST bodies need a world of undecorated convex shapes and planes")
+            convex = (_abi.CC_SHAPE_SPHERE, _abi.CC_SHAPE_BOX, _abi.CC_SHAPE_CAPSULE, _abi.CC_SHAPE_CONVEX_HULL,
+                      _abi.CC_SHAPE_CYLINDER)
+            lc = (bodies["motion_quality"] == _abi.CC_MOTION_QUALITY_LINEAR_CAST) & (bodies["motion_type"] == _abi.CC_MOTION_DYNAMIC)
+            for si in np.unique(bodies["shape"][lc]):  # the cast is restated for convex leaves
+                if shapes[si].type not in convex or shapes[si].decorated:
+                    raise ValueError("LINEAR_CAST bodies must be undecorated convex shapes")
+            for si in np.unique(bodies["shape"]):  # ... against convex leaves, planes and compounds of them
+                s = shapes[si]
+                ok = (s.type in convex or s.type == _abi.CC_SHAPE_PLANE) and not s.decorated
+                if s.type == _abi.CC_SHAPE_COMPOUND:
+                    ok = all((shapes[s.compound_first_child + k].decorated & ~_abi.CC_DECO_TRANSFORMED) == 0
+                             for k in range(s.compound_num_children))
+                if not ok:
+                    raise ValueError("LINEAR_CAST bodies need a world of undecorated convex shapes, planes and "
+                                     "compounds of unscaled convex children")
         pcs = [(a - b0, b - b0, la, lb, ov, op) for (a, b, la, lb, ov, op) in scene.point_constraints if b0 <= a < b1]
         pa = (_abi.PointConstraintDesc * max(len(pcs), 1))()
         for i, (a, b, la, lb, ov, op) in enumerate(pcs):

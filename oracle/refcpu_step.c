@@ -74,6 +74,7 @@ typedef struct {
 /* CCDBody (src/ccd.ts:19-60) */
 typedef struct {
     int body, hit;
+    uint32_t sub2; /* subShapeId2: the hit's sub-shape id on the body hit */
     V3 delta, normal, point;
     double fraction, fps, thrSq, maxPen, friction, restitution;
 } OCcd;
@@ -1187,6 +1188,36 @@ static int ccd_swept_overlap(const OBody* A, V3 rel, const OBody* B) {
     }
     return 1;
 }
+/* one leaf of body j (shape sb at pb, qb, sub-shape id sub) through the cast and the collector's
+ * addHit :1045-1098 */
+static void ccd_leaf(oc_world* w, OCcd* c, int j, int sb, V3 pb, Q4 qb, uint32_t sub, V3 rel, double dt) {
+    const OBody* A = &w->b[c->body];
+    const OBody* B = &w->b[j];
+    double fraction;
+    V3 pA, pen;
+    const int hit = w->shapes[sb].type == CC_SHAPE_PLANE
+                        ? ocx_cast_convex_plane(&w->sh, A->shape, A->pos, A->q, rel, sb, pb, qb, w->s.collide_with_backfaces,
+                                                c->fps, &fraction, &pA, &pen)
+                        : ocx_cast_convex(&w->sh, A->shape, A->pos, A->q, rel, sb, pb, qb, w->s.collide_with_backfaces,
+                                          c->fps, &fraction, &pA, &pen);
+    if (!hit) return;
+    const V3 n = vnormalize(pen);
+    const double den = vdot(n, c->delta);
+    if (den <= c->maxPen) return;
+    const double fps = fraction + c->maxPen / den;
+    if (fps >= c->fps) return;
+    c->hit = j;
+    c->sub2 = sub;
+    c->fraction = fraction;
+    c->fps = fps;
+    c->normal = n;
+    V3 mb = body_motion(B, dt);
+    if (vlen2(mb) > 1e-12) c->point = vadd(pA, vscale(mb, fraction));
+    else c->point = pA;
+    c->friction = combine(A->friction, B->friction, A->fmode, B->fmode);
+    c->restitution = combine(A->restitution, B->restitution, A->rmode, B->rmode);
+}
+
 static void ccd_find(oc_world* w, OCcd* c, double dt) {
     const OBody* A = &w->b[c->body];
     for (int j = 0; j < w->nb; j++) {
@@ -1202,29 +1233,22 @@ static void ccd_find(oc_world* w, OCcd* c, double dt) {
         }
         if (vlen2(rel) < c->thrSq) continue;
         if (!ccd_swept_overlap(A, rel, B)) continue;
-        double fraction;
-        V3 pA, pen;
-        const int hit = w->shapes[B->shape].type == CC_SHAPE_PLANE
-                            ? ocx_cast_convex_plane(&w->sh, A->shape, A->pos, A->q, rel, B->shape, B->pos, B->q,
-                                                    w->s.collide_with_backfaces, c->fps, &fraction, &pA, &pen)
-                            : ocx_cast_convex(&w->sh, A->shape, A->pos, A->q, rel, B->shape, B->pos, B->q,
-                                              w->s.collide_with_backfaces, c->fps, &fraction, &pA, &pen);
-        if (!hit) continue;
-        /* addHit :1045-1098 */
-        const V3 n = vnormalize(pen);
-        const double den = vdot(n, c->delta);
-        if (den <= c->maxPen) continue;
-        const double fps = fraction + c->maxPen / den;
-        if (fps >= c->fps) continue;
-        c->hit = j;
-        c->fraction = fraction;
-        c->fps = fps;
-        c->normal = n;
-        V3 mb = body_motion(B, dt);
-        if (vlen2(mb) > 1e-12) c->point = vadd(pA, vscale(mb, fraction));
-        else c->point = pA;
-        c->friction = combine(A->friction, B->friction, A->fmode, B->fmode);
-        c->restitution = combine(A->restitution, B->restitution, A->rmode, B->rmode);
+        const cc_shape_desc* D = &w->shapes[B->shape];
+        if (D->type != CC_SHAPE_COMPOUND) {
+            ccd_leaf(w, c, j, B->shape, B->pos, B->q, EMPTY_SUB, rel, dt);
+            continue;
+        }
+        /* castShapeVsCompound compound.ts: every child at the body pose composed with its
+         * transform, its sub-shape id pushed */
+        const int nk = D->compound_num_children;
+        const int nbits = nk > 1 ? 32 - __builtin_clz((unsigned)(nk - 1)) : 0;
+        for (int k = 0; k < nk; k++) {
+            const int ci = D->compound_first_child + k;
+            const cc_shape_desc* K = &w->shapes[ci];
+            const Q4 lq = {K->local_quaternion[0], K->local_quaternion[1], K->local_quaternion[2], K->local_quaternion[3]};
+            ccd_leaf(w, c, j, ci, vadd(B->pos, qrotate(B->q, vload(K->local_position))), qmul(B->q, lq),
+                     sub_push(EMPTY_SUB, 0, k, nbits), rel, dt);
+        }
     }
 }
 
@@ -1554,6 +1578,8 @@ int32_t oc_get_ccd_bodies(const oc_world* w, cc_ccd_body* out, int32_t cap) {
         o->max_penetration = c->maxPen;
         o->combined_friction = c->friction;
         o->combined_restitution = c->restitution;
+        o->sub_shape_id_1 = c->hit >= 0 ? EMPTY_SUB : 0u;
+        o->sub_shape_id_2 = c->hit >= 0 ? c->sub2 : 0u;
     }
     return w->nccd;
 }

@@ -219,3 +219,32 @@ def test_plane_cast_measures_against_the_plane_shape_frame():
     sc.end_world()
     b = _one_step(sc)
     assert b["position"][1][1] == pytest.approx(0.25 - 0.02, abs=2e-4)  # not 1.23
+
+
+def test_bullet_into_a_compound_hits_the_child_with_its_pushed_id():
+    """castShapeVsCompound: a sphere at 200 m/s into the near face of a compound's second child
+    (a 0.2 m box 1 m behind the first along z) stops with the allowed penetration, and the CCD
+    record carries the child's pushed sub-shape id (pushIndex(1, 2) on the empty id)."""
+    sc = SC.Scene(_zero_g())
+    sc.add_body(S.compound([((0.0, 0.0, -1.0), (0.0, 0.0, 0.0, 1.0), S.box((0.2, 0.2, 0.2))),
+                            ((0.0, 0.0, 1.0), (0.0, 0.0, 0.0, 1.0), S.box((0.2, 0.2, 0.2)))]),
+                (0.0, 0.0, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+    sc.add_body(S.sphere(0.1), (-2.0, 0.0, 1.0), linear_velocity=(200.0, 0.0, 0.0), motion_quality=LC,
+                linear_damping=0.0, angular_damping=0.0)
+    sc.end_world()
+    o = O.OracleWorld(sc)
+    o.step(DT)
+    b = o.bodies()
+    assert b["position"][1][0] == pytest.approx(-0.2 - 0.1 + min(0.02, 0.25 * 0.1), abs=2e-4)
+    rec = o.ccd_bodies()
+    assert len(rec) == 1 and rec[0]["hit_body_index"] == 0
+    assert rec[0]["sub_shape_id_2"] == 0xFFFFFFFF and rec[0]["sub_shape_id_1"] == 0xFFFFFFFF
+    sc2 = SC.Scene(_zero_g())  # the first child: pushIndex(0, 2) clears the lowest bit
+    sc2.add_body(S.compound([((0.0, 0.0, -1.0), (0.0, 0.0, 0.0, 1.0), S.box((0.2, 0.2, 0.2))),
+                             ((0.0, 0.0, 1.0), (0.0, 0.0, 0.0, 1.0), S.box((0.2, 0.2, 0.2)))]),
+                 (0.0, 0.0, 0.0), motion_type=_abi.CC_MOTION_STATIC)
+    sc2.add_body(S.sphere(0.1), (-2.0, 0.0, -1.0), linear_velocity=(200.0, 0.0, 0.0), motion_quality=LC)
+    sc2.end_world()
+    o2 = O.OracleWorld(sc2)
+    o2.step(DT)
+    assert o2.ccd_bodies()[0]["sub_shape_id_2"] == 0xFFFFFFFE
