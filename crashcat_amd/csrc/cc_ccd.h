@@ -360,6 +360,12 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
             V3<C> pk = pj;
             Q4<C> qk = qj;
             unsigned subk = 0xffffffffu;
+            if (!cmp && (DJ.deco & DECO_TRANSFORMED)) {  // castShapeVsTransformed (transformed.ts:634-700)
+                pk = pj + qrotate(qj, mk3<C>(DJ.lp[0], DJ.lp[1], DJ.lp[2]));
+                Q4<C> lq;
+                lq.x = DJ.lq[0]; lq.y = DJ.lq[1]; lq.z = DJ.lq[2]; lq.w = DJ.lq[3];
+                qk = qmul(qj, lq);
+            }
             if (cmp) {
                 sk = DJ.cfc + k;
                 const ShapeD<C>& K = stc.shapes[sk];

@@ -764,17 +764,15 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                 const cc_shape_desc& q = shapes[bodies[i].shape];
                 const bool convex = q.type == CC_SHAPE_SPHERE || q.type == CC_SHAPE_BOX || q.type == CC_SHAPE_CAPSULE ||
                                     q.type == CC_SHAPE_CONVEX_HULL || q.type == CC_SHAPE_CYLINDER;
-                bool ok = (convex || q.type == CC_SHAPE_PLANE) && q.decorated == 0;
-                if (q.type == CC_SHAPE_COMPOUND) {  // a compound of unscaled convex children, hit but never cast
-                    ok = !(bodies[i].motion_quality == CC_MOTION_QUALITY_LINEAR_CAST && bodies[i].motion_type == CC_MOTION_DYNAMIC);
-                    for (int k = 0; k < q.compound_num_children && ok; k++)
-                        ok = (shapes[q.compound_first_child + k].decorated & ~CC_DECO_TRANSFORMED) == 0;
-                }
+                // the LINEAR_CAST bodies are cast (undecorated convex leaves); the others are hit
+                // (convex leaves with their decorators, planes, compounds of convex leaves)
+                const bool cast = bodies[i].motion_quality == CC_MOTION_QUALITY_LINEAR_CAST &&
+                                  bodies[i].motion_type == CC_MOTION_DYNAMIC;
+                const bool ok = cast ? convex && q.decorated == 0 : convex || q.type == CC_SHAPE_PLANE || q.type == CC_SHAPE_COMPOUND;
                 if (!ok)
-                    return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is on the GPU path in worlds of undecorated "
-                                                     "convex shapes, planes and compounds of unscaled convex children only, the "
-                                                     "LINEAR_CAST bodies convex (world " + std::to_string(w) + ", body " +
-                                                     std::to_string(i) + ")");
+                    return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is on the GPU path for undecorated convex "
+                                                     "LINEAR_CAST bodies in worlds of convex shapes (decorated or not), planes and "
+                                                     "compounds (world " + std::to_string(w) + ", body " + std::to_string(i) + ")");
                 // getShapeInnerRadius: sphere.ts:147-149, box.ts:333-335, capsule.ts:254-256,
                 // cylinder.ts:261-263, convex-hull.ts:550-558 (the nearest face plane)
                 double r = 0;

@@ -735,7 +735,8 @@ def compound_pile(seed=1012, n=200, extent=(12.0, 8.0, 12.0), scene=None):
     return sc
 
 
-def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False, compound_targets=False):
+def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False, compound_targets=False,
+                decorated_targets=False):
     """Fast LINEAR_CAST bodies (MotionQuality.LINEAR_CAST, src/update.ts:932-967) in a walled
     range: a static floor, four thin static walls (0.1 m) around a 16 m square, one tilted thin
     slab in the middle, and `targets` dynamic boxes, spheres, capsules, cylinders and 12-point hulls
@@ -744,7 +745,9 @@ def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False
     spin and restitution in [0, 0.6); pairs of bullets fly at each other, one kinematic LINEAR_CAST
     paddle sweeps the range at 60 m/s, and a few slow LINEAR_CAST bodies stay below the cast
     threshold.  With `plane_floor` the floor is a static plane through the origin; with
-    `compound_targets` every third target is a compound of two or three unscaled leaves."""
+    `compound_targets` every third target is a compound of two or three unscaled leaves; with
+    `decorated_targets` every third is instead scaled, transformed, offset-centre-of-mass or a
+    compound with a scaled child, in turn."""
     sc = scene if scene is not None else Scene()
     rng = Rng(seed)
     if plane_floor:
@@ -775,7 +778,18 @@ def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False
         if abs(p[0]) < 2.5 and abs(p[2]) < 1.5:
             p = (p[0] + 5.0, p[1], p[2])
         tshape = shape(0.2, 0.4)
-        if compound_targets and len(aims) % 3 == 0:
+        if decorated_targets and len(aims) % 3 == 0:
+            kind = (len(aims) // 3) % 4
+            if kind == 0:
+                tshape = S.scaled(S.box((0.2, 0.25, 0.3)), (1.5, 0.8, -1.2))
+            elif kind == 1:
+                tshape = S.transformed(shape(0.15, 0.3), (0.2, 0.1, -0.1), rng.quaternion())
+            elif kind == 2:
+                tshape = S.offset_center_of_mass(shape(0.2, 0.35), (0.05, -0.1, 0.08))
+            else:
+                tshape = S.compound([((0.25, 0.0, 0.0), rng.quaternion(), S.scaled(S.box((0.15, 0.15, 0.15)), (1.4, 1.4, 1.4))),
+                                     ((-0.25, 0.0, 0.0), rng.quaternion(), shape(0.12, 0.2))])
+        elif compound_targets and len(aims) % 3 == 0:
             tshape = S.compound([((rng.uniform(-0.3, 0.3), rng.uniform(-0.2, 0.2), rng.uniform(-0.3, 0.3)), rng.quaternion(),
                                   shape(0.12, 0.25)) for _ in range(2 + int(rng.uniform() * 2))])
         sc.add_body(tshape, p, rng.quaternion(), sleeping=rng.uniform() < 1.0 / 3.0)

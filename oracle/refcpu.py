@@ -115,14 +115,8 @@ class OracleWorld:
                 if shapes[si].type not in convex or shapes[si].decorated:
                     raise ValueError("LINEAR_CAST bodies must be undecorated convex shapes")
             for si in np.unique(bodies["shape"]):  # ... against convex leaves, planes and compounds of them
-                s = shapes[si]
-                ok = (s.type in convex or s.type == _abi.CC_SHAPE_PLANE) and not s.decorated
-                if s.type == _abi.CC_SHAPE_COMPOUND:
-                    ok = all((shapes[s.compound_first_child + k].decorated & ~_abi.CC_DECO_TRANSFORMED) == 0
-                             for k in range(s.compound_num_children))
-                if not ok:
-                    raise ValueError("LINEAR_CAST bodies need a world of undecorated convex shapes, planes and "
-                                     "compounds of unscaled convex children")
+                if shapes[si].type not in convex + (_abi.CC_SHAPE_PLANE, _abi.CC_SHAPE_COMPOUND):
+                    raise ValueError("LINEAR_CAST bodies need a world of convex shapes, planes and compounds")
         pcs = [(a - b0, b - b0, la, lb, ov, op) for (a, b, la, lb, ov, op) in scene.point_constraints if b0 <= a < b1]
         pa = (_abi.PointConstraintDesc * max(len(pcs), 1))()
         for i, (a, b, la, lb, ov, op) in enumerate(pcs):

@@ -1235,7 +1235,17 @@ static void ccd_find(oc_world* w, OCcd* c, double dt) {
         if (!ccd_swept_overlap(A, rel, B)) continue;
         const cc_shape_desc* D = &w->shapes[B->shape];
         if (D->type != CC_SHAPE_COMPOUND) {
-            ccd_leaf(w, c, j, B->shape, B->pos, B->q, EMPTY_SUB, rel, dt);
+            /* castShapeVsTransformed / castShapeVsScaled / castShapeVsOffsetCenterOfMass
+             * (transformed.ts:634-700, scaled.ts:483-560, offset-center-of-mass.ts:458-530): the
+             * leaf at the body pose composed with the local transform, its scale in the support */
+            V3 pb = B->pos;
+            Q4 qb = B->q;
+            if (D->decorated & CC_DECO_TRANSFORMED) {
+                const Q4 lq = {D->local_quaternion[0], D->local_quaternion[1], D->local_quaternion[2], D->local_quaternion[3]};
+                pb = vadd(B->pos, qrotate(B->q, vload(D->local_position)));
+                qb = qmul(B->q, lq);
+            }
+            ccd_leaf(w, c, j, B->shape, pb, qb, EMPTY_SUB, rel, dt);
             continue;
         }
         /* castShapeVsCompound compound.ts: every child at the body pose composed with its

@@ -67,6 +67,26 @@ def test_fp64_ccd_gallery_with_compound_targets_bitwise():
         g.close()
 
 
+def test_fp64_ccd_gallery_with_decorated_targets_bitwise():
+    """Targets in scaled, transformed and offset-centre-of-mass decorators and compounds with scaled
+    children (castShapeVsScaled / Transformed / OffsetCenterOfMass / Compound)."""
+    sc = SC.ccd_gallery(seed=1020, decorated_targets=True)
+    orc = O.OracleWorld(sc)
+    g = Context("fp64", max_pairs_per_body=16)
+    g.upload(sc)
+    try:
+        for s in range(1, 41):
+            orc.step(DT)
+            g.step(DT, 1)
+            oc, gc = orc.ccd_bodies(), g.ccd_bodies()
+            for f in oc.dtype.names:
+                assert np.array_equal(oc[f], gc[f]), (s, f)
+            if s % 4 == 0:
+                _assert_bodies_equal(orc.bodies(), g.bodies(), f"step {s}")
+    finally:
+        g.close()
+
+
 def test_fp64_ccd_gallery_long_bitwise():
     _lockstep_fp64(SC.ccd_gallery(seed=1016, bullets=40, targets=120), 180, check_every=6)
 
@@ -127,14 +147,9 @@ def test_ccd_uploads_refuse_what_the_path_does_not_take():
         _ccd_body(sc)
         sc.end_world()
         bad.append(sc)
-        sc = SC.Scene()
-        sc.add_body(S.scaled(S.box((0.3, 0.3, 0.3)), (2.0, 1.0, 1.0)), (3, 1, 0))
-        _ccd_body(sc)
-        sc.end_world()
-        bad.append(sc)
-        sc = SC.Scene()  # a compound with a scaled child
-        sc.add_body(S.compound([((0, 0, 0), (0, 0, 0, 1), S.scaled(S.box((0.3, 0.3, 0.3)), (2.0, 1.0, 1.0)))]), (3, 1, 0))
-        _ccd_body(sc)
+        sc = SC.Scene()  # a decorated LINEAR_CAST body
+        sc.add_body(S.scaled(S.box((0.3, 0.3, 0.3)), (2.0, 1.0, 1.0)), (3, 1, 0), linear_velocity=(100.0, 0.0, 0.0),
+                    motion_quality=_abi.CC_MOTION_QUALITY_LINEAR_CAST)
         sc.end_world()
         bad.append(sc)
         sc = SC.Scene()  # a LINEAR_CAST compound
