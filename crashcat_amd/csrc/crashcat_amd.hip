@@ -768,11 +768,15 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                 // (convex leaves with their decorators, planes, compounds of convex leaves)
                 const bool cast = bodies[i].motion_quality == CC_MOTION_QUALITY_LINEAR_CAST &&
                                   bodies[i].motion_type == CC_MOTION_DYNAMIC;
-                const bool ok = cast ? convex && q.decorated == 0 : convex || q.type == CC_SHAPE_PLANE || q.type == CC_SHAPE_COMPOUND;
+                // (castScaledVsShape and castOffsetCenterOfMassVsShape pass the pose through, so a cast
+                // body may carry those decorators, not a transformed one)
+                const bool ok = cast ? convex && (q.decorated & CC_DECO_TRANSFORMED) == 0
+                                     : convex || q.type == CC_SHAPE_PLANE || q.type == CC_SHAPE_COMPOUND;
                 if (!ok)
-                    return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is on the GPU path for undecorated convex "
-                                                     "LINEAR_CAST bodies in worlds of convex shapes (decorated or not), planes and "
-                                                     "compounds (world " + std::to_string(w) + ", body " + std::to_string(i) + ")");
+                    return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is on the GPU path for convex LINEAR_CAST "
+                                                     "bodies (scaled or offset-centre-of-mass, not transformed) in worlds of convex "
+                                                     "shapes, planes and compounds (world " + std::to_string(w) + ", body " +
+                                                     std::to_string(i) + ")");
                 // getShapeInnerRadius: sphere.ts:147-149, box.ts:333-335, capsule.ts:254-256,
                 // cylinder.ts:261-263, convex-hull.ts:550-558 (the nearest face plane)
                 double r = 0;
@@ -784,6 +788,8 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                     for (int f = 0; f < q.hull_num_faces; f++) r = std::min(r, -hulls->plane_constants[q.hull_first_face + f]);
                     r = std::max(0.0, r);
                 }
+                if (q.decorated & CC_DECO_SCALED)  // scaled.ts:127-130
+                    r = std::min(std::fabs(q.scale[0]), std::min(std::fabs(q.scale[1]), std::fabs(q.scale[2]))) * r;
                 inner[i] = T(r);
             }
         }

@@ -736,7 +736,7 @@ def compound_pile(seed=1012, n=200, extent=(12.0, 8.0, 12.0), scene=None):
 
 
 def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False, compound_targets=False,
-                decorated_targets=False):
+                decorated_targets=False, decorated_bullets=False):
     """Fast LINEAR_CAST bodies (MotionQuality.LINEAR_CAST, src/update.ts:932-967) in a walled
     range: a static floor, four thin static walls (0.1 m) around a 16 m square, one tilted thin
     slab in the middle, and `targets` dynamic boxes, spheres, capsules, cylinders and 12-point hulls
@@ -747,7 +747,8 @@ def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False
     threshold.  With `plane_floor` the floor is a static plane through the origin; with
     `compound_targets` every third target is a compound of two or three unscaled leaves; with
     `decorated_targets` every third is instead scaled, transformed, offset-centre-of-mass or a
-    compound with a scaled child, in turn."""
+    compound with a scaled child, in turn; with `decorated_bullets` every third bullet is scaled
+    (non-uniformly for boxes and hulls) or has an offset centre of mass."""
     sc = scene if scene is not None else Scene()
     rng = Rng(seed)
     if plane_floor:
@@ -795,11 +796,25 @@ def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False
         sc.add_body(tshape, p, rng.quaternion(), sleeping=rng.uniform() < 1.0 / 3.0)
         aims.append(p)
 
+    nfired = [0]
+
+    def bullet_shape():
+        s = shape(0.1, 0.3)
+        nfired[0] += 1
+        if decorated_bullets and nfired[0] % 3 == 0:
+            if nfired[0] % 2 == 0:
+                u = rng.uniform(0.7, 1.4)
+                s = S.scaled(s, (u, u, u) if s.type in (S.SPHERE, S.CAPSULE, S.CYLINDER)
+                             else (rng.uniform(0.7, 1.4), rng.uniform(0.7, 1.4), rng.uniform(0.7, 1.4)))
+            else:
+                s = S.offset_center_of_mass(s, (rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05)))
+        return s
+
     def fire(pos, aim, speed, **kw):
         d = np.array(aim, dtype=float) - np.array(pos, dtype=float)
         d /= np.linalg.norm(d)
         w = (rng.normal(0, 5), rng.normal(0, 5), rng.normal(0, 5))
-        return sc.add_body(shape(0.1, 0.3), pos, rng.quaternion(), linear_velocity=tuple(speed * d),
+        return sc.add_body(bullet_shape(), pos, rng.quaternion(), linear_velocity=tuple(speed * d),
                            angular_velocity=w, motion_quality=_abi.CC_MOTION_QUALITY_LINEAR_CAST,
                            restitution=rng.uniform(0.0, 0.6), **kw)
 

@@ -69,8 +69,9 @@ def test_fp64_ccd_gallery_with_compound_targets_bitwise():
 
 def test_fp64_ccd_gallery_with_decorated_targets_bitwise():
     """Targets in scaled, transformed and offset-centre-of-mass decorators and compounds with scaled
-    children (castShapeVsScaled / Transformed / OffsetCenterOfMass / Compound)."""
-    sc = SC.ccd_gallery(seed=1020, decorated_targets=True)
+    children (castShapeVsScaled / Transformed / OffsetCenterOfMass / Compound), and scaled and
+    offset-centre-of-mass bullets (castScaledVsShape / castOffsetCenterOfMassVsShape)."""
+    sc = SC.ccd_gallery(seed=1020, decorated_targets=True, decorated_bullets=True)
     orc = O.OracleWorld(sc)
     g = Context("fp64", max_pairs_per_body=16)
     g.upload(sc)
@@ -147,8 +148,8 @@ def test_ccd_uploads_refuse_what_the_path_does_not_take():
         _ccd_body(sc)
         sc.end_world()
         bad.append(sc)
-        sc = SC.Scene()  # a decorated LINEAR_CAST body
-        sc.add_body(S.scaled(S.box((0.3, 0.3, 0.3)), (2.0, 1.0, 1.0)), (3, 1, 0), linear_velocity=(100.0, 0.0, 0.0),
+        sc = SC.Scene()  # a transformed LINEAR_CAST body
+        sc.add_body(S.transformed(S.box((0.3, 0.3, 0.3)), (0.1, 0.0, 0.0), (0, 0, 0, 1)), (3, 1, 0), linear_velocity=(100.0, 0.0, 0.0),
                     motion_quality=_abi.CC_MOTION_QUALITY_LINEAR_CAST)
         sc.end_world()
         bad.append(sc)
