@@ -375,13 +375,24 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
                 qk = qmul(qj, lq);
                 subk = sub_push(0xffffffffu, 0, k, nbits);
             }
+            // castTransformedVsShape (transformed.ts:537-633): A's leaf at the composed pose, the
+            // displacement turned by A's rotation
+            V3<C> pa = cvt<C>(B.pos[a]), da = cvt<C>(rel);
+            Q4<C> qa = cvtq<C>(B.q[a]);
+            const ShapeD<C>& DA = stc.shapes[B.shape[a]];
+            if (DA.deco & DECO_TRANSFORMED) {
+                const Q4<C> q0 = qa;
+                pa = pa + qrotate(q0, mk3<C>(DA.lp[0], DA.lp[1], DA.lp[2]));
+                Q4<C> lq;
+                lq.x = DA.lq[0]; lq.y = DA.lq[1]; lq.z = DA.lq[2]; lq.w = DA.lq[3];
+                qa = qmul(q0, lq);
+                da = qrotate(q0, da);
+            }
             C fraction;
             V3<C> pA, pen;
             const bool hit = stc.shapes[sk].type == SHAPE_PLANE
-                                 ? cast_convex_plane(stc, B.shape[a], cvt<C>(B.pos[a]), cvtq<C>(B.q[a]), cvt<C>(rel), sk, pk,
-                                                     qk, d.s.backfaces != 0, fraction, pA, pen)
-                                 : cast_convex(stc, B.shape[a], cvt<C>(B.pos[a]), cvtq<C>(B.q[a]), cvt<C>(rel), sk, pk, qk,
-                                               d.s.backfaces != 0, fraction, pA, pen);
+                                 ? cast_convex_plane(stc, B.shape[a], pa, qa, da, sk, pk, qk, d.s.backfaces != 0, fraction, pA, pen)
+                                 : cast_convex(stc, B.shape[a], pa, qa, da, sk, pk, qk, d.s.backfaces != 0, fraction, pA, pen);
             if (!hit) continue;
             // addHit :1045-1098
             const V3<T> n = cvt<T>(normalize(pen));

@@ -768,13 +768,13 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                 // (convex leaves with their decorators, planes, compounds of convex leaves)
                 const bool cast = bodies[i].motion_quality == CC_MOTION_QUALITY_LINEAR_CAST &&
                                   bodies[i].motion_type == CC_MOTION_DYNAMIC;
-                // (castScaledVsShape and castOffsetCenterOfMassVsShape pass the pose through, so a cast
-                // body may carry those decorators, not a transformed one)
-                const bool ok = cast ? convex && (q.decorated & CC_DECO_TRANSFORMED) == 0
+                // (castScaledVsShape and castOffsetCenterOfMassVsShape pass the pose through; the one
+                // transformed decorator a chain holds turns the displacement by the body rotation)
+                const bool ok = cast ? convex
                                      : convex || q.type == CC_SHAPE_PLANE || q.type == CC_SHAPE_COMPOUND;
                 if (!ok)
                     return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is on the GPU path for convex LINEAR_CAST "
-                                                     "bodies (scaled or offset-centre-of-mass, not transformed) in worlds of convex "
+                                                     "bodies in worlds of convex "
                                                      "shapes, planes and compounds (world " + std::to_string(w) + ", body " +
                                                      std::to_string(i) + ")");
                 // getShapeInnerRadius: sphere.ts:147-149, box.ts:333-335, capsule.ts:254-256,

@@ -1193,12 +1193,23 @@ static int ccd_swept_overlap(const OBody* A, V3 rel, const OBody* B) {
 static void ccd_leaf(oc_world* w, OCcd* c, int j, int sb, V3 pb, Q4 qb, uint32_t sub, V3 rel, double dt) {
     const OBody* A = &w->b[c->body];
     const OBody* B = &w->b[j];
+    /* castTransformedVsShape transformed.ts:537-633: A's leaf at the composed pose, the
+     * displacement turned by A's rotation; scaled and offset-centre-of-mass pass through */
+    const cc_shape_desc* DA = &w->shapes[A->shape];
+    V3 pa = A->pos, da = rel;
+    Q4 qa = A->q;
+    if (DA->decorated & CC_DECO_TRANSFORMED) {
+        const Q4 lq = {DA->local_quaternion[0], DA->local_quaternion[1], DA->local_quaternion[2], DA->local_quaternion[3]};
+        pa = vadd(A->pos, qrotate(A->q, vload(DA->local_position)));
+        qa = qmul(A->q, lq);
+        da = qrotate(A->q, rel);
+    }
     double fraction;
     V3 pA, pen;
     const int hit = w->shapes[sb].type == CC_SHAPE_PLANE
-                        ? ocx_cast_convex_plane(&w->sh, A->shape, A->pos, A->q, rel, sb, pb, qb, w->s.collide_with_backfaces,
+                        ? ocx_cast_convex_plane(&w->sh, A->shape, pa, qa, da, sb, pb, qb, w->s.collide_with_backfaces,
                                                 c->fps, &fraction, &pA, &pen)
-                        : ocx_cast_convex(&w->sh, A->shape, A->pos, A->q, rel, sb, pb, qb, w->s.collide_with_backfaces,
+                        : ocx_cast_convex(&w->sh, A->shape, pa, qa, da, sb, pb, qb, w->s.collide_with_backfaces,
                                           c->fps, &fraction, &pA, &pen);
     if (!hit) return;
     const V3 n = vnormalize(pen);
