@@ -317,7 +317,7 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
     const V3<double> amn = cvt<double>(B.amin[a]), amx = cvt<double>(B.amax[a]);
     T best = c.fps;
     int bestJ = 0x7fffffff;
-    unsigned bSub = 0xffffffffu;
+    unsigned bSub = 0xffffffffu, bSubA = 0xffffffffu;
     T bFrac = T(0);
     V3<T> bNormal = mk3<T>(0, 0, 0), bPoint = mk3<T>(0, 0, 0);
     for (int j = b0 + lane; j < b1; j += 64) {
@@ -348,6 +348,28 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
         }
         // a leaf, or every child of a compound at the body pose composed with the child's transform
         // with its sub-shape id pushed (castShapeVsCompound compound.ts)
+        // A's leaves: castCompoundVsShape (compound.ts: each child at the composed pose, the
+        // displacement turned by A's rotation, its id pushed) or castTransformedVsShape
+        // (transformed.ts:537-633, the same for the one transform); scaled and
+        // offset-centre-of-mass pass through
+        const ShapeD<C>& DA0 = stc.shapes[B.shape[a]];
+        const bool cmpA = DA0.type == SHAPE_COMPOUND;
+        const int na = cmpA ? DA0.ncc : 1;
+        const int abits = cmpA ? sub_bits(na) : 0;
+        for (int ka = 0; ka < na; ka++) {
+        const int sa = cmpA ? DA0.cfc + ka : B.shape[a];
+        const ShapeD<C>& DA = stc.shapes[sa];
+        V3<C> pa = cvt<C>(B.pos[a]), da = cvt<C>(rel);
+        Q4<C> qa = cvtq<C>(B.q[a]);
+        if (DA.deco & DECO_TRANSFORMED) {
+            const Q4<C> q0 = qa;
+            pa = pa + qrotate(q0, mk3<C>(DA.lp[0], DA.lp[1], DA.lp[2]));
+            Q4<C> lq;
+            lq.x = DA.lq[0]; lq.y = DA.lq[1]; lq.z = DA.lq[2]; lq.w = DA.lq[3];
+            qa = qmul(q0, lq);
+            da = qrotate(q0, da);
+        }
+        const unsigned subA = cmpA ? sub_push(0xffffffffu, 0, ka, abits) : 0xffffffffu;
         const int sj = B.shape[j];
         const ShapeD<C>& DJ = stc.shapes[sj];
         const bool cmp = DJ.type == SHAPE_COMPOUND;
@@ -375,24 +397,11 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
                 qk = qmul(qj, lq);
                 subk = sub_push(0xffffffffu, 0, k, nbits);
             }
-            // castTransformedVsShape (transformed.ts:537-633): A's leaf at the composed pose, the
-            // displacement turned by A's rotation
-            V3<C> pa = cvt<C>(B.pos[a]), da = cvt<C>(rel);
-            Q4<C> qa = cvtq<C>(B.q[a]);
-            const ShapeD<C>& DA = stc.shapes[B.shape[a]];
-            if (DA.deco & DECO_TRANSFORMED) {
-                const Q4<C> q0 = qa;
-                pa = pa + qrotate(q0, mk3<C>(DA.lp[0], DA.lp[1], DA.lp[2]));
-                Q4<C> lq;
-                lq.x = DA.lq[0]; lq.y = DA.lq[1]; lq.z = DA.lq[2]; lq.w = DA.lq[3];
-                qa = qmul(q0, lq);
-                da = qrotate(q0, da);
-            }
             C fraction;
             V3<C> pA, pen;
             const bool hit = stc.shapes[sk].type == SHAPE_PLANE
-                                 ? cast_convex_plane(stc, B.shape[a], pa, qa, da, sk, pk, qk, d.s.backfaces != 0, fraction, pA, pen)
-                                 : cast_convex(stc, B.shape[a], pa, qa, da, sk, pk, qk, d.s.backfaces != 0, fraction, pA, pen);
+                                 ? cast_convex_plane(stc, sa, pa, qa, da, sk, pk, qk, d.s.backfaces != 0, fraction, pA, pen)
+                                 : cast_convex(stc, sa, pa, qa, da, sk, pk, qk, d.s.backfaces != 0, fraction, pA, pen);
             if (!hit) continue;
             // addHit :1045-1098
             const V3<T> n = cvt<T>(normalize(pen));
@@ -403,10 +412,12 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
             best = fps;
             bestJ = j;
             bSub = subk;
+            bSubA = subA;
             bFrac = T(fraction);
             bNormal = n;
             const V3<T> mb = ((fb & FL_LINEAR_CAST) && fl_motion(fb) != MOTION_STATIC) ? B.v[j] * dt : mk3<T>(0, 0, 0);
             bPoint = len2(mb) > T(1e-12) ? cvt<T>(pA) + mb * T(fraction) : cvt<T>(pA);
+        }
         }
     }
     // the wavefront's earliest (fraction-plus-slop, body index)
@@ -420,6 +431,7 @@ __device__ inline void ccd_find(const Dev<T>& d, CcdRec<T>* recs, const int* ccd
     if (bestJ != 0x7fffffff && bestJ == mj) {
         CcdRec<T>& r = recs[cb0 + e];
         r.hit = bestJ;
+        r.sub1 = bSubA;
         r.sub2 = bSub;
         r.fraction = bFrac;
         r.fps = best;

@@ -803,6 +803,7 @@ __device__ inline bool integrate_body(const Dev<T>& d, int i, T dt, bool ccd, Cc
         if (len2(disp) > thr * thr) {
             rec.body = i;
             rec.hit = -1;
+            rec.sub1 = 0;
             rec.sub2 = 0;
             rec.delta = disp;
             rec.normal = mk3<T>(0, 0, 0);

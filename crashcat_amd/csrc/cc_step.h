@@ -21,7 +21,7 @@ constexpr int FL_LINEAR_CAST = 1 << 24;  // Bodies::flags: motionQuality == LINE
 template <typename T>
 struct CcdRec {
     int body, hit;
-    unsigned sub2;  // subShapeId2: the hit's sub-shape id on the body hit
+    unsigned sub1, sub2;  // subShapeId1 / subShapeId2: the hit's sub-shape ids
     V3<T> delta, normal, point;
     T fraction, fps, thrSq, maxPen, friction, restitution;
 };

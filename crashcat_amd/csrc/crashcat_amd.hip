@@ -770,7 +770,7 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                                   bodies[i].motion_type == CC_MOTION_DYNAMIC;
                 // (castScaledVsShape and castOffsetCenterOfMassVsShape pass the pose through; the one
                 // transformed decorator a chain holds turns the displacement by the body rotation)
-                const bool ok = cast ? convex
+                const bool ok = cast ? convex || q.type == CC_SHAPE_COMPOUND
                                      : convex || q.type == CC_SHAPE_PLANE || q.type == CC_SHAPE_COMPOUND;
                 if (!ok)
                     return fail(c, CC_E_UNSUPPORTED, "MotionQuality.LINEAR_CAST (CCD) is on the GPU path for convex LINEAR_CAST "
@@ -778,18 +778,29 @@ static int upload_into(cc_ctx* c, CtxT<T>* X, const cc_settings* s, int32_t nw, 
                                                      "shapes, planes and compounds (world " + std::to_string(w) + ", body " +
                                                      std::to_string(i) + ")");
                 // getShapeInnerRadius: sphere.ts:147-149, box.ts:333-335, capsule.ts:254-256,
-                // cylinder.ts:261-263, convex-hull.ts:550-558 (the nearest face plane)
+                // cylinder.ts:261-263, convex-hull.ts:550-558 (the nearest face plane),
+                // scaled.ts:127-130, compound.ts:283-291 (the smallest child's)
+                auto leafInner = [&](const cc_shape_desc& p) {
+                    double r = 0;
+                    if (p.type == CC_SHAPE_SPHERE || p.type == CC_SHAPE_CAPSULE) r = p.radius;
+                    else if (p.type == CC_SHAPE_BOX) r = std::min(p.half_extents[0], std::min(p.half_extents[1], p.half_extents[2]));
+                    else if (p.type == CC_SHAPE_CYLINDER) r = std::min(p.half_height, p.radius);
+                    else if (p.type == CC_SHAPE_CONVEX_HULL) {
+                        r = 1.7976931348623157e308;
+                        for (int f = 0; f < p.hull_num_faces; f++) r = std::min(r, -hulls->plane_constants[p.hull_first_face + f]);
+                        r = std::max(0.0, r);
+                    }
+                    if (p.decorated & CC_DECO_SCALED)
+                        r = std::min(std::fabs(p.scale[0]), std::min(std::fabs(p.scale[1]), std::fabs(p.scale[2]))) * r;
+                    return r;
+                };
                 double r = 0;
-                if (q.type == CC_SHAPE_SPHERE || q.type == CC_SHAPE_CAPSULE) r = q.radius;
-                else if (q.type == CC_SHAPE_BOX) r = std::min(q.half_extents[0], std::min(q.half_extents[1], q.half_extents[2]));
-                else if (q.type == CC_SHAPE_CYLINDER) r = std::min(q.half_height, q.radius);
-                else if (q.type == CC_SHAPE_CONVEX_HULL) {
+                if (q.type == CC_SHAPE_COMPOUND) {
                     r = 1.7976931348623157e308;
-                    for (int f = 0; f < q.hull_num_faces; f++) r = std::min(r, -hulls->plane_constants[q.hull_first_face + f]);
-                    r = std::max(0.0, r);
+                    for (int k = 0; k < q.compound_num_children; k++) r = std::min(r, leafInner(shapes[q.compound_first_child + k]));
+                } else {
+                    r = leafInner(q);
                 }
-                if (q.decorated & CC_DECO_SCALED)  // scaled.ts:127-130
-                    r = std::min(std::fabs(q.scale[0]), std::min(std::fabs(q.scale[1]), std::fabs(q.scale[2]))) * r;
                 inner[i] = T(r);
             }
         }
@@ -2149,7 +2160,7 @@ static int ccd_bodies_impl(cc_ctx* c, cc_ccd_body* out, int cap, int* count) {
             o.max_penetration = r.maxPen;
             o.combined_friction = r.friction;
             o.combined_restitution = r.restitution;
-            o.sub_shape_id_1 = r.hit >= 0 ? 0xffffffffu : 0u;
+            o.sub_shape_id_1 = r.hit >= 0 ? r.sub1 : 0u;
             o.sub_shape_id_2 = r.hit >= 0 ? r.sub2 : 0u;
         }
     *count = k;

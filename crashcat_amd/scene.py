@@ -748,8 +748,8 @@ def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False
     `compound_targets` every third target is a compound of two or three unscaled leaves; with
     `decorated_targets` every third is instead scaled, transformed, offset-centre-of-mass or a
     compound with a scaled child, in turn; with `decorated_bullets` every third bullet is scaled
-    (non-uniformly for boxes and hulls), in a transformed decorator or has an offset centre of
-    mass."""
+    (non-uniformly for boxes and hulls), a compound of two leaves, in a transformed decorator or
+    has an offset centre of mass."""
     sc = scene if scene is not None else Scene()
     rng = Rng(seed)
     if plane_floor:
@@ -807,6 +807,8 @@ def ccd_gallery(seed=1013, bullets=24, targets=60, scene=None, plane_floor=False
                 u = rng.uniform(0.7, 1.4)
                 s = S.scaled(s, (u, u, u) if s.type in (S.SPHERE, S.CAPSULE, S.CYLINDER)
                              else (rng.uniform(0.7, 1.4), rng.uniform(0.7, 1.4), rng.uniform(0.7, 1.4)))
+            elif nfired[0] % 15 == 0:
+                s = S.compound([((0.12, 0.0, 0.0), rng.quaternion(), s), ((-0.12, 0.0, 0.0), rng.quaternion(), shape(0.08, 0.15))])
             elif nfired[0] % 9 == 0:
                 s = S.transformed(s, (rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1)), rng.quaternion())
             else:

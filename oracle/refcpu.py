@@ -112,8 +112,8 @@ class OracleWorld:
                       _abi.CC_SHAPE_CYLINDER)
             lc = (bodies["motion_quality"] == _abi.CC_MOTION_QUALITY_LINEAR_CAST) & (bodies["motion_type"] == _abi.CC_MOTION_DYNAMIC)
             for si in np.unique(bodies["shape"][lc]):  # the cast is restated for convex leaves
-                if shapes[si].type not in convex:
-                    raise ValueError("LINEAR_CAST bodies must be convex shapes (decorated or not)")
+                if shapes[si].type not in convex + (_abi.CC_SHAPE_COMPOUND,):
+                    raise ValueError("LINEAR_CAST bodies must be convex shapes (decorated or not) or compounds")
             for si in np.unique(bodies["shape"]):  # ... against convex leaves, planes and compounds of them
                 if shapes[si].type not in convex + (_abi.CC_SHAPE_PLANE, _abi.CC_SHAPE_COMPOUND):
                     raise ValueError("LINEAR_CAST bodies need a world of convex shapes, planes and compounds")

@@ -74,7 +74,7 @@ typedef struct {
 /* CCDBody (src/ccd.ts:19-60) */
 typedef struct {
     int body, hit;
-    uint32_t sub2; /* subShapeId2: the hit's sub-shape id on the body hit */
+    uint32_t sub1, sub2; /* subShapeId1 / subShapeId2: the hit's sub-shape ids */
     V3 delta, normal, point;
     double fraction, fps, thrSq, maxPen, friction, restitution;
 } OCcd;
@@ -1146,6 +1146,11 @@ static int solve_position_contact(oc_world* w, OCC* k) {
 static double inner_radius(const oc_world* w, int si) {
     const cc_shape_desc* d = &w->shapes[si];
     double r = 0;
+    if (d->type == CC_SHAPE_COMPOUND) { /* the smallest child's (compound.ts:283-291) */
+        r = DBL_MAX;
+        for (int k = 0; k < d->compound_num_children; k++) r = dmin(r, inner_radius(w, d->compound_first_child + k));
+        return r;
+    }
     switch (d->type) {
     case CC_SHAPE_SPHERE: case CC_SHAPE_CAPSULE: r = d->radius; break;
     case CC_SHAPE_BOX: r = dmin(d->half_extents[0], dmin(d->half_extents[1], d->half_extents[2])); break;
@@ -1190,26 +1195,16 @@ static int ccd_swept_overlap(const OBody* A, V3 rel, const OBody* B) {
 }
 /* one leaf of body j (shape sb at pb, qb, sub-shape id sub) through the cast and the collector's
  * addHit :1045-1098 */
-static void ccd_leaf(oc_world* w, OCcd* c, int j, int sb, V3 pb, Q4 qb, uint32_t sub, V3 rel, double dt) {
+static void ccd_leaf(oc_world* w, OCcd* c, int j, int sa, V3 pa, Q4 qa, V3 da, uint32_t subA, int sb, V3 pb, Q4 qb,
+                     uint32_t sub, double dt) {
     const OBody* A = &w->b[c->body];
     const OBody* B = &w->b[j];
-    /* castTransformedVsShape transformed.ts:537-633: A's leaf at the composed pose, the
-     * displacement turned by A's rotation; scaled and offset-centre-of-mass pass through */
-    const cc_shape_desc* DA = &w->shapes[A->shape];
-    V3 pa = A->pos, da = rel;
-    Q4 qa = A->q;
-    if (DA->decorated & CC_DECO_TRANSFORMED) {
-        const Q4 lq = {DA->local_quaternion[0], DA->local_quaternion[1], DA->local_quaternion[2], DA->local_quaternion[3]};
-        pa = vadd(A->pos, qrotate(A->q, vload(DA->local_position)));
-        qa = qmul(A->q, lq);
-        da = qrotate(A->q, rel);
-    }
     double fraction;
     V3 pA, pen;
     const int hit = w->shapes[sb].type == CC_SHAPE_PLANE
-                        ? ocx_cast_convex_plane(&w->sh, A->shape, pa, qa, da, sb, pb, qb, w->s.collide_with_backfaces,
+                        ? ocx_cast_convex_plane(&w->sh, sa, pa, qa, da, sb, pb, qb, w->s.collide_with_backfaces,
                                                 c->fps, &fraction, &pA, &pen)
-                        : ocx_cast_convex(&w->sh, A->shape, pa, qa, da, sb, pb, qb, w->s.collide_with_backfaces,
+                        : ocx_cast_convex(&w->sh, sa, pa, qa, da, sb, pb, qb, w->s.collide_with_backfaces,
                                           c->fps, &fraction, &pA, &pen);
     if (!hit) return;
     const V3 n = vnormalize(pen);
@@ -1218,6 +1213,7 @@ static void ccd_leaf(oc_world* w, OCcd* c, int j, int sb, V3 pb, Q4 qb, uint32_t
     const double fps = fraction + c->maxPen / den;
     if (fps >= c->fps) return;
     c->hit = j;
+    c->sub1 = subA;
     c->sub2 = sub;
     c->fraction = fraction;
     c->fps = fps;
@@ -1244,6 +1240,26 @@ static void ccd_find(oc_world* w, OCcd* c, double dt) {
         }
         if (vlen2(rel) < c->thrSq) continue;
         if (!ccd_swept_overlap(A, rel, B)) continue;
+        /* A's leaves: castCompoundVsShape (compound.ts: each child at the composed pose, the
+         * displacement turned by A's rotation, its id pushed) or castTransformedVsShape
+         * (transformed.ts:537-633, the same for the one transform); scaled and
+         * offset-centre-of-mass pass through */
+        const cc_shape_desc* DA = &w->shapes[A->shape];
+        const int cmpA = DA->type == CC_SHAPE_COMPOUND;
+        const int na = cmpA ? DA->compound_num_children : 1;
+        const int abits = na > 1 ? 32 - __builtin_clz((unsigned)(na - 1)) : 0;
+        for (int ka = 0; ka < na; ka++) {
+        const int sa = cmpA ? DA->compound_first_child + ka : A->shape;
+        const cc_shape_desc* LA = &w->shapes[sa];
+        V3 pa = A->pos, da = rel;
+        Q4 qa = A->q;
+        if (LA->decorated & CC_DECO_TRANSFORMED) {
+            const Q4 lq = {LA->local_quaternion[0], LA->local_quaternion[1], LA->local_quaternion[2], LA->local_quaternion[3]};
+            pa = vadd(A->pos, qrotate(A->q, vload(LA->local_position)));
+            qa = qmul(A->q, lq);
+            da = qrotate(A->q, rel);
+        }
+        const uint32_t subA = cmpA ? sub_push(EMPTY_SUB, 0, ka, abits) : EMPTY_SUB;
         const cc_shape_desc* D = &w->shapes[B->shape];
         if (D->type != CC_SHAPE_COMPOUND) {
             /* castShapeVsTransformed / castShapeVsScaled / castShapeVsOffsetCenterOfMass
@@ -1256,7 +1272,7 @@ static void ccd_find(oc_world* w, OCcd* c, double dt) {
                 pb = vadd(B->pos, qrotate(B->q, vload(D->local_position)));
                 qb = qmul(B->q, lq);
             }
-            ccd_leaf(w, c, j, B->shape, pb, qb, EMPTY_SUB, rel, dt);
+            ccd_leaf(w, c, j, sa, pa, qa, da, subA, B->shape, pb, qb, EMPTY_SUB, dt);
             continue;
         }
         /* castShapeVsCompound compound.ts: every child at the body pose composed with its
@@ -1267,8 +1283,9 @@ static void ccd_find(oc_world* w, OCcd* c, double dt) {
             const int ci = D->compound_first_child + k;
             const cc_shape_desc* K = &w->shapes[ci];
             const Q4 lq = {K->local_quaternion[0], K->local_quaternion[1], K->local_quaternion[2], K->local_quaternion[3]};
-            ccd_leaf(w, c, j, ci, vadd(B->pos, qrotate(B->q, vload(K->local_position))), qmul(B->q, lq),
-                     sub_push(EMPTY_SUB, 0, k, nbits), rel, dt);
+            ccd_leaf(w, c, j, sa, pa, qa, da, subA, ci, vadd(B->pos, qrotate(B->q, vload(K->local_position))), qmul(B->q, lq),
+                     sub_push(EMPTY_SUB, 0, k, nbits), dt);
+        }
         }
     }
 }
@@ -1599,7 +1616,7 @@ int32_t oc_get_ccd_bodies(const oc_world* w, cc_ccd_body* out, int32_t cap) {
         o->max_penetration = c->maxPen;
         o->combined_friction = c->friction;
         o->combined_restitution = c->restitution;
-        o->sub_shape_id_1 = c->hit >= 0 ? EMPTY_SUB : 0u;
+        o->sub_shape_id_1 = c->hit >= 0 ? c->sub1 : 0u;
         o->sub_shape_id_2 = c->hit >= 0 ? c->sub2 : 0u;
     }
     return w->nccd;

@@ -148,11 +148,6 @@ def test_ccd_uploads_refuse_what_the_path_does_not_take():
         _ccd_body(sc)
         sc.end_world()
         bad.append(sc)
-        sc = SC.Scene()  # a LINEAR_CAST compound
-        sc.add_body(S.compound([((0, 0, 0), (0, 0, 0, 1), S.box((0.3, 0.3, 0.3)))]), (3, 1, 0),
-                    linear_velocity=(100.0, 0.0, 0.0), motion_quality=_abi.CC_MOTION_QUALITY_LINEAR_CAST)
-        sc.end_world()
-        bad.append(sc)
         for sc in bad:
             with pytest.raises(CrashcatError, match="LINEAR_CAST"):
                 g.upload(sc)
